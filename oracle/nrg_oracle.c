@@ -1,0 +1,958 @@
+/*
+ * nrg_oracle.c -- FP64 single-thread CPU restatement of the TwoChNRG
+ * iterative-diagonalisation path (OneChQS Anderson through NRG_main).
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the parity checker for the HIP
+ * library and the timed "port" CPU baseline.  The product library never links,
+ * loads or calls it.  It follows the reference's loops literally (linear
+ * block searches, dense row-major GEMMs, unblocked Householder + implicit QL)
+ * so that the data layouts and orderings it produces are the reference's.
+ *
+ * Pinning (see tests/test_oracle.py): golden run Runs/run1 (636 levels,
+ * block labels, Emin/Ecut, <n_d>,<n_d^2>,<S_d^2>), closed forms (H_{-1},
+ * chi_N), brute-force Fock-space exact diagonalisation, U=0 single-particle
+ * sums, LAPACK (scipy) eigenvalues, CG tables.
+ */
+#include "nrg_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#ifndef M_PI
+#define M_PI 3.14159265358979323846
+#endif
+
+/* ------------------------------------------------------------------ */
+/* growable arrays                                                      */
+/* ------------------------------------------------------------------ */
+typedef struct { double *a; int n, cap; } dvec;
+typedef struct { int *a; int n, cap; } ivec;
+
+static void dres(dvec *v, int cap) {
+  if (cap > v->cap) {
+    int c = v->cap ? v->cap : 16;
+    while (c < cap) c *= 2;
+    v->a = (double *)realloc(v->a, sizeof(double) * (size_t)c);
+    v->cap = c;
+  }
+}
+static void ires(ivec *v, int cap) {
+  if (cap > v->cap) {
+    int c = v->cap ? v->cap : 16;
+    while (c < cap) c *= 2;
+    v->a = (int *)realloc(v->a, sizeof(int) * (size_t)c);
+    v->cap = c;
+  }
+}
+static void dpush(dvec *v, double x) { dres(v, v->n + 1); v->a[v->n++] = x; }
+static void ipush(ivec *v, int x) { ires(v, v->n + 1); v->a[v->n++] = x; }
+static void dcopy_v(dvec *dst, const dvec *src) {
+  dres(dst, src->n); if (src->n) memcpy(dst->a, src->a, sizeof(double) * (size_t)src->n); dst->n = src->n;
+}
+static void icopy_v(ivec *dst, const ivec *src) {
+  ires(dst, src->n); if (src->n) memcpy(dst->a, src->a, sizeof(int) * (size_t)src->n); dst->n = src->n;
+}
+static void derase(dvec *v, int b, int e) { /* erase [b,e) */
+  if (e <= b) return;
+  memmove(v->a + b, v->a + e, sizeof(double) * (size_t)(v->n - e));
+  v->n -= (e - b);
+}
+static void ierase(ivec *v, int b, int e) {
+  if (e <= b) return;
+  memmove(v->a + b, v->a + e, sizeof(int) * (size_t)(v->n - e));
+  v->n -= (e - b);
+}
+
+/* ------------------------------------------------------------------ */
+/* CNRGarray / CNRGbasisarray / CNRGmatrix subset                       */
+/* include/NRGclasses.hpp:47-759                                        */
+/* ------------------------------------------------------------------ */
+typedef struct {
+  int nshell, nq;
+  dvec qn;      /* QNumbers */
+  ivec bbe;     /* BlockBegEnd */
+  dvec en;      /* dEn */
+  dvec ev;      /* dEigVec */
+  ivec kept;    /* Kept */
+  ivec itype;   /* CNRGbasisarray::iType */
+  ivec stcf;    /* CNRGbasisarray::StCameFrom */
+  ivec bbe_bc;  /* BlockBegEndBC */
+  ivec bbe_ev;  /* BlockBegEndEigVec */
+} arr_t;
+
+typedef struct {
+  arr_t b;          /* synced array part */
+  dvec mat;         /* MatEl */
+  ivec map;         /* MatBlockMap */
+  ivec mbe;         /* MatBlockBegEnd */
+  int upper, need_old, calc_avg;
+  int (*check)(const arr_t *, int, int);
+  double (*calc)(const arr_t *, const arr_t *, int, int);
+} mat_t;
+
+static void arr_free(arr_t *A) {
+  free(A->qn.a); free(A->bbe.a); free(A->en.a); free(A->ev.a); free(A->kept.a);
+  free(A->itype.a); free(A->stcf.a); free(A->bbe_bc.a); free(A->bbe_ev.a);
+  memset(A, 0, sizeof(*A));
+}
+static void mat_free(mat_t *M) {
+  arr_free(&M->b); free(M->mat.a); free(M->map.a); free(M->mbe.a);
+  memset(M, 0, sizeof(*M));
+}
+/* CNRGarray::ClearAll  src/NRGarrayClass.cpp (ClearAll) + CNRGbasisarray::ClearAll :1150-1161 */
+static void arr_clear(arr_t *A) {
+  A->qn.n = A->bbe.n = A->en.n = A->ev.n = A->kept.n = 0;
+  A->itype.n = A->stcf.n = A->bbe_bc.n = A->bbe_ev.n = 0;
+}
+static int nblocks(const arr_t *A) { return A->nq ? A->qn.n / A->nq : 0; }
+static int blim(const arr_t *A, int b, int w) { return b < 0 ? 0 : A->bbe.a[2 * b + w]; }
+static int bsize(const arr_t *A, int b) { return A->bbe.a[2 * b + 1] - A->bbe.a[2 * b] + 1; }
+static int nstates(const arr_t *A) { return A->bbe.a[A->bbe.n - 1] - A->bbe.a[0] + 1; }
+static double getqn(const arr_t *A, int b, int q) { return A->qn.n ? A->qn.a[b * A->nq + q] : 0.0; }
+/* GetBlockFromSt  src/NRGarrayClass.cpp:374-390 (linear) */
+static int blk_from_st(const arr_t *A, int ist, int *pos) {
+  int b = 0;
+  while (ist > A->bbe.a[2 * b + 1]) b++;
+  *pos = ist - A->bbe.a[2 * b];
+  return b;
+}
+static double qn_from_st(const arr_t *A, int ist, int q) {
+  int pos; int b = blk_from_st(A, ist, &pos); return getqn(A, b, q);
+}
+/* dEqual  src/dEqual.cpp:6-10 */
+static int dEqual(double a, double b) { return fabs(a - b) < 1e-20; }
+/* GetBlockFromQNumbers  src/NRGarrayClass.cpp:514-542: first block whose QNs all dEqual */
+static int blk_from_qn(const arr_t *A, const double *q) {
+  int nb = nblocks(A);
+  for (int b = 0; b < nb; b++) {
+    int ok = 1;
+    for (int i = 0; i < A->nq; i++) if (!dEqual(A->qn.a[b * A->nq + i], q[i])) { ok = 0; break; }
+    if (ok) return b;
+  }
+  return -1;
+}
+/* GetBlockLimitEigv: CNRGarray sums n^2 (:400-415); CNRGbasisarray uses BlockBegEndEigVec (:1550-1564) */
+static int blim_ev(const arr_t *A, int b) {
+  if (A->bbe_ev.n) return A->bbe_ev.a[2 * b];
+  int pos = 0;
+  for (int i = 0; i < b; i++) { int n = bsize(A, i); pos += n * n; }
+  return pos;
+}
+static int bsize_bc(const arr_t *A, int b) {
+  if (A->bbe_bc.n == 0) return bsize(A, b);
+  return A->bbe_bc.a[2 * b + 1] - A->bbe_bc.a[2 * b] + 1;
+}
+
+/* CNRGmatrix::FindMatBlock  src/NRGmatrixClass.cpp:33-70: first (b1,b2) pair at an even slot */
+static int find_mat_block(const mat_t *M, int b1, int b2) {
+  for (int i = 0; i + 1 < M->map.n; i += 2)
+    if (M->map.a[i] == b1 && M->map.a[i + 1] == b2) return i / 2;
+  return -1;
+}
+static int ij2rUpTr(int n, int i, int j) { /* src/ij2r.cpp:63-68 */
+  if (i > j || i < 0 || j < 0) return 0;
+  return i * n - i * (i - 1) / 2 + (j - i);
+}
+/* GetBlockMatEl  src/NRGmatrixClass.cpp:120-150 */
+static double get_block_matel(const mat_t *M, int b1, int b2, int i, int j) {
+  int im = find_mat_block(M, b1, b2);
+  if (im < 0 || i < 0 || j < 0) return 0.0;
+  int i0 = M->mbe.a[2 * im];
+  int n1 = bsize(&M->b, b1), n2 = bsize(&M->b, b2);
+  int r = n2 * i + j; (void)n1;
+  if (M->upper && b1 == b2) r = (i > j) ? ij2rUpTr(n1, j, i) : ij2rUpTr(n1, i, j);
+  return M->mat.a[i0 + r];
+}
+/* GetMatEl  :160-170 */
+static double get_matel(const mat_t *M, int ist, int jst) {
+  int pi, pj;
+  int b1 = blk_from_st(&M->b, ist, &pi), b2 = blk_from_st(&M->b, jst, &pj);
+  return get_block_matel(M, b1, b2, pi, pj);
+}
+/* SyncNRGarray(CNRGbasisarray)  include/NRGclasses.hpp:446-456 */
+static void mat_sync(mat_t *M, const arr_t *A) {
+  M->b.nshell = A->nshell; M->b.nq = A->nq;
+  dcopy_v(&M->b.qn, &A->qn); icopy_v(&M->b.bbe, &A->bbe); icopy_v(&M->b.kept, &A->kept);
+  icopy_v(&M->b.itype, &A->itype);
+}
+static void mat_clear(mat_t *M) {  /* CNRGmatrix::ClearAll :416-425 */
+  M->b.qn.n = M->b.bbe.n = M->b.en.n = M->b.ev.n = M->b.kept.n = 0;
+  M->mat.n = M->map.n = M->mbe.n = 0;
+}
+static void mat_copy(mat_t *dst, const mat_t *src) {
+  dst->b.nshell = src->b.nshell; dst->b.nq = src->b.nq;
+  dcopy_v(&dst->b.qn, &src->b.qn); icopy_v(&dst->b.bbe, &src->b.bbe);
+  icopy_v(&dst->b.kept, &src->b.kept); icopy_v(&dst->b.itype, &src->b.itype);
+  dcopy_v(&dst->mat, &src->mat); icopy_v(&dst->map, &src->map); icopy_v(&dst->mbe, &src->mbe);
+  dst->upper = src->upper; dst->need_old = src->need_old; dst->calc_avg = src->calc_avg;
+  dst->check = src->check; dst->calc = src->calc;
+}
+/* SetDiagonalMatrix  src/NRGmatrixClass.cpp:1727-1769 */
+static void mat_set_diag(mat_t *M, double el) {
+  M->map.n = M->mbe.n = M->mat.n = 0;
+  int i1 = 0;
+  for (int b = 0; b < nblocks(&M->b); b++) {
+    ipush(&M->map, b); ipush(&M->map, b); ipush(&M->mbe, i1);
+    int n = bsize(&M->b, b);
+    for (int i = 0; i < n; i++) for (int j = 0; j < n; j++) { dpush(&M->mat, i == j ? el : 0.0); i1++; }
+    ipush(&M->mbe, i1 - 1);
+  }
+}
+/* PushBlockMatEl  src/NRGmatrixClass.cpp:267-290 (non-upper) */
+static void mat_put(mat_t *M, int b1, int b2, int i, int j, double el) {
+  int im = find_mat_block(M, b1, b2);
+  if (im < 0) return;
+  M->mat.a[M->mbe.a[2 * im] + bsize(&M->b, b2) * i + j] = el;
+}
+
+/* ------------------------------------------------------------------ */
+/* Clebsch-Gordan  src/CGordan.cpp:13-120 (Racah formula, exact factorials) */
+/* ------------------------------------------------------------------ */
+static double fact(int n) {
+  static double tab[171];
+  static int init = 0;
+  if (!init) { tab[0] = 1.0; for (int i = 1; i < 171; i++) tab[i] = tab[i - 1] * (double)i; init = 1; }
+  if (n < 0 || n > 170) return 0.0;
+  return tab[n];
+}
+static int triang(double J1, double J2, double J) {  /* TriangIneq :13-17 */
+  return !((J - (J1 + J2) > 1.0e-10) || (J - fabs(J1 - J2) < -1.0e-10));
+}
+double orc_cgordan(double J1, double M1, double J2, double M2, double J, double M) {
+  if (J1 < 0.0 || J2 < 0.0 || J < 0.0) return 0.0;
+  if (fabs(M1) > J1 || fabs(M2) > J2 || fabs(M) > J) return 0.0;
+  if (fabs(M - (M1 + M2)) > 1.0e-10) return 0.0;
+  if (J > J1 + J2 || J < fabs(J1 - J2)) return 0.0;
+  double a5 = fact((int)(J1 + J2 - J)) * fact((int)(J1 - J2 + J)) * fact((int)(-J1 + J2 + J)) *
+              (2 * J + 1) / fact((int)(J1 + J2 + J + 1));
+  double rho = a5 > 0.0 ? sqrt(a5) : 0.0;
+  double a6 = fact((int)(J + M)) * fact((int)(J - M)) * fact((int)(J1 + M1)) * fact((int)(J1 - M1)) *
+              fact((int)(J2 + M2)) * fact((int)(J2 - M2));
+  double sigma = a6 > 0.0 ? sqrt(a6) : 0.0;
+  int r1 = (int)(J1 - M1), r2 = (int)(J2 + M2), r3 = (int)(J1 + J2 - J);
+  int rmax = r1 < r2 ? r1 : r2; if (r3 < rmax) rmax = r3;
+  int m1 = -(int)(J - J2 + M1), m2 = -(int)(J - J1 - M2);
+  int rmin = m1 > m2 ? m1 : m2; if (rmin < 0) rmin = 0;
+  double tau = 0.0;
+  for (int r = rmin; r <= rmax; r++) {
+    double d = fact((int)(J1 - M1) - r) * fact((int)(J2 + M2) - r) * fact((int)(J - J2 + M1) + r) *
+               fact((int)(J - J1 - M2) + r) * fact((int)(J1 + J2 - J) - r) * fact(r);
+    tau += ((r & 1) ? -1.0 : 1.0) * (1.0 / d);
+  }
+  return rho * sigma * tau;
+}
+
+/* ------------------------------------------------------------------ */
+/* Symmetric eigensolver: the dspev_/dsyev_ calls of DiagBlock           */
+/* (src/NRGmatrixClass.cpp:566-696).  Householder reduction to           */
+/* tridiagonal form with explicit accumulation of Q, then implicit QL    */
+/* with Wilkinson shifts on (d,e) rotating the columns of Q; eigenpairs  */
+/* sorted ascending and returned as ROWS (row i = eigenvector i), which  */
+/* is the layout DiagBlock copies into dEigVec.                          */
+/* ------------------------------------------------------------------ */
+static void eigh_dense(int n, double *a /* n*n row-major, overwritten by Q */, double *w, double *z) {
+  double *d = (double *)calloc((size_t)n, sizeof(double));
+  double *e = (double *)calloc((size_t)n, sizeof(double));
+  double *v = (double *)calloc((size_t)n, sizeof(double));
+  double *p = (double *)calloc((size_t)n, sizeof(double));
+  /* Q accumulated as product H_0 H_1 ... ; store reflectors then build Q */
+  double *vs = (double *)calloc((size_t)n * (size_t)n, sizeof(double));
+  double *tau = (double *)calloc((size_t)n, sizeof(double));
+#define A(i, j) a[(size_t)(i) * n + (j)]
+  for (int k = 0; k + 2 < n; k++) {
+    /* reflector annihilating A(k+2:n, k) */
+    double nrm = 0.0;
+    for (int i = k + 1; i < n; i++) nrm += A(i, k) * A(i, k);
+    nrm = sqrt(nrm);
+    double x0 = A(k + 1, k);
+    if (nrm == 0.0) { tau[k] = 0.0; continue; }
+    double alpha = (x0 > 0) ? -nrm : nrm;
+    /* v = x - alpha e1, H = I - 2 v v^T / (v^T v) */
+    for (int i = 0; i < n; i++) v[i] = 0.0;
+    v[k + 1] = x0 - alpha;
+    for (int i = k + 2; i < n; i++) v[i] = A(i, k);
+    double vtv = 0.0;
+    for (int i = k + 1; i < n; i++) vtv += v[i] * v[i];
+    if (vtv == 0.0) { tau[k] = 0.0; continue; }
+    double t = 2.0 / vtv;
+    tau[k] = t;
+    for (int i = k + 1; i < n; i++) vs[(size_t)k * n + i] = v[i];
+    /* A <- H A H on the trailing block (rows/cols k..n-1) */
+    /* p = t * A v */
+    for (int i = k; i < n; i++) {
+      double s = 0.0;
+      for (int j = k + 1; j < n; j++) s += A(i, j) * v[j];
+      p[i] = t * s;
+    }
+    double vp = 0.0;
+    for (int i = k + 1; i < n; i++) vp += v[i] * p[i];
+    double K = 0.5 * t * vp;
+    /* q = p - K v ; A <- A - v q^T - q v^T */
+    for (int i = k; i < n; i++) p[i] -= K * v[i];
+    for (int i = k; i < n; i++)
+      for (int j = k; j < n; j++) A(i, j) -= v[i] * p[j] + p[i] * v[j];
+    /* exact zeros below the subdiagonal */
+    A(k + 1, k) = alpha; A(k, k + 1) = alpha;
+    for (int i = k + 2; i < n; i++) { A(i, k) = 0.0; A(k, i) = 0.0; }
+  }
+  for (int i = 0; i < n; i++) d[i] = A(i, i);
+  for (int i = 0; i + 1 < n; i++) e[i] = A(i + 1, i);
+  e[n - 1] = 0.0;
+  /* Q = H_0 H_1 ... H_{n-3}: apply in reverse to the identity */
+  for (int i = 0; i < n; i++) for (int j = 0; j < n; j++) A(i, j) = (i == j) ? 1.0 : 0.0;
+  for (int k = n - 3; k >= 0; k--) {
+    if (tau[k] == 0.0) continue;
+    const double *vk = vs + (size_t)k * n;
+    for (int j = k + 1; j < n; j++) {
+      double s = 0.0;
+      for (int i = k + 1; i < n; i++) s += vk[i] * A(i, j);
+      s *= tau[k];
+      for (int i = k + 1; i < n; i++) A(i, j) -= s * vk[i];
+    }
+  }
+  /* implicit QL on (d,e); rotations applied to columns of Q (=A) */
+  for (int l = 0; l < n; l++) {
+    int iter = 0;
+    for (;;) {
+      int m;
+      for (m = l; m + 1 < n; m++) {
+        double dd = fabs(d[m]) + fabs(d[m + 1]);
+        if (fabs(e[m]) <= 1e-300 || fabs(e[m]) + dd == dd) break;
+      }
+      if (m == l) break;
+      if (++iter > 60) break;
+      double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+      double r = hypot(g, 1.0);
+      g = d[m] - d[l] + e[l] / (g + (g >= 0 ? fabs(r) : -fabs(r)));
+      double s = 1.0, c = 1.0, pp = 0.0;
+      int i;
+      for (i = m - 1; i >= l; i--) {
+        double f = s * e[i], b = c * e[i];
+        r = hypot(f, g);
+        e[i + 1] = r;
+        if (r == 0.0) { d[i + 1] -= pp; e[m] = 0.0; break; }
+        s = f / r; c = g / r;
+        g = d[i + 1] - pp;
+        r = (d[i] - g) * s + 2.0 * c * b;
+        pp = s * r;
+        d[i + 1] = g + pp;
+        g = c * r - b;
+        for (int k = 0; k < n; k++) {
+          double t1 = A(k, i + 1);
+          A(k, i + 1) = s * A(k, i) + c * t1;
+          A(k, i) = c * A(k, i) - s * t1;
+        }
+      }
+      if (r == 0.0 && i >= l) continue;
+      d[l] -= pp; e[l] = g; e[m] = 0.0;
+    }
+  }
+  /* sort ascending (stable selection) and emit rows */
+  int *idx = (int *)malloc(sizeof(int) * (size_t)n);
+  for (int i = 0; i < n; i++) idx[i] = i;
+  for (int i = 0; i < n; i++) {
+    int kmin = i;
+    for (int j = i + 1; j < n; j++) if (d[idx[j]] < d[idx[kmin]]) kmin = j;
+    int t = idx[i]; idx[i] = idx[kmin]; idx[kmin] = t;
+  }
+  for (int i = 0; i < n; i++) {
+    w[i] = d[idx[i]];
+    for (int k = 0; k < n; k++) z[(size_t)i * n + k] = A(k, idx[i]);
+  }
+#undef A
+  free(idx); free(d); free(e); free(v); free(p); free(vs); free(tau);
+}
+
+void orc_eigh_full(int n, const double *a, double *w, double *z) {
+  if (n <= 0) return;
+  double *t = (double *)malloc(sizeof(double) * (size_t)n * (size_t)n);
+  memcpy(t, a, sizeof(double) * (size_t)n * (size_t)n);
+  eigh_dense(n, t, w, z);
+  free(t);
+}
+void orc_eigh_packed(int n, const double *ap, double *w, double *z) {
+  if (n <= 0) return;
+  double *t = (double *)malloc(sizeof(double) * (size_t)n * (size_t)n);
+  for (int i = 0; i < n; i++)
+    for (int j = i; j < n; j++) { double x = ap[ij2rUpTr(n, i, j)]; t[(size_t)i * n + j] = x; t[(size_t)j * n + i] = x; }
+  eigh_dense(n, t, w, z);
+  free(t);
+}
+
+/* ------------------------------------------------------------------ */
+/* SetE0zero / SetDegen / Ecut / SetKept  src/NRGarrayClass.cpp:234-491, 661-691 */
+/* ------------------------------------------------------------------ */
+static double set_e0zero(arr_t *A) {
+  double emin = A->en.a[0];
+  for (int i = 1; i < A->en.n; i++) if (A->en.a[i] < emin) emin = A->en.a[i];
+  for (int i = 0; i < A->en.n; i++) A->en.a[i] -= emin;
+  return emin;
+}
+void orc_setdegen(int n, double *en, double myeps) {
+  for (int ii = 0; ii < n; ii++)
+    for (int jj = ii + 1; jj < n; jj++)
+      if (fabs(en[jj] - en[ii]) < myeps) {
+        if (en[jj] > en[ii]) en[jj] = en[ii]; else en[ii] = en[jj];
+      }
+}
+static int cmpd(const void *x, const void *y) {
+  double a = *(const double *)x, b = *(const double *)y;
+  return (a > b) - (a < b);
+}
+/* The 1% window loop reads Eaux[Ncut+ii] with the bound ii<size only
+ * (:682-684), i.e. possibly past the end.  Here an index past the end stops
+ * the window as if a level far above Ecut were there, which keeps every state. */
+double orc_ecut(int n, const double *en, int ncut) {
+  double *e = (double *)malloc(sizeof(double) * (size_t)n);
+  memcpy(e, en, sizeof(double) * (size_t)n);
+  qsort(e, (size_t)n, sizeof(double), cmpd);
+  double aux;
+  if (ncut >= n - 1) aux = e[n - 1];
+  else {
+    aux = e[ncut - 1];
+    double aux2 = e[ncut];
+    int ii = 1;
+    while (aux2 > 0.0 && fabs((aux2 - aux) / aux) < 0.01 && ii < n) {
+      if (ncut + ii >= n) { aux2 = HUGE_VAL; ii++; break; }
+      aux2 = e[ncut + ii]; ii++;
+    }
+    if (ii > 1) aux = (ncut + ii - 2 < n) ? e[ncut + ii - 2] : e[n - 1];
+  }
+  free(e);
+  return aux;
+}
+static double set_kept(arr_t *A, int ncut, double myeps, int *nkept) {
+  orc_setdegen(A->en.n, A->en.a, myeps);
+  double ec = orc_ecut(A->en.n, A->en.a, ncut);
+  A->kept.n = 0;
+  int nk = 0;
+  for (int i = 0; i < A->en.n; i++) { int k = A->en.a[i] <= ec; ipush(&A->kept, k); nk += k; }
+  if (nkept) *nkept = nk;
+  return ec;
+}
+
+/* ------------------------------------------------------------------ */
+/* CutStates  src/CutStates.cpp:5-96 with RemoveBlock / RemoveStatesFromBlock */
+/* (src/NRGarrayClass.cpp:1307-1548)                                    */
+/* ------------------------------------------------------------------ */
+static void set_bbe_ev(arr_t *A) {  /* SetBlockBegEndEigVec :1285-1305 */
+  A->bbe_ev.n = 0;
+  int nb = A->bbe_bc.n / 2, aux = 0;
+  for (int b = 0; b < nb; b++) {
+    ipush(&A->bbe_ev, aux);
+    int n = bsize_bc(A, b);
+    aux += n * n - 1;
+    ipush(&A->bbe_ev, aux);
+    aux++;
+  }
+}
+static void remove_block(arr_t *A, int ib) {
+  int b0 = blim(A, ib, 0), b1 = blim(A, ib, 1);
+  derase(&A->en, b0, b1 + 1);
+  ierase(&A->stcf, b0, b1 + 1);
+  ierase(&A->kept, b0, b1 + 1);
+  derase(&A->ev, A->bbe_ev.a[2 * ib], A->bbe_ev.a[2 * ib + 1] + 1);
+  derase(&A->qn, A->nq * ib, A->nq * (ib + 1));
+  int nb = nblocks(A);
+  for (int b = ib; b < nb; b++) {
+    int sn = A->bbe.a[2 * (b + 1) + 1] - A->bbe.a[2 * (b + 1)] + 1;
+    A->bbe.a[2 * b + 1] = A->bbe.a[2 * b] + sn - 1;
+    A->bbe.a[2 * b + 2] = A->bbe.a[2 * b] + sn;
+    int bn = A->bbe_bc.a[2 * (b + 1) + 1] - A->bbe_bc.a[2 * (b + 1)] + 1; /* BC size of next */
+    (void)bn;
+    int sn2 = A->bbe_ev.a[2 * (b + 1) + 1] - A->bbe_ev.a[2 * (b + 1)] + 1;
+    /* The reference recomputes the eigvec range from Size_next^2 (:1340-1344):
+       at cut time every block is still square, so the two agree. */
+    A->bbe_ev.a[2 * b + 1] = A->bbe_ev.a[2 * b] + sn2 - 1;
+    A->bbe_ev.a[2 * b + 2] = A->bbe_ev.a[2 * b] + sn2;
+  }
+  A->bbe.n -= 2; A->bbe_ev.n -= 2;
+  ierase(&A->bbe_bc, 2 * ib, 2 * (ib + 1));
+}
+static void remove_states_from_block(arr_t *A, int ib, int ist1, int ist2) {
+  int ngone = ist2 - ist1 + 1;
+  int p0, p1;
+  if (blk_from_st(A, ist1, &p0) != ib || blk_from_st(A, ist2, &p1) != ib) return;
+  int nbc = bsize_bc(A, ib);
+  if (A->bbe.a[2 * ib] == A->bbe.a[2 * ib + 1]) { remove_block(A, ib); return; }
+  derase(&A->en, ist1, ist2 + 1);
+  ierase(&A->stcf, ist1, ist2 + 1);
+  ierase(&A->kept, ist1, ist2 + 1);
+  int off = A->bbe_ev.a[2 * ib] + nbc * p0;
+  derase(&A->ev, off, off + nbc * ngone);
+  A->bbe.a[2 * ib + 1] -= ngone;
+  A->bbe_ev.a[2 * ib + 1] -= nbc * ngone;
+  for (int b = ib + 1; b < nblocks(A); b++) {
+    A->bbe.a[2 * b] -= ngone; A->bbe.a[2 * b + 1] -= ngone;
+    A->bbe_ev.a[2 * b] -= nbc * ngone; A->bbe_ev.a[2 * b + 1] -= nbc * ngone;
+  }
+}
+static double cut_states(const arr_t *Ain, arr_t *Ac, int ncut, double myeps) {
+  arr_clear(Ac);
+  Ac->nshell = Ain->nshell; Ac->nq = Ain->nq;
+  dcopy_v(&Ac->qn, &Ain->qn); dcopy_v(&Ac->en, &Ain->en); dcopy_v(&Ac->ev, &Ain->ev);
+  icopy_v(&Ac->kept, &Ain->kept); icopy_v(&Ac->bbe, &Ain->bbe);
+  Ac->stcf.n = 0;
+  for (int i = 0; i < Ac->en.n; i++) ipush(&Ac->stcf, i);
+  icopy_v(&Ac->bbe_bc, &Ain->bbe);
+  set_bbe_ev(Ac);
+  if (ncut >= nstates(Ain)) return HUGE_VAL;
+  double ec = orc_ecut(Ain->en.n, Ain->en.a, ncut);
+  set_kept(Ac, ncut, myeps, NULL);
+  for (int b = 0; b < nblocks(Ac); b++) {
+    int b0 = blim(Ac, b, 0), b1 = blim(Ac, b, 1);
+    double emin = Ac->en.a[b0];
+    for (int i = b0; i <= b1; i++) if (Ac->en.a[i] < emin) emin = Ac->en.a[i];
+    if (emin > ec) { remove_block(Ac, b); b--; }
+  }
+  for (int b = 0; b < nblocks(Ac); b++) {
+    int b0 = blim(Ac, b, 0), b1 = blim(Ac, b, 1);
+    double emax = Ac->en.a[b0];
+    for (int i = b0; i <= b1; i++) if (Ac->en.a[i] > emax) emax = Ac->en.a[i];
+    if (emax > ec) {
+      int ip = b0;  /* GetBlockEcutPos :450-459 */
+      while (ip <= b1 && Ac->en.a[ip] <= ec) ip++;
+      remove_states_from_block(Ac, b, ip, b1);
+    }
+  }
+  return ec;
+}
+
+/* ------------------------------------------------------------------ */
+/* OneChQS model plugin                                                 */
+/* ------------------------------------------------------------------ */
+/* OneChQS_SetSingleSite  src/OneChQS_SetSingleSite.cpp */
+static void onechqs_site(arr_t *S) {
+  arr_clear(S);
+  S->nq = 3; S->nshell = 0;
+  const double q[12] = {-1, 0, 0, 0, 0.5, 0.5, 0, 0.5, -0.5, 1, 0, 0};
+  for (int i = 0; i < 12; i++) dpush(&S->qn, q[i]);
+  for (int i = 0; i < 4; i++) { ipush(&S->bbe, i); ipush(&S->bbe, i); ipush(&S->itype, i); }
+}
+/* OneCh_fd_table  src/OpTables.cpp:8-33 */
+static double onech_fd(int sigma, int ti, int tj) {
+  if (sigma == 1) {
+    if (ti == 1 && tj == 0) return 1.0;
+    if (ti == 3 && tj == 2) return 1.0;
+    return 0.0;
+  }
+  if (sigma == -1) {
+    if (ti == 2 && tj == 0) return 1.0;
+    if (ti == 3 && tj == 1) return -1.0;
+    return 0.0;
+  }
+  return 0.0;
+}
+/* Diag_check  src/OpMatRules.cpp:12-24 */
+static int diag_check(const arr_t *A, int b1, int b2) { (void)A; return b1 == b2; }
+/* OneChQS_cd_check  src/OpMatRules.cpp:106-125 */
+static int onechqs_cd_check(const arr_t *A, int b1, int b2) {
+  double Qi = getqn(A, b1, 0), Si = getqn(A, b1, 1), Qj = getqn(A, b2, 0), Sj = getqn(A, b2, 1);
+  return dEqual(Qj, Qi + 1.0) && (dEqual(Sj, Si + 0.5) || dEqual(Sj, Si - 0.5));
+}
+/* ImpOnly_MatEl  src/OpMatRules.cpp:265-280 */
+static double imponly_matel(const arr_t *B, const arr_t *S, int i, int j) {
+  (void)S; return B->itype.a[i] == B->itype.a[j] ? 1.0 : 0.0;
+}
+/* OneChQS_fN_MatEl  src/OpMatRules.cpp:1167-1247 */
+static double onechqs_fn_matel(const arr_t *B, const arr_t *S, int ist, int jst) {
+  double Si = qn_from_st(B, ist, 1), Sj = qn_from_st(B, jst, 1);
+  int ti = B->itype.a[ist], tj = B->itype.a[jst];
+  if (B->stcf.a[ist] != B->stcf.a[jst]) return 0.0;
+  double Qti = getqn(S, ti, 0), Sti = getqn(S, ti, 1), Szti = getqn(S, ti, 2);
+  double Qtj = getqn(S, tj, 0), Stj = getqn(S, tj, 1), Sztj = getqn(S, tj, 2);
+  double qi[3] = {Qti, Sti, Szti}, qj[3] = {Qtj, Stj, Sztj};
+  double Scfi = Si - Szti, Scfj = Sj - Sztj;
+  double acc = 0.0, cg1 = 0.0;
+  double sigma = Sj - Si;
+  for (double sz = -Stj; sz <= Stj; sz += 1.0) {
+    qj[2] = sz;
+    int ssj = blk_from_qn(S, qj);
+    qi[2] = sz - sigma;
+    int ssi = blk_from_qn(S, qi);
+    double cg0 = orc_cgordan(Scfi, Si - sz + sigma, Sti, sz - sigma, Si, Si);
+    cg1 = orc_cgordan(Scfj, Sj - sz, Stj, sz, Sj, Sj);
+    acc += onech_fd((int)(sigma / 0.5), ssj, ssi) * cg0 * cg1;
+  }
+  double cgr = orc_cgordan(Si, Si, 0.5, sigma, Sj, Sj);
+  if (!dEqual(cgr, 0.0)) return acc / cgr;
+  return 0.0;  /* auxBasis[1] is never accumulated (i1 resets to 0 each pass) */
+}
+/* OneChQS_cd_MatEl  src/OpMatRules.cpp:1249-1332 */
+static double onechqs_cd_matel(const arr_t *B, const arr_t *S, int ist, int jst) {
+  double Si = qn_from_st(B, ist, 1), Sj = qn_from_st(B, jst, 1);
+  int ti = B->itype.a[ist], tj = B->itype.a[jst];
+  double Qti = getqn(S, ti, 0), Sti = getqn(S, ti, 1), Szti = getqn(S, ti, 2);
+  double Qtj = getqn(S, tj, 0), Stj = getqn(S, tj, 1), Sztj = getqn(S, tj, 2);
+  double qi[3] = {Qti, Sti, Szti}, qj[3] = {Qtj, Stj, Sztj};
+  double Scfi = Si - Szti, Scfj = Sj - Sztj;
+  double sigma = Sj - Si;
+  double cg3 = orc_cgordan(Si, Si, 0.5, sigma, Sj, Sj);
+  if (dEqual(cg3, 0.0)) {
+    sigma = Si - Sj;
+    cg3 = orc_cgordan(Si, Si, 0.5, sigma, Sj, Sj);
+    if (dEqual(cg3, 0.0)) return 0.0;
+  }
+  double fb = 0.0;
+  for (double szcfj = -Scfj; szcfj <= Scfj; szcfj += 1.0) {
+    double szcfi = szcfj - sigma;
+    double sztj = Sj - szcfj, szti = Si - szcfi;
+    qj[2] = sztj; int ssj = blk_from_qn(S, qj);
+    qi[2] = szti; int ssi = blk_from_qn(S, qi);
+    double c0 = orc_cgordan(Scfi, szcfi, Sti, szti, Si, Si);
+    double c1 = orc_cgordan(Scfj, szcfj, Stj, sztj, Sj, Sj);
+    double c2 = orc_cgordan(Scfi, szcfi, 0.5, sigma, Scfj, szcfj);
+    double fs = dEqual(Qtj, 0.0) ? -1.0 : 1.0;
+    if (ssi == ssj) fb += c0 * c1 * c2 * fs;
+  }
+  if (fabs(cg3) > 1e-20) return fb / cg3;
+  return 0.0;
+}
+/* OneChQS_HN_MatEl  src/OpMatRules.cpp:1410-1535 */
+static double onechqs_hn_matel(const double *par, const arr_t *B, const arr_t *S, const mat_t *f, int ist, int jst) {
+  double Lambda = par[0], chi = par[1], eN = par[2];
+  double Qi = qn_from_st(B, ist, 0), Si = qn_from_st(B, ist, 1);
+  double Qj = qn_from_st(B, jst, 0), Sj = qn_from_st(B, jst, 1);
+  if (fabs(Qi - Qj) > 1e-20 || fabs(Si - Sj) > 1e-20) return 0.0;
+  if (ist == jst) {
+    double m = sqrt(Lambda) * B->en.a[ist];
+    double Qt = qn_from_st(S, B->itype.a[ist], 0);
+    return m + eN * (Qt + 1.0);
+  }
+  int ti = B->itype.a[ist], tj = B->itype.a[jst];
+  double old = get_matel(f, B->stcf.a[ist], B->stcf.a[jst]);
+  int typep = ti, type = tj;
+  if (dEqual(old, 0.0)) {
+    old = get_matel(f, B->stcf.a[jst], B->stcf.a[ist]);
+    typep = tj; type = ti;
+  }
+  double Qtp = getqn(S, typep, 0), Stp = getqn(S, typep, 1), Sztp = getqn(S, typep, 2);
+  double Qt = getqn(S, type, 0), St = getqn(S, type, 1), Szt = getqn(S, type, 2);
+  double Scfp = Si - Sztp, Scf = Si - Szt;
+  double fsign = dEqual(Qt, 0.0) ? -1.0 : 1.0;
+  double coef = 0.0;
+  double qp[3] = {Qtp, Stp, Sztp}, qq[3] = {Qt, St, Szt};
+  for (double sz = -St; sz <= St; sz += 1.0) {
+    qq[2] = sz;
+    int ss = blk_from_qn(S, qq);
+    for (int sg = -1; sg <= 1; sg += 2) {
+      double ds = 0.5 * sg;
+      double szcf = Si - sz, szcfp = szcf - ds, sztp = sz + ds;
+      qp[2] = sz + ds;
+      int ssp = blk_from_qn(S, qp);
+      double c0 = orc_cgordan(Scf, szcf, St, sz, Si, Si);
+      double c1 = orc_cgordan(Scfp, szcfp, Stp, sztp, Si, Si);
+      double c2 = orc_cgordan(Scfp, szcfp, 0.5, ds, Scf, szcf);
+      double c3 = onech_fd(sg, ssp, ss);
+      coef += c0 * c1 * c2 * c3;
+    }
+  }
+  return chi * old * coef * fsign;
+}
+
+/* ------------------------------------------------------------------ */
+/* BuildBasis  src/BuildBasis.cpp:8-432 (CommonQNs={2,0,1,0,1}, totSpos={1}) */
+/* ------------------------------------------------------------------ */
+static void filter_qnumbers(arr_t *A) {  /* FilterQNumbers src/NRGarrayClass.cpp:271-309 */
+  int nq = A->nq;
+  for (int b = 0; b < A->qn.n / nq; b++) {
+    for (int c = b + 1; c < A->qn.n / nq;) {
+      int eq = 1;
+      for (int i = 0; i < nq; i++) if (A->qn.a[c * nq + i] != A->qn.a[b * nq + i]) { eq = 0; break; }
+      if (eq) derase(&A->qn, c * nq, (c + 1) * nq); else c++;
+    }
+  }
+}
+static void build_basis_qs(const arr_t *Ac, arr_t *Ab, const arr_t *S) {
+  /* Q adds, S couples (TotalS on position 1); site QN positions {0,1}, Sz at 2 */
+  Ab->nshell = Ac->nshell + 1;
+  Ab->nq = 2;
+  arr_clear(Ab);
+  for (int b = 0; b < nblocks(Ac); b++) {
+    double Qo = getqn(Ac, b, 0), So = getqn(Ac, b, 1);
+    for (int s = 0; s < nblocks(S); s++) {
+      double Qs = getqn(S, s, 0), Ss = getqn(S, s, 1);
+      dpush(&Ab->qn, Qo + Qs); dpush(&Ab->qn, So + Ss);
+      for (double sn = So + Ss - 1.0; sn >= fabs(So - Ss); sn -= 1.0) {
+        dpush(&Ab->qn, Qo + Qs); dpush(&Ab->qn, sn);
+      }
+    }
+  }
+  filter_qnumbers(Ab);
+  int istnew = 0;
+  for (int bn = 0; bn < nblocks(Ab); bn++) {
+    double Qn = getqn(Ab, bn, 0), Sn = getqn(Ab, bn, 1);
+    ipush(&Ab->bbe, istnew);
+    for (int bo = 0; bo < nblocks(Ac); bo++) {
+      double Qo = getqn(Ac, bo, 0), So = getqn(Ac, bo, 1);
+      for (int s = 0; s < nblocks(S); s++) {
+        double Qs = getqn(S, s, 0), Ss = getqn(S, s, 1), Szs = getqn(S, s, 2);
+        int ok = dEqual(Qn, Qs + Qo);
+        if (!dEqual(Sn, Szs + So) || !triang(So, Ss, Sn)) ok = 0;
+        if (!ok) continue;
+        for (int io = blim(Ac, bo, 0); io <= blim(Ac, bo, 1); io++)
+          for (int is = blim(S, s, 0); is <= blim(S, s, 1); is++) {
+            ipush(&Ab->itype, S->itype.a[is]);
+            dpush(&Ab->en, Ac->en.a[io]);
+            ipush(&Ab->stcf, io);
+            istnew++;
+          }
+      }
+    }
+    ipush(&Ab->bbe, istnew - 1);
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* DiagHN  src/DiagHN.cpp:6-119                                         */
+/* ------------------------------------------------------------------ */
+static double diag_hn(const double *par, const arr_t *Ab, const arr_t *S, const mat_t *f, arr_t *Ae,
+                      dvec *hrec) {
+  Ae->nshell = Ab->nshell; Ae->nq = Ab->nq;
+  arr_clear(Ae);
+  dcopy_v(&Ae->qn, &Ab->qn); icopy_v(&Ae->bbe, &Ab->bbe);
+  if (hrec) hrec->n = 0;
+  for (int b = 0; b < nblocks(Ab); b++) {
+    int n = bsize(Ab, b), b0 = blim(Ab, b, 0);
+    double *ap = (double *)malloc(sizeof(double) * (size_t)n * (size_t)(n + 1) / 2);
+    int r = 0;
+    for (int i = 0; i < n; i++)
+      for (int j = i; j < n; j++) ap[r++] = onechqs_hn_matel(par, Ab, S, f, b0 + i, b0 + j);
+    if (hrec) for (int k = 0; k < r; k++) dpush(hrec, ap[k]);
+    dres(&Ae->en, Ae->en.n + n);
+    dres(&Ae->ev, Ae->ev.n + n * n);
+    orc_eigh_packed(n, ap, Ae->en.a + Ae->en.n, Ae->ev.a + Ae->ev.n);
+    Ae->en.n += n; Ae->ev.n += n * n;
+    free(ap);
+  }
+  return set_e0zero(Ae);
+}
+
+/* ------------------------------------------------------------------ */
+/* UpdateMatrices  src/UpdateMatrices.cpp:272-568                       */
+/* ------------------------------------------------------------------ */
+static void gemm_nt(int m, int n, int k, const double *a, const double *b, double *c) {
+  /* c(m x n) = a(m x k) * b(n x k)^T */
+  for (int i = 0; i < m; i++)
+    for (int j = 0; j < n; j++) {
+      double s = 0.0;
+      for (int l = 0; l < k; l++) s += a[(size_t)i * k + l] * b[(size_t)j * k + l];
+      c[(size_t)i * n + j] = s;
+    }
+}
+static void gemm_nn(int m, int n, int k, const double *a, const double *b, double *c) {
+  for (int i = 0; i < m; i++) {
+    for (int j = 0; j < n; j++) c[(size_t)i * n + j] = 0.0;
+    for (int l = 0; l < k; l++) {
+      double x = a[(size_t)i * k + l];
+      for (int j = 0; j < n; j++) c[(size_t)i * n + j] += x * b[(size_t)l * n + j];
+    }
+  }
+}
+void orc_rotate(int ni, int nj, int ki, int kj, const double *zi, const double *f, const double *zj,
+                double *out) {
+  double *aux = (double *)malloc(sizeof(double) * (size_t)ni * (size_t)(kj > 0 ? kj : 1));
+  gemm_nt(ni, kj, nj, f, zj, aux);
+  gemm_nn(ki, kj, ni, zi, aux, out);
+  free(aux);
+}
+static int find_match_block(const arr_t *Ac, int b, const arr_t *Ab) {  /* src/FindMatchBlock.cpp:5-24 */
+  double q[8];
+  for (int i = 0; i < Ac->nq; i++) q[i] = getqn(Ac, b, i);
+  return blk_from_qn(Ab, q);
+}
+static void update_matrices(const arr_t *S, const arr_t *Ac, const arr_t *Ab, mat_t *mats, int nm) {
+  mat_t *old = (mat_t *)calloc((size_t)nm, sizeof(mat_t));
+  for (int m = 0; m < nm; m++) {
+    if (mats[m].need_old) mat_copy(&old[m], &mats[m]);
+    mat_clear(&mats[m]);
+    mat_sync(&mats[m], Ac);
+  }
+  int *icount = (int *)calloc((size_t)nm, sizeof(int));
+  for (int ib = 0; ib < nblocks(Ac); ib++) {
+    int ni = bsize(Ac, ib);
+    int ibc = find_match_block(Ac, ib, Ab);
+    int nibc = bsize(Ab, ibc);
+    for (int jb = 0; jb < nblocks(Ac); jb++) {
+      int nj = bsize(Ac, jb);
+      int jbc = find_match_block(Ac, jb, Ab);
+      int njbc = bsize(Ab, jbc);
+      for (int m = 0; m < nm; m++) {
+        if (!mats[m].check(Ac, ib, jb)) continue;
+        ipush(&mats[m].map, ib); ipush(&mats[m].map, jb);
+        ipush(&mats[m].mbe, icount[m]);
+        icount[m] += ni * nj;
+        ipush(&mats[m].mbe, icount[m] - 1);
+        int zi = blim_ev(Ac, ib), zj = blim_ev(Ac, jb);
+        double *F = (double *)malloc(sizeof(double) * (size_t)nibc * (size_t)njbc);
+        int r = 0;
+        for (int ist = blim(Ab, ibc, 0); ist <= blim(Ab, ibc, 1); ist++)
+          for (int jst = blim(Ab, jbc, 0); jst <= blim(Ab, jbc, 1); jst++) {
+            double el = mats[m].calc(Ab, S, ist, jst);
+            if (mats[m].need_old) el *= get_matel(&old[m], Ab->stcf.a[ist], Ab->stcf.a[jst]);
+            F[r++] = el;
+          }
+        dres(&mats[m].mat, mats[m].mat.n + ni * nj);
+        orc_rotate(nibc, njbc, ni, nj, Ac->ev.a + zi, F, Ac->ev.a + zj, mats[m].mat.a + mats[m].mat.n);
+        mats[m].mat.n += ni * nj;
+        free(F);
+      }
+    }
+  }
+  for (int m = 0; m < nm; m++) mat_free(&old[m]);
+  free(old); free(icount);
+}
+
+/* ------------------------------------------------------------------ */
+/* Thermodynamics  src/CalcThermo.cpp:16-121, src/CalcOpAvg.cpp:13-84    */
+/* ------------------------------------------------------------------ */
+static double calc_suscep(double bb, const arr_t *A) {
+  double s0 = 0, s1 = 0;
+  for (int b = 0; b < nblocks(A); b++) {
+    double S = getqn(A, b, 1);
+    double aS = 2.0 * S + 1.0, szsq = (1.0 / 12.0) * aS * (aS * aS - 1.0);
+    for (int i = blim(A, b, 0); i <= blim(A, b, 1); i++) {
+      double x = exp(-bb * A->en.a[i]);
+      s0 += aS * x; s1 += szsq * x;
+    }
+  }
+  return s1 / s0;
+}
+static double calc_entropy(double bb, const arr_t *A) {
+  double Z = 0, s = 0;
+  for (int b = 0; b < nblocks(A); b++) {
+    double dg = 2.0 * getqn(A, b, 1) + 1.0;
+    for (int i = blim(A, b, 0); i <= blim(A, b, 1); i++) {
+      double E = A->en.a[i], x = exp(-bb * E);
+      s += dg * E * x; Z += dg * x;
+    }
+  }
+  return bb * (s / Z) + log(Z);
+}
+static double calc_opavg(double bb, const arr_t *A, const mat_t *op) {
+  double s = 0, Z = 0;
+  for (int b = 0; b < nblocks(A); b++) {
+    double dg = 2.0 * getqn(A, b, 1) + 1.0;
+    for (int i = blim(A, b, 0); i <= blim(A, b, 1); i++) {
+      double x = exp(-bb * A->en.a[i]);
+      s += dg * get_matel(op, i, i) * x; Z += dg * x;
+    }
+  }
+  if (fabs(Z) > 1e-20) s /= Z;
+  return s;
+}
+
+/* ------------------------------------------------------------------ */
+/* chain  src/NRGchainClass.cpp:132-155                                 */
+/* ------------------------------------------------------------------ */
+double orc_chin(int n, double L) {
+  double d0 = 1.0 - pow(L, -((double)n + 1.0));
+  double d1 = sqrt(1.0 - pow(L, -(2.0 * (double)n + 1.0)));
+  double d2 = sqrt(1.0 - pow(L, -(2.0 * (double)n + 3.0)));
+  return d0 / (d1 * d2);
+}
+
+/* ------------------------------------------------------------------ */
+/* NRG_main loop  src/Main/NRG_main.cpp:45-427                          */
+/* ------------------------------------------------------------------ */
+static void emit(orc_step_cb cb, void *user, const arr_t *Ae, const arr_t *Ac, double emin, double ecut,
+                 int nkept, const dvec *hrec, int rec, const double *thT, const double *sus,
+                 const double *ent, double Tavg, int nops, const double *ops) {
+  if (!cb) return;
+  orc_step st;
+  memset(&st, 0, sizeof(st));
+  st.nshell = Ae->nshell; st.nq = Ae->nq; st.nblocks = nblocks(Ae);
+  st.qn = Ae->qn.a; st.bbe = Ae->bbe.a; st.nstates = Ae->en.n; st.en = Ae->en.a;
+  unsigned char *kp = (unsigned char *)malloc((size_t)(Ae->en.n + 1));
+  for (int i = 0; i < Ae->en.n; i++) kp[i] = (i < Ae->kept.n) ? (unsigned char)Ae->kept.a[i] : 1;
+  st.kept = kp;
+  st.emin = emin; st.ecut = ecut; st.nkept = nkept;
+  st.hpacked = (rec && hrec) ? hrec->a : NULL;
+  st.evec = rec ? Ae->ev.a : NULL;
+  for (int i = 0; i < 4; i++) { st.thermo_T[i] = thT ? thT[i] : 0; st.suscep[i] = sus ? sus[i] : 0; st.entropy[i] = ent ? ent[i] : 0; }
+  st.T_avg = Tavg; st.nops_avg = nops;
+  for (int i = 0; i < nops && i < 8; i++) st.opavg[i] = ops[i];
+  st.cut_nblocks = nblocks(Ac); st.cut_qn = Ac->qn.a; st.cut_bbe = Ac->bbe.a; st.cut_bbe_bc = Ac->bbe_bc.a;
+  cb(user, &st);
+  free(kp);
+}
+
+int orc_run_onechqs(const orc_onechqs_params *p, orc_step_cb cb, void *user) {
+  double Lambda = p->Lambda, bbar = p->betabar > 0 ? p->betabar : 0.727;
+  double myeps = p->myeps > 0 ? p->myeps : 1e-18;
+  double HLF = 0.5 * (1.0 + 1.0 / Lambda);
+  arr_t S = {0}, Ae = {0}, Ac = {0}, Ab = {0};
+  onechqs_site(&S);
+  /* operators: [0] f_N; calcdens 0/2 -> [1..3] n_d, n_d^2, S_d^2; calcdens 1 -> [1] c_d */
+  int nm = (p->calcdens == 1) ? 2 : 4;
+  mat_t mats[4];
+  memset(mats, 0, sizeof(mats));
+  mats[0].need_old = 0; mats[0].upper = 0; mats[0].check = onechqs_cd_check; mats[0].calc = onechqs_fn_matel;
+  for (int m = 1; m < nm; m++) {
+    mats[m].need_old = 1; mats[m].upper = 0;
+    if (p->calcdens == 1) { mats[m].check = onechqs_cd_check; mats[m].calc = onechqs_cd_matel; }
+    else { mats[m].check = diag_check; mats[m].calc = imponly_matel; mats[m].calc_avg = 1; }
+  }
+  /* OneChQS_SetAnderson_Hm1  src/Main/Main_SetH0.cpp:1794-1969 */
+  double chi_m1 = sqrt(2.0 * p->Gamma / M_PI) / (sqrt(Lambda) * HLF);
+  if (p->disc_scheme != 1) chi_m1 *= sqrt(0.5 * log(Lambda) * (Lambda + 1) / (Lambda - 1));
+  Ae.nq = 2; Ae.nshell = -1;
+  const double q0[6] = {-1, 0, 0, 0.5, 1, 0};
+  for (int i = 0; i < 6; i++) dpush(&Ae.qn, q0[i]);
+  for (int i = 0; i < 3; i++) { ipush(&Ae.bbe, i); ipush(&Ae.bbe, i); dpush(&Ae.ev, 1.0); }
+  dpush(&Ae.en, 0.5 * p->U / (Lambda * HLF));
+  dpush(&Ae.en, (p->ed + 0.5 * p->U) / (Lambda * HLF));
+  dpush(&Ae.en, (2.0 * p->ed + 1.5 * p->U) / (Lambda * HLF));
+  for (int m = 0; m < nm; m++) { mats[m].b.nshell = -1; mat_sync(&mats[m], &Ae); }
+  dpush(&mats[0].mat, 1.0); ipush(&mats[0].map, 0); ipush(&mats[0].map, 1); ipush(&mats[0].mbe, 0); ipush(&mats[0].mbe, 0);
+  dpush(&mats[0].mat, -sqrt(2.0)); ipush(&mats[0].map, 1); ipush(&mats[0].map, 2); ipush(&mats[0].mbe, 1); ipush(&mats[0].mbe, 1);
+  if (p->calcdens == 1) {
+    mat_t tmp = {0}; mat_copy(&tmp, &mats[0]);
+    tmp.need_old = 1; tmp.check = onechqs_cd_check; tmp.calc = onechqs_cd_matel;
+    mat_copy(&mats[1], &tmp); mat_free(&tmp);
+  } else {
+    for (int m = 1; m < nm; m++) mat_set_diag(&mats[m], 0.0);
+    for (int b = 0; b < 3; b++) {
+      double nd = getqn(&Ae, b, 0) + 1.0, Sd = getqn(&Ae, b, 1);
+      mat_put(&mats[1], b, b, 0, 0, nd);
+      mat_put(&mats[2], b, b, 0, 0, nd * nd);
+      mat_put(&mats[3], b, b, 0, 0, Sd * (Sd + 1.0));
+    }
+  }
+  cut_states(&Ae, &Ac, p->ncut, myeps);
+  double DN = HLF * pow(Lambda, -(0 - 2) / 2.0);
+  {
+    double ops[3] = {0, 0, 0};
+    int nops = 0;
+    if (p->calcdens != 1) for (int m = 1; m < nm; m++) ops[nops++] = calc_opavg(bbar, &Ac, &mats[m]);
+    emit(cb, user, &Ae, &Ac, 0.0, HUGE_VAL, Ae.en.n, NULL, 0, NULL, NULL, NULL, DN / bbar, nops, ops);
+  }
+  double par[4] = {Lambda, chi_m1, 0.0, 0.0};
+  dvec hrec = {0};
+  for (int N = 0; N < p->nsitesmax; N++) {
+    DN = HLF * pow(Lambda, -(N - 1) / 2.0);
+    build_basis_qs(&Ac, &Ab, &S);
+    double emin = diag_hn(par, &Ab, &S, &mats[0], &Ae, p->record_blocks ? &hrec : NULL);
+    int nkept = 0;
+    double ecut = set_kept(&Ae, p->ncut, myeps, &nkept);
+    double thT[4], sus[4], ent[4];
+    for (int nc = 1; nc <= 4; nc++) {
+      double bN = bbar * pow(Lambda, (nc - 4) / (2.0 * 4));
+      thT[nc - 1] = DN / bN;
+      sus[nc - 1] = calc_suscep(bN, &Ae);
+      ent[nc - 1] = calc_entropy(bN, &Ae);
+    }
+    cut_states(&Ae, &Ac, p->ncut, myeps);
+    update_matrices(&S, &Ac, &Ab, mats, nm);
+    double ops[3] = {0, 0, 0};
+    int nops = 0;
+    if (p->calcdens != 1) for (int m = 1; m < nm; m++) ops[nops++] = calc_opavg(bbar, &Ac, &mats[m]);
+    emit(cb, user, &Ae, &Ac, emin, ecut, nkept, &hrec, p->record_blocks, thT, sus, ent, DN / bbar, nops, ops);
+    par[1] = orc_chin(N, Lambda);
+  }
+  free(hrec.a);
+  for (int m = 0; m < 4; m++) mat_free(&mats[m]);
+  arr_free(&S); arr_free(&Ae); arr_free(&Ac); arr_free(&Ab);
+  return 0;
+}

@@ -1,0 +1,89 @@
+/*
+ * nrg_oracle.h -- CPU restatement (FP64, single thread) of the TwoChNRG
+ * iterative-diagonalisation path.  TEST INFRASTRUCTURE ONLY: this code is the
+ * checker that the HIP path is compared against, and the timed "port" CPU
+ * baseline.  Nothing in the product library links or calls it.
+ *
+ * Every function cites the reference file:line it restates.
+ */
+#ifndef NRG_ORACLE_H
+#define NRG_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* One NRG step as seen after DiagHN + SetKept (Aeig) and CutStates (AeigCut). */
+typedef struct {
+  int nshell;          /* N */
+  int nq;              /* quantum numbers per block */
+  int nblocks;         /* Aeig blocks */
+  const double *qn;    /* nblocks*nq */
+  const int *bbe;      /* BlockBegEnd, 2*nblocks (inclusive) */
+  int nstates;
+  const double *en;    /* rescaled energies (after SetE0zero + SetDegen) */
+  const unsigned char *kept; /* Kept flags (nstates) */
+  double emin;         /* Emin subtracted by SetE0zero */
+  double ecut;         /* Ecut used by SetKept */
+  int nkept;           /* kept states of Aeig */
+  /* optional per-block data: packed upper H (row-wise) and eigvec rows */
+  const double *hpacked;   /* sum n(n+1)/2, NULL if not recorded */
+  const double *evec;      /* sum n*n, NULL if not recorded */
+  /* thermodynamics on Aeig at the 4 betabar values of CalcThermo */
+  double thermo_T[4], suscep[4], entropy[4];
+  /* CalcStuff: op averages on AeigCut at betabar (T = DN/betabar) */
+  double T_avg;
+  int nops_avg;
+  double opavg[8];
+  /* AeigCut structure */
+  int cut_nblocks;
+  const double *cut_qn;
+  const int *cut_bbe;
+  const int *cut_bbe_bc;
+} orc_step;
+
+typedef void (*orc_step_cb)(void *user, const orc_step *st);
+
+/* OneChQS Anderson through the NRG_main driver
+ * (src/Main/NRG_main.cpp:45-427, ModelSwitch src/CodeHandler_ModelSwitch.cpp:121-293). */
+typedef struct {
+  double U, Gamma, ed, Lambda;
+  int nsitesmax;       /* loop runs Nsites = 0 .. nsitesmax-1 */
+  int ncut;
+  int calcdens;        /* 0: n_d,n_d^2,S_d^2 op averages; 2: thermo + ops; 1: f_N + c_d */
+  int disc_scheme;     /* 1: no sqrt(A_Lambda) factor on chi_{-1} (NRG_main -D 1) */
+  double myeps;        /* CNRGarray::MyEPS (default 1e-18) */
+  double betabar;      /* default 0.727 */
+  int record_blocks;   /* pass H and eigvec blocks to the callback */
+} orc_onechqs_params;
+
+int orc_run_onechqs(const orc_onechqs_params *p, orc_step_cb cb, void *user);
+
+/* ---- kernel-level restatements ---- */
+
+/* CGordan  src/CGordan.cpp:34-120 */
+double orc_cgordan(double J1, double M1, double J2, double M2, double J, double M);
+
+/* DiagBlock -> dspev_('V','L') on row-wise packed upper storage
+ * (src/NRGmatrixClass.cpp:566-654).  W ascending, Z rows = eigenvectors. */
+void orc_eigh_packed(int n, const double *ap, double *w, double *z);
+/* same, full row-major symmetric input (dsyev_ path, :655-680) */
+void orc_eigh_full(int n, const double *a, double *w, double *z);
+
+/* CNRGarray::SetDegen src/NRGarrayClass.cpp:244-268 (in place) */
+void orc_setdegen(int n, double *en, double myeps);
+/* CNRGarray::Ecut src/NRGarrayClass.cpp:661-691 */
+double orc_ecut(int n, const double *en, int ncut);
+
+/* The two cblas_dgemm of UpdateMatrices (src/UpdateMatrices.cpp:481-494):
+ * out(ki x kj) = Zi(ki x ni) * F(ni x nj) * Zj(kj x nj)^T */
+void orc_rotate(int ni, int nj, int ki, int kj, const double *zi,
+                const double *f, const double *zj, double *out);
+
+/* chi_N of SetChainWilson src/NRGchainClass.cpp:132-155 */
+double orc_chin(int n, double Lambda);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

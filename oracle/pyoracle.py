@@ -1,0 +1,167 @@
+"""ctypes binding of the CPU oracle (test infrastructure only).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
+this module.  The product package never does.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+
+class OrcStep(ctypes.Structure):
+    _fields_ = [
+        ("nshell", ctypes.c_int), ("nq", ctypes.c_int), ("nblocks", ctypes.c_int),
+        ("qn", ctypes.POINTER(ctypes.c_double)), ("bbe", ctypes.POINTER(ctypes.c_int)),
+        ("nstates", ctypes.c_int), ("en", ctypes.POINTER(ctypes.c_double)),
+        ("kept", ctypes.POINTER(ctypes.c_ubyte)),
+        ("emin", ctypes.c_double), ("ecut", ctypes.c_double), ("nkept", ctypes.c_int),
+        ("hpacked", ctypes.POINTER(ctypes.c_double)), ("evec", ctypes.POINTER(ctypes.c_double)),
+        ("thermo_T", ctypes.c_double * 4), ("suscep", ctypes.c_double * 4),
+        ("entropy", ctypes.c_double * 4),
+        ("T_avg", ctypes.c_double), ("nops_avg", ctypes.c_int), ("opavg", ctypes.c_double * 8),
+        ("cut_nblocks", ctypes.c_int), ("cut_qn", ctypes.POINTER(ctypes.c_double)),
+        ("cut_bbe", ctypes.POINTER(ctypes.c_int)), ("cut_bbe_bc", ctypes.POINTER(ctypes.c_int)),
+    ]
+
+
+STEP_CB = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.POINTER(OrcStep))
+
+
+class OneChQSParams(ctypes.Structure):
+    _fields_ = [
+        ("U", ctypes.c_double), ("Gamma", ctypes.c_double), ("ed", ctypes.c_double),
+        ("Lambda", ctypes.c_double), ("nsitesmax", ctypes.c_int), ("ncut", ctypes.c_int),
+        ("calcdens", ctypes.c_int), ("disc_scheme", ctypes.c_int), ("myeps", ctypes.c_double),
+        ("betabar", ctypes.c_double), ("record_blocks", ctypes.c_int),
+    ]
+
+
+_lib = None
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        subprocess.check_call(["make", "-s", "-C", HERE])
+    lib = ctypes.CDLL(LIB_PATH)
+    lib.orc_run_onechqs.argtypes = [ctypes.POINTER(OneChQSParams), STEP_CB, ctypes.c_void_p]
+    lib.orc_run_onechqs.restype = ctypes.c_int
+    lib.orc_cgordan.argtypes = [ctypes.c_double] * 6
+    lib.orc_cgordan.restype = ctypes.c_double
+    dp = ctypes.POINTER(ctypes.c_double)
+    lib.orc_eigh_packed.argtypes = [ctypes.c_int, dp, dp, dp]
+    lib.orc_eigh_full.argtypes = [ctypes.c_int, dp, dp, dp]
+    lib.orc_setdegen.argtypes = [ctypes.c_int, dp, ctypes.c_double]
+    lib.orc_ecut.argtypes = [ctypes.c_int, dp, ctypes.c_int]
+    lib.orc_ecut.restype = ctypes.c_double
+    lib.orc_rotate.argtypes = [ctypes.c_int] * 4 + [dp, dp, dp, dp]
+    lib.orc_chin.argtypes = [ctypes.c_int, ctypes.c_double]
+    lib.orc_chin.restype = ctypes.c_double
+    _lib = lib
+    return lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def step_to_dict(st):
+    """Copy one step record (shared layout of oracle and product) into numpy."""
+    s = st.contents
+    nb, nq, ns = s.nblocks, s.nq, s.nstates
+    d = {
+        "N": s.nshell, "nq": nq, "nblocks": nb,
+        "qn": np.ctypeslib.as_array(s.qn, (nb * nq,)).reshape(nb, nq).copy(),
+        "bbe": np.ctypeslib.as_array(s.bbe, (2 * nb,)).reshape(nb, 2).copy(),
+        "en": np.ctypeslib.as_array(s.en, (ns,)).copy(),
+        "kept": np.ctypeslib.as_array(s.kept, (ns,)).astype(bool).copy(),
+        "emin": s.emin, "ecut": s.ecut, "nkept": s.nkept,
+        "thermo_T": np.array(s.thermo_T[:]), "suscep": np.array(s.suscep[:]),
+        "entropy": np.array(s.entropy[:]), "T_avg": s.T_avg,
+        "opavg": np.array(s.opavg[: s.nops_avg]),
+        "cut_nblocks": s.cut_nblocks,
+    }
+    cb = s.cut_nblocks
+    d["cut_qn"] = np.ctypeslib.as_array(s.cut_qn, (cb * nq,)).reshape(cb, nq).copy() if cb else np.zeros((0, nq))
+    d["cut_bbe"] = np.ctypeslib.as_array(s.cut_bbe, (2 * cb,)).reshape(cb, 2).copy() if cb else np.zeros((0, 2), int)
+    d["cut_bbe_bc"] = (np.ctypeslib.as_array(s.cut_bbe_bc, (2 * cb,)).reshape(cb, 2).copy()
+                       if cb else np.zeros((0, 2), int))
+    sizes = d["bbe"][:, 1] - d["bbe"][:, 0] + 1
+    if bool(s.hpacked):
+        tot = int(np.sum(sizes * (sizes + 1) // 2))
+        d["hpacked"] = np.ctypeslib.as_array(s.hpacked, (tot,)).copy()
+    if bool(s.evec):
+        tot = int(np.sum(sizes * sizes))
+        d["evec"] = np.ctypeslib.as_array(s.evec, (tot,)).copy()
+    return d
+
+
+def run_onechqs(U=0.5, Gamma=0.0282691, ed=-0.25, Lambda=2.5, nsitesmax=4, ncut=1000,
+                calcdens=0, disc_scheme=0, myeps=1e-18, betabar=0.727, record_blocks=False):
+    lib = load()
+    steps = []
+
+    def cb(_user, st):
+        steps.append(step_to_dict(st))
+
+    p = OneChQSParams(U, Gamma, ed, Lambda, nsitesmax, ncut, calcdens, disc_scheme, myeps,
+                      betabar, int(record_blocks))
+    fn = STEP_CB(cb)
+    rc = lib.orc_run_onechqs(ctypes.byref(p), fn, None)
+    if rc != 0:
+        raise RuntimeError("oracle run failed: %d" % rc)
+    return steps
+
+
+def cgordan(J1, M1, J2, M2, J, M):
+    return load().orc_cgordan(J1, M1, J2, M2, J, M)
+
+
+def eigh_packed(n, ap):
+    ap = np.ascontiguousarray(ap, dtype=np.float64)
+    w = np.zeros(n)
+    z = np.zeros(n * n)
+    load().orc_eigh_packed(n, _dp(ap), _dp(w), _dp(z))
+    return w, z.reshape(n, n)
+
+
+def eigh_full(a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    n = a.shape[0]
+    w = np.zeros(n)
+    z = np.zeros(n * n)
+    load().orc_eigh_full(n, _dp(a), _dp(w), _dp(z))
+    return w, z.reshape(n, n)
+
+
+def setdegen(en, myeps):
+    e = np.ascontiguousarray(en, dtype=np.float64).copy()
+    load().orc_setdegen(len(e), _dp(e), myeps)
+    return e
+
+
+def ecut(en, ncut):
+    e = np.ascontiguousarray(en, dtype=np.float64)
+    return load().orc_ecut(len(e), _dp(e), ncut)
+
+
+def rotate(zi, f, zj):
+    zi = np.ascontiguousarray(zi, dtype=np.float64)
+    zj = np.ascontiguousarray(zj, dtype=np.float64)
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    ki, ni = zi.shape
+    kj, nj = zj.shape
+    out = np.zeros((ki, kj))
+    load().orc_rotate(ni, nj, ki, kj, _dp(zi), _dp(f), _dp(zj), _dp(out))
+    return out
+
+
+def chin(n, Lambda):
+    return load().orc_chin(n, Lambda)
