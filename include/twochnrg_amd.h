@@ -1,0 +1,149 @@
+/*
+ * twochnrg_amd.h -- C-ABI of the MI355X (gfx950) NRG iterative-diagonalisation
+ * hot path.  Plain pointers and sizes only.  Every entry point returns 0 on
+ * success and a negative code on error; nrg_last_error() gives the message
+ * (thread-local).  No exception crosses this boundary.  Calls are synchronous
+ * at the boundary: host buffers passed in are read/written before return.
+ *
+ * Each entry point names the reference interface it replaces
+ * (paths relative to the reference tree lgds/TwoChNRG).
+ */
+#ifndef TWOCHNRG_AMD_H
+#define TWOCHNRG_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NRG_OK 0
+#define NRG_ERR_ARG (-1)
+#define NRG_ERR_HIP (-2)
+#define NRG_ERR_NUMERIC (-3)
+#define NRG_ERR_STATE (-4)
+
+typedef struct nrg_ctx nrg_ctx;
+
+/* One context per host thread: owns the HIP stream, device workspaces and
+ * per-step scratch.  device = HIP ordinal. */
+int nrg_ctx_create(int device, nrg_ctx **out);
+void nrg_ctx_destroy(nrg_ctx *ctx);
+const char *nrg_last_error(void);
+/* Number of HIP devices visible (0 on a machine without a GPU). */
+int nrg_device_count(void);
+
+/*
+ * Batched symmetric eigensolve of nb uneven blocks, FP64, all vectors.
+ * Replaces CNRGmatrix::DiagBlock (src/NRGmatrixClass.cpp:566-696,
+ * include/NRGclasses.hpp:598-600) -> dspev_/dsyev_.
+ *   n[b]      block dimension
+ *   h_off[b]  offset of block b in H: packed != 0 -> n(n+1)/2 doubles, row-wise
+ *             upper (ij2rUpTr, src/ij2r.cpp:63-68, what DiagHN builds with
+ *             UpperTriangular=true); packed == 0 -> n*n row-major full.
+ *   W         sum n doubles, ascending per block, blocks in input order
+ *   Z         sum n*n doubles, block b at the prefix sum of n^2, row i =
+ *             eigenvector i (the dEigVec layout, src/NRGmatrixClass.cpp:647-654)
+ *   info[b]   0 ok, 1 if the QL iteration limit was hit (may be NULL)
+ * Host pointers.
+ */
+int nrg_diag_blocks(nrg_ctx *ctx, int nb, const int *n, const int64_t *h_off, const double *H, int packed,
+                    double *W, double *Z, int *info);
+
+/* Device-resident variant on the context's stream: dA holds full n*n blocks at
+ * a_off (overwritten by eigenvector rows), eigenvalues to dW at w_off. */
+int nrg_diag_blocks_dev(nrg_ctx *ctx, int nb, const int *n, const int64_t *a_off, double *dA,
+                        const int64_t *w_off, double *dW, int *info);
+
+/*
+ * Truncation decision with the reference's exact semantics:
+ * CNRGarray::SetDegen (src/NRGarrayClass.cpp:244-268) on en in place, then
+ * CNRGarray::Ecut (:661-691) and CNRGarray::SetKept (:461-491).
+ *   kept[i] = en[i] <= Ecut; *ecut, *nkept returned.  Host pointers.
+ */
+int nrg_truncate(int nstates, double *en, int ncut, double myeps, uint8_t *kept, double *ecut, int *nkept);
+
+/* CNRGarray::Ecut alone (src/NRGarrayClass.cpp:661-691). */
+int nrg_ecut(int nstates, const double *en, int ncut, double *ecut);
+
+/* One rotation problem of UpdateMatrices (src/UpdateMatrices.cpp:481-494):
+ * out(ki x kj) = Zi(ki x ni) * F(ni x nj) * Zj(kj x nj)^T, offsets in doubles. */
+typedef struct {
+  int ni, nj, ki, kj;
+  int64_t zi_off, zj_off, f_off, out_off;
+} nrg_pair_desc;
+
+/* Batched operator rotation (host buffers).  precision 0 = FP64.
+ * Replaces the cblas_dgemm pair of UpdateMatrices and the uBLAS
+ * prod(trans(Z),prod(M,Z)) of RotateBlock2BLAS (src/NRGmatrixClass.cpp:1471-1512). */
+int nrg_rotate_ops(nrg_ctx *ctx, int npairs, const nrg_pair_desc *pairs, const double *Z, int64_t zlen,
+                   const double *F, int64_t flen, double *out, int64_t outlen, int precision);
+
+/* ---------------- per-kernel timing ---------------- */
+
+/* Kernel classes timed with HIP events on the context stream. */
+#define NRG_K_EIG 0      /* batched eigensolve (eig_block_kernel) */
+#define NRG_K_HN 1       /* H_N assembly */
+#define NRG_K_FBASIS 2   /* operator matrix in the product basis */
+#define NRG_K_GEMM 3     /* rotation GEMMs */
+#define NRG_NKSTAT 4
+
+typedef struct {
+  double ms[NRG_NKSTAT];        /* summed kernel time */
+  long launches[NRG_NKSTAT];
+  double flops[NRG_NKSTAT];     /* algorithmic flops of those launches */
+  double bytes[NRG_NKSTAT];     /* algorithmic bytes of those launches */
+} nrg_stats;
+
+/* Turn event timing on/off (off by default) and read/reset the counters. */
+int nrg_set_timing(nrg_ctx *ctx, int on);
+int nrg_get_stats(nrg_ctx *ctx, nrg_stats *out);
+int nrg_reset_stats(nrg_ctx *ctx);
+
+/* ---------------- whole-chain driver (NRG_main loop) ---------------- */
+
+/* Per-step record (same layout as the CPU checker's record). */
+typedef struct {
+  int nshell, nq, nblocks;
+  const double *qn;
+  const int *bbe;
+  int nstates;
+  const double *en;
+  const unsigned char *kept;
+  double emin, ecut;
+  int nkept;
+  const double *hpacked;
+  const double *evec;
+  double thermo_T[4], suscep[4], entropy[4];
+  double T_avg;
+  int nops_avg;
+  double opavg[8];
+  int cut_nblocks;
+  const double *cut_qn;
+  const int *cut_bbe;
+  const int *cut_bbe_bc;
+} nrg_step;
+
+typedef void (*nrg_step_cb)(void *user, const nrg_step *st);
+
+typedef struct {
+  double U, Gamma, ed, Lambda;
+  int nsitesmax, ncut, calcdens, disc_scheme;
+  double myeps, betabar;
+  int record_blocks;
+} nrg_onechqs_params;
+
+/*
+ * OneChQS Anderson model through the NRG_main loop
+ * (src/Main/NRG_main.cpp:282-427; ModelSwitch src/CodeHandler_ModelSwitch.cpp:121-293):
+ * BuildBasis (host) -> H_N assembly + batched eigensolve (device) ->
+ * SetKept/CalcThermo/CutStates (host decisions on device results) ->
+ * UpdateMatrices rotations (device) -> CalcStuff.  cb may be NULL.
+ * timing_ms (may be NULL) receives the wall time of the step loop.
+ */
+int nrg_run_onechqs(nrg_ctx *ctx, const nrg_onechqs_params *p, nrg_step_cb cb, void *user, double *timing_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

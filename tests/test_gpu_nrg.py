@@ -1,0 +1,69 @@
+"""Whole NRG chain on the GPU path (OneChQS Anderson, NRG_main loop) against
+the CPU oracle: basis/block structure and kept sets bit-exact, rescaled
+eigenvalues within 1e-10 relative (to max(|E|,1)), thermodynamics and
+operator averages within 1e-8 relative."""
+import numpy as np
+import pytest
+
+import pyoracle as po
+import twochnrg_amd as t
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = t.Context(0)
+    yield c
+    c.close()
+
+
+def _compare(gs, os_, eig_tol=1e-10, th_tol=1e-8):
+    assert len(gs) == len(os_)
+    worst = 0.0
+    for g, o in zip(gs, os_):
+        assert g["N"] == o["N"]
+        assert np.array_equal(g["qn"], o["qn"]), g["N"]
+        assert np.array_equal(g["bbe"], o["bbe"]), g["N"]
+        assert np.array_equal(g["kept"], o["kept"]), g["N"]
+        assert np.array_equal(g["cut_qn"], o["cut_qn"]) and np.array_equal(g["cut_bbe"], o["cut_bbe"])
+        assert np.array_equal(g["cut_bbe_bc"], o["cut_bbe_bc"])
+        err = np.max(np.abs(g["en"] - o["en"]) / np.maximum(np.abs(o["en"]), 1.0))
+        worst = max(worst, err)
+        assert err <= eig_tol, (g["N"], err)
+        if g["N"] >= 0:
+            assert abs(g["ecut"] - o["ecut"]) <= eig_tol * max(1, abs(o["ecut"]))
+            assert abs(g["emin"] - o["emin"]) <= eig_tol * max(1, abs(o["emin"]))
+            for k in ("suscep", "entropy"):
+                assert np.allclose(g[k], o[k], rtol=th_tol, atol=0), (k, g["N"])
+        assert np.allclose(g["opavg"], o["opavg"], rtol=th_tol, atol=1e-14)
+    return worst
+
+
+def test_golden_config(ctx):
+    kw = dict(U=0.5, Gamma=0.0282691, ed=-0.25, Lambda=2.5, nsitesmax=4, ncut=1000, calcdens=0,
+              disc_scheme=1)
+    print("nrg golden config", kw, flush=True)
+    gs, _ = ctx.run_onechqs(**kw, record_blocks=True)
+    os_ = po.run_onechqs(**kw, record_blocks=True)
+    _compare(gs, os_)
+    for g, o in zip(gs[1:], os_[1:]):  # H_N blocks assembled on the device
+        assert np.max(np.abs(g["hpacked"] - o["hpacked"])) <= 1e-13 * max(1, np.abs(o["hpacked"]).max())
+
+
+@pytest.mark.parametrize("calcdens", [2, 1])
+def test_truncated_chain(ctx, calcdens):
+    kw = dict(U=0.5, Gamma=0.02, ed=-0.25, Lambda=2.5, nsitesmax=14, ncut=400, calcdens=calcdens)
+    print("nrg truncated", kw, flush=True)
+    gs, _ = ctx.run_onechqs(**kw)
+    os_ = po.run_onechqs(**kw)
+    worst = _compare(gs, os_)
+    print("worst eigenvalue rel err", worst)
+
+
+def test_asymmetric_point(ctx):
+    kw = dict(U=0.3, Gamma=0.05, ed=-0.05, Lambda=3.0, nsitesmax=10, ncut=300, calcdens=2)
+    print("nrg asym", kw, flush=True)
+    gs, _ = ctx.run_onechqs(**kw)
+    os_ = po.run_onechqs(**kw)
+    _compare(gs, os_)

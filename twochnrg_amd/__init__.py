@@ -1,0 +1,276 @@
+"""twochnrg_amd -- MI355X (gfx950) NRG iterative-diagonalisation hot path.
+
+Python face of the C-ABI in include/twochnrg_amd.h (libtwochnrg_amd.so,
+built in-tree by twochnrg_amd/build.py).  The HIP library is required: every
+entry point raises if it is missing; there is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libtwochnrg_amd.so")
+
+# every symbol declared in include/twochnrg_amd.h
+EXPORTS = [
+    "nrg_ctx_create", "nrg_ctx_destroy", "nrg_last_error", "nrg_device_count",
+    "nrg_diag_blocks", "nrg_diag_blocks_dev", "nrg_truncate", "nrg_ecut",
+    "nrg_rotate_ops", "nrg_run_onechqs", "nrg_set_timing", "nrg_get_stats", "nrg_reset_stats",
+]
+
+KSTAT_NAMES = ["eig", "hn", "fbasis", "gemm"]
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_i64p = ctypes.POINTER(ctypes.c_int64)
+
+
+class NrgStep(ctypes.Structure):
+    _fields_ = [
+        ("nshell", ctypes.c_int), ("nq", ctypes.c_int), ("nblocks", ctypes.c_int),
+        ("qn", _dp), ("bbe", _ip), ("nstates", ctypes.c_int), ("en", _dp),
+        ("kept", ctypes.POINTER(ctypes.c_ubyte)),
+        ("emin", ctypes.c_double), ("ecut", ctypes.c_double), ("nkept", ctypes.c_int),
+        ("hpacked", _dp), ("evec", _dp),
+        ("thermo_T", ctypes.c_double * 4), ("suscep", ctypes.c_double * 4),
+        ("entropy", ctypes.c_double * 4),
+        ("T_avg", ctypes.c_double), ("nops_avg", ctypes.c_int), ("opavg", ctypes.c_double * 8),
+        ("cut_nblocks", ctypes.c_int), ("cut_qn", _dp), ("cut_bbe", _ip), ("cut_bbe_bc", _ip),
+    ]
+
+
+STEP_CB = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.POINTER(NrgStep))
+
+
+class OneChQSParams(ctypes.Structure):
+    _fields_ = [
+        ("U", ctypes.c_double), ("Gamma", ctypes.c_double), ("ed", ctypes.c_double),
+        ("Lambda", ctypes.c_double), ("nsitesmax", ctypes.c_int), ("ncut", ctypes.c_int),
+        ("calcdens", ctypes.c_int), ("disc_scheme", ctypes.c_int), ("myeps", ctypes.c_double),
+        ("betabar", ctypes.c_double), ("record_blocks", ctypes.c_int),
+    ]
+
+
+class PairDesc(ctypes.Structure):
+    _fields_ = [("ni", ctypes.c_int), ("nj", ctypes.c_int), ("ki", ctypes.c_int), ("kj", ctypes.c_int),
+                ("zi_off", ctypes.c_int64), ("zj_off", ctypes.c_int64),
+                ("f_off", ctypes.c_int64), ("out_off", ctypes.c_int64)]
+
+
+class NrgStats(ctypes.Structure):
+    _fields_ = [("ms", ctypes.c_double * 4), ("launches", ctypes.c_long * 4),
+                ("flops", ctypes.c_double * 4), ("bytes", ctypes.c_double * 4)]
+
+
+_lib = None
+
+
+class NrgError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load libtwochnrg_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise NrgError("libtwochnrg_amd.so not built: run twochnrg_amd/build.py (or __graft_entry__.build())")
+    L = ctypes.CDLL(LIB_PATH)
+    L.nrg_last_error.restype = ctypes.c_char_p
+    L.nrg_device_count.restype = ctypes.c_int
+    L.nrg_ctx_create.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
+    L.nrg_ctx_destroy.argtypes = [ctypes.c_void_p]
+    L.nrg_ctx_destroy.restype = None
+    L.nrg_diag_blocks.argtypes = [ctypes.c_void_p, ctypes.c_int, _ip, _i64p, _dp, ctypes.c_int, _dp, _dp, _ip]
+    L.nrg_diag_blocks_dev.argtypes = [ctypes.c_void_p, ctypes.c_int, _ip, _i64p, ctypes.c_void_p, _i64p,
+                                      ctypes.c_void_p, _ip]
+    L.nrg_truncate.argtypes = [ctypes.c_int, _dp, ctypes.c_int, ctypes.c_double,
+                               ctypes.POINTER(ctypes.c_uint8), _dp, _ip]
+    L.nrg_ecut.argtypes = [ctypes.c_int, _dp, ctypes.c_int, _dp]
+    L.nrg_rotate_ops.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(PairDesc), _dp, ctypes.c_int64,
+                                 _dp, ctypes.c_int64, _dp, ctypes.c_int64, ctypes.c_int]
+    L.nrg_set_timing.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    L.nrg_get_stats.argtypes = [ctypes.c_void_p, ctypes.POINTER(NrgStats)]
+    L.nrg_reset_stats.argtypes = [ctypes.c_void_p]
+    L.nrg_run_onechqs.argtypes = [ctypes.c_void_p, ctypes.POINTER(OneChQSParams), STEP_CB, ctypes.c_void_p, _dp]
+    _lib = L
+    return L
+
+
+def _check(rc):
+    if rc != 0:
+        raise NrgError("nrg error %d: %s" % (rc, lib().nrg_last_error().decode()))
+
+
+def _ptr(a, t=_dp):
+    return a.ctypes.data_as(t)
+
+
+def device_count():
+    return lib().nrg_device_count()
+
+
+class Context:
+    """One HIP stream + device workspaces (nrg_ctx)."""
+
+    def __init__(self, device=0):
+        h = ctypes.c_void_p()
+        _check(lib().nrg_ctx_create(device, ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().nrg_ctx_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_timing(self, on=True):
+        _check(lib().nrg_set_timing(self.h, int(on)))
+
+    def reset_stats(self):
+        _check(lib().nrg_reset_stats(self.h))
+
+    def stats(self):
+        st = NrgStats()
+        _check(lib().nrg_get_stats(self.h, ctypes.byref(st)))
+        return {k: {"ms": st.ms[i], "launches": st.launches[i], "flops": st.flops[i], "bytes": st.bytes[i]}
+                for i, k in enumerate(KSTAT_NAMES)}
+
+    # DiagBlock replacement -------------------------------------------------
+    def diag_blocks(self, blocks, packed=False):
+        """blocks: list of (n x n) symmetric arrays, or of packed upper vectors
+        with packed=True.  Returns (list of W, list of Z with row i = vector i, info)."""
+        ns = []
+        for b in blocks:
+            b = np.asarray(b)
+            if packed:
+                n = int(round((np.sqrt(8 * b.size + 1) - 1) / 2))
+            else:
+                n = b.shape[0]
+            ns.append(n)
+        n_arr = np.array(ns, dtype=np.int32)
+        flat = np.concatenate([np.ascontiguousarray(b, dtype=np.float64).ravel() for b in blocks]) \
+            if blocks else np.zeros(1)
+        lens = [np.asarray(b).size for b in blocks]
+        h_off = np.concatenate([[0], np.cumsum(lens)[:-1]]).astype(np.int64)
+        W = np.zeros(max(1, int(n_arr.sum())))
+        Z = np.zeros(max(1, int((n_arr.astype(np.int64) ** 2).sum())))
+        info = np.zeros(max(1, len(ns)), dtype=np.int32)
+        _check(lib().nrg_diag_blocks(self.h, len(ns), _ptr(n_arr, _ip), _ptr(h_off, _i64p), _ptr(flat),
+                                     int(packed), _ptr(W), _ptr(Z), _ptr(info, _ip)))
+        outW, outZ = [], []
+        wo = zo = 0
+        for n in ns:
+            outW.append(W[wo:wo + n].copy())
+            outZ.append(Z[zo:zo + n * n].reshape(n, n).copy())
+            wo += n
+            zo += n * n
+        return outW, outZ, info[: len(ns)].copy()
+
+    def rotate(self, problems):
+        """problems: list of (Zi (ki x ni), F (ni x nj), Zj (kj x nj)).
+        Returns list of Zi F Zj^T (FP64 on the device)."""
+        zs, fs, descs = [], [], []
+        zoff = foff = ooff = 0
+        for zi, f, zj in problems:
+            zi = np.ascontiguousarray(zi, np.float64)
+            zj = np.ascontiguousarray(zj, np.float64)
+            f = np.ascontiguousarray(f, np.float64)
+            ki, ni = zi.shape
+            kj, nj = zj.shape
+            d = PairDesc(ni, nj, ki, kj, zoff, zoff + zi.size, foff, ooff)
+            zs += [zi.ravel(), zj.ravel()]
+            fs.append(f.ravel())
+            zoff += zi.size + zj.size
+            foff += f.size
+            ooff += ki * kj
+            descs.append(d)
+        Z = np.concatenate(zs) if zs else np.zeros(1)
+        F = np.concatenate(fs) if fs else np.zeros(1)
+        out = np.zeros(max(1, ooff))
+        arr = (PairDesc * max(1, len(descs)))(*descs)
+        _check(lib().nrg_rotate_ops(self.h, len(descs), arr, _ptr(Z), Z.size, _ptr(F), F.size, _ptr(out),
+                                    out.size, 0))
+        res = []
+        o = 0
+        for d in descs:
+            res.append(out[o:o + d.ki * d.kj].reshape(d.ki, d.kj).copy())
+            o += d.ki * d.kj
+        return res
+
+    # NRG_main loop ---------------------------------------------------------
+    def run_onechqs(self, U=0.5, Gamma=0.0282691, ed=-0.25, Lambda=2.5, nsitesmax=4, ncut=1000,
+                    calcdens=0, disc_scheme=0, myeps=1e-18, betabar=0.727, record_blocks=False,
+                    keep_steps=True, on_step=None):
+        steps = []
+
+        def cb(_u, st):
+            if on_step is not None:
+                on_step(st)
+            if keep_steps:
+                steps.append(step_to_dict(st))
+
+        p = OneChQSParams(U, Gamma, ed, Lambda, nsitesmax, ncut, calcdens, disc_scheme, myeps, betabar,
+                          int(record_blocks))
+        fn = STEP_CB(cb)
+        ms = ctypes.c_double(0.0)
+        _check(lib().nrg_run_onechqs(self.h, ctypes.byref(p), fn, None, ctypes.byref(ms)))
+        return steps, ms.value
+
+
+def truncate(en, ncut, myeps=1e-18):
+    """SetDegen + Ecut + SetKept (host part of the library; no device needed)."""
+    e = np.ascontiguousarray(en, np.float64).copy()
+    kept = np.zeros(len(e), np.uint8)
+    ec = ctypes.c_double()
+    nk = ctypes.c_int()
+    _check(lib().nrg_truncate(len(e), _ptr(e), ncut, myeps, kept.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+                              ctypes.byref(ec), ctypes.byref(nk)))
+    return e, kept.astype(bool), ec.value, nk.value
+
+
+def ecut(en, ncut):
+    e = np.ascontiguousarray(en, np.float64)
+    ec = ctypes.c_double()
+    _check(lib().nrg_ecut(len(e), _ptr(e), ncut, ctypes.byref(ec)))
+    return ec.value
+
+
+def step_to_dict(st):
+    s = st.contents
+    nb, nq, ns = s.nblocks, s.nq, s.nstates
+    d = {
+        "N": s.nshell, "nq": nq, "nblocks": nb,
+        "qn": np.ctypeslib.as_array(s.qn, (nb * nq,)).reshape(nb, nq).copy(),
+        "bbe": np.ctypeslib.as_array(s.bbe, (2 * nb,)).reshape(nb, 2).copy(),
+        "en": np.ctypeslib.as_array(s.en, (ns,)).copy(),
+        "kept": np.ctypeslib.as_array(s.kept, (ns,)).astype(bool).copy(),
+        "emin": s.emin, "ecut": s.ecut, "nkept": s.nkept,
+        "thermo_T": np.array(s.thermo_T[:]), "suscep": np.array(s.suscep[:]),
+        "entropy": np.array(s.entropy[:]), "T_avg": s.T_avg,
+        "opavg": np.array(s.opavg[: s.nops_avg]), "cut_nblocks": s.cut_nblocks,
+    }
+    cb = s.cut_nblocks
+    d["cut_qn"] = np.ctypeslib.as_array(s.cut_qn, (cb * nq,)).reshape(cb, nq).copy() if cb else np.zeros((0, nq))
+    d["cut_bbe"] = np.ctypeslib.as_array(s.cut_bbe, (2 * cb,)).reshape(cb, 2).copy() if cb else np.zeros((0, 2), int)
+    d["cut_bbe_bc"] = (np.ctypeslib.as_array(s.cut_bbe_bc, (2 * cb,)).reshape(cb, 2).copy()
+                       if cb else np.zeros((0, 2), int))
+    sizes = d["bbe"][:, 1] - d["bbe"][:, 0] + 1
+    if bool(s.hpacked):
+        d["hpacked"] = np.ctypeslib.as_array(s.hpacked, (int(np.sum(sizes * (sizes + 1) // 2)),)).copy()
+    if bool(s.evec):
+        d["evec"] = np.ctypeslib.as_array(s.evec, (int(np.sum(sizes * sizes)),)).copy()
+    return d

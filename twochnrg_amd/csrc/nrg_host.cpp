@@ -1,0 +1,430 @@
+// nrg_host.cpp -- host side of the MI355X NRG hot path: the C-ABI, the
+// context (stream + device workspaces), exact truncation bookkeeping, and the
+// kernel-level entry points.  The whole-chain driver lives in nrg_driver.cpp.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/twochnrg_amd.h"
+#include "nrg_internal.h"
+
+namespace nrg {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+// ---------------------------------------------------------------------------
+// SetDegen (src/NRGarrayClass.cpp:244-268): pairs (ii<jj) in index order with
+// |E_jj - E_ii| < eps are snapped to the smaller value, the comparisons using
+// values already snapped.  Values only ever move to another value of the same
+// cluster (maximal run of sorted values with gaps < eps), and values of
+// different clusters stay >= eps apart, so the quadratic loop restricted to
+// each cluster (members in index order) gives the identical result.
+// ---------------------------------------------------------------------------
+void set_degen(std::vector<double>& en, double eps) {
+  const int n = (int)en.size();
+  if (n < 2) return;
+  std::vector<int> idx(n);
+  std::iota(idx.begin(), idx.end(), 0);
+  std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return en[a] < en[b]; });
+  int s = 0;
+  while (s < n) {
+    int e = s + 1;
+    while (e < n && std::fabs(en[idx[e]] - en[idx[e - 1]]) < eps) e++;
+    if (e - s > 1) {
+      std::vector<int> mem(idx.begin() + s, idx.begin() + e);
+      std::sort(mem.begin(), mem.end());
+      for (size_t a = 0; a < mem.size(); a++)
+        for (size_t b = a + 1; b < mem.size(); b++) {
+          double& x = en[mem[a]];
+          double& y = en[mem[b]];
+          if (std::fabs(y - x) < eps) {
+            if (y > x) y = x; else x = y;
+          }
+        }
+    }
+    s = e;
+  }
+}
+
+// CNRGarray::Ecut (src/NRGarrayClass.cpp:661-691).  Its 1% window reads
+// Eaux[Ncut+ii] bounded only by ii<size; an index past the end is taken here
+// as a level far above the cut, which keeps every remaining state.
+double ecut_of(const std::vector<double>& en, int ncut) {
+  std::vector<double> e(en);
+  std::sort(e.begin(), e.end());
+  const int n = (int)e.size();
+  if (n == 0) return 0.0;
+  if (ncut >= n - 1) return e[n - 1];
+  double aux = e[ncut - 1], aux2 = e[ncut];
+  int ii = 1;
+  while (aux2 > 0.0 && std::fabs((aux2 - aux) / aux) < 0.01 && ii < n) {
+    if (ncut + ii >= n) { aux2 = HUGE_VAL; ii++; break; }
+    aux2 = e[ncut + ii];
+    ii++;
+  }
+  if (ii > 1) aux = (ncut + ii - 2 < n) ? e[ncut + ii - 2] : e[n - 1];
+  return aux;
+}
+
+}  // namespace nrg
+
+using namespace nrg;
+
+// ---------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------
+extern "C" const char* nrg_last_error(void) { return g_err.c_str(); }
+
+extern "C" int nrg_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+extern "C" int nrg_ctx_create(int device, nrg_ctx** out) {
+  if (!out) return fail(NRG_ERR_ARG, "nrg_ctx_create: out is NULL");
+  int nd = 0;
+  if (hipGetDeviceCount(&nd) != hipSuccess || nd <= 0)
+    return fail(NRG_ERR_HIP, "nrg_ctx_create: no HIP device available");
+  if (device < 0 || device >= nd) return fail(NRG_ERR_ARG, "nrg_ctx_create: bad device ordinal");
+  nrg_ctx* c = new nrg_ctx();
+  c->device = device;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess) {
+    delete c;
+    return fail(NRG_ERR_HIP, "nrg_ctx_create: hipSetDevice/hipStreamCreate failed");
+  }
+  *out = c;
+  return NRG_OK;
+}
+
+extern "C" void nrg_ctx_destroy(nrg_ctx* c) {
+  if (!c) return;
+  hipSetDevice(c->device);
+  hipStreamSynchronize(c->stream);
+  c->resolve();
+  for (auto e : c->event_pool) hipEventDestroy(e);
+  c->bufs.clear();
+  hipStreamDestroy(c->stream);
+  delete c;
+}
+
+DBuf::~DBuf() {
+  if (p) hipFree(p);
+}
+int DBuf::reserve(size_t bytes) {
+  if (bytes <= cap) return NRG_OK;
+  if (p) hipFree(p);
+  p = nullptr;
+  cap = 0;
+  size_t b = bytes + bytes / 4 + 256;
+  if (hipMalloc(&p, b) != hipSuccess) { p = nullptr; return fail(NRG_ERR_HIP, "hipMalloc failed (" + std::to_string(b) + " bytes)"); }
+  cap = b;
+  return NRG_OK;
+}
+
+hipEvent_t nrg_ctx::get_event() {
+  if (!event_pool.empty()) { hipEvent_t e = event_pool.back(); event_pool.pop_back(); return e; }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+int nrg_ctx::timer_begin(PendingTimer& t, int kind, double flops, double bytes) {
+  t.kind = kind; t.flops = flops; t.bytes = bytes;
+  t.a = t.b = nullptr;
+  if (!timing) return 0;
+  t.a = get_event();
+  t.b = get_event();
+  if (!t.a || !t.b) return fail(NRG_ERR_HIP, "hipEventCreate failed");
+  HIPCHK(hipEventRecord(t.a, stream));
+  return 0;
+}
+
+int nrg_ctx::timer_end(PendingTimer& t) {
+  if (!timing || !t.a) return 0;
+  HIPCHK(hipEventRecord(t.b, stream));
+  pending.push_back(t);
+  return 0;
+}
+
+int nrg_ctx::resolve() {
+  for (auto& t : pending) {
+    HIPCHK(hipEventSynchronize(t.b));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, t.a, t.b));
+    KStat& k = stat[t.kind];
+    k.ms += ms; k.launches++; k.flops += t.flops; k.bytes += t.bytes;
+    event_pool.push_back(t.a);
+    event_pool.push_back(t.b);
+  }
+  pending.clear();
+  return 0;
+}
+
+extern "C" int nrg_set_timing(nrg_ctx* c, int on) {
+  if (!c) return fail(NRG_ERR_ARG, "nrg_set_timing: NULL ctx");
+  c->timing = on != 0;
+  return NRG_OK;
+}
+
+extern "C" int nrg_get_stats(nrg_ctx* c, nrg_stats* out) {
+  if (!c || !out) return fail(NRG_ERR_ARG, "nrg_get_stats: NULL argument");
+  int rc = c->resolve();
+  if (rc) return rc;
+  for (int k = 0; k < NRG_NKSTAT; k++) {
+    out->ms[k] = c->stat[k].ms; out->launches[k] = c->stat[k].launches;
+    out->flops[k] = c->stat[k].flops; out->bytes[k] = c->stat[k].bytes;
+  }
+  return NRG_OK;
+}
+
+extern "C" int nrg_reset_stats(nrg_ctx* c) {
+  if (!c) return fail(NRG_ERR_ARG, "nrg_reset_stats: NULL ctx");
+  int rc = c->resolve();
+  if (rc) return rc;
+  for (int k = 0; k < NRG_NKSTAT; k++) c->stat[k] = KStat();
+  return NRG_OK;
+}
+
+DBuf& nrg_ctx::buf(const std::string& name) {
+  auto it = bufs.find(name);
+  if (it == bufs.end()) it = bufs.emplace(name, std::unique_ptr<DBuf>(new DBuf())).first;
+  return *it->second;
+}
+
+// ---------------------------------------------------------------------------
+// eigensolve over blocks already resident in dA
+// ---------------------------------------------------------------------------
+int nrg::eig_dev(nrg_ctx* c, int nb, const int* n, const int64_t* a_off, double* dA, const int64_t* w_off,
+                 double* dW, int* info_host) {
+  if (nb <= 0) return NRG_OK;
+  std::vector<EigDesc> desc(nb);
+  int64_t ws = 0, vs = 0;
+  int nmax = 0;
+  for (int b = 0; b < nb; b++) {
+    if (n[b] < 0) return fail(NRG_ERR_ARG, "negative block size");
+    desc[b].n = n[b];
+    desc[b].a_off = a_off[b];
+    desc[b].w_off = w_off[b];
+    desc[b].ws_off = ws;
+    ws += (int64_t)n[b] * n[b];
+    nmax = std::max(nmax, n[b]);
+  }
+  for (int b = 0; b < nb; b++) { desc[b].vec_off = ws + vs; vs += 7 * (int64_t)n[b]; }
+  std::vector<int> order(nb);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return n[x] > n[y]; });
+  DBuf& dws = c->buf("eig_ws");
+  DBuf& ddesc = c->buf("eig_desc");
+  DBuf& dord = c->buf("eig_order");
+  DBuf& dinfo = c->buf("eig_info");
+  int rc;
+  if ((rc = dws.reserve(sizeof(double) * (size_t)(ws + vs + 1)))) return rc;
+  if ((rc = ddesc.reserve(sizeof(EigDesc) * nb))) return rc;
+  if ((rc = dord.reserve(sizeof(int) * nb))) return rc;
+  if ((rc = dinfo.reserve(sizeof(int) * nb))) return rc;
+  HIPCHK(hipMemcpyAsync(ddesc.p, desc.data(), sizeof(EigDesc) * nb, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(dord.p, order.data(), sizeof(int) * nb, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  // algorithmic work: (10/3) n^3 flops; 8*(n(n+1)/2 + n^2 + n) bytes per block
+  double fl = 0.0, by = 0.0;
+  for (int b = 0; b < nb; b++) {
+    double x = n[b];
+    fl += 10.0 / 3.0 * x * x * x;
+    by += 8.0 * (x * (x + 1) / 2 + x * x + x);
+  }
+  PendingTimer tm;
+  if ((rc = c->timer_begin(tm, NRG_K_EIG, fl, by))) return rc;
+  HIPCHK(nrgk_launch_eig((EigDesc*)ddesc.p, (int*)dord.p, nb, nmax, dA, dW, (double*)dws.p, (int*)dinfo.p,
+                         c->stream));
+  if ((rc = c->timer_end(tm))) return rc;
+  if (info_host) {
+    HIPCHK(hipMemcpyAsync(info_host, dinfo.p, sizeof(int) * nb, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  }
+  return NRG_OK;
+}
+
+extern "C" int nrg_diag_blocks_dev(nrg_ctx* c, int nb, const int* n, const int64_t* a_off, double* dA,
+                                   const int64_t* w_off, double* dW, int* info) {
+  if (!c || nb < 0 || (nb > 0 && (!n || !a_off || !dA || !w_off || !dW)))
+    return fail(NRG_ERR_ARG, "nrg_diag_blocks_dev: bad arguments");
+  HIPCHK(hipSetDevice(c->device));
+  return eig_dev(c, nb, n, a_off, dA, w_off, dW, info);
+}
+
+extern "C" int nrg_diag_blocks(nrg_ctx* c, int nb, const int* n, const int64_t* h_off, const double* H,
+                               int packed, double* W, double* Z, int* info) {
+  if (!c || nb < 0 || (nb > 0 && (!n || !h_off || !H || !W || !Z)))
+    return fail(NRG_ERR_ARG, "nrg_diag_blocks: bad arguments");
+  if (nb == 0) return NRG_OK;
+  HIPCHK(hipSetDevice(c->device));
+  std::vector<int64_t> a_off(nb), w_off(nb);
+  int64_t za = 0, wa = 0, hlen = 0;
+  int nmax = 0;
+  for (int b = 0; b < nb; b++) {
+    if (n[b] < 0) return fail(NRG_ERR_ARG, "nrg_diag_blocks: negative block size");
+    a_off[b] = za; w_off[b] = wa;
+    za += (int64_t)n[b] * n[b]; wa += n[b];
+    int64_t len = packed ? (int64_t)n[b] * (n[b] + 1) / 2 : (int64_t)n[b] * n[b];
+    hlen = std::max(hlen, h_off[b] + len);
+    nmax = std::max(nmax, n[b]);
+  }
+  DBuf& dA = c->buf("diag_A");
+  DBuf& dW = c->buf("diag_W");
+  int rc;
+  if ((rc = dA.reserve(sizeof(double) * (size_t)(za + 1)))) return rc;
+  if ((rc = dW.reserve(sizeof(double) * (size_t)(wa + 1)))) return rc;
+  if (packed) {
+    DBuf& dP = c->buf("diag_P");
+    DBuf& dpo = c->buf("diag_poff");
+    DBuf& dd = c->buf("diag_desc");
+    if ((rc = dP.reserve(sizeof(double) * (size_t)(hlen + 1)))) return rc;
+    if ((rc = dpo.reserve(sizeof(int64_t) * nb))) return rc;
+    if ((rc = dd.reserve(sizeof(EigDesc) * nb))) return rc;
+    std::vector<EigDesc> desc(nb);
+    for (int b = 0; b < nb; b++) { desc[b] = EigDesc{}; desc[b].n = n[b]; desc[b].a_off = a_off[b]; }
+    HIPCHK(hipMemcpyAsync(dP.p, H, sizeof(double) * hlen, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(dpo.p, h_off, sizeof(int64_t) * nb, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(dd.p, desc.data(), sizeof(EigDesc) * nb, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(nrgk_launch_unpack((EigDesc*)dd.p, (int64_t*)dpo.p, (double*)dP.p, (double*)dA.p, nb, nmax, c->stream));
+  } else {
+    for (int b = 0; b < nb; b++)
+      HIPCHK(hipMemcpyAsync((double*)dA.p + a_off[b], H + h_off[b], sizeof(double) * (size_t)n[b] * n[b],
+                            hipMemcpyHostToDevice, c->stream));
+  }
+  std::vector<int> inf(nb, 0);
+  if ((rc = eig_dev(c, nb, n, a_off.data(), (double*)dA.p, w_off.data(), (double*)dW.p, inf.data()))) return rc;
+  HIPCHK(hipMemcpyAsync(W, dW.p, sizeof(double) * wa, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(Z, dA.p, sizeof(double) * za, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (info) std::memcpy(info, inf.data(), sizeof(int) * nb);
+  return NRG_OK;
+}
+
+extern "C" int nrg_ecut(int nstates, const double* en, int ncut, double* ecut) {
+  if (nstates <= 0 || !en || !ecut || ncut < 1) return fail(NRG_ERR_ARG, "nrg_ecut: bad arguments");
+  std::vector<double> e(en, en + nstates);
+  *ecut = ecut_of(e, ncut);
+  return NRG_OK;
+}
+
+extern "C" int nrg_truncate(int nstates, double* en, int ncut, double myeps, uint8_t* kept, double* ecut,
+                            int* nkept) {
+  if (nstates <= 0 || !en || !kept || ncut < 1) return fail(NRG_ERR_ARG, "nrg_truncate: bad arguments");
+  std::vector<double> e(en, en + nstates);
+  set_degen(e, myeps);
+  double ec = ecut_of(e, ncut);
+  int nk = 0;
+  for (int i = 0; i < nstates; i++) { kept[i] = e[i] <= ec; nk += kept[i]; en[i] = e[i]; }
+  if (ecut) *ecut = ec;
+  if (nkept) *nkept = nk;
+  return NRG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// rotations: out = Zi * F * Zj^T as two NT GEMMs through auxT = Zj * F^T
+// ---------------------------------------------------------------------------
+int nrg::rotate_dev(nrg_ctx* c, const std::vector<RotJob>& jobs, const double* dZ, const double* dF, double* dAux,
+                    double* dOut) {
+  if (jobs.empty()) return NRG_OK;
+  // pass 1: auxT(kj x ni) = Zj(kj x nj) * F(ni x nj)^T ; pass 2: out(ki x kj) = Zi(ki x ni) * auxT(kj x ni)^T
+  std::vector<GemmDesc> p1, p2;
+  std::vector<GemmTile> t1, t2;
+  for (size_t q = 0; q < jobs.size(); q++) {
+    const RotJob& J = jobs[q];
+    if (J.ki == 0 || J.kj == 0) continue;
+    GemmDesc g{};
+    g.m = J.kj; g.n = J.ni; g.k = J.nj; g.lda = J.ldzj; g.ldb = J.nj; g.ldc = J.ni;
+    g.a_off = J.zj_off; g.b_off = J.f_off; g.c_off = J.aux_off; g.alpha = 1.0;
+    p1.push_back(g);
+    GemmDesc h{};
+    h.m = J.ki; h.n = J.kj; h.k = J.ni; h.lda = J.ldzi; h.ldb = J.ni; h.ldc = J.kj;
+    h.a_off = J.zi_off; h.b_off = J.aux_off; h.c_off = J.out_off; h.alpha = 1.0;
+    p2.push_back(h);
+  }
+  auto tiles = [](const std::vector<GemmDesc>& P, std::vector<GemmTile>& T) {
+    for (size_t q = 0; q < P.size(); q++) {
+      int tm = (P[q].m + 63) / 64, tn = (P[q].n + 63) / 64;
+      for (int a = 0; a < tm; a++)
+        for (int b = 0; b < tn; b++) T.push_back(GemmTile{(int)q, a, b, 0});
+    }
+  };
+  tiles(p1, t1);
+  tiles(p2, t2);
+  if (p1.empty()) return NRG_OK;
+  DBuf& dp = c->buf("rot_probs");
+  DBuf& dt = c->buf("rot_tiles");
+  size_t np = p1.size() + p2.size(), nt = t1.size() + t2.size();
+  int rc;
+  if ((rc = dp.reserve(sizeof(GemmDesc) * np))) return rc;
+  if ((rc = dt.reserve(sizeof(GemmTile) * nt))) return rc;
+  std::vector<GemmDesc> pall(p1);
+  pall.insert(pall.end(), p2.begin(), p2.end());
+  std::vector<GemmTile> tall(t1);
+  for (auto& t : t2) tall.push_back(GemmTile{t.prob + (int)p1.size(), t.tm, t.tn, 0});
+  HIPCHK(hipMemcpyAsync(dp.p, pall.data(), sizeof(GemmDesc) * np, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(dt.p, tall.data(), sizeof(GemmTile) * nt, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  double fl = 0.0, by = 0.0;
+  for (auto& g : pall) {
+    fl += 2.0 * g.m * (double)g.n * g.k;
+    by += 8.0 * ((double)g.m * g.k + (double)g.n * g.k + (double)g.m * g.n);
+  }
+  PendingTimer tm;
+  if ((rc = c->timer_begin(tm, NRG_K_GEMM, fl, by))) return rc;
+  HIPCHK(nrgk_launch_gemm_nt((GemmDesc*)dp.p, (GemmTile*)dt.p, (int)t1.size(), dZ, dF, dAux, c->stream));
+  HIPCHK(nrgk_launch_gemm_nt((GemmDesc*)dp.p, (GemmTile*)dt.p + t1.size(), (int)t2.size(), dZ, dAux, dOut,
+                             c->stream));
+  if ((rc = c->timer_end(tm))) return rc;
+  return NRG_OK;
+}
+
+extern "C" int nrg_rotate_ops(nrg_ctx* c, int npairs, const nrg_pair_desc* pairs, const double* Z, int64_t zlen,
+                              const double* F, int64_t flen, double* out, int64_t outlen, int precision) {
+  if (!c || npairs < 0 || (npairs > 0 && (!pairs || !Z || !F || !out)))
+    return fail(NRG_ERR_ARG, "nrg_rotate_ops: bad arguments");
+  if (precision != 0) return fail(NRG_ERR_ARG, "nrg_rotate_ops: only precision 0 (FP64) is available");
+  HIPCHK(hipSetDevice(c->device));
+  std::vector<RotJob> jobs(npairs);
+  int64_t aux = 0;
+  for (int q = 0; q < npairs; q++) {
+    const nrg_pair_desc& P = pairs[q];
+    if (P.ni < 0 || P.nj < 0 || P.ki < 0 || P.kj < 0 || P.ki > P.ni || P.kj > P.nj)
+      return fail(NRG_ERR_ARG, "nrg_rotate_ops: bad pair shape");
+    if (P.zi_off + (int64_t)P.ki * P.ni > zlen || P.zj_off + (int64_t)P.kj * P.nj > zlen ||
+        P.f_off + (int64_t)P.ni * P.nj > flen || P.out_off + (int64_t)P.ki * P.kj > outlen)
+      return fail(NRG_ERR_ARG, "nrg_rotate_ops: pair exceeds buffer");
+    RotJob& J = jobs[q];
+    J.ni = P.ni; J.nj = P.nj; J.ki = P.ki; J.kj = P.kj;
+    J.zi_off = P.zi_off; J.zj_off = P.zj_off; J.ldzi = P.ni; J.ldzj = P.nj;
+    J.f_off = P.f_off; J.out_off = P.out_off; J.aux_off = aux;
+    aux += (int64_t)P.kj * P.ni;
+  }
+  DBuf& dZ = c->buf("rot_Z");
+  DBuf& dF = c->buf("rot_F");
+  DBuf& dA = c->buf("rot_aux");
+  DBuf& dO = c->buf("rot_out");
+  int rc;
+  if ((rc = dZ.reserve(sizeof(double) * (size_t)(zlen + 1)))) return rc;
+  if ((rc = dF.reserve(sizeof(double) * (size_t)(flen + 1)))) return rc;
+  if ((rc = dA.reserve(sizeof(double) * (size_t)(aux + 1)))) return rc;
+  if ((rc = dO.reserve(sizeof(double) * (size_t)(outlen + 1)))) return rc;
+  HIPCHK(hipMemcpyAsync(dZ.p, Z, sizeof(double) * zlen, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(dF.p, F, sizeof(double) * flen, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemsetAsync(dO.p, 0, sizeof(double) * (size_t)(outlen + 1), c->stream));
+  if ((rc = rotate_dev(c, jobs, (double*)dZ.p, (double*)dF.p, (double*)dA.p, (double*)dO.p))) return rc;
+  HIPCHK(hipMemcpyAsync(out, dO.p, sizeof(double) * outlen, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return NRG_OK;
+}

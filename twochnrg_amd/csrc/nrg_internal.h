@@ -1,0 +1,98 @@
+// nrg_internal.h -- shared host-side internals of the HIP library.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/twochnrg_amd.h"
+#include "nrg_kernels.h"
+
+namespace nrg {
+int fail(int code, const std::string& msg);
+}
+
+#define HIPCHK(x)                                                                               \
+  do {                                                                                          \
+    hipError_t _e = (x);                                                                        \
+    if (_e != hipSuccess)                                                                       \
+      return nrg::fail(NRG_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(_e));            \
+  } while (0)
+
+// grow-only device allocation
+struct DBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+  DBuf() = default;
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  ~DBuf();
+  int reserve(size_t bytes);
+  template <class T>
+  T* as() { return (T*)p; }
+};
+
+// per-kernel timing with HIP events on the context stream
+struct KStat {
+  double ms = 0.0;       // summed kernel time
+  long launches = 0;
+  double flops = 0.0;    // algorithmic flops of the timed launches
+  double bytes = 0.0;    // algorithmic bytes of the timed launches
+};
+
+struct PendingTimer {
+  hipEvent_t a, b;
+  int kind;
+  double flops, bytes;
+};
+
+struct nrg_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::map<std::string, std::unique_ptr<DBuf>> bufs;
+  DBuf& buf(const std::string& name);
+  bool timing = false;
+  KStat stat[NRG_NKSTAT];
+  std::vector<PendingTimer> pending;
+  std::vector<hipEvent_t> event_pool;
+  hipEvent_t get_event();
+  // record an event pair around a launch; resolve() after a stream sync
+  int timer_begin(PendingTimer& t, int kind, double flops, double bytes);
+  int timer_end(PendingTimer& t);
+  int resolve();
+};
+
+namespace nrg {
+
+// one rotation problem out(ki x kj) = Zi(ki x ni) F(ni x nj) Zj(kj x nj)^T
+struct RotJob {
+  int ni, nj, ki, kj;
+  int ldzi, ldzj;
+  int64_t zi_off, zj_off, f_off, aux_off, out_off;
+};
+
+int eig_dev(nrg_ctx* c, int nb, const int* n, const int64_t* a_off, double* dA, const int64_t* w_off, double* dW,
+            int* info_host);
+int rotate_dev(nrg_ctx* c, const std::vector<RotJob>& jobs, const double* dZ, const double* dF, double* dAux,
+               double* dOut);
+void set_degen(std::vector<double>& en, double eps);
+double ecut_of(const std::vector<double>& en, int ncut);
+
+// upload helper
+template <class T>
+int upload(nrg_ctx* c, const char* name, const std::vector<T>& v, T** dptr) {
+  DBuf& b = c->buf(name);
+  int rc = b.reserve(sizeof(T) * (v.size() + 1));
+  if (rc) return rc;
+  if (!v.empty()) {
+    // pageable source: wait for the copy before the caller may release it
+    HIPCHK(hipMemcpyAsync(b.p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  }
+  *dptr = (T*)b.p;
+  return 0;
+}
+
+}  // namespace nrg
