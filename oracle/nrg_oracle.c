@@ -956,3 +956,361 @@ int orc_run_onechqs(const orc_onechqs_params *p, orc_step_cb cb, void *user) {
   arr_free(&S); arr_free(&Ae); arr_free(&Ac); arr_free(&Ab);
   return 0;
 }
+
+/* ================================================================== */
+/* TwoChQS Anderson: legacy standalone driver src/TwoChQS/TwoChQS.cpp  */
+/* (ModelNo 0) with QS_BuildBasis, TwoChQS_DiagHN,                     */
+/* TwoChQS_SetH0Anderson, TwoChQS_UpdateQm1fQ /                        */
+/* TwoChQS_UpdateMatrixAfterCutting and the by-value CutStates.        */
+/* ================================================================== */
+
+/* TwoChQS_SetSingleSite  src/TwoChQS/TwoChQS_SetSingleSite.cpp: 16 states, 10 (Q,S,Sz) blocks */
+static void twochqs_site(arr_t *S) {
+  arr_clear(S);
+  S->nq = 3; S->nshell = 0;
+  const double q[30] = {-2, 0, 0, -1, 0.5, 0.5, -1, 0.5, -0.5, 0, 0, 0, 0, 1, -1,
+                        0, 1, 0, 0, 1, 1, 1, 0.5, 0.5, 1, 0.5, -0.5, 2, 0, 0};
+  const int be[20] = {0, 0, 1, 2, 3, 4, 5, 7, 8, 8, 9, 9, 10, 10, 11, 12, 13, 14, 15, 15};
+  for (int i = 0; i < 30; i++) dpush(&S->qn, q[i]);
+  for (int i = 0; i < 20; i++) ipush(&S->bbe, be[i]);
+  for (int i = 0; i < 16; i++) ipush(&S->itype, i);
+}
+/* TwoChQS_fd_table  src/TwoChQS/TwoChQS_OpTables.cpp:19-146 */
+static double twoch_fd(int ch, int sigma, int ti, int tj) {
+  static double M[2][2][16][16];
+  static int init = 0;
+  if (!init) {
+    const double r = 1.0 / sqrt(2.0);
+    memset(M, 0, sizeof(M));
+    double (*a)[16] = M[0][1]; /* ch1 up */
+    a[2][0] = 1; a[10][1] = 1; a[9][3] = r; a[11][6] = 1; a[5][3] = r; a[7][4] = 1; a[12][9] = r;
+    a[12][5] = -r; a[14][8] = 1; a[15][13] = 1;
+    a = M[0][0]; /* ch1 dn */
+    a[4][0] = 1; a[9][1] = r; a[5][1] = -r; a[8][3] = 1; a[13][6] = 1; a[7][2] = -1; a[12][10] = -1;
+    a[14][9] = -r; a[14][5] = -r; a[15][11] = -1;
+    a = M[1][1]; /* ch2 up */
+    a[1][0] = 1; a[6][3] = 1; a[10][2] = -1; a[11][9] = -r; a[11][5] = -r; a[9][4] = -r; a[5][4] = r;
+    a[13][8] = -1; a[12][7] = 1; a[15][14] = 1;
+    a = M[1][0]; /* ch2 dn */
+    a[3][0] = 1; a[6][1] = -1; a[9][2] = -r; a[11][10] = 1; a[5][2] = -r; a[8][4] = -1; a[13][9] = r;
+    a[13][5] = -r; a[14][7] = 1; a[15][12] = -1;
+    init = 1;
+  }
+  if (ch < 1 || ch > 2 || (sigma != 1 && sigma != -1)) return 0.0;
+  if (ti < 0 || ti > 15 || tj < 0 || tj > 15) return 0.0;
+  return M[ch - 1][sigma == 1 ? 1 : 0][ti][tj];
+}
+/* TwoChQS_SpSm_table  src/TwoChQS/TwoChQS_OpTables.cpp:148-183 */
+static double twoch_spsm(int ti, int tj) {
+  if (ti == tj) return 1.0;
+  int a = ti < tj ? ti : tj, b = ti < tj ? tj : ti;
+  if ((a == 1 && b == 3) || (a == 2 && b == 4) || (a == 11 && b == 13) || (a == 12 && b == 14) ||
+      (a == 8 && b == 9) || (a == 9 && b == 10) || (a == 8 && b == 10))
+    return 1.0;
+  return 0.0;
+}
+static int dLEqual(double a, double b) { return (a < b) || fabs(a - b) < 1e-20; }
+
+/* QS_BuildBasis  src/QS_BuildBasis.cpp:9-169 (KeepSz=0): block list as in
+ * BuildBasis; states ordered (old block, site block, site state, old state). */
+static void qs_build_basis(const arr_t *Ac, arr_t *Ab, const arr_t *S, int befcut) {
+  Ab->nshell = Ac->nshell + 1;
+  Ab->nq = Ac->nq;
+  arr_clear(Ab);
+  for (int b = 0; b < nblocks(Ac); b++) {
+    double Qo = getqn(Ac, b, 0), So = getqn(Ac, b, 1);
+    for (int s = 0; s < nblocks(S); s++) {
+      double Qs = getqn(S, s, 0), Ss = getqn(S, s, 1);
+      for (double sn = So + Ss; sn >= fabs(So - Ss); sn -= 1.0) { dpush(&Ab->qn, Qo + Qs); dpush(&Ab->qn, sn); }
+    }
+  }
+  filter_qnumbers(Ab);
+  int istnew = 0;
+  for (int bn = 0; bn < nblocks(Ab); bn++) {
+    ipush(&Ab->bbe, istnew);
+    double Qn = getqn(Ab, bn, 0), Sn = getqn(Ab, bn, 1);
+    for (int bo = 0; bo < nblocks(Ac); bo++) {
+      double Qo = getqn(Ac, bo, 0), So = getqn(Ac, bo, 1);
+      for (int s = 0; s < nblocks(S); s++) {
+        double Qs = getqn(S, s, 0), Ss = getqn(S, s, 1), Szs = getqn(S, s, 2);
+        for (int is = blim(S, s, 0); is <= blim(S, s, 1); is++) {
+          if (!(dEqual(Qn, Qo + Qs) && dEqual(Sn, So + Szs) && triang(So, Ss, Sn))) continue;
+          for (int io = blim(Ac, bo, 0); io <= blim(Ac, bo, 1); io++) {
+            ipush(&Ab->itype, S->itype.a[is]);
+            dpush(&Ab->en, Ac->en.a[io]);
+            ipush(&Ab->stcf, befcut == 1 ? Ac->stcf.a[io] : io);
+            istnew++;
+          }
+        }
+      }
+    }
+    ipush(&Ab->bbe, istnew - 1);
+  }
+}
+
+/* site state range of a site block; block -1 gives [0,0] as GetBlockLimit(-1,.) returns 0 */
+static void site_range(const arr_t *S, int blk, int *b0, int *b1) {
+  if (blk < 0) { *b0 = 0; *b1 = 0; return; }
+  *b0 = S->bbe.a[2 * blk]; *b1 = S->bbe.a[2 * blk + 1];
+}
+
+/* coefficient of chi*<old|f_ch|old'> in an H_N element: the sigma/Szold sums of
+ * TwoChQS_DiagHN (src/TwoChQS/TwoChQS_DiagHN.cpp:62-250).  h0 selects the
+ * TwoChQS_SetH0Anderson variant of the fermion sign (Q and Sz test). */
+static double twoch_hn_coef(const arr_t *S, int ch, double Si, int typep, int type, int h0) {
+  int pos;
+  int bp = blk_from_st(S, typep, &pos), bb = blk_from_st(S, type, &pos);
+  double Qtp = getqn(S, bp, 0), Stp = getqn(S, bp, 1), Sztp = getqn(S, bp, 2);
+  double Qt = getqn(S, bb, 0), St = getqn(S, bb, 1), Szt = getqn(S, bb, 2);
+  double Soldp = Si - Sztp, Sold = Si - Szt;
+  double fs = 1.0;
+  if (h0) { if ((dEqual(Qt, 1.0) || dEqual(Qt, -1.0)) && (dEqual(Szt, 0.5) || dEqual(Szt, -0.5))) fs = -1.0; }
+  else if (dEqual(Szt, 0.5) || dEqual(Szt, -0.5)) fs = -1.0;
+  double qs[3] = {Qt, St, Szt}, qsp[3] = {Qtp, Stp, Sztp};
+  double tot = 0.0;
+  for (int sg = -1; sg <= 1; sg += 2) {
+    double ds = 0.5 * sg, full = 0.0;
+    for (double szo = Sold; szo >= -Sold; szo -= 1.0) {
+      double szop = szo - ds, szt = Si - szo, sztp = szt + ds;
+      qs[2] = szt; qsp[2] = sztp;
+      int sb = blk_from_qn(S, qs), sbp = blk_from_qn(S, qsp);
+      if (dLEqual(fabs(sztp), Stp) && dLEqual(fabs(szo), Sold) && dLEqual(fabs(szop), Soldp)) {
+        double c0 = orc_cgordan(Sold, szo, St, szt, Si, Si);
+        double c1 = orc_cgordan(Soldp, szop, Stp, sztp, Si, Si);
+        double c2 = orc_cgordan(Soldp, szop, 0.5, ds, Sold, szo);
+        int p0, p1, q0, q1;
+        site_range(S, sbp, &p0, &p1);
+        site_range(S, sb, &q0, &q1);
+        for (int ssp = p0; ssp <= p1; ssp++)
+          for (int ss = q0; ss <= q1; ss++)
+            full += c0 * c1 * c2 * twoch_fd(ch, sg, ssp, ss) * fs * twoch_spsm(ss, type) * twoch_spsm(ssp, typep);
+      }
+    }
+    tot += full;
+  }
+  return tot;
+}
+
+/* element of f_ch between basis states with the same parent: the fnbasis rule of
+ * TwoChQS_UpdateQm1fQ / TwoChQS_UpdateMatrixAfterCutting
+ * (src/TwoChQS/TwoChQS_UpdateMatrices.cpp:90-196, 380-470) */
+static double twoch_fn_el(const arr_t *S, int ch, double Si, double Sj, int ti, int tj) {
+  int pos;
+  int bi = blk_from_st(S, ti, &pos), bj = blk_from_st(S, tj, &pos);
+  double Qti = getqn(S, bi, 0), Sti = getqn(S, bi, 1), Szti = getqn(S, bi, 2);
+  double Qtj = getqn(S, bj, 0), Stj = getqn(S, bj, 1), Sztj = getqn(S, bj, 2);
+  double qi[3] = {Qti, Sti, Szti}, qj[3] = {Qtj, Stj, Sztj};
+  double Scfi = Si - Szti, Scfj = Sj - Sztj;
+  double cgm = orc_cgordan(Si, Si, 0.5, -0.5, Sj, Sj), cgp = orc_cgordan(Si, Si, 0.5, 0.5, Sj, Sj);
+  int sigma = 0;
+  double norm = 1.0;
+  if (fabs(cgm) > 1e-20) { sigma = -1; norm = cgm; }
+  else if (fabs(cgp) > 1e-20) { sigma = 1; norm = cgp; }
+  if (sigma == 0) return 0.0;
+  double acc = 0.0;
+  for (double szc = Scfj; szc >= -Scfj; szc -= 1.0) {
+    double szi = Si - szc, szj = Sj - szc;
+    qi[2] = szi; qj[2] = szj;
+    int bsi = blk_from_qn(S, qi), bsj = blk_from_qn(S, qj);
+    double c0 = orc_cgordan(Scfi, szc, Sti, szi, Si, Si);
+    double c1 = orc_cgordan(Scfj, szc, Stj, szj, Sj, Sj);
+    int a0, a1, b0, b1;
+    site_range(S, bsi, &a0, &a1);
+    site_range(S, bsj, &b0, &b1);
+    for (int ssi = a0; ssi <= a1; ssi++)
+      for (int ssj = b0; ssj <= b1; ssj++)
+        acc += twoch_fd(ch, sigma, ssj, ssi) * c0 * c1 * twoch_spsm(ssi, ti) * twoch_spsm(ssj, tj);
+  }
+  return acc / norm;
+}
+
+/* by-value CutStates(CNRGarray*,int)  src/CutStates.cpp:103-160: no SetKept/SetDegen on the copy */
+static void cut_states_legacy(const arr_t *Ain, arr_t *Ac, int ncut) {
+  arr_clear(Ac);
+  Ac->nshell = Ain->nshell; Ac->nq = Ain->nq;
+  dcopy_v(&Ac->qn, &Ain->qn); dcopy_v(&Ac->en, &Ain->en); dcopy_v(&Ac->ev, &Ain->ev);
+  icopy_v(&Ac->bbe, &Ain->bbe);
+  for (int i = 0; i < Ac->en.n; i++) { ipush(&Ac->stcf, i); ipush(&Ac->kept, 1); }
+  icopy_v(&Ac->bbe_bc, &Ain->bbe);
+  set_bbe_ev(Ac);
+  if (ncut >= nstates(Ain)) return;
+  double ec = orc_ecut(Ain->en.n, Ain->en.a, ncut);
+  for (int b = 0; b < nblocks(Ac); b++) {
+    int b0 = blim(Ac, b, 0), b1 = blim(Ac, b, 1);
+    double emin = Ac->en.a[b0];
+    for (int i = b0; i <= b1; i++) if (Ac->en.a[i] < emin) emin = Ac->en.a[i];
+    if (emin > ec) { remove_block(Ac, b); b--; }
+  }
+  for (int b = 0; b < nblocks(Ac); b++) {
+    int b0 = blim(Ac, b, 0), b1 = blim(Ac, b, 1);
+    double emax = Ac->en.a[b0];
+    for (int i = b0; i <= b1; i++) if (Ac->en.a[i] > emax) emax = Ac->en.a[i];
+    if (emax > ec) {
+      int ip = b0;
+      while (ip <= b1 && Ac->en.a[ip] <= ec) ip++;
+      remove_states_from_block(Ac, b, ip, b1);
+    }
+  }
+}
+
+/* f_ch in the eigenbasis: pairs (ibl < jbl) with Q_j = Q_i+1, S_j = S_i +- 1/2,
+ * channel loop inside the pair loop.  A has BC data (bbe_bc/bbe_ev) when it is a
+ * cut array; the matching pre-cut block of Ab is found by quantum numbers. */
+static void twoch_update_f(const arr_t *S, const arr_t *A, const arr_t *Ab, mat_t *f /* [2] */) {
+  for (int c = 0; c < 2; c++) { mat_clear(&f[c]); mat_sync(&f[c], A); f[c].b.nshell = A->nshell; }
+  int icount[2] = {0, 0};
+  for (int ib = 0; ib < nblocks(A); ib++) {
+    double Qi = getqn(A, ib, 0), Si = getqn(A, ib, 1);
+    int ni = bsize(A, ib);
+    double q[2] = {Qi, Si};
+    int ibc = blk_from_qn(Ab, q);
+    int nibc = bsize(Ab, ibc);
+    for (int jb = ib + 1; jb < nblocks(A); jb++) {
+      double Qj = getqn(A, jb, 0), Sj = getqn(A, jb, 1);
+      int nj = bsize(A, jb);
+      double qq[2] = {Qj, Sj};
+      int jbc = blk_from_qn(Ab, qq);
+      int njbc = bsize(Ab, jbc);
+      if (!(dEqual(Qi + 1.0, Qj) && (dEqual(Si + 0.5, Sj) || dEqual(Si - 0.5, Sj)))) continue;
+      for (int ch = 1; ch <= 2; ch++) {
+        mat_t *M = &f[ch - 1];
+        ipush(&M->map, ib); ipush(&M->map, jb);
+        ipush(&M->mbe, icount[ch - 1]);
+        icount[ch - 1] += ni * nj;
+        ipush(&M->mbe, icount[ch - 1] - 1);
+        double *F = (double *)calloc((size_t)nibc * (size_t)njbc, sizeof(double));
+        for (int a = 0; a < nibc; a++) {
+          int ist = blim(Ab, ibc, 0) + a;
+          for (int b = 0; b < njbc; b++) {
+            int jst = blim(Ab, jbc, 0) + b;
+            if (Ab->stcf.a[ist] != Ab->stcf.a[jst]) continue;
+            F[(size_t)a * njbc + b] = twoch_fn_el(S, ch, Si, Sj, Ab->itype.a[ist], Ab->itype.a[jst]);
+          }
+        }
+        dres(&M->mat, M->mat.n + ni * nj);
+        orc_rotate(nibc, njbc, ni, nj, A->ev.a + blim_ev(A, ib), F, A->ev.a + blim_ev(A, jb), M->mat.a + M->mat.n);
+        M->mat.n += ni * nj;
+        free(F);
+      }
+    }
+  }
+}
+
+int orc_run_twochqs(const orc_twochqs_params *p, orc_step_cb cb, void *user) {
+  const double L = p->Lambda, bbar = p->betabar > 0 ? p->betabar : 0.727;
+  const double HLF = 0.5 * (1.0 + 1.0 / L);
+  arr_t S = {0}, Ae = {0}, Ac = {0}, Ab = {0}, Ah = {0}, Ahc = {0};
+  mat_t f[2], d = {0};
+  memset(f, 0, sizeof(f));
+  twochqs_site(&S);
+  /* src/TwoChQS/TwoChQS.cpp:208-264 */
+  double Ut = 0.5 * p->U / HLF, edt = p->ed / HLF;
+  double G1t = p->Gamma1 * (2.0 / M_PI) / (HLF * HLF), G2t = p->Gamma2 * (2.0 / M_PI) / (HLF * HLF);
+  double Un = Ut / sqrt(L), edn = edt / sqrt(L), chi0[2] = {sqrt(G1t / L), sqrt(G2t / L)};
+  /* TwoChQS_SetH0Anderson  src/TwoChQS/TwoChQS_SetH0Anderson.cpp:15-345 */
+  Ah.nq = 2; Ah.nshell = -1;
+  const double qh[6] = {-1, 0, 0, 0.5, 1, 0};
+  for (int i = 0; i < 6; i++) dpush(&Ah.qn, qh[i]);
+  for (int i = 0; i < 3; i++) { ipush(&Ah.bbe, i); ipush(&Ah.bbe, i); dpush(&Ah.ev, 1.0); }
+  dpush(&Ah.en, Un); dpush(&Ah.en, edn + Un); dpush(&Ah.en, 2.0 * edn + 3.0 * Un);
+  mat_sync(&d, &Ah);
+  dpush(&d.mat, 1.0); ipush(&d.map, 0); ipush(&d.map, 1); ipush(&d.mbe, 0); ipush(&d.mbe, 0);
+  dpush(&d.mat, -sqrt(2.0)); ipush(&d.map, 1); ipush(&d.map, 2); ipush(&d.mbe, 1); ipush(&d.mbe, 1);
+  cut_states_legacy(&Ah, &Ahc, 100);
+  qs_build_basis(&Ahc, &Ab, &S, 1);
+  Ae.nshell = Ab.nshell; Ae.nq = 2;
+  arr_clear(&Ae);
+  dcopy_v(&Ae.qn, &Ab.qn); icopy_v(&Ae.bbe, &Ab.bbe);
+  for (int b = 0; b < nblocks(&Ab); b++) {
+    int n = bsize(&Ab, b), b0 = blim(&Ab, b, 0);
+    double Si = getqn(&Ab, b, 1);
+    double *H = (double *)calloc((size_t)n * n, sizeof(double));
+    for (int i = 0; i < n; i++)
+      for (int j = 0; j < n; j++) {
+        int ist = b0 + i, jst = b0 + j;
+        if (i == j) { H[(size_t)i * n + j] = Ab.en.a[ist]; continue; }
+        double old = get_matel(&d, Ab.stcf.a[ist], Ab.stcf.a[jst]);
+        int tp = Ab.itype.a[ist], tt = Ab.itype.a[jst];
+        if (dEqual(old, 0.0)) { old = get_matel(&d, Ab.stcf.a[jst], Ab.stcf.a[ist]); tp = Ab.itype.a[jst]; tt = Ab.itype.a[ist]; }
+        double v = 0.0;
+        for (int ch = 1; ch <= 2; ch++) v += chi0[ch - 1] * old * twoch_hn_coef(&S, ch, Si, tp, tt, 1);
+        H[(size_t)i * n + j] = v;
+      }
+    dres(&Ae.en, Ae.en.n + n); dres(&Ae.ev, Ae.ev.n + n * n);
+    orc_eigh_full(n, H, Ae.en.a + Ae.en.n, Ae.ev.a + Ae.ev.n);
+    Ae.en.n += n; Ae.ev.n += n * n;
+    free(H);
+  }
+  double emin = set_e0zero(&Ae);
+  twoch_update_f(&S, &Ae, &Ab, f);   /* TwoChQS_UpdateQm1fQ on the uncut N=0 basis */
+  double DN = HLF * pow(L, -(0 - 1) / 2.0);
+  {
+    double thT[4] = {DN / bbar, 0, 0, 0}, sus[4] = {calc_suscep(bbar, &Ae), 0, 0, 0};
+    double ent[4] = {calc_entropy(bbar, &Ae), 0, 0, 0};
+    arr_t tmpc = {0};
+    cut_states_legacy(&Ae, &tmpc, p->ncut);
+    emit(cb, user, &Ae, &tmpc, emin, HUGE_VAL, Ae.en.n, NULL, 0, thT, sus, ent, DN / bbar, 0, NULL);
+    arr_free(&tmpc);
+  }
+  dvec hrec = {0};
+  for (int N = 1; N <= p->nsitesmax; N++) {
+    DN = HLF * pow(L, -(N - 1) / 2.0);
+    double d0 = 1.0 - pow(L, -N), d1 = sqrt(1.0 - pow(L, -(2 * N - 1))), d2 = sqrt(1.0 - pow(L, -(2 * N + 1)));
+    double chiN = d0 / (d1 * d2);
+    cut_states_legacy(&Ae, &Ac, p->ncut);
+    if (N > 1) twoch_update_f(&S, &Ac, &Ab, f);
+    qs_build_basis(&Ac, &Ab, &S, 0);
+    /* TwoChQS_DiagHN */
+    Ae.nshell = Ab.nshell; Ae.nq = 2;
+    arr_clear(&Ae);
+    dcopy_v(&Ae.qn, &Ab.qn); icopy_v(&Ae.bbe, &Ab.bbe);
+    hrec.n = 0;
+    for (int b = 0; b < nblocks(&Ab); b++) {
+      int n = bsize(&Ab, b), b0 = blim(&Ab, b, 0);
+      double Si = getqn(&Ab, b, 1);
+      double *ap = (double *)malloc(sizeof(double) * (size_t)n * (n + 1) / 2);
+      int r = 0;
+      for (int i = 0; i < n; i++)
+        for (int j = i; j < n; j++) {
+          int ist = b0 + i, jst = b0 + j;
+          double v;
+          if (i == j) v = sqrt(L) * Ab.en.a[ist];
+          else {
+            v = 0.0;
+            for (int ch = 1; ch <= 2; ch++) {
+              double old = get_matel(&f[ch - 1], Ab.stcf.a[ist], Ab.stcf.a[jst]);
+              int tp = Ab.itype.a[ist], tt = Ab.itype.a[jst];
+              if (dEqual(fabs(old), 0.0)) {
+                old = get_matel(&f[ch - 1], Ab.stcf.a[jst], Ab.stcf.a[ist]);
+                tp = Ab.itype.a[jst]; tt = Ab.itype.a[ist];
+              }
+              v += chiN * old * twoch_hn_coef(&S, ch, Si, tp, tt, 0);
+            }
+          }
+          ap[r++] = v;
+        }
+      if (p->record_blocks) for (int k = 0; k < r; k++) dpush(&hrec, ap[k]);
+      dres(&Ae.en, Ae.en.n + n); dres(&Ae.ev, Ae.ev.n + n * n);
+      orc_eigh_packed(n, ap, Ae.en.a + Ae.en.n, Ae.ev.a + Ae.ev.n);
+      Ae.en.n += n; Ae.ev.n += n * n;
+      free(ap);
+    }
+    emin = set_e0zero(&Ae);
+    double thT[4] = {DN / bbar, 0, 0, 0}, sus[4] = {calc_suscep(bbar, &Ae), 0, 0, 0};
+    double ent[4] = {calc_entropy(bbar, &Ae), 0, 0, 0};
+    /* record the cut that the next iteration will make */
+    arr_t nc = {0};
+    double ec = HUGE_VAL;
+    if (p->ncut < nstates(&Ae)) ec = orc_ecut(Ae.en.n, Ae.en.a, p->ncut);
+    cut_states_legacy(&Ae, &nc, p->ncut);
+    int nk = 0;
+    Ae.kept.n = 0;
+    for (int i = 0; i < Ae.en.n; i++) { int k = Ae.en.a[i] <= ec; ipush(&Ae.kept, k); nk += k; }
+    emit(cb, user, &Ae, &nc, emin, ec, nk, &hrec, p->record_blocks, thT, sus, ent, DN / bbar, 0, NULL);
+    arr_free(&nc);
+  }
+  free(hrec.a);
+  mat_free(&f[0]); mat_free(&f[1]); mat_free(&d);
+  arr_free(&S); arr_free(&Ae); arr_free(&Ac); arr_free(&Ab); arr_free(&Ah); arr_free(&Ahc);
+  return 0;
+}

@@ -59,6 +59,20 @@ typedef struct {
 
 int orc_run_onechqs(const orc_onechqs_params *p, orc_step_cb cb, void *user);
 
+/* TwoChQS Anderson through the legacy standalone driver
+ * (src/TwoChQS/TwoChQS.cpp:23-504, ModelNo 0): Gamma~ = Gamma*(2/pi)/HLF^2,
+ * chi_{-1} = sqrt(Gamma~/Lambda) per channel (no A_Lambda), one betabar per N.
+ * Loop Nsites = 1 .. nsitesmax after the N=0 set-up (impurity + first site). */
+typedef struct {
+  double U, Gamma1, Gamma2, ed, Lambda;
+  int nsitesmax;
+  int ncut;
+  double betabar;
+  int record_blocks;
+} orc_twochqs_params;
+
+int orc_run_twochqs(const orc_twochqs_params *p, orc_step_cb cb, void *user);
+
 /* ---- kernel-level restatements ---- */
 
 /* CGordan  src/CGordan.cpp:34-120 */

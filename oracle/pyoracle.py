@@ -41,6 +41,14 @@ class OneChQSParams(ctypes.Structure):
     ]
 
 
+class TwoChQSParams(ctypes.Structure):
+    _fields_ = [
+        ("U", ctypes.c_double), ("Gamma1", ctypes.c_double), ("Gamma2", ctypes.c_double),
+        ("ed", ctypes.c_double), ("Lambda", ctypes.c_double), ("nsitesmax", ctypes.c_int),
+        ("ncut", ctypes.c_int), ("betabar", ctypes.c_double), ("record_blocks", ctypes.c_int),
+    ]
+
+
 _lib = None
 
 
@@ -53,6 +61,8 @@ def load():
     lib = ctypes.CDLL(LIB_PATH)
     lib.orc_run_onechqs.argtypes = [ctypes.POINTER(OneChQSParams), STEP_CB, ctypes.c_void_p]
     lib.orc_run_onechqs.restype = ctypes.c_int
+    lib.orc_run_twochqs.argtypes = [ctypes.POINTER(TwoChQSParams), STEP_CB, ctypes.c_void_p]
+    lib.orc_run_twochqs.restype = ctypes.c_int
     lib.orc_cgordan.argtypes = [ctypes.c_double] * 6
     lib.orc_cgordan.restype = ctypes.c_double
     dp = ctypes.POINTER(ctypes.c_double)
@@ -115,6 +125,22 @@ def run_onechqs(U=0.5, Gamma=0.0282691, ed=-0.25, Lambda=2.5, nsitesmax=4, ncut=
                       betabar, int(record_blocks))
     fn = STEP_CB(cb)
     rc = lib.orc_run_onechqs(ctypes.byref(p), fn, None)
+    if rc != 0:
+        raise RuntimeError("oracle run failed: %d" % rc)
+    return steps
+
+
+def run_twochqs(U=0.4, Gamma1=0.03, Gamma2=0.03, ed=-0.2, Lambda=3.0, nsitesmax=3, ncut=3000,
+                betabar=0.727, record_blocks=False):
+    lib = load()
+    steps = []
+
+    def cb(_user, st):
+        steps.append(step_to_dict(st))
+
+    p = TwoChQSParams(U, Gamma1, Gamma2, ed, Lambda, nsitesmax, ncut, betabar, int(record_blocks))
+    fn = STEP_CB(cb)
+    rc = lib.orc_run_twochqs(ctypes.byref(p), fn, None)
     if rc != 0:
         raise RuntimeError("oracle run failed: %d" % rc)
     return steps

@@ -47,8 +47,11 @@ def test_golden_config(ctx):
     gs, _ = ctx.run_onechqs(**kw, record_blocks=True)
     os_ = po.run_onechqs(**kw, record_blocks=True)
     _compare(gs, os_)
-    for g, o in zip(gs[1:], os_[1:]):  # H_N blocks assembled on the device
-        assert np.max(np.abs(g["hpacked"] - o["hpacked"])) <= 1e-13 * max(1, np.abs(o["hpacked"]).max())
+    # H_0 assembled on the device: element-wise equal (its f_{-1} has no gauge
+    # freedom).  Later H_N depend on the previous eigenvector signs, which are
+    # arbitrary; those steps are compared through their spectra above.
+    g, o = gs[1], os_[1]
+    assert np.max(np.abs(g["hpacked"] - o["hpacked"])) <= 1e-13 * max(1, np.abs(o["hpacked"]).max())
 
 
 @pytest.mark.parametrize("calcdens", [2, 1])
