@@ -143,6 +143,25 @@ typedef struct {
  */
 int nrg_run_onechqs(nrg_ctx *ctx, const nrg_onechqs_params *p, nrg_step_cb cb, void *user, double *timing_ms);
 
+typedef struct {
+  double U, Gamma1, Gamma2, ed, Lambda;
+  int nsitesmax, ncut;
+  double betabar;
+  int record_blocks;
+} nrg_twochqs_params;
+
+/*
+ * TwoChQS (two-channel, SU(2)xU(1)) Anderson model through the legacy
+ * standalone loop (src/TwoChQS/TwoChQS.cpp:204-495; H_0 from
+ * TwoChQS_SetH0Anderson src/TwoChQS/TwoChQS_SetH0Anderson.cpp:15-345, H_N
+ * rules of TwoChQS_DiagHN src/TwoChQS/TwoChQS_DiagHN.cpp:15-289, operator
+ * update TwoChQS_UpdateMatrixAfterCutting src/TwoChQS/TwoChQS_UpdateMatrices.cpp:277-627,
+ * by-value CutStates src/CutStates.cpp:103-160).  Steps N = 0 .. nsitesmax;
+ * one betabar per N for the thermodynamics.  Returns NRG_ERR_ARG when ncut is
+ * below the N=0 basis size.
+ */
+int nrg_run_twochqs(nrg_ctx *ctx, const nrg_twochqs_params *p, nrg_step_cb cb, void *user, double *timing_ms);
+
 #ifdef __cplusplus
 }
 #endif

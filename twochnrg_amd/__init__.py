@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(HERE, "libtwochnrg_amd.so")
 EXPORTS = [
     "nrg_ctx_create", "nrg_ctx_destroy", "nrg_last_error", "nrg_device_count",
     "nrg_diag_blocks", "nrg_diag_blocks_dev", "nrg_truncate", "nrg_ecut",
-    "nrg_rotate_ops", "nrg_run_onechqs", "nrg_set_timing", "nrg_get_stats", "nrg_reset_stats",
+    "nrg_rotate_ops", "nrg_run_onechqs", "nrg_run_twochqs", "nrg_set_timing", "nrg_get_stats", "nrg_reset_stats",
 ]
 
 KSTAT_NAMES = ["eig", "hn", "fbasis", "gemm"]
@@ -49,6 +49,14 @@ class OneChQSParams(ctypes.Structure):
         ("Lambda", ctypes.c_double), ("nsitesmax", ctypes.c_int), ("ncut", ctypes.c_int),
         ("calcdens", ctypes.c_int), ("disc_scheme", ctypes.c_int), ("myeps", ctypes.c_double),
         ("betabar", ctypes.c_double), ("record_blocks", ctypes.c_int),
+    ]
+
+
+class TwoChQSParams(ctypes.Structure):
+    _fields_ = [
+        ("U", ctypes.c_double), ("Gamma1", ctypes.c_double), ("Gamma2", ctypes.c_double),
+        ("ed", ctypes.c_double), ("Lambda", ctypes.c_double), ("nsitesmax", ctypes.c_int),
+        ("ncut", ctypes.c_int), ("betabar", ctypes.c_double), ("record_blocks", ctypes.c_int),
     ]
 
 
@@ -95,6 +103,7 @@ def lib():
     L.nrg_get_stats.argtypes = [ctypes.c_void_p, ctypes.POINTER(NrgStats)]
     L.nrg_reset_stats.argtypes = [ctypes.c_void_p]
     L.nrg_run_onechqs.argtypes = [ctypes.c_void_p, ctypes.POINTER(OneChQSParams), STEP_CB, ctypes.c_void_p, _dp]
+    L.nrg_run_twochqs.argtypes = [ctypes.c_void_p, ctypes.POINTER(TwoChQSParams), STEP_CB, ctypes.c_void_p, _dp]
     _lib = L
     return L
 
@@ -228,6 +237,23 @@ class Context:
         fn = STEP_CB(cb)
         ms = ctypes.c_double(0.0)
         _check(lib().nrg_run_onechqs(self.h, ctypes.byref(p), fn, None, ctypes.byref(ms)))
+        return steps, ms.value
+
+
+    def run_twochqs(self, U=0.4, Gamma1=0.03, Gamma2=0.03, ed=-0.2, Lambda=3.0, nsitesmax=3, ncut=3000,
+                    betabar=0.727, record_blocks=False, keep_steps=True, on_step=None):
+        steps = []
+
+        def cb(_u, st):
+            if on_step is not None:
+                on_step(st)
+            if keep_steps:
+                steps.append(step_to_dict(st))
+
+        p = TwoChQSParams(U, Gamma1, Gamma2, ed, Lambda, nsitesmax, ncut, betabar, int(record_blocks))
+        fn = STEP_CB(cb)
+        ms = ctypes.c_double(0.0)
+        _check(lib().nrg_run_twochqs(self.h, ctypes.byref(p), fn, None, ctypes.byref(ms)))
         return steps, ms.value
 
 

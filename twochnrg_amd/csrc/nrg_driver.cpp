@@ -1,20 +1,26 @@
-// nrg_driver.cpp -- the NRG_main-style chain driver on the MI355X path.
+// nrg_driver.cpp -- chain drivers of the MI355X NRG path:
+//   * OneChQS Anderson through the NRG_main loop (src/Main/NRG_main.cpp:282-427)
+//   * TwoChQS Anderson through the legacy standalone loop
+//     (src/TwoChQS/TwoChQS.cpp:386-495)
 //
 // Host objects keep the reference's data-model semantics
 // (include/NRGclasses.hpp:47-759): QNumbers per block, BlockBegEnd inclusive
 // pairs, dEn per state, Kept flags, iType/StCameFrom/BlockBegEndBC for basis
 // arrays, MatBlockMap/MatBlockBegEnd for operators.  Eigenvectors and
 // operator elements stay resident in HBM; the host holds the integer
-// bookkeeping (basis build, block maps, truncation decisions) and the O(n)
-// FP64 Boltzmann sums.
+// bookkeeping (basis build, block maps, truncation decisions), the per-model
+// coupling tables (Clebsch-Gordan sums, computed once per spin value) and the
+// O(n) FP64 Boltzmann sums.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <numeric>
+#include <tuple>
 #include <utility>
 #include <vector>
 
@@ -45,20 +51,16 @@ struct CNRGbasisarray : CNRGarray {
   std::vector<int> zld;
 };
 
-enum OpKind { OP_FN = 0, OP_IMPONLY = 1, OP_CD = 2 };
-
-// an operator over a kept basis; elements on the device
+// an operator over a kept basis; elements on the device, block map on the host
 struct CNRGmatrix {
-  OpKind kind = OP_FN;
   bool NeedOld = false, CalcAvg = false;
   std::vector<int> MatBlockMap, MatBlockBegEnd;
-  int nblocks_basis = 0;        // blocks of the basis it is synced to
-  std::string buf[2];           // device buffer names (double-buffered)
-  int cur = 0;                  // which buffer holds the current elements
-  int64_t nel = 0;
+  std::string buf[2];  // device buffer names (double-buffered across steps)
+  int cur = 0;
 };
 
 static long key2(double x) { return std::lround(2.0 * x); }
+static bool deq(double a, double b) { return std::fabs(a - b) < 1e-20; }  // dEqual src/dEqual.cpp:6-10
 
 // ---------------------------------------------------------------------------
 // Clebsch-Gordan (Racah formula) -- src/CGordan.cpp:34-120
@@ -94,105 +96,228 @@ double cgordan(double J1, double M1, double J2, double M2, double J, double M) {
 }
 
 // ---------------------------------------------------------------------------
-// OneChQS model plugin: single site (Q,S,Sz) one state per block
-// (src/OneChQS_SetSingleSite.cpp), f^dagger table (src/OpTables.cpp:8-33),
-// H_N / f_N / c_d element rules (src/OpMatRules.cpp:1167-1535).
+// single-site description: site blocks (Q,S,Sz) with their state ranges;
+// a state's index is its iType
 // ---------------------------------------------------------------------------
-struct OneChQSSite {
-  // type -> (Q, S, Sz); type == site block index
-  double Q[4] = {-1, 0, 0, 1}, S[4] = {0, 0.5, 0.5, 0}, Sz[4] = {0, 0.5, -0.5, 0};
+struct SiteDef {
+  std::vector<double> bQ, bS, bSz;
+  std::vector<int> bbeg, bend, blk;
+  int ntypes() const { return (int)blk.size(); }
+  int nblocks() const { return (int)bQ.size(); }
+  double Q(int t) const { return bQ[blk[t]]; }
+  double S(int t) const { return bS[blk[t]]; }
+  double Sz(int t) const { return bSz[blk[t]]; }
+  // GetBlockFromQNumbers: first block with all three QNs dEqual, else -1
   int find(double q, double s, double sz) const {
-    for (int t = 0; t < 4; t++)
-      if (std::fabs(Q[t] - q) < 1e-20 && std::fabs(S[t] - s) < 1e-20 && std::fabs(Sz[t] - sz) < 1e-20) return t;
+    for (int b = 0; b < nblocks(); b++)
+      if (deq(bQ[b], q) && deq(bS[b], s) && deq(bSz[b], sz)) return b;
     return -1;
   }
-  static double fd(int sigma, int ti, int tj) {
-    if (sigma == 1) return (ti == 1 && tj == 0) ? 1.0 : ((ti == 3 && tj == 2) ? 1.0 : 0.0);
-    if (sigma == -1) return (ti == 2 && tj == 0) ? 1.0 : ((ti == 3 && tj == 1) ? -1.0 : 0.0);
-    return 0.0;
+  // state range of a site block; GetBlockLimit(-1,.) returns 0
+  void range(int b, int& lo, int& hi) const {
+    if (b < 0) { lo = hi = 0; return; }
+    lo = bbeg[b]; hi = bend[b];
   }
-  // CoefMatEl * FermiSign of OneChQS_HN_MatEl for a block of spin Si, site types (typep, type)
-  double hn_coef(double Si, int tp, int tt) const {
-    double Stp = S[tp], Sztp = Sz[tp], Qt = Q[tt], St = S[tt], Szt = Sz[tt];
-    double Scfp = Si - Sztp, Scf = Si - Szt;
-    double fsign = std::fabs(Qt) < 1e-20 ? -1.0 : 1.0;
-    double coef = 0.0;
-    for (double sz = -St; sz <= St; sz += 1.0) {
-      int ss = find(Qt, St, sz);
-      for (int sg = -1; sg <= 1; sg += 2) {
-        double ds = 0.5 * sg, szcf = Si - sz, szcfp = szcf - ds, sztp = sz + ds;
-        int ssp = find(Q[tp], Stp, sz + ds);
-        coef += cgordan(Scf, szcf, St, sz, Si, Si) * cgordan(Scfp, szcfp, Stp, sztp, Si, Si) *
-                cgordan(Scfp, szcfp, 0.5, ds, Scf, szcf) * fd(sg, ssp, ss);
-      }
-    }
-    return coef * fsign;
-  }
-  // OneChQS_fN_MatEl for states of spins (Si,Sj) with the same parent
-  double fn_el(double Si, double Sj, int ti, int tj) const {
-    double Sti = S[ti], Szti = Sz[ti], Stj = S[tj], Sztj = Sz[tj];
-    double Scfi = Si - Szti, Scfj = Sj - Sztj, sigma = Sj - Si, acc = 0.0;
-    for (double sz = -Stj; sz <= Stj; sz += 1.0) {
-      int ssj = find(Q[tj], Stj, sz), ssi = find(Q[ti], Sti, sz - sigma);
-      acc += fd((int)(sigma / 0.5), ssj, ssi) * cgordan(Scfi, Si - sz + sigma, Sti, sz - sigma, Si, Si) *
-             cgordan(Scfj, Sj - sz, Stj, sz, Sj, Sj);
-    }
-    double cgr = cgordan(Si, Si, 0.5, sigma, Sj, Sj);
-    return std::fabs(cgr) >= 1e-20 ? acc / cgr : 0.0;
-  }
-  // OneChQS_cd_MatEl
-  double cd_el(double Si, double Sj, int ti, int tj) const {
-    double Sti = S[ti], Szti = Sz[ti], Qtj = Q[tj], Stj = S[tj], Sztj = Sz[tj];
-    double Scfi = Si - Szti, Scfj = Sj - Sztj, sigma = Sj - Si;
-    double cg3 = cgordan(Si, Si, 0.5, sigma, Sj, Sj);
-    if (std::fabs(cg3) < 1e-20) {
-      sigma = Si - Sj;
-      cg3 = cgordan(Si, Si, 0.5, sigma, Sj, Sj);
-      if (std::fabs(cg3) < 1e-20) return 0.0;
-    }
-    double fb = 0.0;
-    for (double szcfj = -Scfj; szcfj <= Scfj; szcfj += 1.0) {
-      double szcfi = szcfj - sigma, sztj = Sj - szcfj, szti = Si - szcfi;
-      int ssj = find(Qtj, Stj, sztj), ssi = find(Q[ti], Sti, szti);
-      double fs = std::fabs(Qtj) < 1e-20 ? -1.0 : 1.0;
-      if (ssi == ssj)
-        fb += cgordan(Scfi, szcfi, Sti, szti, Si, Si) * cgordan(Scfj, szcfj, Stj, sztj, Sj, Sj) *
-              cgordan(Scfi, szcfi, 0.5, sigma, Scfj, szcfj) * fs;
-    }
-    return std::fabs(cg3) > 1e-20 ? fb / cg3 : 0.0;
+  void add_block(double q, double s, double sz, int nstates) {
+    int lo = ntypes();
+    bQ.push_back(q); bS.push_back(s); bSz.push_back(sz);
+    bbeg.push_back(lo); bend.push_back(lo + nstates - 1);
+    for (int i = 0; i < nstates; i++) blk.push_back(nblocks() - 1);
   }
 };
 
-// OneChQS_cd_check (src/OpMatRules.cpp:106-125) / Diag_check (:12-24)
-static bool cd_check(const CNRGarray& A, int i, int j) {
-  double Qi = A.GetQNumber(i, 0), Si = A.GetQNumber(i, 1), Qj = A.GetQNumber(j, 0), Sj = A.GetQNumber(j, 1);
-  return std::fabs(Qj - (Qi + 1.0)) < 1e-20 &&
-         (std::fabs(Sj - (Si + 0.5)) < 1e-20 || std::fabs(Sj - (Si - 0.5)) < 1e-20);
+// ---------------------------------------------------------------------------
+// OneChQS plugin: OneChQS_SetSingleSite, OneCh_fd_table (src/OpTables.cpp:8-33),
+// OneChQS_HN_MatEl / OneChQS_fN_MatEl / OneChQS_cd_MatEl
+// (src/OpMatRules.cpp:1167-1535)
+// ---------------------------------------------------------------------------
+static SiteDef onechqs_site() {
+  SiteDef s;
+  s.add_block(-1, 0, 0, 1);
+  s.add_block(0, 0.5, 0.5, 1);
+  s.add_block(0, 0.5, -0.5, 1);
+  s.add_block(1, 0, 0, 1);
+  return s;
+}
+static double onech_fd(int sigma, int ti, int tj) {
+  if (sigma == 1) return (ti == 1 && tj == 0) ? 1.0 : ((ti == 3 && tj == 2) ? 1.0 : 0.0);
+  if (sigma == -1) return (ti == 2 && tj == 0) ? 1.0 : ((ti == 3 && tj == 1) ? -1.0 : 0.0);
+  return 0.0;
+}
+// CoefMatEl * FermiSign of OneChQS_HN_MatEl (block spin Si, site types typep, type)
+static double onech_hn_coef(const SiteDef& st, double Si, int tp, int tt) {
+  double Stp = st.S(tp), Sztp = st.Sz(tp), Qt = st.Q(tt), St = st.S(tt), Szt = st.Sz(tt);
+  double Scfp = Si - Sztp, Scf = Si - Szt;
+  double fsign = deq(Qt, 0.0) ? -1.0 : 1.0;
+  double coef = 0.0;
+  for (double sz = -St; sz <= St; sz += 1.0) {
+    int ss = st.find(Qt, St, sz);
+    for (int sg = -1; sg <= 1; sg += 2) {
+      double ds = 0.5 * sg, szcf = Si - sz, szcfp = szcf - ds, sztp = sz + ds;
+      int ssp = st.find(st.Q(tp), Stp, sz + ds);
+      coef += cgordan(Scf, szcf, St, sz, Si, Si) * cgordan(Scfp, szcfp, Stp, sztp, Si, Si) *
+              cgordan(Scfp, szcfp, 0.5, ds, Scf, szcf) * onech_fd(sg, ssp, ss);
+    }
+  }
+  return coef * fsign;
+}
+static double onech_fn_el(const SiteDef& st, double Si, double Sj, int ti, int tj) {
+  double Sti = st.S(ti), Szti = st.Sz(ti), Stj = st.S(tj), Sztj = st.Sz(tj);
+  double Scfi = Si - Szti, Scfj = Sj - Sztj, sigma = Sj - Si, acc = 0.0;
+  for (double sz = -Stj; sz <= Stj; sz += 1.0) {
+    int ssj = st.find(st.Q(tj), Stj, sz), ssi = st.find(st.Q(ti), Sti, sz - sigma);
+    acc += onech_fd((int)(sigma / 0.5), ssj, ssi) * cgordan(Scfi, Si - sz + sigma, Sti, sz - sigma, Si, Si) *
+           cgordan(Scfj, Sj - sz, Stj, sz, Sj, Sj);
+  }
+  double cgr = cgordan(Si, Si, 0.5, sigma, Sj, Sj);
+  return !deq(cgr, 0.0) ? acc / cgr : 0.0;
+}
+static double onech_cd_el(const SiteDef& st, double Si, double Sj, int ti, int tj) {
+  double Sti = st.S(ti), Szti = st.Sz(ti), Qtj = st.Q(tj), Stj = st.S(tj), Sztj = st.Sz(tj);
+  double Scfi = Si - Szti, Scfj = Sj - Sztj, sigma = Sj - Si;
+  double cg3 = cgordan(Si, Si, 0.5, sigma, Sj, Sj);
+  if (deq(cg3, 0.0)) {
+    sigma = Si - Sj;
+    cg3 = cgordan(Si, Si, 0.5, sigma, Sj, Sj);
+    if (deq(cg3, 0.0)) return 0.0;
+  }
+  double fb = 0.0;
+  for (double szcfj = -Scfj; szcfj <= Scfj; szcfj += 1.0) {
+    double szcfi = szcfj - sigma, sztj = Sj - szcfj, szti = Si - szcfi;
+    int ssj = st.find(Qtj, Stj, sztj), ssi = st.find(st.Q(ti), Sti, szti);
+    double fs = deq(Qtj, 0.0) ? -1.0 : 1.0;
+    if (ssi == ssj)
+      fb += cgordan(Scfi, szcfi, Sti, szti, Si, Si) * cgordan(Scfj, szcfj, Stj, sztj, Sj, Sj) *
+            cgordan(Scfi, szcfi, 0.5, sigma, Scfj, szcfj) * fs;
+  }
+  return std::fabs(cg3) > 1e-20 ? fb / cg3 : 0.0;
 }
 
 // ---------------------------------------------------------------------------
-// BuildBasis for (Q,S) with SU(2) coupling (src/BuildBasis.cpp:8-432 with
-// CommonQNs={2,0,1,0,1}, totSpos={1}).  New blocks in first-appearance order
-// of (Q_old+Q_site, S_old+S_site, then S descending to |S_old-S_site|)
-// (FilterQNumbers keeps first occurrences); states per new block ordered by
-// (old block, site block, old state).
+// TwoChQS plugin: TwoChQS_SetSingleSite (src/TwoChQS/TwoChQS_SetSingleSite.cpp),
+// TwoChQS_fd_table / TwoChQS_SpSm_table (src/TwoChQS/TwoChQS_OpTables.cpp:19-183),
+// H_N rules of TwoChQS_DiagHN (src/TwoChQS/TwoChQS_DiagHN.cpp:62-250) and
+// TwoChQS_SetH0Anderson (src/TwoChQS/TwoChQS_SetH0Anderson.cpp:95-190),
+// f_ch rule of TwoChQS_UpdateMatrixAfterCutting (src/TwoChQS/TwoChQS_UpdateMatrices.cpp:380-470)
 // ---------------------------------------------------------------------------
-static void build_basis_qs(const CNRGbasisarray& Ac, CNRGbasisarray& Ab, const OneChQSSite& site) {
+static SiteDef twochqs_site() {
+  SiteDef s;
+  s.add_block(-2, 0, 0, 1);
+  s.add_block(-1, 0.5, 0.5, 2);
+  s.add_block(-1, 0.5, -0.5, 2);
+  s.add_block(0, 0, 0, 3);
+  s.add_block(0, 1, -1, 1);
+  s.add_block(0, 1, 0, 1);
+  s.add_block(0, 1, 1, 1);
+  s.add_block(1, 0.5, 0.5, 2);
+  s.add_block(1, 0.5, -0.5, 2);
+  s.add_block(2, 0, 0, 1);
+  return s;
+}
+struct TwoChTables {
+  double fd[2][2][16][16];  // [channel][sigma>0][i][j] of f^dagger
+  TwoChTables() {
+    std::memset(fd, 0, sizeof(fd));
+    const double r = 1.0 / std::sqrt(2.0);
+    auto set = [&](int ch, int up, std::initializer_list<std::tuple<int, int, double>> L) {
+      for (auto& x : L) fd[ch][up][std::get<0>(x)][std::get<1>(x)] = std::get<2>(x);
+    };
+    set(0, 1, {{2, 0, 1}, {10, 1, 1}, {9, 3, r}, {11, 6, 1}, {5, 3, r}, {7, 4, 1}, {12, 9, r}, {12, 5, -r},
+               {14, 8, 1}, {15, 13, 1}});
+    set(0, 0, {{4, 0, 1}, {9, 1, r}, {5, 1, -r}, {8, 3, 1}, {13, 6, 1}, {7, 2, -1}, {12, 10, -1}, {14, 9, -r},
+               {14, 5, -r}, {15, 11, -1}});
+    set(1, 1, {{1, 0, 1}, {6, 3, 1}, {10, 2, -1}, {11, 9, -r}, {11, 5, -r}, {9, 4, -r}, {5, 4, r}, {13, 8, -1},
+               {12, 7, 1}, {15, 14, 1}});
+    set(1, 0, {{3, 0, 1}, {6, 1, -1}, {9, 2, -r}, {11, 10, 1}, {5, 2, -r}, {8, 4, -1}, {13, 9, r}, {13, 5, -r},
+               {14, 7, 1}, {15, 12, -1}});
+  }
+  double f(int ch, int sigma, int ti, int tj) const {
+    if (ch < 1 || ch > 2 || (sigma != 1 && sigma != -1) || ti < 0 || ti > 15 || tj < 0 || tj > 15) return 0.0;
+    return fd[ch - 1][sigma == 1][ti][tj];
+  }
+  static double spsm(int ti, int tj) {
+    if (ti == tj) return 1.0;
+    int a = std::min(ti, tj), b = std::max(ti, tj);
+    return ((a == 1 && b == 3) || (a == 2 && b == 4) || (a == 11 && b == 13) || (a == 12 && b == 14) ||
+            (a == 8 && b == 9) || (a == 9 && b == 10) || (a == 8 && b == 10))
+               ? 1.0
+               : 0.0;
+  }
+};
+static bool dle(double a, double b) { return (a < b) || deq(a, b); }  // dLEqual
+// h0 = 1: TwoChQS_SetH0Anderson fermion sign (Q = +-1 and Sz = +-1/2)
+static double twoch_hn_coef(const SiteDef& st, const TwoChTables& T, int ch, double Si, int typep, int type, int h0) {
+  double Qtp = st.Q(typep), Stp = st.S(typep), Sztp = st.Sz(typep);
+  double Qt = st.Q(type), St = st.S(type), Szt = st.Sz(type);
+  double Soldp = Si - Sztp, Sold = Si - Szt;
+  double fs = 1.0;
+  if (h0) { if ((deq(Qt, 1.0) || deq(Qt, -1.0)) && (deq(Szt, 0.5) || deq(Szt, -0.5))) fs = -1.0; }
+  else if (deq(Szt, 0.5) || deq(Szt, -0.5)) fs = -1.0;
+  double tot = 0.0;
+  for (int sg = -1; sg <= 1; sg += 2) {
+    double ds = 0.5 * sg;
+    for (double szo = Sold; szo >= -Sold; szo -= 1.0) {
+      double szop = szo - ds, szt = Si - szo, sztp = szt + ds;
+      int sb = st.find(Qt, St, szt), sbp = st.find(Qtp, Stp, sztp);
+      if (dle(std::fabs(sztp), Stp) && dle(std::fabs(szo), Sold) && dle(std::fabs(szop), Soldp)) {
+        double c = cgordan(Sold, szo, St, szt, Si, Si) * cgordan(Soldp, szop, Stp, sztp, Si, Si) *
+                   cgordan(Soldp, szop, 0.5, ds, Sold, szo);
+        int p0, p1, q0, q1;
+        st.range(sbp, p0, p1);
+        st.range(sb, q0, q1);
+        for (int ssp = p0; ssp <= p1; ssp++)
+          for (int ss = q0; ss <= q1; ss++)
+            tot += c * T.f(ch, sg, ssp, ss) * fs * TwoChTables::spsm(ss, type) * TwoChTables::spsm(ssp, typep);
+      }
+    }
+  }
+  return tot;
+}
+static double twoch_fn_el(const SiteDef& st, const TwoChTables& T, int ch, double Si, double Sj, int ti, int tj) {
+  double Qti = st.Q(ti), Sti = st.S(ti), Szti = st.Sz(ti), Qtj = st.Q(tj), Stj = st.S(tj), Sztj = st.Sz(tj);
+  double Scfi = Si - Szti, Scfj = Sj - Sztj;
+  double cgm = cgordan(Si, Si, 0.5, -0.5, Sj, Sj), cgp = cgordan(Si, Si, 0.5, 0.5, Sj, Sj);
+  int sigma = 0;
+  double norm = 1.0;
+  if (std::fabs(cgm) > 1e-20) { sigma = -1; norm = cgm; }
+  else if (std::fabs(cgp) > 1e-20) { sigma = 1; norm = cgp; }
+  if (sigma == 0) return 0.0;
+  double acc = 0.0;
+  for (double szc = Scfj; szc >= -Scfj; szc -= 1.0) {
+    double szi = Si - szc, szj = Sj - szc;
+    int bsi = st.find(Qti, Sti, szi), bsj = st.find(Qtj, Stj, szj);
+    double c = cgordan(Scfi, szc, Sti, szi, Si, Si) * cgordan(Scfj, szc, Stj, szj, Sj, Sj);
+    int a0, a1, b0, b1;
+    st.range(bsi, a0, a1);
+    st.range(bsj, b0, b1);
+    for (int ssi = a0; ssi <= a1; ssi++)
+      for (int ssj = b0; ssj <= b1; ssj++)
+        acc += T.f(ch, sigma, ssj, ssi) * c * TwoChTables::spsm(ssi, ti) * TwoChTables::spsm(ssj, tj);
+  }
+  return acc / norm;
+}
+
+// ---------------------------------------------------------------------------
+// basis construction.  New blocks in first-appearance order of (Q_old+Q_site,
+// S from S_old+S_site down to |S_old-S_site|) (FilterQNumbers keeps first
+// occurrences); a state (old state o, site state s) enters block (Q,S) when
+// Q = Q_o + Q_s, S = S_o + Sz_s and (S_o, S_s, S) obey the triangle rule.
+//   order 0: BuildBasis      (src/BuildBasis.cpp:8-432): old block, site block, old state, site state
+//   order 1: QS_BuildBasis   (src/QS_BuildBasis.cpp:9-169): old block, site block, site state, old state
+// ---------------------------------------------------------------------------
+static void build_basis(const CNRGbasisarray& Ac, CNRGbasisarray& Ab, const SiteDef& site, int order, int befcut) {
   Ab = CNRGbasisarray();
   Ab.Nshell = Ac.Nshell + 1;
   Ab.NQNumbers = 2;
   std::map<std::pair<long, long>, int> seen;
   std::vector<std::pair<double, double>> qns;
-  auto add = [&](double q, double s) {
-    auto k = std::make_pair(key2(q), key2(s));
-    if (seen.emplace(k, (int)qns.size()).second) qns.push_back({q, s});
-  };
   for (int b = 0; b < Ac.NumBlocks(); b++) {
     double Qo = Ac.GetQNumber(b, 0), So = Ac.GetQNumber(b, 1);
-    for (int s = 0; s < 4; s++) {
-      add(Qo + site.Q[s], So + site.S[s]);
-      for (double sn = So + site.S[s] - 1.0; sn >= std::fabs(So - site.S[s]); sn -= 1.0) add(Qo + site.Q[s], sn);
-    }
+    for (int s = 0; s < site.nblocks(); s++)
+      for (double sn = So + site.bS[s]; sn >= std::fabs(So - site.bS[s]); sn -= 1.0) {
+        auto k = std::make_pair(key2(Qo + site.bQ[s]), key2(sn));
+        if (seen.emplace(k, (int)qns.size()).second) qns.push_back({Qo + site.bQ[s], sn});
+      }
   }
   int ist = 0;
   for (auto& qs : qns) {
@@ -201,14 +326,22 @@ static void build_basis_qs(const CNRGbasisarray& Ac, CNRGbasisarray& Ab, const O
     Ab.BlockBegEnd.push_back(ist);
     for (int bo = 0; bo < Ac.NumBlocks(); bo++) {
       double Qo = Ac.GetQNumber(bo, 0), So = Ac.GetQNumber(bo, 1);
-      for (int s = 0; s < 4; s++) {
-        if (key2(qs.first) != key2(site.Q[s] + Qo)) continue;
-        if (key2(qs.second) != key2(site.Sz[s] + So) || !triangle(So, site.S[s], qs.second)) continue;
-        for (int io = Ac.GetBlockLimit(bo, 0); io <= Ac.GetBlockLimit(bo, 1); io++) {
-          Ab.iType.push_back(s);
+      for (int s = 0; s < site.nblocks(); s++) {
+        if (key2(qs.first) != key2(site.bQ[s] + Qo)) continue;
+        if (key2(qs.second) != key2(site.bSz[s] + So) || !triangle(So, site.bS[s], qs.second)) continue;
+        const int o0 = Ac.GetBlockLimit(bo, 0), o1 = Ac.GetBlockLimit(bo, 1);
+        auto push = [&](int io, int is) {
+          Ab.iType.push_back(is);
           Ab.dEn.push_back(Ac.dEn[io]);
-          Ab.StCameFrom.push_back(io);
+          Ab.StCameFrom.push_back(befcut == 1 ? Ac.StCameFrom[io] : io);
           ist++;
+        };
+        if (order == 0) {
+          for (int io = o0; io <= o1; io++)
+            for (int is = site.bbeg[s]; is <= site.bend[s]; is++) push(io, is);
+        } else {
+          for (int is = site.bbeg[s]; is <= site.bend[s]; is++)
+            for (int io = o0; io <= o1; io++) push(io, is);
         }
       }
     }
@@ -217,9 +350,9 @@ static void build_basis_qs(const CNRGbasisarray& Ac, CNRGbasisarray& Ab, const O
 }
 
 // ---------------------------------------------------------------------------
-// device view of an operator over a kept basis
+// device views
 // ---------------------------------------------------------------------------
-struct BasisDev {  // per-state block/position, per-block size of a kept basis
+struct BasisDev {  // per-state block/position and per-block size of a kept basis
   int* ob = nullptr;
   int* op = nullptr;
   int* bsz = nullptr;
@@ -261,7 +394,7 @@ static int make_oldop(nrg_ctx* c, const CNRGmatrix& M, const BasisDev& bd, const
 }
 
 // ---------------------------------------------------------------------------
-// thermodynamics (src/CalcThermo.cpp:16-121, src/CalcOpAvg.cpp:13-84)
+// thermodynamics (src/CalcThermo.cpp:16-121)
 // ---------------------------------------------------------------------------
 static double calc_suscep(double bb, const CNRGarray& A) {
   double s0 = 0, s1 = 0;
@@ -286,55 +419,167 @@ static double calc_entropy(double bb, const CNRGarray& A) {
   return bb * (s / Z) + std::log(Z);
 }
 
+// operator update specification
+enum PairRule { PAIR_DIAG = 0, PAIR_QS_UP = 1, PAIR_QS_UP_UPPER = 2 };
+struct OpSpec {
+  bool need_old = false;
+  bool calc_avg = false;
+  PairRule rule = PAIR_DIAG;
+  std::function<double(double, double, int, int)> g;  // site-type value for (S_i, S_j, t_i, t_j)
+};
+
+// CheckForMatEl rules: Diag_check (src/OpMatRules.cpp:12-24), OneChQS_cd_check
+// (:106-125); the two-channel update only visits jbl > ibl
+// (src/TwoChQS/TwoChQS_UpdateMatrices.cpp:333)
+static bool pair_ok(PairRule r, const CNRGarray& A, int i, int j) {
+  if (r == PAIR_DIAG) return i == j;
+  if (r == PAIR_QS_UP_UPPER && j <= i) return false;
+  double Qi = A.GetQNumber(i, 0), Si = A.GetQNumber(i, 1), Qj = A.GetQNumber(j, 0), Sj = A.GetQNumber(j, 1);
+  return deq(Qj, Qi + 1.0) && (deq(Sj, Si + 0.5) || deq(Sj, Si - 0.5));
+}
+
 // ---------------------------------------------------------------------------
-// the chain driver
+// the step machinery shared by the chain drivers
 // ---------------------------------------------------------------------------
-struct Driver {
+struct Chain {
   nrg_ctx* c;
-  nrg_onechqs_params p;
-  OneChQSSite site;
+  SiteDef site;
+  int ntypes = 4;
   CNRGarray Aeig;
   CNRGbasisarray AeigCut, Abasis;
   std::vector<CNRGmatrix> mats;
-  std::vector<int64_t> Aeig_zoff;   // offset of each Aeig block in the Z buffer
+  std::vector<OpSpec> specs;
+  std::vector<int64_t> Aeig_zoff;  // offset of each Aeig block in the Z buffer
   double* dZ = nullptr;
   double* dW = nullptr;
-  int parity = 0;                   // Z buffer parity (double-buffered per step)
+  BasisDev oldb;                   // device view of the basis the current ops live on
+  int zpar = 0;
+  bool record = false;
+  std::vector<double> hrec, zrec;
 
-  int emit(nrg_step_cb cb, void* user, double emin, double ecut, int nkept, const double* th_T, const double* sus,
-           const double* ent, double Tavg, const std::vector<double>& ops, const std::vector<double>* hrec,
-           const std::vector<double>* zrec) {
-    if (!cb) return 0;
-    nrg_step st;
-    std::memset(&st, 0, sizeof(st));
-    st.nshell = Aeig.Nshell; st.nq = Aeig.NQNumbers; st.nblocks = Aeig.NumBlocks();
-    st.qn = Aeig.QNumbers.data(); st.bbe = Aeig.BlockBegEnd.data();
-    st.nstates = (int)Aeig.dEn.size(); st.en = Aeig.dEn.data();
-    std::vector<unsigned char> kp(Aeig.dEn.size(), 1);
-    for (size_t i = 0; i < kp.size() && i < Aeig.Kept.size(); i++) kp[i] = Aeig.Kept[i];
-    st.kept = kp.data();
-    st.emin = emin; st.ecut = ecut; st.nkept = nkept;
-    st.hpacked = hrec ? hrec->data() : nullptr;
-    st.evec = zrec ? zrec->data() : nullptr;
-    for (int i = 0; i < 4; i++) {
-      st.thermo_T[i] = th_T ? th_T[i] : 0; st.suscep[i] = sus ? sus[i] : 0; st.entropy[i] = ent ? ent[i] : 0;
+  void init_ops(int nm) {
+    mats.assign(nm, CNRGmatrix());
+    for (int m = 0; m < nm; m++) {
+      mats[m].buf[0] = "op" + std::to_string(m) + "_a";
+      mats[m].buf[1] = "op" + std::to_string(m) + "_b";
+      mats[m].NeedOld = specs[m].need_old;
+      mats[m].CalcAvg = specs[m].calc_avg;
     }
-    st.T_avg = Tavg;
-    st.nops_avg = (int)ops.size();
-    for (size_t i = 0; i < ops.size() && i < 8; i++) st.opavg[i] = ops[i];
-    st.cut_nblocks = AeigCut.NumBlocks();
-    st.cut_qn = AeigCut.QNumbers.data();
-    st.cut_bbe = AeigCut.BlockBegEnd.data();
-    st.cut_bbe_bc = AeigCut.BlockBegEndBC.data();
-    cb(user, &st);
+  }
+
+  int set_op(int m, const std::vector<int>& map, const std::vector<int>& be, const std::vector<double>& el) {
+    CNRGmatrix& M = mats[m];
+    M.MatBlockMap = map;
+    M.MatBlockBegEnd = be;
+    M.cur = 0;
+    DBuf& b = c->buf(M.buf[0]);
+    int rc = b.reserve(sizeof(double) * (el.size() + 1));
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(b.p, el.data(), sizeof(double) * el.size(), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
     return 0;
   }
 
-  // CutStates (src/CutStates.cpp:5-96): AeigCut keeps, per surviving block,
-  // the energy-ordered prefix up to the first level above Ecut; blocks whose
-  // lowest level is above Ecut are dropped.  Eigenvector rows are not moved:
-  // AeigCut records where each block's kept rows start in the Z buffer.
-  void cut_states(int ncut) {
+  // H_N assembly on the device for the blocks of Abasis, then the batched
+  // eigensolve; Aeig receives the raw (unshifted) eigenvalues.
+  //   fops[ch]  operator used as f_ch;  coef[ch]  [S index][typep][type] table
+  int diag_step(int nch, const int* fops, const double* chi, const std::vector<double>* coef, double sqrtL,
+                double eN) {
+    int rc;
+    const int nb = Abasis.NumBlocks();
+    const int ns = Abasis.Nstates();
+    std::vector<int> bbeg(nb), bsize(nb), sidx(nb);
+    std::vector<int64_t> h_off(nb), w_off(nb);
+    int64_t ztot = 0;
+    int nmax = 0;
+    for (int b = 0; b < nb; b++) {
+      bbeg[b] = Abasis.GetBlockLimit(b, 0);
+      bsize[b] = Abasis.GetBlockSize(b);
+      sidx[b] = (int)key2(Abasis.GetQNumber(b, 1));
+      h_off[b] = ztot;
+      w_off[b] = bbeg[b];
+      ztot += (int64_t)bsize[b] * bsize[b];
+      nmax = std::max(nmax, bsize[b]);
+    }
+    zpar ^= 1;
+    DBuf& zb = c->buf(zpar ? "Z1" : "Z2");
+    DBuf& wb = c->buf("W");
+    if ((rc = zb.reserve(sizeof(double) * (size_t)(ztot + 1)))) return rc;
+    if ((rc = wb.reserve(sizeof(double) * (size_t)(ns + 1)))) return rc;
+    double* dZn = (double*)zb.p;
+    dW = (double*)wb.p;
+    int *d_bbeg, *d_bsize, *d_sidx, *d_stcf, *d_itype;
+    int64_t* d_hoff;
+    double *d_en, *d_siteq;
+    if ((rc = upload(c, "hb_bbeg", bbeg, &d_bbeg))) return rc;
+    if ((rc = upload(c, "hb_bsize", bsize, &d_bsize))) return rc;
+    if ((rc = upload(c, "hb_sidx", sidx, &d_sidx))) return rc;
+    if ((rc = upload(c, "hb_hoff", h_off, &d_hoff))) return rc;
+    if ((rc = upload(c, "hb_stcf", Abasis.StCameFrom, &d_stcf))) return rc;
+    if ((rc = upload(c, "hb_itype", Abasis.iType, &d_itype))) return rc;
+    if ((rc = upload(c, "hb_en", Abasis.dEn, &d_en))) return rc;
+    std::vector<double> siteq(ntypes);
+    for (int t = 0; t < ntypes; t++) siteq[t] = site.Q(t);
+    if ((rc = upload(c, "hb_siteq", siteq, &d_siteq))) return rc;
+    HnArgs ha{};
+    ha.H = dZn; ha.h_off = d_hoff; ha.bbeg = d_bbeg; ha.bsize = d_bsize; ha.blk_sidx = d_sidx;
+    ha.stcf = d_stcf; ha.itype = d_itype; ha.en_par = d_en; ha.site_q = d_siteq;
+    for (int ch = 0; ch < nch; ch++) {
+      double* dcoef;
+      std::string nm = "hb_coef" + std::to_string(ch);
+      if ((rc = upload(c, nm.c_str(), coef[ch], &dcoef))) return rc;
+      ha.coef[ch] = dcoef;
+      if ((rc = make_oldop(c, mats[fops[ch]], oldb, "hf" + std::to_string(ch), ha.f[ch]))) return rc;
+      ha.chi[ch] = chi[ch];
+    }
+    ha.sqrtL = sqrtL;
+    ha.eN = eN;
+    ha.nch = nch;
+    ha.ntypes = ntypes;
+    {
+      double by = 0.0;
+      for (int b = 0; b < nb; b++) by += 8.0 * (double)bsize[b] * bsize[b];
+      PendingTimer tm;
+      if ((rc = c->timer_begin(tm, NRG_K_HN, 0.0, by))) return rc;
+      HIPCHK(nrgk_launch_hn(&ha, nb, nmax, c->stream));
+      if ((rc = c->timer_end(tm))) return rc;
+    }
+    hrec.clear();
+    if (record) {
+      std::vector<double> hfull(ztot);
+      HIPCHK(hipMemcpyAsync(hfull.data(), dZn, sizeof(double) * ztot, hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(hipStreamSynchronize(c->stream));
+      for (int b = 0; b < nb; b++) {
+        int n = bsize[b];
+        for (int i = 0; i < n; i++)
+          for (int j = i; j < n; j++) hrec.push_back(hfull[h_off[b] + (int64_t)i * n + j]);
+      }
+    }
+    if ((rc = eig_dev(c, nb, bsize.data(), h_off.data(), dZn, w_off.data(), dW, nullptr))) return rc;
+    Aeig = CNRGarray();
+    Aeig.Nshell = Abasis.Nshell; Aeig.NQNumbers = 2;
+    Aeig.QNumbers = Abasis.QNumbers;
+    Aeig.BlockBegEnd = Abasis.BlockBegEnd;
+    Aeig.dEn.resize(ns);
+    HIPCHK(hipMemcpyAsync(Aeig.dEn.data(), dW, sizeof(double) * ns, hipMemcpyDeviceToHost, c->stream));
+    zrec.clear();
+    if (record) {
+      zrec.resize(ztot);
+      HIPCHK(hipMemcpyAsync(zrec.data(), dZn, sizeof(double) * ztot, hipMemcpyDeviceToHost, c->stream));
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    Aeig_zoff = h_off;
+    dZ = dZn;
+    return 0;
+  }
+
+  // CutStates (src/CutStates.cpp:5-96; by-value form :103-160): AeigCut keeps,
+  // per surviving block, the energy-ordered prefix up to the first level above
+  // Ecut; blocks whose lowest level is above Ecut are dropped.  Eigenvector
+  // rows are not moved: AeigCut records where each block's kept rows start in
+  // the Z buffer.  legacy=false repeats SetKept(Ncut) (SetDegen + Ecut) on the
+  // copy as CutStates does; the by-value form of the two-channel driver does not.
+  void cut_states(int ncut, bool legacy, double myeps) {
     CNRGbasisarray A;
     A.Nshell = Aeig.Nshell; A.NQNumbers = Aeig.NQNumbers;
     int nb = Aeig.NumBlocks();
@@ -344,10 +589,12 @@ struct Driver {
     double ec = HUGE_VAL;
     if (!notrunc) {
       ec = ecut_of(Aeig.dEn, ncut);
-      set_degen(en, p.myeps);   // Acut.SetKept(Ncutoff)
-      double ec2 = ecut_of(en, ncut);
-      kept.assign(en.size(), 0);
-      for (size_t i = 0; i < en.size(); i++) kept[i] = en[i] <= ec2;
+      if (!legacy) {
+        set_degen(en, myeps);
+        double ec2 = ecut_of(en, ncut);
+        kept.assign(en.size(), 0);
+        for (size_t i = 0; i < en.size(); i++) kept[i] = en[i] <= ec2;
+      }
     }
     int ist = 0;
     for (int b = 0; b < nb; b++) {
@@ -373,7 +620,7 @@ struct Driver {
       A.BlockBegEndBC.push_back(b1);
       for (int i = 0; i < k; i++) {
         A.dEn.push_back(en[b0 + i]);
-        A.Kept.push_back(i + b0 < (int)kept.size() ? kept[b0 + i] : 1);
+        A.Kept.push_back(b0 + i < (int)kept.size() ? kept[b0 + i] : 1);
         A.StCameFrom.push_back(b0 + i);
       }
       A.zoff.push_back(Aeig_zoff[b]);
@@ -383,87 +630,112 @@ struct Driver {
     AeigCut = std::move(A);
   }
 
-  int run(nrg_step_cb cb, void* user, double* timing_ms);
-};
-
-int Driver::run(nrg_step_cb cb, void* user, double* timing_ms) {
-  const double Lambda = p.Lambda;
-  const double bbar = p.betabar > 0 ? p.betabar : 0.727;
-  if (p.myeps <= 0) p.myeps = 1e-18;
-  const double HLF = 0.5 * (1.0 + 1.0 / Lambda);
-  int rc;
-  // operators: [0] f_N, then n_d, n_d^2, S_d^2 (calcdens 0/2) or c_d (calcdens 1)
-  int nm = (p.calcdens == 1) ? 2 : 4;
-  mats.assign(nm, CNRGmatrix());
-  mats[0].kind = OP_FN;
-  for (int m = 1; m < nm; m++) {
-    mats[m].kind = (p.calcdens == 1) ? OP_CD : OP_IMPONLY;
-    mats[m].NeedOld = true;
-    mats[m].CalcAvg = (p.calcdens != 1);
-  }
-  for (int m = 0; m < nm; m++) {
-    mats[m].buf[0] = "op" + std::to_string(m) + "_a";
-    mats[m].buf[1] = "op" + std::to_string(m) + "_b";
-  }
-  // ---- H_{-1}: OneChQS_SetAnderson_Hm1 (src/Main/Main_SetH0.cpp:1794-1969)
-  double chi_m1 = std::sqrt(2.0 * p.Gamma / M_PI) / (std::sqrt(Lambda) * HLF);
-  if (p.disc_scheme != 1) chi_m1 *= std::sqrt(0.5 * std::log(Lambda) * (Lambda + 1) / (Lambda - 1));
-  Aeig = CNRGarray();
-  Aeig.Nshell = -1; Aeig.NQNumbers = 2;
-  Aeig.QNumbers = {-1, 0, 0, 0.5, 1, 0};
-  Aeig.BlockBegEnd = {0, 0, 1, 1, 2, 2};
-  Aeig.dEn = {0.5 * p.U / (Lambda * HLF), (p.ed + 0.5 * p.U) / (Lambda * HLF), (2.0 * p.ed + 1.5 * p.U) / (Lambda * HLF)};
-  Aeig_zoff = {0, 1, 2};
-  {
-    DBuf& zb = c->buf("Z0");
-    if ((rc = zb.reserve(sizeof(double) * 8))) return rc;
-    double ones[3] = {1, 1, 1};
-    HIPCHK(hipMemcpyAsync(zb.p, ones, sizeof(ones), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
-    dZ = (double*)zb.p;
-  }
-  // operators over the 3 H_{-1} states
-  {
-    std::vector<double> f = {1.0, -std::sqrt(2.0)};
-    mats[0].MatBlockMap = {0, 1, 1, 2};
-    mats[0].MatBlockBegEnd = {0, 0, 1, 1};
-    DBuf& b0 = c->buf(mats[0].buf[0]);
-    if ((rc = b0.reserve(sizeof(double) * 2))) return rc;
-    HIPCHK(hipMemcpyAsync(b0.p, f.data(), sizeof(double) * 2, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
-    mats[0].cur = 0;
-    for (int m = 1; m < nm; m++) {
-      std::vector<double> v(3);
-      if (p.calcdens == 1) {
-        mats[m].MatBlockMap = mats[0].MatBlockMap;
-        mats[m].MatBlockBegEnd = mats[0].MatBlockBegEnd;
-        v = f;
-      } else {
-        mats[m].MatBlockMap = {0, 0, 1, 1, 2, 2};
-        mats[m].MatBlockBegEnd = {0, 0, 1, 1, 2, 2};
-        for (int b = 0; b < 3; b++) {
-          double nd = Aeig.GetQNumber(b, 0) + 1.0, Sd = Aeig.GetQNumber(b, 1);
-          v[b] = (m == 1) ? nd : (m == 2 ? nd * nd : Sd * (Sd + 1.0));
+  // UpdateMatrices (src/UpdateMatrices.cpp:272-568): for each (ib, jb) of
+  // AeigCut in loop order and each operator whose pair rule holds,
+  // F = G[t_i][t_j] * (NeedOld ? O_old(stcf_i,stcf_j) : delta(stcf_i,stcf_j))
+  // over the pre-cut blocks, then O_new = Z_i F Z_j^T on the device.
+  int update_ops() {
+    int rc;
+    const int nm = (int)mats.size();
+    std::vector<FPair> pairs;
+    std::vector<double> G;
+    std::vector<RotJob> jobs;
+    std::vector<int> job_op;
+    std::vector<std::vector<int>> nmap(nm), nbe(nm);
+    std::vector<int64_t> icount(nm, 0);
+    int64_t ftot = 0, auxtot = 0, maxel = 1;
+    const int nbc = AeigCut.NumBlocks();
+    for (int ib = 0; ib < nbc; ib++) {
+      int ki = AeigCut.GetBlockSize(ib);
+      int ibc_beg = AeigCut.BlockBegEndBC[2 * ib];
+      int ni = AeigCut.BlockBegEndBC[2 * ib + 1] - ibc_beg + 1;
+      double Si = AeigCut.GetQNumber(ib, 1);
+      for (int jb = 0; jb < nbc; jb++) {
+        int kj = AeigCut.GetBlockSize(jb);
+        int jbc_beg = AeigCut.BlockBegEndBC[2 * jb];
+        int nj = AeigCut.BlockBegEndBC[2 * jb + 1] - jbc_beg + 1;
+        double Sj = AeigCut.GetQNumber(jb, 1);
+        for (int m = 0; m < nm; m++) {
+          if (!pair_ok(specs[m].rule, AeigCut, ib, jb)) continue;
+          nmap[m].push_back(ib); nmap[m].push_back(jb);
+          nbe[m].push_back((int)icount[m]);
+          int64_t out_off = icount[m];
+          icount[m] += (int64_t)ki * kj;
+          nbe[m].push_back((int)icount[m] - 1);
+          FPair P{};
+          P.ni_bc = ni; P.nj_bc = nj; P.ibeg_bc = ibc_beg; P.jbeg_bc = jbc_beg;
+          P.f_off = ftot; P.g_off = (int64_t)G.size(); P.need_old = specs[m].need_old ? 1 : 0; P.op = m;
+          for (int ti = 0; ti < ntypes; ti++)
+            for (int tj = 0; tj < ntypes; tj++) G.push_back(specs[m].g(Si, Sj, ti, tj));
+          pairs.push_back(P);
+          RotJob J{};
+          J.ni = ni; J.nj = nj; J.ki = ki; J.kj = kj;
+          J.ldzi = AeigCut.zld[ib]; J.ldzj = AeigCut.zld[jb];
+          J.zi_off = AeigCut.zoff[ib]; J.zj_off = AeigCut.zoff[jb];
+          J.f_off = ftot; J.aux_off = auxtot; J.out_off = out_off;
+          jobs.push_back(J);
+          job_op.push_back(m);
+          ftot += (int64_t)ni * nj;
+          auxtot += (int64_t)kj * ni;
+          maxel = std::max(maxel, (int64_t)ni * nj);
         }
       }
-      DBuf& bm = c->buf(mats[m].buf[0]);
-      if ((rc = bm.reserve(sizeof(double) * v.size()))) return rc;
-      HIPCHK(hipMemcpyAsync(bm.p, v.data(), sizeof(double) * v.size(), hipMemcpyHostToDevice, c->stream));
-      HIPCHK(hipStreamSynchronize(c->stream));
-      mats[m].cur = 0;
     }
+    DBuf& fb = c->buf("upd_F");
+    DBuf& ab = c->buf("upd_aux");
+    if ((rc = fb.reserve(sizeof(double) * (size_t)(ftot + 1)))) return rc;
+    if ((rc = ab.reserve(sizeof(double) * (size_t)(auxtot + 1)))) return rc;
+    if (!pairs.empty()) {
+      FPair* d_pairs;
+      double* d_G;
+      int *d_stcf, *d_itype;
+      if ((rc = upload(c, "upd_pairs", pairs, &d_pairs))) return rc;
+      if ((rc = upload(c, "upd_G", G, &d_G))) return rc;
+      if ((rc = upload(c, "upd_stcf", Abasis.StCameFrom, &d_stcf))) return rc;
+      if ((rc = upload(c, "upd_itype", Abasis.iType, &d_itype))) return rc;
+      FArgs fa{};
+      fa.F = (double*)fb.p; fa.pairs = d_pairs; fa.G = d_G; fa.stcf = d_stcf; fa.itype = d_itype;
+      fa.ntypes = ntypes;
+      for (int m = 0; m < nm && m < 8; m++)
+        if (specs[m].need_old)
+          if ((rc = make_oldop(c, mats[m], oldb, "uo" + std::to_string(m), fa.old[m]))) return rc;
+      PendingTimer tm;
+      if ((rc = c->timer_begin(tm, NRG_K_FBASIS, 0.0, 8.0 * (double)ftot))) return rc;
+      HIPCHK(nrgk_launch_fbasis(&fa, (int)pairs.size(), maxel, c->stream));
+      if ((rc = c->timer_end(tm))) return rc;
+    }
+    for (int m = 0; m < nm; m++) {
+      std::vector<RotJob> jm;
+      for (size_t q = 0; q < jobs.size(); q++)
+        if (job_op[q] == m) jm.push_back(jobs[q]);
+      CNRGmatrix& M = mats[m];
+      int nxt = M.cur ^ 1;
+      DBuf& ob = c->buf(M.buf[nxt]);
+      if ((rc = ob.reserve(sizeof(double) * (size_t)(icount[m] + 1)))) return rc;
+      if ((rc = rotate_dev(c, jm, dZ, (double*)fb.p, (double*)ab.p, (double*)ob.p))) return rc;
+      M.cur = nxt;
+      M.MatBlockMap = nmap[m];
+      M.MatBlockBegEnd = nbe[m];
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if ((rc = c->resolve())) return rc;
+    // the new operators live on AeigCut
+    return make_basis_dev(c, AeigCut, "oldb", oldb);
   }
-  cut_states(p.ncut);
-  HIPCHK(hipStreamSynchronize(c->stream));
-  auto op_avgs = [&](std::vector<double>& out) -> int {
+
+  // CalcOpAvg (src/CalcOpAvg.cpp:13-84) on AeigCut for the operators with CalcAvg
+  int op_avgs(double bbar, std::vector<double>& out) {
     out.clear();
-    for (int m = 1; m < nm; m++) {
+    for (size_t m = 0; m < mats.size(); m++) {
       if (!mats[m].CalcAvg) continue;
       const CNRGmatrix& M = mats[m];
       int64_t tot = M.MatBlockBegEnd.empty() ? 0 : M.MatBlockBegEnd.back() + 1;
       std::vector<double> h(tot > 0 ? tot : 1);
-      if (tot > 0)
-        HIPCHK(hipMemcpy(h.data(), c->buf(M.buf[M.cur]).p, sizeof(double) * tot, hipMemcpyDeviceToHost));
+      if (tot > 0) {
+        HIPCHK(hipMemcpyAsync(h.data(), c->buf(M.buf[M.cur]).p, sizeof(double) * tot, hipMemcpyDeviceToHost,
+                              c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+      }
       double s = 0, Z = 0;
       for (int b = 0; b < AeigCut.NumBlocks(); b++) {
         double dg = 2.0 * AeigCut.GetQNumber(b, 1) + 1.0;
@@ -480,221 +752,256 @@ int Driver::run(nrg_step_cb cb, void* user, double* timing_ms) {
       out.push_back(std::fabs(Z) > 1e-20 ? s / Z : s);
     }
     return 0;
-  };
+  }
+
+  void emit(nrg_step_cb cb, void* user, double emin, double ecut, int nkept, const double* th_T, const double* sus,
+            const double* ent, double Tavg, const std::vector<double>& ops) {
+    if (!cb) return;
+    nrg_step st;
+    std::memset(&st, 0, sizeof(st));
+    st.nshell = Aeig.Nshell; st.nq = Aeig.NQNumbers; st.nblocks = Aeig.NumBlocks();
+    st.qn = Aeig.QNumbers.data(); st.bbe = Aeig.BlockBegEnd.data();
+    st.nstates = (int)Aeig.dEn.size(); st.en = Aeig.dEn.data();
+    std::vector<unsigned char> kp(Aeig.dEn.size(), 1);
+    for (size_t i = 0; i < kp.size() && i < Aeig.Kept.size(); i++) kp[i] = Aeig.Kept[i];
+    st.kept = kp.data();
+    st.emin = emin; st.ecut = ecut; st.nkept = nkept;
+    st.hpacked = (record && !hrec.empty()) ? hrec.data() : nullptr;
+    st.evec = (record && !zrec.empty()) ? zrec.data() : nullptr;
+    for (int i = 0; i < 4; i++) {
+      st.thermo_T[i] = th_T ? th_T[i] : 0; st.suscep[i] = sus ? sus[i] : 0; st.entropy[i] = ent ? ent[i] : 0;
+    }
+    st.T_avg = Tavg;
+    st.nops_avg = (int)ops.size();
+    for (size_t i = 0; i < ops.size() && i < 8; i++) st.opavg[i] = ops[i];
+    st.cut_nblocks = AeigCut.NumBlocks();
+    st.cut_qn = AeigCut.QNumbers.data();
+    st.cut_bbe = AeigCut.BlockBegEnd.data();
+    st.cut_bbe_bc = AeigCut.BlockBegEndBC.data();
+    cb(user, &st);
+  }
+};
+
+// per-S coupling table of a model rule, [S index = 2S][typep][type]
+static std::vector<double> coef_table(int smax, int ntypes, const std::function<double(double, int, int)>& f) {
+  std::vector<double> t((size_t)(smax + 1) * ntypes * ntypes, 0.0);
+  for (int si = 0; si <= smax; si++)
+    for (int a = 0; a < ntypes; a++)
+      for (int b = 0; b < ntypes; b++) t[((size_t)si * ntypes + a) * ntypes + b] = f(0.5 * si, a, b);
+  return t;
+}
+static int max_sidx(const CNRGarray& A) {
+  int m = 0;
+  for (int b = 0; b < A.NumBlocks(); b++) m = std::max(m, (int)key2(A.GetQNumber(b, 1)));
+  return m;
+}
+
+// ---------------------------------------------------------------------------
+// OneChQS Anderson: NRG_main loop
+// ---------------------------------------------------------------------------
+static int run_onechqs(nrg_ctx* c, nrg_onechqs_params p, nrg_step_cb cb, void* user, double* timing_ms) {
+  const double Lambda = p.Lambda;
+  const double bbar = p.betabar > 0 ? p.betabar : 0.727;
+  if (p.myeps <= 0) p.myeps = 1e-18;
+  const double HLF = 0.5 * (1.0 + 1.0 / Lambda);
+  int rc;
+  Chain ch;
+  ch.c = c;
+  ch.site = onechqs_site();
+  ch.ntypes = 4;
+  ch.record = p.record_blocks != 0;
+  const SiteDef& st = ch.site;
+  // ModelSwitch (src/CodeHandler_ModelSwitch.cpp:121-293): [0] f_N, then
+  // n_d, n_d^2, S_d^2 (calcdens 0/2, ImpOnly_MatEl x old) or c_d (calcdens 1)
+  const int nm = (p.calcdens == 1) ? 2 : 4;
+  ch.specs.resize(nm);
+  ch.specs[0].rule = PAIR_QS_UP;
+  ch.specs[0].g = [&st](double Si, double Sj, int ti, int tj) { return onech_fn_el(st, Si, Sj, ti, tj); };
+  for (int m = 1; m < nm; m++) {
+    ch.specs[m].need_old = true;
+    if (p.calcdens == 1) {
+      ch.specs[m].rule = PAIR_QS_UP;
+      ch.specs[m].g = [&st](double Si, double Sj, int ti, int tj) { return onech_cd_el(st, Si, Sj, ti, tj); };
+    } else {
+      ch.specs[m].rule = PAIR_DIAG;
+      ch.specs[m].calc_avg = true;
+      ch.specs[m].g = [](double, double, int ti, int tj) { return ti == tj ? 1.0 : 0.0; };
+    }
+  }
+  ch.init_ops(nm);
+  // H_{-1}: OneChQS_SetAnderson_Hm1 (src/Main/Main_SetH0.cpp:1794-1969)
+  double chi_m1 = std::sqrt(2.0 * p.Gamma / M_PI) / (std::sqrt(Lambda) * HLF);
+  if (p.disc_scheme != 1) chi_m1 *= std::sqrt(0.5 * std::log(Lambda) * (Lambda + 1) / (Lambda - 1));
+  ch.Aeig.Nshell = -1; ch.Aeig.NQNumbers = 2;
+  ch.Aeig.QNumbers = {-1, 0, 0, 0.5, 1, 0};
+  ch.Aeig.BlockBegEnd = {0, 0, 1, 1, 2, 2};
+  ch.Aeig.dEn = {0.5 * p.U / (Lambda * HLF), (p.ed + 0.5 * p.U) / (Lambda * HLF),
+                 (2.0 * p.ed + 1.5 * p.U) / (Lambda * HLF)};
+  ch.Aeig_zoff = {0, 1, 2};
+  {
+    DBuf& zb = c->buf("Z0");
+    if ((rc = zb.reserve(sizeof(double) * 8))) return rc;
+    double ones[3] = {1, 1, 1};
+    HIPCHK(hipMemcpyAsync(zb.p, ones, sizeof(ones), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    ch.dZ = (double*)zb.p;
+  }
+  std::vector<double> f = {1.0, -std::sqrt(2.0)};
+  if ((rc = ch.set_op(0, {0, 1, 1, 2}, {0, 0, 1, 1}, f))) return rc;
+  for (int m = 1; m < nm; m++) {
+    if (p.calcdens == 1) {
+      if ((rc = ch.set_op(m, {0, 1, 1, 2}, {0, 0, 1, 1}, f))) return rc;
+    } else {
+      std::vector<double> v(3);
+      for (int b = 0; b < 3; b++) {
+        double nd = ch.Aeig.GetQNumber(b, 0) + 1.0, Sd = ch.Aeig.GetQNumber(b, 1);
+        v[b] = (m == 1) ? nd : (m == 2 ? nd * nd : Sd * (Sd + 1.0));
+      }
+      if ((rc = ch.set_op(m, {0, 0, 1, 1, 2, 2}, {0, 0, 1, 1, 2, 2}, v))) return rc;
+    }
+  }
+  ch.cut_states(p.ncut, false, p.myeps);
+  if ((rc = make_basis_dev(c, ch.AeigCut, "oldb", ch.oldb))) return rc;
   std::vector<double> ops;
-  if ((rc = op_avgs(ops))) return rc;
+  if ((rc = ch.op_avgs(bbar, ops))) return rc;
   double DN = HLF * std::pow(Lambda, -(0 - 2) / 2.0);
-  emit(cb, user, 0.0, HUGE_VAL, Aeig.Nstates(), nullptr, nullptr, nullptr, DN / bbar, ops, nullptr, nullptr);
+  ch.emit(cb, user, 0.0, HUGE_VAL, ch.Aeig.Nstates(), nullptr, nullptr, nullptr, DN / bbar, ops);
 
   double par_chi = chi_m1;
   auto t0 = std::chrono::steady_clock::now();
   for (int N = 0; N < p.nsitesmax; N++) {
     DN = HLF * std::pow(Lambda, -(N - 1) / 2.0);
-    // ---- BuildBasis (host)
-    build_basis_qs(AeigCut, Abasis, site);
-    const int nb = Abasis.NumBlocks();
-    const int ns = Abasis.Nstates();
-    std::vector<int> bbeg(nb), bsize(nb), sidx(nb);
-    std::vector<int64_t> h_off(nb), w_off(nb);
-    int64_t ztot = 0;
-    int nmax = 0, smax = 0;
-    for (int b = 0; b < nb; b++) {
-      bbeg[b] = Abasis.GetBlockLimit(b, 0);
-      bsize[b] = Abasis.GetBlockSize(b);
-      sidx[b] = (int)key2(Abasis.GetQNumber(b, 1));
-      smax = std::max(smax, sidx[b]);
-      h_off[b] = ztot;
-      w_off[b] = bbeg[b];
-      ztot += (int64_t)bsize[b] * bsize[b];
-      nmax = std::max(nmax, bsize[b]);
-    }
-    // coupling tables C[sidx][tp][t]
-    std::vector<double> coef((size_t)(smax + 1) * 16, 0.0);
-    for (int si = 0; si <= smax; si++)
-      for (int tp = 0; tp < 4; tp++)
-        for (int t = 0; t < 4; t++) coef[(size_t)si * 16 + tp * 4 + t] = site.hn_coef(0.5 * si, tp, t);
-    // ---- device: upload basis, assemble H_N, eigensolve
-    parity ^= 1;
-    DBuf& zb = c->buf(parity ? "Z1" : "Z2");
-    DBuf& wb = c->buf("W");
-    if ((rc = zb.reserve(sizeof(double) * (size_t)(ztot + 1)))) return rc;
-    if ((rc = wb.reserve(sizeof(double) * (size_t)(ns + 1)))) return rc;
-    double* dZn = (double*)zb.p;
-    dW = (double*)wb.p;
-    int *d_bbeg, *d_bsize, *d_sidx, *d_stcf, *d_itype;
-    int64_t* d_hoff;
-    double *d_en, *d_coef, *d_siteq;
-    if ((rc = upload(c, "hb_bbeg", bbeg, &d_bbeg))) return rc;
-    if ((rc = upload(c, "hb_bsize", bsize, &d_bsize))) return rc;
-    if ((rc = upload(c, "hb_sidx", sidx, &d_sidx))) return rc;
-    if ((rc = upload(c, "hb_hoff", h_off, &d_hoff))) return rc;
-    if ((rc = upload(c, "hb_stcf", Abasis.StCameFrom, &d_stcf))) return rc;
-    if ((rc = upload(c, "hb_itype", Abasis.iType, &d_itype))) return rc;
-    if ((rc = upload(c, "hb_en", Abasis.dEn, &d_en))) return rc;
-    if ((rc = upload(c, "hb_coef", coef, &d_coef))) return rc;
-    std::vector<double> siteq(site.Q, site.Q + 4);
-    if ((rc = upload(c, "hb_siteq", siteq, &d_siteq))) return rc;
-    BasisDev oldb;
-    if ((rc = make_basis_dev(c, AeigCut, "oldb", oldb))) return rc;
-    HnArgs ha{};
-    ha.H = dZn; ha.h_off = d_hoff; ha.bbeg = d_bbeg; ha.bsize = d_bsize; ha.blk_sidx = d_sidx;
-    ha.stcf = d_stcf; ha.itype = d_itype; ha.en_par = d_en; ha.site_q = d_siteq;
-    ha.coef[0] = d_coef;
-    if ((rc = make_oldop(c, mats[0], oldb, "hf0", ha.f[0]))) return rc;
-    ha.chi[0] = par_chi;
-    ha.sqrtL = std::sqrt(Lambda);
-    ha.eN = 0.0;
-    ha.nch = 1;
-    ha.ntypes = 4;
-    {
-      double by = 0.0;
-      for (int b = 0; b < nb; b++) by += 8.0 * (double)bsize[b] * bsize[b];
-      PendingTimer tm;
-      if ((rc = c->timer_begin(tm, NRG_K_HN, 0.0, by))) return rc;
-      HIPCHK(nrgk_launch_hn(&ha, nb, nmax, c->stream));
-      if ((rc = c->timer_end(tm))) return rc;
-    }
-    std::vector<double> hrec, zrec;
-    if (p.record_blocks) {
-      std::vector<double> hfull(ztot);
-      HIPCHK(hipMemcpyAsync(hfull.data(), dZn, sizeof(double) * ztot, hipMemcpyDeviceToHost, c->stream));
-      HIPCHK(hipStreamSynchronize(c->stream));
-      for (int b = 0; b < nb; b++) {
-        int n = bsize[b];
-        for (int i = 0; i < n; i++)
-          for (int j = i; j < n; j++) hrec.push_back(hfull[h_off[b] + (int64_t)i * n + j]);
-      }
-    }
-    if ((rc = eig_dev(c, nb, bsize.data(), h_off.data(), dZn, w_off.data(), dW, nullptr))) return rc;
-    // ---- eigenvalues to the host: SetE0zero, SetKept, thermo
-    Aeig = CNRGarray();
-    Aeig.Nshell = Abasis.Nshell; Aeig.NQNumbers = 2;
-    Aeig.QNumbers = Abasis.QNumbers;
-    Aeig.BlockBegEnd = Abasis.BlockBegEnd;
-    Aeig.dEn.resize(ns);
-    HIPCHK(hipMemcpyAsync(Aeig.dEn.data(), dW, sizeof(double) * ns, hipMemcpyDeviceToHost, c->stream));
-    if (p.record_blocks) {
-      zrec.resize(ztot);
-      HIPCHK(hipMemcpyAsync(zrec.data(), dZn, sizeof(double) * ztot, hipMemcpyDeviceToHost, c->stream));
-    }
-    HIPCHK(hipStreamSynchronize(c->stream));
-    double emin = *std::min_element(Aeig.dEn.begin(), Aeig.dEn.end());
-    for (auto& x : Aeig.dEn) x -= emin;
-    set_degen(Aeig.dEn, p.myeps);
-    double ecut = ecut_of(Aeig.dEn, p.ncut);
-    Aeig.Kept.assign(ns, 0);
+    build_basis(ch.AeigCut, ch.Abasis, st, 0, 0);
+    std::vector<double> coef = coef_table(max_sidx(ch.Abasis), 4, [&st](double S, int a, int b) {
+      return onech_hn_coef(st, S, a, b);
+    });
+    int fop = 0;
+    if ((rc = ch.diag_step(1, &fop, &par_chi, &coef, std::sqrt(Lambda), 0.0))) return rc;
+    // SetE0zero, SetKept (SetDegen + Ecut), CalcThermo at 4 betabar values
+    double emin = *std::min_element(ch.Aeig.dEn.begin(), ch.Aeig.dEn.end());
+    for (auto& x : ch.Aeig.dEn) x -= emin;
+    set_degen(ch.Aeig.dEn, p.myeps);
+    double ecut = ecut_of(ch.Aeig.dEn, p.ncut);
+    const int ns = (int)ch.Aeig.dEn.size();
+    ch.Aeig.Kept.assign(ns, 0);
     int nkept = 0;
-    for (int i = 0; i < ns; i++) { Aeig.Kept[i] = Aeig.dEn[i] <= ecut; nkept += Aeig.Kept[i]; }
+    for (int i = 0; i < ns; i++) { ch.Aeig.Kept[i] = ch.Aeig.dEn[i] <= ecut; nkept += ch.Aeig.Kept[i]; }
     double thT[4], sus[4], ent[4];
     for (int nc = 1; nc <= 4; nc++) {
       double bN = bbar * std::pow(Lambda, (nc - 4) / 8.0);
       thT[nc - 1] = DN / bN;
-      sus[nc - 1] = calc_suscep(bN, Aeig);
-      ent[nc - 1] = calc_entropy(bN, Aeig);
+      sus[nc - 1] = calc_suscep(bN, ch.Aeig);
+      ent[nc - 1] = calc_entropy(bN, ch.Aeig);
     }
-    Aeig_zoff = h_off;
-    dZ = dZn;
-    CNRGbasisarray prevCut = std::move(AeigCut);
-    cut_states(p.ncut);
-    // ---- UpdateMatrices (src/UpdateMatrices.cpp:272-568) on the device
-    {
-      std::vector<FPair> pairs;
-      std::vector<double> G;
-      std::vector<RotJob> jobs;
-      std::vector<std::vector<int>> nmap(nm), nbe(nm);
-      std::vector<int64_t> icount(nm, 0);
-      int64_t ftot = 0, auxtot = 0, maxel = 1;
-      const int nbc = AeigCut.NumBlocks();
-      std::vector<int> job_op;  // operator of each job
-      for (int ib = 0; ib < nbc; ib++) {
-        int ki = AeigCut.GetBlockSize(ib);
-        int ibc_beg = AeigCut.BlockBegEndBC[2 * ib];
-        int ni = AeigCut.BlockBegEndBC[2 * ib + 1] - ibc_beg + 1;
-        double Si = AeigCut.GetQNumber(ib, 1);
-        for (int jb = 0; jb < nbc; jb++) {
-          int kj = AeigCut.GetBlockSize(jb);
-          int jbc_beg = AeigCut.BlockBegEndBC[2 * jb];
-          int nj = AeigCut.BlockBegEndBC[2 * jb + 1] - jbc_beg + 1;
-          double Sj = AeigCut.GetQNumber(jb, 1);
-          for (int m = 0; m < nm; m++) {
-            bool ok = (mats[m].kind == OP_IMPONLY) ? (ib == jb) : cd_check(AeigCut, ib, jb);
-            if (!ok) continue;
-            nmap[m].push_back(ib); nmap[m].push_back(jb);
-            nbe[m].push_back((int)icount[m]);
-            int64_t out_off = icount[m];
-            icount[m] += (int64_t)ki * kj;
-            nbe[m].push_back((int)icount[m] - 1);
-            FPair P{};
-            P.ni_bc = ni; P.nj_bc = nj; P.ibeg_bc = ibc_beg; P.jbeg_bc = jbc_beg;
-            P.f_off = ftot; P.g_off = (int64_t)G.size(); P.need_old = mats[m].NeedOld ? 1 : 0; P.op = m;
-            for (int ti = 0; ti < 4; ti++)
-              for (int tj = 0; tj < 4; tj++) {
-                double g = 0.0;
-                if (mats[m].kind == OP_FN) g = site.fn_el(Si, Sj, ti, tj);
-                else if (mats[m].kind == OP_IMPONLY) g = (ti == tj) ? 1.0 : 0.0;
-                else g = site.cd_el(Si, Sj, ti, tj);
-                G.push_back(g);
-              }
-            pairs.push_back(P);
-            RotJob J{};
-            J.ni = ni; J.nj = nj; J.ki = ki; J.kj = kj;
-            J.ldzi = AeigCut.zld[ib]; J.ldzj = AeigCut.zld[jb];
-            J.zi_off = AeigCut.zoff[ib]; J.zj_off = AeigCut.zoff[jb];
-            J.f_off = ftot; J.aux_off = auxtot; J.out_off = out_off;
-            jobs.push_back(J);
-            job_op.push_back(m);
-            ftot += (int64_t)ni * nj;
-            auxtot += (int64_t)kj * ni;
-            maxel = std::max(maxel, (int64_t)ni * nj);
-          }
-        }
-      }
-      // F for every job, then per-operator GEMMs into that operator's new buffer
-      DBuf& fb = c->buf("upd_F");
-      DBuf& ab = c->buf("upd_aux");
-      if ((rc = fb.reserve(sizeof(double) * (size_t)(ftot + 1)))) return rc;
-      if ((rc = ab.reserve(sizeof(double) * (size_t)(auxtot + 1)))) return rc;
-      FPair* d_pairs;
-      double* d_G;
-      if ((rc = upload(c, "upd_pairs", pairs, &d_pairs))) return rc;
-      if ((rc = upload(c, "upd_G", G, &d_G))) return rc;
-      FArgs fa{};
-      fa.F = (double*)fb.p; fa.pairs = d_pairs; fa.G = d_G; fa.stcf = d_stcf; fa.itype = d_itype; fa.ntypes = 4;
-      for (int m = 0; m < nm; m++)
-        if (mats[m].NeedOld)
-          if ((rc = make_oldop(c, mats[m], oldb, "uo" + std::to_string(m), fa.old[m]))) return rc;
-      {
-        PendingTimer tm;
-        if ((rc = c->timer_begin(tm, NRG_K_FBASIS, 0.0, 8.0 * (double)ftot))) return rc;
-        HIPCHK(nrgk_launch_fbasis(&fa, (int)pairs.size(), maxel, c->stream));
-        if ((rc = c->timer_end(tm))) return rc;
-      }
-      for (int m = 0; m < nm; m++) {
-        std::vector<RotJob> jm;
-        for (size_t q = 0; q < jobs.size(); q++)
-          if (job_op[q] == m) jm.push_back(jobs[q]);
-        CNRGmatrix& M = mats[m];
-        int nxt = M.cur ^ 1;
-        DBuf& ob = c->buf(M.buf[nxt]);
-        if ((rc = ob.reserve(sizeof(double) * (size_t)(icount[m] + 1)))) return rc;
-        if ((rc = rotate_dev(c, jm, dZ, (double*)fb.p, (double*)ab.p, (double*)ob.p))) return rc;
-        M.cur = nxt;
-        M.MatBlockMap = nmap[m];
-        M.MatBlockBegEnd = nbe[m];
-        M.nel = icount[m];
-      }
-      HIPCHK(hipStreamSynchronize(c->stream));
-      if ((rc = c->resolve())) return rc;
-    }
-    if ((rc = op_avgs(ops))) return rc;
-    emit(cb, user, emin, ecut, nkept, thT, sus, ent, DN / bbar, ops, p.record_blocks ? &hrec : nullptr,
-         p.record_blocks ? &zrec : nullptr);
+    ch.cut_states(p.ncut, false, p.myeps);
+    if ((rc = ch.update_ops())) return rc;
+    if ((rc = ch.op_avgs(bbar, ops))) return rc;
+    ch.emit(cb, user, emin, ecut, nkept, thT, sus, ent, DN / bbar, ops);
     // chi for the next site: SetChainWilson (src/NRGchainClass.cpp:132-155)
     double d0 = 1.0 - std::pow(Lambda, -((double)N + 1.0));
     double d1 = std::sqrt(1.0 - std::pow(Lambda, -(2.0 * (double)N + 1.0)));
     double d2 = std::sqrt(1.0 - std::pow(Lambda, -(2.0 * (double)N + 3.0)));
     par_chi = d0 / (d1 * d2);
-    (void)prevCut;
+  }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  auto t1 = std::chrono::steady_clock::now();
+  if (timing_ms) *timing_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// TwoChQS Anderson: legacy standalone loop (src/TwoChQS/TwoChQS.cpp:204-495)
+// ---------------------------------------------------------------------------
+static int run_twochqs(nrg_ctx* c, nrg_twochqs_params p, nrg_step_cb cb, void* user, double* timing_ms) {
+  const double L = p.Lambda;
+  const double bbar = p.betabar > 0 ? p.betabar : 0.727;
+  const double HLF = 0.5 * (1.0 + 1.0 / L);
+  int rc;
+  static const TwoChTables T;
+  Chain ch;
+  ch.c = c;
+  ch.site = twochqs_site();
+  ch.ntypes = 16;
+  ch.record = p.record_blocks != 0;
+  const SiteDef& st = ch.site;
+  // operators: f_ch1, f_ch2 (TwoChQS_UpdateMatrixAfterCutting), plus d for H_0
+  ch.specs.resize(3);
+  for (int k = 0; k < 2; k++) {
+    ch.specs[k].rule = PAIR_QS_UP_UPPER;
+    ch.specs[k].g = [&st, k](double Si, double Sj, int ti, int tj) {
+      return twoch_fn_el(st, T, k + 1, Si, Sj, ti, tj);
+    };
+  }
+  ch.specs[2].rule = PAIR_QS_UP_UPPER;
+  ch.specs[2].g = [](double, double, int, int) { return 0.0; };
+  ch.init_ops(3);
+  // src/TwoChQS/TwoChQS.cpp:208-264
+  double Ut = 0.5 * p.U / HLF, edt = p.ed / HLF;
+  double G1t = p.Gamma1 * (2.0 / M_PI) / (HLF * HLF), G2t = p.Gamma2 * (2.0 / M_PI) / (HLF * HLF);
+  double Un = Ut / std::sqrt(L), edn = edt / std::sqrt(L);
+  double chi0[2] = {std::sqrt(G1t / L), std::sqrt(G2t / L)};
+  // TwoChQS_SetH0Anderson: impurity (N=-1) states and d, basis with the first
+  // site of both channels, H_0 assembled with the H_0 fermion-sign rule and no
+  // sqrt(Lambda) on the diagonal
+  ch.AeigCut = CNRGbasisarray();
+  ch.AeigCut.Nshell = -1; ch.AeigCut.NQNumbers = 2;
+  ch.AeigCut.QNumbers = {-1, 0, 0, 0.5, 1, 0};
+  ch.AeigCut.BlockBegEnd = {0, 0, 1, 1, 2, 2};
+  ch.AeigCut.BlockBegEndBC = {0, 0, 1, 1, 2, 2};
+  ch.AeigCut.dEn = {Un, edn + Un, 2.0 * edn + 3.0 * Un};
+  ch.AeigCut.StCameFrom = {0, 1, 2};
+  ch.AeigCut.zoff = {0, 1, 2};
+  ch.AeigCut.zld = {1, 1, 1};
+  if ((rc = ch.set_op(2, {0, 1, 1, 2}, {0, 0, 1, 1}, {1.0, -std::sqrt(2.0)}))) return rc;
+  if ((rc = make_basis_dev(c, ch.AeigCut, "oldb", ch.oldb))) return rc;
+  build_basis(ch.AeigCut, ch.Abasis, st, 1, 1);
+  {
+    int smax = max_sidx(ch.Abasis);
+    std::vector<double> coef[2];
+    for (int k = 0; k < 2; k++)
+      coef[k] = coef_table(smax, 16, [&st, k](double S, int a, int b) { return twoch_hn_coef(st, T, k + 1, S, a, b, 1); });
+    int fops[2] = {2, 2};
+    if ((rc = ch.diag_step(2, fops, chi0, coef, 1.0, 0.0))) return rc;
+  }
+  double DN = HLF * std::pow(L, -(0 - 1) / 2.0);
+  double emin = *std::min_element(ch.Aeig.dEn.begin(), ch.Aeig.dEn.end());
+  for (auto& x : ch.Aeig.dEn) x -= emin;
+  if (p.ncut < ch.Aeig.Nstates())
+    return fail(NRG_ERR_ARG, "nrg_run_twochqs: Ncut is below the N=0 basis size; the reference's "
+                             "TwoChQS_UpdateQm1fQ indexes the uncut basis there");
+  auto finish_step = [&](double emin_) -> int {
+    double thT[4] = {DN / bbar, 0, 0, 0}, sus[4] = {calc_suscep(bbar, ch.Aeig), 0, 0, 0};
+    double ent[4] = {calc_entropy(bbar, ch.Aeig), 0, 0, 0};
+    double ec = (p.ncut < ch.Aeig.Nstates()) ? ecut_of(ch.Aeig.dEn, p.ncut) : HUGE_VAL;
+    const int ns = (int)ch.Aeig.dEn.size();
+    ch.Aeig.Kept.assign(ns, 0);
+    int nk = 0;
+    for (int i = 0; i < ns; i++) { ch.Aeig.Kept[i] = ch.Aeig.dEn[i] <= ec; nk += ch.Aeig.Kept[i]; }
+    // the cut the next iteration starts with (by-value CutStates, no SetDegen)
+    ch.cut_states(p.ncut, true, 0.0);
+    int r = ch.update_ops();
+    if (r) return r;
+    std::vector<double> none;
+    ch.emit(cb, user, emin_, ec, nk, thT, sus, ent, DN / bbar, none);
+    return 0;
+  };
+  if ((rc = finish_step(emin))) return rc;
+  auto t0 = std::chrono::steady_clock::now();
+  for (int N = 1; N <= p.nsitesmax; N++) {
+    DN = HLF * std::pow(L, -(N - 1) / 2.0);
+    double d0 = 1.0 - std::pow(L, -N), d1 = std::sqrt(1.0 - std::pow(L, -(2 * N - 1)));
+    double d2 = std::sqrt(1.0 - std::pow(L, -(2 * N + 1)));
+    double chiN[2] = {d0 / (d1 * d2), d0 / (d1 * d2)};
+    build_basis(ch.AeigCut, ch.Abasis, st, 1, 0);
+    int smax = max_sidx(ch.Abasis);
+    std::vector<double> coef[2];
+    for (int k = 0; k < 2; k++)
+      coef[k] = coef_table(smax, 16, [&st, k](double S, int a, int b) { return twoch_hn_coef(st, T, k + 1, S, a, b, 0); });
+    int fops[2] = {0, 1};
+    if ((rc = ch.diag_step(2, fops, chiN, coef, std::sqrt(L), 0.0))) return rc;
+    emin = *std::min_element(ch.Aeig.dEn.begin(), ch.Aeig.dEn.end());
+    for (auto& x : ch.Aeig.dEn) x -= emin;
+    if ((rc = finish_step(emin))) return rc;
   }
   HIPCHK(hipStreamSynchronize(c->stream));
   auto t1 = std::chrono::steady_clock::now();
@@ -710,8 +1017,14 @@ extern "C" int nrg_run_onechqs(nrg_ctx* c, const nrg_onechqs_params* p, nrg_step
   if (p->nsitesmax < 0 || p->ncut < 1 || p->Lambda <= 1.0)
     return nrg::fail(NRG_ERR_ARG, "nrg_run_onechqs: bad parameters");
   HIPCHK(hipSetDevice(c->device));
-  nrg::Driver d;
-  d.c = c;
-  d.p = *p;
-  return d.run(cb, user, timing_ms);
+  return nrg::run_onechqs(c, *p, cb, user, timing_ms);
+}
+
+extern "C" int nrg_run_twochqs(nrg_ctx* c, const nrg_twochqs_params* p, nrg_step_cb cb, void* user,
+                               double* timing_ms) {
+  if (!c || !p) return nrg::fail(NRG_ERR_ARG, "nrg_run_twochqs: NULL argument");
+  if (p->nsitesmax < 0 || p->ncut < 1 || p->Lambda <= 1.0)
+    return nrg::fail(NRG_ERR_ARG, "nrg_run_twochqs: bad parameters");
+  HIPCHK(hipSetDevice(c->device));
+  return nrg::run_twochqs(c, *p, cb, user, timing_ms);
 }
