@@ -50,6 +50,13 @@ int nrg_device_count(void);
 int nrg_diag_blocks(nrg_ctx *ctx, int nb, const int *n, const int64_t *h_off, const double *H, int packed,
                     double *W, double *Z, int *info);
 
+/* Eigensolver used by every entry point of this context:
+ *   2 (default) Householder tridiagonalisation (one workgroup per block up to
+ *     n=256, 256-column slabs above) + divide and conquer + compact-WY
+ *     back-transformation, every stage batched over all blocks;
+ *   1 one workgroup per block: Householder + implicit QL + back-transformation. */
+int nrg_set_eig_impl(nrg_ctx *ctx, int impl);
+
 /* Device-resident variant on the context's stream: dA holds full n*n blocks at
  * a_off (overwritten by eigenvector rows), eigenvalues to dW at w_off. */
 int nrg_diag_blocks_dev(nrg_ctx *ctx, int nb, const int *n, const int64_t *a_off, double *dA,

@@ -17,6 +17,7 @@ EXPORTS = [
     "nrg_ctx_create", "nrg_ctx_destroy", "nrg_last_error", "nrg_device_count",
     "nrg_diag_blocks", "nrg_diag_blocks_dev", "nrg_truncate", "nrg_ecut",
     "nrg_rotate_ops", "nrg_run_onechqs", "nrg_run_twochqs", "nrg_set_timing", "nrg_get_stats", "nrg_reset_stats",
+    "nrg_set_eig_impl",
 ]
 
 KSTAT_NAMES = ["eig", "hn", "fbasis", "gemm"]
@@ -100,6 +101,7 @@ def lib():
     L.nrg_rotate_ops.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(PairDesc), _dp, ctypes.c_int64,
                                  _dp, ctypes.c_int64, _dp, ctypes.c_int64, ctypes.c_int]
     L.nrg_set_timing.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    L.nrg_set_eig_impl.argtypes = [ctypes.c_void_p, ctypes.c_int]
     L.nrg_get_stats.argtypes = [ctypes.c_void_p, ctypes.POINTER(NrgStats)]
     L.nrg_reset_stats.argtypes = [ctypes.c_void_p]
     L.nrg_run_onechqs.argtypes = [ctypes.c_void_p, ctypes.POINTER(OneChQSParams), STEP_CB, ctypes.c_void_p, _dp]
@@ -145,6 +147,11 @@ class Context:
             self.close()
         except Exception:
             pass
+
+    def set_eig_impl(self, impl):
+        """1: one workgroup per block (Householder + QL); 2: tridiagonalisation
+        + divide and conquer + blocked back-transformation (default)."""
+        _check(lib().nrg_set_eig_impl(self.h, int(impl)))
 
     def set_timing(self, on=True):
         _check(lib().nrg_set_timing(self.h, int(on)))

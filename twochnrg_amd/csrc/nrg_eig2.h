@@ -1,0 +1,69 @@
+// nrg_eig2.h -- descriptors of the batched eigensolver (Householder
+// tridiagonalisation, divide and conquer, compact-WY back-transformation).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+// one symmetric block; all offsets in doubles (ints for i_off)
+struct E2Block {
+  int n;
+  int pad;
+  int64_t a_off;   // caller's matrix (in: full symmetric, out: eigenvector rows)
+  int64_t w_off;   // caller's eigenvalue slot
+  int64_t q_off;   // n*n  eigenvectors of T, vector-major
+  int64_t k_off;   // n*n  gathered rows for the merge GEMM
+  int64_t u_off;   // n*n  rank-one eigenvectors
+  int64_t dl_off;  // n*n  d_l - lambda_j
+  int64_t t_off;   // n*n  merge output / reorder scratch
+  int64_t p_off;   // panel scratch: Vt, Gram, T^T, W, W2
+  int64_t v_off;   // 20*n doubles of vectors
+  int64_t i_off;   // 8*n ints
+};
+
+struct E2Leaf {
+  int blk, lo, hi, pad;
+};
+
+struct E2Merge {
+  int blk, lo, mid, hi;
+};
+
+// per-merge results of the deflation pass
+struct E2MState {
+  int k, nd, ngiv, pad;
+  double rho, tol;
+};
+
+struct E2Gemm {
+  int m, n, k;
+  int dyn;           // >= 0: m = k = state[dyn].k
+  int64_t a_off, b_off, c_off;
+  int64_t as0, as1, bs0, bs1, cs0;
+  double alpha;
+  int accumulate;
+  int pad;
+};
+
+struct E2Tile {
+  int prob, tm, tn, pad;
+};
+
+extern "C" {
+hipError_t e2_launch_tridiag_small(const E2Block* b, const int* list, int nlist, double* A, double* WS, hipStream_t s);
+hipError_t e2_launch_tridiag_step(const E2Block* b, const int* list, int nlist, int nslab, int k, double* A,
+                                  double* WS, hipStream_t s);
+hipError_t e2_launch_tridiag_tail(const E2Block* b, const int* list, int nlist, const double* A, double* WS,
+                                  hipStream_t s);
+hipError_t e2_launch_dc_init(const E2Block* b, int nb, int nmax, const E2Merge* merges, int nmerge, const int* mbeg,
+                             double* WS, hipStream_t s);
+hipError_t e2_launch_dc_leaf(const E2Block* b, const E2Leaf* leaves, int nleaf, double* WS, hipStream_t s);
+hipError_t e2_launch_dc_level(const E2Block* b, const E2Merge* merges, int nm, int mmax, E2MState* ms, double* WS,
+                              int* IW, const E2Gemm* gprobs, const E2Tile* gtiles, int ntiles, hipStream_t s);
+hipError_t e2_launch_bt_panel(const E2Block* b, const int* list, const int* p0, int nlist, int nmax, const double* A,
+                              double* WS, const E2Gemm* gprobs, const E2Tile* tiles_g, int nt_g,
+                              const E2Tile* tiles_rest, int nt_w, int nt_w2, int nt_z, hipStream_t s);
+hipError_t e2_launch_output(const E2Block* b, int nb, int nmax, const double* WS, double* A, double* W,
+                            hipStream_t s);
+hipError_t e2_launch_gemm(const E2Gemm* probs, const E2Tile* tiles, int ntiles, const double* A, const double* B,
+                          double* C, hipStream_t s);
+}

@@ -1,0 +1,228 @@
+// nrg_eig2_host.cpp -- host orchestration of the batched eigensolver: builds
+// the per-block workspace layout, the divide-and-conquer trees, the grouped
+// GEMM descriptors of every merge level and back-transformation panel, uploads
+// them once, and enqueues every launch on the context stream (no host
+// synchronisation inside).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <functional>
+#include <vector>
+
+#include "nrg_eig2.h"
+#include "nrg_internal.h"
+
+namespace nrg {
+
+static const int kLeaf = 32;   // = LEAF in nrg_eig2.hip
+static const int kPanel = 32;  // = PANEL in nrg_eig2.hip
+static const int kSmall = 256; // single-workgroup reduction up to this size
+
+int eig2_dev(nrg_ctx* c, int nb, const int* n, const int64_t* a_off, double* dA, const int64_t* w_off, double* dW) {
+  if (nb <= 0) return NRG_OK;
+  int rc;
+  std::vector<E2Block> blk(nb);
+  int64_t ws = 0, iw = 0;
+  int nmax = 0;
+  for (int b = 0; b < nb; b++) {
+    const int64_t nn = n[b], n2 = nn * nn;
+    E2Block& B = blk[b];
+    B = E2Block{};
+    B.n = n[b];
+    B.a_off = a_off[b];
+    B.w_off = w_off[b];
+    B.q_off = ws; ws += n2;
+    B.k_off = ws; ws += n2;
+    B.u_off = ws; ws += n2;
+    B.dl_off = ws; ws += n2;
+    B.t_off = ws; ws += n2;
+    B.p_off = ws; ws += kPanel * nn + 2 * kPanel * kPanel + 2 * nn * kPanel;
+    B.v_off = ws; ws += 20 * nn + 64;
+    B.i_off = iw; iw += 8 * nn + 8;
+    nmax = std::max(nmax, n[b]);
+  }
+  // ---- stage 1 lists
+  std::vector<int> small, large;
+  int nmax_large = 0;
+  for (int b = 0; b < nb; b++) {
+    if (n[b] <= 0) continue;
+    if (n[b] <= kSmall) small.push_back(b);
+    else { large.push_back(b); nmax_large = std::max(nmax_large, n[b]); }
+  }
+  // ---- stage 2 trees
+  std::vector<E2Leaf> leaves;
+  std::vector<E2Merge> all_merges;
+  std::vector<int> mbeg(nb + 1, 0);
+  std::vector<std::vector<E2Merge>> by_h;
+  for (int b = 0; b < nb; b++) {
+    mbeg[b] = (int)all_merges.size();
+    if (n[b] <= 0) continue;
+    std::function<int(int, int)> build = [&](int lo, int hi) -> int {
+      if (hi - lo <= kLeaf) { leaves.push_back(E2Leaf{b, lo, hi, 0}); return 0; }
+      const int mid = lo + (hi - lo) / 2;
+      const int h = std::max(build(lo, mid), build(mid, hi)) + 1;
+      E2Merge M{b, lo, mid, hi};
+      all_merges.push_back(M);
+      if ((int)by_h.size() < h) by_h.resize(h);
+      by_h[h - 1].push_back(M);
+      return h;
+    };
+    build(0, n[b]);
+  }
+  mbeg[nb] = (int)all_merges.size();
+  std::vector<E2Merge> lvl_merges;
+  std::vector<int> lvl_beg, lvl_mmax;
+  std::vector<E2Gemm> gprobs;
+  std::vector<E2Tile> gtiles;
+  std::vector<int> lvl_pbeg, lvl_tbeg, lvl_tcnt;
+  for (auto& L : by_h) {
+    lvl_beg.push_back((int)lvl_merges.size());
+    lvl_pbeg.push_back((int)gprobs.size());
+    lvl_tbeg.push_back((int)gtiles.size());
+    int mm = 0, pbase = (int)gprobs.size();
+    for (size_t q = 0; q < L.size(); q++) {
+      const E2Merge& M = L[q];
+      const E2Block& B = blk[M.blk];
+      const int msz = M.hi - M.lo;
+      mm = std::max(mm, msz);
+      E2Gemm g{};
+      g.m = msz; g.n = msz; g.k = msz; g.dyn = (int)q;
+      g.a_off = B.u_off + (int64_t)M.lo * B.n; g.as0 = msz; g.as1 = 1;
+      g.b_off = B.k_off + (int64_t)M.lo * B.n; g.bs0 = msz; g.bs1 = 1;
+      g.c_off = B.t_off + (int64_t)M.lo * B.n; g.cs0 = msz;
+      g.alpha = 1.0;
+      const int pi = (int)gprobs.size() - pbase;
+      gprobs.push_back(g);
+      const int tt = (msz + 63) / 64;
+      for (int a = 0; a < tt; a++)
+        for (int bb = 0; bb < tt; bb++) gtiles.push_back(E2Tile{pi, a, bb, 0});
+      lvl_merges.push_back(M);
+    }
+    lvl_mmax.push_back(mm);
+    lvl_tcnt.push_back((int)gtiles.size() - lvl_tbeg.back());
+  }
+  const int nlev = (int)by_h.size();
+  // ---- stage 3 panels: round r applies panel P_b-1-r of every block that has it
+  std::vector<int> npan(nb, 0);
+  int maxP = 0;
+  for (int b = 0; b < nb; b++) {
+    npan[b] = n[b] > 2 ? (n[b] - 2 + kPanel - 1) / kPanel : 0;
+    maxP = std::max(maxP, npan[b]);
+  }
+  std::vector<int> bt_list, bt_p0, r_lbeg, r_lcnt, r_pbeg, r_tbeg, r_ng, r_nw, r_nw2, r_nz;
+  std::vector<E2Gemm> bprobs;
+  std::vector<E2Tile> btiles;
+  int nmax_bt = 0;
+  for (int r = 0; r < maxP; r++) {
+    r_lbeg.push_back((int)bt_list.size());
+    r_pbeg.push_back((int)bprobs.size());
+    r_tbeg.push_back((int)btiles.size());
+    std::vector<int> bl;
+    for (int b = 0; b < nb; b++)
+      if (npan[b] > r) bl.push_back(b);
+    const int pbase = (int)bprobs.size();
+    std::vector<E2Tile> tg, tw, tw2, tz;
+    for (int b : bl) {
+      const E2Block& B = blk[b];
+      const int64_t nn = B.n;
+      const int k0 = (npan[b] - 1 - r) * kPanel;
+      bt_list.push_back(b);
+      bt_p0.push_back(k0);
+      nmax_bt = std::max(nmax_bt, B.n);
+      const int64_t vt = B.p_off, G = vt + kPanel * nn, Tg = G + kPanel * kPanel, W = Tg + kPanel * kPanel,
+                    W2 = W + nn * kPanel;
+      E2Gemm g{};
+      g.dyn = -1; g.alpha = 1.0;
+      // Gram G = Vt Vt^T
+      E2Gemm gg = g;
+      gg.m = kPanel; gg.n = kPanel; gg.k = B.n;
+      gg.a_off = vt; gg.as0 = nn; gg.as1 = 1; gg.b_off = vt; gg.bs0 = 1; gg.bs1 = nn; gg.c_off = G; gg.cs0 = kPanel;
+      int pg = (int)bprobs.size() - pbase;
+      bprobs.push_back(gg);
+      tg.push_back(E2Tile{pg, 0, 0, 0});
+      // W = Qv Vt^T
+      E2Gemm gw = g;
+      gw.m = B.n; gw.n = kPanel; gw.k = B.n;
+      gw.a_off = B.q_off; gw.as0 = nn; gw.as1 = 1; gw.b_off = vt; gw.bs0 = 1; gw.bs1 = nn; gw.c_off = W; gw.cs0 = kPanel;
+      int pw = (int)bprobs.size() - pbase;
+      bprobs.push_back(gw);
+      for (int a = 0; a < (B.n + 63) / 64; a++) tw.push_back(E2Tile{pw, a, 0, 0});
+      // W2 = W T^T
+      E2Gemm g2 = g;
+      g2.m = B.n; g2.n = kPanel; g2.k = kPanel;
+      g2.a_off = W; g2.as0 = kPanel; g2.as1 = 1; g2.b_off = Tg; g2.bs0 = kPanel; g2.bs1 = 1; g2.c_off = W2;
+      g2.cs0 = kPanel;
+      int p2 = (int)bprobs.size() - pbase;
+      bprobs.push_back(g2);
+      for (int a = 0; a < (B.n + 63) / 64; a++) tw2.push_back(E2Tile{p2, a, 0, 0});
+      // Qv -= W2 Vt
+      E2Gemm gz = g;
+      gz.m = B.n; gz.n = B.n; gz.k = kPanel;
+      gz.a_off = W2; gz.as0 = kPanel; gz.as1 = 1; gz.b_off = vt; gz.bs0 = nn; gz.bs1 = 1; gz.c_off = B.q_off;
+      gz.cs0 = nn; gz.alpha = -1.0; gz.accumulate = 1;
+      int pz = (int)bprobs.size() - pbase;
+      bprobs.push_back(gz);
+      for (int a = 0; a < (B.n + 63) / 64; a++)
+        for (int bb = 0; bb < (B.n + 63) / 64; bb++) tz.push_back(E2Tile{pz, a, bb, 0});
+    }
+    r_lcnt.push_back((int)bl.size());
+    r_ng.push_back((int)tg.size()); r_nw.push_back((int)tw.size());
+    r_nw2.push_back((int)tw2.size()); r_nz.push_back((int)tz.size());
+    btiles.insert(btiles.end(), tg.begin(), tg.end());
+    btiles.insert(btiles.end(), tw.begin(), tw.end());
+    btiles.insert(btiles.end(), tw2.begin(), tw2.end());
+    btiles.insert(btiles.end(), tz.begin(), tz.end());
+  }
+  // ---- uploads (one synchronisation)
+  DBuf& dws = c->buf("e2_ws");
+  DBuf& diw = c->buf("e2_iw");
+  if ((rc = dws.reserve(sizeof(double) * (size_t)(ws + 1)))) return rc;
+  if ((rc = diw.reserve(sizeof(int) * (size_t)(iw + 1)))) return rc;
+  double* WS = (double*)dws.p;
+  int* IW = (int*)diw.p;
+  E2Block* d_blk;
+  int *d_small, *d_large, *d_mbeg, *d_btl, *d_btp;
+  E2Leaf* d_leaf;
+  E2Merge *d_all, *d_lvl;
+  E2Gemm *d_gp, *d_bp;
+  E2Tile *d_gt, *d_bt;
+  std::vector<E2MState> ms0(lvl_merges.size() + 1);
+  E2MState* d_ms;
+  if ((rc = upload(c, "e2_blk", blk, &d_blk))) return rc;
+  if ((rc = upload(c, "e2_small", small, &d_small))) return rc;
+  if ((rc = upload(c, "e2_large", large, &d_large))) return rc;
+  if ((rc = upload(c, "e2_leaf", leaves, &d_leaf))) return rc;
+  if ((rc = upload(c, "e2_all", all_merges, &d_all))) return rc;
+  if ((rc = upload(c, "e2_mbeg", mbeg, &d_mbeg))) return rc;
+  if ((rc = upload(c, "e2_lvl", lvl_merges, &d_lvl))) return rc;
+  if ((rc = upload(c, "e2_ms", ms0, &d_ms))) return rc;
+  if ((rc = upload(c, "e2_gp", gprobs, &d_gp))) return rc;
+  if ((rc = upload(c, "e2_gt", gtiles, &d_gt))) return rc;
+  if ((rc = upload(c, "e2_btl", bt_list, &d_btl))) return rc;
+  if ((rc = upload(c, "e2_btp", bt_p0, &d_btp))) return rc;
+  if ((rc = upload(c, "e2_bp", bprobs, &d_bp))) return rc;
+  if ((rc = upload(c, "e2_bt", btiles, &d_bt))) return rc;
+  // ---- launches
+  hipStream_t s = c->stream;
+  HIPCHK(e2_launch_tridiag_small(d_blk, d_small, (int)small.size(), dA, WS, s));
+  if (!large.empty()) {
+    const int nslab = (nmax_large + 255) / 256;
+    for (int k = 0; k + 2 < nmax_large; k++)
+      HIPCHK(e2_launch_tridiag_step(d_blk, d_large, (int)large.size(), nslab, k, dA, WS, s));
+    HIPCHK(e2_launch_tridiag_tail(d_blk, d_large, (int)large.size(), dA, WS, s));
+  }
+  HIPCHK(e2_launch_dc_init(d_blk, nb, nmax, d_all, (int)all_merges.size(), d_mbeg, WS, s));
+  HIPCHK(e2_launch_dc_leaf(d_blk, d_leaf, (int)leaves.size(), WS, s));
+  for (int h = 0; h < nlev; h++)
+    HIPCHK(e2_launch_dc_level(d_blk, d_lvl + lvl_beg[h], (int)by_h[h].size(), lvl_mmax[h], d_ms + lvl_beg[h], WS, IW,
+                              d_gp + lvl_pbeg[h], d_gt + lvl_tbeg[h], lvl_tcnt[h], s));
+  for (int r = 0; r < maxP; r++) {
+    const E2Tile* t0 = d_bt + r_tbeg[r];
+    HIPCHK(e2_launch_bt_panel(d_blk, d_btl + r_lbeg[r], d_btp + r_lbeg[r], r_lcnt[r], nmax_bt, dA, WS,
+                              d_bp + r_pbeg[r], t0, r_ng[r], t0 + r_ng[r], r_nw[r], r_nw2[r], r_nz[r], s));
+  }
+  HIPCHK(e2_launch_output(d_blk, nb, nmax, WS, dA, dW, s));
+  return NRG_OK;
+}
+
+}  // namespace nrg

@@ -171,6 +171,12 @@ int nrg_ctx::resolve() {
   return 0;
 }
 
+extern "C" int nrg_set_eig_impl(nrg_ctx* c, int impl) {
+  if (!c || (impl != 1 && impl != 2)) return fail(NRG_ERR_ARG, "nrg_set_eig_impl: impl must be 1 or 2");
+  c->eig_impl = impl;
+  return NRG_OK;
+}
+
 extern "C" int nrg_set_timing(nrg_ctx* c, int on) {
   if (!c) return fail(NRG_ERR_ARG, "nrg_set_timing: NULL ctx");
   c->timing = on != 0;
@@ -245,6 +251,12 @@ int nrg::eig_dev(nrg_ctx* c, int nb, const int* n, const int64_t* a_off, double*
   }
   PendingTimer tm;
   if ((rc = c->timer_begin(tm, NRG_K_EIG, fl, by))) return rc;
+  if (c->eig_impl == 2) {
+    if ((rc = eig2_dev(c, nb, n, a_off, dA, w_off, dW))) return rc;
+    if ((rc = c->timer_end(tm))) return rc;
+    if (info_host) for (int b = 0; b < nb; b++) info_host[b] = 0;
+    return NRG_OK;
+  }
   HIPCHK(nrgk_launch_eig((EigDesc*)ddesc.p, (int*)dord.p, nb, nmax, dA, dW, (double*)dws.p, (int*)dinfo.p,
                          c->stream));
   if ((rc = c->timer_end(tm))) return rc;

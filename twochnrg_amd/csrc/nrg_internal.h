@@ -54,6 +54,7 @@ struct nrg_ctx {
   std::map<std::string, std::unique_ptr<DBuf>> bufs;
   DBuf& buf(const std::string& name);
   bool timing = false;
+  int eig_impl = 2;   // 1: one workgroup per block (Householder + QL); 2: tridiag + D&C + WY
   KStat stat[NRG_NKSTAT];
   std::vector<PendingTimer> pending;
   std::vector<hipEvent_t> event_pool;
@@ -75,6 +76,7 @@ struct RotJob {
 
 int eig_dev(nrg_ctx* c, int nb, const int* n, const int64_t* a_off, double* dA, const int64_t* w_off, double* dW,
             int* info_host);
+int eig2_dev(nrg_ctx* c, int nb, const int* n, const int64_t* a_off, double* dA, const int64_t* w_off, double* dW);
 int rotate_dev(nrg_ctx* c, const std::vector<RotJob>& jobs, const double* dZ, const double* dF, double* dAux,
                double* dOut);
 void set_degen(std::vector<double>& en, double eps);
