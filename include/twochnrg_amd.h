@@ -57,6 +57,19 @@ int nrg_diag_blocks(nrg_ctx *ctx, int nb, const int *n, const int64_t *h_off, co
  *   1 one workgroup per block: Householder + implicit QL + back-transformation. */
 int nrg_set_eig_impl(nrg_ctx *ctx, int impl);
 
+/* on != 0: "plain" launch mode -- the same kernels, but one symmetry block per
+ * eigensolve launch sequence and one block pair per rotation launch (no
+ * batching across blocks); the baseline the batched mode is measured against. */
+int nrg_set_plain(nrg_ctx *ctx, int on);
+
+/* Diagnostics: run the batched solver (impl 2) on one n x n matrix and return
+ * the stage outputs -- the tear-modified tridiagonal diagonal d (after the
+ * divide-and-conquer set-up), the untorn off-diagonal e, the sorted
+ * eigenvalues dd and vector rows qv of the tridiagonal problem, and the final
+ * eigenvalues w and eigenvector rows z.  Any output pointer may be NULL. */
+int nrg_eig2_stages(nrg_ctx *ctx, int n, const double *A, double *d_out, double *e_out, double *dd_out,
+                    double *qv_out, double *w_out, double *z_out);
+
 /* Device-resident variant on the context's stream: dA holds full n*n blocks at
  * a_off (overwritten by eigenvector rows), eigenvalues to dW at w_off. */
 int nrg_diag_blocks_dev(nrg_ctx *ctx, int nb, const int *n, const int64_t *a_off, double *dA,

@@ -17,7 +17,7 @@ EXPORTS = [
     "nrg_ctx_create", "nrg_ctx_destroy", "nrg_last_error", "nrg_device_count",
     "nrg_diag_blocks", "nrg_diag_blocks_dev", "nrg_truncate", "nrg_ecut",
     "nrg_rotate_ops", "nrg_run_onechqs", "nrg_run_twochqs", "nrg_set_timing", "nrg_get_stats", "nrg_reset_stats",
-    "nrg_set_eig_impl",
+    "nrg_set_eig_impl", "nrg_set_plain", "nrg_eig2_stages",
 ]
 
 KSTAT_NAMES = ["eig", "hn", "fbasis", "gemm"]
@@ -102,6 +102,8 @@ def lib():
                                  _dp, ctypes.c_int64, _dp, ctypes.c_int64, ctypes.c_int]
     L.nrg_set_timing.argtypes = [ctypes.c_void_p, ctypes.c_int]
     L.nrg_set_eig_impl.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    L.nrg_set_plain.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    L.nrg_eig2_stages.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
     L.nrg_get_stats.argtypes = [ctypes.c_void_p, ctypes.POINTER(NrgStats)]
     L.nrg_reset_stats.argtypes = [ctypes.c_void_p]
     L.nrg_run_onechqs.argtypes = [ctypes.c_void_p, ctypes.POINTER(OneChQSParams), STEP_CB, ctypes.c_void_p, _dp]
@@ -153,6 +155,10 @@ class Context:
         + divide and conquer + blocked back-transformation (default)."""
         _check(lib().nrg_set_eig_impl(self.h, int(impl)))
 
+    def set_plain(self, on=True):
+        """One block (pair) per launch: the unbatched baseline."""
+        _check(lib().nrg_set_plain(self.h, int(on)))
+
     def set_timing(self, on=True):
         _check(lib().nrg_set_timing(self.h, int(on)))
 
@@ -164,6 +170,15 @@ class Context:
         _check(lib().nrg_get_stats(self.h, ctypes.byref(st)))
         return {k: {"ms": st.ms[i], "launches": st.launches[i], "flops": st.flops[i], "bytes": st.bytes[i]}
                 for i, k in enumerate(KSTAT_NAMES)}
+
+    def eig2_stages(self, a):
+        """Stage outputs of the batched solver for one matrix (diagnostics)."""
+        a = np.ascontiguousarray(a, np.float64)
+        n = a.shape[0]
+        d, e, dd, w = (np.zeros(n) for _ in range(4))
+        qv, z = np.zeros(n * n), np.zeros(n * n)
+        _check(lib().nrg_eig2_stages(self.h, n, _ptr(a), _ptr(d), _ptr(e), _ptr(dd), _ptr(qv), _ptr(w), _ptr(z)))
+        return dict(d=d, e=e, dd=dd, qv=qv.reshape(n, n), w=w, z=z.reshape(n, n))
 
     # DiagBlock replacement -------------------------------------------------
     def diag_blocks(self, blocks, packed=False):

@@ -426,6 +426,18 @@ struct OpSpec {
   bool calc_avg = false;
   PairRule rule = PAIR_DIAG;
   std::function<double(double, double, int, int)> g;  // site-type value for (S_i, S_j, t_i, t_j)
+  // site-type tables already evaluated, keyed by (2S_i, 2S_j): the rules depend
+  // on the block spins only, so each table is computed once per run
+  std::map<std::pair<long, long>, std::vector<double>> cache;
+  const std::vector<double>& table(double Si, double Sj, int ntypes) {
+    auto key = std::make_pair(key2(Si), key2(Sj));
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+    std::vector<double> t((size_t)ntypes * ntypes);
+    for (int a = 0; a < ntypes; a++)
+      for (int b = 0; b < ntypes; b++) t[(size_t)a * ntypes + b] = g(Si, Sj, a, b);
+    return cache.emplace(key, std::move(t)).first->second;
+  }
 };
 
 // CheckForMatEl rules: Diag_check (src/OpMatRules.cpp:12-24), OneChQS_cd_check
@@ -665,8 +677,8 @@ struct Chain {
           FPair P{};
           P.ni_bc = ni; P.nj_bc = nj; P.ibeg_bc = ibc_beg; P.jbeg_bc = jbc_beg;
           P.f_off = ftot; P.g_off = (int64_t)G.size(); P.need_old = specs[m].need_old ? 1 : 0; P.op = m;
-          for (int ti = 0; ti < ntypes; ti++)
-            for (int tj = 0; tj < ntypes; tj++) G.push_back(specs[m].g(Si, Sj, ti, tj));
+          const std::vector<double>& gt = specs[m].table(Si, Sj, ntypes);
+          G.insert(G.end(), gt.begin(), gt.end());
           pairs.push_back(P);
           RotJob J{};
           J.ni = ni; J.nj = nj; J.ki = ki; J.kj = kj;
@@ -782,14 +794,24 @@ struct Chain {
   }
 };
 
-// per-S coupling table of a model rule, [S index = 2S][typep][type]
-static std::vector<double> coef_table(int smax, int ntypes, const std::function<double(double, int, int)>& f) {
-  std::vector<double> t((size_t)(smax + 1) * ntypes * ntypes, 0.0);
-  for (int si = 0; si <= smax; si++)
-    for (int a = 0; a < ntypes; a++)
-      for (int b = 0; b < ntypes; b++) t[((size_t)si * ntypes + a) * ntypes + b] = f(0.5 * si, a, b);
-  return t;
-}
+// per-S coupling table of a model rule, [S index = 2S][typep][type]; rows
+// for spins already evaluated are kept and only new spin values are computed
+struct CoefTable {
+  std::function<double(double, int, int)> f;
+  int ntypes = 0;
+  int done = -1;
+  std::vector<double> t;
+  const std::vector<double>& upto(int smax) {
+    if (smax > done) {
+      t.resize((size_t)(smax + 1) * ntypes * ntypes, 0.0);
+      for (int si = done + 1; si <= smax; si++)
+        for (int a = 0; a < ntypes; a++)
+          for (int b = 0; b < ntypes; b++) t[((size_t)si * ntypes + a) * ntypes + b] = f(0.5 * si, a, b);
+      done = smax;
+    }
+    return t;
+  }
+};
 static int max_sidx(const CNRGarray& A) {
   int m = 0;
   for (int b = 0; b < A.NumBlocks(); b++) m = std::max(m, (int)key2(A.GetQNumber(b, 1)));
@@ -868,13 +890,14 @@ static int run_onechqs(nrg_ctx* c, nrg_onechqs_params p, nrg_step_cb cb, void* u
   ch.emit(cb, user, 0.0, HUGE_VAL, ch.Aeig.Nstates(), nullptr, nullptr, nullptr, DN / bbar, ops);
 
   double par_chi = chi_m1;
+  CoefTable hn_tab;
+  hn_tab.ntypes = 4;
+  hn_tab.f = [&st](double S, int a, int b) { return onech_hn_coef(st, S, a, b); };
   auto t0 = std::chrono::steady_clock::now();
   for (int N = 0; N < p.nsitesmax; N++) {
     DN = HLF * std::pow(Lambda, -(N - 1) / 2.0);
     build_basis(ch.AeigCut, ch.Abasis, st, 0, 0);
-    std::vector<double> coef = coef_table(max_sidx(ch.Abasis), 4, [&st](double S, int a, int b) {
-      return onech_hn_coef(st, S, a, b);
-    });
+    const std::vector<double>& coef = hn_tab.upto(max_sidx(ch.Abasis));
     int fop = 0;
     if ((rc = ch.diag_step(1, &fop, &par_chi, &coef, std::sqrt(Lambda), 0.0))) return rc;
     // SetE0zero, SetKept (SetDegen + Ecut), CalcThermo at 4 betabar values
@@ -955,11 +978,16 @@ static int run_twochqs(nrg_ctx* c, nrg_twochqs_params p, nrg_step_cb cb, void* u
   if ((rc = ch.set_op(2, {0, 1, 1, 2}, {0, 0, 1, 1}, {1.0, -std::sqrt(2.0)}))) return rc;
   if ((rc = make_basis_dev(c, ch.AeigCut, "oldb", ch.oldb))) return rc;
   build_basis(ch.AeigCut, ch.Abasis, st, 1, 1);
+  CoefTable h0_tab[2], hn_tab[2];
+  for (int k = 0; k < 2; k++) {
+    h0_tab[k].ntypes = hn_tab[k].ntypes = 16;
+    h0_tab[k].f = [&st, k](double S, int a, int b) { return twoch_hn_coef(st, T, k + 1, S, a, b, 1); };
+    hn_tab[k].f = [&st, k](double S, int a, int b) { return twoch_hn_coef(st, T, k + 1, S, a, b, 0); };
+  }
   {
     int smax = max_sidx(ch.Abasis);
     std::vector<double> coef[2];
-    for (int k = 0; k < 2; k++)
-      coef[k] = coef_table(smax, 16, [&st, k](double S, int a, int b) { return twoch_hn_coef(st, T, k + 1, S, a, b, 1); });
+    for (int k = 0; k < 2; k++) coef[k] = h0_tab[k].upto(smax);
     int fops[2] = {2, 2};
     if ((rc = ch.diag_step(2, fops, chi0, coef, 1.0, 0.0))) return rc;
   }
@@ -995,8 +1023,7 @@ static int run_twochqs(nrg_ctx* c, nrg_twochqs_params p, nrg_step_cb cb, void* u
     build_basis(ch.AeigCut, ch.Abasis, st, 1, 0);
     int smax = max_sidx(ch.Abasis);
     std::vector<double> coef[2];
-    for (int k = 0; k < 2; k++)
-      coef[k] = coef_table(smax, 16, [&st, k](double S, int a, int b) { return twoch_hn_coef(st, T, k + 1, S, a, b, 0); });
+    for (int k = 0; k < 2; k++) coef[k] = hn_tab[k].upto(smax);
     int fops[2] = {0, 1};
     if ((rc = ch.diag_step(2, fops, chiN, coef, std::sqrt(L), 0.0))) return rc;
     emin = *std::min_element(ch.Aeig.dEn.begin(), ch.Aeig.dEn.end());

@@ -481,7 +481,7 @@ __global__ __launch_bounds__(E2_T) void dc_secular_kernel(const E2Block* __restr
     for (int l = 0; l < k; l++) {
       const double del = (dk[l] - dorg) - tau;
       const double w = zk[l] / del;
-      f += rho * zk[l] * (1.0 / del);
+      f += rho * zk[l] * w;
       fp += rho * w * w;
     }
     if (f == 0.0) break;

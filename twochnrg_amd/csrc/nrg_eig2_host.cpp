@@ -226,3 +226,34 @@ int eig2_dev(nrg_ctx* c, int nb, const int* n, const int64_t* a_off, double* dA,
 }
 
 }  // namespace nrg
+
+// ---------------------------------------------------------------------------
+// stage outputs of the batched solver for one matrix (diagnostics)
+// ---------------------------------------------------------------------------
+extern "C" int nrg_eig2_stages(nrg_ctx* c, int n, const double* A, double* d_out, double* e_out, double* dd_out,
+                               double* qv_out, double* w_out, double* z_out) {
+  using namespace nrg;
+  if (!c || n <= 0 || !A) return fail(NRG_ERR_ARG, "nrg_eig2_stages: bad arguments");
+  HIPCHK(hipSetDevice(c->device));
+  DBuf& dA = c->buf("dbg_A");
+  DBuf& dW = c->buf("dbg_W");
+  int rc;
+  if ((rc = dA.reserve(sizeof(double) * (size_t)n * n))) return rc;
+  if ((rc = dW.reserve(sizeof(double) * (size_t)n))) return rc;
+  HIPCHK(hipMemcpy(dA.p, A, sizeof(double) * (size_t)n * n, hipMemcpyHostToDevice));
+  int64_t a0 = 0, w0 = 0;
+  if ((rc = eig2_dev(c, 1, &n, &a0, (double*)dA.p, &w0, (double*)dW.p))) return rc;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  // the single block's workspace starts at offset 0: q,k,u,dl,t (5 n^2), panel, vectors
+  const int64_t nn = n, q_off = 0;
+  const int64_t p_off = 5 * nn * nn;
+  const int64_t v_off = p_off + 32 * nn + 2 * 32 * 32 + 2 * nn * 32;
+  double* WS = (double*)c->buf("e2_ws").p;
+  if (d_out) HIPCHK(hipMemcpy(d_out, WS + v_off, sizeof(double) * n, hipMemcpyDeviceToHost));
+  if (e_out) HIPCHK(hipMemcpy(e_out, WS + v_off + 16 * nn, sizeof(double) * n, hipMemcpyDeviceToHost));
+  if (dd_out) HIPCHK(hipMemcpy(dd_out, WS + v_off + 5 * nn, sizeof(double) * n, hipMemcpyDeviceToHost));
+  if (qv_out) HIPCHK(hipMemcpy(qv_out, WS + q_off, sizeof(double) * nn * nn, hipMemcpyDeviceToHost));
+  if (w_out) HIPCHK(hipMemcpy(w_out, dW.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+  if (z_out) HIPCHK(hipMemcpy(z_out, dA.p, sizeof(double) * nn * nn, hipMemcpyDeviceToHost));
+  return NRG_OK;
+}

@@ -177,6 +177,12 @@ extern "C" int nrg_set_eig_impl(nrg_ctx* c, int impl) {
   return NRG_OK;
 }
 
+extern "C" int nrg_set_plain(nrg_ctx* c, int on) {
+  if (!c) return fail(NRG_ERR_ARG, "nrg_set_plain: NULL ctx");
+  c->plain = on != 0;
+  return NRG_OK;
+}
+
 extern "C" int nrg_set_timing(nrg_ctx* c, int on) {
   if (!c) return fail(NRG_ERR_ARG, "nrg_set_timing: NULL ctx");
   c->timing = on != 0;
@@ -252,7 +258,12 @@ int nrg::eig_dev(nrg_ctx* c, int nb, const int* n, const int64_t* a_off, double*
   PendingTimer tm;
   if ((rc = c->timer_begin(tm, NRG_K_EIG, fl, by))) return rc;
   if (c->eig_impl == 2) {
-    if ((rc = eig2_dev(c, nb, n, a_off, dA, w_off, dW))) return rc;
+    if (c->plain) {
+      for (int b = 0; b < nb; b++)
+        if ((rc = eig2_dev(c, 1, n + b, a_off + b, dA, w_off + b, dW))) return rc;
+    } else if ((rc = eig2_dev(c, nb, n, a_off, dA, w_off, dW))) {
+      return rc;
+    }
     if ((rc = c->timer_end(tm))) return rc;
     if (info_host) for (int b = 0; b < nb; b++) info_host[b] = 0;
     return NRG_OK;
@@ -350,6 +361,14 @@ extern "C" int nrg_truncate(int nstates, double* en, int ncut, double myeps, uin
 int nrg::rotate_dev(nrg_ctx* c, const std::vector<RotJob>& jobs, const double* dZ, const double* dF, double* dAux,
                     double* dOut) {
   if (jobs.empty()) return NRG_OK;
+  if (c->plain && jobs.size() > 1) {
+    for (size_t q = 0; q < jobs.size(); q++) {
+      std::vector<RotJob> one(1, jobs[q]);
+      int rc = rotate_dev(c, one, dZ, dF, dAux, dOut);
+      if (rc) return rc;
+    }
+    return NRG_OK;
+  }
   // pass 1: auxT(kj x ni) = Zj(kj x nj) * F(ni x nj)^T ; pass 2: out(ki x kj) = Zi(ki x ni) * auxT(kj x ni)^T
   std::vector<GemmDesc> p1, p2;
   std::vector<GemmTile> t1, t2;

@@ -55,6 +55,7 @@ struct nrg_ctx {
   DBuf& buf(const std::string& name);
   bool timing = false;
   int eig_impl = 2;   // 1: one workgroup per block (Householder + QL); 2: tridiag + D&C + WY
+  bool plain = false; // one block (eigensolve) / one block pair (rotation) per launch sequence
   KStat stat[NRG_NKSTAT];
   std::vector<PendingTimer> pending;
   std::vector<hipEvent_t> event_pool;
