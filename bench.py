@@ -57,6 +57,8 @@ def gpu_run(name, steps, warmup, eig_impl=2):
         s = st.contents
         stamps[s.nshell] = time.perf_counter()
         sizes[s.nshell] = [s.bbe[2 * i + 1] - s.bbe[2 * i] + 1 for i in range(s.nblocks)]
+        print("bench: N=%d done, %d blocks, max block %d, %d states" %
+              (s.nshell, s.nblocks, max(sizes[s.nshell]), sum(sizes[s.nshell])), file=sys.stderr, flush=True)
         if s.nshell == t_start_N:
             ctx.reset_stats()
 

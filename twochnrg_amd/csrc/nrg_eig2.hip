@@ -2,10 +2,11 @@
 // blocks, built from three stages that each run over all blocks at once:
 //
 //   1. Householder reduction to tridiagonal form.  Blocks with n <= 256: one
-//      256-thread workgroup per block.  Larger blocks: two launches per column
-//      step, each workgroup owning a 256-column slab (symv, then rank-2 update);
-//      a thread owns one column, so every row access of the wavefront is one
-//      contiguous 512-byte segment.
+//      256-thread workgroup per block.  Larger blocks: blocked reduction in
+//      panels of 32 columns -- per column a 2-D tiled y = A v (a thread owns a
+//      column, so each row access of a wavefront is one contiguous 512-byte
+//      segment) with the panel corrections, per panel one rank-64 GEMM update
+//      of the trailing matrix.
 //   2. Divide and conquer on the tridiagonal matrix (Cuppen's tearing): leaves
 //      of <= 32 rows by implicit QL; merges by z-vector deflation (small
 //      components, and Givens rotation of nearly equal poles), secular-equation
@@ -109,105 +110,230 @@ __global__ __launch_bounds__(E2_T) void tridiag_small_kernel(const E2Block* __re
 }
 
 // ---------------------------------------------------------------------------
-// stage 1b: one column step k for every large block.  Launch 1 (symv):
-// every workgroup recomputes the reflector of row k (O(n)), then q over its
-// column slab and a partial v.q; launch 2 (update): rank-2 update of its slab.
+// stage 1b: blocked reduction of the large blocks (n > 256), panels of PANEL
+// columns.  Inside a panel the trailing matrix is left as it was at the panel
+// start; column k is brought up to date from the panel's reflectors V and the
+// vectors W (A_k -= V W_k^T + W V_k^T), and the product y = A v is corrected
+// the same way (y -= V (W^T v) + W (V^T v)).  At the panel end one GEMM applies
+// A -= [V W] [W V]^T to the trailing matrix.  Two launches per column:
+//   symv(k): the reflector of column k (from partial norms), y = A v over 2-D
+//            tiles (column tile x row tile), v^T A v per tile, and the panel
+//            dot products V^T v, W^T v over column chunks;
+//   fin(k):  w = tau y + alpha v per column chunk, the reflector stored in
+//            row k, and (inside a panel) column k+1 brought up to date with
+//            its partial norms.
+//   norm(k): partial norms of column k at a panel start (after the GEMM).
+// Panel buffer X (3 PANEL rows of n): rows [0,P) V^T, [P,2P) W^T, [2P,3P) V^T,
+// so that rows [0,2P) and [P,3P) are the two operands of the panel GEMM.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void reflector(const double* rowk, int k, int n, double* red, double& nrm2, double& x0) {
-  double part = 0.0;
-  for (int j = k + 1 + threadIdx.x; j < n; j += blockDim.x) { double x = rowk[j]; part += x * x; }
-  nrm2 = block_sum(part, red);
-  x0 = rowk[k + 1];
+#define SYMV_R 128   // rows per symv tile
+#define CHUNK 256    // columns per symv tile / elements per chunk
+
+struct TdbPtr {
+  double *A, *X, *d, *e, *tau, *refl, *npart, *part, *tpart, *cpart;
+};
+__device__ __forceinline__ TdbPtr tdb_ptrs(const E2Block& B, double* A0, double* WS) {
+  const size_t n = B.n;
+  TdbPtr p;
+  p.A = A0 + B.a_off;
+  p.X = WS + B.p_off;
+  p.d = WS + B.v_off;
+  p.e = p.d + n;
+  p.tau = p.e + n;
+  p.refl = WS + B.v_off + 8 * n;    // t, v0, alpha
+  p.npart = WS + B.v_off + 9 * n;   // partial norms, one per chunk
+  p.part = WS + B.k_off;            // row tiles x n partial y
+  p.tpart = WS + B.u_off;           // one v^T A v partial per tile
+  p.cpart = WS + B.dl_off;          // column chunk x 2P panel dot products
+  return p;
 }
 
-__global__ __launch_bounds__(E2_T) void tridiag_symv_kernel(const E2Block* __restrict__ blks,
-                                                             const int* __restrict__ list, int k,
-                                                             double* __restrict__ A0, double* __restrict__ WS) {
-  __shared__ double red[E2_T / WAVE];
-  const E2Block B = blks[list[blockIdx.y]];
-  const int n = B.n;
-  if (k + 2 >= n) return;
-  const int c0 = blockIdx.x * E2_T;
-  if (c0 >= n) return;
-  double* A = A0 + B.a_off;
-  double* d = WS + B.v_off;
-  double* q = d + 3 * (size_t)n;        // q[n]
-  double* part = q + n;                 // partial sums, one per slab
-  double* refl = part + (n + E2_T - 1) / E2_T;  // (t, v0, alpha)
-  const double* rowk = A + (size_t)k * n;
-  double nrm2, x0;
-  reflector(rowk, k, n, red, nrm2, x0);
-  if (nrm2 == 0.0) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) { refl[0] = 0.0; refl[1] = 0.0; refl[2] = 0.0; }
-    return;
-  }
-  const double nrm = sqrt(nrm2), alpha = (x0 > 0.0) ? -nrm : nrm, v0 = x0 - alpha;
-  const double t = 2.0 / (nrm2 - x0 * x0 + v0 * v0);
-  const int i = c0 + threadIdx.x;
-  double qi = 0.0, vi = 0.0;
-  if (i > k && i < n) {
-    // v_j = A[k][j] for j > k+1, v_{k+1} = v0 (row k is read by every lane: broadcast)
-    qi = A[(size_t)(k + 1) * n + i] * v0;
-    for (int j = k + 2; j < n; j++) qi += A[(size_t)j * n + i] * rowk[j];
-    qi *= t;
-    q[i] = qi;
-    vi = (i == k + 1) ? v0 : rowk[i];
-  }
-  const double vp = block_sum(qi * vi, red);
-  if (threadIdx.x == 0) part[blockIdx.x] = vp;
-  if (blockIdx.x == 0 && threadIdx.x == 0) { refl[0] = t; refl[1] = v0; refl[2] = alpha; }
+__device__ __forceinline__ void tdb_reflector(const TdbPtr& P, int k, int n, double& t, double& v0, double& alpha) {
+  const int nch = (n - k + CHUNK - 1) / CHUNK;
+  double nrm2 = 0.0;
+  for (int c = 0; c < nch; c++) nrm2 += P.npart[c];
+  const double x0 = P.A[(size_t)k * n + k + 1];
+  if (nrm2 == 0.0) { t = 0.0; v0 = 0.0; alpha = 0.0; return; }
+  const double nrm = sqrt(nrm2);
+  alpha = (x0 > 0.0) ? -nrm : nrm;
+  v0 = x0 - alpha;
+  t = 2.0 / (nrm2 - x0 * x0 + v0 * v0);
 }
 
-__global__ __launch_bounds__(E2_T) void tridiag_update_kernel(const E2Block* __restrict__ blks,
-                                                               const int* __restrict__ list, int k,
-                                                               double* __restrict__ A0, double* __restrict__ WS) {
+// partial norms of column k (entries i > k) at a panel start; for the last,
+// partial panel also clear the unused panel rows in the two columns the
+// final GEMM reads
+__global__ __launch_bounds__(CHUNK) void tdb_norm_kernel(const E2Block* __restrict__ blks,
+                                                         const int* __restrict__ list, int k, double* __restrict__ A0,
+                                                         double* __restrict__ WS) {
+  __shared__ double red[CHUNK / WAVE];
   const E2Block B = blks[list[blockIdx.y]];
   const int n = B.n;
-  if (k + 2 >= n) return;
-  const int c0 = blockIdx.x * E2_T;
-  if (c0 >= n) return;
-  double* A = A0 + B.a_off;
-  double* d = WS + B.v_off;
-  double* e = d + n;
-  double* tau = e + n;
-  double* q = tau + n;
-  const int nslab = (n + E2_T - 1) / E2_T;
-  double* part = q + n;
-  double* refl = part + nslab;
-  const double t = refl[0], v0 = refl[1], alpha = refl[2];
-  double* rowk = A + (size_t)k * n;
-  if (t == 0.0) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) { d[k] = rowk[k]; e[k] = rowk[k + 1]; tau[k] = 0.0; }
-    return;
-  }
-  double vp = 0.0;
-  for (int s = 0; s < nslab; s++) vp += part[s];
-  const double K = 0.5 * t * vp;
-  const int i = c0 + threadIdx.x;
-  if (i > k && i < n) {
-    const double vi = (i == k + 1) ? v0 : rowk[i];
-    const double q2 = q[i] - K * vi;
-    for (int j = k + 1; j < n; j++) {
-      const double vj = (j == k + 1) ? v0 : rowk[j];
-      const double qj = q[j] - K * vj;
-      const double upd = (j <= i) ? __fma_rn(vj, q2, vi * qj) : __fma_rn(vi, qj, vj * q2);
-      A[(size_t)j * n + i] -= upd;
+  if (k + 2 >= n || (k % PANEL) != 0) return;
+  const int c = blockIdx.x;
+  if (k + c * CHUNK >= n) return;
+  const TdbPtr P = tdb_ptrs(B, A0, WS);
+  const int i = k + c * CHUNK + threadIdx.x;
+  const double a = (i > k && i < n) ? P.A[(size_t)k * n + i] : 0.0;
+  const double s = block_sum(a * a, red);
+  if (threadIdx.x == 0) P.npart[c] = s;
+  const int nbp = min(PANEL, n - 2 - k);
+  if (c == 0 && nbp < PANEL && threadIdx.x < 3 * PANEL) {
+    const int p = threadIdx.x % PANEL, sect = threadIdx.x / PANEL;
+    if (p >= nbp) {
+      P.X[(size_t)(sect * PANEL + p) * n + n - 2] = 0.0;
+      P.X[(size_t)(sect * PANEL + p) * n + n - 1] = 0.0;
     }
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    d[k] = rowk[k]; e[k] = alpha; tau[k] = t;
+}
+
+__global__ __launch_bounds__(CHUNK) void tdb_symv_kernel(const E2Block* __restrict__ blks,
+                                                         const int* __restrict__ list, int k, double* __restrict__ A0,
+                                                         double* __restrict__ WS) {
+  __shared__ double red[CHUNK / WAVE];
+  __shared__ double vs[CHUNK];
+  const E2Block B = blks[list[blockIdx.y]];
+  const int n = B.n;
+  if (k + 2 >= n) return;
+  const int m = n - k - 1;
+  const int CT = (m + CHUNK - 1) / CHUNK, RT = (m + SYMV_R - 1) / SYMV_R;
+  const int x = blockIdx.x;
+  if (x >= CT * RT + CT) return;
+  const TdbPtr P = tdb_ptrs(B, A0, WS);
+  double t, v0, alpha;
+  tdb_reflector(P, k, n, t, v0, alpha);
+  const double* rowk = P.A + (size_t)k * n;
+  const int tid = threadIdx.x;
+  if (x == 0 && tid == 0) {
+    P.d[k] = rowk[k];
+    P.e[k] = alpha;
+    P.tau[k] = t;
+    P.refl[0] = t; P.refl[1] = v0; P.refl[2] = alpha;
+  }
+  if (x < CT * RT) {
+    const int ct = x % CT, rt = x / CT;
+    const int r0 = k + 1 + rt * SYMV_R, c0 = k + 1 + ct * CHUNK;
+    const int nr = min(SYMV_R, n - r0);
+    if (tid < nr) vs[tid] = (r0 + tid == k + 1) ? v0 : rowk[r0 + tid];
+    __syncthreads();
+    const int i = c0 + tid;
+    double s = 0.0;
+    if (i < n) {
+      const double* col = P.A + (size_t)r0 * n + i;
+      double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+      int l = 0;
+      for (; l + 4 <= nr; l += 4) {
+        s0 += col[(size_t)l * n] * vs[l];
+        s1 += col[(size_t)(l + 1) * n] * vs[l + 1];
+        s2 += col[(size_t)(l + 2) * n] * vs[l + 2];
+        s3 += col[(size_t)(l + 3) * n] * vs[l + 3];
+      }
+      for (; l < nr; l++) s0 += col[(size_t)l * n] * vs[l];
+      s = (s0 + s1) + (s2 + s3);
+      P.part[(size_t)rt * n + i] = s;
+    }
+    const double vi = (i < n) ? ((i == k + 1) ? v0 : rowk[i]) : 0.0;
+    const double tv = block_sum(s * vi, red);
+    if (tid == 0) P.tpart[x] = tv;
+  } else {
+    // panel dot products over one column chunk: cpart[c][p] = W_p . v, cpart[c][P+p] = V_p . v
+    const int c = x - CT * RT;
+    const int j = k % PANEL;
+    const int c0 = k + 1 + c * CHUNK;
+    const int i = c0 + tid;
+    vs[tid] = (i < n) ? ((i == k + 1) ? v0 : rowk[i]) : 0.0;
+    __syncthreads();
+    const int lane = tid & (WAVE - 1), wv = tid / WAVE;
+    for (int p = wv; p < j; p += CHUNK / WAVE) {
+      const double* Vp = P.X + (size_t)p * n;
+      const double* Wp = P.X + (size_t)(PANEL + p) * n;
+      double a1 = 0.0, a2 = 0.0;
+      for (int q = lane; q < CHUNK; q += WAVE) {
+        const int ii = c0 + q;
+        if (ii < n) { a1 += Wp[ii] * vs[q]; a2 += Vp[ii] * vs[q]; }
+      }
+      a1 = wave_sum(a1);
+      a2 = wave_sum(a2);
+      if (lane == 0) { P.cpart[(size_t)c * 2 * PANEL + p] = a1; P.cpart[(size_t)c * 2 * PANEL + PANEL + p] = a2; }
+    }
   }
 }
 
-// the reflector head v0 goes into row k only after every slab of step k has
-// read the old A[k][k+1]
-__global__ void tridiag_store_kernel(const E2Block* __restrict__ blks, const int* __restrict__ list, int k,
-                                     double* __restrict__ A0, const double* __restrict__ WS) {
-  const E2Block B = blks[list[blockIdx.x]];
+__global__ __launch_bounds__(CHUNK) void tdb_fin_kernel(const E2Block* __restrict__ blks,
+                                                        const int* __restrict__ list, int k, double* __restrict__ A0,
+                                                        double* __restrict__ WS) {
+  __shared__ double red[CHUNK / WAVE];
+  __shared__ double c1s[PANEL], c2s[PANEL], vk1[PANEL], wk1[PANEL];
+  const E2Block B = blks[list[blockIdx.y]];
   const int n = B.n;
-  if (k + 2 >= n || threadIdx.x != 0) return;
-  const double* refl = WS + B.v_off + 4 * (size_t)n + (n + E2_T - 1) / E2_T;
-  if (refl[0] != 0.0) A0[B.a_off + (size_t)k * n + k + 1] = refl[1];
+  if (k + 2 >= n) return;
+  const int m = n - k - 1;
+  const int CT = (m + CHUNK - 1) / CHUNK, RT = (m + SYMV_R - 1) / SYMV_R;
+  const int c = blockIdx.x;
+  if (c >= CT) return;
+  const TdbPtr P = tdb_ptrs(B, A0, WS);
+  const int tid = threadIdx.x, j = k % PANEL;
+  const double t = P.refl[0], v0 = P.refl[1];
+  if (tid < j) {
+    double a1 = 0.0, a2 = 0.0;
+    for (int cc = 0; cc < CT; cc++) {
+      a1 += P.cpart[(size_t)cc * 2 * PANEL + tid];
+      a2 += P.cpart[(size_t)cc * 2 * PANEL + PANEL + tid];
+    }
+    c1s[tid] = a1;   // W_p . v
+    c2s[tid] = a2;   // V_p . v
+  }
+  double tv = 0.0;
+  for (int q = tid; q < CT * RT; q += CHUNK) tv += P.tpart[q];
+  const double vav = block_sum(tv, red);   // also orders the c1s/c2s stores
+  double cc12 = 0.0;
+  for (int p = 0; p < j; p++) cc12 += c1s[p] * c2s[p];
+  const double alpha2 = -0.5 * t * t * (vav - 2.0 * cc12);
+  const double* rowk = P.A + (size_t)k * n;
+  const int i = k + 1 + c * CHUNK + tid;
+  double vi = 0.0, wi = 0.0;
+  if (i < n) {
+    vi = (i == k + 1) ? v0 : rowk[i];
+    double y = 0.0;
+    for (int rt = 0; rt < RT; rt++) y += P.part[(size_t)rt * n + i];
+    for (int p = 0; p < j; p++) y -= P.X[(size_t)p * n + i] * c1s[p] + P.X[(size_t)(PANEL + p) * n + i] * c2s[p];
+    wi = t * y + alpha2 * vi;
+  }
+  const bool panel_end = (j == PANEL - 1) || (k + 3 >= n);
+  if (!panel_end) {
+    // W(k+1, j) is needed by every chunk: recompute it here
+    if (tid == 0) {
+      double y = 0.0;
+      for (int rt = 0; rt < RT; rt++) y += P.part[(size_t)rt * n + k + 1];
+      for (int p = 0; p < j; p++)
+        y -= P.X[(size_t)p * n + k + 1] * c1s[p] + P.X[(size_t)(PANEL + p) * n + k + 1] * c2s[p];
+      wk1[j] = t * y + alpha2 * v0;
+      vk1[j] = v0;
+    }
+    if (tid < j) {
+      vk1[tid] = P.X[(size_t)tid * n + k + 1];
+      wk1[tid] = P.X[(size_t)(PANEL + tid) * n + k + 1];
+    }
+  }
+  __syncthreads();   // every read of the panel rows at column k+1 precedes the stores below
+  if (i < n) {
+    P.X[(size_t)j * n + i] = vi;
+    P.X[(size_t)(2 * PANEL + j) * n + i] = vi;
+    P.X[(size_t)(PANEL + j) * n + i] = wi;
+    if (i == k + 1) P.A[(size_t)k * n + k + 1] = v0;
+  }
+  if (panel_end) return;
+  // column k+1 brought up to date with the panel's columns 0..j
+  double a = 0.0;
+  if (i < n) {
+    a = P.A[(size_t)(k + 1) * n + i];
+    for (int p = 0; p < j; p++)
+      a -= P.X[(size_t)p * n + i] * wk1[p] + P.X[(size_t)(PANEL + p) * n + i] * vk1[p];
+    a -= vi * wk1[j] + wi * vk1[j];
+    P.A[(size_t)(k + 1) * n + i] = a;
+  }
+  const double s = block_sum((i > k + 1 && i < n) ? a * a : 0.0, red);
+  if (tid == 0) P.npart[c] = s;
 }
 
 __global__ void tridiag_tail_kernel(const E2Block* __restrict__ blks, const int* __restrict__ list,
@@ -348,17 +474,32 @@ __global__ __launch_bounds__(64) void dc_leaf_kernel(const E2Block* __restrict__
   }
 }
 
-// merge preparation: z vector, sort of the two halves, deflation, Givens
+// merge preparation: z vector, merge of the two sorted halves (parallel, by
+// binary search), deflation scan (sequential, staged through LDS), Givens
+// rotations of the deflated pairs
+__device__ __forceinline__ int count_less(const double* a, int n, double x) {   // #{a_i < x}, a ascending
+  int lo = 0, hi = n;
+  while (lo < hi) { const int mid = (lo + hi) >> 1; if (a[mid] < x) lo = mid + 1; else hi = mid; }
+  return lo;
+}
+__device__ __forceinline__ int count_leq(const double* a, int n, double x) {    // #{a_i <= x}, a ascending
+  int lo = 0, hi = n;
+  while (lo < hi) { const int mid = (lo + hi) >> 1; if (a[mid] <= x) lo = mid + 1; else hi = mid; }
+  return lo;
+}
+
 __global__ __launch_bounds__(E2_T) void dc_prep_kernel(const E2Block* __restrict__ blks,
                                                        const E2Merge* __restrict__ merges, E2MState* __restrict__ ms,
                                                        double* __restrict__ WS, int* __restrict__ IW) {
   __shared__ double red[E2_T / WAVE];
+  __shared__ double cds[E2_T], czs[E2_T];
+  __shared__ int cpm[E2_T];
   __shared__ int sh_ngiv;
   const E2Merge M = merges[blockIdx.x];
   const E2Block B = blks[M.blk];
   const int n = B.n, lo = M.lo, mid = M.mid, hi = M.hi, m = hi - lo, n1 = mid - lo, tid = threadIdx.x;
   double* Qv = WS + B.q_off;
-  double* dd = VD(5);
+  const double* dd = VD(5);
   double* z = VD(6) + lo;
   double* ds = VD(7) + lo;
   double* zs = VD(8) + lo;
@@ -383,44 +524,60 @@ __global__ __launch_bounds__(E2_T) void dc_prep_kernel(const E2Block* __restrict
   const double zn2 = block_sum(part, red);
   const double zn = sqrt(zn2);
   const double rho = beta * zn2;
-  if (tid == 0) {
-    // merge the two ascending runs of eigenvalues
-    int a = 0, b = n1, s = 0;
-    while (a < n1 || b < m) {
-      int take;
-      if (a >= n1) take = b++;
-      else if (b >= m) take = a++;
-      else take = (dd[lo + b] < dd[lo + a]) ? b++ : a++;
-      perm[s] = take;
-      ds[s] = dd[lo + take];
-      zs[s] = (zn > 0.0) ? z[take] / zn : 0.0;
-      s++;
-    }
-    double dmax = 0.0;
-    for (int i = 0; i < m; i++) dmax = fmax(dmax, fabs(ds[i]));
-    const double tol = 8.0 * 2.220446049250313e-16 * fmax(dmax, rho);
-    int k = 0, nd = 0, ng = 0, pend = -1;
-    for (int i = 0; i < m; i++) {
-      if (rho * fabs(zs[i]) <= tol) { dval[nd] = ds[i]; drow[nd] = perm[i]; nd++; continue; }
-      if (pend < 0) { pend = i; continue; }
-      const double zp = zs[pend], zc = zs[i];
-      const double tv = hypot(zc, zp);
-      const double c = zc / tv, sv = -zp / tv;
-      const double gap = ds[i] - ds[pend];
-      if (fabs(gap * c * sv) <= tol) {
-        ga[ng] = perm[pend]; gb[ng] = perm[i]; gc[ng] = c; gs[ng] = sv; ng++;
-        zs[i] = tv; zs[pend] = 0.0;
-        const double dp = ds[pend] * c * c + ds[i] * sv * sv;
-        ds[i] = ds[pend] * sv * sv + ds[i] * c * c;
-        ds[pend] = dp;
-        dval[nd] = dp; drow[nd] = perm[pend]; nd++;
-        pend = i;
-      } else {
-        dk[k] = ds[pend]; zk[k] = zs[pend]; krow[k] = perm[pend]; k++;
-        pend = i;
+  // stable merge of the ascending runs [0,n1) and [n1,m): ties keep the left run first
+  double amax = 0.0;
+  for (int i = tid; i < m; i += blockDim.x) {
+    const double x = dd[lo + i];
+    const int pos = (i < n1) ? i + count_less(dd + lo + n1, m - n1, x) : (i - n1) + count_leq(dd + lo, n1, x);
+    perm[pos] = i;
+    ds[pos] = x;
+    zs[pos] = (zn > 0.0) ? z[i] / zn : 0.0;
+    amax = fmax(amax, fabs(x));
+  }
+  {
+    double v = amax;
+#pragma unroll
+    for (int o = WAVE / 2; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, WAVE));
+    const int lane = tid & (WAVE - 1), wid = tid / WAVE;
+    __syncthreads();
+    if (lane == 0) red[wid] = v;
+    __syncthreads();
+    amax = 0.0;
+    for (int w = 0; w < (int)(blockDim.x / WAVE); w++) amax = fmax(amax, red[w]);
+  }
+  const double tol = 8.0 * 2.220446049250313e-16 * fmax(amax, rho);
+  // deflation scan in sorted order: thread 0 walks each LDS-staged chunk
+  int k = 0, nd = 0, ng = 0, pend = -1, pperm = 0;
+  double pds = 0.0, pzs = 0.0;
+  for (int c0 = 0; c0 < m; c0 += E2_T) {
+    const int cnt = min(E2_T, m - c0);
+    __syncthreads();
+    if (tid < cnt) { cds[tid] = ds[c0 + tid]; czs[tid] = zs[c0 + tid]; cpm[tid] = perm[c0 + tid]; }
+    __syncthreads();
+    if (tid == 0) {
+      for (int q = 0; q < cnt; q++) {
+        double di = cds[q], zi = czs[q];
+        const int pi = cpm[q];
+        if (rho * fabs(zi) <= tol) { dval[nd] = di; drow[nd] = pi; nd++; continue; }
+        if (pend < 0) { pend = c0 + q; pds = di; pzs = zi; pperm = pi; continue; }
+        const double tv = hypot(zi, pzs);
+        const double cc = zi / tv, sv = -pzs / tv;
+        const double gap = di - pds;
+        if (fabs(gap * cc * sv) <= tol) {
+          ga[ng] = pperm; gb[ng] = pi; gc[ng] = cc; gs[ng] = sv; ng++;
+          const double dp = pds * cc * cc + di * sv * sv;
+          di = pds * sv * sv + di * cc * cc;
+          zi = tv;
+          dval[nd] = dp; drow[nd] = pperm; nd++;
+        } else {
+          dk[k] = pds; zk[k] = pzs; krow[k] = pperm; k++;
+        }
+        pend = c0 + q; pds = di; pzs = zi; pperm = pi;
       }
     }
-    if (pend >= 0) { dk[k] = ds[pend]; zk[k] = zs[pend]; krow[k] = perm[pend]; k++; }
+  }
+  if (tid == 0) {
+    if (pend >= 0) { dk[k] = pds; zk[k] = pzs; krow[k] = pperm; k++; }
     E2MState st;
     st.k = k; st.nd = nd; st.ngiv = ng; st.pad = 0; st.rho = rho; st.tol = tol;
     ms[blockIdx.x] = st;
@@ -428,8 +585,8 @@ __global__ __launch_bounds__(E2_T) void dc_prep_kernel(const E2Block* __restrict
   }
   __syncthreads();
   // Givens rotations of the deflated pairs, in order: q_a' = c q_a + s q_b, q_b' = -s q_a + c q_b
-  const int ng = sh_ngiv;
-  for (int g = 0; g < ng; g++) {
+  const int ngv = sh_ngiv;
+  for (int g = 0; g < ngv; g++) {
     double* ra = Qv + (size_t)(lo + ga[g]) * n + lo;
     double* rb = Qv + (size_t)(lo + gb[g]) * n + lo;
     const double c = gc[g], s = gs[g];
@@ -442,8 +599,11 @@ __global__ __launch_bounds__(E2_T) void dc_prep_kernel(const E2Block* __restrict
   }
 }
 
-// secular equation root j: f(lambda) = 1 + rho sum_l zk_l^2/(dk_l - lambda);
-// lambda = dk[org] + tau with the nearer pole as origin; delta(j,l) = dk_l - lambda_j
+// secular equation root j: f(lambda) = 1 + rho sum_l zk_l^2/(dk_l - lambda), one
+// wavefront per root.  lambda = dk[org] + tau with the nearer pole as origin;
+// iteration: the two-pole rational model of the left and right sums (matching
+// value and slope at the iterate), safeguarded by bisection of the bracket;
+// delta(j,l) = dk_l - lambda_j
 __global__ __launch_bounds__(E2_T) void dc_secular_kernel(const E2Block* __restrict__ blks,
                                                           const E2Merge* __restrict__ merges,
                                                           const E2MState* __restrict__ ms, double* __restrict__ WS,
@@ -452,7 +612,8 @@ __global__ __launch_bounds__(E2_T) void dc_secular_kernel(const E2Block* __restr
   const E2MState S = ms[blockIdx.y];
   const E2Block B = blks[M.blk];
   const int n = B.n, lo = M.lo, k = S.k;
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.x * (E2_T / WAVE) + threadIdx.x / WAVE;
+  const int lane = threadIdx.x & (WAVE - 1);
   if (j >= k) return;
   const double* dk = VD(9) + lo;
   const double* zk = VD(10) + lo;
@@ -461,40 +622,73 @@ __global__ __launch_bounds__(E2_T) void dc_secular_kernel(const E2Block* __restr
   const int m = M.hi - M.lo;
   double* delta = WS + B.dl_off + (size_t)lo * n;   // k rows of stride m, row j = delta(j, .)
   const double rho = S.rho;
+  const double eps = 2.220446049250313e-16;
   int o;
   double tlo, thi;
-  if (j < k - 1) {
+  const bool last = (j == k - 1);
+  if (!last) {
     const double gap = dk[j + 1] - dk[j], half = 0.5 * gap;
-    double f = 1.0;
-    for (int l = 0; l < k; l++) f += rho * zk[l] * zk[l] / ((dk[l] - dk[j]) - half);
+    double f = 0.0;
+    for (int l = lane; l < k; l += WAVE) f += zk[l] * zk[l] / ((dk[l] - dk[j]) - half);
+    f = 1.0 + rho * wave_sum(f);
     if (f >= 0.0) { o = j; tlo = 0.0; thi = half; }
     else { o = j + 1; tlo = -(gap - half); thi = 0.0; }
   } else {
     double zz = 0.0;
-    for (int l = 0; l < k; l++) zz += zk[l] * zk[l];
-    o = j; tlo = 0.0; thi = rho * zz;
+    for (int l = lane; l < k; l += WAVE) zz += zk[l] * zk[l];
+    o = j; tlo = 0.0; thi = rho * wave_sum(zz);
   }
   const double dorg = dk[o];
+  const double DL = dk[j] - dorg;                  // left pole, relative to the origin
+  const double DR = last ? 0.0 : dk[j + 1] - dorg; // right pole
   double tau = 0.5 * (tlo + thi);
-  for (int it = 0; it < 120; it++) {
-    double f = 1.0, fp = 0.0;
-    for (int l = 0; l < k; l++) {
+  for (int it = 0; it < 64; it++) {
+    double psi = 0.0, dpsi = 0.0, phi = 0.0, dphi = 0.0;
+    for (int l = lane; l < k; l += WAVE) {
       const double del = (dk[l] - dorg) - tau;
       const double w = zk[l] / del;
-      f += rho * zk[l] * w;
-      fp += rho * w * w;
+      if (l <= j) { psi += zk[l] * w; dpsi += w * w; }
+      else { phi += zk[l] * w; dphi += w * w; }
     }
+    psi = rho * wave_sum(psi); dpsi = rho * wave_sum(dpsi);
+    phi = rho * wave_sum(phi); dphi = rho * wave_sum(dphi);
+    const double f = 1.0 + psi + phi;
     if (f == 0.0) break;
     if (f < 0.0) tlo = tau; else thi = tau;
-    double tn = tau - f / fp;
+    // rational model in the step s: g(s) = C + a/(dl - s) + b/(dr - s)
+    const double dl = DL - tau;
+    const double a = dpsi * dl * dl, A = psi - dpsi * dl;
+    double s;
+    bool ok = true;
+    if (last) {
+      const double C = 1.0 + A;
+      if (C > 0.0) s = dl + a / C; else ok = false;
+    } else {
+      const double dr = DR - tau;
+      const double b = dphi * dr * dr, Bc = phi - dphi * dr;
+      const double C = 1.0 + A + Bc;
+      const double qa = C, qb = -(C * (dl + dr) + a + b), qc = f * dl * dr;
+      const double disc = qb * qb - 4.0 * qa * qc;
+      if (disc < 0.0 || qa == 0.0) ok = false;
+      else {
+        const double q = -0.5 * (qb + copysign(sqrt(disc), qb));
+        const double s1 = q / qa, s2 = (q != 0.0) ? qc / q : 0.0;
+        const double blo = tlo - tau, bhi = thi - tau;
+        const bool in1 = s1 > blo && s1 < bhi, in2 = s2 > blo && s2 < bhi;
+        if (in2 && (!in1 || fabs(s2) < fabs(s1))) s = s2;
+        else if (in1) s = s1;
+        else ok = false;
+      }
+    }
+    double tn = ok ? tau + s : 0.5 * (tlo + thi);
     if (!(tn > tlo && tn < thi)) tn = 0.5 * (tlo + thi);
-    if (tn == tau || fabs(thi - tlo) <= 4.0 * 2.220446049250313e-16 * fabs(tau)) { tau = tn; break; }
+    const bool done = fabs(tn - tau) <= 2.0 * eps * fabs(tn) || (thi - tlo) <= 4.0 * eps * fmax(fabs(tlo), fabs(thi));
     tau = tn;
+    if (done) break;
   }
-  lam[j] = dorg + tau;
-  org[j] = o;
+  if (lane == 0) { lam[j] = dorg + tau; org[j] = o; }
   double* dj = delta + (size_t)j * m;
-  for (int l = 0; l < k; l++) dj[l] = (dk[l] - dorg) - tau;
+  for (int l = lane; l < k; l += WAVE) dj[l] = (dk[l] - dorg) - tau;
 }
 
 // Gu-Eisenstat: zhat_l^2 = prod_j (lambda_j - d_l) / (rho prod_{j!=l} (d_j - d_l))
@@ -521,7 +715,8 @@ __global__ __launch_bounds__(E2_T) void dc_zhat_kernel(const E2Block* __restrict
   zh[l] = (zk[l] >= 0.0) ? a : -a;
 }
 
-// eigenvector j of the rank-one problem: u_j(l) = zhat_l / (d_l - lambda_j), normalised
+// eigenvector j of the rank-one problem: u_j(l) = zhat_l / (d_l - lambda_j),
+// normalised; one wavefront per vector
 __global__ __launch_bounds__(E2_T) void dc_vec_kernel(const E2Block* __restrict__ blks,
                                                       const E2Merge* __restrict__ merges,
                                                       const E2MState* __restrict__ ms, double* __restrict__ WS) {
@@ -529,23 +724,25 @@ __global__ __launch_bounds__(E2_T) void dc_vec_kernel(const E2Block* __restrict_
   const E2MState S = ms[blockIdx.y];
   const E2Block B = blks[M.blk];
   const int n = B.n, lo = M.lo, k = S.k;
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.x * (E2_T / WAVE) + threadIdx.x / WAVE;
+  const int lane = threadIdx.x & (WAVE - 1);
   if (j >= k) return;
   const int m = M.hi - M.lo;
   const double* zh = VD(12) + lo;
   const double* dj = WS + B.dl_off + (size_t)lo * n + (size_t)j * m;
   double* uj = WS + B.u_off + (size_t)lo * n + (size_t)j * m;
   double s = 0.0;
-  for (int l = 0; l < k; l++) {
+  for (int l = lane; l < k; l += WAVE) {
     const double u = zh[l] / dj[l];
     uj[l] = u;
     s += u * u;
   }
-  const double inv = 1.0 / sqrt(s);
-  for (int l = 0; l < k; l++) uj[l] *= inv;
+  const double inv = 1.0 / sqrt(wave_sum(s));
+  for (int l = lane; l < k; l += WAVE) uj[l] *= inv;
 }
 
-// gather the non-deflated vector rows (components [lo,hi)) into Qk (k x m)
+// gather the non-deflated vector rows (components [lo,hi)) into Qk (k x m) and
+// the deflated rows into rows k..m-1 of Qt (the GEMM writes rows 0..k-1)
 __global__ void dc_gather_kernel(const E2Block* __restrict__ blks, const E2Merge* __restrict__ merges,
                                  const E2MState* __restrict__ ms, double* __restrict__ WS, const int* __restrict__ IW) {
   const E2Merge M = merges[blockIdx.y];
@@ -553,56 +750,91 @@ __global__ void dc_gather_kernel(const E2Block* __restrict__ blks, const E2Merge
   const E2Block B = blks[M.blk];
   const int n = B.n, lo = M.lo, m = M.hi - M.lo, k = S.k;
   const int* krow = VI(1) + lo;
+  const int* drow = VI(2) + lo;
   const double* Qv = WS + B.q_off;
   double* Qk = WS + B.k_off + (size_t)lo * n;
-  for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < (size_t)k * m;
+  double* Qt = WS + B.t_off + (size_t)lo * n;
+  for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < (size_t)m * m;
        idx += (size_t)gridDim.x * blockDim.x) {
     const int r = (int)(idx / m), c = (int)(idx % m);
-    Qk[idx] = Qv[(size_t)(lo + krow[r]) * n + lo + c];
+    if (r < k) Qk[idx] = Qv[(size_t)(lo + krow[r]) * n + lo + c];
+    else Qt[idx] = Qv[(size_t)(lo + drow[r - k]) * n + lo + c];
   }
 }
 
-// final order: merge the ascending non-deflated roots (rows of Qnew) with the
-// deflated eigenvalues (rows of Qv), write eigenvalues and rows back
-__global__ __launch_bounds__(E2_T) void dc_assemble_kernel(const E2Block* __restrict__ blks,
-                                                           const E2Merge* __restrict__ merges,
-                                                           const E2MState* __restrict__ ms, double* __restrict__ WS,
-                                                           int* __restrict__ IW) {
+// final order of one merge: the deflated values are sorted by rank (ties by
+// index), then merged with the ascending roots by binary search (roots first
+// on ties); writes the eigenvalues and the source row of every output row
+__global__ __launch_bounds__(E2_T) void dc_order_kernel(const E2Block* __restrict__ blks,
+                                                        const E2Merge* __restrict__ merges,
+                                                        const E2MState* __restrict__ ms, double* __restrict__ WS,
+                                                        int* __restrict__ IW) {
+  __shared__ double cv[E2_T];
   const E2Merge M = merges[blockIdx.x];
   const E2MState S = ms[blockIdx.x];
   const E2Block B = blks[M.blk];
-  const int n = B.n, lo = M.lo, m = M.hi - M.lo, k = S.k, nd = S.nd, tid = threadIdx.x;
-  double* Qv = WS + B.q_off;
-  double* Qt = WS + B.t_off + (size_t)lo * n;     // rows 0..k-1 hold the new vectors (GEMM output)
+  const int n = B.n, lo = M.lo, k = S.k, nd = S.nd, tid = threadIdx.x;
   const double* lam = VD(11) + lo;
   const double* dval = VD(13) + lo;
   double* dd = VD(5);
-  const int* drow = VI(2) + lo;
+  double* dvs = VD(7) + lo;   // sorted deflated values (the ds scratch of the merge)
+  int* dix = VI(0) + lo;      // their indices (the perm scratch)
   int* src = VI(6) + lo;
-  // deflated rows -> Qt rows k..k+nd-1
-  for (size_t idx = tid; idx < (size_t)nd * m; idx += blockDim.x) {
-    const int r = (int)(idx / m), c = (int)(idx % m);
-    Qt[(size_t)(k + r) * m + c] = Qv[(size_t)(lo + drow[r]) * n + lo + c];
-  }
-  if (tid == 0) {
-    // deflated values are ascending up to rotation-level perturbations: insertion sort keeps it exact
-    int a = 0, b = 0, o = 0;
-    double* dvs = VD(7) + lo;   // reuse ds as scratch for the sorted deflated values
-    int* dix = VI(0) + lo;      // reuse perm
-    for (int i = 0; i < nd; i++) { dvs[i] = dval[i]; dix[i] = i; }
-    for (int i = 1; i < nd; i++) {
-      double x = dvs[i]; int xi = dix[i]; int j = i - 1;
-      while (j >= 0 && dvs[j] > x) { dvs[j + 1] = dvs[j]; dix[j + 1] = dix[j]; j--; }
-      dvs[j + 1] = x; dix[j + 1] = xi;
+  for (int base = 0; base < nd; base += E2_T * 16) {
+    int rk[16];
+    double xv[16];
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+      const int i = base + q * E2_T + tid;
+      xv[q] = (i < nd) ? dval[i] : 0.0;
+      rk[q] = 0;
     }
-    while (a < k || b < nd) {
-      if (b >= nd || (a < k && lam[a] <= dvs[b])) { dd[lo + o] = lam[a]; src[o] = a; a++; }
-      else { dd[lo + o] = dvs[b]; src[o] = k + dix[b]; b++; }
-      o++;
+    for (int c0 = 0; c0 < nd; c0 += E2_T) {
+      const int cnt = min(E2_T, nd - c0);
+      __syncthreads();
+      if (tid < cnt) cv[tid] = dval[c0 + tid];
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < 16; q++) {
+        const int i = base + q * E2_T + tid;
+        if (i < nd)
+          for (int t = 0; t < cnt; t++) {
+            const double y = cv[t];
+            rk[q] += (y < xv[q]) || (y == xv[q] && c0 + t < i);
+          }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+      const int i = base + q * E2_T + tid;
+      if (i < nd) { dvs[rk[q]] = xv[q]; dix[rk[q]] = i; }
     }
   }
   __syncthreads();
-  for (size_t idx = tid; idx < (size_t)m * m; idx += blockDim.x) {
+  for (int a = tid; a < k; a += E2_T) {
+    const int pos = a + count_less(dvs, nd, lam[a]);
+    dd[lo + pos] = lam[a];
+    src[pos] = a;
+  }
+  for (int b = tid; b < nd; b += E2_T) {
+    const int pos = b + count_leq(lam, k, dvs[b]);
+    dd[lo + pos] = dvs[b];
+    src[pos] = k + dix[b];
+  }
+}
+
+// output rows of one merge: Qv row lo+o <- Qt row src[o] (Qt rows 0..k-1 are
+// the GEMM output, k..m-1 the deflated rows)
+__global__ void dc_copy_kernel(const E2Block* __restrict__ blks, const E2Merge* __restrict__ merges,
+                               double* __restrict__ WS, const int* __restrict__ IW) {
+  const E2Merge M = merges[blockIdx.y];
+  const E2Block B = blks[M.blk];
+  const int n = B.n, lo = M.lo, m = M.hi - M.lo;
+  const int* src = VI(6) + lo;
+  double* Qv = WS + B.q_off;
+  const double* Qt = WS + B.t_off + (size_t)lo * n;
+  for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < (size_t)m * m;
+       idx += (size_t)gridDim.x * blockDim.x) {
     const int o = (int)(idx / m), c = (int)(idx % m);
     Qv[(size_t)(lo + o) * n + lo + c] = Qt[(size_t)src[o] * m + c];
   }
@@ -765,15 +997,31 @@ hipError_t e2_launch_tridiag_small(const E2Block* b, const int* list, int nlist,
   return hipGetLastError();
 }
 
-hipError_t e2_launch_tridiag_step(const E2Block* b, const int* list, int nlist, int nslab, int k, double* A,
-                                  double* WS, hipStream_t s) {
-  if (nlist <= 0) return hipSuccess;
-  hipLaunchKernelGGL(e2::tridiag_symv_kernel, dim3(nslab, nlist), dim3(E2_T), 0, s, b, list, k, A, WS);
+hipError_t e2_launch_tdb_norm(const E2Block* b, const int* list, int nlist, int nmax, int k, double* A, double* WS,
+                              hipStream_t s) {
+  if (nlist <= 0 || k + 2 >= nmax) return hipSuccess;
+  const int gx = (nmax - k + CHUNK - 1) / CHUNK;
+  hipLaunchKernelGGL(e2::tdb_norm_kernel, dim3(gx, nlist), dim3(CHUNK), 0, s, b, list, k, A, WS);
+  return hipGetLastError();
+}
+
+hipError_t e2_launch_tdb_column(const E2Block* b, const int* list, int nlist, int nmax, int k, double* A, double* WS,
+                                hipStream_t s) {
+  if (nlist <= 0 || k + 2 >= nmax) return hipSuccess;
+  const int m = nmax - k - 1;
+  const int CT = (m + CHUNK - 1) / CHUNK, RT = (m + SYMV_R - 1) / SYMV_R;
+  hipLaunchKernelGGL(e2::tdb_symv_kernel, dim3(CT * RT + CT, nlist), dim3(CHUNK), 0, s, b, list, k, A, WS);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(e2::tridiag_update_kernel, dim3(nslab, nlist), dim3(E2_T), 0, s, b, list, k, A, WS);
-  if ((e = hipGetLastError()) != hipSuccess) return e;
-  hipLaunchKernelGGL(e2::tridiag_store_kernel, dim3(nlist), dim3(64), 0, s, b, list, k, A, WS);
+  hipLaunchKernelGGL(e2::tdb_fin_kernel, dim3(CT, nlist), dim3(CHUNK), 0, s, b, list, k, A, WS);
+  return hipGetLastError();
+}
+
+hipError_t e2_launch_gemm3(const E2Gemm* probs, const E2Tile* tiles, int ntiles, const double* A, const double* B,
+                           double* C, hipStream_t s) {
+  if (ntiles <= 0) return hipSuccess;
+  hipLaunchKernelGGL(e2::gemm_kernel, dim3(ntiles), dim3(256), 0, s, probs, tiles, A, B, C,
+                     (const E2MState*)nullptr);
   return hipGetLastError();
 }
 
@@ -805,21 +1053,24 @@ hipError_t e2_launch_dc_level(const E2Block* b, const E2Merge* merges, int nm, i
   hipError_t e;
   hipLaunchKernelGGL(e2::dc_prep_kernel, dim3(nm), dim3(E2_T), 0, s, b, merges, ms, WS, IW);
   if ((e = hipGetLastError()) != hipSuccess) return e;
-  const int gy = (mmax + E2_T - 1) / E2_T;
-  hipLaunchKernelGGL(e2::dc_secular_kernel, dim3(gy, nm), dim3(E2_T), 0, s, b, merges, ms, WS, IW);
+  const int gw = (mmax + E2_T / WAVE - 1) / (E2_T / WAVE);   // one wavefront per root / vector
+  const int gy = (mmax + E2_T - 1) / E2_T;                   // one thread per component
+  hipLaunchKernelGGL(e2::dc_secular_kernel, dim3(gw, nm), dim3(E2_T), 0, s, b, merges, ms, WS, IW);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   hipLaunchKernelGGL(e2::dc_zhat_kernel, dim3(gy, nm), dim3(E2_T), 0, s, b, merges, ms, WS);
   if ((e = hipGetLastError()) != hipSuccess) return e;
-  hipLaunchKernelGGL(e2::dc_vec_kernel, dim3(gy, nm), dim3(E2_T), 0, s, b, merges, ms, WS);
+  hipLaunchKernelGGL(e2::dc_vec_kernel, dim3(gw, nm), dim3(E2_T), 0, s, b, merges, ms, WS);
   if ((e = hipGetLastError()) != hipSuccess) return e;
-  int gx = (int)(((size_t)mmax * mmax + 255) / 256); if (gx > 512) gx = 512; if (gx < 1) gx = 1;
+  int gx = (int)(((size_t)mmax * mmax + 255) / 256); if (gx > 1024) gx = 1024; if (gx < 1) gx = 1;
   hipLaunchKernelGGL(e2::dc_gather_kernel, dim3(gx, nm), dim3(256), 0, s, b, merges, ms, WS, IW);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   if (ntiles > 0) {
     hipLaunchKernelGGL(e2::gemm_kernel, dim3(ntiles), dim3(256), 0, s, gprobs, gtiles, WS, WS, WS, ms);
     if ((e = hipGetLastError()) != hipSuccess) return e;
   }
-  hipLaunchKernelGGL(e2::dc_assemble_kernel, dim3(nm), dim3(E2_T), 0, s, b, merges, ms, WS, IW);
+  hipLaunchKernelGGL(e2::dc_order_kernel, dim3(nm), dim3(E2_T), 0, s, b, merges, ms, WS, IW);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  hipLaunchKernelGGL(e2::dc_copy_kernel, dim3(gx, nm), dim3(256), 0, s, b, merges, WS, IW);
   return hipGetLastError();
 }
 

@@ -49,6 +49,35 @@ int eig2_dev(nrg_ctx* c, int nb, const int* n, const int64_t* a_off, double* dA,
     if (n[b] <= kSmall) small.push_back(b);
     else { large.push_back(b); nmax_large = std::max(nmax_large, n[b]); }
   }
+  // ---- stage 1b panel GEMMs: A[s:,s:] -= [V W][W V]^T at each panel end of each large block
+  std::vector<E2Gemm> tprobs;
+  std::vector<E2Tile> ttiles;
+  std::vector<int> tev_k, tev_pbeg, tev_tbeg, tev_tcnt;
+  for (int k = 0; k + 2 < nmax_large; k++) {
+    const int j = k % kPanel, k0 = k - j;
+    const int pbase = (int)tprobs.size(), tbase = (int)ttiles.size();
+    for (int b : large) {
+      const int nb_ = n[b];
+      if (k + 2 >= nb_ || !(j == kPanel - 1 || k + 3 >= nb_)) continue;
+      const int s0 = k0 + std::min(kPanel, nb_ - 2 - k0);
+      const int64_t nn = nb_;
+      E2Gemm g{};
+      g.m = nb_ - s0; g.n = nb_ - s0; g.k = 2 * kPanel; g.dyn = -1;
+      g.a_off = blk[b].p_off + s0; g.as0 = 1; g.as1 = nn;
+      g.b_off = blk[b].p_off + (int64_t)kPanel * nn + s0; g.bs0 = nn; g.bs1 = 1;
+      g.c_off = blk[b].a_off + (int64_t)s0 * nn + s0; g.cs0 = nn;
+      g.alpha = -1.0; g.accumulate = 1;
+      const int pi = (int)tprobs.size() - pbase;
+      tprobs.push_back(g);
+      const int tt = (g.m + 63) / 64;
+      for (int a = 0; a < tt; a++)
+        for (int bb = 0; bb < tt; bb++) ttiles.push_back(E2Tile{pi, a, bb, 0});
+    }
+    if ((int)tprobs.size() > pbase) {
+      tev_k.push_back(k); tev_pbeg.push_back(pbase); tev_tbeg.push_back(tbase);
+      tev_tcnt.push_back((int)ttiles.size() - tbase);
+    }
+  }
   // ---- stage 2 trees
   std::vector<E2Leaf> leaves;
   std::vector<E2Merge> all_merges;
@@ -202,13 +231,25 @@ int eig2_dev(nrg_ctx* c, int nb, const int* n, const int64_t* a_off, double* dA,
   if ((rc = upload(c, "e2_btp", bt_p0, &d_btp))) return rc;
   if ((rc = upload(c, "e2_bp", bprobs, &d_bp))) return rc;
   if ((rc = upload(c, "e2_bt", btiles, &d_bt))) return rc;
+  E2Gemm* d_tp;
+  E2Tile* d_tt;
+  if ((rc = upload(c, "e2_tp", tprobs, &d_tp))) return rc;
+  if ((rc = upload(c, "e2_tt", ttiles, &d_tt))) return rc;
   // ---- launches
   hipStream_t s = c->stream;
   HIPCHK(e2_launch_tridiag_small(d_blk, d_small, (int)small.size(), dA, WS, s));
   if (!large.empty()) {
-    const int nslab = (nmax_large + 255) / 256;
-    for (int k = 0; k + 2 < nmax_large; k++)
-      HIPCHK(e2_launch_tridiag_step(d_blk, d_large, (int)large.size(), nslab, k, dA, WS, s));
+    const int nl = (int)large.size();
+    HIPCHK(e2_launch_tdb_norm(d_blk, d_large, nl, nmax_large, 0, dA, WS, s));
+    size_t ev = 0;
+    for (int k = 0; k + 2 < nmax_large; k++) {
+      HIPCHK(e2_launch_tdb_column(d_blk, d_large, nl, nmax_large, k, dA, WS, s));
+      if (ev < tev_k.size() && tev_k[ev] == k) {
+        HIPCHK(e2_launch_gemm3(d_tp + tev_pbeg[ev], d_tt + tev_tbeg[ev], tev_tcnt[ev], WS, WS, dA, s));
+        ev++;
+      }
+      if ((k + 1) % kPanel == 0) HIPCHK(e2_launch_tdb_norm(d_blk, d_large, nl, nmax_large, k + 1, dA, WS, s));
+    }
     HIPCHK(e2_launch_tridiag_tail(d_blk, d_large, (int)large.size(), dA, WS, s));
   }
   HIPCHK(e2_launch_dc_init(d_blk, nb, nmax, d_all, (int)all_merges.size(), d_mbeg, WS, s));
