@@ -14,7 +14,8 @@
 //      Gu-Eisenstat recomputation of z for orthogonal vectors, and a GEMM for
 //      the new vectors.
 //   3. Back-transformation with blocked (compact WY) Householder panels:
-//      three GEMMs per 32-reflector panel.
+//      three GEMMs per 64-reflector panel, restricted to the columns the
+//      panel's reflectors touch.
 //
 // Storage: eigenvectors are kept vector-major (row i = eigenvector i), the
 // layout of dEigVec in the reference (src/NRGmatrixClass.cpp:647-654), so the
@@ -27,7 +28,8 @@
 #define E2_T 256
 #define WAVE 64
 #define LEAF 32
-#define PANEL 32
+#define PANEL 32   // reduction panel
+#define BTP 64     // back-transformation panel
 
 namespace e2 {
 
@@ -849,65 +851,69 @@ __global__ void bt_vt_kernel(const E2Block* __restrict__ blks, const int* __rest
   const E2Block B = blks[list[blockIdx.y]];
   const int n = B.n, k0 = p0[blockIdx.y];
   if (k0 < 0) return;
-  const int nb = min(PANEL, n - 2 - k0);
+  const int nb = min(BTP, n - 2 - k0);
   const double* A = A0 + B.a_off;
-  double* Vt = WS + B.p_off;  // PANEL x n
-  for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < (size_t)PANEL * n;
+  double* Vt = WS + B.p_off;  // BTP x n
+  for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < (size_t)BTP * n;
        idx += (size_t)gridDim.x * blockDim.x) {
     const int a = (int)(idx / n), j = (int)(idx % n);
     Vt[idx] = (a < nb && j > k0 + a) ? A[(size_t)(k0 + a) * n + j] : 0.0;
   }
 }
 
+// T of the compact-WY form, one wavefront per block: T_ii = tau_i,
+// T[0:i,i] = -tau_i T[0:i,0:i] G[0:i,i]; the Gram matrix G is read from global
+// memory (a GEMM output), T is built in LDS and stored transposed
 __global__ __launch_bounds__(64) void bt_tmat_kernel(const E2Block* __restrict__ blks, const int* __restrict__ list,
                                                      const int* __restrict__ p0, double* __restrict__ WS) {
-  __shared__ double T[PANEL][PANEL + 1];
-  __shared__ double G[PANEL][PANEL + 1];
+  __shared__ double T[BTP][BTP + 1];
   const E2Block B = blks[list[blockIdx.x]];
   const int n = B.n, k0 = p0[blockIdx.x];
   if (k0 < 0) return;
-  const int nb = min(PANEL, n - 2 - k0);
+  const int nb = min(BTP, n - 2 - k0);
   const double* tau = WS + B.v_off + 2 * (size_t)n;
-  const double* Gg = WS + B.p_off + (size_t)PANEL * n;       // PANEL x PANEL Gram matrix (GEMM output)
-  double* Tg = WS + B.p_off + (size_t)PANEL * n + PANEL * PANEL;  // PANEL x PANEL, T^T stored row-major
+  const double* G = WS + B.p_off + (size_t)BTP * n;                // BTP x BTP Gram matrix
+  double* Tg = WS + B.p_off + (size_t)BTP * n + BTP * BTP;         // BTP x BTP, T^T row-major
   const int tid = threadIdx.x;
-  for (int i = tid; i < PANEL * PANEL; i += blockDim.x) {
-    G[i / PANEL][i % PANEL] = Gg[i];
-    T[i / PANEL][i % PANEL] = 0.0;
-  }
+  for (int i = tid; i < BTP * BTP; i += blockDim.x) T[i / BTP][i % BTP] = 0.0;
   __syncthreads();
   for (int i = 0; i < nb; i++) {
     const double ti = tau[k0 + i];
-    // w_r = sum_{c<i} T[r][c] G[c][i], r < i
     double w = 0.0;
     if (tid < i)
-      for (int c = tid; c < i; c++) w += T[tid][c] * G[c][i];
+      for (int c = tid; c < i; c++) w += T[tid][c] * G[(size_t)c * BTP + i];
     __syncthreads();
     if (tid < i) T[tid][i] = -ti * w;
     if (tid == 0) T[i][i] = ti;
     __syncthreads();
   }
-  for (int i = tid; i < PANEL * PANEL; i += blockDim.x) {
-    const int r = i / PANEL, c = i % PANEL;
+  for (int i = tid; i < BTP * BTP; i += blockDim.x) {
+    const int r = i / BTP, c = i % BTP;
     Tg[i] = T[c][r];   // T^T
   }
 }
 
 // ---------------------------------------------------------------------------
-// batched grouped GEMM, FP64, strided operands:
+// batched grouped GEMM, FP64 on the matrix cores, strided operands:
 //   C(m x n) = alpha * sum_k A(i,k) B(k,j) (+ C if accumulate)
 //   A(i,k) = A[a_off + i*as0 + k*as1],  B(k,j) = B[b_off + k*bs0 + j*bs1]
-// 64x64 tile per 256-thread workgroup, 4x4 per thread; tile loads choose the
-// thread mapping that keeps the contiguous dimension along the wavefront.
-// dyn >= 0: m and k are read from dynk[dyn] (sizes only known on the device).
+// 64x64 tile per 256-thread workgroup; wavefront w owns the 32x32 quadrant
+// (w/2, w%2) as 2x2 blocks of v_mfma_f64_16x16x4.  Operand lanes: a = A(l%16,
+// l/16), b = B(l/16, l%16); accumulator r of lane l is C(l/16 + 4r, l%16).
+// K advances in steps of 16 through LDS (k-major), the next step's global
+// loads issued before the current step's MFMAs.  Tile loads pick the thread
+// mapping that keeps the contiguous dimension along the wavefront.
+// dyn >= 0: m and k are read from dyns[dyn] (sizes only known on the device).
 // ---------------------------------------------------------------------------
+typedef double v4d __attribute__((ext_vector_type(4)));
 #define GT 64
 #define GK 16
+#define GP 4   // LDS row padding (doubles)
 __global__ __launch_bounds__(256) void gemm_kernel(const E2Gemm* __restrict__ probs, const E2Tile* __restrict__ tiles,
                                                    const double* __restrict__ Abase, const double* __restrict__ Bbase,
                                                    double* __restrict__ Cbase, const E2MState* __restrict__ dyns) {
-  __shared__ double As[GK][GT + 1];
-  __shared__ double Bs[GK][GT + 1];
+  __shared__ double As[GK][GT + GP];
+  __shared__ double Bs[GK][GT + GP];
   const E2Tile Tl = tiles[blockIdx.x];
   const E2Gemm P = probs[Tl.prob];
   int M = P.m, K = P.k;
@@ -917,57 +923,70 @@ __global__ __launch_bounds__(256) void gemm_kernel(const E2Gemm* __restrict__ pr
   const double* A = Abase + P.a_off;
   const double* Bm = Bbase + P.b_off;
   double* C = Cbase + P.c_off;
-  const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
-  double acc[4][4];
-#pragma unroll
-  for (int r = 0; r < 4; r++)
-#pragma unroll
-    for (int c = 0; c < 4; c++) acc[r][c] = 0.0;
+  const int tid = threadIdx.x, lane = tid & (WAVE - 1), wv = tid / WAVE;
+  const int wr = (wv >> 1) * 32, wc = (wv & 1) * 32;
   const bool a_kcontig = (P.as1 == 1);
   const bool b_ncontig = (P.bs1 == 1);
-  for (int k0 = 0; k0 < K; k0 += GK) {
+  // this thread's four tile-load slots
+  int arow[4], akk[4], bcol[4], bkk[4];
+#pragma unroll
+  for (int rr = 0; rr < 4; rr++) {
+    if (a_kcontig) { arow[rr] = (tid >> 4) + 16 * rr; akk[rr] = tid & 15; }
+    else { arow[rr] = tid & 63; akk[rr] = (tid >> 6) + 4 * rr; }
+    if (b_ncontig) { bcol[rr] = tid & 63; bkk[rr] = (tid >> 6) + 4 * rr; }
+    else { bcol[rr] = (tid >> 4) + 16 * rr; bkk[rr] = tid & 15; }
+  }
+  double ra[4], rb[4];
+  auto load = [&](int k0) {
 #pragma unroll
     for (int rr = 0; rr < 4; rr++) {
-      int row, kk;
-      if (a_kcontig) { row = ty + 16 * rr; kk = tx; }
-      else { row = tid & 63; kk = (tid >> 6) + 4 * rr; }
-      const int gi = m0 + row, gk = k0 + kk;
-      As[kk][row] = (gi < M && gk < K) ? A[(int64_t)gi * P.as0 + (int64_t)gk * P.as1] : 0.0;
-      int col;
-      if (b_ncontig) { col = tid & 63; kk = (tid >> 6) + 4 * rr; }
-      else { col = ty + 16 * rr; kk = tx; }
-      const int gj = n0 + col, gk2 = k0 + kk;
-      Bs[kk][col] = (gj < P.n && gk2 < K) ? Bm[(int64_t)gk2 * P.bs0 + (int64_t)gj * P.bs1] : 0.0;
+      const int gi = m0 + arow[rr], gk = k0 + akk[rr];
+      ra[rr] = (gi < M && gk < K) ? A[(int64_t)gi * P.as0 + (int64_t)gk * P.as1] : 0.0;
+      const int gj = n0 + bcol[rr], gk2 = k0 + bkk[rr];
+      rb[rr] = (gj < P.n && gk2 < K) ? Bm[(int64_t)gk2 * P.bs0 + (int64_t)gj * P.bs1] : 0.0;
     }
+  };
+  v4d acc[2][2];
+#pragma unroll
+  for (int x = 0; x < 2; x++)
+#pragma unroll
+    for (int y = 0; y < 2; y++) acc[x][y] = (v4d){0.0, 0.0, 0.0, 0.0};
+  load(0);
+  for (int k0 = 0; k0 < K; k0 += GK) {
     __syncthreads();
 #pragma unroll
-    for (int kk = 0; kk < GK; kk++) {
-      double av[4], bv[4];
-#pragma unroll
-      for (int r = 0; r < 4; r++) av[r] = As[kk][ty * 4 + r];
-#pragma unroll
-      for (int c = 0; c < 4; c++) bv[c] = Bs[kk][tx * 4 + c];
-#pragma unroll
-      for (int r = 0; r < 4; r++)
-#pragma unroll
-        for (int c = 0; c < 4; c++) acc[r][c] = __fma_rn(av[r], bv[c], acc[r][c]);
+    for (int rr = 0; rr < 4; rr++) {
+      As[akk[rr]][arow[rr]] = ra[rr];
+      Bs[bkk[rr]][bcol[rr]] = rb[rr];
     }
     __syncthreads();
-  }
+    if (k0 + GK < K) load(k0 + GK);
 #pragma unroll
-  for (int r = 0; r < 4; r++) {
-    const int gi = m0 + ty * 4 + r;
-    if (gi >= M) continue;
-#pragma unroll
-    for (int c = 0; c < 4; c++) {
-      const int gj = n0 + tx * 4 + c;
-      if (gj >= P.n) continue;
-      double v = P.alpha * acc[r][c];
-      double* cp = C + (int64_t)gi * P.cs0 + gj;
-      if (P.accumulate) v += *cp;
-      *cp = v;
+    for (int kk = 0; kk < GK; kk += 4) {
+      const int k = kk + (lane >> 4), c16 = lane & 15;
+      const double a0 = As[k][wr + c16], a1 = As[k][wr + 16 + c16];
+      const double b0 = Bs[k][wc + c16], b1 = Bs[k][wc + 16 + c16];
+      acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+      acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+      acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+      acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
     }
   }
+#pragma unroll
+  for (int x = 0; x < 2; x++)
+#pragma unroll
+    for (int y = 0; y < 2; y++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int gi = m0 + wr + 16 * x + (lane >> 4) + 4 * r;
+        const int gj = n0 + wc + 16 * y + (lane & 15);
+        if (gi < M && gj < P.n) {
+          double v = P.alpha * acc[x][y][r];
+          double* cp = C + (int64_t)gi * P.cs0 + gj;
+          if (P.accumulate) v += *cp;
+          *cp = v;
+        }
+      }
 }
 
 // copy eigenvalues and eigenvector rows to the caller's buffers
@@ -1079,7 +1098,7 @@ hipError_t e2_launch_bt_panel(const E2Block* b, const int* list, const int* p0, 
                               const E2Tile* tiles_rest, int nt_w, int nt_w2, int nt_z, hipStream_t s) {
   if (nlist <= 0) return hipSuccess;
   hipError_t e;
-  int gx = (int)(((size_t)PANEL * nmax + 255) / 256); if (gx > 256) gx = 256; if (gx < 1) gx = 1;
+  int gx = (int)(((size_t)BTP * nmax + 255) / 256); if (gx > 512) gx = 512; if (gx < 1) gx = 1;
   hipLaunchKernelGGL(e2::bt_vt_kernel, dim3(gx, nlist), dim3(256), 0, s, b, list, p0, A, WS);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   if (nt_g > 0) hipLaunchKernelGGL(e2::gemm_kernel, dim3(nt_g), dim3(256), 0, s, gprobs, tiles_g, WS, WS, WS,

@@ -16,6 +16,7 @@ namespace nrg {
 
 static const int kLeaf = 32;   // = LEAF in nrg_eig2.hip
 static const int kPanel = 32;  // = PANEL in nrg_eig2.hip
+static const int kBtp = 64;    // = BTP in nrg_eig2.hip
 static const int kSmall = 256; // single-workgroup reduction up to this size
 
 int eig2_dev(nrg_ctx* c, int nb, const int* n, const int64_t* a_off, double* dA, const int64_t* w_off, double* dW) {
@@ -36,7 +37,9 @@ int eig2_dev(nrg_ctx* c, int nb, const int* n, const int64_t* a_off, double* dA,
     B.u_off = ws; ws += n2;
     B.dl_off = ws; ws += n2;
     B.t_off = ws; ws += n2;
-    B.p_off = ws; ws += kPanel * nn + 2 * kPanel * kPanel + 2 * nn * kPanel;
+    // panel scratch: back-transformation Vt, G, T^T, W, W2 (the reduction's
+    // 3 x 32 panel rows fit inside)
+    B.p_off = ws; ws += kBtp * nn + 2 * kBtp * kBtp + 2 * nn * kBtp;
     B.v_off = ws; ws += 20 * nn + 64;
     B.i_off = iw; iw += 8 * nn + 8;
     nmax = std::max(nmax, n[b]);
@@ -131,11 +134,13 @@ int eig2_dev(nrg_ctx* c, int nb, const int* n, const int64_t* a_off, double* dA,
     lvl_tcnt.push_back((int)gtiles.size() - lvl_tbeg.back());
   }
   const int nlev = (int)by_h.size();
-  // ---- stage 3 panels: round r applies panel P_b-1-r of every block that has it
+  // ---- stage 3 panels: round r applies panel P_b-1-r of every block that has it;
+  // the reflectors of a panel starting at k0 vanish in columns <= k0, so every
+  // product runs over the L = n-k0-1 columns after it
   std::vector<int> npan(nb, 0);
   int maxP = 0;
   for (int b = 0; b < nb; b++) {
-    npan[b] = n[b] > 2 ? (n[b] - 2 + kPanel - 1) / kPanel : 0;
+    npan[b] = n[b] > 2 ? (n[b] - 2 + kBtp - 1) / kBtp : 0;
     maxP = std::max(maxP, npan[b]);
   }
   std::vector<int> bt_list, bt_p0, r_lbeg, r_lcnt, r_pbeg, r_tbeg, r_ng, r_nw, r_nw2, r_nz;
@@ -154,45 +159,48 @@ int eig2_dev(nrg_ctx* c, int nb, const int* n, const int64_t* a_off, double* dA,
     for (int b : bl) {
       const E2Block& B = blk[b];
       const int64_t nn = B.n;
-      const int k0 = (npan[b] - 1 - r) * kPanel;
+      const int k0 = (npan[b] - 1 - r) * kBtp;
+      const int L = B.n - k0 - 1;
       bt_list.push_back(b);
       bt_p0.push_back(k0);
       nmax_bt = std::max(nmax_bt, B.n);
-      const int64_t vt = B.p_off, G = vt + kPanel * nn, Tg = G + kPanel * kPanel, W = Tg + kPanel * kPanel,
-                    W2 = W + nn * kPanel;
+      const int64_t vt = B.p_off, G = vt + kBtp * nn, Tg = G + kBtp * kBtp, W = Tg + kBtp * kBtp,
+                    W2 = W + nn * kBtp;
       E2Gemm g{};
       g.dyn = -1; g.alpha = 1.0;
-      // Gram G = Vt Vt^T
+      // Gram G = Vt Vt^T over columns k0+1..n-1
       E2Gemm gg = g;
-      gg.m = kPanel; gg.n = kPanel; gg.k = B.n;
-      gg.a_off = vt; gg.as0 = nn; gg.as1 = 1; gg.b_off = vt; gg.bs0 = 1; gg.bs1 = nn; gg.c_off = G; gg.cs0 = kPanel;
+      gg.m = kBtp; gg.n = kBtp; gg.k = L;
+      gg.a_off = vt + k0 + 1; gg.as0 = nn; gg.as1 = 1; gg.b_off = vt + k0 + 1; gg.bs0 = 1; gg.bs1 = nn;
+      gg.c_off = G; gg.cs0 = kBtp;
       int pg = (int)bprobs.size() - pbase;
       bprobs.push_back(gg);
       tg.push_back(E2Tile{pg, 0, 0, 0});
-      // W = Qv Vt^T
+      // W = Qv[:, k0+1:] Vt[:, k0+1:]^T
       E2Gemm gw = g;
-      gw.m = B.n; gw.n = kPanel; gw.k = B.n;
-      gw.a_off = B.q_off; gw.as0 = nn; gw.as1 = 1; gw.b_off = vt; gw.bs0 = 1; gw.bs1 = nn; gw.c_off = W; gw.cs0 = kPanel;
+      gw.m = B.n; gw.n = kBtp; gw.k = L;
+      gw.a_off = B.q_off + k0 + 1; gw.as0 = nn; gw.as1 = 1; gw.b_off = vt + k0 + 1; gw.bs0 = 1; gw.bs1 = nn;
+      gw.c_off = W; gw.cs0 = kBtp;
       int pw = (int)bprobs.size() - pbase;
       bprobs.push_back(gw);
       for (int a = 0; a < (B.n + 63) / 64; a++) tw.push_back(E2Tile{pw, a, 0, 0});
       // W2 = W T^T
       E2Gemm g2 = g;
-      g2.m = B.n; g2.n = kPanel; g2.k = kPanel;
-      g2.a_off = W; g2.as0 = kPanel; g2.as1 = 1; g2.b_off = Tg; g2.bs0 = kPanel; g2.bs1 = 1; g2.c_off = W2;
-      g2.cs0 = kPanel;
+      g2.m = B.n; g2.n = kBtp; g2.k = kBtp;
+      g2.a_off = W; g2.as0 = kBtp; g2.as1 = 1; g2.b_off = Tg; g2.bs0 = kBtp; g2.bs1 = 1; g2.c_off = W2;
+      g2.cs0 = kBtp;
       int p2 = (int)bprobs.size() - pbase;
       bprobs.push_back(g2);
       for (int a = 0; a < (B.n + 63) / 64; a++) tw2.push_back(E2Tile{p2, a, 0, 0});
-      // Qv -= W2 Vt
+      // Qv[:, k0+1:] -= W2 Vt[:, k0+1:]
       E2Gemm gz = g;
-      gz.m = B.n; gz.n = B.n; gz.k = kPanel;
-      gz.a_off = W2; gz.as0 = kPanel; gz.as1 = 1; gz.b_off = vt; gz.bs0 = nn; gz.bs1 = 1; gz.c_off = B.q_off;
-      gz.cs0 = nn; gz.alpha = -1.0; gz.accumulate = 1;
+      gz.m = B.n; gz.n = L; gz.k = kBtp;
+      gz.a_off = W2; gz.as0 = kBtp; gz.as1 = 1; gz.b_off = vt + k0 + 1; gz.bs0 = nn; gz.bs1 = 1;
+      gz.c_off = B.q_off + k0 + 1; gz.cs0 = nn; gz.alpha = -1.0; gz.accumulate = 1;
       int pz = (int)bprobs.size() - pbase;
       bprobs.push_back(gz);
       for (int a = 0; a < (B.n + 63) / 64; a++)
-        for (int bb = 0; bb < (B.n + 63) / 64; bb++) tz.push_back(E2Tile{pz, a, bb, 0});
+        for (int bb = 0; bb < (L + 63) / 64; bb++) tz.push_back(E2Tile{pz, a, bb, 0});
     }
     r_lcnt.push_back((int)bl.size());
     r_ng.push_back((int)tg.size()); r_nw.push_back((int)tw.size());
@@ -288,7 +296,7 @@ extern "C" int nrg_eig2_stages(nrg_ctx* c, int n, const double* A, double* d_out
   // the single block's workspace starts at offset 0: q,k,u,dl,t (5 n^2), panel, vectors
   const int64_t nn = n, q_off = 0;
   const int64_t p_off = 5 * nn * nn;
-  const int64_t v_off = p_off + 32 * nn + 2 * 32 * 32 + 2 * nn * 32;
+  const int64_t v_off = p_off + kBtp * nn + 2 * kBtp * kBtp + 2 * nn * kBtp;
   double* WS = (double*)c->buf("e2_ws").p;
   if (d_out) HIPCHK(hipMemcpy(d_out, WS + v_off, sizeof(double) * n, hipMemcpyDeviceToHost));
   if (e_out) HIPCHK(hipMemcpy(e_out, WS + v_off + 16 * nn, sizeof(double) * n, hipMemcpyDeviceToHost));
