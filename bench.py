@@ -41,7 +41,7 @@ def _params(w):
     return model, ramp, w
 
 
-def gpu_run(name, steps, warmup, eig_impl=2):
+def gpu_run(name, steps, warmup, eig_impl=2, plain=False):
     import twochnrg_amd as t
     model, ramp, w = _params(WORKLOADS[name])
     first = -1 if model == "onechqs" else 0   # first record emitted
@@ -50,6 +50,7 @@ def gpu_run(name, steps, warmup, eig_impl=2):
     sizes = {}
     ctx = t.Context(0)
     ctx.set_eig_impl(eig_impl)
+    ctx.set_plain(plain)
     ctx.set_timing(True)
     t_start_N = first + ramp + warmup         # stats reset after this step's record
 
@@ -57,8 +58,9 @@ def gpu_run(name, steps, warmup, eig_impl=2):
         s = st.contents
         stamps[s.nshell] = time.perf_counter()
         sizes[s.nshell] = [s.bbe[2 * i + 1] - s.bbe[2 * i] + 1 for i in range(s.nblocks)]
-        print("bench: N=%d done, %d blocks, max block %d, %d states" %
-              (s.nshell, s.nblocks, max(sizes[s.nshell]), sum(sizes[s.nshell])), file=sys.stderr, flush=True)
+        print("bench%s: N=%d done, %d blocks, max block %d, %d states" %
+              (" (plain)" if plain else "", s.nshell, s.nblocks, max(sizes[s.nshell]), sum(sizes[s.nshell])),
+              file=sys.stderr, flush=True)
         if s.nshell == t_start_N:
             ctx.reset_stats()
 
@@ -121,6 +123,8 @@ def main():
     ap.add_argument("--workload", default="twochqs_c3", choices=sorted(WORKLOADS))
     ap.add_argument("--eig-impl", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--plain-steps", type=int, default=2,
+                    help="steps of the plain one-block-per-launch comparison run (0 = skip)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py: this build measures one GPU only (the NRG step and its batched blocks run on one "
@@ -147,6 +151,12 @@ def main():
                      "eig_ms_per_step": eig["ms"] / nsteps},
         "kernel_ms_per_step": {k: v["ms"] / nsteps for k, v in st.items()},
     }
+    if a.plain_steps > 0:
+        # the same kernels, one symmetry block (eigensolve) or one block pair
+        # (rotation) per launch, host-synchronised, no batching
+        pdt, pst, _, _, _, pn = gpu_run(a.workload, a.plain_steps, 0, a.eig_impl, plain=True)
+        out["plain_gpu"] = {"value": pn / pdt, "unit": "NRG steps/s", "steps": pn,
+                            "ms_per_step": 1e3 * pdt / pn, "batched_over_plain": value / (pn / pdt)}
     if not a.no_cpu:
         cb = cpu_baseline(a.workload, timed)
         if cb:

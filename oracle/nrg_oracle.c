@@ -878,7 +878,11 @@ static void emit(orc_step_cb cb, void *user, const arr_t *Ae, const arr_t *Ac, d
   free(kp);
 }
 
-int orc_run_onechqs(const orc_onechqs_params *p, orc_step_cb cb, void *user) {
+struct orc_dm;
+static void dm_store_step(struct orc_dm *D, const arr_t *Ac, const arr_t *Ab, const mat_t *op);
+static int run_onechqs(const orc_onechqs_params *p, orc_step_cb cb, void *user, struct orc_dm *dm);
+int orc_run_onechqs(const orc_onechqs_params *p, orc_step_cb cb, void *user) { return run_onechqs(p, cb, user, NULL); }
+static int run_onechqs(const orc_onechqs_params *p, orc_step_cb cb, void *user, struct orc_dm *dm) {
   double Lambda = p->Lambda, bbar = p->betabar > 0 ? p->betabar : 0.727;
   double myeps = p->myeps > 0 ? p->myeps : 1e-18;
   double HLF = 0.5 * (1.0 + 1.0 / Lambda);
@@ -945,6 +949,7 @@ int orc_run_onechqs(const orc_onechqs_params *p, orc_step_cb cb, void *user) {
     }
     cut_states(&Ae, &Ac, p->ncut, myeps);
     update_matrices(&S, &Ac, &Ab, mats, nm);
+    if (dm && p->calcdens == 1) dm_store_step(dm, &Ac, &Ab, &mats[1]);   /* SaveArrays src/NRGcodehandlerClass.cpp:887-933 */
     double ops[3] = {0, 0, 0};
     int nops = 0;
     if (p->calcdens != 1) for (int m = 1; m < nm; m++) ops[nops++] = calc_opavg(bbar, &Ac, &mats[m]);
@@ -1313,4 +1318,429 @@ int orc_run_twochqs(const orc_twochqs_params *p, orc_step_cb cb, void *user) {
   mat_free(&f[0]); mat_free(&f[1]); mat_free(&d);
   arr_free(&S); arr_free(&Ae); arr_free(&Ac); arr_free(&Ab); arr_free(&Ah); arr_free(&Ahc);
   return 0;
+}
+
+/* ================================================================== */
+/* DM-NRG: density-matrix sweep and spectral sums (src/DM_NRG,         */
+/* src/SpecFuncClass.cpp, src/SpecFunc_allSTL.cpp, src/BroadDelta.cpp)  */
+/* for the OneChQS Anderson chain run with calcdens 1 (ops f_N, c_d),   */
+/* UpdateBefCut 0, plain DM-NRG (UseCFS 0).                             */
+/* ================================================================== */
+struct orc_dm {
+  orc_dmnrg_params p;
+  int nsh;                 /* shells stored: 0 .. nsh-1 (Nsites0 = 0) */
+  arr_t *acut, *abasis;    /* Acut_N, Abasis_N as SaveArrays writes them */
+  mat_t *op;               /* Mat0_N = c_d (cdot1) in the Acut_N basis */
+  mat_t *rho;              /* rho_N, packed upper diagonal blocks over Acut_N */
+  double *trace;           /* Trace (2S+1) rho_N printed by CalcRhoN / SetRhoNmax */
+  int nshell_max;          /* DM_NRG NshellMax */
+  int spec_nshell_max;     /* CSpecFunction::NshellMax */
+  double temp, betabar_spec;
+  arr_t site;
+  int nomega;
+  double *omega, *rhow;
+};
+
+static void arr_copy(arr_t *d, const arr_t *s) {
+  d->nshell = s->nshell; d->nq = s->nq;
+  dcopy_v(&d->qn, &s->qn); icopy_v(&d->bbe, &s->bbe); dcopy_v(&d->en, &s->en); dcopy_v(&d->ev, &s->ev);
+  icopy_v(&d->kept, &s->kept); icopy_v(&d->itype, &s->itype); icopy_v(&d->stcf, &s->stcf);
+  icopy_v(&d->bbe_bc, &s->bbe_bc); icopy_v(&d->bbe_ev, &s->bbe_ev);
+}
+
+/* CalcDN  src/CalcDN.cpp:4-15 (z_twist = 1) */
+static double calc_dn(double L, int N) {
+  double HLF = (L > 0.0) ? 0.5 * (1.0 + 1.0 / L) : 1.0;
+  return HLF * pow(L, -(N - 1) / 2.0);
+}
+/* CNRGarray::PartitionFunc  src/NRGarrayClass.cpp:997-1030 ((2S+1) per state, S = QN 1) */
+static double partition_func(const arr_t *A, double bb) {
+  double s = 0.0;
+  for (int b = 0; b < nblocks(A); b++) {
+    double Si = getqn(A, b, 1);
+    for (int i = blim(A, b, 0); i <= blim(A, b, 1); i++) s += (2.0 * Si + 1.0) * exp(-bb * A->en.a[i]);
+  }
+  return s;
+}
+/* BroadDelta  src/BroadDelta.cpp:12-29 */
+static double broad_delta(double w, double Ep, double b) {
+  if (Ep == 0.0) Ep = 1.0e-100;
+  if (w * Ep < 0.0) return 0.0;
+  double a0 = exp(-b * b / 4.0) / (b * fabs(Ep) * sqrt(M_PI));
+  double a1 = log(fabs(w / Ep)) / b;
+  return a0 * exp(-a1 * a1);
+}
+/* LorentzDelta  src/BroadDelta.cpp:63-73 */
+static double lorentz_delta(double w, double Ep, double b) {
+  return (1.0 / (2.0 * M_PI)) * (b / ((w - Ep) * (w - Ep) + b * b));
+}
+
+/* DM_NRG_SetRhoNmax  src/DM_NRG/DM_NRG_SetRhoNmax.cpp:9-126: packed upper,
+ * block diagonal, diag exp(-betabar E)/ZN */
+static double set_rho_nmax(double TempBar, double ZN, const arr_t *A, mat_t *R) {
+  mat_clear(R);
+  mat_sync(R, A);
+  R->upper = 1;
+  double bb = (fabs(TempBar) < 1e-20) ? 1.0e10 : 1.0 / TempBar;
+  int i1 = 0;
+  double tr = 0.0;
+  for (int b = 0; b < nblocks(A); b++) {
+    ipush(&R->map, b); ipush(&R->map, b); ipush(&R->mbe, i1);
+    int i0 = blim(A, b, 0), i9 = blim(A, b, 1);
+    for (int i = i0; i <= i9; i++) {
+      double x = exp(-bb * A->en.a[i]) / ZN;
+      dpush(&R->mat, x); i1++;
+      tr += (2.0 * getqn(A, b, 1) + 1.0) * x;
+      for (int j = i + 1; j <= i9; j++) { dpush(&R->mat, 0.0); i1++; }
+    }
+    ipush(&R->mbe, i1 - 1);
+  }
+  return tr;
+}
+
+/* RotateMatrix  src/RotateMatrix.cpp:17-99 with RotateBlock2BLAS
+ * src/NRGmatrixClass.cpp:1471-1512: M_rot = Z1^T (M Z2) per matrix block, Z = kept
+ * eigenvector rows (k x n_BC); the result lives on BlockBegEndBC */
+static void rotate_matrix(const mat_t *M, const arr_t *Ac, mat_t *R) {
+  mat_clear(R);
+  icopy_v(&R->map, &M->map);
+  R->upper = M->upper;
+  R->b.nq = Ac->nq; R->b.nshell = Ac->nshell;
+  dcopy_v(&R->b.qn, &Ac->qn);
+  if (Ac->bbe_bc.n) icopy_v(&R->b.bbe, &Ac->bbe_bc); else icopy_v(&R->b.bbe, &Ac->bbe);
+  int i1 = 0;
+  for (int im = 0; im < M->map.n / 2; im++) {
+    int b1 = M->map.a[2 * im], b2 = M->map.a[2 * im + 1];
+    ipush(&R->mbe, i1);
+    int k1 = bsize(Ac, b1), k2 = bsize(Ac, b2), n1 = bsize_bc(Ac, b1), n2 = bsize_bc(Ac, b2);
+    const double *Z1 = Ac->ev.a + Ac->bbe_ev.a[2 * b1], *Z2 = Ac->ev.a + Ac->bbe_ev.a[2 * b2];
+    double *MZ = (double *)calloc((size_t)k1 * n2 + 1, sizeof(double));
+    double *F = (double *)calloc((size_t)n1 * n2 + 1, sizeof(double));
+    for (int i = 0; i < k1; i++)        /* MZ = M (k1 x k2) * Z2 (k2 x n2) */
+      for (int c = 0; c < n2; c++) {
+        double s = 0.0;
+        for (int j = 0; j < k2; j++) s += get_block_matel(M, b1, b2, i, j) * Z2[(size_t)j * n2 + c];
+        MZ[(size_t)i * n2 + c] = s;
+      }
+    for (int r = 0; r < n1; r++)        /* F = Z1^T (n1 x k1) * MZ (k1 x n2) */
+      for (int c = 0; c < n2; c++) {
+        double s = 0.0;
+        for (int i = 0; i < k1; i++) s += Z1[(size_t)i * n1 + r] * MZ[(size_t)i * n2 + c];
+        F[(size_t)r * n2 + c] = s;
+      }
+    for (int r = 0; r < n1; r++)
+      for (int c = (R->upper && b1 == b2) ? r : 0; c < n2; c++) { dpush(&R->mat, F[(size_t)r * n2 + c]); i1++; }
+    ipush(&R->mbe, i1 - 1);
+    free(MZ); free(F);
+  }
+}
+
+/* DM_NRG_CalcRhoN_withSU2  src/DM_NRG/DM_NRG_CalcRhoN.cpp:252-449 with
+ * DM_NRG_SetChildSt / DM_NRG_SetChildSameType src/DM_NRG/DM_NRG_SetChildSt.cpp:11-113
+ * (no FDM): rho_N(i,j) = sum over children (ci,cj) of the same site type and
+ * the same N+1 block of CG(Sold,Sold; S~,Sz-Sold | Si,Sz)^2 summed over Sz,
+ * times rho_rot(ci,cj) */
+static double calc_rho_n(const arr_t *AcN, const arr_t *AcNp1, const arr_t *AbNp1, const arr_t *S,
+                         const mat_t *RhoNp1, mat_t *RhoN) {
+  mat_t rot = {0};
+  rotate_matrix(RhoNp1, AcNp1, &rot);
+  int ns = AcN->en.n;
+  /* ChildStates: N+1 basis indices per Acut_N state, in basis order */
+  int *ccount = (int *)calloc((size_t)ns + 1, sizeof(int)), *cbeg = (int *)calloc((size_t)ns + 2, sizeof(int));
+  for (int c = 0; c < AbNp1->stcf.n; c++) ccount[AbNp1->stcf.a[c]]++;
+  for (int i = 0; i < ns; i++) cbeg[i + 1] = cbeg[i] + ccount[i];
+  int *child = (int *)malloc(sizeof(int) * (size_t)(cbeg[ns] + 1));
+  int *fill = (int *)calloc((size_t)ns + 1, sizeof(int));
+  for (int c = 0; c < AbNp1->stcf.n; c++) { int s = AbNp1->stcf.a[c]; child[cbeg[s] + fill[s]++] = c; }
+  mat_clear(RhoN);
+  mat_sync(RhoN, AcN);
+  RhoN->upper = 1;
+  int i1 = 0;
+  double tr = 0.0;
+  for (int b = 0; b < nblocks(AcN); b++) {
+    ipush(&RhoN->map, b); ipush(&RhoN->map, b); ipush(&RhoN->mbe, i1);
+    int i0 = blim(AcN, b, 0), i9 = blim(AcN, b, 1);
+    double Sold = getqn(AcN, b, 1);
+    for (int ist = i0; ist <= i9; ist++)
+      for (int jst = ist; jst <= i9; jst++) {
+        double el = 0.0;
+        for (int a = cbeg[ist]; a < cbeg[ist + 1]; a++) {
+          int ci = child[a], pos;
+          int ti = AbNp1->itype.a[ci], bi = blk_from_st(AbNp1, ci, &pos);
+          for (int c = cbeg[jst]; c < cbeg[jst + 1]; c++) {
+            int cj = child[c], pos2;
+            int tj = AbNp1->itype.a[cj], bj = blk_from_st(AbNp1, cj, &pos2);
+            if (ti != tj || bi != bj) continue;
+            double Si = getqn(AbNp1, bi, 1);
+            double St = qn_from_st(S, ti, 1);
+            double cg = 0.0;
+            for (double Sz = -Si; Sz <= Si; Sz += 1.0) {
+              double x = orc_cgordan(Sold, Sold, St, Sz - Sold, Si, Sz);
+              cg += x * x;
+            }
+            el += cg * get_matel(&rot, ci, cj);
+          }
+        }
+        if (ist == jst) tr += (2.0 * Sold + 1.0) * el;
+        dpush(&RhoN->mat, el); i1++;
+      }
+    ipush(&RhoN->mbe, i1 - 1);
+  }
+  free(ccount); free(cbeg); free(child); free(fill);
+  mat_free(&rot);
+  return tr;
+}
+
+/* CG factor of a spin-1/2 operator between blocks of spin S1 and S2
+ * (src/SpecFunc_allSTL.cpp:654-671, src/SpecFuncClass.cpp:621-634) */
+static double op_cgfactor(double S1, double S2) {
+  double cg = 0.0;
+  for (double Sz = -S1; Sz <= S1; Sz += 1.0) { double x = orc_cgordan(S1, Sz, 0.5, 0.5, S2, Sz + 0.5); cg += x * x; }
+  return cg;
+}
+
+/* MijxBDeltaEij  src/SpecFunc_allSTL.cpp:36-114: Mout = Min o BD(omega, Ej-Ei),
+ * Lorentzian (dBroadTemp) instead of the log-Gaussian for |Ej-Ei| < twindow*T/D_N */
+static double bdelta_ij(const struct orc_dm *D, const arr_t *A, double w, double Ei, double Ej) {
+  double TempDN = D->temp / calc_dn(D->p.Lambda, A->nshell);
+  if (fabs(Ej - Ei) < D->p.twindow * TempDN) return lorentz_delta(w, Ej - Ei, D->p.broadtemp);
+  return broad_delta(w, Ej - Ei, D->p.bbroad);
+}
+
+/* CalcSpecCosti_T_N  src/SpecFuncClass.cpp:417-662 (real operators) */
+static double spec_costi(const struct orc_dm *D, int N, double bb, double w) {
+  const arr_t *A = &D->acut[N];
+  const mat_t *Op = &D->op[N];
+  double acc = 0.0;
+  for (int im = 0; im < Op->map.n / 2; im++) {
+    int b1 = Op->map.a[2 * im], b2 = Op->map.a[2 * im + 1];
+    int n1 = bsize(A, b1), n2 = bsize(A, b2), i0 = blim(A, b1, 0), j0 = blim(A, b2, 0);
+    /* Sum12 = opB Rho1 + Rho2 opB (n2 x n1), opB = opA^T; trace of Sum12 opAw */
+    double tr = 0.0;
+    for (int j = 0; j < n2; j++)
+      for (int i = 0; i < n1; i++) {
+        double a = get_block_matel(Op, b1, b2, i, j);
+        double s12 = a * exp(-bb * A->en.a[i0 + i]) + exp(-bb * A->en.a[j0 + j]) * a;
+        tr += s12 * a * bdelta_ij(D, A, w, A->en.a[i0 + i], A->en.a[j0 + j]);
+      }
+    acc += op_cgfactor(getqn(A, b1, 1), getqn(A, b2, 1)) * tr;
+  }
+  return acc / partition_func(A, bb);
+}
+
+/* DMNRG_SpecDens_M_STL  src/SpecFunc_allSTL.cpp:223-692 (real operators,
+ * diagonal GF): per op block pair Termo1 = Tr[A^T rho1 (A o BD)],
+ * Termo2 = Tr[rho2 A^T (A o BD)], weighted by the CG factor; the last shell
+ * NshellMax-1 uses the Costi form with the input betabar */
+double orc_dm_spec_shell(const orc_dm *D, int N, double w) {
+  if (N > D->spec_nshell_max - 1) return 0.0;
+  if (N == D->spec_nshell_max - 1) {
+    double bb = (fabs(D->temp) < 1e-20) ? 1.0e10 : D->betabar_spec;
+    return spec_costi(D, N, bb, w);
+  }
+  const arr_t *A = &D->acut[N];
+  const mat_t *Op = &D->op[N], *R = &D->rho[N];
+  double acc = 0.0;
+  for (int im = 0; im < Op->map.n / 2; im++) {
+    int b1 = Op->map.a[2 * im], b2 = Op->map.a[2 * im + 1];
+    int n1 = bsize(A, b1), n2 = bsize(A, b2), i0 = blim(A, b1, 0), j0 = blim(A, b2, 0);
+    double *Aw = (double *)malloc(sizeof(double) * (size_t)(n1 * n2 + 1));
+    double *P = (double *)malloc(sizeof(double) * (size_t)(n1 * n2 + 1));
+    for (int i = 0; i < n1; i++)
+      for (int j = 0; j < n2; j++)
+        Aw[i * n2 + j] = get_block_matel(Op, b1, b2, i, j) *
+                         bdelta_ij(D, A, w, A->en.a[i0 + i], A->en.a[j0 + j]);
+    /* Termo1: Prod1 = A^T rho1 (n2 x n1); trace(Prod1 Aw) */
+    for (int j = 0; j < n2; j++)
+      for (int i = 0; i < n1; i++) {
+        double s = 0.0;
+        for (int l = 0; l < n1; l++) s += get_block_matel(Op, b1, b2, l, j) * get_block_matel(R, b1, b1, l, i);
+        P[j * n1 + i] = s;
+      }
+    double t1 = 0.0;
+    for (int j = 0; j < n2; j++) for (int i = 0; i < n1; i++) t1 += P[j * n1 + i] * Aw[i * n2 + j];
+    /* Termo2: Prod2 = rho2 A^T (n2 x n1); trace(Prod2 Aw) */
+    for (int j = 0; j < n2; j++)
+      for (int i = 0; i < n1; i++) {
+        double s = 0.0;
+        for (int l = 0; l < n2; l++) s += get_block_matel(R, b2, b2, j, l) * get_block_matel(Op, b1, b2, i, l);
+        P[j * n1 + i] = s;
+      }
+    double t2 = 0.0;
+    for (int j = 0; j < n2; j++) for (int i = 0; i < n1; i++) t2 += P[j * n1 + i] * Aw[i * n2 + j];
+    acc += op_cgfactor(getqn(A, b1, 1), getqn(A, b2, 1)) * (t1 + t2);
+    free(Aw); free(P);
+  }
+  return acc;
+}
+
+static void dm_store_step(struct orc_dm *D, const arr_t *Ac, const arr_t *Ab, const mat_t *op) {
+  int N = D->nsh++;
+  D->acut = (arr_t *)realloc(D->acut, sizeof(arr_t) * D->nsh);
+  D->abasis = (arr_t *)realloc(D->abasis, sizeof(arr_t) * D->nsh);
+  D->op = (mat_t *)realloc(D->op, sizeof(mat_t) * D->nsh);
+  memset(&D->acut[N], 0, sizeof(arr_t)); memset(&D->abasis[N], 0, sizeof(arr_t)); memset(&D->op[N], 0, sizeof(mat_t));
+  arr_copy(&D->acut[N], Ac); arr_copy(&D->abasis[N], Ab); mat_copy(&D->op[N], op);
+}
+
+/* gsl_interp_linear evaluation on ascending knots (GSL interp/linear.c) */
+static double lin_interp(const double *x, const double *y, int n, double t) {
+  int lo = 0, hi = n - 1;
+  while (hi > lo + 1) { int m = (lo + hi) / 2; if (x[m] > t) hi = m; else lo = m; }
+  double dx = x[hi] - x[lo];
+  if (dx > 0.0) return y[lo] + (t - x[lo]) * ((y[hi] - y[lo]) / dx);
+  return y[lo];
+}
+static int cmpd2(const void *a, const void *b) {
+  double x = *(const double *)a, y = *(const double *)b;
+  return (x > y) - (x < y);
+}
+
+/* CalcSpecDM_NRG_FixedOmegas  src/SpecFuncClass.cpp:1932-2233 (NwEachN = 1):
+ * even and odd shells on w = +-Lambda^1.25 D_N, each set interpolated linearly
+ * on the sorted union and averaged */
+static void spec_fixed_omegas(struct orc_dm *D, double fac) {
+  int N0 = 0, NM = D->spec_nshell_max;
+  int Nsh0E = (N0 % 2 == 0) ? N0 : N0 + 1, Nsh0O = (N0 % 2 == 0) ? N0 + 1 : N0;
+  dvec ex = {0}, ey = {0}, ox = {0}, oy = {0}, all = {0};
+  int last = 0;
+  for (int par = 0; par < 2; par++) {
+    dvec *X = par ? &ox : &ex, *Y = par ? &oy : &ey;
+    int n0 = par ? Nsh0O : Nsh0E;
+    last = 0;
+    for (int N = n0; N < NM; N += 2) {
+      double DN = calc_dn(D->p.Lambda, N), WN = fac * DN;
+      double r = orc_dm_spec_shell(D, N, -fac * pow(D->p.Lambda, 0.0)) / DN;
+      dpush(X, -WN); dpush(Y, r); dpush(&all, -WN);
+      last = N;
+    }
+    for (int N = last; N >= n0; N -= 2) {
+      double DN = calc_dn(D->p.Lambda, N), WN = fac * DN;
+      double r = orc_dm_spec_shell(D, N, fac * pow(D->p.Lambda, 0.0)) / DN;
+      dpush(X, WN); dpush(Y, r); dpush(&all, WN);
+    }
+  }
+  qsort(all.a, (size_t)all.n, sizeof(double), cmpd2);
+  D->nomega = all.n;
+  D->omega = (double *)malloc(sizeof(double) * (size_t)(all.n + 1));
+  D->rhow = (double *)malloc(sizeof(double) * (size_t)(all.n + 1));
+  for (int i = 0; i < all.n; i++) {
+    double w = all.a[i];
+    int inE = !(w < ex.a[0] || w > ex.a[ex.n - 1]), inO = !(w < ox.a[0] || w > ox.a[ox.n - 1]);
+    double rE = inE ? lin_interp(ex.a, ey.a, ex.n, w) : 0.0, rO = inO ? lin_interp(ox.a, oy.a, ox.n, w) : 0.0;
+    if (!inE) { if (!inO) { rE = -100.0; rO = -100.0; } else rE = rO; }
+    else if (!inO) rO = rE;
+    D->omega[i] = w;
+    D->rhow[i] = 0.5 * (rE + rO);
+  }
+  free(ex.a); free(ey.a); free(ox.a); free(oy.a); free(all.a);
+}
+
+/* CalcSpec_ManyOmegas  src/SpecFuncClass.cpp:2239-2341: w = +-Lambda^(-i/(2Nw)) Lambda^1.25 D_N,
+ * the shell sum evaluated at +-Lambda^1.25 Lambda^(+i/(2Nw)) as the reference does */
+static void spec_many_omegas(struct orc_dm *D, int Nw, double fac) {
+  int N0 = 0, Nmax = D->spec_nshell_max - 1;
+  int cap = 2 * (Nmax - N0 + 1) * Nw + 1;
+  D->omega = (double *)malloc(sizeof(double) * (size_t)cap);
+  D->rhow = (double *)malloc(sizeof(double) * (size_t)cap);
+  int k = 0;
+  for (int N = N0; N <= Nmax; N++) {
+    double DN = calc_dn(D->p.Lambda, N), WN = fac * DN;
+    for (int i = 0; i < Nw; i++) {
+      double rf = (double)i / (2.0 * Nw), w = pow(D->p.Lambda, -rf) * WN;
+      D->omega[k] = -w;
+      D->rhow[k++] = orc_dm_spec_shell(D, N, -fac * pow(D->p.Lambda, rf)) / DN;
+    }
+  }
+  for (int N = Nmax; N >= N0; N--) {
+    double DN = calc_dn(D->p.Lambda, N), WN = fac * DN;
+    for (int i = Nw - 1; i >= 0; i--) {
+      double rf = (double)i / (2.0 * Nw), w = pow(D->p.Lambda, -rf) * WN;
+      D->omega[k] = w;
+      D->rhow[k++] = orc_dm_spec_shell(D, N, fac * pow(D->p.Lambda, rf)) / DN;
+    }
+  }
+  D->nomega = k;
+}
+
+/* DM_NRG main  src/DM_NRG/DM_NRG.cpp:28-531 (QS symmetry, UseFDM false, UseCFS 0,
+ * NFermiOps 1): temperature from Mtemp (:102-118), rho at NshellMax with
+ * ZN = PartitionFunc at Temp/D_{NshellMax-1} (:203-214, the reference's own
+ * "Why????" shell offset kept), backward sweep (:302-434), spectral function
+ * rho_0_0 (:451-506) on the grid of DM_NRG_CalcSpecFunc_ij
+ * src/DM_NRG/DM_NRG_CalcSpecFuncs.cpp:73-84 */
+orc_dm *orc_dm_run(const orc_dmnrg_params *p) {
+  struct orc_dm *D = (struct orc_dm *)calloc(1, sizeof(struct orc_dm));
+  D->p = *p;
+  if (D->p.twindow <= 0) D->p.twindow = 2.0;
+  if (D->p.broadtemp <= 0) D->p.broadtemp = 0.8;
+  if (D->p.bbroad <= 0) D->p.bbroad = 0.5 * log(p->Lambda);
+  orc_onechqs_params q;
+  memset(&q, 0, sizeof(q));
+  q.U = p->U; q.Gamma = p->Gamma; q.ed = p->ed; q.Lambda = p->Lambda; q.nsitesmax = p->nsitesmax;
+  q.ncut = p->ncut; q.calcdens = 1; q.disc_scheme = p->disc_scheme; q.myeps = p->myeps;
+  q.betabar = p->betabar_nrg;
+  run_onechqs(&q, NULL, NULL, D);
+  onechqs_site(&D->site);
+  int Nsitesmax = p->nsitesmax;
+  double DM, Temp;
+  if (p->Mtemp < Nsitesmax - 1) {
+    Nsitesmax = p->Mtemp + 1;
+    DM = calc_dn(p->Lambda, p->Mtemp);
+    Temp = DM / p->betabar;
+  } else {
+    Temp = calc_dn(p->Lambda, Nsitesmax + 199) / p->betabar;
+    DM = calc_dn(p->Lambda, Nsitesmax - 1);
+  }
+  D->temp = Temp;
+  D->betabar_spec = p->betabar;
+  D->nshell_max = Nsitesmax - 1;
+  D->spec_nshell_max = Nsitesmax;
+  int NM = D->nshell_max;
+  D->rho = (mat_t *)calloc((size_t)NM + 1, sizeof(mat_t));
+  D->trace = (double *)calloc((size_t)NM + 1, sizeof(double));
+  double TempBar = Temp / calc_dn(p->Lambda, NM - 1);
+  double ZN = partition_func(&D->acut[NM], 1.0 / TempBar);
+  D->trace[NM] = set_rho_nmax(Temp / DM, ZN, &D->acut[NM], &D->rho[NM]);
+  for (int N = NM - 1; N >= 0; N--)
+    D->trace[N] = calc_rho_n(&D->acut[N], &D->acut[N + 1], &D->abasis[N + 1], &D->site, &D->rho[N + 1], &D->rho[N]);
+  double fac = pow(p->Lambda, 1.25);
+  if (p->Nw > 0) spec_many_omegas(D, p->Nw, fac);
+  else spec_fixed_omegas(D, fac);
+  return D;
+}
+
+void orc_dm_free(orc_dm *D) {
+  if (!D) return;
+  for (int i = 0; i < D->nsh; i++) { arr_free(&D->acut[i]); arr_free(&D->abasis[i]); mat_free(&D->op[i]); }
+  for (int i = 0; D->rho && i <= D->nshell_max; i++) mat_free(&D->rho[i]);
+  free(D->acut); free(D->abasis); free(D->op); free(D->rho); free(D->trace);
+  arr_free(&D->site);
+  free(D->omega); free(D->rhow);
+  free(D);
+}
+
+int orc_dm_info(const orc_dm *D, int *nshell_max, int *spec_nshell_max, double *temp) {
+  *nshell_max = D->nshell_max; *spec_nshell_max = D->spec_nshell_max; *temp = D->temp;
+  return D->nsh;
+}
+
+int orc_dm_shell(const orc_dm *D, int N, orc_dm_shell_view *v) {
+  if (N < 0 || N > D->nshell_max) return -1;
+  const arr_t *A = &D->acut[N];
+  const mat_t *R = &D->rho[N];
+  v->nblocks = nblocks(A); v->nq = A->nq; v->qn = A->qn.a; v->bbe = A->bbe.a; v->en = A->en.a;
+  v->nstates = A->en.n; v->rho = R->mat.a; v->rho_mbe = R->mbe.a; v->trace = D->trace[N];
+  v->bbe_bc = A->bbe_bc.a; v->ev = A->ev.a; v->bbe_ev = A->bbe_ev.a;
+  const mat_t *O = &D->op[N];
+  v->op_nblocks = O->map.n / 2; v->op_map = O->map.a; v->op_mbe = O->mbe.a; v->op = O->mat.a;
+  const arr_t *B = &D->abasis[N];
+  v->basis_nstates = B->stcf.n; v->basis_stcf = B->stcf.a; v->basis_itype = B->itype.a;
+  v->basis_nblocks = nblocks(B); v->basis_qn = B->qn.a; v->basis_bbe = B->bbe.a;
+  return 0;
+}
+
+int orc_dm_spectrum(const orc_dm *D, const double **omega, const double **rho) {
+  *omega = D->omega; *rho = D->rhow;
+  return D->nomega;
 }

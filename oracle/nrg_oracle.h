@@ -73,6 +73,40 @@ typedef struct {
 
 int orc_run_twochqs(const orc_twochqs_params *p, orc_step_cb cb, void *user);
 
+/* DM-NRG on the OneChQS chain (NRG_main with calcdens 1, UpdateBefCut 0)
+ * followed by the DM_NRG post-processing (src/DM_NRG/DM_NRG.cpp:28-531):
+ * plain DM-NRG (UseCFS 0), spectral function of c_d. */
+typedef struct {
+  double U, Gamma, ed, Lambda;
+  int nsitesmax, ncut, disc_scheme;
+  double myeps, betabar_nrg;  /* NRG run: MyEPS, betabar (0 -> defaults) */
+  int Mtemp;                  /* DM_NRG -M */
+  double betabar;             /* DM_NRG -b */
+  double twindow, broadtemp, bbroad;   /* -w, -B, -e (0 -> 2, 0.8, ln(Lambda)/2) */
+  int Nw;                     /* -n: 0 -> CalcSpecDM_NRG_FixedOmegas, >0 -> CalcSpec_ManyOmegas */
+} orc_dmnrg_params;
+
+typedef struct orc_dm orc_dm;
+
+typedef struct {
+  int nblocks, nq, nstates;
+  const double *qn; const int *bbe; const double *en;    /* Acut_N */
+  const int *bbe_bc; const double *ev; const int *bbe_ev;
+  const double *rho; const int *rho_mbe;                 /* rho_N packed upper, block diagonal */
+  double trace;                                          /* sum (2S+1) rho_ii */
+  int op_nblocks; const int *op_map; const int *op_mbe; const double *op;   /* c_d blocks */
+  int basis_nstates, basis_nblocks; const int *basis_stcf, *basis_itype;     /* Abasis_N */
+  const double *basis_qn; const int *basis_bbe;
+} orc_dm_shell_view;
+
+orc_dm *orc_dm_run(const orc_dmnrg_params *p);
+void orc_dm_free(orc_dm *D);
+int orc_dm_info(const orc_dm *D, int *nshell_max, int *spec_nshell_max, double *temp);
+int orc_dm_shell(const orc_dm *D, int N, orc_dm_shell_view *v);
+/* DMNRG_SpecDens_M_STL(omegabar, N)  src/SpecFunc_allSTL.cpp:223-692 */
+double orc_dm_spec_shell(const orc_dm *D, int N, double omegabar);
+int orc_dm_spectrum(const orc_dm *D, const double **omega, const double **rho);
+
 /* ---- kernel-level restatements ---- */
 
 /* CGordan  src/CGordan.cpp:34-120 */

@@ -191,3 +191,114 @@ def rotate(zi, f, zj):
 
 def chin(n, Lambda):
     return load().orc_chin(n, Lambda)
+
+
+# ---------------------------------------------------------------- DM-NRG
+class DmParams(ctypes.Structure):
+    _fields_ = [
+        ("U", ctypes.c_double), ("Gamma", ctypes.c_double), ("ed", ctypes.c_double),
+        ("Lambda", ctypes.c_double), ("nsitesmax", ctypes.c_int), ("ncut", ctypes.c_int),
+        ("disc_scheme", ctypes.c_int), ("myeps", ctypes.c_double), ("betabar_nrg", ctypes.c_double),
+        ("Mtemp", ctypes.c_int), ("betabar", ctypes.c_double), ("twindow", ctypes.c_double),
+        ("broadtemp", ctypes.c_double), ("bbroad", ctypes.c_double), ("Nw", ctypes.c_int),
+    ]
+
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+
+class DmShellView(ctypes.Structure):
+    _fields_ = [
+        ("nblocks", ctypes.c_int), ("nq", ctypes.c_int), ("nstates", ctypes.c_int),
+        ("qn", _dp), ("bbe", _ip), ("en", _dp), ("bbe_bc", _ip), ("ev", _dp), ("bbe_ev", _ip),
+        ("rho", _dp), ("rho_mbe", _ip), ("trace", ctypes.c_double),
+        ("op_nblocks", ctypes.c_int), ("op_map", _ip), ("op_mbe", _ip), ("op", _dp),
+        ("basis_nstates", ctypes.c_int), ("basis_nblocks", ctypes.c_int),
+        ("basis_stcf", _ip), ("basis_itype", _ip), ("basis_qn", _dp), ("basis_bbe", _ip),
+    ]
+
+
+def _arr(p, n, dt=np.float64):
+    if n <= 0 or not p:
+        return np.zeros(0, dt)
+    return np.ctypeslib.as_array(p, shape=(n,)).astype(dt).copy()
+
+
+class DmRun:
+    """DM-NRG restatement on a OneChQS chain: see orc_dm_run."""
+
+    def __init__(self, U, Gamma, ed, Lambda, nsitesmax, ncut, Mtemp, betabar, disc_scheme=0,
+                 twindow=0.0, broadtemp=0.0, bbroad=0.0, Nw=0, myeps=0.0, betabar_nrg=0.0):
+        L = load()
+        L.orc_dm_run.restype = ctypes.c_void_p
+        L.orc_dm_run.argtypes = [ctypes.POINTER(DmParams)]
+        L.orc_dm_free.argtypes = [ctypes.c_void_p]
+        L.orc_dm_info.argtypes = [ctypes.c_void_p, _ip, _ip, _dp]
+        L.orc_dm_shell.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(DmShellView)]
+        L.orc_dm_spec_shell.restype = ctypes.c_double
+        L.orc_dm_spec_shell.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double]
+        L.orc_dm_spectrum.argtypes = [ctypes.c_void_p, ctypes.POINTER(_dp), ctypes.POINTER(_dp)]
+        p = DmParams(U, Gamma, ed, Lambda, nsitesmax, ncut, disc_scheme, myeps, betabar_nrg, Mtemp, betabar,
+                     twindow, broadtemp, bbroad, Nw)
+        self._L = L
+        self.h = L.orc_dm_run(ctypes.byref(p))
+        a, b, t = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        self.nsh = L.orc_dm_info(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(t))
+        self.nshell_max, self.spec_nshell_max, self.temp = a.value, b.value, t.value
+
+    def close(self):
+        if self.h:
+            self._L.orc_dm_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def shell(self, N):
+        v = DmShellView()
+        if self._L.orc_dm_shell(self.h, N, ctypes.byref(v)) != 0:
+            raise IndexError(N)
+        nb, nq = v.nblocks, v.nq
+        bbe = _arr(v.bbe, 2 * nb, np.int64).reshape(nb, 2)
+        bbe_bc = _arr(v.bbe_bc, 2 * nb, np.int64).reshape(nb, 2)
+        rmbe = _arr(v.rho_mbe, 2 * nb, np.int64).reshape(nb, 2)
+        rho_all = _arr(v.rho, int(rmbe[-1, 1]) + 1 if nb else 0)
+        rho = []
+        for b in range(nb):
+            n = int(bbe[b, 1] - bbe[b, 0] + 1)
+            m = np.zeros((n, n))
+            m[np.triu_indices(n)] = rho_all[rmbe[b, 0]:rmbe[b, 1] + 1]
+            rho.append(m + np.triu(m, 1).T)
+        ev_bbe = _arr(v.bbe_ev, 2 * nb, np.int64).reshape(nb, 2)
+        ev_all = _arr(v.ev, int(ev_bbe[-1, 1]) + 1 if nb else 0)
+        Z = []
+        for b in range(nb):
+            k = int(bbe[b, 1] - bbe[b, 0] + 1)
+            n = int(bbe_bc[b, 1] - bbe_bc[b, 0] + 1)
+            Z.append(ev_all[ev_bbe[b, 0]:ev_bbe[b, 0] + k * n].reshape(k, n))
+        nop = v.op_nblocks
+        omap = _arr(v.op_map, 2 * nop, np.int64).reshape(nop, 2)
+        ombe = _arr(v.op_mbe, 2 * nop, np.int64).reshape(nop, 2)
+        op_all = _arr(v.op, int(ombe[-1, 1]) + 1 if nop else 0)
+        ops = {}
+        for i in range(nop):
+            b1, b2 = int(omap[i, 0]), int(omap[i, 1])
+            n1 = int(bbe[b1, 1] - bbe[b1, 0] + 1)
+            n2 = int(bbe[b2, 1] - bbe[b2, 0] + 1)
+            ops[(b1, b2)] = op_all[ombe[i, 0]:ombe[i, 1] + 1].reshape(n1, n2)
+        bnb = v.basis_nblocks
+        return dict(N=N, qn=_arr(v.qn, nb * nq).reshape(nb, nq), bbe=bbe, bbe_bc=bbe_bc,
+                    en=_arr(v.en, v.nstates), rho=rho, trace=v.trace, Z=Z, op=ops,
+                    basis_stcf=_arr(v.basis_stcf, v.basis_nstates, np.int64),
+                    basis_itype=_arr(v.basis_itype, v.basis_nstates, np.int64),
+                    basis_qn=_arr(v.basis_qn, bnb * nq).reshape(bnb, nq),
+                    basis_bbe=_arr(v.basis_bbe, 2 * bnb, np.int64).reshape(bnb, 2))
+
+    def spec_shell(self, N, w):
+        return self._L.orc_dm_spec_shell(self.h, N, w)
+
+    def spectrum(self):
+        po, pr = _dp(), _dp()
+        n = self._L.orc_dm_spectrum(self.h, ctypes.byref(po), ctypes.byref(pr))
+        return _arr(po, n), _arr(pr, n)

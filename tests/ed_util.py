@@ -131,3 +131,72 @@ def expand_multiplets(en, qn, bbe, s_col=1):
         for i in range(bbe[b, 0], bbe[b, 1] + 1):
             out += [en[i]] * int(round(2 * S + 1))
     return np.sort(np.array(out))
+
+
+# ---------------------------------------------------------------- dense Fock-space operators
+def fock_annihilate(nmode, mode):
+    """Dense c_mode on the full 2^nmode Fock space (Jordan-Wigner sign: occupied
+    modes below `mode`)."""
+    dim = 1 << nmode
+    C = np.zeros((dim, dim))
+    for s in range(dim):
+        if (s >> mode) & 1:
+            sign = (-1) ** bin(s & ((1 << mode) - 1)).count("1")
+            C[s ^ (1 << mode), s] = sign
+    return C
+
+
+def fock_hamiltonian(E):
+    """Dense H of a FockED on the full Fock space."""
+    nm = E.nmode
+    dim = 1 << nm
+    C = [fock_annihilate(nm, m) for m in range(nm)]
+    H = np.eye(dim) * E.const
+    for o, e in E.onsite:
+        for sp in (0, 1):
+            m = 2 * o + sp
+            H += e * C[m].T @ C[m]
+    for o, u in E.hubbard:
+        H += u * (C[2 * o].T @ C[2 * o]) @ (C[2 * o + 1].T @ C[2 * o + 1])
+    for o1, o2, t in E.hops:
+        for sp in (0, 1):
+            a, b = C[2 * o1 + sp], C[2 * o2 + sp]
+            H += t * (a.T @ b + b.T @ a)
+    return H, C
+
+
+def onechqs_chain_fock(U, Gamma, ed, Lam, N, a_lambda=True):
+    """Dense rescaled H_N (minus its minimum) of the OneChQS recursion on the full
+    Fock space of d + sites 0..N, with the annihilators of every mode
+    (mode 2*orbital + spin, orbital 0 = d)."""
+    HLF = 0.5 * (1 + 1 / Lam)
+    chi = np.sqrt(2 * Gamma / np.pi) / (np.sqrt(Lam) * HLF)
+    if a_lambda:
+        chi *= np.sqrt(0.5 * np.log(Lam) * (Lam + 1) / (Lam - 1))
+
+    def chin(n):
+        return (1 - Lam ** -(n + 1.0)) / np.sqrt((1 - Lam ** -(2.0 * n + 1)) * (1 - Lam ** -(2.0 * n + 3)))
+
+    E = FockED(N + 2)
+    s = Lam ** ((N + 1) / 2.0) / (Lam * HLF)
+    E.eps(0, ed * s)
+    E.U(0, U * s)
+    E.const = 0.5 * U * s
+    for n in range(0, N + 1):
+        t = chi if n == 0 else chin(n - 1)
+        E.hop(n, n + 1, t * Lam ** ((N - n) / 2.0))
+    H, C = fock_hamiltonian(E)
+    w = np.linalg.eigvalsh(H)
+    return H - w.min() * np.eye(len(H)), C
+
+
+def partial_trace_last_orbital(rho, nmode):
+    """Trace out the two highest modes (the last orbital): with Jordan-Wigner
+    strings counting lower modes only, operators of the remaining modes act as
+    O (x) 1 on (low bits, high bits)."""
+    dl = 1 << (nmode - 2)
+    out = np.zeros((dl, dl))
+    for h in range(4):
+        idx = np.arange(dl) + (h << (nmode - 2))
+        out += rho[np.ix_(idx, idx)]
+    return out
