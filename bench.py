@@ -76,6 +76,41 @@ def gpu_run(name, steps, warmup, eig_impl=2, plain=False):
     return dt, st, timed, model, w, last - t_start_N
 
 
+# DM-NRG A(omega) of the single-channel Anderson chain at T = 1e-8
+# (40 sites, 400 kept multiplets; T = D_M / betabar with M = 38, the fixed-omega grid)
+DM_C1 = dict(U=0.5, Gamma=0.02, ed=-0.25, Lambda=2.5, nsitesmax=40, ncut=400, Mtemp=38)
+
+
+def dm_betabar(T=1e-8, Lambda=2.5, M=38):
+    hlf = 0.5 * (1.0 + 1.0 / Lambda)
+    return hlf * Lambda ** (-(M - 1) / 2.0) / T
+
+
+def dm_run(cpu=True):
+    import twochnrg_amd as t
+    kw = dict(DM_C1, betabar=dm_betabar())
+    out = {"workload": "onechqs_dmnrg_c1", "T": 1e-8, "betabar": kw["betabar"], "params": DM_C1}
+    with t.Context(0) as ctx:
+        t0 = time.perf_counter()
+        with ctx.dm(**kw) as d:
+            w, r = d.spectrum()
+            t1 = time.perf_counter()
+            tm = d.timing()
+    out.update({"wall_ms": 1e3 * (t1 - t0), "breakdown_ms": tm, "nomega": len(w),
+                "post_ms": tm["sweep"] + tm["poles"] + tm["broaden"]})
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle as po
+        t0 = time.perf_counter()
+        o = po.DmRun(**kw)
+        wo, ro = o.spectrum()
+        t1 = time.perf_counter()
+        import numpy as np
+        out["cpu_oracle_ms"] = 1e3 * (t1 - t0)
+        out["max_rel_err_vs_oracle"] = float(np.max(np.abs(r - ro) / np.maximum(np.abs(ro), 1e-12 * np.abs(ro).max())))
+    return out
+
+
 def cpu_baseline(name, timed_sizes, budget_s=20.0):
     """The CPU oracle (gcc -O2 -funroll-loops, one thread, FP64) on a bounded
     sample of the same workload: the oracle's dspev-equivalent eigensolve on
@@ -91,9 +126,14 @@ def cpu_baseline(name, timed_sizes, budget_s=20.0):
     rng = np.random.default_rng(0)
     done, coef = [], []
     t_all = time.perf_counter()
+    a3_guess = 1.2e-9   # s per n^3 of the oracle eigensolve, refined as blocks are timed
     for n in sizes:
         if n < 64:
             continue
+        a3 = float(np.median(coef)) if coef else a3_guess
+        left = budget_s - (time.perf_counter() - t_all)
+        if a3 * float(n) ** 3 > min(left, budget_s / 3.0):
+            continue   # too large for the sample budget: extrapolated from the timed blocks
         a = rng.standard_normal((n, n))
         a = (a + a.T) / 2
         t0 = time.perf_counter()
@@ -101,6 +141,7 @@ def cpu_baseline(name, timed_sizes, budget_s=20.0):
         dt = time.perf_counter() - t0
         done.append(n)
         coef.append(dt / float(n) ** 3)
+        print("bench: cpu sample n=%d %.2f s" % (n, dt), file=sys.stderr, flush=True)
         if time.perf_counter() - t_all > budget_s:
             break
     if not coef:
@@ -123,6 +164,7 @@ def main():
     ap.add_argument("--workload", default="twochqs_c3", choices=sorted(WORKLOADS))
     ap.add_argument("--eig-impl", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-dm", action="store_true", help="skip the DM-NRG A(omega) leg")
     ap.add_argument("--plain-steps", type=int, default=2,
                     help="steps of the plain one-block-per-launch comparison run (0 = skip)")
     a = ap.parse_args()
@@ -161,6 +203,8 @@ def main():
         cb = cpu_baseline(a.workload, timed)
         if cb:
             out["cpu_baseline"] = cb
+    if not a.no_dm:
+        out["dmnrg"] = dm_run(cpu=not a.no_cpu)
     print(json.dumps(out))
 
 

@@ -182,6 +182,47 @@ typedef struct {
  */
 int nrg_run_twochqs(nrg_ctx *ctx, const nrg_twochqs_params *p, nrg_step_cb cb, void *user, double *timing_ms);
 
+/* ---------------- DM-NRG (density-matrix NRG spectral function) ---------------- */
+
+typedef struct {
+  double U, Gamma, ed, Lambda;     /* OneChQS Anderson chain, run with calcdens 1 (f_N, c_d) */
+  int nsitesmax, ncut, disc_scheme;
+  double myeps, betabar_nrg;       /* NRG run: MyEPS, betabar (0 -> 1e-18, 0.727) */
+  int Mtemp;                       /* DM_NRG -M: T = D_Mtemp / betabar */
+  double betabar;                  /* DM_NRG -b */
+  double twindow, broadtemp, bbroad;   /* -w -B -e (0 -> 2, 0.8, ln(Lambda)/2) */
+  int Nw;                          /* -n: 0 -> fixed omegas (even/odd shells), >0 -> Nw omegas per shell */
+} nrg_dmnrg_params;
+
+typedef struct nrg_dm nrg_dm;
+
+/*
+ * DM_NRG main (src/DM_NRG/DM_NRG.cpp:28-531; QS symmetry, plain DM-NRG, UseCFS 0)
+ * on the device: the OneChQS chain (as nrg_run_onechqs with calcdens 1) keeping
+ * what SaveArrays writes per shell (src/NRGcodehandlerClass.cpp:887-933), then
+ * rho_Nmax (DM_NRG_SetRhoNmax src/DM_NRG/DM_NRG_SetRhoNmax.cpp:9-126) and the
+ * backward sweep (RotateMatrix src/RotateMatrix.cpp:17-99 +
+ * DM_NRG_CalcRhoN_withSU2 src/DM_NRG/DM_NRG_CalcRhoN.cpp:252-449).  The handle
+ * keeps device memory until nrg_dm_destroy; ctx must outlive it.
+ */
+int nrg_dm_create(nrg_ctx *ctx, const nrg_dmnrg_params *p, nrg_dm **out);
+void nrg_dm_destroy(nrg_dm *dm);
+/* NshellMax of the sweep, CSpecFunction::NshellMax and the temperature Temp. */
+int nrg_dm_info(const nrg_dm *dm, int *nshell_max, int *spec_nshell_max, double *temp);
+/* Wall times (ms): forward chain, backward sweep, pole weights, broadening. */
+int nrg_dm_timing(const nrg_dm *dm, double *ms4);
+/* rho_N blocks (k_b x k_b row-major, Acut_N block order) and its trace
+ * sum (2S+1) rho_ii (the value CalcRhoN prints); out may be NULL. */
+int nrg_dm_rho(nrg_dm *dm, int N, double *out, int64_t cap, int *nblocks, double *trace);
+/* CSpecFunction::DMNRG_SpecDens_M_STL(omegabar, N) (src/SpecFunc_allSTL.cpp:223-692;
+ * last shell CalcSpecCosti_T_N src/SpecFuncClass.cpp:417-662) for n (shell, omegabar) pairs. */
+int nrg_dm_spec(nrg_dm *dm, int n, const int *shells, const double *omegabar, double *out);
+/* rho_0_0 on the grid of DM_NRG_CalcSpecFunc_ij (src/DM_NRG/DM_NRG_CalcSpecFuncs.cpp:73-84):
+ * CalcSpecDM_NRG_FixedOmegas (src/SpecFuncClass.cpp:1932-2233) or CalcSpec_ManyOmegas
+ * (:2239-2341) -- the two columns SaveOmegaRhow writes.  *n receives the count;
+ * omega/rho may be NULL to query it. */
+int nrg_dm_spectrum(nrg_dm *dm, double *omega, double *rho, int cap, int *n);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1416,12 +1416,15 @@ static void rotate_matrix(const mat_t *M, const arr_t *Ac, mat_t *R) {
     const double *Z1 = Ac->ev.a + Ac->bbe_ev.a[2 * b1], *Z2 = Ac->ev.a + Ac->bbe_ev.a[2 * b2];
     double *MZ = (double *)calloc((size_t)k1 * n2 + 1, sizeof(double));
     double *F = (double *)calloc((size_t)n1 * n2 + 1, sizeof(double));
+    double *Mb = (double *)calloc((size_t)k1 * k2 + 1, sizeof(double));   /* MatBlock2BLAS */
+    for (int i = 0; i < k1; i++) for (int j = 0; j < k2; j++) Mb[(size_t)i * k2 + j] = get_block_matel(M, b1, b2, i, j);
     for (int i = 0; i < k1; i++)        /* MZ = M (k1 x k2) * Z2 (k2 x n2) */
       for (int c = 0; c < n2; c++) {
         double s = 0.0;
-        for (int j = 0; j < k2; j++) s += get_block_matel(M, b1, b2, i, j) * Z2[(size_t)j * n2 + c];
+        for (int j = 0; j < k2; j++) s += Mb[(size_t)i * k2 + j] * Z2[(size_t)j * n2 + c];
         MZ[(size_t)i * n2 + c] = s;
       }
+    free(Mb);
     for (int r = 0; r < n1; r++)        /* F = Z1^T (n1 x k1) * MZ (k1 x n2) */
       for (int c = 0; c < n2; c++) {
         double s = 0.0;
@@ -1440,10 +1443,28 @@ static void rotate_matrix(const mat_t *M, const arr_t *Ac, mat_t *R) {
  * (no FDM): rho_N(i,j) = sum over children (ci,cj) of the same site type and
  * the same N+1 block of CG(Sold,Sold; S~,Sz-Sold | Si,Sz)^2 summed over Sz,
  * times rho_rot(ci,cj) */
+/* GetMatEl on the rotated rho (block diagonal, packed upper): state -> block by table */
+static double rot_el(const mat_t *R, const int *rblk, const int *rbeg, int ist, int jst) {
+  int b1 = rblk[ist], b2 = rblk[jst];
+  if (b1 < 0 || b2 < 0 || b1 != b2) return get_matel(R, ist, jst);
+  int im = find_mat_block(R, b1, b1);
+  if (im < 0) return 0.0;
+  int n = bsize(&R->b, b1), i = ist - rbeg[b1], j = jst - rbeg[b1];
+  int r = (i > j) ? ij2rUpTr(n, j, i) : ij2rUpTr(n, i, j);
+  return R->mat.a[R->mbe.a[2 * im] + r];
+}
 static double calc_rho_n(const arr_t *AcN, const arr_t *AcNp1, const arr_t *AbNp1, const arr_t *S,
                          const mat_t *RhoNp1, mat_t *RhoN) {
   mat_t rot = {0};
   rotate_matrix(RhoNp1, AcNp1, &rot);
+  int nst1 = AbNp1->stcf.n;
+  int *rblk = (int *)malloc(sizeof(int) * (size_t)(nst1 + 1));
+  int *rbeg = (int *)malloc(sizeof(int) * (size_t)(nblocks(&rot.b) + 1));
+  for (int i = 0; i < nst1; i++) rblk[i] = -1;
+  for (int b = 0; b < nblocks(&rot.b); b++) {
+    rbeg[b] = blim(&rot.b, b, 0);
+    for (int i = blim(&rot.b, b, 0); i <= blim(&rot.b, b, 1) && i < nst1; i++) rblk[i] = b;
+  }
   int ns = AcN->en.n;
   /* ChildStates: N+1 basis indices per Acut_N state, in basis order */
   int *ccount = (int *)calloc((size_t)ns + 1, sizeof(int)), *cbeg = (int *)calloc((size_t)ns + 2, sizeof(int));
@@ -1478,7 +1499,7 @@ static double calc_rho_n(const arr_t *AcN, const arr_t *AcNp1, const arr_t *AbNp
               double x = orc_cgordan(Sold, Sold, St, Sz - Sold, Si, Sz);
               cg += x * x;
             }
-            el += cg * get_matel(&rot, ci, cj);
+            el += cg * rot_el(&rot, rblk, rbeg, ci, cj);
           }
         }
         if (ist == jst) tr += (2.0 * Sold + 1.0) * el;
@@ -1486,7 +1507,7 @@ static double calc_rho_n(const arr_t *AcN, const arr_t *AcNp1, const arr_t *AbNp
       }
     ipush(&RhoN->mbe, i1 - 1);
   }
-  free(ccount); free(cbeg); free(child); free(fill);
+  free(ccount); free(cbeg); free(child); free(fill); free(rblk); free(rbeg);
   mat_free(&rot);
   return tr;
 }
@@ -1546,15 +1567,21 @@ double orc_dm_spec_shell(const orc_dm *D, int N, double w) {
     int n1 = bsize(A, b1), n2 = bsize(A, b2), i0 = blim(A, b1, 0), j0 = blim(A, b2, 0);
     double *Aw = (double *)malloc(sizeof(double) * (size_t)(n1 * n2 + 1));
     double *P = (double *)malloc(sizeof(double) * (size_t)(n1 * n2 + 1));
+    /* MatBlock2STL of the operator block and of the two rho blocks */
+    double *Ab = (double *)malloc(sizeof(double) * (size_t)(n1 * n2 + 1));
+    double *R1 = (double *)malloc(sizeof(double) * (size_t)(n1 * n1 + 1));
+    double *R2 = (double *)malloc(sizeof(double) * (size_t)(n2 * n2 + 1));
+    for (int i = 0; i < n1; i++) for (int j = 0; j < n2; j++) Ab[i * n2 + j] = get_block_matel(Op, b1, b2, i, j);
+    for (int i = 0; i < n1; i++) for (int j = 0; j < n1; j++) R1[i * n1 + j] = get_block_matel(R, b1, b1, i, j);
+    for (int i = 0; i < n2; i++) for (int j = 0; j < n2; j++) R2[i * n2 + j] = get_block_matel(R, b2, b2, i, j);
     for (int i = 0; i < n1; i++)
       for (int j = 0; j < n2; j++)
-        Aw[i * n2 + j] = get_block_matel(Op, b1, b2, i, j) *
-                         bdelta_ij(D, A, w, A->en.a[i0 + i], A->en.a[j0 + j]);
+        Aw[i * n2 + j] = Ab[i * n2 + j] * bdelta_ij(D, A, w, A->en.a[i0 + i], A->en.a[j0 + j]);
     /* Termo1: Prod1 = A^T rho1 (n2 x n1); trace(Prod1 Aw) */
     for (int j = 0; j < n2; j++)
       for (int i = 0; i < n1; i++) {
         double s = 0.0;
-        for (int l = 0; l < n1; l++) s += get_block_matel(Op, b1, b2, l, j) * get_block_matel(R, b1, b1, l, i);
+        for (int l = 0; l < n1; l++) s += Ab[l * n2 + j] * R1[l * n1 + i];
         P[j * n1 + i] = s;
       }
     double t1 = 0.0;
@@ -1563,13 +1590,13 @@ double orc_dm_spec_shell(const orc_dm *D, int N, double w) {
     for (int j = 0; j < n2; j++)
       for (int i = 0; i < n1; i++) {
         double s = 0.0;
-        for (int l = 0; l < n2; l++) s += get_block_matel(R, b2, b2, j, l) * get_block_matel(Op, b1, b2, i, l);
+        for (int l = 0; l < n2; l++) s += R2[j * n2 + l] * Ab[i * n2 + l];
         P[j * n1 + i] = s;
       }
     double t2 = 0.0;
     for (int j = 0; j < n2; j++) for (int i = 0; i < n1; i++) t2 += P[j * n1 + i] * Aw[i * n2 + j];
     acc += op_cgfactor(getqn(A, b1, 1), getqn(A, b2, 1)) * (t1 + t2);
-    free(Aw); free(P);
+    free(Aw); free(P); free(Ab); free(R1); free(R2);
   }
   return acc;
 }

@@ -204,18 +204,18 @@ class DmParams(ctypes.Structure):
     ]
 
 
-_dp = ctypes.POINTER(ctypes.c_double)
-_ip = ctypes.POINTER(ctypes.c_int)
+_PD = ctypes.POINTER(ctypes.c_double)
+_PI = ctypes.POINTER(ctypes.c_int)
 
 
 class DmShellView(ctypes.Structure):
     _fields_ = [
         ("nblocks", ctypes.c_int), ("nq", ctypes.c_int), ("nstates", ctypes.c_int),
-        ("qn", _dp), ("bbe", _ip), ("en", _dp), ("bbe_bc", _ip), ("ev", _dp), ("bbe_ev", _ip),
-        ("rho", _dp), ("rho_mbe", _ip), ("trace", ctypes.c_double),
-        ("op_nblocks", ctypes.c_int), ("op_map", _ip), ("op_mbe", _ip), ("op", _dp),
+        ("qn", _PD), ("bbe", _PI), ("en", _PD), ("bbe_bc", _PI), ("ev", _PD), ("bbe_ev", _PI),
+        ("rho", _PD), ("rho_mbe", _PI), ("trace", ctypes.c_double),
+        ("op_nblocks", ctypes.c_int), ("op_map", _PI), ("op_mbe", _PI), ("op", _PD),
         ("basis_nstates", ctypes.c_int), ("basis_nblocks", ctypes.c_int),
-        ("basis_stcf", _ip), ("basis_itype", _ip), ("basis_qn", _dp), ("basis_bbe", _ip),
+        ("basis_stcf", _PI), ("basis_itype", _PI), ("basis_qn", _PD), ("basis_bbe", _PI),
     ]
 
 
@@ -234,11 +234,11 @@ class DmRun:
         L.orc_dm_run.restype = ctypes.c_void_p
         L.orc_dm_run.argtypes = [ctypes.POINTER(DmParams)]
         L.orc_dm_free.argtypes = [ctypes.c_void_p]
-        L.orc_dm_info.argtypes = [ctypes.c_void_p, _ip, _ip, _dp]
+        L.orc_dm_info.argtypes = [ctypes.c_void_p, _PI, _PI, _PD]
         L.orc_dm_shell.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(DmShellView)]
         L.orc_dm_spec_shell.restype = ctypes.c_double
         L.orc_dm_spec_shell.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double]
-        L.orc_dm_spectrum.argtypes = [ctypes.c_void_p, ctypes.POINTER(_dp), ctypes.POINTER(_dp)]
+        L.orc_dm_spectrum.argtypes = [ctypes.c_void_p, ctypes.POINTER(_PD), ctypes.POINTER(_PD)]
         p = DmParams(U, Gamma, ed, Lambda, nsitesmax, ncut, disc_scheme, myeps, betabar_nrg, Mtemp, betabar,
                      twindow, broadtemp, bbroad, Nw)
         self._L = L
@@ -299,6 +299,6 @@ class DmRun:
         return self._L.orc_dm_spec_shell(self.h, N, w)
 
     def spectrum(self):
-        po, pr = _dp(), _dp()
+        po, pr = _PD(), _PD()
         n = self._L.orc_dm_spectrum(self.h, ctypes.byref(po), ctypes.byref(pr))
         return _arr(po, n), _arr(pr, n)

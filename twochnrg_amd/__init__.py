@@ -18,6 +18,8 @@ EXPORTS = [
     "nrg_diag_blocks", "nrg_diag_blocks_dev", "nrg_truncate", "nrg_ecut",
     "nrg_rotate_ops", "nrg_run_onechqs", "nrg_run_twochqs", "nrg_set_timing", "nrg_get_stats", "nrg_reset_stats",
     "nrg_set_eig_impl", "nrg_set_plain", "nrg_eig2_stages",
+    "nrg_dm_create", "nrg_dm_destroy", "nrg_dm_info", "nrg_dm_timing", "nrg_dm_rho", "nrg_dm_spec",
+    "nrg_dm_spectrum",
 ]
 
 KSTAT_NAMES = ["eig", "hn", "fbasis", "gemm"]
@@ -58,6 +60,16 @@ class TwoChQSParams(ctypes.Structure):
         ("U", ctypes.c_double), ("Gamma1", ctypes.c_double), ("Gamma2", ctypes.c_double),
         ("ed", ctypes.c_double), ("Lambda", ctypes.c_double), ("nsitesmax", ctypes.c_int),
         ("ncut", ctypes.c_int), ("betabar", ctypes.c_double), ("record_blocks", ctypes.c_int),
+    ]
+
+
+class DmParams(ctypes.Structure):
+    _fields_ = [
+        ("U", ctypes.c_double), ("Gamma", ctypes.c_double), ("ed", ctypes.c_double),
+        ("Lambda", ctypes.c_double), ("nsitesmax", ctypes.c_int), ("ncut", ctypes.c_int),
+        ("disc_scheme", ctypes.c_int), ("myeps", ctypes.c_double), ("betabar_nrg", ctypes.c_double),
+        ("Mtemp", ctypes.c_int), ("betabar", ctypes.c_double), ("twindow", ctypes.c_double),
+        ("broadtemp", ctypes.c_double), ("bbroad", ctypes.c_double), ("Nw", ctypes.c_int),
     ]
 
 
@@ -108,6 +120,14 @@ def lib():
     L.nrg_reset_stats.argtypes = [ctypes.c_void_p]
     L.nrg_run_onechqs.argtypes = [ctypes.c_void_p, ctypes.POINTER(OneChQSParams), STEP_CB, ctypes.c_void_p, _dp]
     L.nrg_run_twochqs.argtypes = [ctypes.c_void_p, ctypes.POINTER(TwoChQSParams), STEP_CB, ctypes.c_void_p, _dp]
+    L.nrg_dm_create.argtypes = [ctypes.c_void_p, ctypes.POINTER(DmParams), ctypes.POINTER(ctypes.c_void_p)]
+    L.nrg_dm_destroy.argtypes = [ctypes.c_void_p]
+    L.nrg_dm_destroy.restype = None
+    L.nrg_dm_info.argtypes = [ctypes.c_void_p, _ip, _ip, _dp]
+    L.nrg_dm_timing.argtypes = [ctypes.c_void_p, _dp]
+    L.nrg_dm_rho.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, ctypes.c_int64, _ip, _dp]
+    L.nrg_dm_spec.argtypes = [ctypes.c_void_p, ctypes.c_int, _ip, _dp, _dp]
+    L.nrg_dm_spectrum.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_int, _ip]
     _lib = L
     return L
 
@@ -262,6 +282,12 @@ class Context:
         return steps, ms.value
 
 
+    def dm(self, U=0.5, Gamma=0.02, ed=-0.25, Lambda=2.5, nsitesmax=40, ncut=400, Mtemp=38, betabar=1.0,
+           disc_scheme=0, twindow=0.0, broadtemp=0.0, bbroad=0.0, Nw=0, myeps=0.0, betabar_nrg=0.0):
+        """DM-NRG handle (nrg_dm_create): forward chain + rho sweep on the device."""
+        return DMNRG(self, DmParams(U, Gamma, ed, Lambda, nsitesmax, ncut, disc_scheme, myeps, betabar_nrg,
+                                    Mtemp, betabar, twindow, broadtemp, bbroad, Nw))
+
     def run_twochqs(self, U=0.4, Gamma1=0.03, Gamma2=0.03, ed=-0.2, Lambda=3.0, nsitesmax=3, ncut=3000,
                     betabar=0.727, record_blocks=False, keep_steps=True, on_step=None):
         steps = []
@@ -322,3 +348,58 @@ def step_to_dict(st):
     if bool(s.evec):
         d["evec"] = np.ctypeslib.as_array(s.evec, (int(np.sum(sizes * sizes)),)).copy()
     return d
+
+
+class DMNRG:
+    """Handle of a device DM-NRG run (include/twochnrg_amd.h nrg_dm_*)."""
+
+    def __init__(self, ctx, params):
+        self.ctx = ctx
+        h = ctypes.c_void_p()
+        _check(lib().nrg_dm_create(ctx.h, ctypes.byref(params), ctypes.byref(h)))
+        self.h = h
+        a, b, t = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        _check(lib().nrg_dm_info(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(t)))
+        self.nshell_max, self.spec_nshell_max, self.temp = a.value, b.value, t.value
+
+    def close(self):
+        if self.h:
+            lib().nrg_dm_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def timing(self):
+        ms = np.zeros(4)
+        _check(lib().nrg_dm_timing(self.h, _ptr(ms)))
+        return dict(forward=ms[0], sweep=ms[1], poles=ms[2], broaden=ms[3])
+
+    def rho(self, N, sizes):
+        """rho_N blocks (list of k x k) for the given block sizes, and its trace."""
+        tot = int(sum(k * k for k in sizes))
+        out = np.zeros(max(tot, 1))
+        nb, tr = ctypes.c_int(), ctypes.c_double()
+        _check(lib().nrg_dm_rho(self.h, N, _ptr(out), out.size, ctypes.byref(nb), ctypes.byref(tr)))
+        blocks, o = [], 0
+        for k in sizes:
+            blocks.append(out[o:o + k * k].reshape(k, k).copy())
+            o += k * k
+        return blocks, tr.value
+
+    def spec(self, shells, omegabar):
+        sh = np.ascontiguousarray(shells, np.int32)
+        w = np.ascontiguousarray(omegabar, np.float64)
+        out = np.zeros(len(sh))
+        _check(lib().nrg_dm_spec(self.h, len(sh), _ptr(sh, _ip), _ptr(w), _ptr(out)))
+        return out
+
+    def spectrum(self):
+        n = ctypes.c_int()
+        _check(lib().nrg_dm_spectrum(self.h, None, None, 0, ctypes.byref(n)))
+        w, r = np.zeros(n.value), np.zeros(n.value)
+        _check(lib().nrg_dm_spectrum(self.h, _ptr(w), _ptr(r), n.value, ctypes.byref(n)))
+        return w, r

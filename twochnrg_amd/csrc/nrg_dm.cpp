@@ -1,0 +1,500 @@
+// nrg_dm.cpp -- DM-NRG on the device (DM_NRG main, src/DM_NRG/DM_NRG.cpp:28-531,
+// QS symmetry, plain DM-NRG): the forward OneChQS chain keeps each shell's
+// kept eigenvector rows, c_d blocks and child map; rho_Nmax = e^{-betabar E}/Z
+// (DM_NRG_SetRhoNmax src/DM_NRG/DM_NRG_SetRhoNmax.cpp:9-126); the backward sweep
+// rotates rho_{N+1} into the pre-cut basis with two batched GEMMs per shell
+// (RotateMatrix src/RotateMatrix.cpp:17-99) and traces out the site as CG-
+// weighted sums of child segments (DM_NRG_CalcRhoN_withSU2
+// src/DM_NRG/DM_NRG_CalcRhoN.cpp:252-449); the shell spectral sums
+// (DMNRG_SpecDens_M_STL src/SpecFunc_allSTL.cpp:223-692, last shell
+// CalcSpecCosti_T_N src/SpecFuncClass.cpp:417-662) become pole weights from
+// two GEMMs per operator block pair and one broadening reduction over every
+// (shell, omega); the omega grids follow CalcSpecDM_NRG_FixedOmegas
+// (src/SpecFuncClass.cpp:1932-2233) and CalcSpec_ManyOmegas (:2239-2341).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "nrg_dm.h"
+#include "nrg_eig2.h"
+#include "nrg_internal.h"
+
+namespace nrg {
+int run_onechqs_dm(nrg_ctx* c, const nrg_onechqs_params& p, std::vector<DmShell>& dm);
+double cgordan(double J1, double M1, double J2, double M2, double J, double M);  // nrg_driver.cpp
+}
+
+struct nrg_dm {
+  nrg_ctx* c = nullptr;
+  nrg_dmnrg_params p{};
+  std::vector<DmShell> sh;
+  int NM = 0, specNM = 0;
+  double temp = 0.0, betabar_spec = 0.0;
+  std::vector<std::vector<int64_t>> rho_off;   // per shell and block: k x k block in dmR<N>
+  std::vector<int64_t> rho_len;
+  // pole arena (filled once, all shells): W, dE and per-shell ranges
+  bool poles_ready = false;
+  std::vector<int64_t> pole_beg, pole_end;
+  double ms_forward = 0, ms_sweep = 0, ms_poles = 0, ms_broaden = 0;
+};
+
+namespace {
+
+using namespace nrg;
+
+double calc_dn(double L, int N) {   // CalcDN src/CalcDN.cpp:4-15, z = 1
+  const double HLF = (L > 0.0) ? 0.5 * (1.0 + 1.0 / L) : 1.0;
+  return HLF * std::pow(L, -(N - 1) / 2.0);
+}
+double partition_func(const DmShell& s, double bb) {   // src/NRGarrayClass.cpp:997-1030
+  double z = 0.0;
+  for (int b = 0; b < s.nblocks(); b++)
+    for (int i = s.bbe[2 * b]; i <= s.bbe[2 * b + 1]; i++) z += (2.0 * s.qn[2 * b + 1] + 1.0) * std::exp(-bb * s.en[i]);
+  return z;
+}
+double site_S(int t) { return (t == 1 || t == 2) ? 0.5 : 0.0; }   // OneChQS site: types 0..3 = 0, up, down, updown
+
+int gemm_groups(nrg_ctx* c, const char* tag, const std::vector<E2Gemm>& probs, const double* A, const double* B,
+                double* C) {
+  if (probs.empty()) return NRG_OK;
+  std::vector<E2Tile> tiles;
+  for (size_t q = 0; q < probs.size(); q++) {
+    const int tm = (probs[q].m + 63) / 64, tn = (probs[q].n + 63) / 64;
+    for (int a = 0; a < tm; a++)
+      for (int b = 0; b < tn; b++) tiles.push_back(E2Tile{(int)q, a, b, 0});
+  }
+  E2Gemm* dp;
+  E2Tile* dt;
+  int rc;
+  if ((rc = upload(c, (std::string(tag) + "_p").c_str(), probs, &dp))) return rc;
+  if ((rc = upload(c, (std::string(tag) + "_t").c_str(), tiles, &dt))) return rc;
+  HIPCHK(e2_launch_gemm(dp, dt, (int)tiles.size(), A, B, C, c->stream));
+  return NRG_OK;
+}
+
+E2Gemm gemm(int m, int n, int k, int64_t a_off, int64_t as0, int64_t as1, int64_t b_off, int64_t bs0, int64_t bs1,
+            int64_t c_off, int64_t cs0) {
+  E2Gemm g{};
+  g.m = m; g.n = n; g.k = k; g.dyn = -1;
+  g.a_off = a_off; g.as0 = as0; g.as1 = as1;
+  g.b_off = b_off; g.bs0 = bs0; g.bs1 = bs1;
+  g.c_off = c_off; g.cs0 = cs0;
+  g.alpha = 1.0; g.accumulate = 0;
+  return g;
+}
+
+// rho_{N} from rho_{N+1}: rotation into the pre-cut basis, then the child trace-out
+int sweep_shell(nrg_dm* D, int N) {
+  nrg_ctx* c = D->c;
+  const DmShell& a = D->sh[N];       // Acut_N
+  const DmShell& b = D->sh[N + 1];   // Acut_{N+1} with Abasis_{N+1}
+  int rc;
+  // rot_b = Z_b^T (rho_b Z_b) over every Acut_{N+1} block
+  const int nbb = b.nblocks();
+  std::vector<int64_t> toff(nbb), roff(nbb);
+  int64_t ttot = 0, rtot = 0;
+  std::vector<E2Gemm> g1, g2;
+  for (int q = 0; q < nbb; q++) {
+    const int k = b.bsize(q), n = b.bsize_bc(q);
+    toff[q] = ttot; ttot += (int64_t)k * n;
+    roff[q] = rtot; rtot += (int64_t)n * n;
+    g1.push_back(gemm(k, n, k, D->rho_off[N + 1][q], k, 1, b.zoff[q], n, 1, toff[q], n));
+    g2.push_back(gemm(n, n, k, b.zoff[q], 1, n, toff[q], n, 1, roff[q], n));
+  }
+  DBuf& tb = c->buf("dm_T");
+  DBuf& rb = c->buf("dm_rot");
+  if ((rc = tb.reserve(sizeof(double) * (size_t)(ttot + 1)))) return rc;
+  if ((rc = rb.reserve(sizeof(double) * (size_t)(rtot + 1)))) return rc;
+  double* dR1 = (double*)c->buf("dmR" + std::to_string(N + 1)).p;
+  double* dZ1 = (double*)c->buf(b.zbuf).p;
+  if ((rc = gemm_groups(c, "dm_g1", g1, dR1, dZ1, (double*)tb.p))) return rc;
+  if ((rc = gemm_groups(c, "dm_g2", g2, dZ1, (double*)tb.p, (double*)rb.p))) return rc;
+  // Abasis_{N+1} block -> surviving Acut_{N+1} block (matched by its first pre-cut state)
+  const int nbas = (int)b.bas_qn.size() / 2;
+  std::vector<int> bas2cut(nbas, -1);
+  for (int q = 0; q < nbb; q++) {
+    const int first = b.bbe_bc[2 * q];
+    for (int ab = 0; ab < nbas; ab++)
+      if (b.bas_bbe[2 * ab] == first) { bas2cut[ab] = q; break; }
+  }
+  std::vector<int> st_blk((size_t)b.stcf.size());
+  for (int ab = 0; ab < nbas; ab++)
+    for (int i = b.bas_bbe[2 * ab]; i <= b.bas_bbe[2 * ab + 1]; i++) st_blk[i] = ab;
+  std::vector<std::vector<int>> children(a.en.size());
+  for (size_t cidx = 0; cidx < b.stcf.size(); cidx++) children[b.stcf[cidx]].push_back((int)cidx);
+  // segments: children of a block's first state; the child of state p of the
+  // block with the same type is the segment start + p
+  std::vector<DmGBlk> gb;
+  std::vector<DmSeg> segs;
+  int kmax = 1;
+  const int nba = a.nblocks();
+  D->rho_off[N].assign(nba, 0);
+  int64_t otot = 0;
+  for (int B = 0; B < nba; B++) {
+    const int i0 = a.bbe[2 * B], k = a.bsize(B);
+    const double Sold = a.qn[2 * B + 1];
+    DmGBlk G{};
+    G.out_off = otot; G.k = k; G.seg_beg = (int)segs.size();
+    D->rho_off[N][B] = otot;
+    otot += (int64_t)k * k;
+    kmax = std::max(kmax, k);
+    for (int c0 : children[i0]) {
+      const int t = b.itype[c0], ab = st_blk[c0];
+      for (int p = 1; p < k; p++) {
+        const int cp = c0 + p;
+        if (cp >= (int)b.stcf.size() || b.stcf[cp] != i0 + p || b.itype[cp] != t || st_blk[cp] != ab)
+          return fail(NRG_ERR_STATE, "DM-NRG: child states of a block are not one contiguous segment per type");
+      }
+      const int q = bas2cut[ab];
+      if (q < 0) continue;   // block cut at N+1: its rotated rho is zero
+      const double Si = b.bas_qn[2 * ab + 1], St = site_S(t);
+      double cg = 0.0;
+      for (double Sz = -Si; Sz <= Si; Sz += 1.0) {
+        const double x = cgordan(Sold, Sold, St, Sz - Sold, Si, Sz);
+        cg += x * x;
+      }
+      const int n = b.bsize_bc(q);
+      const int pc = c0 - b.bas_bbe[2 * ab];
+      DmSeg S{};
+      S.rot_off = roff[q] + (int64_t)pc * n + pc;
+      S.ld = n;
+      S.cg = cg;
+      segs.push_back(S);
+    }
+    G.seg_end = (int)segs.size();
+    gb.push_back(G);
+  }
+  D->rho_len[N] = otot;
+  DBuf& ob = c->buf("dmR" + std::to_string(N));
+  if ((rc = ob.reserve(sizeof(double) * (size_t)(otot + 1)))) return rc;
+  DmGBlk* dgb;
+  DmSeg* dsg;
+  if ((rc = upload(c, "dm_gb", gb, &dgb))) return rc;
+  if ((rc = upload(c, "dm_sg", segs, &dsg))) return rc;
+  HIPCHK(dm_launch_gather(dgb, (int)gb.size(), kmax, dsg, (double*)rb.p, (double*)ob.p, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return NRG_OK;
+}
+
+double op_cgfactor(double S1, double S2) {
+  double cg = 0.0;
+  for (double Sz = -S1; Sz <= S1; Sz += 1.0) {
+    const double x = cgordan(S1, Sz, 0.5, 0.5, S2, Sz + 0.5);
+    cg += x * x;
+  }
+  return cg;
+}
+
+// pole weights and energies of every shell 0 .. specNM-1 (omega independent)
+int build_poles(nrg_dm* D) {
+  nrg_ctx* c = D->c;
+  int rc;
+  const int ns = D->specNM;   // shells with spectral weight: 0 .. specNM-1
+  D->pole_beg.assign(ns, 0);
+  D->pole_end.assign(ns, 0);
+  int64_t ptot = 0;
+  for (int N = 0; N < ns; N++) {
+    const DmShell& s = D->sh[N];
+    D->pole_beg[N] = ptot;
+    for (size_t q = 0; q + 1 < s.op_map.size(); q += 2) ptot += (int64_t)s.bsize(s.op_map[q]) * s.bsize(s.op_map[q + 1]);
+    D->pole_end[N] = ptot;
+  }
+  DBuf& wb = c->buf("dm_W");
+  DBuf& eb = c->buf("dm_dE");
+  if ((rc = wb.reserve(sizeof(double) * (size_t)(ptot + 1)))) return rc;
+  if ((rc = eb.reserve(sizeof(double) * (size_t)(ptot + 1)))) return rc;
+  for (int N = 0; N < ns; N++) {
+    const DmShell& s = D->sh[N];
+    const bool costi = (N == D->specNM - 1);
+    const double bb = (std::fabs(D->temp) < 1e-20) ? 1.0e10 : D->betabar_spec;
+    const double invZ = costi ? 1.0 / partition_func(s, bb) : 0.0;
+    std::vector<DmPair> pairs;
+    std::vector<E2Gemm> g1, g2;
+    int64_t prod = 0, pole = D->pole_beg[N];
+    int kkmax = 1;
+    for (size_t q = 0; q + 1 < s.op_map.size(); q += 2) {
+      const int b1 = s.op_map[q], b2 = s.op_map[q + 1];
+      const int k1 = s.bsize(b1), k2 = s.bsize(b2);
+      DmPair P{};
+      P.a_off = s.op_mbe[q];
+      P.k1 = k1; P.k2 = k2;
+      P.e1_off = s.bbe[2 * b1]; P.e2_off = s.bbe[2 * b2];
+      P.cg = op_cgfactor(s.qn[2 * b1 + 1], s.qn[2 * b2 + 1]);
+      P.pole_off = pole;
+      P.costi = costi ? 1 : 0;
+      P.bb = bb; P.invZ = invZ;
+      if (!costi) {
+        P.p1_off = prod;
+        P.p2_off = prod + (int64_t)k1 * k2;
+        // rho1 A (k1 x k2) and A rho2 (k1 x k2)
+        g1.push_back(gemm(k1, k2, k1, D->rho_off[N][b1], k1, 1, P.a_off, k2, 1, P.p1_off, k2));
+        g2.push_back(gemm(k1, k2, k2, P.a_off, k2, 1, D->rho_off[N][b2], k2, 1, P.p2_off, k2));
+        prod += 2 * (int64_t)k1 * k2;
+      }
+      pole += (int64_t)k1 * k2;
+      kkmax = std::max(kkmax, k1 * k2);
+      pairs.push_back(P);
+    }
+    if (pairs.empty()) continue;
+    DBuf& pb = c->buf("dm_prod");
+    if ((rc = pb.reserve(sizeof(double) * (size_t)(prod + 1)))) return rc;
+    double* dO = (double*)c->buf(s.opbuf).p;
+    if (!costi) {
+      double* dR = (double*)c->buf("dmR" + std::to_string(N)).p;
+      if ((rc = gemm_groups(c, "dm_p1", g1, dR, dO, (double*)pb.p))) return rc;
+      if ((rc = gemm_groups(c, "dm_p2", g2, dO, dR, (double*)pb.p))) return rc;
+    }
+    double* dEn;
+    DmPair* dpairs;
+    if ((rc = upload(c, "dm_en", s.en, &dEn))) return rc;
+    if ((rc = upload(c, "dm_pairs", pairs, &dpairs))) return rc;
+    HIPCHK(dm_launch_weights(dpairs, (int)pairs.size(), kkmax, dO, (double*)pb.p, dEn, (double*)wb.p, (double*)eb.p,
+                             c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  }
+  D->poles_ready = true;
+  return NRG_OK;
+}
+
+int spec_tasks(nrg_dm* D, int n, const int* shells, const double* w, double* out) {
+  nrg_ctx* c = D->c;
+  int rc;
+  if (!D->poles_ready) {
+    auto t0 = std::chrono::steady_clock::now();
+    if ((rc = build_poles(D))) return rc;
+    D->ms_poles += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  }
+  auto t0 = std::chrono::steady_clock::now();
+  const int BCH = 256 * 8;   // poles per chunk, = BCH of the broadening kernel
+  std::vector<DmTask> tasks;
+  std::vector<int> slot;
+  int cb = 0, maxchunk = 0;
+  for (int i = 0; i < n; i++) {
+    const int N = shells[i];
+    if (N < 0 || N > D->specNM - 1) { out[i] = 0.0; continue; }
+    DmTask T{};
+    T.pole_beg = D->pole_beg[N];
+    T.pole_end = D->pole_end[N];
+    T.w = w[i];
+    T.twin = D->p.twindow * D->temp / calc_dn(D->p.Lambda, N);
+    T.nchunk = (int)((T.pole_end - T.pole_beg + BCH - 1) / BCH);
+    T.chunk_beg = cb;
+    cb += T.nchunk;
+    maxchunk = std::max(maxchunk, T.nchunk);
+    tasks.push_back(T);
+    slot.push_back(i);
+  }
+  if (tasks.empty()) return NRG_OK;
+  DBuf& pb = c->buf("dm_partial");
+  DBuf& ob = c->buf("dm_out");
+  if ((rc = pb.reserve(sizeof(double) * (size_t)(cb + 1)))) return rc;
+  if ((rc = ob.reserve(sizeof(double) * (size_t)(tasks.size() + 1)))) return rc;
+  DmTask* dt;
+  if ((rc = upload(c, "dm_tasks", tasks, &dt))) return rc;
+  HIPCHK(dm_launch_broaden(dt, (int)tasks.size(), maxchunk, (double*)c->buf("dm_W").p, (double*)c->buf("dm_dE").p,
+                           D->p.bbroad, D->p.broadtemp, (double*)pb.p, (double*)ob.p, c->stream));
+  std::vector<double> res(tasks.size());
+  HIPCHK(hipMemcpyAsync(res.data(), ob.p, sizeof(double) * tasks.size(), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (size_t q = 0; q < tasks.size(); q++) out[slot[q]] = res[q];
+  D->ms_broaden += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return NRG_OK;
+}
+
+// linear interpolation on ascending knots (the gsl_interp_linear the reference calls)
+double lin_interp(const std::vector<double>& x, const std::vector<double>& y, double t) {
+  int lo = 0, hi = (int)x.size() - 1;
+  while (hi > lo + 1) {
+    const int m = (lo + hi) / 2;
+    if (x[m] > t) hi = m; else lo = m;
+  }
+  const double dx = x[hi] - x[lo];
+  return dx > 0.0 ? y[lo] + (t - x[lo]) * ((y[hi] - y[lo]) / dx) : y[lo];
+}
+
+}  // namespace
+
+extern "C" int nrg_dm_create(nrg_ctx* c, const nrg_dmnrg_params* p, nrg_dm** out) {
+  if (!c || !p || !out) return fail(NRG_ERR_ARG, "nrg_dm_create: NULL argument");
+  if (p->nsitesmax < 2 || p->ncut < 1 || p->Lambda <= 1.0 || p->betabar <= 0.0 || p->Mtemp < 1)
+    return fail(NRG_ERR_ARG, "nrg_dm_create: bad parameters");
+  HIPCHK(hipSetDevice(c->device));
+  nrg_dm* D = new nrg_dm();
+  D->c = c;
+  D->p = *p;
+  if (D->p.twindow <= 0) D->p.twindow = 2.0;
+  if (D->p.broadtemp <= 0) D->p.broadtemp = 0.8;
+  if (D->p.bbroad <= 0) D->p.bbroad = 0.5 * std::log(p->Lambda);
+  nrg_onechqs_params q{};
+  q.U = p->U; q.Gamma = p->Gamma; q.ed = p->ed; q.Lambda = p->Lambda; q.nsitesmax = p->nsitesmax;
+  q.ncut = p->ncut; q.calcdens = 1; q.disc_scheme = p->disc_scheme; q.myeps = p->myeps; q.betabar = p->betabar_nrg;
+  int rc;
+  auto t0 = std::chrono::steady_clock::now();
+  if ((rc = run_onechqs_dm(c, q, D->sh))) { delete D; return rc; }
+  auto t1 = std::chrono::steady_clock::now();
+  // temperature from Mtemp (src/DM_NRG/DM_NRG.cpp:102-118)
+  int Nsitesmax = p->nsitesmax;
+  double DM;
+  if (p->Mtemp < Nsitesmax - 1) {
+    Nsitesmax = p->Mtemp + 1;
+    DM = calc_dn(p->Lambda, p->Mtemp);
+    D->temp = DM / p->betabar;
+  } else {
+    D->temp = calc_dn(p->Lambda, Nsitesmax + 199) / p->betabar;
+    DM = calc_dn(p->Lambda, Nsitesmax - 1);
+  }
+  D->betabar_spec = p->betabar;
+  D->NM = Nsitesmax - 1;
+  D->specNM = Nsitesmax;
+  const int NM = D->NM;
+  D->rho_off.assign(NM + 1, std::vector<int64_t>());
+  D->rho_len.assign(NM + 1, 0);
+  // rho_Nmax: diag e^{-E/TempBar}/Z_N, Z_N at Temp/D_{NM-1} (src/DM_NRG/DM_NRG.cpp:203-214)
+  {
+    const DmShell& s = D->sh[NM];
+    const double ZN = partition_func(s, 1.0 / (D->temp / calc_dn(p->Lambda, NM - 1)));
+    const double TempBar = D->temp / DM;
+    const double bb = (std::fabs(TempBar) < 1e-20) ? 1.0e10 : 1.0 / TempBar;
+    std::vector<double> r;
+    D->rho_off[NM].assign(s.nblocks(), 0);
+    for (int b = 0; b < s.nblocks(); b++) {
+      const int k = s.bsize(b), i0 = s.bbe[2 * b];
+      D->rho_off[NM][b] = (int64_t)r.size();
+      for (int i = 0; i < k; i++)
+        for (int j = 0; j < k; j++) r.push_back(i == j ? std::exp(-bb * s.en[i0 + i]) / ZN : 0.0);
+    }
+    D->rho_len[NM] = (int64_t)r.size();
+    DBuf& rb = c->buf("dmR" + std::to_string(NM));
+    if ((rc = rb.reserve(sizeof(double) * (r.size() + 1)))) { delete D; return rc; }
+    HIPCHK(hipMemcpyAsync(rb.p, r.data(), sizeof(double) * r.size(), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  }
+  for (int N = NM - 1; N >= 0; N--)
+    if ((rc = sweep_shell(D, N))) { delete D; return rc; }
+  auto t2 = std::chrono::steady_clock::now();
+  D->ms_forward = std::chrono::duration<double, std::milli>(t1 - t0).count();
+  D->ms_sweep = std::chrono::duration<double, std::milli>(t2 - t1).count();
+  *out = D;
+  return NRG_OK;
+}
+
+extern "C" void nrg_dm_destroy(nrg_dm* D) { delete D; }
+
+extern "C" int nrg_dm_info(const nrg_dm* D, int* nshell_max, int* spec_nshell_max, double* temp) {
+  if (!D) return fail(NRG_ERR_ARG, "nrg_dm_info: NULL handle");
+  if (nshell_max) *nshell_max = D->NM;
+  if (spec_nshell_max) *spec_nshell_max = D->specNM;
+  if (temp) *temp = D->temp;
+  return NRG_OK;
+}
+
+extern "C" int nrg_dm_timing(const nrg_dm* D, double* ms) {
+  if (!D || !ms) return fail(NRG_ERR_ARG, "nrg_dm_timing: NULL argument");
+  ms[0] = D->ms_forward; ms[1] = D->ms_sweep; ms[2] = D->ms_poles; ms[3] = D->ms_broaden;
+  return NRG_OK;
+}
+
+extern "C" int nrg_dm_rho(nrg_dm* D, int N, double* out, int64_t cap, int* nblocks, double* trace) {
+  if (!D || N < 0 || N > D->NM) return fail(NRG_ERR_ARG, "nrg_dm_rho: bad shell");
+  const DmShell& s = D->sh[N];
+  if (nblocks) *nblocks = s.nblocks();
+  const int64_t len = D->rho_len[N];
+  std::vector<double> r(len);
+  HIPCHK(hipMemcpyAsync(r.data(), D->c->buf("dmR" + std::to_string(N)).p, sizeof(double) * len,
+                        hipMemcpyDeviceToHost, D->c->stream));
+  HIPCHK(hipStreamSynchronize(D->c->stream));
+  if (trace) {
+    double t = 0.0;
+    for (int b = 0; b < s.nblocks(); b++) {
+      const int k = s.bsize(b);
+      for (int i = 0; i < k; i++) t += (2.0 * s.qn[2 * b + 1] + 1.0) * r[D->rho_off[N][b] + (int64_t)i * k + i];
+    }
+    *trace = t;
+  }
+  if (out) {
+    if (cap < len) return fail(NRG_ERR_ARG, "nrg_dm_rho: output too small");
+    std::copy(r.begin(), r.end(), out);
+  }
+  return NRG_OK;
+}
+
+extern "C" int nrg_dm_spec(nrg_dm* D, int n, const int* shells, const double* omegabar, double* out) {
+  if (!D || n < 0 || (n > 0 && (!shells || !omegabar || !out))) return fail(NRG_ERR_ARG, "nrg_dm_spec: bad arguments");
+  HIPCHK(hipSetDevice(D->c->device));
+  return spec_tasks(D, n, shells, omegabar, out);
+}
+
+extern "C" int nrg_dm_spectrum(nrg_dm* D, double* omega, double* rho, int cap, int* nout) {
+  if (!D || !nout) return fail(NRG_ERR_ARG, "nrg_dm_spectrum: NULL argument");
+  HIPCHK(hipSetDevice(D->c->device));
+  const double L = D->p.Lambda, fac = std::pow(L, 1.25);
+  const int Nw = D->p.Nw;
+  std::vector<int> sh;
+  std::vector<double> wbar, wout;
+  std::vector<int> parity;
+  if (Nw > 0) {
+    // CalcSpec_ManyOmegas: recorded w = +-Lambda^(-i/(2Nw)) Lambda^1.25 D_N, the
+    // shell sum evaluated at +-Lambda^1.25 Lambda^(+i/(2Nw)) as in the reference
+    const int Nmax = D->specNM - 1;
+    for (int N = 0; N <= Nmax; N++)
+      for (int i = 0; i < Nw; i++) {
+        const double rf = (double)i / (2.0 * Nw);
+        sh.push_back(N); wbar.push_back(-fac * std::pow(L, rf));
+        wout.push_back(-std::pow(L, -rf) * fac * calc_dn(L, N));
+      }
+    for (int N = Nmax; N >= 0; N--)
+      for (int i = Nw - 1; i >= 0; i--) {
+        const double rf = (double)i / (2.0 * Nw);
+        sh.push_back(N); wbar.push_back(fac * std::pow(L, rf));
+        wout.push_back(std::pow(L, -rf) * fac * calc_dn(L, N));
+      }
+  } else {
+    // CalcSpecDM_NRG_FixedOmegas with one omega per shell: even then odd shells
+    for (int par = 0; par < 2; par++) {
+      int last = 0;
+      for (int N = par; N < D->specNM; N += 2) {
+        sh.push_back(N); wbar.push_back(-fac); wout.push_back(-fac * calc_dn(L, N)); parity.push_back(par);
+        last = N;
+      }
+      for (int N = last; N >= par; N -= 2) {
+        sh.push_back(N); wbar.push_back(fac); wout.push_back(fac * calc_dn(L, N)); parity.push_back(par);
+      }
+    }
+  }
+  std::vector<double> val(sh.size());
+  int rc;
+  if ((rc = spec_tasks(D, (int)sh.size(), sh.data(), wbar.data(), val.data()))) return rc;
+  for (size_t i = 0; i < sh.size(); i++) val[i] /= calc_dn(L, sh[i]);
+  std::vector<double> W, R;
+  if (Nw > 0) {
+    W = wout; R = val;
+  } else {
+    std::vector<double> ex, ey, ox, oy;
+    for (size_t i = 0; i < sh.size(); i++) {
+      if (parity[i] == 0) { ex.push_back(wout[i]); ey.push_back(val[i]); }
+      else { ox.push_back(wout[i]); oy.push_back(val[i]); }
+    }
+    W = wout;
+    std::sort(W.begin(), W.end());
+    for (double w : W) {
+      const bool inE = !(ex.empty() || w < ex.front() || w > ex.back());
+      const bool inO = !(ox.empty() || w < ox.front() || w > ox.back());
+      double rE = inE ? lin_interp(ex, ey, w) : 0.0, rO = inO ? lin_interp(ox, oy, w) : 0.0;
+      if (!inE) { if (!inO) { rE = -100.0; rO = -100.0; } else rE = rO; }
+      else if (!inO) rO = rE;
+      R.push_back(0.5 * (rE + rO));
+    }
+  }
+  *nout = (int)W.size();
+  if (omega || rho) {
+    if (cap < (int)W.size()) return fail(NRG_ERR_ARG, "nrg_dm_spectrum: output too small");
+    if (omega) std::copy(W.begin(), W.end(), omega);
+    if (rho) std::copy(R.begin(), R.end(), rho);
+  }
+  return NRG_OK;
+}

@@ -24,6 +24,7 @@
 #include <utility>
 #include <vector>
 
+#include "nrg_dm.h"
 #include "nrg_internal.h"
 
 namespace nrg {
@@ -818,10 +819,45 @@ static int max_sidx(const CNRGarray& A) {
   return m;
 }
 
+// What SaveArrays (src/NRGcodehandlerClass.cpp:887-933) writes after step N:
+// Abasis_N, Acut_N (here with its kept eigenvector rows copied out of the
+// step's double-buffered Z into a buffer of their own) and the operator opidx.
+static int dm_capture(Chain& ch, int N, int opidx, DmShell& s) {
+  nrg_ctx* c = ch.c;
+  const CNRGbasisarray& A = ch.AeigCut;
+  int rc;
+  s.N = N;
+  s.qn = A.QNumbers; s.en = A.dEn; s.bbe = A.BlockBegEnd; s.bbe_bc = A.BlockBegEndBC;
+  s.stcf = ch.Abasis.StCameFrom; s.itype = ch.Abasis.iType;
+  s.bas_bbe = ch.Abasis.BlockBegEnd; s.bas_qn = ch.Abasis.QNumbers;
+  const int nb = A.NumBlocks();
+  s.zoff.assign(nb, 0);
+  int64_t tot = 0;
+  for (int b = 0; b < nb; b++) { s.zoff[b] = tot; tot += (int64_t)A.GetBlockSize(b) * A.zld[b]; }
+  s.zbuf = "dmZ" + std::to_string(N);
+  DBuf& zb = c->buf(s.zbuf);
+  if ((rc = zb.reserve(sizeof(double) * (size_t)(tot + 1)))) return rc;
+  for (int b = 0; b < nb; b++)   // kept rows = the first k rows of the block's n x n slice
+    HIPCHK(hipMemcpyAsync((double*)zb.p + s.zoff[b], ch.dZ + A.zoff[b],
+                          sizeof(double) * (size_t)A.GetBlockSize(b) * A.zld[b], hipMemcpyDeviceToDevice, c->stream));
+  const CNRGmatrix& M = ch.mats[opidx];
+  s.op_map = M.MatBlockMap; s.op_mbe = M.MatBlockBegEnd;
+  const int64_t nel = M.MatBlockBegEnd.empty() ? 0 : (int64_t)M.MatBlockBegEnd.back() + 1;
+  s.opbuf = "dmO" + std::to_string(N);
+  DBuf& ob = c->buf(s.opbuf);
+  if ((rc = ob.reserve(sizeof(double) * (size_t)(nel + 1)))) return rc;
+  if (nel)
+    HIPCHK(hipMemcpyAsync(ob.p, c->buf(M.buf[M.cur]).p, sizeof(double) * (size_t)nel, hipMemcpyDeviceToDevice,
+                          c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
 // ---------------------------------------------------------------------------
 // OneChQS Anderson: NRG_main loop
 // ---------------------------------------------------------------------------
-static int run_onechqs(nrg_ctx* c, nrg_onechqs_params p, nrg_step_cb cb, void* user, double* timing_ms) {
+static int run_onechqs(nrg_ctx* c, nrg_onechqs_params p, nrg_step_cb cb, void* user, double* timing_ms,
+                       std::vector<DmShell>* dm = nullptr) {
   const double Lambda = p.Lambda;
   const double bbar = p.betabar > 0 ? p.betabar : 0.727;
   if (p.myeps <= 0) p.myeps = 1e-18;
@@ -918,6 +954,10 @@ static int run_onechqs(nrg_ctx* c, nrg_onechqs_params p, nrg_step_cb cb, void* u
     }
     ch.cut_states(p.ncut, false, p.myeps);
     if ((rc = ch.update_ops())) return rc;
+    if (dm && p.calcdens == 1) {
+      dm->emplace_back();
+      if ((rc = dm_capture(ch, N, 1, dm->back()))) return rc;
+    }
     if ((rc = ch.op_avgs(bbar, ops))) return rc;
     ch.emit(cb, user, emin, ecut, nkept, thT, sus, ent, DN / bbar, ops);
     // chi for the next site: SetChainWilson (src/NRGchainClass.cpp:132-155)
@@ -1034,6 +1074,10 @@ static int run_twochqs(nrg_ctx* c, nrg_twochqs_params p, nrg_step_cb cb, void* u
   auto t1 = std::chrono::steady_clock::now();
   if (timing_ms) *timing_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
   return 0;
+}
+
+int run_onechqs_dm(nrg_ctx* c, const nrg_onechqs_params& p, std::vector<DmShell>& dm) {
+  return run_onechqs(c, p, nullptr, nullptr, nullptr, &dm);
 }
 
 }  // namespace nrg
