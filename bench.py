@@ -21,6 +21,10 @@ sys.path.insert(0, ROOT)
 
 METRIC = "NRG steps/s at Ncut=3000, batched vs plain GPU (1 GPU); DM-NRG A(ω) wall time"
 FP64_PEAK_TFLOPS = 78.6
+HBM_PEAK_GBS = 8000.0
+# HBM bytes per tdb_symv launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on
+# this workload, scaled by the copy-kernel calibration (profiles/tdb_symv_twochqs_ncut3000_traffic.md)
+TRAFFIC = {}
 
 WORKLOADS = {
     # two-channel Anderson (TwoChQS), the Ncut=3000 configuration of the metric
@@ -41,7 +45,7 @@ def _params(w):
     return model, ramp, w
 
 
-def gpu_run(name, steps, warmup, eig_impl=2, plain=False):
+def gpu_run(name, steps, warmup, eig_impl=2, plain=False, stats_all=False):
     import twochnrg_amd as t
     model, ramp, w = _params(WORKLOADS[name])
     first = -1 if model == "onechqs" else 0   # first record emitted
@@ -61,7 +65,7 @@ def gpu_run(name, steps, warmup, eig_impl=2, plain=False):
         print("bench%s: N=%d done, %d blocks, max block %d, %d states" %
               (" (plain)" if plain else "", s.nshell, s.nblocks, max(sizes[s.nshell]), sum(sizes[s.nshell])),
               file=sys.stderr, flush=True)
-        if s.nshell == t_start_N:
+        if s.nshell == t_start_N and not stats_all:
             ctx.reset_stats()
 
     if model == "onechqs":
@@ -165,6 +169,8 @@ def main():
     ap.add_argument("--eig-impl", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-dm", action="store_true", help="skip the DM-NRG A(omega) leg")
+    ap.add_argument("--stats-all", action="store_true",
+                    help="kernel statistics over the whole run (ramp-up included), for counter runs")
     ap.add_argument("--plain-steps", type=int, default=2,
                     help="steps of the plain one-block-per-launch comparison run (0 = skip)")
     a = ap.parse_args()
@@ -172,10 +178,10 @@ def main():
         print("bench.py: this build measures one GPU only (the NRG step and its batched blocks run on one "
               "device); --gpus %d is not supported" % a.gpus, file=sys.stderr)
         sys.exit(2)
-    dt, st, timed, model, w, nsteps = gpu_run(a.workload, a.steps, a.warmup, a.eig_impl)
+    dt, st, timed, model, w, nsteps = gpu_run(a.workload, a.steps, a.warmup, a.eig_impl, stats_all=a.stats_all)
     value = nsteps / dt
     eig = st["eig"]
-    achieved = eig["flops"] / (eig["ms"] * 1e-3) / 1e12 if eig["ms"] > 0 else 0.0
+    eig_rate = eig["flops"] / (eig["ms"] * 1e-3) / 1e12 if eig["ms"] > 0 else 0.0
     cfg = {"workload": a.workload,
            "model": "TwoChQS Anderson (legacy standalone)" if model == "twochqs" else "OneChQS Anderson (NRG_main)",
            "eig_impl": "tridiag+dc+wy" if a.eig_impl == 2 else "wg-per-block householder+ql",
@@ -183,14 +189,27 @@ def main():
            "max_block": max(max(s) for s in timed),
            "states_per_step": round(sum(sum(s) for s in timed) / len(timed), 1)}
     cfg.update({k: v for k, v in w.items()})
+    sy = st["tdb_symv"]
+    if sy["launches"] > 0:
+        # dominant kernel of the large-block workloads: y = A v of the blocked reduction,
+        # algorithmic bytes = the trailing matrices it reads (8 (n-k-1)^2 per active block)
+        avg_s = sy["ms"] * 1e-3 / sy["launches"]
+        bpl = sy["bytes"] / sy["launches"]
+        ach = bpl / avg_s / 1e9
+        roof = {"bound": "hbm", "kernel": "tdb_symv_kernel", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": ach / HBM_PEAK_GBS, "traffic": TRAFFIC.get(a.workload), "bytes_per_launch": bpl,
+                "avg_launch_us": avg_s * 1e6, "launches_per_step": sy["launches"] / nsteps,
+                "ms_per_step": sy["ms"] / nsteps}
+    else:
+        roof = {"bound": "mfma", "kernel": "eigensolve (tridiag+dc+wy)", "achieved": eig_rate,
+                "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": eig_rate / FP64_PEAK_TFLOPS, "traffic": None}
     out = {
         "metric": METRIC, "value": value, "unit": "NRG steps/s", "n_gpus": 1, "steps": nsteps,
         "warmup": a.warmup, "ms_per_step": 1e3 * dt / nsteps, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic", "config": cfg,
-        "roofline": {"bound": "mfma", "kernel": "eigensolve (tridiag+dc+wy)", "achieved": achieved,
-                     "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
-                     "traffic": None, "flops_per_step": eig["flops"] / nsteps,
-                     "eig_ms_per_step": eig["ms"] / nsteps},
+        "roofline": roof,
+        "eigensolve": {"tflops": eig_rate, "frac_fp64_peak": eig_rate / FP64_PEAK_TFLOPS,
+                       "flops_per_step": eig["flops"] / nsteps, "ms_per_step": eig["ms"] / nsteps},
         "kernel_ms_per_step": {k: v["ms"] / nsteps for k, v in st.items()},
     }
     if a.plain_steps > 0:

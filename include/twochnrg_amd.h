@@ -106,7 +106,8 @@ int nrg_rotate_ops(nrg_ctx *ctx, int npairs, const nrg_pair_desc *pairs, const d
 #define NRG_K_HN 1       /* H_N assembly */
 #define NRG_K_FBASIS 2   /* operator matrix in the product basis */
 #define NRG_K_GEMM 3     /* rotation GEMMs */
-#define NRG_NKSTAT 4
+#define NRG_K_SYMV 4     /* y = A v of the blocked reduction of blocks with n > 256 (one launch per column) */
+#define NRG_NKSTAT 5
 
 typedef struct {
   double ms[NRG_NKSTAT];        /* summed kernel time */

@@ -22,7 +22,7 @@ EXPORTS = [
     "nrg_dm_spectrum",
 ]
 
-KSTAT_NAMES = ["eig", "hn", "fbasis", "gemm"]
+KSTAT_NAMES = ["eig", "hn", "fbasis", "gemm", "tdb_symv"]
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
@@ -80,8 +80,8 @@ class PairDesc(ctypes.Structure):
 
 
 class NrgStats(ctypes.Structure):
-    _fields_ = [("ms", ctypes.c_double * 4), ("launches", ctypes.c_long * 4),
-                ("flops", ctypes.c_double * 4), ("bytes", ctypes.c_double * 4)]
+    _fields_ = [("ms", ctypes.c_double * 5), ("launches", ctypes.c_long * 5),
+                ("flops", ctypes.c_double * 5), ("bytes", ctypes.c_double * 5)]
 
 
 _lib = None

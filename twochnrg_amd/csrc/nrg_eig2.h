@@ -52,8 +52,10 @@ extern "C" {
 hipError_t e2_launch_tridiag_small(const E2Block* b, const int* list, int nlist, double* A, double* WS, hipStream_t s);
 hipError_t e2_launch_tdb_norm(const E2Block* b, const int* list, int nlist, int nmax, int k, double* A, double* WS,
                               hipStream_t s);
-hipError_t e2_launch_tdb_column(const E2Block* b, const int* list, int nlist, int nmax, int k, double* A, double* WS,
-                                hipStream_t s);
+hipError_t e2_launch_tdb_symv(const E2Block* b, const int* list, int nlist, int nmax, int k, double* A, double* WS,
+                              hipStream_t s);
+hipError_t e2_launch_tdb_fin(const E2Block* b, const int* list, int nlist, int nmax, int k, double* A, double* WS,
+                             hipStream_t s);
 hipError_t e2_launch_gemm3(const E2Gemm* probs, const E2Tile* tiles, int ntiles, const double* A, const double* B,
                            double* C, hipStream_t s);
 hipError_t e2_launch_tridiag_tail(const E2Block* b, const int* list, int nlist, const double* A, double* WS,

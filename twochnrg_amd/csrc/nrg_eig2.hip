@@ -1024,14 +1024,20 @@ hipError_t e2_launch_tdb_norm(const E2Block* b, const int* list, int nlist, int 
   return hipGetLastError();
 }
 
-hipError_t e2_launch_tdb_column(const E2Block* b, const int* list, int nlist, int nmax, int k, double* A, double* WS,
-                                hipStream_t s) {
+hipError_t e2_launch_tdb_symv(const E2Block* b, const int* list, int nlist, int nmax, int k, double* A, double* WS,
+                              hipStream_t s) {
   if (nlist <= 0 || k + 2 >= nmax) return hipSuccess;
   const int m = nmax - k - 1;
   const int CT = (m + CHUNK - 1) / CHUNK, RT = (m + SYMV_R - 1) / SYMV_R;
   hipLaunchKernelGGL(e2::tdb_symv_kernel, dim3(CT * RT + CT, nlist), dim3(CHUNK), 0, s, b, list, k, A, WS);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
+  return hipGetLastError();
+}
+
+hipError_t e2_launch_tdb_fin(const E2Block* b, const int* list, int nlist, int nmax, int k, double* A, double* WS,
+                             hipStream_t s) {
+  if (nlist <= 0 || k + 2 >= nmax) return hipSuccess;
+  const int m = nmax - k - 1;
+  const int CT = (m + CHUNK - 1) / CHUNK;
   hipLaunchKernelGGL(e2::tdb_fin_kernel, dim3(CT, nlist), dim3(CHUNK), 0, s, b, list, k, A, WS);
   return hipGetLastError();
 }

@@ -251,7 +251,17 @@ int eig2_dev(nrg_ctx* c, int nb, const int* n, const int64_t* a_off, double* dA,
     HIPCHK(e2_launch_tdb_norm(d_blk, d_large, nl, nmax_large, 0, dA, WS, s));
     size_t ev = 0;
     for (int k = 0; k + 2 < nmax_large; k++) {
-      HIPCHK(e2_launch_tdb_column(d_blk, d_large, nl, nmax_large, k, dA, WS, s));
+      {
+        // algorithmic bytes of this launch: the trailing matrix of every active large block
+        double by = 0.0;
+        for (int b : large)
+          if (k + 2 < n[b]) by += 8.0 * (double)(n[b] - k - 1) * (double)(n[b] - k - 1);
+        PendingTimer tm;
+        if ((rc = c->timer_begin(tm, NRG_K_SYMV, 0.0, by))) return rc;
+        HIPCHK(e2_launch_tdb_symv(d_blk, d_large, nl, nmax_large, k, dA, WS, s));
+        if ((rc = c->timer_end(tm))) return rc;
+      }
+      HIPCHK(e2_launch_tdb_fin(d_blk, d_large, nl, nmax_large, k, dA, WS, s));
       if (ev < tev_k.size() && tev_k[ev] == k) {
         HIPCHK(e2_launch_gemm3(d_tp + tev_pbeg[ev], d_tt + tev_tbeg[ev], tev_tcnt[ev], WS, WS, dA, s));
         ev++;
