@@ -115,6 +115,44 @@ def dm_run(cpu=True):
     return out
 
 
+def c2_run(steps=4, ramp=6, plain_steps=1):
+    """64 seeded single-channel Anderson instances (splitmix64 seed 20261203,
+    tests/golden/c2_instances.json), Lambda 2.5, Ncut 800..2000, stepped in
+    lockstep with one batched eigensolve per step; value = instances x steps / s.
+    The plain leg runs the same instances with one block per launch."""
+    import json
+    import twochnrg_amd as t
+    d = json.load(open(os.path.join(ROOT, "tests", "golden", "c2_instances.json")))
+    inst = d["instances"]
+    ni = len(inst)
+    out = {"workload": "onechqs_batch64", "instances": ni, "seed": d["seed"], "ncut_range": [min(x["ncut"] for x in inst),
+                                                                                                max(x["ncut"] for x in inst)]}
+    for plain in (False, True):
+        nst = plain_steps if plain else steps
+        total = ramp + nst
+        stamps, sizes = {}, {}
+        with t.Context(0) as ctx:
+            ctx.set_plain(plain)
+
+            def on_step(i, st):
+                s = st.contents
+                if i == ni - 1:
+                    stamps[s.nshell] = time.perf_counter()
+                    print("bench c2%s: N=%d done" % (" (plain)" if plain else "", s.nshell), file=sys.stderr,
+                          flush=True)
+                sizes.setdefault(s.nshell, 0)
+                sizes[s.nshell] += s.nstates
+            ctx.run_onechqs_multi(inst, nsitesmax=total, calcdens=2, keep_steps=False, on_step=on_step)
+        t0 = stamps[ramp - 1]
+        t1 = stamps[total - 1]
+        rate = ni * nst / (t1 - t0)
+        key = "plain_gpu" if plain else "batched"
+        out[key] = {"value": rate, "unit": "instance-steps/s", "steps": nst, "ms_per_step": 1e3 * (t1 - t0) / nst,
+                    "states_per_step": sizes[total - 1]}
+    out["batched_over_plain"] = out["batched"]["value"] / out["plain_gpu"]["value"]
+    return out
+
+
 def cpu_baseline(name, timed_sizes, budget_s=20.0):
     """The CPU oracle (gcc -O2 -funroll-loops, one thread, FP64) on a bounded
     sample of the same workload: the oracle's dspev-equivalent eigensolve on
@@ -169,6 +207,7 @@ def main():
     ap.add_argument("--eig-impl", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-dm", action="store_true", help="skip the DM-NRG A(omega) leg")
+    ap.add_argument("--no-c2", action="store_true", help="skip the 64-instance batched leg")
     ap.add_argument("--stats-all", action="store_true",
                     help="kernel statistics over the whole run (ramp-up included), for counter runs")
     ap.add_argument("--plain-steps", type=int, default=2,
@@ -192,14 +231,14 @@ def main():
     sy = st["tdb_symv"]
     if sy["launches"] > 0:
         # dominant kernel of the large-block workloads: y = A v of the blocked reduction,
-        # algorithmic bytes = the trailing matrices it reads (8 (n-k-1)^2 per active block)
+        # algorithmic bytes = the trailing matrices it reads (8 m^2 per active block)
         avg_s = sy["ms"] * 1e-3 / sy["launches"]
         bpl = sy["bytes"] / sy["launches"]
         ach = bpl / avg_s / 1e9
         roof = {"bound": "hbm", "kernel": "tdb_symv_kernel", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": ach / HBM_PEAK_GBS, "traffic": TRAFFIC.get(a.workload), "bytes_per_launch": bpl,
-                "avg_launch_us": avg_s * 1e6, "launches_per_step": sy["launches"] / nsteps,
-                "ms_per_step": sy["ms"] / nsteps}
+                "avg_launch_us": avg_s * 1e6, "timed_launches_per_step": sy["launches"] / nsteps,
+                "timing": "HIP events around every 8th column's launch on the library stream"}
     else:
         roof = {"bound": "mfma", "kernel": "eigensolve (tridiag+dc+wy)", "achieved": eig_rate,
                 "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": eig_rate / FP64_PEAK_TFLOPS, "traffic": None}
@@ -224,6 +263,8 @@ def main():
             out["cpu_baseline"] = cb
     if not a.no_dm:
         out["dmnrg"] = dm_run(cpu=not a.no_cpu)
+    if not a.no_c2:
+        out["batched_instances"] = c2_run()
     print(json.dumps(out))
 
 

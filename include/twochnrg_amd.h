@@ -164,6 +164,17 @@ typedef struct {
  */
 int nrg_run_onechqs(nrg_ctx *ctx, const nrg_onechqs_params *p, nrg_step_cb cb, void *user, double *timing_ms);
 
+/*
+ * Many independent OneChQS instances in lockstep (configuration of 64 seeded
+ * Anderson impurities): per step all instances' H_N blocks go into one batched
+ * eigensolve.  p is an array of ninst parameter sets (same nsitesmax); cb
+ * receives the instance index.  Each instance's records equal those of
+ * nrg_run_onechqs with the same parameters.
+ */
+typedef void (*nrg_multi_cb)(void *user, int instance, const nrg_step *st);
+int nrg_run_onechqs_multi(nrg_ctx *ctx, int ninst, const nrg_onechqs_params *p, nrg_multi_cb cb, void *user,
+                          double *timing_ms);
+
 typedef struct {
   double U, Gamma1, Gamma2, ed, Lambda;
   int nsitesmax, ncut;

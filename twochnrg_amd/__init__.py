@@ -19,7 +19,7 @@ EXPORTS = [
     "nrg_rotate_ops", "nrg_run_onechqs", "nrg_run_twochqs", "nrg_set_timing", "nrg_get_stats", "nrg_reset_stats",
     "nrg_set_eig_impl", "nrg_set_plain", "nrg_eig2_stages",
     "nrg_dm_create", "nrg_dm_destroy", "nrg_dm_info", "nrg_dm_timing", "nrg_dm_rho", "nrg_dm_spec",
-    "nrg_dm_spectrum",
+    "nrg_dm_spectrum", "nrg_run_onechqs_multi",
 ]
 
 KSTAT_NAMES = ["eig", "hn", "fbasis", "gemm", "tdb_symv"]
@@ -44,6 +44,7 @@ class NrgStep(ctypes.Structure):
 
 
 STEP_CB = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.POINTER(NrgStep))
+MULTI_CB = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(NrgStep))
 
 
 class OneChQSParams(ctypes.Structure):
@@ -120,6 +121,8 @@ def lib():
     L.nrg_reset_stats.argtypes = [ctypes.c_void_p]
     L.nrg_run_onechqs.argtypes = [ctypes.c_void_p, ctypes.POINTER(OneChQSParams), STEP_CB, ctypes.c_void_p, _dp]
     L.nrg_run_twochqs.argtypes = [ctypes.c_void_p, ctypes.POINTER(TwoChQSParams), STEP_CB, ctypes.c_void_p, _dp]
+    L.nrg_run_onechqs_multi.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(OneChQSParams), MULTI_CB,
+                                        ctypes.c_void_p, _dp]
     L.nrg_dm_create.argtypes = [ctypes.c_void_p, ctypes.POINTER(DmParams), ctypes.POINTER(ctypes.c_void_p)]
     L.nrg_dm_destroy.argtypes = [ctypes.c_void_p]
     L.nrg_dm_destroy.restype = None
@@ -281,6 +284,28 @@ class Context:
         _check(lib().nrg_run_onechqs(self.h, ctypes.byref(p), fn, None, ctypes.byref(ms)))
         return steps, ms.value
 
+
+    def run_onechqs_multi(self, instances, nsitesmax, calcdens=0, disc_scheme=0, myeps=1e-18, betabar=0.727,
+                          keep_steps=True, on_step=None):
+        """Lockstep run of many OneChQS instances (nrg_run_onechqs_multi).
+        instances: list of dicts with U, Gamma, ed, Lambda, ncut."""
+        n = len(instances)
+        arr = (OneChQSParams * n)()
+        for i, d in enumerate(instances):
+            arr[i] = OneChQSParams(d["U"], d["Gamma"], d["ed"], d.get("Lambda", 2.5), nsitesmax, d["ncut"],
+                                   calcdens, disc_scheme, myeps, betabar, 0)
+        steps = [[] for _ in range(n)]
+
+        def cb(_u, inst, st):
+            if on_step is not None:
+                on_step(inst, st)
+            if keep_steps:
+                steps[inst].append(step_to_dict(st))
+
+        fn = MULTI_CB(cb)
+        ms = ctypes.c_double(0.0)
+        _check(lib().nrg_run_onechqs_multi(self.h, n, arr, fn, None, ctypes.byref(ms)))
+        return steps, ms.value
 
     def dm(self, U=0.5, Gamma=0.02, ed=-0.25, Lambda=2.5, nsitesmax=40, ncut=400, Mtemp=38, betabar=1.0,
            disc_scheme=0, twindow=0.0, broadtemp=0.0, bbroad=0.0, Nw=0, myeps=0.0, betabar_nrg=0.0):

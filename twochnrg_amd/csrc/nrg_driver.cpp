@@ -19,6 +19,7 @@
 #include <cstring>
 #include <functional>
 #include <map>
+#include <memory>
 #include <numeric>
 #include <tuple>
 #include <utility>
@@ -456,6 +457,7 @@ static bool pair_ok(PairRule r, const CNRGarray& A, int i, int j) {
 // ---------------------------------------------------------------------------
 struct Chain {
   nrg_ctx* c;
+  std::string tag;   // prefix of this chain's persistent device buffers
   SiteDef site;
   int ntypes = 4;
   CNRGarray Aeig;
@@ -473,8 +475,8 @@ struct Chain {
   void init_ops(int nm) {
     mats.assign(nm, CNRGmatrix());
     for (int m = 0; m < nm; m++) {
-      mats[m].buf[0] = "op" + std::to_string(m) + "_a";
-      mats[m].buf[1] = "op" + std::to_string(m) + "_b";
+      mats[m].buf[0] = tag + "op" + std::to_string(m) + "_a";
+      mats[m].buf[1] = tag + "op" + std::to_string(m) + "_b";
       mats[m].NeedOld = specs[m].need_old;
       mats[m].CalcAvg = specs[m].calc_avg;
     }
@@ -494,40 +496,48 @@ struct Chain {
   }
 
   // H_N assembly on the device for the blocks of Abasis, then the batched
-  // eigensolve; Aeig receives the raw (unshifted) eigenvalues.
+  // eigensolve; Aeig receives the raw (unshifted) eigenvalues.  Three phases so
+  // that several chains can share one eigensolve: layout (block sizes and
+  // offsets in a shared Z / W buffer), assemble (H into Z), collect (eigen-
+  // values back, Z rows stay on the device).
   //   fops[ch]  operator used as f_ch;  coef[ch]  [S index][typep][type] table
-  int diag_step(int nch, const int* fops, const double* chi, const std::vector<double>* coef, double sqrtL,
-                double eN) {
+  std::vector<int> L_bsize;
+  std::vector<int64_t> L_hoff, L_woff;
+  int64_t L_ztot = 0, L_wtot = 0;
+  int L_nmax = 0;
+
+  void layout(int64_t zbase, int64_t wbase) {
+    const int nb = Abasis.NumBlocks();
+    L_bsize.assign(nb, 0); L_hoff.assign(nb, 0); L_woff.assign(nb, 0);
+    int64_t z = 0;
+    L_nmax = 0;
+    for (int b = 0; b < nb; b++) {
+      L_bsize[b] = Abasis.GetBlockSize(b);
+      L_hoff[b] = zbase + z;
+      L_woff[b] = wbase + Abasis.GetBlockLimit(b, 0);
+      z += (int64_t)L_bsize[b] * L_bsize[b];
+      L_nmax = std::max(L_nmax, L_bsize[b]);
+    }
+    L_ztot = z;
+    L_wtot = Abasis.Nstates();
+  }
+
+  int assemble(int nch, const int* fops, const double* chi, const std::vector<double>* coef, double sqrtL, double eN,
+               double* dZn) {
     int rc;
     const int nb = Abasis.NumBlocks();
-    const int ns = Abasis.Nstates();
-    std::vector<int> bbeg(nb), bsize(nb), sidx(nb);
-    std::vector<int64_t> h_off(nb), w_off(nb);
-    int64_t ztot = 0;
-    int nmax = 0;
+    std::vector<int> bbeg(nb), sidx(nb);
     for (int b = 0; b < nb; b++) {
       bbeg[b] = Abasis.GetBlockLimit(b, 0);
-      bsize[b] = Abasis.GetBlockSize(b);
       sidx[b] = (int)key2(Abasis.GetQNumber(b, 1));
-      h_off[b] = ztot;
-      w_off[b] = bbeg[b];
-      ztot += (int64_t)bsize[b] * bsize[b];
-      nmax = std::max(nmax, bsize[b]);
     }
-    zpar ^= 1;
-    DBuf& zb = c->buf(zpar ? "Z1" : "Z2");
-    DBuf& wb = c->buf("W");
-    if ((rc = zb.reserve(sizeof(double) * (size_t)(ztot + 1)))) return rc;
-    if ((rc = wb.reserve(sizeof(double) * (size_t)(ns + 1)))) return rc;
-    double* dZn = (double*)zb.p;
-    dW = (double*)wb.p;
     int *d_bbeg, *d_bsize, *d_sidx, *d_stcf, *d_itype;
     int64_t* d_hoff;
     double *d_en, *d_siteq;
     if ((rc = upload(c, "hb_bbeg", bbeg, &d_bbeg))) return rc;
-    if ((rc = upload(c, "hb_bsize", bsize, &d_bsize))) return rc;
+    if ((rc = upload(c, "hb_bsize", L_bsize, &d_bsize))) return rc;
     if ((rc = upload(c, "hb_sidx", sidx, &d_sidx))) return rc;
-    if ((rc = upload(c, "hb_hoff", h_off, &d_hoff))) return rc;
+    if ((rc = upload(c, "hb_hoff", L_hoff, &d_hoff))) return rc;
     if ((rc = upload(c, "hb_stcf", Abasis.StCameFrom, &d_stcf))) return rc;
     if ((rc = upload(c, "hb_itype", Abasis.iType, &d_itype))) return rc;
     if ((rc = upload(c, "hb_en", Abasis.dEn, &d_en))) return rc;
@@ -551,39 +561,65 @@ struct Chain {
     ha.ntypes = ntypes;
     {
       double by = 0.0;
-      for (int b = 0; b < nb; b++) by += 8.0 * (double)bsize[b] * bsize[b];
+      for (int b = 0; b < nb; b++) by += 8.0 * (double)L_bsize[b] * L_bsize[b];
       PendingTimer tm;
       if ((rc = c->timer_begin(tm, NRG_K_HN, 0.0, by))) return rc;
-      HIPCHK(nrgk_launch_hn(&ha, nb, nmax, c->stream));
+      HIPCHK(nrgk_launch_hn(&ha, nb, L_nmax, c->stream));
       if ((rc = c->timer_end(tm))) return rc;
     }
     hrec.clear();
     if (record) {
-      std::vector<double> hfull(ztot);
-      HIPCHK(hipMemcpyAsync(hfull.data(), dZn, sizeof(double) * ztot, hipMemcpyDeviceToHost, c->stream));
+      std::vector<double> hfull(L_ztot);
+      HIPCHK(hipMemcpyAsync(hfull.data(), dZn + L_hoff[0], sizeof(double) * L_ztot, hipMemcpyDeviceToHost,
+                            c->stream));
       HIPCHK(hipStreamSynchronize(c->stream));
       for (int b = 0; b < nb; b++) {
-        int n = bsize[b];
+        int n = L_bsize[b];
+        const int64_t o = L_hoff[b] - L_hoff[0];
         for (int i = 0; i < n; i++)
-          for (int j = i; j < n; j++) hrec.push_back(hfull[h_off[b] + (int64_t)i * n + j]);
+          for (int j = i; j < n; j++) hrec.push_back(hfull[o + (int64_t)i * n + j]);
       }
     }
-    if ((rc = eig_dev(c, nb, bsize.data(), h_off.data(), dZn, w_off.data(), dW, nullptr))) return rc;
+    return 0;
+  }
+
+  int collect(double* dZn, double* dWn) {
+    const int ns = Abasis.Nstates();
     Aeig = CNRGarray();
     Aeig.Nshell = Abasis.Nshell; Aeig.NQNumbers = 2;
     Aeig.QNumbers = Abasis.QNumbers;
     Aeig.BlockBegEnd = Abasis.BlockBegEnd;
     Aeig.dEn.resize(ns);
-    HIPCHK(hipMemcpyAsync(Aeig.dEn.data(), dW, sizeof(double) * ns, hipMemcpyDeviceToHost, c->stream));
+    const int64_t w0 = L_woff.empty() ? 0 : L_woff[0] - Abasis.GetBlockLimit(0, 0);
+    HIPCHK(hipMemcpyAsync(Aeig.dEn.data(), dWn + w0, sizeof(double) * ns, hipMemcpyDeviceToHost, c->stream));
     zrec.clear();
     if (record) {
-      zrec.resize(ztot);
-      HIPCHK(hipMemcpyAsync(zrec.data(), dZn, sizeof(double) * ztot, hipMemcpyDeviceToHost, c->stream));
+      zrec.resize(L_ztot);
+      HIPCHK(hipMemcpyAsync(zrec.data(), dZn + L_hoff[0], sizeof(double) * L_ztot, hipMemcpyDeviceToHost,
+                            c->stream));
     }
     HIPCHK(hipStreamSynchronize(c->stream));
-    Aeig_zoff = h_off;
+    Aeig_zoff = L_hoff;
     dZ = dZn;
+    dW = dWn;
     return 0;
+  }
+
+  int diag_step(int nch, const int* fops, const double* chi, const std::vector<double>* coef, double sqrtL,
+                double eN) {
+    int rc;
+    layout(0, 0);
+    zpar ^= 1;
+    DBuf& zb = c->buf(tag + (zpar ? "Z1" : "Z2"));
+    DBuf& wb = c->buf(tag + "W");
+    if ((rc = zb.reserve(sizeof(double) * (size_t)(L_ztot + 1)))) return rc;
+    if ((rc = wb.reserve(sizeof(double) * (size_t)(L_wtot + 1)))) return rc;
+    double* dZn = (double*)zb.p;
+    double* dWn = (double*)wb.p;
+    if ((rc = assemble(nch, fops, chi, coef, sqrtL, eN, dZn))) return rc;
+    const int nb = Abasis.NumBlocks();
+    if ((rc = eig_dev(c, nb, L_bsize.data(), L_hoff.data(), dZn, L_woff.data(), dWn, nullptr))) return rc;
+    return collect(dZn, dWn);
   }
 
   // CutStates (src/CutStates.cpp:5-96; by-value form :103-160): AeigCut keeps,
@@ -733,7 +769,7 @@ struct Chain {
     HIPCHK(hipStreamSynchronize(c->stream));
     if ((rc = c->resolve())) return rc;
     // the new operators live on AeigCut
-    return make_basis_dev(c, AeigCut, "oldb", oldb);
+    return make_basis_dev(c, AeigCut, (tag + "oldb").c_str(), oldb);
   }
 
   // CalcOpAvg (src/CalcOpAvg.cpp:13-84) on AeigCut for the operators with CalcAvg
@@ -856,87 +892,102 @@ static int dm_capture(Chain& ch, int N, int opidx, DmShell& s) {
 // ---------------------------------------------------------------------------
 // OneChQS Anderson: NRG_main loop
 // ---------------------------------------------------------------------------
-static int run_onechqs(nrg_ctx* c, nrg_onechqs_params p, nrg_step_cb cb, void* user, double* timing_ms,
-                       std::vector<DmShell>* dm = nullptr) {
-  const double Lambda = p.Lambda;
-  const double bbar = p.betabar > 0 ? p.betabar : 0.727;
-  if (p.myeps <= 0) p.myeps = 1e-18;
-  const double HLF = 0.5 * (1.0 + 1.0 / Lambda);
-  int rc;
+// One OneChQS instance of the NRG_main loop, in phases so that several
+// instances can step in lockstep around one shared eigensolve.
+struct OneChRun {
   Chain ch;
-  ch.c = c;
-  ch.site = onechqs_site();
-  ch.ntypes = 4;
-  ch.record = p.record_blocks != 0;
-  const SiteDef& st = ch.site;
-  // ModelSwitch (src/CodeHandler_ModelSwitch.cpp:121-293): [0] f_N, then
-  // n_d, n_d^2, S_d^2 (calcdens 0/2, ImpOnly_MatEl x old) or c_d (calcdens 1)
-  const int nm = (p.calcdens == 1) ? 2 : 4;
-  ch.specs.resize(nm);
-  ch.specs[0].rule = PAIR_QS_UP;
-  ch.specs[0].g = [&st](double Si, double Sj, int ti, int tj) { return onech_fn_el(st, Si, Sj, ti, tj); };
-  for (int m = 1; m < nm; m++) {
-    ch.specs[m].need_old = true;
-    if (p.calcdens == 1) {
-      ch.specs[m].rule = PAIR_QS_UP;
-      ch.specs[m].g = [&st](double Si, double Sj, int ti, int tj) { return onech_cd_el(st, Si, Sj, ti, tj); };
-    } else {
-      ch.specs[m].rule = PAIR_DIAG;
-      ch.specs[m].calc_avg = true;
-      ch.specs[m].g = [](double, double, int ti, int tj) { return ti == tj ? 1.0 : 0.0; };
-    }
-  }
-  ch.init_ops(nm);
-  // H_{-1}: OneChQS_SetAnderson_Hm1 (src/Main/Main_SetH0.cpp:1794-1969)
-  double chi_m1 = std::sqrt(2.0 * p.Gamma / M_PI) / (std::sqrt(Lambda) * HLF);
-  if (p.disc_scheme != 1) chi_m1 *= std::sqrt(0.5 * std::log(Lambda) * (Lambda + 1) / (Lambda - 1));
-  ch.Aeig.Nshell = -1; ch.Aeig.NQNumbers = 2;
-  ch.Aeig.QNumbers = {-1, 0, 0, 0.5, 1, 0};
-  ch.Aeig.BlockBegEnd = {0, 0, 1, 1, 2, 2};
-  ch.Aeig.dEn = {0.5 * p.U / (Lambda * HLF), (p.ed + 0.5 * p.U) / (Lambda * HLF),
-                 (2.0 * p.ed + 1.5 * p.U) / (Lambda * HLF)};
-  ch.Aeig_zoff = {0, 1, 2};
-  {
-    DBuf& zb = c->buf("Z0");
-    if ((rc = zb.reserve(sizeof(double) * 8))) return rc;
-    double ones[3] = {1, 1, 1};
-    HIPCHK(hipMemcpyAsync(zb.p, ones, sizeof(ones), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
-    ch.dZ = (double*)zb.p;
-  }
-  std::vector<double> f = {1.0, -std::sqrt(2.0)};
-  if ((rc = ch.set_op(0, {0, 1, 1, 2}, {0, 0, 1, 1}, f))) return rc;
-  for (int m = 1; m < nm; m++) {
-    if (p.calcdens == 1) {
-      if ((rc = ch.set_op(m, {0, 1, 1, 2}, {0, 0, 1, 1}, f))) return rc;
-    } else {
-      std::vector<double> v(3);
-      for (int b = 0; b < 3; b++) {
-        double nd = ch.Aeig.GetQNumber(b, 0) + 1.0, Sd = ch.Aeig.GetQNumber(b, 1);
-        v[b] = (m == 1) ? nd : (m == 2 ? nd * nd : Sd * (Sd + 1.0));
-      }
-      if ((rc = ch.set_op(m, {0, 0, 1, 1, 2, 2}, {0, 0, 1, 1, 2, 2}, v))) return rc;
-    }
-  }
-  ch.cut_states(p.ncut, false, p.myeps);
-  if ((rc = make_basis_dev(c, ch.AeigCut, "oldb", ch.oldb))) return rc;
-  std::vector<double> ops;
-  if ((rc = ch.op_avgs(bbar, ops))) return rc;
-  double DN = HLF * std::pow(Lambda, -(0 - 2) / 2.0);
-  ch.emit(cb, user, 0.0, HUGE_VAL, ch.Aeig.Nstates(), nullptr, nullptr, nullptr, DN / bbar, ops);
-
-  double par_chi = chi_m1;
+  nrg_onechqs_params p{};
+  double bbar = 0.727, HLF = 1.0, Lambda = 2.0, par_chi = 0.0;
+  int nm = 2;
   CoefTable hn_tab;
-  hn_tab.ntypes = 4;
-  hn_tab.f = [&st](double S, int a, int b) { return onech_hn_coef(st, S, a, b); };
-  auto t0 = std::chrono::steady_clock::now();
-  for (int N = 0; N < p.nsitesmax; N++) {
-    DN = HLF * std::pow(Lambda, -(N - 1) / 2.0);
-    build_basis(ch.AeigCut, ch.Abasis, st, 0, 0);
+  std::vector<double> ops;
+
+  int init(nrg_ctx* c, const nrg_onechqs_params& pp, const std::string& tag, nrg_step_cb cb, void* user) {
+    p = pp;
+    Lambda = p.Lambda;
+    bbar = p.betabar > 0 ? p.betabar : 0.727;
+    if (p.myeps <= 0) p.myeps = 1e-18;
+    HLF = 0.5 * (1.0 + 1.0 / Lambda);
+    int rc;
+    ch.c = c;
+    ch.tag = tag;
+    ch.site = onechqs_site();
+    ch.ntypes = 4;
+    ch.record = p.record_blocks != 0;
+    const SiteDef* st = &ch.site;
+    // ModelSwitch (src/CodeHandler_ModelSwitch.cpp:121-293): [0] f_N, then
+    // n_d, n_d^2, S_d^2 (calcdens 0/2, ImpOnly_MatEl x old) or c_d (calcdens 1)
+    nm = (p.calcdens == 1) ? 2 : 4;
+    ch.specs.resize(nm);
+    ch.specs[0].rule = PAIR_QS_UP;
+    ch.specs[0].g = [st](double Si, double Sj, int ti, int tj) { return onech_fn_el(*st, Si, Sj, ti, tj); };
+    for (int m = 1; m < nm; m++) {
+      ch.specs[m].need_old = true;
+      if (p.calcdens == 1) {
+        ch.specs[m].rule = PAIR_QS_UP;
+        ch.specs[m].g = [st](double Si, double Sj, int ti, int tj) { return onech_cd_el(*st, Si, Sj, ti, tj); };
+      } else {
+        ch.specs[m].rule = PAIR_DIAG;
+        ch.specs[m].calc_avg = true;
+        ch.specs[m].g = [](double, double, int ti, int tj) { return ti == tj ? 1.0 : 0.0; };
+      }
+    }
+    ch.init_ops(nm);
+    // H_{-1}: OneChQS_SetAnderson_Hm1 (src/Main/Main_SetH0.cpp:1794-1969)
+    double chi_m1 = std::sqrt(2.0 * p.Gamma / M_PI) / (std::sqrt(Lambda) * HLF);
+    if (p.disc_scheme != 1) chi_m1 *= std::sqrt(0.5 * std::log(Lambda) * (Lambda + 1) / (Lambda - 1));
+    ch.Aeig.Nshell = -1; ch.Aeig.NQNumbers = 2;
+    ch.Aeig.QNumbers = {-1, 0, 0, 0.5, 1, 0};
+    ch.Aeig.BlockBegEnd = {0, 0, 1, 1, 2, 2};
+    ch.Aeig.dEn = {0.5 * p.U / (Lambda * HLF), (p.ed + 0.5 * p.U) / (Lambda * HLF),
+                   (2.0 * p.ed + 1.5 * p.U) / (Lambda * HLF)};
+    ch.Aeig_zoff = {0, 1, 2};
+    {
+      DBuf& zb = c->buf(ch.tag + "Z0");
+      if ((rc = zb.reserve(sizeof(double) * 8))) return rc;
+      double ones[3] = {1, 1, 1};
+      HIPCHK(hipMemcpyAsync(zb.p, ones, sizeof(ones), hipMemcpyHostToDevice, c->stream));
+      HIPCHK(hipStreamSynchronize(c->stream));
+      ch.dZ = (double*)zb.p;
+    }
+    std::vector<double> f = {1.0, -std::sqrt(2.0)};
+    if ((rc = ch.set_op(0, {0, 1, 1, 2}, {0, 0, 1, 1}, f))) return rc;
+    for (int m = 1; m < nm; m++) {
+      if (p.calcdens == 1) {
+        if ((rc = ch.set_op(m, {0, 1, 1, 2}, {0, 0, 1, 1}, f))) return rc;
+      } else {
+        std::vector<double> v(3);
+        for (int b = 0; b < 3; b++) {
+          double nd = ch.Aeig.GetQNumber(b, 0) + 1.0, Sd = ch.Aeig.GetQNumber(b, 1);
+          v[b] = (m == 1) ? nd : (m == 2 ? nd * nd : Sd * (Sd + 1.0));
+        }
+        if ((rc = ch.set_op(m, {0, 0, 1, 1, 2, 2}, {0, 0, 1, 1, 2, 2}, v))) return rc;
+      }
+    }
+    ch.cut_states(p.ncut, false, p.myeps);
+    if ((rc = make_basis_dev(c, ch.AeigCut, (ch.tag + "oldb").c_str(), ch.oldb))) return rc;
+    if ((rc = ch.op_avgs(bbar, ops))) return rc;
+    double DN = HLF * std::pow(Lambda, -(0 - 2) / 2.0);
+    ch.emit(cb, user, 0.0, HUGE_VAL, ch.Aeig.Nstates(), nullptr, nullptr, nullptr, DN / bbar, ops);
+    par_chi = chi_m1;
+    hn_tab.ntypes = 4;
+    hn_tab.f = [st](double S, int a, int b) { return onech_hn_coef(*st, S, a, b); };
+    return 0;
+  }
+
+  void basis() { build_basis(ch.AeigCut, ch.Abasis, ch.site, 0, 0); }
+
+  int assemble(double* dZ) {
     const std::vector<double>& coef = hn_tab.upto(max_sidx(ch.Abasis));
     int fop = 0;
-    if ((rc = ch.diag_step(1, &fop, &par_chi, &coef, std::sqrt(Lambda), 0.0))) return rc;
-    // SetE0zero, SetKept (SetDegen + Ecut), CalcThermo at 4 betabar values
+    return ch.assemble(1, &fop, &par_chi, &coef, std::sqrt(Lambda), 0.0, dZ);
+  }
+
+  // after the eigensolve: SetE0zero, SetKept (SetDegen + Ecut), CalcThermo at
+  // 4 betabar values, CutStates, UpdateMatrices, CalcStuff
+  int post(int N, nrg_step_cb cb, void* user, std::vector<DmShell>* dm) {
+    int rc;
+    const double DN = HLF * std::pow(Lambda, -(N - 1) / 2.0);
     double emin = *std::min_element(ch.Aeig.dEn.begin(), ch.Aeig.dEn.end());
     for (auto& x : ch.Aeig.dEn) x -= emin;
     set_degen(ch.Aeig.dEn, p.myeps);
@@ -965,6 +1016,96 @@ static int run_onechqs(nrg_ctx* c, nrg_onechqs_params p, nrg_step_cb cb, void* u
     double d1 = std::sqrt(1.0 - std::pow(Lambda, -(2.0 * (double)N + 1.0)));
     double d2 = std::sqrt(1.0 - std::pow(Lambda, -(2.0 * (double)N + 3.0)));
     par_chi = d0 / (d1 * d2);
+    return 0;
+  }
+};
+
+// ---------------------------------------------------------------------------
+// OneChQS Anderson: NRG_main loop (src/Main/NRG_main.cpp:282-427)
+// ---------------------------------------------------------------------------
+static int run_onechqs(nrg_ctx* c, nrg_onechqs_params p, nrg_step_cb cb, void* user, double* timing_ms,
+                       std::vector<DmShell>* dm = nullptr) {
+  int rc;
+  OneChRun R;
+  if ((rc = R.init(c, p, "", cb, user))) return rc;
+  auto t0 = std::chrono::steady_clock::now();
+  for (int N = 0; N < p.nsitesmax; N++) {
+    R.basis();
+    R.ch.layout(0, 0);
+    R.ch.zpar ^= 1;
+    DBuf& zb = c->buf(R.ch.zpar ? "Z1" : "Z2");
+    DBuf& wb = c->buf("W");
+    if ((rc = zb.reserve(sizeof(double) * (size_t)(R.ch.L_ztot + 1)))) return rc;
+    if ((rc = wb.reserve(sizeof(double) * (size_t)(R.ch.L_wtot + 1)))) return rc;
+    if ((rc = R.assemble((double*)zb.p))) return rc;
+    if ((rc = eig_dev(c, R.ch.Abasis.NumBlocks(), R.ch.L_bsize.data(), R.ch.L_hoff.data(), (double*)zb.p,
+                      R.ch.L_woff.data(), (double*)wb.p, nullptr)))
+      return rc;
+    if ((rc = R.ch.collect((double*)zb.p, (double*)wb.p))) return rc;
+    if ((rc = R.post(N, cb, user, dm))) return rc;
+  }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  auto t1 = std::chrono::steady_clock::now();
+  if (timing_ms) *timing_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+  return 0;
+}
+
+// Several independent OneChQS instances in lockstep: per step every instance
+// builds its basis and assembles H_N into one shared buffer, one eigensolve
+// covers the blocks of all instances, then each instance truncates, rotates
+// its operators and reports.  All instances run the same number of sites.
+struct MultiCbCtx {
+  nrg_multi_cb cb;
+  void* user;
+  int inst;
+};
+static void multi_tramp(void* u, const nrg_step* st) {
+  MultiCbCtx* m = (MultiCbCtx*)u;
+  if (m->cb) m->cb(m->user, m->inst, st);
+}
+
+static int run_onechqs_multi(nrg_ctx* c, int ninst, const nrg_onechqs_params* ps, nrg_multi_cb cb, void* user,
+                             double* timing_ms) {
+  int rc;
+  const int nsites = ps[0].nsitesmax;
+  for (int i = 0; i < ninst; i++)
+    if (ps[i].nsitesmax != nsites) return fail(NRG_ERR_ARG, "nrg_run_onechqs_multi: all instances need the same nsitesmax");
+  std::vector<std::unique_ptr<OneChRun>> R(ninst);
+  std::vector<MultiCbCtx> cbc(ninst);
+  for (int i = 0; i < ninst; i++) {
+    cbc[i] = MultiCbCtx{cb, user, i};
+    R[i].reset(new OneChRun());
+    if ((rc = R[i]->init(c, ps[i], "m" + std::to_string(i) + "_", multi_tramp, &cbc[i]))) return rc;
+  }
+  auto t0 = std::chrono::steady_clock::now();
+  int zpar = 0;
+  for (int N = 0; N < nsites; N++) {
+    int64_t zb0 = 0, wb0 = 0;
+    for (int i = 0; i < ninst; i++) {
+      R[i]->basis();
+      R[i]->ch.layout(zb0, wb0);
+      zb0 += R[i]->ch.L_ztot;
+      wb0 += R[i]->ch.L_wtot;
+    }
+    zpar ^= 1;
+    DBuf& zb = c->buf(zpar ? "mZ1" : "mZ2");
+    DBuf& wb = c->buf("mW");
+    if ((rc = zb.reserve(sizeof(double) * (size_t)(zb0 + 1)))) return rc;
+    if ((rc = wb.reserve(sizeof(double) * (size_t)(wb0 + 1)))) return rc;
+    std::vector<int> n;
+    std::vector<int64_t> hoff, woff;
+    for (int i = 0; i < ninst; i++) {
+      if ((rc = R[i]->assemble((double*)zb.p))) return rc;
+      n.insert(n.end(), R[i]->ch.L_bsize.begin(), R[i]->ch.L_bsize.end());
+      hoff.insert(hoff.end(), R[i]->ch.L_hoff.begin(), R[i]->ch.L_hoff.end());
+      woff.insert(woff.end(), R[i]->ch.L_woff.begin(), R[i]->ch.L_woff.end());
+    }
+    if ((rc = eig_dev(c, (int)n.size(), n.data(), hoff.data(), (double*)zb.p, woff.data(), (double*)wb.p, nullptr)))
+      return rc;
+    for (int i = 0; i < ninst; i++) {
+      if ((rc = R[i]->ch.collect((double*)zb.p, (double*)wb.p))) return rc;
+      if ((rc = R[i]->post(N, multi_tramp, &cbc[i], nullptr))) return rc;
+    }
   }
   HIPCHK(hipStreamSynchronize(c->stream));
   auto t1 = std::chrono::steady_clock::now();
@@ -1016,7 +1157,7 @@ static int run_twochqs(nrg_ctx* c, nrg_twochqs_params p, nrg_step_cb cb, void* u
   ch.AeigCut.zoff = {0, 1, 2};
   ch.AeigCut.zld = {1, 1, 1};
   if ((rc = ch.set_op(2, {0, 1, 1, 2}, {0, 0, 1, 1}, {1.0, -std::sqrt(2.0)}))) return rc;
-  if ((rc = make_basis_dev(c, ch.AeigCut, "oldb", ch.oldb))) return rc;
+  if ((rc = make_basis_dev(c, ch.AeigCut, (ch.tag + "oldb").c_str(), ch.oldb))) return rc;
   build_basis(ch.AeigCut, ch.Abasis, st, 1, 1);
   CoefTable h0_tab[2], hn_tab[2];
   for (int k = 0; k < 2; k++) {
@@ -1089,6 +1230,16 @@ extern "C" int nrg_run_onechqs(nrg_ctx* c, const nrg_onechqs_params* p, nrg_step
     return nrg::fail(NRG_ERR_ARG, "nrg_run_onechqs: bad parameters");
   HIPCHK(hipSetDevice(c->device));
   return nrg::run_onechqs(c, *p, cb, user, timing_ms);
+}
+
+extern "C" int nrg_run_onechqs_multi(nrg_ctx* c, int ninst, const nrg_onechqs_params* p, nrg_multi_cb cb,
+                                     void* user, double* timing_ms) {
+  if (!c || !p || ninst < 1) return nrg::fail(NRG_ERR_ARG, "nrg_run_onechqs_multi: bad arguments");
+  for (int i = 0; i < ninst; i++)
+    if (p[i].nsitesmax < 0 || p[i].ncut < 1 || p[i].Lambda <= 1.0)
+      return nrg::fail(NRG_ERR_ARG, "nrg_run_onechqs_multi: bad parameters");
+  HIPCHK(hipSetDevice(c->device));
+  return nrg::run_onechqs_multi(c, ninst, p, cb, user, timing_ms);
 }
 
 extern "C" int nrg_run_twochqs(nrg_ctx* c, const nrg_twochqs_params* p, nrg_step_cb cb, void* user,

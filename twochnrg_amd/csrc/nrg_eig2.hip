@@ -276,18 +276,29 @@ __global__ __launch_bounds__(CHUNK) void tdb_fin_kernel(const E2Block* __restric
   const TdbPtr P = tdb_ptrs(B, A0, WS);
   const int tid = threadIdx.x, j = k % PANEL;
   const double t = P.refl[0], v0 = P.refl[1];
-  if (tid < j) {
+  // panel dot products summed over the column chunks, split 8 ways over the workgroup
+  __shared__ double cred[2][CHUNK / PANEL][PANEL];
+  {
+    const int pp = tid % PANEL, g = tid / PANEL;
     double a1 = 0.0, a2 = 0.0;
-    for (int cc = 0; cc < CT; cc++) {
-      a1 += P.cpart[(size_t)cc * 2 * PANEL + tid];
-      a2 += P.cpart[(size_t)cc * 2 * PANEL + PANEL + tid];
-    }
-    c1s[tid] = a1;   // W_p . v
-    c2s[tid] = a2;   // V_p . v
+    if (pp < j)
+      for (int cc = g; cc < CT; cc += CHUNK / PANEL) {
+        a1 += P.cpart[(size_t)cc * 2 * PANEL + pp];
+        a2 += P.cpart[(size_t)cc * 2 * PANEL + PANEL + pp];
+      }
+    cred[0][g][pp] = a1;
+    cred[1][g][pp] = a2;
   }
   double tv = 0.0;
   for (int q = tid; q < CT * RT; q += CHUNK) tv += P.tpart[q];
-  const double vav = block_sum(tv, red);   // also orders the c1s/c2s stores
+  const double vav = block_sum(tv, red);   // also orders the cred stores
+  if (tid < j) {
+    double a1 = 0.0, a2 = 0.0;
+    for (int g = 0; g < CHUNK / PANEL; g++) { a1 += cred[0][g][tid]; a2 += cred[1][g][tid]; }
+    c1s[tid] = a1;   // W_p . v
+    c2s[tid] = a2;   // V_p . v
+  }
+  __syncthreads();
   double cc12 = 0.0;
   for (int p = 0; p < j; p++) cc12 += c1s[p] * c2s[p];
   const double alpha2 = -0.5 * t * t * (vav - 2.0 * cc12);

@@ -251,8 +251,9 @@ int eig2_dev(nrg_ctx* c, int nb, const int* n, const int64_t* a_off, double* dA,
     HIPCHK(e2_launch_tdb_norm(d_blk, d_large, nl, nmax_large, 0, dA, WS, s));
     size_t ev = 0;
     for (int k = 0; k + 2 < nmax_large; k++) {
-      {
-        // algorithmic bytes of this launch: the trailing matrix of every active large block
+      if (k % 8 == 0) {
+        // every 8th column is timed (the events cost more than a short launch):
+        // algorithmic bytes = the trailing matrix of every active large block
         double by = 0.0;
         for (int b : large)
           if (k + 2 < n[b]) by += 8.0 * (double)(n[b] - k - 1) * (double)(n[b] - k - 1);
@@ -260,6 +261,8 @@ int eig2_dev(nrg_ctx* c, int nb, const int* n, const int64_t* a_off, double* dA,
         if ((rc = c->timer_begin(tm, NRG_K_SYMV, 0.0, by))) return rc;
         HIPCHK(e2_launch_tdb_symv(d_blk, d_large, nl, nmax_large, k, dA, WS, s));
         if ((rc = c->timer_end(tm))) return rc;
+      } else {
+        HIPCHK(e2_launch_tdb_symv(d_blk, d_large, nl, nmax_large, k, dA, WS, s));
       }
       HIPCHK(e2_launch_tdb_fin(d_blk, d_large, nl, nmax_large, k, dA, WS, s));
       if (ev < tev_k.size() && tev_k[ev] == k) {
