@@ -664,8 +664,9 @@ static void filter_qnumbers(arr_t *A) {  /* FilterQNumbers src/NRGarrayClass.cpp
     }
   }
 }
-static void build_basis_qs(const arr_t *Ac, arr_t *Ab, const arr_t *S) {
-  /* Q adds, S couples (TotalS on position 1); site QN positions {0,1}, Sz at 2 */
+static void build_basis_qs(const arr_t *Ac, arr_t *Ab, const arr_t *S, int befcut) {
+  /* Q adds, S couples (TotalS on position 1); site QN positions {0,1}, Sz at 2.
+   * befcut = UpdateBefCut 1: StCameFrom is the parent's pre-cut index (:386-391) */
   Ab->nshell = Ac->nshell + 1;
   Ab->nq = 2;
   arr_clear(Ab);
@@ -695,7 +696,7 @@ static void build_basis_qs(const arr_t *Ac, arr_t *Ab, const arr_t *S) {
           for (int is = blim(S, s, 0); is <= blim(S, s, 1); is++) {
             ipush(&Ab->itype, S->itype.a[is]);
             dpush(&Ab->en, Ac->en.a[io]);
-            ipush(&Ab->stcf, io);
+            ipush(&Ab->stcf, befcut == 1 ? Ac->stcf.a[io] : io);
             istnew++;
           }
       }
@@ -880,9 +881,12 @@ static void emit(orc_step_cb cb, void *user, const arr_t *Ae, const arr_t *Ac, d
 
 struct orc_dm;
 static void dm_store_step(struct orc_dm *D, const arr_t *Ac, const arr_t *Ab, const mat_t *op);
-static int run_onechqs(const orc_onechqs_params *p, orc_step_cb cb, void *user, struct orc_dm *dm);
-int orc_run_onechqs(const orc_onechqs_params *p, orc_step_cb cb, void *user) { return run_onechqs(p, cb, user, NULL); }
-static int run_onechqs(const orc_onechqs_params *p, orc_step_cb cb, void *user, struct orc_dm *dm) {
+static int run_onechqs(const orc_onechqs_params *p, orc_step_cb cb, void *user, struct orc_dm *dm, int befcut);
+int orc_run_onechqs(const orc_onechqs_params *p, orc_step_cb cb, void *user) { return run_onechqs(p, cb, user, NULL, 0); }
+/* befcut = UpdateBefCut 1 (src/Main/NRG_main.cpp:334-343,393-396): the operators are
+ * rotated over every eigenstate of AeigCut = Aeig with its Kept flags, SaveArrays
+ * writes that copy, and the real CutStates follows CalcStuff */
+static int run_onechqs(const orc_onechqs_params *p, orc_step_cb cb, void *user, struct orc_dm *dm, int befcut) {
   double Lambda = p->Lambda, bbar = p->betabar > 0 ? p->betabar : 0.727;
   double myeps = p->myeps > 0 ? p->myeps : 1e-18;
   double HLF = 0.5 * (1.0 + 1.0 / Lambda);
@@ -936,7 +940,7 @@ static int run_onechqs(const orc_onechqs_params *p, orc_step_cb cb, void *user, 
   dvec hrec = {0};
   for (int N = 0; N < p->nsitesmax; N++) {
     DN = HLF * pow(Lambda, -(N - 1) / 2.0);
-    build_basis_qs(&Ac, &Ab, &S);
+    build_basis_qs(&Ac, &Ab, &S, befcut);
     double emin = diag_hn(par, &Ab, &S, &mats[0], &Ae, p->record_blocks ? &hrec : NULL);
     int nkept = 0;
     double ecut = set_kept(&Ae, p->ncut, myeps, &nkept);
@@ -947,12 +951,13 @@ static int run_onechqs(const orc_onechqs_params *p, orc_step_cb cb, void *user, 
       sus[nc - 1] = calc_suscep(bN, &Ae);
       ent[nc - 1] = calc_entropy(bN, &Ae);
     }
-    cut_states(&Ae, &Ac, p->ncut, myeps);
+    cut_states(&Ae, &Ac, befcut ? 2 * Ae.en.n : p->ncut, myeps);
     update_matrices(&S, &Ac, &Ab, mats, nm);
     if (dm && p->calcdens == 1) dm_store_step(dm, &Ac, &Ab, &mats[1]);   /* SaveArrays src/NRGcodehandlerClass.cpp:887-933 */
     double ops[3] = {0, 0, 0};
     int nops = 0;
     if (p->calcdens != 1) for (int m = 1; m < nm; m++) ops[nops++] = calc_opavg(bbar, &Ac, &mats[m]);
+    if (befcut) cut_states(&Ae, &Ac, p->ncut, myeps);
     emit(cb, user, &Ae, &Ac, emin, ecut, nkept, &hrec, p->record_blocks, thT, sus, ent, DN / bbar, nops, ops);
     par[1] = orc_chin(N, Lambda);
   }
@@ -1324,7 +1329,7 @@ int orc_run_twochqs(const orc_twochqs_params *p, orc_step_cb cb, void *user) {
 /* DM-NRG: density-matrix sweep and spectral sums (src/DM_NRG,         */
 /* src/SpecFuncClass.cpp, src/SpecFunc_allSTL.cpp, src/BroadDelta.cpp)  */
 /* for the OneChQS Anderson chain run with calcdens 1 (ops f_N, c_d),   */
-/* UpdateBefCut 0, plain DM-NRG (UseCFS 0).                             */
+/* UpdateBefCut 0 or 1; DM-NRG (UseCFS 0), CFS (-C) or FDM (-F).        */
 /* ================================================================== */
 struct orc_dm {
   orc_dmnrg_params p;
@@ -1336,6 +1341,9 @@ struct orc_dm {
   int nshell_max;          /* DM_NRG NshellMax */
   int spec_nshell_max;     /* CSpecFunction::NshellMax */
   double temp, betabar_spec;
+  int method;              /* spec1.UseCFS: 0 DM-NRG, 1 CFS, 2 FDM */
+  double ztot;             /* FDM: Z_tot (src/DM_NRG/DM_NRG.cpp:203-268) */
+  double *wn;              /* FDM: w_N per shell */
   arr_t site;
   int nomega;
   double *omega, *rhow;
@@ -1362,6 +1370,29 @@ static double partition_func(const arr_t *A, double bb) {
   }
   return s;
 }
+/* CNRGarray::PartitionFuncDisc  src/NRGarrayClass.cpp:1032-1058: discarded states only */
+static double partition_func_disc(const arr_t *A, double bb) {
+  if (A->kept.n == 0) return -1.0;
+  double s = 0.0;
+  for (int b = 0; b < nblocks(A); b++) {
+    double Si = getqn(A, b, 1);
+    for (int i = blim(A, b, 0); i <= blim(A, b, 1); i++)
+      if (!A->kept.a[i]) s += (2.0 * Si + 1.0) * exp(-bb * A->en.a[i]);
+  }
+  return s;
+}
+/* CNRGarray::CheckKept(ist, kp)  src/NRGarrayClass.cpp:493-503 */
+static int check_kept(const arr_t *A, int ist, int kp) {
+  if (A->kept.n <= ist) return 0;
+  return (A->kept.a[ist] != 0) == (kp != 0);
+}
+/* CNRGarray::GetBlockSize(ibl, kp)  src/NRGarrayClass.cpp:357-371 */
+static int bsize_kp(const arr_t *A, int b, int kp) {
+  if (A->kept.n <= blim(A, b, 1)) return 0;
+  int n = 0;
+  for (int i = blim(A, b, 0); i <= blim(A, b, 1); i++) n += check_kept(A, i, kp);
+  return n;
+}
 /* BroadDelta  src/BroadDelta.cpp:12-29 */
 static double broad_delta(double w, double Ep, double b) {
   if (Ep == 0.0) Ep = 1.0e-100;
@@ -1373,6 +1404,23 @@ static double broad_delta(double w, double Ep, double b) {
 /* LorentzDelta  src/BroadDelta.cpp:63-73 */
 static double lorentz_delta(double w, double Ep, double b) {
   return (1.0 / (2.0 * M_PI)) * (b / ((w - Ep) * (w - Ep) + b * b));
+}
+/* LorentzDeltaAnders  src/BroadDelta.cpp:76-87 */
+static double lorentz_delta_anders(double w, double Ep, double b) {
+  return (1.0 / M_PI) * (b / ((w - Ep) * (w - Ep) + b * b));
+}
+/* GaussDelta  src/BroadDelta.cpp:96-107 */
+static double gauss_delta(double w, double Ep, double b) {
+  double x = (w - Ep) / b;
+  return (1.0 / (b * sqrt(M_PI))) * exp(-x * x);
+}
+/* LogGaussDelta  src/BroadDelta.cpp:116-132 (Weichselbaum) */
+static double loggauss_delta(double w, double Ep, double b) {
+  if (Ep == 0.0) Ep = 1.0e-100;
+  if (w * Ep < 0.0) return 0.0;
+  double a0 = 1.0 / (b * fabs(Ep) * sqrt(M_PI));
+  double a1 = log(fabs(w / Ep)) / b;
+  return a0 * exp(-(a1 - b / 4.0) * (a1 - b / 4.0));
 }
 
 /* DM_NRG_SetRhoNmax  src/DM_NRG/DM_NRG_SetRhoNmax.cpp:9-126: packed upper,
@@ -1439,10 +1487,11 @@ static void rotate_matrix(const mat_t *M, const arr_t *Ac, mat_t *R) {
 }
 
 /* DM_NRG_CalcRhoN_withSU2  src/DM_NRG/DM_NRG_CalcRhoN.cpp:252-449 with
- * DM_NRG_SetChildSt / DM_NRG_SetChildSameType src/DM_NRG/DM_NRG_SetChildSt.cpp:11-113
- * (no FDM): rho_N(i,j) = sum over children (ci,cj) of the same site type and
- * the same N+1 block of CG(Sold,Sold; S~,Sz-Sold | Si,Sz)^2 summed over Sz,
- * times rho_rot(ci,cj) */
+ * DM_NRG_SetChildSt / DM_NRG_SetChildSameType src/DM_NRG/DM_NRG_SetChildSt.cpp:11-113:
+ * rho_N(i,j) = sum over children (ci,cj) of the same site type and the same
+ * N+1 block of CG(Sold,Sold; S~,Sz-Sold | Si,Sz)^2 summed over Sz, times
+ * rho_rot(ci,cj).  FDM (:393-398): the diagonal of a discarded state is
+ * exp(-E/TempBar)/ZNdisc instead (when ZNdisc != 0). */
 /* GetMatEl on the rotated rho (block diagonal, packed upper): state -> block by table */
 static double rot_el(const mat_t *R, const int *rblk, const int *rbeg, int ist, int jst) {
   int b1 = rblk[ist], b2 = rblk[jst];
@@ -1454,7 +1503,7 @@ static double rot_el(const mat_t *R, const int *rblk, const int *rbeg, int ist, 
   return R->mat.a[R->mbe.a[2 * im] + r];
 }
 static double calc_rho_n(const arr_t *AcN, const arr_t *AcNp1, const arr_t *AbNp1, const arr_t *S,
-                         const mat_t *RhoNp1, mat_t *RhoN) {
+                         const mat_t *RhoNp1, mat_t *RhoN, int fdm, double TempBar, double ZNdisc) {
   mat_t rot = {0};
   rotate_matrix(RhoNp1, AcNp1, &rot);
   int nst1 = AbNp1->stcf.n;
@@ -1502,6 +1551,8 @@ static double calc_rho_n(const arr_t *AcN, const arr_t *AcNp1, const arr_t *AbNp
             el += cg * rot_el(&rot, rblk, rbeg, ci, cj);
           }
         }
+        if (fdm && jst == ist && check_kept(AcN, ist, 0))
+          if (fabs(ZNdisc - 0.0) > 1e-20) el = exp(-(1.0 / TempBar) * AcN->en.a[ist]) / ZNdisc;
         if (ist == jst) tr += (2.0 * Sold + 1.0) * el;
         dpush(&RhoN->mat, el); i1++;
       }
@@ -1520,16 +1571,24 @@ static double op_cgfactor(double S1, double S2) {
   return cg;
 }
 
-/* MijxBDeltaEij  src/SpecFunc_allSTL.cpp:36-114: Mout = Min o BD(omega, Ej-Ei),
- * Lorentzian (dBroadTemp) instead of the log-Gaussian for |Ej-Ei| < twindow*T/D_N */
+/* MijxBDeltaEij  src/SpecFunc_allSTL.cpp:36-219: Mout = Min o BD(omega, Ej-Ei), with
+ * BDeltaTemp(dBroadTemp) instead of BDelta for |Ej-Ei| < twindow*T/D_N.  BDelta /
+ * BDeltaTemp by method (src/DM_NRG/DM_NRG.cpp:474-482): BroadDelta/LorentzDelta
+ * (DM-NRG), BroadDelta/LorentzDeltaAnders (CFS), LogGaussDelta/GaussDelta (FDM) */
 static double bdelta_ij(const struct orc_dm *D, const arr_t *A, double w, double Ei, double Ej) {
   double TempDN = D->temp / calc_dn(D->p.Lambda, A->nshell);
-  if (fabs(Ej - Ei) < D->p.twindow * TempDN) return lorentz_delta(w, Ej - Ei, D->p.broadtemp);
-  return broad_delta(w, Ej - Ei, D->p.bbroad);
+  double x = Ej - Ei;
+  if (fabs(x) < D->p.twindow * TempDN) {
+    if (D->method == 1) return lorentz_delta_anders(w, x, D->p.broadtemp);
+    if (D->method == 2) return gauss_delta(w, x, D->p.broadtemp);
+    return lorentz_delta(w, x, D->p.broadtemp);
+  }
+  if (D->method == 2) return loggauss_delta(w, x, D->p.bbroad);
+  return broad_delta(w, x, D->p.bbroad);
 }
 
-/* CalcSpecCosti_T_N  src/SpecFuncClass.cpp:417-662 (real operators) */
-static double spec_costi(const struct orc_dm *D, int N, double bb, double w) {
+/* CalcSpecCosti_T_N  src/SpecFuncClass.cpp:417-662 (real operators); CalcNorm: BD -> 1 */
+static double spec_costi(const struct orc_dm *D, int N, double bb, double w, int norm) {
   const arr_t *A = &D->acut[N];
   const mat_t *Op = &D->op[N];
   double acc = 0.0;
@@ -1542,7 +1601,7 @@ static double spec_costi(const struct orc_dm *D, int N, double bb, double w) {
       for (int i = 0; i < n1; i++) {
         double a = get_block_matel(Op, b1, b2, i, j);
         double s12 = a * exp(-bb * A->en.a[i0 + i]) + exp(-bb * A->en.a[j0 + j]) * a;
-        tr += s12 * a * bdelta_ij(D, A, w, A->en.a[i0 + i], A->en.a[j0 + j]);
+        tr += s12 * a * (norm ? 1.0 : bdelta_ij(D, A, w, A->en.a[i0 + i], A->en.a[j0 + j]));
       }
     acc += op_cgfactor(getqn(A, b1, 1), getqn(A, b2, 1)) * tr;
   }
@@ -1553,12 +1612,14 @@ static double spec_costi(const struct orc_dm *D, int N, double bb, double w) {
  * diagonal GF): per op block pair Termo1 = Tr[A^T rho1 (A o BD)],
  * Termo2 = Tr[rho2 A^T (A o BD)], weighted by the CG factor; the last shell
  * NshellMax-1 uses the Costi form with the input betabar */
+static double spec_kd(const struct orc_dm *D, int N, double w, int norm);
 double orc_dm_spec_shell(const orc_dm *D, int N, double w) {
   if (N > D->spec_nshell_max - 1) return 0.0;
   if (N == D->spec_nshell_max - 1) {
     double bb = (fabs(D->temp) < 1e-20) ? 1.0e10 : D->betabar_spec;
-    return spec_costi(D, N, bb, w);
+    return spec_costi(D, N, bb, w, 0);
   }
+  if (D->method != 0) return spec_kd(D, N, w, 0);
   const arr_t *A = &D->acut[N];
   const mat_t *Op = &D->op[N], *R = &D->rho[N];
   double acc = 0.0;
@@ -1601,6 +1662,138 @@ double orc_dm_spec_shell(const orc_dm *D, int N, double w) {
   return acc;
 }
 
+/* Terms of the kept/discarded split, on index sets of the two blocks (positions
+ * inside the blocks); A is the n1 x n2 operator block, Dl the n1 x n2 broadening.
+ *   rho1 side (Prod1 = B^T rho1, trace with A o BD):
+ *     sum_{j in J} sum_{i in I} (sum_{l in I} A_lj R1_li) A_ij Dl_ij
+ *   rho2 side (Prod2 = rho2 B^T):
+ *     sum_{j in J} sum_{i in I} (sum_{l in J} R2_jl A_il) A_ij Dl_ij */
+static double kd_term1(const double *A, int n2, const double *R1, int n1, const int *I, int ni, const int *J, int nj,
+                       const double *Dl) {
+  double t = 0.0;
+  for (int jj = 0; jj < nj; jj++) {
+    int j = J[jj];
+    for (int ii = 0; ii < ni; ii++) {
+      int i = I[ii];
+      double p = 0.0;
+      for (int ll = 0; ll < ni; ll++) p += A[(size_t)I[ll] * n2 + j] * R1[(size_t)I[ll] * n1 + i];
+      t += p * A[(size_t)i * n2 + j] * Dl[(size_t)i * n2 + j];
+    }
+  }
+  return t;
+}
+static double kd_term2(const double *A, int n2, const double *R2, const int *I, int ni, const int *J, int nj,
+                       const double *Dl) {
+  double t = 0.0;
+  for (int jj = 0; jj < nj; jj++) {
+    int j = J[jj];
+    for (int ii = 0; ii < ni; ii++) {
+      int i = I[ii];
+      double p = 0.0;
+      for (int ll = 0; ll < nj; ll++) p += R2[(size_t)j * n2 + J[ll]] * A[(size_t)i * n2 + J[ll]];
+      t += p * A[(size_t)i * n2 + j] * Dl[(size_t)i * n2 + j];
+    }
+  }
+  return t;
+}
+
+/* CFS_SpecDens_M_STL  src/SpecFunc_allSTL.cpp:702-1240 (method 1) and
+ * FDM_SpecDens_M_STL  src/SpecFunc_allSTL.cpp:1253-2354 (method 2), real
+ * operators, diagonal GF, below the last shell.  Per operator block pair
+ * (b1 -> b2) with K/D = kept/discarded states (MatBlock2STL with kept flags,
+ * src/NRGmatrixClass.cpp:1188-1212):
+ *   the pair is skipped when neither block has a kept state or neither has a
+ *   discarded one;
+ *   Termo1KK  = T1(K1, D2, rho1_KK)   if K1, D2 and (w >= 0 or CalcNorm)
+ *   Termo2KK  = T2(D1, K2, rho2_KK)   if D1, K2 and (w <  0 or CalcNorm)
+ *   FDM only, if D1 and D2:  Termo1DDADD = T1(D1, D2, rho1_DD),
+ *     Termo1DDADK = T1(D1, K2, rho1_DD) if K2, Termo2DDADD = T2(D1, D2, rho2_DD),
+ *     Termo2DDAKD = T2(K1, D2, rho2_DD) if K1;
+ *   Positive += CG (T1KK + T1DDADD + T2DDADD + T2DDAKD),
+ *   Negative += CG (T2KK + T1DDADD + T1DDADK + T2DDADD)   (CFS: T1KK, T2KK only);
+ * result Positive (w >= 0) or Negative (w < 0); CalcNorm: BD -> 1 and
+ * Positive + Negative. */
+static double spec_kd(const struct orc_dm *D, int N, double w, int norm) {
+  const arr_t *A = &D->acut[N];
+  const mat_t *Op = &D->op[N], *R = &D->rho[N];
+  const int fdm = (D->method == 2);
+  double pos = 0.0, neg = 0.0;
+  for (int im = 0; im < Op->map.n / 2; im++) {
+    int b1 = Op->map.a[2 * im], b2 = Op->map.a[2 * im + 1];
+    int n1 = bsize(A, b1), n2 = bsize(A, b2), i0 = blim(A, b1, 0), j0 = blim(A, b2, 0);
+    int k1 = bsize_kp(A, b1, 1), d1 = bsize_kp(A, b1, 0), k2 = bsize_kp(A, b2, 1), d2 = bsize_kp(A, b2, 0);
+    if (n1 != k1 + d1 || n2 != k2 + d2) return 0.0;   /* "Kept+Disc NOT equal to Nstblock" */
+    if ((k1 == 0 && k2 == 0) || (d1 == 0 && d2 == 0)) continue;
+    int *K1 = (int *)malloc(sizeof(int) * (size_t)(n1 + 1)), *D1 = (int *)malloc(sizeof(int) * (size_t)(n1 + 1));
+    int *K2 = (int *)malloc(sizeof(int) * (size_t)(n2 + 1)), *D2 = (int *)malloc(sizeof(int) * (size_t)(n2 + 1));
+    int a = 0, b = 0;
+    for (int i = 0; i < n1; i++) { if (check_kept(A, i0 + i, 1)) K1[a++] = i; if (check_kept(A, i0 + i, 0)) D1[b++] = i; }
+    a = 0; b = 0;
+    for (int j = 0; j < n2; j++) { if (check_kept(A, j0 + j, 1)) K2[a++] = j; if (check_kept(A, j0 + j, 0)) D2[b++] = j; }
+    double *Ab = (double *)malloc(sizeof(double) * (size_t)(n1 * n2 + 1));
+    double *Dl = (double *)malloc(sizeof(double) * (size_t)(n1 * n2 + 1));
+    double *R1 = (double *)malloc(sizeof(double) * (size_t)(n1 * n1 + 1));
+    double *R2 = (double *)malloc(sizeof(double) * (size_t)(n2 * n2 + 1));
+    for (int i = 0; i < n1; i++)
+      for (int j = 0; j < n2; j++) {
+        Ab[i * n2 + j] = get_block_matel(Op, b1, b2, i, j);
+        Dl[i * n2 + j] = norm ? 1.0 : bdelta_ij(D, A, w, A->en.a[i0 + i], A->en.a[j0 + j]);
+      }
+    for (int i = 0; i < n1; i++) for (int j = 0; j < n1; j++) R1[i * n1 + j] = get_block_matel(R, b1, b1, i, j);
+    for (int i = 0; i < n2; i++) for (int j = 0; j < n2; j++) R2[i * n2 + j] = get_block_matel(R, b2, b2, i, j);
+    double t1kk = 0, t2kk = 0, t1dd = 0, t1dk = 0, t2dd = 0, t2kd = 0;
+    if (k1 && d2 && (w >= 0.0 || norm)) t1kk = kd_term1(Ab, n2, R1, n1, K1, k1, D2, d2, Dl);
+    if (fdm && d1 && d2) {
+      t1dd = kd_term1(Ab, n2, R1, n1, D1, d1, D2, d2, Dl);
+      if (k2) t1dk = kd_term1(Ab, n2, R1, n1, D1, d1, K2, k2, Dl);
+    }
+    if (d1 && k2 && (w < 0.0 || norm)) t2kk = kd_term2(Ab, n2, R2, D1, d1, K2, k2, Dl);
+    if (fdm && d1 && d2) {
+      t2dd = kd_term2(Ab, n2, R2, D1, d1, D2, d2, Dl);
+      if (k1) t2kd = kd_term2(Ab, n2, R2, K1, k1, D2, d2, Dl);
+    }
+    double cg = op_cgfactor(getqn(A, b1, 1), getqn(A, b2, 1));
+    pos += cg * (t1kk + t1dd + t2dd + t2kd);
+    neg += cg * (t2kk + t1dd + t1dk + t2dd);
+    free(K1); free(D1); free(K2); free(D2); free(Ab); free(Dl); free(R1); free(R2);
+  }
+  if (norm) return pos + neg;
+  return (w >= 0.0) ? pos : neg;
+}
+
+/* CalcNfromOmega  src/SpecFuncClass.cpp:41-49 (z = 1) */
+static int n_from_omega(double L, double w) {
+  double HLF = 0.5 * (1.0 + 1.0 / L);
+  int i = (int)(2.0 * log(2.0 * fabs(w) / HLF) / log(L));
+  i = 1 - i;
+  return i < 0 ? 0 : i;
+}
+double orc_dm_spec_omega(const orc_dm *D, double w) {
+  int No = n_from_omega(D->p.Lambda, w);
+  int N0 = (No - 8 > 0) ? No - 8 : 0;
+  int N1 = (No + 14 < D->spec_nshell_max) ? No + 14 : D->spec_nshell_max;
+  double s = 0.0;
+  for (int N = N0; N < N1; N++) {
+    double DN = calc_dn(D->p.Lambda, N);
+    s += orc_dm_spec_shell(D, N, w / DN) / DN;
+  }
+  return s;
+}
+double orc_dm_norm(const orc_dm *D) {
+  if (D->method == 0) return NAN;
+  double s = 0.0;
+  for (int N = 0; N < D->spec_nshell_max; N++) {
+    if (N == D->spec_nshell_max - 1) {
+      double bb = (fabs(D->temp) < 1e-20) ? 1.0e10 : D->betabar_spec;
+      s += spec_costi(D, N, bb, 0.0, 1);
+    } else {
+      s += spec_kd(D, N, 0.0, 1);
+    }
+  }
+  return s;
+}
+double orc_dm_ztot(const orc_dm *D) { return D->ztot; }
+
 static void dm_store_step(struct orc_dm *D, const arr_t *Ac, const arr_t *Ab, const mat_t *op) {
   int N = D->nsh++;
   D->acut = (arr_t *)realloc(D->acut, sizeof(arr_t) * D->nsh);
@@ -1623,12 +1816,20 @@ static int cmpd2(const void *a, const void *b) {
   return (x > y) - (x < y);
 }
 
+/* one grid value: DM-NRG evaluates the shell's own sum at +-Lambda^1.25 (/D_N);
+ * CFS and FDM call CalcSpecDM_NRG(+-omega) (src/SpecFuncClass.cpp:1979-1982) */
+static double grid_value(const struct orc_dm *D, int N, double sgn, double wbar, double omega) {
+  if (D->method != 0) return orc_dm_spec_omega(D, sgn * omega);
+  return orc_dm_spec_shell(D, N, sgn * wbar) / calc_dn(D->p.Lambda, N);
+}
+
 /* CalcSpecDM_NRG_FixedOmegas  src/SpecFuncClass.cpp:1932-2233 (NwEachN = 1):
  * even and odd shells on w = +-Lambda^1.25 D_N, each set interpolated linearly
- * on the sorted union and averaged */
+ * on the sorted union and averaged; CFS/FDM start two shells later (:1954) */
 static void spec_fixed_omegas(struct orc_dm *D, double fac) {
   int N0 = 0, NM = D->spec_nshell_max;
   int Nsh0E = (N0 % 2 == 0) ? N0 : N0 + 1, Nsh0O = (N0 % 2 == 0) ? N0 + 1 : N0;
+  if (D->method != 0) { Nsh0E += 2; Nsh0O += 2; }
   dvec ex = {0}, ey = {0}, ox = {0}, oy = {0}, all = {0};
   int last = 0;
   for (int par = 0; par < 2; par++) {
@@ -1637,13 +1838,13 @@ static void spec_fixed_omegas(struct orc_dm *D, double fac) {
     last = 0;
     for (int N = n0; N < NM; N += 2) {
       double DN = calc_dn(D->p.Lambda, N), WN = fac * DN;
-      double r = orc_dm_spec_shell(D, N, -fac * pow(D->p.Lambda, 0.0)) / DN;
+      double r = grid_value(D, N, -1.0, fac * pow(D->p.Lambda, 0.0), WN);
       dpush(X, -WN); dpush(Y, r); dpush(&all, -WN);
       last = N;
     }
     for (int N = last; N >= n0; N -= 2) {
       double DN = calc_dn(D->p.Lambda, N), WN = fac * DN;
-      double r = orc_dm_spec_shell(D, N, fac * pow(D->p.Lambda, 0.0)) / DN;
+      double r = grid_value(D, N, 1.0, fac * pow(D->p.Lambda, 0.0), WN);
       dpush(X, WN); dpush(Y, r); dpush(&all, WN);
     }
   }
@@ -1663,10 +1864,12 @@ static void spec_fixed_omegas(struct orc_dm *D, double fac) {
   free(ex.a); free(ey.a); free(ox.a); free(oy.a); free(all.a);
 }
 
-/* CalcSpec_ManyOmegas  src/SpecFuncClass.cpp:2239-2341: w = +-Lambda^(-i/(2Nw)) Lambda^1.25 D_N,
- * the shell sum evaluated at +-Lambda^1.25 Lambda^(+i/(2Nw)) as the reference does */
+/* CalcSpec_ManyOmegas  src/SpecFuncClass.cpp:2239-2341: w = +-Lambda^(-i/(2Nw)) Lambda^1.25 D_N;
+ * DM-NRG evaluates the shell sum at +-Lambda^1.25 Lambda^(+i/(2Nw)) as the reference
+ * does, CFS/FDM the window sum at the recorded w, from two shells later (:2256) */
 static void spec_many_omegas(struct orc_dm *D, int Nw, double fac) {
   int N0 = 0, Nmax = D->spec_nshell_max - 1;
+  if (D->method != 0) N0 += 2;
   int cap = 2 * (Nmax - N0 + 1) * Nw + 1;
   D->omega = (double *)malloc(sizeof(double) * (size_t)cap);
   D->rhow = (double *)malloc(sizeof(double) * (size_t)cap);
@@ -1676,7 +1879,8 @@ static void spec_many_omegas(struct orc_dm *D, int Nw, double fac) {
     for (int i = 0; i < Nw; i++) {
       double rf = (double)i / (2.0 * Nw), w = pow(D->p.Lambda, -rf) * WN;
       D->omega[k] = -w;
-      D->rhow[k++] = orc_dm_spec_shell(D, N, -fac * pow(D->p.Lambda, rf)) / DN;
+      D->rhow[k++] = grid_value(D, N, -1.0, fac * pow(D->p.Lambda, rf), w);
+      (void)DN;
     }
   }
   for (int N = Nmax; N >= N0; N--) {
@@ -1684,35 +1888,40 @@ static void spec_many_omegas(struct orc_dm *D, int Nw, double fac) {
     for (int i = Nw - 1; i >= 0; i--) {
       double rf = (double)i / (2.0 * Nw), w = pow(D->p.Lambda, -rf) * WN;
       D->omega[k] = w;
-      D->rhow[k++] = orc_dm_spec_shell(D, N, fac * pow(D->p.Lambda, rf)) / DN;
+      D->rhow[k++] = grid_value(D, N, 1.0, fac * pow(D->p.Lambda, rf), w);
+      (void)DN;
     }
   }
   D->nomega = k;
 }
 
-/* DM_NRG main  src/DM_NRG/DM_NRG.cpp:28-531 (QS symmetry, UseFDM false, UseCFS 0,
- * NFermiOps 1): temperature from Mtemp (:102-118), rho at NshellMax with
- * ZN = PartitionFunc at Temp/D_{NshellMax-1} (:203-214, the reference's own
- * "Why????" shell offset kept), backward sweep (:302-434), spectral function
- * rho_0_0 (:451-506) on the grid of DM_NRG_CalcSpecFunc_ij
- * src/DM_NRG/DM_NRG_CalcSpecFuncs.cpp:73-84 */
+/* DM_NRG main  src/DM_NRG/DM_NRG.cpp:28-531 (QS symmetry, NFermiOps 1):
+ * temperature from Mtemp (:102-118; FDM keeps every shell, :105), rho at
+ * NshellMax with ZN = PartitionFunc at Temp/D_{NshellMax-1} (:203-214, the
+ * reference's own "Why????" shell offset kept); FDM: w_N from Z_N^disc 4^{Nmax-N}
+ * (:221-271) and rho_Nmax normalised by Z_tot with the exponent Temp/DM of
+ * :102-118 (:273-297); backward sweep (:312-434) with TempBar = Temp/D_{N-1} and
+ * ZNdisc = Z_tot/4^{Nmax-N} (:361-366); spectral function rho_0_0 (:451-506) on the grid
+ * of DM_NRG_CalcSpecFunc_ij src/DM_NRG/DM_NRG_CalcSpecFuncs.cpp:73-84 */
 orc_dm *orc_dm_run(const orc_dmnrg_params *p) {
   struct orc_dm *D = (struct orc_dm *)calloc(1, sizeof(struct orc_dm));
   D->p = *p;
   if (D->p.twindow <= 0) D->p.twindow = 2.0;
   if (D->p.broadtemp <= 0) D->p.broadtemp = 0.8;
   if (D->p.bbroad <= 0) D->p.bbroad = 0.5 * log(p->Lambda);
+  D->method = (p->method == 1 || p->method == 2) ? p->method : 0;
+  const int fdm = (D->method == 2);
   orc_onechqs_params q;
   memset(&q, 0, sizeof(q));
   q.U = p->U; q.Gamma = p->Gamma; q.ed = p->ed; q.Lambda = p->Lambda; q.nsitesmax = p->nsitesmax;
   q.ncut = p->ncut; q.calcdens = 1; q.disc_scheme = p->disc_scheme; q.myeps = p->myeps;
   q.betabar = p->betabar_nrg;
-  run_onechqs(&q, NULL, NULL, D);
+  run_onechqs(&q, NULL, NULL, D, p->update_bef_cut ? 1 : 0);
   onechqs_site(&D->site);
   int Nsitesmax = p->nsitesmax;
   double DM, Temp;
   if (p->Mtemp < Nsitesmax - 1) {
-    Nsitesmax = p->Mtemp + 1;
+    if (!fdm) Nsitesmax = p->Mtemp + 1;
     DM = calc_dn(p->Lambda, p->Mtemp);
     Temp = DM / p->betabar;
   } else {
@@ -1728,9 +1937,23 @@ orc_dm *orc_dm_run(const orc_dmnrg_params *p) {
   D->trace = (double *)calloc((size_t)NM + 1, sizeof(double));
   double TempBar = Temp / calc_dn(p->Lambda, NM - 1);
   double ZN = partition_func(&D->acut[NM], 1.0 / TempBar);
-  D->trace[NM] = set_rho_nmax(Temp / DM, ZN, &D->acut[NM], &D->rho[NM]);
-  for (int N = NM - 1; N >= 0; N--)
-    D->trace[N] = calc_rho_n(&D->acut[N], &D->acut[N + 1], &D->abasis[N + 1], &D->site, &D->rho[N + 1], &D->rho[N]);
+  D->wn = (double *)calloc((size_t)NM + 1, sizeof(double));
+  double ZTot = ZN;
+  D->wn[NM] = ZN;
+  for (int N = NM - 1; N >= 0; N--) {
+    double tb = Temp / calc_dn(p->Lambda, N - 1);
+    double zd = partition_func_disc(&D->acut[N], 1.0 / tb) * pow(4.0, NM - N);
+    D->wn[N] = zd;
+    ZTot += zd;
+  }
+  for (int N = 0; N <= NM; N++) D->wn[N] = fdm ? D->wn[N] / ZTot : 0.0;
+  D->ztot = fdm ? ZTot : 0.0;
+  D->trace[NM] = set_rho_nmax(Temp / DM, fdm ? ZTot : ZN, &D->acut[NM], &D->rho[NM]);
+  for (int N = NM - 1; N >= 0; N--) {
+    double tb = Temp / calc_dn(p->Lambda, N - 1);
+    D->trace[N] = calc_rho_n(&D->acut[N], &D->acut[N + 1], &D->abasis[N + 1], &D->site, &D->rho[N + 1], &D->rho[N],
+                             fdm, tb, ZTot / pow(4.0, NM - N));
+  }
   double fac = pow(p->Lambda, 1.25);
   if (p->Nw > 0) spec_many_omegas(D, p->Nw, fac);
   else spec_fixed_omegas(D, fac);
@@ -1741,7 +1964,7 @@ void orc_dm_free(orc_dm *D) {
   if (!D) return;
   for (int i = 0; i < D->nsh; i++) { arr_free(&D->acut[i]); arr_free(&D->abasis[i]); mat_free(&D->op[i]); }
   for (int i = 0; D->rho && i <= D->nshell_max; i++) mat_free(&D->rho[i]);
-  free(D->acut); free(D->abasis); free(D->op); free(D->rho); free(D->trace);
+  free(D->acut); free(D->abasis); free(D->op); free(D->rho); free(D->trace); free(D->wn);
   arr_free(&D->site);
   free(D->omega); free(D->rhow);
   free(D);
@@ -1764,6 +1987,8 @@ int orc_dm_shell(const orc_dm *D, int N, orc_dm_shell_view *v) {
   const arr_t *B = &D->abasis[N];
   v->basis_nstates = B->stcf.n; v->basis_stcf = B->stcf.a; v->basis_itype = B->itype.a;
   v->basis_nblocks = nblocks(B); v->basis_qn = B->qn.a; v->basis_bbe = B->bbe.a;
+  v->kept = A->kept.n ? A->kept.a : NULL;
+  v->wn = D->wn ? D->wn[N] : 0.0;
   return 0;
 }
 

@@ -73,9 +73,9 @@ typedef struct {
 
 int orc_run_twochqs(const orc_twochqs_params *p, orc_step_cb cb, void *user);
 
-/* DM-NRG on the OneChQS chain (NRG_main with calcdens 1, UpdateBefCut 0)
+/* DM-NRG on the OneChQS chain (NRG_main with calcdens 1, UpdateBefCut 0 or 1)
  * followed by the DM_NRG post-processing (src/DM_NRG/DM_NRG.cpp:28-531):
- * plain DM-NRG (UseCFS 0), spectral function of c_d. */
+ * plain DM-NRG (UseCFS 0), CFS (-C) or FDM (-F); spectral function of c_d. */
 typedef struct {
   double U, Gamma, ed, Lambda;
   int nsitesmax, ncut, disc_scheme;
@@ -84,6 +84,8 @@ typedef struct {
   double betabar;             /* DM_NRG -b */
   double twindow, broadtemp, bbroad;   /* -w, -B, -e (0 -> 2, 0.8, ln(Lambda)/2) */
   int Nw;                     /* -n: 0 -> CalcSpecDM_NRG_FixedOmegas, >0 -> CalcSpec_ManyOmegas */
+  int method;                 /* 0: DM-NRG (UseCFS 0); 1: CFS (-C); 2: FDM (-F) */
+  int update_bef_cut;         /* NRG run UpdateBefCut (src/Main/NRG_main.cpp:334-343,393-396) */
 } orc_dmnrg_params;
 
 typedef struct orc_dm orc_dm;
@@ -97,14 +99,25 @@ typedef struct {
   int op_nblocks; const int *op_map; const int *op_mbe; const double *op;   /* c_d blocks */
   int basis_nstates, basis_nblocks; const int *basis_stcf, *basis_itype;     /* Abasis_N */
   const double *basis_qn; const int *basis_bbe;
+  const int *kept;                                       /* Acut_N Kept flags (nstates) */
+  double wn;                                             /* FDM weight w_N (0 unless FDM) */
 } orc_dm_shell_view;
 
 orc_dm *orc_dm_run(const orc_dmnrg_params *p);
 void orc_dm_free(orc_dm *D);
 int orc_dm_info(const orc_dm *D, int *nshell_max, int *spec_nshell_max, double *temp);
 int orc_dm_shell(const orc_dm *D, int N, orc_dm_shell_view *v);
-/* DMNRG_SpecDens_M_STL(omegabar, N)  src/SpecFunc_allSTL.cpp:223-692 */
+/* Shell sum at omegabar: DMNRG_SpecDens_M_STL (src/SpecFunc_allSTL.cpp:223-692),
+ * CFS_SpecDens_M_STL (:702-1240) or FDM_SpecDens_M_STL (:1253-2354) by method;
+ * the last shell is CalcSpecCosti_T_N (src/SpecFuncClass.cpp:417-662). */
 double orc_dm_spec_shell(const orc_dm *D, int N, double omegabar);
+/* CSpecFunction::CalcSpecDM_NRG(omega) src/SpecFuncClass.cpp:1827-1885 (CFS/FDM):
+ * sum over the shell window [N_w-8, N_w+14) of A_N(omega/D_N)/D_N */
+double orc_dm_spec_omega(const orc_dm *D, double omega);
+/* CSpecFunction::CalcNorm src/SpecFuncClass.cpp:1887-1930 (CFS/FDM; NaN for DM-NRG) */
+double orc_dm_norm(const orc_dm *D);
+/* Z_tot of the FDM weights (src/DM_NRG/DM_NRG.cpp:203-268; 0 unless FDM) */
+double orc_dm_ztot(const orc_dm *D);
 int orc_dm_spectrum(const orc_dm *D, const double **omega, const double **rho);
 
 /* ---- kernel-level restatements ---- */

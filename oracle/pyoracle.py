@@ -201,6 +201,7 @@ class DmParams(ctypes.Structure):
         ("disc_scheme", ctypes.c_int), ("myeps", ctypes.c_double), ("betabar_nrg", ctypes.c_double),
         ("Mtemp", ctypes.c_int), ("betabar", ctypes.c_double), ("twindow", ctypes.c_double),
         ("broadtemp", ctypes.c_double), ("bbroad", ctypes.c_double), ("Nw", ctypes.c_int),
+        ("method", ctypes.c_int), ("update_bef_cut", ctypes.c_int),
     ]
 
 
@@ -216,6 +217,7 @@ class DmShellView(ctypes.Structure):
         ("op_nblocks", ctypes.c_int), ("op_map", _PI), ("op_mbe", _PI), ("op", _PD),
         ("basis_nstates", ctypes.c_int), ("basis_nblocks", ctypes.c_int),
         ("basis_stcf", _PI), ("basis_itype", _PI), ("basis_qn", _PD), ("basis_bbe", _PI),
+        ("kept", _PI), ("wn", ctypes.c_double),
     ]
 
 
@@ -226,10 +228,12 @@ def _arr(p, n, dt=np.float64):
 
 
 class DmRun:
-    """DM-NRG restatement on a OneChQS chain: see orc_dm_run."""
+    """DM-NRG restatement on a OneChQS chain: see orc_dm_run.
+    method 0: DM-NRG, 1: CFS, 2: FDM; update_bef_cut: the NRG run's UpdateBefCut."""
 
     def __init__(self, U, Gamma, ed, Lambda, nsitesmax, ncut, Mtemp, betabar, disc_scheme=0,
-                 twindow=0.0, broadtemp=0.0, bbroad=0.0, Nw=0, myeps=0.0, betabar_nrg=0.0):
+                 twindow=0.0, broadtemp=0.0, bbroad=0.0, Nw=0, myeps=0.0, betabar_nrg=0.0,
+                 method=0, update_bef_cut=0):
         L = load()
         L.orc_dm_run.restype = ctypes.c_void_p
         L.orc_dm_run.argtypes = [ctypes.POINTER(DmParams)]
@@ -239,13 +243,20 @@ class DmRun:
         L.orc_dm_spec_shell.restype = ctypes.c_double
         L.orc_dm_spec_shell.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double]
         L.orc_dm_spectrum.argtypes = [ctypes.c_void_p, ctypes.POINTER(_PD), ctypes.POINTER(_PD)]
+        for f in (L.orc_dm_spec_omega, L.orc_dm_norm, L.orc_dm_ztot):
+            f.restype = ctypes.c_double
+        L.orc_dm_spec_omega.argtypes = [ctypes.c_void_p, ctypes.c_double]
+        L.orc_dm_norm.argtypes = [ctypes.c_void_p]
+        L.orc_dm_ztot.argtypes = [ctypes.c_void_p]
         p = DmParams(U, Gamma, ed, Lambda, nsitesmax, ncut, disc_scheme, myeps, betabar_nrg, Mtemp, betabar,
-                     twindow, broadtemp, bbroad, Nw)
+                     twindow, broadtemp, bbroad, Nw, method, update_bef_cut)
+        self.method = method
         self._L = L
         self.h = L.orc_dm_run(ctypes.byref(p))
         a, b, t = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
         self.nsh = L.orc_dm_info(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(t))
         self.nshell_max, self.spec_nshell_max, self.temp = a.value, b.value, t.value
+        self.ztot = L.orc_dm_ztot(self.h)
 
     def close(self):
         if self.h:
@@ -293,10 +304,18 @@ class DmRun:
                     basis_stcf=_arr(v.basis_stcf, v.basis_nstates, np.int64),
                     basis_itype=_arr(v.basis_itype, v.basis_nstates, np.int64),
                     basis_qn=_arr(v.basis_qn, bnb * nq).reshape(bnb, nq),
-                    basis_bbe=_arr(v.basis_bbe, 2 * bnb, np.int64).reshape(bnb, 2))
+                    basis_bbe=_arr(v.basis_bbe, 2 * bnb, np.int64).reshape(bnb, 2),
+                    kept=_arr(v.kept, v.nstates, np.int64) if v.kept else np.ones(v.nstates, np.int64),
+                    wn=v.wn)
 
     def spec_shell(self, N, w):
         return self._L.orc_dm_spec_shell(self.h, N, w)
+
+    def spec_omega(self, w):
+        return self._L.orc_dm_spec_omega(self.h, w)
+
+    def norm(self):
+        return self._L.orc_dm_norm(self.h)
 
     def spectrum(self):
         po, pr = _PD(), _PD()
