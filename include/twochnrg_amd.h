@@ -204,17 +204,21 @@ typedef struct {
   double betabar;                  /* DM_NRG -b */
   double twindow, broadtemp, bbroad;   /* -w -B -e (0 -> 2, 0.8, ln(Lambda)/2) */
   int Nw;                          /* -n: 0 -> fixed omegas (even/odd shells), >0 -> Nw omegas per shell */
+  int method;                      /* 0: DM-NRG (UseCFS 0); 1: CFS (-C); 2: FDM (-F) */
+  int update_bef_cut;              /* NRG run UpdateBefCut (src/Main/NRG_main.cpp:334-343,393-396);
+                                      CFS and FDM need 1: the discarded states and their operator
+                                      elements are kept per shell */
 } nrg_dmnrg_params;
 
 typedef struct nrg_dm nrg_dm;
 
 /*
- * DM_NRG main (src/DM_NRG/DM_NRG.cpp:28-531; QS symmetry, plain DM-NRG, UseCFS 0)
+ * DM_NRG main (src/DM_NRG/DM_NRG.cpp:28-531; QS symmetry; DM-NRG, CFS or FDM)
  * on the device: the OneChQS chain (as nrg_run_onechqs with calcdens 1) keeping
  * what SaveArrays writes per shell (src/NRGcodehandlerClass.cpp:887-933), then
- * rho_Nmax (DM_NRG_SetRhoNmax src/DM_NRG/DM_NRG_SetRhoNmax.cpp:9-126) and the
- * backward sweep (RotateMatrix src/RotateMatrix.cpp:17-99 +
- * DM_NRG_CalcRhoN_withSU2 src/DM_NRG/DM_NRG_CalcRhoN.cpp:252-449).  The handle
+ * rho_Nmax (DM_NRG_SetRhoNmax src/DM_NRG/DM_NRG_SetRhoNmax.cpp:9-126; FDM
+ * weights :203-297) and the backward sweep (RotateMatrix src/RotateMatrix.cpp:17-99
+ * + DM_NRG_CalcRhoN_withSU2 src/DM_NRG/DM_NRG_CalcRhoN.cpp:252-449).  The handle
  * keeps device memory until nrg_dm_destroy; ctx must outlive it.
  */
 int nrg_dm_create(nrg_ctx *ctx, const nrg_dmnrg_params *p, nrg_dm **out);
@@ -226,9 +230,22 @@ int nrg_dm_timing(const nrg_dm *dm, double *ms4);
 /* rho_N blocks (k_b x k_b row-major, Acut_N block order) and its trace
  * sum (2S+1) rho_ii (the value CalcRhoN prints); out may be NULL. */
 int nrg_dm_rho(nrg_dm *dm, int N, double *out, int64_t cap, int *nblocks, double *trace);
-/* CSpecFunction::DMNRG_SpecDens_M_STL(omegabar, N) (src/SpecFunc_allSTL.cpp:223-692;
- * last shell CalcSpecCosti_T_N src/SpecFuncClass.cpp:417-662) for n (shell, omegabar) pairs. */
+/* The shell sum at omegabar for n (shell, omegabar) pairs: DMNRG_SpecDens_M_STL
+ * (src/SpecFunc_allSTL.cpp:223-692), CFS_SpecDens_M_STL (:702-1240) or
+ * FDM_SpecDens_M_STL (:1253-2354) by method; the last shell is
+ * CalcSpecCosti_T_N (src/SpecFuncClass.cpp:417-662). */
 int nrg_dm_spec(nrg_dm *dm, int n, const int *shells, const double *omegabar, double *out);
+/* CSpecFunction::CalcSpecDM_NRG(omega) (src/SpecFuncClass.cpp:1827-1885) for n
+ * omegas: the sum over the shell window [N_w-8, N_w+14) of A_N(omega/D_N)/D_N.
+ * CFS and FDM only (NRG_ERR_ARG for DM-NRG, whose reference path for it is the
+ * unused non-STL DMNRG_SpecDens_M). */
+int nrg_dm_spec_omega(nrg_dm *dm, int n, const double *omega, double *out);
+/* CSpecFunction::CalcNorm (src/SpecFuncClass.cpp:1887-1930), the "NormCFS" that
+ * DM_NRG_CalcSpecFunc_ij prints (src/DM_NRG/DM_NRG_CalcSpecFuncs.cpp:77-79); CFS/FDM. */
+int nrg_dm_norm(nrg_dm *dm, double *norm);
+/* FDM weights w_N (N = 0 .. NshellMax) and Z_tot (src/DM_NRG/DM_NRG.cpp:203-271);
+ * zeros unless FDM.  wn may be NULL. */
+int nrg_dm_weights(const nrg_dm *dm, double *wn, int cap, double *ztot);
 /* rho_0_0 on the grid of DM_NRG_CalcSpecFunc_ij (src/DM_NRG/DM_NRG_CalcSpecFuncs.cpp:73-84):
  * CalcSpecDM_NRG_FixedOmegas (src/SpecFuncClass.cpp:1932-2233) or CalcSpec_ManyOmegas
  * (:2239-2341) -- the two columns SaveOmegaRhow writes.  *n receives the count;

@@ -19,7 +19,7 @@ EXPORTS = [
     "nrg_rotate_ops", "nrg_run_onechqs", "nrg_run_twochqs", "nrg_set_timing", "nrg_get_stats", "nrg_reset_stats",
     "nrg_set_eig_impl", "nrg_set_plain", "nrg_eig2_stages",
     "nrg_dm_create", "nrg_dm_destroy", "nrg_dm_info", "nrg_dm_timing", "nrg_dm_rho", "nrg_dm_spec",
-    "nrg_dm_spectrum", "nrg_run_onechqs_multi",
+    "nrg_dm_spectrum", "nrg_run_onechqs_multi", "nrg_dm_spec_omega", "nrg_dm_norm", "nrg_dm_weights",
 ]
 
 KSTAT_NAMES = ["eig", "hn", "fbasis", "gemm", "tdb_symv"]
@@ -71,6 +71,7 @@ class DmParams(ctypes.Structure):
         ("disc_scheme", ctypes.c_int), ("myeps", ctypes.c_double), ("betabar_nrg", ctypes.c_double),
         ("Mtemp", ctypes.c_int), ("betabar", ctypes.c_double), ("twindow", ctypes.c_double),
         ("broadtemp", ctypes.c_double), ("bbroad", ctypes.c_double), ("Nw", ctypes.c_int),
+        ("method", ctypes.c_int), ("update_bef_cut", ctypes.c_int),
     ]
 
 
@@ -131,6 +132,9 @@ def lib():
     L.nrg_dm_rho.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, ctypes.c_int64, _ip, _dp]
     L.nrg_dm_spec.argtypes = [ctypes.c_void_p, ctypes.c_int, _ip, _dp, _dp]
     L.nrg_dm_spectrum.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_int, _ip]
+    L.nrg_dm_spec_omega.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp]
+    L.nrg_dm_norm.argtypes = [ctypes.c_void_p, _dp]
+    L.nrg_dm_weights.argtypes = [ctypes.c_void_p, _dp, ctypes.c_int, _dp]
     _lib = L
     return L
 
@@ -308,10 +312,13 @@ class Context:
         return steps, ms.value
 
     def dm(self, U=0.5, Gamma=0.02, ed=-0.25, Lambda=2.5, nsitesmax=40, ncut=400, Mtemp=38, betabar=1.0,
-           disc_scheme=0, twindow=0.0, broadtemp=0.0, bbroad=0.0, Nw=0, myeps=0.0, betabar_nrg=0.0):
-        """DM-NRG handle (nrg_dm_create): forward chain + rho sweep on the device."""
+           disc_scheme=0, twindow=0.0, broadtemp=0.0, bbroad=0.0, Nw=0, myeps=0.0, betabar_nrg=0.0,
+           method=0, update_bef_cut=0):
+        """DM-NRG handle (nrg_dm_create): forward chain + rho sweep on the device.
+        method 0: DM-NRG, 1: CFS (DM_NRG -C), 2: FDM (DM_NRG -F); CFS/FDM want
+        update_bef_cut=1 (the chain keeps the discarded states per shell)."""
         return DMNRG(self, DmParams(U, Gamma, ed, Lambda, nsitesmax, ncut, disc_scheme, myeps, betabar_nrg,
-                                    Mtemp, betabar, twindow, broadtemp, bbroad, Nw))
+                                    Mtemp, betabar, twindow, broadtemp, bbroad, Nw, method, update_bef_cut))
 
     def run_twochqs(self, U=0.4, Gamma1=0.03, Gamma2=0.03, ed=-0.2, Lambda=3.0, nsitesmax=3, ncut=3000,
                     betabar=0.727, record_blocks=False, keep_steps=True, on_step=None):
@@ -421,6 +428,26 @@ class DMNRG:
         out = np.zeros(len(sh))
         _check(lib().nrg_dm_spec(self.h, len(sh), _ptr(sh, _ip), _ptr(w), _ptr(out)))
         return out
+
+    def spec_omega(self, omega):
+        """CalcSpecDM_NRG(omega): the shell-window sum (CFS/FDM)."""
+        w = np.ascontiguousarray(np.atleast_1d(omega), np.float64)
+        out = np.zeros(len(w))
+        _check(lib().nrg_dm_spec_omega(self.h, len(w), _ptr(w), _ptr(out)))
+        return out
+
+    def norm(self):
+        """CalcNorm (CFS/FDM)."""
+        v = ctypes.c_double()
+        _check(lib().nrg_dm_norm(self.h, ctypes.byref(v)))
+        return v.value
+
+    def weights(self):
+        """FDM w_N (N = 0 .. nshell_max) and Z_tot."""
+        wn = np.zeros(self.nshell_max + 1)
+        z = ctypes.c_double()
+        _check(lib().nrg_dm_weights(self.h, _ptr(wn), len(wn), ctypes.byref(z)))
+        return wn, z.value
 
     def spectrum(self):
         n = ctypes.c_int()

@@ -1,16 +1,20 @@
 // nrg_dm.cpp -- DM-NRG on the device (DM_NRG main, src/DM_NRG/DM_NRG.cpp:28-531,
-// QS symmetry, plain DM-NRG): the forward OneChQS chain keeps each shell's
-// kept eigenvector rows, c_d blocks and child map; rho_Nmax = e^{-betabar E}/Z
-// (DM_NRG_SetRhoNmax src/DM_NRG/DM_NRG_SetRhoNmax.cpp:9-126); the backward sweep
-// rotates rho_{N+1} into the pre-cut basis with two batched GEMMs per shell
-// (RotateMatrix src/RotateMatrix.cpp:17-99) and traces out the site as CG-
-// weighted sums of child segments (DM_NRG_CalcRhoN_withSU2
-// src/DM_NRG/DM_NRG_CalcRhoN.cpp:252-449); the shell spectral sums
-// (DMNRG_SpecDens_M_STL src/SpecFunc_allSTL.cpp:223-692, last shell
-// CalcSpecCosti_T_N src/SpecFuncClass.cpp:417-662) become pole weights from
-// two GEMMs per operator block pair and one broadening reduction over every
-// (shell, omega); the omega grids follow CalcSpecDM_NRG_FixedOmegas
-// (src/SpecFuncClass.cpp:1932-2233) and CalcSpec_ManyOmegas (:2239-2341).
+// QS symmetry; DM-NRG, CFS or FDM): the forward OneChQS chain keeps each shell's
+// eigenvector rows (kept, or every state with UpdateBefCut 1), c_d blocks and
+// child map; rho_Nmax = e^{-betabar E}/Z (DM_NRG_SetRhoNmax
+// src/DM_NRG/DM_NRG_SetRhoNmax.cpp:9-126; FDM: Z_tot of the shell weights,
+// src/DM_NRG/DM_NRG.cpp:203-297); the backward sweep rotates rho_{N+1} into the
+// pre-cut basis with two batched GEMMs per shell (RotateMatrix
+// src/RotateMatrix.cpp:17-99) and traces out the site as CG-weighted sums of
+// child segments (DM_NRG_CalcRhoN_withSU2 src/DM_NRG/DM_NRG_CalcRhoN.cpp:252-449,
+// FDM discarded diagonal :393-398); the shell spectral sums (DMNRG_SpecDens_M_STL
+// src/SpecFunc_allSTL.cpp:223-692, CFS_SpecDens_M_STL :702-1240,
+// FDM_SpecDens_M_STL :1253-2354, last shell CalcSpecCosti_T_N
+// src/SpecFuncClass.cpp:417-662) become pole weights from two GEMMs per operator
+// block pair and one broadening reduction over every (shell, omega); CFS/FDM
+// sum a window of shells per omega (CalcSpecDM_NRG src/SpecFuncClass.cpp:1827-1885);
+// the omega grids follow CalcSpecDM_NRG_FixedOmegas (src/SpecFuncClass.cpp:1932-2233)
+// and CalcSpec_ManyOmegas (:2239-2341).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -25,7 +29,7 @@
 #include "nrg_internal.h"
 
 namespace nrg {
-int run_onechqs_dm(nrg_ctx* c, const nrg_onechqs_params& p, std::vector<DmShell>& dm);
+int run_onechqs_dm(nrg_ctx* c, const nrg_onechqs_params& p, std::vector<DmShell>& dm, int befcut);
 double cgordan(double J1, double M1, double J2, double M2, double J, double M);  // nrg_driver.cpp
 }
 
@@ -34,7 +38,10 @@ struct nrg_dm {
   nrg_dmnrg_params p{};
   std::vector<DmShell> sh;
   int NM = 0, specNM = 0;
+  int method = 0;                              // spec1.UseCFS: 0 DM-NRG, 1 CFS, 2 FDM
   double temp = 0.0, betabar_spec = 0.0;
+  double ztot = 0.0;                           // FDM Z_tot
+  std::vector<double> wn;                      // FDM w_N
   std::vector<std::vector<int64_t>> rho_off;   // per shell and block: k x k block in dmR<N>
   std::vector<int64_t> rho_len;
   // pole arena (filled once, all shells): W, dE and per-shell ranges
@@ -56,6 +63,22 @@ double partition_func(const DmShell& s, double bb) {   // src/NRGarrayClass.cpp:
   for (int b = 0; b < s.nblocks(); b++)
     for (int i = s.bbe[2 * b]; i <= s.bbe[2 * b + 1]; i++) z += (2.0 * s.qn[2 * b + 1] + 1.0) * std::exp(-bb * s.en[i]);
   return z;
+}
+double partition_func_disc(const DmShell& s, double bb) {   // src/NRGarrayClass.cpp:1032-1058
+  if (s.kept.empty()) return -1.0;
+  double z = 0.0;
+  for (int b = 0; b < s.nblocks(); b++)
+    for (int i = s.bbe[2 * b]; i <= s.bbe[2 * b + 1]; i++)
+      if (!s.kept[i]) z += (2.0 * s.qn[2 * b + 1] + 1.0) * std::exp(-bb * s.en[i]);
+  return z;
+}
+int kept_prefix(const DmShell& s, int b, int& kc) {   // Kept states lead each energy-ordered block
+  const int i0 = s.bbe[2 * b], k = s.bsize(b);
+  kc = 0;
+  while (kc < k && (s.kept.empty() || s.kept[i0 + kc])) kc++;
+  for (int p = kc; p < k; p++)
+    if (s.kept[i0 + p]) return fail(NRG_ERR_STATE, "DM-NRG: kept states are not a prefix of their block");
+  return NRG_OK;
 }
 double site_S(int t) { return (t == 1 || t == 2) ? 0.5 : 0.0; }   // OneChQS site: types 0..3 = 0, up, down, updown
 
@@ -88,8 +111,9 @@ E2Gemm gemm(int m, int n, int k, int64_t a_off, int64_t as0, int64_t as1, int64_
   return g;
 }
 
-// rho_{N} from rho_{N+1}: rotation into the pre-cut basis, then the child trace-out
-int sweep_shell(nrg_dm* D, int N) {
+// rho_{N} from rho_{N+1}: rotation into the pre-cut basis, then the child trace-out;
+// fdm: e^{-E/TempBar}/zdisc on the diagonal of the discarded states (zdisc != 0)
+int sweep_shell(nrg_dm* D, int N, bool fdm, double TempBar, double zdisc) {
   nrg_ctx* c = D->c;
   const DmShell& a = D->sh[N];       // Acut_N
   const DmShell& b = D->sh[N + 1];   // Acut_{N+1} with Abasis_{N+1}
@@ -126,7 +150,11 @@ int sweep_shell(nrg_dm* D, int N) {
   for (int ab = 0; ab < nbas; ab++)
     for (int i = b.bas_bbe[2 * ab]; i <= b.bas_bbe[2 * ab + 1]; i++) st_blk[i] = ab;
   std::vector<std::vector<int>> children(a.en.size());
-  for (size_t cidx = 0; cidx < b.stcf.size(); cidx++) children[b.stcf[cidx]].push_back((int)cidx);
+  for (size_t cidx = 0; cidx < b.stcf.size(); cidx++) {
+    if (b.stcf[cidx] < 0 || b.stcf[cidx] >= (int)a.en.size())
+      return fail(NRG_ERR_STATE, "DM-NRG: StCameFrom outside the parent shell");
+    children[b.stcf[cidx]].push_back((int)cidx);
+  }
   // segments: children of a block's first state; the child of state p of the
   // block with the same type is the segment start + p
   std::vector<DmGBlk> gb;
@@ -135,17 +163,23 @@ int sweep_shell(nrg_dm* D, int N) {
   const int nba = a.nblocks();
   D->rho_off[N].assign(nba, 0);
   int64_t otot = 0;
+  const bool fdm_diag = fdm && std::fabs(zdisc - 0.0) > 1e-20;
   for (int B = 0; B < nba; B++) {
     const int i0 = a.bbe[2 * B], k = a.bsize(B);
     const double Sold = a.qn[2 * B + 1];
+    int kc;
+    if ((rc = kept_prefix(a, B, kc))) return rc;
+    for (int p = 0; p < k; p++)   // the kept states are the parents of the N+1 basis
+      if (children[i0 + p].empty() != (p >= kc))
+        return fail(NRG_ERR_STATE, "DM-NRG: child states do not match the kept states");
     DmGBlk G{};
-    G.out_off = otot; G.k = k; G.seg_beg = (int)segs.size();
+    G.out_off = otot; G.e_off = i0; G.k = k; G.kc = kc; G.kd = fdm_diag ? kc : k; G.seg_beg = (int)segs.size();
     D->rho_off[N][B] = otot;
     otot += (int64_t)k * k;
     kmax = std::max(kmax, k);
-    for (int c0 : children[i0]) {
+    if (kc > 0) for (int c0 : children[i0]) {
       const int t = b.itype[c0], ab = st_blk[c0];
-      for (int p = 1; p < k; p++) {
+      for (int p = 1; p < kc; p++) {
         const int cp = c0 + p;
         if (cp >= (int)b.stcf.size() || b.stcf[cp] != i0 + p || b.itype[cp] != t || st_blk[cp] != ab)
           return fail(NRG_ERR_STATE, "DM-NRG: child states of a block are not one contiguous segment per type");
@@ -174,9 +208,12 @@ int sweep_shell(nrg_dm* D, int N) {
   if ((rc = ob.reserve(sizeof(double) * (size_t)(otot + 1)))) return rc;
   DmGBlk* dgb;
   DmSeg* dsg;
+  double* den;
   if ((rc = upload(c, "dm_gb", gb, &dgb))) return rc;
   if ((rc = upload(c, "dm_sg", segs, &dsg))) return rc;
-  HIPCHK(dm_launch_gather(dgb, (int)gb.size(), kmax, dsg, (double*)rb.p, (double*)ob.p, c->stream));
+  if ((rc = upload(c, "dm_gen", a.en, &den))) return rc;
+  HIPCHK(dm_launch_gather(dgb, (int)gb.size(), kmax, dsg, (double*)rb.p, den, fdm_diag ? 1.0 / TempBar : 0.0,
+                          fdm_diag ? zdisc : 1.0, (double*)ob.p, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   return NRG_OK;
 }
@@ -190,33 +227,27 @@ double op_cgfactor(double S1, double S2) {
   return cg;
 }
 
-// pole weights and energies of every shell 0 .. specNM-1 (omega independent)
+// pole weights (Wp for omega >= 0, Wn for omega < 0) and energies of every shell
+// 0 .. specNM-1 (omega independent).  CFS/FDM: a block pair with no kept state
+// on either side, or no discarded state on either side, has no poles
+// (src/SpecFunc_allSTL.cpp:1375-1381, :824-830).
 int build_poles(nrg_dm* D) {
   nrg_ctx* c = D->c;
   int rc;
   const int ns = D->specNM;   // shells with spectral weight: 0 .. specNM-1
+  const double bb = (std::fabs(D->temp) < 1e-20) ? 1.0e10 : D->betabar_spec;
+  // per shell: the pairs that carry poles, with their kept prefixes
+  std::vector<std::vector<DmPair>> shell_pairs(ns);
+  std::vector<std::vector<std::pair<int, int>>> shell_blks(ns);   // (b1, b2) of each pair
   D->pole_beg.assign(ns, 0);
   D->pole_end.assign(ns, 0);
   int64_t ptot = 0;
   for (int N = 0; N < ns; N++) {
     const DmShell& s = D->sh[N];
-    D->pole_beg[N] = ptot;
-    for (size_t q = 0; q + 1 < s.op_map.size(); q += 2) ptot += (int64_t)s.bsize(s.op_map[q]) * s.bsize(s.op_map[q + 1]);
-    D->pole_end[N] = ptot;
-  }
-  DBuf& wb = c->buf("dm_W");
-  DBuf& eb = c->buf("dm_dE");
-  if ((rc = wb.reserve(sizeof(double) * (size_t)(ptot + 1)))) return rc;
-  if ((rc = eb.reserve(sizeof(double) * (size_t)(ptot + 1)))) return rc;
-  for (int N = 0; N < ns; N++) {
-    const DmShell& s = D->sh[N];
     const bool costi = (N == D->specNM - 1);
-    const double bb = (std::fabs(D->temp) < 1e-20) ? 1.0e10 : D->betabar_spec;
+    const int form = costi ? DM_FORM_COSTI : D->method;
     const double invZ = costi ? 1.0 / partition_func(s, bb) : 0.0;
-    std::vector<DmPair> pairs;
-    std::vector<E2Gemm> g1, g2;
-    int64_t prod = 0, pole = D->pole_beg[N];
-    int kkmax = 1;
+    D->pole_beg[N] = ptot;
     for (size_t q = 0; q + 1 < s.op_map.size(); q += 2) {
       const int b1 = s.op_map[q], b2 = s.op_map[q + 1];
       const int k1 = s.bsize(b1), k2 = s.bsize(b2);
@@ -225,22 +256,50 @@ int build_poles(nrg_dm* D) {
       P.k1 = k1; P.k2 = k2;
       P.e1_off = s.bbe[2 * b1]; P.e2_off = s.bbe[2 * b2];
       P.cg = op_cgfactor(s.qn[2 * b1 + 1], s.qn[2 * b2 + 1]);
-      P.pole_off = pole;
-      P.costi = costi ? 1 : 0;
+      P.form = form;
       P.bb = bb; P.invZ = invZ;
-      if (!costi) {
-        P.p1_off = prod;
-        P.p2_off = prod + (int64_t)k1 * k2;
-        // rho1 A (k1 x k2) and A rho2 (k1 x k2)
-        g1.push_back(gemm(k1, k2, k1, D->rho_off[N][b1], k1, 1, P.a_off, k2, 1, P.p1_off, k2));
-        g2.push_back(gemm(k1, k2, k2, P.a_off, k2, 1, D->rho_off[N][b2], k2, 1, P.p2_off, k2));
-        prod += 2 * (int64_t)k1 * k2;
+      if ((rc = kept_prefix(s, b1, P.kc1))) return rc;
+      if ((rc = kept_prefix(s, b2, P.kc2))) return rc;
+      if (form == DM_FORM_CFS || form == DM_FORM_FDM) {
+        const bool nokept = P.kc1 == 0 && P.kc2 == 0, nodisc = P.kc1 == k1 && P.kc2 == k2;
+        if (nokept || nodisc) continue;
       }
-      pole += (int64_t)k1 * k2;
-      kkmax = std::max(kkmax, k1 * k2);
-      pairs.push_back(P);
+      P.pole_off = ptot;
+      ptot += (int64_t)k1 * k2;
+      shell_pairs[N].push_back(P);
+      shell_blks[N].emplace_back(b1, b2);
     }
+    D->pole_end[N] = ptot;
+  }
+  DBuf& wpb = c->buf("dm_Wp");
+  DBuf& wnb = c->buf("dm_Wn");
+  DBuf& eb = c->buf("dm_dE");
+  if ((rc = wpb.reserve(sizeof(double) * (size_t)(ptot + 1)))) return rc;
+  if ((rc = wnb.reserve(sizeof(double) * (size_t)(ptot + 1)))) return rc;
+  if ((rc = eb.reserve(sizeof(double) * (size_t)(ptot + 1)))) return rc;
+  for (int N = 0; N < ns; N++) {
+    const DmShell& s = D->sh[N];
+    std::vector<DmPair>& pairs = shell_pairs[N];
     if (pairs.empty()) continue;
+    const bool costi = (N == D->specNM - 1);
+    std::vector<E2Gemm> g1, g2;
+    int64_t prod = 0;
+    int kkmax = 1;
+    // rho1 A (k1 x k2) and A rho2 (k1 x k2) over the whole blocks: rho has no
+    // kept-discarded elements, so the products of the reference's kept/discarded
+    // sub-blocks are sub-blocks of these
+    for (size_t q = 0; q < pairs.size(); q++) {
+      DmPair& P = pairs[q];
+      if (!costi) {
+        const int b1 = shell_blks[N][q].first, b2 = shell_blks[N][q].second;
+        P.p1_off = prod;
+        P.p2_off = prod + (int64_t)P.k1 * P.k2;
+        prod += 2 * (int64_t)P.k1 * P.k2;
+        g1.push_back(gemm(P.k1, P.k2, P.k1, D->rho_off[N][b1], P.k1, 1, P.a_off, P.k2, 1, P.p1_off, P.k2));
+        g2.push_back(gemm(P.k1, P.k2, P.k2, P.a_off, P.k2, 1, D->rho_off[N][b2], P.k2, 1, P.p2_off, P.k2));
+      }
+      kkmax = std::max(kkmax, P.k1 * P.k2);
+    }
     DBuf& pb = c->buf("dm_prod");
     if ((rc = pb.reserve(sizeof(double) * (size_t)(prod + 1)))) return rc;
     double* dO = (double*)c->buf(s.opbuf).p;
@@ -253,15 +312,21 @@ int build_poles(nrg_dm* D) {
     DmPair* dpairs;
     if ((rc = upload(c, "dm_en", s.en, &dEn))) return rc;
     if ((rc = upload(c, "dm_pairs", pairs, &dpairs))) return rc;
-    HIPCHK(dm_launch_weights(dpairs, (int)pairs.size(), kkmax, dO, (double*)pb.p, dEn, (double*)wb.p, (double*)eb.p,
-                             c->stream));
+    HIPCHK(dm_launch_weights(dpairs, (int)pairs.size(), kkmax, dO, (double*)pb.p, dEn, (double*)wpb.p,
+                             (double*)wnb.p, (double*)eb.p, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
   }
   D->poles_ready = true;
   return NRG_OK;
 }
 
-int spec_tasks(nrg_dm* D, int n, const int* shells, const double* w, double* out) {
+int bd_kind(const nrg_dm* D) {   // BDelta / BDeltaTemp by method (src/DM_NRG/DM_NRG.cpp:474-482)
+  return D->method == 2 ? DM_BD_LOGGAUSS_GAUSS : (D->method == 1 ? DM_BD_BROAD_ANDERS : DM_BD_BROAD_LORENTZ);
+}
+
+// shell sums at (shell, omegabar); norm: CalcNorm (broadening 1, Wp + Wn below
+// the last shell, the Costi sum once at the last)
+int spec_tasks(nrg_dm* D, int n, const int* shells, const double* w, double* out, bool norm = false) {
   nrg_ctx* c = D->c;
   int rc;
   if (!D->poles_ready) {
@@ -276,18 +341,23 @@ int spec_tasks(nrg_dm* D, int n, const int* shells, const double* w, double* out
   int cb = 0, maxchunk = 0;
   for (int i = 0; i < n; i++) {
     const int N = shells[i];
-    if (N < 0 || N > D->specNM - 1) { out[i] = 0.0; continue; }
-    DmTask T{};
-    T.pole_beg = D->pole_beg[N];
-    T.pole_end = D->pole_end[N];
-    T.w = w[i];
-    T.twin = D->p.twindow * D->temp / calc_dn(D->p.Lambda, N);
-    T.nchunk = (int)((T.pole_end - T.pole_beg + BCH - 1) / BCH);
-    T.chunk_beg = cb;
-    cb += T.nchunk;
-    maxchunk = std::max(maxchunk, T.nchunk);
-    tasks.push_back(T);
-    slot.push_back(i);
+    out[i] = 0.0;
+    if (N < 0 || N > D->specNM - 1) continue;
+    const int nsign = (norm && N < D->specNM - 1) ? 2 : 1;
+    for (int sg = 0; sg < nsign; sg++) {
+      DmTask T{};
+      T.pole_beg = D->pole_beg[N];
+      T.pole_end = D->pole_end[N];
+      T.w = norm ? (sg == 0 ? 1.0 : -1.0) : w[i];
+      T.twin = D->p.twindow * D->temp / calc_dn(D->p.Lambda, N);
+      T.norm = norm ? 1 : 0;
+      T.nchunk = (int)((T.pole_end - T.pole_beg + BCH - 1) / BCH);
+      T.chunk_beg = cb;
+      cb += T.nchunk;
+      maxchunk = std::max(maxchunk, T.nchunk);
+      tasks.push_back(T);
+      slot.push_back(i);
+    }
   }
   if (tasks.empty()) return NRG_OK;
   DBuf& pb = c->buf("dm_partial");
@@ -296,13 +366,46 @@ int spec_tasks(nrg_dm* D, int n, const int* shells, const double* w, double* out
   if ((rc = ob.reserve(sizeof(double) * (size_t)(tasks.size() + 1)))) return rc;
   DmTask* dt;
   if ((rc = upload(c, "dm_tasks", tasks, &dt))) return rc;
-  HIPCHK(dm_launch_broaden(dt, (int)tasks.size(), maxchunk, (double*)c->buf("dm_W").p, (double*)c->buf("dm_dE").p,
-                           D->p.bbroad, D->p.broadtemp, (double*)pb.p, (double*)ob.p, c->stream));
+  HIPCHK(dm_launch_broaden(dt, (int)tasks.size(), maxchunk, (double*)c->buf("dm_Wp").p, (double*)c->buf("dm_Wn").p,
+                           (double*)c->buf("dm_dE").p, bd_kind(D), D->p.bbroad, D->p.broadtemp, (double*)pb.p,
+                           (double*)ob.p, c->stream));
   std::vector<double> res(tasks.size());
   HIPCHK(hipMemcpyAsync(res.data(), ob.p, sizeof(double) * tasks.size(), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
-  for (size_t q = 0; q < tasks.size(); q++) out[slot[q]] = res[q];
+  for (size_t q = 0; q < tasks.size(); q++) out[slot[q]] += res[q];
   D->ms_broaden += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return NRG_OK;
+}
+
+// CalcNfromOmega  src/SpecFuncClass.cpp:41-49 (z = 1)
+int n_from_omega(double L, double w) {
+  const double HLF = 0.5 * (1.0 + 1.0 / L);
+  int i = (int)(2.0 * std::log(2.0 * std::fabs(w) / HLF) / std::log(L));
+  i = 1 - i;
+  return i < 0 ? 0 : i;
+}
+
+// CalcSpecDM_NRG(omega) for n omegas: every (omega, shell in its window) is one
+// task of a single broadening launch; the window sums run on the host in the
+// reference's shell order
+int spec_omegas(nrg_dm* D, int n, const double* omega, double* out) {
+  std::vector<int> sh, owner;
+  std::vector<double> wb;
+  for (int i = 0; i < n; i++) {
+    const int No = n_from_omega(D->p.Lambda, omega[i]);
+    const int N0 = (No - 8 > 0) ? No - 8 : 0;
+    const int N1 = (No + 14 < D->specNM) ? No + 14 : D->specNM;
+    for (int N = N0; N < N1; N++) {
+      sh.push_back(N);
+      wb.push_back(omega[i] / calc_dn(D->p.Lambda, N));
+      owner.push_back(i);
+    }
+  }
+  std::vector<double> val(sh.size());
+  int rc;
+  if ((rc = spec_tasks(D, (int)sh.size(), sh.data(), wb.data(), val.data()))) return rc;
+  for (int i = 0; i < n; i++) out[i] = 0.0;
+  for (size_t q = 0; q < sh.size(); q++) out[owner[q]] += val[q] / calc_dn(D->p.Lambda, sh[q]);
   return NRG_OK;
 }
 
@@ -321,12 +424,15 @@ double lin_interp(const std::vector<double>& x, const std::vector<double>& y, do
 
 extern "C" int nrg_dm_create(nrg_ctx* c, const nrg_dmnrg_params* p, nrg_dm** out) {
   if (!c || !p || !out) return fail(NRG_ERR_ARG, "nrg_dm_create: NULL argument");
-  if (p->nsitesmax < 2 || p->ncut < 1 || p->Lambda <= 1.0 || p->betabar <= 0.0 || p->Mtemp < 1)
+  if (p->nsitesmax < 2 || p->ncut < 1 || p->Lambda <= 1.0 || p->betabar <= 0.0 || p->Mtemp < 1 ||
+      p->method < 0 || p->method > 2)
     return fail(NRG_ERR_ARG, "nrg_dm_create: bad parameters");
   HIPCHK(hipSetDevice(c->device));
   nrg_dm* D = new nrg_dm();
   D->c = c;
   D->p = *p;
+  D->method = p->method;
+  const bool fdm = (p->method == 2);
   if (D->p.twindow <= 0) D->p.twindow = 2.0;
   if (D->p.broadtemp <= 0) D->p.broadtemp = 0.8;
   if (D->p.bbroad <= 0) D->p.bbroad = 0.5 * std::log(p->Lambda);
@@ -335,13 +441,13 @@ extern "C" int nrg_dm_create(nrg_ctx* c, const nrg_dmnrg_params* p, nrg_dm** out
   q.ncut = p->ncut; q.calcdens = 1; q.disc_scheme = p->disc_scheme; q.myeps = p->myeps; q.betabar = p->betabar_nrg;
   int rc;
   auto t0 = std::chrono::steady_clock::now();
-  if ((rc = run_onechqs_dm(c, q, D->sh))) { delete D; return rc; }
+  if ((rc = run_onechqs_dm(c, q, D->sh, p->update_bef_cut ? 1 : 0))) { delete D; return rc; }
   auto t1 = std::chrono::steady_clock::now();
-  // temperature from Mtemp (src/DM_NRG/DM_NRG.cpp:102-118)
+  // temperature from Mtemp (src/DM_NRG/DM_NRG.cpp:102-118); FDM keeps every shell (:105)
   int Nsitesmax = p->nsitesmax;
   double DM;
   if (p->Mtemp < Nsitesmax - 1) {
-    Nsitesmax = p->Mtemp + 1;
+    if (!fdm) Nsitesmax = p->Mtemp + 1;
     DM = calc_dn(p->Lambda, p->Mtemp);
     D->temp = DM / p->betabar;
   } else {
@@ -354,19 +460,36 @@ extern "C" int nrg_dm_create(nrg_ctx* c, const nrg_dmnrg_params* p, nrg_dm** out
   const int NM = D->NM;
   D->rho_off.assign(NM + 1, std::vector<int64_t>());
   D->rho_len.assign(NM + 1, 0);
-  // rho_Nmax: diag e^{-E/TempBar}/Z_N, Z_N at Temp/D_{NM-1} (src/DM_NRG/DM_NRG.cpp:203-214)
+  // Z_N at Temp/D_{NM-1} (src/DM_NRG/DM_NRG.cpp:203-214); FDM: Z_tot = Z_N +
+  // sum_N Z_N^disc 4^{NM-N} and w_N (:221-271)
+  const double ZN = partition_func(D->sh[NM], 1.0 / (D->temp / calc_dn(p->Lambda, NM - 1)));
+  D->wn.assign(NM + 1, 0.0);
+  double ZTot = ZN;
+  std::vector<double> wraw(NM + 1, 0.0);
+  wraw[NM] = ZN;
+  for (int N = NM - 1; N >= 0; N--) {
+    const double zd = partition_func_disc(D->sh[N], 1.0 / (D->temp / calc_dn(p->Lambda, N - 1))) * std::pow(4.0, NM - N);
+    wraw[N] = zd;
+    ZTot += zd;
+  }
+  if (fdm) {
+    for (int N = 0; N <= NM; N++) D->wn[N] = wraw[N] / ZTot;
+    D->ztot = ZTot;
+  }
+  // rho_Nmax: diag e^{-E/TempBar}/Z with TempBar = Temp/DM of :102-118 and
+  // Z = Z_N (DM-NRG, CFS) or Z_tot (FDM) (src/DM_NRG/DM_NRG.cpp:273-297)
   {
     const DmShell& s = D->sh[NM];
-    const double ZN = partition_func(s, 1.0 / (D->temp / calc_dn(p->Lambda, NM - 1)));
     const double TempBar = D->temp / DM;
     const double bb = (std::fabs(TempBar) < 1e-20) ? 1.0e10 : 1.0 / TempBar;
+    const double Z = fdm ? ZTot : ZN;
     std::vector<double> r;
     D->rho_off[NM].assign(s.nblocks(), 0);
     for (int b = 0; b < s.nblocks(); b++) {
       const int k = s.bsize(b), i0 = s.bbe[2 * b];
       D->rho_off[NM][b] = (int64_t)r.size();
       for (int i = 0; i < k; i++)
-        for (int j = 0; j < k; j++) r.push_back(i == j ? std::exp(-bb * s.en[i0 + i]) / ZN : 0.0);
+        for (int j = 0; j < k; j++) r.push_back(i == j ? std::exp(-bb * s.en[i0 + i]) / Z : 0.0);
     }
     D->rho_len[NM] = (int64_t)r.size();
     DBuf& rb = c->buf("dmR" + std::to_string(NM));
@@ -374,8 +497,12 @@ extern "C" int nrg_dm_create(nrg_ctx* c, const nrg_dmnrg_params* p, nrg_dm** out
     HIPCHK(hipMemcpyAsync(rb.p, r.data(), sizeof(double) * r.size(), hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
   }
+  // the sweep: TempBar = Temp/D_{N-1}, ZNdisc = Z_tot/4^{NM-N} (:361-366)
   for (int N = NM - 1; N >= 0; N--)
-    if ((rc = sweep_shell(D, N))) { delete D; return rc; }
+    if ((rc = sweep_shell(D, N, fdm, D->temp / calc_dn(p->Lambda, N - 1), ZTot / std::pow(4.0, NM - N)))) {
+      delete D;
+      return rc;
+    }
   auto t2 = std::chrono::steady_clock::now();
   D->ms_forward = std::chrono::duration<double, std::milli>(t1 - t0).count();
   D->ms_sweep = std::chrono::duration<double, std::milli>(t2 - t1).count();
@@ -429,25 +556,59 @@ extern "C" int nrg_dm_spec(nrg_dm* D, int n, const int* shells, const double* om
   return spec_tasks(D, n, shells, omegabar, out);
 }
 
+extern "C" int nrg_dm_spec_omega(nrg_dm* D, int n, const double* omega, double* out) {
+  if (!D || n < 0 || (n > 0 && (!omega || !out))) return fail(NRG_ERR_ARG, "nrg_dm_spec_omega: bad arguments");
+  if (D->method == 0) return fail(NRG_ERR_ARG, "nrg_dm_spec_omega: CFS or FDM only");
+  HIPCHK(hipSetDevice(D->c->device));
+  return spec_omegas(D, n, omega, out);
+}
+
+extern "C" int nrg_dm_norm(nrg_dm* D, double* norm) {
+  if (!D || !norm) return fail(NRG_ERR_ARG, "nrg_dm_norm: NULL argument");
+  if (D->method == 0) return fail(NRG_ERR_ARG, "nrg_dm_norm: CFS or FDM only");
+  HIPCHK(hipSetDevice(D->c->device));
+  std::vector<int> sh(D->specNM);
+  std::vector<double> w(D->specNM, 0.0), val(D->specNM);
+  for (int N = 0; N < D->specNM; N++) sh[N] = N;
+  int rc;
+  if ((rc = spec_tasks(D, D->specNM, sh.data(), w.data(), val.data(), true))) return rc;
+  double s = 0.0;
+  for (int N = 0; N < D->specNM; N++) s += val[N];
+  *norm = s;
+  return NRG_OK;
+}
+
+extern "C" int nrg_dm_weights(const nrg_dm* D, double* wn, int cap, double* ztot) {
+  if (!D) return fail(NRG_ERR_ARG, "nrg_dm_weights: NULL handle");
+  if (wn) {
+    if (cap < D->NM + 1) return fail(NRG_ERR_ARG, "nrg_dm_weights: output too small");
+    std::copy(D->wn.begin(), D->wn.end(), wn);
+  }
+  if (ztot) *ztot = D->ztot;
+  return NRG_OK;
+}
+
 extern "C" int nrg_dm_spectrum(nrg_dm* D, double* omega, double* rho, int cap, int* nout) {
   if (!D || !nout) return fail(NRG_ERR_ARG, "nrg_dm_spectrum: NULL argument");
   HIPCHK(hipSetDevice(D->c->device));
   const double L = D->p.Lambda, fac = std::pow(L, 1.25);
   const int Nw = D->p.Nw;
+  const bool win = D->method != 0;   // CFS/FDM: CalcSpecDM_NRG(+-omega), two shells later
+  const int N0 = win ? 2 : 0;
   std::vector<int> sh;
   std::vector<double> wbar, wout;
   std::vector<int> parity;
   if (Nw > 0) {
-    // CalcSpec_ManyOmegas: recorded w = +-Lambda^(-i/(2Nw)) Lambda^1.25 D_N, the
-    // shell sum evaluated at +-Lambda^1.25 Lambda^(+i/(2Nw)) as in the reference
+    // CalcSpec_ManyOmegas: recorded w = +-Lambda^(-i/(2Nw)) Lambda^1.25 D_N; DM-NRG
+    // evaluates the shell sum at +-Lambda^1.25 Lambda^(+i/(2Nw)) as in the reference
     const int Nmax = D->specNM - 1;
-    for (int N = 0; N <= Nmax; N++)
+    for (int N = N0; N <= Nmax; N++)
       for (int i = 0; i < Nw; i++) {
         const double rf = (double)i / (2.0 * Nw);
         sh.push_back(N); wbar.push_back(-fac * std::pow(L, rf));
         wout.push_back(-std::pow(L, -rf) * fac * calc_dn(L, N));
       }
-    for (int N = Nmax; N >= 0; N--)
+    for (int N = Nmax; N >= N0; N--)
       for (int i = Nw - 1; i >= 0; i--) {
         const double rf = (double)i / (2.0 * Nw);
         sh.push_back(N); wbar.push_back(fac * std::pow(L, rf));
@@ -457,19 +618,23 @@ extern "C" int nrg_dm_spectrum(nrg_dm* D, double* omega, double* rho, int cap, i
     // CalcSpecDM_NRG_FixedOmegas with one omega per shell: even then odd shells
     for (int par = 0; par < 2; par++) {
       int last = 0;
-      for (int N = par; N < D->specNM; N += 2) {
+      for (int N = par + N0; N < D->specNM; N += 2) {
         sh.push_back(N); wbar.push_back(-fac); wout.push_back(-fac * calc_dn(L, N)); parity.push_back(par);
         last = N;
       }
-      for (int N = last; N >= par; N -= 2) {
+      for (int N = last; N >= par + N0; N -= 2) {
         sh.push_back(N); wbar.push_back(fac); wout.push_back(fac * calc_dn(L, N)); parity.push_back(par);
       }
     }
   }
   std::vector<double> val(sh.size());
   int rc;
-  if ((rc = spec_tasks(D, (int)sh.size(), sh.data(), wbar.data(), val.data()))) return rc;
-  for (size_t i = 0; i < sh.size(); i++) val[i] /= calc_dn(L, sh[i]);
+  if (win) {
+    if ((rc = spec_omegas(D, (int)wout.size(), wout.data(), val.data()))) return rc;
+  } else {
+    if ((rc = spec_tasks(D, (int)sh.size(), sh.data(), wbar.data(), val.data()))) return rc;
+    for (size_t i = 0; i < sh.size(); i++) val[i] /= calc_dn(L, sh[i]);
+  }
   std::vector<double> W, R;
   if (Nw > 0) {
     W = wout; R = val;

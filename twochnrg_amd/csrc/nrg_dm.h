@@ -16,35 +16,46 @@ struct DmSeg {
   double cg;
 };
 // one Acut_N block of the gather: output k x k at out_off, segments [seg_beg, seg_end)
+// over the leading kc x kc (the states with children: the kept prefix); FDM puts
+// e^{-bb E}/Z_disc on the diagonal of the discarded states p >= kd
 struct DmGBlk {
   int64_t out_off;
-  int k, seg_beg, seg_end, pad;
+  int64_t e_off;    // energies of the block in the shell's energy array
+  int k, kc, kd, seg_beg, seg_end, pad;
 };
-// one operator block pair (b1 -> b2) at a shell: k1 x k2 poles
+// one operator block pair (b1 -> b2) at a shell: k1 x k2 poles, each with a weight
+// for omega >= 0 (Wp) and one for omega < 0 (Wn)
+enum { DM_FORM_DMNRG = 0, DM_FORM_CFS = 1, DM_FORM_FDM = 2, DM_FORM_COSTI = 3 };
 struct DmPair {
   int64_t a_off;    // operator block (k1 x k2) in the shell's operator buffer
-  int64_t p1_off;   // rho1 A (k1 x k2) in the product buffer (DM-NRG form)
+  int64_t p1_off;   // rho1 A (k1 x k2) in the product buffer
   int64_t p2_off;   // A rho2 (k1 x k2)
   int64_t pole_off; // first pole of the pair
   int64_t e1_off, e2_off;   // energies of the two blocks in the shell's energy array
   int k1, k2;
+  int kc1, kc2;     // kept prefixes of the two blocks (CFS/FDM classes)
   double cg;
-  int costi;        // 1: last shell, Costi weights
+  int form;         // DM_FORM_*
   int pad;
   double bb, invZ;  // Costi betabar and 1/Z
 };
+// the broadening of a method (BDelta / BDeltaTemp, src/DM_NRG/DM_NRG.cpp:474-482)
+enum { DM_BD_BROAD_LORENTZ = 0, DM_BD_BROAD_ANDERS = 1, DM_BD_LOGGAUSS_GAUSS = 2 };
 // one (shell, omega) task of the broadening sum
 struct DmTask {
   int64_t pole_beg, pole_end;
-  double w;         // omegabar
-  double twin;      // Lorentzian window |dE| < twindow*T/D_N
+  double w;         // omegabar (its sign selects Wp / Wn)
+  double twin;      // BDeltaTemp window |dE| < twindow*T/D_N
   int chunk_beg;    // first partial-sum slot
   int nchunk;
+  int norm;         // 1: CalcNorm, the broadening is replaced by 1
+  int pad;
 };
 
 struct DmShell {
   int N = 0;
-  std::vector<double> qn, en;          // Acut_N: (Q,S) per block, kept energies
+  std::vector<double> qn, en;          // Acut_N: (Q,S) per block, energies of its states
+  std::vector<unsigned char> kept;     // Kept flags (all 1 unless UpdateBefCut 1)
   std::vector<int> bbe, bbe_bc;        // block limits, pre-cut limits
   std::vector<int64_t> zoff;           // kept rows (k_b x n_b) of block b in zbuf
   std::string zbuf;
@@ -59,9 +70,10 @@ struct DmShell {
 
 extern "C" {
 hipError_t dm_launch_gather(const DmGBlk* blks, int nblk, int kmax, const DmSeg* segs, const double* rot,
-                            double* out, hipStream_t s);
+                            const double* en, double bb, double zdisc, double* out, hipStream_t s);
 hipError_t dm_launch_weights(const DmPair* pairs, int npair, int kkmax, const double* ops, const double* prod,
-                             const double* en, double* W, double* dE, hipStream_t s);
-hipError_t dm_launch_broaden(const DmTask* tasks, int ntask, int maxchunk, const double* W, const double* dE,
-                             double bbroad, double btemp, double* partial, double* out, hipStream_t s);
+                             const double* en, double* Wp, double* Wn, double* dE, hipStream_t s);
+hipError_t dm_launch_broaden(const DmTask* tasks, int ntask, int maxchunk, const double* Wp, const double* Wn,
+                             const double* dE, int bd, double bbroad, double btemp, double* partial, double* out,
+                             hipStream_t s);
 }

@@ -1,8 +1,8 @@
 // nrg_dm.hip -- DM-NRG kernels: the child trace-out of the rotated density
-// matrix (src/DM_NRG/DM_NRG_CalcRhoN.cpp:252-449), the pole weights of the
-// shell spectral sums (src/SpecFunc_allSTL.cpp:223-692, src/SpecFuncClass.cpp:
-// 417-662) and their broadening (src/BroadDelta.cpp:12-73, MijxBDeltaEij
-// src/SpecFunc_allSTL.cpp:36-114).
+// matrix (src/DM_NRG/DM_NRG_CalcRhoN.cpp:252-449, FDM diagonal :393-398), the
+// pole weights of the shell spectral sums (DM-NRG src/SpecFunc_allSTL.cpp:223-692,
+// CFS :702-1240, FDM :1253-2354, Costi src/SpecFuncClass.cpp:417-662) and their
+// broadening (src/BroadDelta.cpp:12-132, MijxBDeltaEij src/SpecFunc_allSTL.cpp:36-219).
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdint.h>
@@ -11,59 +11,102 @@
 
 namespace dmk {
 
-// rho_N block = sum over its child segments of cg * (segment of the rotated rho_{N+1})
+// rho_N block = sum over its child segments of cg * (segment of the rotated
+// rho_{N+1}) on the kept prefix (the states with children), zero elsewhere; FDM:
+// e^{-bb E_p} / zdisc on the diagonal of the discarded states p >= kd
 __global__ void gather_kernel(const DmGBlk* __restrict__ blks, const DmSeg* __restrict__ segs,
-                              const double* __restrict__ rot, double* __restrict__ out) {
+                              const double* __restrict__ rot, const double* __restrict__ en, double bb, double zdisc,
+                              double* __restrict__ out) {
   const DmGBlk B = blks[blockIdx.y];
   const int64_t kk = (int64_t)B.k * B.k;
   for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < kk; idx += (int64_t)gridDim.x * blockDim.x) {
     const int p = (int)(idx / B.k), q = (int)(idx % B.k);
     double s = 0.0;
-    for (int g = B.seg_beg; g < B.seg_end; g++) {
-      const DmSeg S = segs[g];
-      s += S.cg * rot[S.rot_off + (int64_t)p * S.ld + q];
-    }
+    if (p < B.kc && q < B.kc)
+      for (int g = B.seg_beg; g < B.seg_end; g++) {
+        const DmSeg S = segs[g];
+        s += S.cg * rot[S.rot_off + (int64_t)p * S.ld + q];
+      }
+    if (p == q && p >= B.kd) s = exp(-bb * en[B.e_off + p]) / zdisc;
     out[B.out_off + idx] = s;
   }
 }
 
-// pole weights and energies of one operator block pair:
-//   DM-NRG form  W = cg (rho1 A o A + A o A rho2)   (Termo1 + Termo2)
-//   Costi form   W = cg A o A (e^{-bb E1_i} + e^{-bb E2_j}) / Z
+// pole weights (omega >= 0: Wp, omega < 0: Wn) and energies of one operator
+// block pair, with w1 = (rho1 A)_ij A_ij (Termo1) and w2 = A_ij (A rho2)_ij (Termo2):
+//   DM-NRG   Wp = Wn = cg (w1 + w2)
+//   Costi    Wp = Wn = cg A_ij^2 (e^{-bb E1_i} + e^{-bb E2_j}) / Z
+//   CFS      (i kept, j disc): Wp = cg w1;  (i disc, j kept): Wn = cg w2
+//   FDM      (i kept, j disc): Wp = cg (w1 + [block 1 has discarded] w2)
+//            (i disc, j kept): Wn = cg (w2 + [block 2 has discarded] w1)
+//            (i disc, j disc): Wp = Wn = cg (w1 + w2)
 //   dE = E2_j - E1_i
 __global__ void weights_kernel(const DmPair* __restrict__ pairs, const double* __restrict__ ops,
                                const double* __restrict__ prod, const double* __restrict__ en,
-                               double* __restrict__ W, double* __restrict__ dE) {
+                               double* __restrict__ Wp, double* __restrict__ Wn, double* __restrict__ dE) {
   const DmPair P = pairs[blockIdx.y];
   const int64_t n = (int64_t)P.k1 * P.k2;
   for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < n; idx += (int64_t)gridDim.x * blockDim.x) {
     const int i = (int)(idx / P.k2), j = (int)(idx % P.k2);
     const double a = ops[P.a_off + idx];
     const double e1 = en[P.e1_off + i], e2 = en[P.e2_off + j];
-    double w;
-    if (P.costi) w = P.cg * a * a * (exp(-P.bb * e1) + exp(-P.bb * e2)) * P.invZ;
-    else w = P.cg * (prod[P.p1_off + idx] * a + a * prod[P.p2_off + idx]);
-    W[P.pole_off + idx] = w;
+    double wp = 0.0, wn = 0.0;
+    if (P.form == DM_FORM_COSTI) {
+      wp = wn = P.cg * a * a * (exp(-P.bb * e1) + exp(-P.bb * e2)) * P.invZ;
+    } else {
+      const double w1 = prod[P.p1_off + idx] * a, w2 = a * prod[P.p2_off + idx];
+      if (P.form == DM_FORM_DMNRG) {
+        wp = wn = P.cg * (w1 + w2);
+      } else {
+        const bool ik = i < P.kc1, jk = j < P.kc2;
+        if (P.form == DM_FORM_CFS) {
+          if (ik && !jk) wp = P.cg * w1;
+          else if (!ik && jk) wn = P.cg * w2;
+        } else {
+          const bool d1 = P.kc1 < P.k1, d2 = P.kc2 < P.k2;
+          if (ik && !jk) wp = P.cg * (w1 + (d1 ? w2 : 0.0));
+          else if (!ik && jk) wn = P.cg * (w2 + (d2 ? w1 : 0.0));
+          else if (!ik && !jk) wp = wn = P.cg * (w1 + w2);
+        }
+      }
+    }
+    Wp[P.pole_off + idx] = wp;
+    Wn[P.pole_off + idx] = wn;
     dE[P.pole_off + idx] = e2 - e1;
   }
 }
 
-__device__ __forceinline__ double bdelta(double w, double x, double twin, double bbroad, double btemp) {
-  if (fabs(x) < twin) {   // LorentzDelta with the broadening dBroadTemp
+// BDelta(w, x, dBroad), or BDeltaTemp(w, x, dBroadTemp) inside the window:
+//   DM_BD_BROAD_LORENTZ   BroadDelta / LorentzDelta        (DM-NRG)
+//   DM_BD_BROAD_ANDERS    BroadDelta / LorentzDeltaAnders  (CFS)
+//   DM_BD_LOGGAUSS_GAUSS  LogGaussDelta / GaussDelta       (FDM)
+__device__ __forceinline__ double bdelta(int bd, double w, double x, double twin, double bbroad, double btemp) {
+  if (fabs(x) < twin) {
     const double d = w - x;
-    return (1.0 / (2.0 * M_PI)) * (btemp / (d * d + btemp * btemp));
+    if (bd == DM_BD_LOGGAUSS_GAUSS) {
+      const double t = d / btemp;
+      return (1.0 / (btemp * sqrt(M_PI))) * exp(-t * t);
+    }
+    const double l = btemp / (d * d + btemp * btemp);
+    return (bd == DM_BD_BROAD_ANDERS) ? (1.0 / M_PI) * l : (1.0 / (2.0 * M_PI)) * l;
   }
   if (x == 0.0) x = 1.0e-100;
-  if (w * x < 0.0) return 0.0;   // BroadDelta: log-Gaussian, same-sign poles only
-  const double a0 = exp(-bbroad * bbroad / 4.0) / (bbroad * fabs(x) * sqrt(M_PI));
+  if (w * x < 0.0) return 0.0;   // log-Gaussians: same-sign poles only
   const double a1 = log(fabs(w / x)) / bbroad;
+  if (bd == DM_BD_LOGGAUSS_GAUSS) {
+    const double a0 = 1.0 / (bbroad * fabs(x) * sqrt(M_PI));
+    const double t = a1 - bbroad / 4.0;
+    return a0 * exp(-t * t);
+  }
+  const double a0 = exp(-bbroad * bbroad / 4.0) / (bbroad * fabs(x) * sqrt(M_PI));
   return a0 * exp(-a1 * a1);
 }
 
 #define BT 256
 #define BCH (BT * 8)   // poles per workgroup chunk
-__global__ __launch_bounds__(BT) void broaden_kernel(const DmTask* __restrict__ tasks, const double* __restrict__ W,
-                                                     const double* __restrict__ dE, double bbroad, double btemp,
+__global__ __launch_bounds__(BT) void broaden_kernel(const DmTask* __restrict__ tasks, const double* __restrict__ Wp,
+                                                     const double* __restrict__ Wn, const double* __restrict__ dE,
+                                                     int bd, double bbroad, double btemp,
                                                      double* __restrict__ partial) {
   __shared__ double red[BT / 64];
   const DmTask T = tasks[blockIdx.y];
@@ -71,10 +114,15 @@ __global__ __launch_bounds__(BT) void broaden_kernel(const DmTask* __restrict__ 
   if (c >= T.nchunk) return;
   const int64_t b0 = T.pole_beg + (int64_t)c * BCH;
   const int64_t b1 = min(b0 + (int64_t)BCH, T.pole_end);
+  const double* __restrict__ W = (T.w >= 0.0) ? Wp : Wn;
   double s = 0.0;
-  for (int64_t p = b0 + threadIdx.x; p < b1; p += BT) {
-    const double w = W[p];
-    if (w != 0.0) s += w * bdelta(T.w, dE[p], T.twin, bbroad, btemp);
+  if (T.norm) {
+    for (int64_t p = b0 + threadIdx.x; p < b1; p += BT) s += W[p];
+  } else {
+    for (int64_t p = b0 + threadIdx.x; p < b1; p += BT) {
+      const double w = W[p];
+      if (w != 0.0) s += w * bdelta(bd, T.w, dE[p], T.twin, bbroad, btemp);
+    }
   }
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
@@ -99,31 +147,32 @@ __global__ void reduce_kernel(const DmTask* __restrict__ tasks, int ntask, const
 extern "C" {
 
 hipError_t dm_launch_gather(const DmGBlk* blks, int nblk, int kmax, const DmSeg* segs, const double* rot,
-                            double* out, hipStream_t s) {
+                            const double* en, double bb, double zdisc, double* out, hipStream_t s) {
   if (nblk <= 0) return hipSuccess;
   int gx = (int)(((int64_t)kmax * kmax + 255) / 256);
   if (gx > 256) gx = 256;
   if (gx < 1) gx = 1;
-  hipLaunchKernelGGL(dmk::gather_kernel, dim3(gx, nblk), dim3(256), 0, s, blks, segs, rot, out);
+  hipLaunchKernelGGL(dmk::gather_kernel, dim3(gx, nblk), dim3(256), 0, s, blks, segs, rot, en, bb, zdisc, out);
   return hipGetLastError();
 }
 
 hipError_t dm_launch_weights(const DmPair* pairs, int npair, int kkmax, const double* ops, const double* prod,
-                             const double* en, double* W, double* dE, hipStream_t s) {
+                             const double* en, double* Wp, double* Wn, double* dE, hipStream_t s) {
   if (npair <= 0) return hipSuccess;
   int gx = (kkmax + 255) / 256;
   if (gx > 256) gx = 256;
   if (gx < 1) gx = 1;
-  hipLaunchKernelGGL(dmk::weights_kernel, dim3(gx, npair), dim3(256), 0, s, pairs, ops, prod, en, W, dE);
+  hipLaunchKernelGGL(dmk::weights_kernel, dim3(gx, npair), dim3(256), 0, s, pairs, ops, prod, en, Wp, Wn, dE);
   return hipGetLastError();
 }
 
-hipError_t dm_launch_broaden(const DmTask* tasks, int ntask, int maxchunk, const double* W, const double* dE,
-                             double bbroad, double btemp, double* partial, double* out, hipStream_t s) {
+hipError_t dm_launch_broaden(const DmTask* tasks, int ntask, int maxchunk, const double* Wp, const double* Wn,
+                             const double* dE, int bd, double bbroad, double btemp, double* partial, double* out,
+                             hipStream_t s) {
   if (ntask <= 0) return hipSuccess;
   if (maxchunk > 0) {
-    hipLaunchKernelGGL(dmk::broaden_kernel, dim3(maxchunk, ntask), dim3(BT), 0, s, tasks, W, dE, bbroad, btemp,
-                       partial);
+    hipLaunchKernelGGL(dmk::broaden_kernel, dim3(maxchunk, ntask), dim3(BT), 0, s, tasks, Wp, Wn, dE, bd, bbroad,
+                       btemp, partial);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }

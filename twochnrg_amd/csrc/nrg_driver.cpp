@@ -864,6 +864,7 @@ static int dm_capture(Chain& ch, int N, int opidx, DmShell& s) {
   int rc;
   s.N = N;
   s.qn = A.QNumbers; s.en = A.dEn; s.bbe = A.BlockBegEnd; s.bbe_bc = A.BlockBegEndBC;
+  s.kept = A.Kept;
   s.stcf = ch.Abasis.StCameFrom; s.itype = ch.Abasis.iType;
   s.bas_bbe = ch.Abasis.BlockBegEnd; s.bas_qn = ch.Abasis.QNumbers;
   const int nb = A.NumBlocks();
@@ -899,6 +900,7 @@ struct OneChRun {
   nrg_onechqs_params p{};
   double bbar = 0.727, HLF = 1.0, Lambda = 2.0, par_chi = 0.0;
   int nm = 2;
+  int befcut = 0;   // UpdateBefCut (src/Main/NRG_main.cpp:334-343,393-396)
   CoefTable hn_tab;
   std::vector<double> ops;
 
@@ -975,7 +977,7 @@ struct OneChRun {
     return 0;
   }
 
-  void basis() { build_basis(ch.AeigCut, ch.Abasis, ch.site, 0, 0); }
+  void basis() { build_basis(ch.AeigCut, ch.Abasis, ch.site, 0, befcut); }
 
   int assemble(double* dZ) {
     const std::vector<double>& coef = hn_tab.upto(max_sidx(ch.Abasis));
@@ -1003,13 +1005,18 @@ struct OneChRun {
       sus[nc - 1] = calc_suscep(bN, ch.Aeig);
       ent[nc - 1] = calc_entropy(bN, ch.Aeig);
     }
-    ch.cut_states(p.ncut, false, p.myeps);
+    // UpdateBefCut 1: the operators are rotated over every eigenstate of the copy
+    // AeigCut = Aeig with its Kept flags (CutStates with 2 Nstates), SaveArrays
+    // writes that copy, and the real cut follows CalcStuff; the operators stay on
+    // the pre-cut blocks, which the next basis indexes through StCameFrom
+    ch.cut_states(befcut ? 2 * ns : p.ncut, false, p.myeps);
     if ((rc = ch.update_ops())) return rc;
     if (dm && p.calcdens == 1) {
       dm->emplace_back();
       if ((rc = dm_capture(ch, N, 1, dm->back()))) return rc;
     }
     if ((rc = ch.op_avgs(bbar, ops))) return rc;
+    if (befcut) ch.cut_states(p.ncut, false, p.myeps);
     ch.emit(cb, user, emin, ecut, nkept, thT, sus, ent, DN / bbar, ops);
     // chi for the next site: SetChainWilson (src/NRGchainClass.cpp:132-155)
     double d0 = 1.0 - std::pow(Lambda, -((double)N + 1.0));
@@ -1024,9 +1031,10 @@ struct OneChRun {
 // OneChQS Anderson: NRG_main loop (src/Main/NRG_main.cpp:282-427)
 // ---------------------------------------------------------------------------
 static int run_onechqs(nrg_ctx* c, nrg_onechqs_params p, nrg_step_cb cb, void* user, double* timing_ms,
-                       std::vector<DmShell>* dm = nullptr) {
+                       std::vector<DmShell>* dm = nullptr, int befcut = 0) {
   int rc;
   OneChRun R;
+  R.befcut = befcut;
   if ((rc = R.init(c, p, "", cb, user))) return rc;
   auto t0 = std::chrono::steady_clock::now();
   for (int N = 0; N < p.nsitesmax; N++) {
@@ -1217,8 +1225,8 @@ static int run_twochqs(nrg_ctx* c, nrg_twochqs_params p, nrg_step_cb cb, void* u
   return 0;
 }
 
-int run_onechqs_dm(nrg_ctx* c, const nrg_onechqs_params& p, std::vector<DmShell>& dm) {
-  return run_onechqs(c, p, nullptr, nullptr, nullptr, &dm);
+int run_onechqs_dm(nrg_ctx* c, const nrg_onechqs_params& p, std::vector<DmShell>& dm, int befcut) {
+  return run_onechqs(c, p, nullptr, nullptr, nullptr, &dm, befcut);
 }
 
 }  // namespace nrg
