@@ -115,6 +115,54 @@ def dm_run(cpu=True):
     return out
 
 
+def fdm_omegas():
+    """omega = +-logspace(-10, 0, 1000): 2000 points, negative first (ascending)."""
+    import numpy as np
+    pos = np.logspace(-10.0, 0.0, 1000)
+    return np.concatenate([-pos[::-1], pos])
+
+
+def fdm_run(cpu=True, cpu_omegas=12):
+    """FDM A(omega) (DM_NRG -F -e 0.6) at T = 1e-8 on the single-channel chain of
+    the Ncut=400 workload run with UpdateBefCut 1 (every state and its operator
+    elements kept per shell), evaluated through CalcSpecDM_NRG's shell window on
+    2000 omegas; wall = forward chain + FDM sweep + pole weights + all omegas.
+    The CPU oracle runs the same chain and sweep, and a sample of the omegas
+    (every len/cpu_omegas-th), extrapolated to the 2000."""
+    import numpy as np
+    import twochnrg_amd as t
+    kw = dict(DM_C1, betabar=dm_betabar(), method=2, update_bef_cut=1, bbroad=0.6)
+    om = fdm_omegas()
+    out = {"workload": "onechqs_fdm_c1", "T": 1e-8, "betabar": kw["betabar"], "bbroad": 0.6, "nomega": len(om),
+           "params": DM_C1}
+    with t.Context(0) as ctx:
+        t0 = time.perf_counter()
+        with ctx.dm(**kw) as d:
+            r = d.spec_omega(om)
+            t1 = time.perf_counter()
+            tm = d.timing()
+            norm = d.norm()
+            wn, ztot = d.weights()
+    out.update({"wall_ms": 1e3 * (t1 - t0), "breakdown_ms": tm, "post_ms": tm["sweep"] + tm["poles"] + tm["broaden"],
+                "norm": norm, "ztot": ztot, "shell_weight_max": int(np.argmax(wn))})
+    print("bench fdm: device done (%.0f ms)" % out["wall_ms"], file=sys.stderr, flush=True)
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle as po
+        t0 = time.perf_counter()
+        o = po.DmRun(**kw)
+        t1 = time.perf_counter()
+        print("bench fdm: oracle chain + sweep %.1f s" % (t1 - t0), file=sys.stderr, flush=True)
+        idx = np.linspace(0, len(om) - 1, cpu_omegas).astype(int)
+        ro = np.array([o.spec_omega(x) for x in om[idx]])
+        t2 = time.perf_counter()
+        out["cpu_oracle_ms"] = 1e3 * ((t1 - t0) + (t2 - t1) * len(om) / len(idx))
+        out["cpu_sample"] = "chain + sweep timed, %d of %d omegas timed and scaled" % (len(idx), len(om))
+        out["max_rel_err_vs_oracle"] = float(np.max(np.abs(r[idx] - ro) / np.maximum(np.abs(ro), 1e-12 * np.abs(ro).max())))
+        out["norm_oracle"] = o.norm()
+    return out
+
+
 def c2_run(steps=4, ramp=6, plain_steps=1):
     """64 seeded single-channel Anderson instances (splitmix64 seed 20261203,
     tests/golden/c2_instances.json), Lambda 2.5, Ncut 800..2000, stepped in
@@ -263,6 +311,7 @@ def main():
             out["cpu_baseline"] = cb
     if not a.no_dm:
         out["dmnrg"] = dm_run(cpu=not a.no_cpu)
+        out["fdm"] = fdm_run(cpu=not a.no_cpu)
     if not a.no_c2:
         out["batched_instances"] = c2_run()
     print(json.dumps(out))

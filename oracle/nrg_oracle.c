@@ -1954,9 +1954,6 @@ orc_dm *orc_dm_run(const orc_dmnrg_params *p) {
     D->trace[N] = calc_rho_n(&D->acut[N], &D->acut[N + 1], &D->abasis[N + 1], &D->site, &D->rho[N + 1], &D->rho[N],
                              fdm, tb, ZTot / pow(4.0, NM - N));
   }
-  double fac = pow(p->Lambda, 1.25);
-  if (p->Nw > 0) spec_many_omegas(D, p->Nw, fac);
-  else spec_fixed_omegas(D, fac);
   return D;
 }
 
@@ -1992,7 +1989,13 @@ int orc_dm_shell(const orc_dm *D, int N, orc_dm_shell_view *v) {
   return 0;
 }
 
-int orc_dm_spectrum(const orc_dm *D, const double **omega, const double **rho) {
+/* the grid is evaluated on the first call */
+int orc_dm_spectrum(orc_dm *D, const double **omega, const double **rho) {
+  if (!D->omega) {
+    double fac = pow(D->p.Lambda, 1.25);
+    if (D->p.Nw > 0) spec_many_omegas(D, D->p.Nw, fac);
+    else spec_fixed_omegas(D, fac);
+  }
   *omega = D->omega; *rho = D->rhow;
   return D->nomega;
 }

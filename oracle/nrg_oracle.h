@@ -118,7 +118,8 @@ double orc_dm_spec_omega(const orc_dm *D, double omega);
 double orc_dm_norm(const orc_dm *D);
 /* Z_tot of the FDM weights (src/DM_NRG/DM_NRG.cpp:203-268; 0 unless FDM) */
 double orc_dm_ztot(const orc_dm *D);
-int orc_dm_spectrum(const orc_dm *D, const double **omega, const double **rho);
+/* rho_0_0 on the DM_NRG_CalcSpecFunc_ij grid (evaluated on the first call) */
+int orc_dm_spectrum(orc_dm *D, const double **omega, const double **rho);
 
 /* ---- kernel-level restatements ---- */
 
