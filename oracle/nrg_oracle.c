@@ -1166,7 +1166,8 @@ static void cut_states_legacy(const arr_t *Ain, arr_t *Ac, int ncut) {
 /* f_ch in the eigenbasis: pairs (ibl < jbl) with Q_j = Q_i+1, S_j = S_i +- 1/2,
  * channel loop inside the pair loop.  A has BC data (bbe_bc/bbe_ev) when it is a
  * cut array; the matching pre-cut block of Ab is found by quantum numbers. */
-static void twoch_update_f(const arr_t *S, const arr_t *A, const arr_t *Ab, mat_t *f /* [2] */) {
+static double qsz_fn_el(int ch, int ti, int tj);
+static void twoch_update_f(const arr_t *S, const arr_t *A, const arr_t *Ab, mat_t *f /* [2] */, int qsz) {
   for (int c = 0; c < 2; c++) { mat_clear(&f[c]); mat_sync(&f[c], A); f[c].b.nshell = A->nshell; }
   int icount[2] = {0, 0};
   for (int ib = 0; ib < nblocks(A); ib++) {
@@ -1194,7 +1195,8 @@ static void twoch_update_f(const arr_t *S, const arr_t *A, const arr_t *Ab, mat_
           for (int b = 0; b < njbc; b++) {
             int jst = blim(Ab, jbc, 0) + b;
             if (Ab->stcf.a[ist] != Ab->stcf.a[jst]) continue;
-            F[(size_t)a * njbc + b] = twoch_fn_el(S, ch, Si, Sj, Ab->itype.a[ist], Ab->itype.a[jst]);
+            F[(size_t)a * njbc + b] = qsz ? qsz_fn_el(ch, Ab->itype.a[ist], Ab->itype.a[jst])
+                                          : twoch_fn_el(S, ch, Si, Sj, Ab->itype.a[ist], Ab->itype.a[jst]);
           }
         }
         dres(&M->mat, M->mat.n + ni * nj);
@@ -1252,7 +1254,7 @@ int orc_run_twochqs(const orc_twochqs_params *p, orc_step_cb cb, void *user) {
     free(H);
   }
   double emin = set_e0zero(&Ae);
-  twoch_update_f(&S, &Ae, &Ab, f);   /* TwoChQS_UpdateQm1fQ on the uncut N=0 basis */
+  twoch_update_f(&S, &Ae, &Ab, f, 0);   /* TwoChQS_UpdateQm1fQ on the uncut N=0 basis */
   double DN = HLF * pow(L, -(0 - 1) / 2.0);
   {
     double thT[4] = {DN / bbar, 0, 0, 0}, sus[4] = {calc_suscep(bbar, &Ae), 0, 0, 0};
@@ -1268,7 +1270,7 @@ int orc_run_twochqs(const orc_twochqs_params *p, orc_step_cb cb, void *user) {
     double d0 = 1.0 - pow(L, -N), d1 = sqrt(1.0 - pow(L, -(2 * N - 1))), d2 = sqrt(1.0 - pow(L, -(2 * N + 1)));
     double chiN = d0 / (d1 * d2);
     cut_states_legacy(&Ae, &Ac, p->ncut);
-    if (N > 1) twoch_update_f(&S, &Ac, &Ab, f);
+    if (N > 1) twoch_update_f(&S, &Ac, &Ab, f, 0);
     qs_build_basis(&Ac, &Ab, &S, 0);
     /* TwoChQS_DiagHN */
     Ae.nshell = Ab.nshell; Ae.nq = 2;
@@ -1309,6 +1311,227 @@ int orc_run_twochqs(const orc_twochqs_params *p, orc_step_cb cb, void *user) {
     double thT[4] = {DN / bbar, 0, 0, 0}, sus[4] = {calc_suscep(bbar, &Ae), 0, 0, 0};
     double ent[4] = {calc_entropy(bbar, &Ae), 0, 0, 0};
     /* record the cut that the next iteration will make */
+    arr_t nc = {0};
+    double ec = HUGE_VAL;
+    if (p->ncut < nstates(&Ae)) ec = orc_ecut(Ae.en.n, Ae.en.a, p->ncut);
+    cut_states_legacy(&Ae, &nc, p->ncut);
+    int nk = 0;
+    Ae.kept.n = 0;
+    for (int i = 0; i < Ae.en.n; i++) { int k = Ae.en.a[i] <= ec; ipush(&Ae.kept, k); nk += k; }
+    emit(cb, user, &Ae, &nc, emin, ec, nk, &hrec, p->record_blocks, thT, sus, ent, DN / bbar, 0, NULL);
+    arr_free(&nc);
+  }
+  free(hrec.a);
+  mat_free(&f[0]); mat_free(&f[1]); mat_free(&d);
+  arr_free(&S); arr_free(&Ae); arr_free(&Ac); arr_free(&Ab); arr_free(&Ah); arr_free(&Ahc);
+  return 0;
+}
+
+/* ================================================================== */
+/* TwoChQSz Anderson: the standalone driver src/TwoChQSz/TwoChQSz.cpp  */
+/* :103-425 (the part after its self-test exit): (Q, Sz) blocks, no    */
+/* SU(2); QSz_BuildBasis, TwoChQSz_DiagHN, TwoChQSz_SetH0Anderson,      */
+/* TwoChQSz_UpdateQm1fQ / _UpdateMatrixAfterCutting, CalcSuscep with    */
+/* totalS false.                                                        */
+/* ================================================================== */
+
+/* TwoChQSz_SetSingleSite  src/TwoChQSz/TwoChQSz_SetSingleSite.cpp:5-85: 16 states,
+ * each its own (Q, Sz) block, iType = state index */
+static void twochqsz_site(arr_t *S) {
+  arr_clear(S);
+  S->nq = 2; S->nshell = 0;
+  const double q[32] = {-2, 0, -1, 0.5, -1, -0.5, 0, 0, -1, 0.5, 0, 1, 0, 0, 1, 0.5,
+                        -1, -0.5, 0, 0, 0, -1, 1, -0.5, 0, 0, 1, 0.5, 1, -0.5, 2, 0};
+  for (int i = 0; i < 32; i++) dpush(&S->qn, q[i]);
+  for (int i = 0; i < 16; i++) { ipush(&S->bbe, i); ipush(&S->bbe, i); ipush(&S->itype, i); }
+}
+/* fd_table  src/TwoChQSz/TwoChQSz_OpTables.cpp:3-78: <type_i| f^dagger_{ch,sigma} |type_j>;
+ * site state bits: 1 = ch2 up, 2 = ch2 down, 4 = ch1 up, 8 = ch1 down */
+static double qsz_fd(int ch, int sigma, int ti, int tj) {
+  if (ti < 0 || ti > 15 || tj < 0 || tj > 15) return 0.0;
+  int bit = (ch == 1) ? (sigma == 1 ? 4 : (sigma == -1 ? 8 : 0)) : (ch == 2 ? (sigma == 1 ? 1 : (sigma == -1 ? 2 : 0)) : 0);
+  if (!bit || (tj & bit) || ti != (tj | bit)) return 0.0;
+  /* the signs of the table: ch1 up +1; ch1 dn -1 when ch1 up is occupied; ch2 up
+   * -1 when exactly one of the ch1 modes is occupied; ch2 dn: that sign times
+   * -1 when ch2 up is occupied */
+  if (bit == 4) return 1.0;
+  if (bit == 8) return (tj & 4) ? -1.0 : 1.0;
+  double s = (((tj >> 2) & 1) ^ ((tj >> 3) & 1)) ? -1.0 : 1.0;
+  if (bit == 1) return s;
+  return (tj & 1) ? -s : s;
+}
+/* fnbasis of TwoChQSz_UpdateQm1fQ / _UpdateMatrixAfterCutting
+ * (src/TwoChQSz/TwoChQSz_UpdateMatrices.cpp:110-111, 288-289), same parent */
+static double qsz_fn_el(int ch, int ti, int tj) { return qsz_fd(ch, -1, tj, ti) + qsz_fd(ch, 1, tj, ti); }
+/* H_N coupling of TwoChQSz_DiagHN (src/TwoChQSz/TwoChQSz_DiagHN.cpp:74-97, h0 = 0)
+ * and TwoChQSz_SetH0Anderson (src/TwoChQSz/TwoChQSz_SetH0Anderson.cpp:180-215,
+ * h0 = 1): sum over sigma of fd(ch, sigma, typep, type) times the fermion sign
+ * of the site state `type` (-1 for Sz = +-1/2; at N = 0 also Q = +-1) */
+static double qsz_hn_coef(const arr_t *S, int ch, int typep, int type, int h0) {
+  double Qt = getqn(S, type, 0), Szt = getqn(S, type, 1);
+  double fs = 1.0;
+  if (h0) { if ((dEqual(Qt, 1.0) || dEqual(Qt, -1.0)) && (dEqual(Szt, 0.5) || dEqual(Szt, -0.5))) fs = -1.0; }
+  else if (dEqual(Szt, 0.5) || dEqual(Szt, -0.5)) fs = -1.0;
+  double v = 0.0;
+  for (int sg = -1; sg <= 1; sg += 2) v += qsz_fd(ch, sg, typep, type) * fs;
+  return v;
+}
+/* QSz_BuildBasis  src/QSz_BuildBasis.cpp:9-117: (Q, Sz) of old block + site
+ * block, FilterQNumbers (first appearance), then per new block: old block,
+ * site block, old state */
+static void qsz_build_basis(const arr_t *Ac, arr_t *Ab, const arr_t *S, int befcut) {
+  Ab->nshell = Ac->nshell + 1;
+  Ab->nq = 2;
+  arr_clear(Ab);
+  for (int b = 0; b < nblocks(Ac); b++)
+    for (int s = 0; s < nblocks(S); s++) {
+      dpush(&Ab->qn, getqn(Ac, b, 0) + getqn(S, s, 0));
+      dpush(&Ab->qn, getqn(Ac, b, 1) + getqn(S, s, 1));
+    }
+  filter_qnumbers(Ab);
+  int istnew = 0;
+  for (int bn = 0; bn < nblocks(Ab); bn++) {
+    double Qn = getqn(Ab, bn, 0), Sn = getqn(Ab, bn, 1);
+    ipush(&Ab->bbe, istnew);
+    for (int bo = 0; bo < nblocks(Ac); bo++) {
+      double Qo = getqn(Ac, bo, 0), So = getqn(Ac, bo, 1);
+      for (int s = 0; s < nblocks(S); s++) {
+        if (!(dEqual(Qn, Qo + getqn(S, s, 0)) && dEqual(Sn, So + getqn(S, s, 1)))) continue;
+        for (int io = blim(Ac, bo, 0); io <= blim(Ac, bo, 1); io++) {
+          ipush(&Ab->itype, S->itype.a[s]);
+          dpush(&Ab->en, Ac->en.a[io]);
+          ipush(&Ab->stcf, befcut == 1 ? Ac->stcf.a[io] : io);
+          istnew++;
+        }
+      }
+    }
+    ipush(&Ab->bbe, istnew - 1);
+  }
+}
+/* CalcSuscep / CalcEntropy with totalS false (src/CalcThermo.cpp:16-121):
+ * <Sz^2> and S with degeneracy 1 per state */
+static double calc_suscep_sz(double bb, const arr_t *A) {
+  double s0 = 0, s1 = 0;
+  for (int b = 0; b < nblocks(A); b++) {
+    double Sz = getqn(A, b, 1);
+    for (int i = blim(A, b, 0); i <= blim(A, b, 1); i++) {
+      double x = exp(-bb * A->en.a[i]);
+      s0 += x; s1 += Sz * Sz * x;
+    }
+  }
+  return s1 / s0;
+}
+static double calc_entropy_sz(double bb, const arr_t *A) {
+  double Z = 0, s = 0;
+  for (int i = 0; i < A->en.n; i++) { double E = A->en.a[i], x = exp(-bb * E); s += E * x; Z += x; }
+  return bb * (s / Z) + log(Z);
+}
+
+int orc_run_twochqsz(const orc_twochqs_params *p, orc_step_cb cb, void *user) {
+  const double L = p->Lambda, bbar = p->betabar > 0 ? p->betabar : 0.727;
+  const double HLF = 0.5 * (1.0 + 1.0 / L);
+  arr_t S = {0}, Ae = {0}, Ac = {0}, Ab = {0}, Ah = {0}, Ahc = {0};
+  mat_t f[2], d = {0};
+  memset(f, 0, sizeof(f));
+  twochqsz_site(&S);
+  /* src/TwoChQSz/TwoChQSz.cpp:156-185 */
+  double Ut = 0.5 * p->U / HLF, edt = p->ed / HLF;
+  double G1t = p->Gamma1 * (2.0 / M_PI) / (HLF * HLF), G2t = p->Gamma2 * (2.0 / M_PI) / (HLF * HLF);
+  double Un = Ut / sqrt(L), edn = edt / sqrt(L), chi0[2] = {sqrt(G1t / L), sqrt(G2t / L)};
+  /* TwoChQSz_SetH0Anderson  src/TwoChQSz/TwoChQSz_SetH0Anderson.cpp:15-251: four
+   * impurity blocks (Q, Sz) and the impurity operator of its :90-135 */
+  Ah.nq = 2; Ah.nshell = -1;
+  const double qh[8] = {-1, 0, 0, 0.5, 0, -0.5, 1, 0};
+  for (int i = 0; i < 8; i++) dpush(&Ah.qn, qh[i]);
+  for (int i = 0; i < 4; i++) { ipush(&Ah.bbe, i); ipush(&Ah.bbe, i); dpush(&Ah.ev, 1.0); }
+  dpush(&Ah.en, Un); dpush(&Ah.en, edn + Un); dpush(&Ah.en, edn + Un); dpush(&Ah.en, 2.0 * edn + 3.0 * Un);
+  mat_sync(&d, &Ah);
+  const int dmap[8] = {0, 1, 0, 2, 1, 3, 2, 3};
+  const double dval[4] = {1.0, 1.0, -1.0, 1.0};
+  for (int k = 0; k < 4; k++) {
+    dpush(&d.mat, dval[k]); ipush(&d.map, dmap[2 * k]); ipush(&d.map, dmap[2 * k + 1]);
+    ipush(&d.mbe, k); ipush(&d.mbe, k);
+  }
+  cut_states_legacy(&Ah, &Ahc, 100);
+  qsz_build_basis(&Ahc, &Ab, &S, 1);
+  Ae.nshell = Ab.nshell; Ae.nq = 2;
+  arr_clear(&Ae);
+  dcopy_v(&Ae.qn, &Ab.qn); icopy_v(&Ae.bbe, &Ab.bbe);
+  for (int b = 0; b < nblocks(&Ab); b++) {
+    int n = bsize(&Ab, b), b0 = blim(&Ab, b, 0);
+    double *H = (double *)calloc((size_t)n * n, sizeof(double));
+    for (int i = 0; i < n; i++)
+      for (int j = 0; j < n; j++) {
+        int ist = b0 + i, jst = b0 + j;
+        if (i == j) { H[(size_t)i * n + j] = Ab.en.a[ist]; continue; }
+        double old = get_matel(&d, Ab.stcf.a[ist], Ab.stcf.a[jst]);
+        int tp = Ab.itype.a[ist], tt = Ab.itype.a[jst];
+        if (dEqual(old, 0.0)) { old = get_matel(&d, Ab.stcf.a[jst], Ab.stcf.a[ist]); tp = Ab.itype.a[jst]; tt = Ab.itype.a[ist]; }
+        double v = 0.0;
+        for (int ch = 1; ch <= 2; ch++) v += chi0[ch - 1] * old * qsz_hn_coef(&S, ch, tp, tt, 1);
+        H[(size_t)i * n + j] = v;
+      }
+    dres(&Ae.en, Ae.en.n + n); dres(&Ae.ev, Ae.ev.n + n * n);
+    orc_eigh_full(n, H, Ae.en.a + Ae.en.n, Ae.ev.a + Ae.ev.n);
+    Ae.en.n += n; Ae.ev.n += n * n;
+    free(H);
+  }
+  double emin = set_e0zero(&Ae);
+  twoch_update_f(&S, &Ae, &Ab, f, 1);   /* TwoChQSz_UpdateQm1fQ on the uncut N=0 basis */
+  double DN = HLF * pow(L, -(0 - 1) / 2.0);
+  {
+    double thT[4] = {DN / bbar, 0, 0, 0}, sus[4] = {calc_suscep_sz(bbar, &Ae), 0, 0, 0};
+    double ent[4] = {calc_entropy_sz(bbar, &Ae), 0, 0, 0};
+    arr_t tmpc = {0};
+    cut_states_legacy(&Ae, &tmpc, p->ncut);
+    emit(cb, user, &Ae, &tmpc, emin, HUGE_VAL, Ae.en.n, NULL, 0, thT, sus, ent, DN / bbar, 0, NULL);
+    arr_free(&tmpc);
+  }
+  dvec hrec = {0};
+  for (int N = 1; N <= p->nsitesmax; N++) {
+    DN = HLF * pow(L, -(N - 1) / 2.0);
+    double d0 = 1.0 - pow(L, -N), d1 = sqrt(1.0 - pow(L, -(2 * N - 1))), d2 = sqrt(1.0 - pow(L, -(2 * N + 1)));
+    double chiN = d0 / (d1 * d2);
+    cut_states_legacy(&Ae, &Ac, p->ncut);
+    if (N > 1) twoch_update_f(&S, &Ac, &Ab, f, 1);
+    qsz_build_basis(&Ac, &Ab, &S, 0);
+    /* TwoChQSz_DiagHN: the rules give a symmetric H; the upper triangle is used */
+    Ae.nshell = Ab.nshell; Ae.nq = 2;
+    arr_clear(&Ae);
+    dcopy_v(&Ae.qn, &Ab.qn); icopy_v(&Ae.bbe, &Ab.bbe);
+    hrec.n = 0;
+    for (int b = 0; b < nblocks(&Ab); b++) {
+      int n = bsize(&Ab, b), b0 = blim(&Ab, b, 0);
+      double *ap = (double *)malloc(sizeof(double) * ((size_t)n * (n + 1) / 2 + 1));
+      int r = 0;
+      for (int i = 0; i < n; i++)
+        for (int j = i; j < n; j++) {
+          int ist = b0 + i, jst = b0 + j;
+          double v;
+          if (i == j) v = sqrt(L) * Ab.en.a[ist];
+          else {
+            v = 0.0;
+            for (int ch = 1; ch <= 2; ch++) {
+              double old = get_matel(&f[ch - 1], Ab.stcf.a[ist], Ab.stcf.a[jst]);
+              int tp = Ab.itype.a[ist], tt = Ab.itype.a[jst];
+              if (dEqual(fabs(old), 0.0)) {
+                old = get_matel(&f[ch - 1], Ab.stcf.a[jst], Ab.stcf.a[ist]);
+                tp = Ab.itype.a[jst]; tt = Ab.itype.a[ist];
+              }
+              v += chiN * old * qsz_hn_coef(&S, ch, tp, tt, 0);
+            }
+          }
+          ap[r++] = v;
+        }
+      if (p->record_blocks) for (int k = 0; k < r; k++) dpush(&hrec, ap[k]);
+      dres(&Ae.en, Ae.en.n + n); dres(&Ae.ev, Ae.ev.n + n * n);
+      orc_eigh_packed(n, ap, Ae.en.a + Ae.en.n, Ae.ev.a + Ae.ev.n);
+      Ae.en.n += n; Ae.ev.n += n * n;
+      free(ap);
+    }
+    emin = set_e0zero(&Ae);
+    double thT[4] = {DN / bbar, 0, 0, 0}, sus[4] = {calc_suscep_sz(bbar, &Ae), 0, 0, 0};
+    double ent[4] = {calc_entropy_sz(bbar, &Ae), 0, 0, 0};
     arr_t nc = {0};
     double ec = HUGE_VAL;
     if (p->ncut < nstates(&Ae)) ec = orc_ecut(Ae.en.n, Ae.en.a, p->ncut);

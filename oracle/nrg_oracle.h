@@ -73,6 +73,11 @@ typedef struct {
 
 int orc_run_twochqs(const orc_twochqs_params *p, orc_step_cb cb, void *user);
 
+/* TwoChQSz Anderson through its standalone driver (src/TwoChQSz/TwoChQSz.cpp:103-425,
+ * the part after the self-test exit at :100): (Q, Sz) blocks, same parameters
+ * and conventions as orc_run_twochqs; T*chi = <Sz^2>, entropy with degeneracy 1. */
+int orc_run_twochqsz(const orc_twochqs_params *p, orc_step_cb cb, void *user);
+
 /* DM-NRG on the OneChQS chain (NRG_main with calcdens 1, UpdateBefCut 0 or 1)
  * followed by the DM_NRG post-processing (src/DM_NRG/DM_NRG.cpp:28-531):
  * plain DM-NRG (UseCFS 0), CFS (-C) or FDM (-F); spectral function of c_d. */

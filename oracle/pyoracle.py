@@ -63,6 +63,8 @@ def load():
     lib.orc_run_onechqs.restype = ctypes.c_int
     lib.orc_run_twochqs.argtypes = [ctypes.POINTER(TwoChQSParams), STEP_CB, ctypes.c_void_p]
     lib.orc_run_twochqs.restype = ctypes.c_int
+    lib.orc_run_twochqsz.argtypes = [ctypes.POINTER(TwoChQSParams), STEP_CB, ctypes.c_void_p]
+    lib.orc_run_twochqsz.restype = ctypes.c_int
     lib.orc_cgordan.argtypes = [ctypes.c_double] * 6
     lib.orc_cgordan.restype = ctypes.c_double
     dp = ctypes.POINTER(ctypes.c_double)
@@ -141,6 +143,23 @@ def run_twochqs(U=0.4, Gamma1=0.03, Gamma2=0.03, ed=-0.2, Lambda=3.0, nsitesmax=
     p = TwoChQSParams(U, Gamma1, Gamma2, ed, Lambda, nsitesmax, ncut, betabar, int(record_blocks))
     fn = STEP_CB(cb)
     rc = lib.orc_run_twochqs(ctypes.byref(p), fn, None)
+    if rc != 0:
+        raise RuntimeError("oracle run failed: %d" % rc)
+    return steps
+
+
+def run_twochqsz(U=0.5, Gamma1=0.03, Gamma2=0.01, ed=-0.25, Lambda=2.5, nsitesmax=3, ncut=4000,
+                 betabar=0.727, record_blocks=False):
+    """TwoChQSz standalone driver: (Q, Sz) blocks, parameters as run_twochqs."""
+    lib = load()
+    steps = []
+
+    def cb(_user, st):
+        steps.append(step_to_dict(st))
+
+    p = TwoChQSParams(U, Gamma1, Gamma2, ed, Lambda, nsitesmax, ncut, betabar, int(record_blocks))
+    fn = STEP_CB(cb)
+    rc = lib.orc_run_twochqsz(ctypes.byref(p), fn, None)
     if rc != 0:
         raise RuntimeError("oracle run failed: %d" % rc)
     return steps
