@@ -194,6 +194,18 @@ typedef struct {
  */
 int nrg_run_twochqs(nrg_ctx *ctx, const nrg_twochqs_params *p, nrg_step_cb cb, void *user, double *timing_ms);
 
+/*
+ * TwoChQSz Anderson through its standalone driver (src/TwoChQSz/TwoChQSz.cpp:103-425,
+ * the code after the self-test exit at :100; NRG_main's ModelSwitch has no case
+ * for it): (Q, Sz) blocks with no SU(2) (QSz_BuildBasis src/QSz_BuildBasis.cpp:9-117,
+ * TwoChQSz_DiagHN src/TwoChQSz/TwoChQSz_DiagHN.cpp:14-133, TwoChQSz_SetH0Anderson
+ * src/TwoChQSz/TwoChQSz_SetH0Anderson.cpp:15-251, TwoChQSz_UpdateQm1fQ /
+ * _UpdateMatrixAfterCutting src/TwoChQSz/TwoChQSz_UpdateMatrices.cpp:16-341).
+ * Parameters and step loop as nrg_run_twochqs; Ncut counts states; T*chi is
+ * <Sz^2> (CalcSuscep with totalS false, src/CalcThermo.cpp:16-78).
+ */
+int nrg_run_twochqsz(nrg_ctx *ctx, const nrg_twochqs_params *p, nrg_step_cb cb, void *user, double *timing_ms);
+
 /* ---------------- DM-NRG (density-matrix NRG spectral function) ---------------- */
 
 typedef struct {

@@ -97,3 +97,26 @@ def test_twochqs_h0_blocks(ctx):
     # H_1 is assembled from f_ch over the N=0 eigenbasis (sign gauge free), so
     # only its spectrum is compared; the N=1 step's record still carries H
     assert gs[1]["hpacked"].shape == os_[1]["hpacked"].shape
+
+
+@pytest.mark.parametrize("couplings,nsites,ncut", [((0.03, 0.03), 5, 400), ((0.03, 0.01), 4, 300)])
+def test_twochqsz_chain(ctx, couplings, nsites, ncut):
+    # TwoChQSz: (Q, Sz) blocks, Ncut counts states; the (0.03, 0.01) run passes
+    # a step (N = 3) whose block order hides f couplings from the next H_N (the
+    # reference's jbl > ibl update), which both sides must reproduce
+    g1, g2 = couplings
+    kw = dict(U=0.5, Gamma1=g1, Gamma2=g2, ed=-0.25, Lambda=2.5, nsitesmax=nsites, ncut=ncut)
+    print("nrg twochqsz", kw, flush=True)
+    gs, _ = ctx.run_twochqsz(**kw)
+    os_ = po.run_twochqsz(**kw)
+    worst = _compare(gs, os_)
+    print("worst eigenvalue rel err", worst)
+
+
+def test_twochqsz_h0_blocks(ctx):
+    kw = dict(U=0.5, Gamma1=0.03, Gamma2=0.01, ed=-0.1, Lambda=2.5, nsitesmax=1, ncut=2000)
+    print("nrg twochqsz h0", kw, flush=True)
+    gs, _ = ctx.run_twochqsz(**kw, record_blocks=True)
+    os_ = po.run_twochqsz(**kw, record_blocks=True)
+    _compare(gs, os_)
+    assert gs[1]["hpacked"].shape == os_[1]["hpacked"].shape

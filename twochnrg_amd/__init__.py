@@ -20,6 +20,7 @@ EXPORTS = [
     "nrg_set_eig_impl", "nrg_set_plain", "nrg_eig2_stages",
     "nrg_dm_create", "nrg_dm_destroy", "nrg_dm_info", "nrg_dm_timing", "nrg_dm_rho", "nrg_dm_spec",
     "nrg_dm_spectrum", "nrg_run_onechqs_multi", "nrg_dm_spec_omega", "nrg_dm_norm", "nrg_dm_weights",
+    "nrg_run_twochqsz",
 ]
 
 KSTAT_NAMES = ["eig", "hn", "fbasis", "gemm", "tdb_symv"]
@@ -122,6 +123,7 @@ def lib():
     L.nrg_reset_stats.argtypes = [ctypes.c_void_p]
     L.nrg_run_onechqs.argtypes = [ctypes.c_void_p, ctypes.POINTER(OneChQSParams), STEP_CB, ctypes.c_void_p, _dp]
     L.nrg_run_twochqs.argtypes = [ctypes.c_void_p, ctypes.POINTER(TwoChQSParams), STEP_CB, ctypes.c_void_p, _dp]
+    L.nrg_run_twochqsz.argtypes = [ctypes.c_void_p, ctypes.POINTER(TwoChQSParams), STEP_CB, ctypes.c_void_p, _dp]
     L.nrg_run_onechqs_multi.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(OneChQSParams), MULTI_CB,
                                         ctypes.c_void_p, _dp]
     L.nrg_dm_create.argtypes = [ctypes.c_void_p, ctypes.POINTER(DmParams), ctypes.POINTER(ctypes.c_void_p)]
@@ -321,7 +323,8 @@ class Context:
                                     Mtemp, betabar, twindow, broadtemp, bbroad, Nw, method, update_bef_cut))
 
     def run_twochqs(self, U=0.4, Gamma1=0.03, Gamma2=0.03, ed=-0.2, Lambda=3.0, nsitesmax=3, ncut=3000,
-                    betabar=0.727, record_blocks=False, keep_steps=True, on_step=None):
+                    betabar=0.727, record_blocks=False, keep_steps=True, on_step=None, qsz=False):
+        """TwoChQS (SU(2), multiplets) or, with qsz=True, TwoChQSz ((Q, Sz) blocks, states)."""
         steps = []
 
         def cb(_u, st):
@@ -333,8 +336,14 @@ class Context:
         p = TwoChQSParams(U, Gamma1, Gamma2, ed, Lambda, nsitesmax, ncut, betabar, int(record_blocks))
         fn = STEP_CB(cb)
         ms = ctypes.c_double(0.0)
-        _check(lib().nrg_run_twochqs(self.h, ctypes.byref(p), fn, None, ctypes.byref(ms)))
+        run = lib().nrg_run_twochqsz if qsz else lib().nrg_run_twochqs
+        _check(run(self.h, ctypes.byref(p), fn, None, ctypes.byref(ms)))
         return steps, ms.value
+
+    def run_twochqsz(self, U=0.5, Gamma1=0.03, Gamma2=0.01, ed=-0.25, Lambda=2.5, nsitesmax=3, ncut=4000,
+                     betabar=0.727, record_blocks=False, keep_steps=True, on_step=None):
+        return self.run_twochqs(U, Gamma1, Gamma2, ed, Lambda, nsitesmax, ncut, betabar, record_blocks, keep_steps,
+                                on_step, qsz=True)
 
 
 def truncate(en, ncut, myeps=1e-18):

@@ -300,6 +300,81 @@ static double twoch_fn_el(const SiteDef& st, const TwoChTables& T, int ch, doubl
 }
 
 // ---------------------------------------------------------------------------
+// TwoChQSz plugin: TwoChQSz_SetSingleSite (src/TwoChQSz/TwoChQSz_SetSingleSite.cpp:5-85,
+// 16 one-state (Q, Sz) blocks), fd_table (src/TwoChQSz/TwoChQSz_OpTables.cpp:3-78),
+// H_N rules of TwoChQSz_DiagHN (src/TwoChQSz/TwoChQSz_DiagHN.cpp:74-97) and
+// TwoChQSz_SetH0Anderson (src/TwoChQSz/TwoChQSz_SetH0Anderson.cpp:180-215), f_ch
+// rule of TwoChQSz_UpdateMatrixAfterCutting (src/TwoChQSz/TwoChQSz_UpdateMatrices.cpp:288-289)
+// ---------------------------------------------------------------------------
+static SiteDef twochqsz_site() {
+  static const double q[16][2] = {{-2, 0}, {-1, 0.5}, {-1, -0.5}, {0, 0}, {-1, 0.5}, {0, 1}, {0, 0}, {1, 0.5},
+                                  {-1, -0.5}, {0, 0}, {0, -1}, {1, -0.5}, {0, 0}, {1, 0.5}, {1, -0.5}, {2, 0}};
+  SiteDef s;
+  for (int t = 0; t < 16; t++) s.add_block(q[t][0], std::fabs(q[t][1]), q[t][1], 1);
+  return s;
+}
+// <t_i| f^dagger_{ch,sigma} |t_j>; state bits 1 = ch2 up, 2 = ch2 down, 4 = ch1 up,
+// 8 = ch1 down; the table's signs: ch1 down -1 past ch1 up, ch2 up -1 past an odd
+// number of ch1 modes, ch2 down that sign and -1 past ch2 up
+static double qsz_fd(int ch, int sigma, int ti, int tj) {
+  if (ti < 0 || ti > 15 || tj < 0 || tj > 15) return 0.0;
+  const int bit = ch == 1 ? (sigma == 1 ? 4 : (sigma == -1 ? 8 : 0)) : (ch == 2 ? (sigma == 1 ? 1 : (sigma == -1 ? 2 : 0)) : 0);
+  if (!bit || (tj & bit) || ti != (tj | bit)) return 0.0;
+  if (bit == 4) return 1.0;
+  if (bit == 8) return (tj & 4) ? -1.0 : 1.0;
+  const double sg = (((tj >> 2) & 1) ^ ((tj >> 3) & 1)) ? -1.0 : 1.0;
+  if (bit == 1) return sg;
+  return (tj & 1) ? -sg : sg;
+}
+// H_N coupling: sum over sigma of fd(ch, sigma, typep, type) times the fermion sign
+// of the site state `type` (-1 for Sz = +-1/2; at N = 0 also Q = +-1)
+static double qsz_hn_coef(const SiteDef& st, int ch, int typep, int type, int h0) {
+  const double Qt = st.Q(type), Szt = st.Sz(type);
+  double fs = 1.0;
+  if (h0) { if ((deq(Qt, 1.0) || deq(Qt, -1.0)) && (deq(Szt, 0.5) || deq(Szt, -0.5))) fs = -1.0; }
+  else if (deq(Szt, 0.5) || deq(Szt, -0.5)) fs = -1.0;
+  double v = 0.0;
+  for (int sg = -1; sg <= 1; sg += 2) v += qsz_fd(ch, sg, typep, type) * fs;
+  return v;
+}
+static double qsz_fn_el(int ch, int ti, int tj) { return qsz_fd(ch, -1, tj, ti) + qsz_fd(ch, 1, tj, ti); }
+
+// QSz_BuildBasis (src/QSz_BuildBasis.cpp:9-117): blocks (Q_old+Q_site, Sz_old+Sz_site)
+// in first-appearance order; per new block: old block, site block, old state
+static void build_basis_qsz(const CNRGbasisarray& Ac, CNRGbasisarray& Ab, const SiteDef& site, int befcut) {
+  Ab = CNRGbasisarray();
+  Ab.Nshell = Ac.Nshell + 1;
+  Ab.NQNumbers = 2;
+  std::map<std::pair<long, long>, int> seen;
+  std::vector<std::pair<double, double>> qns;
+  for (int b = 0; b < Ac.NumBlocks(); b++)
+    for (int s = 0; s < site.nblocks(); s++) {
+      const double q = Ac.GetQNumber(b, 0) + site.bQ[s], sz = Ac.GetQNumber(b, 1) + site.bSz[s];
+      if (seen.emplace(std::make_pair(key2(q), key2(sz)), (int)qns.size()).second) qns.push_back({q, sz});
+    }
+  int ist = 0;
+  for (auto& qs : qns) {
+    Ab.QNumbers.push_back(qs.first);
+    Ab.QNumbers.push_back(qs.second);
+    Ab.BlockBegEnd.push_back(ist);
+    for (int bo = 0; bo < Ac.NumBlocks(); bo++) {
+      const double Qo = Ac.GetQNumber(bo, 0), So = Ac.GetQNumber(bo, 1);
+      for (int s = 0; s < site.nblocks(); s++) {
+        if (key2(qs.first) != key2(Qo + site.bQ[s]) || key2(qs.second) != key2(So + site.bSz[s])) continue;
+        for (int io = Ac.GetBlockLimit(bo, 0); io <= Ac.GetBlockLimit(bo, 1); io++)
+          for (int is = site.bbeg[s]; is <= site.bend[s]; is++) {
+            Ab.iType.push_back(is);
+            Ab.dEn.push_back(Ac.dEn[io]);
+            Ab.StCameFrom.push_back(befcut == 1 ? Ac.StCameFrom[io] : io);
+            ist++;
+          }
+      }
+    }
+    Ab.BlockBegEnd.push_back(ist - 1);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // basis construction.  New blocks in first-appearance order of (Q_old+Q_site,
 // S from S_old+S_site down to |S_old-S_site|) (FilterQNumbers keeps first
 // occurrences); a state (old state o, site state s) enters block (Q,S) when
@@ -409,6 +484,27 @@ static double calc_suscep(double bb, const CNRGarray& A) {
   }
   return s1 / s0;
 }
+// CalcSuscep / CalcEntropy with totalS false (src/CalcThermo.cpp:16-121): <Sz^2>
+// with Sz the block's second quantum number, degeneracy 1 per state
+static double calc_suscep_sz(double bb, const CNRGarray& A) {
+  double s0 = 0, s1 = 0;
+  for (int b = 0; b < A.NumBlocks(); b++) {
+    const double Sz = A.GetQNumber(b, 1);
+    for (int i = A.GetBlockLimit(b, 0); i <= A.GetBlockLimit(b, 1); i++) {
+      const double x = std::exp(-bb * A.dEn[i]);
+      s0 += x; s1 += Sz * Sz * x;
+    }
+  }
+  return s1 / s0;
+}
+static double calc_entropy_sz(double bb, const CNRGarray& A) {
+  double Z = 0, s = 0;
+  for (size_t i = 0; i < A.dEn.size(); i++) {
+    const double E = A.dEn[i], x = std::exp(-bb * E);
+    s += E * x; Z += x;
+  }
+  return bb * (s / Z) + std::log(Z);
+}
 static double calc_entropy(double bb, const CNRGarray& A) {
   double Z = 0, s = 0;
   for (int b = 0; b < A.NumBlocks(); b++) {
@@ -470,6 +566,7 @@ struct Chain {
   BasisDev oldb;                   // device view of the basis the current ops live on
   int zpar = 0;
   bool record = false;
+  bool sz_only = false;            // (Q, Sz) blocks: one coupling table (no spin index)
   std::vector<double> hrec, zrec;
 
   void init_ops(int nm) {
@@ -529,7 +626,7 @@ struct Chain {
     std::vector<int> bbeg(nb), sidx(nb);
     for (int b = 0; b < nb; b++) {
       bbeg[b] = Abasis.GetBlockLimit(b, 0);
-      sidx[b] = (int)key2(Abasis.GetQNumber(b, 1));
+      sidx[b] = sz_only ? 0 : (int)key2(Abasis.GetQNumber(b, 1));
     }
     int *d_bbeg, *d_bsize, *d_sidx, *d_stcf, *d_itype;
     int64_t* d_hoff;
@@ -1122,9 +1219,11 @@ static int run_onechqs_multi(nrg_ctx* c, int ninst, const nrg_onechqs_params* ps
 }
 
 // ---------------------------------------------------------------------------
-// TwoChQS Anderson: legacy standalone loop (src/TwoChQS/TwoChQS.cpp:204-495)
+// TwoChQS Anderson: legacy standalone loop (src/TwoChQS/TwoChQS.cpp:204-495);
+// qsz: TwoChQSz, the same loop on (Q, Sz) blocks (src/TwoChQSz/TwoChQSz.cpp:103-425)
 // ---------------------------------------------------------------------------
-static int run_twochqs(nrg_ctx* c, nrg_twochqs_params p, nrg_step_cb cb, void* user, double* timing_ms) {
+static int run_twochqs(nrg_ctx* c, nrg_twochqs_params p, nrg_step_cb cb, void* user, double* timing_ms,
+                       bool qsz = false) {
   const double L = p.Lambda;
   const double bbar = p.betabar > 0 ? p.betabar : 0.727;
   const double HLF = 0.5 * (1.0 + 1.0 / L);
@@ -1132,17 +1231,21 @@ static int run_twochqs(nrg_ctx* c, nrg_twochqs_params p, nrg_step_cb cb, void* u
   static const TwoChTables T;
   Chain ch;
   ch.c = c;
-  ch.site = twochqs_site();
+  ch.site = qsz ? twochqsz_site() : twochqs_site();
+  ch.sz_only = qsz;
   ch.ntypes = 16;
   ch.record = p.record_blocks != 0;
   const SiteDef& st = ch.site;
-  // operators: f_ch1, f_ch2 (TwoChQS_UpdateMatrixAfterCutting), plus d for H_0
+  // operators: f_ch1, f_ch2 (TwoChQS(z)_UpdateMatrixAfterCutting), plus d for H_0
   ch.specs.resize(3);
   for (int k = 0; k < 2; k++) {
     ch.specs[k].rule = PAIR_QS_UP_UPPER;
-    ch.specs[k].g = [&st, k](double Si, double Sj, int ti, int tj) {
-      return twoch_fn_el(st, T, k + 1, Si, Sj, ti, tj);
-    };
+    if (qsz)
+      ch.specs[k].g = [k](double, double, int ti, int tj) { return qsz_fn_el(k + 1, ti, tj); };
+    else
+      ch.specs[k].g = [&st, k](double Si, double Sj, int ti, int tj) {
+        return twoch_fn_el(st, T, k + 1, Si, Sj, ti, tj);
+      };
   }
   ch.specs[2].rule = PAIR_QS_UP_UPPER;
   ch.specs[2].g = [](double, double, int, int) { return 0.0; };
@@ -1152,29 +1255,49 @@ static int run_twochqs(nrg_ctx* c, nrg_twochqs_params p, nrg_step_cb cb, void* u
   double G1t = p.Gamma1 * (2.0 / M_PI) / (HLF * HLF), G2t = p.Gamma2 * (2.0 / M_PI) / (HLF * HLF);
   double Un = Ut / std::sqrt(L), edn = edt / std::sqrt(L);
   double chi0[2] = {std::sqrt(G1t / L), std::sqrt(G2t / L)};
-  // TwoChQS_SetH0Anderson: impurity (N=-1) states and d, basis with the first
+  // TwoChQS(z)_SetH0Anderson: impurity (N=-1) states and d, basis with the first
   // site of both channels, H_0 assembled with the H_0 fermion-sign rule and no
   // sqrt(Lambda) on the diagonal
   ch.AeigCut = CNRGbasisarray();
   ch.AeigCut.Nshell = -1; ch.AeigCut.NQNumbers = 2;
-  ch.AeigCut.QNumbers = {-1, 0, 0, 0.5, 1, 0};
-  ch.AeigCut.BlockBegEnd = {0, 0, 1, 1, 2, 2};
-  ch.AeigCut.BlockBegEndBC = {0, 0, 1, 1, 2, 2};
-  ch.AeigCut.dEn = {Un, edn + Un, 2.0 * edn + 3.0 * Un};
-  ch.AeigCut.StCameFrom = {0, 1, 2};
-  ch.AeigCut.zoff = {0, 1, 2};
-  ch.AeigCut.zld = {1, 1, 1};
-  if ((rc = ch.set_op(2, {0, 1, 1, 2}, {0, 0, 1, 1}, {1.0, -std::sqrt(2.0)}))) return rc;
+  if (qsz) {   // src/TwoChQSz/TwoChQSz_SetH0Anderson.cpp:24-135
+    ch.AeigCut.QNumbers = {-1, 0, 0, 0.5, 0, -0.5, 1, 0};
+    ch.AeigCut.BlockBegEnd = {0, 0, 1, 1, 2, 2, 3, 3};
+    ch.AeigCut.BlockBegEndBC = {0, 0, 1, 1, 2, 2, 3, 3};
+    ch.AeigCut.dEn = {Un, edn + Un, edn + Un, 2.0 * edn + 3.0 * Un};
+    ch.AeigCut.StCameFrom = {0, 1, 2, 3};
+    ch.AeigCut.zoff = {0, 1, 2, 3};
+    ch.AeigCut.zld = {1, 1, 1, 1};
+    if ((rc = ch.set_op(2, {0, 1, 0, 2, 1, 3, 2, 3}, {0, 0, 1, 1, 2, 2, 3, 3}, {1.0, 1.0, -1.0, 1.0}))) return rc;
+  } else {
+    ch.AeigCut.QNumbers = {-1, 0, 0, 0.5, 1, 0};
+    ch.AeigCut.BlockBegEnd = {0, 0, 1, 1, 2, 2};
+    ch.AeigCut.BlockBegEndBC = {0, 0, 1, 1, 2, 2};
+    ch.AeigCut.dEn = {Un, edn + Un, 2.0 * edn + 3.0 * Un};
+    ch.AeigCut.StCameFrom = {0, 1, 2};
+    ch.AeigCut.zoff = {0, 1, 2};
+    ch.AeigCut.zld = {1, 1, 1};
+    if ((rc = ch.set_op(2, {0, 1, 1, 2}, {0, 0, 1, 1}, {1.0, -std::sqrt(2.0)}))) return rc;
+  }
   if ((rc = make_basis_dev(c, ch.AeigCut, (ch.tag + "oldb").c_str(), ch.oldb))) return rc;
-  build_basis(ch.AeigCut, ch.Abasis, st, 1, 1);
+  auto basis = [&](int befcut) {
+    if (qsz) build_basis_qsz(ch.AeigCut, ch.Abasis, st, befcut);
+    else build_basis(ch.AeigCut, ch.Abasis, st, 1, befcut);
+  };
+  basis(1);
   CoefTable h0_tab[2], hn_tab[2];
   for (int k = 0; k < 2; k++) {
     h0_tab[k].ntypes = hn_tab[k].ntypes = 16;
-    h0_tab[k].f = [&st, k](double S, int a, int b) { return twoch_hn_coef(st, T, k + 1, S, a, b, 1); };
-    hn_tab[k].f = [&st, k](double S, int a, int b) { return twoch_hn_coef(st, T, k + 1, S, a, b, 0); };
+    if (qsz) {
+      h0_tab[k].f = [&st, k](double, int a, int b) { return qsz_hn_coef(st, k + 1, a, b, 1); };
+      hn_tab[k].f = [&st, k](double, int a, int b) { return qsz_hn_coef(st, k + 1, a, b, 0); };
+    } else {
+      h0_tab[k].f = [&st, k](double S, int a, int b) { return twoch_hn_coef(st, T, k + 1, S, a, b, 1); };
+      hn_tab[k].f = [&st, k](double S, int a, int b) { return twoch_hn_coef(st, T, k + 1, S, a, b, 0); };
+    }
   }
   {
-    int smax = max_sidx(ch.Abasis);
+    int smax = qsz ? 0 : max_sidx(ch.Abasis);
     std::vector<double> coef[2];
     for (int k = 0; k < 2; k++) coef[k] = h0_tab[k].upto(smax);
     int fops[2] = {2, 2};
@@ -1184,11 +1307,14 @@ static int run_twochqs(nrg_ctx* c, nrg_twochqs_params p, nrg_step_cb cb, void* u
   double emin = *std::min_element(ch.Aeig.dEn.begin(), ch.Aeig.dEn.end());
   for (auto& x : ch.Aeig.dEn) x -= emin;
   if (p.ncut < ch.Aeig.Nstates())
-    return fail(NRG_ERR_ARG, "nrg_run_twochqs: Ncut is below the N=0 basis size; the reference's "
-                             "TwoChQS_UpdateQm1fQ indexes the uncut basis there");
+    return fail(NRG_ERR_ARG, qsz ? "nrg_run_twochqsz: Ncut is below the N=0 basis size; the reference's "
+                                   "TwoChQSz_UpdateQm1fQ indexes the uncut basis there"
+                                 : "nrg_run_twochqs: Ncut is below the N=0 basis size; the reference's "
+                                   "TwoChQS_UpdateQm1fQ indexes the uncut basis there");
   auto finish_step = [&](double emin_) -> int {
-    double thT[4] = {DN / bbar, 0, 0, 0}, sus[4] = {calc_suscep(bbar, ch.Aeig), 0, 0, 0};
-    double ent[4] = {calc_entropy(bbar, ch.Aeig), 0, 0, 0};
+    double thT[4] = {DN / bbar, 0, 0, 0};
+    double sus[4] = {qsz ? calc_suscep_sz(bbar, ch.Aeig) : calc_suscep(bbar, ch.Aeig), 0, 0, 0};
+    double ent[4] = {qsz ? calc_entropy_sz(bbar, ch.Aeig) : calc_entropy(bbar, ch.Aeig), 0, 0, 0};
     double ec = (p.ncut < ch.Aeig.Nstates()) ? ecut_of(ch.Aeig.dEn, p.ncut) : HUGE_VAL;
     const int ns = (int)ch.Aeig.dEn.size();
     ch.Aeig.Kept.assign(ns, 0);
@@ -1209,8 +1335,8 @@ static int run_twochqs(nrg_ctx* c, nrg_twochqs_params p, nrg_step_cb cb, void* u
     double d0 = 1.0 - std::pow(L, -N), d1 = std::sqrt(1.0 - std::pow(L, -(2 * N - 1)));
     double d2 = std::sqrt(1.0 - std::pow(L, -(2 * N + 1)));
     double chiN[2] = {d0 / (d1 * d2), d0 / (d1 * d2)};
-    build_basis(ch.AeigCut, ch.Abasis, st, 1, 0);
-    int smax = max_sidx(ch.Abasis);
+    basis(0);
+    int smax = qsz ? 0 : max_sidx(ch.Abasis);
     std::vector<double> coef[2];
     for (int k = 0; k < 2; k++) coef[k] = hn_tab[k].upto(smax);
     int fops[2] = {0, 1};
@@ -1238,6 +1364,15 @@ extern "C" int nrg_run_onechqs(nrg_ctx* c, const nrg_onechqs_params* p, nrg_step
     return nrg::fail(NRG_ERR_ARG, "nrg_run_onechqs: bad parameters");
   HIPCHK(hipSetDevice(c->device));
   return nrg::run_onechqs(c, *p, cb, user, timing_ms);
+}
+
+extern "C" int nrg_run_twochqsz(nrg_ctx* c, const nrg_twochqs_params* p, nrg_step_cb cb, void* user,
+                                double* timing_ms) {
+  if (!c || !p) return nrg::fail(NRG_ERR_ARG, "nrg_run_twochqsz: NULL argument");
+  if (p->nsitesmax < 0 || p->ncut < 1 || p->Lambda <= 1.0)
+    return nrg::fail(NRG_ERR_ARG, "nrg_run_twochqsz: bad parameters");
+  HIPCHK(hipSetDevice(c->device));
+  return nrg::run_twochqs(c, *p, cb, user, timing_ms, true);
 }
 
 extern "C" int nrg_run_onechqs_multi(nrg_ctx* c, int ninst, const nrg_onechqs_params* p, nrg_multi_cb cb,
