@@ -165,6 +165,22 @@ typedef struct {
 int nrg_run_onechqs(nrg_ctx *ctx, const nrg_onechqs_params *p, nrg_step_cb cb, void *user, double *timing_ms);
 
 /*
+ * Impurity thermodynamics of NRG_main's CalcThermo (src/NRGcodehandlerClass.cpp:627-652)
+ * with its chain subtraction (CNRGthermo::AddValue / SaveNValue,
+ * src/NRGthermoClass.cpp:10-151): two device chains with the parameters of p,
+ * calcdens 2 -- the chain run with the impurity decoupled (U = ed = Gamma = 0),
+ * whose values minus the free-orbital 1/8 and 2 ln 2 (dImpValue,
+ * src/CodeHandler_ModelSwitch.cpp:217-220) are what the chain files
+ * SuscepChain1Ch_QS*.dat / EntropyChain1Ch_QS*.dat hold -- and the impurity run.
+ * Writes 4 rows per step N = 0 .. nsitesmax-1 (betabar_nc, nc = 1..4; row
+ * 4N+nc-1 = the reference's Nvalue) of 7 columns:
+ *   T, Tchi_tot, Tchi_chain, Tchi_imp, S_tot, S_chain, S_imp
+ * i.e. the four columns of SuscepImp1ChQS_Anderson*.dat and of
+ * EntropyImp1ChQS_Anderson*.dat.  out may be NULL to query *nrows.
+ */
+int nrg_onechqs_thermo(nrg_ctx *ctx, const nrg_onechqs_params *p, double *out, int cap_rows, int *nrows);
+
+/*
  * Many independent OneChQS instances in lockstep (configuration of 64 seeded
  * Anderson impurities): per step all instances' H_N blocks go into one batched
  * eigensolve.  p is an array of ninst parameter sets (same nsitesmax); cb

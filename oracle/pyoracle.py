@@ -148,6 +148,27 @@ def run_twochqs(U=0.4, Gamma1=0.03, Gamma2=0.03, ed=-0.2, Lambda=3.0, nsitesmax=
     return steps
 
 
+def onechqs_thermo(U=0.5, Gamma=0.02, ed=-0.25, Lambda=2.5, nsitesmax=40, ncut=400, disc_scheme=0,
+                   myeps=0.0, betabar=0.727):
+    """Impurity thermodynamics of CalcThermo with the chain subtraction
+    (src/NRGcodehandlerClass.cpp:627-652, src/NRGthermoClass.cpp:10-151): the
+    chain run (U = ed = Gamma = 0) minus the free orbital's 1/8 and 2 ln 2
+    (src/CodeHandler_ModelSwitch.cpp:217-220), then the impurity run; rows
+    (4 per step N >= 0) of T, Tchi_tot, Tchi_chain, Tchi_imp, S_tot, S_chain, S_imp."""
+    kw = dict(Lambda=Lambda, nsitesmax=nsitesmax, ncut=ncut, calcdens=2, disc_scheme=disc_scheme, myeps=myeps,
+              betabar=betabar)
+    ch = [s for s in run_onechqs(U=0.0, Gamma=0.0, ed=0.0, **kw) if s["N"] >= 0]
+    im = [s for s in run_onechqs(U=U, Gamma=Gamma, ed=ed, **kw) if s["N"] >= 0]
+    rows = []
+    for a, b in zip(im, ch):
+        for nc in range(4):
+            cchi = b["suscep"][nc] - 1.0 / 8.0
+            cS = b["entropy"][nc] - 2.0 * np.log(2.0)
+            rows.append([a["thermo_T"][nc], a["suscep"][nc], cchi, a["suscep"][nc] - cchi,
+                         a["entropy"][nc], cS, a["entropy"][nc] - cS])
+    return np.array(rows)
+
+
 def run_twochqsz(U=0.5, Gamma1=0.03, Gamma2=0.01, ed=-0.25, Lambda=2.5, nsitesmax=3, ncut=4000,
                  betabar=0.727, record_blocks=False):
     """TwoChQSz standalone driver: (Q, Sz) blocks, parameters as run_twochqs."""

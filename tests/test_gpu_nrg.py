@@ -120,3 +120,15 @@ def test_twochqsz_h0_blocks(ctx):
     os_ = po.run_twochqsz(**kw, record_blocks=True)
     _compare(gs, os_)
     assert gs[1]["hpacked"].shape == os_[1]["hpacked"].shape
+
+
+def test_impurity_thermo(ctx):
+    # CalcThermo with the chain subtraction: the device's two chains against the
+    # oracle's, every (N, betabar_nc) row
+    kw = dict(U=0.5, Gamma=0.02, ed=-0.25, Lambda=2.5, nsitesmax=24, ncut=300)
+    print("nrg impurity thermo", kw, flush=True)
+    g = ctx.onechqs_thermo(**kw)
+    o = po.onechqs_thermo(**kw)
+    assert g.shape == o.shape == (96, 7)
+    np.testing.assert_allclose(g[:, 0], o[:, 0], rtol=1e-15, atol=0)
+    np.testing.assert_allclose(g[:, 1:], o[:, 1:], rtol=1e-8, atol=1e-10)

@@ -20,7 +20,7 @@ EXPORTS = [
     "nrg_set_eig_impl", "nrg_set_plain", "nrg_eig2_stages",
     "nrg_dm_create", "nrg_dm_destroy", "nrg_dm_info", "nrg_dm_timing", "nrg_dm_rho", "nrg_dm_spec",
     "nrg_dm_spectrum", "nrg_run_onechqs_multi", "nrg_dm_spec_omega", "nrg_dm_norm", "nrg_dm_weights",
-    "nrg_run_twochqsz",
+    "nrg_run_twochqsz", "nrg_onechqs_thermo",
 ]
 
 KSTAT_NAMES = ["eig", "hn", "fbasis", "gemm", "tdb_symv"]
@@ -124,6 +124,7 @@ def lib():
     L.nrg_run_onechqs.argtypes = [ctypes.c_void_p, ctypes.POINTER(OneChQSParams), STEP_CB, ctypes.c_void_p, _dp]
     L.nrg_run_twochqs.argtypes = [ctypes.c_void_p, ctypes.POINTER(TwoChQSParams), STEP_CB, ctypes.c_void_p, _dp]
     L.nrg_run_twochqsz.argtypes = [ctypes.c_void_p, ctypes.POINTER(TwoChQSParams), STEP_CB, ctypes.c_void_p, _dp]
+    L.nrg_onechqs_thermo.argtypes = [ctypes.c_void_p, ctypes.POINTER(OneChQSParams), _dp, ctypes.c_int, _ip]
     L.nrg_run_onechqs_multi.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(OneChQSParams), MULTI_CB,
                                         ctypes.c_void_p, _dp]
     L.nrg_dm_create.argtypes = [ctypes.c_void_p, ctypes.POINTER(DmParams), ctypes.POINTER(ctypes.c_void_p)]
@@ -290,6 +291,17 @@ class Context:
         _check(lib().nrg_run_onechqs(self.h, ctypes.byref(p), fn, None, ctypes.byref(ms)))
         return steps, ms.value
 
+
+    def onechqs_thermo(self, U=0.5, Gamma=0.02, ed=-0.25, Lambda=2.5, nsitesmax=40, ncut=400, disc_scheme=0,
+                       myeps=1e-18, betabar=0.727):
+        """Impurity thermodynamics with the chain subtraction (nrg_onechqs_thermo):
+        rows (4 per step) of T, Tchi_tot, Tchi_chain, Tchi_imp, S_tot, S_chain, S_imp."""
+        p = OneChQSParams(U, Gamma, ed, Lambda, nsitesmax, ncut, 2, disc_scheme, myeps, betabar, 0)
+        n = ctypes.c_int()
+        _check(lib().nrg_onechqs_thermo(self.h, ctypes.byref(p), None, 0, ctypes.byref(n)))
+        out = np.zeros((n.value, 7))
+        _check(lib().nrg_onechqs_thermo(self.h, ctypes.byref(p), _ptr(out), n.value, ctypes.byref(n)))
+        return out
 
     def run_onechqs_multi(self, instances, nsitesmax, calcdens=0, disc_scheme=0, myeps=1e-18, betabar=0.727,
                           keep_steps=True, on_step=None):

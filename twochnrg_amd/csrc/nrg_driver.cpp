@@ -1375,6 +1375,47 @@ extern "C" int nrg_run_twochqsz(nrg_ctx* c, const nrg_twochqs_params* p, nrg_ste
   return nrg::run_twochqs(c, *p, cb, user, timing_ms, true);
 }
 
+extern "C" int nrg_onechqs_thermo(nrg_ctx* c, const nrg_onechqs_params* p, double* out, int cap_rows, int* nrows) {
+  if (!c || !p || !nrows) return nrg::fail(NRG_ERR_ARG, "nrg_onechqs_thermo: NULL argument");
+  if (p->nsitesmax < 1 || p->ncut < 1 || p->Lambda <= 1.0)
+    return nrg::fail(NRG_ERR_ARG, "nrg_onechqs_thermo: bad parameters");
+  const int rows = 4 * p->nsitesmax;
+  *nrows = rows;
+  if (!out) return NRG_OK;
+  if (cap_rows < rows) return nrg::fail(NRG_ERR_ARG, "nrg_onechqs_thermo: output too small");
+  HIPCHK(hipSetDevice(c->device));
+  struct Acc {
+    std::vector<double> T, chi, S;
+  } imp, chain;
+  auto collect = [](void* u, const nrg_step* st) {
+    if (st->nshell < 0) return;   // CalcThermo runs for N >= Nsites0 = 0
+    Acc* a = (Acc*)u;
+    for (int nc = 0; nc < 4; nc++) {
+      a->T.push_back(st->thermo_T[nc]);
+      a->chi.push_back(st->suscep[nc]);
+      a->S.push_back(st->entropy[nc]);
+    }
+  };
+  nrg_onechqs_params q = *p;
+  q.calcdens = 2;
+  q.record_blocks = 0;
+  nrg_onechqs_params qc = q;
+  qc.U = 0.0; qc.ed = 0.0; qc.Gamma = 0.0;
+  int rc;
+  if ((rc = nrg::run_onechqs(c, qc, collect, &chain, nullptr))) return rc;
+  if ((rc = nrg::run_onechqs(c, q, collect, &imp, nullptr))) return rc;
+  if ((int)imp.T.size() != rows || (int)chain.T.size() != rows)
+    return nrg::fail(NRG_ERR_STATE, "nrg_onechqs_thermo: step count");
+  for (int r = 0; r < rows; r++) {
+    const double cchi = chain.chi[r] - 1.0 / 8.0, cS = chain.S[r] - 2.0 * std::log(2.0);
+    double* o = out + (size_t)7 * r;
+    o[0] = imp.T[r];
+    o[1] = imp.chi[r]; o[2] = cchi; o[3] = imp.chi[r] - cchi;
+    o[4] = imp.S[r]; o[5] = cS; o[6] = imp.S[r] - cS;
+  }
+  return NRG_OK;
+}
+
 extern "C" int nrg_run_onechqs_multi(nrg_ctx* c, int ninst, const nrg_onechqs_params* p, nrg_multi_cb cb,
                                      void* user, double* timing_ms) {
   if (!c || !p || ninst < 1) return nrg::fail(NRG_ERR_ARG, "nrg_run_onechqs_multi: bad arguments");
