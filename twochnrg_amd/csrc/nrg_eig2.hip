@@ -78,10 +78,19 @@ __global__ __launch_bounds__(E2_T) void tridiag_small_kernel(const E2Block* __re
     const double t = 2.0 / (nrm2 - x0 * x0 + v0 * v0);
     if (tid < n) vsh[tid] = (tid > k + 1) ? rowk[tid] : (tid == k + 1 ? v0 : 0.0);
     __syncthreads();
-    // q_i = t * sum_j A[j][i] v_j  (thread i owns column i)
+    // q_i = t * sum_j A[j][i] v_j  (thread i owns column i); 16 rows' loads in
+    // flight per thread, products accumulated in row order
     double qi = 0.0;
     if (tid > k && tid < n) {
-      for (int j = k + 1; j < n; j++) qi += A[(size_t)j * n + tid] * vsh[j];
+      int j = k + 1;
+      for (; j + 16 <= n; j += 16) {
+        double a[16];
+#pragma unroll
+        for (int u = 0; u < 16; u++) a[u] = A[(size_t)(j + u) * n + tid];
+#pragma unroll
+        for (int u = 0; u < 16; u++) qi += a[u] * vsh[j + u];
+      }
+      for (; j < n; j++) qi += A[(size_t)j * n + tid] * vsh[j];
       qi *= t;
     }
     const double vp = block_sum((tid > k && tid < n) ? qi * vsh[tid] : 0.0, red);
@@ -89,8 +98,24 @@ __global__ __launch_bounds__(E2_T) void tridiag_small_kernel(const E2Block* __re
     if (tid < n) qsh[tid] = (tid > k) ? qi - K * vsh[tid] : 0.0;
     __syncthreads();
     if (tid > k && tid < n) {
+      // rank-2 update of column tid in batches of 8 rows: the in-place stores
+      // would otherwise keep the compiler from issuing the next row's load early
       const double vi = vsh[tid], q2 = qsh[tid];
-      for (int j = k + 1; j < n; j++) {
+      int j = k + 1;
+      for (; j + 8 <= n; j += 8) {
+        double a[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) a[u] = A[(size_t)(j + u) * n + tid];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+          const int jj = j + u;
+          const double vj = vsh[jj], qj = qsh[jj];
+          a[u] -= (jj <= tid) ? __fma_rn(vj, q2, vi * qj) : __fma_rn(vi, qj, vj * q2);
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++) A[(size_t)(j + u) * n + tid] = a[u];
+      }
+      for (; j < n; j++) {
         const double vj = vsh[j], qj = qsh[j];
         const double upd = (j <= tid) ? __fma_rn(vj, q2, vi * qj) : __fma_rn(vi, qj, vj * q2);
         A[(size_t)j * n + tid] -= upd;
@@ -224,11 +249,19 @@ __global__ __launch_bounds__(CHUNK) void tdb_symv_kernel(const E2Block* __restri
       const double* col = P.A + (size_t)r0 * n + i;
       double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
       int l = 0;
-      for (; l + 4 <= nr; l += 4) {
-        s0 += col[(size_t)l * n] * vs[l];
-        s1 += col[(size_t)(l + 1) * n] * vs[l + 1];
-        s2 += col[(size_t)(l + 2) * n] * vs[l + 2];
-        s3 += col[(size_t)(l + 3) * n] * vs[l + 3];
+      // 16 rows' loads issued before their products: enough reads in flight
+      // per wavefront to cover the HBM latency at this occupancy
+      for (; l + 16 <= nr; l += 16) {
+        double a[16];
+#pragma unroll
+        for (int u = 0; u < 16; u++) a[u] = col[(size_t)(l + u) * n];
+#pragma unroll
+        for (int u = 0; u < 16; u += 4) {
+          s0 += a[u] * vs[l + u];
+          s1 += a[u + 1] * vs[l + u + 1];
+          s2 += a[u + 2] * vs[l + u + 2];
+          s3 += a[u + 3] * vs[l + u + 3];
+        }
       }
       for (; l < nr; l++) s0 += col[(size_t)l * n] * vs[l];
       s = (s0 + s1) + (s2 + s3);
@@ -308,8 +341,24 @@ __global__ __launch_bounds__(CHUNK) void tdb_fin_kernel(const E2Block* __restric
   if (i < n) {
     vi = (i == k + 1) ? v0 : rowk[i];
     double y = 0.0;
-    for (int rt = 0; rt < RT; rt++) y += P.part[(size_t)rt * n + i];
-    for (int p = 0; p < j; p++) y -= P.X[(size_t)p * n + i] * c1s[p] + P.X[(size_t)(PANEL + p) * n + i] * c2s[p];
+    int rt = 0;
+    for (; rt + 8 <= RT; rt += 8) {   // eight partials in flight, summed in row-tile order
+      double a[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) a[u] = P.part[(size_t)(rt + u) * n + i];
+#pragma unroll
+      for (int u = 0; u < 8; u++) y += a[u];
+    }
+    for (; rt < RT; rt++) y += P.part[(size_t)rt * n + i];
+    int p = 0;
+    for (; p + 4 <= j; p += 4) {
+      double a[4], b[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) { a[u] = P.X[(size_t)(p + u) * n + i]; b[u] = P.X[(size_t)(PANEL + p + u) * n + i]; }
+#pragma unroll
+      for (int u = 0; u < 4; u++) y -= a[u] * c1s[p + u] + b[u] * c2s[p + u];
+    }
+    for (; p < j; p++) y -= P.X[(size_t)p * n + i] * c1s[p] + P.X[(size_t)(PANEL + p) * n + i] * c2s[p];
     wi = t * y + alpha2 * vi;
   }
   const bool panel_end = (j == PANEL - 1) || (k + 3 >= n);
@@ -340,8 +389,15 @@ __global__ __launch_bounds__(CHUNK) void tdb_fin_kernel(const E2Block* __restric
   double a = 0.0;
   if (i < n) {
     a = P.A[(size_t)(k + 1) * n + i];
-    for (int p = 0; p < j; p++)
-      a -= P.X[(size_t)p * n + i] * wk1[p] + P.X[(size_t)(PANEL + p) * n + i] * vk1[p];
+    int p = 0;
+    for (; p + 4 <= j; p += 4) {
+      double x1[4], x2[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) { x1[u] = P.X[(size_t)(p + u) * n + i]; x2[u] = P.X[(size_t)(PANEL + p + u) * n + i]; }
+#pragma unroll
+      for (int u = 0; u < 4; u++) a -= x1[u] * wk1[p + u] + x2[u] * vk1[p + u];
+    }
+    for (; p < j; p++) a -= P.X[(size_t)p * n + i] * wk1[p] + P.X[(size_t)(PANEL + p) * n + i] * vk1[p];
     a -= vi * wk1[j] + wi * vk1[j];
     P.A[(size_t)(k + 1) * n + i] = a;
   }
