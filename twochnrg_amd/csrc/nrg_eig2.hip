@@ -55,76 +55,108 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
 __global__ __launch_bounds__(E2_T) void tridiag_small_kernel(const E2Block* __restrict__ blks,
                                                               const int* __restrict__ list, double* __restrict__ A0,
                                                               double* __restrict__ WS) {
+  // Householder reduction, one workgroup per block, thread i owning column i.
+  // The rank-2 update of column k-1 (A -= v w^T + w v^T) is applied inside the
+  // pass of column k that also forms A v: every trailing element is read once,
+  // updated, written and used in the product, instead of the separate product
+  // and update passes.  The arithmetic is that of the two-pass form: row k
+  // gets the pending update before its reflector is formed, and the update of
+  // element (j, i) uses the same operations as that of (i, j).
   __shared__ double red[E2_T / WAVE];
-  __shared__ double vsh[256];
-  __shared__ double qsh[256];
+  __shared__ double vsh[256], wsh[256], vnew[256], ash[256];
   const E2Block B = blks[list[blockIdx.x]];
   const int n = B.n, tid = threadIdx.x;
   double* A = A0 + B.a_off;
   double* d = WS + B.v_off;
   double* e = d + n;
   double* tau = e + n;
+  bool pending = false;   // vsh/wsh hold the update of the previous column
+  double vi_old = 0.0, wi_old = 0.0;
+  vsh[tid] = 0.0;
+  wsh[tid] = 0.0;
+  __syncthreads();
   for (int k = 0; k + 2 < n; k++) {
-    const double* rowk = A + (size_t)k * n;
-    double x = (tid > k && tid < n) ? rowk[tid] : 0.0;
-    const double nrm2 = block_sum(x * x, red);
-    const double x0 = rowk[k + 1];
-    if (nrm2 == 0.0) {
-      if (tid == 0) { d[k] = rowk[k]; e[k] = 0.0; tau[k] = 0.0; }
-      __syncthreads();
-      continue;
+    // row k of the current matrix
+    double ak = 0.0;
+    if (tid >= k && tid < n) {
+      ak = A[(size_t)k * n + tid];
+      if (pending) {
+        const double vj = vsh[k], qj = wsh[k];
+        ak -= (k <= tid) ? __fma_rn(vj, wi_old, vi_old * qj) : __fma_rn(vi_old, qj, vj * wi_old);
+      }
     }
-    const double nrm = sqrt(nrm2), alpha = (x0 > 0.0) ? -nrm : nrm, v0 = x0 - alpha;
-    const double t = 2.0 / (nrm2 - x0 * x0 + v0 * v0);
-    if (tid < n) vsh[tid] = (tid > k + 1) ? rowk[tid] : (tid == k + 1 ? v0 : 0.0);
+    ash[tid] = ak;
+    const double x = (tid > k && tid < n) ? ak : 0.0;
+    const double nrm2 = block_sum(x * x, red);   // its barriers order the ash stores
+    const double x0 = ash[k + 1];
+    double t = 0.0, v0 = x0, alpha = 0.0;
+    if (nrm2 != 0.0) {
+      const double nrm = sqrt(nrm2);
+      alpha = (x0 > 0.0) ? -nrm : nrm;
+      v0 = x0 - alpha;
+      t = 2.0 / (nrm2 - x0 * x0 + v0 * v0);
+    }
+    const double vi = (tid > k + 1 && tid < n) ? ak : (tid == k + 1 ? v0 : 0.0);
+    vnew[tid] = vi;
     __syncthreads();
-    // q_i = t * sum_j A[j][i] v_j  (thread i owns column i); 16 rows' loads in
-    // flight per thread, products accumulated in row order
+    // rows > k: apply the pending update, write back, accumulate q_i = t sum_j A_ji v_j
     double qi = 0.0;
     if (tid > k && tid < n) {
       int j = k + 1;
-      for (; j + 16 <= n; j += 16) {
-        double a[16];
+      if (pending) {
+        for (; j + 8 <= n; j += 8) {
+          double a[8];
 #pragma unroll
-        for (int u = 0; u < 16; u++) a[u] = A[(size_t)(j + u) * n + tid];
+          for (int u = 0; u < 8; u++) a[u] = A[(size_t)(j + u) * n + tid];
 #pragma unroll
-        for (int u = 0; u < 16; u++) qi += a[u] * vsh[j + u];
+          for (int u = 0; u < 8; u++) {
+            const int jj = j + u;
+            const double vj = vsh[jj], qj = wsh[jj];
+            a[u] -= (jj <= tid) ? __fma_rn(vj, wi_old, vi_old * qj) : __fma_rn(vi_old, qj, vj * wi_old);
+            qi += a[u] * vnew[jj];
+          }
+#pragma unroll
+          for (int u = 0; u < 8; u++) A[(size_t)(j + u) * n + tid] = a[u];
+        }
+        for (; j < n; j++) {
+          const double vj = vsh[j], qj = wsh[j];
+          double a = A[(size_t)j * n + tid];
+          a -= (j <= tid) ? __fma_rn(vj, wi_old, vi_old * qj) : __fma_rn(vi_old, qj, vj * wi_old);
+          A[(size_t)j * n + tid] = a;
+          qi += a * vnew[j];
+        }
+      } else {
+        for (; j + 16 <= n; j += 16) {
+          double a[16];
+#pragma unroll
+          for (int u = 0; u < 16; u++) a[u] = A[(size_t)(j + u) * n + tid];
+#pragma unroll
+          for (int u = 0; u < 16; u++) qi += a[u] * vnew[j + u];
+        }
+        for (; j < n; j++) qi += A[(size_t)j * n + tid] * vnew[j];
       }
-      for (; j < n; j++) qi += A[(size_t)j * n + tid] * vsh[j];
       qi *= t;
     }
-    const double vp = block_sum((tid > k && tid < n) ? qi * vsh[tid] : 0.0, red);
+    const double vp = block_sum((tid > k && tid < n) ? qi * vi : 0.0, red);   // barriers: pass done
     const double K = 0.5 * t * vp;
-    if (tid < n) qsh[tid] = (tid > k) ? qi - K * vsh[tid] : 0.0;
-    __syncthreads();
-    if (tid > k && tid < n) {
-      // rank-2 update of column tid in batches of 8 rows: the in-place stores
-      // would otherwise keep the compiler from issuing the next row's load early
-      const double vi = vsh[tid], q2 = qsh[tid];
-      int j = k + 1;
-      for (; j + 8 <= n; j += 8) {
-        double a[8];
-#pragma unroll
-        for (int u = 0; u < 8; u++) a[u] = A[(size_t)(j + u) * n + tid];
-#pragma unroll
-        for (int u = 0; u < 8; u++) {
-          const int jj = j + u;
-          const double vj = vsh[jj], qj = qsh[jj];
-          a[u] -= (jj <= tid) ? __fma_rn(vj, q2, vi * qj) : __fma_rn(vi, qj, vj * q2);
-        }
-#pragma unroll
-        for (int u = 0; u < 8; u++) A[(size_t)(j + u) * n + tid] = a[u];
-      }
-      for (; j < n; j++) {
-        const double vj = vsh[j], qj = qsh[j];
-        const double upd = (j <= tid) ? __fma_rn(vj, q2, vi * qj) : __fma_rn(vi, qj, vj * q2);
-        A[(size_t)j * n + tid] -= upd;
-      }
-    }
-    __syncthreads();
-    if (tid == 0) { A[(size_t)k * n + k + 1] = v0; d[k] = A[(size_t)k * n + k]; e[k] = alpha; tau[k] = t; }
+    const double wi = (tid > k && tid < n) ? qi - K * vi : 0.0;
+    vsh[tid] = vi;
+    wsh[tid] = wi;
+    vi_old = vi;
+    wi_old = wi;
+    pending = true;
+    if (tid == k) { d[k] = ak; e[k] = alpha; tau[k] = t; }
+    if (tid > k && tid < n) A[(size_t)k * n + tid] = vi;   // reflector k in row k
     __syncthreads();
   }
+  // the last update reaches the trailing 2 x 2 block
+  if (n >= 2 && pending && tid >= n - 2 && tid < n) {
+    for (int j = n - 2; j < n; j++) {
+      const double vj = vsh[j], qj = wsh[j];
+      A[(size_t)j * n + tid] -= (j <= tid) ? __fma_rn(vj, wi_old, vi_old * qj) : __fma_rn(vi_old, qj, vj * wi_old);
+    }
+  }
+  __syncthreads();
   if (tid == 0) {
     if (n >= 2) {
       d[n - 2] = A[(size_t)(n - 2) * n + (n - 2)];
