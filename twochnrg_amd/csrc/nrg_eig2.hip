@@ -27,7 +27,7 @@
 
 #define E2_T 256
 #define WAVE 64
-#define LEAF 32
+#define LEAF 16
 #define PANEL 32   // reduction panel
 #define BTP 64     // back-transformation panel
 

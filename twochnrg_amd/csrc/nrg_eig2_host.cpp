@@ -14,7 +14,7 @@
 
 namespace nrg {
 
-static const int kLeaf = 32;   // = LEAF in nrg_eig2.hip
+static const int kLeaf = 16;   // = LEAF in nrg_eig2.hip
 static const int kPanel = 32;  // = PANEL in nrg_eig2.hip
 static const int kBtp = 64;    // = BTP in nrg_eig2.hip
 static const int kSmall = 256; // single-workgroup reduction up to this size
