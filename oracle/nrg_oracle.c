@@ -609,24 +609,29 @@ static double onechqs_cd_matel(const arr_t *B, const arr_t *S, int ist, int jst)
   if (fabs(cg3) > 1e-20) return fb / cg3;
   return 0.0;
 }
-/* OneChQS_HN_MatEl  src/OpMatRules.cpp:1410-1535 */
-static double onechqs_hn_matel(const double *par, const arr_t *B, const arr_t *S, const mat_t *f, int ist, int jst) {
-  double Lambda = par[0], chi = par[1], eN = par[2];
-  double Qi = qn_from_st(B, ist, 0), Si = qn_from_st(B, ist, 1);
-  double Qj = qn_from_st(B, jst, 0), Sj = qn_from_st(B, jst, 1);
-  if (fabs(Qi - Qj) > 1e-20 || fabs(Si - Sj) > 1e-20) return 0.0;
-  if (ist == jst) {
-    double m = sqrt(Lambda) * B->en.a[ist];
-    double Qt = qn_from_st(S, B->itype.a[ist], 0);
-    return m + eN * (Qt + 1.0);
-  }
+/* Coupling part of OneChQS_HN_MatEl (src/OpMatRules.cpp:1460-1530) for
+ * ist != jst of one (Q,S) block: returns CoefMatEl*FermiSign and the old
+ * state pair (*pa, *pb) whose f element it multiplies.  structural = 0 is the
+ * reference's rule (the (jst, ist) element with swapped site types when the
+ * (ist, jst) one is zero, :1477-1488); structural = 1 takes the orientation
+ * in which f stores the block pair, the same choice except where a stored
+ * element happens to vanish (what a derivative of the element needs). */
+static double hn_coupling(const arr_t *B, const arr_t *S, const mat_t *f, int ist, int jst, int structural,
+                          int *pa, int *pb) {
+  double Si = qn_from_st(B, ist, 1);
   int ti = B->itype.a[ist], tj = B->itype.a[jst];
-  double old = get_matel(f, B->stcf.a[ist], B->stcf.a[jst]);
-  int typep = ti, type = tj;
-  if (dEqual(old, 0.0)) {
-    old = get_matel(f, B->stcf.a[jst], B->stcf.a[ist]);
-    typep = tj; type = ti;
+  int a = B->stcf.a[ist], b = B->stcf.a[jst];
+  int swap;
+  if (structural) {
+    int p1, p2;
+    int b1 = blk_from_st(&f->b, a, &p1), b2 = blk_from_st(&f->b, b, &p2);
+    swap = find_mat_block(f, b1, b2) < 0;
+  } else {
+    swap = dEqual(get_matel(f, a, b), 0.0);
   }
+  int typep = ti, type = tj;
+  if (swap) { typep = tj; type = ti; int x = a; a = b; b = x; }
+  *pa = a; *pb = b;
   double Qtp = getqn(S, typep, 0), Stp = getqn(S, typep, 1), Sztp = getqn(S, typep, 2);
   double Qt = getqn(S, type, 0), St = getqn(S, type, 1), Szt = getqn(S, type, 2);
   double Scfp = Si - Sztp, Scf = Si - Szt;
@@ -648,7 +653,22 @@ static double onechqs_hn_matel(const double *par, const arr_t *B, const arr_t *S
       coef += c0 * c1 * c2 * c3;
     }
   }
-  return chi * old * coef * fsign;
+  return coef * fsign;
+}
+/* OneChQS_HN_MatEl  src/OpMatRules.cpp:1410-1535 */
+static double onechqs_hn_matel(const double *par, const arr_t *B, const arr_t *S, const mat_t *f, int ist, int jst) {
+  double Lambda = par[0], chi = par[1], eN = par[2];
+  double Qi = qn_from_st(B, ist, 0), Si = qn_from_st(B, ist, 1);
+  double Qj = qn_from_st(B, jst, 0), Sj = qn_from_st(B, jst, 1);
+  if (fabs(Qi - Qj) > 1e-20 || fabs(Si - Sj) > 1e-20) return 0.0;
+  if (ist == jst) {
+    double m = sqrt(Lambda) * B->en.a[ist];
+    double Qt = qn_from_st(S, B->itype.a[ist], 0);
+    return m + eN * (Qt + 1.0);
+  }
+  int a, b;
+  double c = hn_coupling(B, S, f, ist, jst, 0, &a, &b);
+  return chi * get_matel(f, a, b) * c;
 }
 
 /* ------------------------------------------------------------------ */
@@ -857,10 +877,264 @@ double orc_chin(int n, double L) {
 /* ------------------------------------------------------------------ */
 /* NRG_main loop  src/Main/NRG_main.cpp:45-427                          */
 /* ------------------------------------------------------------------ */
+/* ================================================================== */
+/* Derivative of the fixed-truncation NRG_main loop with respect to    */
+/* (U, Gamma, ed), forward mode.  No counterpart in the reference.      */
+/* Restated from first-order perturbation theory of each symmetric     */
+/* block (Z rows = eigenvectors, E ascending):                         */
+/*   G = Z dH Z^T,  dE_i = G_ii;                                        */
+/*   a kept row moves only towards the discarded rows of its block,    */
+/*     dz_i = sum_{j discarded} G_ij / (E_i - E_j) z_j,                 */
+/*   so the kept-space Hamiltonian tangent is G on the kept rows and    */
+/*   columns (full: rotations among kept states are not undone, every   */
+/*   later quantity is invariant under them) and an operator            */
+/*   O' = Z_k F Z_k^T moves by dZ_k F Z_k^T + Z_k F dZ_k^T + Z_k dF Z_k^T. */
+/*   Thermal sums over a non-diagonal tangent use the divided           */
+/*   differences of exp(-beta E) (Daleckii-Krein).                     */
+/* The kept set of every step is frozen (given, or the base run's).    */
+/* ================================================================== */
+typedef struct {
+  int ntan;
+  double dpar[ORC_MAX_TAN][3];      /* (dU, dGamma, ded) per tangent */
+  dvec th[ORC_MAX_TAN];             /* kept-space H tangent: AeigCut block b, k x k at th_off[b] */
+  ivec th_off;
+  dvec tm[ORC_MAX_TAN][4];          /* operator tangents, element layout of mats[m] */
+  dvec G[ORC_MAX_TAN];              /* G of every Aeig block, n x n at ae_off[b] */
+  ivec ae_off;
+  dvec dz[ORC_MAX_TAN];             /* dZ_k of every AeigCut block, laid out as its ev rows */
+  dvec den[ORC_MAX_TAN];            /* d(rescaled E) of Aeig */
+  double shift[ORC_MAX_TAN];        /* dEmin (SetE0zero) */
+  double dsus[ORC_MAX_TAN][4], dent[ORC_MAX_TAN][4], dops[ORC_MAX_TAN][8];
+  orc_tan_cb cb;
+  void *user;
+} orc_tan;
+
+static void tan_free(orc_tan *T) {
+  for (int t = 0; t < ORC_MAX_TAN; t++) {
+    free(T->th[t].a); free(T->G[t].a); free(T->dz[t].a); free(T->den[t].a);
+    for (int m = 0; m < 4; m++) free(T->tm[t][m].a);
+  }
+  free(T->th_off.a); free(T->ae_off.a);
+}
+/* element of an operator tangent stored with the block layout of M */
+static double tan_matel(const mat_t *M, const dvec *tv, int ist, int jst) {
+  int pi, pj;
+  int b1 = blk_from_st(&M->b, ist, &pi), b2 = blk_from_st(&M->b, jst, &pj);
+  int im = find_mat_block(M, b1, b2);
+  if (im < 0) return 0.0;
+  return tv->a[M->mbe.a[2 * im] + bsize(&M->b, b2) * pi + pj];
+}
+/* dH of one Abasis block (n x n, full): sqrt(Lambda) th(parent, parent') on
+ * equal site types, plus chi df + dchi f through the coupling rule */
+static void tan_hn_block(const orc_tan *T, int t, double sqrtL, double chi, double dchi, const arr_t *Ab,
+                         const arr_t *S, const mat_t *f, const arr_t *Ac, int b0, int n, double *dH) {
+  for (int i = 0; i < n; i++)
+    for (int j = i; j < n; j++) {
+      int ist = b0 + i, jst = b0 + j;
+      double v = 0.0;
+      if (Ab->itype.a[ist] == Ab->itype.a[jst]) {
+        int pp, pq;
+        int bp = blk_from_st(Ac, Ab->stcf.a[ist], &pp), bq = blk_from_st(Ac, Ab->stcf.a[jst], &pq);
+        if (bp == bq) v += sqrtL * T->th[t].a[T->th_off.a[bp] + pp * bsize(Ac, bp) + pq];
+      }
+      if (i != j) {
+        int a, b;
+        double c = hn_coupling(Ab, S, f, ist, jst, 1, &a, &b);
+        if (c != 0.0) v += chi * tan_matel(f, &T->tm[t][0], a, b) * c + dchi * get_matel(f, a, b) * c;
+      }
+      dH[i * n + j] = v;
+      dH[j * n + i] = v;
+    }
+}
+/* G = Z dH Z^T for every block of Aeig, the shift dEmin and dE */
+static void tan_diag(orc_tan *T, double sqrtL, double chi, const double *dchi, const arr_t *Ab, const arr_t *S,
+                     const mat_t *f, const arr_t *Ac, const arr_t *Ae) {
+  int nb = nblocks(Ae), ns = Ae->en.n, tot = 0, nmax = 1;
+  T->ae_off.n = 0;
+  for (int b = 0; b < nb; b++) {
+    int n = bsize(Ae, b);
+    ipush(&T->ae_off, tot);
+    tot += n * n;
+    if (n > nmax) nmax = n;
+  }
+  double *dH = (double *)malloc(sizeof(double) * (size_t)nmax * nmax);
+  double *Y = (double *)malloc(sizeof(double) * (size_t)nmax * nmax);
+  int gs = 0;   /* SetE0zero's minimum: the first lowest level */
+  for (int i = 1; i < ns; i++) if (Ae->en.a[i] < Ae->en.a[gs]) gs = i;
+  int gp, gb = blk_from_st(Ae, gs, &gp);
+  for (int t = 0; t < T->ntan; t++) {
+    dres(&T->G[t], tot); T->G[t].n = tot;
+    dres(&T->den[t], ns); T->den[t].n = ns;
+    for (int b = 0; b < nb; b++) {
+      int n = bsize(Ae, b);
+      const double *Z = Ae->ev.a + T->ae_off.a[b];
+      tan_hn_block(T, t, sqrtL, chi, dchi[t], Ab, S, f, Ac, blim(Ab, b, 0), n, dH);
+      gemm_nn(n, n, n, Z, dH, Y);
+      gemm_nt(n, n, n, Y, Z, T->G[t].a + T->ae_off.a[b]);
+    }
+    double s = T->G[t].a[T->ae_off.a[gb] + gp * bsize(Ae, gb) + gp];
+    T->shift[t] = s;
+    for (int b = 0; b < nb; b++) {
+      int n = bsize(Ae, b), b0 = blim(Ae, b, 0);
+      for (int i = 0; i < n; i++) T->den[t].a[b0 + i] = T->G[t].a[T->ae_off.a[b] + i * n + i] - s;
+    }
+  }
+  free(dH); free(Y);
+}
+/* CalcSuscep / CalcEntropy (src/CalcThermo.cpp:16-121) and their derivatives along dE */
+static void tan_thermo(double bb, const arr_t *A, const double *dE, double *dsus, double *dent) {
+  double s0 = 0, s1 = 0, d0 = 0, d1 = 0, Z = 0, se = 0, dZ = 0, dse = 0;
+  for (int b = 0; b < nblocks(A); b++) {
+    double S = getqn(A, b, 1);
+    double aS = 2.0 * S + 1.0, szsq = (1.0 / 12.0) * aS * (aS * aS - 1.0);
+    for (int i = blim(A, b, 0); i <= blim(A, b, 1); i++) {
+      double E = A->en.a[i], x = exp(-bb * E), dx = -bb * dE[i] * x;
+      s0 += aS * x; s1 += szsq * x; d0 += aS * dx; d1 += szsq * dx;
+      Z += aS * x; dZ += aS * dx; se += aS * E * x; dse += aS * (dE[i] * x + E * dx);
+    }
+  }
+  *dsus = (d1 * s0 - s1 * d0) / (s0 * s0);
+  *dent = bb * (dse * Z - se * dZ) / (Z * Z) + dZ / Z;
+}
+/* the new kept-space tangent, C = G_kd / (E_k - E_d) and dZ_k = C Z_d per AeigCut block */
+static void tan_cut(orc_tan *T, const arr_t *Ae, const arr_t *Ac) {
+  int nbc = nblocks(Ac), tot = 0;
+  T->th_off.n = 0;
+  for (int b = 0; b < nbc; b++) { ipush(&T->th_off, tot); tot += bsize(Ac, b) * bsize(Ac, b); }
+  for (int t = 0; t < T->ntan; t++) {
+    dres(&T->th[t], tot); T->th[t].n = tot;
+    dres(&T->dz[t], Ac->ev.n); T->dz[t].n = Ac->ev.n;
+    for (int b = 0; b < nbc; b++) {
+      int k = bsize(Ac, b), n = bsize_bc(Ac, b), be = 0;
+      while (blim(Ae, be, 0) != Ac->bbe_bc.a[2 * b]) be++;
+      const double *G = T->G[t].a + T->ae_off.a[be];
+      const double *E = Ae->en.a + blim(Ae, be, 0);
+      const double *Z = Ae->ev.a + T->ae_off.a[be];
+      double *th = T->th[t].a + T->th_off.a[b];
+      for (int i = 0; i < k; i++)
+        for (int j = 0; j < k; j++) th[i * k + j] = G[i * n + j] - (i == j ? T->shift[t] : 0.0);
+      double *dz = T->dz[t].a + blim_ev(Ac, b);
+      if (k == n) { memset(dz, 0, sizeof(double) * (size_t)k * n); continue; }
+      double *C = (double *)malloc(sizeof(double) * (size_t)k * (n - k));
+      for (int i = 0; i < k; i++)
+        for (int j = k; j < n; j++) {
+          double d = E[i] - E[j];
+          C[i * (n - k) + (j - k)] = d != 0.0 ? G[i * n + j] / d : 0.0;
+        }
+      gemm_nn(k, n, n - k, C, Z + (size_t)k * n, dz);
+      free(C);
+    }
+  }
+}
+/* operator tangents over the new AeigCut, in UpdateMatrices' (ib, jb, m) order;
+ * called with the operators of the previous step still in mats */
+static void tan_ops(orc_tan *T, const arr_t *S, const arr_t *Ac, const arr_t *Ab, const mat_t *mats, int nm) {
+  dvec nt[ORC_MAX_TAN][4];
+  memset(nt, 0, sizeof(nt));
+  for (int ib = 0; ib < nblocks(Ac); ib++) {
+    int ki = bsize(Ac, ib), ibc = find_match_block(Ac, ib, Ab), ni = bsize(Ab, ibc);
+    for (int jb = 0; jb < nblocks(Ac); jb++) {
+      int kj = bsize(Ac, jb), jbc = find_match_block(Ac, jb, Ab), nj = bsize(Ab, jbc);
+      for (int m = 0; m < nm; m++) {
+        if (!mats[m].check(Ac, ib, jb)) continue;
+        double *F = (double *)malloc(sizeof(double) * (size_t)ni * nj);
+        double *dF = (double *)malloc(sizeof(double) * (size_t)ni * nj);
+        double *o = (double *)malloc(sizeof(double) * (size_t)ki * kj);
+        const double *Zi = Ac->ev.a + blim_ev(Ac, ib), *Zj = Ac->ev.a + blim_ev(Ac, jb);
+        int r = 0;
+        for (int ist = blim(Ab, ibc, 0); ist <= blim(Ab, ibc, 1); ist++)
+          for (int jst = blim(Ab, jbc, 0); jst <= blim(Ab, jbc, 1); jst++, r++) {
+            double g = mats[m].calc(Ab, S, ist, jst);
+            F[r] = mats[m].need_old ? g * get_matel(&mats[m], Ab->stcf.a[ist], Ab->stcf.a[jst]) : g;
+          }
+        for (int t = 0; t < T->ntan; t++) {
+          const double *dZi = T->dz[t].a + blim_ev(Ac, ib), *dZj = T->dz[t].a + blim_ev(Ac, jb);
+          dres(&nt[t][m], nt[t][m].n + ki * kj);
+          double *out = nt[t][m].a + nt[t][m].n;
+          orc_rotate(ni, nj, ki, kj, dZi, F, Zj, out);
+          orc_rotate(ni, nj, ki, kj, Zi, F, dZj, o);
+          for (int q = 0; q < ki * kj; q++) out[q] += o[q];
+          if (mats[m].need_old) {
+            r = 0;
+            for (int ist = blim(Ab, ibc, 0); ist <= blim(Ab, ibc, 1); ist++)
+              for (int jst = blim(Ab, jbc, 0); jst <= blim(Ab, jbc, 1); jst++, r++)
+                dF[r] = mats[m].calc(Ab, S, ist, jst) *
+                        tan_matel(&mats[m], &T->tm[t][m], Ab->stcf.a[ist], Ab->stcf.a[jst]);
+            orc_rotate(ni, nj, ki, kj, Zi, dF, Zj, o);
+            for (int q = 0; q < ki * kj; q++) out[q] += o[q];
+          }
+          nt[t][m].n += ki * kj;
+        }
+        free(F); free(dF); free(o);
+      }
+    }
+  }
+  for (int t = 0; t < T->ntan; t++)
+    for (int m = 0; m < nm; m++) { free(T->tm[t][m].a); T->tm[t][m] = nt[t][m]; }
+}
+/* derivative of CalcOpAvg (src/CalcOpAvg.cpp:13-84) on AeigCut: the kept-space
+ * Hamiltonian is diag(E) + th, so sum_i O_ii x_i becomes Tr[O exp(-beta H)] */
+static double tan_opavg(double bb, const arr_t *A, const mat_t *op, const dvec *top, const orc_tan *T, int t) {
+  double s = 0, Z = 0, ds = 0, dZ = 0;
+  for (int b = 0; b < nblocks(A); b++) {
+    double dg = 2.0 * getqn(A, b, 1) + 1.0;
+    int k = bsize(A, b), b0 = blim(A, b, 0);
+    int im = find_mat_block(op, b, b);
+    const double *O = im >= 0 ? op->mat.a + op->mbe.a[2 * im] : NULL;
+    const double *dO = im >= 0 ? top->a + op->mbe.a[2 * im] : NULL;
+    const double *th = T->th[t].a + T->th_off.a[b];
+    const double *E = A->en.a + b0;
+    for (int i = 0; i < k; i++) {
+      double x = exp(-bb * E[i]);
+      Z += dg * x;
+      dZ += dg * th[i * k + i] * (-bb * x);
+      if (!O) continue;
+      s += dg * O[i * k + i] * x;
+      ds += dg * dO[i * k + i] * x;
+      for (int j = 0; j < k; j++) {
+        double d = E[i] - E[j], xj = exp(-bb * E[j]);
+        double phi = d != 0.0 ? xj * expm1(-bb * d) / d : -bb * xj;
+        ds += dg * O[j * k + i] * th[i * k + j] * phi;
+      }
+    }
+  }
+  return (ds * Z - s * dZ) / (Z * Z);
+}
+/* CutStates with a given Kept set (src/CutStates.cpp:5-96 after SetKept):
+ * blocks without kept states go, then each block keeps its kept prefix */
+static void cut_states_fixed(const arr_t *Ain, arr_t *Ac) {
+  arr_clear(Ac);
+  Ac->nshell = Ain->nshell; Ac->nq = Ain->nq;
+  dcopy_v(&Ac->qn, &Ain->qn); dcopy_v(&Ac->en, &Ain->en); dcopy_v(&Ac->ev, &Ain->ev);
+  icopy_v(&Ac->kept, &Ain->kept); icopy_v(&Ac->bbe, &Ain->bbe);
+  Ac->stcf.n = 0;
+  for (int i = 0; i < Ac->en.n; i++) ipush(&Ac->stcf, i);
+  icopy_v(&Ac->bbe_bc, &Ain->bbe);
+  set_bbe_ev(Ac);
+  for (int b = 0; b < nblocks(Ac); b++) {
+    int nk = 0;
+    for (int i = blim(Ac, b, 0); i <= blim(Ac, b, 1); i++) nk += Ac->kept.a[i] != 0;
+    if (nk == 0) { remove_block(Ac, b); b--; }
+  }
+  for (int b = 0; b < nblocks(Ac); b++) {
+    int b0 = blim(Ac, b, 0), b1 = blim(Ac, b, 1), nk = 0;
+    for (int i = b0; i <= b1; i++) nk += Ac->kept.a[i] != 0;
+    if (b0 + nk <= b1) remove_states_from_block(Ac, b, b0 + nk, b1);
+  }
+}
+
+static void emit_tan(orc_step_cb cb, void *user, const arr_t *Ae, const arr_t *Ac, double emin, double ecut,
+                     int nkept, const dvec *hrec, int rec, const double *thT, const double *sus,
+                     const double *ent, double Tavg, int nops, const double *ops, const void *tan);
 static void emit(orc_step_cb cb, void *user, const arr_t *Ae, const arr_t *Ac, double emin, double ecut,
                  int nkept, const dvec *hrec, int rec, const double *thT, const double *sus,
                  const double *ent, double Tavg, int nops, const double *ops) {
-  if (!cb) return;
+  emit_tan(cb, user, Ae, Ac, emin, ecut, nkept, hrec, rec, thT, sus, ent, Tavg, nops, ops, NULL);
+}
+static void emit_tan(orc_step_cb cb, void *user, const arr_t *Ae, const arr_t *Ac, double emin, double ecut,
+                     int nkept, const dvec *hrec, int rec, const double *thT, const double *sus,
+                     const double *ent, double Tavg, int nops, const double *ops, const void *tan) {
+  if (!cb && !tan) return;
   orc_step st;
   memset(&st, 0, sizeof(st));
   st.nshell = Ae->nshell; st.nq = Ae->nq; st.nblocks = nblocks(Ae);
@@ -875,18 +1149,54 @@ static void emit(orc_step_cb cb, void *user, const arr_t *Ae, const arr_t *Ac, d
   st.T_avg = Tavg; st.nops_avg = nops;
   for (int i = 0; i < nops && i < 8; i++) st.opavg[i] = ops[i];
   st.cut_nblocks = nblocks(Ac); st.cut_qn = Ac->qn.a; st.cut_bbe = Ac->bbe.a; st.cut_bbe_bc = Ac->bbe_bc.a;
-  cb(user, &st);
+  if (tan) {
+    const orc_tan *T = (const orc_tan *)tan;
+    orc_tan_step dt;
+    memset(&dt, 0, sizeof(dt));
+    dt.nshell = Ae->nshell; dt.ntan = T->ntan; dt.nstates = Ae->en.n; dt.nops_avg = nops;
+    double *den = (double *)malloc(sizeof(double) * (size_t)(T->ntan * Ae->en.n + 1));
+    for (int t = 0; t < T->ntan; t++) {
+      memcpy(den + (size_t)t * Ae->en.n, T->den[t].a, sizeof(double) * (size_t)Ae->en.n);
+      for (int i = 0; i < 4; i++) { dt.dsuscep[t][i] = T->dsus[t][i]; dt.dentropy[t][i] = T->dent[t][i]; }
+      for (int i = 0; i < nops && i < 8; i++) dt.dopavg[t][i] = T->dops[t][i];
+    }
+    dt.den = den;
+    if (T->cb) T->cb(T->user, &st, &dt);
+    free(den);
+  } else {
+    cb(user, &st);
+  }
   free(kp);
 }
 
 struct orc_dm;
 static void dm_store_step(struct orc_dm *D, const arr_t *Ac, const arr_t *Ab, const mat_t *op);
-static int run_onechqs(const orc_onechqs_params *p, orc_step_cb cb, void *user, struct orc_dm *dm, int befcut);
-int orc_run_onechqs(const orc_onechqs_params *p, orc_step_cb cb, void *user) { return run_onechqs(p, cb, user, NULL, 0); }
+static int run_onechqs(const orc_onechqs_params *p, orc_step_cb cb, void *user, struct orc_dm *dm, int befcut,
+                       const unsigned char *kfix, const long long *kfix_off, orc_tan *T);
+int orc_run_onechqs(const orc_onechqs_params *p, orc_step_cb cb, void *user) {
+  return run_onechqs(p, cb, user, NULL, 0, NULL, NULL, NULL);
+}
+/* The NRG_main loop with every step's Kept set given (kept + kept_off[N] for
+ * N = 0 .. nsitesmax-1, Aeig state order; NULL: SetKept as usual) and ntan
+ * tangents along dpar[t] = (dU, dGamma, ded). */
+int orc_run_onechqs_jvp(const orc_onechqs_params *p, const unsigned char *kept, const long long *kept_off, int ntan,
+                        const double *dpar, orc_tan_cb cb, void *user) {
+  if (ntan < 0 || ntan > ORC_MAX_TAN || (ntan > 0 && !dpar) || (kept && !kept_off)) return -1;
+  if (ntan > 0 && !(p->Gamma > 0.0)) return -1;
+  orc_tan T;
+  memset(&T, 0, sizeof(T));
+  T.ntan = ntan; T.cb = cb; T.user = user;
+  for (int t = 0; t < ntan; t++)
+    for (int q = 0; q < 3; q++) T.dpar[t][q] = dpar[3 * t + q];
+  int rc = run_onechqs(p, NULL, NULL, NULL, 0, kept, kept_off, &T);
+  tan_free(&T);
+  return rc;
+}
 /* befcut = UpdateBefCut 1 (src/Main/NRG_main.cpp:334-343,393-396): the operators are
  * rotated over every eigenstate of AeigCut = Aeig with its Kept flags, SaveArrays
  * writes that copy, and the real CutStates follows CalcStuff */
-static int run_onechqs(const orc_onechqs_params *p, orc_step_cb cb, void *user, struct orc_dm *dm, int befcut) {
+static int run_onechqs(const orc_onechqs_params *p, orc_step_cb cb, void *user, struct orc_dm *dm, int befcut,
+                       const unsigned char *kfix, const long long *kfix_off, orc_tan *T) {
   double Lambda = p->Lambda, bbar = p->betabar > 0 ? p->betabar : 0.727;
   double myeps = p->myeps > 0 ? p->myeps : 1e-18;
   double HLF = 0.5 * (1.0 + 1.0 / Lambda);
@@ -930,36 +1240,74 @@ static int run_onechqs(const orc_onechqs_params *p, orc_step_cb cb, void *user, 
   }
   cut_states(&Ae, &Ac, p->ncut, myeps);
   double DN = HLF * pow(Lambda, -(0 - 2) / 2.0);
+  if (T) {   /* tangents of H_{-1}: one-state blocks, f and the impurity operators are constants */
+    const double sc = 1.0 / (Lambda * HLF);
+    T->th_off.n = 0;
+    for (int b = 0; b < 3; b++) ipush(&T->th_off, b);
+    for (int t = 0; t < T->ntan; t++) {
+      double dU = T->dpar[t][0], ded = T->dpar[t][2];
+      T->th[t].n = 0; T->den[t].n = 0;
+      double e3[3] = {0.5 * dU * sc, (ded + 0.5 * dU) * sc, (2.0 * ded + 1.5 * dU) * sc};
+      for (int b = 0; b < 3; b++) { dpush(&T->th[t], e3[b]); dpush(&T->den[t], e3[b]); }
+      for (int m = 0; m < nm; m++) { T->tm[t][m].n = 0; for (int q = 0; q < mats[m].mat.n; q++) dpush(&T->tm[t][m], 0.0); }
+    }
+  }
   {
     double ops[3] = {0, 0, 0};
     int nops = 0;
     if (p->calcdens != 1) for (int m = 1; m < nm; m++) ops[nops++] = calc_opavg(bbar, &Ac, &mats[m]);
-    emit(cb, user, &Ae, &Ac, 0.0, HUGE_VAL, Ae.en.n, NULL, 0, NULL, NULL, NULL, DN / bbar, nops, ops);
+    if (T)
+      for (int t = 0; t < T->ntan; t++)
+        for (int m = 1, q = 0; m < nm && p->calcdens != 1; m++, q++) T->dops[t][q] = tan_opavg(bbar, &Ac, &mats[m], &T->tm[t][m], T, t);
+    emit_tan(cb, user, &Ae, &Ac, 0.0, HUGE_VAL, Ae.en.n, NULL, 0, NULL, NULL, NULL, DN / bbar, nops, ops, T);
   }
   double par[4] = {Lambda, chi_m1, 0.0, 0.0};
+  double dchi[ORC_MAX_TAN] = {0, 0, 0, 0};
+  if (T) for (int t = 0; t < T->ntan; t++) dchi[t] = chi_m1 * T->dpar[t][1] / (2.0 * p->Gamma);
   dvec hrec = {0};
   for (int N = 0; N < p->nsitesmax; N++) {
     DN = HLF * pow(Lambda, -(N - 1) / 2.0);
     build_basis_qs(&Ac, &Ab, &S, befcut);
     double emin = diag_hn(par, &Ab, &S, &mats[0], &Ae, p->record_blocks ? &hrec : NULL);
+    if (T) tan_diag(T, sqrt(Lambda), par[1], dchi, &Ab, &S, &mats[0], &Ac, &Ae);
     int nkept = 0;
-    double ecut = set_kept(&Ae, p->ncut, myeps, &nkept);
+    double ecut;
+    if (kfix) {   /* the given Kept set; Ecut reported as its highest level */
+      orc_setdegen(Ae.en.n, Ae.en.a, myeps);
+      Ae.kept.n = 0;
+      ecut = 0.0;
+      for (int i = 0; i < Ae.en.n; i++) {
+        int k = kfix[kfix_off[N] + i] != 0;
+        ipush(&Ae.kept, k); nkept += k;
+        if (k && Ae.en.a[i] > ecut) ecut = Ae.en.a[i];
+      }
+    } else {
+      ecut = set_kept(&Ae, p->ncut, myeps, &nkept);
+    }
     double thT[4], sus[4], ent[4];
     for (int nc = 1; nc <= 4; nc++) {
       double bN = bbar * pow(Lambda, (nc - 4) / (2.0 * 4));
       thT[nc - 1] = DN / bN;
       sus[nc - 1] = calc_suscep(bN, &Ae);
       ent[nc - 1] = calc_entropy(bN, &Ae);
+      if (T)
+        for (int t = 0; t < T->ntan; t++) tan_thermo(bN, &Ae, T->den[t].a, &T->dsus[t][nc - 1], &T->dent[t][nc - 1]);
     }
-    cut_states(&Ae, &Ac, befcut ? 2 * Ae.en.n : p->ncut, myeps);
+    if (kfix) cut_states_fixed(&Ae, &Ac);
+    else cut_states(&Ae, &Ac, befcut ? 2 * Ae.en.n : p->ncut, myeps);
+    if (T) { tan_cut(T, &Ae, &Ac); tan_ops(T, &S, &Ac, &Ab, mats, nm); }
     update_matrices(&S, &Ac, &Ab, mats, nm);
     if (dm && p->calcdens == 1) dm_store_step(dm, &Ac, &Ab, &mats[1]);   /* SaveArrays src/NRGcodehandlerClass.cpp:887-933 */
     double ops[3] = {0, 0, 0};
     int nops = 0;
     if (p->calcdens != 1) for (int m = 1; m < nm; m++) ops[nops++] = calc_opavg(bbar, &Ac, &mats[m]);
+    if (T)
+      for (int t = 0; t < T->ntan; t++)
+        for (int m = 1, q = 0; m < nm && p->calcdens != 1; m++, q++) T->dops[t][q] = tan_opavg(bbar, &Ac, &mats[m], &T->tm[t][m], T, t);
     if (befcut) cut_states(&Ae, &Ac, p->ncut, myeps);
-    emit(cb, user, &Ae, &Ac, emin, ecut, nkept, &hrec, p->record_blocks, thT, sus, ent, DN / bbar, nops, ops);
+    emit_tan(cb, user, &Ae, &Ac, emin, ecut, nkept, &hrec, p->record_blocks, thT, sus, ent, DN / bbar, nops, ops, T);
     par[1] = orc_chin(N, Lambda);
+    for (int t = 0; t < ORC_MAX_TAN; t++) dchi[t] = 0.0;
   }
   free(hrec.a);
   for (int m = 0; m < 4; m++) mat_free(&mats[m]);
@@ -2139,7 +2487,7 @@ orc_dm *orc_dm_run(const orc_dmnrg_params *p) {
   q.U = p->U; q.Gamma = p->Gamma; q.ed = p->ed; q.Lambda = p->Lambda; q.nsitesmax = p->nsitesmax;
   q.ncut = p->ncut; q.calcdens = 1; q.disc_scheme = p->disc_scheme; q.myeps = p->myeps;
   q.betabar = p->betabar_nrg;
-  run_onechqs(&q, NULL, NULL, D, p->update_bef_cut ? 1 : 0);
+  run_onechqs(&q, NULL, NULL, D, p->update_bef_cut ? 1 : 0, NULL, NULL, NULL);
   onechqs_site(&D->site);
   int Nsitesmax = p->nsitesmax;
   double DM, Temp;

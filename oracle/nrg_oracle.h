@@ -59,6 +59,25 @@ typedef struct {
 
 int orc_run_onechqs(const orc_onechqs_params *p, orc_step_cb cb, void *user);
 
+/* Derivatives of the same loop with every step's Kept set frozen (forward
+ * mode; no counterpart in the reference).  kept + kept_off[N] (N = 0 ..
+ * nsitesmax-1) gives Aeig's Kept flags of step N, or kept = NULL for the
+ * usual SetKept.  Tangent t is the direction dpar[3t..3t+2] = (dU, dGamma,
+ * ded).  The callback receives the usual record and the tangent record:
+ * d(rescaled E) of Aeig, d(T chi) and dS at the 4 CalcThermo betabar, and
+ * d<O> of the CalcOpAvg operators.  Needs Gamma > 0 when ntan > 0. */
+#define ORC_MAX_TAN 4
+typedef struct {
+  int nshell, ntan, nstates;
+  const double *den;                  /* ntan x nstates */
+  double dsuscep[ORC_MAX_TAN][4], dentropy[ORC_MAX_TAN][4];
+  int nops_avg;
+  double dopavg[ORC_MAX_TAN][8];
+} orc_tan_step;
+typedef void (*orc_tan_cb)(void *user, const orc_step *st, const orc_tan_step *dt);
+int orc_run_onechqs_jvp(const orc_onechqs_params *p, const unsigned char *kept, const long long *kept_off, int ntan,
+                        const double *dpar, orc_tan_cb cb, void *user);
+
 /* TwoChQS Anderson through the legacy standalone driver
  * (src/TwoChQS/TwoChQS.cpp:23-504, ModelNo 0): Gamma~ = Gamma*(2/pi)/HLF^2,
  * chi_{-1} = sqrt(Gamma~/Lambda) per channel (no A_Lambda), one betabar per N.

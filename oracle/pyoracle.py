@@ -41,6 +41,18 @@ class OneChQSParams(ctypes.Structure):
     ]
 
 
+class OrcTanStep(ctypes.Structure):
+    _fields_ = [
+        ("nshell", ctypes.c_int), ("ntan", ctypes.c_int), ("nstates", ctypes.c_int),
+        ("den", ctypes.POINTER(ctypes.c_double)),
+        ("dsuscep", (ctypes.c_double * 4) * 4), ("dentropy", (ctypes.c_double * 4) * 4),
+        ("nops_avg", ctypes.c_int), ("dopavg", (ctypes.c_double * 8) * 4),
+    ]
+
+
+TAN_CB = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.POINTER(OrcStep), ctypes.POINTER(OrcTanStep))
+
+
 class TwoChQSParams(ctypes.Structure):
     _fields_ = [
         ("U", ctypes.c_double), ("Gamma1", ctypes.c_double), ("Gamma2", ctypes.c_double),
@@ -65,6 +77,10 @@ def load():
     lib.orc_run_twochqs.restype = ctypes.c_int
     lib.orc_run_twochqsz.argtypes = [ctypes.POINTER(TwoChQSParams), STEP_CB, ctypes.c_void_p]
     lib.orc_run_twochqsz.restype = ctypes.c_int
+    lib.orc_run_onechqs_jvp.argtypes = [ctypes.POINTER(OneChQSParams), ctypes.POINTER(ctypes.c_ubyte),
+                                        ctypes.POINTER(ctypes.c_longlong), ctypes.c_int,
+                                        ctypes.POINTER(ctypes.c_double), TAN_CB, ctypes.c_void_p]
+    lib.orc_run_onechqs_jvp.restype = ctypes.c_int
     lib.orc_cgordan.argtypes = [ctypes.c_double] * 6
     lib.orc_cgordan.restype = ctypes.c_double
     dp = ctypes.POINTER(ctypes.c_double)
@@ -130,6 +146,54 @@ def run_onechqs(U=0.5, Gamma=0.0282691, ed=-0.25, Lambda=2.5, nsitesmax=4, ncut=
     if rc != 0:
         raise RuntimeError("oracle run failed: %d" % rc)
     return steps
+
+
+def tan_step_to_dict(dt):
+    """Copy one tangent record into numpy (ntan x ... arrays)."""
+    d = dt.contents
+    nt, ns = d.ntan, d.nstates
+    return {"N": d.nshell,
+            "den": (np.ctypeslib.as_array(d.den, (nt * ns,)).reshape(nt, ns).copy() if nt else np.zeros((0, ns))),
+            "dsuscep": np.array([[d.dsuscep[t][i] for i in range(4)] for t in range(nt)]).reshape(nt, 4),
+            "dentropy": np.array([[d.dentropy[t][i] for i in range(4)] for t in range(nt)]).reshape(nt, 4),
+            "dopavg": np.array([[d.dopavg[t][i] for i in range(d.nops_avg)] for t in range(nt)]).reshape(
+                nt, d.nops_avg)}
+
+
+def kept_arrays(steps):
+    """Concatenated Kept flags of steps N >= 0 and their offsets (the frozen-truncation input)."""
+    ks = [s["kept"].astype(np.uint8) for s in steps if s["N"] >= 0]
+    off = np.zeros(len(ks) + 1, dtype=np.int64)
+    off[1:] = np.cumsum([len(k) for k in ks])
+    return (np.concatenate(ks) if ks else np.zeros(0, np.uint8)), off
+
+
+def run_onechqs_jvp(U=0.5, Gamma=0.02, ed=-0.25, Lambda=2.5, nsitesmax=4, ncut=1000, calcdens=2,
+                    disc_scheme=0, myeps=1e-18, betabar=0.727, kept=None, dpar=None):
+    """The NRG_main loop with frozen Kept sets (kept = (flags, offsets) from
+    kept_arrays, or None) and tangents along the rows of dpar (dU, dGamma, ded).
+    Returns (steps, tangents)."""
+    lib = load()
+    steps, tans = [], []
+
+    def cb(_user, st, dt):
+        steps.append(step_to_dict(st))
+        tans.append(tan_step_to_dict(dt))
+
+    p = OneChQSParams(U, Gamma, ed, Lambda, nsitesmax, ncut, calcdens, disc_scheme, myeps, betabar, 0)
+    dp_ = np.ascontiguousarray(np.zeros((0, 3)) if dpar is None else np.asarray(dpar, np.float64).reshape(-1, 3))
+    if kept is not None:
+        kf = np.ascontiguousarray(kept[0], dtype=np.uint8)
+        ko = np.ascontiguousarray(kept[1], dtype=np.int64)
+        kfp = kf.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte))
+        kop = ko.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong))
+    else:
+        kfp = kop = None
+    fn = TAN_CB(cb)
+    rc = lib.orc_run_onechqs_jvp(ctypes.byref(p), kfp, kop, dp_.shape[0], _dp(dp_) if dp_.size else None, fn, None)
+    if rc != 0:
+        raise RuntimeError("oracle jvp run failed: %d" % rc)
+    return steps, tans
 
 
 def run_twochqs(U=0.4, Gamma1=0.03, Gamma2=0.03, ed=-0.2, Lambda=3.0, nsitesmax=3, ncut=3000,
