@@ -222,6 +222,35 @@ int nrg_run_twochqs(nrg_ctx *ctx, const nrg_twochqs_params *p, nrg_step_cb cb, v
  */
 int nrg_run_twochqsz(nrg_ctx *ctx, const nrg_twochqs_params *p, nrg_step_cb cb, void *user, double *timing_ms);
 
+/* ---------------- derivative of the fixed-truncation iteration ---------------- */
+
+/*
+ * No counterpart in the reference (which has no derivative; its "gradient" is
+ * a finite difference of whole NRG_main runs).  Forward-mode derivative of the
+ * OneChQS NRG_main loop of nrg_run_onechqs with respect to the Anderson
+ * parameters, every step's Kept set frozen:
+ *   kept + kept_off[N] (N = 0 .. nsitesmax-1; kept_off has nsitesmax+1
+ *   entries) are step N's Kept flags in Aeig state order (the `kept` of the
+ *   nrg_step records of a plain run), or kept = NULL for the usual SetKept;
+ *   tangent t is the direction dpar[3t .. 3t+2] = (dU, dGamma, ded).
+ * Per step and tangent: d(rescaled E) of Aeig (SetE0zero's shift included),
+ * d(T chi) and dS at the 4 CalcThermo betabar values, d<O> of the CalcOpAvg
+ * operators (n_d, n_d^2, S_d^2 with calcdens 0/2).  With ntan = 0 it is
+ * nrg_run_onechqs with the given Kept sets (finite differences).  cb receives
+ * the usual record and the tangent record.  Needs Gamma > 0 when ntan > 0.
+ */
+#define NRG_MAX_TAN 4
+typedef struct {
+  int nshell, ntan, nstates;
+  const double *den;                         /* ntan x nstates */
+  double dsuscep[NRG_MAX_TAN][4], dentropy[NRG_MAX_TAN][4];
+  int nops_avg;
+  double dopavg[NRG_MAX_TAN][8];
+} nrg_tangent_step;
+typedef void (*nrg_tangent_cb)(void *user, const nrg_step *st, const nrg_tangent_step *dt);
+int nrg_onechqs_jvp(nrg_ctx *ctx, const nrg_onechqs_params *p, const unsigned char *kept, const int64_t *kept_off,
+                    int ntan, const double *dpar, nrg_tangent_cb cb, void *user);
+
 /* ---------------- DM-NRG (density-matrix NRG spectral function) ---------------- */
 
 typedef struct {

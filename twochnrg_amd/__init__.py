@@ -20,7 +20,7 @@ EXPORTS = [
     "nrg_set_eig_impl", "nrg_set_plain", "nrg_eig2_stages",
     "nrg_dm_create", "nrg_dm_destroy", "nrg_dm_info", "nrg_dm_timing", "nrg_dm_rho", "nrg_dm_spec",
     "nrg_dm_spectrum", "nrg_run_onechqs_multi", "nrg_dm_spec_omega", "nrg_dm_norm", "nrg_dm_weights",
-    "nrg_run_twochqsz", "nrg_onechqs_thermo",
+    "nrg_run_twochqsz", "nrg_onechqs_thermo", "nrg_onechqs_jvp",
 ]
 
 KSTAT_NAMES = ["eig", "hn", "fbasis", "gemm", "tdb_symv"]
@@ -45,6 +45,17 @@ class NrgStep(ctypes.Structure):
 
 
 STEP_CB = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.POINTER(NrgStep))
+
+
+class NrgTangentStep(ctypes.Structure):
+    _fields_ = [
+        ("nshell", ctypes.c_int), ("ntan", ctypes.c_int), ("nstates", ctypes.c_int), ("den", _dp),
+        ("dsuscep", (ctypes.c_double * 4) * 4), ("dentropy", (ctypes.c_double * 4) * 4),
+        ("nops_avg", ctypes.c_int), ("dopavg", (ctypes.c_double * 8) * 4),
+    ]
+
+
+TAN_CB = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.POINTER(NrgStep), ctypes.POINTER(NrgTangentStep))
 MULTI_CB = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(NrgStep))
 
 
@@ -125,6 +136,8 @@ def lib():
     L.nrg_run_twochqs.argtypes = [ctypes.c_void_p, ctypes.POINTER(TwoChQSParams), STEP_CB, ctypes.c_void_p, _dp]
     L.nrg_run_twochqsz.argtypes = [ctypes.c_void_p, ctypes.POINTER(TwoChQSParams), STEP_CB, ctypes.c_void_p, _dp]
     L.nrg_onechqs_thermo.argtypes = [ctypes.c_void_p, ctypes.POINTER(OneChQSParams), _dp, ctypes.c_int, _ip]
+    L.nrg_onechqs_jvp.argtypes = [ctypes.c_void_p, ctypes.POINTER(OneChQSParams), ctypes.POINTER(ctypes.c_ubyte),
+                                  _i64p, ctypes.c_int, _dp, TAN_CB, ctypes.c_void_p]
     L.nrg_run_onechqs_multi.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(OneChQSParams), MULTI_CB,
                                         ctypes.c_void_p, _dp]
     L.nrg_dm_create.argtypes = [ctypes.c_void_p, ctypes.POINTER(DmParams), ctypes.POINTER(ctypes.c_void_p)]
@@ -292,6 +305,32 @@ class Context:
         return steps, ms.value
 
 
+    def onechqs_jvp(self, U=0.5, Gamma=0.02, ed=-0.25, Lambda=2.5, nsitesmax=4, ncut=1000, calcdens=2,
+                    disc_scheme=0, myeps=1e-18, betabar=0.727, kept=None, dpar=None):
+        """Derivative pass of the fixed-truncation loop (nrg_onechqs_jvp).  kept =
+        (flags, offsets) as from kept_arrays() of a plain run, or None; dpar rows
+        are tangent directions (dU, dGamma, ded).  Returns (steps, tangents)."""
+        steps, tans = [], []
+
+        def cb(_u, st, dt):
+            steps.append(step_to_dict(st))
+            tans.append(tangent_to_dict(dt))
+
+        p = OneChQSParams(U, Gamma, ed, Lambda, nsitesmax, ncut, calcdens, disc_scheme, myeps, betabar, 0)
+        dp_ = np.ascontiguousarray(np.zeros((0, 3)) if dpar is None else np.asarray(dpar, np.float64).reshape(-1, 3))
+        if kept is not None:
+            kf = np.ascontiguousarray(kept[0], dtype=np.uint8)
+            ko = np.ascontiguousarray(kept[1], dtype=np.int64)
+            if len(ko) < nsitesmax + 1:
+                raise ValueError("kept offsets need nsitesmax+1 entries")
+            kfp, kop = kf.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte)), _ptr(ko, _i64p)
+        else:
+            kfp = kop = None
+        fn = TAN_CB(cb)
+        _check(lib().nrg_onechqs_jvp(self.h, ctypes.byref(p), kfp, kop, dp_.shape[0],
+                                     _ptr(dp_) if dp_.size else None, fn, None))
+        return steps, tans
+
     def onechqs_thermo(self, U=0.5, Gamma=0.02, ed=-0.25, Lambda=2.5, nsitesmax=40, ncut=400, disc_scheme=0,
                        myeps=1e-18, betabar=0.727):
         """Impurity thermodynamics with the chain subtraction (nrg_onechqs_thermo):
@@ -401,6 +440,26 @@ def step_to_dict(st):
     if bool(s.evec):
         d["evec"] = np.ctypeslib.as_array(s.evec, (int(np.sum(sizes * sizes)),)).copy()
     return d
+
+
+def tangent_to_dict(dt):
+    d = dt.contents
+    nt, ns = d.ntan, d.nstates
+    return {"N": d.nshell,
+            "den": (np.ctypeslib.as_array(d.den, (nt * ns,)).reshape(nt, ns).copy() if nt else np.zeros((0, ns))),
+            "dsuscep": np.array([[d.dsuscep[t][i] for i in range(4)] for t in range(nt)]).reshape(nt, 4),
+            "dentropy": np.array([[d.dentropy[t][i] for i in range(4)] for t in range(nt)]).reshape(nt, 4),
+            "dopavg": np.array([[d.dopavg[t][i] for i in range(d.nops_avg)] for t in range(nt)]).reshape(
+                nt, d.nops_avg)}
+
+
+def kept_arrays(steps):
+    """Concatenated Kept flags of the records N >= 0 and their offsets: the
+    frozen-truncation input of onechqs_jvp."""
+    ks = [s["kept"].astype(np.uint8) for s in steps if s["N"] >= 0]
+    off = np.zeros(len(ks) + 1, dtype=np.int64)
+    off[1:] = np.cumsum([len(k) for k in ks])
+    return (np.concatenate(ks) if ks else np.zeros(0, np.uint8)), off
 
 
 class DMNRG:

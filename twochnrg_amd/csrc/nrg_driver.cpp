@@ -26,6 +26,7 @@
 #include <vector>
 
 #include "nrg_dm.h"
+#include "nrg_eig2.h"
 #include "nrg_internal.h"
 
 namespace nrg {
@@ -725,15 +726,19 @@ struct Chain {
   // rows are not moved: AeigCut records where each block's kept rows start in
   // the Z buffer.  legacy=false repeats SetKept(Ncut) (SetDegen + Ecut) on the
   // copy as CutStates does; the by-value form of the two-channel driver does not.
-  void cut_states(int ncut, bool legacy, double myeps) {
+  //   fixed != nullptr: the given Kept set (Aeig state order; each block keeps as
+  //   many of its lowest states as it has kept flags) instead of SetKept.
+  void cut_states(int ncut, bool legacy, double myeps, const unsigned char* fixed = nullptr) {
     CNRGbasisarray A;
     A.Nshell = Aeig.Nshell; A.NQNumbers = Aeig.NQNumbers;
     int nb = Aeig.NumBlocks();
-    const bool notrunc = ncut >= Aeig.Nstates();
+    const bool notrunc = !fixed && ncut >= Aeig.Nstates();
     std::vector<double> en = Aeig.dEn;
     std::vector<unsigned char> kept = Aeig.Kept;
     double ec = HUGE_VAL;
-    if (!notrunc) {
+    if (fixed) {
+      kept.assign(fixed, fixed + en.size());
+    } else if (!notrunc) {
       ec = ecut_of(Aeig.dEn, ncut);
       if (!legacy) {
         set_degen(en, myeps);
@@ -747,7 +752,11 @@ struct Chain {
       int b0 = Aeig.GetBlockLimit(b, 0), b1 = Aeig.GetBlockLimit(b, 1);
       int nbc = b1 - b0 + 1;
       int k = nbc;
-      if (!notrunc) {
+      if (fixed) {
+        k = 0;
+        for (int i = b0; i <= b1; i++) k += kept[i] != 0;
+        if (k == 0) continue;
+      } else if (!notrunc) {
         double emin = *std::min_element(en.begin() + b0, en.begin() + b1 + 1);
         if (emin > ec) continue;
         double emax = *std::max_element(en.begin() + b0, en.begin() + b1 + 1);
@@ -780,15 +789,34 @@ struct Chain {
   // AeigCut in loop order and each operator whose pair rule holds,
   // F = G[t_i][t_j] * (NeedOld ? O_old(stcf_i,stcf_j) : delta(stcf_i,stcf_j))
   // over the pre-cut blocks, then O_new = Z_i F Z_j^T on the device.
+  //   plan_update: pair list, F on the device; finish_update: the rotations and
+  //   the switch of every operator to AeigCut.  Between the two, F (upd_F), the
+  //   jobs and the previous operators with their block maps are all valid.
+  std::vector<FPair> up_pairs;
+  std::vector<RotJob> up_jobs;
+  std::vector<int> up_job_op;
+  std::vector<std::vector<int>> up_nmap, up_nbe;
+  std::vector<int64_t> up_icount;
+  int64_t up_ftot = 0, up_auxtot = 0;
+  FArgs up_fa{};
   int update_ops() {
     int rc;
+    if ((rc = plan_update())) return rc;
+    return finish_update();
+  }
+  int plan_update() {
+    int rc;
     const int nm = (int)mats.size();
-    std::vector<FPair> pairs;
+    std::vector<FPair>& pairs = up_pairs;
     std::vector<double> G;
-    std::vector<RotJob> jobs;
-    std::vector<int> job_op;
-    std::vector<std::vector<int>> nmap(nm), nbe(nm);
-    std::vector<int64_t> icount(nm, 0);
+    std::vector<RotJob>& jobs = up_jobs;
+    std::vector<int>& job_op = up_job_op;
+    pairs.clear(); jobs.clear(); job_op.clear();
+    up_nmap.assign(nm, {}); up_nbe.assign(nm, {});
+    up_icount.assign(nm, 0);
+    std::vector<std::vector<int>>& nmap = up_nmap;
+    std::vector<std::vector<int>>& nbe = up_nbe;
+    std::vector<int64_t>& icount = up_icount;
     int64_t ftot = 0, auxtot = 0, maxel = 1;
     const int nbc = AeigCut.NumBlocks();
     for (int ib = 0; ib < nbc; ib++) {
@@ -849,19 +877,29 @@ struct Chain {
       if ((rc = c->timer_begin(tm, NRG_K_FBASIS, 0.0, 8.0 * (double)ftot))) return rc;
       HIPCHK(nrgk_launch_fbasis(&fa, (int)pairs.size(), maxel, c->stream));
       if ((rc = c->timer_end(tm))) return rc;
+      up_fa = fa;
     }
+    up_ftot = ftot;
+    up_auxtot = auxtot;
+    return 0;
+  }
+  int finish_update() {
+    int rc;
+    const int nm = (int)mats.size();
+    DBuf& fb = c->buf("upd_F");
+    DBuf& ab = c->buf("upd_aux");
     for (int m = 0; m < nm; m++) {
       std::vector<RotJob> jm;
-      for (size_t q = 0; q < jobs.size(); q++)
-        if (job_op[q] == m) jm.push_back(jobs[q]);
+      for (size_t q = 0; q < up_jobs.size(); q++)
+        if (up_job_op[q] == m) jm.push_back(up_jobs[q]);
       CNRGmatrix& M = mats[m];
       int nxt = M.cur ^ 1;
       DBuf& ob = c->buf(M.buf[nxt]);
-      if ((rc = ob.reserve(sizeof(double) * (size_t)(icount[m] + 1)))) return rc;
+      if ((rc = ob.reserve(sizeof(double) * (size_t)(up_icount[m] + 1)))) return rc;
       if ((rc = rotate_dev(c, jm, dZ, (double*)fb.p, (double*)ab.p, (double*)ob.p))) return rc;
       M.cur = nxt;
-      M.MatBlockMap = nmap[m];
-      M.MatBlockBegEnd = nbe[m];
+      M.MatBlockMap = up_nmap[m];
+      M.MatBlockBegEnd = up_nbe[m];
     }
     HIPCHK(hipStreamSynchronize(c->stream));
     if ((rc = c->resolve())) return rc;
@@ -988,6 +1026,429 @@ static int dm_capture(Chain& ch, int N, int opidx, DmShell& s) {
 }
 
 // ---------------------------------------------------------------------------
+// Derivative of the fixed-truncation OneChQS loop with respect to (U, Gamma,
+// ed), forward mode (no counterpart in the reference).  Each step, for every
+// tangent t:
+//   dH  = sqrt(Lambda) th_{N-1} (x) 1 + chi df_{N-1} + dchi f_{N-1}   (tan_hn_kernel)
+//   G   = Z dH Z^T on the kept rows (two MFMA GEMMs), dE = diag G      (tan_diag_kernel)
+//   th_N = G_kk - dEmin I,  C = G_kd / (E_k - E_d)                    (tan_cut_kernel)
+//   dZ_k = C Z_d                                                       (MFMA GEMM)
+//   dO  = dZ_i F Z_j^T + Z_i F dZ_j^T + Z_i dF Z_j^T per operator block pair
+// with the kept-space Hamiltonian tangent th kept as a full k x k block per
+// kept block (rotations inside the kept space are not undone).  T chi, S and
+// <O> derivatives are host sums over dE and th (Daleckii-Krein divided
+// differences of exp(-beta E) for the off-diagonal part).
+// ---------------------------------------------------------------------------
+static E2Gemm gemm_desc(int m, int n, int k, int64_t a_off, int64_t as0, int64_t as1, int64_t b_off, int64_t bs0,
+                        int64_t bs1, int64_t c_off, int64_t cs0, int acc) {
+  E2Gemm g{};
+  g.m = m; g.n = n; g.k = k; g.dyn = -1;
+  g.a_off = a_off; g.as0 = as0; g.as1 = as1;
+  g.b_off = b_off; g.bs0 = bs0; g.bs1 = bs1;
+  g.c_off = c_off; g.cs0 = cs0;
+  g.alpha = 1.0; g.accumulate = acc;
+  return g;
+}
+// one launch of the grouped FP64 MFMA GEMM over problems sharing A, B, C bases
+static int gemm_batch(nrg_ctx* c, const std::vector<E2Gemm>& P, const double* A, const double* B, double* C,
+                      const std::string& tag) {
+  std::vector<E2Tile> T;
+  double fl = 0.0, by = 0.0;
+  for (size_t q = 0; q < P.size(); q++) {
+    const int tm = (P[q].m + 63) / 64, tn = (P[q].n + 63) / 64;
+    for (int a = 0; a < tm; a++)
+      for (int b = 0; b < tn; b++) T.push_back(E2Tile{(int)q, a, b, 0});
+    fl += 2.0 * P[q].m * (double)P[q].n * P[q].k;
+    by += 8.0 * ((double)P[q].m * P[q].k + (double)P[q].k * P[q].n + (double)P[q].m * P[q].n);
+  }
+  if (T.empty()) return 0;
+  E2Gemm* dp;
+  E2Tile* dt;
+  int rc;
+  if ((rc = upload(c, (tag + "_p").c_str(), P, &dp))) return rc;
+  if ((rc = upload(c, (tag + "_t").c_str(), T, &dt))) return rc;
+  PendingTimer tm;
+  if ((rc = c->timer_begin(tm, NRG_K_GEMM, fl, by))) return rc;
+  HIPCHK(e2_launch_gemm(dp, dt, (int)T.size(), A, B, C, c->stream));
+  return c->timer_end(tm);
+}
+
+struct Tangent {
+  nrg_ctx* c = nullptr;
+  int T = 0;
+  double dpar[NRGK_MAX_TAN][3] = {};
+  double dchi[NRGK_MAX_TAN] = {};
+  int cur = 0;                          // th / operator tangents: buffer "tg_th<cur>", "tg_op<cur>"
+  std::vector<int64_t> th_off;          // per current kept block
+  int64_t th_tot = 0;
+  std::vector<int64_t> op_cnt, op_base; // per operator: elements, start (T copies follow)
+  // per step
+  std::vector<int> kb;                  // kept rows of each Aeig block
+  std::vector<int64_t> goff;
+  int64_t gtot = 0;
+  int64_t gs = 0;                       // SetE0zero's ground state
+  std::vector<double> den;              // T x nstates, after the shift
+  std::vector<int64_t> dzoff;           // per kept block: its k x n rows of dZ
+  int64_t dztot = 0, ctot = 0;
+  std::vector<double> dsus, dent, dops;
+
+  double* buf(const std::string& n) { return (double*)c->buf(n).p; }
+  std::string thb(int w) const { return w ? "tg_th1" : "tg_th0"; }
+  std::string opb(int w) const { return w ? "tg_op1" : "tg_op0"; }
+
+  // H_{-1}: one-state blocks (src/Main/Main_SetH0.cpp:1847-1849); f and the
+  // impurity operators are constants
+  int init(nrg_ctx* cc, int ntan, const double* dp, const Chain& ch, double Lambda, double HLF, double chi_m1,
+           double Gamma) {
+    c = cc; T = ntan;
+    for (int t = 0; t < T; t++)
+      for (int q = 0; q < 3; q++) dpar[t][q] = dp[3 * t + q];
+    const double sc = 1.0 / (Lambda * HLF);
+    th_off = {0, 1, 2};
+    th_tot = 3;
+    std::vector<double> th((size_t)T * 3 + 1);
+    den.assign((size_t)T * 3, 0.0);
+    for (int t = 0; t < T; t++) {
+      const double dU = dpar[t][0], ded = dpar[t][2];
+      const double e3[3] = {0.5 * dU * sc, (ded + 0.5 * dU) * sc, (2.0 * ded + 1.5 * dU) * sc};
+      for (int b = 0; b < 3; b++) th[(size_t)t * 3 + b] = den[(size_t)t * 3 + b] = e3[b];
+      dchi[t] = chi_m1 * dpar[t][1] / (2.0 * Gamma);
+    }
+    int rc;
+    DBuf& tb = c->buf(thb(cur));
+    if ((rc = tb.reserve(sizeof(double) * th.size()))) return rc;
+    HIPCHK(hipMemcpyAsync(tb.p, th.data(), sizeof(double) * th.size(), hipMemcpyHostToDevice, c->stream));
+    const int nm = (int)ch.mats.size();
+    op_cnt.assign(nm, 0); op_base.assign(nm, 0);
+    int64_t tot = 0;
+    for (int m = 0; m < nm; m++) {
+      op_cnt[m] = ch.mats[m].MatBlockBegEnd.empty() ? 0 : ch.mats[m].MatBlockBegEnd.back() + 1;
+      op_base[m] = tot;
+      tot += (int64_t)T * op_cnt[m];
+    }
+    DBuf& ob = c->buf(opb(cur));
+    if ((rc = ob.reserve(sizeof(double) * (size_t)(tot + 1)))) return rc;
+    HIPCHK(hipMemsetAsync(ob.p, 0, sizeof(double) * (size_t)(tot + 1), c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+  }
+
+  // dH of every Abasis block, right after the base H_N assembly
+  int assemble(Chain& ch, const std::vector<double>& coef, double chi, double sqrtL) {
+    int rc;
+    const CNRGbasisarray& Ab = ch.Abasis;
+    const int nb = Ab.NumBlocks();
+    std::vector<int> bbeg(nb), sidx(nb);
+    for (int b = 0; b < nb; b++) {
+      bbeg[b] = Ab.GetBlockLimit(b, 0);
+      sidx[b] = (int)key2(Ab.GetQNumber(b, 1));
+    }
+    TanHnArgs a{};
+    int *d_bbeg, *d_bsize, *d_sidx, *d_stcf, *d_itype;
+    int64_t *d_hoff, *d_thoff;
+    double* d_coef;
+    if ((rc = upload(c, "tg_bbeg", bbeg, &d_bbeg))) return rc;
+    if ((rc = upload(c, "tg_bsize", ch.L_bsize, &d_bsize))) return rc;
+    if ((rc = upload(c, "tg_sidx", sidx, &d_sidx))) return rc;
+    if ((rc = upload(c, "tg_hoff", ch.L_hoff, &d_hoff))) return rc;
+    if ((rc = upload(c, "tg_stcf", Ab.StCameFrom, &d_stcf))) return rc;
+    if ((rc = upload(c, "tg_itype", Ab.iType, &d_itype))) return rc;
+    if ((rc = upload(c, "tg_thoff", th_off, &d_thoff))) return rc;
+    if ((rc = upload(c, "tg_coef", coef, &d_coef))) return rc;
+    if ((rc = make_oldop(c, ch.mats[0], ch.oldb, "tg_f", a.f))) return rc;
+    DBuf& hb = c->buf("tg_dH");
+    if ((rc = hb.reserve(sizeof(double) * (size_t)(T * ch.L_ztot + 1)))) return rc;
+    a.dH = (double*)hb.p; a.dh_stride = ch.L_ztot;
+    a.h_off = d_hoff; a.bbeg = d_bbeg; a.bsize = d_bsize; a.blk_sidx = d_sidx; a.stcf = d_stcf; a.itype = d_itype;
+    a.th = buf(thb(cur)); a.th_off = d_thoff; a.th_stride = th_tot;
+    a.coef = d_coef;
+    a.df = buf(opb(cur)) + op_base[0]; a.df_stride = op_cnt[0];
+    a.chi = chi; a.sqrtL = sqrtL;
+    for (int t = 0; t < T; t++) a.dchi[t] = dchi[t];
+    a.ntypes = ch.ntypes;
+    HIPCHK(nrgk_launch_tan_hn(&a, nb, ch.L_nmax, T, c->stream));
+    return 0;
+  }
+
+  // after the Kept decision: G on the kept rows, dE of every state (host copy, shifted)
+  int diag(Chain& ch) {
+    int rc;
+    const CNRGarray& A = ch.Aeig;
+    const int nb = A.NumBlocks(), ns = A.Nstates();
+    const int64_t zt = ch.L_ztot;
+    kb.assign(nb, 0); goff.assign(nb, 0);
+    gtot = 0;
+    gs = std::min_element(A.dEn.begin(), A.dEn.end()) - A.dEn.begin();
+    std::vector<TanBlk> tb(nb);
+    std::vector<E2Gemm> py, pg;
+    for (int b = 0; b < nb; b++) {
+      const int n = ch.L_bsize[b], b0 = A.GetBlockLimit(b, 0);
+      int k = 0;
+      for (int i = 0; i < n; i++) k += A.Kept[b0 + i] != 0;
+      kb[b] = k; goff[b] = gtot;
+      tb[b] = TanBlk{n, k, ch.L_hoff[b], gtot, b0};
+      for (int t = 0; t < T; t++) {
+        const int64_t h = (int64_t)t * zt + ch.L_hoff[b];
+        py.push_back(gemm_desc(n, n, n, ch.L_hoff[b], n, 1, h, n, 1, h, n, 0));      // Y = Z dH
+      }
+      gtot += (int64_t)k * n;
+    }
+    for (int b = 0; b < nb; b++) {
+      const int n = ch.L_bsize[b], k = kb[b];
+      for (int t = 0; t < T; t++)   // G_k = Y_k Z^T
+        pg.push_back(gemm_desc(k, n, n, (int64_t)t * zt + ch.L_hoff[b], n, 1, ch.L_hoff[b], 1, n,
+                               (int64_t)t * gtot + goff[b], n, 0));
+    }
+    DBuf& yb = c->buf("tg_Y");
+    DBuf& gb = c->buf("tg_G");
+    DBuf& eb = c->buf("tg_dE");
+    if ((rc = yb.reserve(sizeof(double) * (size_t)(T * zt + 1)))) return rc;
+    if ((rc = gb.reserve(sizeof(double) * (size_t)(T * gtot + 1)))) return rc;
+    if ((rc = eb.reserve(sizeof(double) * (size_t)(T * (int64_t)ns + 1)))) return rc;
+    if ((rc = gemm_batch(c, py, ch.dZ, buf("tg_dH"), (double*)yb.p, "tg_gy"))) return rc;
+    if ((rc = gemm_batch(c, pg, (double*)yb.p, ch.dZ, (double*)gb.p, "tg_gg"))) return rc;
+    TanBlk* d_tb;
+    if ((rc = upload(c, "tg_blk", tb, &d_tb))) return rc;
+    HIPCHK(nrgk_launch_tan_diag(d_tb, nb, ch.L_nmax, T, ch.dZ, (double*)yb.p, zt, (double*)gb.p, gtot,
+                                (double*)eb.p, ns, c->stream));
+    den.resize((size_t)T * ns);
+    HIPCHK(hipMemcpyAsync(den.data(), eb.p, sizeof(double) * den.size(), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int t = 0; t < T; t++) {
+      const double s = den[(size_t)t * ns + gs];
+      for (int i = 0; i < ns; i++) den[(size_t)t * ns + i] -= s;
+    }
+    return c->resolve();
+  }
+
+  // CalcSuscep / CalcEntropy (src/CalcThermo.cpp:16-121) differentiated along dE
+  void thermo(const CNRGarray& A, const double* bN4) {
+    const int ns = A.Nstates();
+    dsus.assign((size_t)T * 4, 0.0); dent.assign((size_t)T * 4, 0.0);
+    for (int t = 0; t < T; t++)
+      for (int nc = 0; nc < 4; nc++) {
+        const double bb = bN4[nc];
+        double s0 = 0, s1 = 0, d0 = 0, d1 = 0, se = 0, dse = 0;
+        for (int b = 0; b < A.NumBlocks(); b++) {
+          const double S = A.GetQNumber(b, 1), aS = 2.0 * S + 1.0, szsq = (1.0 / 12.0) * aS * (aS * aS - 1.0);
+          for (int i = A.GetBlockLimit(b, 0); i <= A.GetBlockLimit(b, 1); i++) {
+            const double E = A.dEn[i], dE = den[(size_t)t * ns + i], x = std::exp(-bb * E), dx = -bb * dE * x;
+            s0 += aS * x; s1 += szsq * x; d0 += aS * dx; d1 += szsq * dx;
+            se += aS * E * x; dse += aS * (dE * x + E * dx);
+          }
+        }
+        dsus[(size_t)t * 4 + nc] = (d1 * s0 - s1 * d0) / (s0 * s0);
+        dent[(size_t)t * 4 + nc] = bb * (dse * s0 - se * d0) / (s0 * s0) + d0 / s0;
+      }
+  }
+
+  // after CutStates: th of the new kept blocks, C, dZ_k = C Z_d
+  int cut(Chain& ch) {
+    int rc;
+    const CNRGarray& A = ch.Aeig;
+    const CNRGbasisarray& Ac = ch.AeigCut;
+    const int nbc = Ac.NumBlocks(), ns = A.Nstates();
+    std::map<int, int> from_beg;
+    for (int b = 0; b < A.NumBlocks(); b++) from_beg[A.GetBlockLimit(b, 0)] = b;
+    std::vector<TanCut> tc(nbc);
+    std::vector<int64_t> nth(nbc), coff(nbc);
+    dzoff.assign(nbc, 0);
+    int64_t tht = 0, maxel = 1;
+    ctot = 0; dztot = 0;
+    for (int cb = 0; cb < nbc; cb++) {
+      const int be = from_beg.at(Ac.BlockBegEndBC[2 * cb]);
+      const int n = ch.L_bsize[be], k = Ac.GetBlockSize(cb);
+      if (k != kb[be]) return fail(NRG_ERR_STATE, "tangent: kept rows differ from the cut");
+      nth[cb] = tht; coff[cb] = ctot; dzoff[cb] = dztot;
+      tc[cb] = TanCut{n, k, goff[be], A.GetBlockLimit(be, 0), tht, ctot};
+      tht += (int64_t)k * k; ctot += (int64_t)k * (n - k); dztot += (int64_t)k * n;
+      maxel = std::max(maxel, (int64_t)k * n);
+    }
+    double* d_E;
+    TanCut* d_tc;
+    if ((rc = upload(c, "tg_E", A.dEn, &d_E))) return rc;
+    if ((rc = upload(c, "tg_cut", tc, &d_tc))) return rc;
+    DBuf& thn = c->buf(thb(cur ^ 1));
+    DBuf& cbf = c->buf("tg_C");
+    DBuf& dzb = c->buf("tg_dZ");
+    if ((rc = thn.reserve(sizeof(double) * (size_t)(T * tht + 1)))) return rc;
+    if ((rc = cbf.reserve(sizeof(double) * (size_t)(T * ctot + 1)))) return rc;
+    if ((rc = dzb.reserve(sizeof(double) * (size_t)(T * dztot + 1)))) return rc;
+    HIPCHK(nrgk_launch_tan_cut(d_tc, nbc, maxel, T, buf("tg_G"), gtot, d_E, buf("tg_dE"), ns, gs, (double*)thn.p, tht,
+                               (double*)cbf.p, ctot, c->stream));
+    std::vector<E2Gemm> pz;
+    for (int cb = 0; cb < nbc; cb++) {
+      const int n = tc[cb].n, k = tc[cb].k;
+      const int64_t z = Ac.zoff[cb];
+      for (int t = 0; t < T; t++)   // dZ_k = C Z[k:n, :]  (K = 0 writes the zeros of a fully kept block)
+        pz.push_back(gemm_desc(k, n, n - k, (int64_t)t * ctot + coff[cb], n - k, 1, z + (int64_t)k * n, n, 1,
+                               (int64_t)t * dztot + dzoff[cb], n, 0));
+    }
+    if ((rc = gemm_batch(c, pz, (double*)cbf.p, ch.dZ, (double*)dzb.p, "tg_gz"))) return rc;
+    th_off = nth;   // th of the new kept basis is in buffer cur^1 from here on
+    th_tot = tht;
+    return 0;
+  }
+
+  // operator tangents, between Chain::plan_update and Chain::finish_update
+  int ops(Chain& ch) {
+    int rc;
+    const int nm = (int)ch.mats.size();
+    const CNRGbasisarray& Ac = ch.AeigCut;
+    std::map<int64_t, int> cut_of_z;
+    for (int cb = 0; cb < Ac.NumBlocks(); cb++) cut_of_z[Ac.zoff[cb]] = cb;
+    std::vector<int64_t> ncnt(nm), nbase(nm);
+    int64_t tot = 0;
+    for (int m = 0; m < nm; m++) { ncnt[m] = ch.up_icount[m]; nbase[m] = tot; tot += (int64_t)T * ncnt[m]; }
+    DBuf& onew = c->buf(opb(cur ^ 1));
+    if ((rc = onew.reserve(sizeof(double) * (size_t)(tot + 1)))) return rc;
+    const std::vector<RotJob>& J = ch.up_jobs;
+    std::vector<int64_t> poff(J.size()), qoff(J.size());
+    int64_t ptot = 0, qtot = 0;
+    std::vector<E2Gemm> pp, pq, po1, po2;
+    for (size_t q = 0; q < J.size(); q++) {
+      const RotJob& j = J[q];
+      poff[q] = ptot; qoff[q] = qtot;
+      ptot += (int64_t)j.ni * j.kj; qtot += (int64_t)j.ki * j.nj;
+      pp.push_back(gemm_desc(j.ni, j.kj, j.nj, j.f_off, j.nj, 1, j.zj_off, 1, j.ldzj, poff[q], j.kj, 0));   // F Z_j^T
+      pq.push_back(gemm_desc(j.ki, j.nj, j.ni, j.zi_off, j.ldzi, 1, j.f_off, j.nj, 1, qoff[q], j.nj, 0));   // Z_i F
+      const int m = ch.up_job_op[q];
+      const int ci = cut_of_z.at(j.zi_off), cj = cut_of_z.at(j.zj_off);
+      for (int t = 0; t < T; t++) {
+        const int64_t o = nbase[m] + (int64_t)t * ncnt[m] + j.out_off;
+        po1.push_back(gemm_desc(j.ki, j.kj, j.ni, (int64_t)t * dztot + dzoff[ci], j.ni, 1, poff[q], j.kj, 1, o, j.kj,
+                                0));   // dZ_i (F Z_j^T)
+        po2.push_back(gemm_desc(j.ki, j.kj, j.nj, qoff[q], j.nj, 1, (int64_t)t * dztot + dzoff[cj], 1, j.nj, o, j.kj,
+                                1));   // + (Z_i F) dZ_j^T
+      }
+    }
+    DBuf& pb = c->buf("tg_P");
+    DBuf& qb = c->buf("tg_Q");
+    if ((rc = pb.reserve(sizeof(double) * (size_t)(ptot + 1)))) return rc;
+    if ((rc = qb.reserve(sizeof(double) * (size_t)(qtot + 1)))) return rc;
+    double* F = buf("upd_F");
+    if ((rc = gemm_batch(c, pp, F, ch.dZ, (double*)pb.p, "tg_gp"))) return rc;
+    if ((rc = gemm_batch(c, pq, ch.dZ, F, (double*)qb.p, "tg_gq"))) return rc;
+    if ((rc = gemm_batch(c, po1, buf("tg_dZ"), (double*)pb.p, (double*)onew.p, "tg_go1"))) return rc;
+    if ((rc = gemm_batch(c, po2, (double*)qb.p, buf("tg_dZ"), (double*)onew.p, "tg_go2"))) return rc;
+    // + Z_i dF Z_j^T for the operators built on their previous values
+    std::vector<FPair> sub;
+    std::vector<size_t> subq;
+    int64_t maxel = 1;
+    for (size_t q = 0; q < J.size(); q++)
+      if (ch.specs[ch.up_job_op[q]].need_old) {
+        sub.push_back(ch.up_pairs[q]);
+        subq.push_back(q);
+        maxel = std::max(maxel, (int64_t)J[q].ni * J[q].nj);
+      }
+    if (!sub.empty()) {
+      FPair* d_sub;
+      if ((rc = upload(c, "tg_pairs", sub, &d_sub))) return rc;
+      const int64_t ftot = ch.up_ftot;
+      DBuf& dfb = c->buf("tg_dF");
+      if ((rc = dfb.reserve(sizeof(double) * (size_t)(T * ftot + 1)))) return rc;
+      int64_t rtot = 0;
+      std::vector<int64_t> roff(subq.size());
+      for (size_t u = 0; u < subq.size(); u++) { roff[u] = rtot; rtot += (int64_t)J[subq[u]].ni * J[subq[u]].kj; }
+      DBuf& rb = c->buf("tg_R");
+      if ((rc = rb.reserve(sizeof(double) * (size_t)(T * rtot + 1)))) return rc;
+      std::vector<E2Gemm> pr, po3;
+      for (int t = 0; t < T; t++) {
+        FArgs fa = ch.up_fa;
+        fa.F = (double*)dfb.p + (int64_t)t * ftot;
+        fa.pairs = d_sub;
+        for (int m = 0; m < nm && m < 8; m++)
+          if (ch.specs[m].need_old) fa.old[m].mat = buf(opb(cur)) + op_base[m] + (int64_t)t * op_cnt[m];
+        HIPCHK(nrgk_launch_fbasis(&fa, (int)sub.size(), maxel, c->stream));
+        for (size_t u = 0; u < subq.size(); u++) {
+          const RotJob& j = J[subq[u]];
+          const int m = ch.up_job_op[subq[u]];
+          const int64_t r = (int64_t)t * rtot + roff[u];
+          pr.push_back(gemm_desc(j.ni, j.kj, j.nj, (int64_t)t * ftot + j.f_off, j.nj, 1, j.zj_off, 1, j.ldzj, r,
+                                 j.kj, 0));   // dF Z_j^T
+          po3.push_back(gemm_desc(j.ki, j.kj, j.ni, j.zi_off, j.ldzi, 1, r, j.kj, 1,
+                                  nbase[m] + (int64_t)t * ncnt[m] + j.out_off, j.kj, 1));   // + Z_i (dF Z_j^T)
+        }
+      }
+      if ((rc = gemm_batch(c, pr, (double*)dfb.p, ch.dZ, (double*)rb.p, "tg_gr"))) return rc;
+      if ((rc = gemm_batch(c, po3, ch.dZ, (double*)rb.p, (double*)onew.p, "tg_go3"))) return rc;
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    op_cnt = ncnt;
+    op_base = nbase;
+    cur ^= 1;   // th (from cut) and the operator tangents now live in buffer cur
+    return c->resolve();
+  }
+
+  // CalcOpAvg (src/CalcOpAvg.cpp:13-84) differentiated: with the kept-space
+  // Hamiltonian diag(E) + th, sum_i O_ii x_i becomes Tr[O exp(-beta H)]
+  int op_avgs(Chain& ch, double bb) {
+    const CNRGbasisarray& A = ch.AeigCut;
+    const int nm = (int)ch.mats.size();
+    dops.assign((size_t)T * 8, 0.0);
+    std::vector<double> th((size_t)T * th_tot + 1);
+    HIPCHK(hipMemcpyAsync(th.data(), buf(thb(cur)), sizeof(double) * (size_t)(T * th_tot), hipMemcpyDeviceToHost,
+                          c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    int q = 0;
+    for (int m = 0; m < nm; m++) {
+      if (!ch.mats[m].CalcAvg) continue;
+      const CNRGmatrix& M = ch.mats[m];
+      const int64_t cnt = op_cnt[m];
+      std::vector<double> h(cnt + 1), dh((size_t)T * cnt + 1);
+      if (cnt > 0) {
+        HIPCHK(hipMemcpyAsync(h.data(), c->buf(M.buf[M.cur]).p, sizeof(double) * cnt, hipMemcpyDeviceToHost,
+                              c->stream));
+        HIPCHK(hipMemcpyAsync(dh.data(), buf(opb(cur)) + op_base[m], sizeof(double) * (size_t)(T * cnt),
+                              hipMemcpyDeviceToHost, c->stream));
+      }
+      HIPCHK(hipStreamSynchronize(c->stream));
+      for (int t = 0; t < T; t++) {
+        double s = 0, Z = 0, ds = 0, dZ = 0;
+        for (int b = 0; b < A.NumBlocks(); b++) {
+          const double dg = 2.0 * A.GetQNumber(b, 1) + 1.0;
+          const int k = A.GetBlockSize(b), b0 = A.GetBlockLimit(b, 0);
+          int64_t off = -1;
+          for (size_t r = 0; r + 1 < M.MatBlockMap.size(); r += 2)
+            if (M.MatBlockMap[r] == b && M.MatBlockMap[r + 1] == b) { off = M.MatBlockBegEnd[r]; break; }
+          const double* tb = th.data() + (size_t)t * th_tot + th_off[b];
+          for (int i = 0; i < k; i++) {
+            const double x = std::exp(-bb * A.dEn[b0 + i]);
+            Z += dg * x;
+            dZ += dg * tb[(int64_t)i * k + i] * (-bb * x);
+            if (off < 0) continue;
+            const double* O = h.data() + off;
+            s += dg * O[(int64_t)i * k + i] * x;
+            ds += dg * dh[(size_t)t * cnt + off + (int64_t)i * k + i] * x;
+            for (int j = 0; j < k; j++) {
+              const double d = A.dEn[b0 + i] - A.dEn[b0 + j], xj = std::exp(-bb * A.dEn[b0 + j]);
+              const double phi = d != 0.0 ? xj * std::expm1(-bb * d) / d : -bb * xj;
+              ds += dg * O[(int64_t)j * k + i] * tb[(int64_t)i * k + j] * phi;
+            }
+          }
+        }
+        dops[(size_t)t * 8 + q] = (ds * Z - s * dZ) / (Z * Z);
+      }
+      q++;
+    }
+    return 0;
+  }
+
+  void fill(nrg_tangent_step& dt, int N, int ns, int nops) const {
+    std::memset(&dt, 0, sizeof(dt));
+    dt.nshell = N; dt.ntan = T; dt.nstates = ns; dt.nops_avg = nops;
+    dt.den = den.data();
+    for (int t = 0; t < T; t++) {
+      for (int i = 0; i < 4; i++) {
+        dt.dsuscep[t][i] = dsus.empty() ? 0.0 : dsus[(size_t)t * 4 + i];
+        dt.dentropy[t][i] = dent.empty() ? 0.0 : dent[(size_t)t * 4 + i];
+      }
+      for (int i = 0; i < nops && i < 8; i++) dt.dopavg[t][i] = dops.empty() ? 0.0 : dops[(size_t)t * 8 + i];
+    }
+  }
+};
+
+// ---------------------------------------------------------------------------
 // OneChQS Anderson: NRG_main loop
 // ---------------------------------------------------------------------------
 // One OneChQS instance of the NRG_main loop, in phases so that several
@@ -1084,37 +1545,60 @@ struct OneChRun {
 
   // after the eigensolve: SetE0zero, SetKept (SetDegen + Ecut), CalcThermo at
   // 4 betabar values, CutStates, UpdateMatrices, CalcStuff
-  int post(int N, nrg_step_cb cb, void* user, std::vector<DmShell>* dm) {
+  //   tg: carry the derivative pass along; fixed: this step's Kept set (Aeig order)
+  int post(int N, nrg_step_cb cb, void* user, std::vector<DmShell>* dm, Tangent* tg = nullptr,
+           const unsigned char* fixed = nullptr) {
     int rc;
     const double DN = HLF * std::pow(Lambda, -(N - 1) / 2.0);
     double emin = *std::min_element(ch.Aeig.dEn.begin(), ch.Aeig.dEn.end());
     for (auto& x : ch.Aeig.dEn) x -= emin;
     set_degen(ch.Aeig.dEn, p.myeps);
-    double ecut = ecut_of(ch.Aeig.dEn, p.ncut);
     const int ns = (int)ch.Aeig.dEn.size();
-    ch.Aeig.Kept.assign(ns, 0);
+    double ecut;
     int nkept = 0;
-    for (int i = 0; i < ns; i++) { ch.Aeig.Kept[i] = ch.Aeig.dEn[i] <= ecut; nkept += ch.Aeig.Kept[i]; }
-    double thT[4], sus[4], ent[4];
+    if (fixed) {   // the given Kept set; Ecut reported as its highest level
+      ch.Aeig.Kept.assign(fixed, fixed + ns);
+      ecut = 0.0;
+      for (int i = 0; i < ns; i++)
+        if (ch.Aeig.Kept[i]) { nkept++; ecut = std::max(ecut, ch.Aeig.dEn[i]); }
+    } else {
+      ecut = ecut_of(ch.Aeig.dEn, p.ncut);
+      ch.Aeig.Kept.assign(ns, 0);
+      for (int i = 0; i < ns; i++) { ch.Aeig.Kept[i] = ch.Aeig.dEn[i] <= ecut; nkept += ch.Aeig.Kept[i]; }
+    }
+    if (tg && (rc = tg->diag(ch))) return rc;
+    double thT[4], sus[4], ent[4], bN4[4];
     for (int nc = 1; nc <= 4; nc++) {
       double bN = bbar * std::pow(Lambda, (nc - 4) / 8.0);
+      bN4[nc - 1] = bN;
       thT[nc - 1] = DN / bN;
       sus[nc - 1] = calc_suscep(bN, ch.Aeig);
       ent[nc - 1] = calc_entropy(bN, ch.Aeig);
     }
+    if (tg) tg->thermo(ch.Aeig, bN4);
     // UpdateBefCut 1: the operators are rotated over every eigenstate of the copy
     // AeigCut = Aeig with its Kept flags (CutStates with 2 Nstates), SaveArrays
     // writes that copy, and the real cut follows CalcStuff; the operators stay on
     // the pre-cut blocks, which the next basis indexes through StCameFrom
-    ch.cut_states(befcut ? 2 * ns : p.ncut, false, p.myeps);
-    if ((rc = ch.update_ops())) return rc;
+    ch.cut_states(befcut ? 2 * ns : p.ncut, false, p.myeps, fixed);
+    if (tg) {
+      if ((rc = tg->cut(ch))) return rc;
+      if ((rc = ch.plan_update())) return rc;
+      if ((rc = tg->ops(ch))) return rc;
+      if ((rc = ch.finish_update())) return rc;
+    } else if ((rc = ch.update_ops())) {
+      return rc;
+    }
     if (dm && p.calcdens == 1) {
       dm->emplace_back();
       if ((rc = dm_capture(ch, N, 1, dm->back()))) return rc;
     }
     if ((rc = ch.op_avgs(bbar, ops))) return rc;
+    if (tg && (rc = tg->op_avgs(ch, bbar))) return rc;
     if (befcut) ch.cut_states(p.ncut, false, p.myeps);
     ch.emit(cb, user, emin, ecut, nkept, thT, sus, ent, DN / bbar, ops);
+    if (tg)
+      for (int t = 0; t < NRGK_MAX_TAN; t++) tg->dchi[t] = 0.0;   // Gamma enters chi_{-1} only
     // chi for the next site: SetChainWilson (src/NRGchainClass.cpp:132-155)
     double d0 = 1.0 - std::pow(Lambda, -((double)N + 1.0));
     double d1 = std::sqrt(1.0 - std::pow(Lambda, -(2.0 * (double)N + 1.0)));
@@ -1351,6 +1835,60 @@ static int run_twochqs(nrg_ctx* c, nrg_twochqs_params p, nrg_step_cb cb, void* u
   return 0;
 }
 
+// The derivative pass: the NRG_main loop with every step's Kept set given (or
+// SetKept's when kept is NULL) and ntan tangents along dpar (dU, dGamma, ded).
+struct JvpCtx {
+  nrg_tangent_cb cb;
+  void* user;
+  Tangent* tg;
+};
+static void jvp_tramp(void* u, const nrg_step* st) {
+  JvpCtx* j = (JvpCtx*)u;
+  nrg_tangent_step dt;
+  j->tg->fill(dt, st->nshell, st->nstates, st->nops_avg);
+  if (j->cb) j->cb(j->user, st, &dt);
+}
+static int run_onechqs_jvp(nrg_ctx* c, const nrg_onechqs_params& p, const unsigned char* kept, const int64_t* kept_off,
+                           int ntan, const double* dpar, nrg_tangent_cb cb, void* user) {
+  int rc;
+  OneChRun R;
+  Tangent tg;
+  JvpCtx j{cb, user, &tg};
+  if ((rc = R.init(c, p, "", nullptr, nullptr))) return rc;
+  if ((rc = tg.init(c, ntan, dpar, R.ch, R.Lambda, R.HLF, R.par_chi, p.Gamma))) return rc;
+  if ((rc = tg.op_avgs(R.ch, R.bbar))) return rc;
+  const double DNm1 = R.HLF * R.Lambda;
+  R.ch.emit(jvp_tramp, &j, 0.0, HUGE_VAL, R.ch.Aeig.Nstates(), nullptr, nullptr, nullptr, DNm1 / R.bbar, R.ops);
+  for (int N = 0; N < p.nsitesmax; N++) {
+    R.basis();
+    R.ch.layout(0, 0);
+    R.ch.zpar ^= 1;
+    DBuf& zb = c->buf(R.ch.zpar ? "Z1" : "Z2");
+    DBuf& wb = c->buf("W");
+    if ((rc = zb.reserve(sizeof(double) * (size_t)(R.ch.L_ztot + 1)))) return rc;
+    if ((rc = wb.reserve(sizeof(double) * (size_t)(R.ch.L_wtot + 1)))) return rc;
+    if ((rc = R.assemble((double*)zb.p))) return rc;
+    if (ntan > 0) {
+      const std::vector<double>& coef = R.hn_tab.upto(max_sidx(R.ch.Abasis));
+      if ((rc = tg.assemble(R.ch, coef, R.par_chi, std::sqrt(R.Lambda)))) return rc;
+    }
+    if ((rc = eig_dev(c, R.ch.Abasis.NumBlocks(), R.ch.L_bsize.data(), R.ch.L_hoff.data(), (double*)zb.p,
+                      R.ch.L_woff.data(), (double*)wb.p, nullptr)))
+      return rc;
+    if ((rc = R.ch.collect((double*)zb.p, (double*)wb.p))) return rc;
+    const unsigned char* fixed = nullptr;
+    if (kept) {
+      if (kept_off[N + 1] - kept_off[N] != R.ch.Aeig.Nstates())
+        return fail(NRG_ERR_ARG, "nrg_onechqs_jvp: kept flags of step " + std::to_string(N) + " do not match its " +
+                                     std::to_string(R.ch.Aeig.Nstates()) + " states");
+      fixed = kept + kept_off[N];
+    }
+    if ((rc = R.post(N, jvp_tramp, &j, nullptr, ntan > 0 ? &tg : nullptr, fixed))) return rc;
+  }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
 int run_onechqs_dm(nrg_ctx* c, const nrg_onechqs_params& p, std::vector<DmShell>& dm, int befcut) {
   return run_onechqs(c, p, nullptr, nullptr, nullptr, &dm, befcut);
 }
@@ -1364,6 +1902,19 @@ extern "C" int nrg_run_onechqs(nrg_ctx* c, const nrg_onechqs_params* p, nrg_step
     return nrg::fail(NRG_ERR_ARG, "nrg_run_onechqs: bad parameters");
   HIPCHK(hipSetDevice(c->device));
   return nrg::run_onechqs(c, *p, cb, user, timing_ms);
+}
+
+extern "C" int nrg_onechqs_jvp(nrg_ctx* c, const nrg_onechqs_params* p, const unsigned char* kept,
+                               const int64_t* kept_off, int ntan, const double* dpar, nrg_tangent_cb cb, void* user) {
+  if (!c || !p) return nrg::fail(NRG_ERR_ARG, "nrg_onechqs_jvp: NULL argument");
+  if (p->nsitesmax < 0 || p->ncut < 1 || p->Lambda <= 1.0)
+    return nrg::fail(NRG_ERR_ARG, "nrg_onechqs_jvp: bad parameters");
+  if (ntan < 0 || ntan > NRG_MAX_TAN || (ntan > 0 && !dpar))
+    return nrg::fail(NRG_ERR_ARG, "nrg_onechqs_jvp: ntan must be 0..4 with dpar given");
+  if (ntan > 0 && !(p->Gamma > 0.0)) return nrg::fail(NRG_ERR_ARG, "nrg_onechqs_jvp: derivatives need Gamma > 0");
+  if (kept && !kept_off) return nrg::fail(NRG_ERR_ARG, "nrg_onechqs_jvp: kept without kept_off");
+  HIPCHK(hipSetDevice(c->device));
+  return nrg::run_onechqs_jvp(c, *p, kept, kept_off, ntan, dpar, cb, user);
 }
 
 extern "C" int nrg_run_twochqsz(nrg_ctx* c, const nrg_twochqs_params* p, nrg_step_cb cb, void* user,

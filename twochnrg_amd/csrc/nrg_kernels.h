@@ -83,7 +83,55 @@ struct CompactDesc {
   int64_t src_off, dst_off;
 };
 
+// ---- derivative (tangent) pass of the fixed-truncation iteration ----
+#define NRGK_MAX_TAN 4
+
+// dH of every Abasis block per tangent t (full n x n at t*dh_stride + h_off[b])
+struct TanHnArgs {
+  double* dH;
+  int64_t dh_stride;
+  const int64_t* h_off;
+  const int* bbeg;
+  const int* bsize;
+  const int* blk_sidx;
+  const int* stcf;
+  const int* itype;
+  const double* th;          // kept-space Hamiltonian tangent of the previous step
+  const int64_t* th_off;     // per previous kept block
+  int64_t th_stride;
+  const double* coef;        // coupling table [S index][typep][type]
+  OldOp f;                   // f of the previous step (its block map and kept-basis view)
+  const double* df;          // tangent of f, layout of f.mat
+  int64_t df_stride;
+  double chi, sqrtL;
+  double dchi[NRGK_MAX_TAN];
+  int ntypes, pad;
+};
+
+// one Aeig block for the per-state dE
+struct TanBlk {
+  int n, k;                  // block size, kept rows
+  int64_t zoff;              // Z (and dH, Y) offset
+  int64_t goff;              // G kept rows (k x n) offset
+  int64_t e0;                // first state
+};
+
+// one AeigCut block: new kept-space tangent and kept-discarded mixing C
+struct TanCut {
+  int n, k;
+  int64_t goff, e0;
+  int64_t thoff;             // k x k
+  int64_t coff;              // k x (n-k)
+};
+
 extern "C" {
+hipError_t nrgk_launch_tan_hn(const TanHnArgs* a, int nb, int nmax, int ntan, hipStream_t s);
+hipError_t nrgk_launch_tan_diag(const TanBlk* d_blk, int nb, int nmax, int ntan, const double* Z, const double* Y,
+                                int64_t y_stride, const double* G, int64_t g_stride, double* dE, int64_t e_stride,
+                                hipStream_t s);
+hipError_t nrgk_launch_tan_cut(const TanCut* d_cut, int nbc, int64_t maxel, int ntan, const double* G,
+                               int64_t g_stride, const double* E, const double* dE, int64_t e_stride, int64_t gs,
+                               double* th, int64_t th_stride, double* C, int64_t c_stride, hipStream_t s);
 hipError_t nrgk_launch_eig(const EigDesc* d_desc, const int* d_order, int nb, int nmax, double* A, double* W,
                            double* WS, int* d_info, hipStream_t s);
 hipError_t nrgk_launch_unpack(const EigDesc* d_desc, const int64_t* d_poff, const double* P, double* A, int nb,
