@@ -1171,6 +1171,7 @@ static void emit_tan(orc_step_cb cb, void *user, const arr_t *Ae, const arr_t *A
 
 struct orc_dm;
 static void dm_store_step(struct orc_dm *D, const arr_t *Ac, const arr_t *Ab, const mat_t *op);
+static void dm_store_tan(struct orc_dm *D, const orc_tan *T);
 static int run_onechqs(const orc_onechqs_params *p, orc_step_cb cb, void *user, struct orc_dm *dm, int befcut,
                        const unsigned char *kfix, const long long *kfix_off, orc_tan *T);
 int orc_run_onechqs(const orc_onechqs_params *p, orc_step_cb cb, void *user) {
@@ -1298,6 +1299,7 @@ static int run_onechqs(const orc_onechqs_params *p, orc_step_cb cb, void *user, 
     if (T) { tan_cut(T, &Ae, &Ac); tan_ops(T, &S, &Ac, &Ab, mats, nm); }
     update_matrices(&S, &Ac, &Ab, mats, nm);
     if (dm && p->calcdens == 1) dm_store_step(dm, &Ac, &Ab, &mats[1]);   /* SaveArrays src/NRGcodehandlerClass.cpp:887-933 */
+    if (dm && T && p->calcdens == 1) dm_store_tan(dm, T);
     double ops[3] = {0, 0, 0};
     int nops = 0;
     if (p->calcdens != 1) for (int m = 1; m < nm; m++) ops[nops++] = calc_opavg(bbar, &Ac, &mats[m]);
@@ -1918,6 +1920,14 @@ struct orc_dm {
   arr_t site;
   int nomega;
   double *omega, *rhow;
+  /* derivative pass (method 0): per shell the kept-space tangents of the chain
+   * (th per kept block, c_d, kept rows dZ) and of rho_N, per tangent t */
+  int ntan;
+  dvec (*th)[ORC_MAX_TAN];
+  ivec *th_off;
+  dvec (*dop)[ORC_MAX_TAN];
+  dvec (*dz)[ORC_MAX_TAN];
+  dvec (*drho)[ORC_MAX_TAN];
 };
 
 static void arr_copy(arr_t *d, const arr_t *s) {
@@ -2073,10 +2083,20 @@ static double rot_el(const mat_t *R, const int *rblk, const int *rbeg, int ist, 
   int r = (i > j) ? ij2rUpTr(n, j, i) : ij2rUpTr(n, i, j);
   return R->mat.a[R->mbe.a[2 * im] + r];
 }
+static double gather_rho(const arr_t *AcN, const arr_t *AbNp1, const arr_t *S, const mat_t *rot, mat_t *RhoN,
+                         int fdm, double TempBar, double ZNdisc);
 static double calc_rho_n(const arr_t *AcN, const arr_t *AcNp1, const arr_t *AbNp1, const arr_t *S,
                          const mat_t *RhoNp1, mat_t *RhoN, int fdm, double TempBar, double ZNdisc) {
   mat_t rot = {0};
   rotate_matrix(RhoNp1, AcNp1, &rot);
+  double tr = gather_rho(AcN, AbNp1, S, &rot, RhoN, fdm, TempBar, ZNdisc);
+  mat_free(&rot);
+  return tr;
+}
+/* the child trace-out of DM_NRG_CalcRhoN_withSU2 on an already rotated rho_{N+1} */
+static double gather_rho(const arr_t *AcN, const arr_t *AbNp1, const arr_t *S, const mat_t *rotp, mat_t *RhoN,
+                         int fdm, double TempBar, double ZNdisc) {
+  const mat_t rot = *rotp;
   int nst1 = AbNp1->stcf.n;
   int *rblk = (int *)malloc(sizeof(int) * (size_t)(nst1 + 1));
   int *rbeg = (int *)malloc(sizeof(int) * (size_t)(nblocks(&rot.b) + 1));
@@ -2130,7 +2150,6 @@ static double calc_rho_n(const arr_t *AcN, const arr_t *AcNp1, const arr_t *AbNp
     ipush(&RhoN->mbe, i1 - 1);
   }
   free(ccount); free(cbeg); free(child); free(fill); free(rblk); free(rbeg);
-  mat_free(&rot);
   return tr;
 }
 
@@ -2474,7 +2493,12 @@ static void spec_many_omegas(struct orc_dm *D, int Nw, double fac) {
  * :102-118 (:273-297); backward sweep (:312-434) with TempBar = Temp/D_{N-1} and
  * ZNdisc = Z_tot/4^{Nmax-N} (:361-366); spectral function rho_0_0 (:451-506) on the grid
  * of DM_NRG_CalcSpecFunc_ij src/DM_NRG/DM_NRG_CalcSpecFuncs.cpp:73-84 */
-orc_dm *orc_dm_run(const orc_dmnrg_params *p) {
+static void tan_dm_sweep(struct orc_dm *D, double bb_r, double bb_z, double ZN);
+orc_dm *orc_dm_run(const orc_dmnrg_params *p) { return orc_dm_run_jvp(p, NULL, NULL, 0, NULL); }
+orc_dm *orc_dm_run_jvp(const orc_dmnrg_params *p, const unsigned char *kept, const long long *kept_off, int ntan,
+                       const double *dpar) {
+  if (ntan < 0 || ntan > ORC_MAX_TAN || (ntan > 0 && (!dpar || p->method != 0 || p->update_bef_cut || !(p->Gamma > 0))))
+    return NULL;
   struct orc_dm *D = (struct orc_dm *)calloc(1, sizeof(struct orc_dm));
   D->p = *p;
   if (D->p.twindow <= 0) D->p.twindow = 2.0;
@@ -2487,7 +2511,14 @@ orc_dm *orc_dm_run(const orc_dmnrg_params *p) {
   q.U = p->U; q.Gamma = p->Gamma; q.ed = p->ed; q.Lambda = p->Lambda; q.nsitesmax = p->nsitesmax;
   q.ncut = p->ncut; q.calcdens = 1; q.disc_scheme = p->disc_scheme; q.myeps = p->myeps;
   q.betabar = p->betabar_nrg;
-  run_onechqs(&q, NULL, NULL, D, p->update_bef_cut ? 1 : 0, NULL, NULL, NULL);
+  orc_tan T;
+  memset(&T, 0, sizeof(T));
+  T.ntan = ntan;
+  for (int t = 0; t < ntan; t++)
+    for (int c = 0; c < 3; c++) T.dpar[t][c] = dpar[3 * t + c];
+  D->ntan = ntan;
+  run_onechqs(&q, NULL, NULL, D, p->update_bef_cut ? 1 : 0, kept, kept_off, ntan > 0 ? &T : NULL);
+  tan_free(&T);
   onechqs_site(&D->site);
   int Nsitesmax = p->nsitesmax;
   double DM, Temp;
@@ -2525,13 +2556,237 @@ orc_dm *orc_dm_run(const orc_dmnrg_params *p) {
     D->trace[N] = calc_rho_n(&D->acut[N], &D->acut[N + 1], &D->abasis[N + 1], &D->site, &D->rho[N + 1], &D->rho[N],
                              fdm, tb, ZTot / pow(4.0, NM - N));
   }
+  if (ntan > 0) tan_dm_sweep(D, fabs(Temp / DM) < 1e-20 ? 1e10 : DM / Temp, 1.0 / TempBar, ZN);
   return D;
+}
+
+/* ------------------------------------------------------------------ */
+/* Derivative of the DM-NRG shell sums (method 0) along the tangents of  */
+/* the fixed-truncation chain; no counterpart in the reference.         */
+/* The chain's tangents live in the kept-space frame (kept rows move    */
+/* only towards discarded ones; the kept-space Hamiltonian tangent th   */
+/* is a full block).  rho_Nmax = exp(-b H)/Z_N and the backward sweep   */
+/* rho_N = Tr_site[Z^T rho_{N+1} Z] are differentiated in that frame;   */
+/* a shell sum is then taken to the perturbed eigenbasis by            */
+/* M_ij = th_ij / (E_i - E_j) for non-degenerate pairs (dX -> dX +      */
+/* M1 X - X M2), while inside degenerate clusters th stays and moves    */
+/* the poles: with W_ij = cg A_ij Q_ij, Q = rho1 A + A rho2,            */
+/*   dA_N(w) = sum_ij W1_ij delta(w, E_j - E_i) + W2_ij d delta/dE,     */
+/*   W1 = cg (dA Q + A dQ), W2 = cg Q o (A th2_cl - th1_cl A).         */
+/* ------------------------------------------------------------------ */
+#define ORC_DEG_TOL 1e-9
+static void dm_store_tan(struct orc_dm *D, const orc_tan *T) {
+  int N = D->nsh - 1, nsh = D->nsh;
+  D->th = realloc(D->th, sizeof(*D->th) * (size_t)nsh);
+  D->th_off = (ivec *)realloc(D->th_off, sizeof(ivec) * (size_t)nsh);
+  D->dop = realloc(D->dop, sizeof(*D->dop) * (size_t)nsh);
+  D->dz = realloc(D->dz, sizeof(*D->dz) * (size_t)nsh);
+  memset(&D->th[N], 0, sizeof(D->th[N])); memset(&D->th_off[N], 0, sizeof(ivec));
+  memset(&D->dop[N], 0, sizeof(D->dop[N])); memset(&D->dz[N], 0, sizeof(D->dz[N]));
+  icopy_v(&D->th_off[N], &T->th_off);
+  for (int t = 0; t < T->ntan; t++) {
+    dcopy_v(&D->th[N][t], &T->th[t]);
+    dcopy_v(&D->dop[N][t], &T->tm[t][1]);
+    dcopy_v(&D->dz[N][t], &T->dz[t]);
+  }
+}
+/* element (i, j) of block pair (b1, b2) of a tangent stored in M's layout */
+static double get_block_tan(const mat_t *M, const dvec *tv, int b1, int b2, int i, int j) {
+  int im = find_mat_block(M, b1, b2);
+  if (im < 0) return 0.0;
+  int n1 = bsize(&M->b, b1), n2 = bsize(&M->b, b2), r = n2 * i + j;
+  if (M->upper && b1 == b2) r = (i > j) ? ij2rUpTr(n1, j, i) : ij2rUpTr(n1, i, j);
+  return tv->a[M->mbe.a[2 * im] + r];
+}
+/* divided difference of exp(-b E) between Ei and Ej */
+static double dd_exp(double b, double Ei, double Ej) {
+  double d = Ei - Ej, xj = exp(-b * Ej);
+  return d != 0.0 ? xj * expm1(-b * d) / d : -b * xj;
+}
+static void tan_dm_sweep(struct orc_dm *D, double bb_r, double bb_z, double ZN) {
+  int NM = D->nshell_max;
+  D->drho = calloc((size_t)NM + 1, sizeof(*D->drho));
+  const arr_t *A = &D->acut[NM];
+  for (int t = 0; t < D->ntan; t++) {
+    /* rho_Nmax = exp(-bb_r H) / Z_N, Z_N = sum (2S+1) exp(-bb_z E) (the reference's two shells) */
+    const double *th = D->th[NM][t].a;
+    const int *off = D->th_off[NM].a;
+    double dZ = 0.0;
+    for (int b = 0; b < nblocks(A); b++) {
+      int k = bsize(A, b), b0 = blim(A, b, 0);
+      double dg = 2.0 * getqn(A, b, 1) + 1.0;
+      for (int i = 0; i < k; i++) dZ += dg * th[off[b] + i * k + i] * (-bb_z * exp(-bb_z * A->en.a[b0 + i]));
+    }
+    dvec *out = &D->drho[NM][t];
+    for (int b = 0; b < nblocks(A); b++) {   /* packed upper, the layout of set_rho_nmax */
+      int k = bsize(A, b), b0 = blim(A, b, 0);
+      for (int i = 0; i < k; i++)
+        for (int j = i; j < k; j++) {
+          double Ei = A->en.a[b0 + i], Ej = A->en.a[b0 + j];
+          double rho = (i == j) ? exp(-bb_r * Ei) / ZN : 0.0;
+          dpush(out, dd_exp(bb_r, Ei, Ej) * th[off[b] + i * k + j] / ZN - rho * dZ / ZN);
+        }
+    }
+  }
+  for (int N = NM - 1; N >= 0; N--) {
+    const arr_t *Ac = &D->acut[N + 1];
+    const mat_t *R = &D->rho[N + 1];
+    for (int t = 0; t < D->ntan; t++) {
+      /* Z^T drho Z + dZ^T rho Z + Z^T rho dZ over the kept rows of shell N+1 */
+      mat_t tmp = {0}, rot = {0}, g = {0};
+      mat_copy(&tmp, R);
+      dcopy_v(&tmp.mat, &D->drho[N + 1][t]);
+      rotate_matrix(&tmp, Ac, &rot);
+      const double *dzb = D->dz[N + 1][t].a;
+      int i1 = 0;
+      for (int im = 0; im < R->map.n / 2; im++) {
+        int b1 = R->map.a[2 * im], b2 = R->map.a[2 * im + 1];
+        int k1 = bsize(Ac, b1), k2 = bsize(Ac, b2), n1 = bsize_bc(Ac, b1), n2 = bsize_bc(Ac, b2);
+        const double *Z1 = Ac->ev.a + Ac->bbe_ev.a[2 * b1], *Z2 = Ac->ev.a + Ac->bbe_ev.a[2 * b2];
+        const double *dZ1 = dzb + Ac->bbe_ev.a[2 * b1], *dZ2 = dzb + Ac->bbe_ev.a[2 * b2];
+        double *MZ = (double *)calloc((size_t)k1 * n2 + 1, sizeof(double));
+        double *MdZ = (double *)calloc((size_t)k1 * n2 + 1, sizeof(double));
+        for (int i = 0; i < k1; i++)
+          for (int j = 0; j < k2; j++) {
+            double m = get_block_matel(R, b1, b2, i, j);
+            for (int c = 0; c < n2; c++) { MZ[i * n2 + c] += m * Z2[j * n2 + c]; MdZ[i * n2 + c] += m * dZ2[j * n2 + c]; }
+          }
+        for (int r = 0; r < n1; r++)
+          for (int c = (rot.upper && b1 == b2) ? r : 0; c < n2; c++) {
+            double v = 0.0;
+            for (int i = 0; i < k1; i++) v += dZ1[i * n1 + r] * MZ[i * n2 + c] + Z1[i * n1 + r] * MdZ[i * n2 + c];
+            rot.mat.a[i1++] += v;
+          }
+        free(MZ); free(MdZ);
+      }
+      gather_rho(&D->acut[N], &D->abasis[N + 1], &D->site, &rot, &g, 0, 1.0, 0.0);
+      dcopy_v(&D->drho[N][t], &g.mat);
+      mat_free(&tmp); mat_free(&rot); mat_free(&g);
+    }
+  }
+}
+/* BDelta / BDeltaTemp of bdelta_ij and their derivatives with respect to the pole energy */
+static double bdelta_ij_d(const struct orc_dm *D, const arr_t *A, double w, double Ei, double Ej, double *gp) {
+  double TempDN = D->temp / calc_dn(D->p.Lambda, A->nshell);
+  double x = Ej - Ei;
+  if (fabs(x) < D->p.twindow * TempDN) {
+    double b = D->p.broadtemp, u = w - x, q = u * u + b * b;
+    *gp = (1.0 / (2.0 * M_PI)) * b * 2.0 * u / (q * q);
+    return lorentz_delta(w, x, b);
+  }
+  double g = broad_delta(w, x, D->p.bbroad);
+  double Ep = (x == 0.0) ? 1.0e-100 : x;
+  *gp = (w * Ep < 0.0) ? 0.0 : g * (2.0 * log(fabs(w / Ep)) / (D->p.bbroad * D->p.bbroad) - 1.0) / Ep;
+  return g;
+}
+/* d/dtheta_t of orc_dm_spec_shell(D, N, w) for every tangent t */
+int orc_dm_spec_shell_jvp(const orc_dm *D, int N, double w, double *out) {
+  if (D->ntan <= 0 || N < 0 || N > D->spec_nshell_max - 1) return -1;
+  const arr_t *A = &D->acut[N];
+  const mat_t *Op = &D->op[N];
+  const int costi = (N == D->spec_nshell_max - 1);
+  const double bb = (fabs(D->temp) < 1e-20) ? 1.0e10 : D->betabar_spec;
+  const double Zc = costi ? partition_func(A, bb) : 1.0;
+  for (int t = 0; t < D->ntan; t++) {
+    const double *th = D->th[N][t].a;
+    const int *off = D->th_off[N].a;
+    double dZc = 0.0;
+    if (costi)
+      for (int b = 0; b < nblocks(A); b++) {
+        int k = bsize(A, b), b0 = blim(A, b, 0);
+        double dg = 2.0 * getqn(A, b, 1) + 1.0;
+        for (int i = 0; i < k; i++) dZc += dg * th[off[b] + i * k + i] * (-bb * exp(-bb * A->en.a[b0 + i]));
+      }
+    double acc = 0.0;
+    for (int im = 0; im < Op->map.n / 2; im++) {
+      int bl[2] = {Op->map.a[2 * im], Op->map.a[2 * im + 1]};
+      int n[2], e0[2];
+      double *R[2], *dR[2], *Mr[2], *Tc[2];
+      for (int s = 0; s < 2; s++) {
+        int b = bl[s], k = bsize(A, b);
+        n[s] = k; e0[s] = blim(A, b, 0);
+        R[s] = (double *)calloc((size_t)k * k + 1, sizeof(double));
+        dR[s] = (double *)calloc((size_t)k * k + 1, sizeof(double));
+        Mr[s] = (double *)calloc((size_t)k * k + 1, sizeof(double));
+        Tc[s] = (double *)calloc((size_t)k * k + 1, sizeof(double));
+        const double *E = A->en.a + e0[s], *tb = th + off[b];
+        for (int i = 0; i < k; i++)
+          for (int j = 0; j < k; j++) {
+            double d = E[i] - E[j];
+            if (fabs(d) > ORC_DEG_TOL) Mr[s][i * k + j] = tb[i * k + j] / d; else Tc[s][i * k + j] = tb[i * k + j];
+            if (costi) {   /* rho = exp(-bb H)/Z in the kept-space frame */
+              R[s][i * k + j] = (i == j) ? exp(-bb * E[i]) / Zc : 0.0;
+              dR[s][i * k + j] = dd_exp(bb, E[i], E[j]) * tb[i * k + j] / Zc - R[s][i * k + j] * dZc / Zc;
+            } else {
+              R[s][i * k + j] = get_block_matel(&D->rho[N], b, b, i, j);
+              dR[s][i * k + j] = get_block_tan(&D->rho[N], &D->drho[N][t], b, b, i, j);
+            }
+          }
+      }
+      int n1 = n[0], n2 = n[1];
+      double *Ab = (double *)malloc(sizeof(double) * (size_t)(n1 * n2 + 1));
+      double *dA = (double *)malloc(sizeof(double) * (size_t)(n1 * n2 + 1));
+      for (int i = 0; i < n1; i++)
+        for (int j = 0; j < n2; j++) {
+          Ab[i * n2 + j] = get_block_matel(Op, bl[0], bl[1], i, j);
+          dA[i * n2 + j] = get_block_tan(Op, &D->dop[N][t], bl[0], bl[1], i, j);
+        }
+      /* to the perturbed eigenbasis: dA += M1 A - A M2, drho += M rho - rho M */
+      double *dAe = (double *)malloc(sizeof(double) * (size_t)(n1 * n2 + 1));
+      for (int i = 0; i < n1; i++)
+        for (int j = 0; j < n2; j++) {
+          double v = dA[i * n2 + j];
+          for (int l = 0; l < n1; l++) v += Mr[0][i * n1 + l] * Ab[l * n2 + j];
+          for (int l = 0; l < n2; l++) v -= Ab[i * n2 + l] * Mr[1][l * n2 + j];
+          dAe[i * n2 + j] = v;
+        }
+      double *dRe[2];
+      for (int s = 0; s < 2; s++) {
+        int k = n[s];
+        dRe[s] = (double *)malloc(sizeof(double) * (size_t)(k * k + 1));
+        for (int i = 0; i < k; i++)
+          for (int j = 0; j < k; j++) {
+            double v = dR[s][i * k + j];
+            for (int l = 0; l < k; l++) v += Mr[s][i * k + l] * R[s][l * k + j] - R[s][i * k + l] * Mr[s][l * k + j];
+            dRe[s][i * k + j] = v;
+          }
+      }
+      double cg = op_cgfactor(getqn(A, bl[0], 1), getqn(A, bl[1], 1));
+      for (int i = 0; i < n1; i++)
+        for (int j = 0; j < n2; j++) {
+          double Q = 0.0, dQ = 0.0, cl = 0.0;
+          for (int l = 0; l < n1; l++) {
+            Q += R[0][i * n1 + l] * Ab[l * n2 + j];
+            dQ += dRe[0][i * n1 + l] * Ab[l * n2 + j] + R[0][i * n1 + l] * dAe[l * n2 + j];
+            cl -= Tc[0][i * n1 + l] * Ab[l * n2 + j];
+          }
+          for (int l = 0; l < n2; l++) {
+            Q += Ab[i * n2 + l] * R[1][l * n2 + j];
+            dQ += dAe[i * n2 + l] * R[1][l * n2 + j] + Ab[i * n2 + l] * dRe[1][l * n2 + j];
+            cl += Ab[i * n2 + l] * Tc[1][l * n2 + j];
+          }
+          double gp, g = bdelta_ij_d(D, A, w, A->en.a[e0[0] + i], A->en.a[e0[1] + j], &gp);
+          acc += cg * ((dAe[i * n2 + j] * Q + Ab[i * n2 + j] * dQ) * g + Q * cl * gp);
+        }
+      for (int s = 0; s < 2; s++) { free(R[s]); free(dR[s]); free(Mr[s]); free(Tc[s]); free(dRe[s]); }
+      free(Ab); free(dA); free(dAe);
+    }
+    out[t] = acc;
+  }
+  return 0;
 }
 
 void orc_dm_free(orc_dm *D) {
   if (!D) return;
   for (int i = 0; i < D->nsh; i++) { arr_free(&D->acut[i]); arr_free(&D->abasis[i]); mat_free(&D->op[i]); }
   for (int i = 0; D->rho && i <= D->nshell_max; i++) mat_free(&D->rho[i]);
+  for (int i = 0; D->th && i < D->nsh; i++) {
+    free(D->th_off[i].a);
+    for (int t = 0; t < ORC_MAX_TAN; t++) { free(D->th[i][t].a); free(D->dop[i][t].a); free(D->dz[i][t].a); }
+  }
+  for (int i = 0; D->drho && i <= D->nshell_max; i++)
+    for (int t = 0; t < ORC_MAX_TAN; t++) free(D->drho[i][t].a);
+  free(D->th); free(D->th_off); free(D->dop); free(D->dz); free(D->drho);
   free(D->acut); free(D->abasis); free(D->op); free(D->rho); free(D->trace); free(D->wn);
   arr_free(&D->site);
   free(D->omega); free(D->rhow);

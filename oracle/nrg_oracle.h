@@ -128,6 +128,14 @@ typedef struct {
 } orc_dm_shell_view;
 
 orc_dm *orc_dm_run(const orc_dmnrg_params *p);
+/* The same run with every NRG step's Kept set given (as orc_run_onechqs_jvp;
+ * kept = NULL: SetKept) and ntan tangents along dpar (dU, dGamma, ded) carried
+ * through the chain, rho_Nmax and the backward sweep (DM-NRG, method 0,
+ * UpdateBefCut 0 only; NULL otherwise).  No counterpart in the reference. */
+orc_dm *orc_dm_run_jvp(const orc_dmnrg_params *p, const unsigned char *kept, const long long *kept_off, int ntan,
+                       const double *dpar);
+/* d/dtheta of orc_dm_spec_shell(D, N, omegabar) along each tangent (out[ntan]) */
+int orc_dm_spec_shell_jvp(const orc_dm *D, int N, double omegabar, double *out);
 void orc_dm_free(orc_dm *D);
 int orc_dm_info(const orc_dm *D, int *nshell_max, int *spec_nshell_max, double *temp);
 int orc_dm_shell(const orc_dm *D, int N, orc_dm_shell_view *v);

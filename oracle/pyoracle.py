@@ -337,8 +337,15 @@ class DmRun:
 
     def __init__(self, U, Gamma, ed, Lambda, nsitesmax, ncut, Mtemp, betabar, disc_scheme=0,
                  twindow=0.0, broadtemp=0.0, bbroad=0.0, Nw=0, myeps=0.0, betabar_nrg=0.0,
-                 method=0, update_bef_cut=0):
+                 method=0, update_bef_cut=0, kept=None, dpar=None):
+        """kept = (flags, offsets) from kept_arrays of an NRG run freezes every
+        step's Kept set; dpar rows are tangent directions (dU, dGamma, ded)
+        carried through the chain and the sweep (orc_dm_run_jvp)."""
         L = load()
+        L.orc_dm_run_jvp.restype = ctypes.c_void_p
+        L.orc_dm_run_jvp.argtypes = [ctypes.POINTER(DmParams), ctypes.POINTER(ctypes.c_ubyte),
+                                     ctypes.POINTER(ctypes.c_longlong), ctypes.c_int, _PD]
+        L.orc_dm_spec_shell_jvp.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, _PD]
         L.orc_dm_run.restype = ctypes.c_void_p
         L.orc_dm_run.argtypes = [ctypes.POINTER(DmParams)]
         L.orc_dm_free.argtypes = [ctypes.c_void_p]
@@ -356,7 +363,18 @@ class DmRun:
                      twindow, broadtemp, bbroad, Nw, method, update_bef_cut)
         self.method = method
         self._L = L
-        self.h = L.orc_dm_run(ctypes.byref(p))
+        dp_ = np.ascontiguousarray(np.zeros((0, 3)) if dpar is None else np.asarray(dpar, np.float64).reshape(-1, 3))
+        self.ntan = dp_.shape[0]
+        if kept is not None:
+            self._kf = np.ascontiguousarray(kept[0], dtype=np.uint8)
+            self._ko = np.ascontiguousarray(kept[1], dtype=np.int64)
+            kfp = self._kf.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte))
+            kop = self._ko.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong))
+        else:
+            kfp = kop = None
+        self.h = L.orc_dm_run_jvp(ctypes.byref(p), kfp, kop, self.ntan, _dp(dp_) if dp_.size else None)
+        if not self.h:
+            raise RuntimeError("orc_dm_run_jvp refused the arguments")
         a, b, t = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
         self.nsh = L.orc_dm_info(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(t))
         self.nshell_max, self.spec_nshell_max, self.temp = a.value, b.value, t.value
@@ -417,6 +435,12 @@ class DmRun:
 
     def spec_omega(self, w):
         return self._L.orc_dm_spec_omega(self.h, w)
+
+    def spec_shell_jvp(self, N, w):
+        out = np.zeros(max(self.ntan, 1))
+        if self._L.orc_dm_spec_shell_jvp(self.h, N, w, _dp(out)) != 0:
+            raise RuntimeError("orc_dm_spec_shell_jvp failed")
+        return out[:self.ntan]
 
     def norm(self):
         return self._L.orc_dm_norm(self.h)

@@ -105,3 +105,43 @@ def test_h_minus1_closed_form():
     assert np.allclose(den[0], np.array([0.5, 0.5, 1.5]) / (L * hlf), rtol=0, atol=1e-15)
     assert np.allclose(den[1], 0.0, atol=0)
     assert np.allclose(den[2], np.array([0, 1, 2]) / (L * hlf), rtol=0, atol=1e-15)
+
+
+def _dm_fd(kw, kept, key, h, shells, ws):
+    kp, km = dict(kw), dict(kw)
+    kp[key] += h
+    km[key] -= h
+    dp, dm = po.DmRun(kept=kept, **kp), po.DmRun(kept=kept, **km)
+    return {(N, w): (dp.spec_shell(N, w) - dm.spec_shell(N, w)) / (2 * h) for N in shells for w in ws}
+
+
+@pytest.mark.parametrize("ed", [-0.2, -0.25])
+def test_dmnrg_shell_sums_match_central_fd(ed):
+    """d/dtheta of the DM-NRG shell sums (DMNRG_SpecDens_M_STL, Costi form at the
+    last shell) against Richardson central differences of frozen-Kept oracle
+    runs, every shell, 7 omegas per shell on both sides of the BroadDelta /
+    LorentzDelta switch."""
+    kw = dict(U=0.5, Gamma=0.03, ed=ed, Lambda=2.5, nsitesmax=11, ncut=150, Mtemp=8, betabar=0.9)
+    base = po.run_onechqs(U=kw["U"], Gamma=kw["Gamma"], ed=ed, Lambda=2.5, nsitesmax=11, ncut=150, calcdens=1)
+    assert any(not s["kept"].all() for s in base)
+    kept = po.kept_arrays(base)
+    D = po.DmRun(kept=kept, dpar=np.eye(3), **kw)
+    shells = list(range(D.spec_nshell_max))
+    ws = [0.3, -0.7, 1.5, -2.2, 3.0, 0.05, -0.02]
+    an = {(N, w): D.spec_shell_jvp(N, w) for N in shells for w in ws}
+    for t, key in enumerate(KEYS):
+        # levels 4e-4 apart at the particle-hole point make larger steps nonlinear
+        h = 1e-5 * abs(kw[key])
+        d1 = _dm_fd(kw, kept, key, h, shells, ws)
+        d2 = _dm_fd(kw, kept, key, h / 2, shells, ws)
+        fd = {k: (4 * d2[k] - d1[k]) / 3 for k in d1}
+        scale = max(abs(v) for v in fd.values())
+        err = max(abs(fd[k] - an[k][t]) for k in fd)
+        assert err <= 1e-8 * scale, (ed, key, err, scale)
+    if ed == -0.25:
+        # particle-hole symmetry keeps A(w) = A(-w) along (1, 0, -1/2) and Gamma
+        Dp = po.DmRun(kept=kept, dpar=[[1, 0, -0.5], [0, 1, 0]], **kw)
+        for N in shells:
+            for w in (0.3, 1.5, 0.05):
+                a, b = Dp.spec_shell_jvp(N, w), Dp.spec_shell_jvp(N, -w)
+                assert np.allclose(a, b, rtol=1e-9, atol=1e-9 * max(1.0, np.max(np.abs(a)))), (N, w, a, b)
