@@ -280,6 +280,20 @@ typedef struct nrg_dm nrg_dm;
  */
 int nrg_dm_create(nrg_ctx *ctx, const nrg_dmnrg_params *p, nrg_dm **out);
 void nrg_dm_destroy(nrg_dm *dm);
+/*
+ * No counterpart in the reference: nrg_dm_create with every NRG step's Kept set
+ * frozen (kept/kept_off as in nrg_onechqs_jvp, from a calcdens-1 run with the
+ * same parameters; NULL: SetKept) and ntan tangents along dpar = (dU, dGamma,
+ * ded) carried through the chain, rho_Nmax and the backward sweep.  Tangents
+ * need method 0 (DM-NRG), update_bef_cut 0 and Gamma > 0; ntan = 0 with kept
+ * is the frozen-Kept run of finite differences.  The handle works with every
+ * nrg_dm_* call.
+ */
+int nrg_dm_create_jvp(nrg_ctx *ctx, const nrg_dmnrg_params *p, const unsigned char *kept, const int64_t *kept_off,
+                      int ntan, const double *dpar, nrg_dm **out);
+/* d/dtheta_t of nrg_dm_spec (the shell sums, Costi form at the last shell):
+ * out[i * ntan + t] for the n (shell, omegabar) pairs. */
+int nrg_dm_spec_jvp(nrg_dm *dm, int n, const int *shells, const double *omegabar, double *out);
 /* NshellMax of the sweep, CSpecFunction::NshellMax and the temperature Temp. */
 int nrg_dm_info(const nrg_dm *dm, int *nshell_max, int *spec_nshell_max, double *temp);
 /* Wall times (ms): forward chain, backward sweep, pole weights, broadening. */

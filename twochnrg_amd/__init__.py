@@ -20,7 +20,7 @@ EXPORTS = [
     "nrg_set_eig_impl", "nrg_set_plain", "nrg_eig2_stages",
     "nrg_dm_create", "nrg_dm_destroy", "nrg_dm_info", "nrg_dm_timing", "nrg_dm_rho", "nrg_dm_spec",
     "nrg_dm_spectrum", "nrg_run_onechqs_multi", "nrg_dm_spec_omega", "nrg_dm_norm", "nrg_dm_weights",
-    "nrg_run_twochqsz", "nrg_onechqs_thermo", "nrg_onechqs_jvp",
+    "nrg_run_twochqsz", "nrg_onechqs_thermo", "nrg_onechqs_jvp", "nrg_dm_create_jvp", "nrg_dm_spec_jvp",
 ]
 
 KSTAT_NAMES = ["eig", "hn", "fbasis", "gemm", "tdb_symv"]
@@ -142,6 +142,9 @@ def lib():
                                         ctypes.c_void_p, _dp]
     L.nrg_dm_create.argtypes = [ctypes.c_void_p, ctypes.POINTER(DmParams), ctypes.POINTER(ctypes.c_void_p)]
     L.nrg_dm_destroy.argtypes = [ctypes.c_void_p]
+    L.nrg_dm_create_jvp.argtypes = [ctypes.c_void_p, ctypes.POINTER(DmParams), ctypes.POINTER(ctypes.c_ubyte), _i64p,
+                                    ctypes.c_int, _dp, ctypes.POINTER(ctypes.c_void_p)]
+    L.nrg_dm_spec_jvp.argtypes = [ctypes.c_void_p, ctypes.c_int, _ip, _dp, _dp]
     L.nrg_dm_destroy.restype = None
     L.nrg_dm_info.argtypes = [ctypes.c_void_p, _ip, _ip, _dp]
     L.nrg_dm_timing.argtypes = [ctypes.c_void_p, _dp]
@@ -366,12 +369,16 @@ class Context:
 
     def dm(self, U=0.5, Gamma=0.02, ed=-0.25, Lambda=2.5, nsitesmax=40, ncut=400, Mtemp=38, betabar=1.0,
            disc_scheme=0, twindow=0.0, broadtemp=0.0, bbroad=0.0, Nw=0, myeps=0.0, betabar_nrg=0.0,
-           method=0, update_bef_cut=0):
+           method=0, update_bef_cut=0, kept=None, dpar=None):
         """DM-NRG handle (nrg_dm_create): forward chain + rho sweep on the device.
         method 0: DM-NRG, 1: CFS (DM_NRG -C), 2: FDM (DM_NRG -F); CFS/FDM want
-        update_bef_cut=1 (the chain keeps the discarded states per shell)."""
+        update_bef_cut=1 (the chain keeps the discarded states per shell).
+        kept = (flags, offsets) from kept_arrays of a calcdens-1 run freezes every
+        step's Kept set; dpar rows are tangent directions (dU, dGamma, ded)
+        (nrg_dm_create_jvp; DM-NRG method only)."""
         return DMNRG(self, DmParams(U, Gamma, ed, Lambda, nsitesmax, ncut, disc_scheme, myeps, betabar_nrg,
-                                    Mtemp, betabar, twindow, broadtemp, bbroad, Nw, method, update_bef_cut))
+                                    Mtemp, betabar, twindow, broadtemp, bbroad, Nw, method, update_bef_cut),
+                     kept=kept, dpar=dpar)
 
     def run_twochqs(self, U=0.4, Gamma1=0.03, Gamma2=0.03, ed=-0.2, Lambda=3.0, nsitesmax=3, ncut=3000,
                     betabar=0.727, record_blocks=False, keep_steps=True, on_step=None, qsz=False):
@@ -465,10 +472,23 @@ def kept_arrays(steps):
 class DMNRG:
     """Handle of a device DM-NRG run (include/twochnrg_amd.h nrg_dm_*)."""
 
-    def __init__(self, ctx, params):
+    def __init__(self, ctx, params, kept=None, dpar=None):
         self.ctx = ctx
         h = ctypes.c_void_p()
-        _check(lib().nrg_dm_create(ctx.h, ctypes.byref(params), ctypes.byref(h)))
+        if kept is None and dpar is None:
+            self.ntan = 0
+            _check(lib().nrg_dm_create(ctx.h, ctypes.byref(params), ctypes.byref(h)))
+        else:
+            dp_ = np.ascontiguousarray(np.zeros((0, 3)) if dpar is None else np.asarray(dpar, np.float64).reshape(-1, 3))
+            self.ntan = dp_.shape[0]
+            if kept is not None:
+                kf = np.ascontiguousarray(kept[0], dtype=np.uint8)
+                ko = np.ascontiguousarray(kept[1], dtype=np.int64)
+                kfp, kop = kf.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte)), _ptr(ko, _i64p)
+            else:
+                kfp = kop = None
+            _check(lib().nrg_dm_create_jvp(ctx.h, ctypes.byref(params), kfp, kop, self.ntan,
+                                           _ptr(dp_) if dp_.size else None, ctypes.byref(h)))
         self.h = h
         a, b, t = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
         _check(lib().nrg_dm_info(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(t)))
@@ -508,6 +528,14 @@ class DMNRG:
         out = np.zeros(len(sh))
         _check(lib().nrg_dm_spec(self.h, len(sh), _ptr(sh, _ip), _ptr(w), _ptr(out)))
         return out
+
+    def spec_jvp(self, shells, omegabar):
+        """d/dtheta of spec(): array (len(shells), ntan)."""
+        sh = np.ascontiguousarray(shells, np.int32)
+        w = np.ascontiguousarray(omegabar, np.float64)
+        out = np.zeros(max(len(sh) * self.ntan, 1))
+        _check(lib().nrg_dm_spec_jvp(self.h, len(sh), _ptr(sh, _ip), _ptr(w), _ptr(out)))
+        return out[:len(sh) * self.ntan].reshape(len(sh), self.ntan)
 
     def spec_omega(self, omega):
         """CalcSpecDM_NRG(omega): the shell-window sum (CFS/FDM)."""

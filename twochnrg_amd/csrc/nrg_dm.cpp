@@ -30,6 +30,8 @@
 
 namespace nrg {
 int run_onechqs_dm(nrg_ctx* c, const nrg_onechqs_params& p, std::vector<DmShell>& dm, int befcut);
+int run_onechqs_dm_jvp(nrg_ctx* c, const nrg_onechqs_params& p, std::vector<DmShell>& dm, const unsigned char* kept,
+                       const int64_t* kept_off, int ntan, const double* dpar);
 double cgordan(double J1, double M1, double J2, double M2, double J, double M);  // nrg_driver.cpp
 }
 
@@ -47,6 +49,11 @@ struct nrg_dm {
   // pole arena (filled once, all shells): W, dE and per-shell ranges
   bool poles_ready = false;
   std::vector<int64_t> pole_beg, pole_end;
+  // derivative pass (DM-NRG method): tangents carried through the chain and the
+  // sweep (dmdR<N>), pole weights of delta and of its derivative (dm_W1, dm_W2)
+  int ntan = 0;
+  bool tpoles_ready = false;
+  int64_t tpole_tot = 0;
   double ms_forward = 0, ms_sweep = 0, ms_poles = 0, ms_broaden = 0;
 };
 
@@ -110,6 +117,19 @@ E2Gemm gemm(int m, int n, int k, int64_t a_off, int64_t as0, int64_t as1, int64_
   g.alpha = 1.0; g.accumulate = 0;
   return g;
 }
+E2Gemm acc(E2Gemm g, double alpha = 1.0) {   // C += alpha A B
+  g.accumulate = 1;
+  g.alpha = alpha;
+  return g;
+}
+double* dbuf(nrg_ctx* c, const std::string& n) { return (double*)c->buf(n).p; }
+// divided difference of exp(-b E) between Ei and Ej
+double dd_exp(double b, double Ei, double Ej) {
+  const double d = Ei - Ej, xj = std::exp(-b * Ej);
+  return d != 0.0 ? xj * std::expm1(-b * d) / d : -b * xj;
+}
+// pairs of the derivative of the shell sums closer than this are one degenerate cluster
+const double kDegTol = 1e-9;
 
 // rho_{N} from rho_{N+1}: rotation into the pre-cut basis, then the child trace-out;
 // fdm: e^{-E/TempBar}/zdisc on the diagonal of the discarded states (zdisc != 0)
@@ -215,6 +235,40 @@ int sweep_shell(nrg_dm* D, int N, bool fdm, double TempBar, double zdisc) {
   HIPCHK(dm_launch_gather(dgb, (int)gb.size(), kmax, dsg, (double*)rb.p, den, fdm_diag ? 1.0 / TempBar : 0.0,
                           fdm_diag ? zdisc : 1.0, (double*)ob.p, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
+  if (D->ntan > 0) {
+    // derivative, in the chain's kept-space frame: the rotation of rho_{N+1}
+    // moves by Z^T (drho Z + rho dZ) + dZ^T (rho Z); the trace-out is linear
+    const int T = D->ntan;
+    std::vector<E2Gemm> h1, h2, h3, h4;
+    for (int q = 0; q < nbb; q++) {
+      const int k = b.bsize(q), n = b.bsize_bc(q);
+      for (int t = 0; t < T; t++) {
+        const int64_t tt = (int64_t)t * ttot + toff[q], tr = (int64_t)t * rtot + roff[q];
+        const int64_t dz = (int64_t)t * b.dz_tot + b.dz_off[q];
+        h1.push_back(gemm(k, n, k, (int64_t)t * D->rho_len[N + 1] + D->rho_off[N + 1][q], k, 1, b.zoff[q], n, 1, tt,
+                          n));                                                                  // drho Z
+        h2.push_back(acc(gemm(k, n, k, D->rho_off[N + 1][q], k, 1, dz, n, 1, tt, n)));          // + rho dZ
+        h3.push_back(gemm(n, n, k, b.zoff[q], 1, n, tt, n, 1, tr, n));                          // Z^T (.)
+        h4.push_back(acc(gemm(n, n, k, dz, 1, n, toff[q], n, 1, tr, n)));                       // + dZ^T (rho Z)
+      }
+    }
+    DBuf& ttb = c->buf("dm_tT");
+    DBuf& trb = c->buf("dm_trot");
+    DBuf& dob = c->buf("dmdR" + std::to_string(N));
+    if ((rc = ttb.reserve(sizeof(double) * (size_t)(T * ttot + 1)))) return rc;
+    if ((rc = trb.reserve(sizeof(double) * (size_t)(T * rtot + 1)))) return rc;
+    if ((rc = dob.reserve(sizeof(double) * (size_t)(T * otot + 1)))) return rc;
+    double* dR1t = dbuf(c, "dmdR" + std::to_string(N + 1));
+    double* dZ1t = dbuf(c, b.dzbuf);
+    if ((rc = gemm_groups(c, "dm_h1", h1, dR1t, dZ1, (double*)ttb.p))) return rc;
+    if ((rc = gemm_groups(c, "dm_h2", h2, dR1, dZ1t, (double*)ttb.p))) return rc;
+    if ((rc = gemm_groups(c, "dm_h3", h3, dZ1, (double*)ttb.p, (double*)trb.p))) return rc;
+    if ((rc = gemm_groups(c, "dm_h4", h4, dZ1t, (double*)tb.p, (double*)trb.p))) return rc;
+    for (int t = 0; t < T; t++)
+      HIPCHK(dm_launch_gather(dgb, (int)gb.size(), kmax, dsg, (double*)trb.p + (int64_t)t * rtot, den, 0.0, 1.0,
+                              (double*)dob.p + (int64_t)t * otot, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  }
   return NRG_OK;
 }
 
@@ -320,6 +374,179 @@ int build_poles(nrg_dm* D) {
   return NRG_OK;
 }
 
+// Derivative pole weights of every shell (DM-NRG method; the last shell in the
+// Costi form).  The shell's tangents live in the chain's kept-space frame; the
+// perturbed eigenbasis is reached with M = th / (E_i - E_j) on non-degenerate
+// pairs (dX -> dX + M1 X - X M2), while inside degenerate clusters th (Tc)
+// moves the poles:  W1 = cg (dA o Q + A o dQ), W2 = cg Q o (A Tc2 - Tc1 A),
+// Q = rho1 A + A rho2, and the shell sum's derivative is
+// sum_p W1_p delta(w, dE_p) + W2_p d delta / d dE_p.
+int build_poles_jvp(nrg_dm* D) {
+  nrg_ctx* c = D->c;
+  int rc;
+  const int T = D->ntan, ns = D->specNM;
+  const double bb = (std::fabs(D->temp) < 1e-20) ? 1.0e10 : D->betabar_spec;
+  int64_t ptot = 0;
+  for (int N = 0; N < ns; N++)
+    for (size_t q = 0; q + 1 < D->sh[N].op_map.size(); q += 2)
+      ptot += (int64_t)D->sh[N].bsize(D->sh[N].op_map[q]) * D->sh[N].bsize(D->sh[N].op_map[q + 1]);
+  if (ptot != D->pole_end[ns - 1]) return fail(NRG_ERR_STATE, "DM-NRG derivative: pole layout differs");
+  DBuf& w1b = c->buf("dm_W1");
+  DBuf& w2b = c->buf("dm_W2");
+  if ((rc = w1b.reserve(sizeof(double) * (size_t)(T * ptot + 1)))) return rc;
+  if ((rc = w2b.reserve(sizeof(double) * (size_t)(T * ptot + 1)))) return rc;
+  int64_t pole = 0;
+  for (int N = 0; N < ns; N++) {
+    const DmShell& s = D->sh[N];
+    const bool costi = (N == ns - 1);
+    const int nb = s.nblocks();
+    const int64_t btot = s.th_tot, opc = s.op_cnt;
+    std::vector<int64_t> boff(s.th_off);
+    if ((int)boff.size() != nb) return fail(NRG_ERR_STATE, "DM-NRG derivative: block layout");
+    std::vector<DmFrame> fr(nb);
+    int kkmax = 1;
+    for (int b = 0; b < nb; b++) {
+      const int k = s.bsize(b);
+      if (!costi && D->rho_off[N][b] != boff[b]) return fail(NRG_ERR_STATE, "DM-NRG derivative: rho layout");
+      fr[b] = DmFrame{boff[b], (int64_t)s.bbe[2 * b], k, 0};
+      kkmax = std::max(kkmax, k * k);
+    }
+    DmFrame* dfr;
+    double* den;
+    if ((rc = upload(c, "dmt_fr", fr, &dfr))) return rc;
+    if ((rc = upload(c, "dmt_en", s.en, &den))) return rc;
+    DBuf& mb = c->buf("dmt_M");
+    DBuf& tcb = c->buf("dmt_Tc");
+    DBuf& drb = c->buf("dmt_dRe");
+    if ((rc = mb.reserve(sizeof(double) * (size_t)(T * btot + 1)))) return rc;
+    if ((rc = tcb.reserve(sizeof(double) * (size_t)(T * btot + 1)))) return rc;
+    if ((rc = drb.reserve(sizeof(double) * (size_t)(T * btot + 1)))) return rc;
+    HIPCHK(dm_launch_frame(dfr, nb, kkmax, T, dbuf(c, s.thbuf), s.th_tot, den, kDegTol, (double*)mb.p,
+                           (double*)tcb.p, btot, c->stream));
+    double* R;
+    if (costi) {
+      // rho = exp(-bb H)/Z on the kept space; in the eigenbasis its tangent lives
+      // in the degenerate clusters: dd_exp(bb) th / Z, minus rho dZ / Z
+      std::vector<double> th((size_t)T * btot + 1);
+      HIPCHK(hipMemcpyAsync(th.data(), dbuf(c, s.thbuf), sizeof(double) * (size_t)(T * btot), hipMemcpyDeviceToHost,
+                            c->stream));
+      HIPCHK(hipStreamSynchronize(c->stream));
+      const double Z = partition_func(s, bb);
+      std::vector<double> r((size_t)btot + 1, 0.0), dr((size_t)T * btot + 1, 0.0);
+      for (int b = 0; b < nb; b++) {
+        const int k = s.bsize(b), i0 = s.bbe[2 * b];
+        for (int i = 0; i < k; i++) r[boff[b] + (int64_t)i * k + i] = std::exp(-bb * s.en[i0 + i]) / Z;
+      }
+      for (int t = 0; t < T; t++) {
+        double dZ = 0.0;
+        for (int b = 0; b < nb; b++) {
+          const int k = s.bsize(b), i0 = s.bbe[2 * b];
+          const double dg = 2.0 * s.qn[2 * b + 1] + 1.0;
+          for (int i = 0; i < k; i++)
+            dZ += dg * th[(size_t)t * btot + boff[b] + (int64_t)i * k + i] * (-bb * std::exp(-bb * s.en[i0 + i]));
+        }
+        for (int b = 0; b < nb; b++) {
+          const int k = s.bsize(b), i0 = s.bbe[2 * b];
+          for (int i = 0; i < k; i++)
+            for (int j = 0; j < k; j++) {
+              const double Ei = s.en[i0 + i], Ej = s.en[i0 + j];
+              const int64_t e = boff[b] + (int64_t)i * k + j;
+              double v = std::fabs(Ei - Ej) <= kDegTol ? dd_exp(bb, Ei, Ej) * th[(size_t)t * btot + e] / Z : 0.0;
+              if (i == j) v -= r[e] * dZ / Z;
+              dr[(size_t)t * btot + e] = v;
+            }
+        }
+      }
+      double* dRc;
+      if ((rc = upload(c, "dmt_Rc", r, &dRc))) return rc;
+      R = dRc;
+      HIPCHK(hipMemcpyAsync(drb.p, dr.data(), sizeof(double) * (size_t)(T * btot), hipMemcpyHostToDevice, c->stream));
+      HIPCHK(hipStreamSynchronize(c->stream));
+    } else {
+      R = dbuf(c, "dmR" + std::to_string(N));
+      HIPCHK(hipMemcpyAsync(drb.p, dbuf(c, "dmdR" + std::to_string(N)), sizeof(double) * (size_t)(T * btot),
+                            hipMemcpyDeviceToDevice, c->stream));
+      std::vector<E2Gemm> g1, g2;   // drho += M rho - rho M
+      for (int b = 0; b < nb; b++) {
+        const int k = s.bsize(b);
+        for (int t = 0; t < T; t++) {
+          const int64_t o = (int64_t)t * btot + boff[b];
+          g1.push_back(acc(gemm(k, k, k, o, k, 1, boff[b], k, 1, o, k)));
+          g2.push_back(acc(gemm(k, k, k, boff[b], k, 1, o, k, 1, o, k), -1.0));
+        }
+      }
+      if ((rc = gemm_groups(c, "dmt_r1", g1, (double*)mb.p, R, (double*)drb.p))) return rc;
+      if ((rc = gemm_groups(c, "dmt_r2", g2, R, (double*)mb.p, (double*)drb.p))) return rc;
+    }
+    // operator block pairs
+    std::vector<DmTanPair> tp;
+    int64_t qtot = 0;
+    int kk2 = 1;
+    for (size_t q = 0; q + 1 < s.op_map.size(); q += 2) {
+      const int b1 = s.op_map[q], b2 = s.op_map[q + 1];
+      DmTanPair P{};
+      P.a_off = s.op_mbe[q]; P.k1 = s.bsize(b1); P.k2 = s.bsize(b2);
+      P.q_off = qtot; P.pole_off = pole;
+      P.cg = op_cgfactor(s.qn[2 * b1 + 1], s.qn[2 * b2 + 1]);
+      qtot += (int64_t)P.k1 * P.k2;
+      pole += (int64_t)P.k1 * P.k2;
+      kk2 = std::max(kk2, P.k1 * P.k2);
+      tp.push_back(P);
+    }
+    if (tp.empty()) continue;
+    DBuf& dab = c->buf("dmt_dAe");
+    DBuf& qb = c->buf("dmt_Q");
+    DBuf& dqb = c->buf("dmt_dQ");
+    DBuf& clb = c->buf("dmt_cl");
+    if ((rc = dab.reserve(sizeof(double) * (size_t)(T * opc + 1)))) return rc;
+    if ((rc = qb.reserve(sizeof(double) * (size_t)(qtot + 1)))) return rc;
+    if ((rc = dqb.reserve(sizeof(double) * (size_t)(T * qtot + 1)))) return rc;
+    if ((rc = clb.reserve(sizeof(double) * (size_t)(T * qtot + 1)))) return rc;
+    HIPCHK(hipMemcpyAsync(dab.p, dbuf(c, s.dopbuf), sizeof(double) * (size_t)(T * opc), hipMemcpyDeviceToDevice,
+                          c->stream));
+    double* A = dbuf(c, s.opbuf);
+    std::vector<E2Gemm> ga1, ga2, gq1, gq2, gd1, gd2, gd3, gd4, gc1, gc2;
+    for (size_t u = 0; u < tp.size(); u++) {
+      const DmTanPair& P = tp[u];
+      const int b1 = s.op_map[2 * u], b2 = s.op_map[2 * u + 1];
+      const int k1 = P.k1, k2 = P.k2;
+      gq1.push_back(gemm(k1, k2, k1, boff[b1], k1, 1, P.a_off, k2, 1, P.q_off, k2));             // rho1 A
+      gq2.push_back(acc(gemm(k1, k2, k2, P.a_off, k2, 1, boff[b2], k2, 1, P.q_off, k2)));        // + A rho2
+      for (int t = 0; t < T; t++) {
+        const int64_t m1 = (int64_t)t * btot + boff[b1], m2 = (int64_t)t * btot + boff[b2];
+        const int64_t da = (int64_t)t * opc + P.a_off, dq = (int64_t)t * qtot + P.q_off;
+        ga1.push_back(acc(gemm(k1, k2, k1, m1, k1, 1, P.a_off, k2, 1, da, k2)));              // dA += M1 A
+        ga2.push_back(acc(gemm(k1, k2, k2, P.a_off, k2, 1, m2, k2, 1, da, k2), -1.0));        //    -= A M2
+        gd1.push_back(gemm(k1, k2, k1, m1, k1, 1, P.a_off, k2, 1, dq, k2));                   // dQ = drho1 A
+        gd2.push_back(acc(gemm(k1, k2, k1, boff[b1], k1, 1, da, k2, 1, dq, k2)));             //  + rho1 dA
+        gd3.push_back(acc(gemm(k1, k2, k2, da, k2, 1, boff[b2], k2, 1, dq, k2)));             //  + dA rho2
+        gd4.push_back(acc(gemm(k1, k2, k2, P.a_off, k2, 1, m2, k2, 1, dq, k2)));              //  + A drho2
+        gc1.push_back(gemm(k1, k2, k2, P.a_off, k2, 1, m2, k2, 1, dq, k2));                   // cl = A Tc2
+        gc2.push_back(acc(gemm(k1, k2, k1, m1, k1, 1, P.a_off, k2, 1, dq, k2), -1.0));        //    - Tc1 A
+      }
+    }
+    double *M = (double*)mb.p, *Tc = (double*)tcb.p, *dRe = (double*)drb.p, *dAe = (double*)dab.p;
+    if ((rc = gemm_groups(c, "dmt_a1", ga1, M, A, dAe))) return rc;
+    if ((rc = gemm_groups(c, "dmt_a2", ga2, A, M, dAe))) return rc;
+    if ((rc = gemm_groups(c, "dmt_q1", gq1, R, A, (double*)qb.p))) return rc;
+    if ((rc = gemm_groups(c, "dmt_q2", gq2, A, R, (double*)qb.p))) return rc;
+    if ((rc = gemm_groups(c, "dmt_d1", gd1, dRe, A, (double*)dqb.p))) return rc;
+    if ((rc = gemm_groups(c, "dmt_d2", gd2, R, dAe, (double*)dqb.p))) return rc;
+    if ((rc = gemm_groups(c, "dmt_d3", gd3, dAe, R, (double*)dqb.p))) return rc;
+    if ((rc = gemm_groups(c, "dmt_d4", gd4, A, dRe, (double*)dqb.p))) return rc;
+    if ((rc = gemm_groups(c, "dmt_c1", gc1, A, Tc, (double*)clb.p))) return rc;
+    if ((rc = gemm_groups(c, "dmt_c2", gc2, Tc, A, (double*)clb.p))) return rc;
+    DmTanPair* dtp;
+    if ((rc = upload(c, "dmt_pairs", tp, &dtp))) return rc;
+    HIPCHK(dm_launch_tan_weights(dtp, (int)tp.size(), kk2, T, A, dAe, opc, (double*)qb.p, (double*)dqb.p,
+                                 (double*)clb.p, qtot, (double*)w1b.p, (double*)w2b.p, ptot, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  }
+  D->tpole_tot = ptot;
+  D->tpoles_ready = true;
+  return NRG_OK;
+}
+
 int bd_kind(const nrg_dm* D) {   // BDelta / BDeltaTemp by method (src/DM_NRG/DM_NRG.cpp:474-482)
   return D->method == 2 ? DM_BD_LOGGAUSS_GAUSS : (D->method == 1 ? DM_BD_BROAD_ANDERS : DM_BD_BROAD_LORENTZ);
 }
@@ -377,6 +604,62 @@ int spec_tasks(nrg_dm* D, int n, const int* shells, const double* w, double* out
   return NRG_OK;
 }
 
+// derivative of the shell sums at (shell, omegabar): out[i * ntan + t]
+int spec_tasks_jvp(nrg_dm* D, int n, const int* shells, const double* w, double* out) {
+  nrg_ctx* c = D->c;
+  int rc;
+  const int T = D->ntan;
+  if (!D->poles_ready) {
+    auto t0 = std::chrono::steady_clock::now();
+    if ((rc = build_poles(D))) return rc;
+    D->ms_poles += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  }
+  if (!D->tpoles_ready) {
+    auto t0 = std::chrono::steady_clock::now();
+    if ((rc = build_poles_jvp(D))) return rc;
+    D->ms_poles += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  }
+  auto t0 = std::chrono::steady_clock::now();
+  const int BCH = 256 * 8;
+  std::vector<DmTask> tasks;
+  std::vector<int> slot;
+  int cb = 0, maxchunk = 0;
+  for (int i = 0; i < n; i++) {
+    for (int t = 0; t < T; t++) out[(size_t)i * T + t] = 0.0;
+    const int N = shells[i];
+    if (N < 0 || N > D->specNM - 1) continue;
+    DmTask Tk{};
+    Tk.pole_beg = D->pole_beg[N];
+    Tk.pole_end = D->pole_end[N];
+    Tk.w = w[i];
+    Tk.twin = D->p.twindow * D->temp / calc_dn(D->p.Lambda, N);
+    Tk.nchunk = (int)((Tk.pole_end - Tk.pole_beg + BCH - 1) / BCH);
+    Tk.chunk_beg = cb;
+    cb += Tk.nchunk;
+    maxchunk = std::max(maxchunk, Tk.nchunk);
+    tasks.push_back(Tk);
+    slot.push_back(i);
+  }
+  if (tasks.empty()) return NRG_OK;
+  const int nt = (int)tasks.size();
+  DBuf& pb = c->buf("dmt_partial");
+  DBuf& ob = c->buf("dmt_out");
+  if ((rc = pb.reserve(sizeof(double) * (size_t)(T * cb + 1)))) return rc;
+  if ((rc = ob.reserve(sizeof(double) * (size_t)(T * nt + 1)))) return rc;
+  DmTask* dt;
+  if ((rc = upload(c, "dmt_tasks", tasks, &dt))) return rc;
+  HIPCHK(dm_launch_broaden_jvp(dt, nt, maxchunk, T, dbuf(c, "dm_W1"), dbuf(c, "dm_W2"), D->tpole_tot,
+                               dbuf(c, "dm_dE"), D->p.bbroad, D->p.broadtemp, (double*)pb.p, cb, (double*)ob.p,
+                               c->stream));
+  std::vector<double> res((size_t)T * nt);
+  HIPCHK(hipMemcpyAsync(res.data(), ob.p, sizeof(double) * res.size(), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (int q = 0; q < nt; q++)
+    for (int t = 0; t < T; t++) out[(size_t)slot[q] * T + t] += res[(size_t)t * nt + q];
+  D->ms_broaden += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return NRG_OK;
+}
+
 // CalcNfromOmega  src/SpecFuncClass.cpp:41-49 (z = 1)
 int n_from_omega(double L, double w) {
   const double HLF = 0.5 * (1.0 + 1.0 / L);
@@ -422,8 +705,13 @@ double lin_interp(const std::vector<double>& x, const std::vector<double>& y, do
 
 }  // namespace
 
-extern "C" int nrg_dm_create(nrg_ctx* c, const nrg_dmnrg_params* p, nrg_dm** out) {
+static int dm_create(nrg_ctx* c, const nrg_dmnrg_params* p, const unsigned char* kept, const int64_t* kept_off,
+                     int ntan, const double* dpar, nrg_dm** out) {
   if (!c || !p || !out) return fail(NRG_ERR_ARG, "nrg_dm_create: NULL argument");
+  if (ntan < 0 || ntan > NRG_MAX_TAN || (ntan > 0 && !dpar) || (kept && !kept_off))
+    return fail(NRG_ERR_ARG, "nrg_dm_create_jvp: bad tangent or kept arguments");
+  if (ntan > 0 && (p->method != 0 || p->update_bef_cut || !(p->Gamma > 0.0)))
+    return fail(NRG_ERR_ARG, "nrg_dm_create_jvp: derivatives need the DM-NRG method, UpdateBefCut 0 and Gamma > 0");
   if (p->nsitesmax < 2 || p->ncut < 1 || p->Lambda <= 1.0 || p->betabar <= 0.0 || p->Mtemp < 1 ||
       p->method < 0 || p->method > 2)
     return fail(NRG_ERR_ARG, "nrg_dm_create: bad parameters");
@@ -441,7 +729,14 @@ extern "C" int nrg_dm_create(nrg_ctx* c, const nrg_dmnrg_params* p, nrg_dm** out
   q.ncut = p->ncut; q.calcdens = 1; q.disc_scheme = p->disc_scheme; q.myeps = p->myeps; q.betabar = p->betabar_nrg;
   int rc;
   auto t0 = std::chrono::steady_clock::now();
-  if ((rc = run_onechqs_dm(c, q, D->sh, p->update_bef_cut ? 1 : 0))) { delete D; return rc; }
+  D->ntan = ntan;
+  if (kept || ntan > 0) {
+    if (p->update_bef_cut) { delete D; return fail(NRG_ERR_ARG, "nrg_dm_create_jvp: frozen Kept sets need UpdateBefCut 0"); }
+    if ((rc = run_onechqs_dm_jvp(c, q, D->sh, kept, kept_off, ntan, dpar))) { delete D; return rc; }
+  } else if ((rc = run_onechqs_dm(c, q, D->sh, p->update_bef_cut ? 1 : 0))) {
+    delete D;
+    return rc;
+  }
   auto t1 = std::chrono::steady_clock::now();
   // temperature from Mtemp (src/DM_NRG/DM_NRG.cpp:102-118); FDM keeps every shell (:105)
   int Nsitesmax = p->nsitesmax;
@@ -496,6 +791,39 @@ extern "C" int nrg_dm_create(nrg_ctx* c, const nrg_dmnrg_params* p, nrg_dm** out
     if ((rc = rb.reserve(sizeof(double) * (r.size() + 1)))) { delete D; return rc; }
     HIPCHK(hipMemcpyAsync(rb.p, r.data(), sizeof(double) * r.size(), hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
+    if (ntan > 0) {
+      // its derivative in the kept-space frame of the chain: exp(-bb H)/Z_N with
+      // H = diag(E) + th and Z_N = sum (2S+1) exp(-bz E) at the reference's other shell
+      const double bz = 1.0 / (D->temp / calc_dn(p->Lambda, NM - 1));
+      const int64_t tht = s.th_tot, rl = D->rho_len[NM];
+      std::vector<double> th((size_t)ntan * tht + 1), dr((size_t)ntan * rl + 1, 0.0);
+      HIPCHK(hipMemcpyAsync(th.data(), c->buf(s.thbuf).p, sizeof(double) * (size_t)(ntan * tht),
+                            hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(hipStreamSynchronize(c->stream));
+      for (int t = 0; t < ntan; t++) {
+        double dZ = 0.0;
+        for (int b = 0; b < s.nblocks(); b++) {
+          const int k = s.bsize(b), i0 = s.bbe[2 * b];
+          const double dg = 2.0 * s.qn[2 * b + 1] + 1.0;
+          for (int i = 0; i < k; i++)
+            dZ += dg * th[(size_t)t * tht + s.th_off[b] + (int64_t)i * k + i] * (-bz * std::exp(-bz * s.en[i0 + i]));
+        }
+        for (int b = 0; b < s.nblocks(); b++) {
+          const int k = s.bsize(b), i0 = s.bbe[2 * b];
+          for (int i = 0; i < k; i++)
+            for (int j = 0; j < k; j++) {
+              const double Ei = s.en[i0 + i], Ej = s.en[i0 + j];
+              double v = dd_exp(bb, Ei, Ej) * th[(size_t)t * tht + s.th_off[b] + (int64_t)i * k + j] / Z;
+              if (i == j) v -= std::exp(-bb * Ei) / Z * dZ / Z;
+              dr[(size_t)t * rl + D->rho_off[NM][b] + (int64_t)i * k + j] = v;
+            }
+        }
+      }
+      DBuf& drb = c->buf("dmdR" + std::to_string(NM));
+      if ((rc = drb.reserve(sizeof(double) * dr.size()))) { delete D; return rc; }
+      HIPCHK(hipMemcpyAsync(drb.p, dr.data(), sizeof(double) * dr.size(), hipMemcpyHostToDevice, c->stream));
+      HIPCHK(hipStreamSynchronize(c->stream));
+    }
   }
   // the sweep: TempBar = Temp/D_{N-1}, ZNdisc = Z_tot/4^{NM-N} (:361-366)
   for (int N = NM - 1; N >= 0; N--)
@@ -508,6 +836,22 @@ extern "C" int nrg_dm_create(nrg_ctx* c, const nrg_dmnrg_params* p, nrg_dm** out
   D->ms_sweep = std::chrono::duration<double, std::milli>(t2 - t1).count();
   *out = D;
   return NRG_OK;
+}
+
+extern "C" int nrg_dm_create(nrg_ctx* c, const nrg_dmnrg_params* p, nrg_dm** out) {
+  return dm_create(c, p, nullptr, nullptr, 0, nullptr, out);
+}
+
+extern "C" int nrg_dm_create_jvp(nrg_ctx* c, const nrg_dmnrg_params* p, const unsigned char* kept,
+                                 const int64_t* kept_off, int ntan, const double* dpar, nrg_dm** out) {
+  return dm_create(c, p, kept, kept_off, ntan, dpar, out);
+}
+
+extern "C" int nrg_dm_spec_jvp(nrg_dm* D, int n, const int* shells, const double* omegabar, double* out) {
+  if (!D || n < 0 || (n > 0 && (!shells || !omegabar || !out))) return fail(NRG_ERR_ARG, "nrg_dm_spec_jvp: bad arguments");
+  if (D->ntan <= 0) return fail(NRG_ERR_STATE, "nrg_dm_spec_jvp: the handle carries no tangents");
+  HIPCHK(hipSetDevice(D->c->device));
+  return spec_tasks_jvp(D, n, shells, omegabar, out);
 }
 
 extern "C" void nrg_dm_destroy(nrg_dm* D) { delete D; }

@@ -63,12 +63,43 @@ struct DmShell {
   std::vector<double> bas_qn;
   std::vector<int> op_map, op_mbe;     // c_d blocks over Acut_N
   std::string opbuf;
+  // derivative pass: per tangent, the kept-space Hamiltonian tangent (k x k per
+  // block at th_off), the kept rows' tangent (k x n at dz_off) and the c_d
+  // tangent (layout of opbuf); tangent t at t * (th_tot | dz_tot | op_cnt)
+  int ntan = 0;
+  std::vector<int64_t> th_off, dz_off;
+  int64_t th_tot = 0, dz_tot = 0, op_cnt = 0;
+  std::string thbuf, dzbuf, dopbuf;
   int nblocks() const { return (int)qn.size() / 2; }
   int bsize(int b) const { return bbe[2 * b + 1] - bbe[2 * b] + 1; }
   int bsize_bc(int b) const { return bbe_bc[2 * b + 1] - bbe_bc[2 * b] + 1; }
 };
 
+// derivative pass: one Acut_N block of the change to the perturbed eigenbasis
+struct DmFrame {
+  int64_t off;      // k x k block (th, M, Tc)
+  int64_t e_off;
+  int k, pad;
+};
+// one operator block pair of the derivative pole weights
+struct DmTanPair {
+  int64_t a_off;    // operator block (k1 x k2)
+  int64_t q_off;    // Q = rho1 A + A rho2 (k1 x k2)
+  int64_t pole_off;
+  int k1, k2;
+  double cg;
+};
+
 extern "C" {
+hipError_t dm_launch_frame(const DmFrame* blks, int nblk, int kkmax, int ntan, const double* th, int64_t th_stride,
+                           const double* en, double tol, double* M, double* Tc, int64_t stride, hipStream_t s);
+hipError_t dm_launch_tan_weights(const DmTanPair* pairs, int npair, int kkmax, int ntan, const double* ops,
+                                 const double* dAe, int64_t a_stride, const double* Q, const double* dQ,
+                                 const double* cl, int64_t q_stride, double* W1, double* W2, int64_t w_stride,
+                                 hipStream_t s);
+hipError_t dm_launch_broaden_jvp(const DmTask* tasks, int ntask, int maxchunk, int ntan, const double* W1,
+                                 const double* W2, int64_t w_stride, const double* dE, double bbroad, double btemp,
+                                 double* partial, int64_t p_stride, double* out, hipStream_t s);
 hipError_t dm_launch_gather(const DmGBlk* blks, int nblk, int kmax, const DmSeg* segs, const double* rot,
                             const double* en, double bb, double zdisc, double* out, hipStream_t s);
 hipError_t dm_launch_weights(const DmPair* pairs, int npair, int kkmax, const double* ops, const double* prod,

@@ -142,9 +142,133 @@ __global__ void reduce_kernel(const DmTask* __restrict__ tasks, int ntask, const
   out[t] = s;
 }
 
+// ---------------------------------------------------------------------------
+// derivative pass of the DM-NRG shell sums (no counterpart in the reference)
+// ---------------------------------------------------------------------------
+// change of frame of one block: M = th / (E_i - E_j) for non-degenerate pairs
+// (the rotation to the perturbed eigenbasis), Tc = th inside degenerate
+// clusters (|E_i - E_j| <= tol, the diagonal included)
+__global__ void frame_kernel(const DmFrame* __restrict__ blks, const double* __restrict__ th, int64_t th_stride,
+                             const double* __restrict__ en, double tol, double* __restrict__ M,
+                             double* __restrict__ Tc, int64_t stride) {
+  const DmFrame B = blks[blockIdx.y];
+  const int t = blockIdx.z;
+  const int64_t n = (int64_t)B.k * B.k;
+  for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < n; idx += (int64_t)gridDim.x * blockDim.x) {
+    const int i = (int)(idx / B.k), j = (int)(idx % B.k);
+    const double d = en[B.e_off + i] - en[B.e_off + j];
+    const double h = th[t * th_stride + B.off + idx];
+    const bool deg = fabs(d) <= tol;
+    M[t * stride + B.off + idx] = deg ? 0.0 : h / d;
+    Tc[t * stride + B.off + idx] = deg ? h : 0.0;
+  }
+}
+
+// W1 = cg (dA o Q + A o dQ), W2 = cg Q o (A Tc2 - Tc1 A): the weights of
+// delta(w, E_j - E_i) and of its derivative in the pole energy
+__global__ void tan_weights_kernel(const DmTanPair* __restrict__ pairs, const double* __restrict__ ops,
+                                   const double* __restrict__ dAe, int64_t a_stride, const double* __restrict__ Q,
+                                   const double* __restrict__ dQ, const double* __restrict__ cl, int64_t q_stride,
+                                   double* __restrict__ W1, double* __restrict__ W2, int64_t w_stride) {
+  const DmTanPair P = pairs[blockIdx.y];
+  const int t = blockIdx.z;
+  const int64_t n = (int64_t)P.k1 * P.k2;
+  for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < n; idx += (int64_t)gridDim.x * blockDim.x) {
+    const double a = ops[P.a_off + idx], q = Q[P.q_off + idx];
+    const double da = dAe[t * a_stride + P.a_off + idx];
+    const double dq = dQ[t * q_stride + P.q_off + idx], c = cl[t * q_stride + P.q_off + idx];
+    W1[t * w_stride + P.pole_off + idx] = P.cg * (da * q + a * dq);
+    W2[t * w_stride + P.pole_off + idx] = P.cg * q * c;
+  }
+}
+
+// BroadDelta / LorentzDelta of the DM-NRG method and their derivatives in the
+// pole energy x:  d/dx BroadDelta = BroadDelta (2 ln|w/x| / b^2 - 1) / x,
+// d/dx LorentzDelta = b (w - x) / (pi ((w - x)^2 + b^2)^2)
+__device__ __forceinline__ double bdelta_dx(double w, double x, double twin, double bbroad, double btemp,
+                                            double* gp) {
+  if (fabs(x) < twin) {
+    const double d = w - x, q = d * d + btemp * btemp;
+    *gp = (1.0 / (2.0 * M_PI)) * btemp * 2.0 * d / (q * q);
+    return (1.0 / (2.0 * M_PI)) * (btemp / q);
+  }
+  if (x == 0.0) x = 1.0e-100;
+  if (w * x < 0.0) { *gp = 0.0; return 0.0; }
+  const double a1 = log(fabs(w / x)) / bbroad;
+  const double g = exp(-bbroad * bbroad / 4.0) / (bbroad * fabs(x) * sqrt(M_PI)) * exp(-a1 * a1);
+  *gp = g * (2.0 * a1 / bbroad - 1.0) / x;
+  return g;
+}
+
+__global__ __launch_bounds__(BT) void broaden_jvp_kernel(const DmTask* __restrict__ tasks,
+                                                         const double* __restrict__ W1,
+                                                         const double* __restrict__ W2, int64_t w_stride,
+                                                         const double* __restrict__ dE, double bbroad, double btemp,
+                                                         double* __restrict__ partial, int64_t p_stride) {
+  __shared__ double red[BT / 64];
+  const DmTask T = tasks[blockIdx.y];
+  const int c = blockIdx.x, t = blockIdx.z;
+  if (c >= T.nchunk) return;
+  const int64_t b0 = T.pole_beg + (int64_t)c * BCH;
+  const int64_t b1 = min(b0 + (int64_t)BCH, T.pole_end);
+  const double* __restrict__ w1 = W1 + t * w_stride;
+  const double* __restrict__ w2 = W2 + t * w_stride;
+  double s = 0.0;
+  for (int64_t p = b0 + threadIdx.x; p < b1; p += BT) {
+    double gp;
+    const double g = bdelta_dx(T.w, dE[p], T.twin, bbroad, btemp, &gp);
+    s += w1[p] * g + w2[p] * gp;
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) partial[t * p_stride + T.chunk_beg + c] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
 }  // namespace dmk
 
 extern "C" {
+
+hipError_t dm_launch_frame(const DmFrame* blks, int nblk, int kkmax, int ntan, const double* th, int64_t th_stride,
+                           const double* en, double tol, double* M, double* Tc, int64_t stride, hipStream_t s) {
+  if (nblk <= 0 || ntan <= 0) return hipSuccess;
+  int gx = (kkmax + 255) / 256; if (gx > 1024) gx = 1024; if (gx < 1) gx = 1;
+  hipLaunchKernelGGL(dmk::frame_kernel, dim3(gx, nblk, ntan), dim3(256), 0, s, blks, th, th_stride, en, tol, M, Tc,
+                     stride);
+  return hipGetLastError();
+}
+
+hipError_t dm_launch_tan_weights(const DmTanPair* pairs, int npair, int kkmax, int ntan, const double* ops,
+                                 const double* dAe, int64_t a_stride, const double* Q, const double* dQ,
+                                 const double* cl, int64_t q_stride, double* W1, double* W2, int64_t w_stride,
+                                 hipStream_t s) {
+  if (npair <= 0 || ntan <= 0) return hipSuccess;
+  int gx = (kkmax + 255) / 256; if (gx > 1024) gx = 1024; if (gx < 1) gx = 1;
+  hipLaunchKernelGGL(dmk::tan_weights_kernel, dim3(gx, npair, ntan), dim3(256), 0, s, pairs, ops, dAe, a_stride, Q,
+                     dQ, cl, q_stride, W1, W2, w_stride);
+  return hipGetLastError();
+}
+
+hipError_t dm_launch_broaden_jvp(const DmTask* tasks, int ntask, int maxchunk, int ntan, const double* W1,
+                                 const double* W2, int64_t w_stride, const double* dE, double bbroad, double btemp,
+                                 double* partial, int64_t p_stride, double* out, hipStream_t s) {
+  if (ntask <= 0 || ntan <= 0) return hipSuccess;
+  if (maxchunk > 0) {
+    hipLaunchKernelGGL(dmk::broaden_jvp_kernel, dim3(maxchunk, ntask, ntan), dim3(BT), 0, s, tasks, W1, W2, w_stride,
+                       dE, bbroad, btemp, partial, p_stride);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  for (int t = 0; t < ntan; t++) {
+    hipLaunchKernelGGL(dmk::reduce_kernel, dim3((ntask + 255) / 256), dim3(256), 0, s, tasks, ntask,
+                       partial + t * p_stride, out + (int64_t)t * ntask);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
 
 hipError_t dm_launch_gather(const DmGBlk* blks, int nblk, int kmax, const DmSeg* segs, const double* rot,
                             const double* en, double bb, double zdisc, double* out, hipStream_t s) {

@@ -1434,6 +1434,31 @@ struct Tangent {
     return 0;
   }
 
+  // What the DM-NRG derivative needs of shell N, next to dm_capture's data:
+  // th (k x k per kept block), dZ_k (k x n per kept block) and the c_d tangent
+  int capture(DmShell& s, int opidx) {
+    s.ntan = T;
+    s.th_off = th_off; s.th_tot = th_tot;
+    s.dz_off = dzoff; s.dz_tot = dztot;
+    s.op_cnt = op_cnt[opidx];
+    const std::string n = std::to_string(s.N);
+    s.thbuf = "dmTH" + n; s.dzbuf = "dmdZ" + n; s.dopbuf = "dmdO" + n;
+    const std::pair<std::string, std::pair<const double*, int64_t>> cp[3] = {
+        {s.thbuf, {buf(thb(cur)), T * th_tot}},
+        {s.dzbuf, {buf("tg_dZ"), T * dztot}},
+        {s.dopbuf, {buf(opb(cur)) + op_base[opidx], T * op_cnt[opidx]}}};
+    for (const auto& x : cp) {
+      DBuf& b = c->buf(x.first);
+      int rc = b.reserve(sizeof(double) * (size_t)(x.second.second + 1));
+      if (rc) return rc;
+      if (x.second.second > 0)
+        HIPCHK(hipMemcpyAsync(b.p, x.second.first, sizeof(double) * (size_t)x.second.second, hipMemcpyDeviceToDevice,
+                              c->stream));
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+  }
+
   void fill(nrg_tangent_step& dt, int N, int ns, int nops) const {
     std::memset(&dt, 0, sizeof(dt));
     dt.nshell = N; dt.ntan = T; dt.nstates = ns; dt.nops_avg = nops;
@@ -1592,6 +1617,7 @@ struct OneChRun {
     if (dm && p.calcdens == 1) {
       dm->emplace_back();
       if ((rc = dm_capture(ch, N, 1, dm->back()))) return rc;
+      if (tg && (rc = tg->capture(dm->back(), 1))) return rc;
     }
     if ((rc = ch.op_avgs(bbar, ops))) return rc;
     if (tg && (rc = tg->op_avgs(ch, bbar))) return rc;
@@ -1849,7 +1875,8 @@ static void jvp_tramp(void* u, const nrg_step* st) {
   if (j->cb) j->cb(j->user, st, &dt);
 }
 static int run_onechqs_jvp(nrg_ctx* c, const nrg_onechqs_params& p, const unsigned char* kept, const int64_t* kept_off,
-                           int ntan, const double* dpar, nrg_tangent_cb cb, void* user) {
+                           int ntan, const double* dpar, nrg_tangent_cb cb, void* user,
+                           std::vector<DmShell>* dm = nullptr) {
   int rc;
   OneChRun R;
   Tangent tg;
@@ -1883,10 +1910,16 @@ static int run_onechqs_jvp(nrg_ctx* c, const nrg_onechqs_params& p, const unsign
                                      std::to_string(R.ch.Aeig.Nstates()) + " states");
       fixed = kept + kept_off[N];
     }
-    if ((rc = R.post(N, jvp_tramp, &j, nullptr, ntan > 0 ? &tg : nullptr, fixed))) return rc;
+    if ((rc = R.post(N, jvp_tramp, &j, dm, ntan > 0 ? &tg : nullptr, fixed))) return rc;
   }
   HIPCHK(hipStreamSynchronize(c->stream));
   return 0;
+}
+
+// the DM-NRG chain with frozen Kept sets and tangents captured per shell
+int run_onechqs_dm_jvp(nrg_ctx* c, const nrg_onechqs_params& p, std::vector<DmShell>& dm, const unsigned char* kept,
+                       const int64_t* kept_off, int ntan, const double* dpar) {
+  return run_onechqs_jvp(c, p, kept, kept_off, ntan, dpar, nullptr, nullptr, &dm);
 }
 
 int run_onechqs_dm(nrg_ctx* c, const nrg_onechqs_params& p, std::vector<DmShell>& dm, int befcut) {
