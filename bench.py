@@ -201,6 +201,88 @@ def c2_run(steps=4, ramp=6, plain_steps=1):
     return out
 
 
+def adjoint_run(cpu=True):
+    """Derivative pass (forward mode, 3 tangents U/Gamma/ed) of the single-channel
+    Anderson chain, Lambda 2.5, Ncut 1000 with the Kept sets frozen, to N = 31
+    (T = 1e-6 with betabar = D_31/T): d<n_d>, d(T chi) per step; then the DM-NRG
+    A(omega) and its Jacobian on omega = +-logspace(-8, 0, 100).  The CPU leg is
+    the oracle's derivative pass on the same chain to N = 6, timed per
+    full-size step."""
+    import numpy as np
+    import twochnrg_amd as t
+    from twochnrg_amd import autograd as ag
+    L, Nout = 2.5, 31
+    bb = ag.calc_dn(L, Nout) / 1e-6
+    kw = dict(U=0.5, Gamma=0.02, ed=-0.2, Lambda=L, nsitesmax=Nout + 1, ncut=1000, calcdens=2, betabar=bb)
+    out = {"workload": "onechqs_adjoint_c4", "params": kw, "T": 1e-6, "ntan": 3}
+    with t.Context(0) as ctx:
+        ctx.run_onechqs(**dict(kw, nsitesmax=4))   # first-touch of the buffers
+        t0 = time.perf_counter()
+        steps, _ = ctx.run_onechqs(**kw)
+        t1 = time.perf_counter()
+        kept = t.kept_arrays(steps)
+        st, tn = ctx.onechqs_jvp(kept=kept, dpar=np.eye(3), **kw)
+        t2 = time.perf_counter()
+        nst = len(steps) - 1
+        out["forward_ms"] = 1e3 * (t1 - t0)
+        out["jvp_ms"] = 1e3 * (t2 - t1)
+        out["forward_steps_per_s"] = nst / (t1 - t0)
+        out["jvp_steps_per_s"] = nst / (t2 - t1)
+        out["jvp_over_forward"] = (t2 - t1) / (t1 - t0)
+        out["jacobian_nd_Tchi"] = [tn[-1]["dopavg"][:, 0].tolist(), tn[-1]["dsuscep"][:, 3].tolist()]
+        print("bench adjoint: chain forward %.0f ms, derivative pass %.0f ms" % (out["forward_ms"], out["jvp_ms"]),
+              file=sys.stderr, flush=True)
+        om = np.concatenate([-np.logspace(-8, 0, 100)[::-1], np.logspace(-8, 0, 100)])
+        dkw = dict(Lambda=L, nsitesmax=40, ncut=1000, Mtemp=Nout, betabar=bb)
+        run = ag.make_spectral_run(ctx, (0.5, 0.02, -0.2), om, **dkw)
+        grid = run["grid"]
+        sh, wb = [g[0] for g in grid], [g[1] for g in grid]
+        t3 = time.perf_counter()
+        with ctx.dm(U=0.5, Gamma=0.02, ed=-0.2, kept=run["kept"], **dkw) as d:
+            d.spec(sh, wb)
+        t4 = time.perf_counter()
+        with ctx.dm(U=0.5, Gamma=0.02, ed=-0.2, kept=run["kept"], dpar=np.eye(3), **dkw) as d:
+            d.spec(sh, wb)
+            d.spec_jvp(sh, wb)
+            tm = d.timing()
+        t5 = time.perf_counter()
+        out["dm_A_ms"] = 1e3 * (t4 - t3)
+        out["dm_A_and_jacobian_ms"] = 1e3 * (t5 - t4)
+        out["dm_breakdown_ms"] = tm
+        out["nomega"] = len(om)
+        print("bench adjoint: A(omega) %.0f ms, with its Jacobian %.0f ms" % (out["dm_A_ms"], out["dm_A_and_jacobian_ms"]),
+              file=sys.stderr, flush=True)
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import ctypes
+        import pyoracle as po
+        okw = dict(kw, nsitesmax=6)
+        base = po.run_onechqs(**{k: v for k, v in okw.items()})
+        okept = po.kept_arrays(base)
+        lib = po.load()
+        stamps = []
+
+        def cb(_u, st_, dt_):
+            stamps.append((st_.contents.nshell, time.perf_counter(), st_.contents.nstates))
+        p = po.OneChQSParams(okw["U"], okw["Gamma"], okw["ed"], L, okw["nsitesmax"], okw["ncut"], 2, 0, 1e-18, bb, 0)
+        dp = np.ascontiguousarray(np.eye(3))
+        kf, ko = np.ascontiguousarray(okept[0]), np.ascontiguousarray(okept[1])
+        fn = po.TAN_CB(cb)
+        t0 = time.perf_counter()
+        lib.orc_run_onechqs_jvp(ctypes.byref(p), kf.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte)),
+                                ko.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)), 3,
+                                dp.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), fn, None)
+        dt = stamps[-1][1] - stamps[-2][1]
+        out["cpu_oracle"] = {"value": 1.0 / dt, "unit": "derivative-pass steps/s", "cores": 1, "kind": "port",
+                             "sample": "oracle/nrg_oracle.c derivative pass (3 tangents) on the same chain to N = %d "
+                                       "(%d states at the last step), last step timed: %.2f s, run %.1f s" % (
+                                           stamps[-1][0], stamps[-1][2], dt, time.perf_counter() - t0)}
+        # the device's rate over its full-size steps, for the same unit
+        out["device_vs_cpu_oracle"] = out["jvp_steps_per_s"] * dt
+        print("bench adjoint: oracle step %.2f s" % dt, file=sys.stderr, flush=True)
+    return out
+
+
 def cpu_baseline(name, timed_sizes, budget_s=20.0):
     """The CPU oracle (gcc -O2 -funroll-loops, one thread, FP64) on a bounded
     sample of the same workload: the oracle's dspev-equivalent eigensolve on
@@ -256,6 +338,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-dm", action="store_true", help="skip the DM-NRG A(omega) leg")
     ap.add_argument("--no-c2", action="store_true", help="skip the 64-instance batched leg")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the derivative-pass leg")
     ap.add_argument("--stats-all", action="store_true",
                     help="kernel statistics over the whole run (ramp-up included), for counter runs")
     ap.add_argument("--plain-steps", type=int, default=2,
@@ -314,6 +397,8 @@ def main():
         out["fdm"] = fdm_run(cpu=not a.no_cpu)
     if not a.no_c2:
         out["batched_instances"] = c2_run()
+    if not a.no_adjoint:
+        out["adjoint"] = adjoint_run(cpu=not a.no_cpu)
     print(json.dumps(out))
 
 
