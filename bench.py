@@ -283,6 +283,41 @@ def adjoint_run(cpu=True):
     return out
 
 
+def hbm_probe_child():
+    """hbm_probe in a child process: torch's HIP runtime cannot open the device
+    in a process where the library's runtime already holds it."""
+    import subprocess
+    r = subprocess.run([sys.executable, os.path.abspath(__file__), "--hbm-probe"], stdout=subprocess.PIPE,
+                       stderr=subprocess.PIPE, text=True, timeout=300)
+    if r.returncode != 0:
+        return {"error": r.stderr.strip().splitlines()[-1] if r.stderr.strip() else "rc %d" % r.returncode}
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
+def hbm_probe(nbytes=2 << 30, reps=10):
+    """STREAM-like calibration of the nominal HBM peak on this device: a 2 GiB
+    FP64 copy (torch's copy kernel, read + write) and a 2 GiB read (torch.sum),
+    timed with events on torch's stream."""
+    import torch
+    n = nbytes // 8
+    x = torch.ones(n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    out = {}
+    for name, fn, moved in (("copy", lambda: y.copy_(x), 2 * nbytes), ("read", lambda: x.sum(), nbytes)):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        out[name + "_GBs"] = moved * reps / (e0.elapsed_time(e1) * 1e-3) / 1e9
+    del x, y
+    torch.cuda.empty_cache()
+    return out
+
+
 def cpu_baseline(name, timed_sizes, budget_s=20.0):
     """The CPU oracle (gcc -O2 -funroll-loops, one thread, FP64) on a bounded
     sample of the same workload: the oracle's dspev-equivalent eigensolve on
@@ -339,15 +374,24 @@ def main():
     ap.add_argument("--no-dm", action="store_true", help="skip the DM-NRG A(omega) leg")
     ap.add_argument("--no-c2", action="store_true", help="skip the 64-instance batched leg")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the derivative-pass leg")
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="run only the derivative-pass leg (device part) and print its JSON object")
     ap.add_argument("--stats-all", action="store_true",
                     help="kernel statistics over the whole run (ramp-up included), for counter runs")
     ap.add_argument("--plain-steps", type=int, default=2,
                     help="steps of the plain one-block-per-launch comparison run (0 = skip)")
+    ap.add_argument("--hbm-probe", action="store_true", help=argparse.SUPPRESS)
     a = ap.parse_args()
+    if a.hbm_probe:
+        print(json.dumps(hbm_probe()))
+        return
     if a.gpus != 1:
         print("bench.py: this build measures one GPU only (the NRG step and its batched blocks run on one "
               "device); --gpus %d is not supported" % a.gpus, file=sys.stderr)
         sys.exit(2)
+    if a.adjoint_only:
+        print(json.dumps(adjoint_run(cpu=False)))
+        return
     dt, st, timed, model, w, nsteps = gpu_run(a.workload, a.steps, a.warmup, a.eig_impl, stats_all=a.stats_all)
     value = nsteps / dt
     eig = st["eig"]
@@ -382,6 +426,10 @@ def main():
                        "flops_per_step": eig["flops"] / nsteps, "ms_per_step": eig["ms"] / nsteps},
         "kernel_ms_per_step": {k: v["ms"] / nsteps for k, v in st.items()},
     }
+    cal = hbm_probe_child()
+    roof["measured_hbm"] = cal
+    if roof["unit"] == "GB/s" and "read_GBs" in cal:
+        roof["frac_of_measured_read"] = roof["achieved"] / cal["read_GBs"]
     if a.plain_steps > 0:
         # the same kernels, one symmetry block (eigensolve) or one block pair
         # (rotation) per launch, host-synchronised, no batching

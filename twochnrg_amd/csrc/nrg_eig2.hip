@@ -185,7 +185,12 @@ __global__ __launch_bounds__(E2_T) void tridiag_small_kernel(const E2Block* __re
 // Panel buffer X (3 PANEL rows of n): rows [0,P) V^T, [P,2P) W^T, [2P,3P) V^T,
 // so that rows [0,2P) and [P,3P) are the two operands of the panel GEMM.
 // ---------------------------------------------------------------------------
-#define SYMV_R 128   // rows per symv tile
+#ifndef SYMV_R
+#define SYMV_R 128   // rows per symv tile (at most CHUNK)
+#endif
+#ifndef SYMV_B
+#define SYMV_B 16    // rows whose loads a symv thread issues before their products
+#endif
 #define CHUNK 256    // columns per symv tile / elements per chunk
 
 struct TdbPtr {
@@ -281,14 +286,14 @@ __global__ __launch_bounds__(CHUNK) void tdb_symv_kernel(const E2Block* __restri
       const double* col = P.A + (size_t)r0 * n + i;
       double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
       int l = 0;
-      // 16 rows' loads issued before their products: enough reads in flight
+      // SYMV_B rows' loads issued before their products: enough reads in flight
       // per wavefront to cover the HBM latency at this occupancy
-      for (; l + 16 <= nr; l += 16) {
-        double a[16];
+      for (; l + SYMV_B <= nr; l += SYMV_B) {
+        double a[SYMV_B];
 #pragma unroll
-        for (int u = 0; u < 16; u++) a[u] = col[(size_t)(l + u) * n];
+        for (int u = 0; u < SYMV_B; u++) a[u] = col[(size_t)(l + u) * n];
 #pragma unroll
-        for (int u = 0; u < 16; u += 4) {
+        for (int u = 0; u < SYMV_B; u += 4) {
           s0 += a[u] * vs[l + u];
           s1 += a[u + 1] * vs[l + u + 1];
           s2 += a[u + 2] * vs[l + u + 2];
