@@ -52,41 +52,45 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
 // ---------------------------------------------------------------------------
 // stage 1a: whole reduction inside one workgroup (n <= 256)
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(E2_T) void tridiag_small_kernel(const E2Block* __restrict__ blks,
+#define SM_G 4              // row groups of the small-block reduction
+#define SM_T (256 * SM_G)    // its workgroup: column c = tid % 256, row group g = tid / 256
+__global__ __launch_bounds__(SM_T) void tridiag_small_kernel(const E2Block* __restrict__ blks,
                                                               const int* __restrict__ list, double* __restrict__ A0,
                                                               double* __restrict__ WS) {
-  // Householder reduction, one workgroup per block, thread i owning column i.
-  // The rank-2 update of column k-1 (A -= v w^T + w v^T) is applied inside the
-  // pass of column k that also forms A v: every trailing element is read once,
-  // updated, written and used in the product, instead of the separate product
-  // and update passes.  The arithmetic is that of the two-pass form: row k
-  // gets the pending update before its reflector is formed, and the update of
-  // element (j, i) uses the same operations as that of (i, j).
-  __shared__ double red[E2_T / WAVE];
+  // Householder reduction, one workgroup per block.  Column c is owned by the
+  // SM_G threads (c, g); for each column k the rows below k are split into SM_G
+  // contiguous ranges, one per row group, so that SM_G times more loads are in
+  // flight per block, and the row groups' partial sums of (A v)_c are added in
+  // group order.  The rank-2 update of column k-1 (A -= v w^T + w v^T) is
+  // applied inside the pass of column k that also forms A v: every trailing
+  // element is read once, updated, written and used in the product.  The
+  // update of element (j, i) uses the same operations as that of (i, j).
+  __shared__ double red[SM_T / WAVE];
   __shared__ double vsh[256], wsh[256], vnew[256], ash[256];
+  __shared__ double qpart[SM_G][256];
   const E2Block B = blks[list[blockIdx.x]];
   const int n = B.n, tid = threadIdx.x;
+  const int c = tid & 255, g = tid >> 8;
   double* A = A0 + B.a_off;
   double* d = WS + B.v_off;
   double* e = d + n;
   double* tau = e + n;
   bool pending = false;   // vsh/wsh hold the update of the previous column
   double vi_old = 0.0, wi_old = 0.0;
-  vsh[tid] = 0.0;
-  wsh[tid] = 0.0;
+  if (g == 0) { vsh[c] = 0.0; wsh[c] = 0.0; }
   __syncthreads();
   for (int k = 0; k + 2 < n; k++) {
-    // row k of the current matrix
+    // row k of the current matrix (row group 0)
     double ak = 0.0;
-    if (tid >= k && tid < n) {
-      ak = A[(size_t)k * n + tid];
+    if (g == 0 && c >= k && c < n) {
+      ak = A[(size_t)k * n + c];
       if (pending) {
         const double vj = vsh[k], qj = wsh[k];
-        ak -= (k <= tid) ? __fma_rn(vj, wi_old, vi_old * qj) : __fma_rn(vi_old, qj, vj * wi_old);
+        ak -= (k <= c) ? __fma_rn(vj, wi_old, vi_old * qj) : __fma_rn(vi_old, qj, vj * wi_old);
       }
     }
-    ash[tid] = ak;
-    const double x = (tid > k && tid < n) ? ak : 0.0;
+    if (g == 0) ash[c] = ak;
+    const double x = (g == 0 && c > k && c < n) ? ak : 0.0;
     const double nrm2 = block_sum(x * x, red);   // its barriers order the ash stores
     const double x0 = ash[k + 1];
     double t = 0.0, v0 = x0, alpha = 0.0;
@@ -96,64 +100,71 @@ __global__ __launch_bounds__(E2_T) void tridiag_small_kernel(const E2Block* __re
       v0 = x0 - alpha;
       t = 2.0 / (nrm2 - x0 * x0 + v0 * v0);
     }
-    const double vi = (tid > k + 1 && tid < n) ? ak : (tid == k + 1 ? v0 : 0.0);
-    vnew[tid] = vi;
+    const double ac = ash[c];
+    const double vi = (c > k + 1 && c < n) ? ac : (c == k + 1 ? v0 : 0.0);
+    if (g == 0) vnew[c] = vi;
     __syncthreads();
-    // rows > k: apply the pending update, write back, accumulate q_i = t sum_j A_ji v_j
+    // this group's rows > k: apply the pending update, write back, accumulate A v
     double qi = 0.0;
-    if (tid > k && tid < n) {
-      int j = k + 1;
+    if (c > k && c < n) {
+      const int L = (n - k - 1 + SM_G - 1) / SM_G;
+      int j = k + 1 + g * L;
+      const int j1 = min(n, j + L);
       if (pending) {
-        for (; j + 8 <= n; j += 8) {
+        for (; j + 8 <= j1; j += 8) {
           double a[8];
 #pragma unroll
-          for (int u = 0; u < 8; u++) a[u] = A[(size_t)(j + u) * n + tid];
+          for (int u = 0; u < 8; u++) a[u] = A[(size_t)(j + u) * n + c];
 #pragma unroll
           for (int u = 0; u < 8; u++) {
             const int jj = j + u;
             const double vj = vsh[jj], qj = wsh[jj];
-            a[u] -= (jj <= tid) ? __fma_rn(vj, wi_old, vi_old * qj) : __fma_rn(vi_old, qj, vj * wi_old);
+            a[u] -= (jj <= c) ? __fma_rn(vj, wi_old, vi_old * qj) : __fma_rn(vi_old, qj, vj * wi_old);
             qi += a[u] * vnew[jj];
           }
 #pragma unroll
-          for (int u = 0; u < 8; u++) A[(size_t)(j + u) * n + tid] = a[u];
+          for (int u = 0; u < 8; u++) A[(size_t)(j + u) * n + c] = a[u];
         }
-        for (; j < n; j++) {
+        for (; j < j1; j++) {
           const double vj = vsh[j], qj = wsh[j];
-          double a = A[(size_t)j * n + tid];
-          a -= (j <= tid) ? __fma_rn(vj, wi_old, vi_old * qj) : __fma_rn(vi_old, qj, vj * wi_old);
-          A[(size_t)j * n + tid] = a;
+          double a = A[(size_t)j * n + c];
+          a -= (j <= c) ? __fma_rn(vj, wi_old, vi_old * qj) : __fma_rn(vi_old, qj, vj * wi_old);
+          A[(size_t)j * n + c] = a;
           qi += a * vnew[j];
         }
       } else {
-        for (; j + 16 <= n; j += 16) {
-          double a[16];
+        for (; j + 8 <= j1; j += 8) {
+          double a[8];
 #pragma unroll
-          for (int u = 0; u < 16; u++) a[u] = A[(size_t)(j + u) * n + tid];
+          for (int u = 0; u < 8; u++) a[u] = A[(size_t)(j + u) * n + c];
 #pragma unroll
-          for (int u = 0; u < 16; u++) qi += a[u] * vnew[j + u];
+          for (int u = 0; u < 8; u++) qi += a[u] * vnew[j + u];
         }
-        for (; j < n; j++) qi += A[(size_t)j * n + tid] * vnew[j];
+        for (; j < j1; j++) qi += A[(size_t)j * n + c] * vnew[j];
       }
-      qi *= t;
     }
-    const double vp = block_sum((tid > k && tid < n) ? qi * vi : 0.0, red);   // barriers: pass done
+    qpart[g][c] = qi;
+    __syncthreads();
+    qi = 0.0;
+#pragma unroll
+    for (int h = 0; h < SM_G; h++) qi += qpart[h][c];
+    qi *= t;
+    const double vp = block_sum((g == 0 && c > k && c < n) ? qi * vi : 0.0, red);   // barriers: qpart read
     const double K = 0.5 * t * vp;
-    const double wi = (tid > k && tid < n) ? qi - K * vi : 0.0;
-    vsh[tid] = vi;
-    wsh[tid] = wi;
+    const double wi = (c > k && c < n) ? qi - K * vi : 0.0;
+    if (g == 0) { vsh[c] = vi; wsh[c] = wi; }
     vi_old = vi;
     wi_old = wi;
     pending = true;
-    if (tid == k) { d[k] = ak; e[k] = alpha; tau[k] = t; }
-    if (tid > k && tid < n) A[(size_t)k * n + tid] = vi;   // reflector k in row k
+    if (g == 0 && c == k) { d[k] = ak; e[k] = alpha; tau[k] = t; }
+    if (g == 0 && c > k && c < n) A[(size_t)k * n + c] = vi;   // reflector k in row k
     __syncthreads();
   }
   // the last update reaches the trailing 2 x 2 block
-  if (n >= 2 && pending && tid >= n - 2 && tid < n) {
+  if (g == 0 && n >= 2 && pending && c >= n - 2 && c < n) {
     for (int j = n - 2; j < n; j++) {
       const double vj = vsh[j], qj = wsh[j];
-      A[(size_t)j * n + tid] -= (j <= tid) ? __fma_rn(vj, wi_old, vi_old * qj) : __fma_rn(vi_old, qj, vj * wi_old);
+      A[(size_t)j * n + c] -= (j <= c) ? __fma_rn(vj, wi_old, vi_old * qj) : __fma_rn(vi_old, qj, vj * wi_old);
     }
   }
   __syncthreads();
@@ -1116,7 +1127,7 @@ extern "C" {
 
 hipError_t e2_launch_tridiag_small(const E2Block* b, const int* list, int nlist, double* A, double* WS, hipStream_t s) {
   if (nlist <= 0) return hipSuccess;
-  hipLaunchKernelGGL(e2::tridiag_small_kernel, dim3(nlist), dim3(E2_T), 0, s, b, list, A, WS);
+  hipLaunchKernelGGL(e2::tridiag_small_kernel, dim3(nlist), dim3(SM_T), 0, s, b, list, A, WS);
   return hipGetLastError();
 }
 
