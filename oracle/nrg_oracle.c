@@ -79,6 +79,7 @@ typedef struct {
   ivec stcf;    /* CNRGbasisarray::StCameFrom */
   ivec bbe_bc;  /* BlockBegEndBC */
   ivec bbe_ev;  /* BlockBegEndEigVec */
+  int sz;       /* (Q, Sz) blocks without SU(2): degeneracy 1, no Clebsch-Gordan factors */
 } arr_t;
 
 typedef struct {
@@ -1777,7 +1778,67 @@ static double calc_entropy_sz(double bb, const arr_t *A) {
   return bb * (s / Z) + log(Z);
 }
 
-int orc_run_twochqsz(const orc_twochqs_params *p, orc_step_cb cb, void *user) {
+/* d_up on every state of a shell (for the DM-NRG post-processing of TwoChQSz,
+ * which the reference does not have): for each block pair (i, j) of Aeig with
+ * Q_j = Q_i + 1 and Sz_j = Sz_i + 1/2, Z_i F Z_j^T over all eigenvector rows,
+ * F(ist, jst) = delta(type_ist, type_jst) (-1)^{n(type)} d_old(stcf_ist, stcf_jst):
+ * d acts on the old part, past the new site's n(type) fermions (the ordering of
+ * the coupling signs of qsz_hn_coef). */
+static void qsz_dup_all(const arr_t *Ae, const arr_t *Ab, const mat_t *dold, mat_t *D) {
+  mat_clear(D);
+  mat_sync(D, Ae);
+  int icount = 0;
+  for (int ib = 0; ib < nblocks(Ae); ib++)
+    for (int jb = 0; jb < nblocks(Ae); jb++) {
+      if (!(dEqual(getqn(Ae, jb, 0), getqn(Ae, ib, 0) + 1.0) && dEqual(getqn(Ae, jb, 1), getqn(Ae, ib, 1) + 0.5)))
+        continue;
+      int ni = bsize(Ae, ib), nj = bsize(Ae, jb);
+      ipush(&D->map, ib); ipush(&D->map, jb);
+      ipush(&D->mbe, icount);
+      icount += ni * nj;
+      ipush(&D->mbe, icount - 1);
+      double *F = (double *)calloc((size_t)ni * nj + 1, sizeof(double));
+      for (int a = 0; a < ni; a++) {
+        int ist = blim(Ab, ib, 0) + a, t = Ab->itype.a[ist];
+        for (int b = 0; b < nj; b++) {
+          int jst = blim(Ab, jb, 0) + b;
+          if (Ab->itype.a[jst] != t) continue;
+          double sg = (__builtin_popcount((unsigned)t) & 1) ? -1.0 : 1.0;
+          F[(size_t)a * nj + b] = sg * get_matel(dold, Ab->stcf.a[ist], Ab->stcf.a[jst]);
+        }
+      }
+      dres(&D->mat, D->mat.n + ni * nj);
+      orc_rotate(ni, nj, ni, nj, Ae->ev.a + blim_ev(Ae, ib), F, Ae->ev.a + blim_ev(Ae, jb), D->mat.a + D->mat.n);
+      D->mat.n += ni * nj;
+      free(F);
+    }
+}
+/* the kept part of an all-states operator: the kept prefixes of the surviving blocks */
+static void op_kept_part(const mat_t *Dall, const arr_t *Ae, const arr_t *Ac, mat_t *Dk) {
+  mat_clear(Dk);
+  mat_sync(Dk, Ac);
+  int icount = 0;
+  for (int ib = 0; ib < nblocks(Ac); ib++)
+    for (int jb = 0; jb < nblocks(Ac); jb++) {
+      int bi = blk_from_qn(Ae, Ac->qn.a + ib * Ac->nq), bj = blk_from_qn(Ae, Ac->qn.a + jb * Ac->nq);
+      int im = find_mat_block(Dall, bi, bj);
+      if (im < 0) continue;
+      int ki = bsize(Ac, ib), kj = bsize(Ac, jb), nj = bsize(Ae, bj);
+      ipush(&Dk->map, ib); ipush(&Dk->map, jb);
+      ipush(&Dk->mbe, icount);
+      icount += ki * kj;
+      ipush(&Dk->mbe, icount - 1);
+      for (int a = 0; a < ki; a++)
+        for (int b = 0; b < kj; b++) dpush(&Dk->mat, Dall->mat.a[Dall->mbe.a[2 * im] + a * nj + b]);
+    }
+}
+static void dm_store_qsz(struct orc_dm *D, const arr_t *Ae, const arr_t *Ab, const arr_t *Acprev, const mat_t *dall);
+static int run_twochqsz(const orc_twochqs_params *p, orc_step_cb cb, void *user, struct orc_dm *dm);
+int orc_run_twochqsz(const orc_twochqs_params *p, orc_step_cb cb, void *user) { return run_twochqsz(p, cb, user, NULL); }
+/* dm != NULL: every shell (N = 0 .. nsitesmax) is stored with all its states,
+ * their Kept flags and d_up on them, Abasis with StCameFrom as indices of the
+ * previous shell's states before the cut */
+static int run_twochqsz(const orc_twochqs_params *p, orc_step_cb cb, void *user, struct orc_dm *dm) {
   const double L = p->Lambda, bbar = p->betabar > 0 ? p->betabar : 0.727;
   const double HLF = 0.5 * (1.0 + 1.0 / L);
   arr_t S = {0}, Ae = {0}, Ac = {0}, Ab = {0}, Ah = {0}, Ahc = {0};
@@ -1828,6 +1889,24 @@ int orc_run_twochqsz(const orc_twochqs_params *p, orc_step_cb cb, void *user) {
   }
   double emin = set_e0zero(&Ae);
   twoch_update_f(&S, &Ae, &Ab, f, 1);   /* TwoChQSz_UpdateQm1fQ on the uncut N=0 basis */
+  mat_t dup = {0}, dall = {0};          /* d_up: impurity table (up: (0,1) and (2,3)), then per shell */
+  if (dm) {
+    mat_sync(&dup, &Ah);
+    const int um[4] = {0, 1, 2, 3};
+    for (int k = 0; k < 2; k++) {
+      dpush(&dup.mat, 1.0); ipush(&dup.map, um[2 * k]); ipush(&dup.map, um[2 * k + 1]);
+      ipush(&dup.mbe, k); ipush(&dup.mbe, k);
+    }
+    qsz_dup_all(&Ae, &Ab, &dup, &dall);
+    arr_t kc = {0};
+    cut_states_legacy(&Ae, &kc, p->ncut);
+    Ae.kept.n = 0;
+    double ec0 = (p->ncut < nstates(&Ae)) ? orc_ecut(Ae.en.n, Ae.en.a, p->ncut) : HUGE_VAL;
+    for (int i = 0; i < Ae.en.n; i++) ipush(&Ae.kept, Ae.en.a[i] <= ec0);
+    dm_store_qsz(dm, &Ae, &Ab, &Ahc, &dall);
+    op_kept_part(&dall, &Ae, &kc, &dup);
+    arr_free(&kc);
+  }
   double DN = HLF * pow(L, -(0 - 1) / 2.0);
   {
     double thT[4] = {DN / bbar, 0, 0, 0}, sus[4] = {calc_suscep_sz(bbar, &Ae), 0, 0, 0};
@@ -1844,6 +1923,7 @@ int orc_run_twochqsz(const orc_twochqs_params *p, orc_step_cb cb, void *user) {
     double chiN = d0 / (d1 * d2);
     cut_states_legacy(&Ae, &Ac, p->ncut);
     if (N > 1) twoch_update_f(&S, &Ac, &Ab, f, 1);
+    if (dm && N > 1) op_kept_part(&dall, &Ae, &Ac, &dup);
     qsz_build_basis(&Ac, &Ab, &S, 0);
     /* TwoChQSz_DiagHN: the rules give a symmetric H; the upper triangle is used */
     Ae.nshell = Ab.nshell; Ae.nq = 2;
@@ -1889,10 +1969,15 @@ int orc_run_twochqsz(const orc_twochqs_params *p, orc_step_cb cb, void *user) {
     int nk = 0;
     Ae.kept.n = 0;
     for (int i = 0; i < Ae.en.n; i++) { int k = Ae.en.a[i] <= ec; ipush(&Ae.kept, k); nk += k; }
+    if (dm) {
+      qsz_dup_all(&Ae, &Ab, &dup, &dall);
+      dm_store_qsz(dm, &Ae, &Ab, &Ac, &dall);
+    }
     emit(cb, user, &Ae, &nc, emin, ec, nk, &hrec, p->record_blocks, thT, sus, ent, DN / bbar, 0, NULL);
     arr_free(&nc);
   }
   free(hrec.a);
+  mat_free(&dup); mat_free(&dall);
   mat_free(&f[0]); mat_free(&f[1]); mat_free(&d);
   arr_free(&S); arr_free(&Ae); arr_free(&Ac); arr_free(&Ab); arr_free(&Ah); arr_free(&Ahc);
   return 0;
@@ -1931,7 +2016,7 @@ struct orc_dm {
 };
 
 static void arr_copy(arr_t *d, const arr_t *s) {
-  d->nshell = s->nshell; d->nq = s->nq;
+  d->nshell = s->nshell; d->nq = s->nq; d->sz = s->sz;
   dcopy_v(&d->qn, &s->qn); icopy_v(&d->bbe, &s->bbe); dcopy_v(&d->en, &s->en); dcopy_v(&d->ev, &s->ev);
   icopy_v(&d->kept, &s->kept); icopy_v(&d->itype, &s->itype); icopy_v(&d->stcf, &s->stcf);
   icopy_v(&d->bbe_bc, &s->bbe_bc); icopy_v(&d->bbe_ev, &s->bbe_ev);
@@ -1943,23 +2028,21 @@ static double calc_dn(double L, int N) {
   return HLF * pow(L, -(N - 1) / 2.0);
 }
 /* CNRGarray::PartitionFunc  src/NRGarrayClass.cpp:997-1030 ((2S+1) per state, S = QN 1) */
+/* degeneracy of a state of block b: 2S+1 (SU(2) blocks), 1 (Sz blocks) */
+static double blk_deg(const arr_t *A, int b) { return A->sz ? 1.0 : 2.0 * getqn(A, b, 1) + 1.0; }
 static double partition_func(const arr_t *A, double bb) {
   double s = 0.0;
-  for (int b = 0; b < nblocks(A); b++) {
-    double Si = getqn(A, b, 1);
-    for (int i = blim(A, b, 0); i <= blim(A, b, 1); i++) s += (2.0 * Si + 1.0) * exp(-bb * A->en.a[i]);
-  }
+  for (int b = 0; b < nblocks(A); b++)
+    for (int i = blim(A, b, 0); i <= blim(A, b, 1); i++) s += blk_deg(A, b) * exp(-bb * A->en.a[i]);
   return s;
 }
 /* CNRGarray::PartitionFuncDisc  src/NRGarrayClass.cpp:1032-1058: discarded states only */
 static double partition_func_disc(const arr_t *A, double bb) {
   if (A->kept.n == 0) return -1.0;
   double s = 0.0;
-  for (int b = 0; b < nblocks(A); b++) {
-    double Si = getqn(A, b, 1);
+  for (int b = 0; b < nblocks(A); b++)
     for (int i = blim(A, b, 0); i <= blim(A, b, 1); i++)
-      if (!A->kept.a[i]) s += (2.0 * Si + 1.0) * exp(-bb * A->en.a[i]);
-  }
+      if (!A->kept.a[i]) s += blk_deg(A, b) * exp(-bb * A->en.a[i]);
   return s;
 }
 /* CNRGarray::CheckKept(ist, kp)  src/NRGarrayClass.cpp:493-503 */
@@ -2019,7 +2102,7 @@ static double set_rho_nmax(double TempBar, double ZN, const arr_t *A, mat_t *R) 
     for (int i = i0; i <= i9; i++) {
       double x = exp(-bb * A->en.a[i]) / ZN;
       dpush(&R->mat, x); i1++;
-      tr += (2.0 * getqn(A, b, 1) + 1.0) * x;
+      tr += blk_deg(A, b) * x;
       for (int j = i + 1; j <= i9; j++) { dpush(&R->mat, 0.0); i1++; }
     }
     ipush(&R->mbe, i1 - 1);
@@ -2135,16 +2218,18 @@ static double gather_rho(const arr_t *AcN, const arr_t *AbNp1, const arr_t *S, c
             double Si = getqn(AbNp1, bi, 1);
             double St = qn_from_st(S, ti, 1);
             double cg = 0.0;
-            for (double Sz = -Si; Sz <= Si; Sz += 1.0) {
-              double x = orc_cgordan(Sold, Sold, St, Sz - Sold, Si, Sz);
-              cg += x * x;
-            }
+            if (AcN->sz) cg = 1.0;   /* Sz blocks: the plain partial trace */
+            else
+              for (double Sz = -Si; Sz <= Si; Sz += 1.0) {
+                double x = orc_cgordan(Sold, Sold, St, Sz - Sold, Si, Sz);
+                cg += x * x;
+              }
             el += cg * rot_el(&rot, rblk, rbeg, ci, cj);
           }
         }
         if (fdm && jst == ist && check_kept(AcN, ist, 0))
           if (fabs(ZNdisc - 0.0) > 1e-20) el = exp(-(1.0 / TempBar) * AcN->en.a[ist]) / ZNdisc;
-        if (ist == jst) tr += (2.0 * Sold + 1.0) * el;
+        if (ist == jst) tr += blk_deg(AcN, b) * el;
         dpush(&RhoN->mat, el); i1++;
       }
     ipush(&RhoN->mbe, i1 - 1);
@@ -2193,7 +2278,7 @@ static double spec_costi(const struct orc_dm *D, int N, double bb, double w, int
         double s12 = a * exp(-bb * A->en.a[i0 + i]) + exp(-bb * A->en.a[j0 + j]) * a;
         tr += s12 * a * (norm ? 1.0 : bdelta_ij(D, A, w, A->en.a[i0 + i], A->en.a[j0 + j]));
       }
-    acc += op_cgfactor(getqn(A, b1, 1), getqn(A, b2, 1)) * tr;
+    acc += (A->sz ? 1.0 : op_cgfactor(getqn(A, b1, 1), getqn(A, b2, 1))) * tr;
   }
   return acc / partition_func(A, bb);
 }
@@ -2246,7 +2331,7 @@ double orc_dm_spec_shell(const orc_dm *D, int N, double w) {
       }
     double t2 = 0.0;
     for (int j = 0; j < n2; j++) for (int i = 0; i < n1; i++) t2 += P[j * n1 + i] * Aw[i * n2 + j];
-    acc += op_cgfactor(getqn(A, b1, 1), getqn(A, b2, 1)) * (t1 + t2);
+    acc += (A->sz ? 1.0 : op_cgfactor(getqn(A, b1, 1), getqn(A, b2, 1))) * (t1 + t2);
     free(Aw); free(P); free(Ab); free(R1); free(R2);
   }
   return acc;
@@ -2342,7 +2427,7 @@ static double spec_kd(const struct orc_dm *D, int N, double w, int norm) {
       t2dd = kd_term2(Ab, n2, R2, D1, d1, D2, d2, Dl);
       if (k1) t2kd = kd_term2(Ab, n2, R2, K1, k1, D2, d2, Dl);
     }
-    double cg = op_cgfactor(getqn(A, b1, 1), getqn(A, b2, 1));
+    double cg = A->sz ? 1.0 : op_cgfactor(getqn(A, b1, 1), getqn(A, b2, 1));
     pos += cg * (t1kk + t1dd + t2dd + t2kd);
     neg += cg * (t2kk + t1dd + t1dk + t2dd);
     free(K1); free(D1); free(K2); free(D2); free(Ab); free(Dl); free(R1); free(R2);
@@ -2494,11 +2579,33 @@ static void spec_many_omegas(struct orc_dm *D, int Nw, double fac) {
  * ZNdisc = Z_tot/4^{Nmax-N} (:361-366); spectral function rho_0_0 (:451-506) on the grid
  * of DM_NRG_CalcSpecFunc_ij src/DM_NRG/DM_NRG_CalcSpecFuncs.cpp:73-84 */
 static void tan_dm_sweep(struct orc_dm *D, double bb_r, double bb_z, double ZN);
+/* a TwoChQSz shell as SaveArrays would write it with UpdateBefCut 1: every state
+ * of Aeig with its Kept flags, Abasis with StCameFrom as the previous shell's
+ * indices before its cut, d_up on every state */
+static void dm_store_qsz(struct orc_dm *D, const arr_t *Ae, const arr_t *Ab, const arr_t *Acprev, const mat_t *dall) {
+  arr_t A = {0}, B = {0};
+  arr_copy(&A, Ae);
+  A.sz = 1;
+  icopy_v(&A.bbe_bc, &Ae->bbe);
+  A.stcf.n = 0;
+  for (int i = 0; i < A.en.n; i++) ipush(&A.stcf, i);
+  set_bbe_ev(&A);
+  arr_copy(&B, Ab);
+  B.sz = 1;
+  for (int i = 0; i < B.stcf.n; i++) B.stcf.a[i] = Acprev->stcf.a[Ab->stcf.a[i]];
+  mat_t O = {0};
+  mat_copy(&O, dall);
+  O.b.sz = 1;
+  dm_store_step(D, &A, &B, &O);
+  mat_free(&O);
+  arr_free(&A); arr_free(&B);
+}
 orc_dm *orc_dm_run(const orc_dmnrg_params *p) { return orc_dm_run_jvp(p, NULL, NULL, 0, NULL); }
 orc_dm *orc_dm_run_jvp(const orc_dmnrg_params *p, const unsigned char *kept, const long long *kept_off, int ntan,
                        const double *dpar) {
   if (ntan < 0 || ntan > ORC_MAX_TAN || (ntan > 0 && (!dpar || p->method != 0 || p->update_bef_cut || !(p->Gamma > 0))))
     return NULL;
+  if (p->model == 1 && (ntan > 0 || kept)) return NULL;
   struct orc_dm *D = (struct orc_dm *)calloc(1, sizeof(struct orc_dm));
   D->p = *p;
   if (D->p.twindow <= 0) D->p.twindow = 2.0;
@@ -2517,10 +2624,21 @@ orc_dm *orc_dm_run_jvp(const orc_dmnrg_params *p, const unsigned char *kept, con
   for (int t = 0; t < ntan; t++)
     for (int c = 0; c < 3; c++) T.dpar[t][c] = dpar[3 * t + c];
   D->ntan = ntan;
-  run_onechqs(&q, NULL, NULL, D, p->update_bef_cut ? 1 : 0, kept, kept_off, ntan > 0 ? &T : NULL);
+  const int qsz = (p->model == 1);
+  if (qsz) {   /* TwoChQSz standalone loop (src/TwoChQSz/TwoChQSz.cpp:103-425), shells 0 .. nsitesmax */
+    orc_twochqs_params q2;
+    memset(&q2, 0, sizeof(q2));
+    q2.U = p->U; q2.Gamma1 = p->Gamma; q2.Gamma2 = p->Gamma2; q2.ed = p->ed; q2.Lambda = p->Lambda;
+    q2.nsitesmax = p->nsitesmax; q2.ncut = p->ncut; q2.betabar = p->betabar_nrg;
+    run_twochqsz(&q2, NULL, NULL, D);
+    twochqsz_site(&D->site);
+  } else {
+    run_onechqs(&q, NULL, NULL, D, p->update_bef_cut ? 1 : 0, kept, kept_off, ntan > 0 ? &T : NULL);
+    onechqs_site(&D->site);
+  }
   tan_free(&T);
-  onechqs_site(&D->site);
-  int Nsitesmax = p->nsitesmax;
+  const double site_dim = qsz ? 16.0 : 4.0;   /* 4: the reference's one-channel value (src/DM_NRG/DM_NRG.cpp:244) */
+  int Nsitesmax = D->nsh;
   double DM, Temp;
   if (p->Mtemp < Nsitesmax - 1) {
     if (!fdm) Nsitesmax = p->Mtemp + 1;
@@ -2544,7 +2662,7 @@ orc_dm *orc_dm_run_jvp(const orc_dmnrg_params *p, const unsigned char *kept, con
   D->wn[NM] = ZN;
   for (int N = NM - 1; N >= 0; N--) {
     double tb = Temp / calc_dn(p->Lambda, N - 1);
-    double zd = partition_func_disc(&D->acut[N], 1.0 / tb) * pow(4.0, NM - N);
+    double zd = partition_func_disc(&D->acut[N], 1.0 / tb) * pow(site_dim, NM - N);
     D->wn[N] = zd;
     ZTot += zd;
   }
@@ -2554,7 +2672,7 @@ orc_dm *orc_dm_run_jvp(const orc_dmnrg_params *p, const unsigned char *kept, con
   for (int N = NM - 1; N >= 0; N--) {
     double tb = Temp / calc_dn(p->Lambda, N - 1);
     D->trace[N] = calc_rho_n(&D->acut[N], &D->acut[N + 1], &D->abasis[N + 1], &D->site, &D->rho[N + 1], &D->rho[N],
-                             fdm, tb, ZTot / pow(4.0, NM - N));
+                             fdm, tb, ZTot / pow(site_dim, NM - N));
   }
   if (ntan > 0) tan_dm_sweep(D, fabs(Temp / DM) < 1e-20 ? 1e10 : DM / Temp, 1.0 / TempBar, ZN);
   return D;

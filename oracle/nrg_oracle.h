@@ -110,6 +110,10 @@ typedef struct {
   int Nw;                     /* -n: 0 -> CalcSpecDM_NRG_FixedOmegas, >0 -> CalcSpec_ManyOmegas */
   int method;                 /* 0: DM-NRG (UseCFS 0); 1: CFS (-C); 2: FDM (-F) */
   int update_bef_cut;         /* NRG run UpdateBefCut (src/Main/NRG_main.cpp:334-343,393-396) */
+  int model;                  /* 0: OneChQS (NRG_main); 1: TwoChQSz (standalone loop, Gamma1 = Gamma), which
+                                 the reference never hands to DM_NRG: every shell keeps all its states and
+                                 d_up on them, Sz blocks (no CG factors), FDM site dimension 16 */
+  double Gamma2;              /* TwoChQSz second channel */
 } orc_dmnrg_params;
 
 typedef struct orc_dm orc_dm;

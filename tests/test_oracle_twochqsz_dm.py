@@ -1,0 +1,145 @@
+"""The oracle's DM-NRG / FDM post-processing of the TwoChQSz chain (config C5;
+the reference never hands a TwoChQSz run to DM_NRG, so this part restates its
+OneChQS path on (Q, Sz) blocks: degeneracy 1, no Clebsch-Gordan factors, d_up
+on every state of a shell, FDM site dimension 16).
+
+Pinned by a Fock-space construction of the same two-channel chain (impurity +
+both channels' sites 0..N, the legacy TwoChQS(z) rescaling): the operator d_up
+in the shell eigenbasis (the impurity table of TwoChQSz_SetH0Anderson and the
+impurity-only update with the new site's fermion sign) through the thermal
+Lehmann sum of the last shell, and the backward sweep rho_0 = Tr_site rho_1 by
+the Fock-space partial trace, both untruncated.  With truncation: the trace of
+rho_N is conserved down the sweep and the FDM shell weights sum to 1."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle"))
+import pyoracle as po  # noqa: E402
+import ed_util as ed  # noqa: E402
+
+L = 2.5
+HLF = 0.5 * (1 + 1 / L)
+CASES = [dict(U=0.5, G1=0.03, G2=0.01, ed=-0.2), dict(U=0.3, G1=0.05, G2=0.05, ed=-0.15)]
+
+
+def _fock(p, N):
+    """Dense rescaled H_N (minus its minimum) and the annihilators: orbital 0 = d,
+    orbital 1 + 2n + ch = site n of channel ch (mode 2 orbital + spin)."""
+    v = [np.sqrt(g * (2 / np.pi) / HLF ** 2 / L) for g in (p["G1"], p["G2"])]
+
+    def chi(n):
+        return (1 - L ** -float(n)) / np.sqrt((1 - L ** -(2.0 * n - 1)) * (1 - L ** -(2.0 * n + 1)))
+    E = ed.FockED(1 + 2 * (N + 1))
+    s = L ** (N / 2.0) / (HLF * np.sqrt(L))
+    E.eps(0, p["ed"] * s)
+    E.U(0, p["U"] * s)
+    E.const = 0.5 * p["U"] * s
+    for ch in range(2):
+        E.hop(0, 1 + ch, v[ch] * L ** (N / 2.0))
+        for n in range(1, N + 1):
+            E.hop(1 + 2 * (n - 1) + ch, 1 + 2 * n + ch, chi(n) * L ** ((N - n) / 2.0))
+    H, C = ed.fock_hamiltonian(E)
+    return H - np.linalg.eigvalsh(H).min() * np.eye(len(H)), C
+
+
+def _loggauss(w, E, b):
+    if E == 0.0:
+        E = 1e-100
+    if w * E < 0:
+        return 0.0
+    a1 = np.log(abs(w / E)) / b
+    return np.exp(-(a1 - b / 4) ** 2) / (b * abs(E) * np.sqrt(np.pi))
+
+
+def _gauss(w, E, b):
+    return np.exp(-((w - E) / b) ** 2) / (b * np.sqrt(np.pi))
+
+
+@pytest.mark.parametrize("p", CASES)
+def test_last_shell_is_the_lehmann_sum(p):
+    """FDM on an untruncated 2-shell chain: the window sum is the Costi form of the
+    last shell, the thermal Lehmann sum of d_up with LogGaussDelta / GaussDelta."""
+    bb = 0.9
+    D = po.DmRun(U=p["U"], Gamma=p["G1"], ed=p["ed"], Lambda=L, nsitesmax=1, ncut=10 ** 6, Mtemp=0, betabar=bb,
+                 method=2, model=1, Gamma2=p["G2"], bbroad=0.6)
+    assert D.nsh == 2
+    N = D.spec_nshell_max - 1
+    H, C = _fock(p, N)
+    ev, V = np.linalg.eigh(H)
+    c = V.T @ C[0] @ V
+    x = np.exp(-bb * ev)
+    Z = x.sum()
+    tdn = D.temp / (HLF * L ** (-(N - 1) / 2.0))
+    for w in (-2.1, -0.6, -0.05, 0.03, 0.4, 1.7):
+        ref = 0.0
+        for a in range(len(ev)):
+            for b in np.nonzero(np.abs(c[a]) > 1e-14)[0]:
+                dE = ev[b] - ev[a]
+                bd = _gauss(w, dE, 0.8) if abs(dE) < 2.0 * tdn else _loggauss(w, dE, 0.6)
+                ref += c[a, b] ** 2 * (x[a] + x[b]) / Z * bd
+        got = D.spec_shell(N, w)
+        assert abs(got - ref) <= 1e-12 * max(abs(ref), 1e-6), (w, got, ref)
+
+
+def _trace_out_site(rho, nmode):
+    """Trace out the highest 4 modes (site N of both channels)."""
+    r = ed.partial_trace_last_orbital(rho, nmode)
+    return ed.partial_trace_last_orbital(r, nmode - 2)
+
+
+@pytest.mark.parametrize("p", CASES)
+def test_sweep_is_the_partial_trace(p):
+    """rho_0 = Tr_site rho_1 with rho_1 = exp(-betabar H_1)/Z (FDM, untruncated):
+    block by block against the Fock-space partial trace, compared through
+    Frobenius norms between degenerate clusters of H_0 (gauge invariant)."""
+    bb = 0.9
+    D = po.DmRun(U=p["U"], Gamma=p["G1"], ed=p["ed"], Lambda=L, nsitesmax=1, ncut=10 ** 6, Mtemp=0, betabar=bb,
+                 method=2, model=1, Gamma2=p["G2"], bbroad=0.6)
+    H1, _ = _fock(p, 1)
+    w, V = np.linalg.eigh(H1)
+    rho1 = (V * np.exp(-bb * w)) @ V.T / np.sum(np.exp(-bb * w))
+    rho0 = _trace_out_site(rho1, 10)
+    H0, _ = _fock(p, 0)
+    nm = 6
+    ne = np.array([bin(s).count("1") for s in range(1 << nm)])
+    sz2 = np.array([sum(((s >> (2 * o)) & 1) - ((s >> (2 * o + 1)) & 1) for o in range(nm // 2)) for s in range(1 << nm)])
+    sh = D.shell(0)
+    assert abs(sh["trace"] - 1.0) <= 1e-12 and abs(np.trace(rho0) - 1.0) <= 1e-12
+    for b in range(len(sh["rho"])):
+        Q, Sz = sh["qn"][b]
+        sel = np.where((ne - 3 == int(round(Q))) & (sz2 == int(round(2 * Sz))))[0]
+        e, U = np.linalg.eigh(H0[np.ix_(sel, sel)])
+        en = sh["en"][sh["bbe"][b, 0]:sh["bbe"][b, 1] + 1]
+        np.testing.assert_allclose(e, en, atol=1e-10)
+        red = U.T @ rho0[np.ix_(sel, sel)] @ U
+        R = sh["rho"][b]
+        cl, cur = [], [0]
+        for i in range(1, len(en)):
+            if abs(en[i] - en[cur[0]]) < 1e-8:
+                cur.append(i)
+            else:
+                cl.append(cur)
+                cur = [i]
+        cl.append(cur)
+        for c1 in cl:
+            for c2 in cl:
+                a, x = np.linalg.norm(R[np.ix_(c1, c2)]), np.linalg.norm(red[np.ix_(c1, c2)])
+                assert abs(a - x) <= 1e-11, (b, c1, c2, a, x)
+
+
+def test_truncated_run_invariants():
+    """C5-like TwoChQSz FDM run with truncation from N = 2: the printed traces
+    sum rho_ii are conserved down the sweep, the FDM weights sum to 1, and the
+    window sums are finite and non-negative."""
+    D = po.DmRun(U=0.5, Gamma=0.03, ed=-0.25, Lambda=2.5, nsitesmax=6, ncut=400, Mtemp=4, betabar=0.779,
+                 method=2, model=1, Gamma2=0.01, bbroad=0.6)
+    kept = [D.shell(N)["kept"] for N in range(D.nshell_max + 1)]
+    assert any(not k.all() for k in kept)
+    wn = np.array([D.shell(N)["wn"] for N in range(D.nshell_max + 1)])
+    assert abs(wn.sum() - 1.0) <= 1e-12
+    for w in (1e-3, -1e-3, 0.05, -0.3, 0.8):
+        a = D.spec_omega(w)
+        assert np.isfinite(a) and a >= -1e-12, (w, a)
