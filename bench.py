@@ -163,6 +163,70 @@ def fdm_run(cpu=True, cpu_omegas=12):
     return out
 
 
+C5 = dict(U=0.5, Gamma=0.03, Gamma2=0.01, ed=-0.25, Lambda=2.5, nsitesmax=49, ncut=4000, model=1)
+
+
+def c5_run(cpu=True, nsitesmax=None, ncut=None, site_dims=(16.0, 4.0)):
+    """FDM A(omega) of d_up (DM_NRG -F -e 0.6) at T = 1e-9 (Mtemp 46) on the
+    TwoChQSz two-channel chain, 50 sites (N = 0 .. 49) keeping 4000 states, on
+    omega = +-logspace(-10, 0, 1000); wall = forward chain with every shell's
+    states and d_up kept + FDM sweep + pole weights + the 2000 omegas.  Once with
+    the two-channel environment count d = 16 per later site and once with the
+    reference's one-channel d = 4.  The CPU oracle runs the same model at a
+    size it finishes in seconds (5 sites, 200 states), with 8 omegas."""
+    import numpy as np
+    import twochnrg_amd as t
+    base = dict(C5)
+    if nsitesmax is not None:
+        base["nsitesmax"] = nsitesmax
+    if ncut is not None:
+        base["ncut"] = ncut
+    M = min(46, base["nsitesmax"] - 1)   # shorter probe chains keep betabar = D_46 / 1e-9
+    T = 1e-9 if M == 46 else dm_betabar(1.0, 2.5, M) / dm_betabar(1e-9, 2.5, 46)
+    kw = dict(base, Mtemp=M, betabar=dm_betabar(T, 2.5, M), method=2, bbroad=0.6)
+    om = fdm_omegas()
+    out = {"workload": "twochqsz_fdm", "T": T, "betabar": kw["betabar"], "bbroad": 0.6, "nomega": len(om),
+           "params": base, "runs": {}}
+    with t.Context(0) as ctx:
+        for d in site_dims:
+            t0 = time.perf_counter()
+            with ctx.dm(**kw, fdm_site_dim=d) as h:
+                r = h.spec_omega(om)
+                t1 = time.perf_counter()
+                tm = h.timing()
+                norm = h.norm()
+                wn, ztot = h.weights()
+            rec = {"wall_ms": 1e3 * (t1 - t0), "breakdown_ms": tm,
+                   "post_ms": tm["sweep"] + tm["poles"] + tm["broaden"], "norm": norm, "ztot": ztot,
+                   "shell_weight_max": int(np.argmax(wn)), "A_at_omega_pm1e-10": [float(r[999]), float(r[1000])],
+                   "A_max": float(r.max())}
+            out["runs"]["d%d" % int(d)] = rec
+            print("bench c5: site dim %d done (%.0f ms, forward %.0f ms)" % (d, rec["wall_ms"], tm["forward"]),
+                  file=sys.stderr, flush=True)
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle as po
+        small = dict(kw, nsitesmax=5, ncut=200, Mtemp=3, betabar=0.779)
+        idx = np.linspace(0, len(om) - 1, 8).astype(int)
+        t0 = time.perf_counter()
+        o = po.DmRun(**small)
+        t1 = time.perf_counter()
+        ro = np.array([o.spec_omega(x) for x in om[idx]])
+        t2 = time.perf_counter()
+        with t.Context(0) as ctx:
+            t3 = time.perf_counter()
+            with ctx.dm(**small) as h:
+                rg = h.spec_omega(om)
+            t4 = time.perf_counter()
+        out["cpu_oracle_small"] = {
+            "sample": "5 sites, 200 states, Mtemp 3: chain + sweep, 8 of the 2000 omegas timed and scaled",
+            "cpu_ms": 1e3 * ((t1 - t0) + (t2 - t1) * len(om) / len(idx)), "gpu_ms": 1e3 * (t4 - t3),
+            "max_rel_err_vs_oracle": float(np.max(np.abs(rg[idx] - ro) / np.maximum(np.abs(ro),
+                                                                                   1e-12 * np.abs(ro).max())))}
+        print("bench c5: oracle sample %.1f s" % (t2 - t0), file=sys.stderr, flush=True)
+    return out
+
+
 def c2_run(steps=4, ramp=6, plain_steps=1):
     """64 seeded single-channel Anderson instances (splitmix64 seed 20261203,
     tests/golden/c2_instances.json), Lambda 2.5, Ncut 800..2000, stepped in
@@ -374,6 +438,9 @@ def main():
     ap.add_argument("--no-dm", action="store_true", help="skip the DM-NRG A(omega) leg")
     ap.add_argument("--no-c2", action="store_true", help="skip the 64-instance batched leg")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the derivative-pass leg")
+    ap.add_argument("--no-c5", action="store_true", help="skip the two-channel FDM leg")
+    ap.add_argument("--c5-only", action="store_true",
+                    help="run only the two-channel FDM leg and print its JSON object")
     ap.add_argument("--adjoint-only", action="store_true",
                     help="run only the derivative-pass leg (device part) and print its JSON object")
     ap.add_argument("--stats-all", action="store_true",
@@ -391,6 +458,9 @@ def main():
         sys.exit(2)
     if a.adjoint_only:
         print(json.dumps(adjoint_run(cpu=False)))
+        return
+    if a.c5_only:
+        print(json.dumps(c5_run(cpu=not a.no_cpu)))
         return
     dt, st, timed, model, w, nsteps = gpu_run(a.workload, a.steps, a.warmup, a.eig_impl, stats_all=a.stats_all)
     value = nsteps / dt
@@ -447,6 +517,8 @@ def main():
         out["batched_instances"] = c2_run()
     if not a.no_adjoint:
         out["adjoint"] = adjoint_run(cpu=not a.no_cpu)
+    if not a.no_c5:
+        out["twochqsz_fdm"] = c5_run(cpu=not a.no_cpu)
     print(json.dumps(out))
 
 

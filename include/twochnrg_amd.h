@@ -265,6 +265,16 @@ typedef struct {
   int update_bef_cut;              /* NRG run UpdateBefCut (src/Main/NRG_main.cpp:334-343,393-396);
                                       CFS and FDM need 1: the discarded states and their operator
                                       elements are kept per shell */
+  int model;                       /* 0: OneChQS Anderson (U, Gamma, ed); 1: TwoChQSz two-channel
+                                      Anderson (U, Gamma, Gamma2, ed; the chain of nrg_run_twochqsz),
+                                      run with d_up kept per shell and the sweep
+                                      over (Q, Sz) blocks.  Model 1 needs method 1 or 2 and no
+                                      derivatives. */
+  double Gamma2;                   /* model 1: hybridisation of the second channel */
+  double fdm_site_dim;             /* FDM states per later site in w_N = Z_N^disc d^{Nmax-N}/Z_tot and
+                                      rho_N's Z_tot/d^{Nmax-N}: 0 -> 4 for model 0 (the reference's
+                                      value, src/DM_NRG/DM_NRG.cpp:244-247,365-366) and 16 for model 1;
+                                      any other positive value is used as given */
 } nrg_dmnrg_params;
 
 typedef struct nrg_dm nrg_dm;

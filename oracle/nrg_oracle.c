@@ -2637,7 +2637,8 @@ orc_dm *orc_dm_run_jvp(const orc_dmnrg_params *p, const unsigned char *kept, con
     onechqs_site(&D->site);
   }
   tan_free(&T);
-  const double site_dim = qsz ? 16.0 : 4.0;   /* 4: the reference's one-channel value (src/DM_NRG/DM_NRG.cpp:244) */
+  /* 4: the reference's one-channel value (src/DM_NRG/DM_NRG.cpp:244); 16 the two-channel site */
+  const double site_dim = p->fdm_site_dim > 0.0 ? p->fdm_site_dim : (qsz ? 16.0 : 4.0);
   int Nsitesmax = D->nsh;
   double DM, Temp;
   if (p->Mtemp < Nsitesmax - 1) {

@@ -112,8 +112,10 @@ typedef struct {
   int update_bef_cut;         /* NRG run UpdateBefCut (src/Main/NRG_main.cpp:334-343,393-396) */
   int model;                  /* 0: OneChQS (NRG_main); 1: TwoChQSz (standalone loop, Gamma1 = Gamma), which
                                  the reference never hands to DM_NRG: every shell keeps all its states and
-                                 d_up on them, Sz blocks (no CG factors), FDM site dimension 16 */
+                                 d_up on them, Sz blocks (no CG factors) */
   double Gamma2;              /* TwoChQSz second channel */
+  double fdm_site_dim;        /* FDM environment count per later site: 0 -> 4 for model 0 (the reference's
+                                 value, src/DM_NRG/DM_NRG.cpp:244,365) and 16 for model 1 (two channels) */
 } orc_dmnrg_params;
 
 typedef struct orc_dm orc_dm;

@@ -1,5 +1,5 @@
-"""The oracle's DM-NRG / FDM post-processing of the TwoChQSz chain (config C5;
-the reference never hands a TwoChQSz run to DM_NRG, so this part restates its
+"""The oracle's DM-NRG / FDM post-processing of the TwoChQSz chain (the
+reference never hands a TwoChQSz run to DM_NRG, so this part restates its
 OneChQS path on (Q, Sz) blocks: degeneracy 1, no Clebsch-Gordan factors, d_up
 on every state of a shell, FDM site dimension 16).
 
@@ -131,7 +131,7 @@ def test_sweep_is_the_partial_trace(p):
 
 
 def test_truncated_run_invariants():
-    """C5-like TwoChQSz FDM run with truncation from N = 2: the printed traces
+    """A TwoChQSz FDM run with truncation from N = 2: the printed traces
     sum rho_ii are conserved down the sweep, the FDM weights sum to 1, and the
     window sums are finite and non-negative."""
     D = po.DmRun(U=0.5, Gamma=0.03, ed=-0.25, Lambda=2.5, nsitesmax=6, ncut=400, Mtemp=4, betabar=0.779,
@@ -143,3 +143,20 @@ def test_truncated_run_invariants():
     for w in (1e-3, -1e-3, 0.05, -0.3, 0.8):
         a = D.spec_omega(w)
         assert np.isfinite(a) and a >= -1e-12, (w, a)
+
+
+def test_fdm_site_dimension():
+    """The FDM environment count d enters w_N only as Z_N^disc d^{Nmax-N}: the
+    unnormalised weights w_N Z_tot of d = 16 and d = 4 differ by 4^{Nmax-N} on
+    the same chain, and both sets of w_N sum to 1."""
+    c = dict(U=0.5, Gamma=0.03, ed=-0.25, Lambda=2.5, nsitesmax=4, ncut=200, Mtemp=3, betabar=0.779,
+             method=2, model=1, Gamma2=0.01, bbroad=0.6)
+    D16 = po.DmRun(**c)
+    D4 = po.DmRun(**dict(c, fdm_site_dim=4.0))
+    NM = D16.nshell_max
+    w16 = np.array([D16.shell(N)["wn"] for N in range(NM + 1)]) * D16.ztot
+    w4 = np.array([D4.shell(N)["wn"] for N in range(NM + 1)]) * D4.ztot
+    assert w4[0] == 0.0 == w16[0] and (w4[1:] > 0).all()   # N = 0 keeps every state
+    np.testing.assert_allclose(w16[1:] / w4[1:], 4.0 ** (NM - np.arange(1, NM + 1)), rtol=1e-12)
+    for D in (D4, D16):
+        assert abs(sum(D.shell(N)["wn"] for N in range(NM + 1)) - 1.0) <= 1e-12

@@ -30,6 +30,7 @@
 
 namespace nrg {
 int run_onechqs_dm(nrg_ctx* c, const nrg_onechqs_params& p, std::vector<DmShell>& dm, int befcut);
+int run_twochqsz_dm(nrg_ctx* c, const nrg_twochqs_params& p, std::vector<DmShell>& dm);
 int run_onechqs_dm_jvp(nrg_ctx* c, const nrg_onechqs_params& p, std::vector<DmShell>& dm, const unsigned char* kept,
                        const int64_t* kept_off, int ntan, const double* dpar);
 double cgordan(double J1, double M1, double J2, double M2, double J, double M);  // nrg_driver.cpp
@@ -68,7 +69,7 @@ double calc_dn(double L, int N) {   // CalcDN src/CalcDN.cpp:4-15, z = 1
 double partition_func(const DmShell& s, double bb) {   // src/NRGarrayClass.cpp:997-1030
   double z = 0.0;
   for (int b = 0; b < s.nblocks(); b++)
-    for (int i = s.bbe[2 * b]; i <= s.bbe[2 * b + 1]; i++) z += (2.0 * s.qn[2 * b + 1] + 1.0) * std::exp(-bb * s.en[i]);
+    for (int i = s.bbe[2 * b]; i <= s.bbe[2 * b + 1]; i++) z += s.deg(b) * std::exp(-bb * s.en[i]);
   return z;
 }
 double partition_func_disc(const DmShell& s, double bb) {   // src/NRGarrayClass.cpp:1032-1058
@@ -76,7 +77,7 @@ double partition_func_disc(const DmShell& s, double bb) {   // src/NRGarrayClass
   double z = 0.0;
   for (int b = 0; b < s.nblocks(); b++)
     for (int i = s.bbe[2 * b]; i <= s.bbe[2 * b + 1]; i++)
-      if (!s.kept[i]) z += (2.0 * s.qn[2 * b + 1] + 1.0) * std::exp(-bb * s.en[i]);
+      if (!s.kept[i]) z += s.deg(b) * std::exp(-bb * s.en[i]);
   return z;
 }
 int kept_prefix(const DmShell& s, int b, int& kc) {   // Kept states lead each energy-ordered block
@@ -207,11 +208,12 @@ int sweep_shell(nrg_dm* D, int N, bool fdm, double TempBar, double zdisc) {
       const int q = bas2cut[ab];
       if (q < 0) continue;   // block cut at N+1: its rotated rho is zero
       const double Si = b.bas_qn[2 * ab + 1], St = site_S(t);
-      double cg = 0.0;
-      for (double Sz = -Si; Sz <= Si; Sz += 1.0) {
-        const double x = cgordan(Sold, Sold, St, Sz - Sold, Si, Sz);
-        cg += x * x;
-      }
+      double cg = b.sz ? 1.0 : 0.0;   // (Q, Sz) blocks: the plain partial trace
+      if (!b.sz)
+        for (double Sz = -Si; Sz <= Si; Sz += 1.0) {
+          const double x = cgordan(Sold, Sold, St, Sz - Sold, Si, Sz);
+          cg += x * x;
+        }
       const int n = b.bsize_bc(q);
       const int pc = c0 - b.bas_bbe[2 * ab];
       DmSeg S{};
@@ -309,7 +311,7 @@ int build_poles(nrg_dm* D) {
       P.a_off = s.op_mbe[q];
       P.k1 = k1; P.k2 = k2;
       P.e1_off = s.bbe[2 * b1]; P.e2_off = s.bbe[2 * b2];
-      P.cg = op_cgfactor(s.qn[2 * b1 + 1], s.qn[2 * b2 + 1]);
+      P.cg = s.sz ? 1.0 : op_cgfactor(s.qn[2 * b1 + 1], s.qn[2 * b2 + 1]);
       P.form = form;
       P.bb = bb; P.invZ = invZ;
       if ((rc = kept_prefix(s, b1, P.kc1))) return rc;
@@ -441,7 +443,7 @@ int build_poles_jvp(nrg_dm* D) {
         double dZ = 0.0;
         for (int b = 0; b < nb; b++) {
           const int k = s.bsize(b), i0 = s.bbe[2 * b];
-          const double dg = 2.0 * s.qn[2 * b + 1] + 1.0;
+          const double dg = s.deg(b);
           for (int i = 0; i < k; i++)
             dZ += dg * th[(size_t)t * btot + boff[b] + (int64_t)i * k + i] * (-bb * std::exp(-bb * s.en[i0 + i]));
         }
@@ -487,7 +489,7 @@ int build_poles_jvp(nrg_dm* D) {
       DmTanPair P{};
       P.a_off = s.op_mbe[q]; P.k1 = s.bsize(b1); P.k2 = s.bsize(b2);
       P.q_off = qtot; P.pole_off = pole;
-      P.cg = op_cgfactor(s.qn[2 * b1 + 1], s.qn[2 * b2 + 1]);
+      P.cg = s.sz ? 1.0 : op_cgfactor(s.qn[2 * b1 + 1], s.qn[2 * b2 + 1]);
       qtot += (int64_t)P.k1 * P.k2;
       pole += (int64_t)P.k1 * P.k2;
       kk2 = std::max(kk2, P.k1 * P.k2);
@@ -715,6 +717,10 @@ static int dm_create(nrg_ctx* c, const nrg_dmnrg_params* p, const unsigned char*
   if (p->nsitesmax < 2 || p->ncut < 1 || p->Lambda <= 1.0 || p->betabar <= 0.0 || p->Mtemp < 1 ||
       p->method < 0 || p->method > 2)
     return fail(NRG_ERR_ARG, "nrg_dm_create: bad parameters");
+  if (!(p->fdm_site_dim >= 0.0)) return fail(NRG_ERR_ARG, "nrg_dm_create: fdm_site_dim must be 0 or positive");
+  if (p->model < 0 || p->model > 1) return fail(NRG_ERR_ARG, "nrg_dm_create: model must be 0 (OneChQS) or 1 (TwoChQSz)");
+  if (p->model == 1 && (p->method == 0 || ntan > 0 || kept))
+    return fail(NRG_ERR_ARG, "nrg_dm_create: the TwoChQSz chain supports CFS and FDM without derivatives");
   HIPCHK(hipSetDevice(c->device));
   nrg_dm* D = new nrg_dm();
   D->c = c;
@@ -730,7 +736,14 @@ static int dm_create(nrg_ctx* c, const nrg_dmnrg_params* p, const unsigned char*
   int rc;
   auto t0 = std::chrono::steady_clock::now();
   D->ntan = ntan;
-  if (kept || ntan > 0) {
+  if (p->model == 1) {
+    // every state of each shell is kept for the sweep (UpdateBefCut 1 is implied)
+    nrg_twochqs_params q2{};
+    q2.U = p->U; q2.Gamma1 = p->Gamma; q2.Gamma2 = p->Gamma2; q2.ed = p->ed; q2.Lambda = p->Lambda;
+    q2.nsitesmax = p->nsitesmax; q2.ncut = p->ncut; q2.betabar = p->betabar_nrg;
+    if ((rc = run_twochqsz_dm(c, q2, D->sh))) { delete D; return rc; }
+    for (DmShell& s : D->sh) s.sz = true;
+  } else if (kept || ntan > 0) {
     if (p->update_bef_cut) { delete D; return fail(NRG_ERR_ARG, "nrg_dm_create_jvp: frozen Kept sets need UpdateBefCut 0"); }
     if ((rc = run_onechqs_dm_jvp(c, q, D->sh, kept, kept_off, ntan, dpar))) { delete D; return rc; }
   } else if ((rc = run_onechqs_dm(c, q, D->sh, p->update_bef_cut ? 1 : 0))) {
@@ -739,7 +752,8 @@ static int dm_create(nrg_ctx* c, const nrg_dmnrg_params* p, const unsigned char*
   }
   auto t1 = std::chrono::steady_clock::now();
   // temperature from Mtemp (src/DM_NRG/DM_NRG.cpp:102-118); FDM keeps every shell (:105)
-  int Nsitesmax = p->nsitesmax;
+  // the TwoChQSz loop diagonalises N = 0 .. nsitesmax: one shell more than NRG_main's sites
+  int Nsitesmax = (p->model == 1) ? (int)D->sh.size() : p->nsitesmax;
   double DM;
   if (p->Mtemp < Nsitesmax - 1) {
     if (!fdm) Nsitesmax = p->Mtemp + 1;
@@ -750,6 +764,10 @@ static int dm_create(nrg_ctx* c, const nrg_dmnrg_params* p, const unsigned char*
     DM = calc_dn(p->Lambda, Nsitesmax - 1);
   }
   D->betabar_spec = p->betabar;
+  if ((int)D->sh.size() < Nsitesmax) { delete D; return fail(NRG_ERR_STATE, "nrg_dm_create: fewer shells than NRG sites"); }
+  // FDM's count of the discarded states' environment: the site dimension per
+  // later site (4 for one channel, as src/DM_NRG/DM_NRG.cpp:244; 16 for two)
+  const double site_dim = p->fdm_site_dim > 0.0 ? p->fdm_site_dim : ((p->model == 1) ? 16.0 : 4.0);
   D->NM = Nsitesmax - 1;
   D->specNM = Nsitesmax;
   const int NM = D->NM;
@@ -763,7 +781,7 @@ static int dm_create(nrg_ctx* c, const nrg_dmnrg_params* p, const unsigned char*
   std::vector<double> wraw(NM + 1, 0.0);
   wraw[NM] = ZN;
   for (int N = NM - 1; N >= 0; N--) {
-    const double zd = partition_func_disc(D->sh[N], 1.0 / (D->temp / calc_dn(p->Lambda, N - 1))) * std::pow(4.0, NM - N);
+    const double zd = partition_func_disc(D->sh[N], 1.0 / (D->temp / calc_dn(p->Lambda, N - 1))) * std::pow(site_dim, NM - N);
     wraw[N] = zd;
     ZTot += zd;
   }
@@ -804,7 +822,7 @@ static int dm_create(nrg_ctx* c, const nrg_dmnrg_params* p, const unsigned char*
         double dZ = 0.0;
         for (int b = 0; b < s.nblocks(); b++) {
           const int k = s.bsize(b), i0 = s.bbe[2 * b];
-          const double dg = 2.0 * s.qn[2 * b + 1] + 1.0;
+          const double dg = s.deg(b);
           for (int i = 0; i < k; i++)
             dZ += dg * th[(size_t)t * tht + s.th_off[b] + (int64_t)i * k + i] * (-bz * std::exp(-bz * s.en[i0 + i]));
         }
@@ -827,7 +845,7 @@ static int dm_create(nrg_ctx* c, const nrg_dmnrg_params* p, const unsigned char*
   }
   // the sweep: TempBar = Temp/D_{N-1}, ZNdisc = Z_tot/4^{NM-N} (:361-366)
   for (int N = NM - 1; N >= 0; N--)
-    if ((rc = sweep_shell(D, N, fdm, D->temp / calc_dn(p->Lambda, N - 1), ZTot / std::pow(4.0, NM - N)))) {
+    if ((rc = sweep_shell(D, N, fdm, D->temp / calc_dn(p->Lambda, N - 1), ZTot / std::pow(site_dim, NM - N)))) {
       delete D;
       return rc;
     }
@@ -883,7 +901,7 @@ extern "C" int nrg_dm_rho(nrg_dm* D, int N, double* out, int64_t cap, int* nbloc
     double t = 0.0;
     for (int b = 0; b < s.nblocks(); b++) {
       const int k = s.bsize(b);
-      for (int i = 0; i < k; i++) t += (2.0 * s.qn[2 * b + 1] + 1.0) * r[D->rho_off[N][b] + (int64_t)i * k + i];
+      for (int i = 0; i < k; i++) t += s.deg(b) * r[D->rho_off[N][b] + (int64_t)i * k + i];
     }
     *trace = t;
   }

@@ -70,7 +70,9 @@ struct DmShell {
   std::vector<int64_t> th_off, dz_off;
   int64_t th_tot = 0, dz_tot = 0, op_cnt = 0;
   std::string thbuf, dzbuf, dopbuf;
+  bool sz = false;                     // (Q, Sz) blocks: degeneracy 1, no Clebsch-Gordan factors
   int nblocks() const { return (int)qn.size() / 2; }
+  double deg(int b) const { return sz ? 1.0 : 2.0 * qn[2 * b + 1] + 1.0; }
   int bsize(int b) const { return bbe[2 * b + 1] - bbe[2 * b] + 1; }
   int bsize_bc(int b) const { return bbe_bc[2 * b + 1] - bbe_bc[2 * b] + 1; }
 };

@@ -519,7 +519,7 @@ static double calc_entropy(double bb, const CNRGarray& A) {
 }
 
 // operator update specification
-enum PairRule { PAIR_DIAG = 0, PAIR_QS_UP = 1, PAIR_QS_UP_UPPER = 2 };
+enum PairRule { PAIR_DIAG = 0, PAIR_QS_UP = 1, PAIR_QS_UP_UPPER = 2, PAIR_QSZ_UP = 3 };
 struct OpSpec {
   bool need_old = false;
   bool calc_avg = false;
@@ -544,6 +544,8 @@ struct OpSpec {
 // (src/TwoChQS/TwoChQS_UpdateMatrices.cpp:333)
 static bool pair_ok(PairRule r, const CNRGarray& A, int i, int j) {
   if (r == PAIR_DIAG) return i == j;
+  if (r == PAIR_QSZ_UP)   // d_up on (Q, Sz) blocks: (Q, Sz) -> (Q+1, Sz+1/2)
+    return deq(A.GetQNumber(j, 0), A.GetQNumber(i, 0) + 1.0) && deq(A.GetQNumber(j, 1), A.GetQNumber(i, 1) + 0.5);
   if (r == PAIR_QS_UP_UPPER && j <= i) return false;
   double Qi = A.GetQNumber(i, 0), Si = A.GetQNumber(i, 1), Qj = A.GetQNumber(j, 0), Sj = A.GetQNumber(j, 1);
   return deq(Qj, Qi + 1.0) && (deq(Sj, Si + 0.5) || deq(Sj, Si - 0.5));
@@ -1023,6 +1025,90 @@ static int dm_capture(Chain& ch, int N, int opidx, DmShell& s) {
                           c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   return 0;
+}
+
+// What DM_NRG would read of a TwoChQSz shell with UpdateBefCut 1 (the reference
+// has no such path): every state of Aeig with its Kept flags, all
+// eigenvector rows, Abasis with StCameFrom as indices of the previous shell's
+// states before its cut, and d_up on every pair of Aeig blocks that its rule
+// connects, Z_i F Z_j^T over all rows with F from the previous step's d_up
+// (which lives on the previous AeigCut: call before Chain::update_ops).
+static int dm_capture_all(Chain& ch, int N, int opidx, DmShell& s) {
+  nrg_ctx* c = ch.c;
+  const CNRGarray& A = ch.Aeig;
+  const CNRGbasisarray& Ab = ch.Abasis;
+  int rc;
+  s.N = N;
+  s.qn = A.QNumbers; s.en = A.dEn; s.bbe = A.BlockBegEnd; s.bbe_bc = A.BlockBegEnd;
+  s.kept = A.Kept;
+  s.itype = Ab.iType;
+  s.stcf.resize(Ab.StCameFrom.size());
+  for (size_t i = 0; i < Ab.StCameFrom.size(); i++) s.stcf[i] = ch.AeigCut.StCameFrom[Ab.StCameFrom[i]];
+  s.bas_bbe = Ab.BlockBegEnd; s.bas_qn = Ab.QNumbers;
+  const int nb = A.NumBlocks();
+  s.zoff.assign(nb, 0);
+  int64_t ztot = 0;
+  for (int b = 0; b < nb; b++) { s.zoff[b] = ztot; ztot += (int64_t)A.GetBlockSize(b) * A.GetBlockSize(b); }
+  s.zbuf = "dmZ" + std::to_string(N);
+  DBuf& zb = c->buf(s.zbuf);
+  if ((rc = zb.reserve(sizeof(double) * (size_t)(ztot + 1)))) return rc;
+  for (int b = 0; b < nb; b++)
+    HIPCHK(hipMemcpyAsync((double*)zb.p + s.zoff[b], ch.dZ + ch.Aeig_zoff[b],
+                          sizeof(double) * (size_t)A.GetBlockSize(b) * A.GetBlockSize(b), hipMemcpyDeviceToDevice,
+                          c->stream));
+  // pairs and product-basis F of the operator over every Aeig block pair
+  OpSpec& sp = ch.specs[opidx];
+  std::vector<FPair> pairs;
+  std::vector<double> G;
+  std::vector<RotJob> jobs;
+  s.op_map.clear(); s.op_mbe.clear();
+  int64_t ftot = 0, aux = 0, cnt = 0, maxel = 1;
+  for (int ib = 0; ib < nb; ib++)
+    for (int jb = 0; jb < nb; jb++) {
+      if (!pair_ok(sp.rule, A, ib, jb)) continue;
+      const int ni = A.GetBlockSize(ib), nj = A.GetBlockSize(jb);
+      s.op_map.push_back(ib); s.op_map.push_back(jb);
+      s.op_mbe.push_back((int)cnt);
+      FPair P{};
+      P.ni_bc = ni; P.nj_bc = nj; P.ibeg_bc = A.GetBlockLimit(ib, 0); P.jbeg_bc = A.GetBlockLimit(jb, 0);
+      P.f_off = ftot; P.g_off = (int64_t)G.size(); P.need_old = 1; P.op = opidx;
+      const std::vector<double>& gt = sp.table(A.GetQNumber(ib, 1), A.GetQNumber(jb, 1), ch.ntypes);
+      G.insert(G.end(), gt.begin(), gt.end());
+      pairs.push_back(P);
+      RotJob J{};
+      J.ni = ni; J.nj = nj; J.ki = ni; J.kj = nj; J.ldzi = ni; J.ldzj = nj;
+      J.zi_off = ch.Aeig_zoff[ib]; J.zj_off = ch.Aeig_zoff[jb];
+      J.f_off = ftot; J.aux_off = aux; J.out_off = cnt;
+      jobs.push_back(J);
+      ftot += (int64_t)ni * nj;
+      aux += (int64_t)nj * ni;
+      cnt += (int64_t)ni * nj;
+      s.op_mbe.push_back((int)cnt - 1);
+      maxel = std::max(maxel, (int64_t)ni * nj);
+    }
+  s.opbuf = "dmO" + std::to_string(N);
+  DBuf& ob = c->buf(s.opbuf);
+  if ((rc = ob.reserve(sizeof(double) * (size_t)(cnt + 1)))) return rc;
+  if (!pairs.empty()) {
+    DBuf& fb = c->buf("dmc_F");
+    DBuf& ab = c->buf("dmc_aux");
+    if ((rc = fb.reserve(sizeof(double) * (size_t)(ftot + 1)))) return rc;
+    if ((rc = ab.reserve(sizeof(double) * (size_t)(aux + 1)))) return rc;
+    FPair* d_pairs;
+    double* d_G;
+    int *d_stcf, *d_itype;
+    if ((rc = upload(c, "dmc_pairs", pairs, &d_pairs))) return rc;
+    if ((rc = upload(c, "dmc_G", G, &d_G))) return rc;
+    if ((rc = upload(c, "dmc_stcf", Ab.StCameFrom, &d_stcf))) return rc;
+    if ((rc = upload(c, "dmc_itype", Ab.iType, &d_itype))) return rc;
+    FArgs fa{};
+    fa.F = (double*)fb.p; fa.pairs = d_pairs; fa.G = d_G; fa.stcf = d_stcf; fa.itype = d_itype; fa.ntypes = ch.ntypes;
+    if ((rc = make_oldop(c, ch.mats[opidx], ch.oldb, "dmc_old", fa.old[opidx]))) return rc;
+    HIPCHK(nrgk_launch_fbasis(&fa, (int)pairs.size(), maxel, c->stream));
+    if ((rc = rotate_dev(c, jobs, ch.dZ, (double*)fb.p, (double*)ab.p, (double*)ob.p))) return rc;
+  }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return c->resolve();
 }
 
 // ---------------------------------------------------------------------------
@@ -1732,8 +1818,10 @@ static int run_onechqs_multi(nrg_ctx* c, int ninst, const nrg_onechqs_params* ps
 // TwoChQS Anderson: legacy standalone loop (src/TwoChQS/TwoChQS.cpp:204-495);
 // qsz: TwoChQSz, the same loop on (Q, Sz) blocks (src/TwoChQSz/TwoChQSz.cpp:103-425)
 // ---------------------------------------------------------------------------
+//   dm != nullptr (TwoChQSz): also d_up (operator 3), and every shell captured
+//   with all its states for the DM-NRG post-processing (dm_capture_all)
 static int run_twochqs(nrg_ctx* c, nrg_twochqs_params p, nrg_step_cb cb, void* user, double* timing_ms,
-                       bool qsz = false) {
+                       bool qsz = false, std::vector<DmShell>* dm = nullptr) {
   const double L = p.Lambda;
   const double bbar = p.betabar > 0 ? p.betabar : 0.727;
   const double HLF = 0.5 * (1.0 + 1.0 / L);
@@ -1759,7 +1847,16 @@ static int run_twochqs(nrg_ctx* c, nrg_twochqs_params p, nrg_step_cb cb, void* u
   }
   ch.specs[2].rule = PAIR_QS_UP_UPPER;
   ch.specs[2].g = [](double, double, int, int) { return 0.0; };
-  ch.init_ops(3);
+  const bool with_d = qsz && dm;
+  if (with_d) {   // d_up: impurity only, past the new site's fermions
+    ch.specs.resize(4);
+    ch.specs[3].rule = PAIR_QSZ_UP;
+    ch.specs[3].need_old = true;
+    ch.specs[3].g = [](double, double, int ti, int tj) {
+      return ti == tj ? ((__builtin_popcount((unsigned)ti) & 1) ? -1.0 : 1.0) : 0.0;
+    };
+  }
+  ch.init_ops(with_d ? 4 : 3);
   // src/TwoChQS/TwoChQS.cpp:208-264
   double Ut = 0.5 * p.U / HLF, edt = p.ed / HLF;
   double G1t = p.Gamma1 * (2.0 / M_PI) / (HLF * HLF), G2t = p.Gamma2 * (2.0 / M_PI) / (HLF * HLF);
@@ -1779,6 +1876,8 @@ static int run_twochqs(nrg_ctx* c, nrg_twochqs_params p, nrg_step_cb cb, void* u
     ch.AeigCut.zoff = {0, 1, 2, 3};
     ch.AeigCut.zld = {1, 1, 1, 1};
     if ((rc = ch.set_op(2, {0, 1, 0, 2, 1, 3, 2, 3}, {0, 0, 1, 1, 2, 2, 3, 3}, {1.0, 1.0, -1.0, 1.0}))) return rc;
+    // d_up of the impurity (the up entries of the table above)
+    if (with_d && (rc = ch.set_op(3, {0, 1, 2, 3}, {0, 0, 1, 1}, {1.0, 1.0}))) return rc;
   } else {
     ch.AeigCut.QNumbers = {-1, 0, 0, 0.5, 1, 0};
     ch.AeigCut.BlockBegEnd = {0, 0, 1, 1, 2, 2};
@@ -1830,6 +1929,11 @@ static int run_twochqs(nrg_ctx* c, nrg_twochqs_params p, nrg_step_cb cb, void* u
     ch.Aeig.Kept.assign(ns, 0);
     int nk = 0;
     for (int i = 0; i < ns; i++) { ch.Aeig.Kept[i] = ch.Aeig.dEn[i] <= ec; nk += ch.Aeig.Kept[i]; }
+    if (with_d) {
+      dm->emplace_back();
+      int r = dm_capture_all(ch, ch.Aeig.Nshell, 3, dm->back());
+      if (r) return r;
+    }
     // the cut the next iteration starts with (by-value CutStates, no SetDegen)
     ch.cut_states(p.ncut, true, 0.0);
     int r = ch.update_ops();
@@ -1920,6 +2024,11 @@ static int run_onechqs_jvp(nrg_ctx* c, const nrg_onechqs_params& p, const unsign
 int run_onechqs_dm_jvp(nrg_ctx* c, const nrg_onechqs_params& p, std::vector<DmShell>& dm, const unsigned char* kept,
                        const int64_t* kept_off, int ntan, const double* dpar) {
   return run_onechqs_jvp(c, p, kept, kept_off, ntan, dpar, nullptr, nullptr, &dm);
+}
+
+// the TwoChQSz chain with every shell captured for the DM-NRG post-processing
+int run_twochqsz_dm(nrg_ctx* c, const nrg_twochqs_params& p, std::vector<DmShell>& dm) {
+  return run_twochqs(c, p, nullptr, nullptr, nullptr, true, &dm);
 }
 
 int run_onechqs_dm(nrg_ctx* c, const nrg_onechqs_params& p, std::vector<DmShell>& dm, int befcut) {
