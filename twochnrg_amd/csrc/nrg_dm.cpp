@@ -38,6 +38,7 @@ double cgordan(double J1, double M1, double J2, double M2, double J, double M); 
 
 struct nrg_dm {
   nrg_ctx* c = nullptr;
+  std::string pfx;                             // names of the handle's own buffers in the context
   nrg_dmnrg_params p{};
   std::vector<DmShell> sh;
   int NM = 0, specNM = 0;
@@ -47,15 +48,27 @@ struct nrg_dm {
   std::vector<double> wn;                      // FDM w_N
   std::vector<std::vector<int64_t>> rho_off;   // per shell and block: k x k block in dmR<N>
   std::vector<int64_t> rho_len;
-  // pole arena (filled once, all shells): W, dE and per-shell ranges
-  bool poles_ready = false;
+  // poles (W, dE) of shells [ar_lo, ar_hi) in the arena, at most arena_max at
+  // once; pole_beg/pole_end: every shell's range over all shells (the arena
+  // holds them from ar_base on)
+  bool layout_ready = false;
+  std::vector<std::vector<DmPair>> lay_pairs;
+  std::vector<std::vector<std::pair<int, int>>> lay_blks;
   std::vector<int64_t> pole_beg, pole_end;
+  int ar_lo = -1, ar_hi = -1;
+  int64_t ar_base = 0, arena_max = 0;
   // derivative pass (DM-NRG method): tangents carried through the chain and the
   // sweep (dmdR<N>), pole weights of delta and of its derivative (dm_W1, dm_W2)
   int ntan = 0;
   bool tpoles_ready = false;
   int64_t tpole_tot = 0;
   double ms_forward = 0, ms_sweep = 0, ms_poles = 0, ms_broaden = 0;
+  ~nrg_dm() {   // the handle's buffers go back to the device with it
+    if (!c || pfx.empty()) return;
+    for (auto it = c->bufs.begin(); it != c->bufs.end();)
+      if (it->first.compare(0, pfx.size(), pfx) == 0) it = c->bufs.erase(it);
+      else ++it;
+  }
 };
 
 namespace {
@@ -155,7 +168,7 @@ int sweep_shell(nrg_dm* D, int N, bool fdm, double TempBar, double zdisc) {
   DBuf& rb = c->buf("dm_rot");
   if ((rc = tb.reserve(sizeof(double) * (size_t)(ttot + 1)))) return rc;
   if ((rc = rb.reserve(sizeof(double) * (size_t)(rtot + 1)))) return rc;
-  double* dR1 = (double*)c->buf("dmR" + std::to_string(N + 1)).p;
+  double* dR1 = (double*)c->buf(D->pfx + "R" + std::to_string(N + 1)).p;
   double* dZ1 = (double*)c->buf(b.zbuf).p;
   if ((rc = gemm_groups(c, "dm_g1", g1, dR1, dZ1, (double*)tb.p))) return rc;
   if ((rc = gemm_groups(c, "dm_g2", g2, dZ1, (double*)tb.p, (double*)rb.p))) return rc;
@@ -226,8 +239,8 @@ int sweep_shell(nrg_dm* D, int N, bool fdm, double TempBar, double zdisc) {
     gb.push_back(G);
   }
   D->rho_len[N] = otot;
-  DBuf& ob = c->buf("dmR" + std::to_string(N));
-  if ((rc = ob.reserve(sizeof(double) * (size_t)(otot + 1)))) return rc;
+  DBuf& ob = c->buf(D->pfx + "R" + std::to_string(N));
+  if ((rc = ob.reserve(sizeof(double) * (size_t)(otot + 1), true))) return rc;
   DmGBlk* dgb;
   DmSeg* dsg;
   double* den;
@@ -256,11 +269,11 @@ int sweep_shell(nrg_dm* D, int N, bool fdm, double TempBar, double zdisc) {
     }
     DBuf& ttb = c->buf("dm_tT");
     DBuf& trb = c->buf("dm_trot");
-    DBuf& dob = c->buf("dmdR" + std::to_string(N));
+    DBuf& dob = c->buf(D->pfx + "dR" + std::to_string(N));
     if ((rc = ttb.reserve(sizeof(double) * (size_t)(T * ttot + 1)))) return rc;
     if ((rc = trb.reserve(sizeof(double) * (size_t)(T * rtot + 1)))) return rc;
-    if ((rc = dob.reserve(sizeof(double) * (size_t)(T * otot + 1)))) return rc;
-    double* dR1t = dbuf(c, "dmdR" + std::to_string(N + 1));
+    if ((rc = dob.reserve(sizeof(double) * (size_t)(T * otot + 1), true))) return rc;
+    double* dR1t = dbuf(c, D->pfx + "dR" + std::to_string(N + 1));
     double* dZ1t = dbuf(c, b.dzbuf);
     if ((rc = gemm_groups(c, "dm_h1", h1, dR1t, dZ1, (double*)ttb.p))) return rc;
     if ((rc = gemm_groups(c, "dm_h2", h2, dR1, dZ1t, (double*)ttb.p))) return rc;
@@ -287,14 +300,13 @@ double op_cgfactor(double S1, double S2) {
 // 0 .. specNM-1 (omega independent).  CFS/FDM: a block pair with no kept state
 // on either side, or no discarded state on either side, has no poles
 // (src/SpecFunc_allSTL.cpp:1375-1381, :824-830).
-int build_poles(nrg_dm* D) {
-  nrg_ctx* c = D->c;
+int pole_layout(nrg_dm* D) {
+  if (D->layout_ready) return NRG_OK;
   int rc;
   const int ns = D->specNM;   // shells with spectral weight: 0 .. specNM-1
   const double bb = (std::fabs(D->temp) < 1e-20) ? 1.0e10 : D->betabar_spec;
-  // per shell: the pairs that carry poles, with their kept prefixes
-  std::vector<std::vector<DmPair>> shell_pairs(ns);
-  std::vector<std::vector<std::pair<int, int>>> shell_blks(ns);   // (b1, b2) of each pair
+  D->lay_pairs.assign(ns, std::vector<DmPair>());
+  D->lay_blks.assign(ns, std::vector<std::pair<int, int>>());
   D->pole_beg.assign(ns, 0);
   D->pole_end.assign(ns, 0);
   int64_t ptot = 0;
@@ -322,20 +334,41 @@ int build_poles(nrg_dm* D) {
       }
       P.pole_off = ptot;
       ptot += (int64_t)k1 * k2;
-      shell_pairs[N].push_back(P);
-      shell_blks[N].emplace_back(b1, b2);
+      D->lay_pairs[N].push_back(P);
+      D->lay_blks[N].emplace_back(b1, b2);
     }
     D->pole_end[N] = ptot;
   }
-  DBuf& wpb = c->buf("dm_Wp");
-  DBuf& wnb = c->buf("dm_Wn");
-  DBuf& eb = c->buf("dm_dE");
-  if ((rc = wpb.reserve(sizeof(double) * (size_t)(ptot + 1)))) return rc;
-  if ((rc = wnb.reserve(sizeof(double) * (size_t)(ptot + 1)))) return rc;
-  if ((rc = eb.reserve(sizeof(double) * (size_t)(ptot + 1)))) return rc;
-  for (int N = 0; N < ns; N++) {
+  // the arena holds at most this many poles (three doubles each) at once: 40%
+  // of the device memory free now, which leaves the per-shell products room
+  size_t fr = 0, tot = 0;
+  HIPCHK(hipMemGetInfo(&fr, &tot));
+  D->arena_max = std::max<int64_t>((int64_t)(0.4 * (double)fr / 24.0), 1 << 20);
+  D->layout_ready = true;
+  return NRG_OK;
+}
+
+// pole weights and energies of shells N0 .. N1-1 into the arena (offsets from
+// the first pole of N0).  Costi at the last shell; CFS/FDM: a block pair with
+// no kept state on either side, or no discarded state on either side, has no
+// poles (src/SpecFunc_allSTL.cpp:1375-1381, :824-830).
+int build_pole_range(nrg_dm* D, int N0, int N1) {
+  nrg_ctx* c = D->c;
+  int rc;
+  if ((rc = pole_layout(D))) return rc;
+  if (D->ar_lo == N0 && D->ar_hi == N1) return NRG_OK;
+  auto t0 = std::chrono::steady_clock::now();
+  const int64_t base = D->pole_beg[N0], np = D->pole_end[N1 - 1] - base;
+  DBuf& wpb = c->buf(D->pfx + "Wp");
+  DBuf& wnb = c->buf(D->pfx + "Wn");
+  DBuf& eb = c->buf(D->pfx + "dE");
+  D->ar_lo = D->ar_hi = -1;
+  if ((rc = wpb.reserve(sizeof(double) * (size_t)(np + 1), true))) return rc;
+  if ((rc = wnb.reserve(sizeof(double) * (size_t)(np + 1), true))) return rc;
+  if ((rc = eb.reserve(sizeof(double) * (size_t)(np + 1), true))) return rc;
+  for (int N = N0; N < N1; N++) {
     const DmShell& s = D->sh[N];
-    std::vector<DmPair>& pairs = shell_pairs[N];
+    std::vector<DmPair> pairs = D->lay_pairs[N];
     if (pairs.empty()) continue;
     const bool costi = (N == D->specNM - 1);
     std::vector<E2Gemm> g1, g2;
@@ -346,8 +379,9 @@ int build_poles(nrg_dm* D) {
     // sub-blocks are sub-blocks of these
     for (size_t q = 0; q < pairs.size(); q++) {
       DmPair& P = pairs[q];
+      P.pole_off -= base;
       if (!costi) {
-        const int b1 = shell_blks[N][q].first, b2 = shell_blks[N][q].second;
+        const int b1 = D->lay_blks[N][q].first, b2 = D->lay_blks[N][q].second;
         P.p1_off = prod;
         P.p2_off = prod + (int64_t)P.k1 * P.k2;
         prod += 2 * (int64_t)P.k1 * P.k2;
@@ -360,7 +394,7 @@ int build_poles(nrg_dm* D) {
     if ((rc = pb.reserve(sizeof(double) * (size_t)(prod + 1)))) return rc;
     double* dO = (double*)c->buf(s.opbuf).p;
     if (!costi) {
-      double* dR = (double*)c->buf("dmR" + std::to_string(N)).p;
+      double* dR = (double*)c->buf(D->pfx + "R" + std::to_string(N)).p;
       if ((rc = gemm_groups(c, "dm_p1", g1, dR, dO, (double*)pb.p))) return rc;
       if ((rc = gemm_groups(c, "dm_p2", g2, dO, dR, (double*)pb.p))) return rc;
     }
@@ -372,8 +406,21 @@ int build_poles(nrg_dm* D) {
                              (double*)wnb.p, (double*)eb.p, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
   }
-  D->poles_ready = true;
+  D->ar_lo = N0;
+  D->ar_hi = N1;
+  D->ar_base = base;
+  D->ms_poles += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return NRG_OK;
+}
+
+// every shell's poles at once (the derivative pass reads them all)
+int build_poles(nrg_dm* D) {
+  int rc;
+  if ((rc = pole_layout(D))) return rc;
+  const int ns = D->specNM;
+  if (D->pole_end[ns - 1] > D->arena_max)
+    return fail(NRG_ERR_HIP, "DM-NRG: the poles of all shells do not fit in device memory");
+  return build_pole_range(D, 0, ns);
 }
 
 // Derivative pole weights of every shell (DM-NRG method; the last shell in the
@@ -393,8 +440,8 @@ int build_poles_jvp(nrg_dm* D) {
     for (size_t q = 0; q + 1 < D->sh[N].op_map.size(); q += 2)
       ptot += (int64_t)D->sh[N].bsize(D->sh[N].op_map[q]) * D->sh[N].bsize(D->sh[N].op_map[q + 1]);
   if (ptot != D->pole_end[ns - 1]) return fail(NRG_ERR_STATE, "DM-NRG derivative: pole layout differs");
-  DBuf& w1b = c->buf("dm_W1");
-  DBuf& w2b = c->buf("dm_W2");
+  DBuf& w1b = c->buf(D->pfx + "W1");
+  DBuf& w2b = c->buf(D->pfx + "W2");
   if ((rc = w1b.reserve(sizeof(double) * (size_t)(T * ptot + 1)))) return rc;
   if ((rc = w2b.reserve(sizeof(double) * (size_t)(T * ptot + 1)))) return rc;
   int64_t pole = 0;
@@ -465,8 +512,8 @@ int build_poles_jvp(nrg_dm* D) {
       HIPCHK(hipMemcpyAsync(drb.p, dr.data(), sizeof(double) * (size_t)(T * btot), hipMemcpyHostToDevice, c->stream));
       HIPCHK(hipStreamSynchronize(c->stream));
     } else {
-      R = dbuf(c, "dmR" + std::to_string(N));
-      HIPCHK(hipMemcpyAsync(drb.p, dbuf(c, "dmdR" + std::to_string(N)), sizeof(double) * (size_t)(T * btot),
+      R = dbuf(c, D->pfx + "R" + std::to_string(N));
+      HIPCHK(hipMemcpyAsync(drb.p, dbuf(c, D->pfx + "dR" + std::to_string(N)), sizeof(double) * (size_t)(T * btot),
                             hipMemcpyDeviceToDevice, c->stream));
       std::vector<E2Gemm> g1, g2;   // drho += M rho - rho M
       for (int b = 0; b < nb; b++) {
@@ -558,51 +605,112 @@ int bd_kind(const nrg_dm* D) {   // BDelta / BDeltaTemp by method (src/DM_NRG/DM
 int spec_tasks(nrg_dm* D, int n, const int* shells, const double* w, double* out, bool norm = false) {
   nrg_ctx* c = D->c;
   int rc;
-  if (!D->poles_ready) {
-    auto t0 = std::chrono::steady_clock::now();
-    if ((rc = build_poles(D))) return rc;
-    D->ms_poles += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-  }
-  auto t0 = std::chrono::steady_clock::now();
-  const int BCH = 256 * 8;   // poles per chunk, = BCH of the broadening kernel
-  std::vector<DmTask> tasks;
-  std::vector<int> slot;
-  int cb = 0, maxchunk = 0;
+  if ((rc = pole_layout(D))) return rc;
+  const int ns = D->specNM;
+  // the (shell, omegabar) sums by shell and weight array: tasks[N][sel] = slots
+  struct Task { double w; int slot; };
+  std::vector<std::vector<Task>> tk[2];
+  tk[0].assign(ns, std::vector<Task>());
+  tk[1].assign(ns, std::vector<Task>());
   for (int i = 0; i < n; i++) {
-    const int N = shells[i];
     out[i] = 0.0;
-    if (N < 0 || N > D->specNM - 1) continue;
-    const int nsign = (norm && N < D->specNM - 1) ? 2 : 1;
-    for (int sg = 0; sg < nsign; sg++) {
-      DmTask T{};
-      T.pole_beg = D->pole_beg[N];
-      T.pole_end = D->pole_end[N];
-      T.w = norm ? (sg == 0 ? 1.0 : -1.0) : w[i];
-      T.twin = D->p.twindow * D->temp / calc_dn(D->p.Lambda, N);
-      T.norm = norm ? 1 : 0;
-      T.nchunk = (int)((T.pole_end - T.pole_beg + BCH - 1) / BCH);
-      T.chunk_beg = cb;
-      cb += T.nchunk;
-      maxchunk = std::max(maxchunk, T.nchunk);
-      tasks.push_back(T);
-      slot.push_back(i);
+    const int N = shells[i];
+    if (N < 0 || N > ns - 1) continue;
+    if (norm) {   // CalcNorm: Wp + Wn below the last shell, the Costi sum once at the last
+      tk[0][N].push_back(Task{1.0, i});
+      if (N < ns - 1) tk[1][N].push_back(Task{-1.0, i});
+    } else {
+      tk[w[i] >= 0.0 ? 0 : 1][N].push_back(Task{w[i], i});
     }
   }
-  if (tasks.empty()) return NRG_OK;
-  DBuf& pb = c->buf("dm_partial");
-  DBuf& ob = c->buf("dm_out");
-  if ((rc = pb.reserve(sizeof(double) * (size_t)(cb + 1)))) return rc;
-  if ((rc = ob.reserve(sizeof(double) * (size_t)(tasks.size() + 1)))) return rc;
-  DmTask* dt;
-  if ((rc = upload(c, "dm_tasks", tasks, &dt))) return rc;
-  HIPCHK(dm_launch_broaden(dt, (int)tasks.size(), maxchunk, (double*)c->buf("dm_Wp").p, (double*)c->buf("dm_Wn").p,
-                           (double*)c->buf("dm_dE").p, bd_kind(D), D->p.bbroad, D->p.broadtemp, (double*)pb.p,
-                           (double*)ob.p, c->stream));
-  std::vector<double> res(tasks.size());
-  HIPCHK(hipMemcpyAsync(res.data(), ob.p, sizeof(double) * tasks.size(), hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(hipStreamSynchronize(c->stream));
-  for (size_t q = 0; q < tasks.size(); q++) out[slot[q]] += res[q];
-  D->ms_broaden += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  // shell groups whose poles fit the arena together, in shell order; the
+  // arena's current range first when it covers every shell asked for
+  std::vector<std::pair<int, int>> groups;
+  {
+    int lo = ns, hi = -1;
+    for (int N = 0; N < ns; N++)
+      if (!tk[0][N].empty() || !tk[1][N].empty()) { lo = std::min(lo, N); hi = std::max(hi, N); }
+    if (hi < 0) return NRG_OK;
+    if (D->ar_lo >= 0 && D->ar_lo <= lo && hi < D->ar_hi) {
+      groups.emplace_back(D->ar_lo, D->ar_hi);
+    } else if (D->pole_end[ns - 1] <= D->arena_max) {
+      groups.emplace_back(0, ns);
+    } else {
+      int g0 = lo;
+      while (g0 <= hi) {
+        int g1 = g0 + 1;
+        if (D->pole_end[g0] - D->pole_beg[g0] > D->arena_max)
+          return fail(NRG_ERR_HIP, "DM-NRG: the poles of shell " + std::to_string(g0) + " do not fit in device memory");
+        while (g1 <= hi && D->pole_end[g1] - D->pole_beg[g0] <= D->arena_max) g1++;
+        groups.emplace_back(g0, g1);
+        g0 = g1;
+      }
+    }
+  }
+  for (const auto& g : groups) {
+    std::vector<DmTile> tiles;
+    std::vector<double> wl;
+    std::vector<int> slot;
+    int64_t part = 0;
+    int maxslab = 0;
+    for (int N = g.first; N < g.second; N++)
+      for (int sel = 0; sel < 2; sel++) {
+        const std::vector<Task>& L = tk[sel][N];
+        if (L.empty()) continue;
+        const int64_t np = D->pole_end[N] - D->pole_beg[N];
+        // slabs of at least 64 poles per lane and at most 1024 per shell
+        int64_t nslab = std::max<int64_t>(1, std::min<int64_t>((np + 16383) / 16384, 1024));
+        int64_t len = (np + nslab - 1) / nslab;
+        len = std::max<int64_t>(256, (len + 255) / 256 * 256);
+        nslab = std::max<int64_t>(1, (np + len - 1) / len);
+        const int step = norm ? 1 : DM_TILE_W;
+        for (size_t q0 = 0; q0 < L.size(); q0 += step) {
+          DmTile T{};
+          T.pole_beg = D->pole_beg[N] - D->pole_beg[g.first];
+          T.pole_end = D->pole_end[N] - D->pole_beg[g.first];
+          T.slab_len = len;
+          T.nslab = (int)nslab;
+          T.task_beg = (int)wl.size();
+          T.ntask = (int)std::min<size_t>(step, L.size() - q0);
+          T.part_beg = (int)part;
+          T.wsel = sel;
+          T.norm = norm ? 1 : 0;
+          T.twin = D->p.twindow * D->temp / calc_dn(D->p.Lambda, N);
+          bool pos = true, neg = true;
+          for (int j = 0; j < T.ntask; j++) {
+            const double x = L[q0 + j].w;
+            wl.push_back(x);
+            slot.push_back(L[q0 + j].slot);
+            pos = pos && x > 0.0;
+            neg = neg && x < 0.0;
+          }
+          T.sign = norm ? 0 : (pos ? 1 : (neg ? -1 : 0));
+          part += (int64_t)T.ntask * nslab;
+          if (part > (int64_t)1 << 30) return fail(NRG_ERR_ARG, "DM-NRG: too many omegas in one call");
+          maxslab = std::max(maxslab, T.nslab);
+          tiles.push_back(T);
+        }
+      }
+    if (tiles.empty()) continue;
+    if ((rc = build_pole_range(D, g.first, g.second))) return rc;
+    auto t0 = std::chrono::steady_clock::now();
+    DBuf& pb = c->buf("dm_partial");
+    DBuf& ob = c->buf("dm_out");
+    if ((rc = pb.reserve(sizeof(double) * (size_t)(part + 1)))) return rc;
+    if ((rc = ob.reserve(sizeof(double) * (wl.size() + 1)))) return rc;
+    DmTile* dt;
+    double* dw;
+    if ((rc = upload(c, "dm_tiles", tiles, &dt))) return rc;
+    if ((rc = upload(c, "dm_wl", wl, &dw))) return rc;
+    HIPCHK(dm_launch_broaden_tiles(dt, (int)tiles.size(), maxslab, dw, dbuf(c, D->pfx + "Wp"),
+                                   dbuf(c, D->pfx + "Wn"), dbuf(c, D->pfx + "dE"), bd_kind(D), D->p.bbroad,
+                                   D->p.broadtemp, (double*)pb.p, (double*)ob.p, c->stream));
+    std::vector<double> res(wl.size());
+    HIPCHK(hipMemcpyAsync(res.data(), ob.p, sizeof(double) * res.size(), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (size_t q = 0; q < res.size(); q++) out[slot[q]] += res[q];
+    D->ms_broaden += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  }
   return NRG_OK;
 }
 
@@ -611,11 +719,7 @@ int spec_tasks_jvp(nrg_dm* D, int n, const int* shells, const double* w, double*
   nrg_ctx* c = D->c;
   int rc;
   const int T = D->ntan;
-  if (!D->poles_ready) {
-    auto t0 = std::chrono::steady_clock::now();
-    if ((rc = build_poles(D))) return rc;
-    D->ms_poles += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-  }
+  if ((rc = build_poles(D))) return rc;
   if (!D->tpoles_ready) {
     auto t0 = std::chrono::steady_clock::now();
     if ((rc = build_poles_jvp(D))) return rc;
@@ -650,8 +754,8 @@ int spec_tasks_jvp(nrg_dm* D, int n, const int* shells, const double* w, double*
   if ((rc = ob.reserve(sizeof(double) * (size_t)(T * nt + 1)))) return rc;
   DmTask* dt;
   if ((rc = upload(c, "dmt_tasks", tasks, &dt))) return rc;
-  HIPCHK(dm_launch_broaden_jvp(dt, nt, maxchunk, T, dbuf(c, "dm_W1"), dbuf(c, "dm_W2"), D->tpole_tot,
-                               dbuf(c, "dm_dE"), D->p.bbroad, D->p.broadtemp, (double*)pb.p, cb, (double*)ob.p,
+  HIPCHK(dm_launch_broaden_jvp(dt, nt, maxchunk, T, dbuf(c, D->pfx + "W1"), dbuf(c, D->pfx + "W2"), D->tpole_tot,
+                               dbuf(c, D->pfx + "dE"), D->p.bbroad, D->p.broadtemp, (double*)pb.p, cb, (double*)ob.p,
                                c->stream));
   std::vector<double> res((size_t)T * nt);
   HIPCHK(hipMemcpyAsync(res.data(), ob.p, sizeof(double) * res.size(), hipMemcpyDeviceToHost, c->stream));
@@ -724,6 +828,7 @@ static int dm_create(nrg_ctx* c, const nrg_dmnrg_params* p, const unsigned char*
   HIPCHK(hipSetDevice(c->device));
   nrg_dm* D = new nrg_dm();
   D->c = c;
+  D->pfx = "dm#" + std::to_string(c->dm_seq++) + ":";
   D->p = *p;
   D->method = p->method;
   const bool fdm = (p->method == 2);
@@ -751,6 +856,13 @@ static int dm_create(nrg_ctx* c, const nrg_dmnrg_params* p, const unsigned char*
     return rc;
   }
   auto t1 = std::chrono::steady_clock::now();
+  // the shells' buffers become the handle's own
+  for (DmShell& s : D->sh)
+    for (std::string* nm : {&s.zbuf, &s.opbuf, &s.thbuf, &s.dzbuf, &s.dopbuf})
+      if (!nm->empty()) {
+        c->rename(*nm, D->pfx + *nm);
+        *nm = D->pfx + *nm;
+      }
   // temperature from Mtemp (src/DM_NRG/DM_NRG.cpp:102-118); FDM keeps every shell (:105)
   // the TwoChQSz loop diagonalises N = 0 .. nsitesmax: one shell more than NRG_main's sites
   int Nsitesmax = (p->model == 1) ? (int)D->sh.size() : p->nsitesmax;
@@ -805,8 +917,8 @@ static int dm_create(nrg_ctx* c, const nrg_dmnrg_params* p, const unsigned char*
         for (int j = 0; j < k; j++) r.push_back(i == j ? std::exp(-bb * s.en[i0 + i]) / Z : 0.0);
     }
     D->rho_len[NM] = (int64_t)r.size();
-    DBuf& rb = c->buf("dmR" + std::to_string(NM));
-    if ((rc = rb.reserve(sizeof(double) * (r.size() + 1)))) { delete D; return rc; }
+    DBuf& rb = c->buf(D->pfx + "R" + std::to_string(NM));
+    if ((rc = rb.reserve(sizeof(double) * (r.size() + 1), true))) { delete D; return rc; }
     HIPCHK(hipMemcpyAsync(rb.p, r.data(), sizeof(double) * r.size(), hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     if (ntan > 0) {
@@ -837,8 +949,8 @@ static int dm_create(nrg_ctx* c, const nrg_dmnrg_params* p, const unsigned char*
             }
         }
       }
-      DBuf& drb = c->buf("dmdR" + std::to_string(NM));
-      if ((rc = drb.reserve(sizeof(double) * dr.size()))) { delete D; return rc; }
+      DBuf& drb = c->buf(D->pfx + "dR" + std::to_string(NM));
+      if ((rc = drb.reserve(sizeof(double) * dr.size(), true))) { delete D; return rc; }
       HIPCHK(hipMemcpyAsync(drb.p, dr.data(), sizeof(double) * dr.size(), hipMemcpyHostToDevice, c->stream));
       HIPCHK(hipStreamSynchronize(c->stream));
     }
@@ -849,6 +961,16 @@ static int dm_create(nrg_ctx* c, const nrg_dmnrg_params* p, const unsigned char*
       delete D;
       return rc;
     }
+  // the eigenvectors (and their tangents) served the sweep only; the chain's
+  // work buffers are released too, which leaves the context holding the
+  // DM-NRG handles' own buffers (rho_N, the operators, energies' tangents)
+  for (DmShell& s : D->sh) {
+    if (!s.zbuf.empty()) c->drop(s.zbuf);
+    if (!s.dzbuf.empty()) c->drop(s.dzbuf);
+    s.zbuf.clear();
+    s.dzbuf.clear();
+  }
+  c->trim("dm#");
   auto t2 = std::chrono::steady_clock::now();
   D->ms_forward = std::chrono::duration<double, std::milli>(t1 - t0).count();
   D->ms_sweep = std::chrono::duration<double, std::milli>(t2 - t1).count();
@@ -894,7 +1016,7 @@ extern "C" int nrg_dm_rho(nrg_dm* D, int N, double* out, int64_t cap, int* nbloc
   if (nblocks) *nblocks = s.nblocks();
   const int64_t len = D->rho_len[N];
   std::vector<double> r(len);
-  HIPCHK(hipMemcpyAsync(r.data(), D->c->buf("dmR" + std::to_string(N)).p, sizeof(double) * len,
+  HIPCHK(hipMemcpyAsync(r.data(), D->c->buf(D->pfx + "R" + std::to_string(N)).p, sizeof(double) * len,
                         hipMemcpyDeviceToHost, D->c->stream));
   HIPCHK(hipStreamSynchronize(D->c->stream));
   if (trace) {

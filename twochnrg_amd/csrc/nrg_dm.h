@@ -52,6 +52,22 @@ struct DmTask {
   int pad;
 };
 
+// up to DM_TILE_W (shell, omega) sums that read the same poles: one shell, one
+// weight array (Wp for omega >= 0, Wn below); the shell's poles are cut into
+// nslab slabs of slab_len, one workgroup per (slab, tile)
+#define DM_TILE_W 16
+struct DmTile {
+  int64_t pole_beg, pole_end;   // the shell's poles in the arena
+  int64_t slab_len;
+  int task_beg, ntask;          // omegas task_beg .. task_beg + ntask - 1
+  int nslab, part_beg;          // partial sums at part_beg + j * nslab + slab
+  int wsel;                     // 0: Wp, 1: Wn
+  int norm;                     // CalcNorm: the broadening is replaced by 1 (ntask 1)
+  int sign;                     // +1 / -1: every omega of the tile > 0 / < 0; 0 otherwise
+  int pad;
+  double twin;                  // BDeltaTemp window |dE| < twindow*T/D_N
+};
+
 struct DmShell {
   int N = 0;
   std::vector<double> qn, en;          // Acut_N: (Q,S) per block, energies of its states
@@ -109,4 +125,8 @@ hipError_t dm_launch_weights(const DmPair* pairs, int npair, int kkmax, const do
 hipError_t dm_launch_broaden(const DmTask* tasks, int ntask, int maxchunk, const double* Wp, const double* Wn,
                              const double* dE, int bd, double bbroad, double btemp, double* partial, double* out,
                              hipStream_t s);
+// the shell sums of ntile tiles: out[task_beg + j] for the omegas w[task_beg + j]
+hipError_t dm_launch_broaden_tiles(const DmTile* tiles, int ntile, int maxslab, const double* w, const double* Wp,
+                                   const double* Wn, const double* dE, int bd, double bbroad, double btemp,
+                                   double* partial, double* out, hipStream_t s);
 }

@@ -131,6 +131,71 @@ __global__ __launch_bounds__(BT) void broaden_kernel(const DmTask* __restrict__ 
   if (threadIdx.x == 0) partial[T.chunk_beg + c] = (red[0] + red[1]) + (red[2] + red[3]);
 }
 
+// the shell sums of a tile of omegas over one slab of the shell's poles: each
+// pole is read once and broadened at every omega of the tile (the omegas of
+// one shell share its poles, so the pole traffic falls by the tile width).
+// A pole on the other side of zero than every omega of the tile, outside the
+// BDeltaTemp window, contributes the exact 0 of bdelta's sign test and is
+// skipped.
+__global__ __launch_bounds__(BT) void broaden_tiles_kernel(const DmTile* __restrict__ tiles,
+                                                           const double* __restrict__ wlist,
+                                                           const double* __restrict__ Wp,
+                                                           const double* __restrict__ Wn,
+                                                           const double* __restrict__ dE, int bd, double bbroad,
+                                                           double btemp, double* __restrict__ partial) {
+  __shared__ double red[BT / 64][DM_TILE_W];
+  const DmTile T = tiles[blockIdx.y];
+  const int slab = blockIdx.x;
+  if (slab >= T.nslab) return;
+  const int64_t b0 = T.pole_beg + (int64_t)slab * T.slab_len;
+  const int64_t b1 = min(b0 + T.slab_len, T.pole_end);
+  const double* __restrict__ W = T.wsel ? Wn : Wp;
+  double w[DM_TILE_W], acc[DM_TILE_W];
+#pragma unroll
+  for (int j = 0; j < DM_TILE_W; j++) {
+    w[j] = (j < T.ntask) ? wlist[T.task_beg + j] : 0.0;
+    acc[j] = 0.0;
+  }
+  if (T.norm) {
+    for (int64_t p = b0 + threadIdx.x; p < b1; p += BT) acc[0] += W[p];
+  } else {
+    for (int64_t p = b0 + threadIdx.x; p < b1; p += BT) {
+      const double wt = W[p];
+      if (wt == 0.0) continue;
+      const double x = dE[p];
+      if (!(fabs(x) < T.twin) && ((T.sign > 0 && x < 0.0) || (T.sign < 0 && x >= 0.0))) continue;
+#pragma unroll
+      for (int j = 0; j < DM_TILE_W; j++)
+        if (j < T.ntask) acc[j] += wt * bdelta(bd, w[j], x, T.twin, bbroad, btemp);
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < DM_TILE_W; j++) {
+    double v = acc[j];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6][j] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < T.ntask) {
+    const int j = threadIdx.x;
+    partial[T.part_beg + (int64_t)j * T.nslab + slab] = (red[0][j] + red[1][j]) + (red[2][j] + red[3][j]);
+  }
+}
+
+// per tile omega: sum of its slab partials in slab order
+__global__ void reduce_tiles_kernel(const DmTile* __restrict__ tiles, int ntile, const double* __restrict__ partial,
+                                    double* __restrict__ out) {
+  const int t = blockIdx.x;
+  if (t >= ntile) return;
+  const DmTile T = tiles[t];
+  for (int j = threadIdx.x; j < T.ntask; j += blockDim.x) {
+    double s = 0.0;
+    for (int c = 0; c < T.nslab; c++) s += partial[T.part_beg + (int64_t)j * T.nslab + c];
+    out[T.task_beg + j] = s;
+  }
+}
+
 // per task: sum of its chunk partials in chunk order
 __global__ void reduce_kernel(const DmTask* __restrict__ tasks, int ntask, const double* __restrict__ partial,
                               double* __restrict__ out) {
@@ -301,6 +366,20 @@ hipError_t dm_launch_broaden(const DmTask* tasks, int ntask, int maxchunk, const
     if (e != hipSuccess) return e;
   }
   hipLaunchKernelGGL(dmk::reduce_kernel, dim3((ntask + 255) / 256), dim3(256), 0, s, tasks, ntask, partial, out);
+  return hipGetLastError();
+}
+
+hipError_t dm_launch_broaden_tiles(const DmTile* tiles, int ntile, int maxslab, const double* w, const double* Wp,
+                                   const double* Wn, const double* dE, int bd, double bbroad, double btemp,
+                                   double* partial, double* out, hipStream_t s) {
+  if (ntile <= 0) return hipSuccess;
+  if (maxslab > 0) {
+    hipLaunchKernelGGL(dmk::broaden_tiles_kernel, dim3(maxslab, ntile), dim3(BT), 0, s, tiles, w, Wp, Wn, dE, bd,
+                       bbroad, btemp, partial);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(dmk::reduce_tiles_kernel, dim3(ntile), dim3(64), 0, s, tiles, ntile, partial, out);
   return hipGetLastError();
 }
 

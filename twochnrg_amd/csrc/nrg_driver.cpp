@@ -1010,7 +1010,7 @@ static int dm_capture(Chain& ch, int N, int opidx, DmShell& s) {
   for (int b = 0; b < nb; b++) { s.zoff[b] = tot; tot += (int64_t)A.GetBlockSize(b) * A.zld[b]; }
   s.zbuf = "dmZ" + std::to_string(N);
   DBuf& zb = c->buf(s.zbuf);
-  if ((rc = zb.reserve(sizeof(double) * (size_t)(tot + 1)))) return rc;
+  if ((rc = zb.reserve(sizeof(double) * (size_t)(tot + 1), true))) return rc;
   for (int b = 0; b < nb; b++)   // kept rows = the first k rows of the block's n x n slice
     HIPCHK(hipMemcpyAsync((double*)zb.p + s.zoff[b], ch.dZ + A.zoff[b],
                           sizeof(double) * (size_t)A.GetBlockSize(b) * A.zld[b], hipMemcpyDeviceToDevice, c->stream));
@@ -1019,7 +1019,7 @@ static int dm_capture(Chain& ch, int N, int opidx, DmShell& s) {
   const int64_t nel = M.MatBlockBegEnd.empty() ? 0 : (int64_t)M.MatBlockBegEnd.back() + 1;
   s.opbuf = "dmO" + std::to_string(N);
   DBuf& ob = c->buf(s.opbuf);
-  if ((rc = ob.reserve(sizeof(double) * (size_t)(nel + 1)))) return rc;
+  if ((rc = ob.reserve(sizeof(double) * (size_t)(nel + 1), true))) return rc;
   if (nel)
     HIPCHK(hipMemcpyAsync(ob.p, c->buf(M.buf[M.cur]).p, sizeof(double) * (size_t)nel, hipMemcpyDeviceToDevice,
                           c->stream));
@@ -1051,7 +1051,7 @@ static int dm_capture_all(Chain& ch, int N, int opidx, DmShell& s) {
   for (int b = 0; b < nb; b++) { s.zoff[b] = ztot; ztot += (int64_t)A.GetBlockSize(b) * A.GetBlockSize(b); }
   s.zbuf = "dmZ" + std::to_string(N);
   DBuf& zb = c->buf(s.zbuf);
-  if ((rc = zb.reserve(sizeof(double) * (size_t)(ztot + 1)))) return rc;
+  if ((rc = zb.reserve(sizeof(double) * (size_t)(ztot + 1), true))) return rc;
   for (int b = 0; b < nb; b++)
     HIPCHK(hipMemcpyAsync((double*)zb.p + s.zoff[b], ch.dZ + ch.Aeig_zoff[b],
                           sizeof(double) * (size_t)A.GetBlockSize(b) * A.GetBlockSize(b), hipMemcpyDeviceToDevice,
@@ -1088,7 +1088,7 @@ static int dm_capture_all(Chain& ch, int N, int opidx, DmShell& s) {
     }
   s.opbuf = "dmO" + std::to_string(N);
   DBuf& ob = c->buf(s.opbuf);
-  if ((rc = ob.reserve(sizeof(double) * (size_t)(cnt + 1)))) return rc;
+  if ((rc = ob.reserve(sizeof(double) * (size_t)(cnt + 1), true))) return rc;
   if (!pairs.empty()) {
     DBuf& fb = c->buf("dmc_F");
     DBuf& ab = c->buf("dmc_aux");
@@ -1535,7 +1535,7 @@ struct Tangent {
         {s.dopbuf, {buf(opb(cur)) + op_base[opidx], T * op_cnt[opidx]}}};
     for (const auto& x : cp) {
       DBuf& b = c->buf(x.first);
-      int rc = b.reserve(sizeof(double) * (size_t)(x.second.second + 1));
+      int rc = b.reserve(sizeof(double) * (size_t)(x.second.second + 1), true);
       if (rc) return rc;
       if (x.second.second > 0)
         HIPCHK(hipMemcpyAsync(b.p, x.second.first, sizeof(double) * (size_t)x.second.second, hipMemcpyDeviceToDevice,

@@ -121,12 +121,12 @@ extern "C" void nrg_ctx_destroy(nrg_ctx* c) {
 DBuf::~DBuf() {
   if (p) hipFree(p);
 }
-int DBuf::reserve(size_t bytes) {
+int DBuf::reserve(size_t bytes, bool exact) {
   if (bytes <= cap) return NRG_OK;
   if (p) hipFree(p);
   p = nullptr;
   cap = 0;
-  size_t b = bytes + bytes / 4 + 256;
+  size_t b = exact ? bytes : bytes + bytes / 4 + 256;
   if (hipMalloc(&p, b) != hipSuccess) { p = nullptr; return fail(NRG_ERR_HIP, "hipMalloc failed (" + std::to_string(b) + " bytes)"); }
   cap = b;
   return NRG_OK;
@@ -212,6 +212,20 @@ DBuf& nrg_ctx::buf(const std::string& name) {
   auto it = bufs.find(name);
   if (it == bufs.end()) it = bufs.emplace(name, std::unique_ptr<DBuf>(new DBuf())).first;
   return *it->second;
+}
+void nrg_ctx::drop(const std::string& name) { bufs.erase(name); }
+void nrg_ctx::trim(const std::string& keep) {
+  for (auto it = bufs.begin(); it != bufs.end();)
+    if (it->first.compare(0, keep.size(), keep) != 0) it = bufs.erase(it);
+    else ++it;
+}
+void nrg_ctx::rename(const std::string& from, const std::string& to) {
+  if (from == to) return;
+  auto it = bufs.find(from);
+  if (it == bufs.end()) return;
+  std::unique_ptr<DBuf> b = std::move(it->second);
+  bufs.erase(it);
+  bufs[to] = std::move(b);
 }
 
 // ---------------------------------------------------------------------------

@@ -21,7 +21,7 @@ int fail(int code, const std::string& msg);
       return nrg::fail(NRG_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(_e));            \
   } while (0)
 
-// grow-only device allocation
+// grow-only device allocation (exact: no growth slack, for buffers written once)
 struct DBuf {
   void* p = nullptr;
   size_t cap = 0;
@@ -29,7 +29,7 @@ struct DBuf {
   DBuf(const DBuf&) = delete;
   DBuf& operator=(const DBuf&) = delete;
   ~DBuf();
-  int reserve(size_t bytes);
+  int reserve(size_t bytes, bool exact = false);
   template <class T>
   T* as() { return (T*)p; }
 };
@@ -53,6 +53,12 @@ struct nrg_ctx {
   hipStream_t stream = nullptr;
   std::map<std::string, std::unique_ptr<DBuf>> bufs;
   DBuf& buf(const std::string& name);
+  // release one named buffer / every buffer whose name does not start with keep
+  void drop(const std::string& name);
+  void trim(const std::string& keep);
+  // move a buffer to another name (replacing what that name held)
+  void rename(const std::string& from, const std::string& to);
+  int dm_seq = 0;      // DM-NRG handles created on this context (buffer name prefixes)
   bool timing = false;
   int eig_impl = 2;   // 1: one workgroup per block (Householder + QL); 2: tridiag + D&C + WY
   bool plain = false; // one block (eigensolve) / one block pair (rotation) per launch sequence
