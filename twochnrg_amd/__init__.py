@@ -18,7 +18,7 @@ EXPORTS = [
     "nrg_diag_blocks", "nrg_diag_blocks_dev", "nrg_truncate", "nrg_ecut",
     "nrg_rotate_ops", "nrg_run_onechqs", "nrg_run_twochqs", "nrg_set_timing", "nrg_get_stats", "nrg_reset_stats",
     "nrg_set_eig_impl", "nrg_set_plain", "nrg_eig2_stages",
-    "nrg_dm_create", "nrg_dm_destroy", "nrg_dm_info", "nrg_dm_timing", "nrg_dm_rho", "nrg_dm_spec",
+    "nrg_dm_create", "nrg_dm_destroy", "nrg_dm_info", "nrg_dm_timing", "nrg_dm_memory", "nrg_dm_rho", "nrg_dm_spec",
     "nrg_dm_spectrum", "nrg_run_onechqs_multi", "nrg_dm_spec_omega", "nrg_dm_norm", "nrg_dm_weights",
     "nrg_run_twochqsz", "nrg_onechqs_thermo", "nrg_onechqs_jvp", "nrg_dm_create_jvp", "nrg_dm_spec_jvp",
 ]
@@ -149,6 +149,7 @@ def lib():
     L.nrg_dm_destroy.restype = None
     L.nrg_dm_info.argtypes = [ctypes.c_void_p, _ip, _ip, _dp]
     L.nrg_dm_timing.argtypes = [ctypes.c_void_p, _dp]
+    L.nrg_dm_memory.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)]
     L.nrg_dm_rho.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, ctypes.c_int64, _ip, _dp]
     L.nrg_dm_spec.argtypes = [ctypes.c_void_p, ctypes.c_int, _ip, _dp, _dp]
     L.nrg_dm_spectrum.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_int, _ip]
@@ -515,6 +516,12 @@ class DMNRG:
         ms = np.zeros(4)
         _check(lib().nrg_dm_timing(self.h, _ptr(ms)))
         return dict(forward=ms[0], sweep=ms[1], poles=ms[2], broaden=ms[3])
+
+    def memory(self):
+        """Device bytes held by the context: at the end of the sweep, and now."""
+        a, b = ctypes.c_uint64(), ctypes.c_uint64()
+        _check(lib().nrg_dm_memory(self.h, ctypes.byref(a), ctypes.byref(b)))
+        return dict(after_sweep=a.value, now=b.value)
 
     def rho(self, N, sizes):
         """rho_N blocks (list of k x k) for the given block sizes, and its trace."""

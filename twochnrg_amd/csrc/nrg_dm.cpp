@@ -63,6 +63,7 @@ struct nrg_dm {
   bool tpoles_ready = false;
   int64_t tpole_tot = 0;
   double ms_forward = 0, ms_sweep = 0, ms_poles = 0, ms_broaden = 0;
+  uint64_t mem_sweep = 0;
   ~nrg_dm() {   // the handle's buffers go back to the device with it
     if (!c || pfx.empty()) return;
     for (auto it = c->bufs.begin(); it != c->bufs.end();)
@@ -961,6 +962,7 @@ static int dm_create(nrg_ctx* c, const nrg_dmnrg_params* p, const unsigned char*
       delete D;
       return rc;
     }
+  D->mem_sweep = c->held();
   // the eigenvectors (and their tangents) served the sweep only; the chain's
   // work buffers are released too, which leaves the context holding the
   // DM-NRG handles' own buffers (rho_N, the operators, energies' tangents)
@@ -1001,6 +1003,13 @@ extern "C" int nrg_dm_info(const nrg_dm* D, int* nshell_max, int* spec_nshell_ma
   if (nshell_max) *nshell_max = D->NM;
   if (spec_nshell_max) *spec_nshell_max = D->specNM;
   if (temp) *temp = D->temp;
+  return NRG_OK;
+}
+
+extern "C" int nrg_dm_memory(const nrg_dm* D, uint64_t* after_sweep, uint64_t* now) {
+  if (!D) return fail(NRG_ERR_ARG, "nrg_dm_memory: NULL handle");
+  if (after_sweep) *after_sweep = D->mem_sweep;
+  if (now) *now = D->c->held();
   return NRG_OK;
 }
 

@@ -214,6 +214,11 @@ DBuf& nrg_ctx::buf(const std::string& name) {
   return *it->second;
 }
 void nrg_ctx::drop(const std::string& name) { bufs.erase(name); }
+uint64_t nrg_ctx::held() const {
+  uint64_t b = 0;
+  for (const auto& x : bufs) b += x.second->cap;
+  return b;
+}
 void nrg_ctx::trim(const std::string& keep) {
   for (auto it = bufs.begin(); it != bufs.end();)
     if (it->first.compare(0, keep.size(), keep) != 0) it = bufs.erase(it);

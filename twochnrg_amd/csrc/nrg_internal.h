@@ -59,6 +59,7 @@ struct nrg_ctx {
   // move a buffer to another name (replacing what that name held)
   void rename(const std::string& from, const std::string& to);
   int dm_seq = 0;      // DM-NRG handles created on this context (buffer name prefixes)
+  uint64_t held() const;   // bytes of every named buffer
   bool timing = false;
   int eig_impl = 2;   // 1: one workgroup per block (Householder + QL); 2: tridiag + D&C + WY
   bool plain = false; // one block (eigensolve) / one block pair (rotation) per launch sequence
