@@ -498,6 +498,11 @@ def main():
                        "flops_per_step": eig["flops"] / nsteps, "ms_per_step": eig["ms"] / nsteps},
         "kernel_ms_per_step": {k: v["ms"] / nsteps for k, v in st.items()},
     }
+    gm = st.get("gemm")
+    if gm and gm["ms"] > 0:
+        # operator rotations Z_i F Z_j^T (FP64 MFMA tile kernel): algorithmic flops / HIP-event time
+        out["rotation"] = {"ms_per_step": gm["ms"] / nsteps, "tflops": gm["flops"] / (gm["ms"] * 1e-3) / 1e12,
+                           "frac_fp64_peak": gm["flops"] / (gm["ms"] * 1e-3) / 1e12 / FP64_PEAK_TFLOPS}
     cal = hbm_probe_child()
     roof["measured_hbm"] = cal
     if roof["unit"] == "GB/s" and "read_GBs" in cal:

@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "../../include/twochnrg_amd.h"
+#include "nrg_eig2.h"
 #include "nrg_internal.h"
 
 namespace nrg {
@@ -388,43 +389,50 @@ int nrg::rotate_dev(nrg_ctx* c, const std::vector<RotJob>& jobs, const double* d
     }
     return NRG_OK;
   }
-  // pass 1: auxT(kj x ni) = Zj(kj x nj) * F(ni x nj)^T ; pass 2: out(ki x kj) = Zi(ki x ni) * auxT(kj x ni)^T
-  std::vector<GemmDesc> p1, p2;
-  std::vector<GemmTile> t1, t2;
+  // pass 1: auxT(kj x ni) = Zj(kj x nj) * F(ni x nj)^T ; pass 2: out(ki x kj) = Zi(ki x ni) * auxT(kj x ni)^T,
+  // both on the FP64 MFMA tile kernel of the eigensolver (B^T read through its strides)
+  std::vector<E2Gemm> pall;
+  std::vector<E2Tile> t1, t2;
   for (size_t q = 0; q < jobs.size(); q++) {
     const RotJob& J = jobs[q];
     if (J.ki == 0 || J.kj == 0) continue;
-    GemmDesc g{};
-    g.m = J.kj; g.n = J.ni; g.k = J.nj; g.lda = J.ldzj; g.ldb = J.nj; g.ldc = J.ni;
-    g.a_off = J.zj_off; g.b_off = J.f_off; g.c_off = J.aux_off; g.alpha = 1.0;
-    p1.push_back(g);
-    GemmDesc h{};
-    h.m = J.ki; h.n = J.kj; h.k = J.ni; h.lda = J.ldzi; h.ldb = J.ni; h.ldc = J.kj;
-    h.a_off = J.zi_off; h.b_off = J.aux_off; h.c_off = J.out_off; h.alpha = 1.0;
-    p2.push_back(h);
+    E2Gemm g{};
+    g.dyn = -1; g.alpha = 1.0;
+    g.m = J.kj; g.n = J.ni; g.k = J.nj;
+    g.a_off = J.zj_off; g.as0 = J.ldzj; g.as1 = 1;
+    g.b_off = J.f_off; g.bs0 = 1; g.bs1 = J.nj;
+    g.c_off = J.aux_off; g.cs0 = J.ni;
+    pall.push_back(g);
   }
-  auto tiles = [](const std::vector<GemmDesc>& P, std::vector<GemmTile>& T) {
-    for (size_t q = 0; q < P.size(); q++) {
-      int tm = (P[q].m + 63) / 64, tn = (P[q].n + 63) / 64;
-      for (int a = 0; a < tm; a++)
-        for (int b = 0; b < tn; b++) T.push_back(GemmTile{(int)q, a, b, 0});
-    }
-  };
-  tiles(p1, t1);
-  tiles(p2, t2);
-  if (p1.empty()) return NRG_OK;
+  const int n1 = (int)pall.size();
+  for (size_t q = 0; q < jobs.size(); q++) {
+    const RotJob& J = jobs[q];
+    if (J.ki == 0 || J.kj == 0) continue;
+    E2Gemm h{};
+    h.dyn = -1; h.alpha = 1.0;
+    h.m = J.ki; h.n = J.kj; h.k = J.ni;
+    h.a_off = J.zi_off; h.as0 = J.ldzi; h.as1 = 1;
+    h.b_off = J.aux_off; h.bs0 = 1; h.bs1 = J.ni;
+    h.c_off = J.out_off; h.cs0 = J.kj;
+    pall.push_back(h);
+  }
+  if (n1 == 0) return NRG_OK;
+  for (int q = 0; q < (int)pall.size(); q++) {
+    const int tm = (pall[q].m + 63) / 64, tn = (pall[q].n + 63) / 64;
+    std::vector<E2Tile>& T = (q < n1) ? t1 : t2;
+    for (int a = 0; a < tm; a++)
+      for (int b = 0; b < tn; b++) T.push_back(E2Tile{q, a, b, 0});
+  }
   DBuf& dp = c->buf("rot_probs");
   DBuf& dt = c->buf("rot_tiles");
-  size_t np = p1.size() + p2.size(), nt = t1.size() + t2.size();
+  const size_t np = pall.size(), nt = t1.size() + t2.size();
   int rc;
-  if ((rc = dp.reserve(sizeof(GemmDesc) * np))) return rc;
-  if ((rc = dt.reserve(sizeof(GemmTile) * nt))) return rc;
-  std::vector<GemmDesc> pall(p1);
-  pall.insert(pall.end(), p2.begin(), p2.end());
-  std::vector<GemmTile> tall(t1);
-  for (auto& t : t2) tall.push_back(GemmTile{t.prob + (int)p1.size(), t.tm, t.tn, 0});
-  HIPCHK(hipMemcpyAsync(dp.p, pall.data(), sizeof(GemmDesc) * np, hipMemcpyHostToDevice, c->stream));
-  HIPCHK(hipMemcpyAsync(dt.p, tall.data(), sizeof(GemmTile) * nt, hipMemcpyHostToDevice, c->stream));
+  if ((rc = dp.reserve(sizeof(E2Gemm) * np))) return rc;
+  if ((rc = dt.reserve(sizeof(E2Tile) * nt))) return rc;
+  std::vector<E2Tile> tall(t1);
+  tall.insert(tall.end(), t2.begin(), t2.end());
+  HIPCHK(hipMemcpyAsync(dp.p, pall.data(), sizeof(E2Gemm) * np, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(dt.p, tall.data(), sizeof(E2Tile) * nt, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   double fl = 0.0, by = 0.0;
   for (auto& g : pall) {
@@ -433,9 +441,8 @@ int nrg::rotate_dev(nrg_ctx* c, const std::vector<RotJob>& jobs, const double* d
   }
   PendingTimer tm;
   if ((rc = c->timer_begin(tm, NRG_K_GEMM, fl, by))) return rc;
-  HIPCHK(nrgk_launch_gemm_nt((GemmDesc*)dp.p, (GemmTile*)dt.p, (int)t1.size(), dZ, dF, dAux, c->stream));
-  HIPCHK(nrgk_launch_gemm_nt((GemmDesc*)dp.p, (GemmTile*)dt.p + t1.size(), (int)t2.size(), dZ, dAux, dOut,
-                             c->stream));
+  HIPCHK(e2_launch_gemm3((E2Gemm*)dp.p, (E2Tile*)dt.p, (int)t1.size(), dZ, dF, dAux, c->stream));
+  HIPCHK(e2_launch_gemm3((E2Gemm*)dp.p, (E2Tile*)dt.p + t1.size(), (int)t2.size(), dZ, dAux, dOut, c->stream));
   if ((rc = c->timer_end(tm))) return rc;
   return NRG_OK;
 }
