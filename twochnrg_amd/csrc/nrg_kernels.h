@@ -65,18 +65,6 @@ struct FArgs {
   int pad;
 };
 
-struct GemmDesc {
-  int m, n, k;
-  int lda, ldb, ldc;
-  int64_t a_off, b_off, c_off;
-  double alpha;
-  int accumulate;
-  int pad;
-};
-
-struct GemmTile {
-  int prob, tm, tn, pad;
-};
 
 struct CompactDesc {
   int k, n;
@@ -138,8 +126,6 @@ hipError_t nrgk_launch_unpack(const EigDesc* d_desc, const int64_t* d_poff, cons
                               int nmax, hipStream_t s);
 hipError_t nrgk_launch_hn(const HnArgs* a, int nb, int nmax, hipStream_t s);
 hipError_t nrgk_launch_fbasis(const FArgs* a, int npairs, int64_t maxel, hipStream_t s);
-hipError_t nrgk_launch_gemm_nt(const GemmDesc* d_probs, const GemmTile* d_tiles, int ntiles, const double* A,
-                               const double* B, double* C, hipStream_t s);
 hipError_t nrgk_launch_compact(const CompactDesc* d_cd, int nb, int64_t maxel, const double* src, double* dst,
                                hipStream_t s);
 }

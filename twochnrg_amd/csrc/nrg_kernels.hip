@@ -6,7 +6,6 @@
 //                          f-operator and per-(S, site type) coupling tables
 //   * fbasis_kernel      : operator matrix in the product basis (F of
 //                          UpdateMatrices) for the rotation GEMMs
-//   * gemm_nt_f64_kernel : batched grouped C = alpha * A * B^T in FP64
 //   * unpack/compact helpers
 //
 // Layouts follow the reference's storage contract (include/NRGclasses.hpp):
@@ -338,72 +337,6 @@ __global__ void fbasis_kernel(const FArgs a) {
   }
 }
 
-// ---------------------------------------------------------------------------
-// Batched grouped GEMM, FP64: C(m x n) = alpha * A(m x k) * B(n x k)^T
-// (+ C if accumulate).  64x64 output tile per 256-thread workgroup, 4x4 per
-// thread, K staged through LDS in chunks of 16 (k-major, padded).
-// ---------------------------------------------------------------------------
-#define GT 64
-#define GK 16
-__global__ __launch_bounds__(256) void gemm_nt_f64_kernel(const GemmDesc* __restrict__ probs,
-                                                          const GemmTile* __restrict__ tiles,
-                                                          const double* __restrict__ Abase,
-                                                          const double* __restrict__ Bbase,
-                                                          double* __restrict__ Cbase) {
-  __shared__ double As[GK][GT + 1];
-  __shared__ double Bs[GK][GT + 1];
-  const GemmTile T = tiles[blockIdx.x];
-  const GemmDesc P = probs[T.prob];
-  const double* A = Abase + P.a_off;
-  const double* B = Bbase + P.b_off;
-  double* C = Cbase + P.c_off;
-  const int m0 = T.tm * GT, n0 = T.tn * GT;
-  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-  double acc[4][4];
-#pragma unroll
-  for (int r = 0; r < 4; r++)
-#pragma unroll
-    for (int c = 0; c < 4; c++) acc[r][c] = 0.0;
-  for (int k0 = 0; k0 < P.k; k0 += GK) {
-    // load 64x16 of A and B: thread loads rows (tid/16)+{0,16,32,48}, column tid%16
-#pragma unroll
-    for (int rr = 0; rr < 4; rr++) {
-      const int row = ty + 16 * rr, kk = tx;
-      const int gi = m0 + row, gk = k0 + kk;
-      As[kk][row] = (gi < P.m && gk < P.k) ? A[(int64_t)gi * P.lda + gk] : 0.0;
-      const int gj = n0 + row;
-      Bs[kk][row] = (gj < P.n && gk < P.k) ? B[(int64_t)gj * P.ldb + gk] : 0.0;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int kk = 0; kk < GK; kk++) {
-      double av[4], bv[4];
-#pragma unroll
-      for (int r = 0; r < 4; r++) av[r] = As[kk][ty * 4 + r];
-#pragma unroll
-      for (int c = 0; c < 4; c++) bv[c] = Bs[kk][tx * 4 + c];
-#pragma unroll
-      for (int r = 0; r < 4; r++)
-#pragma unroll
-        for (int c = 0; c < 4; c++) acc[r][c] = __fma_rn(av[r], bv[c], acc[r][c]);
-    }
-    __syncthreads();
-  }
-#pragma unroll
-  for (int r = 0; r < 4; r++) {
-    const int gi = m0 + ty * 4 + r;
-    if (gi >= P.m) continue;
-#pragma unroll
-    for (int c = 0; c < 4; c++) {
-      const int gj = n0 + tx * 4 + c;
-      if (gj >= P.n) continue;
-      double v = P.alpha * acc[r][c];
-      if (P.accumulate) v += C[(int64_t)gi * P.ldc + gj];
-      C[(int64_t)gi * P.ldc + gj] = v;
-    }
-  }
-}
-
 // copy the first k rows (length n) of each block -- CutStates' eigenvector
 // compaction (src/NRGarrayClass.cpp:1467-1548): kept states are the lowest
 // rows of each energy-ordered block.
@@ -458,12 +391,6 @@ hipError_t nrgk_launch_fbasis(const FArgs* a, int npairs, int64_t maxel, hipStre
   return hipGetLastError();
 }
 
-hipError_t nrgk_launch_gemm_nt(const GemmDesc* d_probs, const GemmTile* d_tiles, int ntiles, const double* A,
-                               const double* B, double* C, hipStream_t s) {
-  if (ntiles <= 0) return hipSuccess;
-  hipLaunchKernelGGL(nrgk::gemm_nt_f64_kernel, dim3(ntiles), dim3(256), 0, s, d_probs, d_tiles, A, B, C);
-  return hipGetLastError();
-}
 
 hipError_t nrgk_launch_compact(const CompactDesc* d_cd, int nb, int64_t maxel, const double* src, double* dst,
                                hipStream_t s) {
