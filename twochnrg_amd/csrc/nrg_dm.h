@@ -122,9 +122,6 @@ hipError_t dm_launch_gather(const DmGBlk* blks, int nblk, int kmax, const DmSeg*
                             const double* en, double bb, double zdisc, double* out, hipStream_t s);
 hipError_t dm_launch_weights(const DmPair* pairs, int npair, int kkmax, const double* ops, const double* prod,
                              const double* en, double* Wp, double* Wn, double* dE, hipStream_t s);
-hipError_t dm_launch_broaden(const DmTask* tasks, int ntask, int maxchunk, const double* Wp, const double* Wn,
-                             const double* dE, int bd, double bbroad, double btemp, double* partial, double* out,
-                             hipStream_t s);
 // the shell sums of ntile tiles: out[task_beg + j] for the omegas w[task_beg + j]
 hipError_t dm_launch_broaden_tiles(const DmTile* tiles, int ntile, int maxslab, const double* w, const double* Wp,
                                    const double* Wn, const double* dE, int bd, double bbroad, double btemp,

@@ -104,32 +104,6 @@ __device__ __forceinline__ double bdelta(int bd, double w, double x, double twin
 
 #define BT 256
 #define BCH (BT * 8)   // poles per workgroup chunk
-__global__ __launch_bounds__(BT) void broaden_kernel(const DmTask* __restrict__ tasks, const double* __restrict__ Wp,
-                                                     const double* __restrict__ Wn, const double* __restrict__ dE,
-                                                     int bd, double bbroad, double btemp,
-                                                     double* __restrict__ partial) {
-  __shared__ double red[BT / 64];
-  const DmTask T = tasks[blockIdx.y];
-  const int c = blockIdx.x;
-  if (c >= T.nchunk) return;
-  const int64_t b0 = T.pole_beg + (int64_t)c * BCH;
-  const int64_t b1 = min(b0 + (int64_t)BCH, T.pole_end);
-  const double* __restrict__ W = (T.w >= 0.0) ? Wp : Wn;
-  double s = 0.0;
-  if (T.norm) {
-    for (int64_t p = b0 + threadIdx.x; p < b1; p += BT) s += W[p];
-  } else {
-    for (int64_t p = b0 + threadIdx.x; p < b1; p += BT) {
-      const double w = W[p];
-      if (w != 0.0) s += w * bdelta(bd, T.w, dE[p], T.twin, bbroad, btemp);
-    }
-  }
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
-  __syncthreads();
-  if (threadIdx.x == 0) partial[T.chunk_beg + c] = (red[0] + red[1]) + (red[2] + red[3]);
-}
 
 // the shell sums of a tile of omegas over one slab of the shell's poles: each
 // pole is read once and broadened at every omega of the tile (the omegas of
@@ -159,14 +133,35 @@ __global__ __launch_bounds__(BT) void broaden_tiles_kernel(const DmTile* __restr
   if (T.norm) {
     for (int64_t p = b0 + threadIdx.x; p < b1; p += BT) acc[0] += W[p];
   } else {
+    // outside the window both broadenings are pref/(b|x| sqrt(pi)) exp(-t^2) with
+    // t = ln|w/x|/b - shift = u_w - c_x: ln|w|/b once per omega, ln|x|/b once per
+    // pole, so each (pole, omega) costs one exp (the log is shared by the tile)
+    const bool lg = (bd == DM_BD_LOGGAUSS_GAUSS);
+    const double shift = lg ? 0.25 * bbroad : 0.0;
+    const double pref = (lg ? 1.0 : exp(-0.25 * bbroad * bbroad)) / (bbroad * sqrt(M_PI));
+    double u[DM_TILE_W];
+#pragma unroll
+    for (int j = 0; j < DM_TILE_W; j++) u[j] = log(fabs(w[j])) / bbroad;
     for (int64_t p = b0 + threadIdx.x; p < b1; p += BT) {
       const double wt = W[p];
       if (wt == 0.0) continue;
       const double x = dE[p];
-      if (!(fabs(x) < T.twin) && ((T.sign > 0 && x < 0.0) || (T.sign < 0 && x >= 0.0))) continue;
+      if (fabs(x) < T.twin) {
+#pragma unroll
+        for (int j = 0; j < DM_TILE_W; j++)
+          if (j < T.ntask) acc[j] += wt * bdelta(bd, w[j], x, T.twin, bbroad, btemp);
+        continue;
+      }
+      if ((T.sign > 0 && x < 0.0) || (T.sign < 0 && x >= 0.0)) continue;
+      const double xx = (x == 0.0) ? 1.0e-100 : x;
+      const double c = log(fabs(xx)) / bbroad + shift;
+      const double f = wt * pref / fabs(xx);
 #pragma unroll
       for (int j = 0; j < DM_TILE_W; j++)
-        if (j < T.ntask) acc[j] += wt * bdelta(bd, w[j], x, T.twin, bbroad, btemp);
+        if (j < T.ntask && !(w[j] * xx < 0.0)) {
+          const double t = u[j] - c;
+          acc[j] += f * exp(-t * t);
+        }
     }
   }
 #pragma unroll
@@ -352,20 +347,6 @@ hipError_t dm_launch_weights(const DmPair* pairs, int npair, int kkmax, const do
   if (gx > 256) gx = 256;
   if (gx < 1) gx = 1;
   hipLaunchKernelGGL(dmk::weights_kernel, dim3(gx, npair), dim3(256), 0, s, pairs, ops, prod, en, Wp, Wn, dE);
-  return hipGetLastError();
-}
-
-hipError_t dm_launch_broaden(const DmTask* tasks, int ntask, int maxchunk, const double* Wp, const double* Wn,
-                             const double* dE, int bd, double bbroad, double btemp, double* partial, double* out,
-                             hipStream_t s) {
-  if (ntask <= 0) return hipSuccess;
-  if (maxchunk > 0) {
-    hipLaunchKernelGGL(dmk::broaden_kernel, dim3(maxchunk, ntask), dim3(BT), 0, s, tasks, Wp, Wn, dE, bd, bbroad,
-                       btemp, partial);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-  }
-  hipLaunchKernelGGL(dmk::reduce_kernel, dim3((ntask + 255) / 256), dim3(256), 0, s, tasks, ntask, partial, out);
   return hipGetLastError();
 }
 
