@@ -153,26 +153,8 @@ int sweep_shell(nrg_dm* D, int N, bool fdm, double TempBar, double zdisc) {
   const DmShell& a = D->sh[N];       // Acut_N
   const DmShell& b = D->sh[N + 1];   // Acut_{N+1} with Abasis_{N+1}
   int rc;
-  // rot_b = Z_b^T (rho_b Z_b) over every Acut_{N+1} block
   const int nbb = b.nblocks();
-  std::vector<int64_t> toff(nbb), roff(nbb);
-  int64_t ttot = 0, rtot = 0;
-  std::vector<E2Gemm> g1, g2;
-  for (int q = 0; q < nbb; q++) {
-    const int k = b.bsize(q), n = b.bsize_bc(q);
-    toff[q] = ttot; ttot += (int64_t)k * n;
-    roff[q] = rtot; rtot += (int64_t)n * n;
-    g1.push_back(gemm(k, n, k, D->rho_off[N + 1][q], k, 1, b.zoff[q], n, 1, toff[q], n));
-    g2.push_back(gemm(n, n, k, b.zoff[q], 1, n, toff[q], n, 1, roff[q], n));
-  }
-  DBuf& tb = c->buf("dm_T");
-  DBuf& rb = c->buf("dm_rot");
-  if ((rc = tb.reserve(sizeof(double) * (size_t)(ttot + 1)))) return rc;
-  if ((rc = rb.reserve(sizeof(double) * (size_t)(rtot + 1)))) return rc;
-  double* dR1 = (double*)c->buf(D->pfx + "R" + std::to_string(N + 1)).p;
-  double* dZ1 = (double*)c->buf(b.zbuf).p;
-  if ((rc = gemm_groups(c, "dm_g1", g1, dR1, dZ1, (double*)tb.p))) return rc;
-  if ((rc = gemm_groups(c, "dm_g2", g2, dZ1, (double*)tb.p, (double*)rb.p))) return rc;
+  const bool full = D->ntan > 0;   // the derivative pass gathers from the whole rotated blocks
   // Abasis_{N+1} block -> surviving Acut_{N+1} block (matched by its first pre-cut state)
   const int nbas = (int)b.bas_qn.size() / 2;
   std::vector<int> bas2cut(nbas, -1);
@@ -190,14 +172,26 @@ int sweep_shell(nrg_dm* D, int N, bool fdm, double TempBar, double zdisc) {
       return fail(NRG_ERR_STATE, "DM-NRG: StCameFrom outside the parent shell");
     children[b.stcf[cidx]].push_back((int)cidx);
   }
+  // T_b = rho_b Z_b (k x n) per Acut_{N+1} block; the whole rotation Z_b^T T_b
+  // (n x n) only for the derivative pass
+  std::vector<int64_t> toff(nbb), roff(nbb);
+  int64_t ttot = 0, rtot = 0;
+  for (int q = 0; q < nbb; q++) {
+    const int k = b.bsize(q), n = b.bsize_bc(q);
+    toff[q] = ttot; ttot += (int64_t)k * n;
+    roff[q] = rtot; rtot += full ? (int64_t)n * n : 0;
+  }
   // segments: children of a block's first state; the child of state p of the
-  // block with the same type is the segment start + p
+  // block with the same type is the segment start + p.  Without tangents each
+  // segment's kc x kc diagonal block of Z^T T is its own small product (only
+  // those blocks survive the trace over the site), stored compactly.
   std::vector<DmGBlk> gb;
   std::vector<DmSeg> segs;
+  std::vector<E2Gemm> gs;
   int kmax = 1;
   const int nba = a.nblocks();
   D->rho_off[N].assign(nba, 0);
-  int64_t otot = 0;
+  int64_t otot = 0, stot = 0;
   const bool fdm_diag = fdm && std::fabs(zdisc - 0.0) > 1e-20;
   for (int B = 0; B < nba; B++) {
     const int i0 = a.bbe[2 * B], k = a.bsize(B);
@@ -228,16 +222,64 @@ int sweep_shell(nrg_dm* D, int N, bool fdm, double TempBar, double zdisc) {
           const double x = cgordan(Sold, Sold, St, Sz - Sold, Si, Sz);
           cg += x * x;
         }
-      const int n = b.bsize_bc(q);
+      const int n = b.bsize_bc(q), kq = b.bsize(q);
       const int pc = c0 - b.bas_bbe[2 * ab];
+      if (pc < 0 || pc + kc > n) return fail(NRG_ERR_STATE, "DM-NRG: a child segment leaves its block");
       DmSeg S{};
-      S.rot_off = roff[q] + (int64_t)pc * n + pc;
-      S.ld = n;
+      if (full) {
+        S.rot_off = roff[q] + (int64_t)pc * n + pc;
+        S.ld = n;
+      } else {
+        // (Z^T T)[pc+p][pc+p'] = sum_i Z[i][pc+p] T[i][pc+p']
+        S.rot_off = stot;
+        S.ld = kc;
+        gs.push_back(gemm(kc, kc, kq, b.zoff[q] + pc, 1, n, toff[q] + pc, n, 1, stot, kc));
+        stot += (int64_t)kc * kc;
+      }
       S.cg = cg;
       segs.push_back(S);
     }
     G.seg_end = (int)segs.size();
     gb.push_back(G);
+  }
+  DBuf& tb = c->buf("dm_T");
+  DBuf& rb = c->buf("dm_rot");
+  if ((rc = tb.reserve(sizeof(double) * (size_t)(ttot + 1)))) return rc;
+  if ((rc = rb.reserve(sizeof(double) * (size_t)((full ? rtot : stot) + 1)))) return rc;
+  double* dR1 = (double*)c->buf(D->pfx + "R" + std::to_string(N + 1)).p;
+  double* dZ1 = (double*)c->buf(b.zbuf).p;
+  if (full) {
+    std::vector<E2Gemm> g1, g2;
+    for (int q = 0; q < nbb; q++) {
+      const int k = b.bsize(q), n = b.bsize_bc(q);
+      g1.push_back(gemm(k, n, k, D->rho_off[N + 1][q], k, 1, b.zoff[q], n, 1, toff[q], n));
+      g2.push_back(gemm(n, n, k, b.zoff[q], 1, n, toff[q], n, 1, roff[q], n));
+    }
+    if ((rc = gemm_groups(c, "dm_g1", g1, dR1, dZ1, (double*)tb.p))) return rc;
+    if ((rc = gemm_groups(c, "dm_g2", g2, dZ1, (double*)tb.p, (double*)rb.p))) return rc;
+  } else {
+    // rho_{N+1} is its kept block plus a diagonal on the discarded states (the
+    // gather writes nothing else): T = [rho_KK Z_K ; diag(rho_DD) Z_D]
+    std::vector<E2Gemm> g1;
+    std::vector<DmScale> sc;
+    int64_t smax = 1;
+    for (int q = 0; q < nbb; q++) {
+      const int k = b.bsize(q), n = b.bsize_bc(q);
+      int kcq;
+      if ((rc = kept_prefix(b, q, kcq))) return rc;
+      if (kcq > 0) g1.push_back(gemm(kcq, n, kcq, D->rho_off[N + 1][q], k, 1, b.zoff[q], n, 1, toff[q], n));
+      if (kcq < k) {
+        sc.push_back(DmScale{toff[q], b.zoff[q], D->rho_off[N + 1][q], k, kcq, n});
+        smax = std::max<int64_t>(smax, (int64_t)(k - kcq) * n);
+      }
+    }
+    if ((rc = gemm_groups(c, "dm_g1", g1, dR1, dZ1, (double*)tb.p))) return rc;
+    if (!sc.empty()) {
+      DmScale* dsc;
+      if ((rc = upload(c, "dm_sc", sc, &dsc))) return rc;
+      HIPCHK(dm_launch_scale_rows(dsc, (int)sc.size(), smax, dR1, dZ1, (double*)tb.p, c->stream));
+    }
+    if ((rc = gemm_groups(c, "dm_g2", gs, dZ1, (double*)tb.p, (double*)rb.p))) return rc;
   }
   D->rho_len[N] = otot;
   DBuf& ob = c->buf(D->pfx + "R" + std::to_string(N));

@@ -23,6 +23,12 @@ struct DmGBlk {
   int64_t e_off;    // energies of the block in the shell's energy array
   int k, kc, kd, seg_beg, seg_end, pad;
 };
+// rows kc..k-1 of T = rho Z for one block whose rho is diagonal there:
+// T[i][:] = rho[i][i] Z[i][:] (T and Z k x n, rho k x k)
+struct DmScale {
+  int64_t t_off, z_off, r_off;
+  int k, kc, n, pad;
+};
 // one operator block pair (b1 -> b2) at a shell: k1 x k2 poles, each with a weight
 // for omega >= 0 (Wp) and one for omega < 0 (Wn)
 enum { DM_FORM_DMNRG = 0, DM_FORM_CFS = 1, DM_FORM_FDM = 2, DM_FORM_COSTI = 3 };
@@ -118,6 +124,8 @@ hipError_t dm_launch_tan_weights(const DmTanPair* pairs, int npair, int kkmax, i
 hipError_t dm_launch_broaden_jvp(const DmTask* tasks, int ntask, int maxchunk, int ntan, const double* W1,
                                  const double* W2, int64_t w_stride, const double* dE, double bbroad, double btemp,
                                  double* partial, int64_t p_stride, double* out, hipStream_t s);
+hipError_t dm_launch_scale_rows(const DmScale* blks, int nblk, int64_t maxel, const double* rho, const double* Z,
+                                double* T, hipStream_t s);
 hipError_t dm_launch_gather(const DmGBlk* blks, int nblk, int kmax, const DmSeg* segs, const double* rot,
                             const double* en, double bb, double zdisc, double* out, hipStream_t s);
 hipError_t dm_launch_weights(const DmPair* pairs, int npair, int kkmax, const double* ops, const double* prod,

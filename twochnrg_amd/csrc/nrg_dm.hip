@@ -32,6 +32,18 @@ __global__ void gather_kernel(const DmGBlk* __restrict__ blks, const DmSeg* __re
   }
 }
 
+// the discarded rows of T = rho Z, where rho is diagonal
+__global__ void scale_rows_kernel(const DmScale* __restrict__ blks, const double* __restrict__ rho,
+                                  const double* __restrict__ Z, double* __restrict__ T) {
+  const DmScale B = blks[blockIdx.y];
+  const int64_t tot = (int64_t)(B.k - B.kc) * B.n;
+  for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < tot; idx += (int64_t)gridDim.x * blockDim.x) {
+    const int i = B.kc + (int)(idx / B.n);
+    const int64_t e = (int64_t)i * B.n + idx % B.n;
+    T[B.t_off + e] = rho[B.r_off + (int64_t)i * B.k + i] * Z[B.z_off + e];
+  }
+}
+
 // pole weights (omega >= 0: Wp, omega < 0: Wn) and energies of one operator
 // block pair, with w1 = (rho1 A)_ij A_ij (Termo1) and w2 = A_ij (A rho2)_ij (Termo2):
 //   DM-NRG   Wp = Wn = cg (w1 + w2)
@@ -347,6 +359,16 @@ hipError_t dm_launch_weights(const DmPair* pairs, int npair, int kkmax, const do
   if (gx > 256) gx = 256;
   if (gx < 1) gx = 1;
   hipLaunchKernelGGL(dmk::weights_kernel, dim3(gx, npair), dim3(256), 0, s, pairs, ops, prod, en, Wp, Wn, dE);
+  return hipGetLastError();
+}
+
+hipError_t dm_launch_scale_rows(const DmScale* blks, int nblk, int64_t maxel, const double* rho, const double* Z,
+                                double* T, hipStream_t s) {
+  if (nblk <= 0) return hipSuccess;
+  int64_t gx = (maxel + 255) / 256;
+  if (gx > 1024) gx = 1024;
+  if (gx < 1) gx = 1;
+  hipLaunchKernelGGL(dmk::scale_rows_kernel, dim3((int)gx, nblk), dim3(256), 0, s, blks, rho, Z, T);
   return hipGetLastError();
 }
 
