@@ -198,7 +198,7 @@ def c5_run(cpu=True, nsitesmax=None, ncut=None, site_dims=(16.0, 4.0)):
                 wn, ztot = h.weights()
                 mem = h.memory()
             rec = {"wall_ms": 1e3 * (t1 - t0), "breakdown_ms": tm,
-                   "device_GB": {"after_sweep": mem["after_sweep"] / 1e9, "with_poles": mem["now"] / 1e9},
+                   "device_GB": {"before_sweep": mem["before_sweep"] / 1e9, "with_poles": mem["now"] / 1e9},
                    "post_ms": tm["sweep"] + tm["poles"] + tm["broaden"], "norm": norm, "ztot": ztot,
                    "shell_weight_max": int(np.argmax(wn)), "A_at_omega_pm1e-10": [float(r[999]), float(r[1000])],
                    "A_max": float(r.max())}

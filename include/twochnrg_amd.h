@@ -309,9 +309,9 @@ int nrg_dm_info(const nrg_dm *dm, int *nshell_max, int *spec_nshell_max, double 
 /* Wall times (ms): forward chain, backward sweep, pole weights, broadening. */
 int nrg_dm_timing(const nrg_dm *dm, double *ms4);
 /* No counterpart in the reference: device bytes held by the context's buffers
- * at the end of the sweep (every shell's eigenvectors, operators and rho_N, and
- * the chain's work buffers; the most nrg_dm_create holds) and now. */
-int nrg_dm_memory(const nrg_dm *dm, uint64_t *after_sweep, uint64_t *now);
+ * when the sweep starts (every shell's eigenvectors and operators: the most
+ * nrg_dm_create holds, besides one shell's temporaries) and now. */
+int nrg_dm_memory(const nrg_dm *dm, uint64_t *before_sweep, uint64_t *now);
 /* rho_N blocks (k_b x k_b row-major, Acut_N block order) and its trace
  * sum (2S+1) rho_ii (the value CalcRhoN prints); out may be NULL. */
 int nrg_dm_rho(nrg_dm *dm, int N, double *out, int64_t cap, int *nblocks, double *trace);

@@ -518,10 +518,10 @@ class DMNRG:
         return dict(forward=ms[0], sweep=ms[1], poles=ms[2], broaden=ms[3])
 
     def memory(self):
-        """Device bytes held by the context: at the end of the sweep, and now."""
+        """Device bytes held by the context: when the sweep starts (the peak), and now."""
         a, b = ctypes.c_uint64(), ctypes.c_uint64()
         _check(lib().nrg_dm_memory(self.h, ctypes.byref(a), ctypes.byref(b)))
-        return dict(after_sweep=a.value, now=b.value)
+        return dict(before_sweep=a.value, now=b.value)
 
     def rho(self, N, sizes):
         """rho_N blocks (list of k x k) for the given block sizes, and its trace."""

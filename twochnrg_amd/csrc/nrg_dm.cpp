@@ -46,7 +46,11 @@ struct nrg_dm {
   double temp = 0.0, betabar_spec = 0.0;
   double ztot = 0.0;                           // FDM Z_tot
   std::vector<double> wn;                      // FDM w_N
-  std::vector<std::vector<int64_t>> rho_off;   // per shell and block: k x k block in dmR<N>
+  // rho_N per shell and block in <pfx>R<N>: the k x k block at rho_off, or when
+  // compact (no tangents) the kc x kc kept block at rho_off and the k - kc
+  // discarded diagonal at rho_doff (rho has nothing else)
+  bool compact = false;
+  std::vector<std::vector<int64_t>> rho_off, rho_doff;
   std::vector<int64_t> rho_len;
   // poles (W, dE) of shells [ar_lo, ar_hi) in the arena, at most arena_max at
   // once; pole_beg/pole_end: every shell's range over all shells (the arena
@@ -63,7 +67,7 @@ struct nrg_dm {
   bool tpoles_ready = false;
   int64_t tpole_tot = 0;
   double ms_forward = 0, ms_sweep = 0, ms_poles = 0, ms_broaden = 0;
-  uint64_t mem_sweep = 0;
+  uint64_t mem_peak = 0;
   ~nrg_dm() {   // the handle's buffers go back to the device with it
     if (!c || pfx.empty()) return;
     for (auto it = c->bufs.begin(); it != c->bufs.end();)
@@ -191,6 +195,7 @@ int sweep_shell(nrg_dm* D, int N, bool fdm, double TempBar, double zdisc) {
   int kmax = 1;
   const int nba = a.nblocks();
   D->rho_off[N].assign(nba, 0);
+  D->rho_doff[N].assign(nba, 0);
   int64_t otot = 0, stot = 0;
   const bool fdm_diag = fdm && std::fabs(zdisc - 0.0) > 1e-20;
   for (int B = 0; B < nba; B++) {
@@ -203,8 +208,16 @@ int sweep_shell(nrg_dm* D, int N, bool fdm, double TempBar, double zdisc) {
         return fail(NRG_ERR_STATE, "DM-NRG: child states do not match the kept states");
     DmGBlk G{};
     G.out_off = otot; G.e_off = i0; G.k = k; G.kc = kc; G.kd = fdm_diag ? kc : k; G.seg_beg = (int)segs.size();
+    G.compact = D->compact ? 1 : 0;
     D->rho_off[N][B] = otot;
-    otot += (int64_t)k * k;
+    if (D->compact) {
+      otot += (int64_t)kc * kc;
+      G.d_off = otot;
+      D->rho_doff[N][B] = otot;
+      otot += k - kc;
+    } else {
+      otot += (int64_t)k * k;
+    }
     kmax = std::max(kmax, k);
     if (kc > 0) for (int c0 : children[i0]) {
       const int t = b.itype[c0], ab = st_blk[c0];
@@ -267,9 +280,9 @@ int sweep_shell(nrg_dm* D, int N, bool fdm, double TempBar, double zdisc) {
       const int k = b.bsize(q), n = b.bsize_bc(q);
       int kcq;
       if ((rc = kept_prefix(b, q, kcq))) return rc;
-      if (kcq > 0) g1.push_back(gemm(kcq, n, kcq, D->rho_off[N + 1][q], k, 1, b.zoff[q], n, 1, toff[q], n));
+      if (kcq > 0) g1.push_back(gemm(kcq, n, kcq, D->rho_off[N + 1][q], kcq, 1, b.zoff[q], n, 1, toff[q], n));
       if (kcq < k) {
-        sc.push_back(DmScale{toff[q], b.zoff[q], D->rho_off[N + 1][q], k, kcq, n});
+        sc.push_back(DmScale{toff[q], b.zoff[q], D->rho_doff[N + 1][q], kcq, k, 0, n, n, 0});
         smax = std::max<int64_t>(smax, (int64_t)(k - kcq) * n);
       }
     }
@@ -277,7 +290,7 @@ int sweep_shell(nrg_dm* D, int N, bool fdm, double TempBar, double zdisc) {
     if (!sc.empty()) {
       DmScale* dsc;
       if ((rc = upload(c, "dm_sc", sc, &dsc))) return rc;
-      HIPCHK(dm_launch_scale_rows(dsc, (int)sc.size(), smax, dR1, dZ1, (double*)tb.p, c->stream));
+      HIPCHK(dm_launch_scale(dsc, (int)sc.size(), smax, dR1, dZ1, (double*)tb.p, c->stream));
     }
     if ((rc = gemm_groups(c, "dm_g2", gs, dZ1, (double*)tb.p, (double*)rb.p))) return rc;
   }
@@ -420,6 +433,8 @@ int build_pole_range(nrg_dm* D, int N0, int N1) {
     // rho1 A (k1 x k2) and A rho2 (k1 x k2) over the whole blocks: rho has no
     // kept-discarded elements, so the products of the reference's kept/discarded
     // sub-blocks are sub-blocks of these
+    std::vector<DmScale> sc;
+    int64_t smax = 1;
     for (size_t q = 0; q < pairs.size(); q++) {
       DmPair& P = pairs[q];
       P.pole_off -= base;
@@ -428,8 +443,21 @@ int build_pole_range(nrg_dm* D, int N0, int N1) {
         P.p1_off = prod;
         P.p2_off = prod + (int64_t)P.k1 * P.k2;
         prod += 2 * (int64_t)P.k1 * P.k2;
-        g1.push_back(gemm(P.k1, P.k2, P.k1, D->rho_off[N][b1], P.k1, 1, P.a_off, P.k2, 1, P.p1_off, P.k2));
-        g2.push_back(gemm(P.k1, P.k2, P.k2, P.a_off, P.k2, 1, D->rho_off[N][b2], P.k2, 1, P.p2_off, P.k2));
+        if (!D->compact) {
+          g1.push_back(gemm(P.k1, P.k2, P.k1, D->rho_off[N][b1], P.k1, 1, P.a_off, P.k2, 1, P.p1_off, P.k2));
+          g2.push_back(gemm(P.k1, P.k2, P.k2, P.a_off, P.k2, 1, D->rho_off[N][b2], P.k2, 1, P.p2_off, P.k2));
+        } else {
+          // rho1 A: kept rows through rho1's kept block, discarded rows scaled by its diagonal;
+          // A rho2: the same by columns
+          const int c1 = P.kc1, c2 = P.kc2;
+          if (c1 > 0) g1.push_back(gemm(c1, P.k2, c1, D->rho_off[N][b1], c1, 1, P.a_off, P.k2, 1, P.p1_off, P.k2));
+          if (c1 < P.k1)
+            sc.push_back(DmScale{P.p1_off, P.a_off, D->rho_doff[N][b1], c1, P.k1, 0, P.k2, P.k2, 0});
+          if (c2 > 0) g2.push_back(gemm(P.k1, c2, c2, P.a_off, P.k2, 1, D->rho_off[N][b2], c2, 1, P.p2_off, P.k2));
+          if (c2 < P.k2)
+            sc.push_back(DmScale{P.p2_off, P.a_off, D->rho_doff[N][b2], 0, P.k1, c2, P.k2, P.k2, 1});
+          smax = std::max<int64_t>(smax, (int64_t)P.k1 * P.k2);
+        }
       }
       kkmax = std::max(kkmax, P.k1 * P.k2);
     }
@@ -440,6 +468,11 @@ int build_pole_range(nrg_dm* D, int N0, int N1) {
       double* dR = (double*)c->buf(D->pfx + "R" + std::to_string(N)).p;
       if ((rc = gemm_groups(c, "dm_p1", g1, dR, dO, (double*)pb.p))) return rc;
       if ((rc = gemm_groups(c, "dm_p2", g2, dO, dR, (double*)pb.p))) return rc;
+      if (!sc.empty()) {
+        DmScale* dsc;
+        if ((rc = upload(c, "dm_psc", sc, &dsc))) return rc;
+        HIPCHK(dm_launch_scale(dsc, (int)sc.size(), smax, dR, dO, (double*)pb.p, c->stream));
+      }
     }
     double* dEn;
     DmPair* dpairs;
@@ -906,6 +939,10 @@ static int dm_create(nrg_ctx* c, const nrg_dmnrg_params* p, const unsigned char*
         c->rename(*nm, D->pfx + *nm);
         *nm = D->pfx + *nm;
       }
+  // the chain's work buffers are not needed any more: what the context holds
+  // now (every shell's eigenvectors and operators) is the most it will hold
+  c->trim("dm#");
+  D->mem_peak = c->held();
   // temperature from Mtemp (src/DM_NRG/DM_NRG.cpp:102-118); FDM keeps every shell (:105)
   // the TwoChQSz loop diagonalises N = 0 .. nsitesmax: one shell more than NRG_main's sites
   int Nsitesmax = (p->model == 1) ? (int)D->sh.size() : p->nsitesmax;
@@ -927,7 +964,9 @@ static int dm_create(nrg_ctx* c, const nrg_dmnrg_params* p, const unsigned char*
   D->specNM = Nsitesmax;
   const int NM = D->NM;
   D->rho_off.assign(NM + 1, std::vector<int64_t>());
+  D->rho_doff.assign(NM + 1, std::vector<int64_t>());
   D->rho_len.assign(NM + 1, 0);
+  D->compact = (ntan == 0);
   // Z_N at Temp/D_{NM-1} (src/DM_NRG/DM_NRG.cpp:203-214); FDM: Z_tot = Z_N +
   // sum_N Z_N^disc 4^{NM-N} and w_N (:221-271)
   const double ZN = partition_func(D->sh[NM], 1.0 / (D->temp / calc_dn(p->Lambda, NM - 1)));
@@ -953,11 +992,16 @@ static int dm_create(nrg_ctx* c, const nrg_dmnrg_params* p, const unsigned char*
     const double Z = fdm ? ZTot : ZN;
     std::vector<double> r;
     D->rho_off[NM].assign(s.nblocks(), 0);
+    D->rho_doff[NM].assign(s.nblocks(), 0);
     for (int b = 0; b < s.nblocks(); b++) {
       const int k = s.bsize(b), i0 = s.bbe[2 * b];
+      int kc = k;
+      if (D->compact && (rc = kept_prefix(s, b, kc))) { delete D; return rc; }
       D->rho_off[NM][b] = (int64_t)r.size();
-      for (int i = 0; i < k; i++)
-        for (int j = 0; j < k; j++) r.push_back(i == j ? std::exp(-bb * s.en[i0 + i]) / Z : 0.0);
+      for (int i = 0; i < kc; i++)
+        for (int j = 0; j < kc; j++) r.push_back(i == j ? std::exp(-bb * s.en[i0 + i]) / Z : 0.0);
+      D->rho_doff[NM][b] = (int64_t)r.size();
+      for (int i = kc; i < k; i++) r.push_back(std::exp(-bb * s.en[i0 + i]) / Z);
     }
     D->rho_len[NM] = (int64_t)r.size();
     DBuf& rb = c->buf(D->pfx + "R" + std::to_string(NM));
@@ -999,21 +1043,23 @@ static int dm_create(nrg_ctx* c, const nrg_dmnrg_params* p, const unsigned char*
     }
   }
   // the sweep: TempBar = Temp/D_{N-1}, ZNdisc = Z_tot/4^{NM-N} (:361-366)
-  for (int N = NM - 1; N >= 0; N--)
-    if ((rc = sweep_shell(D, N, fdm, D->temp / calc_dn(p->Lambda, N - 1), ZTot / std::pow(site_dim, NM - N)))) {
-      delete D;
-      return rc;
-    }
-  D->mem_sweep = c->held();
-  // the eigenvectors (and their tangents) served the sweep only; the chain's
-  // work buffers are released too, which leaves the context holding the
-  // DM-NRG handles' own buffers (rho_N, the operators, energies' tangents)
-  for (DmShell& s : D->sh) {
+  // the eigenvectors of shell N+1 (and their tangents) serve the step to N
+  // only, and go back to the device after it
+  auto drop_z = [&](DmShell& s) {
     if (!s.zbuf.empty()) c->drop(s.zbuf);
     if (!s.dzbuf.empty()) c->drop(s.dzbuf);
     s.zbuf.clear();
     s.dzbuf.clear();
+  };
+  for (int N = NM - 1; N >= 0; N--) {
+    if ((rc = sweep_shell(D, N, fdm, D->temp / calc_dn(p->Lambda, N - 1), ZTot / std::pow(site_dim, NM - N)))) {
+      delete D;
+      return rc;
+    }
+    drop_z(D->sh[N + 1]);
   }
+  // what is left is the handle's own: rho_N, the operators, the energies' tangents
+  for (DmShell& s : D->sh) drop_z(s);
   c->trim("dm#");
   auto t2 = std::chrono::steady_clock::now();
   D->ms_forward = std::chrono::duration<double, std::milli>(t1 - t0).count();
@@ -1048,9 +1094,9 @@ extern "C" int nrg_dm_info(const nrg_dm* D, int* nshell_max, int* spec_nshell_ma
   return NRG_OK;
 }
 
-extern "C" int nrg_dm_memory(const nrg_dm* D, uint64_t* after_sweep, uint64_t* now) {
+extern "C" int nrg_dm_memory(const nrg_dm* D, uint64_t* before_sweep, uint64_t* now) {
   if (!D) return fail(NRG_ERR_ARG, "nrg_dm_memory: NULL handle");
-  if (after_sweep) *after_sweep = D->mem_sweep;
+  if (before_sweep) *before_sweep = D->mem_peak;
   if (now) *now = D->c->held();
   return NRG_OK;
 }
@@ -1064,23 +1110,46 @@ extern "C" int nrg_dm_timing(const nrg_dm* D, double* ms) {
 extern "C" int nrg_dm_rho(nrg_dm* D, int N, double* out, int64_t cap, int* nblocks, double* trace) {
   if (!D || N < 0 || N > D->NM) return fail(NRG_ERR_ARG, "nrg_dm_rho: bad shell");
   const DmShell& s = D->sh[N];
-  if (nblocks) *nblocks = s.nblocks();
+  const int nb = s.nblocks();
+  if (nblocks) *nblocks = nb;
   const int64_t len = D->rho_len[N];
   std::vector<double> r(len);
   HIPCHK(hipMemcpyAsync(r.data(), D->c->buf(D->pfx + "R" + std::to_string(N)).p, sizeof(double) * len,
                         hipMemcpyDeviceToHost, D->c->stream));
   HIPCHK(hipStreamSynchronize(D->c->stream));
+  // the k x k blocks (a compact rho is its kept block plus the discarded diagonal)
+  int64_t full = 0;
+  for (int b = 0; b < nb; b++) full += (int64_t)s.bsize(b) * s.bsize(b);
+  std::vector<double> f;
+  if (D->compact) {
+    f.assign(full, 0.0);
+    int64_t o = 0;
+    for (int b = 0; b < nb; b++) {
+      const int k = s.bsize(b);
+      int kc = k;
+      int rc = kept_prefix(s, b, kc);
+      if (rc) return rc;
+      for (int i = 0; i < kc; i++)
+        for (int j = 0; j < kc; j++) f[o + (int64_t)i * k + j] = r[D->rho_off[N][b] + (int64_t)i * kc + j];
+      for (int i = kc; i < k; i++) f[o + (int64_t)i * k + i] = r[D->rho_doff[N][b] + (i - kc)];
+      o += (int64_t)k * k;
+    }
+  } else {
+    f.swap(r);
+  }
   if (trace) {
     double t = 0.0;
-    for (int b = 0; b < s.nblocks(); b++) {
+    int64_t o = 0;
+    for (int b = 0; b < nb; b++) {
       const int k = s.bsize(b);
-      for (int i = 0; i < k; i++) t += s.deg(b) * r[D->rho_off[N][b] + (int64_t)i * k + i];
+      for (int i = 0; i < k; i++) t += s.deg(b) * f[o + (int64_t)i * k + i];
+      o += (int64_t)k * k;
     }
     *trace = t;
   }
   if (out) {
-    if (cap < len) return fail(NRG_ERR_ARG, "nrg_dm_rho: output too small");
-    std::copy(r.begin(), r.end(), out);
+    if (cap < full) return fail(NRG_ERR_ARG, "nrg_dm_rho: output too small");
+    std::copy(f.begin(), f.end(), out);
   }
   return NRG_OK;
 }

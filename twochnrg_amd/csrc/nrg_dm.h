@@ -18,16 +18,20 @@ struct DmSeg {
 // one Acut_N block of the gather: output k x k at out_off, segments [seg_beg, seg_end)
 // over the leading kc x kc (the states with children: the kept prefix); FDM puts
 // e^{-bb E}/Z_disc on the diagonal of the discarded states p >= kd
+// compact (no tangents): only the kc x kc kept block at out_off and the k - kc
+// discarded diagonal at d_off are stored (the rest of rho is zero)
 struct DmGBlk {
   int64_t out_off;
   int64_t e_off;    // energies of the block in the shell's energy array
-  int k, kc, kd, seg_beg, seg_end, pad;
+  int64_t d_off;
+  int k, kc, kd, seg_beg, seg_end, compact;
 };
-// rows kc..k-1 of T = rho Z for one block whose rho is diagonal there:
-// T[i][:] = rho[i][i] Z[i][:] (T and Z k x n, rho k x k)
+// out[i][j] = a[i][j] * d[i - m0] (rows, mode 0) or d[j - n0] (columns, mode 1)
+// for i in [m0, m1), j in [n0, n1); out and a share the leading dimension ld:
+// products of a matrix with the discarded diagonal of a compact rho
 struct DmScale {
-  int64_t t_off, z_off, r_off;
-  int k, kc, n, pad;
+  int64_t o_off, a_off, d_off;
+  int m0, m1, n0, n1, ld, mode;
 };
 // one operator block pair (b1 -> b2) at a shell: k1 x k2 poles, each with a weight
 // for omega >= 0 (Wp) and one for omega < 0 (Wn)
@@ -124,8 +128,8 @@ hipError_t dm_launch_tan_weights(const DmTanPair* pairs, int npair, int kkmax, i
 hipError_t dm_launch_broaden_jvp(const DmTask* tasks, int ntask, int maxchunk, int ntan, const double* W1,
                                  const double* W2, int64_t w_stride, const double* dE, double bbroad, double btemp,
                                  double* partial, int64_t p_stride, double* out, hipStream_t s);
-hipError_t dm_launch_scale_rows(const DmScale* blks, int nblk, int64_t maxel, const double* rho, const double* Z,
-                                double* T, hipStream_t s);
+hipError_t dm_launch_scale(const DmScale* blks, int nblk, int64_t maxel, const double* d, const double* a, double* out,
+                           hipStream_t s);
 hipError_t dm_launch_gather(const DmGBlk* blks, int nblk, int kmax, const DmSeg* segs, const double* rot,
                             const double* en, double bb, double zdisc, double* out, hipStream_t s);
 hipError_t dm_launch_weights(const DmPair* pairs, int npair, int kkmax, const double* ops, const double* prod,

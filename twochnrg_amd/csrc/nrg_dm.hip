@@ -18,6 +18,24 @@ __global__ void gather_kernel(const DmGBlk* __restrict__ blks, const DmSeg* __re
                               const double* __restrict__ rot, const double* __restrict__ en, double bb, double zdisc,
                               double* __restrict__ out) {
   const DmGBlk B = blks[blockIdx.y];
+  if (B.compact) {
+    const int64_t kk = (int64_t)B.kc * B.kc, tot = kk + (B.k - B.kc);
+    for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < tot; idx += (int64_t)gridDim.x * blockDim.x) {
+      if (idx < kk) {
+        const int p = (int)(idx / B.kc), q = (int)(idx % B.kc);
+        double s = 0.0;
+        for (int g = B.seg_beg; g < B.seg_end; g++) {
+          const DmSeg S = segs[g];
+          s += S.cg * rot[S.rot_off + (int64_t)p * S.ld + q];
+        }
+        out[B.out_off + idx] = s;
+      } else {
+        const int p = B.kc + (int)(idx - kk);
+        out[B.d_off + (p - B.kc)] = (p >= B.kd) ? exp(-bb * en[B.e_off + p]) / zdisc : 0.0;
+      }
+    }
+    return;
+  }
   const int64_t kk = (int64_t)B.k * B.k;
   for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < kk; idx += (int64_t)gridDim.x * blockDim.x) {
     const int p = (int)(idx / B.k), q = (int)(idx % B.k);
@@ -32,15 +50,17 @@ __global__ void gather_kernel(const DmGBlk* __restrict__ blks, const DmSeg* __re
   }
 }
 
-// the discarded rows of T = rho Z, where rho is diagonal
-__global__ void scale_rows_kernel(const DmScale* __restrict__ blks, const double* __restrict__ rho,
-                                  const double* __restrict__ Z, double* __restrict__ T) {
+// a matrix times the discarded diagonal of a compact rho, from the left (rows)
+// or from the right (columns)
+__global__ void scale_kernel(const DmScale* __restrict__ blks, const double* __restrict__ d,
+                             const double* __restrict__ a, double* __restrict__ out) {
   const DmScale B = blks[blockIdx.y];
-  const int64_t tot = (int64_t)(B.k - B.kc) * B.n;
+  const int w = B.n1 - B.n0;
+  const int64_t tot = (int64_t)(B.m1 - B.m0) * w;
   for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < tot; idx += (int64_t)gridDim.x * blockDim.x) {
-    const int i = B.kc + (int)(idx / B.n);
-    const int64_t e = (int64_t)i * B.n + idx % B.n;
-    T[B.t_off + e] = rho[B.r_off + (int64_t)i * B.k + i] * Z[B.z_off + e];
+    const int i = B.m0 + (int)(idx / w), j = B.n0 + (int)(idx % w);
+    const int64_t e = (int64_t)i * B.ld + j;
+    out[B.o_off + e] = a[B.a_off + e] * d[B.d_off + (B.mode ? j - B.n0 : i - B.m0)];
   }
 }
 
@@ -362,13 +382,13 @@ hipError_t dm_launch_weights(const DmPair* pairs, int npair, int kkmax, const do
   return hipGetLastError();
 }
 
-hipError_t dm_launch_scale_rows(const DmScale* blks, int nblk, int64_t maxel, const double* rho, const double* Z,
-                                double* T, hipStream_t s) {
+hipError_t dm_launch_scale(const DmScale* blks, int nblk, int64_t maxel, const double* d, const double* a, double* out,
+                           hipStream_t s) {
   if (nblk <= 0) return hipSuccess;
   int64_t gx = (maxel + 255) / 256;
   if (gx > 1024) gx = 1024;
   if (gx < 1) gx = 1;
-  hipLaunchKernelGGL(dmk::scale_rows_kernel, dim3((int)gx, nblk), dim3(256), 0, s, blks, rho, Z, T);
+  hipLaunchKernelGGL(dmk::scale_kernel, dim3((int)gx, nblk), dim3(256), 0, s, blks, d, a, out);
   return hipGetLastError();
 }
 
