@@ -21,6 +21,7 @@
 #include <chrono>
 #include <cmath>
 #include <map>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -887,6 +888,9 @@ double lin_interp(const std::vector<double>& x, const std::vector<double>& y, do
 
 }  // namespace
 
+static int dm_fill(nrg_ctx* c, const nrg_dmnrg_params* p, const unsigned char* kept, const int64_t* kept_off,
+                   int ntan, const double* dpar, nrg_dm* D);
+
 static int dm_create(nrg_ctx* c, const nrg_dmnrg_params* p, const unsigned char* kept, const int64_t* kept_off,
                      int ntan, const double* dpar, nrg_dm** out) {
   if (!c || !p || !out) return fail(NRG_ERR_ARG, "nrg_dm_create: NULL argument");
@@ -902,7 +906,16 @@ static int dm_create(nrg_ctx* c, const nrg_dmnrg_params* p, const unsigned char*
   if (p->model == 1 && (p->method == 0 || ntan > 0 || kept))
     return fail(NRG_ERR_ARG, "nrg_dm_create: the TwoChQSz chain supports CFS and FDM without derivatives");
   HIPCHK(hipSetDevice(c->device));
-  nrg_dm* D = new nrg_dm();
+  // the handle owns its device buffers: on any error they go back with it
+  std::unique_ptr<nrg_dm> D(new nrg_dm());
+  const int rc = dm_fill(c, p, kept, kept_off, ntan, dpar, D.get());
+  if (rc) return rc;
+  *out = D.release();
+  return NRG_OK;
+}
+
+static int dm_fill(nrg_ctx* c, const nrg_dmnrg_params* p, const unsigned char* kept, const int64_t* kept_off,
+                   int ntan, const double* dpar, nrg_dm* D) {
   D->c = c;
   D->pfx = "dm#" + std::to_string(c->dm_seq++) + ":";
   D->p = *p;
@@ -922,13 +935,12 @@ static int dm_create(nrg_ctx* c, const nrg_dmnrg_params* p, const unsigned char*
     nrg_twochqs_params q2{};
     q2.U = p->U; q2.Gamma1 = p->Gamma; q2.Gamma2 = p->Gamma2; q2.ed = p->ed; q2.Lambda = p->Lambda;
     q2.nsitesmax = p->nsitesmax; q2.ncut = p->ncut; q2.betabar = p->betabar_nrg;
-    if ((rc = run_twochqsz_dm(c, q2, D->sh))) { delete D; return rc; }
+    if ((rc = run_twochqsz_dm(c, q2, D->sh))) return rc;
     for (DmShell& s : D->sh) s.sz = true;
   } else if (kept || ntan > 0) {
-    if (p->update_bef_cut) { delete D; return fail(NRG_ERR_ARG, "nrg_dm_create_jvp: frozen Kept sets need UpdateBefCut 0"); }
-    if ((rc = run_onechqs_dm_jvp(c, q, D->sh, kept, kept_off, ntan, dpar))) { delete D; return rc; }
+    if (p->update_bef_cut) return fail(NRG_ERR_ARG, "nrg_dm_create_jvp: frozen Kept sets need UpdateBefCut 0");
+    if ((rc = run_onechqs_dm_jvp(c, q, D->sh, kept, kept_off, ntan, dpar))) return rc;
   } else if ((rc = run_onechqs_dm(c, q, D->sh, p->update_bef_cut ? 1 : 0))) {
-    delete D;
     return rc;
   }
   auto t1 = std::chrono::steady_clock::now();
@@ -956,7 +968,7 @@ static int dm_create(nrg_ctx* c, const nrg_dmnrg_params* p, const unsigned char*
     DM = calc_dn(p->Lambda, Nsitesmax - 1);
   }
   D->betabar_spec = p->betabar;
-  if ((int)D->sh.size() < Nsitesmax) { delete D; return fail(NRG_ERR_STATE, "nrg_dm_create: fewer shells than NRG sites"); }
+  if ((int)D->sh.size() < Nsitesmax) return fail(NRG_ERR_STATE, "nrg_dm_create: fewer shells than NRG sites");
   // FDM's count of the discarded states' environment: the site dimension per
   // later site (4 for one channel, as src/DM_NRG/DM_NRG.cpp:244; 16 for two)
   const double site_dim = p->fdm_site_dim > 0.0 ? p->fdm_site_dim : ((p->model == 1) ? 16.0 : 4.0);
@@ -996,7 +1008,7 @@ static int dm_create(nrg_ctx* c, const nrg_dmnrg_params* p, const unsigned char*
     for (int b = 0; b < s.nblocks(); b++) {
       const int k = s.bsize(b), i0 = s.bbe[2 * b];
       int kc = k;
-      if (D->compact && (rc = kept_prefix(s, b, kc))) { delete D; return rc; }
+      if (D->compact && (rc = kept_prefix(s, b, kc))) return rc;
       D->rho_off[NM][b] = (int64_t)r.size();
       for (int i = 0; i < kc; i++)
         for (int j = 0; j < kc; j++) r.push_back(i == j ? std::exp(-bb * s.en[i0 + i]) / Z : 0.0);
@@ -1005,7 +1017,7 @@ static int dm_create(nrg_ctx* c, const nrg_dmnrg_params* p, const unsigned char*
     }
     D->rho_len[NM] = (int64_t)r.size();
     DBuf& rb = c->buf(D->pfx + "R" + std::to_string(NM));
-    if ((rc = rb.reserve(sizeof(double) * (r.size() + 1), true))) { delete D; return rc; }
+    if ((rc = rb.reserve(sizeof(double) * (r.size() + 1), true))) return rc;
     HIPCHK(hipMemcpyAsync(rb.p, r.data(), sizeof(double) * r.size(), hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     if (ntan > 0) {
@@ -1037,7 +1049,7 @@ static int dm_create(nrg_ctx* c, const nrg_dmnrg_params* p, const unsigned char*
         }
       }
       DBuf& drb = c->buf(D->pfx + "dR" + std::to_string(NM));
-      if ((rc = drb.reserve(sizeof(double) * dr.size(), true))) { delete D; return rc; }
+      if ((rc = drb.reserve(sizeof(double) * dr.size(), true))) return rc;
       HIPCHK(hipMemcpyAsync(drb.p, dr.data(), sizeof(double) * dr.size(), hipMemcpyHostToDevice, c->stream));
       HIPCHK(hipStreamSynchronize(c->stream));
     }
@@ -1053,7 +1065,6 @@ static int dm_create(nrg_ctx* c, const nrg_dmnrg_params* p, const unsigned char*
   };
   for (int N = NM - 1; N >= 0; N--) {
     if ((rc = sweep_shell(D, N, fdm, D->temp / calc_dn(p->Lambda, N - 1), ZTot / std::pow(site_dim, NM - N)))) {
-      delete D;
       return rc;
     }
     drop_z(D->sh[N + 1]);
@@ -1064,7 +1075,6 @@ static int dm_create(nrg_ctx* c, const nrg_dmnrg_params* p, const unsigned char*
   auto t2 = std::chrono::steady_clock::now();
   D->ms_forward = std::chrono::duration<double, std::milli>(t1 - t0).count();
   D->ms_sweep = std::chrono::duration<double, std::milli>(t2 - t1).count();
-  *out = D;
   return NRG_OK;
 }
 
