@@ -342,11 +342,17 @@ __global__ __launch_bounds__(CHUNK) void tdb_symv_kernel(const E2Block* __restri
   }
 }
 
-__global__ __launch_bounds__(CHUNK) void tdb_fin_kernel(const E2Block* __restrict__ blks,
-                                                        const int* __restrict__ list, int k, double* __restrict__ A0,
-                                                        double* __restrict__ WS) {
-  __shared__ double red[CHUNK / WAVE];
+// FG thread groups share a chunk's elements: group g sums the row-tile
+// partials rt = g (mod FG) and the panel terms p = g (mod FG) of each element,
+// so each thread keeps a quarter of the loads in flight; group 0 adds the four
+// sums in a fixed order and finishes the element.
+#define FG 4
+__global__ __launch_bounds__(CHUNK * FG) void tdb_fin_kernel(const E2Block* __restrict__ blks,
+                                                             const int* __restrict__ list, int k,
+                                                             double* __restrict__ A0, double* __restrict__ WS) {
+  __shared__ double red[CHUNK * FG / WAVE];
   __shared__ double c1s[PANEL], c2s[PANEL], vk1[PANEL], wk1[PANEL];
+  __shared__ double ys[FG][CHUNK];
   const E2Block B = blks[list[blockIdx.y]];
   const int n = B.n;
   if (k + 2 >= n) return;
@@ -356,26 +362,27 @@ __global__ __launch_bounds__(CHUNK) void tdb_fin_kernel(const E2Block* __restric
   if (c >= CT) return;
   const TdbPtr P = tdb_ptrs(B, A0, WS);
   const int tid = threadIdx.x, j = k % PANEL;
+  const int e = tid % CHUNK, g = tid / CHUNK;
   const double t = P.refl[0], v0 = P.refl[1];
-  // panel dot products summed over the column chunks, split 8 ways over the workgroup
+  // panel dot products summed over the column chunks, split 8 ways over group 0
   __shared__ double cred[2][CHUNK / PANEL][PANEL];
-  {
-    const int pp = tid % PANEL, g = tid / PANEL;
+  if (tid < CHUNK) {
+    const int pp = tid % PANEL, gg = tid / PANEL;
     double a1 = 0.0, a2 = 0.0;
     if (pp < j)
-      for (int cc = g; cc < CT; cc += CHUNK / PANEL) {
+      for (int cc = gg; cc < CT; cc += CHUNK / PANEL) {
         a1 += P.cpart[(size_t)cc * 2 * PANEL + pp];
         a2 += P.cpart[(size_t)cc * 2 * PANEL + PANEL + pp];
       }
-    cred[0][g][pp] = a1;
-    cred[1][g][pp] = a2;
+    cred[0][gg][pp] = a1;
+    cred[1][gg][pp] = a2;
   }
   double tv = 0.0;
-  for (int q = tid; q < CT * RT; q += CHUNK) tv += P.tpart[q];
+  for (int q = tid; q < CT * RT; q += CHUNK * FG) tv += P.tpart[q];
   const double vav = block_sum(tv, red);   // also orders the cred stores
   if (tid < j) {
     double a1 = 0.0, a2 = 0.0;
-    for (int g = 0; g < CHUNK / PANEL; g++) { a1 += cred[0][g][tid]; a2 += cred[1][g][tid]; }
+    for (int gg = 0; gg < CHUNK / PANEL; gg++) { a1 += cred[0][gg][tid]; a2 += cred[1][gg][tid]; }
     c1s[tid] = a1;   // W_p . v
     c2s[tid] = a2;   // V_p . v
   }
@@ -384,49 +391,51 @@ __global__ __launch_bounds__(CHUNK) void tdb_fin_kernel(const E2Block* __restric
   for (int p = 0; p < j; p++) cc12 += c1s[p] * c2s[p];
   const double alpha2 = -0.5 * t * t * (vav - 2.0 * cc12);
   const double* rowk = P.A + (size_t)k * n;
-  const int i = k + 1 + c * CHUNK + tid;
-  double vi = 0.0, wi = 0.0;
-  if (i < n) {
-    vi = (i == k + 1) ? v0 : rowk[i];
-    double y = 0.0;
-    int rt = 0;
-    for (; rt + 8 <= RT; rt += 8) {   // eight partials in flight, summed in row-tile order
-      double a[8];
+  const int i = k + 1 + c * CHUNK + e;
+  // y_i = (A v)_i - sum_p (X_p c1_p + X_{P+p} c2_p), this group's share
+  {
+    double yp = 0.0;
+    if (i < n) {
+      int rt = g;
+      for (; rt + 3 * FG < RT; rt += 4 * FG) {   // four partials in flight
+        double a[4];
 #pragma unroll
-      for (int u = 0; u < 8; u++) a[u] = P.part[(size_t)(rt + u) * n + i];
+        for (int u = 0; u < 4; u++) a[u] = P.part[(size_t)(rt + u * FG) * n + i];
 #pragma unroll
-      for (int u = 0; u < 8; u++) y += a[u];
+        for (int u = 0; u < 4; u++) yp += a[u];
+      }
+      for (; rt < RT; rt += FG) yp += P.part[(size_t)rt * n + i];
+      for (int p = g; p < j; p += FG) yp -= P.X[(size_t)p * n + i] * c1s[p] + P.X[(size_t)(PANEL + p) * n + i] * c2s[p];
     }
-    for (; rt < RT; rt++) y += P.part[(size_t)rt * n + i];
-    int p = 0;
-    for (; p + 4 <= j; p += 4) {
-      double a[4], b[4];
-#pragma unroll
-      for (int u = 0; u < 4; u++) { a[u] = P.X[(size_t)(p + u) * n + i]; b[u] = P.X[(size_t)(PANEL + p + u) * n + i]; }
-#pragma unroll
-      for (int u = 0; u < 4; u++) y -= a[u] * c1s[p + u] + b[u] * c2s[p + u];
-    }
-    for (; p < j; p++) y -= P.X[(size_t)p * n + i] * c1s[p] + P.X[(size_t)(PANEL + p) * n + i] * c2s[p];
-    wi = t * y + alpha2 * vi;
+    ys[g][e] = yp;
   }
   const bool panel_end = (j == PANEL - 1) || (k + 3 >= n);
   if (!panel_end) {
-    // W(k+1, j) is needed by every chunk: recompute it here
-    if (tid == 0) {
+    // W(k+1, j) is needed by every chunk: the second wavefront recomputes it here
+    if (tid >= WAVE && tid < 2 * WAVE) {
+      const int l = tid - WAVE;
       double y = 0.0;
-      for (int rt = 0; rt < RT; rt++) y += P.part[(size_t)rt * n + k + 1];
-      for (int p = 0; p < j; p++)
-        y -= P.X[(size_t)p * n + k + 1] * c1s[p] + P.X[(size_t)(PANEL + p) * n + k + 1] * c2s[p];
-      wk1[j] = t * y + alpha2 * v0;
-      vk1[j] = v0;
+      for (int rt = l; rt < RT; rt += WAVE) y += P.part[(size_t)rt * n + k + 1];
+      if (l < j) y -= P.X[(size_t)l * n + k + 1] * c1s[l] + P.X[(size_t)(PANEL + l) * n + k + 1] * c2s[l];
+      y = wave_sum(y);
+      if (l == 0) {
+        wk1[j] = t * y + alpha2 * v0;
+        vk1[j] = v0;
+      }
     }
     if (tid < j) {
       vk1[tid] = P.X[(size_t)tid * n + k + 1];
       wk1[tid] = P.X[(size_t)(PANEL + tid) * n + k + 1];
     }
   }
-  __syncthreads();   // every read of the panel rows at column k+1 precedes the stores below
+  __syncthreads();   // the group sums, and every read of the panel rows at column k+1, precede the stores below
+  double vi = 0.0, wi = 0.0;
   if (i < n) {
+    vi = (i == k + 1) ? v0 : rowk[i];
+    const double y = (ys[0][e] + ys[1][e]) + (ys[2][e] + ys[3][e]);
+    wi = t * y + alpha2 * vi;
+  }
+  if (g == 0 && i < n) {
     P.X[(size_t)j * n + i] = vi;
     P.X[(size_t)(2 * PANEL + j) * n + i] = vi;
     P.X[(size_t)(PANEL + j) * n + i] = wi;
@@ -434,23 +443,22 @@ __global__ __launch_bounds__(CHUNK) void tdb_fin_kernel(const E2Block* __restric
   }
   if (panel_end) return;
   // column k+1 brought up to date with the panel's columns 0..j
+  __syncthreads();   // group 0's reads of ys precede their reuse
+  {
+    double ap = 0.0;
+    if (i < n)
+      for (int p = g; p < j; p += FG) ap -= P.X[(size_t)p * n + i] * wk1[p] + P.X[(size_t)(PANEL + p) * n + i] * vk1[p];
+    ys[g][e] = ap;
+  }
+  __syncthreads();
   double a = 0.0;
-  if (i < n) {
-    a = P.A[(size_t)(k + 1) * n + i];
-    int p = 0;
-    for (; p + 4 <= j; p += 4) {
-      double x1[4], x2[4];
-#pragma unroll
-      for (int u = 0; u < 4; u++) { x1[u] = P.X[(size_t)(p + u) * n + i]; x2[u] = P.X[(size_t)(PANEL + p + u) * n + i]; }
-#pragma unroll
-      for (int u = 0; u < 4; u++) a -= x1[u] * wk1[p + u] + x2[u] * vk1[p + u];
-    }
-    for (; p < j; p++) a -= P.X[(size_t)p * n + i] * wk1[p] + P.X[(size_t)(PANEL + p) * n + i] * vk1[p];
+  if (g == 0 && i < n) {
+    a = P.A[(size_t)(k + 1) * n + i] + ((ys[0][e] + ys[1][e]) + (ys[2][e] + ys[3][e]));
     a -= vi * wk1[j] + wi * vk1[j];
     P.A[(size_t)(k + 1) * n + i] = a;
   }
-  const double s = block_sum((i > k + 1 && i < n) ? a * a : 0.0, red);
-  if (tid == 0) P.npart[c] = s;
+  const double sq = block_sum((g == 0 && i > k + 1 && i < n) ? a * a : 0.0, red);
+  if (tid == 0) P.npart[c] = sq;
 }
 
 __global__ void tridiag_tail_kernel(const E2Block* __restrict__ blks, const int* __restrict__ list,
@@ -1153,7 +1161,7 @@ hipError_t e2_launch_tdb_fin(const E2Block* b, const int* list, int nlist, int n
   if (nlist <= 0 || k + 2 >= nmax) return hipSuccess;
   const int m = nmax - k - 1;
   const int CT = (m + CHUNK - 1) / CHUNK;
-  hipLaunchKernelGGL(e2::tdb_fin_kernel, dim3(CT, nlist), dim3(CHUNK), 0, s, b, list, k, A, WS);
+  hipLaunchKernelGGL(e2::tdb_fin_kernel, dim3(CT, nlist), dim3(CHUNK * FG), 0, s, b, list, k, A, WS);
   return hipGetLastError();
 }
 
