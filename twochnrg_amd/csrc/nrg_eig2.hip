@@ -987,9 +987,11 @@ __global__ void bt_vt_kernel(const E2Block* __restrict__ blks, const int* __rest
 // T of the compact-WY form, one wavefront per block: T_ii = tau_i,
 // T[0:i,i] = -tau_i T[0:i,0:i] G[0:i,i]; the Gram matrix G is read from global
 // memory (a GEMM output), T is built in LDS and stored transposed
-__global__ __launch_bounds__(64) void bt_tmat_kernel(const E2Block* __restrict__ blks, const int* __restrict__ list,
+__global__ __launch_bounds__(4 * BTP) void bt_tmat_kernel(const E2Block* __restrict__ blks, const int* __restrict__ list,
                                                      const int* __restrict__ p0, double* __restrict__ WS) {
   __shared__ double T[BTP][BTP + 1];
+  __shared__ double Gs[BTP][BTP + 1];   // the Gram matrix, staged once: the recurrence below reads it 32 times
+  __shared__ double ts[BTP];
   const E2Block B = blks[list[blockIdx.x]];
   const int n = B.n, k0 = p0[blockIdx.x];
   if (k0 < 0) return;
@@ -998,15 +1000,24 @@ __global__ __launch_bounds__(64) void bt_tmat_kernel(const E2Block* __restrict__
   const double* G = WS + B.p_off + (size_t)BTP * n;                // BTP x BTP Gram matrix
   double* Tg = WS + B.p_off + (size_t)BTP * n + BTP * BTP;         // BTP x BTP, T^T row-major
   const int tid = threadIdx.x;
-  for (int i = tid; i < BTP * BTP; i += blockDim.x) T[i / BTP][i % BTP] = 0.0;
+  for (int i = tid; i < BTP * BTP; i += blockDim.x) {
+    T[i / BTP][i % BTP] = 0.0;
+    Gs[i / BTP][i % BTP] = G[i];
+  }
+  if (tid < nb) ts[tid] = tau[k0 + tid];
   __syncthreads();
+  // column i of T: T(0:i-1, i) = -tau_i T(0:i-1, 0:i-1) G(0:i-1, i); four
+  // neighbouring lanes share row r (every fourth term each), summed by shuffles
+  const int r = tid >> 2, q = tid & 3;
   for (int i = 0; i < nb; i++) {
-    const double ti = tau[k0 + i];
+    const double ti = ts[i];
     double w = 0.0;
-    if (tid < i)
-      for (int c = tid; c < i; c++) w += T[tid][c] * G[(size_t)c * BTP + i];
+    if (r < i)
+      for (int c = r + q; c < i; c += 4) w += T[r][c] * Gs[c][i];
+    w += __shfl_xor(w, 1, WAVE);
+    w += __shfl_xor(w, 2, WAVE);
     __syncthreads();
-    if (tid < i) T[tid][i] = -ti * w;
+    if (q == 0 && r < i) T[r][i] = -ti * w;
     if (tid == 0) T[i][i] = ti;
     __syncthreads();
   }
@@ -1233,7 +1244,7 @@ hipError_t e2_launch_bt_panel(const E2Block* b, const int* list, const int* p0, 
   if (nt_g > 0) hipLaunchKernelGGL(e2::gemm_kernel, dim3(nt_g), dim3(256), 0, s, gprobs, tiles_g, WS, WS, WS,
                                    (const E2MState*)nullptr);
   if ((e = hipGetLastError()) != hipSuccess) return e;
-  hipLaunchKernelGGL(e2::bt_tmat_kernel, dim3(nlist), dim3(64), 0, s, b, list, p0, WS);
+  hipLaunchKernelGGL(e2::bt_tmat_kernel, dim3(nlist), dim3(4 * BTP), 0, s, b, list, p0, WS);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   const E2Tile* t = tiles_rest;
   if (nt_w > 0) hipLaunchKernelGGL(e2::gemm_kernel, dim3(nt_w), dim3(256), 0, s, gprobs, t, WS, WS, WS,
