@@ -1083,6 +1083,7 @@ static int dm_capture_all(Chain& ch, int N, int opidx, DmShell& s) {
       ftot += (int64_t)ni * nj;
       aux += (int64_t)nj * ni;
       cnt += (int64_t)ni * nj;
+      if (cnt > INT32_MAX) return fail(NRG_ERR_ARG, "DM-NRG: d_up of one shell exceeds 2^31 elements");
       s.op_mbe.push_back((int)cnt - 1);
       maxel = std::max(maxel, (int64_t)ni * nj);
     }
