@@ -1,6 +1,7 @@
 """C-ABI checks that need no GPU: the library loads, exports every symbol the
-public header declares, and its host-side truncation (SetDegen/Ecut/SetKept)
-matches the oracle bit-exactly."""
+public header declares, the structs the binding passes have the header's
+layout, and its host-side truncation (SetDegen/Ecut/SetKept) matches the
+oracle bit-exactly."""
 import os
 import re
 
@@ -66,3 +67,34 @@ def test_bad_arguments_raise():
         t.truncate(np.zeros(0), 1)
     with pytest.raises(t.NrgError):
         t.ecut(np.ones(3), 0)
+
+
+def test_struct_layouts_match_header(tmp_path):
+    """Every struct the Python binding passes through the C-ABI has the size and
+    field offsets the public header gives it (compiled with the host C compiler)."""
+    import ctypes
+    import shutil
+    import subprocess
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no host C compiler")
+    pairs = [("nrg_step", t.NrgStep), ("nrg_tangent_step", t.NrgTangentStep),
+             ("nrg_onechqs_params", t.OneChQSParams), ("nrg_twochqs_params", t.TwoChQSParams),
+             ("nrg_dmnrg_params", t.DmParams), ("nrg_pair_desc", t.PairDesc), ("nrg_stats", t.NrgStats)]
+    lines = ['#include <stddef.h>', '#include <stdio.h>', '#include "twochnrg_amd.h"', "int main(void) {"]
+    expect = []
+    for cname, cls in pairs:
+        lines.append('  printf("%%zu\\n", sizeof(%s));' % cname)
+        expect.append((cname, "sizeof", ctypes.sizeof(cls)))
+        for f in cls._fields_:
+            lines.append('  printf("%%zu\\n", offsetof(%s, %s));' % (cname, f[0]))
+            expect.append((cname, f[0], getattr(cls, f[0]).offset))
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.check_call([cc, "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)])
+    got = [int(x) for x in subprocess.check_output([str(exe)]).split()]
+    assert len(got) == len(expect)
+    bad = [(e[0], e[1], e[2], g) for e, g in zip(expect, got) if e[2] != g]
+    assert not bad, bad
