@@ -115,6 +115,7 @@ extern "C" void nrg_ctx_destroy(nrg_ctx* c) {
   c->resolve();
   for (auto e : c->event_pool) hipEventDestroy(e);
   c->bufs.clear();
+  if (c->ring) hipHostFree(c->ring);
   hipStreamDestroy(c->stream);
   delete c;
 }
@@ -215,6 +216,31 @@ DBuf& nrg_ctx::buf(const std::string& name) {
   return *it->second;
 }
 void nrg_ctx::drop(const std::string& name) { bufs.erase(name); }
+int nrg_ctx::stage_h2d(void* dst, const void* src, size_t bytes) {
+  const size_t kRing = (size_t)64 << 20;
+  const size_t need = (bytes + 255) & ~(size_t)255;
+  if (need > kRing / 4) {
+    // large copies: from the caller's memory, finished before it may be released
+    HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    return NRG_OK;
+  }
+  if (!ring) {
+    void* p = nullptr;
+    HIPCHK(hipHostMalloc(&p, kRing, hipHostMallocDefault));
+    ring = (char*)p;
+    ring_cap = kRing;
+    ring_head = 0;
+  }
+  if (ring_head + need > ring_cap) {   // every copy out of the ring has completed after this
+    HIPCHK(hipStreamSynchronize(stream));
+    ring_head = 0;
+  }
+  std::memcpy(ring + ring_head, src, bytes);
+  HIPCHK(hipMemcpyAsync(dst, ring + ring_head, bytes, hipMemcpyHostToDevice, stream));
+  ring_head += need;
+  return NRG_OK;
+}
 uint64_t nrg_ctx::held() const {
   uint64_t b = 0;
   for (const auto& x : bufs) b += x.second->cap;

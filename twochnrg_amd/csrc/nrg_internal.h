@@ -59,6 +59,11 @@ struct nrg_ctx {
   // move a buffer to another name (replacing what that name held)
   void rename(const std::string& from, const std::string& to);
   int dm_seq = 0;      // DM-NRG handles created on this context (buffer name prefixes)
+  // small host-to-device copies go through a pinned ring on the stream, with no
+  // wait: the ring is reused only after a stream synchronisation
+  char* ring = nullptr;
+  size_t ring_cap = 0, ring_head = 0;
+  int stage_h2d(void* dst, const void* src, size_t bytes);
   uint64_t held() const;   // bytes of every named buffer
   bool timing = false;
   int eig_impl = 2;   // 1: one workgroup per block (Householder + QL); 2: tridiag + D&C + WY
@@ -96,11 +101,7 @@ int upload(nrg_ctx* c, const char* name, const std::vector<T>& v, T** dptr) {
   DBuf& b = c->buf(name);
   int rc = b.reserve(sizeof(T) * (v.size() + 1));
   if (rc) return rc;
-  if (!v.empty()) {
-    // pageable source: wait for the copy before the caller may release it
-    HIPCHK(hipMemcpyAsync(b.p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
-  }
+  if (!v.empty() && (rc = c->stage_h2d(b.p, v.data(), sizeof(T) * v.size()))) return rc;
   *dptr = (T*)b.p;
   return 0;
 }
