@@ -204,7 +204,7 @@ __global__ __launch_bounds__(BT) void broaden_tiles_kernel(const DmTile* __restr
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6][j] = v;
   }
   __syncthreads();
-  if (threadIdx.x < T.ntask) {
+  if ((int)threadIdx.x < T.ntask) {
     const int j = threadIdx.x;
     partial[T.part_beg + (int64_t)j * T.nslab + slab] = (red[0][j] + red[1][j]) + (red[2][j] + red[3][j]);
   }
