@@ -352,7 +352,7 @@ __global__ __launch_bounds__(CHUNK * FG) void tdb_fin_kernel(const E2Block* __re
                                                              double* __restrict__ A0, double* __restrict__ WS) {
   __shared__ double red[CHUNK * FG / WAVE];
   __shared__ double c1s[PANEL], c2s[PANEL], vk1[PANEL], wk1[PANEL];
-  __shared__ double ys[FG][CHUNK];
+  __shared__ double ys[FG][CHUNK], as[FG][CHUNK];
   const E2Block B = blks[list[blockIdx.y]];
   const int n = B.n;
   if (k + 2 >= n) return;
@@ -392,23 +392,6 @@ __global__ __launch_bounds__(CHUNK * FG) void tdb_fin_kernel(const E2Block* __re
   const double alpha2 = -0.5 * t * t * (vav - 2.0 * cc12);
   const double* rowk = P.A + (size_t)k * n;
   const int i = k + 1 + c * CHUNK + e;
-  // y_i = (A v)_i - sum_p (X_p c1_p + X_{P+p} c2_p), this group's share
-  {
-    double yp = 0.0;
-    if (i < n) {
-      int rt = g;
-      for (; rt + 3 * FG < RT; rt += 4 * FG) {   // four partials in flight
-        double a[4];
-#pragma unroll
-        for (int u = 0; u < 4; u++) a[u] = P.part[(size_t)(rt + u * FG) * n + i];
-#pragma unroll
-        for (int u = 0; u < 4; u++) yp += a[u];
-      }
-      for (; rt < RT; rt += FG) yp += P.part[(size_t)rt * n + i];
-      for (int p = g; p < j; p += FG) yp -= P.X[(size_t)p * n + i] * c1s[p] + P.X[(size_t)(PANEL + p) * n + i] * c2s[p];
-    }
-    ys[g][e] = yp;
-  }
   const bool panel_end = (j == PANEL - 1) || (k + 3 >= n);
   if (!panel_end) {
     // W(k+1, j) is needed by every chunk: the second wavefront recomputes it here
@@ -427,36 +410,54 @@ __global__ __launch_bounds__(CHUNK * FG) void tdb_fin_kernel(const E2Block* __re
       vk1[tid] = P.X[(size_t)tid * n + k + 1];
       wk1[tid] = P.X[(size_t)(PANEL + tid) * n + k + 1];
     }
+    __syncthreads();   // wk1, vk1 before the element pass
   }
-  __syncthreads();   // the group sums, and every read of the panel rows at column k+1, precede the stores below
-  double vi = 0.0, wi = 0.0;
-  if (i < n) {
+  // one pass over this group's share of each element: y_i = (A v)_i -
+  // sum_p (X_p c1_p + X_{P+p} c2_p) and, inside a panel, the correction
+  // -sum_p (X_p wk1_p + X_{P+p} vk1_p) of column k+1 from the same X loads
+  {
+    double yp = 0.0, ap = 0.0;
+    if (i < n) {
+      int rt = g;
+      for (; rt + 3 * FG < RT; rt += 4 * FG) {   // four partials in flight
+        double pa[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) pa[u] = P.part[(size_t)(rt + u * FG) * n + i];
+#pragma unroll
+        for (int u = 0; u < 4; u++) yp += pa[u];
+      }
+      for (; rt < RT; rt += FG) yp += P.part[(size_t)rt * n + i];
+      if (panel_end) {
+        for (int p = g; p < j; p += FG) yp -= P.X[(size_t)p * n + i] * c1s[p] + P.X[(size_t)(PANEL + p) * n + i] * c2s[p];
+      } else {
+        for (int p = g; p < j; p += FG) {
+          const double x1 = P.X[(size_t)p * n + i], x2 = P.X[(size_t)(PANEL + p) * n + i];
+          yp -= x1 * c1s[p] + x2 * c2s[p];
+          ap -= x1 * wk1[p] + x2 * vk1[p];
+        }
+      }
+    }
+    ys[g][e] = yp;
+    as[g][e] = ap;
+  }
+  __syncthreads();   // the group sums, and every read of the panel rows, precede the stores below
+  double vi = 0.0, wi = 0.0, a = 0.0;
+  if (g == 0 && i < n) {
     vi = (i == k + 1) ? v0 : rowk[i];
     const double y = (ys[0][e] + ys[1][e]) + (ys[2][e] + ys[3][e]);
     wi = t * y + alpha2 * vi;
-  }
-  if (g == 0 && i < n) {
     P.X[(size_t)j * n + i] = vi;
     P.X[(size_t)(2 * PANEL + j) * n + i] = vi;
     P.X[(size_t)(PANEL + j) * n + i] = wi;
     if (i == k + 1) P.A[(size_t)k * n + k + 1] = v0;
+    if (!panel_end) {
+      // column k+1 brought up to date with the panel's columns 0..j
+      a = P.A[(size_t)(k + 1) * n + i] + ((as[0][e] + as[1][e]) + (as[2][e] + as[3][e]));
+      a -= vi * wk1[j] + wi * vk1[j];
+      P.A[(size_t)(k + 1) * n + i] = a;
+    }
   }
   if (panel_end) return;
-  // column k+1 brought up to date with the panel's columns 0..j
-  __syncthreads();   // group 0's reads of ys precede their reuse
-  {
-    double ap = 0.0;
-    if (i < n)
-      for (int p = g; p < j; p += FG) ap -= P.X[(size_t)p * n + i] * wk1[p] + P.X[(size_t)(PANEL + p) * n + i] * vk1[p];
-    ys[g][e] = ap;
-  }
-  __syncthreads();
-  double a = 0.0;
-  if (g == 0 && i < n) {
-    a = P.A[(size_t)(k + 1) * n + i] + ((ys[0][e] + ys[1][e]) + (ys[2][e] + ys[3][e]));
-    a -= vi * wk1[j] + wi * vk1[j];
-    P.A[(size_t)(k + 1) * n + i] = a;
-  }
   const double sq = block_sum((g == 0 && i > k + 1 && i < n) ? a * a : 0.0, red);
   if (tid == 0) P.npart[c] = sq;
 }
