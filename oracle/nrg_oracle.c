@@ -2002,6 +2002,8 @@ struct orc_dm {
   int method;              /* spec1.UseCFS: 0 DM-NRG, 1 CFS, 2 FDM */
   double ztot;             /* FDM: Z_tot (src/DM_NRG/DM_NRG.cpp:203-268) */
   double *wn;              /* FDM: w_N per shell */
+  int fix;                 /* FDM: ORC_FDM_FIX_* bits in effect (0 unless FDM) */
+  double bb_last;          /* FDM with ORC_FDM_FIX_LAST: 1/TempBar of the last shell */
   arr_t site;
   int nomega;
   double *omega, *rhow;
@@ -2263,7 +2265,7 @@ static double bdelta_ij(const struct orc_dm *D, const arr_t *A, double w, double
 }
 
 /* CalcSpecCosti_T_N  src/SpecFuncClass.cpp:417-662 (real operators); CalcNorm: BD -> 1 */
-static double spec_costi(const struct orc_dm *D, int N, double bb, double w, int norm) {
+static double spec_costi(const struct orc_dm *D, int N, double bb, double w, int norm, double zdiv) {
   const arr_t *A = &D->acut[N];
   const mat_t *Op = &D->op[N];
   double acc = 0.0;
@@ -2280,7 +2282,7 @@ static double spec_costi(const struct orc_dm *D, int N, double bb, double w, int
       }
     acc += (A->sz ? 1.0 : op_cgfactor(getqn(A, b1, 1), getqn(A, b2, 1))) * tr;
   }
-  return acc / partition_func(A, bb);
+  return acc / (zdiv > 0.0 ? zdiv : partition_func(A, bb));
 }
 
 /* DMNRG_SpecDens_M_STL  src/SpecFunc_allSTL.cpp:223-692 (real operators,
@@ -2291,8 +2293,11 @@ static double spec_kd(const struct orc_dm *D, int N, double w, int norm);
 double orc_dm_spec_shell(const orc_dm *D, int N, double w) {
   if (N > D->spec_nshell_max - 1) return 0.0;
   if (N == D->spec_nshell_max - 1) {
+    /* FDM without the Costi quirk: the last shell's states are all discarded,
+     * rho_Nmax = e^{-bE}/Z_tot at the scale of its FDM weight */
+    if (D->fix & ORC_FDM_FIX_LAST) return spec_costi(D, N, D->bb_last, w, 0, D->ztot);
     double bb = (fabs(D->temp) < 1e-20) ? 1.0e10 : D->betabar_spec;
-    return spec_costi(D, N, bb, w, 0);
+    return spec_costi(D, N, bb, w, 0, 0.0);
   }
   if (D->method != 0) return spec_kd(D, N, w, 0);
   const arr_t *A = &D->acut[N];
@@ -2398,7 +2403,8 @@ static double spec_kd(const struct orc_dm *D, int N, double w, int norm) {
     int n1 = bsize(A, b1), n2 = bsize(A, b2), i0 = blim(A, b1, 0), j0 = blim(A, b2, 0);
     int k1 = bsize_kp(A, b1, 1), d1 = bsize_kp(A, b1, 0), k2 = bsize_kp(A, b2, 1), d2 = bsize_kp(A, b2, 0);
     if (n1 != k1 + d1 || n2 != k2 + d2) return 0.0;   /* "Kept+Disc NOT equal to Nstblock" */
-    if ((k1 == 0 && k2 == 0) || (d1 == 0 && d2 == 0)) continue;
+    if (d1 == 0 && d2 == 0) continue;   /* kept-kept only: no pole at this shell */
+    if (k1 == 0 && k2 == 0 && !(D->fix & ORC_FDM_FIX_PAIRS)) continue;
     int *K1 = (int *)malloc(sizeof(int) * (size_t)(n1 + 1)), *D1 = (int *)malloc(sizeof(int) * (size_t)(n1 + 1));
     int *K2 = (int *)malloc(sizeof(int) * (size_t)(n2 + 1)), *D2 = (int *)malloc(sizeof(int) * (size_t)(n2 + 1));
     int a = 0, b = 0;
@@ -2416,22 +2422,26 @@ static double spec_kd(const struct orc_dm *D, int N, double w, int norm) {
       }
     for (int i = 0; i < n1; i++) for (int j = 0; j < n1; j++) R1[i * n1 + j] = get_block_matel(R, b1, b1, i, j);
     for (int i = 0; i < n2; i++) for (int j = 0; j < n2; j++) R2[i * n2 + j] = get_block_matel(R, b2, b2, i, j);
+    /* ORC_FDM_FIX_NORM: every term at both signs of omega (the broadening picks the side) */
+    const int both = fdm && (D->fix & ORC_FDM_FIX_NORM);
+    const int onesided = fdm && (D->fix & ORC_FDM_FIX_ONESIDED);
     double t1kk = 0, t2kk = 0, t1dd = 0, t1dk = 0, t2dd = 0, t2kd = 0;
-    if (k1 && d2 && (w >= 0.0 || norm)) t1kk = kd_term1(Ab, n2, R1, n1, K1, k1, D2, d2, Dl);
-    if (fdm && d1 && d2) {
-      t1dd = kd_term1(Ab, n2, R1, n1, D1, d1, D2, d2, Dl);
-      if (k2) t1dk = kd_term1(Ab, n2, R1, n1, D1, d1, K2, k2, Dl);
-    }
-    if (d1 && k2 && (w < 0.0 || norm)) t2kk = kd_term2(Ab, n2, R2, D1, d1, K2, k2, Dl);
-    if (fdm && d1 && d2) {
-      t2dd = kd_term2(Ab, n2, R2, D1, d1, D2, d2, Dl);
-      if (k1) t2kd = kd_term2(Ab, n2, R2, K1, k1, D2, d2, Dl);
-    }
+    if (k1 && d2 && (w >= 0.0 || norm || both)) t1kk = kd_term1(Ab, n2, R1, n1, K1, k1, D2, d2, Dl);
+    if (fdm && d1 && d2) t1dd = kd_term1(Ab, n2, R1, n1, D1, d1, D2, d2, Dl);
+    if (fdm && d1 && k2 && (d2 || onesided)) t1dk = kd_term1(Ab, n2, R1, n1, D1, d1, K2, k2, Dl);
+    if (d1 && k2 && (w < 0.0 || norm || both)) t2kk = kd_term2(Ab, n2, R2, D1, d1, K2, k2, Dl);
+    if (fdm && d1 && d2) t2dd = kd_term2(Ab, n2, R2, D1, d1, D2, d2, Dl);
+    if (fdm && k1 && d2 && (d1 || onesided)) t2kd = kd_term2(Ab, n2, R2, K1, k1, D2, d2, Dl);
     double cg = A->sz ? 1.0 : op_cgfactor(getqn(A, b1, 1), getqn(A, b2, 1));
-    pos += cg * (t1kk + t1dd + t2dd + t2kd);
-    neg += cg * (t2kk + t1dd + t1dk + t2dd);
+    if (both) {
+      pos += cg * (t1kk + t2kk + t1dd + t1dk + t2dd + t2kd);
+    } else {
+      pos += cg * (t1kk + t1dd + t2dd + t2kd);
+      neg += cg * (t2kk + t1dd + t1dk + t2dd);
+    }
     free(K1); free(D1); free(K2); free(D2); free(Ab); free(Dl); free(R1); free(R2);
   }
+  if (fdm && (D->fix & ORC_FDM_FIX_NORM)) return pos;
   if (norm) return pos + neg;
   return (w >= 0.0) ? pos : neg;
 }
@@ -2459,8 +2469,9 @@ double orc_dm_norm(const orc_dm *D) {
   double s = 0.0;
   for (int N = 0; N < D->spec_nshell_max; N++) {
     if (N == D->spec_nshell_max - 1) {
+      if (D->fix & ORC_FDM_FIX_LAST) { s += spec_costi(D, N, D->bb_last, 0.0, 1, D->ztot); continue; }
       double bb = (fabs(D->temp) < 1e-20) ? 1.0e10 : D->betabar_spec;
-      s += spec_costi(D, N, bb, 0.0, 1);
+      s += spec_costi(D, N, bb, 0.0, 1, 0.0);
     } else {
       s += spec_kd(D, N, 0.0, 1);
     }
@@ -2653,25 +2664,33 @@ orc_dm *orc_dm_run_jvp(const orc_dmnrg_params *p, const unsigned char *kept, con
   D->betabar_spec = p->betabar;
   D->nshell_max = Nsitesmax - 1;
   D->spec_nshell_max = Nsitesmax;
+  D->fix = fdm ? (p->fdm_fix & ORC_FDM_EXACT) : 0;
+  /* shell N's Boltzmann factors: Temp/D_{N-1} as the reference (src/DM_NRG/DM_NRG.cpp:203-214,240,349),
+   * or Temp/D_N without that quirk */
+  const int dsh = (D->fix & ORC_FDM_FIX_DN) ? 0 : 1;
   int NM = D->nshell_max;
   D->rho = (mat_t *)calloc((size_t)NM + 1, sizeof(mat_t));
   D->trace = (double *)calloc((size_t)NM + 1, sizeof(double));
-  double TempBar = Temp / calc_dn(p->Lambda, NM - 1);
+  double TempBar = Temp / calc_dn(p->Lambda, NM - dsh);
   double ZN = partition_func(&D->acut[NM], 1.0 / TempBar);
   D->wn = (double *)calloc((size_t)NM + 1, sizeof(double));
   double ZTot = ZN;
   D->wn[NM] = ZN;
   for (int N = NM - 1; N >= 0; N--) {
-    double tb = Temp / calc_dn(p->Lambda, N - 1);
+    double tb = Temp / calc_dn(p->Lambda, N - dsh);
     double zd = partition_func_disc(&D->acut[N], 1.0 / tb) * pow(site_dim, NM - N);
     D->wn[N] = zd;
     ZTot += zd;
   }
   for (int N = 0; N <= NM; N++) D->wn[N] = fdm ? D->wn[N] / ZTot : 0.0;
   D->ztot = fdm ? ZTot : 0.0;
-  D->trace[NM] = set_rho_nmax(Temp / DM, fdm ? ZTot : ZN, &D->acut[NM], &D->rho[NM]);
+  D->bb_last = 1.0 / TempBar;
+  /* rho_Nmax's exponent: Temp/D_Mtemp as the reference (src/DM_NRG/DM_NRG.cpp:102-118,273-297), or the
+   * scale of Z_tot and w_Nmax */
+  D->trace[NM] = set_rho_nmax((D->fix & ORC_FDM_FIX_RHOMAX) ? TempBar : Temp / DM, fdm ? ZTot : ZN, &D->acut[NM],
+                              &D->rho[NM]);
   for (int N = NM - 1; N >= 0; N--) {
-    double tb = Temp / calc_dn(p->Lambda, N - 1);
+    double tb = Temp / calc_dn(p->Lambda, N - dsh);
     D->trace[N] = calc_rho_n(&D->acut[N], &D->acut[N + 1], &D->abasis[N + 1], &D->site, &D->rho[N + 1], &D->rho[N],
                              fdm, tb, ZTot / pow(site_dim, NM - N));
   }

@@ -116,7 +116,27 @@ typedef struct {
   double Gamma2;              /* TwoChQSz second channel */
   double fdm_site_dim;        /* FDM environment count per later site: 0 -> 4 for model 0 (the reference's
                                  value, src/DM_NRG/DM_NRG.cpp:244,365) and 16 for model 1 (two channels) */
+  int fdm_fix;                /* FDM only: ORC_FDM_FIX_* bits switch the reference's FDM quirks off one by
+                                 one; 0 = the reference, ORC_FDM_EXACT = the full density matrix of
+                                 Weichselbaum and von Delft (sum rule exact) */
 } orc_dmnrg_params;
+
+/* Quirks of the reference's FDM path, each switched off by one bit of fdm_fix. */
+#define ORC_FDM_FIX_RHOMAX 1    /* rho_Nmax's exponent at the same shell scale as Z_tot and w_Nmax
+                                   (reference: Temp/D_Mtemp, src/DM_NRG/DM_NRG.cpp:102-118,273-297) */
+#define ORC_FDM_FIX_DN 2        /* shell N's Boltzmann factors at D_N/Temp (reference: D_{N-1},
+                                   src/DM_NRG/DM_NRG.cpp:203-214,340) */
+#define ORC_FDM_FIX_LAST 4      /* the last shell as an FDM shell with rho_Nmax = e^{-bE}/Z_tot (reference:
+                                   the Costi form with its own 1/Z, src/SpecFunc_allSTL.cpp:1270-1277) */
+#define ORC_FDM_FIX_PAIRS 8     /* block pairs without a kept state keep their discarded-discarded terms
+                                   (reference skips them, src/SpecFunc_allSTL.cpp:1375-1381) */
+#define ORC_FDM_FIX_ONESIDED 16 /* kept-discarded terms with rho on the discarded side also when the other
+                                   block has no discarded state (reference: only inside the
+                                   both-blocks-discarded branch) */
+#define ORC_FDM_FIX_NORM 32     /* every pole carries its whole weight at both signs of omega and CalcNorm
+                                   counts it once (reference: DD terms in Positive and Negative, so CalcNorm
+                                   counts them twice, src/SpecFunc_allSTL.cpp:2328-2348) */
+#define ORC_FDM_EXACT 63
 
 typedef struct orc_dm orc_dm;
 

@@ -307,7 +307,13 @@ class DmParams(ctypes.Structure):
         ("broadtemp", ctypes.c_double), ("bbroad", ctypes.c_double), ("Nw", ctypes.c_int),
         ("method", ctypes.c_int), ("update_bef_cut", ctypes.c_int),
         ("model", ctypes.c_int), ("Gamma2", ctypes.c_double), ("fdm_site_dim", ctypes.c_double),
+        ("fdm_fix", ctypes.c_int),
     ]
+
+
+# bits of fdm_fix (nrg_oracle.h ORC_FDM_FIX_*): each switches one quirk of the reference's FDM off
+FDM_FIX_RHOMAX, FDM_FIX_DN, FDM_FIX_LAST, FDM_FIX_PAIRS, FDM_FIX_ONESIDED, FDM_FIX_NORM = 1, 2, 4, 8, 16, 32
+FDM_EXACT = 63
 
 
 _PD = ctypes.POINTER(ctypes.c_double)
@@ -338,7 +344,7 @@ class DmRun:
 
     def __init__(self, U, Gamma, ed, Lambda, nsitesmax, ncut, Mtemp, betabar, disc_scheme=0,
                  twindow=0.0, broadtemp=0.0, bbroad=0.0, Nw=0, myeps=0.0, betabar_nrg=0.0,
-                 method=0, update_bef_cut=0, kept=None, dpar=None, model=0, Gamma2=0.0, fdm_site_dim=0.0):
+                 method=0, update_bef_cut=0, kept=None, dpar=None, model=0, Gamma2=0.0, fdm_site_dim=0.0, fdm_fix=0):
         """kept = (flags, offsets) from kept_arrays of an NRG run freezes every
         step's Kept set; dpar rows are tangent directions (dU, dGamma, ded)
         carried through the chain and the sweep (orc_dm_run_jvp)."""
@@ -361,7 +367,7 @@ class DmRun:
         L.orc_dm_norm.argtypes = [ctypes.c_void_p]
         L.orc_dm_ztot.argtypes = [ctypes.c_void_p]
         p = DmParams(U, Gamma, ed, Lambda, nsitesmax, ncut, disc_scheme, myeps, betabar_nrg, Mtemp, betabar,
-                     twindow, broadtemp, bbroad, Nw, method, update_bef_cut, model, Gamma2, fdm_site_dim)
+                     twindow, broadtemp, bbroad, Nw, method, update_bef_cut, model, Gamma2, fdm_site_dim, fdm_fix)
         self.method = method
         self._L = L
         dp_ = np.ascontiguousarray(np.zeros((0, 3)) if dpar is None else np.asarray(dpar, np.float64).reshape(-1, 3))

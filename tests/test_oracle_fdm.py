@@ -111,23 +111,28 @@ def _nrg_fock(p, d):
     return shells
 
 
-def _rho_fock(sh, d, mtemp, betabar, fdm):
+def _rho_fock(sh, d, mtemp, betabar, fdm, exact=False):
+    """exact: the full density matrix of Weichselbaum and von Delft, every shell's
+    Boltzmann factors at e^{-E D_N/T} (rho_Nmax included); otherwise the
+    reference's D_{N-1} scales and rho_Nmax exponent betabar = D_Mtemp/T."""
     NM = d.nshell_max
     temp = dn(mtemp) / betabar
-    ZN = np.sum(np.exp(-sh[NM]["E"] * dn(NM - 1) / temp))
+    sc = (lambda N: dn(N)) if exact else (lambda N: dn(N - 1))
+    ZN = np.sum(np.exp(-sh[NM]["E"] * sc(NM) / temp))
     ztot = ZN
     for N in range(NM - 1, -1, -1):
         dsc = sh[N]["kept"] == 0
-        ztot += np.sum(np.exp(-sh[N]["E"][dsc] * dn(N - 1) / temp)) * 4.0 ** (NM - N)
+        ztot += np.sum(np.exp(-sh[N]["E"][dsc] * sc(N) / temp)) * 4.0 ** (NM - N)
     norm = ztot if fdm else ZN
     V = sh[NM]["V"]
-    rho = {NM: (V * (np.exp(-betabar * sh[NM]["E"]) / norm)) @ V.T}
+    bmax = sc(NM) / temp if exact else betabar
+    rho = {NM: (V * (np.exp(-bmax * sh[NM]["E"]) / norm)) @ V.T}
     for N in range(NM - 1, -1, -1):
         R = partial_trace_last_orbital(rho[N + 1], 2 * (N + 3))
         if fdm:
             dsc = sh[N]["kept"] == 0
             Vd = sh[N]["V"][:, dsc]
-            w = np.exp(-sh[N]["E"][dsc] * dn(N - 1) / temp) * 4.0 ** (NM - N) / ztot
+            w = np.exp(-sh[N]["E"][dsc] * sc(N) / temp) * 4.0 ** (NM - N) / ztot
             R = R + (Vd * w) @ Vd.T
         rho[N] = R
     return rho, temp, (ztot if fdm else 0.0)
@@ -163,7 +168,9 @@ def _bd_(method, w, x, tdn, twindow, bt, b):
     return np.where(np.abs(x) < twindow * tdn, tw, lg)
 
 
-def _shell_sum(s, rho, method, w, temp, N, norm=False):
+def _shell_sum(s, rho, method, w, temp, N, norm=False, exact=False):
+    """exact: every pole with a discarded state at either end, weight
+    (rho_a + rho_b)|c_ab|^2, at both signs of omega (the broadening picks the side)."""
     V = s["V"]
     cm = V.T @ s["C"][0] @ V
     r = V.T @ rho @ V
@@ -178,6 +185,8 @@ def _shell_sum(s, rho, method, w, temp, N, norm=False):
     skip = ((kc[b1] == 0) & (kc[b2] == 0)) | ((dc[b1] == 0) & (dc[b2] == 0))
     aK, aD = (kp == 1)[:, None], (kp == 0)[:, None]
     bK, bD = (kp == 1)[None, :], (kp == 0)[None, :]
+    if exact:
+        return np.sum(np.where(aK & bK, 0.0, W1 + W2) * Dl)
     if method == 2:
         pos = np.where(aK & bD, W1 + np.where(dc[b1] > 0, W2, 0.0), 0.0) + np.where(aD & bD, W1 + W2, 0.0)
         neg = np.where(aD & bK, W2 + np.where(dc[b2] > 0, W1, 0.0), 0.0) + np.where(aD & bD, W1 + W2, 0.0)
@@ -191,12 +200,13 @@ def _shell_sum(s, rho, method, w, temp, N, norm=False):
     return pos if w >= 0 else neg
 
 
-def _costi(s, method, w, temp, N, betabar, norm=False):
+def _costi(s, method, w, temp, N, betabar, norm=False, ztot=None):
+    """ztot given: the last shell of the exact FDM, rho = e^{-E D_N/T}/Z_tot."""
     V = s["V"]
     cm = V.T @ s["C"][0] @ V
     E = s["E"]
-    x = np.exp(-betabar * E)
-    Z = np.sum(x)
+    x = np.exp(-(dn(N) / temp if ztot else betabar) * E)
+    Z = ztot if ztot else np.sum(x)
     dE = E[None, :] - E[:, None]
     Dl = np.ones_like(dE) if norm else _bd(method, w, dE, temp / dn(N))
     return np.sum(cm * cm * (x[:, None] + x[None, :]) * Dl) / Z
@@ -209,6 +219,10 @@ CASES = [
     dict(p=dict(U=0.6, Gamma=0.07, ed=-0.2), method=2, nsites=4, ncut=6, mtemp=1, betabar=0.9),
     # CFS: rho from the plain sweep over kept + discarded states, the chain cut at Mtemp+1
     dict(p=dict(U=0.5, Gamma=0.05, ed=-0.25), method=1, nsites=6, ncut=5, mtemp=3, betabar=1.1),
+    # exact FDM (every quirk switched off), Mtemp < Nmax-1, off particle-hole symmetry
+    dict(p=dict(U=0.6, Gamma=0.07, ed=-0.2), method=2, nsites=4, ncut=6, mtemp=1, betabar=0.9, fix=63),
+    # exact FDM, Mtemp = Nmax-1
+    dict(p=dict(U=0.5, Gamma=0.05, ed=-0.25), method=2, nsites=4, ncut=5, mtemp=2, betabar=1.3, fix=63),
 ]
 
 
@@ -217,9 +231,10 @@ def case(request):
     c = CASES[request.param]
     p = c["p"]
     d = po.DmRun(U=p["U"], Gamma=p["Gamma"], ed=p["ed"], Lambda=LAM, nsitesmax=c["nsites"], ncut=c["ncut"],
-                 Mtemp=c["mtemp"], betabar=c["betabar"], method=c["method"], update_bef_cut=1)
+                 Mtemp=c["mtemp"], betabar=c["betabar"], method=c["method"], update_bef_cut=1,
+                 fdm_fix=c.get("fix", 0))
     sh = _nrg_fock(p, d)
-    rho, temp, ztot = _rho_fock(sh, d, c["mtemp"], c["betabar"], c["method"] == 2)
+    rho, temp, ztot = _rho_fock(sh, d, c["mtemp"], c["betabar"], c["method"] == 2, c.get("fix", 0) == 63)
     return c, d, sh, rho, temp, ztot
 
 
@@ -256,13 +271,15 @@ def test_rho_matches_fock_space(case):
 def test_shell_sums_match_fock_space(case):
     c, d, sh, rho, temp, ztot = case
     NM = d.nshell_max
+    ex = c.get("fix", 0) == 63
+    zl = ztot if ex else None
     for N in range(NM):
         for w in (-2.3, -0.6, -0.04, 0.0, 0.03, 0.5, 1.7):
-            ref = _shell_sum(sh[N], rho[N], c["method"], w, temp, N)
+            ref = _shell_sum(sh[N], rho[N], c["method"], w, temp, N, exact=ex)
             got = d.spec_shell(N, w)
             assert abs(got - ref) <= 1e-10 * max(1e-8, abs(ref)), (N, w, got, ref)
     for w in (-1.1, -0.1, 0.2, 2.0):
-        ref = _costi(sh[NM], c["method"], w, temp, NM, c["betabar"])
+        ref = _costi(sh[NM], c["method"], w, temp, NM, c["betabar"], ztot=zl)
         got = d.spec_shell(NM, w)
         assert abs(got - ref) <= 1e-10 * max(1e-8, abs(ref)), (w, got, ref)
 
@@ -270,9 +287,16 @@ def test_shell_sums_match_fock_space(case):
 def test_norm_and_window_sum(case):
     c, d, sh, rho, temp, ztot = case
     NM = d.nshell_max
-    nrm = sum(_shell_sum(sh[N], rho[N], c["method"], 0.0, temp, N, norm=True) for N in range(NM))
-    nrm += _costi(sh[NM], c["method"], 0.0, temp, NM, c["betabar"], norm=True)
+    ex = c.get("fix", 0) == 63
+    zl = ztot if ex else None
+    nrm = sum(_shell_sum(sh[N], rho[N], c["method"], 0.0, temp, N, norm=True, exact=ex) for N in range(NM))
+    nrm += _costi(sh[NM], c["method"], 0.0, temp, NM, c["betabar"], norm=True, ztot=zl)
     assert abs(d.norm() - nrm) <= 1e-11 * abs(nrm)
+    if ex:   # the full density matrix: sum rule <{c, c^dag}> = 1 and unit trace
+        assert abs(nrm - 1.0) <= 1e-12, nrm
+        tr = sum(np.trace(rho[N]) - np.trace(partial_trace_last_orbital(rho[N + 1], 2 * (N + 3)))
+                 for N in range(NM)) + np.trace(rho[NM])
+        assert abs(tr - 1.0) <= 1e-12, tr
     # CalcSpecDM_NRG: shells [N_w-8, N_w+14) of the spectral chain, A_N(w/D_N)/D_N
     for w in (-0.9, -0.05, 0.004, 0.3, 1.2):
         Nw = 1 - int(2.0 * np.log(2.0 * abs(w) / HLF) / np.log(LAM))
@@ -280,9 +304,9 @@ def test_norm_and_window_sum(case):
         ref = 0.0
         for N in range(max(Nw - 8, 0), min(Nw + 14, d.spec_nshell_max)):
             if N == NM:
-                ref += _costi(sh[N], c["method"], w / dn(N), temp, N, c["betabar"]) / dn(N)
+                ref += _costi(sh[N], c["method"], w / dn(N), temp, N, c["betabar"], ztot=zl) / dn(N)
             else:
-                ref += _shell_sum(sh[N], rho[N], c["method"], w / dn(N), temp, N) / dn(N)
+                ref += _shell_sum(sh[N], rho[N], c["method"], w / dn(N), temp, N, exact=ex) / dn(N)
         got = d.spec_omega(w)
         assert abs(got - ref) <= 1e-10 * max(1e-8, abs(ref)), (w, got, ref)
 

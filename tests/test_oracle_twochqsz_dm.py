@@ -160,3 +160,32 @@ def test_fdm_site_dimension():
     np.testing.assert_allclose(w16[1:] / w4[1:], 4.0 ** (NM - np.arange(1, NM + 1)), rtol=1e-12)
     for D in (D4, D16):
         assert abs(sum(D.shell(N)["wn"] for N in range(NM + 1)) - 1.0) <= 1e-12
+
+
+@pytest.mark.parametrize("site_dim", [16.0, 4.0])
+def test_exact_fdm_sum_rule(site_dim):
+    """The full density matrix (every FDM quirk of the reference switched off,
+    fdm_fix = 63) on a truncated TwoChQSz chain with Mtemp < Nmax-1: CalcNorm is
+    the sum rule <{d, d^dag}> = 1 and the density matrix has unit trace
+    (sum over shells of the discarded diagonal plus rho_Nmax), whatever the
+    environment count d.  Each quirk that touches the weights, switched back
+    on alone, breaks the sum rule: rho_Nmax's exponent at D_Mtemp, the Costi
+    last shell, the skipped all-discarded block pairs and CalcNorm's double
+    count of the discarded-discarded terms."""
+    c = dict(U=0.5, Gamma=0.03, ed=-0.25, Lambda=2.5, nsitesmax=5, ncut=120, Mtemp=2, betabar=0.779,
+             method=2, model=1, Gamma2=0.01, bbroad=0.6, fdm_site_dim=site_dim)
+    D = po.DmRun(**c, fdm_fix=po.FDM_EXACT)
+    NM = D.nshell_max
+    sh = [D.shell(N) for N in range(NM + 1)]
+    assert any(not s["kept"].all() for s in sh[:NM])
+    assert abs(D.norm() - 1.0) <= 1e-10, D.norm()
+    tr = sh[NM]["trace"]
+    for N in range(NM):
+        dsc = sh[N]["kept"] == 0
+        tr += sum(np.diag(r)[dsc[b0:b1 + 1]].sum() for r, (b0, b1) in zip(sh[N]["rho"], sh[N]["bbe"]))
+    assert abs(tr - 1.0) <= 1e-12, tr
+    for bit in (po.FDM_FIX_RHOMAX, po.FDM_FIX_LAST, po.FDM_FIX_PAIRS, po.FDM_FIX_NORM):
+        n = po.DmRun(**c, fdm_fix=po.FDM_EXACT & ~bit).norm()
+        assert abs(n - 1.0) > 1e-4, (bit, n)
+    # the reference's own FDM (no bit) is not normalised on this chain
+    assert abs(po.DmRun(**c).norm() - 1.0) > 1e-2
