@@ -12,6 +12,7 @@ a bounded sample of the same workload.
 """
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -22,9 +23,6 @@ sys.path.insert(0, ROOT)
 METRIC = "NRG steps/s at Ncut=3000, batched vs plain GPU (1 GPU); DM-NRG A(ω) wall time"
 FP64_PEAK_TFLOPS = 78.6
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per tdb_symv launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on
-# this workload, scaled by the copy-kernel calibration (profiles/tdb_symv_twochqs_ncut3000_traffic.md)
-TRAFFIC = {}
 
 WORKLOADS = {
     # two-channel Anderson (TwoChQS), the Ncut=3000 configuration of the metric
@@ -143,8 +141,11 @@ def fdm_run(cpu=True, cpu_omegas=12):
             tm = d.timing()
             norm = d.norm()
             wn, ztot = d.weights()
+    a0 = 0.5 * (float(r[999]) + float(r[1000]))   # A(omega = +-1e-10), omega << T << T_K
     out.update({"wall_ms": 1e3 * (t1 - t0), "breakdown_ms": tm, "post_ms": tm["sweep"] + tm["poles"] + tm["broaden"],
-                "norm": norm, "ztot": ztot, "shell_weight_max": int(np.argmax(wn))})
+                "norm": norm, "ztot": ztot, "shell_weight_max": int(np.argmax(wn)),
+                # Friedel sum rule at particle-hole symmetry: pi Gamma A(0) = 1 (the chain carries A_Lambda)
+                "friedel_pi_gamma_A0": math.pi * DM_C1["Gamma"] * a0})
     print("bench fdm: device done (%.0f ms)" % out["wall_ms"], file=sys.stderr, flush=True)
     if cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -166,14 +167,18 @@ def fdm_run(cpu=True, cpu_omegas=12):
 C5 = dict(U=0.5, Gamma=0.03, Gamma2=0.01, ed=-0.25, Lambda=2.5, nsitesmax=49, ncut=4000, model=1)
 
 
-def c5_run(cpu=True, nsitesmax=None, ncut=None, site_dims=(16.0, 4.0)):
+def c5_run(cpu=True, nsitesmax=None, ncut=None, modes=((16.0, 0), (4.0, 0), (16.0, 63))):
     """FDM A(omega) of d_up (DM_NRG -F -e 0.6) at T = 1e-9 (Mtemp 46) on the
     TwoChQSz two-channel chain, 50 sites (N = 0 .. 49) keeping 4000 states, on
     omega = +-logspace(-10, 0, 1000); wall = forward chain with every shell's
-    states and d_up kept + FDM sweep + pole weights + the 2000 omegas.  Once with
-    the two-channel environment count d = 16 per later site and once with the
-    reference's one-channel d = 4.  The CPU oracle runs the same model at a
-    size it finishes in seconds (5 sites, 200 states), with 8 omegas."""
+    states and d_up kept + FDM sweep + pole weights + the 2000 omegas.  Runs
+    (d, fdm_fix): the reference's FDM with the two-channel environment count
+    d = 16 per later site and with its one-channel d = 4, and the exact full
+    density matrix (fdm_fix 63, every quirk of the reference's FDM switched off)
+    at d = 16.  Each run reports CalcNorm and the Friedel ratio
+    pi Gamma_tot A(0) / A_Lambda (the legacy two-channel chain has no A_Lambda;
+    1 at particle-hole symmetry for T << T_K).  The CPU oracle runs the same
+    model at a size it finishes in seconds (5 sites, 200 states), with 8 omegas."""
     import numpy as np
     import twochnrg_amd as t
     base = dict(C5)
@@ -187,10 +192,13 @@ def c5_run(cpu=True, nsitesmax=None, ncut=None, site_dims=(16.0, 4.0)):
     om = fdm_omegas()
     out = {"workload": "twochqsz_fdm", "T": T, "betabar": kw["betabar"], "bbroad": 0.6, "nomega": len(om),
            "params": base, "runs": {}}
+    L = base["Lambda"]
+    a_lam = 0.5 * math.log(L) * (L + 1.0) / (L - 1.0)
+    g_tot = base["Gamma"] + base["Gamma2"]
     with t.Context(0) as ctx:
-        for d in site_dims:
+        for d, fix in modes:
             t0 = time.perf_counter()
-            with ctx.dm(**kw, fdm_site_dim=d) as h:
+            with ctx.dm(**kw, fdm_site_dim=d, fdm_fix=fix) as h:
                 r = h.spec_omega(om)
                 t1 = time.perf_counter()
                 tm = h.timing()
@@ -201,10 +209,11 @@ def c5_run(cpu=True, nsitesmax=None, ncut=None, site_dims=(16.0, 4.0)):
                    "device_GB": {"before_sweep": mem["before_sweep"] / 1e9, "with_poles": mem["now"] / 1e9},
                    "post_ms": tm["sweep"] + tm["poles"] + tm["broaden"], "norm": norm, "ztot": ztot,
                    "shell_weight_max": int(np.argmax(wn)), "A_at_omega_pm1e-10": [float(r[999]), float(r[1000])],
-                   "A_max": float(r.max())}
-            out["runs"]["d%d" % int(d)] = rec
-            print("bench c5: site dim %d done (%.0f ms, forward %.0f ms)" % (d, rec["wall_ms"], tm["forward"]),
-                  file=sys.stderr, flush=True)
+                   "friedel_ratio": math.pi * g_tot * 0.5 * (float(r[999]) + float(r[1000])) / a_lam,
+                   "A_max": float(r.max()), "fdm_fix": fix}
+            out["runs"]["d%d%s" % (int(d), "_exact" if fix == 63 else ("" if fix == 0 else "_fix%d" % fix))] = rec
+            print("bench c5: site dim %d fdm_fix %d done (%.0f ms, forward %.0f ms, norm %.12f, friedel %.4f)"
+                  % (d, fix, rec["wall_ms"], tm["forward"], norm, rec["friedel_ratio"]), file=sys.stderr, flush=True)
     if cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import pyoracle as po
@@ -475,6 +484,10 @@ def main():
            "max_block": max(max(s) for s in timed),
            "states_per_step": round(sum(sum(s) for s in timed) / len(timed), 1)}
     cfg.update({k: v for k, v in w.items()})
+    _, ramp, _ = _params(WORKLOADS[a.workload])
+    # sites run: the timed steps sit at the truncated steady state (reached from N = 3),
+    # so a longer chain repeats the same per-step work
+    cfg["nsitesmax"] = ramp + a.warmup + a.steps
     sy = st["tdb_symv"]
     if sy["launches"] > 0:
         # dominant kernel of the large-block workloads: y = A v of the blocked reduction,
@@ -483,7 +496,10 @@ def main():
         bpl = sy["bytes"] / sy["launches"]
         ach = bpl / avg_s / 1e9
         roof = {"bound": "hbm", "kernel": "tdb_symv_kernel", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": ach / HBM_PEAK_GBS, "traffic": TRAFFIC.get(a.workload), "bytes_per_launch": bpl,
+                "frac": ach / HBM_PEAK_GBS, "traffic": None,
+                "traffic_note": "rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE aborted in the profiler at this kernel's "
+                                "first dispatch; not collected",
+                "bytes_per_launch": bpl,
                 "avg_launch_us": avg_s * 1e6, "timed_launches_per_step": sy["launches"] / nsteps,
                 "timing": "HIP events around every 8th column's launch on the library stream"}
     else:

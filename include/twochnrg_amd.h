@@ -275,7 +275,32 @@ typedef struct {
                                       rho_N's Z_tot/d^{Nmax-N}: 0 -> 4 for model 0 (the reference's
                                       value, src/DM_NRG/DM_NRG.cpp:244-247,365-366) and 16 for model 1;
                                       any other positive value is used as given */
+  int fdm_fix;                     /* FDM only: NRG_FDM_FIX_* bits switch the reference's FDM quirks off
+                                      one by one; 0 = the reference's FDM_SpecDens_M_STL path,
+                                      NRG_FDM_EXACT = the full density matrix (Weichselbaum and von Delft),
+                                      whose CalcNorm is the sum rule <{B, B^dag}> = 1 */
 } nrg_dmnrg_params;
+
+/* Quirks of the reference's FDM path, each switched off by one bit of fdm_fix:
+ *   RHOMAX   rho_Nmax's exponent at the scale of Z_tot and w_Nmax (reference: Temp/D_Mtemp,
+ *            src/DM_NRG/DM_NRG.cpp:102-118,273-297)
+ *   DN       shell N's Boltzmann factors at D_N/Temp (reference: D_{N-1}, src/DM_NRG/DM_NRG.cpp:203-214,240,349)
+ *   LAST     the last shell as an FDM shell with rho_Nmax = e^{-bE}/Z_tot (reference: the Costi form
+ *            with its own 1/Z, src/SpecFunc_allSTL.cpp:1270-1277)
+ *   PAIRS    block pairs with no kept state keep their discarded-discarded terms (reference skips
+ *            them, src/SpecFunc_allSTL.cpp:1375-1381)
+ *   ONESIDED kept-discarded terms with rho on the discarded side also when the other block has no
+ *            discarded state (reference: only inside its both-blocks-discarded branch)
+ *   NORM     every pole carries its whole weight at both signs of omega and CalcNorm counts it once
+ *            (reference: the discarded-discarded terms sit in Positive and Negative, so CalcNorm counts
+ *            them twice, src/SpecFunc_allSTL.cpp:2328-2348) */
+#define NRG_FDM_FIX_RHOMAX 1
+#define NRG_FDM_FIX_DN 2
+#define NRG_FDM_FIX_LAST 4
+#define NRG_FDM_FIX_PAIRS 8
+#define NRG_FDM_FIX_ONESIDED 16
+#define NRG_FDM_FIX_NORM 32
+#define NRG_FDM_EXACT 63
 
 typedef struct nrg_dm nrg_dm;
 

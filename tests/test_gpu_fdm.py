@@ -22,6 +22,10 @@ CASES = [
     # CFS: the chain is cut at Mtemp+1, rho from the plain sweep over kept + discarded states
     dict(U=0.5, Gamma=0.02, ed=-0.25, Lambda=2.5, nsitesmax=14, ncut=120, Mtemp=10, betabar=1.0, method=1,
          update_bef_cut=1),
+    # the exact full density matrix (every FDM quirk of the reference switched off), off particle-hole
+    # symmetry, Mtemp below Nmax - 1
+    dict(U=0.6, Gamma=0.04, ed=-0.2, Lambda=2.5, nsitesmax=12, ncut=150, Mtemp=8, betabar=0.8, method=2,
+         update_bef_cut=1, bbroad=0.6, fdm_fix=63),
 ]
 
 
@@ -103,6 +107,8 @@ def test_fdm_shell_sums_and_norm(ctx, case):
         assert err.max() <= 1e-8, (err.max(), ws[int(err.argmax())])
         n = d.norm()
         assert abs(n - o.norm()) <= 1e-10 * abs(o.norm())
+        if case.get("fdm_fix") == 63:   # the sum rule of the full density matrix
+            assert abs(n - 1.0) <= 1e-10, n
 
 
 @pytest.mark.parametrize("Nw", [0, 2])

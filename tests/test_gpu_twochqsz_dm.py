@@ -24,6 +24,13 @@ CASES = [
     dict(BASE, nsitesmax=5, ncut=200, Mtemp=3, betabar=0.779, method=1, bbroad=0.6),
     # FDM with the one-channel environment count 4 of the reference's weights
     dict(BASE, nsitesmax=4, ncut=200, Mtemp=3, betabar=0.779, method=2, bbroad=0.6, fdm_site_dim=4.0),
+    # the exact full density matrix (every FDM quirk of the reference switched off), truncated,
+    # Mtemp below Nmax - 1, environment count 16 and 4
+    dict(BASE, nsitesmax=5, ncut=200, Mtemp=2, betabar=0.779, method=2, bbroad=0.6, fdm_fix=63),
+    dict(BASE, nsitesmax=5, ncut=200, Mtemp=2, betabar=0.779, method=2, bbroad=0.6, fdm_fix=63,
+         fdm_site_dim=4.0),
+    # one quirk switched off alone: the all-discarded block pairs keep their terms
+    dict(BASE, nsitesmax=5, ncut=200, Mtemp=2, betabar=0.779, method=2, bbroad=0.6, fdm_fix=8),
 ]
 
 
@@ -118,6 +125,8 @@ def test_qsz_shell_sums_and_norm(ctx, runs, i):
         assert err.max() <= 1e-8, (err.max(), ws[int(err.argmax())])
         n = d.norm()
         assert abs(n - o.norm()) <= 1e-10 * abs(o.norm())
+        if case.get("fdm_fix") == 63:   # the sum rule of the full density matrix
+            assert abs(n - 1.0) <= 1e-10, n
 
 
 @pytest.mark.parametrize("Nw", [0, 1])

@@ -85,7 +85,13 @@ class DmParams(ctypes.Structure):
         ("broadtemp", ctypes.c_double), ("bbroad", ctypes.c_double), ("Nw", ctypes.c_int),
         ("method", ctypes.c_int), ("update_bef_cut", ctypes.c_int),
         ("model", ctypes.c_int), ("Gamma2", ctypes.c_double), ("fdm_site_dim", ctypes.c_double),
+        ("fdm_fix", ctypes.c_int),
     ]
+
+
+# bits of nrg_dmnrg_params.fdm_fix (NRG_FDM_FIX_* of the C header)
+FDM_FIX_RHOMAX, FDM_FIX_DN, FDM_FIX_LAST, FDM_FIX_PAIRS, FDM_FIX_ONESIDED, FDM_FIX_NORM = 1, 2, 4, 8, 16, 32
+FDM_EXACT = 63
 
 
 class PairDesc(ctypes.Structure):
@@ -372,7 +378,7 @@ class Context:
     def dm(self, U=0.5, Gamma=0.02, ed=-0.25, Lambda=2.5, nsitesmax=40, ncut=400, Mtemp=38, betabar=1.0,
            disc_scheme=0, twindow=0.0, broadtemp=0.0, bbroad=0.0, Nw=0, myeps=0.0, betabar_nrg=0.0,
            method=0, update_bef_cut=0, kept=None, dpar=None, model=0, Gamma2=0.0,
-           fdm_site_dim=0.0):
+           fdm_site_dim=0.0, fdm_fix=0):
         """DM-NRG handle (nrg_dm_create): forward chain + rho sweep on the device.
         method 0: DM-NRG, 1: CFS (DM_NRG -C), 2: FDM (DM_NRG -F); CFS/FDM want
         update_bef_cut=1 (the chain keeps the discarded states per shell).
@@ -381,10 +387,12 @@ class Context:
         (nrg_dm_create_jvp; DM-NRG method only).  model=1 runs the TwoChQSz
         two-channel chain (Gamma, Gamma2 the two hybridisations; CFS or FDM) and
         the spectral function of d_up.  fdm_site_dim: the FDM environment count
-        per later site (0: 4 for model 0 as the reference, 16 for model 1)."""
+        per later site (0: 4 for model 0 as the reference, 16 for model 1).
+        fdm_fix: FDM_FIX_* bits switching the reference's FDM quirks off
+        (FDM_EXACT: the full density matrix, whose CalcNorm is 1)."""
         return DMNRG(self, DmParams(U, Gamma, ed, Lambda, nsitesmax, ncut, disc_scheme, myeps, betabar_nrg,
                                     Mtemp, betabar, twindow, broadtemp, bbroad, Nw, method, update_bef_cut,
-                                    model, Gamma2, fdm_site_dim),
+                                    model, Gamma2, fdm_site_dim, fdm_fix),
                      kept=kept, dpar=dpar)
 
     def run_twochqs(self, U=0.4, Gamma1=0.03, Gamma2=0.03, ed=-0.2, Lambda=3.0, nsitesmax=3, ncut=3000,

@@ -47,6 +47,8 @@ struct nrg_dm {
   double temp = 0.0, betabar_spec = 0.0;
   double ztot = 0.0;                           // FDM Z_tot
   std::vector<double> wn;                      // FDM w_N
+  int fix = 0;                                 // FDM: NRG_FDM_FIX_* bits in effect (0 unless FDM)
+  double bb_last = 0.0;                        // FDM with NRG_FDM_FIX_LAST: 1/TempBar of the last shell
   // rho_N per shell and block in <pfx>R<N>: the k x k block at rho_off, or when
   // compact (no tangents) the kc x kc kept block at rho_off and the k - kc
   // discarded diagonal at rho_doff (rho has nothing else)
@@ -371,7 +373,11 @@ int pole_layout(nrg_dm* D) {
     const DmShell& s = D->sh[N];
     const bool costi = (N == D->specNM - 1);
     const int form = costi ? DM_FORM_COSTI : D->method;
-    const double invZ = costi ? 1.0 / partition_func(s, bb) : 0.0;
+    // the last shell: the Costi form with its own 1/Z, or (NRG_FDM_FIX_LAST) the
+    // FDM shell of rho_Nmax = e^{-b_last E}/Z_tot, the same sum with other scalars
+    const bool fdm_last = costi && (D->fix & NRG_FDM_FIX_LAST);
+    const double bbN = fdm_last ? D->bb_last : bb;
+    const double invZ = costi ? (fdm_last ? 1.0 / D->ztot : 1.0 / partition_func(s, bb)) : 0.0;
     D->pole_beg[N] = ptot;
     for (size_t q = 0; q + 1 < s.op_map.size(); q += 2) {
       const int b1 = s.op_map[q], b2 = s.op_map[q + 1];
@@ -382,12 +388,13 @@ int pole_layout(nrg_dm* D) {
       P.e1_off = s.bbe[2 * b1]; P.e2_off = s.bbe[2 * b2];
       P.cg = s.sz ? 1.0 : op_cgfactor(s.qn[2 * b1 + 1], s.qn[2 * b2 + 1]);
       P.form = form;
-      P.bb = bb; P.invZ = invZ;
+      P.fix = D->fix;
+      P.bb = bbN; P.invZ = invZ;
       if ((rc = kept_prefix(s, b1, P.kc1))) return rc;
       if ((rc = kept_prefix(s, b2, P.kc2))) return rc;
       if (form == DM_FORM_CFS || form == DM_FORM_FDM) {
         const bool nokept = P.kc1 == 0 && P.kc2 == 0, nodisc = P.kc1 == k1 && P.kc2 == k2;
-        if (nokept || nodisc) continue;
+        if (nodisc || (nokept && !(D->fix & NRG_FDM_FIX_PAIRS))) continue;
       }
       P.pole_off = ptot;
       ptot += (int64_t)k1 * k2;
@@ -694,8 +701,8 @@ int spec_tasks(nrg_dm* D, int n, const int* shells, const double* w, double* out
     const int N = shells[i];
     if (N < 0 || N > ns - 1) continue;
     if (norm) {   // CalcNorm: Wp + Wn below the last shell, the Costi sum once at the last
-      tk[0][N].push_back(Task{1.0, i});
-      if (N < ns - 1) tk[1][N].push_back(Task{-1.0, i});
+      tk[0][N].push_back(Task{1.0, i});   // (NRG_FDM_FIX_NORM: Wp = Wn, every pole once)
+      if (N < ns - 1 && !(D->fix & NRG_FDM_FIX_NORM)) tk[1][N].push_back(Task{-1.0, i});
     } else {
       tk[w[i] >= 0.0 ? 0 : 1][N].push_back(Task{w[i], i});
     }
@@ -902,6 +909,8 @@ static int dm_create(nrg_ctx* c, const nrg_dmnrg_params* p, const unsigned char*
       p->method < 0 || p->method > 2)
     return fail(NRG_ERR_ARG, "nrg_dm_create: bad parameters");
   if (!(p->fdm_site_dim >= 0.0)) return fail(NRG_ERR_ARG, "nrg_dm_create: fdm_site_dim must be 0 or positive");
+  if (p->fdm_fix < 0 || p->fdm_fix > NRG_FDM_EXACT || (p->fdm_fix != 0 && p->method != 2))
+    return fail(NRG_ERR_ARG, "nrg_dm_create: fdm_fix takes NRG_FDM_FIX_* bits and needs FDM (method 2)");
   if (p->model < 0 || p->model > 1) return fail(NRG_ERR_ARG, "nrg_dm_create: model must be 0 (OneChQS) or 1 (TwoChQSz)");
   if (p->model == 1 && (p->method == 0 || ntan > 0 || kept))
     return fail(NRG_ERR_ARG, "nrg_dm_create: the TwoChQSz chain supports CFS and FDM without derivatives");
@@ -979,15 +988,20 @@ static int dm_fill(nrg_ctx* c, const nrg_dmnrg_params* p, const unsigned char* k
   D->rho_doff.assign(NM + 1, std::vector<int64_t>());
   D->rho_len.assign(NM + 1, 0);
   D->compact = (ntan == 0);
+  D->fix = fdm ? p->fdm_fix : 0;
+  // shell N's Boltzmann factors at Temp/D_{N-1} (src/DM_NRG/DM_NRG.cpp:203-214,240,349),
+  // or at Temp/D_N with NRG_FDM_FIX_DN
+  const int dsh = (D->fix & NRG_FDM_FIX_DN) ? 0 : 1;
   // Z_N at Temp/D_{NM-1} (src/DM_NRG/DM_NRG.cpp:203-214); FDM: Z_tot = Z_N +
   // sum_N Z_N^disc 4^{NM-N} and w_N (:221-271)
-  const double ZN = partition_func(D->sh[NM], 1.0 / (D->temp / calc_dn(p->Lambda, NM - 1)));
+  const double ZN = partition_func(D->sh[NM], 1.0 / (D->temp / calc_dn(p->Lambda, NM - dsh)));
+  D->bb_last = 1.0 / (D->temp / calc_dn(p->Lambda, NM - dsh));
   D->wn.assign(NM + 1, 0.0);
   double ZTot = ZN;
   std::vector<double> wraw(NM + 1, 0.0);
   wraw[NM] = ZN;
   for (int N = NM - 1; N >= 0; N--) {
-    const double zd = partition_func_disc(D->sh[N], 1.0 / (D->temp / calc_dn(p->Lambda, N - 1))) * std::pow(site_dim, NM - N);
+    const double zd = partition_func_disc(D->sh[N], 1.0 / (D->temp / calc_dn(p->Lambda, N - dsh))) * std::pow(site_dim, NM - N);
     wraw[N] = zd;
     ZTot += zd;
   }
@@ -996,10 +1010,11 @@ static int dm_fill(nrg_ctx* c, const nrg_dmnrg_params* p, const unsigned char* k
     D->ztot = ZTot;
   }
   // rho_Nmax: diag e^{-E/TempBar}/Z with TempBar = Temp/DM of :102-118 and
-  // Z = Z_N (DM-NRG, CFS) or Z_tot (FDM) (src/DM_NRG/DM_NRG.cpp:273-297)
+  // Z = Z_N (DM-NRG, CFS) or Z_tot (FDM) (src/DM_NRG/DM_NRG.cpp:273-297);
+  // NRG_FDM_FIX_RHOMAX: TempBar at the scale of Z_tot and w_Nmax
   {
     const DmShell& s = D->sh[NM];
-    const double TempBar = D->temp / DM;
+    const double TempBar = (D->fix & NRG_FDM_FIX_RHOMAX) ? 1.0 / D->bb_last : D->temp / DM;
     const double bb = (std::fabs(TempBar) < 1e-20) ? 1.0e10 : 1.0 / TempBar;
     const double Z = fdm ? ZTot : ZN;
     std::vector<double> r;
@@ -1064,7 +1079,7 @@ static int dm_fill(nrg_ctx* c, const nrg_dmnrg_params* p, const unsigned char* k
     s.dzbuf.clear();
   };
   for (int N = NM - 1; N >= 0; N--) {
-    if ((rc = sweep_shell(D, N, fdm, D->temp / calc_dn(p->Lambda, N - 1), ZTot / std::pow(site_dim, NM - N)))) {
+    if ((rc = sweep_shell(D, N, fdm, D->temp / calc_dn(p->Lambda, N - dsh), ZTot / std::pow(site_dim, NM - N)))) {
       return rc;
     }
     drop_z(D->sh[N + 1]);

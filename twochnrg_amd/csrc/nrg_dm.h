@@ -46,7 +46,7 @@ struct DmPair {
   int kc1, kc2;     // kept prefixes of the two blocks (CFS/FDM classes)
   double cg;
   int form;         // DM_FORM_*
-  int pad;
+  int fix;          // FDM: NRG_FDM_FIX_* bits in effect (ONESIDED and NORM act on the weights)
   double bb, invZ;  // Costi betabar and 1/Z
 };
 // the broadening of a method (BDelta / BDeltaTemp, src/DM_NRG/DM_NRG.cpp:474-482)

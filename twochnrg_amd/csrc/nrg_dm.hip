@@ -8,6 +8,7 @@
 #include <stdint.h>
 
 #include "nrg_dm.h"
+#include "../../include/twochnrg_amd.h"
 
 namespace dmk {
 
@@ -72,6 +73,8 @@ __global__ void scale_kernel(const DmScale* __restrict__ blks, const double* __r
 //   FDM      (i kept, j disc): Wp = cg (w1 + [block 1 has discarded] w2)
 //            (i disc, j kept): Wn = cg (w2 + [block 2 has discarded] w1)
 //            (i disc, j disc): Wp = Wn = cg (w1 + w2)
+//            the ONESIDED bit drops the [block has discarded] conditions; the
+//            NORM bit gives every pole its weight at both signs (Wp = Wn)
 //   dE = E2_j - E1_i
 __global__ void weights_kernel(const DmPair* __restrict__ pairs, const double* __restrict__ ops,
                                const double* __restrict__ prod, const double* __restrict__ en,
@@ -95,10 +98,12 @@ __global__ void weights_kernel(const DmPair* __restrict__ pairs, const double* _
           if (ik && !jk) wp = P.cg * w1;
           else if (!ik && jk) wn = P.cg * w2;
         } else {
-          const bool d1 = P.kc1 < P.k1, d2 = P.kc2 < P.k2;
+          const bool ons = (P.fix & NRG_FDM_FIX_ONESIDED) != 0;
+          const bool d1 = ons || P.kc1 < P.k1, d2 = ons || P.kc2 < P.k2;
           if (ik && !jk) wp = P.cg * (w1 + (d1 ? w2 : 0.0));
           else if (!ik && jk) wn = P.cg * (w2 + (d2 ? w1 : 0.0));
           else if (!ik && !jk) wp = wn = P.cg * (w1 + w2);
+          if (P.fix & NRG_FDM_FIX_NORM) wp = wn = (ik && !jk) ? wp : wn;
         }
       }
     }
