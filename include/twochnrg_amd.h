@@ -362,6 +362,27 @@ int nrg_dm_weights(const nrg_dm *dm, double *wn, int cap, double *ztot);
  * omega/rho may be NULL to query it. */
 int nrg_dm_spectrum(nrg_dm *dm, double *omega, double *rho, int cap, int *n);
 
+/*
+ * The omega-grid drivers on their own, for shell sums computed elsewhere:
+ * CalcSpecDM_NRG_FixedOmegas (Nw = 0, one omega per shell as DM_NRG_CalcSpecFunc_ij
+ * calls it; src/SpecFuncClass.cpp:1932-2233) and CalcSpec_ManyOmegas (Nw > 0;
+ * :2239-2341) over nshell spectral shells (CSpecFunction::NshellMax); windowed != 0
+ * for CFS/FDM, which start two shells later.
+ * nrg_spectrum_plan: the n evaluation points in the reference's recording order:
+ *   shell[i], omegabar[i] (DM-NRG: the shell sum is evaluated there, then / D_shell),
+ *   omega[i] (the recorded omega; CFS/FDM: CalcSpecDM_NRG(omega) is evaluated there),
+ *   parity[i] (FixedOmegas: 0 even / 1 odd shell set; ManyOmegas: -1).
+ *   All outputs NULL queries *n.
+ * nrg_spectrum_combine: the two SaveOmegaRhow columns from value[i] at those points
+ *   (FixedOmegas: even and odd sets interpolated linearly on the sorted union and
+ *   averaged, gsl_interp_linear as :2112-2222; ManyOmegas: the values as recorded).
+ *   omega_out and rho_out hold n doubles.
+ */
+int nrg_spectrum_plan(double Lambda, int nshell, int windowed, int Nw, int cap, int *n, int *shell, double *omegabar,
+                      double *omega, int *parity);
+int nrg_spectrum_combine(int n, const double *omega, const int *parity, const double *value, int Nw,
+                         double *omega_out, double *rho_out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -2502,83 +2502,94 @@ static int cmpd2(const void *a, const void *b) {
   return (x > y) - (x < y);
 }
 
-/* one grid value: DM-NRG evaluates the shell's own sum at +-Lambda^1.25 (/D_N);
- * CFS and FDM call CalcSpecDM_NRG(+-omega) (src/SpecFuncClass.cpp:1979-1982) */
-static double grid_value(const struct orc_dm *D, int N, double sgn, double wbar, double omega) {
-  if (D->method != 0) return orc_dm_spec_omega(D, sgn * omega);
-  return orc_dm_spec_shell(D, N, sgn * wbar) / calc_dn(D->p.Lambda, N);
-}
-
-/* CalcSpecDM_NRG_FixedOmegas  src/SpecFuncClass.cpp:1932-2233 (NwEachN = 1):
- * even and odd shells on w = +-Lambda^1.25 D_N, each set interpolated linearly
- * on the sorted union and averaged; CFS/FDM start two shells later (:1954) */
-static void spec_fixed_omegas(struct orc_dm *D, double fac) {
-  int N0 = 0, NM = D->spec_nshell_max;
-  int Nsh0E = (N0 % 2 == 0) ? N0 : N0 + 1, Nsh0O = (N0 % 2 == 0) ? N0 + 1 : N0;
-  if (D->method != 0) { Nsh0E += 2; Nsh0O += 2; }
-  dvec ex = {0}, ey = {0}, ox = {0}, oy = {0}, all = {0};
-  int last = 0;
-  for (int par = 0; par < 2; par++) {
-    dvec *X = par ? &ox : &ex, *Y = par ? &oy : &ey;
-    int n0 = par ? Nsh0O : Nsh0E;
-    last = 0;
-    for (int N = n0; N < NM; N += 2) {
-      double DN = calc_dn(D->p.Lambda, N), WN = fac * DN;
-      double r = grid_value(D, N, -1.0, fac * pow(D->p.Lambda, 0.0), WN);
-      dpush(X, -WN); dpush(Y, r); dpush(&all, -WN);
-      last = N;
-    }
-    for (int N = last; N >= n0; N -= 2) {
-      double DN = calc_dn(D->p.Lambda, N), WN = fac * DN;
-      double r = grid_value(D, N, 1.0, fac * pow(D->p.Lambda, 0.0), WN);
-      dpush(X, WN); dpush(Y, r); dpush(&all, WN);
+/* The evaluation points of the omega-grid drivers, in the order the reference
+ * records them: shell N, the omegabar its DM-NRG shell sum is evaluated at, the
+ * recorded omega, and the even/odd set (FixedOmegas; -1 for ManyOmegas).
+ *   CalcSpecDM_NRG_FixedOmegas  src/SpecFuncClass.cpp:1932-2233 (NwEachN = 1, the call of
+ *     DM_NRG_CalcSpecFunc_ij src/DM_NRG/DM_NRG_CalcSpecFuncs.cpp:83-86): even shells from
+ *     Nsh0Even, w = -Lambda^1.25 D_N ascending in N, then +w back down; the same for the odd
+ *     shells; windowed (CFS/FDM) drivers start two shells later (:1954);
+ *   CalcSpec_ManyOmegas  :2239-2341: shells Nsh0 .. NshellMax-1 (+2 windowed), recorded
+ *     w = -+Lambda^(-i/(2Nw)) Lambda^1.25 D_N, i = 0 .. Nw-1 (negative: N up, i up; positive:
+ *     N down, i down); the DM-NRG shell sum is evaluated at -+Lambda^1.25 Lambda^(+i/(2Nw)).
+ * Returns the count; the arrays may be NULL to query it. */
+int orc_grid_plan(double L, int nshell, int windowed, int Nw, int *shell, double *wbar, double *omega,
+                  int *parity) {
+  const double fac = pow(L, 1.25);
+  int k = 0;
+#define ORC_GP(N_, wb_, w_, par_)                                    \
+  do {                                                               \
+    if (shell) { shell[k] = (N_); wbar[k] = (wb_); omega[k] = (w_); parity[k] = (par_); } \
+    k++;                                                             \
+  } while (0)
+  if (Nw > 0) {
+    int N0 = windowed ? 2 : 0, Nmax = nshell - 1;
+    for (int N = N0; N <= Nmax; N++)
+      for (int i = 0; i < Nw; i++) {
+        double rf = (double)i / (2.0 * Nw);
+        ORC_GP(N, -fac * pow(L, rf), -pow(L, -rf) * fac * calc_dn(L, N), -1);
+      }
+    for (int N = Nmax; N >= N0; N--)
+      for (int i = Nw - 1; i >= 0; i--) {
+        double rf = (double)i / (2.0 * Nw);
+        ORC_GP(N, fac * pow(L, rf), pow(L, -rf) * fac * calc_dn(L, N), -1);
+      }
+  } else {
+    for (int par = 0; par < 2; par++) {
+      int n0 = par + (windowed ? 2 : 0), last = 0;
+      for (int N = n0; N < nshell; N += 2) { ORC_GP(N, -fac, -fac * calc_dn(L, N), par); last = N; }
+      for (int N = last; N >= n0; N -= 2) ORC_GP(N, fac, fac * calc_dn(L, N), par);
     }
   }
-  qsort(all.a, (size_t)all.n, sizeof(double), cmpd2);
-  D->nomega = all.n;
-  D->omega = (double *)malloc(sizeof(double) * (size_t)(all.n + 1));
-  D->rhow = (double *)malloc(sizeof(double) * (size_t)(all.n + 1));
-  for (int i = 0; i < all.n; i++) {
-    double w = all.a[i];
-    int inE = !(w < ex.a[0] || w > ex.a[ex.n - 1]), inO = !(w < ox.a[0] || w > ox.a[ox.n - 1]);
+#undef ORC_GP
+  return k;
+}
+
+/* The spectrum from the values at the plan's points: ManyOmegas records them
+ * as they are; FixedOmegas interpolates the even and the odd set linearly
+ * (gsl_interp_linear) on the sorted union of their omegas and averages, a
+ * point outside one set's range taking the other's value (outside both: -100)
+ * (src/SpecFuncClass.cpp:2112-2222).  Returns the count of (w_out, rho_out). */
+int orc_grid_combine(int n, const double *omega, const int *parity, const double *val, int Nw, double *w_out,
+                     double *rho_out) {
+  if (Nw > 0) {
+    for (int i = 0; i < n; i++) { w_out[i] = omega[i]; rho_out[i] = val[i]; }
+    return n;
+  }
+  dvec ex = {0}, ey = {0}, ox = {0}, oy = {0};
+  for (int i = 0; i < n; i++) {
+    if (parity[i] == 0) { dpush(&ex, omega[i]); dpush(&ey, val[i]); }
+    else { dpush(&ox, omega[i]); dpush(&oy, val[i]); }
+    w_out[i] = omega[i];
+  }
+  qsort(w_out, (size_t)n, sizeof(double), cmpd2);
+  for (int i = 0; i < n; i++) {
+    double w = w_out[i];
+    int inE = ex.n && !(w < ex.a[0] || w > ex.a[ex.n - 1]), inO = ox.n && !(w < ox.a[0] || w > ox.a[ox.n - 1]);
     double rE = inE ? lin_interp(ex.a, ey.a, ex.n, w) : 0.0, rO = inO ? lin_interp(ox.a, oy.a, ox.n, w) : 0.0;
     if (!inE) { if (!inO) { rE = -100.0; rO = -100.0; } else rE = rO; }
     else if (!inO) rO = rE;
-    D->omega[i] = w;
-    D->rhow[i] = 0.5 * (rE + rO);
+    rho_out[i] = 0.5 * (rE + rO);
   }
-  free(ex.a); free(ey.a); free(ox.a); free(oy.a); free(all.a);
+  free(ex.a); free(ey.a); free(ox.a); free(oy.a);
+  return n;
 }
 
-/* CalcSpec_ManyOmegas  src/SpecFuncClass.cpp:2239-2341: w = +-Lambda^(-i/(2Nw)) Lambda^1.25 D_N;
- * DM-NRG evaluates the shell sum at +-Lambda^1.25 Lambda^(+i/(2Nw)) as the reference
- * does, CFS/FDM the window sum at the recorded w, from two shells later (:2256) */
-static void spec_many_omegas(struct orc_dm *D, int Nw, double fac) {
-  int N0 = 0, Nmax = D->spec_nshell_max - 1;
-  if (D->method != 0) N0 += 2;
-  int cap = 2 * (Nmax - N0 + 1) * Nw + 1;
-  D->omega = (double *)malloc(sizeof(double) * (size_t)cap);
-  D->rhow = (double *)malloc(sizeof(double) * (size_t)cap);
-  int k = 0;
-  for (int N = N0; N <= Nmax; N++) {
-    double DN = calc_dn(D->p.Lambda, N), WN = fac * DN;
-    for (int i = 0; i < Nw; i++) {
-      double rf = (double)i / (2.0 * Nw), w = pow(D->p.Lambda, -rf) * WN;
-      D->omega[k] = -w;
-      D->rhow[k++] = grid_value(D, N, -1.0, fac * pow(D->p.Lambda, rf), w);
-      (void)DN;
-    }
-  }
-  for (int N = Nmax; N >= N0; N--) {
-    double DN = calc_dn(D->p.Lambda, N), WN = fac * DN;
-    for (int i = Nw - 1; i >= 0; i--) {
-      double rf = (double)i / (2.0 * Nw), w = pow(D->p.Lambda, -rf) * WN;
-      D->omega[k] = w;
-      D->rhow[k++] = grid_value(D, N, 1.0, fac * pow(D->p.Lambda, rf), w);
-      (void)DN;
-    }
-  }
-  D->nomega = k;
+/* rho_0_0 on the grid: DM-NRG evaluates each shell's own sum (/D_N); CFS and
+ * FDM call CalcSpecDM_NRG at the recorded omega (src/SpecFuncClass.cpp:1979-1982) */
+static void spec_grid(struct orc_dm *D) {
+  const int win = D->method != 0;
+  int n = orc_grid_plan(D->p.Lambda, D->spec_nshell_max, win, D->p.Nw, NULL, NULL, NULL, NULL);
+  int *sh = (int *)malloc(sizeof(int) * (size_t)(n + 1)), *par = (int *)malloc(sizeof(int) * (size_t)(n + 1));
+  double *wb = (double *)malloc(sizeof(double) * (size_t)(n + 1)), *w = (double *)malloc(sizeof(double) * (size_t)(n + 1));
+  double *v = (double *)malloc(sizeof(double) * (size_t)(n + 1));
+  orc_grid_plan(D->p.Lambda, D->spec_nshell_max, win, D->p.Nw, sh, wb, w, par);
+  for (int i = 0; i < n; i++)
+    v[i] = win ? orc_dm_spec_omega(D, w[i]) : orc_dm_spec_shell(D, sh[i], wb[i]) / calc_dn(D->p.Lambda, sh[i]);
+  D->omega = (double *)malloc(sizeof(double) * (size_t)(n + 1));
+  D->rhow = (double *)malloc(sizeof(double) * (size_t)(n + 1));
+  D->nomega = orc_grid_combine(n, w, par, v, D->p.Nw, D->omega, D->rhow);
+  free(sh); free(par); free(wb); free(w); free(v);
 }
 
 /* DM_NRG main  src/DM_NRG/DM_NRG.cpp:28-531 (QS symmetry, NFermiOps 1):
@@ -2955,11 +2966,7 @@ int orc_dm_shell(const orc_dm *D, int N, orc_dm_shell_view *v) {
 
 /* the grid is evaluated on the first call */
 int orc_dm_spectrum(orc_dm *D, const double **omega, const double **rho) {
-  if (!D->omega) {
-    double fac = pow(D->p.Lambda, 1.25);
-    if (D->p.Nw > 0) spec_many_omegas(D, D->p.Nw, fac);
-    else spec_fixed_omegas(D, fac);
-  }
+  if (!D->omega) spec_grid(D);
   *omega = D->omega; *rho = D->rhow;
   return D->nomega;
 }

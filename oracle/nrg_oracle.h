@@ -178,6 +178,14 @@ double orc_dm_norm(const orc_dm *D);
 double orc_dm_ztot(const orc_dm *D);
 /* rho_0_0 on the DM_NRG_CalcSpecFunc_ij grid (evaluated on the first call) */
 int orc_dm_spectrum(orc_dm *D, const double **omega, const double **rho);
+/* The omega-grid drivers split from the shell sums: the evaluation points of
+ * CalcSpecDM_NRG_FixedOmegas (Nw = 0; src/SpecFuncClass.cpp:1932-2233) or
+ * CalcSpec_ManyOmegas (Nw > 0; :2239-2341) for nshell spectral shells
+ * (windowed = CFS/FDM), and the spectrum from the values at those points
+ * (DM-NRG: shell sum at (shell, wbar) / D_shell; windowed: CalcSpecDM_NRG(omega)). */
+int orc_grid_plan(double L, int nshell, int windowed, int Nw, int *shell, double *wbar, double *omega, int *parity);
+int orc_grid_combine(int n, const double *omega, const int *parity, const double *val, int Nw, double *w_out,
+                     double *rho_out);
 
 /* ---- kernel-level restatements ---- */
 

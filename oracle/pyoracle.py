@@ -456,3 +456,31 @@ class DmRun:
         po, pr = _PD(), _PD()
         n = self._L.orc_dm_spectrum(self.h, ctypes.byref(po), ctypes.byref(pr))
         return _arr(po, n), _arr(pr, n)
+
+
+def grid_plan(Lambda, nshell, windowed, Nw):
+    """orc_grid_plan: (shell, omegabar, omega, parity) of the omega-grid drivers."""
+    L = load()
+    L.orc_grid_plan.restype = ctypes.c_int
+    L.orc_grid_plan.argtypes = [ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                ctypes.POINTER(ctypes.c_int), _PD, _PD, ctypes.POINTER(ctypes.c_int)]
+    n = L.orc_grid_plan(Lambda, nshell, int(windowed), Nw, None, None, None, None)
+    sh, par = np.zeros(n, np.int32), np.zeros(n, np.int32)
+    wb, w = np.zeros(n), np.zeros(n)
+    L.orc_grid_plan(Lambda, nshell, int(windowed), Nw, sh.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), _dp(wb),
+                    _dp(w), par.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+    return sh, wb, w, par
+
+
+def grid_combine(omega, parity, value, Nw):
+    """orc_grid_combine: the spectrum from the values at the plan's points."""
+    L = load()
+    L.orc_grid_combine.restype = ctypes.c_int
+    L.orc_grid_combine.argtypes = [ctypes.c_int, _PD, ctypes.POINTER(ctypes.c_int), _PD, ctypes.c_int, _PD, _PD]
+    w = np.ascontiguousarray(omega, np.float64)
+    par = np.ascontiguousarray(parity, np.int32)
+    v = np.ascontiguousarray(value, np.float64)
+    wo, ro = np.zeros(len(w)), np.zeros(len(w))
+    L.orc_grid_combine(len(w), _dp(w), par.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), _dp(v), Nw, _dp(wo),
+                       _dp(ro))
+    return wo, ro

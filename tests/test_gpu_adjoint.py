@@ -160,6 +160,38 @@ def _dm_fd(ctx, x0, kept, kw, sh, wb, dn, q, h):
     return (ap - am) / (2 * h)
 
 
+def _window_crossing_shells(ctx, x0, kept, kw, q, h, temp):
+    """Shells N where a pole E_j - E_i of c_d (kept states of blocks (Q, S) and
+    (Q+1, S +- 1/2)) lies on different sides of the BDeltaTemp window edge
+    twindow T / D_N at x0 - h and x0 + h (frozen Kept sets)."""
+    from twochnrg_amd import autograd as ag
+    tw = kw["twindow"] if kw["twindow"] > 0 else 2.0
+    side = []
+    for sgn in (1.0, -1.0):
+        x = list(x0)
+        x[q] += sgn * h
+        steps, _ = ctx.onechqs_jvp(U=x[0], Gamma=x[1], ed=x[2], Lambda=kw["Lambda"], nsitesmax=kw["nsitesmax"],
+                                   ncut=kw["ncut"], calcdens=1, kept=kept)
+        per = {}
+        for st in steps:
+            N = st["N"]
+            if N < 0:
+                continue
+            edge = tw * temp / ag.calc_dn(kw["Lambda"], N)
+            blk = []
+            for (Q, S), (b0, b1) in zip(st["qn"][:, :2], st["bbe"]):
+                e = st["en"][b0:b1 + 1][st["kept"][b0:b1 + 1]]
+                blk.append((Q, S, e))
+            inside = []
+            for Q1, S1, e1 in blk:
+                for Q2, S2, e2 in blk:
+                    if abs(Q2 - Q1 - 1) < 1e-9 and abs(abs(S2 - S1) - 0.5) < 1e-9 and len(e1) and len(e2):
+                        inside.append((np.abs(e2[None, :] - e1[:, None]) < edge).ravel())
+            per[N] = np.concatenate(inside) if inside else np.zeros(0, bool)
+        side.append(per)
+    return {N for N in side[0] if N in side[1] and np.any(side[0][N] != side[1][N])}
+
+
 def test_c4_spectral_gradients_vs_central_fd(ctx):
     """dA(omega)/d(U, Gamma, ed) on omega = +-logspace(-8, 0, 100) at T = 1e-6
     (Mtemp 31, betabar = D_31 / T), single-channel Anderson Lambda 2.5, Ncut
@@ -171,8 +203,10 @@ def test_c4_spectral_gradients_vs_central_fd(ctx):
     jumps where a pole crosses that edge; the derivative is that of the
     piecewise function.  The tight check therefore uses a vanishing window
     (twindow 1e-12, A smooth): Richardson differences of relative steps 1e-5
-    and 5e-6.  With the default window (2) at the relative step 1e-5 the
-    omegas away from a crossing agree (median)."""
+    and 5e-6.  With the default window (2) at the relative step 1e-5 every
+    omega whose shell has no pole crossing the window edge between x0 - h and
+    x0 + h agrees to 1e-6, and the omegas that disagree by more than 1e-4 all
+    sit on such shells."""
     import torch
     from twochnrg_amd import autograd as ag
     L, M = 2.5, 31
@@ -206,10 +240,14 @@ def test_c4_spectral_gradients_vs_central_fd(ctx):
                 assert err.max() <= 1e-6, (KEYS[q], err.max())
             else:
                 err = np.abs(d1 - J[:, q]) / scale
-                bad = int(np.sum(err > 1e-4))
-                print("c4 dA/d%s (default window): max |dA| %.4g, median rel err %.3g, %d of 200 above 1e-4" % (
-                    KEYS[q], scale, np.median(err), bad), flush=True)
-                # the omegas next to a pole that the step carries across the window edge
-                # see the jump; the rest agree (the derivative with this window is pinned
-                # by the parity test against the oracle)
-                assert np.median(err) <= 1e-6, (KEYS[q], np.median(err), bad)
+                cross = _window_crossing_shells(ctx, x0, run["kept"], kw, q, h, 1e-6)
+                hit = np.array([g in cross for g in sh])
+                bad = err > 1e-4
+                print("c4 dA/d%s (default window): max |dA| %.4g, median rel err %.3g, %d of 200 above 1e-4; "
+                      "shells with a pole crossing the window edge %s (%d omegas), worst elsewhere %.3g" % (
+                          KEYS[q], scale, np.median(err), int(bad.sum()), sorted(cross), int(hit.sum()),
+                          err[~hit].max() if (~hit).any() else 0.0), flush=True)
+                # the omegas of a shell with a pole that the step carries across the
+                # window edge see the jump; every other omega agrees
+                assert not np.any(bad & ~hit), (KEYS[q], np.flatnonzero(bad & ~hit))
+                assert np.all(err[~hit] <= 1e-6), (KEYS[q], err[~hit].max())

@@ -48,3 +48,28 @@ def test_handles_side_by_side(ctx):
     assert np.array_equal(a.spec_omega(ws), ra)
     assert a.memory()["now"] < held_all   # the closed handles' buffers left the context
     a.close()
+
+
+def test_context_closed_before_its_handles():
+    """A context destroyed while DM handles are open: through the binding the
+    handles are closed first; through the C-ABI the handles are detached, so
+    their calls fail with NRG_ERR_STATE and nrg_dm_destroy frees nothing twice."""
+    import ctypes
+    import twochnrg_amd as t
+    print("dm handles outliving their context", flush=True)
+    c = t.Context(0)
+    d = c.dm(**QSZ)
+    c.close()
+    assert d.h is None and c.h is None
+    # C-ABI order: context first, handle second
+    L = t.lib()
+    ch, dh = ctypes.c_void_p(), ctypes.c_void_p()
+    assert L.nrg_ctx_create(0, ctypes.byref(ch)) == 0
+    p = t.DmParams(**{k: v for k, v in QSZ.items()})
+    assert L.nrg_dm_create(ch, ctypes.byref(p), ctypes.byref(dh)) == 0
+    L.nrg_ctx_destroy(ch)
+    out = np.zeros(1)
+    rc = L.nrg_dm_norm(dh, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    assert rc == -4, rc   # NRG_ERR_STATE
+    assert b"context" in L.nrg_last_error()
+    L.nrg_dm_destroy(dh)

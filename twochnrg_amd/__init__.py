@@ -5,6 +5,7 @@ built in-tree by twochnrg_amd/build.py).  The HIP library is required: every
 entry point raises if it is missing; there is no CPU fallback.
 """
 import ctypes
+import weakref
 import os
 
 import numpy as np
@@ -21,6 +22,7 @@ EXPORTS = [
     "nrg_dm_create", "nrg_dm_destroy", "nrg_dm_info", "nrg_dm_timing", "nrg_dm_memory", "nrg_dm_rho", "nrg_dm_spec",
     "nrg_dm_spectrum", "nrg_run_onechqs_multi", "nrg_dm_spec_omega", "nrg_dm_norm", "nrg_dm_weights",
     "nrg_run_twochqsz", "nrg_onechqs_thermo", "nrg_onechqs_jvp", "nrg_dm_create_jvp", "nrg_dm_spec_jvp",
+    "nrg_spectrum_plan", "nrg_spectrum_combine",
 ]
 
 KSTAT_NAMES = ["eig", "hn", "fbasis", "gemm", "tdb_symv"]
@@ -162,6 +164,9 @@ def lib():
     L.nrg_dm_spec_omega.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp]
     L.nrg_dm_norm.argtypes = [ctypes.c_void_p, _dp]
     L.nrg_dm_weights.argtypes = [ctypes.c_void_p, _dp, ctypes.c_int, _dp]
+    L.nrg_spectrum_plan.argtypes = [ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _ip,
+                                    _ip, _dp, _dp, _ip]
+    L.nrg_spectrum_combine.argtypes = [ctypes.c_int, _dp, _ip, _dp, ctypes.c_int, _dp, _dp]
     _lib = L
     return L
 
@@ -179,6 +184,28 @@ def device_count():
     return lib().nrg_device_count()
 
 
+def spectrum_plan(Lambda, nshell, windowed, Nw):
+    """Evaluation points of the omega-grid drivers (nrg_spectrum_plan):
+    (shell, omegabar, omega, parity) arrays in the reference's recording order."""
+    n = ctypes.c_int()
+    _check(lib().nrg_spectrum_plan(Lambda, nshell, int(windowed), Nw, 0, ctypes.byref(n), None, None, None, None))
+    sh, par = np.zeros(n.value, np.int32), np.zeros(n.value, np.int32)
+    wb, w = np.zeros(n.value), np.zeros(n.value)
+    _check(lib().nrg_spectrum_plan(Lambda, nshell, int(windowed), Nw, n.value, ctypes.byref(n), _ptr(sh, _ip),
+                                   _ptr(wb), _ptr(w), _ptr(par, _ip)))
+    return sh, wb, w, par
+
+
+def spectrum_combine(omega, parity, value, Nw):
+    """The spectrum from the values at the plan's points (nrg_spectrum_combine)."""
+    w = np.ascontiguousarray(omega, np.float64)
+    par = np.ascontiguousarray(parity, np.int32)
+    v = np.ascontiguousarray(value, np.float64)
+    wo, ro = np.zeros(len(w)), np.zeros(len(w))
+    _check(lib().nrg_spectrum_combine(len(w), _ptr(w), _ptr(par, _ip), _ptr(v), Nw, _ptr(wo), _ptr(ro)))
+    return wo, ro
+
+
 class Context:
     """One HIP stream + device workspaces (nrg_ctx)."""
 
@@ -186,9 +213,13 @@ class Context:
         h = ctypes.c_void_p()
         _check(lib().nrg_ctx_create(device, ctypes.byref(h)))
         self.h = h
+        self._dms = weakref.WeakSet()   # DM-NRG handles made on this context
 
     def close(self):
         if self.h:
+            # the handles' buffers live in the context: release them first
+            for d in list(self._dms):
+                d.close()
             lib().nrg_ctx_destroy(self.h)
             self.h = None
 
@@ -489,6 +520,7 @@ class DMNRG:
 
     def __init__(self, ctx, params, kept=None, dpar=None):
         self.ctx = ctx
+        self.h = None
         h = ctypes.c_void_p()
         if kept is None and dpar is None:
             self.ntan = 0
@@ -505,6 +537,7 @@ class DMNRG:
             _check(lib().nrg_dm_create_jvp(ctx.h, ctypes.byref(params), kfp, kop, self.ntan,
                                            _ptr(dp_) if dp_.size else None, ctypes.byref(h)))
         self.h = h
+        ctx._dms.add(self)
         a, b, t = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
         _check(lib().nrg_dm_info(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(t)))
         self.nshell_max, self.spec_nshell_max, self.temp = a.value, b.value, t.value
@@ -513,6 +546,13 @@ class DMNRG:
         if self.h:
             lib().nrg_dm_destroy(self.h)
             self.h = None
+            self.ctx._dms.discard(self)
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
     def __enter__(self):
         return self

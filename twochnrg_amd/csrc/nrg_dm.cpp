@@ -72,12 +72,19 @@ struct nrg_dm {
   double ms_forward = 0, ms_sweep = 0, ms_poles = 0, ms_broaden = 0;
   uint64_t mem_peak = 0;
   ~nrg_dm() {   // the handle's buffers go back to the device with it
-    if (!c || pfx.empty()) return;
+    if (!c) return;   // detached: the context and its buffers are gone already
+    c->dms.erase(std::remove(c->dms.begin(), c->dms.end(), this), c->dms.end());
+    if (pfx.empty()) return;
     for (auto it = c->bufs.begin(); it != c->bufs.end();)
       if (it->first.compare(0, pfx.size(), pfx) == 0) it = c->bufs.erase(it);
       else ++it;
   }
 };
+
+void nrg::dm_detach_all(nrg_ctx* c) {
+  for (nrg_dm* d : c->dms) d->c = nullptr;
+  c->dms.clear();
+}
 
 namespace {
 
@@ -926,6 +933,7 @@ static int dm_create(nrg_ctx* c, const nrg_dmnrg_params* p, const unsigned char*
 static int dm_fill(nrg_ctx* c, const nrg_dmnrg_params* p, const unsigned char* kept, const int64_t* kept_off,
                    int ntan, const double* dpar, nrg_dm* D) {
   D->c = c;
+  c->dms.push_back(D);
   D->pfx = "dm#" + std::to_string(c->dm_seq++) + ":";
   D->p = *p;
   D->method = p->method;
@@ -1104,6 +1112,7 @@ extern "C" int nrg_dm_create_jvp(nrg_ctx* c, const nrg_dmnrg_params* p, const un
 
 extern "C" int nrg_dm_spec_jvp(nrg_dm* D, int n, const int* shells, const double* omegabar, double* out) {
   if (!D || n < 0 || (n > 0 && (!shells || !omegabar || !out))) return fail(NRG_ERR_ARG, "nrg_dm_spec_jvp: bad arguments");
+  if (!D->c) return fail(NRG_ERR_STATE, "nrg_dm_spec_jvp: the handle's context was destroyed");
   if (D->ntan <= 0) return fail(NRG_ERR_STATE, "nrg_dm_spec_jvp: the handle carries no tangents");
   HIPCHK(hipSetDevice(D->c->device));
   return spec_tasks_jvp(D, n, shells, omegabar, out);
@@ -1121,6 +1130,7 @@ extern "C" int nrg_dm_info(const nrg_dm* D, int* nshell_max, int* spec_nshell_ma
 
 extern "C" int nrg_dm_memory(const nrg_dm* D, uint64_t* before_sweep, uint64_t* now) {
   if (!D) return fail(NRG_ERR_ARG, "nrg_dm_memory: NULL handle");
+  if (!D->c) return fail(NRG_ERR_STATE, "nrg_dm_memory: the handle's context was destroyed");
   if (before_sweep) *before_sweep = D->mem_peak;
   if (now) *now = D->c->held();
   return NRG_OK;
@@ -1134,6 +1144,7 @@ extern "C" int nrg_dm_timing(const nrg_dm* D, double* ms) {
 
 extern "C" int nrg_dm_rho(nrg_dm* D, int N, double* out, int64_t cap, int* nblocks, double* trace) {
   if (!D || N < 0 || N > D->NM) return fail(NRG_ERR_ARG, "nrg_dm_rho: bad shell");
+  if (!D->c) return fail(NRG_ERR_STATE, "nrg_dm_rho: the handle's context was destroyed");
   const DmShell& s = D->sh[N];
   const int nb = s.nblocks();
   if (nblocks) *nblocks = nb;
@@ -1181,12 +1192,14 @@ extern "C" int nrg_dm_rho(nrg_dm* D, int N, double* out, int64_t cap, int* nbloc
 
 extern "C" int nrg_dm_spec(nrg_dm* D, int n, const int* shells, const double* omegabar, double* out) {
   if (!D || n < 0 || (n > 0 && (!shells || !omegabar || !out))) return fail(NRG_ERR_ARG, "nrg_dm_spec: bad arguments");
+  if (!D->c) return fail(NRG_ERR_STATE, "nrg_dm_spec: the handle's context was destroyed");
   HIPCHK(hipSetDevice(D->c->device));
   return spec_tasks(D, n, shells, omegabar, out);
 }
 
 extern "C" int nrg_dm_spec_omega(nrg_dm* D, int n, const double* omega, double* out) {
   if (!D || n < 0 || (n > 0 && (!omega || !out))) return fail(NRG_ERR_ARG, "nrg_dm_spec_omega: bad arguments");
+  if (!D->c) return fail(NRG_ERR_STATE, "nrg_dm_spec_omega: the handle's context was destroyed");
   if (D->method == 0) return fail(NRG_ERR_ARG, "nrg_dm_spec_omega: CFS or FDM only");
   HIPCHK(hipSetDevice(D->c->device));
   return spec_omegas(D, n, omega, out);
@@ -1194,6 +1207,7 @@ extern "C" int nrg_dm_spec_omega(nrg_dm* D, int n, const double* omega, double* 
 
 extern "C" int nrg_dm_norm(nrg_dm* D, double* norm) {
   if (!D || !norm) return fail(NRG_ERR_ARG, "nrg_dm_norm: NULL argument");
+  if (!D->c) return fail(NRG_ERR_STATE, "nrg_dm_norm: the handle's context was destroyed");
   if (D->method == 0) return fail(NRG_ERR_ARG, "nrg_dm_norm: CFS or FDM only");
   HIPCHK(hipSetDevice(D->c->device));
   std::vector<int> sh(D->specNM);
@@ -1217,46 +1231,119 @@ extern "C" int nrg_dm_weights(const nrg_dm* D, double* wn, int cap, double* ztot
   return NRG_OK;
 }
 
-extern "C" int nrg_dm_spectrum(nrg_dm* D, double* omega, double* rho, int cap, int* nout) {
-  if (!D || !nout) return fail(NRG_ERR_ARG, "nrg_dm_spectrum: NULL argument");
-  HIPCHK(hipSetDevice(D->c->device));
-  const double L = D->p.Lambda, fac = std::pow(L, 1.25);
-  const int Nw = D->p.Nw;
-  const bool win = D->method != 0;   // CFS/FDM: CalcSpecDM_NRG(+-omega), two shells later
-  const int N0 = win ? 2 : 0;
-  std::vector<int> sh;
-  std::vector<double> wbar, wout;
-  std::vector<int> parity;
+// The omega-grid drivers apart from the shell sums (CalcSpecDM_NRG_FixedOmegas
+// src/SpecFuncClass.cpp:1932-2233 with NwEachN = 1, CalcSpec_ManyOmegas :2239-2341):
+// the evaluation points in the reference's recording order, and the spectrum
+// from the values at them.
+namespace {
+struct GridPoint {
+  int shell;
+  double wbar;   // DM-NRG: the shell sum is evaluated at this omegabar
+  double w;      // recorded omega (CFS/FDM: the window sum is evaluated here)
+  int parity;    // FixedOmegas: 0 even, 1 odd shells; ManyOmegas: -1
+};
+std::vector<GridPoint> grid_plan(double L, int nshell, bool windowed, int Nw) {
+  const double fac = std::pow(L, 1.25);
+  std::vector<GridPoint> g;
   if (Nw > 0) {
-    // CalcSpec_ManyOmegas: recorded w = +-Lambda^(-i/(2Nw)) Lambda^1.25 D_N; DM-NRG
-    // evaluates the shell sum at +-Lambda^1.25 Lambda^(+i/(2Nw)) as in the reference
-    const int Nmax = D->specNM - 1;
+    // recorded w = +-Lambda^(-i/(2Nw)) Lambda^1.25 D_N; the DM-NRG shell sum at
+    // +-Lambda^1.25 Lambda^(+i/(2Nw)), as the reference evaluates it
+    const int N0 = windowed ? 2 : 0, Nmax = nshell - 1;
     for (int N = N0; N <= Nmax; N++)
       for (int i = 0; i < Nw; i++) {
         const double rf = (double)i / (2.0 * Nw);
-        sh.push_back(N); wbar.push_back(-fac * std::pow(L, rf));
-        wout.push_back(-std::pow(L, -rf) * fac * calc_dn(L, N));
+        g.push_back({N, -fac * std::pow(L, rf), -std::pow(L, -rf) * fac * calc_dn(L, N), -1});
       }
     for (int N = Nmax; N >= N0; N--)
       for (int i = Nw - 1; i >= 0; i--) {
         const double rf = (double)i / (2.0 * Nw);
-        sh.push_back(N); wbar.push_back(fac * std::pow(L, rf));
-        wout.push_back(std::pow(L, -rf) * fac * calc_dn(L, N));
+        g.push_back({N, fac * std::pow(L, rf), std::pow(L, -rf) * fac * calc_dn(L, N), -1});
       }
   } else {
-    // CalcSpecDM_NRG_FixedOmegas with one omega per shell: even then odd shells
+    // even shells then odd ones: -w with N going up, +w coming back down;
+    // CFS/FDM two shells later (:1954)
     for (int par = 0; par < 2; par++) {
+      const int n0 = par + (windowed ? 2 : 0);
       int last = 0;
-      for (int N = par + N0; N < D->specNM; N += 2) {
-        sh.push_back(N); wbar.push_back(-fac); wout.push_back(-fac * calc_dn(L, N)); parity.push_back(par);
+      for (int N = n0; N < nshell; N += 2) {
+        g.push_back({N, -fac, -fac * calc_dn(L, N), par});
         last = N;
       }
-      for (int N = last; N >= par + N0; N -= 2) {
-        sh.push_back(N); wbar.push_back(fac); wout.push_back(fac * calc_dn(L, N)); parity.push_back(par);
-      }
+      for (int N = last; N >= n0; N -= 2) g.push_back({N, fac, fac * calc_dn(L, N), par});
     }
   }
-  std::vector<double> val(sh.size());
+  return g;
+}
+// ManyOmegas records the values as they are; FixedOmegas interpolates the even
+// and the odd set linearly on the sorted union of their omegas and averages, a
+// point outside one set's range taking the other's (outside both: -100)
+// (src/SpecFuncClass.cpp:2112-2222)
+void grid_combine(const std::vector<GridPoint>& g, const std::vector<double>& val, int Nw, std::vector<double>& W,
+                  std::vector<double>& R) {
+  W.clear();
+  R.clear();
+  if (Nw > 0) {
+    for (size_t i = 0; i < g.size(); i++) { W.push_back(g[i].w); R.push_back(val[i]); }
+    return;
+  }
+  std::vector<double> ex, ey, ox, oy;
+  for (size_t i = 0; i < g.size(); i++) {
+    if (g[i].parity == 0) { ex.push_back(g[i].w); ey.push_back(val[i]); }
+    else { ox.push_back(g[i].w); oy.push_back(val[i]); }
+    W.push_back(g[i].w);
+  }
+  std::sort(W.begin(), W.end());
+  for (double w : W) {
+    const bool inE = !(ex.empty() || w < ex.front() || w > ex.back());
+    const bool inO = !(ox.empty() || w < ox.front() || w > ox.back());
+    double rE = inE ? lin_interp(ex, ey, w) : 0.0, rO = inO ? lin_interp(ox, oy, w) : 0.0;
+    if (!inE) { if (!inO) { rE = -100.0; rO = -100.0; } else rE = rO; }
+    else if (!inO) rO = rE;
+    R.push_back(0.5 * (rE + rO));
+  }
+}
+}  // namespace
+
+extern "C" int nrg_spectrum_plan(double Lambda, int nshell, int windowed, int Nw, int cap, int* n, int* shell,
+                                 double* omegabar, double* omega, int* parity) {
+  if (!n || !(Lambda > 1.0) || nshell < 1 || Nw < 0) return fail(NRG_ERR_ARG, "nrg_spectrum_plan: bad arguments");
+  const std::vector<GridPoint> g = grid_plan(Lambda, nshell, windowed != 0, Nw);
+  *n = (int)g.size();
+  if (!shell && !omegabar && !omega && !parity) return NRG_OK;
+  if (cap < (int)g.size()) return fail(NRG_ERR_ARG, "nrg_spectrum_plan: output too small");
+  for (size_t i = 0; i < g.size(); i++) {
+    if (shell) shell[i] = g[i].shell;
+    if (omegabar) omegabar[i] = g[i].wbar;
+    if (omega) omega[i] = g[i].w;
+    if (parity) parity[i] = g[i].parity;
+  }
+  return NRG_OK;
+}
+
+extern "C" int nrg_spectrum_combine(int n, const double* omega, const int* parity, const double* value, int Nw,
+                                    double* omega_out, double* rho_out) {
+  if (n < 0 || (n > 0 && (!omega || !parity || !value || !omega_out || !rho_out)) || Nw < 0)
+    return fail(NRG_ERR_ARG, "nrg_spectrum_combine: bad arguments");
+  std::vector<GridPoint> g(n);
+  for (int i = 0; i < n; i++) g[i] = GridPoint{0, 0.0, omega[i], parity[i]};
+  std::vector<double> W, R;
+  grid_combine(g, std::vector<double>(value, value + n), Nw, W, R);
+  std::copy(W.begin(), W.end(), omega_out);
+  std::copy(R.begin(), R.end(), rho_out);
+  return NRG_OK;
+}
+
+extern "C" int nrg_dm_spectrum(nrg_dm* D, double* omega, double* rho, int cap, int* nout) {
+  if (!D || !nout) return fail(NRG_ERR_ARG, "nrg_dm_spectrum: NULL argument");
+  if (!D->c) return fail(NRG_ERR_STATE, "nrg_dm_spectrum: the handle's context was destroyed");
+  HIPCHK(hipSetDevice(D->c->device));
+  const double L = D->p.Lambda;
+  const bool win = D->method != 0;   // CFS/FDM: CalcSpecDM_NRG(w) at the recorded omegas
+  const std::vector<GridPoint> g = grid_plan(L, D->specNM, win, D->p.Nw);
+  std::vector<int> sh;
+  std::vector<double> wbar, wout;
+  for (const GridPoint& q : g) { sh.push_back(q.shell); wbar.push_back(q.wbar); wout.push_back(q.w); }
+  std::vector<double> val(g.size());
   int rc;
   if (win) {
     if ((rc = spec_omegas(D, (int)wout.size(), wout.data(), val.data()))) return rc;
@@ -1265,25 +1352,7 @@ extern "C" int nrg_dm_spectrum(nrg_dm* D, double* omega, double* rho, int cap, i
     for (size_t i = 0; i < sh.size(); i++) val[i] /= calc_dn(L, sh[i]);
   }
   std::vector<double> W, R;
-  if (Nw > 0) {
-    W = wout; R = val;
-  } else {
-    std::vector<double> ex, ey, ox, oy;
-    for (size_t i = 0; i < sh.size(); i++) {
-      if (parity[i] == 0) { ex.push_back(wout[i]); ey.push_back(val[i]); }
-      else { ox.push_back(wout[i]); oy.push_back(val[i]); }
-    }
-    W = wout;
-    std::sort(W.begin(), W.end());
-    for (double w : W) {
-      const bool inE = !(ex.empty() || w < ex.front() || w > ex.back());
-      const bool inO = !(ox.empty() || w < ox.front() || w > ox.back());
-      double rE = inE ? lin_interp(ex, ey, w) : 0.0, rO = inO ? lin_interp(ox, oy, w) : 0.0;
-      if (!inE) { if (!inO) { rE = -100.0; rO = -100.0; } else rE = rO; }
-      else if (!inO) rO = rE;
-      R.push_back(0.5 * (rE + rO));
-    }
-  }
+  grid_combine(g, val, D->p.Nw, W, R);
   *nout = (int)W.size();
   if (omega || rho) {
     if (cap < (int)W.size()) return fail(NRG_ERR_ARG, "nrg_dm_spectrum: output too small");

@@ -112,6 +112,7 @@ extern "C" void nrg_ctx_destroy(nrg_ctx* c) {
   if (!c) return;
   hipSetDevice(c->device);
   hipStreamSynchronize(c->stream);
+  nrg::dm_detach_all(c);
   c->resolve();
   for (auto e : c->event_pool) hipEventDestroy(e);
   c->bufs.clear();

@@ -12,6 +12,8 @@
 
 namespace nrg {
 int fail(int code, const std::string& msg);
+// every live DM-NRG handle of c forgets its context (nrg_ctx_destroy before nrg_dm_destroy)
+void dm_detach_all(nrg_ctx* c);
 }
 
 #define HIPCHK(x)                                                                               \
@@ -59,6 +61,7 @@ struct nrg_ctx {
   // move a buffer to another name (replacing what that name held)
   void rename(const std::string& from, const std::string& to);
   int dm_seq = 0;      // DM-NRG handles created on this context (buffer name prefixes)
+  std::vector<nrg_dm*> dms;   // live DM-NRG handles: detached when the context goes first
   // small host-to-device copies go through a pinned ring on the stream, with no
   // wait: the ring is reused only after a stream synchronisation
   char* ring = nullptr;
