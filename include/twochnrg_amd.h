@@ -99,6 +99,22 @@ typedef struct {
 int nrg_rotate_ops(nrg_ctx *ctx, int npairs, const nrg_pair_desc *pairs, const double *Z, int64_t zlen,
                    const double *F, int64_t flen, double *out, int64_t outlen, int precision);
 
+/* No counterpart in the reference.  nrg_rotate_ops on device buffers that
+ * allocates nothing, for callers that cannot call hipMalloc (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid allocation).
+ * dZ, dF, dOut are device pointers on the current HIP device (zlen, flen,
+ * outlen doubles, pair offsets as in nrg_rotate_ops); stream is a hipStream_t
+ * (NULL: the null stream); the call returns after the stream has finished it.
+ * dOut is zero wherever no pair writes, as nrg_rotate_ops leaves it.  The
+ * intermediate products (sum over pairs of kj*ni doubles) and the GEMM
+ * descriptors live in a fixed scratch area of NRG_NOALLOC_SCRATCH_BYTES per
+ * device, which the library holds until the process ends; a call that needs
+ * more fails with NRG_ERR_ARG before any work.  Every call on a device shares
+ * that device's scratch area: calls that overlap on one device race. */
+#define NRG_NOALLOC_SCRATCH_BYTES (64ll << 20)
+int nrg_rotate_ops_noalloc(void *stream, int npairs, const nrg_pair_desc *pairs, const double *dZ, int64_t zlen,
+                           const double *dF, int64_t flen, double *dOut, int64_t outlen, int precision);
+
 /* ---------------- per-kernel timing ---------------- */
 
 /* Kernel classes timed with HIP events on the context stream. */

@@ -22,7 +22,7 @@ EXPORTS = [
     "nrg_dm_create", "nrg_dm_destroy", "nrg_dm_info", "nrg_dm_timing", "nrg_dm_memory", "nrg_dm_rho", "nrg_dm_spec",
     "nrg_dm_spectrum", "nrg_run_onechqs_multi", "nrg_dm_spec_omega", "nrg_dm_norm", "nrg_dm_weights",
     "nrg_run_twochqsz", "nrg_onechqs_thermo", "nrg_onechqs_jvp", "nrg_dm_create_jvp", "nrg_dm_spec_jvp",
-    "nrg_spectrum_plan", "nrg_spectrum_combine",
+    "nrg_spectrum_plan", "nrg_spectrum_combine", "nrg_rotate_ops_noalloc",
 ]
 
 KSTAT_NAMES = ["eig", "hn", "fbasis", "gemm", "tdb_symv"]
@@ -167,6 +167,9 @@ def lib():
     L.nrg_spectrum_plan.argtypes = [ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _ip,
                                     _ip, _dp, _dp, _ip]
     L.nrg_spectrum_combine.argtypes = [ctypes.c_int, _dp, _ip, _dp, ctypes.c_int, _dp, _dp]
+    L.nrg_rotate_ops_noalloc.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(PairDesc), ctypes.c_void_p,
+                                         ctypes.c_int64, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p,
+                                         ctypes.c_int64, ctypes.c_int]
     _lib = L
     return L
 

@@ -405,21 +405,16 @@ extern "C" int nrg_truncate(int nstates, double* en, int ncut, double myeps, uin
 // ---------------------------------------------------------------------------
 // rotations: out = Zi * F * Zj^T as two NT GEMMs through auxT = Zj * F^T
 // ---------------------------------------------------------------------------
-int nrg::rotate_dev(nrg_ctx* c, const std::vector<RotJob>& jobs, const double* dZ, const double* dF, double* dAux,
-                    double* dOut) {
-  if (jobs.empty()) return NRG_OK;
-  if (c->plain && jobs.size() > 1) {
-    for (size_t q = 0; q < jobs.size(); q++) {
-      std::vector<RotJob> one(1, jobs[q]);
-      int rc = rotate_dev(c, one, dZ, dF, dAux, dOut);
-      if (rc) return rc;
-    }
-    return NRG_OK;
-  }
-  // pass 1: auxT(kj x ni) = Zj(kj x nj) * F(ni x nj)^T ; pass 2: out(ki x kj) = Zi(ki x ni) * auxT(kj x ni)^T,
-  // both on the FP64 MFMA tile kernel of the eigensolver (B^T read through its strides)
-  std::vector<E2Gemm> pall;
-  std::vector<E2Tile> t1, t2;
+// The two grouped GEMM passes of a batch of rotations, on the FP64 MFMA tile
+// kernel of the eigensolver (B^T read through its strides):
+//   pass 1: auxT(kj x ni) = Zj(kj x nj) * F(ni x nj)^T
+//   pass 2: out(ki x kj)  = Zi(ki x ni) * auxT(kj x ni)^T
+// probs holds pass 1's problems then pass 2's; tiles likewise, the first n1tiles
+// of them pass 1's.  Returns the number of pass-1 problems (0: nothing to do).
+int nrg::rot_plan(const std::vector<RotJob>& jobs, std::vector<E2Gemm>& pall, std::vector<E2Tile>& tall,
+                  size_t& n1tiles) {
+  pall.clear();
+  tall.clear();
   for (size_t q = 0; q < jobs.size(); q++) {
     const RotJob& J = jobs[q];
     if (J.ki == 0 || J.kj == 0) continue;
@@ -443,21 +438,38 @@ int nrg::rotate_dev(nrg_ctx* c, const std::vector<RotJob>& jobs, const double* d
     h.c_off = J.out_off; h.cs0 = J.kj;
     pall.push_back(h);
   }
-  if (n1 == 0) return NRG_OK;
+  n1tiles = 0;
   for (int q = 0; q < (int)pall.size(); q++) {
     const int tm = (pall[q].m + 63) / 64, tn = (pall[q].n + 63) / 64;
-    std::vector<E2Tile>& T = (q < n1) ? t1 : t2;
     for (int a = 0; a < tm; a++)
-      for (int b = 0; b < tn; b++) T.push_back(E2Tile{q, a, b, 0});
+      for (int b = 0; b < tn; b++) tall.push_back(E2Tile{q, a, b, 0});
+    if (q == n1 - 1) n1tiles = tall.size();
   }
+  return n1;
+}
+
+int nrg::rotate_dev(nrg_ctx* c, const std::vector<RotJob>& jobs, const double* dZ, const double* dF, double* dAux,
+                    double* dOut) {
+  if (jobs.empty()) return NRG_OK;
+  if (c->plain && jobs.size() > 1) {
+    for (size_t q = 0; q < jobs.size(); q++) {
+      std::vector<RotJob> one(1, jobs[q]);
+      int rc = rotate_dev(c, one, dZ, dF, dAux, dOut);
+      if (rc) return rc;
+    }
+    return NRG_OK;
+  }
+  std::vector<E2Gemm> pall;
+  std::vector<E2Tile> tall;
+  size_t n1t = 0;
+  if (rot_plan(jobs, pall, tall, n1t) == 0) return NRG_OK;
+  const size_t n1tiles = n1t;
   DBuf& dp = c->buf("rot_probs");
   DBuf& dt = c->buf("rot_tiles");
-  const size_t np = pall.size(), nt = t1.size() + t2.size();
+  const size_t np = pall.size(), nt = tall.size();
   int rc;
   if ((rc = dp.reserve(sizeof(E2Gemm) * np))) return rc;
   if ((rc = dt.reserve(sizeof(E2Tile) * nt))) return rc;
-  std::vector<E2Tile> tall(t1);
-  tall.insert(tall.end(), t2.begin(), t2.end());
   HIPCHK(hipMemcpyAsync(dp.p, pall.data(), sizeof(E2Gemm) * np, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(dt.p, tall.data(), sizeof(E2Tile) * nt, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -468,9 +480,30 @@ int nrg::rotate_dev(nrg_ctx* c, const std::vector<RotJob>& jobs, const double* d
   }
   PendingTimer tm;
   if ((rc = c->timer_begin(tm, NRG_K_GEMM, fl, by))) return rc;
-  HIPCHK(e2_launch_gemm3((E2Gemm*)dp.p, (E2Tile*)dt.p, (int)t1.size(), dZ, dF, dAux, c->stream));
-  HIPCHK(e2_launch_gemm3((E2Gemm*)dp.p, (E2Tile*)dt.p + t1.size(), (int)t2.size(), dZ, dAux, dOut, c->stream));
+  HIPCHK(e2_launch_gemm3((E2Gemm*)dp.p, (E2Tile*)dt.p, (int)n1tiles, dZ, dF, dAux, c->stream));
+  HIPCHK(e2_launch_gemm3((E2Gemm*)dp.p, (E2Tile*)dt.p + n1tiles, (int)(nt - n1tiles), dZ, dAux, dOut, c->stream));
   if ((rc = c->timer_end(tm))) return rc;
+  return NRG_OK;
+}
+
+int nrg::rot_jobs(int npairs, const nrg_pair_desc* pairs, int64_t zlen, int64_t flen, int64_t outlen,
+                  std::vector<RotJob>& jobs, int64_t& aux, const char* who) {
+  jobs.assign(npairs, RotJob{});
+  aux = 0;
+  for (int q = 0; q < npairs; q++) {
+    const nrg_pair_desc& P = pairs[q];
+    if (P.ni < 0 || P.nj < 0 || P.ki < 0 || P.kj < 0 || P.ki > P.ni || P.kj > P.nj)
+      return fail(NRG_ERR_ARG, std::string(who) + ": bad pair shape");
+    if (P.zi_off < 0 || P.zj_off < 0 || P.f_off < 0 || P.out_off < 0 ||
+        P.zi_off + (int64_t)P.ki * P.ni > zlen || P.zj_off + (int64_t)P.kj * P.nj > zlen ||
+        P.f_off + (int64_t)P.ni * P.nj > flen || P.out_off + (int64_t)P.ki * P.kj > outlen)
+      return fail(NRG_ERR_ARG, std::string(who) + ": pair exceeds buffer");
+    RotJob& J = jobs[q];
+    J.ni = P.ni; J.nj = P.nj; J.ki = P.ki; J.kj = P.kj;
+    J.zi_off = P.zi_off; J.zj_off = P.zj_off; J.ldzi = P.ni; J.ldzj = P.nj;
+    J.f_off = P.f_off; J.out_off = P.out_off; J.aux_off = aux;
+    aux += (int64_t)P.kj * P.ni;
+  }
   return NRG_OK;
 }
 
@@ -480,21 +513,10 @@ extern "C" int nrg_rotate_ops(nrg_ctx* c, int npairs, const nrg_pair_desc* pairs
     return fail(NRG_ERR_ARG, "nrg_rotate_ops: bad arguments");
   if (precision != 0) return fail(NRG_ERR_ARG, "nrg_rotate_ops: only precision 0 (FP64) is available");
   HIPCHK(hipSetDevice(c->device));
-  std::vector<RotJob> jobs(npairs);
+  std::vector<RotJob> jobs;
   int64_t aux = 0;
-  for (int q = 0; q < npairs; q++) {
-    const nrg_pair_desc& P = pairs[q];
-    if (P.ni < 0 || P.nj < 0 || P.ki < 0 || P.kj < 0 || P.ki > P.ni || P.kj > P.nj)
-      return fail(NRG_ERR_ARG, "nrg_rotate_ops: bad pair shape");
-    if (P.zi_off + (int64_t)P.ki * P.ni > zlen || P.zj_off + (int64_t)P.kj * P.nj > zlen ||
-        P.f_off + (int64_t)P.ni * P.nj > flen || P.out_off + (int64_t)P.ki * P.kj > outlen)
-      return fail(NRG_ERR_ARG, "nrg_rotate_ops: pair exceeds buffer");
-    RotJob& J = jobs[q];
-    J.ni = P.ni; J.nj = P.nj; J.ki = P.ki; J.kj = P.kj;
-    J.zi_off = P.zi_off; J.zj_off = P.zj_off; J.ldzi = P.ni; J.ldzj = P.nj;
-    J.f_off = P.f_off; J.out_off = P.out_off; J.aux_off = aux;
-    aux += (int64_t)P.kj * P.ni;
-  }
+  int rc0 = rot_jobs(npairs, pairs, zlen, flen, outlen, jobs, aux, "nrg_rotate_ops");
+  if (rc0) return rc0;
   DBuf& dZ = c->buf("rot_Z");
   DBuf& dF = c->buf("rot_F");
   DBuf& dA = c->buf("rot_aux");

@@ -9,6 +9,7 @@
 
 #include "../../include/twochnrg_amd.h"
 #include "nrg_kernels.h"
+#include "nrg_eig2.h"
 
 namespace nrg {
 int fail(int code, const std::string& msg);
@@ -95,6 +96,13 @@ int eig_dev(nrg_ctx* c, int nb, const int* n, const int64_t* a_off, double* dA, 
 int eig2_dev(nrg_ctx* c, int nb, const int* n, const int64_t* a_off, double* dA, const int64_t* w_off, double* dW);
 int rotate_dev(nrg_ctx* c, const std::vector<RotJob>& jobs, const double* dZ, const double* dF, double* dAux,
                double* dOut);
+// the descriptors of rotate_dev's two GEMM passes (see nrg_host.cpp)
+int rot_plan(const std::vector<RotJob>& jobs, std::vector<E2Gemm>& pall, std::vector<E2Tile>& tall,
+             size_t& n1tiles);
+// RotJobs of nrg_rotate_ops' pairs after its bounds checks; aux_len receives the
+// doubles of the intermediate products
+int rot_jobs(int npairs, const nrg_pair_desc* pairs, int64_t zlen, int64_t flen, int64_t outlen,
+             std::vector<RotJob>& jobs, int64_t& aux_len, const char* who);
 void set_degen(std::vector<double>& en, double eps);
 double ecut_of(const std::vector<double>& en, int ncut);
 
