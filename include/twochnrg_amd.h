@@ -309,7 +309,7 @@ typedef struct {
  *            discarded state (reference: only inside its both-blocks-discarded branch)
  *   NORM     every pole carries its whole weight at both signs of omega and CalcNorm counts it once
  *            (reference: the discarded-discarded terms sit in Positive and Negative, so CalcNorm counts
- *            them twice, src/SpecFunc_allSTL.cpp:2328-2348) */
+ *            them twice, src/SpecFunc_allSTL.cpp:2326-2348) */
 #define NRG_FDM_FIX_RHOMAX 1
 #define NRG_FDM_FIX_DN 2
 #define NRG_FDM_FIX_LAST 4

@@ -135,7 +135,7 @@ typedef struct {
                                    both-blocks-discarded branch) */
 #define ORC_FDM_FIX_NORM 32     /* every pole carries its whole weight at both signs of omega and CalcNorm
                                    counts it once (reference: DD terms in Positive and Negative, so CalcNorm
-                                   counts them twice, src/SpecFunc_allSTL.cpp:2328-2348) */
+                                   counts them twice, src/SpecFunc_allSTL.cpp:2326-2348) */
 #define ORC_FDM_EXACT 63
 
 typedef struct orc_dm orc_dm;
