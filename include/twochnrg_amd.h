@@ -378,6 +378,76 @@ int nrg_dm_weights(const nrg_dm *dm, double *wn, int cap, double *ztot);
  * omega/rho may be NULL to query it. */
 int nrg_dm_spectrum(nrg_dm *dm, double *omega, double *rho, int cap, int *n);
 
+/* ---------------- DM-NRG over the caller's shells ---------------- */
+
+/*
+ * One NRG shell N as SaveArrays writes it and DM_NRG reads it back
+ * (src/NRGcodehandlerClass.cpp:887-933, src/DM_NRG/DM_NRG.cpp:177-298), as
+ * plain arrays in the layout of the reference's objects (host pointers):
+ *   Acut_N (CNRGbasisarray): nblocks blocks of nq = 2 quantum numbers ((Q, S),
+ *     or (Q, Sz) for model 1) in qn; BlockBegEnd in bbe and BlockBegEndBC in
+ *     bbe_bc (inclusive pairs, 2*nblocks); dEn in en (nstates); Kept in kept
+ *     (nstates bytes, NULL: every state kept); dEigVec in evec with
+ *     BlockBegEndEigVec in bbe_ev: block b's rows (bsize x bsize_bc, row i =
+ *     eigenvector i over the block's pre-cut basis) start at bbe_ev[2b];
+ *     evec_len doubles in all;
+ *   Abasis_N (CNRGbasisarray): basis_nblocks blocks, basis_qn, basis_bbe, and
+ *     per basis state basis_itype (site state) and basis_stcf (StCameFrom: the
+ *     Acut_{N-1} state it came from; unused at N = 0);
+ *   the spectral operator at N (the Mat0 file, CNRGmatrix): op_nblocks block
+ *     pairs MatBlockMap in op_map, MatBlockBegEnd in op_mbe (inclusive), row-major
+ *     k1 x k2 blocks over Acut_N in op (MatEl, op_len doubles).
+ */
+typedef struct {
+  int nblocks, nq, nstates;
+  const double *qn;
+  const int *bbe, *bbe_bc, *bbe_ev;
+  const double *en;
+  const unsigned char *kept;
+  const double *evec;
+  int64_t evec_len;
+  int basis_nblocks, basis_nstates;
+  const double *basis_qn;
+  const int *basis_bbe, *basis_itype, *basis_stcf;
+  int op_nblocks;
+  const int *op_map, *op_mbe;
+  const double *op;
+  int64_t op_len;
+} nrg_dm_shell;
+
+/*
+ * DM_NRG on shells the caller owns: DM_NRG_SetRhoNmax, DM_NRG_CalcRhoN_withSU2 /
+ * DM_NRG_CalcRhoN (include/DM_NRG.hpp:40-55) and the CSpecFunction shell sums
+ * (include/SpecFuncClass.hpp:107-149) over shells[0 .. nshells-1] = N = 0 ..
+ * nshells-1 (the reference's Nsites0 = 0; nshells = its Nsitesmax).  p gives
+ * the DM_NRG switches (Lambda, Mtemp, betabar, twindow, broadtemp, bbroad, Nw,
+ * method, fdm_site_dim, fdm_fix) and the site through model (0: the OneChQS
+ * site with SU(2) Clebsch-Gordan weights; 1: the two-channel (Q, Sz) site,
+ * plain partial traces); the chain parameters are not used.  The arrays are
+ * read during the call only.  The handle works with every nrg_dm_* call.
+ */
+int nrg_dm_create_from(nrg_ctx *ctx, const nrg_dmnrg_params *p, int nshells, const nrg_dm_shell *shells,
+                       nrg_dm **out);
+
+/*
+ * The reference's .bin hand-off files, byte for byte (int32, 1-byte bool,
+ * double, little-endian; CNRGarray::SaveBin/ReadBin src/NRGarrayClass.cpp:778-945,
+ * CNRGbasisarray::SaveBin/ReadBin :1947-2075, CNRGmatrix::SaveBin/ReadBin
+ * src/NRGmatrixClass.cpp:957-1077):
+ * nrg_save_arrays runs the chain of p as nrg_dm_create does and writes, per shell
+ * N, what CNRGCodeHandler::SaveArrays (src/NRGcodehandlerClass.cpp:887-933) writes:
+ * <dir>/Abasis_<id>_N<N>.bin, <dir>/Acut_<id>_N<N>.bin and <dir>/Mat0_<id>_N<N>.bin
+ * (the spectral operator: c_d, or d_up for model 1); *nshells receives the count.
+ * nrg_dm_create_from_files reads those files for N = 0 .. nshells-1 (as
+ * src/DM_NRG/DM_NRG.cpp:177-298 and :312-434 do) and calls nrg_dm_create_from.
+ * nrg_dm_save_rho writes the handle's rho_N as <dir>/rhoDM_<id>_N<N>.bin (the
+ * files DM_NRG saves for rho_Nmax and after each CalcRhoN, src/DM_NRG/DM_NRG.cpp:286-297,372-394: packed upper blocks).
+ */
+int nrg_save_arrays(nrg_ctx *ctx, const nrg_dmnrg_params *p, const char *dir, const char *id, int *nshells);
+int nrg_dm_create_from_files(nrg_ctx *ctx, const nrg_dmnrg_params *p, const char *dir, const char *id, int nshells,
+                             nrg_dm **out);
+int nrg_dm_save_rho(nrg_dm *dm, const char *dir, const char *id);
+
 /*
  * The omega-grid drivers on their own, for shell sums computed elsewhere:
  * CalcSpecDM_NRG_FixedOmegas (Nw = 0, one omega per shell as DM_NRG_CalcSpecFunc_ij

@@ -80,7 +80,8 @@ def test_struct_layouts_match_header(tmp_path):
         pytest.skip("no host C compiler")
     pairs = [("nrg_step", t.NrgStep), ("nrg_tangent_step", t.NrgTangentStep),
              ("nrg_onechqs_params", t.OneChQSParams), ("nrg_twochqs_params", t.TwoChQSParams),
-             ("nrg_dmnrg_params", t.DmParams), ("nrg_pair_desc", t.PairDesc), ("nrg_stats", t.NrgStats)]
+             ("nrg_dmnrg_params", t.DmParams), ("nrg_pair_desc", t.PairDesc), ("nrg_stats", t.NrgStats),
+             ("nrg_dm_shell", t.DmShellView)]
     lines = ['#include <stddef.h>', '#include <stdio.h>', '#include "twochnrg_amd.h"', "int main(void) {"]
     expect = []
     for cname, cls in pairs:

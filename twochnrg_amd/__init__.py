@@ -23,6 +23,7 @@ EXPORTS = [
     "nrg_dm_spectrum", "nrg_run_onechqs_multi", "nrg_dm_spec_omega", "nrg_dm_norm", "nrg_dm_weights",
     "nrg_run_twochqsz", "nrg_onechqs_thermo", "nrg_onechqs_jvp", "nrg_dm_create_jvp", "nrg_dm_spec_jvp",
     "nrg_spectrum_plan", "nrg_spectrum_combine", "nrg_rotate_ops_noalloc",
+    "nrg_dm_create_from", "nrg_save_arrays", "nrg_dm_create_from_files", "nrg_dm_save_rho",
 ]
 
 KSTAT_NAMES = ["eig", "hn", "fbasis", "gemm", "tdb_symv"]
@@ -96,6 +97,18 @@ FDM_FIX_RHOMAX, FDM_FIX_DN, FDM_FIX_LAST, FDM_FIX_PAIRS, FDM_FIX_ONESIDED, FDM_F
 FDM_EXACT = 63
 
 
+class DmShellView(ctypes.Structure):
+    """nrg_dm_shell: one shell's Acut_N, Abasis_N and operator arrays (host pointers)."""
+    _fields_ = [
+        ("nblocks", ctypes.c_int), ("nq", ctypes.c_int), ("nstates", ctypes.c_int),
+        ("qn", _dp), ("bbe", _ip), ("bbe_bc", _ip), ("bbe_ev", _ip), ("en", _dp),
+        ("kept", ctypes.POINTER(ctypes.c_ubyte)), ("evec", _dp), ("evec_len", ctypes.c_int64),
+        ("basis_nblocks", ctypes.c_int), ("basis_nstates", ctypes.c_int), ("basis_qn", _dp),
+        ("basis_bbe", _ip), ("basis_itype", _ip), ("basis_stcf", _ip),
+        ("op_nblocks", ctypes.c_int), ("op_map", _ip), ("op_mbe", _ip), ("op", _dp), ("op_len", ctypes.c_int64),
+    ]
+
+
 class PairDesc(ctypes.Structure):
     _fields_ = [("ni", ctypes.c_int), ("nj", ctypes.c_int), ("ki", ctypes.c_int), ("kj", ctypes.c_int),
                 ("zi_off", ctypes.c_int64), ("zj_off", ctypes.c_int64),
@@ -167,6 +180,12 @@ def lib():
     L.nrg_spectrum_plan.argtypes = [ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _ip,
                                     _ip, _dp, _dp, _ip]
     L.nrg_spectrum_combine.argtypes = [ctypes.c_int, _dp, _ip, _dp, ctypes.c_int, _dp, _dp]
+    L.nrg_dm_create_from.argtypes = [ctypes.c_void_p, ctypes.POINTER(DmParams), ctypes.c_int,
+                                     ctypes.POINTER(DmShellView), ctypes.POINTER(ctypes.c_void_p)]
+    L.nrg_save_arrays.argtypes = [ctypes.c_void_p, ctypes.POINTER(DmParams), ctypes.c_char_p, ctypes.c_char_p, _ip]
+    L.nrg_dm_create_from_files.argtypes = [ctypes.c_void_p, ctypes.POINTER(DmParams), ctypes.c_char_p,
+                                           ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
+    L.nrg_dm_save_rho.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_char_p]
     L.nrg_rotate_ops_noalloc.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(PairDesc), ctypes.c_void_p,
                                          ctypes.c_int64, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p,
                                          ctypes.c_int64, ctypes.c_int]
@@ -429,6 +448,59 @@ class Context:
                                     model, Gamma2, fdm_site_dim, fdm_fix),
                      kept=kept, dpar=dpar)
 
+    @staticmethod
+    def _dm_params(U=0.5, Gamma=0.02, ed=-0.25, Lambda=2.5, nsitesmax=40, ncut=400, Mtemp=38, betabar=1.0,
+                   disc_scheme=0, twindow=0.0, broadtemp=0.0, bbroad=0.0, Nw=0, myeps=0.0, betabar_nrg=0.0,
+                   method=0, update_bef_cut=0, model=0, Gamma2=0.0, fdm_site_dim=0.0, fdm_fix=0):
+        return DmParams(U, Gamma, ed, Lambda, nsitesmax, ncut, disc_scheme, myeps, betabar_nrg, Mtemp, betabar,
+                        twindow, broadtemp, bbroad, Nw, method, update_bef_cut, model, Gamma2, fdm_site_dim, fdm_fix)
+
+    def save_arrays(self, directory, ident, **kw):
+        """nrg_save_arrays: run the chain of kw and write SaveArrays' Abasis_/Acut_/Mat0_
+        .bin files of every shell into directory; returns the number of shells."""
+        n = ctypes.c_int()
+        p = self._dm_params(**kw)
+        _check(lib().nrg_save_arrays(self.h, ctypes.byref(p), str(directory).encode(), ident.encode(),
+                                     ctypes.byref(n)))
+        return n.value
+
+    def dm_from_files(self, directory, ident, nshells, **kw):
+        """nrg_dm_create_from_files: DM_NRG over the .bin files of shells 0 .. nshells-1."""
+        p = self._dm_params(**kw)
+        h = ctypes.c_void_p()
+        _check(lib().nrg_dm_create_from_files(self.h, ctypes.byref(p), str(directory).encode(), ident.encode(),
+                                              nshells, ctypes.byref(h)))
+        return DMNRG(self, p, handle=h)
+
+    def dm_from_shells(self, shells, **kw):
+        """nrg_dm_create_from: shells = list of dicts with the nrg_dm_shell arrays
+        (qn, bbe, bbe_bc, bbe_ev, en, kept, evec, basis_qn, basis_bbe, basis_itype,
+        basis_stcf, op_map, op_mbe, op)."""
+        keep, views = [], []
+        for sh in shells:
+            a = {k: np.ascontiguousarray(v, dtype=(np.float64 if k in ("qn", "en", "evec", "basis_qn", "op")
+                                                   else np.uint8 if k == "kept" else np.int32))
+                 for k, v in sh.items() if v is not None}
+            keep.append(a)
+            v = DmShellView()
+            v.nblocks, v.nq, v.nstates = len(a["qn"]) // 2, 2, len(a["en"])
+            v.qn, v.en, v.evec = _ptr(a["qn"]), _ptr(a["en"]), _ptr(a["evec"])
+            v.bbe, v.bbe_bc, v.bbe_ev = _ptr(a["bbe"], _ip), _ptr(a["bbe_bc"], _ip), _ptr(a["bbe_ev"], _ip)
+            v.kept = a["kept"].ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte)) if "kept" in a else None
+            v.evec_len = len(a["evec"])
+            v.basis_nblocks, v.basis_nstates = len(a["basis_qn"]) // 2, len(a["basis_stcf"])
+            v.basis_qn, v.basis_bbe = _ptr(a["basis_qn"]), _ptr(a["basis_bbe"], _ip)
+            v.basis_itype, v.basis_stcf = _ptr(a["basis_itype"], _ip), _ptr(a["basis_stcf"], _ip)
+            v.op_nblocks = len(a["op_map"]) // 2
+            v.op_map, v.op_mbe, v.op = _ptr(a["op_map"], _ip), _ptr(a["op_mbe"], _ip), _ptr(a["op"])
+            v.op_len = len(a["op"])
+            views.append(v)
+        arr = (DmShellView * len(views))(*views)
+        p = self._dm_params(**kw)
+        h = ctypes.c_void_p()
+        _check(lib().nrg_dm_create_from(self.h, ctypes.byref(p), len(views), arr, ctypes.byref(h)))
+        return DMNRG(self, p, handle=h)
+
     def run_twochqs(self, U=0.4, Gamma1=0.03, Gamma2=0.03, ed=-0.2, Lambda=3.0, nsitesmax=3, ncut=3000,
                     betabar=0.727, record_blocks=False, keep_steps=True, on_step=None, qsz=False):
         """TwoChQS (SU(2), multiplets) or, with qsz=True, TwoChQSz ((Q, Sz) blocks, states)."""
@@ -521,11 +593,14 @@ def kept_arrays(steps):
 class DMNRG:
     """Handle of a device DM-NRG run (include/twochnrg_amd.h nrg_dm_*)."""
 
-    def __init__(self, ctx, params, kept=None, dpar=None):
+    def __init__(self, ctx, params, kept=None, dpar=None, handle=None):
         self.ctx = ctx
         self.h = None
         h = ctypes.c_void_p()
-        if kept is None and dpar is None:
+        if handle is not None:
+            self.ntan = 0
+            h = handle
+        elif kept is None and dpar is None:
             self.ntan = 0
             _check(lib().nrg_dm_create(ctx.h, ctypes.byref(params), ctypes.byref(h)))
         else:
@@ -562,6 +637,10 @@ class DMNRG:
 
     def __exit__(self, *a):
         self.close()
+
+    def save_rho(self, directory, ident):
+        """nrg_dm_save_rho: rhoDM_<ident>_N<N>.bin of every shell."""
+        _check(lib().nrg_dm_save_rho(self.h, str(directory).encode(), ident.encode()))
 
     def timing(self):
         ms = np.zeros(4)

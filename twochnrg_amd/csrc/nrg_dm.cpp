@@ -19,12 +19,14 @@
 
 #include <algorithm>
 #include <chrono>
+#include <climits>
 #include <cmath>
 #include <map>
 #include <memory>
 #include <string>
 #include <vector>
 
+#include "nrg_bin.h"
 #include "nrg_dm.h"
 #include "nrg_eig2.h"
 #include "nrg_internal.h"
@@ -904,6 +906,7 @@ double lin_interp(const std::vector<double>& x, const std::vector<double>& y, do
 
 static int dm_fill(nrg_ctx* c, const nrg_dmnrg_params* p, const unsigned char* kept, const int64_t* kept_off,
                    int ntan, const double* dpar, nrg_dm* D);
+static int dm_post(nrg_ctx* c, const nrg_dmnrg_params* p, int Nsitesmax, nrg_dm* D);
 
 static int dm_create(nrg_ctx* c, const nrg_dmnrg_params* p, const unsigned char* kept, const int64_t* kept_off,
                      int ntan, const double* dpar, nrg_dm** out) {
@@ -972,9 +975,22 @@ static int dm_fill(nrg_ctx* c, const nrg_dmnrg_params* p, const unsigned char* k
   // now (every shell's eigenvectors and operators) is the most it will hold
   c->trim("dm#");
   D->mem_peak = c->held();
-  // temperature from Mtemp (src/DM_NRG/DM_NRG.cpp:102-118); FDM keeps every shell (:105)
   // the TwoChQSz loop diagonalises N = 0 .. nsitesmax: one shell more than NRG_main's sites
-  int Nsitesmax = (p->model == 1) ? (int)D->sh.size() : p->nsitesmax;
+  rc = dm_post(c, p, (p->model == 1) ? (int)D->sh.size() : p->nsitesmax, D);
+  auto t2 = std::chrono::steady_clock::now();
+  D->ms_forward = std::chrono::duration<double, std::milli>(t1 - t0).count();
+  D->ms_sweep = std::chrono::duration<double, std::milli>(t2 - t1).count();
+  return rc;
+}
+
+// DM_NRG main after its shells are in place (src/DM_NRG/DM_NRG.cpp:102-434):
+// temperature, FDM weights, rho_Nmax and the backward sweep over D->sh, whose
+// eigenvector rows are released shell by shell
+static int dm_post(nrg_ctx* c, const nrg_dmnrg_params* p, int Nsitesmax, nrg_dm* D) {
+  const bool fdm = (p->method == 2);
+  const int ntan = D->ntan;
+  int rc;
+  // temperature from Mtemp (src/DM_NRG/DM_NRG.cpp:102-118); FDM keeps every shell (:105)
   double DM;
   if (p->Mtemp < Nsitesmax - 1) {
     if (!fdm) Nsitesmax = p->Mtemp + 1;
@@ -1095,9 +1111,6 @@ static int dm_fill(nrg_ctx* c, const nrg_dmnrg_params* p, const unsigned char* k
   // what is left is the handle's own: rho_N, the operators, the energies' tangents
   for (DmShell& s : D->sh) drop_z(s);
   c->trim("dm#");
-  auto t2 = std::chrono::steady_clock::now();
-  D->ms_forward = std::chrono::duration<double, std::milli>(t1 - t0).count();
-  D->ms_sweep = std::chrono::duration<double, std::milli>(t2 - t1).count();
   return NRG_OK;
 }
 
@@ -1358,6 +1371,293 @@ extern "C" int nrg_dm_spectrum(nrg_dm* D, double* omega, double* rho, int cap, i
     if (cap < (int)W.size()) return fail(NRG_ERR_ARG, "nrg_dm_spectrum: output too small");
     if (omega) std::copy(W.begin(), W.end(), omega);
     if (rho) std::copy(R.begin(), R.end(), rho);
+  }
+  return NRG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// DM_NRG over the caller's shells and the reference's .bin hand-off files
+// ---------------------------------------------------------------------------
+
+// the DM_NRG switches of p (the chain parameters are not used here)
+static int dm_check_switches(const nrg_dmnrg_params* p, const char* who) {
+  if (!(p->Lambda > 1.0) || !(p->betabar > 0.0) || p->Mtemp < 1 || p->method < 0 || p->method > 2 || p->Nw < 0)
+    return fail(NRG_ERR_ARG, std::string(who) + ": bad parameters");
+  if (!(p->fdm_site_dim >= 0.0)) return fail(NRG_ERR_ARG, std::string(who) + ": fdm_site_dim must be 0 or positive");
+  if (p->fdm_fix < 0 || p->fdm_fix > NRG_FDM_EXACT || (p->fdm_fix != 0 && p->method != 2))
+    return fail(NRG_ERR_ARG, std::string(who) + ": fdm_fix takes NRG_FDM_FIX_* bits and needs FDM (method 2)");
+  if (p->model < 0 || p->model > 1) return fail(NRG_ERR_ARG, std::string(who) + ": model must be 0 or 1");
+  return NRG_OK;
+}
+
+// inclusive [beg, end] pairs that tile [0, n) in order (each block non-empty)
+static bool tiles(const int* bbe, int nb, int64_t n) {
+  int64_t next = 0;
+  for (int b = 0; b < nb; b++) {
+    if (bbe[2 * b] != next || bbe[2 * b + 1] < bbe[2 * b]) return false;
+    next = (int64_t)bbe[2 * b + 1] + 1;
+  }
+  return next == n;
+}
+
+// one caller shell into a DmShell; its eigenvector rows and operator blocks go
+// to the device (buffers named for the handle).  Every index the device reads
+// is checked here.
+static int shell_from(nrg_dm* D, int N, const nrg_dm_shell& v, int prev_nstates, DmShell& s) {
+  nrg_ctx* c = D->c;
+  const std::string who = "nrg_dm_create_from: shell " + std::to_string(N);
+  if (v.nq != 2) return fail(NRG_ERR_ARG, who + ": two quantum numbers per block are expected");
+  if (v.nblocks < 1 || v.nstates < 1 || !v.qn || !v.bbe || !v.bbe_bc || !v.bbe_ev || !v.en || !v.evec)
+    return fail(NRG_ERR_ARG, who + ": missing Acut arrays");
+  if (!tiles(v.bbe, v.nblocks, v.nstates)) return fail(NRG_ERR_ARG, who + ": BlockBegEnd does not tile the states");
+  if (v.basis_nblocks < 1 || v.basis_nstates < 1 || !v.basis_qn || !v.basis_bbe || !v.basis_itype || !v.basis_stcf)
+    return fail(NRG_ERR_ARG, who + ": missing Abasis arrays");
+  if (!tiles(v.basis_bbe, v.basis_nblocks, v.basis_nstates))
+    return fail(NRG_ERR_ARG, who + ": Abasis BlockBegEnd does not tile its states");
+  s.N = N;
+  s.sz = (D->p.model == 1);
+  s.qn.assign(v.qn, v.qn + 2 * v.nblocks);
+  s.bbe.assign(v.bbe, v.bbe + 2 * v.nblocks);
+  s.bbe_bc.assign(v.bbe_bc, v.bbe_bc + 2 * v.nblocks);
+  s.en.assign(v.en, v.en + v.nstates);
+  s.kept.assign(v.nstates, 1);
+  if (v.kept) s.kept.assign(v.kept, v.kept + v.nstates);
+  s.zoff.assign(v.nblocks, 0);
+  for (int b = 0; b < v.nblocks; b++) {
+    const int lo = v.bbe_bc[2 * b], hi = v.bbe_bc[2 * b + 1];
+    if (lo < 0 || hi < lo || hi >= v.basis_nstates) return fail(NRG_ERR_ARG, who + ": BlockBegEndBC outside Abasis");
+    const int64_t k = s.bsize(b), n = s.bsize_bc(b);
+    if (k > n) return fail(NRG_ERR_ARG, who + ": a block has more states than its pre-cut basis");
+    s.zoff[b] = v.bbe_ev[2 * b];
+    if (s.zoff[b] < 0 || s.zoff[b] + k * n > v.evec_len)
+      return fail(NRG_ERR_ARG, who + ": eigenvector rows outside evec");
+  }
+  s.stcf.assign(v.basis_stcf, v.basis_stcf + v.basis_nstates);
+  s.itype.assign(v.basis_itype, v.basis_itype + v.basis_nstates);
+  const int ntypes = D->p.model == 1 ? 16 : 4;
+  for (int i = 0; i < v.basis_nstates; i++) {
+    if (N > 0 && (s.stcf[i] < 0 || s.stcf[i] >= prev_nstates))
+      return fail(NRG_ERR_ARG, who + ": StCameFrom outside the previous shell");
+    if (s.itype[i] < 0 || s.itype[i] >= ntypes) return fail(NRG_ERR_ARG, who + ": iType outside the site");
+  }
+  s.bas_bbe.assign(v.basis_bbe, v.basis_bbe + 2 * v.basis_nblocks);
+  s.bas_qn.assign(v.basis_qn, v.basis_qn + 2 * v.basis_nblocks);
+  if (v.op_nblocks < 0 || (v.op_nblocks > 0 && (!v.op_map || !v.op_mbe || !v.op)))
+    return fail(NRG_ERR_ARG, who + ": missing operator arrays");
+  s.op_map.assign(v.op_map, v.op_map + 2 * v.op_nblocks);
+  s.op_mbe.assign(v.op_mbe, v.op_mbe + 2 * v.op_nblocks);
+  for (int q = 0; q < v.op_nblocks; q++) {
+    const int b1 = s.op_map[2 * q], b2 = s.op_map[2 * q + 1];
+    if (b1 < 0 || b1 >= v.nblocks || b2 < 0 || b2 >= v.nblocks) return fail(NRG_ERR_ARG, who + ": MatBlockMap outside Acut");
+    const int64_t lo = s.op_mbe[2 * q], hi = s.op_mbe[2 * q + 1];
+    if (lo < 0 || hi - lo + 1 != (int64_t)s.bsize(b1) * s.bsize(b2) || hi >= v.op_len)
+      return fail(NRG_ERR_ARG, who + ": MatBlockBegEnd does not match the block sizes");
+  }
+  int rc;
+  s.zbuf = D->pfx + "dmZ" + std::to_string(N);
+  s.opbuf = D->pfx + "dmO" + std::to_string(N);
+  DBuf& zb = c->buf(s.zbuf);
+  DBuf& ob = c->buf(s.opbuf);
+  if ((rc = zb.reserve(sizeof(double) * (size_t)(v.evec_len + 1), true))) return rc;
+  if ((rc = ob.reserve(sizeof(double) * (size_t)(v.op_len + 1), true))) return rc;
+  HIPCHK(hipMemcpy(zb.p, v.evec, sizeof(double) * (size_t)v.evec_len, hipMemcpyHostToDevice));
+  if (v.op_len > 0) HIPCHK(hipMemcpy(ob.p, v.op, sizeof(double) * (size_t)v.op_len, hipMemcpyHostToDevice));
+  return NRG_OK;
+}
+
+extern "C" int nrg_dm_create_from(nrg_ctx* c, const nrg_dmnrg_params* p, int nshells, const nrg_dm_shell* shells,
+                                  nrg_dm** out) {
+  if (!c || !p || !out || nshells < 2 || !shells) return fail(NRG_ERR_ARG, "nrg_dm_create_from: bad arguments");
+  int rc;
+  if ((rc = dm_check_switches(p, "nrg_dm_create_from"))) return rc;
+  HIPCHK(hipSetDevice(c->device));
+  std::unique_ptr<nrg_dm> D(new nrg_dm());
+  D->c = c;
+  c->dms.push_back(D.get());
+  D->pfx = "dm#" + std::to_string(c->dm_seq++) + ":";
+  D->p = *p;
+  D->method = p->method;
+  if (D->p.twindow <= 0) D->p.twindow = 2.0;
+  if (D->p.broadtemp <= 0) D->p.broadtemp = 0.8;
+  if (D->p.bbroad <= 0) D->p.bbroad = 0.5 * std::log(p->Lambda);
+  auto t0 = std::chrono::steady_clock::now();
+  D->sh.resize(nshells);
+  for (int N = 0; N < nshells; N++)
+    if ((rc = shell_from(D.get(), N, shells[N], N > 0 ? shells[N - 1].nstates : 0, D->sh[N]))) return rc;
+  D->mem_peak = c->held();
+  auto t1 = std::chrono::steady_clock::now();
+  if ((rc = dm_post(c, p, nshells, D.get()))) return rc;
+  auto t2 = std::chrono::steady_clock::now();
+  D->ms_forward = std::chrono::duration<double, std::milli>(t1 - t0).count();
+  D->ms_sweep = std::chrono::duration<double, std::milli>(t2 - t1).count();
+  *out = D.release();
+  return NRG_OK;
+}
+
+// CNRGarray's quantum-number bookkeeping for a shell's arrays
+static void bin_qn_part(BinArray& a, int N, const std::vector<double>& qn, const std::vector<int>& bbe, bool sz) {
+  a.nshell = N;
+  a.nq = 2;
+  a.qn = qn;
+  a.bbe = bbe;
+  a.totalS = sz ? 0 : 1;
+  a.sqn.clear();
+  if (!sz) a.sqn.push_back(1);   // total S is quantum number 1
+}
+
+extern "C" int nrg_save_arrays(nrg_ctx* c, const nrg_dmnrg_params* p, const char* dir, const char* id,
+                               int* nshells) {
+  if (!c || !p || !id || !nshells) return fail(NRG_ERR_ARG, "nrg_save_arrays: NULL argument");
+  if (p->nsitesmax < 2 || p->ncut < 1 || !(p->Lambda > 1.0) || p->model < 0 || p->model > 1)
+    return fail(NRG_ERR_ARG, "nrg_save_arrays: bad parameters");
+  HIPCHK(hipSetDevice(c->device));
+  std::vector<DmShell> sh;
+  int rc;
+  if (p->model == 1) {
+    nrg_twochqs_params q2{};
+    q2.U = p->U; q2.Gamma1 = p->Gamma; q2.Gamma2 = p->Gamma2; q2.ed = p->ed; q2.Lambda = p->Lambda;
+    q2.nsitesmax = p->nsitesmax; q2.ncut = p->ncut; q2.betabar = p->betabar_nrg;
+    rc = run_twochqsz_dm(c, q2, sh);
+  } else {
+    nrg_onechqs_params q{};
+    q.U = p->U; q.Gamma = p->Gamma; q.ed = p->ed; q.Lambda = p->Lambda; q.nsitesmax = p->nsitesmax;
+    q.ncut = p->ncut; q.calcdens = 1; q.disc_scheme = p->disc_scheme; q.myeps = p->myeps;
+    q.betabar = p->betabar_nrg;
+    rc = run_onechqs_dm(c, q, sh, p->update_bef_cut ? 1 : 0);
+  }
+  // the captured shells' device buffers go back whatever happens below
+  struct Drop {
+    nrg_ctx* c;
+    std::vector<DmShell>& sh;
+    ~Drop() {
+      for (DmShell& s : sh)
+        for (const std::string& nm : {s.zbuf, s.opbuf})
+          if (!nm.empty()) c->drop(nm);
+      c->trim("dm#");
+    }
+  } drop{c, sh};
+  if (rc) return rc;
+  const bool sz = (p->model == 1);
+  std::string err;
+  for (size_t N = 0; N < sh.size(); N++) {
+    const DmShell& s = sh[N];
+    const int nb = s.nblocks();
+    // Acut_N: kept (or, with UpdateBefCut 1 and model 1, every) state, eigenvector rows
+    BinBasis a;
+    bin_qn_part(a, (int)N, s.qn, s.bbe, sz);
+    a.en = s.en;
+    a.kept = s.kept;
+    int64_t zlen = 0;
+    std::vector<int> bbe_ev(2 * nb);
+    for (int b = 0; b < nb; b++) {
+      const int64_t len = (int64_t)s.bsize(b) * s.bsize_bc(b);
+      if (s.zoff[b] != zlen) return fail(NRG_ERR_STATE, "nrg_save_arrays: eigenvector rows are not contiguous");
+      bbe_ev[2 * b] = (int)zlen;
+      bbe_ev[2 * b + 1] = (int)(zlen + len - 1);
+      zlen += len;
+    }
+    if (zlen > INT32_MAX) return fail(NRG_ERR_ARG, "nrg_save_arrays: a shell's eigenvectors exceed the int32 counts of the format");
+    a.evec.resize(zlen);
+    HIPCHK(hipMemcpy(a.evec.data(), c->buf(s.zbuf).p, sizeof(double) * (size_t)zlen, hipMemcpyDeviceToHost));
+    a.bbe_bc = s.bbe_bc;
+    a.bbe_ev = bbe_ev;
+    a.stcf.resize(s.en.size());   // SetSCFfromdEn: each state's index before the cut
+    for (int b = 0; b < nb; b++)
+      for (int i = 0; i < s.bsize(b); i++) a.stcf[s.bbe[2 * b] + i] = s.bbe_bc[2 * b] + i;
+    if (nrg::bin_write(nrg::bin_name(dir, "Acut", id, (int)N), a, err)) return fail(NRG_ERR_ARG, "nrg_save_arrays: " + err);
+    // Abasis_N
+    BinBasis ab;
+    bin_qn_part(ab, (int)N, s.bas_qn, s.bas_bbe, sz);
+    ab.itype = s.itype;
+    ab.stcf = s.stcf;
+    if (nrg::bin_write(nrg::bin_name(dir, "Abasis", id, (int)N), ab, err)) return fail(NRG_ERR_ARG, "nrg_save_arrays: " + err);
+    // Mat0: the spectral operator over Acut_N
+    BinMatrix m;
+    bin_qn_part(m, (int)N, s.qn, s.bbe, sz);
+    m.kept = s.kept;
+    m.map = s.op_map;
+    m.mbe = s.op_mbe;
+    const int64_t nel = s.op_mbe.empty() ? 0 : (int64_t)s.op_mbe.back() + 1;
+    m.matel.resize(nel);
+    if (nel) HIPCHK(hipMemcpy(m.matel.data(), c->buf(s.opbuf).p, sizeof(double) * (size_t)nel, hipMemcpyDeviceToHost));
+    m.need_old = 1;
+    m.wigner_eckart_l = 0.5;
+    if (nrg::bin_write(nrg::bin_name(dir, "Mat0", id, (int)N), m, err)) return fail(NRG_ERR_ARG, "nrg_save_arrays: " + err);
+  }
+  *nshells = (int)sh.size();
+  return NRG_OK;
+}
+
+extern "C" int nrg_dm_create_from_files(nrg_ctx* c, const nrg_dmnrg_params* p, const char* dir, const char* id,
+                                        int nshells, nrg_dm** out) {
+  if (!c || !p || !id || !out || nshells < 2) return fail(NRG_ERR_ARG, "nrg_dm_create_from_files: bad arguments");
+  std::vector<BinBasis> acut(nshells), abas(nshells);
+  std::vector<BinMatrix> mat(nshells);
+  std::vector<nrg_dm_shell> v(nshells);
+  std::string err;
+  for (int N = 0; N < nshells; N++) {
+    if (nrg::bin_read(nrg::bin_name(dir, "Acut", id, N), acut[N], err) ||
+        nrg::bin_read(nrg::bin_name(dir, "Abasis", id, N), abas[N], err) ||
+        nrg::bin_read(nrg::bin_name(dir, "Mat0", id, N), mat[N], err))
+      return fail(NRG_ERR_ARG, "nrg_dm_create_from_files: " + err);
+    const BinBasis& a = acut[N];
+    const BinBasis& b = abas[N];
+    const BinMatrix& m = mat[N];
+    if (a.nq != 2 || b.nq != 2 || a.qn.size() % 2 || b.qn.size() % 2 || a.bbe.size() != a.qn.size() ||
+        b.bbe.size() != b.qn.size() || a.bbe_bc.size() != a.bbe.size() || a.bbe_ev.size() != a.bbe.size() ||
+        (!a.kept.empty() && a.kept.size() != a.en.size()) || b.itype.size() != b.stcf.size() ||
+        m.map.size() != m.mbe.size() || m.is_complex)
+      return fail(NRG_ERR_ARG, "nrg_dm_create_from_files: shell " + std::to_string(N) +
+                                   ": arrays of inconsistent sizes (or complex operators)");
+    nrg_dm_shell& s = v[N];
+    s.nblocks = (int)a.qn.size() / 2;
+    s.nq = 2;
+    s.nstates = (int)a.en.size();
+    s.qn = a.qn.data(); s.bbe = a.bbe.data(); s.bbe_bc = a.bbe_bc.data(); s.bbe_ev = a.bbe_ev.data();
+    s.en = a.en.data();
+    s.kept = a.kept.empty() ? nullptr : a.kept.data();
+    s.evec = a.evec.data(); s.evec_len = (int64_t)a.evec.size();
+    s.basis_nblocks = (int)b.qn.size() / 2;
+    s.basis_nstates = (int)b.stcf.size();
+    s.basis_qn = b.qn.data(); s.basis_bbe = b.bbe.data(); s.basis_itype = b.itype.data();
+    s.basis_stcf = b.stcf.data();
+    s.op_nblocks = (int)m.map.size() / 2;
+    s.op_map = m.map.data(); s.op_mbe = m.mbe.data(); s.op = m.matel.data(); s.op_len = (int64_t)m.matel.size();
+  }
+  return nrg_dm_create_from(c, p, nshells, v.data(), out);
+}
+
+extern "C" int nrg_dm_save_rho(nrg_dm* D, const char* dir, const char* id) {
+  if (!D || !id) return fail(NRG_ERR_ARG, "nrg_dm_save_rho: NULL argument");
+  if (!D->c) return fail(NRG_ERR_STATE, "nrg_dm_save_rho: the handle's context was destroyed");
+  HIPCHK(hipSetDevice(D->c->device));
+  std::string err;
+  for (int N = 0; N <= D->NM; N++) {
+    const DmShell& s = D->sh[N];
+    const int nb = s.nblocks();
+    int64_t full = 0;
+    for (int b = 0; b < nb; b++) full += (int64_t)s.bsize(b) * s.bsize(b);
+    std::vector<double> f(full + 1);
+    int nbo = 0;
+    int rc = nrg_dm_rho(D, N, f.data(), full + 1, &nbo, nullptr);
+    if (rc) return rc;
+    // CNRGmatrix with UpperTriangular blocks (the layout DM_NRG_SetRhoNmax and
+    // CalcRhoN build): each diagonal block's upper triangle row by row
+    BinMatrix m;
+    bin_qn_part(m, N, s.qn, s.bbe, s.sz);
+    m.kept = s.kept;
+    m.upper = 1;
+    int64_t o = 0;
+    for (int b = 0; b < nb; b++) {
+      const int k = s.bsize(b);
+      m.map.push_back(b); m.map.push_back(b);
+      m.mbe.push_back((int)m.matel.size());
+      for (int i = 0; i < k; i++)
+        for (int j = i; j < k; j++) m.matel.push_back(f[o + (int64_t)i * k + j]);
+      m.mbe.push_back((int)m.matel.size() - 1);
+      o += (int64_t)k * k;
+    }
+    if (nrg::bin_write(nrg::bin_name(dir, "rhoDM", id, N), m, err)) return fail(NRG_ERR_ARG, "nrg_dm_save_rho: " + err);
   }
   return NRG_OK;
 }
