@@ -570,7 +570,31 @@ struct Chain {
   int zpar = 0;
   bool record = false;
   bool sz_only = false;            // (Q, Sz) blocks: one coupling table (no spin index)
+  // the eigenvectors are formed after the truncation, kept rows only (nothing
+  // downstream reads a discarded state's eigenvector); finish_vectors forms them
+  bool defer = false;
+  bool vec_pending = false;
   std::vector<double> hrec, zrec;
+
+  // rows of each Aeig block that AeigCut keeps (a prefix of the block), at krow[base + b]
+  void kept_rows(std::vector<int>& krow, int base) const {
+    const int nb = Aeig.NumBlocks();
+    for (int b = 0; b < nb; b++) krow[base + b] = 0;
+    for (int q = 0; q < AeigCut.NumBlocks(); q++) {
+      const int first = AeigCut.BlockBegEndBC[2 * q];
+      for (int b = 0; b < nb; b++)
+        if (Aeig.GetBlockLimit(b, 0) == first) { krow[base + b] = AeigCut.GetBlockSize(q); break; }
+    }
+  }
+  // the deferred eigenvectors of this chain's last eigensolve: the kept rows, or every row
+  int finish_vectors(bool all) {
+    if (!vec_pending) return 0;
+    vec_pending = false;
+    if (all) return eig_finish(c, nullptr);
+    std::vector<int> krow(Aeig.NumBlocks());
+    kept_rows(krow, 0);
+    return eig_finish(c, krow.data());
+  }
 
   void init_ops(int nm) {
     mats.assign(nm, CNRGmatrix());
@@ -718,7 +742,12 @@ struct Chain {
     double* dWn = (double*)wb.p;
     if ((rc = assemble(nch, fops, chi, coef, sqrtL, eN, dZn))) return rc;
     const int nb = Abasis.NumBlocks();
-    if ((rc = eig_dev(c, nb, L_bsize.data(), L_hoff.data(), dZn, L_woff.data(), dWn, nullptr))) return rc;
+    if (defer && !record) {
+      if ((rc = eig_begin(c, nb, L_bsize.data(), L_hoff.data(), dZn, L_woff.data(), dWn))) return rc;
+      vec_pending = true;
+    } else if ((rc = eig_dev(c, nb, L_bsize.data(), L_hoff.data(), dZn, L_woff.data(), dWn, nullptr))) {
+      return rc;
+    }
     return collect(dZn, dWn);
   }
 
@@ -1661,6 +1690,19 @@ struct OneChRun {
   int post(int N, nrg_step_cb cb, void* user, std::vector<DmShell>* dm, Tangent* tg = nullptr,
            const unsigned char* fixed = nullptr) {
     int rc;
+    if ((rc = post_cut(N, tg, fixed))) return rc;
+    // the deferred eigenvectors: every row with UpdateBefCut 1, else the kept ones
+    if ((rc = ch.finish_vectors(befcut != 0))) return rc;
+    return post_rest(N, cb, user, dm, tg);
+  }
+
+  // the step's results between post_cut and post_rest
+  double s_DN = 0, s_emin = 0, s_ecut = 0, s_thT[4] = {}, s_sus[4] = {}, s_ent[4] = {};
+  int s_nkept = 0;
+
+  // SetE0zero, SetKept, CalcThermo, CutStates: everything the truncation decides
+  int post_cut(int N, Tangent* tg, const unsigned char* fixed) {
+    int rc;
     const double DN = HLF * std::pow(Lambda, -(N - 1) / 2.0);
     double emin = *std::min_element(ch.Aeig.dEn.begin(), ch.Aeig.dEn.end());
     for (auto& x : ch.Aeig.dEn) x -= emin;
@@ -1693,6 +1735,19 @@ struct OneChRun {
     // writes that copy, and the real cut follows CalcStuff; the operators stay on
     // the pre-cut blocks, which the next basis indexes through StCameFrom
     ch.cut_states(befcut ? 2 * ns : p.ncut, false, p.myeps, fixed);
+    s_DN = DN; s_emin = emin; s_ecut = ecut; s_nkept = nkept;
+    for (int q = 0; q < 4; q++) { s_thT[q] = thT[q]; s_sus[q] = sus[q]; s_ent[q] = ent[q]; }
+    return 0;
+  }
+
+  // UpdateMatrices, the DM-NRG capture, CalcStuff, the record, chi for the next site
+  int post_rest(int N, nrg_step_cb cb, void* user, std::vector<DmShell>* dm, Tangent* tg) {
+    int rc;
+    const double DN = s_DN, emin = s_emin, ecut = s_ecut;
+    const int nkept = s_nkept;
+    double* thT = s_thT;
+    double* sus = s_sus;
+    double* ent = s_ent;
     if (tg) {
       if ((rc = tg->cut(ch))) return rc;
       if ((rc = ch.plan_update())) return rc;
@@ -1730,6 +1785,9 @@ static int run_onechqs(nrg_ctx* c, nrg_onechqs_params p, nrg_step_cb cb, void* u
   OneChRun R;
   R.befcut = befcut;
   if ((rc = R.init(c, p, "", cb, user))) return rc;
+  // UpdateBefCut 1 rotates the operators over every eigenstate; otherwise only
+  // the kept rows are formed, after the truncation
+  R.ch.defer = (befcut == 0) && !R.ch.record;
   auto t0 = std::chrono::steady_clock::now();
   for (int N = 0; N < p.nsitesmax; N++) {
     R.basis();
@@ -1740,9 +1798,15 @@ static int run_onechqs(nrg_ctx* c, nrg_onechqs_params p, nrg_step_cb cb, void* u
     if ((rc = zb.reserve(sizeof(double) * (size_t)(R.ch.L_ztot + 1)))) return rc;
     if ((rc = wb.reserve(sizeof(double) * (size_t)(R.ch.L_wtot + 1)))) return rc;
     if ((rc = R.assemble((double*)zb.p))) return rc;
-    if ((rc = eig_dev(c, R.ch.Abasis.NumBlocks(), R.ch.L_bsize.data(), R.ch.L_hoff.data(), (double*)zb.p,
-                      R.ch.L_woff.data(), (double*)wb.p, nullptr)))
+    if (R.ch.defer) {
+      if ((rc = eig_begin(c, R.ch.Abasis.NumBlocks(), R.ch.L_bsize.data(), R.ch.L_hoff.data(), (double*)zb.p,
+                          R.ch.L_woff.data(), (double*)wb.p)))
+        return rc;
+      R.ch.vec_pending = true;
+    } else if ((rc = eig_dev(c, R.ch.Abasis.NumBlocks(), R.ch.L_bsize.data(), R.ch.L_hoff.data(), (double*)zb.p,
+                             R.ch.L_woff.data(), (double*)wb.p, nullptr))) {
       return rc;
+    }
     if ((rc = R.ch.collect((double*)zb.p, (double*)wb.p))) return rc;
     if ((rc = R.post(N, cb, user, dm))) return rc;
   }
@@ -1779,6 +1843,8 @@ static int run_onechqs_multi(nrg_ctx* c, int ninst, const nrg_onechqs_params* ps
     R[i].reset(new OneChRun());
     if ((rc = R[i]->init(c, ps[i], "m" + std::to_string(i) + "_", multi_tramp, &cbc[i]))) return rc;
   }
+  bool record_any = false;
+  for (int i = 0; i < ninst; i++) record_any = record_any || R[i]->ch.record;
   auto t0 = std::chrono::steady_clock::now();
   int zpar = 0;
   for (int N = 0; N < nsites; N++) {
@@ -1802,12 +1868,27 @@ static int run_onechqs_multi(nrg_ctx* c, int ninst, const nrg_onechqs_params* ps
       hoff.insert(hoff.end(), R[i]->ch.L_hoff.begin(), R[i]->ch.L_hoff.end());
       woff.insert(woff.end(), R[i]->ch.L_woff.begin(), R[i]->ch.L_woff.end());
     }
-    if ((rc = eig_dev(c, (int)n.size(), n.data(), hoff.data(), (double*)zb.p, woff.data(), (double*)wb.p, nullptr)))
+    // every instance's truncation first, then the kept eigenvector rows of all
+    // instances in one back-transformation, then the instances' updates
+    const bool defer = !record_any;
+    if (defer) {
+      if ((rc = eig_begin(c, (int)n.size(), n.data(), hoff.data(), (double*)zb.p, woff.data(), (double*)wb.p)))
+        return rc;
+    } else if ((rc = eig_dev(c, (int)n.size(), n.data(), hoff.data(), (double*)zb.p, woff.data(), (double*)wb.p,
+                             nullptr))) {
       return rc;
+    }
+    std::vector<int> krow(n.size(), 0);
+    int base = 0;
     for (int i = 0; i < ninst; i++) {
       if ((rc = R[i]->ch.collect((double*)zb.p, (double*)wb.p))) return rc;
-      if ((rc = R[i]->post(N, multi_tramp, &cbc[i], nullptr))) return rc;
+      if ((rc = R[i]->post_cut(N, nullptr, nullptr))) return rc;
+      R[i]->ch.kept_rows(krow, base);
+      base += R[i]->ch.Abasis.NumBlocks();
     }
+    if (defer && (rc = eig_finish(c, krow.data()))) return rc;
+    for (int i = 0; i < ninst; i++)
+      if ((rc = R[i]->post_rest(N, multi_tramp, &cbc[i], nullptr, nullptr))) return rc;
   }
   HIPCHK(hipStreamSynchronize(c->stream));
   auto t1 = std::chrono::steady_clock::now();
@@ -1858,6 +1939,7 @@ static int run_twochqs(nrg_ctx* c, nrg_twochqs_params p, nrg_step_cb cb, void* u
     };
   }
   ch.init_ops(with_d ? 4 : 3);
+  ch.defer = !with_d;
   // src/TwoChQS/TwoChQS.cpp:208-264
   double Ut = 0.5 * p.U / HLF, edt = p.ed / HLF;
   double G1t = p.Gamma1 * (2.0 / M_PI) / (HLF * HLF), G2t = p.Gamma2 * (2.0 / M_PI) / (HLF * HLF);
@@ -1931,13 +2013,17 @@ static int run_twochqs(nrg_ctx* c, nrg_twochqs_params p, nrg_step_cb cb, void* u
     int nk = 0;
     for (int i = 0; i < ns; i++) { ch.Aeig.Kept[i] = ch.Aeig.dEn[i] <= ec; nk += ch.Aeig.Kept[i]; }
     if (with_d) {
+      int r = ch.finish_vectors(true);
+      if (r) return r;
       dm->emplace_back();
-      int r = dm_capture_all(ch, ch.Aeig.Nshell, 3, dm->back());
+      r = dm_capture_all(ch, ch.Aeig.Nshell, 3, dm->back());
       if (r) return r;
     }
     // the cut the next iteration starts with (by-value CutStates, no SetDegen)
     ch.cut_states(p.ncut, true, 0.0);
-    int r = ch.update_ops();
+    int r = ch.finish_vectors(false);
+    if (r) return r;
+    r = ch.update_ops();
     if (r) return r;
     std::vector<double> none;
     ch.emit(cb, user, emin_, ec, nk, thT, sus, ent, DN / bbar, none);

@@ -7,7 +7,7 @@
 // one symmetric block; all offsets in doubles (ints for i_off)
 struct E2Block {
   int n;
-  int pad;
+  int krow;        // rows of Z the back-transformation forms and the output writes (a prefix)
   int64_t a_off;   // caller's matrix (in: full symmetric, out: eigenvector rows)
   int64_t w_off;   // caller's eigenvalue slot
   int64_t q_off;   // n*n  eigenvectors of T, vector-major
@@ -68,7 +68,8 @@ hipError_t e2_launch_dc_level(const E2Block* b, const E2Merge* merges, int nm, i
 hipError_t e2_launch_bt_panel(const E2Block* b, const int* list, const int* p0, int nlist, int nmax, const double* A,
                               double* WS, const E2Gemm* gprobs, const E2Tile* tiles_g, int nt_g,
                               const E2Tile* tiles_rest, int nt_w, int nt_w2, int nt_z, hipStream_t s);
-hipError_t e2_launch_output(const E2Block* b, int nb, int nmax, const double* WS, double* A, double* W,
+// what: 1 eigenvalues to W, 2 the first krow eigenvector rows to A, 3 both
+hipError_t e2_launch_output(const E2Block* b, int nb, int nmax, const double* WS, double* A, double* W, int what,
                             hipStream_t s);
 hipError_t e2_launch_gemm(const E2Gemm* probs, const E2Tile* tiles, int ntiles, const double* A, const double* B,
                           double* C, hipStream_t s);

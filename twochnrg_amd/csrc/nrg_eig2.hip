@@ -1124,17 +1124,19 @@ __global__ __launch_bounds__(256) void gemm_kernel(const E2Gemm* __restrict__ pr
       }
 }
 
-// copy eigenvalues and eigenvector rows to the caller's buffers
+// copy eigenvalues (what & 1) and the first krow eigenvector rows (what & 2)
+// to the caller's buffers
 __global__ void e2_output_kernel(const E2Block* __restrict__ blks, const double* __restrict__ WS,
-                                 double* __restrict__ A0, double* __restrict__ W0) {
+                                 double* __restrict__ A0, double* __restrict__ W0, int what) {
   const E2Block B = blks[blockIdx.y];
   const int n = B.n;
   const double* Qv = WS + B.q_off;
   double* A = A0 + B.a_off;
-  for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < (size_t)n * n;
-       idx += (size_t)gridDim.x * blockDim.x)
-    A[idx] = Qv[idx];
-  if (blockIdx.x == 0)
+  if (what & 2)
+    for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < (size_t)B.krow * n;
+         idx += (size_t)gridDim.x * blockDim.x)
+      A[idx] = Qv[idx];
+  if ((what & 1) && blockIdx.x == 0)
     for (int i = threadIdx.x; i < n; i += blockDim.x) W0[B.w_off + i] = WS[B.v_off + 5 * (size_t)n + i];
 }
 
@@ -1261,11 +1263,11 @@ hipError_t e2_launch_bt_panel(const E2Block* b, const int* list, const int* p0, 
   return hipGetLastError();
 }
 
-hipError_t e2_launch_output(const E2Block* b, int nb, int nmax, const double* WS, double* A, double* W,
+hipError_t e2_launch_output(const E2Block* b, int nb, int nmax, const double* WS, double* A, double* W, int what,
                             hipStream_t s) {
   if (nb <= 0) return hipSuccess;
-  int gx = (int)(((size_t)nmax * nmax + 255) / 256); if (gx > 1024) gx = 1024; if (gx < 1) gx = 1;
-  hipLaunchKernelGGL(e2::e2_output_kernel, dim3(gx, nb), dim3(256), 0, s, b, WS, A, W);
+  int gx = (what & 2) ? (int)(((size_t)nmax * nmax + 255) / 256) : 1; if (gx > 1024) gx = 1024; if (gx < 1) gx = 1;
+  hipLaunchKernelGGL(e2::e2_output_kernel, dim3(gx, nb), dim3(256), 0, s, b, WS, A, W, what);
   return hipGetLastError();
 }
 

@@ -264,9 +264,50 @@ void nrg_ctx::rename(const std::string& from, const std::string& to) {
 // ---------------------------------------------------------------------------
 // eigensolve over blocks already resident in dA
 // ---------------------------------------------------------------------------
+// algorithmic work of an eigensolve: (10/3) n^3 flops; 8*(n(n+1)/2 + n^2 + n) bytes per block
+static void eig_work(int nb, const int* n, double& fl, double& by) {
+  fl = by = 0.0;
+  for (int b = 0; b < nb; b++) {
+    double x = n[b];
+    fl += 10.0 / 3.0 * x * x * x;
+    by += 8.0 * (x * (x + 1) / 2 + x * x + x);
+  }
+}
+
+int nrg::eig_begin(nrg_ctx* c, int nb, const int* n, const int64_t* a_off, double* dA, const int64_t* w_off,
+                   double* dW) {
+  if (nb <= 0) return NRG_OK;
+  if (c->eig_impl != 2) return eig_dev(c, nb, n, a_off, dA, w_off, dW, nullptr);
+  for (int b = 0; b < nb; b++)
+    if (n[b] < 0) return fail(NRG_ERR_ARG, "negative block size");
+  double fl, by;
+  eig_work(nb, n, fl, by);
+  PendingTimer tm;
+  int rc;
+  if ((rc = c->timer_begin(tm, NRG_K_EIG, fl, by))) return rc;
+  if ((rc = eig2_begin(c, nb, n, a_off, dA, w_off, dW))) return rc;
+  return c->timer_end(tm);
+}
+
+int nrg::eig_finish(nrg_ctx* c, const int* krow) {
+  if (!c->e2p.pending) return NRG_OK;
+  PendingTimer tm;
+  int rc;
+  if ((rc = c->timer_begin(tm, NRG_K_EIG, 0.0, 0.0))) return rc;
+  if ((rc = eig2_finish(c, krow))) return rc;
+  return c->timer_end(tm);
+}
+
 int nrg::eig_dev(nrg_ctx* c, int nb, const int* n, const int64_t* a_off, double* dA, const int64_t* w_off,
                  double* dW, int* info_host) {
   if (nb <= 0) return NRG_OK;
+  if (c->eig_impl == 2) {
+    int rc;
+    if ((rc = eig_begin(c, nb, n, a_off, dA, w_off, dW))) return rc;
+    if ((rc = eig_finish(c, nullptr))) return rc;
+    if (info_host) for (int b = 0; b < nb; b++) info_host[b] = 0;
+    return NRG_OK;
+  }
   std::vector<EigDesc> desc(nb);
   int64_t ws = 0, vs = 0;
   int nmax = 0;
@@ -295,26 +336,10 @@ int nrg::eig_dev(nrg_ctx* c, int nb, const int* n, const int64_t* a_off, double*
   HIPCHK(hipMemcpyAsync(ddesc.p, desc.data(), sizeof(EigDesc) * nb, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(dord.p, order.data(), sizeof(int) * nb, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
-  // algorithmic work: (10/3) n^3 flops; 8*(n(n+1)/2 + n^2 + n) bytes per block
-  double fl = 0.0, by = 0.0;
-  for (int b = 0; b < nb; b++) {
-    double x = n[b];
-    fl += 10.0 / 3.0 * x * x * x;
-    by += 8.0 * (x * (x + 1) / 2 + x * x + x);
-  }
+  double fl, by;
+  eig_work(nb, n, fl, by);
   PendingTimer tm;
   if ((rc = c->timer_begin(tm, NRG_K_EIG, fl, by))) return rc;
-  if (c->eig_impl == 2) {
-    if (c->plain) {
-      for (int b = 0; b < nb; b++)
-        if ((rc = eig2_dev(c, 1, n + b, a_off + b, dA, w_off + b, dW))) return rc;
-    } else if ((rc = eig2_dev(c, nb, n, a_off, dA, w_off, dW))) {
-      return rc;
-    }
-    if ((rc = c->timer_end(tm))) return rc;
-    if (info_host) for (int b = 0; b < nb; b++) info_host[b] = 0;
-    return NRG_OK;
-  }
   HIPCHK(nrgk_launch_eig((EigDesc*)ddesc.p, (int*)dord.p, nb, nmax, dA, dW, (double*)dws.p, (int*)dinfo.p,
                          c->stream));
   if ((rc = c->timer_end(tm))) return rc;

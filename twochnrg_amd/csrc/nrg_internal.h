@@ -61,6 +61,13 @@ struct nrg_ctx {
   void trim(const std::string& keep);
   // move a buffer to another name (replacing what that name held)
   void rename(const std::string& from, const std::string& to);
+  // an eigensolve whose eigenvalues are out and whose vectors wait for the
+  // truncation (eig_begin / eig_finish)
+  struct E2Pending {
+    bool pending = false;
+    std::vector<E2Block> blk;
+    double* dA = nullptr;
+  } e2p;
   int dm_seq = 0;      // DM-NRG handles created on this context (buffer name prefixes)
   std::vector<nrg_dm*> dms;   // live DM-NRG handles: detached when the context goes first
   // small host-to-device copies go through a pinned ring on the stream, with no
@@ -94,6 +101,14 @@ struct RotJob {
 int eig_dev(nrg_ctx* c, int nb, const int* n, const int64_t* a_off, double* dA, const int64_t* w_off, double* dW,
             int* info_host);
 int eig2_dev(nrg_ctx* c, int nb, const int* n, const int64_t* a_off, double* dA, const int64_t* w_off, double* dW);
+int eig2_begin(nrg_ctx* c, int nb, const int* n, const int64_t* a_off, double* dA, const int64_t* w_off, double* dW);
+int eig2_finish(nrg_ctx* c, const int* krow);
+// eig_dev in two halves: eig_begin writes the eigenvalues (ascending per block)
+// to dW; eig_finish writes the first krow[b] eigenvector rows of block b to dA
+// (krow NULL: every row).  Between the two nothing may run another eigensolve
+// on the context.  With the one-workgroup solver (impl 1) eig_begin does it all.
+int eig_begin(nrg_ctx* c, int nb, const int* n, const int64_t* a_off, double* dA, const int64_t* w_off, double* dW);
+int eig_finish(nrg_ctx* c, const int* krow);
 int rotate_dev(nrg_ctx* c, const std::vector<RotJob>& jobs, const double* dZ, const double* dF, double* dAux,
                double* dOut);
 // the descriptors of rotate_dev's two GEMM passes (see nrg_host.cpp)
