@@ -186,9 +186,11 @@ __global__ __launch_bounds__(SM_T) void tridiag_small_kernel(const E2Block* __re
 // vectors W (A_k -= V W_k^T + W V_k^T), and the product y = A v is corrected
 // the same way (y -= V (W^T v) + W (V^T v)).  At the panel end one GEMM applies
 // A -= [V W] [W V]^T to the trailing matrix.  Two launches per column:
-//   symv(k): the reflector of column k (from partial norms), y = A v over 2-D
-//            tiles (column tile x row tile), v^T A v per tile, and the panel
-//            dot products V^T v, W^T v over column chunks;
+//   symv(k): the reflector of column k (from partial norms), y = A v over the
+//            lower-triangle tiles of the (symmetric) trailing matrix, each
+//            off-diagonal tile read once for its column and its row products,
+//            v^T A v per tile, and the panel dot products V^T v, W^T v over
+//            column chunks;
 //   fin(k):  w = tau y + alpha v per column chunk, the reflector stored in
 //            row k, and (inside a panel) column k+1 brought up to date with
 //            its partial norms.
@@ -196,13 +198,8 @@ __global__ __launch_bounds__(SM_T) void tridiag_small_kernel(const E2Block* __re
 // Panel buffer X (3 PANEL rows of n): rows [0,P) V^T, [P,2P) W^T, [2P,3P) V^T,
 // so that rows [0,2P) and [P,3P) are the two operands of the panel GEMM.
 // ---------------------------------------------------------------------------
-#ifndef SYMV_R
-#define SYMV_R 128   // rows per symv tile (at most CHUNK)
-#endif
-#ifndef SYMV_B
-#define SYMV_B 16    // rows whose loads a symv thread issues before their products
-#endif
-#define CHUNK 256    // columns per symv tile / elements per chunk
+#define SYMV_R 128   // rows and columns of a symv tile
+#define CHUNK 256    // threads of a symv tile / elements per chunk
 
 struct TdbPtr {
   double *A, *X, *d, *e, *tau, *refl, *npart, *part, *tpart, *cpart;
@@ -262,18 +259,40 @@ __global__ __launch_bounds__(CHUNK) void tdb_norm_kernel(const E2Block* __restri
   }
 }
 
+// lower-triangle tile x of MT(MT+1)/2 in row order: row tile I >= column tile J
+__device__ __forceinline__ void tri_tile(int x, int& I, int& J) {
+  int i = (int)((sqrt(8.0 * x + 1.0) - 1.0) * 0.5);
+  while ((i + 1) * (i + 2) / 2 <= x) i++;
+  while (i * (i + 1) / 2 > x) i--;
+  I = i;
+  J = x - i * (i + 1) / 2;
+}
+
+// y = A v over the trailing matrix A[k+1:, k+1:], which is symmetric and still
+// as it was at the panel start.  Tiles of SYMV_R x SYMV_R with row tile
+// I >= column tile J; each is read once.  A diagonal tile gives the column
+// products sum_r A[r][c] v_r of its elements (slot I of their partials); an
+// off-diagonal tile also stands for its mirror tile (J, I) and gives the row
+// products sum_c A[r][c] v_c of its row elements (slot J).  Element i of row
+// tile K so receives exactly one partial per slot 0..MT-1: K from the diagonal
+// tile, I > K from tiles below it, J < K from tiles left of it.
+// Threads: wave w, lane (cl = lane % 16, rl = lane / 16) reads rows
+// w*32 + rl + 4s (s < 8) at columns cl + 16g (g < 8): every load instruction
+// covers four rows x 128 contiguous bytes.
 __global__ __launch_bounds__(CHUNK) void tdb_symv_kernel(const E2Block* __restrict__ blks,
                                                          const int* __restrict__ list, int k, double* __restrict__ A0,
                                                          double* __restrict__ WS) {
+  constexpr int T = SYMV_R, LP = T + 4;   // LP: padded LDS row of the partials
   __shared__ double red[CHUNK / WAVE];
   __shared__ double vs[CHUNK];
+  __shared__ double colp[16][T], rowp[16][LP];
   const E2Block B = blks[list[blockIdx.y]];
   const int n = B.n;
   if (k + 2 >= n) return;
   const int m = n - k - 1;
-  const int CT = (m + CHUNK - 1) / CHUNK, RT = (m + SYMV_R - 1) / SYMV_R;
+  const int CT = (m + CHUNK - 1) / CHUNK, MT = (m + T - 1) / T, NT = MT * (MT + 1) / 2;
   const int x = blockIdx.x;
-  if (x >= CT * RT + CT) return;
+  if (x >= NT + CT) return;
   const TdbPtr P = tdb_ptrs(B, A0, WS);
   double t, v0, alpha;
   tdb_reflector(P, k, n, t, v0, alpha);
@@ -285,42 +304,66 @@ __global__ __launch_bounds__(CHUNK) void tdb_symv_kernel(const E2Block* __restri
     P.tau[k] = t;
     P.refl[0] = t; P.refl[1] = v0; P.refl[2] = alpha;
   }
-  if (x < CT * RT) {
-    const int ct = x % CT, rt = x / CT;
-    const int r0 = k + 1 + rt * SYMV_R, c0 = k + 1 + ct * CHUNK;
-    const int nr = min(SYMV_R, n - r0);
-    if (tid < nr) vs[tid] = (r0 + tid == k + 1) ? v0 : rowk[r0 + tid];
-    __syncthreads();
-    const int i = c0 + tid;
-    double s = 0.0;
-    if (i < n) {
-      const double* col = P.A + (size_t)r0 * n + i;
-      double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-      int l = 0;
-      // SYMV_B rows' loads issued before their products: enough reads in flight
-      // per wavefront to cover the HBM latency at this occupancy
-      for (; l + SYMV_B <= nr; l += SYMV_B) {
-        double a[SYMV_B];
-#pragma unroll
-        for (int u = 0; u < SYMV_B; u++) a[u] = col[(size_t)(l + u) * n];
-#pragma unroll
-        for (int u = 0; u < SYMV_B; u += 4) {
-          s0 += a[u] * vs[l + u];
-          s1 += a[u + 1] * vs[l + u + 1];
-          s2 += a[u + 2] * vs[l + u + 2];
-          s3 += a[u + 3] * vs[l + u + 3];
-        }
-      }
-      for (; l < nr; l++) s0 += col[(size_t)l * n] * vs[l];
-      s = (s0 + s1) + (s2 + s3);
-      P.part[(size_t)rt * n + i] = s;
+  if (x < NT) {
+    int I, J;
+    tri_tile(x, I, J);
+    const int r0 = k + 1 + I * T, c0 = k + 1 + J * T;
+    // vs[0,T): v over the tile's rows, vs[T,2T): v over its columns
+    {
+      const int q = (tid < T) ? r0 + tid : c0 + tid - T;
+      vs[tid] = (q < n) ? ((q == k + 1) ? v0 : rowk[q]) : 0.0;
     }
-    const double vi = (i < n) ? ((i == k + 1) ? v0 : rowk[i]) : 0.0;
-    const double tv = block_sum(s * vi, red);
+    __syncthreads();
+    const int lane = tid & (WAVE - 1), w = tid / WAVE;
+    const int cl = lane & 15, rl = lane >> 4;
+    const int rb = w * 32 + rl;
+    double a[8][8];
+#pragma unroll
+    for (int s = 0; s < 8; s++) {
+      const int r = r0 + rb + 4 * s;
+      const double* row = P.A + (size_t)r * n + c0 + cl;
+#pragma unroll
+      for (int g = 0; g < 8; g++) a[s][g] = (r < n && c0 + cl + 16 * g < n) ? row[16 * g] : 0.0;
+    }
+    double cs[8], rs[8];
+#pragma unroll
+    for (int g = 0; g < 8; g++) cs[g] = 0.0;
+#pragma unroll
+    for (int s = 0; s < 8; s++) {
+      const double vr = vs[rb + 4 * s];
+      double acc = 0.0;
+#pragma unroll
+      for (int g = 0; g < 8; g++) {
+        cs[g] += a[s][g] * vr;
+        acc += a[s][g] * vs[T + cl + 16 * g];
+      }
+      rs[s] = acc;
+    }
+#pragma unroll
+    for (int g = 0; g < 8; g++) colp[w * 4 + rl][cl + 16 * g] = cs[g];
+#pragma unroll
+    for (int s = 0; s < 8; s++) rowp[cl][rb + 4 * s] = rs[s];
+    __syncthreads();
+    double tv = 0.0;
+    if (tid < T) {
+      double y = 0.0;
+#pragma unroll
+      for (int q = 0; q < 16; q++) y += colp[q][tid];
+      if (c0 + tid < n) P.part[(size_t)I * n + c0 + tid] = y;
+      tv = y * vs[T + tid];
+    } else if (I != J) {
+      const int e = tid - T;
+      double y = 0.0;
+#pragma unroll
+      for (int q = 0; q < 16; q++) y += rowp[q][e];
+      if (r0 + e < n) P.part[(size_t)J * n + r0 + e] = y;
+      tv = y * vs[e];
+    }
+    tv = block_sum(tv, red);
     if (tid == 0) P.tpart[x] = tv;
   } else {
     // panel dot products over one column chunk: cpart[c][p] = W_p . v, cpart[c][P+p] = V_p . v
-    const int c = x - CT * RT;
+    const int c = x - NT;
     const int j = k % PANEL;
     const int c0 = k + 1 + c * CHUNK;
     const int i = c0 + tid;
@@ -378,7 +421,7 @@ __global__ __launch_bounds__(CHUNK * FG) void tdb_fin_kernel(const E2Block* __re
     cred[1][gg][pp] = a2;
   }
   double tv = 0.0;
-  for (int q = tid; q < CT * RT; q += CHUNK * FG) tv += P.tpart[q];
+  for (int q = tid; q < RT * (RT + 1) / 2; q += CHUNK * FG) tv += P.tpart[q];   // one per lower-triangle tile
   const double vav = block_sum(tv, red);   // also orders the cred stores
   if (tid < j) {
     double a1 = 0.0, a2 = 0.0;
@@ -1165,8 +1208,8 @@ hipError_t e2_launch_tdb_symv(const E2Block* b, const int* list, int nlist, int 
                               hipStream_t s) {
   if (nlist <= 0 || k + 2 >= nmax) return hipSuccess;
   const int m = nmax - k - 1;
-  const int CT = (m + CHUNK - 1) / CHUNK, RT = (m + SYMV_R - 1) / SYMV_R;
-  hipLaunchKernelGGL(e2::tdb_symv_kernel, dim3(CT * RT + CT, nlist), dim3(CHUNK), 0, s, b, list, k, A, WS);
+  const int CT = (m + CHUNK - 1) / CHUNK, MT = (m + SYMV_R - 1) / SYMV_R;
+  hipLaunchKernelGGL(e2::tdb_symv_kernel, dim3(MT * (MT + 1) / 2 + CT, nlist), dim3(CHUNK), 0, s, b, list, k, A, WS);
   return hipGetLastError();
 }
 

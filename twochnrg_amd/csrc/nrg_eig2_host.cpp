@@ -177,10 +177,14 @@ static int e2_phase_a(nrg_ctx* c, const std::vector<E2Block>& blk, double* dA, d
     for (int k = 0; k + 2 < nmax_large; k++) {
       if (k % 8 == 0) {
         // every 8th column is timed (the events cost more than a short launch):
-        // algorithmic bytes = the trailing matrix of every active large block
+        // algorithmic bytes = the lower triangle (diagonal included) of the
+        // symmetric trailing matrix of every active large block
         double by = 0.0;
         for (int b : large)
-          if (k + 2 < blk[b].n) by += 8.0 * (double)(blk[b].n - k - 1) * (double)(blk[b].n - k - 1);
+          if (k + 2 < blk[b].n) {
+            const double mm = (double)(blk[b].n - k - 1);
+            by += 8.0 * mm * (mm + 1.0) * 0.5;
+          }
         PendingTimer tm;
         if ((rc = c->timer_begin(tm, NRG_K_SYMV, 0.0, by))) return rc;
         HIPCHK(e2_launch_tdb_symv(d_blk, d_large, nl, nmax_large, k, dA, WS, s));
