@@ -259,6 +259,17 @@ __global__ __launch_bounds__(CHUNK) void tdb_norm_kernel(const E2Block* __restri
   }
 }
 
+// the entry of G that workgroup x belongs to: the last i with cum[i] <= x
+__device__ __forceinline__ int tdb_find(const E2TdbGrid& G, int x) {
+  int lo = 0, hi = G.nb - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (G.cum[mid] <= x) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
 // lower-triangle tile x of MT(MT+1)/2 in row order: row tile I >= column tile J
 __device__ __forceinline__ void tri_tile(int x, int& I, int& J) {
   int i = (int)((sqrt(8.0 * x + 1.0) - 1.0) * 0.5);
@@ -279,44 +290,31 @@ __device__ __forceinline__ void tri_tile(int x, int& I, int& J) {
 // Threads: wave w, lane (cl = lane % 16, rl = lane / 16) reads rows
 // w*32 + rl + 4s (s < 8) at columns cl + 16g (g < 8): every load instruction
 // covers four rows x 128 contiguous bytes.
-__global__ __launch_bounds__(CHUNK) void tdb_symv_kernel(const E2Block* __restrict__ blks,
-                                                         const int* __restrict__ list, int k, double* __restrict__ A0,
-                                                         double* __restrict__ WS) {
+__global__ __launch_bounds__(CHUNK) void tdb_symv_kernel(const E2Block* __restrict__ blks, const E2TdbGrid G, int k,
+                                                         double* __restrict__ A0, double* __restrict__ WS) {
   constexpr int T = SYMV_R, LP = T + 4;   // LP: padded LDS row of the partials
   __shared__ double red[CHUNK / WAVE];
   __shared__ double vs[CHUNK];
   __shared__ double colp[16][T], rowp[16][LP];
-  const E2Block B = blks[list[blockIdx.y]];
+  const int gi = tdb_find(G, blockIdx.x);
+  const E2Block B = blks[G.blk[gi]];
   const int n = B.n;
   if (k + 2 >= n) return;
   const int m = n - k - 1;
   const int CT = (m + CHUNK - 1) / CHUNK, MT = (m + T - 1) / T, NT = MT * (MT + 1) / 2;
-  const int x = blockIdx.x;
+  const int x = blockIdx.x - G.cum[gi];
   if (x >= NT + CT) return;
   const TdbPtr P = tdb_ptrs(B, A0, WS);
-  double t, v0, alpha;
-  tdb_reflector(P, k, n, t, v0, alpha);
   const double* rowk = P.A + (size_t)k * n;
   const int tid = threadIdx.x;
-  if (x == 0 && tid == 0) {
-    P.d[k] = rowk[k];
-    P.e[k] = alpha;
-    P.tau[k] = t;
-    P.refl[0] = t; P.refl[1] = v0; P.refl[2] = alpha;
-  }
   if (x < NT) {
     int I, J;
     tri_tile(x, I, J);
     const int r0 = k + 1 + I * T, c0 = k + 1 + J * T;
-    // vs[0,T): v over the tile's rows, vs[T,2T): v over its columns
-    {
-      const int q = (tid < T) ? r0 + tid : c0 + tid - T;
-      vs[tid] = (q < n) ? ((q == k + 1) ? v0 : rowk[q]) : 0.0;
-    }
-    __syncthreads();
     const int lane = tid & (WAVE - 1), w = tid / WAVE;
     const int cl = lane & 15, rl = lane >> 4;
     const int rb = w * 32 + rl;
+    // the tile's loads first: they depend on nothing computed in this kernel
     double a[8][8];
 #pragma unroll
     for (int s = 0; s < 8; s++) {
@@ -325,6 +323,23 @@ __global__ __launch_bounds__(CHUNK) void tdb_symv_kernel(const E2Block* __restri
 #pragma unroll
       for (int g = 0; g < 8; g++) a[s][g] = (r < n && c0 + cl + 16 * g < n) ? row[16 * g] : 0.0;
     }
+    // vs[0,T): v over the tile's rows, vs[T,2T): v over its columns; element
+    // k+1 (in column tile 0 only) is v0, which needs the reflector
+    const int q = (tid < T) ? r0 + tid : c0 + tid - T;
+    double vq = (q < n) ? rowk[q] : 0.0;
+    if (J == 0) {
+      double t, v0, alpha;
+      tdb_reflector(P, k, n, t, v0, alpha);
+      if (q == k + 1) vq = v0;
+      if (x == 0 && tid == 0) {
+        P.d[k] = rowk[k];
+        P.e[k] = alpha;
+        P.tau[k] = t;
+        P.refl[0] = t; P.refl[1] = v0; P.refl[2] = alpha;
+      }
+    }
+    vs[tid] = vq;
+    __syncthreads();
     double cs[8], rs[8];
 #pragma unroll
     for (int g = 0; g < 8; g++) cs[g] = 0.0;
@@ -363,6 +378,8 @@ __global__ __launch_bounds__(CHUNK) void tdb_symv_kernel(const E2Block* __restri
     if (tid == 0) P.tpart[x] = tv;
   } else {
     // panel dot products over one column chunk: cpart[c][p] = W_p . v, cpart[c][P+p] = V_p . v
+    double t, v0, alpha;
+    tdb_reflector(P, k, n, t, v0, alpha);
     const int c = x - NT;
     const int j = k % PANEL;
     const int c0 = k + 1 + c * CHUNK;
@@ -386,29 +403,62 @@ __global__ __launch_bounds__(CHUNK) void tdb_symv_kernel(const E2Block* __restri
 }
 
 // FG thread groups share a chunk's elements: group g sums the row-tile
-// partials rt = g (mod FG) and the panel terms p = g (mod FG) of each element,
-// so each thread keeps a quarter of the loads in flight; group 0 adds the four
-// sums in a fixed order and finishes the element.
+// partials rt = g (mod FG) and the panel terms p = g (mod FG) of each element;
+// group 0 adds the four sums in a fixed order and finishes the element.  Every
+// global load that depends on nothing computed here (the partials, the panel
+// rows, the reflector entries, row k+1, the per-tile and per-chunk partial dot
+// products) is issued before the first barrier, so the kernel waits on memory
+// about once.
 #define FG 4
-__global__ __launch_bounds__(CHUNK * FG) void tdb_fin_kernel(const E2Block* __restrict__ blks,
-                                                             const int* __restrict__ list, int k,
-                                                             double* __restrict__ A0, double* __restrict__ WS) {
+#define FIN_PR 12   // row-tile partials per thread loaded at once (all of them while RT <= FG * FIN_PR)
+__global__ __launch_bounds__(CHUNK * FG) void tdb_fin_kernel(const E2Block* __restrict__ blks, const E2TdbGrid G,
+                                                             int k, double* __restrict__ A0, double* __restrict__ WS) {
+  constexpr int NQ = PANEL / FG;   // panel terms per thread
   __shared__ double red[CHUNK * FG / WAVE];
   __shared__ double c1s[PANEL], c2s[PANEL], vk1[PANEL], wk1[PANEL];
   __shared__ double ys[FG][CHUNK], as[FG][CHUNK];
-  const E2Block B = blks[list[blockIdx.y]];
+  __shared__ double cred[2][CHUNK / PANEL][PANEL];
+  const int gi = tdb_find(G, blockIdx.x);
+  const E2Block B = blks[G.blk[gi]];
   const int n = B.n;
   if (k + 2 >= n) return;
   const int m = n - k - 1;
-  const int CT = (m + CHUNK - 1) / CHUNK, RT = (m + SYMV_R - 1) / SYMV_R;
-  const int c = blockIdx.x;
+  const int CT = (m + CHUNK - 1) / CHUNK, RT = (m + SYMV_R - 1) / SYMV_R, NT = RT * (RT + 1) / 2;
+  const int c = blockIdx.x - G.cum[gi];
   if (c >= CT) return;
   const TdbPtr P = tdb_ptrs(B, A0, WS);
   const int tid = threadIdx.x, j = k % PANEL;
   const int e = tid % CHUNK, g = tid / CHUNK;
-  const double t = P.refl[0], v0 = P.refl[1];
-  // panel dot products summed over the column chunks, split 8 ways over group 0
-  __shared__ double cred[2][CHUNK / PANEL][PANEL];
+  const double* rowk = P.A + (size_t)k * n;
+  const int i = k + 1 + c * CHUNK + e;
+  const bool panel_end = (j == PANEL - 1) || (k + 3 >= n);
+  const bool in = i < n;
+  // ---- loads
+  double pr[FIN_PR];
+#pragma unroll
+  for (int u = 0; u < FIN_PR; u++) {
+    const int rt = g + FG * u;
+    pr[u] = (in && rt < RT) ? P.part[(size_t)rt * n + i] : 0.0;
+  }
+  double x1r[NQ], x2r[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; q++) {
+    const int p = g + FG * q;
+    x1r[q] = (in && p < j) ? P.X[(size_t)p * n + i] : 0.0;
+    x2r[q] = (in && p < j) ? P.X[(size_t)(PANEL + p) * n + i] : 0.0;
+  }
+  const double vraw = (in && g == 0) ? rowk[i] : 0.0;
+  const double arow = (in && g == 0 && !panel_end) ? P.A[(size_t)(k + 1) * n + i] : 0.0;
+  // wave 1 also gathers the partials of element k+1 for W(k+1, j), which every chunk needs
+  const bool wk = !panel_end && tid >= WAVE && tid < 2 * WAVE;
+  double yk1 = 0.0;
+  if (wk)
+    for (int rt = tid - WAVE; rt < RT; rt += WAVE) yk1 += P.part[(size_t)rt * n + k + 1];
+  if (!panel_end && tid < j) {
+    vk1[tid] = P.X[(size_t)tid * n + k + 1];
+    wk1[tid] = P.X[(size_t)(PANEL + tid) * n + k + 1];
+  }
+  // panel dot products summed over the column chunks, split 8 ways over the first CHUNK threads
   if (tid < CHUNK) {
     const int pp = tid % PANEL, gg = tid / PANEL;
     double a1 = 0.0, a2 = 0.0;
@@ -421,8 +471,15 @@ __global__ __launch_bounds__(CHUNK * FG) void tdb_fin_kernel(const E2Block* __re
     cred[1][gg][pp] = a2;
   }
   double tv = 0.0;
-  for (int q = tid; q < RT * (RT + 1) / 2; q += CHUNK * FG) tv += P.tpart[q];   // one per lower-triangle tile
-  const double vav = block_sum(tv, red);   // also orders the cred stores
+  for (int q = tid; q < NT; q += CHUNK * FG) tv += P.tpart[q];   // one per lower-triangle tile
+  const double t = P.refl[0], v0 = P.refl[1];
+  // this group's share of y_i = (A v)_i
+  double yp = 0.0;
+#pragma unroll
+  for (int u = 0; u < FIN_PR; u++) yp += pr[u];
+  for (int rt = g + FG * FIN_PR; in && rt < RT; rt += FG) yp += P.part[(size_t)rt * n + i];
+  // ---- reductions
+  const double vav = block_sum(tv, red);   // also orders the cred, vk1, wk1 stores
   if (tid < j) {
     double a1 = 0.0, a2 = 0.0;
     for (int gg = 0; gg < CHUNK / PANEL; gg++) { a1 += cred[0][gg][tid]; a2 += cred[1][gg][tid]; }
@@ -433,75 +490,49 @@ __global__ __launch_bounds__(CHUNK * FG) void tdb_fin_kernel(const E2Block* __re
   double cc12 = 0.0;
   for (int p = 0; p < j; p++) cc12 += c1s[p] * c2s[p];
   const double alpha2 = -0.5 * t * t * (vav - 2.0 * cc12);
-  const double* rowk = P.A + (size_t)k * n;
-  const int i = k + 1 + c * CHUNK + e;
-  const bool panel_end = (j == PANEL - 1) || (k + 3 >= n);
-  if (!panel_end) {
-    // W(k+1, j) is needed by every chunk: the second wavefront recomputes it here
-    if (tid >= WAVE && tid < 2 * WAVE) {
-      const int l = tid - WAVE;
-      double y = 0.0;
-      for (int rt = l; rt < RT; rt += WAVE) y += P.part[(size_t)rt * n + k + 1];
-      if (l < j) y -= P.X[(size_t)l * n + k + 1] * c1s[l] + P.X[(size_t)(PANEL + l) * n + k + 1] * c2s[l];
-      y = wave_sum(y);
-      if (l == 0) {
-        wk1[j] = t * y + alpha2 * v0;
-        vk1[j] = v0;
-      }
+  if (wk) {
+    // W(k+1, j) = t (y_{k+1} - sum_p (V_p c1_p + W_p c2_p)_{k+1}) + alpha2 v0
+    const int l = tid - WAVE;
+    double y = yk1;
+    if (l < j) y -= vk1[l] * c1s[l] + wk1[l] * c2s[l];
+    y = wave_sum(y);
+    if (l == 0) {
+      wk1[j] = t * y + alpha2 * v0;
+      vk1[j] = v0;
     }
-    if (tid < j) {
-      vk1[tid] = P.X[(size_t)tid * n + k + 1];
-      wk1[tid] = P.X[(size_t)(PANEL + tid) * n + k + 1];
-    }
-    __syncthreads();   // wk1, vk1 before the element pass
   }
-  // one pass over this group's share of each element: y_i = (A v)_i -
-  // sum_p (X_p c1_p + X_{P+p} c2_p) and, inside a panel, the correction
-  // -sum_p (X_p wk1_p + X_{P+p} vk1_p) of column k+1 from the same X loads
-  {
-    double yp = 0.0, ap = 0.0;
-    if (i < n) {
-      int rt = g;
-      for (; rt + 3 * FG < RT; rt += 4 * FG) {   // four partials in flight
-        double pa[4];
+  // y_i -= sum_p (X_p c1_p + X_{P+p} c2_p) and, inside a panel, the correction
+  // -sum_p (X_p wk1_p + X_{P+p} vk1_p) of column k+1 from the same X values
+  double ap = 0.0;
 #pragma unroll
-        for (int u = 0; u < 4; u++) pa[u] = P.part[(size_t)(rt + u * FG) * n + i];
-#pragma unroll
-        for (int u = 0; u < 4; u++) yp += pa[u];
-      }
-      for (; rt < RT; rt += FG) yp += P.part[(size_t)rt * n + i];
-      if (panel_end) {
-        for (int p = g; p < j; p += FG) yp -= P.X[(size_t)p * n + i] * c1s[p] + P.X[(size_t)(PANEL + p) * n + i] * c2s[p];
-      } else {
-        for (int p = g; p < j; p += FG) {
-          const double x1 = P.X[(size_t)p * n + i], x2 = P.X[(size_t)(PANEL + p) * n + i];
-          yp -= x1 * c1s[p] + x2 * c2s[p];
-          ap -= x1 * wk1[p] + x2 * vk1[p];
-        }
-      }
+  for (int q = 0; q < NQ; q++) {
+    const int p = g + FG * q;
+    if (p < j) {
+      yp -= x1r[q] * c1s[p] + x2r[q] * c2s[p];
+      if (!panel_end) ap -= x1r[q] * wk1[p] + x2r[q] * vk1[p];
     }
-    ys[g][e] = yp;
-    as[g][e] = ap;
   }
-  __syncthreads();   // the group sums, and every read of the panel rows, precede the stores below
-  double vi = 0.0, wi = 0.0, a = 0.0;
-  if (g == 0 && i < n) {
-    vi = (i == k + 1) ? v0 : rowk[i];
+  ys[g][e] = yp;
+  as[g][e] = ap;
+  __syncthreads();   // the group sums and wk1[j] precede the stores below; every read of the panel rows precedes them
+  double a = 0.0;
+  if (g == 0 && in) {
+    const double vi = (i == k + 1) ? v0 : vraw;
     const double y = (ys[0][e] + ys[1][e]) + (ys[2][e] + ys[3][e]);
-    wi = t * y + alpha2 * vi;
+    const double wi = t * y + alpha2 * vi;
     P.X[(size_t)j * n + i] = vi;
     P.X[(size_t)(2 * PANEL + j) * n + i] = vi;
     P.X[(size_t)(PANEL + j) * n + i] = wi;
     if (i == k + 1) P.A[(size_t)k * n + k + 1] = v0;
     if (!panel_end) {
       // column k+1 brought up to date with the panel's columns 0..j
-      a = P.A[(size_t)(k + 1) * n + i] + ((as[0][e] + as[1][e]) + (as[2][e] + as[3][e]));
+      a = arow + ((as[0][e] + as[1][e]) + (as[2][e] + as[3][e]));
       a -= vi * wk1[j] + wi * vk1[j];
       P.A[(size_t)(k + 1) * n + i] = a;
     }
   }
   if (panel_end) return;
-  const double sq = block_sum((g == 0 && i > k + 1 && i < n) ? a * a : 0.0, red);
+  const double sq = block_sum((g == 0 && i > k + 1 && in) ? a * a : 0.0, red);
   if (tid == 0) P.npart[c] = sq;
 }
 
@@ -1010,43 +1041,48 @@ __global__ void dc_copy_kernel(const E2Block* __restrict__ blks, const E2Merge* 
 }
 
 // ---------------------------------------------------------------------------
-// stage 3: compact-WY panels.  Vt = masked reflector rows, T from the Gram
-// matrix G = Vt Vt^T (T_ii = tau_i, T[0:i,i] = -tau_i T[0:i,0:i] G[0:i,i]).
+// stage 3: compact-WY panels.  Reflector i of a panel is row k0+i of the
+// caller's matrix from column k0+1 on (zero up to and including the diagonal),
+// T from the Gram matrix G = V V^T (T_ii = tau_i, T[0:i,i] = -tau_i T[0:i,0:i]
+// G[0:i,i]), and each round applies Z[:, k0+1:] -= (Z V^T) T^T V.
 // ---------------------------------------------------------------------------
-__global__ void bt_vt_kernel(const E2Block* __restrict__ blks, const int* __restrict__ list, const int* __restrict__ p0,
-                             const double* __restrict__ A0, double* __restrict__ WS) {
-  const E2Block B = blks[list[blockIdx.y]];
-  const int n = B.n, k0 = p0[blockIdx.y];
-  if (k0 < 0) return;
-  const int nb = min(BTP, n - 2 - k0);
-  const double* A = A0 + B.a_off;
-  double* Vt = WS + B.p_off;  // BTP x n
-  for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < (size_t)BTP * n;
-       idx += (size_t)gridDim.x * blockDim.x) {
-    const int a = (int)(idx / n), j = (int)(idx % n);
-    Vt[idx] = (a < nb && j > k0 + a) ? A[(size_t)(k0 + a) * n + j] : 0.0;
+// entries k0+1 .. k0+a of reflector row k0+a: left of or on the diagonal, so
+// zero in the reflector; the reduction left other values there
+__global__ void bt_zero_kernel(const E2Block* __restrict__ blks, const E2BtPanel* __restrict__ pans,
+                               double* __restrict__ A0) {
+  const E2BtPanel Pn = pans[blockIdx.x];
+  const E2Block B = blks[Pn.blk];
+  const int n = B.n;
+  double* A = A0 + B.a_off;
+  for (int idx = threadIdx.x; idx < BTP * BTP; idx += blockDim.x) {
+    const int a = idx / BTP, jj = idx % BTP;
+    if (a < Pn.nb && jj < a) A[(size_t)(Pn.k0 + a) * n + Pn.k0 + 1 + jj] = 0.0;
   }
 }
 
-// T of the compact-WY form, one wavefront per block: T_ii = tau_i,
-// T[0:i,i] = -tau_i T[0:i,0:i] G[0:i,i]; the Gram matrix G is read from global
-// memory (a GEMM output), T is built in LDS and stored transposed
-__global__ __launch_bounds__(4 * BTP) void bt_tmat_kernel(const E2Block* __restrict__ blks, const int* __restrict__ list,
-                                                     const int* __restrict__ p0, double* __restrict__ WS) {
+// T of the compact-WY form, one workgroup per panel: G summed from its ns
+// partial products, T built in LDS (T_ii = tau_i, T[0:i,i] = -tau_i
+// T[0:i,0:i] G[0:i,i]) and stored transposed
+__global__ __launch_bounds__(4 * BTP) void bt_tmat_kernel(const E2Block* __restrict__ blks,
+                                                          const E2BtPanel* __restrict__ pans,
+                                                          const double* __restrict__ WS, double* __restrict__ aux) {
   __shared__ double T[BTP][BTP + 1];
-  __shared__ double Gs[BTP][BTP + 1];   // the Gram matrix, staged once: the recurrence below reads it 32 times
+  __shared__ double Gs[BTP][BTP + 1];   // the Gram matrix, staged once: the recurrence below reads it nb times
   __shared__ double ts[BTP];
-  const E2Block B = blks[list[blockIdx.x]];
-  const int n = B.n, k0 = p0[blockIdx.x];
-  if (k0 < 0) return;
-  const int nb = min(BTP, n - 2 - k0);
+  const E2BtPanel Pn = pans[blockIdx.x];
+  const E2Block B = blks[Pn.blk];
+  const int n = B.n, k0 = Pn.k0, nb = Pn.nb;
   const double* tau = WS + B.v_off + 2 * (size_t)n;
-  const double* G = WS + B.p_off + (size_t)BTP * n;                // BTP x BTP Gram matrix
-  double* Tg = WS + B.p_off + (size_t)BTP * n + BTP * BTP;         // BTP x BTP, T^T row-major
+  const double* G = aux + Pn.g_off;
+  double* Tg = aux + Pn.t_off;   // BTP x BTP, T^T row-major
   const int tid = threadIdx.x;
   for (int i = tid; i < BTP * BTP; i += blockDim.x) {
-    T[i / BTP][i % BTP] = 0.0;
-    Gs[i / BTP][i % BTP] = G[i];
+    const int r = i / BTP, c = i % BTP;
+    double g = 0.0;
+    if (r < nb && c < nb)
+      for (int q = 0; q < Pn.ns; q++) g += G[(size_t)q * BTP * BTP + i];
+    T[r][c] = 0.0;
+    Gs[r][c] = g;
   }
   if (tid < nb) ts[tid] = tau[k0 + tid];
   __syncthreads();
@@ -1068,6 +1104,45 @@ __global__ __launch_bounds__(4 * BTP) void bt_tmat_kernel(const E2Block* __restr
   for (int i = tid; i < BTP * BTP; i += blockDim.x) {
     const int r = i / BTP, c = i % BTP;
     Tg[i] = T[c][r];   // T^T
+  }
+}
+
+// W2 = (sum_s W_s) T^T for one 64-row tile: the partial products of the split
+// K range summed in a fixed order, columns >= nb (no reflector) taken as zero;
+// each thread forms a 4 x 4 block of the tile
+__global__ __launch_bounds__(256) void bt_w2_kernel(const E2BtRows* __restrict__ rows, double* __restrict__ aux) {
+  __shared__ double Ws[BTP][BTP + 1];
+  __shared__ double Ts[BTP][BTP + 1];
+  const E2BtRows R = rows[blockIdx.x];
+  const int tid = threadIdx.x;
+  const double* Tg = aux + R.t_off;
+  for (int i = tid; i < BTP * BTP; i += blockDim.x) {
+    const int r = i / BTP, c = i % BTP;
+    const int row = R.row0 + r;
+    double w = 0.0;
+    if (row < R.krow && c < R.nb)
+      for (int q = 0; q < R.ns; q++) w += aux[R.wp_off + ((size_t)q * R.krow + row) * BTP + c];
+    Ws[r][c] = w;
+    Ts[r][c] = Tg[i];
+  }
+  __syncthreads();
+  const int r0 = (tid / 16) * 4, c0 = (tid % 16) * 4;
+  double acc[4][4] = {};
+  for (int l = 0; l < BTP; l++) {
+    double a[4], b[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) { a[u] = Ws[r0 + u][l]; b[u] = Ts[l][c0 + u]; }
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) acc[u][v] += a[u] * b[v];
+  }
+#pragma unroll
+  for (int u = 0; u < 4; u++) {
+    const int row = R.row0 + r0 + u;
+    if (row < R.krow)
+#pragma unroll
+      for (int v = 0; v < 4; v++) aux[R.w2_off + (size_t)row * BTP + c0 + v] = acc[u][v];
   }
 }
 
@@ -1188,6 +1263,30 @@ __global__ void e2_output_kernel(const E2Block* __restrict__ blks, const double*
 // ---------------------------------------------------------------------------
 // launch wrappers
 // ---------------------------------------------------------------------------
+// the launches of one reduction kernel over the blocks still being reduced at
+// column k, E2_TDB_MAXB blocks per launch; per(m) = workgroups of a block with
+// an m x m trailing matrix
+template <class Per, class Launch>
+static hipError_t tdb_launch_all(const int* blk_idx, const int* blk_n, int nlist, int k, Per per, Launch launch) {
+  E2TdbGrid G;
+  G.nb = 0;
+  G.cum[0] = 0;
+  for (int i = 0; i <= nlist; i++) {
+    if (G.nb == E2_TDB_MAXB || (i == nlist && G.nb > 0)) {
+      launch(G);
+      const hipError_t e = hipGetLastError();
+      if (e != hipSuccess) return e;
+      G.nb = 0;
+    }
+    if (i == nlist) break;
+    if (k + 2 >= blk_n[i]) continue;
+    G.blk[G.nb] = blk_idx[i];
+    G.cum[G.nb + 1] = G.cum[G.nb] + per(blk_n[i] - k - 1);
+    G.nb++;
+  }
+  return hipSuccess;
+}
+
 extern "C" {
 
 hipError_t e2_launch_tridiag_small(const E2Block* b, const int* list, int nlist, double* A, double* WS, hipStream_t s) {
@@ -1204,22 +1303,23 @@ hipError_t e2_launch_tdb_norm(const E2Block* b, const int* list, int nlist, int 
   return hipGetLastError();
 }
 
-hipError_t e2_launch_tdb_symv(const E2Block* b, const int* list, int nlist, int nmax, int k, double* A, double* WS,
-                              hipStream_t s) {
-  if (nlist <= 0 || k + 2 >= nmax) return hipSuccess;
-  const int m = nmax - k - 1;
-  const int CT = (m + CHUNK - 1) / CHUNK, MT = (m + SYMV_R - 1) / SYMV_R;
-  hipLaunchKernelGGL(e2::tdb_symv_kernel, dim3(MT * (MT + 1) / 2 + CT, nlist), dim3(CHUNK), 0, s, b, list, k, A, WS);
-  return hipGetLastError();
+hipError_t e2_launch_tdb_symv(const E2Block* b, const int* blk_idx, const int* blk_n, int nlist, int k, double* A,
+                              double* WS, hipStream_t s) {
+  auto per = [](int m) {
+    const int MT = (m + SYMV_R - 1) / SYMV_R;
+    return MT * (MT + 1) / 2 + (m + CHUNK - 1) / CHUNK;
+  };
+  return tdb_launch_all(blk_idx, blk_n, nlist, k, per, [&](const E2TdbGrid& G) {
+    hipLaunchKernelGGL(e2::tdb_symv_kernel, dim3(G.cum[G.nb]), dim3(CHUNK), 0, s, b, G, k, A, WS);
+  });
 }
 
-hipError_t e2_launch_tdb_fin(const E2Block* b, const int* list, int nlist, int nmax, int k, double* A, double* WS,
-                             hipStream_t s) {
-  if (nlist <= 0 || k + 2 >= nmax) return hipSuccess;
-  const int m = nmax - k - 1;
-  const int CT = (m + CHUNK - 1) / CHUNK;
-  hipLaunchKernelGGL(e2::tdb_fin_kernel, dim3(CT, nlist), dim3(CHUNK * FG), 0, s, b, list, k, A, WS);
-  return hipGetLastError();
+hipError_t e2_launch_tdb_fin(const E2Block* b, const int* blk_idx, const int* blk_n, int nlist, int k, double* A,
+                             double* WS, hipStream_t s) {
+  auto per = [](int m) { return (m + CHUNK - 1) / CHUNK; };
+  return tdb_launch_all(blk_idx, blk_n, nlist, k, per, [&](const E2TdbGrid& G) {
+    hipLaunchKernelGGL(e2::tdb_fin_kernel, dim3(G.cum[G.nb]), dim3(CHUNK * FG), 0, s, b, G, k, A, WS);
+  });
 }
 
 hipError_t e2_launch_gemm3(const E2Gemm* probs, const E2Tile* tiles, int ntiles, const double* A, const double* B,
@@ -1279,30 +1379,22 @@ hipError_t e2_launch_dc_level(const E2Block* b, const E2Merge* merges, int nm, i
   return hipGetLastError();
 }
 
-hipError_t e2_launch_bt_panel(const E2Block* b, const int* list, const int* p0, int nlist, int nmax, const double* A,
-                              double* WS, const E2Gemm* gprobs, const E2Tile* tiles_g, int nt_g,
-                              const E2Tile* tiles_rest, int nt_w, int nt_w2, int nt_z, hipStream_t s) {
-  if (nlist <= 0) return hipSuccess;
-  hipError_t e;
-  int gx = (int)(((size_t)BTP * nmax + 255) / 256); if (gx > 512) gx = 512; if (gx < 1) gx = 1;
-  hipLaunchKernelGGL(e2::bt_vt_kernel, dim3(gx, nlist), dim3(256), 0, s, b, list, p0, A, WS);
-  if ((e = hipGetLastError()) != hipSuccess) return e;
-  if (nt_g > 0) hipLaunchKernelGGL(e2::gemm_kernel, dim3(nt_g), dim3(256), 0, s, gprobs, tiles_g, WS, WS, WS,
-                                   (const E2MState*)nullptr);
-  if ((e = hipGetLastError()) != hipSuccess) return e;
-  hipLaunchKernelGGL(e2::bt_tmat_kernel, dim3(nlist), dim3(4 * BTP), 0, s, b, list, p0, WS);
-  if ((e = hipGetLastError()) != hipSuccess) return e;
-  const E2Tile* t = tiles_rest;
-  if (nt_w > 0) hipLaunchKernelGGL(e2::gemm_kernel, dim3(nt_w), dim3(256), 0, s, gprobs, t, WS, WS, WS,
-                                   (const E2MState*)nullptr);
-  if ((e = hipGetLastError()) != hipSuccess) return e;
-  t += nt_w;
-  if (nt_w2 > 0) hipLaunchKernelGGL(e2::gemm_kernel, dim3(nt_w2), dim3(256), 0, s, gprobs, t, WS, WS, WS,
-                                    (const E2MState*)nullptr);
-  if ((e = hipGetLastError()) != hipSuccess) return e;
-  t += nt_w2;
-  if (nt_z > 0) hipLaunchKernelGGL(e2::gemm_kernel, dim3(nt_z), dim3(256), 0, s, gprobs, t, WS, WS, WS,
-                                   (const E2MState*)nullptr);
+hipError_t e2_launch_bt_zero(const E2Block* b, const E2BtPanel* pans, int npan, double* A, hipStream_t s) {
+  if (npan <= 0) return hipSuccess;
+  hipLaunchKernelGGL(e2::bt_zero_kernel, dim3(npan), dim3(256), 0, s, b, pans, A);
+  return hipGetLastError();
+}
+
+hipError_t e2_launch_bt_tmat(const E2Block* b, const E2BtPanel* pans, int npan, const double* WS, double* aux,
+                             hipStream_t s) {
+  if (npan <= 0) return hipSuccess;
+  hipLaunchKernelGGL(e2::bt_tmat_kernel, dim3(npan), dim3(4 * BTP), 0, s, b, pans, WS, aux);
+  return hipGetLastError();
+}
+
+hipError_t e2_launch_bt_w2(const E2BtRows* rows, int nrows, double* aux, hipStream_t s) {
+  if (nrows <= 0) return hipSuccess;
+  hipLaunchKernelGGL(e2::bt_w2_kernel, dim3(nrows), dim3(256), 0, s, rows, aux);
   return hipGetLastError();
 }
 

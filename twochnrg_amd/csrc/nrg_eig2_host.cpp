@@ -18,6 +18,10 @@ static const int kLeaf = 16;   // = LEAF in nrg_eig2.hip
 static const int kPanel = 32;  // = PANEL in nrg_eig2.hip
 static const int kBtp = 64;    // = BTP in nrg_eig2.hip
 static const int kSmall = 256; // single-workgroup reduction up to this size
+static const int kBtKs = 256;  // K chunk of the back-transformation products Z V^T
+static const int kBtKsG = 512; // K chunk of the Gram products V V^T
+
+static int64_t e2_panel_len(int64_t nn) { return 3 * (int64_t)kPanel * nn; }
 
 // The workspace layout of every block of one eigensolve (offsets into e2_ws / e2_iw)
 static void e2_layout(int nb, const int* n, const int64_t* a_off, const int64_t* w_off, std::vector<E2Block>& blk,
@@ -37,9 +41,8 @@ static void e2_layout(int nb, const int* n, const int64_t* a_off, const int64_t*
     B.u_off = ws; ws += n2;
     B.dl_off = ws; ws += n2;
     B.t_off = ws; ws += n2;
-    // panel scratch: back-transformation Vt, G, T^T, W, W2 (the reduction's
-    // 3 x 32 panel rows fit inside)
-    B.p_off = ws; ws += kBtp * nn + 2 * kBtp * kBtp + 2 * nn * kBtp;
+    // panel scratch: the reduction's 3 x 32 panel rows
+    B.p_off = ws; ws += e2_panel_len(nn);
     B.v_off = ws; ws += 20 * nn + 64;
     B.i_off = iw; iw += 8 * nn + 8;
   }
@@ -172,6 +175,8 @@ static int e2_phase_a(nrg_ctx* c, const std::vector<E2Block>& blk, double* dA, d
   HIPCHK(e2_launch_tridiag_small(d_blk, d_small, (int)small.size(), dA, WS, s));
   if (!large.empty()) {
     const int nl = (int)large.size();
+    std::vector<int> large_n(nl);
+    for (int i = 0; i < nl; i++) large_n[i] = blk[large[i]].n;
     HIPCHK(e2_launch_tdb_norm(d_blk, d_large, nl, nmax_large, 0, dA, WS, s));
     size_t ev = 0;
     for (int k = 0; k + 2 < nmax_large; k++) {
@@ -187,12 +192,12 @@ static int e2_phase_a(nrg_ctx* c, const std::vector<E2Block>& blk, double* dA, d
           }
         PendingTimer tm;
         if ((rc = c->timer_begin(tm, NRG_K_SYMV, 0.0, by))) return rc;
-        HIPCHK(e2_launch_tdb_symv(d_blk, d_large, nl, nmax_large, k, dA, WS, s));
+        HIPCHK(e2_launch_tdb_symv(d_blk, large.data(), large_n.data(), nl, k, dA, WS, s));
         if ((rc = c->timer_end(tm))) return rc;
       } else {
-        HIPCHK(e2_launch_tdb_symv(d_blk, d_large, nl, nmax_large, k, dA, WS, s));
+        HIPCHK(e2_launch_tdb_symv(d_blk, large.data(), large_n.data(), nl, k, dA, WS, s));
       }
-      HIPCHK(e2_launch_tdb_fin(d_blk, d_large, nl, nmax_large, k, dA, WS, s));
+      HIPCHK(e2_launch_tdb_fin(d_blk, large.data(), large_n.data(), nl, k, dA, WS, s));
       if (ev < tev_k.size() && tev_k[ev] == k) {
         HIPCHK(e2_launch_gemm3(d_tp + tev_pbeg[ev], d_tt + tev_tbeg[ev], tev_tcnt[ev], WS, WS, dA, s));
         ev++;
@@ -222,96 +227,139 @@ static int e2_phase_b(nrg_ctx* c, const std::vector<E2Block>& blk, double* dA, d
   if (act.empty()) return NRG_OK;
   // round r applies panel P_b-1-r of every block that has it; the reflectors of
   // a panel starting at k0 vanish in columns <= k0, so every product runs over
-  // the L = n-k0-1 columns after it; only the krow rows are formed
+  // the L = n-k0-1 columns after it; only the krow rows are formed.  The K
+  // range L of the products Z V^T and V V^T is split into chunks (partial
+  // products summed afterwards in a fixed order), so that a round is not
+  // carried by the few workgroups of its 64-column output.
   const int nb = (int)blk.size();
-  std::vector<int> npan(nb, 0);
+  std::vector<int> npan(nb, 0), pan_beg(nb, 0);
   int maxP = 0;
   for (int b : act) {
     npan[b] = blk[b].n > 2 ? (blk[b].n - 2 + kBtp - 1) / kBtp : 0;
     maxP = std::max(maxP, npan[b]);
   }
-  std::vector<int> bt_list, bt_p0, r_lbeg, r_lcnt, r_pbeg, r_tbeg, r_ng, r_nw, r_nw2, r_nz;
-  std::vector<E2Gemm> bprobs;
-  std::vector<E2Tile> btiles;
-  int nmax_bt = 0;
-  for (int r = 0; r < maxP; r++) {
-    r_lbeg.push_back((int)bt_list.size());
-    r_pbeg.push_back((int)bprobs.size());
-    r_tbeg.push_back((int)btiles.size());
-    std::vector<int> bl;
-    for (int b : act)
-      if (npan[b] > r) bl.push_back(b);
-    const int pbase = (int)bprobs.size();
-    std::vector<E2Tile> tg, tw, tw2, tz;
-    for (int b : bl) {
-      const E2Block& B = blk[b];
-      const int64_t nn = B.n;
-      const int m = B.krow;
-      const int k0 = (npan[b] - 1 - r) * kBtp;
-      const int L = B.n - k0 - 1;
-      bt_list.push_back(b);
-      bt_p0.push_back(k0);
-      nmax_bt = std::max(nmax_bt, B.n);
-      const int64_t vt = B.p_off, G = vt + kBtp * nn, Tg = G + kBtp * kBtp, W = Tg + kBtp * kBtp,
-                    W2 = W + nn * kBtp;
+  // the auxiliary buffer: partial Gram matrices and T^T per panel, then the
+  // partial products W_s and W2 per block
+  std::vector<E2BtPanel> pans;
+  int64_t aux = 0;
+  for (int b : act) {
+    pan_beg[b] = (int)pans.size();
+    for (int p = 0; p < npan[b]; p++) {
+      E2BtPanel P{};
+      P.blk = b;
+      P.k0 = p * kBtp;
+      P.nb = std::min(kBtp, blk[b].n - 2 - P.k0);
+      P.ns = (blk[b].n - P.k0 - 1 + kBtKsG - 1) / kBtKsG;
+      P.g_off = aux;
+      aux += (int64_t)P.ns * kBtp * kBtp;
+      pans.push_back(P);
+    }
+  }
+  for (E2BtPanel& P : pans) { P.t_off = aux; aux += kBtp * kBtp; }
+  std::vector<int64_t> wp_off(nb, 0), w2_off(nb, 0);
+  for (int b : act) {
+    wp_off[b] = aux;
+    aux += (int64_t)blk[b].krow * kBtp * ((blk[b].n - 1 + kBtKs - 1) / kBtKs);
+    w2_off[b] = aux;
+    aux += (int64_t)blk[b].krow * kBtp;
+  }
+  // Gram products of every panel: G_s = V[:, chunk s] V[:, chunk s]^T
+  std::vector<E2Gemm> gprobs;
+  std::vector<E2Tile> gtiles;
+  for (const E2BtPanel& P : pans) {
+    const E2Block& B = blk[P.blk];
+    const int64_t nn = B.n;
+    const int L = B.n - P.k0 - 1;
+    for (int q = 0; q < P.ns; q++) {
       E2Gemm g{};
       g.dyn = -1; g.alpha = 1.0;
-      // Gram G = Vt Vt^T over columns k0+1..n-1
-      E2Gemm gg = g;
-      gg.m = kBtp; gg.n = kBtp; gg.k = L;
-      gg.a_off = vt + k0 + 1; gg.as0 = nn; gg.as1 = 1; gg.b_off = vt + k0 + 1; gg.bs0 = 1; gg.bs1 = nn;
-      gg.c_off = G; gg.cs0 = kBtp;
-      int pg = (int)bprobs.size() - pbase;
-      bprobs.push_back(gg);
-      tg.push_back(E2Tile{pg, 0, 0, 0});
-      // W = Qv[:m, k0+1:] Vt[:, k0+1:]^T
-      E2Gemm gw = g;
-      gw.m = m; gw.n = kBtp; gw.k = L;
-      gw.a_off = B.q_off + k0 + 1; gw.as0 = nn; gw.as1 = 1; gw.b_off = vt + k0 + 1; gw.bs0 = 1; gw.bs1 = nn;
-      gw.c_off = W; gw.cs0 = kBtp;
-      int pw = (int)bprobs.size() - pbase;
-      bprobs.push_back(gw);
-      for (int a = 0; a < (m + 63) / 64; a++) tw.push_back(E2Tile{pw, a, 0, 0});
-      // W2 = W T^T
-      E2Gemm g2 = g;
-      g2.m = m; g2.n = kBtp; g2.k = kBtp;
-      g2.a_off = W; g2.as0 = kBtp; g2.as1 = 1; g2.b_off = Tg; g2.bs0 = kBtp; g2.bs1 = 1; g2.c_off = W2;
-      g2.cs0 = kBtp;
-      int p2 = (int)bprobs.size() - pbase;
-      bprobs.push_back(g2);
-      for (int a = 0; a < (m + 63) / 64; a++) tw2.push_back(E2Tile{p2, a, 0, 0});
-      // Qv[:m, k0+1:] -= W2 Vt[:, k0+1:]
-      E2Gemm gz = g;
-      gz.m = m; gz.n = L; gz.k = kBtp;
-      gz.a_off = W2; gz.as0 = kBtp; gz.as1 = 1; gz.b_off = vt + k0 + 1; gz.bs0 = nn; gz.bs1 = 1;
-      gz.c_off = B.q_off + k0 + 1; gz.cs0 = nn; gz.alpha = -1.0; gz.accumulate = 1;
-      int pz = (int)bprobs.size() - pbase;
-      bprobs.push_back(gz);
+      g.m = P.nb; g.n = P.nb; g.k = std::min(kBtKsG, L - q * kBtKsG);
+      g.a_off = B.a_off + (int64_t)P.k0 * nn + P.k0 + 1 + (int64_t)q * kBtKsG; g.as0 = nn; g.as1 = 1;
+      g.b_off = g.a_off; g.bs0 = 1; g.bs1 = nn;
+      g.c_off = P.g_off + (int64_t)q * kBtp * kBtp; g.cs0 = kBtp;
+      gtiles.push_back(E2Tile{(int)gprobs.size(), 0, 0, 0});
+      gprobs.push_back(g);
+    }
+  }
+  // the rounds: W_s = Z[:, chunk s] V[:, chunk s]^T, W2 = (sum W_s) T^T,
+  // Z[:, k0+1:] -= W2 V
+  std::vector<E2Gemm> rprobs;
+  std::vector<E2Tile> rtiles;
+  std::vector<E2BtRows> rrows;
+  std::vector<int> r_pbeg, r_wt, r_zt, r_tbeg, r_rbeg, r_nr;
+  for (int r = 0; r < maxP; r++) {
+    r_pbeg.push_back((int)rprobs.size());
+    r_tbeg.push_back((int)rtiles.size());
+    r_rbeg.push_back((int)rrows.size());
+    const int pbase = (int)rprobs.size();
+    std::vector<E2Tile> tw, tz;
+    for (int b : act) {
+      if (npan[b] <= r) continue;
+      const E2Block& B = blk[b];
+      const E2BtPanel& P = pans[pan_beg[b] + npan[b] - 1 - r];
+      const int64_t nn = B.n;
+      const int m = B.krow, L = B.n - P.k0 - 1, ns = (L + kBtKs - 1) / kBtKs;
+      const int64_t vrow = B.a_off + (int64_t)P.k0 * nn + P.k0 + 1;   // V(i, j) = A[vrow + i*n + j]
+      for (int q = 0; q < ns; q++) {
+        E2Gemm g{};
+        g.dyn = -1; g.alpha = 1.0;
+        g.m = m; g.n = P.nb; g.k = std::min(kBtKs, L - q * kBtKs);
+        g.a_off = B.q_off + P.k0 + 1 + (int64_t)q * kBtKs; g.as0 = nn; g.as1 = 1;
+        g.b_off = vrow + (int64_t)q * kBtKs; g.bs0 = 1; g.bs1 = nn;
+        g.c_off = wp_off[b] + (int64_t)q * m * kBtp; g.cs0 = kBtp;
+        const int pi = (int)rprobs.size() - pbase;
+        rprobs.push_back(g);
+        for (int a = 0; a < (m + 63) / 64; a++) tw.push_back(E2Tile{pi, a, 0, 0});
+      }
+      for (int a = 0; a < (m + 63) / 64; a++) {
+        E2BtRows R{};
+        R.row0 = a * 64; R.krow = m; R.nb = P.nb; R.ns = ns;
+        R.wp_off = wp_off[b]; R.t_off = P.t_off; R.w2_off = w2_off[b];
+        rrows.push_back(R);
+      }
+      E2Gemm gz{};
+      gz.dyn = -1; gz.alpha = -1.0; gz.accumulate = 1;
+      gz.m = m; gz.n = L; gz.k = P.nb;
+      gz.a_off = w2_off[b]; gz.as0 = kBtp; gz.as1 = 1;
+      gz.b_off = vrow; gz.bs0 = nn; gz.bs1 = 1;
+      gz.c_off = B.q_off + P.k0 + 1; gz.cs0 = nn;
+      const int pz = (int)rprobs.size() - pbase;
+      rprobs.push_back(gz);
       for (int a = 0; a < (m + 63) / 64; a++)
         for (int bb = 0; bb < (L + 63) / 64; bb++) tz.push_back(E2Tile{pz, a, bb, 0});
     }
-    r_lcnt.push_back((int)bl.size());
-    r_ng.push_back((int)tg.size()); r_nw.push_back((int)tw.size());
-    r_nw2.push_back((int)tw2.size()); r_nz.push_back((int)tz.size());
-    btiles.insert(btiles.end(), tg.begin(), tg.end());
-    btiles.insert(btiles.end(), tw.begin(), tw.end());
-    btiles.insert(btiles.end(), tw2.begin(), tw2.end());
-    btiles.insert(btiles.end(), tz.begin(), tz.end());
+    r_wt.push_back((int)tw.size());
+    r_zt.push_back((int)tz.size());
+    r_nr.push_back((int)rrows.size() - r_rbeg.back());
+    rtiles.insert(rtiles.end(), tw.begin(), tw.end());
+    rtiles.insert(rtiles.end(), tz.begin(), tz.end());
   }
   E2Block* d_blk;
-  int *d_btl, *d_btp;
-  E2Gemm* d_bp;
-  E2Tile* d_bt;
+  E2BtPanel* d_pan;
+  E2Gemm *d_gp, *d_rp;
+  E2Tile *d_gt, *d_rt;
+  E2BtRows* d_rr;
   if ((rc = upload(c, "e2b_blk", blk, &d_blk))) return rc;
-  if ((rc = upload(c, "e2_btl", bt_list, &d_btl))) return rc;
-  if ((rc = upload(c, "e2_btp", bt_p0, &d_btp))) return rc;
-  if ((rc = upload(c, "e2_bp", bprobs, &d_bp))) return rc;
-  if ((rc = upload(c, "e2_bt", btiles, &d_bt))) return rc;
+  if ((rc = upload(c, "e2b_pan", pans, &d_pan))) return rc;
+  if ((rc = upload(c, "e2b_gp", gprobs, &d_gp))) return rc;
+  if ((rc = upload(c, "e2b_gt", gtiles, &d_gt))) return rc;
+  if ((rc = upload(c, "e2b_rp", rprobs, &d_rp))) return rc;
+  if ((rc = upload(c, "e2b_rt", rtiles, &d_rt))) return rc;
+  if ((rc = upload(c, "e2b_rr", rrows, &d_rr))) return rc;
+  DBuf& daux = c->buf("e2b_aux");
+  if ((rc = daux.reserve(sizeof(double) * (size_t)(aux + 1)))) return rc;
+  double* AX = (double*)daux.p;
   hipStream_t s = c->stream;
+  const int np = (int)pans.size();
+  HIPCHK(e2_launch_bt_zero(d_blk, d_pan, np, dA, s));
+  HIPCHK(e2_launch_gemm3(d_gp, d_gt, (int)gtiles.size(), dA, dA, AX, s));
+  HIPCHK(e2_launch_bt_tmat(d_blk, d_pan, np, WS, AX, s));
   for (int r = 0; r < maxP; r++) {
-    const E2Tile* t0 = d_bt + r_tbeg[r];
-    HIPCHK(e2_launch_bt_panel(d_blk, d_btl + r_lbeg[r], d_btp + r_lbeg[r], r_lcnt[r], nmax_bt, dA, WS,
-                              d_bp + r_pbeg[r], t0, r_ng[r], t0 + r_ng[r], r_nw[r], r_nw2[r], r_nz[r], s));
+    const E2Gemm* pr = d_rp + r_pbeg[r];
+    const E2Tile* t0 = d_rt + r_tbeg[r];
+    HIPCHK(e2_launch_gemm3(pr, t0, r_wt[r], WS, dA, AX, s));
+    HIPCHK(e2_launch_bt_w2(d_rr + r_rbeg[r], r_nr[r], AX, s));
+    HIPCHK(e2_launch_gemm3(pr, t0 + r_wt[r], r_zt[r], AX, dA, WS, s));
   }
   HIPCHK(e2_launch_output(d_blk, nb, nmax, WS, dA, nullptr, 2, s));
   return NRG_OK;
@@ -389,7 +437,7 @@ extern "C" int nrg_eig2_stages(nrg_ctx* c, int n, const double* A, double* d_out
   // the single block's workspace starts at offset 0: q,k,u,dl,t (5 n^2), panel, vectors
   const int64_t nn = n, q_off = 0;
   const int64_t p_off = 5 * nn * nn;
-  const int64_t v_off = p_off + kBtp * nn + 2 * kBtp * kBtp + 2 * nn * kBtp;
+  const int64_t v_off = p_off + e2_panel_len(nn);
   double* WS = (double*)c->buf("e2_ws").p;
   if (d_out) HIPCHK(hipMemcpy(d_out, WS + v_off, sizeof(double) * n, hipMemcpyDeviceToHost));
   if (e_out) HIPCHK(hipMemcpy(e_out, WS + v_off + 16 * nn, sizeof(double) * n, hipMemcpyDeviceToHost));
