@@ -694,7 +694,6 @@ __global__ __launch_bounds__(E2_T) void dc_prep_kernel(const E2Block* __restrict
   __shared__ double red[E2_T / WAVE];
   __shared__ double cds[E2_T], czs[E2_T];
   __shared__ int cpm[E2_T];
-  __shared__ int sh_ngiv;
   const E2Merge M = merges[blockIdx.x];
   const E2Block B = blks[M.blk];
   const int n = B.n, lo = M.lo, mid = M.mid, hi = M.hi, m = hi - lo, n1 = mid - lo, tid = threadIdx.x;
@@ -781,21 +780,60 @@ __global__ __launch_bounds__(E2_T) void dc_prep_kernel(const E2Block* __restrict
     E2MState st;
     st.k = k; st.nd = nd; st.ngiv = ng; st.pad = 0; st.rho = rho; st.tol = tol;
     ms[blockIdx.x] = st;
-    sh_ngiv = ng;
   }
-  __syncthreads();
-  // Givens rotations of the deflated pairs, in order: q_a' = c q_a + s q_b, q_b' = -s q_a + c q_b
-  const int ngv = sh_ngiv;
-  for (int g = 0; g < ngv; g++) {
-    double* ra = Qv + (size_t)(lo + ga[g]) * n + lo;
-    double* rb = Qv + (size_t)(lo + gb[g]) * n + lo;
-    const double c = gc[g], s = gs[g];
-    for (int i = tid; i < m; i += blockDim.x) {
-      const double xa = ra[i], xb = rb[i];
-      ra[i] = c * xa + s * xb;
-      rb[i] = -s * xa + c * xb;
+}
+
+// Givens rotations of the deflated pairs of each merge, in the order the
+// deflation scan found them: q_a' = c q_a + s q_b, q_b' = -s q_a + c q_b on
+// rows a, b of the eigenvector matrix over the merge range.  Every thread owns
+// one column and applies the whole sequence to it, GV rotations at a time with
+// their loads issued together.  The scan makes row b of a rotation one not
+// rotated before, and row a either such a row or the b row of the rotation
+// just before (a chain, carried in a register); an a row is never rotated
+// again.  So the values loaded ahead are current.
+#define GV 16
+__global__ __launch_bounds__(E2_T) void dc_givens_kernel(const E2Block* __restrict__ blks,
+                                                         const E2Merge* __restrict__ merges,
+                                                         const E2MState* __restrict__ ms, double* __restrict__ WS,
+                                                         int* __restrict__ IW) {
+  const E2Merge M = merges[blockIdx.y];
+  const E2Block B = blks[M.blk];
+  const int n = B.n, lo = M.lo, m = M.hi - M.lo;
+  const int ng = ms[blockIdx.y].ngiv;
+  if (ng == 0 || (int)blockIdx.x * E2_T >= m) return;
+  const int i = blockIdx.x * E2_T + threadIdx.x;
+  const bool act = i < m;
+  double* col = WS + B.q_off + lo + (act ? i : 0);   // element (row r, column lo+i) at col[(lo+r)*n]
+  const int* ga = VI(3) + lo;
+  const int* gb = VI(4) + lo;
+  const double* gc = VD(14) + lo;
+  const double* gs = VD(15) + lo;
+  for (int g0 = 0; g0 < ng; g0 += GV) {
+    const int cnt = min(GV, ng - g0);
+    int ra[GV], rb[GV];
+    double xa[GV], xb[GV];
+#pragma unroll
+    for (int u = 0; u < GV; u++) {
+      ra[u] = (u < cnt) ? ga[g0 + u] : 0;
+      rb[u] = (u < cnt) ? gb[g0 + u] : 0;
+      xa[u] = (act && u < cnt) ? col[(size_t)(lo + ra[u]) * n] : 0.0;
+      xb[u] = (act && u < cnt) ? col[(size_t)(lo + rb[u]) * n] : 0.0;
     }
-    __syncthreads();
+    int carry = -1;
+    double xc = 0.0;
+#pragma unroll
+    for (int u = 0; u < GV; u++) {
+      if (u < cnt) {
+        const double c = gc[g0 + u], sn = gs[g0 + u];
+        double a = xa[u];
+        if (ra[u] == carry) a = xc;
+        else if (carry >= 0 && act) col[(size_t)(lo + carry) * n] = xc;
+        if (act) col[(size_t)(lo + ra[u]) * n] = c * a + sn * xb[u];
+        xc = -sn * a + c * xb[u];
+        carry = rb[u];
+      }
+    }
+    if (carry >= 0 && act) col[(size_t)(lo + carry) * n] = xc;
   }
 }
 
@@ -1113,17 +1151,43 @@ __global__ __launch_bounds__(4 * BTP) void bt_tmat_kernel(const E2Block* __restr
 __global__ __launch_bounds__(256) void bt_w2_kernel(const E2BtRows* __restrict__ rows, double* __restrict__ aux) {
   __shared__ double Ws[BTP][BTP + 1];
   __shared__ double Ts[BTP][BTP + 1];
+  constexpr int EPT = BTP * BTP / 256;   // elements of the tile per thread
   const E2BtRows R = rows[blockIdx.x];
   const int tid = threadIdx.x;
   const double* Tg = aux + R.t_off;
-  for (int i = tid; i < BTP * BTP; i += blockDim.x) {
-    const int r = i / BTP, c = i % BTP;
-    const int row = R.row0 + r;
-    double w = 0.0;
-    if (row < R.krow && c < R.nb)
-      for (int q = 0; q < R.ns; q++) w += aux[R.wp_off + ((size_t)q * R.krow + row) * BTP + c];
-    Ws[r][c] = w;
-    Ts[r][c] = Tg[i];
+  // the partials summed in order q = 0, 1, ...; the loads of two partials of
+  // all this thread's elements are issued together
+  double w[EPT];
+  bool ok[EPT];
+  size_t off[EPT];
+#pragma unroll
+  for (int u = 0; u < EPT; u++) {
+    const int i = tid + 256 * u, r = i / BTP, c = i % BTP, row = R.row0 + r;
+    w[u] = 0.0;
+    ok[u] = row < R.krow && c < R.nb;
+    off[u] = R.wp_off + (size_t)row * BTP + c;
+  }
+  const size_t qstride = (size_t)R.krow * BTP;
+  int q = 0;
+  for (; q + 2 <= R.ns; q += 2) {
+    double x0[EPT], x1[EPT];
+#pragma unroll
+    for (int u = 0; u < EPT; u++) {
+      x0[u] = ok[u] ? aux[off[u] + q * qstride] : 0.0;
+      x1[u] = ok[u] ? aux[off[u] + (q + 1) * qstride] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < EPT; u++) w[u] = (w[u] + x0[u]) + x1[u];
+  }
+  if (q < R.ns) {
+#pragma unroll
+    for (int u = 0; u < EPT; u++) w[u] += ok[u] ? aux[off[u] + q * qstride] : 0.0;
+  }
+#pragma unroll
+  for (int u = 0; u < EPT; u++) {
+    const int i = tid + 256 * u;
+    Ws[i / BTP][i % BTP] = w[u];
+    Ts[i / BTP][i % BTP] = Tg[i];
   }
   __syncthreads();
   const int r0 = (tid / 16) * 4, c0 = (tid % 16) * 4;
@@ -1360,6 +1424,8 @@ hipError_t e2_launch_dc_level(const E2Block* b, const E2Merge* merges, int nm, i
   if ((e = hipGetLastError()) != hipSuccess) return e;
   const int gw = (mmax + E2_T / WAVE - 1) / (E2_T / WAVE);   // one wavefront per root / vector
   const int gy = (mmax + E2_T - 1) / E2_T;                   // one thread per component
+  hipLaunchKernelGGL(e2::dc_givens_kernel, dim3(gy, nm), dim3(E2_T), 0, s, b, merges, ms, WS, IW);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
   hipLaunchKernelGGL(e2::dc_secular_kernel, dim3(gw, nm), dim3(E2_T), 0, s, b, merges, ms, WS, IW);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   hipLaunchKernelGGL(e2::dc_zhat_kernel, dim3(gy, nm), dim3(E2_T), 0, s, b, merges, ms, WS);
