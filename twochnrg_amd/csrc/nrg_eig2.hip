@@ -402,6 +402,34 @@ __global__ __launch_bounds__(CHUNK) void tdb_symv_kernel(const E2Block* __restri
   }
 }
 
+// At a panel start s0 (after the panel GEMM, which brings only the lower
+// triangle and a band above it up to date), rows s0..s0+PANEL-1 are rebuilt
+// from their columns: A[r][j] = A[j][r] for j > r.  Those rows are the
+// reflector sources of the panel and must be current over their whole length.
+// One workgroup per 64 columns j of a block: the 64 x PANEL strip A[j][s0..]
+// staged through LDS and stored as PANEL rows of 64 contiguous entries.
+__global__ __launch_bounds__(256) void tdb_refresh_kernel(const E2Block* __restrict__ blks, const E2TdbGrid G,
+                                                          int s0, double* __restrict__ A0) {
+  __shared__ double t[64][PANEL + 1];
+  const int gi = tdb_find(G, blockIdx.x);
+  const E2Block B = blks[G.blk[gi]];
+  const int n = B.n;
+  double* A = A0 + B.a_off;
+  const int j0 = s0 + 64 * (blockIdx.x - G.cum[gi]);
+  const int tid = threadIdx.x;
+  for (int idx = tid; idx < 64 * PANEL; idx += 256) {
+    const int jj = idx / PANEL, r = idx % PANEL;
+    const int j = j0 + jj;
+    t[jj][r] = (j < n && s0 + r < n) ? A[(size_t)j * n + s0 + r] : 0.0;
+  }
+  __syncthreads();
+  for (int idx = tid; idx < 64 * PANEL; idx += 256) {
+    const int r = idx / 64, jj = idx % 64;
+    const int j = j0 + jj, row = s0 + r;
+    if (row < n && j < n && j > row) A[(size_t)row * n + j] = t[jj][r];
+  }
+}
+
 // FG thread groups share a chunk's elements: group g sums the row-tile
 // partials rt = g (mod FG) and the panel terms p = g (mod FG) of each element;
 // group 0 adds the four sums in a fixed order and finishes the element.  Every
@@ -1375,6 +1403,15 @@ hipError_t e2_launch_tdb_symv(const E2Block* b, const int* blk_idx, const int* b
   };
   return tdb_launch_all(blk_idx, blk_n, nlist, k, per, [&](const E2TdbGrid& G) {
     hipLaunchKernelGGL(e2::tdb_symv_kernel, dim3(G.cum[G.nb]), dim3(CHUNK), 0, s, b, G, k, A, WS);
+  });
+}
+
+hipError_t e2_launch_tdb_refresh(const E2Block* b, const int* blk_idx, const int* blk_n, int nlist, int s0, double* A,
+                                 hipStream_t s) {
+  // the blocks that reduce column s0 (n > s0 + 2), ceil((n - s0) / 64) workgroups each (m = n - s0 - 1)
+  auto per = [](int m) { return (m + 64) / 64; };
+  return tdb_launch_all(blk_idx, blk_n, nlist, s0, per, [&](const E2TdbGrid& G) {
+    hipLaunchKernelGGL(e2::tdb_refresh_kernel, dim3(G.cum[G.nb]), dim3(256), 0, s, b, G, s0, A);
   });
 }
 

@@ -65,7 +65,10 @@ static int e2_phase_a(nrg_ctx* c, const std::vector<E2Block>& blk, double* dA, d
     if (blk[b].n <= kSmall) small.push_back(b);
     else { large.push_back(b); nmax_large = std::max(nmax_large, blk[b].n); }
   }
-  // ---- stage 1b panel GEMMs: A[s:,s:] -= [V W][W V]^T at each panel end of each large block
+  // ---- stage 1b panel GEMMs: A[s:,s:] -= [V W][W V]^T at each panel end of each large block,
+  // on the 64 x 64 tiles that hold the lower triangle or lie within 127 of the
+  // diagonal: the product y = A v reads nothing else, and the rows of the next
+  // panel are rebuilt from their columns before it starts
   std::vector<E2Gemm> tprobs;
   std::vector<E2Tile> ttiles;
   std::vector<int> tev_k, tev_pbeg, tev_tbeg, tev_tcnt;
@@ -87,7 +90,7 @@ static int e2_phase_a(nrg_ctx* c, const std::vector<E2Block>& blk, double* dA, d
       tprobs.push_back(g);
       const int tt = (g.m + 63) / 64;
       for (int a = 0; a < tt; a++)
-        for (int bb = 0; bb < tt; bb++) ttiles.push_back(E2Tile{pi, a, bb, 0});
+        for (int bb = 0; bb < tt && bb <= a + 2; bb++) ttiles.push_back(E2Tile{pi, a, bb, 0});
     }
     if ((int)tprobs.size() > pbase) {
       tev_k.push_back(k); tev_pbeg.push_back(pbase); tev_tbeg.push_back(tbase);
@@ -202,7 +205,10 @@ static int e2_phase_a(nrg_ctx* c, const std::vector<E2Block>& blk, double* dA, d
         HIPCHK(e2_launch_gemm3(d_tp + tev_pbeg[ev], d_tt + tev_tbeg[ev], tev_tcnt[ev], WS, WS, dA, s));
         ev++;
       }
-      if ((k + 1) % kPanel == 0) HIPCHK(e2_launch_tdb_norm(d_blk, d_large, nl, nmax_large, k + 1, dA, WS, s));
+      if ((k + 1) % kPanel == 0) {
+        HIPCHK(e2_launch_tdb_refresh(d_blk, large.data(), large_n.data(), nl, k + 1, dA, s));
+        HIPCHK(e2_launch_tdb_norm(d_blk, d_large, nl, nmax_large, k + 1, dA, WS, s));
+      }
     }
     HIPCHK(e2_launch_tridiag_tail(d_blk, d_large, (int)large.size(), dA, WS, s));
   }
