@@ -62,6 +62,13 @@ int nrg_set_eig_impl(nrg_ctx *ctx, int impl);
  * batching across blocks); the baseline the batched mode is measured against. */
 int nrg_set_plain(nrg_ctx *ctx, int on);
 
+/* Streams of the blocked Householder reduction (impl 2, blocks above n=256):
+ * the blocks are split into n groups of about equal sum n^2, each reduced by
+ * its own chain of column launches on its own stream, so that one group's
+ * latency-bound launches overlap another's bandwidth-bound ones (results do
+ * not depend on n).  1..4, default 3. */
+int nrg_set_red_streams(nrg_ctx *ctx, int n);
+
 /* Diagnostics: run the batched solver (impl 2) on one n x n matrix and return
  * the stage outputs -- the tear-modified tridiagonal diagonal d (after the
  * divide-and-conquer set-up), the untorn off-diagonal e, the sorted

@@ -109,3 +109,31 @@ def test_nrg_like_graded(ctx):
     print("eig graded n=400", flush=True)
     W, Z, info = ctx.diag_blocks([a])
     _check(a, W[0], Z[0], tol=1e-12)
+
+
+def test_reduction_streams_give_identical_results(ctx):
+    # the large blocks split over 1..4 reduction streams: every block runs the
+    # same kernels on the same data, so the results are bit-identical
+    if ctx.impl != 2:
+        pytest.skip("the reduction streams belong to the batched solver")
+    rng = np.random.default_rng(21)
+    sizes = [257, 700, 300, 1100, 520, 90, 400, 33]
+    blocks = [_sym(rng, n, "degen" if n % 2 else "rand") for n in sizes]
+    ref = None
+    try:
+        for ns in (1, 2, 3, 4):
+            ctx.set_red_streams(ns)
+            print("eig streams", ns, sizes, flush=True)
+            W, Z, info = ctx.diag_blocks(blocks)
+            assert not info.any()
+            if ref is None:
+                ref = (W, Z)
+                for a, w, z in zip(blocks, W, Z):
+                    _check(a, w, z, tol=1e-11)
+            else:
+                for w0, z0, w, z in zip(ref[0], ref[1], W, Z):
+                    assert np.array_equal(w0, w) and np.array_equal(z0, z)
+    finally:
+        ctx.set_red_streams(3)
+    with pytest.raises(t.NrgError):
+        ctx.set_red_streams(5)

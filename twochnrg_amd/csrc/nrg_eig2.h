@@ -67,7 +67,7 @@ struct E2BtRows {
 // The live blocks of one reduction launch, passed by value: workgroup x works
 // on block blk[i] for cum[i] <= x < cum[i+1], so no workgroup is launched for
 // a block that is smaller than the largest or already reduced.
-#define E2_TDB_MAXB 128
+#define E2_TDB_MAXB 32
 struct E2TdbGrid {
   int nb;
   int cum[E2_TDB_MAXB + 1];

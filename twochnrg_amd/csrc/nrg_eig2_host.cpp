@@ -59,42 +59,60 @@ static int e2_phase_a(nrg_ctx* c, const std::vector<E2Block>& blk, double* dA, d
   for (const E2Block& B : blk) nmax = std::max(nmax, B.n);
   // ---- stage 1 lists
   std::vector<int> small, large;
-  int nmax_large = 0;
   for (int b = 0; b < nb; b++) {
     if (blk[b].n <= 0) continue;
     if (blk[b].n <= kSmall) small.push_back(b);
-    else { large.push_back(b); nmax_large = std::max(nmax_large, blk[b].n); }
+    else large.push_back(b);
+  }
+  // ---- stage 1b groups: the large blocks split into groups of about equal
+  // sum n^2 (largest first, each to the lightest group); each group's chain of
+  // column launches runs on a stream of its own, so that one group's
+  // latency-bound launches overlap another's bandwidth-bound products
+  std::sort(large.begin(), large.end(), [&](int x, int y) { return blk[x].n > blk[y].n; });
+  const int ngrp = std::min<int>(c->plain ? 1 : c->red_streams, (int)large.size());
+  std::vector<std::vector<int>> grp(ngrp);
+  {
+    std::vector<double> load(ngrp, 0.0);
+    for (int b : large) {
+      const int g = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+      grp[g].push_back(b);
+      load[g] += (double)blk[b].n * blk[b].n;
+    }
   }
   // ---- stage 1b panel GEMMs: A[s:,s:] -= [V W][W V]^T at each panel end of each large block,
   // on the 64 x 64 tiles that hold the lower triangle or lie within 127 of the
   // diagonal: the product y = A v reads nothing else, and the rows of the next
-  // panel are rebuilt from their columns before it starts
+  // panel are rebuilt from their columns before it starts.  Per group: the
+  // launch at column k covers the group's blocks whose panel ends there.
   std::vector<E2Gemm> tprobs;
   std::vector<E2Tile> ttiles;
-  std::vector<int> tev_k, tev_pbeg, tev_tbeg, tev_tcnt;
-  for (int k = 0; k + 2 < nmax_large; k++) {
-    const int j = k % kPanel, k0 = k - j;
-    const int pbase = (int)tprobs.size(), tbase = (int)ttiles.size();
-    for (int b : large) {
-      const int nb_ = blk[b].n;
-      if (k + 2 >= nb_ || !(j == kPanel - 1 || k + 3 >= nb_)) continue;
-      const int s0 = k0 + std::min(kPanel, nb_ - 2 - k0);
-      const int64_t nn = nb_;
-      E2Gemm g{};
-      g.m = nb_ - s0; g.n = nb_ - s0; g.k = 2 * kPanel; g.dyn = -1;
-      g.a_off = blk[b].p_off + s0; g.as0 = 1; g.as1 = nn;
-      g.b_off = blk[b].p_off + (int64_t)kPanel * nn + s0; g.bs0 = nn; g.bs1 = 1;
-      g.c_off = blk[b].a_off + (int64_t)s0 * nn + s0; g.cs0 = nn;
-      g.alpha = -1.0; g.accumulate = 1;
-      const int pi = (int)tprobs.size() - pbase;
-      tprobs.push_back(g);
-      const int tt = (g.m + 63) / 64;
-      for (int a = 0; a < tt; a++)
-        for (int bb = 0; bb < tt && bb <= a + 2; bb++) ttiles.push_back(E2Tile{pi, a, bb, 0});
-    }
-    if ((int)tprobs.size() > pbase) {
-      tev_k.push_back(k); tev_pbeg.push_back(pbase); tev_tbeg.push_back(tbase);
-      tev_tcnt.push_back((int)ttiles.size() - tbase);
+  std::vector<std::vector<int>> tev_k(ngrp), tev_pbeg(ngrp), tev_tbeg(ngrp), tev_tcnt(ngrp);
+  for (int g = 0; g < ngrp; g++) {
+    const int nmax_g = grp[g].empty() ? 0 : blk[grp[g][0]].n;
+    for (int k = 0; k + 2 < nmax_g; k++) {
+      const int j = k % kPanel, k0 = k - j;
+      const int pbase = (int)tprobs.size(), tbase = (int)ttiles.size();
+      for (int b : grp[g]) {
+        const int nb_ = blk[b].n;
+        if (k + 2 >= nb_ || !(j == kPanel - 1 || k + 3 >= nb_)) continue;
+        const int s0 = k0 + std::min(kPanel, nb_ - 2 - k0);
+        const int64_t nn = nb_;
+        E2Gemm gm{};
+        gm.m = nb_ - s0; gm.n = nb_ - s0; gm.k = 2 * kPanel; gm.dyn = -1;
+        gm.a_off = blk[b].p_off + s0; gm.as0 = 1; gm.as1 = nn;
+        gm.b_off = blk[b].p_off + (int64_t)kPanel * nn + s0; gm.bs0 = nn; gm.bs1 = 1;
+        gm.c_off = blk[b].a_off + (int64_t)s0 * nn + s0; gm.cs0 = nn;
+        gm.alpha = -1.0; gm.accumulate = 1;
+        const int pi = (int)tprobs.size() - pbase;
+        tprobs.push_back(gm);
+        const int tt = (gm.m + 63) / 64;
+        for (int a = 0; a < tt; a++)
+          for (int bb = 0; bb < tt && bb <= a + 2; bb++) ttiles.push_back(E2Tile{pi, a, bb, 0});
+      }
+      if ((int)tprobs.size() > pbase) {
+        tev_k[g].push_back(k); tev_pbeg[g].push_back(pbase); tev_tbeg[g].push_back(tbase);
+        tev_tcnt[g].push_back((int)ttiles.size() - tbase);
+      }
     }
   }
   // ---- stage 2 trees
@@ -161,7 +179,12 @@ static int e2_phase_a(nrg_ctx* c, const std::vector<E2Block>& blk, double* dA, d
   E2MState* d_ms;
   if ((rc = upload(c, "e2_blk", blk, &d_blk))) return rc;
   if ((rc = upload(c, "e2_small", small, &d_small))) return rc;
-  if ((rc = upload(c, "e2_large", large, &d_large))) return rc;
+  std::vector<int> glist, goff(ngrp + 1, 0);
+  for (int g = 0; g < ngrp; g++) {
+    glist.insert(glist.end(), grp[g].begin(), grp[g].end());
+    goff[g + 1] = (int)glist.size();
+  }
+  if ((rc = upload(c, "e2_large", glist, &d_large))) return rc;
   if ((rc = upload(c, "e2_leaf", leaves, &d_leaf))) return rc;
   if ((rc = upload(c, "e2_all", all_merges, &d_all))) return rc;
   if ((rc = upload(c, "e2_mbeg", mbeg, &d_mbeg))) return rc;
@@ -175,43 +198,66 @@ static int e2_phase_a(nrg_ctx* c, const std::vector<E2Block>& blk, double* dA, d
   if ((rc = upload(c, "e2_tt", ttiles, &d_tt))) return rc;
   // ---- launches
   hipStream_t s = c->stream;
-  HIPCHK(e2_launch_tridiag_small(d_blk, d_small, (int)small.size(), dA, WS, s));
-  if (!large.empty()) {
-    const int nl = (int)large.size();
-    std::vector<int> large_n(nl);
-    for (int i = 0; i < nl; i++) large_n[i] = blk[large[i]].n;
-    HIPCHK(e2_launch_tdb_norm(d_blk, d_large, nl, nmax_large, 0, dA, WS, s));
-    size_t ev = 0;
-    for (int k = 0; k + 2 < nmax_large; k++) {
-      if (k % 8 == 0) {
-        // every 8th column is timed (the events cost more than a short launch):
-        // algorithmic bytes = the lower triangle (diagonal included) of the
-        // symmetric trailing matrix of every active large block
-        double by = 0.0;
-        for (int b : large)
-          if (k + 2 < blk[b].n) {
-            const double mm = (double)(blk[b].n - k - 1);
-            by += 8.0 * mm * (mm + 1.0) * 0.5;
-          }
-        PendingTimer tm;
-        if ((rc = c->timer_begin(tm, NRG_K_SYMV, 0.0, by))) return rc;
-        HIPCHK(e2_launch_tdb_symv(d_blk, large.data(), large_n.data(), nl, k, dA, WS, s));
-        if ((rc = c->timer_end(tm))) return rc;
-      } else {
-        HIPCHK(e2_launch_tdb_symv(d_blk, large.data(), large_n.data(), nl, k, dA, WS, s));
-      }
-      HIPCHK(e2_launch_tdb_fin(d_blk, large.data(), large_n.data(), nl, k, dA, WS, s));
-      if (ev < tev_k.size() && tev_k[ev] == k) {
-        HIPCHK(e2_launch_gemm3(d_tp + tev_pbeg[ev], d_tt + tev_tbeg[ev], tev_tcnt[ev], WS, WS, dA, s));
-        ev++;
-      }
-      if ((k + 1) % kPanel == 0) {
-        HIPCHK(e2_launch_tdb_refresh(d_blk, large.data(), large_n.data(), nl, k + 1, dA, s));
-        HIPCHK(e2_launch_tdb_norm(d_blk, d_large, nl, nmax_large, k + 1, dA, WS, s));
+  // the small blocks' single-workgroup reductions run beside the large groups'
+  // chains, on a stream of their own when there are large blocks
+  const int nfork = ngrp > 0 ? (ngrp - 1) + (small.empty() ? 0 : 1) : 0;   // aux streams used below
+  if (nfork > 0 && (rc = c->fork(nfork))) return rc;   // after everything enqueued so far on s
+  const bool small_aside = ngrp > 0 && !small.empty();   // then aux[ngrp - 1] is the small blocks' stream
+  HIPCHK(e2_launch_tridiag_small(d_blk, d_small, (int)small.size(), dA, WS, small_aside ? c->aux[ngrp - 1] : s));
+  if (ngrp > 0) {
+    std::vector<hipStream_t> gs(ngrp, s);
+    for (int g = 1; g < ngrp; g++) gs[g] = c->aux[g - 1];
+    std::vector<std::vector<int>> gn(ngrp);
+    std::vector<int> nmax_g(ngrp);
+    std::vector<size_t> ev(ngrp, 0);
+    int kmax = 0;
+    for (int g = 0; g < ngrp; g++) {
+      for (int b : grp[g]) gn[g].push_back(blk[b].n);
+      nmax_g[g] = gn[g][0];
+      kmax = std::max(kmax, nmax_g[g]);
+      HIPCHK(e2_launch_tdb_norm(d_blk, d_large + goff[g], (int)grp[g].size(), nmax_g[g], 0, dA, WS, gs[g]));
+    }
+    for (int k = 0; k + 2 < kmax; k++) {
+      for (int g = 0; g < ngrp; g++) {
+        if (k + 2 >= nmax_g[g]) continue;
+        const hipStream_t sg = gs[g];
+        const int nl = (int)grp[g].size();
+        const int* bl = grp[g].data();
+        const int* bn = gn[g].data();
+        if (k % 8 == 0) {
+          // every 8th column is timed (the events cost more than a short launch):
+          // algorithmic bytes = the lower triangle (diagonal included) of the
+          // symmetric trailing matrix of every active large block of the group
+          double by = 0.0;
+          for (int b : grp[g])
+            if (k + 2 < blk[b].n) {
+              const double mm = (double)(blk[b].n - k - 1);
+              by += 8.0 * mm * (mm + 1.0) * 0.5;
+            }
+          PendingTimer tm;
+          if ((rc = c->timer_begin(tm, NRG_K_SYMV, 0.0, by, sg))) return rc;
+          HIPCHK(e2_launch_tdb_symv(d_blk, bl, bn, nl, k, dA, WS, sg));
+          if ((rc = c->timer_end(tm, sg))) return rc;
+        } else {
+          HIPCHK(e2_launch_tdb_symv(d_blk, bl, bn, nl, k, dA, WS, sg));
+        }
+        HIPCHK(e2_launch_tdb_fin(d_blk, bl, bn, nl, k, dA, WS, sg));
+        size_t& e = ev[g];
+        if (e < tev_k[g].size() && tev_k[g][e] == k) {
+          HIPCHK(e2_launch_gemm3(d_tp + tev_pbeg[g][e], d_tt + tev_tbeg[g][e], tev_tcnt[g][e], WS, WS, dA, sg));
+          e++;
+        }
+        if ((k + 1) % kPanel == 0) {
+          HIPCHK(e2_launch_tdb_refresh(d_blk, bl, bn, nl, k + 1, dA, sg));
+          HIPCHK(e2_launch_tdb_norm(d_blk, d_large + goff[g], nl, nmax_g[g], k + 1, dA, WS, sg));
+        }
       }
     }
-    HIPCHK(e2_launch_tridiag_tail(d_blk, d_large, (int)large.size(), dA, WS, s));
+    for (int g = 0; g < ngrp; g++)
+      HIPCHK(e2_launch_tridiag_tail(d_blk, d_large + goff[g], (int)grp[g].size(), dA, WS, gs[g]));
   }
+  // s continues after every group's chain and the small blocks
+  if (nfork > 0 && (rc = c->join(nfork))) return rc;
   HIPCHK(e2_launch_dc_init(d_blk, nb, nmax, d_all, (int)all_merges.size(), d_mbeg, WS, s));
   HIPCHK(e2_launch_dc_leaf(d_blk, d_leaf, (int)leaves.size(), WS, s));
   for (int h = 0; h < nlev; h++)

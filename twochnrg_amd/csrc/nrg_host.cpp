@@ -115,6 +115,11 @@ extern "C" void nrg_ctx_destroy(nrg_ctx* c) {
   nrg::dm_detach_all(c);
   c->resolve();
   for (auto e : c->event_pool) hipEventDestroy(e);
+  for (auto a : c->aux) {
+    hipStreamSynchronize(a);
+    hipStreamDestroy(a);
+  }
+  for (auto e : c->sync_ev) hipEventDestroy(e);
   c->bufs.clear();
   if (c->ring) hipHostFree(c->ring);
   hipStreamDestroy(c->stream);
@@ -142,21 +147,45 @@ hipEvent_t nrg_ctx::get_event() {
   return e;
 }
 
-int nrg_ctx::timer_begin(PendingTimer& t, int kind, double flops, double bytes) {
+int nrg_ctx::timer_begin(PendingTimer& t, int kind, double flops, double bytes, hipStream_t s) {
   t.kind = kind; t.flops = flops; t.bytes = bytes;
   t.a = t.b = nullptr;
   if (!timing) return 0;
   t.a = get_event();
   t.b = get_event();
   if (!t.a || !t.b) return fail(NRG_ERR_HIP, "hipEventCreate failed");
-  HIPCHK(hipEventRecord(t.a, stream));
+  HIPCHK(hipEventRecord(t.a, s ? s : stream));
   return 0;
 }
 
-int nrg_ctx::timer_end(PendingTimer& t) {
+int nrg_ctx::timer_end(PendingTimer& t, hipStream_t s) {
   if (!timing || !t.a) return 0;
-  HIPCHK(hipEventRecord(t.b, stream));
+  HIPCHK(hipEventRecord(t.b, s ? s : stream));
   pending.push_back(t);
+  return 0;
+}
+
+int nrg_ctx::fork(int n) {
+  while ((int)aux.size() < n) {
+    hipStream_t a = nullptr;
+    HIPCHK(hipStreamCreate(&a));
+    aux.push_back(a);
+  }
+  while ((int)sync_ev.size() < n + 1) {
+    hipEvent_t e = nullptr;
+    HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    sync_ev.push_back(e);
+  }
+  HIPCHK(hipEventRecord(sync_ev[0], stream));
+  for (int i = 0; i < n; i++) HIPCHK(hipStreamWaitEvent(aux[i], sync_ev[0], 0));
+  return 0;
+}
+
+int nrg_ctx::join(int n) {
+  for (int i = 0; i < n; i++) {
+    HIPCHK(hipEventRecord(sync_ev[i + 1], aux[i]));
+    HIPCHK(hipStreamWaitEvent(stream, sync_ev[i + 1], 0));
+  }
   return 0;
 }
 
@@ -183,6 +212,13 @@ extern "C" int nrg_set_eig_impl(nrg_ctx* c, int impl) {
 extern "C" int nrg_set_plain(nrg_ctx* c, int on) {
   if (!c) return fail(NRG_ERR_ARG, "nrg_set_plain: NULL ctx");
   c->plain = on != 0;
+  return NRG_OK;
+}
+
+extern "C" int nrg_set_red_streams(nrg_ctx* c, int n) {
+  if (!c) return fail(NRG_ERR_ARG, "nrg_set_red_streams: NULL ctx");
+  if (n < 1 || n > 4) return fail(NRG_ERR_ARG, "nrg_set_red_streams: n must be 1..4");
+  c->red_streams = n;
   return NRG_OK;
 }
 

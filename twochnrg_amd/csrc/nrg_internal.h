@@ -79,13 +79,22 @@ struct nrg_ctx {
   bool timing = false;
   int eig_impl = 2;   // 1: one workgroup per block (Householder + QL); 2: tridiag + D&C + WY
   bool plain = false; // one block (eigensolve) / one block pair (rotation) per launch sequence
+  int red_streams = 3; // streams (block groups) of the blocked reduction
   KStat stat[NRG_NKSTAT];
   std::vector<PendingTimer> pending;
   std::vector<hipEvent_t> event_pool;
   hipEvent_t get_event();
-  // record an event pair around a launch; resolve() after a stream sync
-  int timer_begin(PendingTimer& t, int kind, double flops, double bytes);
-  int timer_end(PendingTimer& t);
+  // record an event pair around a launch (on stream s, default the context
+  // stream); resolve() after a stream sync
+  int timer_begin(PendingTimer& t, int kind, double flops, double bytes, hipStream_t s = nullptr);
+  int timer_end(PendingTimer& t, hipStream_t s = nullptr);
+  // auxiliary streams (default priority, created on first use): fork(n) makes
+  // aux[0..n) wait for the work enqueued on stream so far, join(n) makes
+  // stream wait for theirs
+  std::vector<hipStream_t> aux;
+  std::vector<hipEvent_t> sync_ev;
+  int fork(int n);
+  int join(int n);
   int resolve();
 };
 
