@@ -604,16 +604,19 @@ __global__ void dc_init_kernel(const E2Block* __restrict__ blks, const E2Merge* 
   for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < (size_t)n * n;
        idx += (size_t)gridDim.x * blockDim.x)
     Qv[idx] = 0.0;
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
+  if (blockIdx.x == 0) {
+    // the raw e saved, then the tears: split points are at least LEAF/2 apart,
+    // so no two merges touch the same d or e entry and they run in parallel
     double* d = VD(0);
     double* e = VD(1);
     double* erow = VD(16);
-    for (int i = 0; i < n; i++) erow[i] = e[i];
-    for (int m = mbeg[bi]; m < mbeg[bi + 1]; m++) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) erow[i] = e[i];
+    __syncthreads();
+    for (int m = mbeg[bi] + threadIdx.x; m < mbeg[bi + 1]; m += blockDim.x) {
       const int mid = merges[m].mid;
-      const double r = e[mid - 1];
-      d[mid - 1] -= fabs(r);
-      d[mid] -= fabs(r);
+      const double r = fabs(erow[mid - 1]);
+      d[mid - 1] -= r;
+      d[mid] -= r;
       e[mid - 1] = 0.0;
     }
   }
@@ -787,10 +790,13 @@ __global__ __launch_bounds__(E2_T) void dc_prep_kernel(const E2Block* __restrict
         const int pi = cpm[q];
         if (rho * fabs(zi) <= tol) { dval[nd] = di; drow[nd] = pi; nd++; continue; }
         if (pend < 0) { pend = c0 + q; pds = di; pzs = zi; pperm = pi; continue; }
-        const double tv = hypot(zi, pzs);
-        const double cc = zi / tv, sv = -pzs / tv;
+        // the test |gap c s| <= tol with c = zi/t, s = -pzs/t, t^2 = zi^2 + pzs^2,
+        // multiplied through by t^2 (z is normalised, so t^2 <= 1 does not overflow)
+        const double t2 = zi * zi + pzs * pzs;
         const double gap = di - pds;
-        if (fabs(gap * cc * sv) <= tol) {
+        if (fabs(gap * zi * pzs) <= tol * t2) {
+          const double tv = sqrt(t2);
+          const double cc = zi / tv, sv = -pzs / tv;
           ga[ng] = pperm; gb[ng] = pi; gc[ng] = cc; gs[ng] = sv; ng++;
           const double dp = pds * cc * cc + di * sv * sv;
           di = pds * sv * sv + di * cc * cc;
@@ -865,6 +871,7 @@ __global__ __launch_bounds__(E2_T) void dc_givens_kernel(const E2Block* __restri
   }
 }
 
+#define SEC_LDS 2048   // merges up to this many poles keep them in LDS for the root iterations
 // secular equation root j: f(lambda) = 1 + rho sum_l zk_l^2/(dk_l - lambda), one
 // wavefront per root.  lambda = dk[org] + tau with the nearer pole as origin;
 // iteration: the two-pole rational model of the left and right sums (matching
@@ -874,15 +881,28 @@ __global__ __launch_bounds__(E2_T) void dc_secular_kernel(const E2Block* __restr
                                                           const E2Merge* __restrict__ merges,
                                                           const E2MState* __restrict__ ms, double* __restrict__ WS,
                                                           int* __restrict__ IW) {
+  // the poles and weights of the merge, staged once per workgroup (its
+  // wavefronts solve roots of the same merge); larger merges read them from
+  // global memory
+  __shared__ double sdk[SEC_LDS], szk[SEC_LDS];
   const E2Merge M = merges[blockIdx.y];
   const E2MState S = ms[blockIdx.y];
   const E2Block B = blks[M.blk];
   const int n = B.n, lo = M.lo, k = S.k;
-  const int j = blockIdx.x * (E2_T / WAVE) + threadIdx.x / WAVE;
+  const int j0 = blockIdx.x * (E2_T / WAVE);
+  if (j0 >= k) return;
+  const int j = j0 + threadIdx.x / WAVE;
   const int lane = threadIdx.x & (WAVE - 1);
+  const double* gdk = VD(9) + lo;
+  const double* gzk = VD(10) + lo;
+  const bool staged = k <= SEC_LDS;
+  if (staged) {
+    for (int l = threadIdx.x; l < k; l += E2_T) { sdk[l] = gdk[l]; szk[l] = gzk[l]; }
+    __syncthreads();
+  }
   if (j >= k) return;
-  const double* dk = VD(9) + lo;
-  const double* zk = VD(10) + lo;
+  const double* dk = staged ? sdk : gdk;
+  const double* zk = staged ? szk : gzk;
   double* lam = VD(11) + lo;
   int* org = VI(5) + lo;
   const int m = M.hi - M.lo;
@@ -957,28 +977,43 @@ __global__ __launch_bounds__(E2_T) void dc_secular_kernel(const E2Block* __restr
   for (int l = lane; l < k; l += WAVE) dj[l] = (dk[l] - dorg) - tau;
 }
 
-// Gu-Eisenstat: zhat_l^2 = prod_j (lambda_j - d_l) / (rho prod_{j!=l} (d_j - d_l))
+// Gu-Eisenstat: zhat_l^2 = prod_j (lambda_j - d_l) / (rho prod_{j!=l} (d_j - d_l)).
+// ZG threads per element l, each multiplying the factors of a contiguous range
+// of j in order; the ZG partial products are then multiplied in range order
+#define ZG 8
 __global__ __launch_bounds__(E2_T) void dc_zhat_kernel(const E2Block* __restrict__ blks,
                                                        const E2Merge* __restrict__ merges,
                                                        const E2MState* __restrict__ ms, double* __restrict__ WS) {
+  __shared__ double part[E2_T];
   const E2Merge M = merges[blockIdx.y];
   const E2MState S = ms[blockIdx.y];
   const E2Block B = blks[M.blk];
   const int n = B.n, lo = M.lo, k = S.k;
-  const int l = blockIdx.x * blockDim.x + threadIdx.x;
-  if (l >= k) return;
+  if ((int)blockIdx.x * (E2_T / ZG) >= k) return;
+  const int tid = threadIdx.x, g = tid % ZG;
+  const int l = blockIdx.x * (E2_T / ZG) + tid / ZG;
   const double* dk = VD(9) + lo;
   const double* zk = VD(10) + lo;
   double* zh = VD(12) + lo;
   const int m = M.hi - M.lo;
   const double* delta = WS + B.dl_off + (size_t)lo * n;
-  double p = -delta[(size_t)l * m + l] / S.rho;
-  for (int j = 0; j < k; j++) {
-    if (j == l) continue;
-    p *= -delta[(size_t)j * m + l] / (dk[j] - dk[l]);
+  double p = 1.0;
+  if (l < k) {
+    const int c = (k + ZG - 1) / ZG, j0 = g * c, j1 = min(k, j0 + c);
+    const double dl = dk[l];
+    for (int j = j0; j < j1; j++) {
+      if (j == l) continue;
+      p *= -delta[(size_t)j * m + l] / (dk[j] - dl);
+    }
   }
-  const double a = sqrt(fabs(p));
-  zh[l] = (zk[l] >= 0.0) ? a : -a;
+  part[tid] = p;
+  __syncthreads();
+  if (g == 0 && l < k) {
+    double q = -delta[(size_t)l * m + l] / S.rho;
+    for (int h = 0; h < ZG; h++) q *= part[tid + h];
+    const double a = sqrt(fabs(q));
+    zh[l] = (zk[l] >= 0.0) ? a : -a;
+  }
 }
 
 // eigenvector j of the rank-one problem: u_j(l) = zhat_l / (d_l - lambda_j),
@@ -1031,52 +1066,54 @@ __global__ void dc_gather_kernel(const E2Block* __restrict__ blks, const E2Merge
 // final order of one merge: the deflated values are sorted by rank (ties by
 // index), then merged with the ascending roots by binary search (roots first
 // on ties); writes the eigenvalues and the source row of every output row
+// rank of each deflated value (ties by index): element i goes to position
+// #{y < x_i} + #{y == x_i, index < i} of the sorted deflated list; one thread
+// per element, E2_T elements per workgroup, every value staged through LDS
+__global__ __launch_bounds__(E2_T) void dc_rank_kernel(const E2Block* __restrict__ blks,
+                                                       const E2Merge* __restrict__ merges,
+                                                       const E2MState* __restrict__ ms, double* __restrict__ WS,
+                                                       int* __restrict__ IW) {
+  __shared__ double cv[E2_T];
+  const E2Merge M = merges[blockIdx.y];
+  const E2MState S = ms[blockIdx.y];
+  const E2Block B = blks[M.blk];
+  const int n = B.n, lo = M.lo, nd = S.nd, tid = threadIdx.x;
+  if ((int)blockIdx.x * E2_T >= nd) return;
+  const double* dval = VD(13) + lo;
+  double* dvs = VD(7) + lo;   // sorted deflated values (the ds scratch of the merge)
+  int* dix = VI(0) + lo;      // their indices (the perm scratch)
+  const int i = blockIdx.x * E2_T + tid;
+  const double xv = (i < nd) ? dval[i] : 0.0;
+  int rk = 0;
+  for (int c0 = 0; c0 < nd; c0 += E2_T) {
+    const int cnt = min(E2_T, nd - c0);
+    __syncthreads();
+    if (tid < cnt) cv[tid] = dval[c0 + tid];
+    __syncthreads();
+    if (i < nd)
+      for (int t = 0; t < cnt; t++) {
+        const double y = cv[t];
+        rk += (y < xv) || (y == xv && c0 + t < i);
+      }
+  }
+  if (i < nd) { dvs[rk] = xv; dix[rk] = i; }
+}
+
+// final order of a merge: the roots and the sorted deflated values merged by
+// binary search (ties: roots first)
 __global__ __launch_bounds__(E2_T) void dc_order_kernel(const E2Block* __restrict__ blks,
                                                         const E2Merge* __restrict__ merges,
                                                         const E2MState* __restrict__ ms, double* __restrict__ WS,
                                                         int* __restrict__ IW) {
-  __shared__ double cv[E2_T];
   const E2Merge M = merges[blockIdx.x];
   const E2MState S = ms[blockIdx.x];
   const E2Block B = blks[M.blk];
   const int n = B.n, lo = M.lo, k = S.k, nd = S.nd, tid = threadIdx.x;
   const double* lam = VD(11) + lo;
-  const double* dval = VD(13) + lo;
   double* dd = VD(5);
-  double* dvs = VD(7) + lo;   // sorted deflated values (the ds scratch of the merge)
-  int* dix = VI(0) + lo;      // their indices (the perm scratch)
+  const double* dvs = VD(7) + lo;
+  const int* dix = VI(0) + lo;
   int* src = VI(6) + lo;
-  for (int base = 0; base < nd; base += E2_T * 16) {
-    int rk[16];
-    double xv[16];
-#pragma unroll
-    for (int q = 0; q < 16; q++) {
-      const int i = base + q * E2_T + tid;
-      xv[q] = (i < nd) ? dval[i] : 0.0;
-      rk[q] = 0;
-    }
-    for (int c0 = 0; c0 < nd; c0 += E2_T) {
-      const int cnt = min(E2_T, nd - c0);
-      __syncthreads();
-      if (tid < cnt) cv[tid] = dval[c0 + tid];
-      __syncthreads();
-#pragma unroll
-      for (int q = 0; q < 16; q++) {
-        const int i = base + q * E2_T + tid;
-        if (i < nd)
-          for (int t = 0; t < cnt; t++) {
-            const double y = cv[t];
-            rk[q] += (y < xv[q]) || (y == xv[q] && c0 + t < i);
-          }
-      }
-    }
-#pragma unroll
-    for (int q = 0; q < 16; q++) {
-      const int i = base + q * E2_T + tid;
-      if (i < nd) { dvs[rk[q]] = xv[q]; dix[rk[q]] = i; }
-    }
-  }
-  __syncthreads();
   for (int a = tid; a < k; a += E2_T) {
     const int pos = a + count_less(dvs, nd, lam[a]);
     dd[lo + pos] = lam[a];
@@ -1465,7 +1502,8 @@ hipError_t e2_launch_dc_level(const E2Block* b, const E2Merge* merges, int nm, i
   if ((e = hipGetLastError()) != hipSuccess) return e;
   hipLaunchKernelGGL(e2::dc_secular_kernel, dim3(gw, nm), dim3(E2_T), 0, s, b, merges, ms, WS, IW);
   if ((e = hipGetLastError()) != hipSuccess) return e;
-  hipLaunchKernelGGL(e2::dc_zhat_kernel, dim3(gy, nm), dim3(E2_T), 0, s, b, merges, ms, WS);
+  hipLaunchKernelGGL(e2::dc_zhat_kernel, dim3((mmax + E2_T / ZG - 1) / (E2_T / ZG), nm), dim3(E2_T), 0, s, b, merges,
+                     ms, WS);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   hipLaunchKernelGGL(e2::dc_vec_kernel, dim3(gw, nm), dim3(E2_T), 0, s, b, merges, ms, WS);
   if ((e = hipGetLastError()) != hipSuccess) return e;
@@ -1476,6 +1514,8 @@ hipError_t e2_launch_dc_level(const E2Block* b, const E2Merge* merges, int nm, i
     hipLaunchKernelGGL(e2::gemm_kernel, dim3(ntiles), dim3(256), 0, s, gprobs, gtiles, WS, WS, WS, ms);
     if ((e = hipGetLastError()) != hipSuccess) return e;
   }
+  hipLaunchKernelGGL(e2::dc_rank_kernel, dim3(gy, nm), dim3(E2_T), 0, s, b, merges, ms, WS, IW);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
   hipLaunchKernelGGL(e2::dc_order_kernel, dim3(nm), dim3(E2_T), 0, s, b, merges, ms, WS, IW);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   hipLaunchKernelGGL(e2::dc_copy_kernel, dim3(gx, nm), dim3(256), 0, s, b, merges, WS, IW);
