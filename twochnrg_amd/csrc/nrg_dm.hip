@@ -8,6 +8,7 @@
 #include <stdint.h>
 
 #include "nrg_dm.h"
+#include "nrg_wave.h"
 #include "../../include/twochnrg_amd.h"
 
 namespace dmk {
@@ -203,9 +204,7 @@ __global__ __launch_bounds__(BT) void broaden_tiles_kernel(const DmTile* __restr
   }
 #pragma unroll
   for (int j = 0; j < DM_TILE_W; j++) {
-    double v = acc[j];
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    const double v = nrgw::wave_sum(acc[j]);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6][j] = v;
   }
   __syncthreads();
@@ -316,8 +315,7 @@ __global__ __launch_bounds__(BT) void broaden_jvp_kernel(const DmTask* __restric
     const double g = bdelta_dx(T.w, dE[p], T.twin, bbroad, btemp, &gp);
     s += w1[p] * g + w2[p] * gp;
   }
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  s = nrgw::wave_sum(s);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
   __syncthreads();
   if (threadIdx.x == 0) partial[t * p_stride + T.chunk_beg + c] = (red[0] + red[1]) + (red[2] + red[3]);

@@ -24,6 +24,7 @@
 #include <stdint.h>
 
 #include "nrg_eig2.h"
+#include "nrg_wave.h"
 
 #define E2_T 256
 #define WAVE 64
@@ -33,11 +34,7 @@
 
 namespace e2 {
 
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int o = WAVE / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
-  return v;
-}
+__device__ __forceinline__ double wave_sum(double v) { return nrgw::wave_sum(v); }
 __device__ __forceinline__ double block_sum(double v, double* red) {
   v = wave_sum(v);
   const int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
@@ -54,6 +51,139 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
 // ---------------------------------------------------------------------------
 #define SM_G 4              // row groups of the small-block reduction
 #define SM_T (256 * SM_G)    // its workgroup: column c = tid % 256, row group g = tid / 256
+#define RS_REG 36            // resident path: matrix elements per thread held in registers
+#define RS_LDS 16            // ... and in LDS
+#define RS_MAXN (SM_G * (RS_REG + RS_LDS))   // largest block the resident path holds (208)
+
+// Resident Householder reduction of one block with n <= RS_MAXN: the whole
+// matrix stays in the workgroup, column c's rows r = g (mod SM_G) with thread
+// (c, g), the first RS_REG of them in registers and the rest in LDS, so a
+// column step touches global memory only to store its reflector.  Per column
+// k: row k from its owners, the reflector from a block reduction, y = A v
+// from each thread's rows (groups added in order), w = t y - (t/2)(v.ty) v,
+// and the rank-2 update A -= v w^T + w v^T in place.  The update of element
+// (r, c) rounds the same two products and their sum as that of (c, r), so the
+// matrix stays exactly symmetric.
+__device__ __forceinline__ double rs_get(const double (&a)[RS_REG], const double (*slots)[SM_T], int s, int tid) {
+  if (s >= RS_REG) return slots[s - RS_REG][tid];
+  double v = 0.0;
+#pragma unroll
+  for (int q = 0; q < RS_REG; q++)
+    if (q == s) v = a[q];
+  return v;
+}
+
+__device__ void tridiag_resident(const int n, double* __restrict__ A, double* __restrict__ d, double* __restrict__ e,
+                                 double* __restrict__ tau, double* red, double* ash, double* vsh, double* wsh,
+                                 double (*qpart)[256], double (*slots)[SM_T]) {
+  const int tid = threadIdx.x, c = tid & 255, g = tid >> 8;
+  const int gu = __builtin_amdgcn_readfirstlane(g);   // g as a scalar (uniform per wavefront)
+  double a[RS_REG];
+#pragma unroll
+  for (int s = 0; s < RS_REG; s++) {
+    const int r = g + SM_G * s;
+    a[s] = (r < n && c < n) ? A[(size_t)r * n + c] : 0.0;
+  }
+  for (int s = 0; s < RS_LDS; s++) {
+    const int r = g + SM_G * (RS_REG + s);
+    slots[s][tid] = (r < n && c < n) ? A[(size_t)r * n + c] : 0.0;   // each thread's own slots: no barrier
+  }
+  // v and w are zero outside rows k+1..n-1 of the current column (rows >= n
+  // never written), so the products and updates below run over every slot
+  // with no per-slot condition: a slot of row r <= k or r >= n gains exact zeros
+  if (tid < 256) { vsh[tid] = 0.0; wsh[tid] = 0.0; }
+  __syncthreads();
+  for (int k = 0; k + 2 < n; k++) {
+    // row k of the current matrix: element (k, c) is thread (c, k mod SM_G)'s slot k / SM_G
+    const bool own = (g == k % SM_G) && c < n;
+    const double ak = own ? rs_get(a, slots, k / SM_G, tid) : 0.0;
+    if (own) ash[c] = ak;
+    const double nrm2 = block_sum((own && c > k) ? ak * ak : 0.0, red);   // its barriers order the ash stores
+    const double x0 = ash[k + 1];
+    double t = 0.0, v0 = x0, alpha = 0.0;
+    if (nrm2 != 0.0) {
+      const double nrm = sqrt(nrm2);
+      alpha = (x0 > 0.0) ? -nrm : nrm;
+      v0 = x0 - alpha;
+      t = 2.0 / (nrm2 - x0 * x0 + v0 * v0);
+    }
+    if (g == 0 && c < n) {
+      const double vc = (c > k + 1) ? ash[c] : (c == k + 1 ? v0 : 0.0);
+      vsh[c] = vc;
+      if (c > k) A[(size_t)k * n + c] = vc;   // reflector k in row k
+    }
+    if (tid == 0) { d[k] = ash[k]; e[k] = alpha; tau[k] = t; }
+    __syncthreads();
+    // y_c over this thread's rows r > k
+    const bool act = c > k && c < n;
+    double q = 0.0;
+    // slots below s_lo hold rows <= k, finished.  g is the same across a
+    // wavefront, so s_lo is a scalar and whole groups of 4 slots are skipped by
+    // scalar branches (a live group's rows <= k meet zeros of v and w)
+    const int s_lo = (k + 1 - gu + SM_G - 1) / SM_G;
+    if (act) {
+#pragma unroll
+      for (int s0 = 0; s0 < RS_REG; s0 += 4)
+        if (s0 + 3 >= s_lo) {
+#pragma unroll
+          for (int u = 0; u < 4; u++) q = __fma_rn(a[s0 + u], vsh[gu + SM_G * (s0 + u)], q);
+        }
+      for (int s0 = (max(0, s_lo - RS_REG) / 4) * 4; s0 < RS_LDS; s0 += 4) {
+#pragma unroll
+        for (int u = 0; u < 4; u++) q = __fma_rn(slots[s0 + u][tid], vsh[gu + SM_G * (RS_REG + s0 + u)], q);
+      }
+    }
+    qpart[g][c] = q;
+    __syncthreads();
+    double qc = 0.0;
+    if (act) {
+#pragma unroll
+      for (int h = 0; h < SM_G; h++) qc += qpart[h][c];
+      qc *= t;
+    }
+    const double vc = (c < n) ? vsh[c] : 0.0;
+    const double vp = block_sum((g == 0 && act) ? qc * vc : 0.0, red);
+    const double K = 0.5 * t * vp;
+    const double wc = act ? qc - K * vc : 0.0;
+    if (g == 0) wsh[c] = wc;
+    __syncthreads();
+    // A -= v w^T + w v^T as a - (v_r w_c + v_c w_r), rounded products and sum
+    // (no contraction): the sum commutes exactly, so (r, c) and (c, r) agree
+    if (act) {
+#pragma unroll
+      for (int s0 = 0; s0 < RS_REG; s0 += 4)
+        if (s0 + 3 >= s_lo) {
+#pragma unroll
+          for (int u = 0; u < 4; u++) {
+            const int r = gu + SM_G * (s0 + u);
+            a[s0 + u] = __dsub_rn(a[s0 + u], __dadd_rn(__dmul_rn(vsh[r], wc), __dmul_rn(vc, wsh[r])));
+          }
+        }
+      for (int s0 = (max(0, s_lo - RS_REG) / 4) * 4; s0 < RS_LDS; s0 += 4) {
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          const int r = gu + SM_G * (RS_REG + s0 + u);
+          slots[s0 + u][tid] =
+              __dsub_rn(slots[s0 + u][tid], __dadd_rn(__dmul_rn(vsh[r], wc), __dmul_rn(vc, wsh[r])));
+        }
+      }
+    }
+  }
+  // the trailing 2 x 2 (or 1 x 1) block from its owners
+  if (c < n) {
+    if (n >= 2 && g == (n - 2) % SM_G) {
+      const double x = rs_get(a, slots, (n - 2) / SM_G, tid);   // element (n-2, c)
+      if (c == n - 2) d[n - 2] = x;
+      if (c == n - 1) e[n - 2] = x;
+    }
+    if (g == (n - 1) % SM_G && c == n - 1) d[n - 1] = rs_get(a, slots, (n - 1) / SM_G, tid);
+  }
+  if (tid == 0) {
+    e[n - 1] = 0.0;
+    for (int k = (n >= 2 ? n - 2 : 0); k < n; k++) tau[k] = 0.0;
+  }
+}
+
 __global__ __launch_bounds__(SM_T) void tridiag_small_kernel(const E2Block* __restrict__ blks,
                                                               const int* __restrict__ list, double* __restrict__ A0,
                                                               double* __restrict__ WS) {
@@ -68,6 +198,7 @@ __global__ __launch_bounds__(SM_T) void tridiag_small_kernel(const E2Block* __re
   __shared__ double red[SM_T / WAVE];
   __shared__ double vsh[256], wsh[256], vnew[256], ash[256];
   __shared__ double qpart[SM_G][256];
+  __shared__ double slots[RS_LDS][SM_T];
   const E2Block B = blks[list[blockIdx.x]];
   const int n = B.n, tid = threadIdx.x;
   const int c = tid & 255, g = tid >> 8;
@@ -75,6 +206,10 @@ __global__ __launch_bounds__(SM_T) void tridiag_small_kernel(const E2Block* __re
   double* d = WS + B.v_off;
   double* e = d + n;
   double* tau = e + n;
+  if (n <= RS_MAXN) {
+    tridiag_resident(n, A, d, e, tau, red, ash, vsh, wsh, qpart, slots);
+    return;
+  }
   bool pending = false;   // vsh/wsh hold the update of the previous column
   double vi_old = 0.0, wi_old = 0.0;
   if (g == 0) { vsh[c] = 0.0; wsh[c] = 0.0; }

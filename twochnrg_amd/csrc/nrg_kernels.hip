@@ -15,17 +15,14 @@
 #include <stdint.h>
 
 #include "nrg_kernels.h"
+#include "nrg_wave.h"
 
 #define EIG_THREADS 256
 #define WAVE 64
 
 namespace nrgk {
 
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int o = WAVE / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
-  return v;
-}
+__device__ __forceinline__ double wave_sum(double v) { return nrgw::wave_sum(v); }
 
 // block-wide sum over EIG_THREADS threads; result broadcast to all threads
 __device__ __forceinline__ double block_sum(double v, double* red) {

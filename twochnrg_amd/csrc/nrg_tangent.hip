@@ -14,6 +14,7 @@
 #include <stdint.h>
 
 #include "nrg_kernels.h"
+#include "nrg_wave.h"
 
 #define WAVE 64
 
@@ -79,8 +80,7 @@ __global__ __launch_bounds__(256) void tan_diag_kernel(const TanBlk* __restrict_
     const double* z = Z + B.zoff + (int64_t)row * B.n;
     v = 0.0;
     for (int j = lane; j < B.n; j += WAVE) v += y[j] * z[j];
-#pragma unroll
-    for (int o = WAVE / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
+    v = nrgw::wave_sum(v);
   }
   if (lane == 0) dE[t * e_stride + B.e0 + row] = v;
 }
