@@ -517,6 +517,22 @@ static double calc_entropy(double bb, const CNRGarray& A) {
   }
   return bb * (s / Z) + std::log(Z);
 }
+// calc_suscep and calc_entropy in one pass: each Boltzmann factor is evaluated
+// once and feeds both sets of sums, which are accumulated in the same order as
+// in the two functions (so the results are theirs, bit for bit)
+static void calc_suscep_entropy(double bb, const CNRGarray& A, double& sus, double& ent) {
+  double s0 = 0, s1 = 0, Z = 0, s = 0;
+  for (int b = 0; b < A.NumBlocks(); b++) {
+    const double aS = 2.0 * A.GetQNumber(b, 1) + 1.0, szsq = (1.0 / 12.0) * aS * (aS * aS - 1.0);
+    for (int i = A.GetBlockLimit(b, 0); i <= A.GetBlockLimit(b, 1); i++) {
+      const double E = A.dEn[i], x = std::exp(-bb * E);
+      s0 += aS * x; s1 += szsq * x;
+      s += aS * E * x; Z += aS * x;
+    }
+  }
+  sus = s1 / s0;
+  ent = bb * (s / Z) + std::log(Z);
+}
 
 // operator update specification
 enum PairRule { PAIR_DIAG = 0, PAIR_QS_UP = 1, PAIR_QS_UP_UPPER = 2, PAIR_QSZ_UP = 3 };
@@ -938,29 +954,49 @@ struct Chain {
     return make_basis_dev(c, AeigCut, (tag + "oldb").c_str(), oldb);
   }
 
-  // CalcOpAvg (src/CalcOpAvg.cpp:13-84) on AeigCut for the operators with CalcAvg
+  // CalcOpAvg (src/CalcOpAvg.cpp:13-84) on AeigCut for the operators with CalcAvg.
+  // Only the diagonals of the diagonal blocks enter; they are gathered on the
+  // device for every such operator and come back in one transfer.
   int op_avgs(double bbar, std::vector<double>& out) {
     out.clear();
-    for (size_t m = 0; m < mats.size(); m++) {
-      if (!mats[m].CalcAvg) continue;
-      const CNRGmatrix& M = mats[m];
-      int64_t tot = M.MatBlockBegEnd.empty() ? 0 : M.MatBlockBegEnd.back() + 1;
-      std::vector<double> h(tot > 0 ? tot : 1);
-      if (tot > 0) {
-        HIPCHK(hipMemcpyAsync(h.data(), c->buf(M.buf[M.cur]).p, sizeof(double) * tot, hipMemcpyDeviceToHost,
-                              c->stream));
-        HIPCHK(hipStreamSynchronize(c->stream));
+    const int nb = AeigCut.NumBlocks();
+    const int64_t ns = AeigCut.dEn.empty() ? 0 : (int64_t)AeigCut.dEn.size();
+    std::vector<int> avg;
+    for (size_t m = 0; m < mats.size(); m++)
+      if (mats[m].CalcAvg) avg.push_back((int)m);
+    if (avg.empty()) return 0;
+    std::vector<DiagJob> jobs;
+    std::vector<unsigned char> has((size_t)avg.size() * nb, 0);
+    for (size_t a = 0; a < avg.size(); a++) {
+      const CNRGmatrix& M = mats[avg[a]];
+      const double* base = (const double*)c->buf(M.buf[M.cur]).p;
+      for (size_t q = 0; q + 1 < M.MatBlockMap.size(); q += 2) {
+        const int b = M.MatBlockMap[q];
+        if (b != M.MatBlockMap[q + 1] || b < 0 || b >= nb || has[a * nb + b]) continue;
+        has[a * nb + b] = 1;   // the first (b, b) entry of the map, as a linear search finds it
+        jobs.push_back(DiagJob{base + M.MatBlockBegEnd[q], (int64_t)a * ns + AeigCut.GetBlockLimit(b, 0),
+                               AeigCut.GetBlockSize(b), 0});
       }
+    }
+    std::vector<double> h((size_t)avg.size() * ns + 1, 0.0);
+    if (!jobs.empty() && ns > 0) {
+      DiagJob* dj;
+      int rc = upload(c, "opavg_jobs", jobs, &dj);
+      if (rc) return rc;
+      DBuf& db = c->buf("opavg_diag");
+      if ((rc = db.reserve(sizeof(double) * h.size()))) return rc;
+      HIPCHK(nrgk_launch_diag_gather(dj, (int)jobs.size(), (double*)db.p, c->stream));
+      HIPCHK(hipMemcpyAsync(h.data(), db.p, sizeof(double) * (h.size() - 1), hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(hipStreamSynchronize(c->stream));
+    }
+    for (size_t a = 0; a < avg.size(); a++) {
       double s = 0, Z = 0;
-      for (int b = 0; b < AeigCut.NumBlocks(); b++) {
-        double dg = 2.0 * AeigCut.GetQNumber(b, 1) + 1.0;
-        int n = AeigCut.GetBlockSize(b);
-        int64_t off = -1;
-        for (size_t q = 0; q + 1 < M.MatBlockMap.size(); q += 2)
-          if (M.MatBlockMap[q] == b && M.MatBlockMap[q + 1] == b) { off = M.MatBlockBegEnd[q]; break; }
+      for (int b = 0; b < nb; b++) {
+        const double dg = 2.0 * AeigCut.GetQNumber(b, 1) + 1.0;
+        const int n = AeigCut.GetBlockSize(b), i0 = AeigCut.GetBlockLimit(b, 0);
         for (int i = 0; i < n; i++) {
-          double x = std::exp(-bbar * AeigCut.dEn[AeigCut.GetBlockLimit(b, 0) + i]);
-          double el = off >= 0 ? h[off + (int64_t)i * n + i] : 0.0;
+          const double x = std::exp(-bbar * AeigCut.dEn[i0 + i]);
+          const double el = has[a * nb + b] ? h[(size_t)a * ns + i0 + i] : 0.0;
           s += dg * el * x; Z += dg * x;
         }
       }
@@ -1726,8 +1762,7 @@ struct OneChRun {
       double bN = bbar * std::pow(Lambda, (nc - 4) / 8.0);
       bN4[nc - 1] = bN;
       thT[nc - 1] = DN / bN;
-      sus[nc - 1] = calc_suscep(bN, ch.Aeig);
-      ent[nc - 1] = calc_entropy(bN, ch.Aeig);
+      calc_suscep_entropy(bN, ch.Aeig, sus[nc - 1], ent[nc - 1]);
     }
     if (tg) tg->thermo(ch.Aeig, bN4);
     // UpdateBefCut 1: the operators are rotated over every eigenstate of the copy

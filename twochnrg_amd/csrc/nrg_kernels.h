@@ -112,7 +112,15 @@ struct TanCut {
   int64_t coff;              // k x (n-k)
 };
 
+// the diagonal of one n x n operator block, to out[dst .. dst+n)
+struct DiagJob {
+  const double* src;
+  int64_t dst;
+  int n, pad;
+};
+
 extern "C" {
+hipError_t nrgk_launch_diag_gather(const DiagJob* d_jobs, int njobs, double* out, hipStream_t s);
 hipError_t nrgk_launch_tan_hn(const TanHnArgs* a, int nb, int nmax, int ntan, hipStream_t s);
 hipError_t nrgk_launch_tan_diag(const TanBlk* d_blk, int nb, int nmax, int ntan, const double* Z, const double* Y,
                                 int64_t y_stride, const double* G, int64_t g_stride, double* dE, int64_t e_stride,

@@ -351,7 +351,19 @@ __global__ void compact_rows_kernel(const CompactDesc* __restrict__ cd, const do
 // ---------------------------------------------------------------------------
 // launch wrappers (host)
 // ---------------------------------------------------------------------------
+// diagonals of operator blocks (for averages over the kept states)
+__global__ void diag_gather_kernel(const DiagJob* __restrict__ jobs, double* __restrict__ out) {
+  const DiagJob J = jobs[blockIdx.x];
+  for (int i = threadIdx.x; i < J.n; i += blockDim.x) out[J.dst + i] = J.src[(size_t)i * J.n + i];
+}
+
 extern "C" {
+
+hipError_t nrgk_launch_diag_gather(const DiagJob* d_jobs, int njobs, double* out, hipStream_t s) {
+  if (njobs <= 0) return hipSuccess;
+  hipLaunchKernelGGL(diag_gather_kernel, dim3(njobs), dim3(256), 0, s, d_jobs, out);
+  return hipGetLastError();
+}
 
 hipError_t nrgk_launch_eig(const EigDesc* d_desc, const int* d_order, int nb, int nmax, double* A,
                            double* W, double* WS, int* d_info, hipStream_t s) {
