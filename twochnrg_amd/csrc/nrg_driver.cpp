@@ -1727,8 +1727,10 @@ struct OneChRun {
            const unsigned char* fixed = nullptr) {
     int rc;
     if ((rc = post_cut(N, tg, fixed))) return rc;
-    // the deferred eigenvectors: every row with UpdateBefCut 1, else the kept ones
+    // the deferred eigenvectors: every row with UpdateBefCut 1, else the kept
+    // ones; the host work that does not decide them runs while they are formed
     if ((rc = ch.finish_vectors(befcut != 0))) return rc;
+    if ((rc = post_thermo(N, tg))) return rc;
     return post_rest(N, cb, user, dm, tg);
   }
 
@@ -1757,6 +1759,19 @@ struct OneChRun {
       for (int i = 0; i < ns; i++) { ch.Aeig.Kept[i] = ch.Aeig.dEn[i] <= ecut; nkept += ch.Aeig.Kept[i]; }
     }
     if (tg && (rc = tg->diag(ch))) return rc;
+    // UpdateBefCut 1: the operators are rotated over every eigenstate of the copy
+    // AeigCut = Aeig with its Kept flags (CutStates with 2 Nstates), SaveArrays
+    // writes that copy, and the real cut follows CalcStuff; the operators stay on
+    // the pre-cut blocks, which the next basis indexes through StCameFrom
+    ch.cut_states(befcut ? 2 * ns : p.ncut, false, p.myeps, fixed);
+    s_DN = DN; s_emin = emin; s_ecut = ecut; s_nkept = nkept;
+    return 0;
+  }
+
+  // CalcThermo at the 4 betabar of the step: host work that the eigenvector
+  // formation does not wait for
+  int post_thermo(int N, Tangent* tg) {
+    const double DN = s_DN;
     double thT[4], sus[4], ent[4], bN4[4];
     for (int nc = 1; nc <= 4; nc++) {
       double bN = bbar * std::pow(Lambda, (nc - 4) / 8.0);
@@ -1765,13 +1780,8 @@ struct OneChRun {
       calc_suscep_entropy(bN, ch.Aeig, sus[nc - 1], ent[nc - 1]);
     }
     if (tg) tg->thermo(ch.Aeig, bN4);
-    // UpdateBefCut 1: the operators are rotated over every eigenstate of the copy
-    // AeigCut = Aeig with its Kept flags (CutStates with 2 Nstates), SaveArrays
-    // writes that copy, and the real cut follows CalcStuff; the operators stay on
-    // the pre-cut blocks, which the next basis indexes through StCameFrom
-    ch.cut_states(befcut ? 2 * ns : p.ncut, false, p.myeps, fixed);
-    s_DN = DN; s_emin = emin; s_ecut = ecut; s_nkept = nkept;
     for (int q = 0; q < 4; q++) { s_thT[q] = thT[q]; s_sus[q] = sus[q]; s_ent[q] = ent[q]; }
+    (void)N;
     return 0;
   }
 
@@ -1922,8 +1932,10 @@ static int run_onechqs_multi(nrg_ctx* c, int ninst, const nrg_onechqs_params* ps
       base += R[i]->ch.Abasis.NumBlocks();
     }
     if (defer && (rc = eig_finish(c, krow.data()))) return rc;
-    for (int i = 0; i < ninst; i++)
+    for (int i = 0; i < ninst; i++) {
+      if ((rc = R[i]->post_thermo(N, nullptr))) return rc;
       if ((rc = R[i]->post_rest(N, multi_tramp, &cbc[i], nullptr, nullptr))) return rc;
+    }
   }
   HIPCHK(hipStreamSynchronize(c->stream));
   auto t1 = std::chrono::steady_clock::now();
