@@ -1102,6 +1102,10 @@ __global__ __launch_bounds__(E2_T) void dc_secular_kernel(const E2Block* __restr
       }
     }
     double tn = ok ? tau + s : 0.5 * (tlo + thi);
+    // a model step below the resolution of tau is convergence: tested before
+    // the bracket check, which rejects tn == tau once the bracket has closed on
+    // tau and would send the iterate back to the bracket's far half
+    if (ok && fabs(tn - tau) <= 2.0 * eps * fabs(tau)) { tau = tn; break; }
     if (!(tn > tlo && tn < thi)) tn = 0.5 * (tlo + thi);
     const bool done = fabs(tn - tau) <= 2.0 * eps * fabs(tn) || (thi - tlo) <= 4.0 * eps * fmax(fabs(tlo), fabs(thi));
     tau = tn;
