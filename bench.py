@@ -375,6 +375,39 @@ def hbm_probe_child():
     return json.loads(r.stdout.strip().splitlines()[-1])
 
 
+def symv_probe(n=4419, reps=2, copy_bytes=1 << 30):
+    """For hardware-counter runs of the reduction's y = A v kernel: one random
+    n x n symmetric eigensolve through the library (reduction on one stream),
+    then a device-to-device copy of copy_bytes through the HIP runtime (the
+    counters' calibration: its kernel reads and writes copy_bytes).  Prints the
+    algorithmic bytes of the average symv launch of the solve: the lower
+    triangle of the trailing matrix, 4 m (m + 1) bytes for m = n - k - 1, over
+    the columns k = 0 .. n - 3."""
+    import ctypes
+    import numpy as np
+    import twochnrg_amd as t
+    rng = np.random.default_rng(0)
+    a = rng.standard_normal((n, n))
+    a = (a + a.T) / 2
+    with t.Context(0) as ctx:
+        ctx.set_red_streams(1)
+        for _ in range(reps):
+            ctx.diag_blocks([a])
+    hip = ctypes.CDLL("libamdhip64.so.7")
+    x, y = ctypes.c_void_p(), ctypes.c_void_p()
+    hip.hipMalloc(ctypes.byref(x), ctypes.c_size_t(copy_bytes))
+    hip.hipMalloc(ctypes.byref(y), ctypes.c_size_t(copy_bytes))
+    hip.hipMemset(x, 0, ctypes.c_size_t(copy_bytes))
+    hip.hipMemcpy(y, x, ctypes.c_size_t(copy_bytes), 3)   # device to device
+    hip.hipDeviceSynchronize()
+    hip.hipFree(x)
+    hip.hipFree(y)
+    cols = range(0, n - 2)
+    alg = sum(4.0 * (n - k - 1) * (n - k) for k in cols) / len(cols)
+    return {"n": n, "solves": reps, "symv_launches_per_solve": len(cols), "alg_bytes_per_launch": alg,
+            "copy_bytes": copy_bytes}
+
+
 def hbm_probe(nbytes=2 << 30, reps=10):
     """STREAM-like calibration of the nominal HBM peak on this device: a 2 GiB
     FP64 copy (torch's copy kernel, read + write) and a 2 GiB read (torch.sum),
@@ -540,9 +573,14 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=2,
                     help="steps of the plain one-block-per-launch comparison run (0 = skip)")
     ap.add_argument("--hbm-probe", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--symv-probe", action="store_true",
+                    help="one 4419-row eigensolve and a 1 GiB copy, for rocprofv3 --pmc runs of tdb_symv_kernel")
     a = ap.parse_args()
     if a.hbm_probe:
         print(json.dumps(hbm_probe()))
+        return
+    if a.symv_probe:
+        print(json.dumps(symv_probe()))
         return
     if a.gpus != 1:
         print("bench.py: this build measures one GPU only (the NRG step and its batched blocks run on one "
