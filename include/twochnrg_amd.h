@@ -90,6 +90,21 @@ int nrg_diag_blocks_dev(nrg_ctx *ctx, int nb, const int *n, const int64_t *a_off
  */
 int nrg_truncate(int nstates, double *en, int ncut, double myeps, uint8_t *kept, double *ecut, int *nkept);
 
+/*
+ * CutStates' eigenvector compaction (src/CutStates.cpp:130-215: RemoveBlock
+ * and CNRGarray::RemoveStatesFromBlock, src/NRGarrayClass.cpp:1467-1548).
+ * Block b of dZ holds n[b] eigenvector rows of length n[b] at z_off[b] (the
+ * dEigVec layout, row i = eigenvector i, energies ascending within the block);
+ * its first keep[b] rows (the kept states of an energy-ordered block are its
+ * lowest) are copied, in block order, to dZcut; blocks with keep[b] = 0 are
+ * dropped.  zcut_off[b] (may be NULL) receives the block's offset in dZcut, or
+ * -1 when dropped, *zcut_len (may be NULL) the total length.  dZ and dZcut are
+ * device arrays that must not overlap; dZcut = NULL only reports the offsets.
+ * Synchronous on the context's stream.
+ */
+int nrg_compact(nrg_ctx *ctx, int nb, const int *n, const int64_t *z_off, const int *keep, const double *dZ,
+                double *dZcut, int64_t *zcut_off, int64_t *zcut_len);
+
 /* CNRGarray::Ecut alone (src/NRGarrayClass.cpp:661-691). */
 int nrg_ecut(int nstates, const double *en, int ncut, double *ecut);
 

@@ -18,7 +18,7 @@ EXPORTS = [
     "nrg_ctx_create", "nrg_ctx_destroy", "nrg_last_error", "nrg_device_count",
     "nrg_diag_blocks", "nrg_diag_blocks_dev", "nrg_truncate", "nrg_ecut",
     "nrg_rotate_ops", "nrg_run_onechqs", "nrg_run_twochqs", "nrg_set_timing", "nrg_get_stats", "nrg_reset_stats",
-    "nrg_set_eig_impl", "nrg_set_plain", "nrg_set_red_streams", "nrg_eig2_stages",
+    "nrg_set_eig_impl", "nrg_set_plain", "nrg_set_red_streams", "nrg_eig2_stages", "nrg_compact",
     "nrg_dm_create", "nrg_dm_destroy", "nrg_dm_info", "nrg_dm_timing", "nrg_dm_memory", "nrg_dm_rho", "nrg_dm_spec",
     "nrg_dm_spectrum", "nrg_run_onechqs_multi", "nrg_dm_spec_omega", "nrg_dm_norm", "nrg_dm_weights",
     "nrg_run_twochqsz", "nrg_onechqs_thermo", "nrg_onechqs_jvp", "nrg_dm_create_jvp", "nrg_dm_spec_jvp",
@@ -152,6 +152,9 @@ def lib():
     L.nrg_set_eig_impl.argtypes = [ctypes.c_void_p, ctypes.c_int]
     L.nrg_set_plain.argtypes = [ctypes.c_void_p, ctypes.c_int]
     L.nrg_set_red_streams.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    L.nrg_compact.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                              ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int), ctypes.c_void_p,
+                              ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]
     L.nrg_eig2_stages.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
     L.nrg_get_stats.argtypes = [ctypes.c_void_p, ctypes.POINTER(NrgStats)]
     L.nrg_reset_stats.argtypes = [ctypes.c_void_p]
@@ -271,6 +274,24 @@ class Context:
         """Streams (block groups) of the blocked Householder reduction, 1..4
         (default 3); the results do not depend on it."""
         _check(lib().nrg_set_red_streams(self.h, int(n)))
+
+    def compact(self, n, z_off, keep, dZ=None, dZcut=None):
+        """CutStates' eigenvector compaction on device arrays (nrg_compact):
+        the first keep[b] rows of each n[b] x n[b] block at z_off[b] of dZ, in
+        block order, to dZcut (device addresses as ints; dZcut None only sizes
+        the result).  Returns (offset of each block in dZcut or -1, length)."""
+        n = np.ascontiguousarray(n, dtype=np.int32)
+        z_off = np.ascontiguousarray(z_off, dtype=np.int64)
+        keep = np.ascontiguousarray(keep, dtype=np.int32)
+        nb = len(n)
+        off = np.zeros(max(nb, 1), dtype=np.int64)
+        tot = ctypes.c_int64(0)
+        _check(lib().nrg_compact(self.h, nb, n.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                 z_off.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                                 keep.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                 ctypes.c_void_p(dZ) if dZ else None, ctypes.c_void_p(dZcut) if dZcut else None,
+                                 off.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), ctypes.byref(tot)))
+        return off[:nb], int(tot.value)
 
     def set_timing(self, on=True):
         _check(lib().nrg_set_timing(self.h, int(on)))

@@ -463,6 +463,36 @@ extern "C" int nrg_truncate(int nstates, double* en, int ncut, double myeps, uin
   return NRG_OK;
 }
 
+extern "C" int nrg_compact(nrg_ctx* c, int nb, const int* n, const int64_t* z_off, const int* keep, const double* dZ,
+                           double* dZcut, int64_t* zcut_off, int64_t* zcut_len) {
+  if (!c || nb < 0 || (nb > 0 && (!n || !z_off || !keep))) return fail(NRG_ERR_ARG, "nrg_compact: bad arguments");
+  std::vector<CompactDesc> cd;
+  int64_t off = 0, maxel = 0;
+  for (int b = 0; b < nb; b++) {
+    if (n[b] < 0 || keep[b] < 0 || keep[b] > n[b] || z_off[b] < 0)
+      return fail(NRG_ERR_ARG, "nrg_compact: block " + std::to_string(b) + " has bad sizes");
+    if (keep[b] == 0) {
+      if (zcut_off) zcut_off[b] = -1;
+      continue;
+    }
+    if (zcut_off) zcut_off[b] = off;
+    cd.push_back(CompactDesc{keep[b], n[b], z_off[b], off});
+    const int64_t el = (int64_t)keep[b] * n[b];
+    off += el;
+    maxel = std::max(maxel, el);
+  }
+  if (zcut_len) *zcut_len = off;
+  if (!dZcut || cd.empty()) return NRG_OK;
+  if (!dZ) return fail(NRG_ERR_ARG, "nrg_compact: dZ is NULL");
+  HIPCHK(hipSetDevice(c->device));
+  CompactDesc* d_cd;
+  int rc = upload(c, "compact_desc", cd, &d_cd);
+  if (rc) return rc;
+  HIPCHK(nrgk_launch_compact(d_cd, (int)cd.size(), maxel, dZ, dZcut, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return NRG_OK;
+}
+
 // ---------------------------------------------------------------------------
 // rotations: out = Zi * F * Zj^T as two NT GEMMs through auxT = Zj * F^T
 // ---------------------------------------------------------------------------
