@@ -432,6 +432,57 @@ def hbm_probe(nbytes=2 << 30, reps=10):
     return out
 
 
+def twochqs_error(name):
+    """Errors of the device eigensolve at the timed size, outside the timed
+    region: the workload's first truncated step (N = 3: 44-45k multiplets in
+    ~53 blocks of up to ~4300 rows, the sizes of the timed steps) run again
+    with its H_N blocks recorded.  Per block the residual max|H z - E z| /
+    max|H| over every eigenvector (for symmetric H it bounds the eigenvalue
+    error), the orthogonality max|Z Z^T - 1| of the largest block, and the
+    eigenvalues against numpy's eigvalsh for blocks up to 1500 rows."""
+    import numpy as np
+    import twochnrg_amd as t
+    model, ramp, w = _params(WORKLOADS[name])
+    if model != "twochqs":
+        return None
+    rec = {}
+
+    def on_step(st):
+        d = t.step_to_dict(st)
+        if d["N"] == 3:
+            rec.update(d)
+
+    t0 = time.perf_counter()
+    with t.Context(0) as ctx:
+        ctx.run_twochqs(nsitesmax=4, record_blocks=True, keep_steps=False, on_step=on_step, **w)
+    sizes = (rec["bbe"][:, 1] - rec["bbe"][:, 0] + 1).astype(int)
+    hoff = np.concatenate([[0], np.cumsum(sizes * (sizes + 1) // 2)])
+    zoff = np.concatenate([[0], np.cumsum(sizes * sizes)])
+    res = eig = 0.0
+    orth = None
+    for b in np.argsort(-sizes):
+        n = int(sizes[b])
+        A = np.zeros((n, n))
+        A[np.triu_indices(n)] = rec["hpacked"][hoff[b]:hoff[b + 1]]
+        H = A + np.triu(A, 1).T
+        Z = rec["evec"][zoff[b]:zoff[b + 1]].reshape(n, n)
+        i0 = int(rec["bbe"][b, 0])
+        E = rec["en"][i0:i0 + n] + rec["emin"]
+        scale = max(np.abs(H).max(), 1e-300)
+        res = max(res, float(np.abs(H @ Z.T - Z.T * E[None, :]).max() / scale))
+        if orth is None:
+            orth = float(np.abs(Z @ Z.T - np.eye(n)).max())
+        if n <= 1500:
+            wl = np.linalg.eigvalsh(H)
+            eig = max(eig, float(np.max(np.abs(np.sort(E) - wl) / np.maximum(np.abs(wl), 1.0))))
+    out = {"step": 3, "blocks": int(len(sizes)), "max_block": int(sizes.max()), "states": int(sizes.sum()),
+           "max_residual_rel": res, "orthogonality_largest_block": orth,
+           "max_eigenvalue_rel_err_vs_lapack_n_le_1500": eig, "wall_s": time.perf_counter() - t0}
+    print("bench: error check at N=3: residual %.2e, orthogonality %.2e, eigenvalues %.2e (%.1f s)"
+          % (res, orth, eig, out["wall_s"]), file=sys.stderr, flush=True)
+    return out
+
+
 def _host_cpu():
     """The host's CPU model (/proc/cpuinfo, as lscpu reports it), the logical
     CPU count and the CPUs this process may run on."""
@@ -568,6 +619,8 @@ def main():
                     help="run only the two-channel FDM leg and print its JSON object")
     ap.add_argument("--adjoint-only", action="store_true",
                     help="run only the derivative-pass leg (device part) and print its JSON object")
+    ap.add_argument("--no-error", action="store_true",
+                    help="skip the eigensolve error check at the timed size (a recorded first truncated step)")
     ap.add_argument("--stats-all", action="store_true",
                     help="kernel statistics over the whole run (ramp-up included), for counter runs")
     ap.add_argument("--plain-steps", type=int, default=2,
@@ -645,6 +698,8 @@ def main():
         # operator rotations Z_i F Z_j^T (FP64 MFMA tile kernel): algorithmic flops / HIP-event time
         out["rotation"] = {"ms_per_step": gm["ms"] / nsteps, "tflops": gm["flops"] / (gm["ms"] * 1e-3) / 1e12,
                            "frac_fp64_peak": gm["flops"] / (gm["ms"] * 1e-3) / 1e12 / FP64_PEAK_TFLOPS}
+    if model == "twochqs" and not a.no_error:
+        out["error_vs_fp64"] = twochqs_error(a.workload)
     cal = hbm_probe_child()
     roof["measured_hbm"] = cal
     if roof["unit"] == "GB/s" and "read_GBs" in cal:
