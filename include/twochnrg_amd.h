@@ -69,6 +69,16 @@ int nrg_set_plain(nrg_ctx *ctx, int on);
  * not depend on n).  1..4, default 3. */
 int nrg_set_red_streams(nrg_ctx *ctx, int n);
 
+/* Two-stage reduction (impl 2) of blocks with n >= min_n (0: none, the
+ * default): dense -> band of width 16 (TSQR panels and two GEMMs per panel)
+ * -> tridiagonal (pipelined bulge chase in LDS), and a grouped compact-WY
+ * back-transformation of the chase before that of the panels.  Replaces the
+ * one-column-at-a-time reduction of those blocks inside DiagBlock's
+ * eigensolve (reference src/NRGmatrixClass.cpp:566-696, dsyev_/dspev_);
+ * results agree with the one-stage path to rounding.  Blocks above
+ * 16384 + 16 rows keep the one-stage reduction. */
+int nrg_set_two_stage(nrg_ctx *ctx, int min_n);
+
 /* Diagnostics: run the batched solver (impl 2) on one n x n matrix and return
  * the stage outputs -- the tear-modified tridiagonal diagonal d (after the
  * divide-and-conquer set-up), the untorn off-diagonal e, the sorted

@@ -1288,17 +1288,20 @@ __global__ void dc_copy_kernel(const E2Block* __restrict__ blks, const E2Merge* 
 // T from the Gram matrix G = V V^T (T_ii = tau_i, T[0:i,i] = -tau_i T[0:i,0:i]
 // G[0:i,i]), and each round applies Z[:, k0+1:] -= (Z V^T) T^T V.
 // ---------------------------------------------------------------------------
-// entries k0+1 .. k0+a of reflector row k0+a: left of or on the diagonal, so
-// zero in the reflector; the reduction left other values there
+// the entries of reflector row k0+a left of where its reflector starts: zero
+// in the reflector; the reduction left other values there
 __global__ void bt_zero_kernel(const E2Block* __restrict__ blks, const E2BtPanel* __restrict__ pans,
                                double* __restrict__ A0) {
   const E2BtPanel Pn = pans[blockIdx.x];
   const E2Block B = blks[Pn.blk];
   const int n = B.n;
   double* A = A0 + B.a_off;
+  // one-stage: reflector k0+a starts at column k0+a+1; two-stage (voff = E2_TSB):
+  // reflector k0+a starts at column k0+E2_TSB*(a/E2_TSB+1) (its stage-1 panel's band end)
   for (int idx = threadIdx.x; idx < BTP * BTP; idx += blockDim.x) {
     const int a = idx / BTP, jj = idx % BTP;
-    if (a < Pn.nb && jj < a) A[(size_t)(Pn.k0 + a) * n + Pn.k0 + 1 + jj] = 0.0;
+    const int nz = Pn.voff == 1 ? a : E2_TSB * (a / E2_TSB);
+    if (a < Pn.nb && jj < nz) A[(size_t)(Pn.k0 + a) * n + Pn.k0 + Pn.voff + jj] = 0.0;
   }
 }
 

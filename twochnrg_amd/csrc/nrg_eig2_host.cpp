@@ -48,6 +48,129 @@ static void e2_layout(int nb, const int* n, const int64_t* a_off, const int64_t*
   }
 }
 
+// Stage 1 and stage 2 of the two-stage blocks tsl (enqueued on stream st):
+// per panel step (column k = 16 p of every block still being reduced) the
+// leaf QRs, the root (T, S R), the Y rows and U = Y T, X = A22 U (split-K
+// GEMM), W, and A22 -= Y W^T + W Y^T; then the band, the bulge chase and d, e.
+static int e2s_phase_a(nrg_ctx* c, const std::vector<E2Block>& blk, const std::vector<int>& tsl, const E2Block* d_blk,
+                       double* dA, double* WS, hipStream_t st) {
+  int rc;
+  double* S = (double*)c->buf("e2s_ws").p;
+  double* V2 = (double*)c->buf("e2s_v2").p;
+  int* FL = (int*)c->buf("e2s_fl").p;
+  int nmax = 0;
+  for (int b : tsl) nmax = std::max(nmax, blk[b].n);
+  std::vector<E2TsStep> steps;
+  std::vector<E2TsItem> leaves, tiles;
+  std::vector<E2Gemm> gp;
+  std::vector<E2Tile> gt;
+  struct PStep { int st0, nst, lf0, nlf, tl0, ntl, sp0, st_tiles0, nst_tiles, rp0, rt0, nrt; };
+  std::vector<PStep> ps;
+  for (int k = 0; k + E2_TSB + 1 < nmax; k += E2_TSB) {
+    PStep P{};
+    P.st0 = (int)steps.size(); P.lf0 = (int)leaves.size(); P.tl0 = (int)tiles.size();
+    for (int b : tsl) {
+      const int n = blk[b].n;
+      if (k + E2_TSB + 1 >= n) continue;
+      const int m = n - k - E2_TSB;
+      const int q = std::min(E2_TS_QMAX, (m + E2_TS_LEAF - 1) / E2_TS_LEAF);
+      const int si = (int)steps.size() - P.st0;
+      steps.push_back(E2TsStep{b, k, m, q});
+      for (int i = 0; i < q; i++) leaves.push_back(E2TsItem{si, i});
+      for (int t = 0; t < (m + 63) / 64; t++) tiles.push_back(E2TsItem{si, t});
+    }
+    P.nst = (int)steps.size() - P.st0; P.nlf = (int)leaves.size() - P.lf0; P.ntl = (int)tiles.size() - P.tl0;
+    // X_s = A22[:, chunk s] U[chunk s]
+    P.sp0 = (int)gp.size(); P.st_tiles0 = (int)gt.size();
+    for (int i = 0; i < P.nst; i++) {
+      const E2TsStep& T = steps[P.st0 + i];
+      const E2Block& B = blk[T.blk];
+      const int64_t nn = B.n, r0 = T.k + E2_TSB;
+      const E2TsOff o = e2_ts_off(B.s_off, B.n);
+      const int nk = (T.m + E2_TS_NK - 1) / E2_TS_NK;
+      for (int q = 0; q < nk; q++) {
+        E2Gemm g{};
+        g.dyn = -1; g.alpha = 1.0;
+        g.m = T.m; g.n = E2_TSB; g.k = std::min(E2_TS_NK, T.m - q * E2_TS_NK);
+        g.a_off = B.a_off + r0 * nn + r0 + (int64_t)q * E2_TS_NK; g.as0 = nn; g.as1 = 1;
+        g.b_off = o.u + (int64_t)q * E2_TS_NK * E2_TSB; g.bs0 = E2_TSB; g.bs1 = 1;
+        g.c_off = o.xp + (int64_t)q * nn * E2_TSB; g.cs0 = E2_TSB;
+        const int pi = (int)gp.size() - P.sp0;
+        gp.push_back(g);
+        for (int a = 0; a < (T.m + 63) / 64; a++) gt.push_back(E2Tile{pi, a, 0, 0});
+      }
+    }
+    P.nst_tiles = (int)gt.size() - P.st_tiles0;
+    // A22 -= [Y W] [W Y]^T
+    P.rp0 = (int)gp.size(); P.rt0 = (int)gt.size();
+    for (int i = 0; i < P.nst; i++) {
+      const E2TsStep& T = steps[P.st0 + i];
+      const E2Block& B = blk[T.blk];
+      const int64_t nn = B.n, r0 = T.k + E2_TSB;
+      const E2TsOff o = e2_ts_off(B.s_off, B.n);
+      E2Gemm g{};
+      g.dyn = -1; g.alpha = -1.0; g.accumulate = 1;
+      g.m = T.m; g.n = T.m; g.k = 2 * E2_TSB;
+      g.a_off = o.yw; g.as0 = 2 * E2_TSB; g.as1 = 1;
+      g.b_off = o.wy; g.bs0 = 1; g.bs1 = 2 * E2_TSB;
+      g.c_off = B.a_off + r0 * nn + r0; g.cs0 = nn;
+      const int pi = (int)gp.size() - P.rp0;
+      gp.push_back(g);
+      const int tt = (T.m + 63) / 64;
+      for (int a = 0; a < tt; a++)
+        for (int bb = 0; bb < tt; bb++) gt.push_back(E2Tile{pi, a, bb, 0});
+    }
+    P.nrt = (int)gt.size() - P.rt0;
+    ps.push_back(P);
+  }
+  E2TsStep* d_st;
+  E2TsItem *d_lf, *d_tl;
+  E2Gemm* d_gp;
+  E2Tile* d_gt;
+  int* d_tsl;
+  if ((rc = upload(c, "e2s_st", steps, &d_st))) return rc;
+  if ((rc = upload(c, "e2s_lf", leaves, &d_lf))) return rc;
+  if ((rc = upload(c, "e2s_tl", tiles, &d_tl))) return rc;
+  if ((rc = upload(c, "e2s_gp", gp, &d_gp))) return rc;
+  if ((rc = upload(c, "e2s_gt", gt, &d_gt))) return rc;
+  if ((rc = upload(c, "e2s_tsl", tsl, &d_tsl))) return rc;
+  const int nts = (int)tsl.size();
+  HIPCHK(e2s_launch_init(d_blk, d_tsl, nts, nmax, WS, FL, st));
+  for (const PStep& P : ps) {
+    const E2TsStep* sp = d_st + P.st0;
+    HIPCHK(e2s_launch_leaf(d_blk, sp, d_lf + P.lf0, P.nlf, dA, S, st));
+    HIPCHK(e2s_launch_root(d_blk, sp, P.nst, dA, S, WS, st));
+    HIPCHK(e2s_launch_form(d_blk, sp, d_lf + P.lf0, P.nlf, dA, S, st));
+    HIPCHK(e2_launch_gemm3(d_gp + P.sp0, d_gt + P.st_tiles0, P.nst_tiles, dA, S, S, st));
+    HIPCHK(e2s_launch_fin1(d_blk, sp, d_tl + P.tl0, P.ntl, S, st));
+    HIPCHK(e2s_launch_fin2(d_blk, sp, d_tl + P.tl0, P.ntl, S, st));
+    HIPCHK(e2_launch_gemm3(d_gp + P.rp0, d_gt + P.rt0, P.nrt, S, S, dA, st));
+  }
+  HIPCHK(e2s_launch_band(d_blk, d_tsl, nts, nmax, dA, WS, st));
+  // the chase: G workgroups per block, every workgroup resident (cooperative launch)
+  const int cap = e2s_chase_max_groups();
+  for (int t0 = 0; t0 < nts; t0 += E2_SB_MAXB) {
+    E2SbGrid G{};
+    G.nb = std::min(E2_SB_MAXB, nts - t0);
+    int nbm = 0;
+    for (int i = 0; i < G.nb; i++) {
+      G.blk[i] = tsl[t0 + i];
+      nbm = std::max(nbm, e2_sb_nbatch(blk[tsl[t0 + i]].n));
+    }
+    // batches in flight: a batch trails its predecessor by about 2 P ticks and lasts n/16 + 2 P ticks
+    int want = std::min(nbm, nmax / (2 * E2_SB_P * E2_TSB) + 4);
+    want = std::min(want, 32);
+    if (cap > 0) want = std::min(want, cap / G.nb);
+    G.G = std::max(1, want);
+    if (cap > 0 && G.G * G.nb > cap) return fail(NRG_ERR_HIP, "two-stage chase: too many blocks for one launch");
+    HIPCHK(e2s_launch_chase(d_blk, G, WS, V2, FL, st));
+  }
+  HIPCHK(e2s_launch_tri(d_blk, d_tsl, nts, nmax, WS, st));
+  // the blocks' abort flags, read back in eig2_finish
+  HIPCHK(hipMemcpyAsync(c->e2p.abort_host, FL, sizeof(int) * (size_t)c->e2p.nflag, hipMemcpyDeviceToHost, st));
+  return NRG_OK;
+}
+
 // Phase A on blocks blk (offsets into the whole workspace): Householder
 // reduction to tridiagonal form, divide and conquer, eigenvalues to dW.  The
 // eigenvector matrix of T stays in the workspace (q_off) for phase B.
@@ -58,10 +181,11 @@ static int e2_phase_a(nrg_ctx* c, const std::vector<E2Block>& blk, double* dA, d
   int nmax = 0;
   for (const E2Block& B : blk) nmax = std::max(nmax, B.n);
   // ---- stage 1 lists
-  std::vector<int> small, large;
+  std::vector<int> small, large, tsl;
   for (int b = 0; b < nb; b++) {
     if (blk[b].n <= 0) continue;
-    if (blk[b].n <= kSmall) small.push_back(b);
+    if (blk[b].ts) tsl.push_back(b);
+    else if (blk[b].n <= kSmall) small.push_back(b);
     else large.push_back(b);
   }
   // ---- stage 1b groups: the large blocks split into groups of about equal
@@ -198,15 +322,21 @@ static int e2_phase_a(nrg_ctx* c, const std::vector<E2Block>& blk, double* dA, d
   if ((rc = upload(c, "e2_tt", ttiles, &d_tt))) return rc;
   // ---- launches
   hipStream_t s = c->stream;
-  // the small blocks' single-workgroup reductions run beside the large groups'
-  // chains, on a stream of their own when there are large blocks
-  const int nfork = ngrp > 0 ? (ngrp - 1) + (small.empty() ? 0 : 1) : 0;   // aux streams used below
+  // lanes of concurrent work: the two-stage blocks' chain, the large groups'
+  // column chains, the small blocks' single-workgroup reductions; lane 0 is
+  // the context stream, lane i the auxiliary stream i-1
+  const int lane_ts = tsl.empty() ? -1 : 0;
+  const int lane_g0 = tsl.empty() ? 0 : 1;
+  const int lane_small = small.empty() ? -1 : lane_g0 + ngrp;
+  const int nlanes = lane_g0 + ngrp + (small.empty() ? 0 : 1);
+  const int nfork = std::max(0, nlanes - 1);
   if (nfork > 0 && (rc = c->fork(nfork))) return rc;   // after everything enqueued so far on s
-  const bool small_aside = ngrp > 0 && !small.empty();   // then aux[ngrp - 1] is the small blocks' stream
-  HIPCHK(e2_launch_tridiag_small(d_blk, d_small, (int)small.size(), dA, WS, small_aside ? c->aux[ngrp - 1] : s));
+  auto lane_stream = [&](int l) { return l <= 0 ? s : c->aux[l - 1]; };
+  HIPCHK(e2_launch_tridiag_small(d_blk, d_small, (int)small.size(), dA, WS, lane_stream(lane_small)));
+  if (!tsl.empty() && (rc = e2s_phase_a(c, blk, tsl, d_blk, dA, WS, lane_stream(lane_ts)))) return rc;
   if (ngrp > 0) {
     std::vector<hipStream_t> gs(ngrp, s);
-    for (int g = 1; g < ngrp; g++) gs[g] = c->aux[g - 1];
+    for (int g = 0; g < ngrp; g++) gs[g] = lane_stream(lane_g0 + g);
     std::vector<std::vector<int>> gn(ngrp);
     std::vector<int> nmax_g(ngrp);
     std::vector<size_t> ev(ngrp, 0);
@@ -286,8 +416,12 @@ static int e2_phase_b(nrg_ctx* c, const std::vector<E2Block>& blk, double* dA, d
   const int nb = (int)blk.size();
   std::vector<int> npan(nb, 0), pan_beg(nb, 0);
   int maxP = 0;
+  // reflector columns: one-stage 0..n-3 (stored from column k+1 of row k);
+  // two-stage 0..ncref-1 (stage-1 panels, stored from the end of their band)
+  auto nref_of = [&](int b) { return blk[b].ts ? blk[b].ncref : std::max(0, blk[b].n - 2); };
+  auto voff_of = [&](int b) { return blk[b].ts ? E2_TSB : 1; };
   for (int b : act) {
-    npan[b] = blk[b].n > 2 ? (blk[b].n - 2 + kBtp - 1) / kBtp : 0;
+    npan[b] = (nref_of(b) + kBtp - 1) / kBtp;
     maxP = std::max(maxP, npan[b]);
   }
   // the auxiliary buffer: partial Gram matrices and T^T per panel, then the
@@ -300,8 +434,9 @@ static int e2_phase_b(nrg_ctx* c, const std::vector<E2Block>& blk, double* dA, d
       E2BtPanel P{};
       P.blk = b;
       P.k0 = p * kBtp;
-      P.nb = std::min(kBtp, blk[b].n - 2 - P.k0);
-      P.ns = (blk[b].n - P.k0 - 1 + kBtKsG - 1) / kBtKsG;
+      P.nb = std::min(kBtp, nref_of(b) - P.k0);
+      P.voff = voff_of(b);
+      P.ns = (blk[b].n - P.k0 - P.voff + kBtKsG - 1) / kBtKsG;
       P.g_off = aux;
       aux += (int64_t)P.ns * kBtp * kBtp;
       pans.push_back(P);
@@ -321,12 +456,12 @@ static int e2_phase_b(nrg_ctx* c, const std::vector<E2Block>& blk, double* dA, d
   for (const E2BtPanel& P : pans) {
     const E2Block& B = blk[P.blk];
     const int64_t nn = B.n;
-    const int L = B.n - P.k0 - 1;
+    const int L = B.n - P.k0 - P.voff;
     for (int q = 0; q < P.ns; q++) {
       E2Gemm g{};
       g.dyn = -1; g.alpha = 1.0;
       g.m = P.nb; g.n = P.nb; g.k = std::min(kBtKsG, L - q * kBtKsG);
-      g.a_off = B.a_off + (int64_t)P.k0 * nn + P.k0 + 1 + (int64_t)q * kBtKsG; g.as0 = nn; g.as1 = 1;
+      g.a_off = B.a_off + (int64_t)P.k0 * nn + P.k0 + P.voff + (int64_t)q * kBtKsG; g.as0 = nn; g.as1 = 1;
       g.b_off = g.a_off; g.bs0 = 1; g.bs1 = nn;
       g.c_off = P.g_off + (int64_t)q * kBtp * kBtp; g.cs0 = kBtp;
       gtiles.push_back(E2Tile{(int)gprobs.size(), 0, 0, 0});
@@ -350,13 +485,13 @@ static int e2_phase_b(nrg_ctx* c, const std::vector<E2Block>& blk, double* dA, d
       const E2Block& B = blk[b];
       const E2BtPanel& P = pans[pan_beg[b] + npan[b] - 1 - r];
       const int64_t nn = B.n;
-      const int m = B.krow, L = B.n - P.k0 - 1, ns = (L + kBtKs - 1) / kBtKs;
-      const int64_t vrow = B.a_off + (int64_t)P.k0 * nn + P.k0 + 1;   // V(i, j) = A[vrow + i*n + j]
+      const int m = B.krow, L = B.n - P.k0 - P.voff, ns = (L + kBtKs - 1) / kBtKs;
+      const int64_t vrow = B.a_off + (int64_t)P.k0 * nn + P.k0 + P.voff;   // V(i, j) = A[vrow + i*n + j]
       for (int q = 0; q < ns; q++) {
         E2Gemm g{};
         g.dyn = -1; g.alpha = 1.0;
         g.m = m; g.n = P.nb; g.k = std::min(kBtKs, L - q * kBtKs);
-        g.a_off = B.q_off + P.k0 + 1 + (int64_t)q * kBtKs; g.as0 = nn; g.as1 = 1;
+        g.a_off = B.q_off + P.k0 + P.voff + (int64_t)q * kBtKs; g.as0 = nn; g.as1 = 1;
         g.b_off = vrow + (int64_t)q * kBtKs; g.bs0 = 1; g.bs1 = nn;
         g.c_off = wp_off[b] + (int64_t)q * m * kBtp; g.cs0 = kBtp;
         const int pi = (int)rprobs.size() - pbase;
@@ -374,7 +509,7 @@ static int e2_phase_b(nrg_ctx* c, const std::vector<E2Block>& blk, double* dA, d
       gz.m = m; gz.n = L; gz.k = P.nb;
       gz.a_off = w2_off[b]; gz.as0 = kBtp; gz.as1 = 1;
       gz.b_off = vrow; gz.bs0 = nn; gz.bs1 = 1;
-      gz.c_off = B.q_off + P.k0 + 1; gz.cs0 = nn;
+      gz.c_off = B.q_off + P.k0 + P.voff; gz.cs0 = nn;
       const int pz = (int)rprobs.size() - pbase;
       rprobs.push_back(gz);
       for (int a = 0; a < (m + 63) / 64; a++)
@@ -402,6 +537,44 @@ static int e2_phase_b(nrg_ctx* c, const std::vector<E2Block>& blk, double* dA, d
   if ((rc = daux.reserve(sizeof(double) * (size_t)(aux + 1)))) return rc;
   double* AX = (double*)daux.p;
   hipStream_t s = c->stream;
+  // the bulge-chase reflectors of the two-stage blocks first (x = Q1 Q2 z)
+  {
+    std::vector<int> tsa;
+    for (int b : act)
+      if (blk[b].ts) tsa.push_back(b);
+    const double* V2 = tsa.empty() ? nullptr : (const double*)c->buf("e2s_v2").p;
+    for (size_t t0 = 0; t0 < tsa.size(); t0 += E2_SB_MAXB) {
+      const int nbl = (int)std::min<size_t>(E2_SB_MAXB, tsa.size() - t0);
+      // group g (sweeps 16g..16g+15) at position j runs at tick (ng-1-g) + j
+      int nt = 0;
+      for (int i = 0; i < nbl; i++) {
+        const int n = blk[tsa[t0 + i]].n, ng = (n - 2 + E2_TSB - 1) / E2_TSB;
+        for (int g = 0; g < ng; g++) nt = std::max(nt, (ng - 1 - g) + (n - 1 - E2_TSB * g + E2_TSB - 1) / E2_TSB);
+      }
+      for (int t = 0; t < nt; t++) {
+        E2Q2Grid G{};
+        G.nb = nbl;
+        G.tick = t;
+        G.cum[0] = 0;
+        for (int i = 0; i < nbl; i++) {
+          const E2Block& B = blk[tsa[t0 + i]];
+          const int n = B.n, ng = (n - 2 + E2_TSB - 1) / E2_TSB;
+          int glo = -1, ghi = -2;
+          for (int g = std::max(0, ng - 1 - t); g < ng; g++) {
+            const int j = t - (ng - 1 - g);
+            if (E2_TSB * g + 1 + j * E2_TSB >= n) break;   // r0 past the end: larger g only larger r0
+            if (glo < 0) glo = g;
+            ghi = g;
+          }
+          G.blk[i] = tsa[t0 + i];
+          G.glo[i] = glo < 0 ? 0 : glo;
+          G.nch[i] = (B.krow + 63) / 64;
+          G.cum[i + 1] = G.cum[i] + (glo < 0 ? 0 : (ghi - glo + 1) * G.nch[i]);
+        }
+        HIPCHK(e2s_launch_q2(d_blk, G, WS, V2, s));
+      }
+    }
+  }
   const int np = (int)pans.size();
   HIPCHK(e2_launch_bt_zero(d_blk, d_pan, np, dA, s));
   HIPCHK(e2_launch_gemm3(d_gp, d_gt, (int)gtiles.size(), dA, dA, AX, s));
@@ -425,10 +598,37 @@ int eig2_begin(nrg_ctx* c, int nb, const int* n, const int64_t* a_off, double* d
   if (nb <= 0) return NRG_OK;
   int64_t ws = 0, iw = 0;
   e2_layout(nb, n, a_off, w_off, c->e2p.blk, ws, iw);
+  // two-stage blocks: n >= ts_min (0: none), within the TSQR limits
+  int64_t sws = 0, sv2 = 0, sfl = 0;
+  c->e2p.ts_foff.clear();
+  for (E2Block& B : c->e2p.blk) {
+    B.ts = 0;
+    if (c->ts_min <= 0 || B.n < c->ts_min || B.n < 2 * E2_TSB + 2 ||
+        B.n - E2_TSB > E2_TS_QMAX * E2_TS_LR)
+      continue;
+    B.ts = 1;
+    B.ncref = E2_TSB * ((B.n - E2_TSB - 2) / E2_TSB + 1);   // panels k = 0, 16, .. with k + 17 < n
+    B.s_off = sws; sws += e2_ts_off(0, B.n).tot;
+    B.v2_off = sv2; sv2 += e2_v2_len(B.n);
+    B.f_off = sfl; sfl += 1 + e2_sb_nbatch(B.n);
+    c->e2p.ts_foff.push_back((int)B.f_off);
+  }
   DBuf& dws = c->buf("e2_ws");
   DBuf& diw = c->buf("e2_iw");
   if ((rc = dws.reserve(sizeof(double) * (size_t)(ws + 1)))) return rc;
   if ((rc = diw.reserve(sizeof(int) * (size_t)(iw + 1)))) return rc;
+  c->e2p.nflag = (int)sfl;
+  if (sfl > 0) {
+    if ((rc = c->buf("e2s_ws").reserve(sizeof(double) * (size_t)(sws + 1)))) return rc;
+    if ((rc = c->buf("e2s_v2").reserve(sizeof(double) * (size_t)(sv2 + 1)))) return rc;
+    if ((rc = c->buf("e2s_fl").reserve(sizeof(int) * (size_t)(sfl + 1)))) return rc;
+    if ((size_t)sfl > c->e2p.abort_cap) {
+      if (c->e2p.abort_host) HIPCHK(hipHostFree(c->e2p.abort_host));
+      c->e2p.abort_host = nullptr;
+      HIPCHK(hipHostMalloc((void**)&c->e2p.abort_host, sizeof(int) * (size_t)sfl));
+      c->e2p.abort_cap = (size_t)sfl;
+    }
+  }
   double* WS = (double*)dws.p;
   int* IW = (int*)diw.p;
   if (c->plain) {
@@ -448,6 +648,12 @@ int eig2_begin(nrg_ctx* c, int nb, const int* n, const int64_t* a_off, double* d
 int eig2_finish(nrg_ctx* c, const int* krow) {
   if (!c->e2p.pending) return NRG_OK;
   c->e2p.pending = false;
+  if (!c->e2p.ts_foff.empty()) {
+    // a bulge chase that gave up waiting (its bounded wait ran out) left garbage
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int f : c->e2p.ts_foff)
+      if (c->e2p.abort_host[f]) return fail(NRG_ERR_NUMERIC, "two-stage reduction: bulge chase aborted (wait limit)");
+  }
   std::vector<E2Block>& blk = c->e2p.blk;
   for (size_t b = 0; b < blk.size(); b++)
     blk[b].krow = krow ? std::max(0, std::min(krow[b], blk[b].n)) : blk[b].n;

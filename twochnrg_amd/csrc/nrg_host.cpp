@@ -122,6 +122,7 @@ extern "C" void nrg_ctx_destroy(nrg_ctx* c) {
   for (auto e : c->sync_ev) hipEventDestroy(e);
   c->bufs.clear();
   if (c->ring) hipHostFree(c->ring);
+  if (c->e2p.abort_host) hipHostFree(c->e2p.abort_host);
   hipStreamDestroy(c->stream);
   delete c;
 }
@@ -219,6 +220,13 @@ extern "C" int nrg_set_red_streams(nrg_ctx* c, int n) {
   if (!c) return fail(NRG_ERR_ARG, "nrg_set_red_streams: NULL ctx");
   if (n < 1 || n > 4) return fail(NRG_ERR_ARG, "nrg_set_red_streams: n must be 1..4");
   c->red_streams = n;
+  return NRG_OK;
+}
+
+extern "C" int nrg_set_two_stage(nrg_ctx* c, int min_n) {
+  if (!c) return fail(NRG_ERR_ARG, "nrg_set_two_stage: NULL ctx");
+  if (min_n < 0) return fail(NRG_ERR_ARG, "nrg_set_two_stage: min_n must be >= 0");
+  c->ts_min = min_n;
   return NRG_OK;
 }
 

@@ -67,7 +67,12 @@ struct nrg_ctx {
     bool pending = false;
     std::vector<E2Block> blk;
     double* dA = nullptr;
+    std::vector<int> ts_foff;    // abort-flag slots of the two-stage blocks
+    int nflag = 0;
+    int* abort_host = nullptr;   // pinned copy of the chase flags
+    size_t abort_cap = 0;
   } e2p;
+  int ts_min = 0;      // blocks with n >= ts_min use the two-stage reduction (0: none)
   int dm_seq = 0;      // DM-NRG handles created on this context (buffer name prefixes)
   std::vector<nrg_dm*> dms;   // live DM-NRG handles: detached when the context goes first
   // small host-to-device copies go through a pinned ring on the stream, with no
