@@ -43,7 +43,7 @@ def _params(w):
     return model, ramp, w
 
 
-def gpu_run(name, steps, warmup, eig_impl=2, plain=False, stats_all=False, red_streams=None):
+def gpu_run(name, steps, warmup, eig_impl=2, plain=False, stats_all=False, red_streams=None, two_stage=None):
     import twochnrg_amd as t
     model, ramp, w = _params(WORKLOADS[name])
     first = -1 if model == "onechqs" else 0   # first record emitted
@@ -55,6 +55,8 @@ def gpu_run(name, steps, warmup, eig_impl=2, plain=False, stats_all=False, red_s
     ctx.set_plain(plain)
     if red_streams is not None:
         ctx.set_red_streams(red_streams)
+    if two_stage is not None:
+        ctx.set_two_stage(two_stage)
     ctx.set_timing(True)
     t_start_N = first + ramp + warmup         # stats reset after this step's record
 
@@ -610,6 +612,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="twochqs_c3", choices=sorted(WORKLOADS))
     ap.add_argument("--eig-impl", type=int, default=2)
+    ap.add_argument("--two-stage", type=int, default=None,
+                    help="blocks with n >= this use the two-stage reduction (0: none)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-dm", action="store_true", help="skip the DM-NRG A(omega) leg")
     ap.add_argument("--no-c2", action="store_true", help="skip the 64-instance batched leg")
@@ -645,7 +649,8 @@ def main():
     if a.c5_only:
         print(json.dumps(c5_run(cpu=not a.no_cpu)))
         return
-    dt, st, timed, model, w, nsteps = gpu_run(a.workload, a.steps, a.warmup, a.eig_impl, stats_all=a.stats_all)
+    dt, st, timed, model, w, nsteps = gpu_run(a.workload, a.steps, a.warmup, a.eig_impl, stats_all=a.stats_all,
+                                              two_stage=a.two_stage)
     value = nsteps / dt
     eig = st["eig"]
     eig_rate = eig["flops"] / (eig["ms"] * 1e-3) / 1e12 if eig["ms"] > 0 else 0.0

@@ -34,7 +34,7 @@ struct E2Block {
 #define E2_TS_QMAX 32      // most leaves per panel
 #define E2_TS_NK 512       // K chunk of the split-K product X = A22 (Y T)
 #define E2_SB_P 8          // sweeps per bulge-chase batch (one workgroup, P x 16 threads)
-#define E2_SB_WIN 288      // band columns held by a batch (LDS ring); >= 2 P TSB - P + 1
+#define E2_SB_WIN 256      // band columns held by a batch (LDS ring, a power of 2); >= 2 P TSB - P + 1
 #define E2_SB_LDB 32       // stored diagonals per band column (band + bulge)
 #define E2_SB_MAXB 32      // two-stage blocks per chase launch
 

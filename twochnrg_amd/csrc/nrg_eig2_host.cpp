@@ -568,8 +568,9 @@ static int e2_phase_b(nrg_ctx* c, const std::vector<E2Block>& blk, double* dA, d
           }
           G.blk[i] = tsa[t0 + i];
           G.glo[i] = glo < 0 ? 0 : glo;
-          G.nch[i] = (B.krow + 63) / 64;
-          G.cum[i + 1] = G.cum[i] + (glo < 0 ? 0 : (ghi - glo + 1) * G.nch[i]);
+          G.nch[i] = 1;   // one workgroup per group covers every kept row
+          G.cum[i + 1] = G.cum[i] + (glo < 0 ? 0 : (ghi - glo + 1));
+          (void)B;
         }
         HIPCHK(e2s_launch_q2(d_blk, G, WS, V2, s));
       }

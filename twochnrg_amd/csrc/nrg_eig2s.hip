@@ -54,32 +54,37 @@ __device__ __forceinline__ double sum16(double v) {
   v += dpp_d<0x140>(v);
   return v;
 }
-// lane t of this thread's 16-lane row
+// lane t of this thread's 16-lane row (DPP row_newbcast: a VALU move, no LDS round trip)
 template <int T>
 __device__ __forceinline__ double bc16(double v) {
-  return __shfl(v, T, 16);
+  return dpp_d<0x150 + T>(v);
 }
 
 // ---------------------------------------------------------------------------
-// panel QR in LDS: P (rows x TB, row stride TB+1, rows <= LRW) <- [R; V] in
-// LAPACK form (beta on the diagonal, v below it, v_j = 1 implicit), tau[j]
-// (0 where no reflector).  256 threads: column c = tid / 16, row phase g =
-// tid % 16 (rows r = g (mod 16) of column c); a column's 16 threads are one
-// DPP row, so its sums are row reductions.
+// panel QR in LDS.  The panel is held column-major, P[c][r] (row pitch
+// LRW + 1), so that the 64 lanes of a wavefront read consecutive rows of one
+// column.  1024 threads: wavefront c owns column c (lanes r = lane (mod 64)),
+// so a column's sums are wavefront reductions.
+// qr_cm: P (rows x TB) <- [R; V] in LAPACK form (beta on the diagonal, v
+// below it, v_j = 1 implicit), tau[j] (0 where no reflector).
 // ---------------------------------------------------------------------------
-__device__ void qr_lds(double (*P)[TB + 1], int rows, double* tau_s, double* scal_s, double* beta_s) {
-  const int tid = threadIdx.x, c = tid >> 4, g = tid & 15;
+#define QT 1024
+#define LDP (LRW + 1)
+__device__ __forceinline__ double wsum(double v) { return nrgw::wave_sum(v); }
+
+__device__ void qr_cm(double (*P)[LDP], int rows, double* tau_s, double* scal_s, double* beta_s) {
+  const int tid = threadIdx.x, c = tid >> 6, ln = tid & 63;
   const int nref = rows < TB ? rows : TB;
   for (int j = 0; j < nref; j++) {
     if (c == j) {
-      double s = 0.0;
-      for (int r = j + 1 + g; r < rows; r += 16) s += P[r][j] * P[r][j];
-      s = sum16(s);
-      if (g == 0) {
+      double sg = 0.0;
+      for (int r = j + 1 + ln; r < rows; r += 64) sg += P[j][r] * P[j][r];
+      sg = wsum(sg);
+      if (ln == 0) {
         const double alpha = P[j][j];
         double tau = 0.0, scal = 0.0, beta = alpha;
-        if (s > 0.0) {
-          beta = -copysign(sqrt(alpha * alpha + s), alpha);
+        if (sg > 0.0) {
+          beta = -copysign(sqrt(alpha * alpha + sg), alpha);
           tau = (beta - alpha) / beta;
           scal = 1.0 / (alpha - beta);
         }
@@ -92,43 +97,53 @@ __device__ void qr_lds(double (*P)[TB + 1], int rows, double* tau_s, double* sca
     const double tau = tau_s[j], scal = scal_s[j];
     if (c > j && tau != 0.0) {
       double dot = 0.0;
-      for (int r = j + 1 + g; r < rows; r += 16) dot += P[r][j] * P[r][c];
-      dot = sum16(dot) * scal + P[j][c];
-      const double f = tau * dot;
-      if (g == 0) P[j][c] -= f;
-      const double fs = f * scal;
-      for (int r = j + 1 + g; r < rows; r += 16) P[r][c] -= fs * P[r][j];
+      for (int r = j + 1 + ln; r < rows; r += 64) dot += P[j][r] * P[c][r];
+      dot = wsum(dot) * scal + P[c][j];
+      const double f = tau * dot, fs = f * scal;
+      for (int r = j + 1 + ln; r < rows; r += 64) P[c][r] -= fs * P[j][r];
+      if (ln == 0) P[c][j] -= f;
     }
     __syncthreads();
   }
   // column j below the diagonal still holds x: scale it to v; beta on the diagonal
-  for (int idx = tid; idx < rows * TB; idx += blockDim.x) {
-    const int r = idx / TB, cc = idx % TB;
-    if (cc < nref) {
-      if (r > cc) P[r][cc] *= scal_s[cc];
-      else if (r == cc) P[r][cc] = beta_s[cc];
-    }
+  if (c < nref) {
+    const double scal = scal_s[c];
+    for (int r = c + 1 + ln; r < rows; r += 64) P[c][r] *= scal;
+    if (ln == 0) P[c][c] = beta_s[c];
   }
   if (tid >= nref && tid < TB) tau_s[tid] = 0.0;
   __syncthreads();
 }
 
-// X (rows x TB in LDS) <- H_0 H_1 ... H_{nref-1} X with the reflectors of V
-// (LAPACK form: the diagonal read as 1) and tau; same thread layout
-__device__ void apply_q_lds(const double (*V)[TB + 1], const double* tau_s, double (*X)[TB + 1], int rows) {
-  const int tid = threadIdx.x, c = tid >> 4, g = tid & 15;
+// X (rows x TB, column-major in LDS) <- H_0 H_1 ... H_{nref-1} X with the
+// reflectors of V (LAPACK form: the diagonal read as 1) and tau
+__device__ void apply_q_cm(const double (*V)[LDP], const double* tau_s, double (*X)[LDP], int rows) {
+  const int tid = threadIdx.x, c = tid >> 6, ln = tid & 63;
   const int nref = rows < TB ? rows : TB;
   for (int j = nref - 1; j >= 0; j--) {
     const double tau = tau_s[j];
     if (tau != 0.0) {
       double dot = 0.0;
-      for (int r = j + 1 + g; r < rows; r += 16) dot += V[r][j] * X[r][c];
-      dot = sum16(dot) + X[j][c];
+      for (int r = j + 1 + ln; r < rows; r += 64) dot += V[j][r] * X[c][r];
+      dot = wsum(dot) + X[c][j];
       const double f = tau * dot;
-      if (g == 0) X[j][c] -= f;
-      for (int r = j + 1 + g; r < rows; r += 16) X[r][c] -= f * V[r][j];
+      for (int r = j + 1 + ln; r < rows; r += 64) X[c][r] -= f * V[j][r];
+      if (ln == 0) X[c][j] -= f;
     }
     __syncthreads();
+  }
+}
+
+// row r (r < 16) of T for 16 reflectors with Gram matrix G = V^T V and tau,
+// in registers: T[r][i] = -tau_i sum_{c=r}^{i-1} T[r][c] G[c][i]
+__device__ __forceinline__ void tmat_row(const double (*G)[TB + 1], const double* tau_s, int r, double* trow) {
+#pragma unroll
+  for (int i = 0; i < TB; i++) {
+    double w = 0.0;
+#pragma unroll
+    for (int c = 0; c < i; c++)
+      if (c >= r) w += trow[c] * G[c][i];
+    trow[i] = (i < r) ? 0.0 : (i == r ? tau_s[i] : -tau_s[i] * w);
   }
 }
 
@@ -155,10 +170,10 @@ __global__ void init_kernel(const E2Block* __restrict__ blks, const int* __restr
 
 // one leaf of one panel: Householder QR in LDS; V and R back in place (the
 // panel rows of A), tau and R to the stage workspace
-__global__ __launch_bounds__(256) void leaf_kernel(const E2Block* __restrict__ blks, const E2TsStep* __restrict__ steps,
-                                                   const E2TsItem* __restrict__ leaves, double* __restrict__ A0,
-                                                   double* __restrict__ S) {
-  __shared__ double P[LRW][TB + 1];
+__global__ __launch_bounds__(QT) void leaf_kernel(const E2Block* __restrict__ blks, const E2TsStep* __restrict__ steps,
+                                                  const E2TsItem* __restrict__ leaves, double* __restrict__ A0,
+                                                  double* __restrict__ S) {
+  __shared__ double P[TB][LDP];
   __shared__ double tau_s[TB], scal_s[TB], beta_s[TB];
   const E2TsItem it = leaves[blockIdx.x];
   const E2TsStep st = steps[it.step];
@@ -169,20 +184,20 @@ __global__ __launch_bounds__(256) void leaf_kernel(const E2Block* __restrict__ b
   const int rows = hi - lo;
   double* A = A0 + B.a_off;
   const E2TsOff o = e2_ts_off(B.s_off, n);
-  for (int idx = tid; idx < rows * TB; idx += blockDim.x) {
+  for (int idx = tid; idx < rows * TB; idx += QT) {
     const int r = idx / TB, c = idx % TB;
-    P[r][c] = A[(size_t)(r0 + lo + r) * n + st.k + c];
+    P[c][r] = A[(size_t)(r0 + lo + r) * n + st.k + c];
   }
   __syncthreads();
-  qr_lds(P, rows, tau_s, scal_s, beta_s);
-  for (int idx = tid; idx < rows * TB; idx += blockDim.x) {
+  qr_cm(P, rows, tau_s, scal_s, beta_s);
+  for (int idx = tid; idx < rows * TB; idx += QT) {
     const int r = idx / TB, c = idx % TB;
-    A[(size_t)(r0 + lo + r) * n + st.k + c] = P[r][c];
+    A[(size_t)(r0 + lo + r) * n + st.k + c] = P[c][r];
   }
   if (tid < TB) S[o.lt + it.i * TB + tid] = tau_s[tid];
-  {
-    const int r = tid / TB, c = tid % TB;   // 256 threads = one 16 x 16 R
-    S[o.rs + it.i * 256 + tid] = (r <= c && r < rows) ? P[r][c] : 0.0;
+  if (tid < 256) {
+    const int r = tid / TB, c = tid % TB;   // one 16 x 16 R
+    S[o.rs + it.i * 256 + tid] = (r <= c && r < rows) ? P[c][r] : 0.0;
   }
 }
 
@@ -191,77 +206,74 @@ __global__ __launch_bounds__(256) void leaf_kernel(const E2Block* __restrict__ b
 // R factors, C = Q_root [I; 0], the top of Q1 = Q_leaf0 [C_0; 0], the signed
 // LU of that top (S, U, L), T = -U S L^{-T}, and S R written over the band
 // part of the panel.  tau_j = T_jj goes to the block's tau slots.
-__global__ __launch_bounds__(256) void root_kernel(const E2Block* __restrict__ blks, const E2TsStep* __restrict__ steps,
-                                                   double* __restrict__ A0, double* __restrict__ S,
-                                                   double* __restrict__ WS) {
-  __shared__ double P[LRW][TB + 1];
-  __shared__ double X[LRW][TB + 1];
+__global__ __launch_bounds__(QT) void root_kernel(const E2Block* __restrict__ blks, const E2TsStep* __restrict__ steps,
+                                                  double* __restrict__ A0, double* __restrict__ S,
+                                                  double* __restrict__ WS) {
+  __shared__ double P[TB][LDP];
+  __shared__ double X[TB][LDP];
   __shared__ double tau_s[TB], scal_s[TB], beta_s[TB], sg[TB];
   __shared__ double M[TB][TB + 1], Tm[TB][TB + 1], Li[TB][TB + 1], Rr[TB][TB + 1];
   const E2TsStep st = steps[blockIdx.x];
   const E2Block B = blks[st.blk];
   const int n = B.n, r0 = st.k + TB, tid = threadIdx.x, q = st.q, m = st.m;
-  const int ti = tid / TB, tj = tid % TB;
+  const bool sq = tid < 256;   // the threads of the 16 x 16 work
+  const int ti = (tid / TB) & (TB - 1), tj = tid % TB;
   double* A = A0 + B.a_off;
   const E2TsOff o = e2_ts_off(B.s_off, n);
   double* taucol = WS + B.v_off + 2 * (size_t)n;
   if (q == 1) {
-    for (int idx = tid; idx < m * TB; idx += blockDim.x) {
+    for (int idx = tid; idx < m * TB; idx += QT) {
       const int r = idx / TB, c = idx % TB;
-      P[r][c] = r > c ? A[(size_t)(r0 + r) * n + st.k + c] : (r == c ? 1.0 : 0.0);
+      P[c][r] = r > c ? A[(size_t)(r0 + r) * n + st.k + c] : (r == c ? 1.0 : 0.0);
     }
     if (tid < TB) { tau_s[tid] = S[o.lt + tid]; sg[tid] = 1.0; }
     __syncthreads();
-    {
+    if (sq) {
       double gsum = 0.0;
-      for (int r = 0; r < m; r++) gsum += P[r][ti] * P[r][tj];
+      for (int r = 0; r < m; r++) gsum += P[ti][r] * P[tj][r];
       M[ti][tj] = gsum;   // Gram matrix Y^T Y
-      Tm[ti][tj] = 0.0;
     }
     __syncthreads();
-    for (int i = 0; i < TB; i++) {
-      // T[r][i] = -tau_i sum_{c=r}^{i-1} T[r][c] G[c][i]
-      double w = 0.0;
-      if (tid < i)
-        for (int c = tid; c < i; c++) w += Tm[tid][c] * M[c][i];
-      __syncthreads();
-      if (tid < i) Tm[tid][i] = -tau_s[i] * w;
-      if (tid == i) Tm[i][i] = tau_s[i];
-      __syncthreads();
+    if (tid < TB) {
+      double trow[TB];
+      tmat_row(M, tau_s, tid, trow);
+#pragma unroll
+      for (int i = 0; i < TB; i++) Tm[tid][i] = trow[i];
     }
+    __syncthreads();
   } else {
-    for (int idx = tid; idx < q * 256; idx += blockDim.x) {
+    for (int idx = tid; idx < q * 256; idx += QT) {
       const int lf = idx / 256, r = (idx / TB) % TB, c = idx % TB;
-      P[lf * TB + r][c] = S[o.rs + idx];
+      P[c][lf * TB + r] = S[o.rs + idx];
     }
     __syncthreads();
-    qr_lds(P, q * TB, tau_s, scal_s, beta_s);
-    Rr[ti][tj] = ti <= tj ? P[ti][tj] : 0.0;
-    for (int idx = tid; idx < q * TB * TB; idx += blockDim.x) {
+    qr_cm(P, q * TB, tau_s, scal_s, beta_s);
+    if (sq) Rr[ti][tj] = ti <= tj ? P[tj][ti] : 0.0;
+    for (int idx = tid; idx < q * TB * TB; idx += QT) {
       const int r = idx / TB, c = idx % TB;
-      X[r][c] = (r == c) ? 1.0 : 0.0;
+      X[c][r] = (r == c) ? 1.0 : 0.0;
     }
     __syncthreads();
-    apply_q_lds(P, tau_s, X, q * TB);   // X = C = Q_root [I; 0]
-    for (int idx = tid; idx < q * TB * TB; idx += blockDim.x) {
+    apply_q_cm(P, tau_s, X, q * TB);   // X = C = Q_root [I; 0]
+    for (int idx = tid; idx < q * TB * TB; idx += QT) {
       const int r = idx / TB, c = idx % TB;
-      S[o.cb + idx] = X[r][c];
+      S[o.cb + idx] = X[c][r];
     }
     __syncthreads();
     // the top of Q1: leaf 0's reflectors applied to [C_0; 0]
     int lo0, hi0;
     leaf_rows(st, 0, lo0, hi0);
     const int rows0 = hi0 - lo0;
-    for (int idx = tid; idx < rows0 * TB; idx += blockDim.x) {
+    for (int idx = tid; idx < rows0 * TB; idx += QT) {
       const int r = idx / TB, c = idx % TB;
-      P[r][c] = A[(size_t)(r0 + r) * n + st.k + c];
-      if (r >= TB) X[r][c] = 0.0;
+      P[c][r] = A[(size_t)(r0 + r) * n + st.k + c];
+      if (r >= TB) X[c][r] = 0.0;
     }
     if (tid < TB) tau_s[tid] = S[o.lt + tid];
     __syncthreads();
-    apply_q_lds(P, tau_s, X, rows0);
+    apply_q_cm(P, tau_s, X, rows0);
     // signed LU of the top TB x TB: Q1_top - diag(S) = L U
-    M[ti][tj] = X[ti][tj];
+    if (sq) M[ti][tj] = X[tj][ti];
     __syncthreads();
     for (int j = 0; j < TB; j++) {
       if (tid == 0) {
@@ -270,9 +282,9 @@ __global__ __launch_bounds__(256) void root_kernel(const E2Block* __restrict__ b
         M[j][j] -= sj;
       }
       __syncthreads();
-      if (ti > j && tj == j) M[ti][j] /= M[j][j];
+      if (sq && ti > j && tj == j) M[ti][j] /= M[j][j];
       __syncthreads();
-      if (ti > j && tj > j) M[ti][tj] -= M[ti][j] * M[j][tj];
+      if (sq && ti > j && tj > j) M[ti][tj] -= M[ti][j] * M[j][tj];
       __syncthreads();
     }
     // Li = L^{-1} (unit lower), column tid
@@ -289,7 +301,7 @@ __global__ __launch_bounds__(256) void root_kernel(const E2Block* __restrict__ b
     }
     __syncthreads();
     // T[i][j] = -sum_{k=i}^{j} U[i][k] S_k Li[j][k]
-    {
+    if (sq) {
       double t = 0.0;
       if (tj >= ti)
         for (int k2 = ti; k2 <= tj; k2++) t -= M[ti][k2] * sg[k2] * Li[tj][k2];
@@ -297,11 +309,13 @@ __global__ __launch_bounds__(256) void root_kernel(const E2Block* __restrict__ b
     }
     __syncthreads();
     // S R over the band part of the panel (upper triangle of its top TB rows)
-    if (ti <= tj) A[(size_t)(r0 + ti) * n + st.k + tj] = sg[ti] * Rr[ti][tj];
-    S[o.sm + 256 + tid] = ti <= tj ? M[ti][tj] : 0.0;                         // U
-    S[o.sm + 512 + tid] = ti > tj ? M[ti][tj] : (ti == tj ? 1.0 : 0.0);       // L
+    if (sq) {
+      if (ti <= tj) A[(size_t)(r0 + ti) * n + st.k + tj] = sg[ti] * Rr[ti][tj];
+      S[o.sm + 256 + tid] = ti <= tj ? M[ti][tj] : 0.0;                         // U
+      S[o.sm + 512 + tid] = ti > tj ? M[ti][tj] : (ti == tj ? 1.0 : 0.0);       // L
+    }
   }
-  S[o.sm + tid] = Tm[ti][tj];
+  if (sq) S[o.sm + tid] = Tm[ti][tj];
   if (tid < TB) {
     S[o.sm + 768 + tid] = sg[tid];
     taucol[st.k + tid] = Tm[tid][tid];
@@ -312,11 +326,11 @@ __global__ __launch_bounds__(256) void root_kernel(const E2Block* __restrict__ b
 // (Q1 - S) U^{-1}; one leaf: the reflectors themselves), U = Y T, Y into the
 // stage workspace and, transposed, into the rows k..k+TB-1 of A right of the
 // band (where the back-transformation reads the reflectors)
-__global__ __launch_bounds__(256) void form_kernel(const E2Block* __restrict__ blks, const E2TsStep* __restrict__ steps,
-                                                   const E2TsItem* __restrict__ leaves, double* __restrict__ A0,
-                                                   double* __restrict__ S) {
-  __shared__ double P[LRW][TB + 1];
-  __shared__ double X[LRW][TB + 1];
+__global__ __launch_bounds__(QT) void form_kernel(const E2Block* __restrict__ blks, const E2TsStep* __restrict__ steps,
+                                                  const E2TsItem* __restrict__ leaves, double* __restrict__ A0,
+                                                  double* __restrict__ S) {
+  __shared__ double P[TB][LDP];
+  __shared__ double X[TB][LDP];
   __shared__ double tau_s[TB];
   __shared__ double Ts[TB][TB + 1], Us[TB][TB + 1], Ls[TB][TB + 1];
   const E2TsItem it = leaves[blockIdx.x];
@@ -328,29 +342,29 @@ __global__ __launch_bounds__(256) void form_kernel(const E2Block* __restrict__ b
   const int rows = hi - lo;
   double* A = A0 + B.a_off;
   const E2TsOff o = e2_ts_off(B.s_off, n);
-  {
+  if (tid < 256) {
     const int ti = tid / TB, tj = tid % TB;
     Ts[ti][tj] = S[o.sm + tid];
     Us[ti][tj] = S[o.sm + 256 + tid];
     Ls[ti][tj] = S[o.sm + 512 + tid];
   }
   if (q == 1) {
-    for (int idx = tid; idx < rows * TB; idx += blockDim.x) {
+    for (int idx = tid; idx < rows * TB; idx += QT) {
       const int r = idx / TB, c = idx % TB;
-      X[r][c] = r > c ? A[(size_t)(r0 + lo + r) * n + st.k + c] : (r == c ? 1.0 : 0.0);
+      X[c][r] = r > c ? A[(size_t)(r0 + lo + r) * n + st.k + c] : (r == c ? 1.0 : 0.0);
     }
     __syncthreads();
   } else {
-    for (int idx = tid; idx < rows * TB; idx += blockDim.x) {
+    for (int idx = tid; idx < rows * TB; idx += QT) {
       const int r = idx / TB, c = idx % TB;
-      P[r][c] = A[(size_t)(r0 + lo + r) * n + st.k + c];
-      X[r][c] = r < TB ? S[o.cb + (size_t)(it.i * TB + r) * TB + c] : 0.0;
+      P[c][r] = A[(size_t)(r0 + lo + r) * n + st.k + c];
+      X[c][r] = r < TB ? S[o.cb + (size_t)(it.i * TB + r) * TB + c] : 0.0;
     }
     if (tid < TB) tau_s[tid] = S[o.lt + it.i * TB + tid];
     __syncthreads();
-    apply_q_lds(P, tau_s, X, rows);   // this leaf's rows of Q1
+    apply_q_cm(P, tau_s, X, rows);   // this leaf's rows of Q1
     // rows of Y: (Q1 - S) U^{-1}; leaf 0's first TB rows are L
-    for (int r = tid; r < rows; r += blockDim.x) {
+    for (int r = tid; r < rows; r += QT) {
       double y[TB];
       if (it.i == 0 && r < TB) {
 #pragma unroll
@@ -358,23 +372,23 @@ __global__ __launch_bounds__(256) void form_kernel(const E2Block* __restrict__ b
       } else {
 #pragma unroll
         for (int c = 0; c < TB; c++) {
-          double v = X[r][c];
+          double v = X[c][r];
 #pragma unroll
           for (int t = 0; t < c; t++) v -= y[t] * Us[t][c];
           y[c] = v / Us[c][c];
         }
       }
 #pragma unroll
-      for (int c = 0; c < TB; c++) X[r][c] = y[c];
+      for (int c = 0; c < TB; c++) X[c][r] = y[c];
     }
     __syncthreads();
   }
   // U = Y T; Y and U out
-  for (int r = tid; r < rows; r += blockDim.x) {
+  for (int r = tid; r < rows; r += QT) {
     const size_t row = (size_t)(lo + r);
     double y[TB];
 #pragma unroll
-    for (int c = 0; c < TB; c++) y[c] = X[r][c];
+    for (int c = 0; c < TB; c++) y[c] = X[c][r];
 #pragma unroll
     for (int c = 0; c < TB; c++) {
       double u = 0.0;
@@ -384,8 +398,10 @@ __global__ __launch_bounds__(256) void form_kernel(const E2Block* __restrict__ b
       S[o.yw + row * 32 + c] = y[c];
     }
   }
-  for (int c = 0; c < TB; c++)
-    for (int r = tid; r < rows; r += blockDim.x) A[(size_t)(st.k + c) * n + r0 + lo + r] = X[r][c];
+  for (int idx = tid; idx < rows * TB; idx += QT) {
+    const int c = idx / rows, r = idx % rows;
+    A[(size_t)(st.k + c) * n + r0 + lo + r] = X[c][r];
+  }
 }
 
 // 64-row tile of X = sum of the split-K partial products; partial Y^T X
@@ -479,6 +495,9 @@ __global__ void band_kernel(const E2Block* __restrict__ blks, const int* __restr
 #define SBW E2_SB_WIN
 #define LDB E2_SB_LDB
 #define SB_T (SBP * 16)
+#ifndef E2_SB_PUB
+#define E2_SB_PUB 4   // progress published every E2_SB_PUB ticks (a power of 2)
+#endif
 
 __device__ __forceinline__ int ld_acq(int* p) { return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT); }
 __device__ __forceinline__ void st_rel(int* p, int v) {
@@ -492,134 +511,213 @@ __device__ __forceinline__ void st_rlx(int* p, int v) {
 // last chase position of sweep s (tasks j = 0..J(s))
 __device__ __forceinline__ int jlast(int s, int n) { return (n - 1 - s + TB - 1) / TB - 1; }
 
-// Householder reflector of x (lane l holds x_l, zero for l >= L): v (v_0 = 1),
-// tau, beta with (I - tau v v^T) x = beta e_0
-__device__ __forceinline__ void house16(double x, int l, double& v, double& tau, double& beta) {
-  const double alpha = bc16<0>(x);
-  const double sig = sum16(l >= 1 ? x * x : 0.0);
-  if (sig > 0.0) {
-    beta = -copysign(sqrt(alpha * alpha + sig), alpha);
-    tau = (beta - alpha) / beta;
-    v = (l == 0) ? 1.0 : x / (alpha - beta);
-  } else {
-    beta = alpha;
-    tau = 0.0;
-    v = (l == 0) ? 1.0 : 0.0;
-  }
-}
-
 #define UNROLL16(M) M(0) M(1) M(2) M(3) M(4) M(5) M(6) M(7) M(8) M(9) M(10) M(11) M(12) M(13) M(14) M(15)
 
+// LDS writes of this wavefront's lanes visible to its other lanes (LDS
+// operations of a wavefront complete in order; the fence keeps the compiler
+// from moving accesses across it)
+__device__ __forceinline__ void wave_lds_fence() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront"); }
+
+// band columns that tick tk of a batch starting at sweep s0 brings in:
+// [s0, s0+17) at tick 0, [s0+16tk+1, s0+16tk+17) after (clipped at n)
+__device__ __forceinline__ void tick_cols(int s0, int n, int tk, int& lo, int& hi) {
+  lo = tk == 0 ? s0 : s0 + TB * tk + 1;
+  hi = s0 + TB * (tk + 1) + 1;
+  lo = min(lo, n);
+  hi = min(hi, n);
+}
+
+// One workgroup = one batch of SBP sweeps at a time; group g (16 lanes) runs
+// sweep s0+g, lane t owning column t of the task's blocks.  A tick: the
+// columns prefetched two ticks earlier go into the ring, every sweep does its
+// task, the columns no later task of the batch touches go back to the band,
+// and every fourth tick the progress is published (each wavefront fences its
+// write-backs first).  Loads of tick tk+2's columns are issued at tick tk,
+// after the predecessor batch has published them.
 __global__ __launch_bounds__(SB_T) void chase_kernel(const E2Block* __restrict__ blks, const E2SbGrid G,
                                                      double* __restrict__ WS, double* __restrict__ V2,
                                                      int* __restrict__ FL) {
-  __shared__ double W[SBW][LDB + 1];
+  // the ring: band column c at slot c mod SBW, LDB+1 doubles per slot; after
+  // it one dummy double per lane, the target of masked stores (so that the
+  // task code has no per-element branches)
+  __shared__ double Wf[SBW * (LDB + 1) + 64];
   __shared__ int sh_abort;
   const int bi = blockIdx.x / G.G, gq = blockIdx.x % G.G;
   if (bi >= G.nb) return;
   const E2Block B = blks[G.blk[bi]];
-  const int n = B.n, tid = threadIdx.x, grp = tid >> 4, l = tid & 15;
+  const int n = B.n, tid = threadIdx.x, grp = tid >> 4, t = tid & 15;
+  const int DUM = SBW * (LDB + 1) + (tid & 63);
   double* AB = WS + B.q_off;
   double* VB = V2 + B.v2_off;
   int* abortf = FL + B.f_off;
   int* ret = abortf + 1;
   const int nbatch = e2_sb_nbatch(n);
-#define WR(c, d) W[(c) % SBW][(d)]
+  constexpr int PF = (TB * LDB) / SB_T;   // prefetched doubles per thread per tick
+#define WA(c, d) ((((c) & (SBW - 1)) * (LDB + 1)) + (d))
   for (int q = gq; q < nbatch; q += G.G) {
     const int s0 = q * SBP, s1 = min(s0 + SBP, n - 2);
     const int s = s0 + grp;   // this group's sweep
-    int loaded_hi = s0, retired_lo = s0;
-    double vprev = 0.0, tprev = 0.0;
+    int retired_lo = s0, published = s0;
+    int known = q > 0 ? 0 : n;   // thread 0: predecessor's progress last read
+    // thread 0 waits until the predecessor has written back every column below hi
+    auto wait_pred = [&](int hi) -> int {
+      if (q == 0 || known >= hi) return 0;
+      known = ld_acq(&ret[q - 1]);
+      const uint64_t t0 = wall_clock64();
+      while (known < hi) {
+        if (ld_rlx(abortf)) return 1;
+        if (wall_clock64() - t0 > 200000000ull) { st_rlx(abortf, 1); return 1; }   // ~2 s at 100 MHz
+        __builtin_amdgcn_s_sleep(1);
+        known = ld_acq(&ret[q - 1]);
+      }
+      return 0;
+    };
+    double pf[2][PF];
+    // prologue: tick 0's columns straight into the ring, ticks 1 and 2 into the prefetch buffers
+    {
+      int lo, hi;
+      tick_cols(s0, n, 2, lo, hi);
+      if (tid == 0) sh_abort = wait_pred(hi);
+      __syncthreads();
+      if (sh_abort) return;
+      tick_cols(s0, n, 0, lo, hi);
+      for (int idx = tid; idx < (hi - lo) * LDB; idx += SB_T) {
+        const int c = lo + idx / LDB, d = idx % LDB;
+        Wf[WA(c, d)] = AB[(size_t)c * LDB + d];
+      }
+#pragma unroll
+      for (int k = 1; k <= 2; k++) {
+        tick_cols(s0, n, k, lo, hi);
+#pragma unroll
+        for (int u = 0; u < PF; u++) {
+          const int idx = tid + u * SB_T, c = lo + idx / LDB;
+          pf[k & 1][u] = c < hi ? AB[(size_t)c * LDB + idx % LDB] : 0.0;
+        }
+      }
+    }
+    double vfull[TB];   // the sweep's last reflector, every lane holds all of it
+#pragma unroll
+    for (int l = 0; l < TB; l++) vfull[l] = 0.0;
+    double tprev = 0.0;
     for (int tk = 0;; tk++) {
-      // ---- columns up to the front of the first sweep, once batch q-1 has written them back
-      const int front = min(n - 1, s0 + (tk + 1) * TB);
-      if (front >= loaded_hi) {
-        if (q > 0) {
-          if (tid == 0) {
-            int ab = 0;
-            if (ld_acq(&ret[q - 1]) <= front) {
-              const uint64_t t0 = wall_clock64();
-              while (ld_acq(&ret[q - 1]) <= front) {
-                if (ld_rlx(abortf)) { ab = 1; break; }
-                if (wall_clock64() - t0 > 200000000ull) { st_rlx(abortf, 1); ab = 1; break; }   // ~2 s at 100 MHz
-                __builtin_amdgcn_s_sleep(2);
-              }
-            }
-            sh_abort = ab;
-          }
-          __syncthreads();
-          if (sh_abort) return;
+      if (tk >= 1) {
+        int lo, hi;
+        tick_cols(s0, n, tk, lo, hi);
+        if (hi - retired_lo > SBW) {   // the ring cannot hold the span (not reached: span <= 2 P TSB - P + 1)
+          if (tid == 0) st_rlx(abortf, 1);
+          return;
         }
-        for (int idx = tid; idx < (front + 1 - loaded_hi) * LDB; idx += SB_T) {
-          const int c = loaded_hi + idx / LDB, d = idx % LDB;
-          WR(c, d) = AB[(size_t)c * LDB + d];
+#pragma unroll
+        for (int u = 0; u < PF; u++) {
+          const int idx = tid + u * SB_T, c = lo + idx / LDB;
+          if (c < hi) Wf[WA(c, idx % LDB)] = pf[tk & 1][u];
         }
-        loaded_hi = front + 1;
+        tick_cols(s0, n, tk + 2, lo, hi);   // its wait was done at the end of the previous tick
+#pragma unroll
+        for (int u = 0; u < PF; u++) {
+          const int idx = tid + u * SB_T, c = lo + idx / LDB;
+          pf[tk & 1][u] = c < hi ? AB[(size_t)c * LDB + idx % LDB] : 0.0;
+        }
       }
       __syncthreads();
       // ---- this tick's task of each sweep: (s, j) with 2 (s - s0) + j = tk
       const int j = tk - 2 * grp;
       if (s < s1 && j >= 0 && j <= jlast(s, n)) {
         const int lo = s + 1 + j * TB, L = min(TB, n - lo);
-        double v, tau;
+        const bool tin = t < L;
+        double x[TB];      // lane 0: the vector to eliminate
+        double bcol[TB];   // lane t: column t of B (j >= 1)
+        // column t of B = A[lo.., lo-TB+t]: contiguous from bb (d = TB - t + l <= 31)
+        const int bb = WA(lo - TB + t, TB - t);
         if (j == 0) {
-          const double x = (l < L) ? WR(s, 1 + l) : 0.0;
-          double beta;
-          house16(x, l, v, tau, beta);
-          if (l < L) WR(s, 1 + l) = (l == 0) ? beta : 0.0;
-        } else {
-          // B = A[lo.., lo-TB..lo-1], row l: b[t] = A(lo+l, lo-TB+t) = AB[lo-TB+t][TB+l-t]
-          double b[TB];
+          const int xb = WA(s, 1);
 #pragma unroll
-          for (int t = 0; t < TB; t++) b[t] = (l < L) ? WR(lo - TB + t, TB + l - t) : 0.0;
-          // B <- B (I - tprev vprev vprev^T)
-          double dot = 0.0;
-#define RM1(t) dot += b[t] * bc16<t>(vprev);
-          UNROLL16(RM1)
-#undef RM1
-          const double f = tprev * dot;
-#define RM2(t) b[t] -= f * bc16<t>(vprev);
-          UNROLL16(RM2)
-#undef RM2
-          if (L >= 2) {
-            double beta;
-            house16(b[0], l, v, tau, beta);
-            // B <- (I - tau v v^T) B: column sums over the 16 lanes
-#pragma unroll
-            for (int t = 1; t < TB; t++) {
-              const double dt = sum16(v * b[t]);
-              b[t] -= tau * v * dt;
-            }
-            b[0] = (l == 0) ? beta : 0.0;
-          } else {
-            v = (l == 0) ? 1.0 : 0.0;
-            tau = 0.0;
+          for (int l = 0; l < TB; l++) {
+            const double v = Wf[xb + l];
+            x[l] = (l < L) ? v : 0.0;
           }
-          if (l < L)
+        } else {
+          // row t of B times the previous reflector, back to the ring
+          double b[TB], d4[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-            for (int t = 0; t < TB; t++) WR(lo - TB + t, TB + l - t) = b[t];
+          for (int u = 0; u < TB; u++) {
+            b[u] = Wf[WA(lo - TB + u, TB + t - u)];
+            d4[u & 3] += b[u] * vfull[u];
+          }
+          const double f = tprev * ((d4[0] + d4[1]) + (d4[2] + d4[3]));
+#pragma unroll
+          for (int u = 0; u < TB; u++) Wf[tin ? WA(lo - TB + u, TB + t - u) : DUM] = b[u] - f * vfull[u];
+          wave_lds_fence();
+#pragma unroll
+          for (int l = 0; l < TB; l++) {
+            const double v = Wf[bb + l];
+            bcol[l] = (l < L) ? v : 0.0;
+            x[l] = bcol[l];
+          }
         }
-        // D = A[lo..lo+L-1, same] <- H D H; lane l holds row l
+        // reflector of lane 0's x, broadcast to the group
+        double tau = 0.0, beta = x[0], vv[TB];
+        {
+          double s4[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+          for (int l = 1; l < TB; l++) s4[l & 3] += x[l] * x[l];
+          const double sig = (s4[0] + s4[1]) + (s4[2] + s4[3]);
+          const double alpha = x[0];
+          const double bnew = -copysign(sqrt(alpha * alpha + sig), alpha);
+          const bool nz = sig > 0.0;
+          beta = nz ? bnew : alpha;
+          tau = nz ? (bnew - alpha) / bnew : 0.0;
+          const double scal = nz ? 1.0 / (alpha - bnew) : 0.0;
+          vv[0] = 1.0;
+#pragma unroll
+          for (int l = 1; l < TB; l++) vv[l] = x[l] * scal;
+        }
+        tau = bc16<0>(tau);
+        beta = bc16<0>(beta);
+#define VB16(l) vfull[l] = bc16<0>(vv[l]);
+        UNROLL16(VB16)
+#undef VB16
+        double vown = 0.0;
+#pragma unroll
+        for (int l = 0; l < TB; l++) vown = (l == t) ? vfull[l] : vown;
+        if (j == 0) {
+          Wf[tin ? WA(s, 1 + t) : DUM] = (t == 0) ? beta : 0.0;
+        } else if (L >= 2) {
+          // column t of (I - tau v v^T) B; column 0 becomes (beta, 0, ...)
+          double d4[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+          for (int l = 0; l < TB; l++) d4[l & 3] += vfull[l] * bcol[l];
+          const double f = tau * ((d4[0] + d4[1]) + (d4[2] + d4[3]));
+#pragma unroll
+          for (int l = 0; l < TB; l++) {
+            const double nv = (t == 0) ? (l == 0 ? beta : 0.0) : bcol[l] - f * vfull[l];
+            Wf[(l < L) ? bb + l : DUM] = nv;
+          }
+        }
+        // D = A[lo..lo+L-1, same] <- H D H; lane t holds column t:
+        // (l, t) at column lo+t, d = l-t for l >= t, at column lo+l, d = t-l above
+        const int bt0 = WA(lo + t, 0) - t;
         double dr[TB];
 #pragma unroll
-        for (int t = 0; t < TB; t++)
-          dr[t] = (l < L && t < L) ? (t <= l ? WR(lo + t, l - t) : WR(lo + l, t - l)) : 0.0;
-        double p = 0.0;
-#define DP(t) p += dr[t] * bc16<t>(v);
-        UNROLL16(DP)
-#undef DP
-        p *= tau;
-        const double vp = sum16(v * p);
-        const double w = p - 0.5 * tau * vp * v;
-#define DU(t) dr[t] -= v * bc16<t>(w) + w * bc16<t>(v);
-        UNROLL16(DU)
-#undef DU
-        if (l < L)
+        for (int l = 0; l < TB; l++) {
+          const int ad = (l >= t) ? bt0 + l : WA(lo + l, t - l);
+          const double v = Wf[ad];
+          dr[l] = (l < L && tin) ? v : 0.0;
+        }
+        double p4[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-          for (int t = 0; t < TB; t++)
-            if (t <= l) WR(lo + t, l - t) = dr[t];
-        VB[((size_t)j * n + s) * TB + l] = (l == 0) ? tau : v;
-        vprev = v;
+        for (int l = 0; l < TB; l++) p4[l & 3] += dr[l] * vfull[l];
+        const double p = tau * ((p4[0] + p4[1]) + (p4[2] + p4[3]));
+        const double vp = sum16(vown * p);
+        const double w = p - 0.5 * tau * vp * vown;
+        double wfull[TB];
+#define WB16(l) wfull[l] = bc16<l>(w);
+        UNROLL16(WB16)
+#undef WB16
+#pragma unroll
+        for (int l = 0; l < TB; l++)
+          Wf[(l >= t && l < L && tin) ? bt0 + l : DUM] = dr[l] - (vfull[l] * w + wfull[l] * vown);
+        VB[((size_t)j * n + s) * TB + t] = (t == 0) ? tau : vown;
         tprev = tau;
       }
       __syncthreads();
@@ -634,26 +732,29 @@ __global__ __launch_bounds__(SB_T) void chase_kernel(const E2Block* __restrict__
         const int mc = (jj <= 0) ? ss : ss + 1 + (jj - 1) * TB;
         low = min(low, mc);
       }
-      const int upto = live ? min(low, loaded_hi) : loaded_hi;
-      if (upto > retired_lo) {
-        for (int idx = tid; idx < (upto - retired_lo) * LDB; idx += SB_T) {
-          const int c = retired_lo + idx / LDB, d = idx % LDB;
-          AB[(size_t)c * LDB + d] = WR(c, d);
-        }
-        retired_lo = upto;
+      int lo_, hi_;
+      tick_cols(s0, n, tk, lo_, hi_);   // loaded so far: columns below hi_
+      const int upto = live ? min(low, hi_) : hi_;   // finished: every loaded column (hi_ = n by then)
+      for (int idx = tid; idx < (upto - retired_lo) * LDB; idx += SB_T) {
+        const int c = retired_lo + idx / LDB, d = idx % LDB;
+        AB[(size_t)c * LDB + d] = Wf[WA(c, d)];
       }
-      if (!live) {
-        __syncthreads();
-        if (tid == 0) st_rel(&ret[q], n);
-        break;
+      retired_lo = max(retired_lo, upto);
+      const bool pub = !live || ((tk & (E2_SB_PUB - 1)) == E2_SB_PUB - 1 && retired_lo > published);
+      if (pub) __threadfence();   // each wavefront's write-backs complete and visible at agent scope
+      if (tid == 0) {
+        int lo3, hi3;
+        tick_cols(s0, n, tk + 3, lo3, hi3);
+        sh_abort = live ? wait_pred(hi3) : 0;
       }
-      if (upto > 0) {
-        __syncthreads();
-        if (tid == 0) st_rel(&ret[q], retired_lo);
-      }
+      __syncthreads();
+      if (pub && tid == 0) st_rel(&ret[q], live ? retired_lo : n);
+      published = pub ? retired_lo : published;
+      if (sh_abort) return;
+      if (!live) break;
     }
   }
-#undef WR
+#undef WA
 }
 
 // d and e of the tridiagonal matrix from the chased band
@@ -687,8 +788,7 @@ __global__ __launch_bounds__(256) void q2_kernel(const E2Block* __restrict__ blk
   while (bi + 1 < G.nb && (int)blockIdx.x >= G.cum[bi + 1]) bi++;
   const E2Block B = blks[G.blk[bi]];
   const int n = B.n, krow = B.krow, tid = threadIdx.x;
-  const int local = blockIdx.x - G.cum[bi];
-  const int g = G.glo[bi] + local / G.nch[bi], rc = local % G.nch[bi];
+  const int g = G.glo[bi] + (blockIdx.x - G.cum[bi]);
   const int ng = (n - 2 + TB - 1) / TB;
   const int j = G.tick - (ng - 1 - g);
   const int s0 = g * TB, r0 = s0 + 1 + j * TB;
@@ -715,42 +815,36 @@ __global__ __launch_bounds__(256) void q2_kernel(const E2Block* __restrict__ blk
     double a = 0.0;
     for (int r = 0; r < h; r++) a += V[r][ti] * V[r][tj];
     Gm[ti][tj] = a;
-    Tm[ti][tj] = 0.0;
   }
   __syncthreads();
-  for (int i = 0; i < TB; i++) {
-    double w = 0.0;
-    if (tid < i)
-      for (int c = tid; c < i; c++) w += Tm[tid][c] * Gm[c][i];
-    __syncthreads();
-    if (tid < i) Tm[tid][i] = -tau_s[i] * w;
-    if (tid == i) Tm[i][i] = tau_s[i];
-    __syncthreads();
+  if (tid < TB) {
+    double trow[TB];
+    tmat_row(Gm, tau_s, tid, trow);
+#pragma unroll
+    for (int i = 0; i < TB; i++) Tm[tid][i] = trow[i];
   }
-  // Z rows rc*64 .. : columns r0 .. r0+h-1
+  __syncthreads();
+  // every 64-row chunk of the kept rows: Z <- Z - (Z V) T^T V^T on columns r0 .. r0+h-1
   double* Z = WS + B.q_off;
-  const int e0 = rc * 64, ne = min(64, krow - e0);
-  for (int idx = tid; idx < 64 * Q2H; idx += blockDim.x) {
-    const int e = idx / Q2H, r = idx % Q2H;
-    Zs[e][r] = (e < ne && r < h) ? Z[(size_t)(e0 + e) * n + r0 + r] : 0.0;
-  }
-  __syncthreads();
-  // P1 = Z V (64 x 16): thread (e = tid / 4, 4 columns)
-  {
-    const int e = tid >> 2, c0 = (tid & 3) * 4;
-    double a[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int r = 0; r < h; r++) {
-      const double z = Zs[e][r];
-#pragma unroll
-      for (int u = 0; u < 4; u++) a[u] += z * V[r][c0 + u];
+  for (int e0 = 0; e0 < krow; e0 += 64) {
+    const int ne = min(64, krow - e0);
+    for (int idx = tid; idx < 64 * Q2H; idx += blockDim.x) {
+      const int e = idx / Q2H, r = idx % Q2H;
+      Zs[e][r] = (e < ne && r < h) ? Z[(size_t)(e0 + e) * n + r0 + r] : 0.0;
     }
-    // P2 = P1 T^T: needs the full row of P1, so stage it first
-#pragma unroll
-    for (int u = 0; u < 4; u++) P1[e][c0 + u] = a[u];
-  }
-  __syncthreads();
-  {
+    __syncthreads();
     const int e = tid >> 2, c0 = (tid & 3) * 4;
+    {
+      double a[4] = {0.0, 0.0, 0.0, 0.0};
+      for (int r = 0; r < h; r++) {
+        const double z = Zs[e][r];
+#pragma unroll
+        for (int u = 0; u < 4; u++) a[u] += z * V[r][c0 + u];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; u++) P1[e][c0 + u] = a[u];
+    }
+    __syncthreads();
     double p2[4];
 #pragma unroll
     for (int u = 0; u < 4; u++) {
@@ -761,16 +855,16 @@ __global__ __launch_bounds__(256) void q2_kernel(const E2Block* __restrict__ blk
     __syncthreads();
 #pragma unroll
     for (int u = 0; u < 4; u++) P1[e][c0 + u] = p2[u];
-  }
-  __syncthreads();
-  // Z -= P2 V^T
-  for (int idx = tid; idx < 64 * Q2H; idx += blockDim.x) {
-    const int e = idx / Q2H, r = idx % Q2H;
-    if (e < ne && r < h) {
-      double a = Zs[e][r];
-      for (int c = 0; c < TB; c++) a -= P1[e][c] * V[r][c];
-      Z[(size_t)(e0 + e) * n + r0 + r] = a;
+    __syncthreads();
+    for (int idx = tid; idx < 64 * Q2H; idx += blockDim.x) {
+      const int ee = idx / Q2H, r = idx % Q2H;
+      if (ee < ne && r < h) {
+        double a = Zs[ee][r];
+        for (int c = 0; c < TB; c++) a -= P1[ee][c] * V[r][c];
+        Z[(size_t)(e0 + ee) * n + r0 + r] = a;
+      }
     }
+    __syncthreads();
   }
 }
 
@@ -793,21 +887,21 @@ hipError_t e2s_launch_init(const E2Block* b, const int* list, int nlist, int nma
 hipError_t e2s_launch_leaf(const E2Block* b, const E2TsStep* st, const E2TsItem* leaves, int nleaf, double* A,
                            double* S, hipStream_t s) {
   if (nleaf <= 0) return hipSuccess;
-  hipLaunchKernelGGL(e2s::leaf_kernel, dim3(nleaf), dim3(256), 0, s, b, st, leaves, A, S);
+  hipLaunchKernelGGL(e2s::leaf_kernel, dim3(nleaf), dim3(QT), 0, s, b, st, leaves, A, S);
   return hipGetLastError();
 }
 
 hipError_t e2s_launch_root(const E2Block* b, const E2TsStep* st, int nstep, double* A, double* S, double* WS,
                            hipStream_t s) {
   if (nstep <= 0) return hipSuccess;
-  hipLaunchKernelGGL(e2s::root_kernel, dim3(nstep), dim3(256), 0, s, b, st, A, S, WS);
+  hipLaunchKernelGGL(e2s::root_kernel, dim3(nstep), dim3(QT), 0, s, b, st, A, S, WS);
   return hipGetLastError();
 }
 
 hipError_t e2s_launch_form(const E2Block* b, const E2TsStep* st, const E2TsItem* leaves, int nleaf, double* A,
                            double* S, hipStream_t s) {
   if (nleaf <= 0) return hipSuccess;
-  hipLaunchKernelGGL(e2s::form_kernel, dim3(nleaf), dim3(256), 0, s, b, st, leaves, A, S);
+  hipLaunchKernelGGL(e2s::form_kernel, dim3(nleaf), dim3(QT), 0, s, b, st, leaves, A, S);
   return hipGetLastError();
 }
 
