@@ -43,7 +43,8 @@ def _params(w):
     return model, ramp, w
 
 
-def gpu_run(name, steps, warmup, eig_impl=2, plain=False, stats_all=False, red_streams=None, two_stage=None):
+def gpu_run(name, steps, warmup, eig_impl=2, plain=False, stats_all=False, red_streams=None, two_stage=None,
+            graph=False, info=None):
     import twochnrg_amd as t
     model, ramp, w = _params(WORKLOADS[name])
     first = -1 if model == "onechqs" else 0   # first record emitted
@@ -57,6 +58,8 @@ def gpu_run(name, steps, warmup, eig_impl=2, plain=False, stats_all=False, red_s
         ctx.set_red_streams(red_streams)
     if two_stage is not None:
         ctx.set_two_stage(two_stage)
+    if graph:
+        ctx.set_graph(True)
     ctx.set_timing(True)
     t_start_N = first + ramp + warmup         # stats reset after this step's record
 
@@ -69,12 +72,17 @@ def gpu_run(name, steps, warmup, eig_impl=2, plain=False, stats_all=False, red_s
               file=sys.stderr, flush=True)
         if s.nshell == t_start_N and not stats_all:
             ctx.reset_stats()
+            if graph:
+                ctx.graph_stats()   # reset
 
     if model == "onechqs":
         ctx.run_onechqs(nsitesmax=total, keep_steps=False, on_step=on_step, **w)
     else:
         ctx.run_twochqs(nsitesmax=total, keep_steps=False, on_step=on_step, **w)
     st = ctx.stats()
+    if graph and info is not None:
+        ms, nodes, graphs = ctx.graph_stats()
+        info.update(graph_host_ms=ms, graph_nodes=nodes, graphs=graphs)
     ctx.close()
     last = max(stamps)
     dt = stamps[last] - stamps[t_start_N]
@@ -431,6 +439,24 @@ def hbm_probe(nbytes=2 << 30, reps=10):
         out[name + "_GBs"] = moved * reps / (e0.elapsed_time(e1) * 1e-3) / 1e9
     del x, y
     torch.cuda.empty_cache()
+    # the FP64 matrix-core ceiling as the vendor library reaches it: torch's
+    # DGEMM (hipBLASLt / rocBLAS) on 8192^3, the rate the rotation and
+    # back-transformation GEMMs are compared with
+    m = 8192
+    a = torch.randn(m, m, dtype=torch.float64, device="cuda")
+    b = torch.randn(m, m, dtype=torch.float64, device="cuda")
+    c = a @ b
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(5):
+        c = a @ b
+    e1.record()
+    torch.cuda.synchronize()
+    out["dgemm_TFLOPs"] = 5 * 2.0 * m ** 3 / (e0.elapsed_time(e1) * 1e-3) / 1e12
+    out["dgemm_shape"] = [m, m, m]
+    del a, b, c
+    torch.cuda.empty_cache()
     return out
 
 
@@ -612,6 +638,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="twochqs_c3", choices=sorted(WORKLOADS))
     ap.add_argument("--eig-impl", type=int, default=2)
+    ap.add_argument("--graph", action="store_true",
+                    help="capture each eigensolve's launch sequences into a hipGraph (launched once)")
     ap.add_argument("--two-stage", type=int, default=None,
                     help="blocks with n >= this use the two-stage reduction (0: none)")
     ap.add_argument("--no-cpu", action="store_true")
@@ -649,8 +677,9 @@ def main():
     if a.c5_only:
         print(json.dumps(c5_run(cpu=not a.no_cpu)))
         return
+    ginfo = {}
     dt, st, timed, model, w, nsteps = gpu_run(a.workload, a.steps, a.warmup, a.eig_impl, stats_all=a.stats_all,
-                                              two_stage=a.two_stage)
+                                              two_stage=a.two_stage, graph=a.graph, info=ginfo)
     value = nsteps / dt
     eig = st["eig"]
     eig_rate = eig["flops"] / (eig["ms"] * 1e-3) / 1e12 if eig["ms"] > 0 else 0.0
@@ -698,6 +727,10 @@ def main():
                        "flops_per_step": eig["flops"] / nsteps, "ms_per_step": eig["ms"] / nsteps},
         "kernel_ms_per_step": {k: v["ms"] / nsteps for k, v in st.items()},
     }
+    if a.graph:
+        out["graph"] = {"graphs_per_step": ginfo.get("graphs", 0) / nsteps,
+                        "nodes_per_step": ginfo.get("graph_nodes", 0) / nsteps,
+                        "host_ms_per_step": ginfo.get("graph_host_ms", 0.0) / nsteps}
     gm = st.get("gemm")
     if gm and gm["ms"] > 0:
         # operator rotations Z_i F Z_j^T (FP64 MFMA tile kernel): algorithmic flops / HIP-event time
@@ -706,9 +739,17 @@ def main():
     if model == "twochqs" and not a.no_error:
         out["error_vs_fp64"] = twochqs_error(a.workload)
     cal = hbm_probe_child()
-    roof["measured_hbm"] = cal
+    roof["measured_hbm"] = {k: v for k, v in cal.items() if not k.startswith("dgemm")}
     if roof["unit"] == "GB/s" and "read_GBs" in cal:
         roof["frac_of_measured_read"] = roof["achieved"] / cal["read_GBs"]
+    if "dgemm_TFLOPs" in cal:
+        # measured FP64 matrix-core ceiling (vendor DGEMM on the box) beside the nominal 78.6
+        out["measured_dgemm"] = {"TFLOPs": cal["dgemm_TFLOPs"], "shape": cal.get("dgemm_shape"),
+                                 "frac_of_nominal": cal["dgemm_TFLOPs"] / FP64_PEAK_TFLOPS}
+        if "rotation" in out:
+            out["rotation"]["frac_of_measured_dgemm"] = out["rotation"]["tflops"] / cal["dgemm_TFLOPs"]
+        if roof["unit"] == "TFLOP/s":
+            roof["frac_of_measured_dgemm"] = roof["achieved"] / cal["dgemm_TFLOPs"]
     if a.plain_steps > 0:
         # the same kernels, one symmetry block (eigensolve) or one block pair
         # (rotation) per launch, host-synchronised, no batching

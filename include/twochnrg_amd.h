@@ -79,6 +79,18 @@ int nrg_set_red_streams(nrg_ctx *ctx, int n);
  * 16384 + 16 rows keep the one-stage reduction. */
 int nrg_set_two_stage(nrg_ctx *ctx, int min_n);
 
+/* hipGraph mode (impl 2): each eigensolve's launch sequences (the reduction,
+ * divide and conquer and back-transformation, with their auxiliary streams)
+ * are captured into a graph after their descriptor uploads, instantiated and
+ * launched once -- the per-step capture of the NRG iteration's dominant launch
+ * sequence.  Two-stage blocks run uncaptured (cooperative launch).  Results
+ * are identical to the eager launches.  No reference counterpart (the
+ * reference runs DiagBlock on the host). */
+int nrg_set_graph(nrg_ctx *ctx, int on);
+/* host milliseconds spent capturing, instantiating and launching graphs, the
+ * graph nodes and graphs since the last call (then reset) */
+int nrg_graph_stats(nrg_ctx *ctx, double *host_ms, long *nodes, long *graphs);
+
 /* Diagnostics: run the batched solver (impl 2) on one n x n matrix and return
  * the stage outputs -- the tear-modified tridiagonal diagonal d (after the
  * divide-and-conquer set-up), the untorn off-diagonal e, the sorted

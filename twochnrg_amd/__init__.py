@@ -18,7 +18,7 @@ EXPORTS = [
     "nrg_ctx_create", "nrg_ctx_destroy", "nrg_last_error", "nrg_device_count",
     "nrg_diag_blocks", "nrg_diag_blocks_dev", "nrg_truncate", "nrg_ecut",
     "nrg_rotate_ops", "nrg_run_onechqs", "nrg_run_twochqs", "nrg_set_timing", "nrg_get_stats", "nrg_reset_stats",
-    "nrg_set_eig_impl", "nrg_set_plain", "nrg_set_red_streams", "nrg_set_two_stage", "nrg_eig2_stages", "nrg_compact",
+    "nrg_set_eig_impl", "nrg_set_plain", "nrg_set_red_streams", "nrg_set_two_stage", "nrg_set_graph", "nrg_graph_stats", "nrg_eig2_stages", "nrg_compact",
     "nrg_dm_create", "nrg_dm_destroy", "nrg_dm_info", "nrg_dm_timing", "nrg_dm_memory", "nrg_dm_rho", "nrg_dm_spec",
     "nrg_dm_spectrum", "nrg_run_onechqs_multi", "nrg_dm_spec_omega", "nrg_dm_norm", "nrg_dm_weights",
     "nrg_run_twochqsz", "nrg_onechqs_thermo", "nrg_onechqs_jvp", "nrg_dm_create_jvp", "nrg_dm_spec_jvp",
@@ -153,6 +153,9 @@ def lib():
     L.nrg_set_plain.argtypes = [ctypes.c_void_p, ctypes.c_int]
     L.nrg_set_red_streams.argtypes = [ctypes.c_void_p, ctypes.c_int]
     L.nrg_set_two_stage.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    L.nrg_set_graph.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    L.nrg_graph_stats.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_long),
+                                  ctypes.POINTER(ctypes.c_long)]
     L.nrg_compact.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
                               ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int), ctypes.c_void_p,
                               ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]
@@ -280,6 +283,16 @@ class Context:
         """Blocks with n >= min_n use the two-stage reduction (dense -> band ->
         tridiagonal); 0 turns it off."""
         _check(lib().nrg_set_two_stage(self.h, int(min_n)))
+
+    def set_graph(self, on=True):
+        """hipGraph mode: each eigensolve's launch sequences captured and launched as a graph."""
+        _check(lib().nrg_set_graph(self.h, int(on)))
+
+    def graph_stats(self):
+        """(host ms capturing + instantiating + launching, nodes, graphs) since the last call."""
+        ms, nodes, graphs = ctypes.c_double(), ctypes.c_long(), ctypes.c_long()
+        _check(lib().nrg_graph_stats(self.h, ctypes.byref(ms), ctypes.byref(nodes), ctypes.byref(graphs)))
+        return ms.value, nodes.value, graphs.value
 
     def compact(self, n, z_off, keep, dZ=None, dZcut=None):
         """CutStates' eigenvector compaction on device arrays (nrg_compact):

@@ -12,7 +12,58 @@
 #include "nrg_eig2.h"
 #include "nrg_internal.h"
 
+#include <chrono>
+
 namespace nrg {
+
+// Graph mode: a launch sequence enqueued on c->stream (and on auxiliary
+// streams forked from it and joined back) is captured, instantiated and
+// launched once.  Uploads come before begin(), so the graph holds kernels and
+// the fork/join dependencies only.  An error path ends the capture and drops
+// the graph.
+struct GraphScope {
+  nrg_ctx* c;
+  bool on = false;
+  std::chrono::steady_clock::time_point t0;
+  GraphScope(nrg_ctx* c_, bool want) : c(c_) {
+    if (!want) return;
+    t0 = std::chrono::steady_clock::now();
+    if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed) != hipSuccess) return;
+    on = true;
+    c->capturing = true;
+  }
+  int finish() {
+    if (!on) return NRG_OK;
+    on = false;
+    c->capturing = false;
+    hipGraph_t g = nullptr;
+    HIPCHK(hipStreamEndCapture(c->stream, &g));
+    hipGraphExec_t ge = nullptr;
+    size_t nn = 0;
+    HIPCHK(hipGraphGetNodes(g, nullptr, &nn));
+    hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+    hipError_t e2 = hipGraphDestroy(g);
+    if (e != hipSuccess) return fail(NRG_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    HIPCHK(e2);
+    HIPCHK(hipGraphLaunch(ge, c->stream));
+    c->gexec.push_back(ge);
+    // graphs of earlier eigensolves have finished (each step reads its results back)
+    while (c->gexec.size() > 4) {
+      HIPCHK(hipGraphExecDestroy(c->gexec.front()));
+      c->gexec.erase(c->gexec.begin());
+    }
+    c->graph_nodes += (long)nn;
+    c->graph_count++;
+    c->graph_host_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return NRG_OK;
+  }
+  ~GraphScope() {
+    if (!on) return;
+    c->capturing = false;
+    hipGraph_t g = nullptr;
+    if (hipStreamEndCapture(c->stream, &g) == hipSuccess && g) (void)hipGraphDestroy(g);
+  }
+};
 
 static const int kLeaf = 16;   // = LEAF in nrg_eig2.hip
 static const int kPanel = 32;  // = PANEL in nrg_eig2.hip
@@ -322,6 +373,7 @@ static int e2_phase_a(nrg_ctx* c, const std::vector<E2Block>& blk, double* dA, d
   if ((rc = upload(c, "e2_tt", ttiles, &d_tt))) return rc;
   // ---- launches
   hipStream_t s = c->stream;
+  GraphScope gs_(c, c->graph && tsl.empty());
   // lanes of concurrent work: the two-stage blocks' chain, the large groups'
   // column chains, the small blocks' single-workgroup reductions; lane 0 is
   // the context stream, lane i the auxiliary stream i-1
@@ -394,7 +446,7 @@ static int e2_phase_a(nrg_ctx* c, const std::vector<E2Block>& blk, double* dA, d
     HIPCHK(e2_launch_dc_level(d_blk, d_lvl + lvl_beg[h], (int)by_h[h].size(), lvl_mmax[h], d_ms + lvl_beg[h], WS, IW,
                               d_gp + lvl_pbeg[h], d_gt + lvl_tbeg[h], lvl_tcnt[h], s));
   HIPCHK(e2_launch_output(d_blk, nb, nmax, WS, dA, dW, 1, s));
-  return NRG_OK;
+  return gs_.finish();
 }
 
 // Phase B on blocks blk: the compact-WY back-transformation of the first
@@ -537,6 +589,9 @@ static int e2_phase_b(nrg_ctx* c, const std::vector<E2Block>& blk, double* dA, d
   if ((rc = daux.reserve(sizeof(double) * (size_t)(aux + 1)))) return rc;
   double* AX = (double*)daux.p;
   hipStream_t s = c->stream;
+  bool any_ts = false;
+  for (int b : act) any_ts = any_ts || blk[b].ts;
+  GraphScope gs_(c, c->graph && !any_ts);
   // the bulge-chase reflectors of the two-stage blocks first (x = Q1 Q2 z)
   {
     std::vector<int> tsa;
@@ -588,7 +643,7 @@ static int e2_phase_b(nrg_ctx* c, const std::vector<E2Block>& blk, double* dA, d
     HIPCHK(e2_launch_gemm3(pr, t0 + r_wt[r], r_zt[r], AX, dA, WS, s));
   }
   HIPCHK(e2_launch_output(d_blk, nb, nmax, WS, dA, nullptr, 2, s));
-  return NRG_OK;
+  return gs_.finish();
 }
 
 // Phase A of every block (batched, or one block per launch sequence in plain

@@ -123,6 +123,7 @@ extern "C" void nrg_ctx_destroy(nrg_ctx* c) {
   c->bufs.clear();
   if (c->ring) hipHostFree(c->ring);
   if (c->e2p.abort_host) hipHostFree(c->e2p.abort_host);
+  for (auto g : c->gexec) hipGraphExecDestroy(g);
   hipStreamDestroy(c->stream);
   delete c;
 }
@@ -151,7 +152,7 @@ hipEvent_t nrg_ctx::get_event() {
 int nrg_ctx::timer_begin(PendingTimer& t, int kind, double flops, double bytes, hipStream_t s) {
   t.kind = kind; t.flops = flops; t.bytes = bytes;
   t.a = t.b = nullptr;
-  if (!timing) return 0;
+  if (!timing || capturing) return 0;
   t.a = get_event();
   t.b = get_event();
   if (!t.a || !t.b) return fail(NRG_ERR_HIP, "hipEventCreate failed");
@@ -220,6 +221,22 @@ extern "C" int nrg_set_red_streams(nrg_ctx* c, int n) {
   if (!c) return fail(NRG_ERR_ARG, "nrg_set_red_streams: NULL ctx");
   if (n < 1 || n > 4) return fail(NRG_ERR_ARG, "nrg_set_red_streams: n must be 1..4");
   c->red_streams = n;
+  return NRG_OK;
+}
+
+extern "C" int nrg_set_graph(nrg_ctx* c, int on) {
+  if (!c) return fail(NRG_ERR_ARG, "nrg_set_graph: NULL ctx");
+  c->graph = on != 0;
+  return NRG_OK;
+}
+
+extern "C" int nrg_graph_stats(nrg_ctx* c, double* host_ms, long* nodes, long* graphs) {
+  if (!c) return fail(NRG_ERR_ARG, "nrg_graph_stats: NULL ctx");
+  if (host_ms) *host_ms = c->graph_host_ms;
+  if (nodes) *nodes = c->graph_nodes;
+  if (graphs) *graphs = c->graph_count;
+  c->graph_host_ms = 0.0;
+  c->graph_nodes = c->graph_count = 0;
   return NRG_OK;
 }
 

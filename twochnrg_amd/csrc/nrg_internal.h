@@ -73,6 +73,13 @@ struct nrg_ctx {
     size_t abort_cap = 0;
   } e2p;
   int ts_min = 0;      // blocks with n >= ts_min use the two-stage reduction (0: none)
+  // hipGraph mode: the eigensolver's launch sequences are captured into a graph
+  // (after their descriptor uploads), instantiated and launched once
+  bool graph = false;
+  bool capturing = false;
+  std::vector<hipGraphExec_t> gexec;   // launched graphs, released two eigensolves later
+  double graph_host_ms = 0.0;          // host time spent capturing + instantiating + launching
+  long graph_nodes = 0, graph_count = 0;
   int dm_seq = 0;      // DM-NRG handles created on this context (buffer name prefixes)
   std::vector<nrg_dm*> dms;   // live DM-NRG handles: detached when the context goes first
   // small host-to-device copies go through a pinned ring on the stream, with no
