@@ -374,6 +374,33 @@ def adjoint_run(cpu=True):
     return out
 
 
+def autotune_run(name):
+    """The eigensolver's launch configuration for this workload, picked by the
+    library's autotuner (nrg_autotune_eig) on random blocks of the sizes of a
+    truncated step of the workload (recorded from a short chain), with the
+    step's kept fraction of eigenvector rows.  Outside the timed region."""
+    import twochnrg_amd as t
+    model, ramp, w = _params(WORKLOADS[name])
+    sizes = {}
+
+    def on_step(st):
+        s = st.contents
+        sizes[s.nshell] = [s.bbe[2 * i + 1] - s.bbe[2 * i] + 1 for i in range(s.nblocks)]
+
+    t0 = time.perf_counter()
+    with t.Context(0) as ctx:
+        if model == "onechqs":
+            ctx.run_onechqs(nsitesmax=ramp + 1, keep_steps=False, on_step=on_step, **w)
+        else:
+            ctx.run_twochqs(nsitesmax=ramp + 1, keep_steps=False, on_step=on_step, **w)
+        last = max(sizes)
+        sz = sizes[last]
+        frac = min(1.0, w["ncut"] / float(sum(sz)))
+        best, rows = ctx.autotune(sz, krow_frac=frac, reps=1)
+    return {"sizes_from_step": last, "blocks": len(sz), "max_block": max(sz), "krow_frac": frac,
+            "chosen": best, "table": rows, "wall_s": time.perf_counter() - t0}
+
+
 def hbm_probe_child():
     """hbm_probe in a child process: torch's HIP runtime cannot open the device
     in a process where the library's runtime already holds it."""
@@ -638,6 +665,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="twochqs_c3", choices=sorted(WORKLOADS))
     ap.add_argument("--eig-impl", type=int, default=2)
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="skip the eigensolver launch-configuration autotuner (library defaults)")
     ap.add_argument("--graph", action="store_true",
                     help="capture each eigensolve's launch sequences into a hipGraph (launched once)")
     ap.add_argument("--two-stage", type=int, default=None,
@@ -678,8 +707,13 @@ def main():
         print(json.dumps(c5_run(cpu=not a.no_cpu)))
         return
     ginfo = {}
+    tune = None
+    red, ts, gr = None, a.two_stage, a.graph
+    if not a.no_autotune and a.eig_impl == 2 and a.two_stage is None and not a.graph:
+        tune = autotune_run(a.workload)
+        red, ts, gr = tune["chosen"]["red_streams"], tune["chosen"]["ts_min"], bool(tune["chosen"]["graph"])
     dt, st, timed, model, w, nsteps = gpu_run(a.workload, a.steps, a.warmup, a.eig_impl, stats_all=a.stats_all,
-                                              two_stage=a.two_stage, graph=a.graph, info=ginfo)
+                                              red_streams=red, two_stage=ts, graph=gr, info=ginfo)
     value = nsteps / dt
     eig = st["eig"]
     eig_rate = eig["flops"] / (eig["ms"] * 1e-3) / 1e12 if eig["ms"] > 0 else 0.0
@@ -727,7 +761,9 @@ def main():
                        "flops_per_step": eig["flops"] / nsteps, "ms_per_step": eig["ms"] / nsteps},
         "kernel_ms_per_step": {k: v["ms"] / nsteps for k, v in st.items()},
     }
-    if a.graph:
+    if tune is not None:
+        out["autotune"] = tune
+    if gr:
         out["graph"] = {"graphs_per_step": ginfo.get("graphs", 0) / nsteps,
                         "nodes_per_step": ginfo.get("graph_nodes", 0) / nsteps,
                         "host_ms_per_step": ginfo.get("graph_host_ms", 0.0) / nsteps}

@@ -91,6 +91,26 @@ int nrg_set_graph(nrg_ctx *ctx, int on);
  * graph nodes and graphs since the last call (then reset) */
 int nrg_graph_stats(nrg_ctx *ctx, double *host_ms, long *nodes, long *graphs);
 
+/* Autotuner of the eigensolver's launch configuration for one step shape
+ * (impl 2).  Random symmetric blocks of sizes n[0..nb-1] are generated on the
+ * device and one eigensolve (every eigenvalue, krow_frac of each block's
+ * eigenvector rows: the kept rows of a truncated step) is timed per candidate,
+ * best of reps after a warm-up run: the reduction's stream count 1..3, then
+ * the two-stage thresholds 2048 and 1024 (where blocks that large exist) with
+ * the best stream count, then graph launches with the best of those; a
+ * candidate replaces the default (3 streams, one-stage, no graph) only when
+ * it is more than 2 % faster.  The choice is applied to ctx (as
+ * nrg_set_red_streams / nrg_set_two_stage / nrg_set_graph would) and every
+ * candidate's time is written to table
+ * (*n_entries of them, at most max_entries stored), the choice to *chosen.  No reference counterpart
+ * (LAPACK's dspev_/dsyev_ have no launch configuration). */
+typedef struct {
+  int red_streams, ts_min, graph, pad;
+  double ms;
+} nrg_tune_entry;
+int nrg_autotune_eig(nrg_ctx *ctx, int nb, const int *n, double krow_frac, int reps, nrg_tune_entry *table,
+                     int max_entries, int *n_entries, nrg_tune_entry *chosen);
+
 /* Diagnostics: run the batched solver (impl 2) on one n x n matrix and return
  * the stage outputs -- the tear-modified tridiagonal diagonal d (after the
  * divide-and-conquer set-up), the untorn off-diagonal e, the sorted

@@ -18,7 +18,7 @@ EXPORTS = [
     "nrg_ctx_create", "nrg_ctx_destroy", "nrg_last_error", "nrg_device_count",
     "nrg_diag_blocks", "nrg_diag_blocks_dev", "nrg_truncate", "nrg_ecut",
     "nrg_rotate_ops", "nrg_run_onechqs", "nrg_run_twochqs", "nrg_set_timing", "nrg_get_stats", "nrg_reset_stats",
-    "nrg_set_eig_impl", "nrg_set_plain", "nrg_set_red_streams", "nrg_set_two_stage", "nrg_set_graph", "nrg_graph_stats", "nrg_eig2_stages", "nrg_compact",
+    "nrg_set_eig_impl", "nrg_set_plain", "nrg_set_red_streams", "nrg_set_two_stage", "nrg_set_graph", "nrg_graph_stats", "nrg_autotune_eig", "nrg_eig2_stages", "nrg_compact",
     "nrg_dm_create", "nrg_dm_destroy", "nrg_dm_info", "nrg_dm_timing", "nrg_dm_memory", "nrg_dm_rho", "nrg_dm_spec",
     "nrg_dm_spectrum", "nrg_run_onechqs_multi", "nrg_dm_spec_omega", "nrg_dm_norm", "nrg_dm_weights",
     "nrg_run_twochqsz", "nrg_onechqs_thermo", "nrg_onechqs_jvp", "nrg_dm_create_jvp", "nrg_dm_spec_jvp",
@@ -123,6 +123,11 @@ class NrgStats(ctypes.Structure):
 _lib = None
 
 
+class TuneEntry(ctypes.Structure):
+    _fields_ = [("red_streams", ctypes.c_int), ("ts_min", ctypes.c_int), ("graph", ctypes.c_int),
+                ("pad", ctypes.c_int), ("ms", ctypes.c_double)]
+
+
 class NrgError(RuntimeError):
     pass
 
@@ -154,6 +159,9 @@ def lib():
     L.nrg_set_red_streams.argtypes = [ctypes.c_void_p, ctypes.c_int]
     L.nrg_set_two_stage.argtypes = [ctypes.c_void_p, ctypes.c_int]
     L.nrg_set_graph.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    L.nrg_autotune_eig.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.c_double,
+                                   ctypes.c_int, ctypes.POINTER(TuneEntry), ctypes.c_int,
+                                   ctypes.POINTER(ctypes.c_int), ctypes.POINTER(TuneEntry)]
     L.nrg_graph_stats.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_long),
                                   ctypes.POINTER(ctypes.c_long)]
     L.nrg_compact.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
@@ -293,6 +301,18 @@ class Context:
         ms, nodes, graphs = ctypes.c_double(), ctypes.c_long(), ctypes.c_long()
         _check(lib().nrg_graph_stats(self.h, ctypes.byref(ms), ctypes.byref(nodes), ctypes.byref(graphs)))
         return ms.value, nodes.value, graphs.value
+
+    def autotune(self, sizes, krow_frac=0.07, reps=1):
+        """Time the eigensolver's launch configurations on random blocks of these
+        sizes (nrg_autotune_eig), apply the choice, return (chosen, table) with
+        dicts {red_streams, ts_min, graph, ms}."""
+        n = np.ascontiguousarray(sizes, dtype=np.int32)
+        tab = (TuneEntry * 16)()
+        ne, ch = ctypes.c_int(), TuneEntry()
+        _check(lib().nrg_autotune_eig(self.h, len(n), n.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                      float(krow_frac), int(reps), tab, 16, ctypes.byref(ne), ctypes.byref(ch)))
+        rows = [dict(red_streams=e.red_streams, ts_min=e.ts_min, graph=e.graph, ms=e.ms) for e in tab[:ne.value]]
+        return dict(red_streams=ch.red_streams, ts_min=ch.ts_min, graph=ch.graph, ms=ch.ms), rows
 
     def compact(self, n, z_off, keep, dZ=None, dZcut=None):
         """CutStates' eigenvector compaction on device arrays (nrg_compact):
