@@ -6,7 +6,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libtwochnrg_amd.so")
-SOURCES = ["nrg_kernels.hip", "nrg_eig2.hip", "nrg_eig2s.hip", "nrg_host.cpp", "nrg_eig2_host.cpp", "nrg_driver.cpp", "nrg_dm.hip", "nrg_dm.cpp", "nrg_tangent.hip", "nrg_noalloc.hip", "nrg_bin.cpp", "nrg_tune.hip"]
+SOURCES = ["nrg_kernels.hip", "nrg_eig2.hip", "nrg_eig2s.hip", "nrg_host.cpp", "nrg_eig2_host.cpp", "nrg_driver.cpp", "nrg_dm.hip", "nrg_dm.cpp", "nrg_tangent.hip", "nrg_noalloc.hip", "nrg_bin.cpp", "nrg_tune.hip", "nrg_trunc.cpp"]
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-fPIC", "-shared", "-std=c++17",
     # keep FP32 arithmetic out of packed (v_pk_*_f32) instructions

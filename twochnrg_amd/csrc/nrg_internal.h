@@ -10,6 +10,7 @@
 #include "../../include/twochnrg_amd.h"
 #include "nrg_kernels.h"
 #include "nrg_eig2.h"
+#include "nrg_trunc.h"
 
 namespace nrg {
 int fail(int code, const std::string& msg);
