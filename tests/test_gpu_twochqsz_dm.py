@@ -1,4 +1,4 @@
-"""CFS and FDM of the TwoChQSz two-channel chain on the device (nrg_dm_create
+"""DM-NRG, CFS and FDM of the TwoChQSz two-channel chain on the device (nrg_dm_create
 with model 1: the chain keeps every state and d_up per shell, the sweep runs
 over (Q, Sz) blocks with degeneracy 1 and no Clebsch-Gordan factors, the FDM
 weights count 16 states per later site, or 4 as the reference's one-channel
@@ -141,9 +141,57 @@ def test_qsz_spectrum(ctx, runs, Nw):
     assert _rel(r, ro).max() <= 1e-8
 
 
-def test_qsz_rejects_dm_nrg_and_derivatives(ctx):
+# plain DM-NRG (UseCFS 0) over the same shells: rho_N from the kept states'
+# children, the shell sums over every pair of each shell
+DMNRG = [
+    dict(BASE, nsitesmax=5, ncut=200, Mtemp=3, betabar=0.9, method=0),
+    dict(BASE, U=0.3, Gamma=0.05, Gamma2=0.05, ed=-0.1, nsitesmax=4, ncut=300, Mtemp=3, betabar=1.1, method=0),
+]
+
+
+@pytest.mark.parametrize("i", range(len(DMNRG)))
+def test_qsz_dmnrg(ctx, i):
+    case = DMNRG[i]
+    print("twochqsz plain dm-nrg", case, flush=True)
+    o = po.DmRun(**case)
+    with ctx.dm(**case) as d:
+        assert (d.nshell_max, d.spec_nshell_max) == (o.nshell_max, o.spec_nshell_max)
+        assert abs(d.temp - o.temp) <= 1e-15 * o.temp
+        for N in range(o.nshell_max, -1, -1):
+            sh = o.shell(N)
+            sizes = [int(b1 - b0 + 1) for b0, b1 in sh["bbe"]]
+            blocks, tr = d.rho(N, sizes)
+            assert abs(tr - sh["trace"]) <= 1e-10 * abs(sh["trace"]), (N, tr, sh["trace"])
+            for b, (R, Ro) in enumerate(zip(blocks, sh["rho"])):
+                en = sh["en"][sh["bbe"][b, 0]:sh["bbe"][b, 1] + 1]
+                scale = max(1e-300, np.abs(Ro).max())
+                for c1 in _clusters(en):
+                    for c2 in _clusters(en):
+                        a = np.linalg.norm(R[np.ix_(c1, c2)])
+                        x = np.linalg.norm(Ro[np.ix_(c1, c2)])
+                        assert abs(a - x) <= 1e-9 * scale, (N, b, c1, c2, a, x)
+        sh, w = [], []
+        for N in range(o.spec_nshell_max):
+            for x in (-4.3, -1.0, -0.2, -1e-3, 1e-3, 0.05, 0.7, 5.0):
+                sh.append(N)
+                w.append(x)
+        got = d.spec(sh, w)
+        ref = np.array([o.spec_shell(N, x) for N, x in zip(sh, w)])
+        err = _rel(got, ref)
+        assert err.max() <= 1e-8, (err.max(), sh[int(err.argmax())], w[int(err.argmax())])
+    for Nw in (0, 1):
+        c2 = dict(case, Nw=Nw)
+        o2 = po.DmRun(**c2)
+        wo, ro = o2.spectrum()
+        with ctx.dm(**c2) as d:
+            wg, rg = d.spectrum()
+        np.testing.assert_allclose(wg, wo, rtol=1e-14, atol=0)
+        assert _rel(rg, ro).max() <= 1e-8
+
+
+def test_qsz_rejects_derivatives_and_bad_model(ctx):
     import twochnrg_amd as t
     with pytest.raises(t.NrgError):
-        ctx.dm(**dict(CASES[0], method=0))
-    with pytest.raises(t.NrgError):
         ctx.dm(**dict(CASES[0], method=2, model=2))
+    with pytest.raises(t.NrgError):
+        ctx.dm(**dict(DMNRG[0]), dpar=[[1.0, 0.0, 0.0]])

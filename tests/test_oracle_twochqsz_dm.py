@@ -189,3 +189,64 @@ def test_exact_fdm_sum_rule(site_dim):
         assert abs(n - 1.0) > 1e-4, (bit, n)
     # the reference's own FDM (no bit) is not normalised on this chain
     assert abs(po.DmRun(**c).norm() - 1.0) > 1e-2
+
+
+def _lehmann_dmnrg(Hn, c, rho, w, temp, dn, twindow=2.0, bt=0.8, b=0.5 * np.log(L)):
+    """DMNRG_SpecDens_M_STL's shell sum in an orthonormal basis:
+    sum_{a,a',b} rho_{a'a} c_ab c_a'b BD(E_b - E_a') + sum_{a,b,b'} rho_bb' c_ab' c_ab BD(E_b - E_a),
+    BD = BroadDelta, the Lorentzian BDeltaTemp for |dE| < twindow T / D_N."""
+    e, V = np.linalg.eigh(Hn)
+    cm = V.T @ c @ V
+    r = V.T @ rho @ V
+    dE = e[None, :] - e[:, None]
+    x = np.where(dE == 0.0, 1e-100, dE)
+    lg = np.exp(-b * b / 4) / (b * np.abs(x) * np.sqrt(np.pi)) * np.exp(-(np.log(np.abs(w / x)) / b) ** 2)
+    lg = np.where(w * x < 0, 0.0, lg)
+    lor = (1 / (2 * np.pi)) * bt / ((w - x) ** 2 + bt ** 2)
+    D = np.where(np.abs(x) < twindow * temp / dn, lor, lg)
+    return np.sum((r.T @ cm) * cm * D) + np.sum((cm @ r) * cm * D)
+
+
+@pytest.mark.parametrize("p", CASES)
+def test_dmnrg_shell_sum_is_the_reduced_lehmann_sum(p):
+    """Plain DM-NRG (UseCFS 0) on the untruncated 2-shell TwoChQSz chain at a
+    finite temperature: with Mtemp >= Nsitesmax-1 the reference takes
+    T = D_{Nsitesmax+199} / betabar (Nsitesmax = 2 shells here), so betabar is
+    chosen to give D_1 / T = 0.9.
+    rho_1 = exp(-E D_1/T) / sum exp(-E D_0/T) (DM_NRG's two scales), rho_0 its
+    Fock-space partial trace over site 1, and shell 0's sum equals the
+    DMNRG_SpecDens_M_STL expression in the Fock space of the first shell with
+    c = d_up, BroadDelta and the Lorentzian window."""
+    def dn(N):
+        return HLF * L ** (-(N - 1) / 2.0)
+    target = 0.9
+    bb = target * dn(2 + 199) / dn(1)
+    D = po.DmRun(U=p["U"], Gamma=p["G1"], ed=p["ed"], Lambda=L, nsitesmax=1, ncut=10 ** 6, Mtemp=1, betabar=bb,
+                 method=0, model=1, Gamma2=p["G2"])
+    assert D.nsh == 2 and D.nshell_max == 1
+    T = D.temp
+    assert abs(dn(1) / T - target) <= 1e-12 * target
+    H1, _ = _fock(p, 1)
+    w1, V1 = np.linalg.eigh(H1)
+    rho1 = (V1 * np.exp(-w1 * dn(1) / T)) @ V1.T / np.sum(np.exp(-w1 * dn(0) / T))
+    rho0 = _trace_out_site(rho1, 10)
+    H0, C0 = _fock(p, 0)
+    assert abs(D.shell(0)["trace"] - np.trace(rho0)) <= 1e-12 * abs(np.trace(rho0))
+    for w in (-2.3, -0.6, -0.04, 0.02, 0.35, 1.8):
+        ref = _lehmann_dmnrg(H0, C0[0], rho0, w, T, dn(0))
+        got = D.spec_shell(0, w)
+        assert abs(got - ref) <= 1e-10 * max(1e-6, abs(ref)), (w, got, ref)
+
+
+def test_dmnrg_truncated_trace_conserved():
+    """Plain DM-NRG on a truncated TwoChQSz chain: sum rho_ii is passed down the
+    sweep unchanged (the child map and the (Q, Sz) weights), and the spectrum
+    is finite."""
+    D = po.DmRun(U=0.5, Gamma=0.03, ed=-0.25, Lambda=2.5, nsitesmax=6, ncut=300, Mtemp=5, betabar=0.9,
+                 method=0, model=1, Gamma2=0.01)
+    kept = [D.shell(N)["kept"] for N in range(D.nshell_max + 1)]
+    assert any(not k.all() for k in kept)
+    tr = [D.shell(N)["trace"] for N in range(D.nshell_max + 1)]
+    assert np.ptp(tr) <= 1e-12 * abs(tr[0])
+    w, r = D.spectrum()
+    assert np.all(np.isfinite(r)) and len(w) > 0

@@ -504,8 +504,8 @@ class Context:
         kept = (flags, offsets) from kept_arrays of a calcdens-1 run freezes every
         step's Kept set; dpar rows are tangent directions (dU, dGamma, ded)
         (nrg_dm_create_jvp; DM-NRG method only).  model=1 runs the TwoChQSz
-        two-channel chain (Gamma, Gamma2 the two hybridisations; CFS or FDM) and
-        the spectral function of d_up.  fdm_site_dim: the FDM environment count
+        two-channel chain (Gamma, Gamma2 the two hybridisations; DM-NRG, CFS or
+        FDM over shells that keep every state) and the spectral function of d_up.  fdm_site_dim: the FDM environment count
         per later site (0: 4 for model 0 as the reference, 16 for model 1).
         fdm_fix: FDM_FIX_* bits switching the reference's FDM quirks off
         (FDM_EXACT: the full density matrix, whose CalcNorm is 1)."""

@@ -922,8 +922,8 @@ static int dm_create(nrg_ctx* c, const nrg_dmnrg_params* p, const unsigned char*
   if (p->fdm_fix < 0 || p->fdm_fix > NRG_FDM_EXACT || (p->fdm_fix != 0 && p->method != 2))
     return fail(NRG_ERR_ARG, "nrg_dm_create: fdm_fix takes NRG_FDM_FIX_* bits and needs FDM (method 2)");
   if (p->model < 0 || p->model > 1) return fail(NRG_ERR_ARG, "nrg_dm_create: model must be 0 (OneChQS) or 1 (TwoChQSz)");
-  if (p->model == 1 && (p->method == 0 || ntan > 0 || kept))
-    return fail(NRG_ERR_ARG, "nrg_dm_create: the TwoChQSz chain supports CFS and FDM without derivatives");
+  if (p->model == 1 && (ntan > 0 || kept))
+    return fail(NRG_ERR_ARG, "nrg_dm_create: the TwoChQSz chain has no derivative pass");
   HIPCHK(hipSetDevice(c->device));
   // the handle owns its device buffers: on any error they go back with it
   std::unique_ptr<nrg_dm> D(new nrg_dm());
