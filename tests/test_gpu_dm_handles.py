@@ -35,7 +35,7 @@ def test_handles_side_by_side(ctx):
     q = ctx.dm(**QSZ)
     rq = q.spec_omega(ws)
     with pytest.raises(t.NrgError):
-        ctx.dm(**dict(QSZ, method=0))
+        ctx.dm(**dict(QSZ, model=2))   # no such chain
     # every live handle still answers as before, to the bit
     assert np.array_equal(a.spec_omega(ws), ra)
     assert np.array_equal(b.spec(sh, [0.1, -0.2, 0.3]), rb)

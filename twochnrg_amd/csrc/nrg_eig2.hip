@@ -49,139 +49,417 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
 // ---------------------------------------------------------------------------
 // stage 1a: whole reduction inside one workgroup (n <= 256)
 // ---------------------------------------------------------------------------
-#define SM_G 4              // row groups of the small-block reduction
+#define SM_G 4              // row groups of the non-resident small-block reduction
 #define SM_T (256 * SM_G)    // its workgroup: column c = tid % 256, row group g = tid / 256
-#define RS_REG 36            // resident path: matrix elements per thread held in registers
-#define RS_LDS 16            // ... and in LDS
-#define RS_MAXN (SM_G * (RS_REG + RS_LDS))   // largest block the resident path holds (208)
+// resident path: row r = w + R2_W * i belongs to wave w, column c = lane + 64 * j to the lane
+#define R2_W (SM_T / WAVE)   // 16 waves = row groups
+#define R2_NI 13             // row slots per thread (largest variant)
+#define R2_NL 3              // its slots 0 .. R2_NL-1 (rows < 48) live in LDS, the rest in registers
+#define R2_NJ 4              // column slots (largest variant)
+#define R2_NP (R2_NI + 1)    // row-slot stride of the by-row vectors (even, so slot pairs are 16-byte aligned)
+#define RS_MAXN (R2_W * R2_NI)   // largest block the resident path holds (208)
 
-// Resident Householder reduction of one block with n <= RS_MAXN: the whole
-// matrix stays in the workgroup, column c's rows r = g (mod SM_G) with thread
-// (c, g), the first RS_REG of them in registers and the rest in LDS, so a
-// column step touches global memory only to store its reflector.  Per column
-// k: row k from its owners, the reflector from a block reduction, y = A v
-// from each thread's rows (groups added in order), w = t y - (t/2)(v.ty) v,
-// and the rank-2 update A -= v w^T + w v^T in place.  The update of element
-// (r, c) rounds the same two products and their sum as that of (c, r), so the
-// matrix stays exactly symmetric.
-__device__ __forceinline__ double rs_get(const double (&a)[RS_REG], const double (*slots)[SM_T], int s, int tid) {
-  if (s >= RS_REG) return slots[s - RS_REG][tid];
-  double v = 0.0;
-#pragma unroll
-  for (int q = 0; q < RS_REG; q++)
-    if (q == s) v = a[q];
+// LDS of the small-block kernel: the resident path's arrays (the small, often
+// read ones first, inside the 64 KiB reach of a DS instruction's offset) and
+// the non-resident path's, which are never live together
+struct alignas(16) R2Lds {
+  double vt[R2_W][R2_NP];               // v by row: vt[w][i] = v[w + 16 i]
+  double vwt[R2_W][R2_NP][2];           // (v, w) by row
+  double vn[256];                       // v by column
+  double vw[256][2];                    // (v, w) by column
+  double pw[R2_W];                      // per-wave partials of v . (A v)
+  double ts[2];                         // t of the current reflector
+  double qp[R2_W / 2][256][2];          // per-wave partial sums of (A v)_c, waves 2h and 2h+1 side by side
+  double la[R2_NL][R2_NJ / 2][SM_T][2];   // the LDS-held slots: columns (2p, 2p+1) of each thread's row
+};
+struct NrLds {
+  double red[SM_T / WAVE];
+  double vsh[256], wsh[256], vnew[256], ash[256];
+  double qpart[SM_G][256];
+};
+union alignas(16) SmallLds {
+  R2Lds r2;
+  NrLds nr;
+};
+
+__device__ __forceinline__ double2 ld2(const double* p) { return *reinterpret_cast<const double2*>(p); }
+__device__ __forceinline__ void st2(double* p, double x, double y) {
+  *reinterpret_cast<double2*>(p) = make_double2(x, y);
+}
+// the thread index as a value the compiler cannot carry across this point:
+// per-lane LDS addresses are recomputed per phase (two integer operations)
+// instead of being kept in registers over the column loop
+__device__ __forceinline__ int r2_tid() {
+  int t = threadIdx.x;
+  asm volatile("" : "+v"(t));
+  return t;
+}
+__device__ __forceinline__ double r2_sum4(double v) {   // quad sum, the same bits in each lane
+  v += nrgw::dpp_d<0xB1>(v);
+  v += nrgw::dpp_d<0x4E>(v);
+  return v;
+}
+__device__ __forceinline__ double r2_sum16(double v) {   // 16-lane row sum, the same bits in each lane
+  v += nrgw::dpp_d<0xB1>(v);
+  v += nrgw::dpp_d<0x4E>(v);
+  v += nrgw::dpp_d<0x141>(v);
+  v += nrgw::dpp_d<0x140>(v);
   return v;
 }
 
-__device__ void tridiag_resident(const int n, double* __restrict__ A, double* __restrict__ d, double* __restrict__ e,
-                                 double* __restrict__ tau, double* red, double* ash, double* vsh, double* wsh,
-                                 double (*qpart)[256], double (*slots)[SM_T]) {
-  const int tid = threadIdx.x, c = tid & 255, g = tid >> 8;
-  const int gu = __builtin_amdgcn_readfirstlane(g);   // g as a scalar (uniform per wavefront)
-  double a[RS_REG];
+// One variant of the resident reduction: NI row slots (n <= 16 NI), NJ column
+// slots (n <= 64 NJ), the first NL row slots in LDS.
+template <int NI, int NJ, int NL>
+struct R2 {
+  static constexpr int NR = NI - NL;   // row slots in registers
+  typedef double Tile[NR][NJ];
+
+  template <int I>
+  static __device__ __forceinline__ void row(const Tile& a, const R2Lds& S, int tid, double (&x)[NJ]) {
+    if constexpr (I < NL) {
 #pragma unroll
-  for (int s = 0; s < RS_REG; s++) {
-    const int r = g + SM_G * s;
-    a[s] = (r < n && c < n) ? A[(size_t)r * n + c] : 0.0;
+      for (int p = 0; p < NJ / 2; p++) {
+        const double2 v = ld2(S.la[I][p][tid]);
+        x[2 * p] = v.x;
+        x[2 * p + 1] = v.y;
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < NJ; j++) x[j] = a[I - NL][j];
+    }
   }
-  for (int s = 0; s < RS_LDS; s++) {
-    const int r = g + SM_G * (RS_REG + s);
-    slots[s][tid] = (r < n && c < n) ? A[(size_t)r * n + c] : 0.0;   // each thread's own slots: no barrier
+  template <int I>
+  static __device__ __forceinline__ void put(Tile& a, R2Lds& S, int tid, const double (&x)[NJ]) {
+    if constexpr (I < NL) {
+#pragma unroll
+      for (int p = 0; p < NJ / 2; p++) st2(S.la[I][p][tid], x[2 * p], x[2 * p + 1]);
+    } else {
+#pragma unroll
+      for (int j = 0; j < NJ; j++) a[I - NL][j] = x[j];
+    }
   }
-  // v and w are zero outside rows k+1..n-1 of the current column (rows >= n
-  // never written), so the products and updates below run over every slot
-  // with no per-slot condition: a slot of row r <= k or r >= n gains exact zeros
-  if (tid < 256) { vsh[tid] = 0.0; wsh[tid] = 0.0; }
-  __syncthreads();
-  for (int k = 0; k + 2 < n; k++) {
-    // row k of the current matrix: element (k, c) is thread (c, k mod SM_G)'s slot k / SM_G
-    const bool own = (g == k % SM_G) && c < n;
-    const double ak = own ? rs_get(a, slots, k / SM_G, tid) : 0.0;
-    if (own) ash[c] = ak;
-    const double nrm2 = block_sum((own && c > k) ? ak * ak : 0.0, red);   // its barriers order the ash stores
-    const double x0 = ash[k + 1];
+  // row slot s, s uniform at run time
+  template <int I = 0>
+  static __device__ __forceinline__ void row_rt(const Tile& a, const R2Lds& S, int tid, int s, double (&x)[NJ]) {
+    if constexpr (I < NI) {
+      if (s == I) row<I>(a, S, tid, x);
+      else row_rt<I + 1>(a, S, tid, s, x);
+    }
+  }
+
+  // q_j += a[i][j] v_(w + 16 i) over slots I .. NI-1 (I even: v of two slots per 16-byte read)
+  template <int I, int I0, int J0>
+  static __device__ __forceinline__ void symv(const Tile& a, const R2Lds& S, int tid, int w, double (&q)[NJ]) {
+    if constexpr (I < NI) {
+      const double2 vv = ld2(&S.vt[w][I]);
+      if constexpr (I >= I0) {
+        double x[NJ];
+        row<I>(a, S, tid, x);
+#pragma unroll
+        for (int j = J0; j < NJ; j++) q[j] = __fma_rn(x[j], vv.x, q[j]);
+      }
+      if constexpr (I + 1 < NI) {
+        double x[NJ];
+        row<I + 1>(a, S, tid, x);
+#pragma unroll
+        for (int j = J0; j < NJ; j++) q[j] = __fma_rn(x[j], vv.y, q[j]);
+      }
+      symv<I + 2, I0, J0>(a, S, tid, w, q);
+    }
+  }
+  // a[i][j] -= v_r w_c + w_r v_c over slots I .. NI-1
+  template <int I, int J0>
+  static __device__ __forceinline__ void update(Tile& a, R2Lds& S, int tid, int w, const double2 (&cw)[NJ]) {
+    if constexpr (I < NI) {
+      const double2 rw = ld2(S.vwt[w][I]);   // (v_r, w_r)
+      double x[NJ];
+      row<I>(a, S, tid, x);
+#pragma unroll
+      for (int j = J0; j < NJ; j++) {
+        x[j] = __fma_rn(-rw.x, cw[j].y, x[j]);
+        x[j] = __fma_rn(-rw.y, cw[j].x, x[j]);
+      }
+      put<I>(a, S, tid, x);
+      if constexpr ((I & 1) == 1) __builtin_amdgcn_sched_barrier(0);   // (v_r, w_r) of two rows in flight at most
+      update<I + 1, J0>(a, S, tid, w, cw);
+    }
+  }
+
+  // reflector kk from row kk of the matrix (x: this wave's row slot, all lanes),
+  // published by column (vn), by row (vt), in row kk of A, and as d, e, tau, t
+  static __device__ __forceinline__ void reflector(const int n, const int kk, const double (&x)[NJ], R2Lds& S,
+                                                   double* __restrict__ A, double* __restrict__ d,
+                                                   double* __restrict__ e, double* __restrict__ tau, int l) {
+    double ss = 0.0, xk1 = 0.0, xkk = 0.0;
+#pragma unroll
+    for (int j = 0; j < NJ; j++) {
+      const int c = l + WAVE * j;
+      if (c > kk) ss = __fma_rn(x[j], x[j], ss);
+      if (j == (kk + 1) / WAVE) xk1 = x[j];
+      if (j == kk / WAVE) xkk = x[j];
+    }
+    const double nrm2 = nrgw::wave_sum_bc(ss);
+    const double x0 = nrgw::readlane_d(xk1, (kk + 1) & (WAVE - 1));   // element (kk, kk+1)
+    const double akk = nrgw::readlane_d(xkk, kk & (WAVE - 1));        // element (kk, kk)
     double t = 0.0, v0 = x0, alpha = 0.0;
     if (nrm2 != 0.0) {
       const double nrm = sqrt(nrm2);
       alpha = (x0 > 0.0) ? -nrm : nrm;
       v0 = x0 - alpha;
-      t = 2.0 / (nrm2 - x0 * x0 + v0 * v0);
+      // t = 2 / |v|^2 = 1 / (nrm (nrm + |x0|)): hardware reciprocal and one
+      // Newton step (within an ulp or so, the rounding of any other t)
+      const double den = nrm * (nrm + fabs(x0));
+      const double r0 = __builtin_amdgcn_rcp(den);
+      t = __fma_rn(r0, __fma_rn(-den, r0, 1.0), r0);
     }
-    if (g == 0 && c < n) {
-      const double vc = (c > k + 1) ? ash[c] : (c == k + 1 ? v0 : 0.0);
-      vsh[c] = vc;
-      if (c > k) A[(size_t)k * n + c] = vc;   // reflector k in row k
+#pragma unroll
+    for (int j = 0; j < NJ; j++) {
+      const int c = l + WAVE * j;
+      const double vc = (c > kk + 1) ? x[j] : (c == kk + 1 ? v0 : 0.0);
+      S.vn[c] = vc;
+      if (c < R2_W * NI) S.vt[c & (R2_W - 1)][c / R2_W] = vc;
+      if (c > kk && c < n) A[(size_t)kk * n + c] = vc;   // reflector kk in row kk
     }
-    if (tid == 0) { d[k] = ash[k]; e[k] = alpha; tau[k] = t; }
-    __syncthreads();
-    // y_c over this thread's rows r > k
-    const bool act = c > k && c < n;
-    double q = 0.0;
-    // slots below s_lo hold rows <= k, finished.  g is the same across a
-    // wavefront, so s_lo is a scalar and whole groups of 4 slots are skipped by
-    // scalar branches (a live group's rows <= k meet zeros of v and w)
-    const int s_lo = (k + 1 - gu + SM_G - 1) / SM_G;
-    if (act) {
+    if (l == 0) {
+      d[kk] = akk;
+      e[kk] = alpha;
+      tau[kk] = t;
+      S.ts[0] = t;
+    }
+  }
+
+  // row k+1 (slot I of this wave) after the update of reflector k, then
+  // reflector k+1 from it
+  template <int I, int J0>
+  static __device__ __forceinline__ void next_row(const int n, const int k, Tile& a, R2Lds& S,
+                                                  double* __restrict__ A, double* __restrict__ d,
+                                                  double* __restrict__ e, double* __restrict__ tau, int tid, int l,
+                                                  double vr, double wr, const double (&vc)[NJ],
+                                                  const double (&wc)[NJ]) {
+    double x[NJ];
+    row<I>(a, S, tid, x);
 #pragma unroll
-      for (int s0 = 0; s0 < RS_REG; s0 += 4)
-        if (s0 + 3 >= s_lo) {
+    for (int j = J0; j < NJ; j++) {
+      x[j] = __fma_rn(-vr, wc[j], x[j]);
+      x[j] = __fma_rn(-wr, vc[j], x[j]);
+    }
+    put<I>(a, S, tid, x);
+    if (k + 3 < n) reflector(n, k + 1, x, S, A, d, e, tau, l);
+  }
+
+  // columns k of stage I0 (k in [16 I0, 16 I0 + 16), k + 2 < n): rows k and
+  // k + 1 are in slots I0 and I0 or I0 + 1, every row of slots below I0 is
+  // finished (v = w = 0 there, so slot I0's finished rows and any column <= k
+  // are left unchanged, exactly), and column slots below J0 hold only finished
+  // columns.  Reflector k is published when the stage's column k begins.
+  template <int I0>
+  static __device__ __forceinline__ void stage(const int n, Tile& a, R2Lds& S, double* __restrict__ A,
+                                               double* __restrict__ d, double* __restrict__ e,
+                                               double* __restrict__ tau, int w) {
+    constexpr int J0 = ((R2_W * I0 + 1) / WAVE < NJ) ? (R2_W * I0 + 1) / WAVE : NJ - 1;
+    const int k1 = min(R2_W * (I0 + 1), n - 2);
+    for (int k = R2_W * I0; k < k1; k++) {
+      // 1. this wave's partial (A v)_c over its rows, and its partial of v . (A v)
+      {
+        const int tid = r2_tid(), l = tid & (WAVE - 1);
+        double q[NJ];
 #pragma unroll
-          for (int u = 0; u < 4; u++) q = __fma_rn(a[s0 + u], vsh[gu + SM_G * (s0 + u)], q);
+        for (int j = 0; j < NJ; j++) q[j] = 0.0;
+        symv<I0 & ~1, I0, J0>(a, S, tid, w, q);
+        double ss = 0.0;
+#pragma unroll
+        for (int j = J0; j < NJ; j++) {
+          const int c = l + WAVE * j;
+          S.qp[w >> 1][c][w & 1] = q[j];
+          ss = __fma_rn(S.vn[c], q[j], ss);   // v_c = 0 for c <= k and c >= n
         }
-      for (int s0 = (max(0, s_lo - RS_REG) / 4) * 4; s0 < RS_LDS; s0 += 4) {
-#pragma unroll
-        for (int u = 0; u < 4; u++) q = __fma_rn(slots[s0 + u][tid], vsh[gu + SM_G * (RS_REG + s0 + u)], q);
+        const double pv = nrgw::wave_sum_bc(ss);
+        if (l == 0) S.pw[w] = pv;
       }
-    }
-    qpart[g][c] = q;
-    __syncthreads();
-    double qc = 0.0;
-    if (act) {
+      __syncthreads();
+      // 2. the wave holding row k+1: (A v)_c from the 16 partials, w = t A v -
+      // (t^2/2)(v.A v) v published by column and by row (zero for row k+1,
+      // which it updates itself), then row k+1 and reflector k+1
+      if (w == ((k + 1) & (R2_W - 1))) {
+        const int tid = r2_tid(), l = tid & (WAVE - 1);
+        const double vav = r2_sum16(S.pw[l & (R2_W - 1)]);
+        const double t = S.ts[0];
+        const double K = 0.5 * t * (t * vav);
+        double vc[NJ], wc[NJ];
 #pragma unroll
-      for (int h = 0; h < SM_G; h++) qc += qpart[h][c];
-      qc *= t;
-    }
-    const double vc = (c < n) ? vsh[c] : 0.0;
-    const double vp = block_sum((g == 0 && act) ? qc * vc : 0.0, red);
-    const double K = 0.5 * t * vp;
-    const double wc = act ? qc - K * vc : 0.0;
-    if (g == 0) wsh[c] = wc;
-    __syncthreads();
-    // A -= v w^T + w v^T as a - (v_r w_c + v_c w_r), rounded products and sum
-    // (no contraction): the sum commutes exactly, so (r, c) and (c, r) agree
-    if (act) {
+        for (int j = 0; j < NJ; j++) {
+          vc[j] = 0.0;
+          wc[j] = 0.0;
+        }
 #pragma unroll
-      for (int s0 = 0; s0 < RS_REG; s0 += 4)
-        if (s0 + 3 >= s_lo) {
+        for (int j = J0; j < NJ; j++) {
+          const int c = l + WAVE * j;
+          double p[R2_W / 2];
 #pragma unroll
-          for (int u = 0; u < 4; u++) {
-            const int r = gu + SM_G * (s0 + u);
-            a[s0 + u] = __dsub_rn(a[s0 + u], __dadd_rn(__dmul_rn(vsh[r], wc), __dmul_rn(vc, wsh[r])));
+          for (int h = 0; h < R2_W / 2; h++) {   // waves 2h + 2h+1, then pairs of those, ...
+            const double2 pr = ld2(S.qp[h][c]);
+            p[h] = pr.x + pr.y;
+          }
+#pragma unroll
+          for (int m = 1; m < R2_W / 2; m *= 2)
+#pragma unroll
+            for (int h = 0; h < R2_W / 2; h += 2 * m) p[h] += p[h + m];
+          const double y = p[0];
+          vc[j] = S.vn[c];
+          wc[j] = (c > k && c < n) ? t * y - K * vc[j] : 0.0;
+          st2(S.vw[c], vc[j], wc[j]);
+          if (c < R2_W * NI) {
+            const bool mine = c == k + 1;
+            st2(S.vwt[c & (R2_W - 1)][c / R2_W], mine ? 0.0 : vc[j], mine ? 0.0 : wc[j]);
           }
         }
-      for (int s0 = (max(0, s_lo - RS_REG) / 4) * 4; s0 < RS_LDS; s0 += 4) {
+        double vr = 0.0, wr = 0.0;   // v and w at k+1
 #pragma unroll
-        for (int u = 0; u < 4; u++) {
-          const int r = gu + SM_G * (RS_REG + s0 + u);
-          slots[s0 + u][tid] =
-              __dsub_rn(slots[s0 + u][tid], __dadd_rn(__dmul_rn(vsh[r], wc), __dmul_rn(vc, wsh[r])));
+        for (int j = J0; j < NJ; j++)
+          if (j == (k + 1) / WAVE) {
+            vr = vc[j];
+            wr = wc[j];
+          }
+        vr = nrgw::readlane_d(vr, (k + 1) & (WAVE - 1));
+        wr = nrgw::readlane_d(wr, (k + 1) & (WAVE - 1));
+        if ((k + 1) / R2_W == I0) {
+          next_row<I0, J0>(n, k, a, S, A, d, e, tau, tid, l, vr, wr, vc, wc);
+        } else {
+          if constexpr (I0 + 1 < NI) next_row<I0 + 1, J0>(n, k, a, S, A, d, e, tau, tid, l, vr, wr, vc, wc);
         }
+      }
+      __syncthreads();
+      // 3. A -= v w^T + w v^T on this thread's slots I0 .. NI-1
+      {
+        const int tid = r2_tid(), l = tid & (WAVE - 1);
+        double2 cw[NJ];
+#pragma unroll
+        for (int j = J0; j < NJ; j++) cw[j] = ld2(S.vw[l + WAVE * j]);
+        update<I0, J0>(a, S, tid, w, cw);
+      }
+    }
+    if (k1 == n - 2) {   // the last stage: rows n-2 and n-1 (slots I0 or I0 + 1) are final
+      const int tid = r2_tid(), l = tid & (WAVE - 1);
+      trailing<I0>(n, a, S, d, e, tid, w, l);
+      if constexpr (I0 + 1 < NI) trailing<I0 + 1>(n, a, S, d, e, tid, w, l);
+    }
+  }
+  // d[n-2], e[n-2] and d[n-1] from the trailing 2 x 2 block's owners, if slot I holds them
+  template <int I>
+  static __device__ __forceinline__ void trailing(const int n, const Tile& a, const R2Lds& S, double* __restrict__ d,
+                                                  double* __restrict__ e, int tid, int w, int l) {
+    if (w == ((n - 2) & (R2_W - 1)) && (n - 2) / R2_W == I) {
+      double x[NJ];
+      row<I>(a, S, tid, x);
+#pragma unroll
+      for (int j = 0; j < NJ; j++) {
+        const int c = l + WAVE * j;
+        if (c == n - 2) d[n - 2] = x[j];
+        if (c == n - 1) e[n - 2] = x[j];
+      }
+    }
+    if (w == ((n - 1) & (R2_W - 1)) && (n - 1) / R2_W == I) {
+      double x[NJ];
+      row<I>(a, S, tid, x);
+#pragma unroll
+      for (int j = 0; j < NJ; j++)
+        if (l + WAVE * j == n - 1) d[n - 1] = x[j];
+    }
+  }
+  template <int I0>
+  static __device__ __forceinline__ void stages(const int n, Tile& a, R2Lds& S, double* __restrict__ A,
+                                                double* __restrict__ d, double* __restrict__ e,
+                                                double* __restrict__ tau, int w) {
+    if constexpr (I0 < NI) {
+      if (R2_W * I0 + 2 < n) {
+        stage<I0>(n, a, S, A, d, e, tau, w);
+        stages<I0 + 1>(n, a, S, A, d, e, tau, w);
       }
     }
   }
-  // the trailing 2 x 2 (or 1 x 1) block from its owners
-  if (c < n) {
-    if (n >= 2 && g == (n - 2) % SM_G) {
-      const double x = rs_get(a, slots, (n - 2) / SM_G, tid);   // element (n-2, c)
-      if (c == n - 2) d[n - 2] = x;
-      if (c == n - 1) e[n - 2] = x;
+
+  static __device__ __forceinline__ void run(const int n, double* __restrict__ A, double* __restrict__ d, double* __restrict__ e,
+                             double* __restrict__ tau, R2Lds& S) {
+    const int tid = threadIdx.x, l = tid & (WAVE - 1);
+    const int w = __builtin_amdgcn_readfirstlane(tid / WAVE);
+    if (n <= 2) {   // already tridiagonal
+      if (tid == 0) {
+        d[0] = A[0];
+        e[n - 1] = 0.0;
+        if (n == 2) {
+          e[0] = A[1];
+          d[1] = A[3];
+        }
+        for (int k = 0; k < n; k++) tau[k] = 0.0;
+      }
+      return;
     }
-    if (g == (n - 1) % SM_G && c == n - 1) d[n - 1] = rs_get(a, slots, (n - 1) / SM_G, tid);
+    Tile a;
+#pragma unroll
+    for (int i = 0; i < NI; i++) {
+      const int r = w + R2_W * i;
+      double x[NJ];
+#pragma unroll
+      for (int j = 0; j < NJ; j++) {
+        const int c = l + WAVE * j;
+        x[j] = (r < n && c < n) ? A[(size_t)r * n + c] : 0.0;
+      }
+      if (i < NL) {
+#pragma unroll
+        for (int p = 0; p < NJ / 2; p++) st2(S.la[i][p][tid], x[2 * p], x[2 * p + 1]);
+      } else {
+#pragma unroll
+        for (int j = 0; j < NJ; j++) a[i - NL][j] = x[j];
+      }
+      __builtin_amdgcn_sched_barrier(0);   // one row of loads in flight at a time: the registers hold the matrix
+    }
+    // v and w by row: the padding slots and the rows >= 16 NI are read (as zeros) but never written
+    for (int q = tid; q < R2_W * R2_NP; q += SM_T) {
+      (&S.vt[0][0])[q] = 0.0;
+      st2(&S.vwt[0][0][0] + 2 * q, 0.0, 0.0);
+    }
+    __syncthreads();
+    if (w == 0) {   // reflector 0 from row 0 (slot 0 of wave 0)
+      double x[NJ];
+      row<0>(a, S, tid, x);
+      reflector(n, 0, x, S, A, d, e, tau, l);
+    }
+    __syncthreads();
+    stages<0>(n, a, S, A, d, e, tau, w);   // the last one also writes the trailing 2 x 2 block
+    if (tid == 0) {
+      e[n - 1] = 0.0;
+      tau[n - 2] = 0.0;
+      tau[n - 1] = 0.0;
+    }
   }
-  if (tid == 0) {
-    e[n - 1] = 0.0;
-    for (int k = (n >= 2 ? n - 2 : 0); k < n; k++) tau[k] = 0.0;
-  }
+};
+
+// Resident Householder reduction of one block with n <= RS_MAXN (one 1024-thread
+// workgroup).  The whole matrix stays in the workgroup: wave w holds rows
+// w + 16 i, each lane columns lane + 64 j, the two first row slots of the
+// largest variant in LDS and the rest in registers, so that every value of v
+// or w read from LDS serves all of a thread's columns.  Reflector k is
+// published before column k begins; per column k, two barriers:
+//   1. every wave forms its rows' partial (A v)_c and its partial of v.(A v);
+//   2. the wave holding row k+1 adds the 16 partials of each column, forms
+//      w = t A v - (t^2/2)(v.A v) v and publishes (v, w) by column and by row,
+//      updates row k+1 and forms and publishes reflector k+1 (norm by a wave
+//      reduction);
+// then each thread applies A -= v w^T + w v^T to its other elements (two
+// fused multiply-adds each).  The column loop runs in stages of 16 columns,
+// each compiled for the row and column slots that can still change, so no
+// per-slot condition is evaluated at run time; six variants sized to n (up
+// to 64, 96, 128, 160, 192, 208 rows) keep the slots past n out of the work.
+// The reflector is written to row k of A (the back-transformation reads it
+// there), with d, e and tau.
+__device__ __forceinline__ void tridiag_resident(const int n, double* __restrict__ A, double* __restrict__ d, double* __restrict__ e,
+                                 double* __restrict__ tau, R2Lds& S) {
+  if (n <= 64) R2<4, 1, 0>::run(n, A, d, e, tau, S);
+  else if (n <= 96) R2<6, 2, 0>::run(n, A, d, e, tau, S);
+  else if (n <= 128) R2<8, 2, 0>::run(n, A, d, e, tau, S);
+  else if (n <= 160) R2<10, 3, 0>::run(n, A, d, e, tau, S);
+  else if (n <= 192) R2<12, 3, 0>::run(n, A, d, e, tau, S);
+  else R2<R2_NI, R2_NJ, R2_NL>::run(n, A, d, e, tau, S);
 }
 
 __global__ __launch_bounds__(SM_T) void tridiag_small_kernel(const E2Block* __restrict__ blks,
@@ -195,10 +473,13 @@ __global__ __launch_bounds__(SM_T) void tridiag_small_kernel(const E2Block* __re
   // applied inside the pass of column k that also forms A v: every trailing
   // element is read once, updated, written and used in the product.  The
   // update of element (j, i) uses the same operations as that of (i, j).
-  __shared__ double red[SM_T / WAVE];
-  __shared__ double vsh[256], wsh[256], vnew[256], ash[256];
-  __shared__ double qpart[SM_G][256];
-  __shared__ double slots[RS_LDS][SM_T];
+  __shared__ SmallLds sm;
+  double* red = sm.nr.red;
+  double* vsh = sm.nr.vsh;
+  double* wsh = sm.nr.wsh;
+  double* vnew = sm.nr.vnew;
+  double* ash = sm.nr.ash;
+  double(*qpart)[256] = sm.nr.qpart;
   const E2Block B = blks[list[blockIdx.x]];
   const int n = B.n, tid = threadIdx.x;
   const int c = tid & 255, g = tid >> 8;
@@ -207,7 +488,7 @@ __global__ __launch_bounds__(SM_T) void tridiag_small_kernel(const E2Block* __re
   double* e = d + n;
   double* tau = e + n;
   if (n <= RS_MAXN) {
-    tridiag_resident(n, A, d, e, tau, red, ash, vsh, wsh, qpart, slots);
+    tridiag_resident(n, A, d, e, tau, sm.r2);
     return;
   }
   bool pending = false;   // vsh/wsh hold the update of the previous column

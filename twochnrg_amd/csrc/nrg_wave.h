@@ -33,4 +33,24 @@ __device__ __forceinline__ double wave_sum(double v) {
   return (readlane_d(v, 0) + readlane_d(v, 16)) + (readlane_d(v, 32) + readlane_d(v, 48));
 }
 
+// The same sum with the cross-row steps done by DPP row broadcasts (row_bcast:15
+// into rows 1 and 3, then row_bcast:31 into rows 2 and 3), so lane 63 holds
+// (r2 + r3) + (r0 + r1) -- bitwise the value wave_sum returns, addition being
+// commutative -- and two lane reads instead of eight hand it to every lane.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_rows_d(double v) {   // rows outside ROWS read 0
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROWS, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROWS, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double wave_sum_bc(double v) {
+  v += dpp_d<0xB1>(v);
+  v += dpp_d<0x4E>(v);
+  v += dpp_d<0x141>(v);
+  v += dpp_d<0x140>(v);                 // every lane: its row's sum
+  v += dpp_rows_d<0x142, 0xA>(v);       // rows 1, 3: + lane 15 of rows 0, 2
+  v += dpp_rows_d<0x143, 0xC>(v);       // rows 2, 3: + lane 31 (rows 0 + 1)
+  return readlane_d(v, 63);
+}
+
 }  // namespace nrgw
