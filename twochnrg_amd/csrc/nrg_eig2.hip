@@ -54,9 +54,8 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
 // resident path: row r = w + R2_W * i belongs to wave w, column c = lane + 64 * j to the lane
 #define R2_W (SM_T / WAVE)   // 16 waves = row groups
 #define R2_NI 13             // row slots per thread (largest variant)
-#define R2_NL 3              // its slots 0 .. R2_NL-1 (rows < 48) live in LDS, the rest in registers
-#define R2_NJ 4              // column slots (largest variant)
-#define R2_NP (R2_NI + 1)    // row-slot stride of the by-row vectors (even, so slot pairs are 16-byte aligned)
+#define R2_NP 16             // row-slot stride of the by-row vectors (even: slot pairs are 16-byte aligned;
+                             // slots NI .. 15 stay zero)
 #define RS_MAXN (R2_W * R2_NI)   // largest block the resident path holds (208)
 
 // LDS of the small-block kernel: the resident path's arrays (the small, often
@@ -68,9 +67,10 @@ struct alignas(16) R2Lds {
   double vn[256];                       // v by column
   double vw[256][2];                    // (v, w) by column
   double pw[R2_W];                      // per-wave partials of v . (A v)
+  double ycol[R2_W];                    // (A v) at the corner columns, one per corner row
+  double fold[R2_W];                    // corner rows' elements at the next reflector's column
   double ts[2];                         // t of the current reflector
   double qp[R2_W / 2][256][2];          // per-wave partial sums of (A v)_c, waves 2h and 2h+1 side by side
-  double la[R2_NL][R2_NJ / 2][SM_T][2];   // the LDS-held slots: columns (2p, 2p+1) of each thread's row
 };
 struct NrLds {
   double red[SM_T / WAVE];
@@ -94,11 +94,6 @@ __device__ __forceinline__ int r2_tid() {
   asm volatile("" : "+v"(t));
   return t;
 }
-__device__ __forceinline__ double r2_sum4(double v) {   // quad sum, the same bits in each lane
-  v += nrgw::dpp_d<0xB1>(v);
-  v += nrgw::dpp_d<0x4E>(v);
-  return v;
-}
 __device__ __forceinline__ double r2_sum16(double v) {   // 16-lane row sum, the same bits in each lane
   v += nrgw::dpp_d<0xB1>(v);
   v += nrgw::dpp_d<0x4E>(v);
@@ -107,92 +102,64 @@ __device__ __forceinline__ double r2_sum16(double v) {   // 16-lane row sum, the
   return v;
 }
 
-// One variant of the resident reduction: NI row slots (n <= 16 NI), NJ column
-// slots (n <= 64 NJ), the first NL row slots in LDS.
-template <int NI, int NJ, int NL>
+// One variant of the resident reduction: NI row slots (n <= 16 NI) and NJ
+// column slots of 64 columns, held in registers.  With CORNER the block also
+// has the 16 columns 64 NJ .. 64 NJ + 15 (NI = 4 NJ + 1: rows 64 NJ .. are the
+// last row slot): rows below 64 NJ keep no copy of them -- by symmetry they are
+// the last row slot's rows, element (r, c) = (c, r) -- and the last slot's rows
+// keep their 16 x 16 corner in one register (lane m: column 64 NJ + m).  The
+// vectors v and w see these columns as column slot NJ, lanes 16 .. 63 past n;
+// (A v)_c there is the row c of the last slot times v, one wave sum by the wave
+// holding it, and the wave forming reflector kk reads row kk's entries there
+// from the holders' elements (c, kk), published before the barrier.
+template <int NI, int NJ, bool CORNER>
 struct R2 {
-  static constexpr int NR = NI - NL;   // row slots in registers
-  typedef double Tile[NR][NJ];
-
-  template <int I>
-  static __device__ __forceinline__ void row(const Tile& a, const R2Lds& S, int tid, double (&x)[NJ]) {
-    if constexpr (I < NL) {
-#pragma unroll
-      for (int p = 0; p < NJ / 2; p++) {
-        const double2 v = ld2(S.la[I][p][tid]);
-        x[2 * p] = v.x;
-        x[2 * p + 1] = v.y;
-      }
-    } else {
-#pragma unroll
-      for (int j = 0; j < NJ; j++) x[j] = a[I - NL][j];
-    }
-  }
-  template <int I>
-  static __device__ __forceinline__ void put(Tile& a, R2Lds& S, int tid, const double (&x)[NJ]) {
-    if constexpr (I < NL) {
-#pragma unroll
-      for (int p = 0; p < NJ / 2; p++) st2(S.la[I][p][tid], x[2 * p], x[2 * p + 1]);
-    } else {
-#pragma unroll
-      for (int j = 0; j < NJ; j++) a[I - NL][j] = x[j];
-    }
-  }
-  // row slot s, s uniform at run time
-  template <int I = 0>
-  static __device__ __forceinline__ void row_rt(const Tile& a, const R2Lds& S, int tid, int s, double (&x)[NJ]) {
-    if constexpr (I < NI) {
-      if (s == I) row<I>(a, S, tid, x);
-      else row_rt<I + 1>(a, S, tid, s, x);
-    }
-  }
+  static constexpr int NJL = NJ + (CORNER ? 1 : 0);   // column slots of the vectors (column lane + 64 j)
+  static constexpr int CF = WAVE * NJ;                 // first corner column
+  static constexpr int IC = NI - 1;                    // the corner rows' slot
+  static_assert(!CORNER || R2_W * IC == CF, "the corner rows are the columns past the column slots");
+  typedef double Tile[NI][NJ];
 
   // q_j += a[i][j] v_(w + 16 i) over slots I .. NI-1 (I even: v of two slots per 16-byte read)
   template <int I, int I0, int J0>
-  static __device__ __forceinline__ void symv(const Tile& a, const R2Lds& S, int tid, int w, double (&q)[NJ]) {
+  static __device__ __forceinline__ void symv(const Tile& a, const R2Lds& S, int w, double (&q)[NJL]) {
     if constexpr (I < NI) {
       const double2 vv = ld2(&S.vt[w][I]);
       if constexpr (I >= I0) {
-        double x[NJ];
-        row<I>(a, S, tid, x);
 #pragma unroll
-        for (int j = J0; j < NJ; j++) q[j] = __fma_rn(x[j], vv.x, q[j]);
+        for (int j = J0; j < NJ; j++) q[j] = __fma_rn(a[I][j], vv.x, q[j]);
       }
       if constexpr (I + 1 < NI) {
-        double x[NJ];
-        row<I + 1>(a, S, tid, x);
 #pragma unroll
-        for (int j = J0; j < NJ; j++) q[j] = __fma_rn(x[j], vv.y, q[j]);
+        for (int j = J0; j < NJ; j++) q[j] = __fma_rn(a[I + 1][j], vv.y, q[j]);
       }
-      symv<I + 2, I0, J0>(a, S, tid, w, q);
+      symv<I + 2, I0, J0>(a, S, w, q);
     }
   }
   // a[i][j] -= v_r w_c + w_r v_c over slots I .. NI-1
   template <int I, int J0>
-  static __device__ __forceinline__ void update(Tile& a, R2Lds& S, int tid, int w, const double2 (&cw)[NJ]) {
+  static __device__ __forceinline__ void update(Tile& a, const R2Lds& S, int w, const double2 (&cw)[NJ]) {
     if constexpr (I < NI) {
       const double2 rw = ld2(S.vwt[w][I]);   // (v_r, w_r)
-      double x[NJ];
-      row<I>(a, S, tid, x);
 #pragma unroll
       for (int j = J0; j < NJ; j++) {
-        x[j] = __fma_rn(-rw.x, cw[j].y, x[j]);
-        x[j] = __fma_rn(-rw.y, cw[j].x, x[j]);
+        a[I][j] = __fma_rn(-rw.x, cw[j].y, a[I][j]);
+        a[I][j] = __fma_rn(-rw.y, cw[j].x, a[I][j]);
       }
-      put<I>(a, S, tid, x);
       if constexpr ((I & 1) == 1) __builtin_amdgcn_sched_barrier(0);   // (v_r, w_r) of two rows in flight at most
-      update<I + 1, J0>(a, S, tid, w, cw);
+      update<I + 1, J0>(a, S, w, cw);
     }
   }
 
-  // reflector kk from row kk of the matrix (x: this wave's row slot, all lanes),
-  // published by column (vn), by row (vt), in row kk of A, and as d, e, tau, t
-  static __device__ __forceinline__ void reflector(const int n, const int kk, const double (&x)[NJ], R2Lds& S,
+  // reflector kk from row kk of the matrix (x: column slots of this wave's row,
+  // all lanes), published by column (vn), by row (vt), in row kk of A, and as
+  // d, e, tau, t
+  static __device__ __forceinline__ void reflector(const int n, const int kk, const double (&x)[NJL], R2Lds& S,
                                                    double* __restrict__ A, double* __restrict__ d,
                                                    double* __restrict__ e, double* __restrict__ tau, int l) {
     double ss = 0.0, xk1 = 0.0, xkk = 0.0;
 #pragma unroll
-    for (int j = 0; j < NJ; j++) {
+    for (int j = 0; j < NJL; j++) {
       const int c = l + WAVE * j;
       if (c > kk) ss = __fma_rn(x[j], x[j], ss);
       if (j == (kk + 1) / WAVE) xk1 = x[j];
@@ -213,7 +180,7 @@ struct R2 {
       t = __fma_rn(r0, __fma_rn(-den, r0, 1.0), r0);
     }
 #pragma unroll
-    for (int j = 0; j < NJ; j++) {
+    for (int j = 0; j < NJL; j++) {
       const int c = l + WAVE * j;
       const double vc = (c > kk + 1) ? x[j] : (c == kk + 1 ? v0 : 0.0);
       S.vn[c] = vc;
@@ -228,23 +195,45 @@ struct R2 {
     }
   }
 
-  // row k+1 (slot I of this wave) after the update of reflector k, then
-  // reflector k+1 from it
+  // row kk = k+1 (slot I of this wave) after the update of reflector k; then
+  // reflector kk from it, or, after the last column, d and e of the trailing
+  // 2 x 2 block
   template <int I, int J0>
-  static __device__ __forceinline__ void next_row(const int n, const int k, Tile& a, R2Lds& S,
+  static __device__ __forceinline__ void next_row(const int n, const int k, Tile& a, double& cr, R2Lds& S,
                                                   double* __restrict__ A, double* __restrict__ d,
-                                                  double* __restrict__ e, double* __restrict__ tau, int tid, int l,
-                                                  double vr, double wr, const double (&vc)[NJ],
-                                                  const double (&wc)[NJ]) {
-    double x[NJ];
-    row<I>(a, S, tid, x);
+                                                  double* __restrict__ e, double* __restrict__ tau, int l, double vr,
+                                                  double wr, const double (&vc)[NJL], const double (&wc)[NJL]) {
+    const int kk = k + 1;
+    double x[NJL];
 #pragma unroll
     for (int j = J0; j < NJ; j++) {
-      x[j] = __fma_rn(-vr, wc[j], x[j]);
-      x[j] = __fma_rn(-wr, vc[j], x[j]);
+      a[I][j] = __fma_rn(-vr, wc[j], a[I][j]);
+      a[I][j] = __fma_rn(-wr, vc[j], a[I][j]);
     }
-    put<I>(a, S, tid, x);
-    if (k + 3 < n) reflector(n, k + 1, x, S, A, d, e, tau, l);
+#pragma unroll
+    for (int j = 0; j < NJ; j++) x[j] = a[I][j];
+    if constexpr (CORNER) {
+      if constexpr (I == IC) {   // the row's own corner
+        cr = __fma_rn(-vr, wc[NJ], cr);
+        cr = __fma_rn(-wr, vc[NJ], cr);
+        x[NJ] = cr;
+      } else {   // elements (CF + m, kk) of the corner rows, updated as their holders will
+        double y = l < 16 ? S.fold[l] : 0.0;
+        y = __fma_rn(-vc[NJ], wr, y);
+        y = __fma_rn(-wc[NJ], vr, y);
+        x[NJ] = y;
+      }
+    }
+    if (kk + 2 < n) {
+      reflector(n, kk, x, S, A, d, e, tau, l);
+    } else {
+#pragma unroll
+      for (int j = 0; j < NJL; j++) {
+        const int c = l + WAVE * j;
+        if (c == n - 2) d[n - 2] = x[j];
+        if (c == n - 1) e[n - 2] = x[j];
+      }
+    }
   }
 
   // columns k of stage I0 (k in [16 I0, 16 I0 + 16), k + 2 < n): rows k and
@@ -253,19 +242,20 @@ struct R2 {
   // are left unchanged, exactly), and column slots below J0 hold only finished
   // columns.  Reflector k is published when the stage's column k begins.
   template <int I0>
-  static __device__ __forceinline__ void stage(const int n, Tile& a, R2Lds& S, double* __restrict__ A,
+  static __device__ __forceinline__ void stage(const int n, Tile& a, double& cr, R2Lds& S, double* __restrict__ A,
                                                double* __restrict__ d, double* __restrict__ e,
                                                double* __restrict__ tau, int w) {
-    constexpr int J0 = ((R2_W * I0 + 1) / WAVE < NJ) ? (R2_W * I0 + 1) / WAVE : NJ - 1;
+    constexpr int J0 = ((R2_W * I0 + 1) / WAVE < NJL) ? (R2_W * I0 + 1) / WAVE : NJL - 1;
     const int k1 = min(R2_W * (I0 + 1), n - 2);
+#pragma unroll 1
     for (int k = R2_W * I0; k < k1; k++) {
       // 1. this wave's partial (A v)_c over its rows, and its partial of v . (A v)
       {
-        const int tid = r2_tid(), l = tid & (WAVE - 1);
-        double q[NJ];
+        const int l = r2_tid() & (WAVE - 1);
+        double q[NJL];
 #pragma unroll
-        for (int j = 0; j < NJ; j++) q[j] = 0.0;
-        symv<I0 & ~1, I0, J0>(a, S, tid, w, q);
+        for (int j = 0; j < NJL; j++) q[j] = 0.0;
+        symv<I0 & ~1, I0, J0>(a, S, w, q);
         double ss = 0.0;
 #pragma unroll
         for (int j = J0; j < NJ; j++) {
@@ -273,7 +263,29 @@ struct R2 {
           S.qp[w >> 1][c][w & 1] = q[j];
           ss = __fma_rn(S.vn[c], q[j], ss);   // v_c = 0 for c <= k and c >= n
         }
-        const double pv = nrgw::wave_sum_bc(ss);
+        double pv;
+        if constexpr (CORNER) {
+          // (A v) at column CF + w: row CF + w (this wave's corner row) times v
+          double yr = 0.0;
+#pragma unroll
+          for (int j = J0; j < NJ; j++) yr = __fma_rn(a[IC][j], S.vn[l + WAVE * j], yr);
+          yr = __fma_rn(cr, S.vn[CF + l], yr);   // lanes 16 .. 63: cr = 0
+          pv = ss;
+          nrgw::wave_sum_bc2(pv, yr);
+          pv = __fma_rn(S.vn[CF + w], yr, pv);
+          if (l == 0) S.ycol[w] = yr;
+          // element (CF + w, k + 1) for the wave forming reflector k + 1
+          const int kk = k + 1;
+          if (kk < CF && l == (kk & (WAVE - 1))) {
+            double x = 0.0;
+#pragma unroll
+            for (int j = 0; j < NJ; j++)
+              if (j == kk / WAVE) x = a[IC][j];
+            S.fold[w] = x;
+          }
+        } else {
+          pv = nrgw::wave_sum_bc(ss);
+        }
         if (l == 0) S.pw[w] = pv;
       }
       __syncthreads();
@@ -281,30 +293,35 @@ struct R2 {
       // (t^2/2)(v.A v) v published by column and by row (zero for row k+1,
       // which it updates itself), then row k+1 and reflector k+1
       if (w == ((k + 1) & (R2_W - 1))) {
-        const int tid = r2_tid(), l = tid & (WAVE - 1);
+        const int l = r2_tid() & (WAVE - 1);
         const double vav = r2_sum16(S.pw[l & (R2_W - 1)]);
         const double t = S.ts[0];
         const double K = 0.5 * t * (t * vav);
-        double vc[NJ], wc[NJ];
+        double vc[NJL], wc[NJL];
 #pragma unroll
-        for (int j = 0; j < NJ; j++) {
+        for (int j = 0; j < NJL; j++) {
           vc[j] = 0.0;
           wc[j] = 0.0;
         }
 #pragma unroll
-        for (int j = J0; j < NJ; j++) {
+        for (int j = J0; j < NJL; j++) {
           const int c = l + WAVE * j;
-          double p[R2_W / 2];
+          double y;
+          if (j < NJ) {
+            double p[R2_W / 2];
 #pragma unroll
-          for (int h = 0; h < R2_W / 2; h++) {   // waves 2h + 2h+1, then pairs of those, ...
-            const double2 pr = ld2(S.qp[h][c]);
-            p[h] = pr.x + pr.y;
+            for (int h = 0; h < R2_W / 2; h++) {   // waves 2h + 2h+1, then pairs of those, ...
+              const double2 pr = ld2(S.qp[h][c]);
+              p[h] = pr.x + pr.y;
+            }
+#pragma unroll
+            for (int m = 1; m < R2_W / 2; m *= 2)
+#pragma unroll
+              for (int h = 0; h < R2_W / 2; h += 2 * m) p[h] += p[h + m];
+            y = p[0];
+          } else {
+            y = S.ycol[l & (R2_W - 1)];   // lanes 16 .. 63 are past n
           }
-#pragma unroll
-          for (int m = 1; m < R2_W / 2; m *= 2)
-#pragma unroll
-            for (int h = 0; h < R2_W / 2; h += 2 * m) p[h] += p[h + m];
-          const double y = p[0];
           vc[j] = S.vn[c];
           wc[j] = (c > k && c < n) ? t * y - K * vc[j] : 0.0;
           st2(S.vw[c], vc[j], wc[j]);
@@ -312,10 +329,11 @@ struct R2 {
             const bool mine = c == k + 1;
             st2(S.vwt[c & (R2_W - 1)][c / R2_W], mine ? 0.0 : vc[j], mine ? 0.0 : wc[j]);
           }
+          __builtin_amdgcn_sched_barrier(0);   // one column slot's partials in flight at a time
         }
         double vr = 0.0, wr = 0.0;   // v and w at k+1
 #pragma unroll
-        for (int j = J0; j < NJ; j++)
+        for (int j = J0; j < NJL; j++)
           if (j == (k + 1) / WAVE) {
             vr = vc[j];
             wr = wc[j];
@@ -323,63 +341,59 @@ struct R2 {
         vr = nrgw::readlane_d(vr, (k + 1) & (WAVE - 1));
         wr = nrgw::readlane_d(wr, (k + 1) & (WAVE - 1));
         if ((k + 1) / R2_W == I0) {
-          next_row<I0, J0>(n, k, a, S, A, d, e, tau, tid, l, vr, wr, vc, wc);
+          next_row<I0, J0>(n, k, a, cr, S, A, d, e, tau, l, vr, wr, vc, wc);
         } else {
-          if constexpr (I0 + 1 < NI) next_row<I0 + 1, J0>(n, k, a, S, A, d, e, tau, tid, l, vr, wr, vc, wc);
+          if constexpr (I0 + 1 < NI) next_row<I0 + 1, J0>(n, k, a, cr, S, A, d, e, tau, l, vr, wr, vc, wc);
         }
       }
       __syncthreads();
       // 3. A -= v w^T + w v^T on this thread's slots I0 .. NI-1
       {
-        const int tid = r2_tid(), l = tid & (WAVE - 1);
+        const int l = r2_tid() & (WAVE - 1);
         double2 cw[NJ];
 #pragma unroll
         for (int j = J0; j < NJ; j++) cw[j] = ld2(S.vw[l + WAVE * j]);
-        update<I0, J0>(a, S, tid, w, cw);
+        update<I0, J0>(a, S, w, cw);
+        if constexpr (CORNER) {   // lanes 16 .. 63 read (v, w) = 0 and keep cr = 0
+          const double2 rw = ld2(S.vwt[w][IC]);
+          const double2 c2 = ld2(S.vw[CF + l]);
+          cr = __fma_rn(-rw.x, c2.y, cr);
+          cr = __fma_rn(-rw.y, c2.x, cr);
+        }
       }
     }
-    if (k1 == n - 2) {   // the last stage: rows n-2 and n-1 (slots I0 or I0 + 1) are final
-      const int tid = r2_tid(), l = tid & (WAVE - 1);
-      trailing<I0>(n, a, S, d, e, tid, w, l);
-      if constexpr (I0 + 1 < NI) trailing<I0 + 1>(n, a, S, d, e, tid, w, l);
+    if (k1 == n - 2) {   // the last stage: row n-1 (slot I0 or I0 + 1) is final
+      const int l = r2_tid() & (WAVE - 1);
+      trailing<I0>(n, a, cr, d, w, l);
+      if constexpr (I0 + 1 < NI) trailing<I0 + 1>(n, a, cr, d, w, l);
     }
   }
-  // d[n-2], e[n-2] and d[n-1] from the trailing 2 x 2 block's owners, if slot I holds them
+  // d[n-1] from row n-1's owner, if slot I holds it
   template <int I>
-  static __device__ __forceinline__ void trailing(const int n, const Tile& a, const R2Lds& S, double* __restrict__ d,
-                                                  double* __restrict__ e, int tid, int w, int l) {
-    if (w == ((n - 2) & (R2_W - 1)) && (n - 2) / R2_W == I) {
-      double x[NJ];
-      row<I>(a, S, tid, x);
-#pragma unroll
-      for (int j = 0; j < NJ; j++) {
-        const int c = l + WAVE * j;
-        if (c == n - 2) d[n - 2] = x[j];
-        if (c == n - 1) e[n - 2] = x[j];
-      }
-    }
+  static __device__ __forceinline__ void trailing(const int n, const Tile& a, double cr, double* __restrict__ d, int w,
+                                                  int l) {
     if (w == ((n - 1) & (R2_W - 1)) && (n - 1) / R2_W == I) {
-      double x[NJ];
-      row<I>(a, S, tid, x);
 #pragma unroll
       for (int j = 0; j < NJ; j++)
-        if (l + WAVE * j == n - 1) d[n - 1] = x[j];
+        if (l + WAVE * j == n - 1) d[n - 1] = a[I][j];
+      if constexpr (CORNER && I == IC)
+        if (CF + l == n - 1) d[n - 1] = cr;
     }
   }
   template <int I0>
-  static __device__ __forceinline__ void stages(const int n, Tile& a, R2Lds& S, double* __restrict__ A,
+  static __device__ __forceinline__ void stages(const int n, Tile& a, double& cr, R2Lds& S, double* __restrict__ A,
                                                 double* __restrict__ d, double* __restrict__ e,
                                                 double* __restrict__ tau, int w) {
     if constexpr (I0 < NI) {
       if (R2_W * I0 + 2 < n) {
-        stage<I0>(n, a, S, A, d, e, tau, w);
-        stages<I0 + 1>(n, a, S, A, d, e, tau, w);
+        stage<I0>(n, a, cr, S, A, d, e, tau, w);
+        stages<I0 + 1>(n, a, cr, S, A, d, e, tau, w);
       }
     }
   }
 
-  static __device__ __forceinline__ void run(const int n, double* __restrict__ A, double* __restrict__ d, double* __restrict__ e,
-                             double* __restrict__ tau, R2Lds& S) {
+  static __device__ __forceinline__ void run(const int n, double* __restrict__ A, double* __restrict__ d,
+                                             double* __restrict__ e, double* __restrict__ tau, R2Lds& S) {
     const int tid = threadIdx.x, l = tid & (WAVE - 1);
     const int w = __builtin_amdgcn_readfirstlane(tid / WAVE);
     if (n <= 2) {   // already tridiagonal
@@ -398,34 +412,31 @@ struct R2 {
 #pragma unroll
     for (int i = 0; i < NI; i++) {
       const int r = w + R2_W * i;
-      double x[NJ];
 #pragma unroll
       for (int j = 0; j < NJ; j++) {
         const int c = l + WAVE * j;
-        x[j] = (r < n && c < n) ? A[(size_t)r * n + c] : 0.0;
-      }
-      if (i < NL) {
-#pragma unroll
-        for (int p = 0; p < NJ / 2; p++) st2(S.la[i][p][tid], x[2 * p], x[2 * p + 1]);
-      } else {
-#pragma unroll
-        for (int j = 0; j < NJ; j++) a[i - NL][j] = x[j];
+        a[i][j] = (r < n && c < n) ? A[(size_t)r * n + c] : 0.0;
       }
       __builtin_amdgcn_sched_barrier(0);   // one row of loads in flight at a time: the registers hold the matrix
     }
+    double cr = 0.0;   // corner: element (CF + w, CF + l), lanes 0 .. 15
+    if (CORNER && CF + w < n && l < 16 && CF + l < n) cr = A[(size_t)(CF + w) * n + CF + l];
     // v and w by row: the padding slots and the rows >= 16 NI are read (as zeros) but never written
     for (int q = tid; q < R2_W * R2_NP; q += SM_T) {
       (&S.vt[0][0])[q] = 0.0;
       st2(&S.vwt[0][0][0] + 2 * q, 0.0, 0.0);
     }
+    if (CORNER && l == 0) S.fold[w] = a[IC][0];   // element (CF + w, 0) for reflector 0
     __syncthreads();
     if (w == 0) {   // reflector 0 from row 0 (slot 0 of wave 0)
-      double x[NJ];
-      row<0>(a, S, tid, x);
+      double x[NJL];
+#pragma unroll
+      for (int j = 0; j < NJ; j++) x[j] = a[0][j];
+      if constexpr (CORNER) x[NJ] = l < 16 ? S.fold[l] : 0.0;
       reflector(n, 0, x, S, A, d, e, tau, l);
     }
     __syncthreads();
-    stages<0>(n, a, S, A, d, e, tau, w);   // the last one also writes the trailing 2 x 2 block
+    stages<0>(n, a, cr, S, A, d, e, tau, w);   // the last one also writes the trailing 2 x 2 block
     if (tid == 0) {
       e[n - 1] = 0.0;
       tau[n - 2] = 0.0;
@@ -435,10 +446,10 @@ struct R2 {
 };
 
 // Resident Householder reduction of one block with n <= RS_MAXN (one 1024-thread
-// workgroup).  The whole matrix stays in the workgroup: wave w holds rows
-// w + 16 i, each lane columns lane + 64 j, the two first row slots of the
-// largest variant in LDS and the rest in registers, so that every value of v
-// or w read from LDS serves all of a thread's columns.  Reflector k is
+// workgroup).  The whole matrix stays in registers: wave w holds rows
+// w + 16 i, each lane columns lane + 64 j (past 192 rows, the last 16 columns
+// by symmetry, with a 16 x 16 corner), so that every value of v or w read from
+// LDS serves all of a thread's columns.  Reflector k is
 // published before column k begins; per column k, two barriers:
 //   1. every wave forms its rows' partial (A v)_c and its partial of v.(A v);
 //   2. the wave holding row k+1 adds the 16 partials of each column, forms
@@ -454,12 +465,12 @@ struct R2 {
 // there), with d, e and tau.
 __device__ __forceinline__ void tridiag_resident(const int n, double* __restrict__ A, double* __restrict__ d, double* __restrict__ e,
                                  double* __restrict__ tau, R2Lds& S) {
-  if (n <= 64) R2<4, 1, 0>::run(n, A, d, e, tau, S);
-  else if (n <= 96) R2<6, 2, 0>::run(n, A, d, e, tau, S);
-  else if (n <= 128) R2<8, 2, 0>::run(n, A, d, e, tau, S);
-  else if (n <= 160) R2<10, 3, 0>::run(n, A, d, e, tau, S);
-  else if (n <= 192) R2<12, 3, 0>::run(n, A, d, e, tau, S);
-  else R2<R2_NI, R2_NJ, R2_NL>::run(n, A, d, e, tau, S);
+  if (n <= 64) R2<4, 1, false>::run(n, A, d, e, tau, S);
+  else if (n <= 96) R2<6, 2, false>::run(n, A, d, e, tau, S);
+  else if (n <= 128) R2<8, 2, false>::run(n, A, d, e, tau, S);
+  else if (n <= 160) R2<10, 3, false>::run(n, A, d, e, tau, S);
+  else if (n <= 192) R2<12, 3, false>::run(n, A, d, e, tau, S);
+  else R2<R2_NI, 3, true>::run(n, A, d, e, tau, S);
 }
 
 __global__ __launch_bounds__(SM_T) void tridiag_small_kernel(const E2Block* __restrict__ blks,
