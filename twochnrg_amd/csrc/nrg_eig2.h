@@ -139,6 +139,13 @@ struct E2BtRows {
 // on block blk[i] for cum[i] <= x < cum[i+1], so no workgroup is launched for
 // a block that is smaller than the largest or already reduced.
 #define E2_TDB_MAXB 32
+#define E2_RS_MAXN 208     // largest matrix the register-resident reduction holds
+// a large block's column chain stops at panel boundary k0 (n - k0 <= E2_RS_MAXN)
+// and the resident reduction finishes its trailing matrix
+struct E2Hand {
+  int blk;
+  int k0;
+};
 struct E2TdbGrid {
   int nb;
   int cum[E2_TDB_MAXB + 1];
@@ -160,8 +167,7 @@ hipError_t e2_launch_tdb_fin(const E2Block* b, const int* blk_idx, const int* bl
                              double* WS, hipStream_t s);
 hipError_t e2_launch_gemm3(const E2Gemm* probs, const E2Tile* tiles, int ntiles, const double* A, const double* B,
                            double* C, hipStream_t s);
-hipError_t e2_launch_tridiag_tail(const E2Block* b, const int* list, int nlist, const double* A, double* WS,
-                                  hipStream_t s);
+hipError_t e2_launch_tridiag_hand(const E2Block* b, const E2Hand* hands, int nh, double* A, double* WS, hipStream_t s);
 hipError_t e2_launch_dc_init(const E2Block* b, int nb, int nmax, const E2Merge* merges, int nmerge, const int* mbeg,
                              double* WS, hipStream_t s);
 hipError_t e2_launch_dc_leaf(const E2Block* b, const E2Leaf* leaves, int nleaf, double* WS, hipStream_t s);

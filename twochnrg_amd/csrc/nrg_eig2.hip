@@ -57,6 +57,7 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
 #define R2_NP 16             // row-slot stride of the by-row vectors (even: slot pairs are 16-byte aligned;
                              // slots NI .. 15 stay zero)
 #define RS_MAXN (R2_W * R2_NI)   // largest block the resident path holds (208)
+static_assert(RS_MAXN == E2_RS_MAXN, "the host's hand-off size");
 
 // LDS of the small-block kernel: the resident path's arrays (the small, often
 // read ones first, inside the 64 KiB reach of a DS instruction's offset) and
@@ -154,8 +155,8 @@ struct R2 {
   // reflector kk from row kk of the matrix (x: column slots of this wave's row,
   // all lanes), published by column (vn), by row (vt), in row kk of A, and as
   // d, e, tau, t
-  static __device__ __forceinline__ void reflector(const int n, const int kk, const double (&x)[NJL], R2Lds& S,
-                                                   double* __restrict__ A, double* __restrict__ d,
+  static __device__ __forceinline__ void reflector(const int n, const int ld, const int kk, const double (&x)[NJL],
+                                                   R2Lds& S, double* __restrict__ A, double* __restrict__ d,
                                                    double* __restrict__ e, double* __restrict__ tau, int l) {
     double ss = 0.0, xk1 = 0.0, xkk = 0.0;
 #pragma unroll
@@ -185,7 +186,7 @@ struct R2 {
       const double vc = (c > kk + 1) ? x[j] : (c == kk + 1 ? v0 : 0.0);
       S.vn[c] = vc;
       if (c < R2_W * NI) S.vt[c & (R2_W - 1)][c / R2_W] = vc;
-      if (c > kk && c < n) A[(size_t)kk * n + c] = vc;   // reflector kk in row kk
+      if (c > kk && c < n) A[(size_t)kk * ld + c] = vc;   // reflector kk in row kk
     }
     if (l == 0) {
       d[kk] = akk;
@@ -199,7 +200,7 @@ struct R2 {
   // reflector kk from it, or, after the last column, d and e of the trailing
   // 2 x 2 block
   template <int I, int J0>
-  static __device__ __forceinline__ void next_row(const int n, const int k, Tile& a, double& cr, R2Lds& S,
+  static __device__ __forceinline__ void next_row(const int n, const int ld, const int k, Tile& a, double& cr, R2Lds& S,
                                                   double* __restrict__ A, double* __restrict__ d,
                                                   double* __restrict__ e, double* __restrict__ tau, int l, double vr,
                                                   double wr, const double (&vc)[NJL], const double (&wc)[NJL]) {
@@ -225,7 +226,7 @@ struct R2 {
       }
     }
     if (kk + 2 < n) {
-      reflector(n, kk, x, S, A, d, e, tau, l);
+      reflector(n, ld, kk, x, S, A, d, e, tau, l);
     } else {
 #pragma unroll
       for (int j = 0; j < NJL; j++) {
@@ -242,7 +243,8 @@ struct R2 {
   // are left unchanged, exactly), and column slots below J0 hold only finished
   // columns.  Reflector k is published when the stage's column k begins.
   template <int I0>
-  static __device__ __forceinline__ void stage(const int n, Tile& a, double& cr, R2Lds& S, double* __restrict__ A,
+  static __device__ __forceinline__ void stage(const int n, const int ld, Tile& a, double& cr, R2Lds& S,
+                                               double* __restrict__ A,
                                                double* __restrict__ d, double* __restrict__ e,
                                                double* __restrict__ tau, int w) {
     constexpr int J0 = ((R2_W * I0 + 1) / WAVE < NJL) ? (R2_W * I0 + 1) / WAVE : NJL - 1;
@@ -341,9 +343,9 @@ struct R2 {
         vr = nrgw::readlane_d(vr, (k + 1) & (WAVE - 1));
         wr = nrgw::readlane_d(wr, (k + 1) & (WAVE - 1));
         if ((k + 1) / R2_W == I0) {
-          next_row<I0, J0>(n, k, a, cr, S, A, d, e, tau, l, vr, wr, vc, wc);
+          next_row<I0, J0>(n, ld, k, a, cr, S, A, d, e, tau, l, vr, wr, vc, wc);
         } else {
-          if constexpr (I0 + 1 < NI) next_row<I0 + 1, J0>(n, k, a, cr, S, A, d, e, tau, l, vr, wr, vc, wc);
+          if constexpr (I0 + 1 < NI) next_row<I0 + 1, J0>(n, ld, k, a, cr, S, A, d, e, tau, l, vr, wr, vc, wc);
         }
       }
       __syncthreads();
@@ -381,19 +383,26 @@ struct R2 {
     }
   }
   template <int I0>
-  static __device__ __forceinline__ void stages(const int n, Tile& a, double& cr, R2Lds& S, double* __restrict__ A,
+  static __device__ __forceinline__ void stages(const int n, const int ld, Tile& a, double& cr, R2Lds& S,
+                                                double* __restrict__ A,
                                                 double* __restrict__ d, double* __restrict__ e,
                                                 double* __restrict__ tau, int w) {
     if constexpr (I0 < NI) {
       if (R2_W * I0 + 2 < n) {
-        stage<I0>(n, a, cr, S, A, d, e, tau, w);
-        stages<I0 + 1>(n, a, cr, S, A, d, e, tau, w);
+        stage<I0>(n, ld, a, cr, S, A, d, e, tau, w);
+        stages<I0 + 1>(n, ld, a, cr, S, A, d, e, tau, w);
       }
     }
   }
 
-  static __device__ __forceinline__ void run(const int n, double* __restrict__ A, double* __restrict__ d,
-                                             double* __restrict__ e, double* __restrict__ tau, R2Lds& S) {
+  // the n x n matrix at A with row stride ld: in full (lower = false) or its
+  // lower triangle only, the upper triangle read from it by symmetry
+  static __device__ __forceinline__ double load(const double* __restrict__ A, int ld, bool lower, int r, int c) {
+    return (lower && r < c) ? A[(size_t)c * ld + r] : A[(size_t)r * ld + c];
+  }
+  static __device__ __forceinline__ void run(const int n, double* __restrict__ A, const int ld, const bool lower,
+                                             double* __restrict__ d, double* __restrict__ e, double* __restrict__ tau,
+                                             R2Lds& S) {
     const int tid = threadIdx.x, l = tid & (WAVE - 1);
     const int w = __builtin_amdgcn_readfirstlane(tid / WAVE);
     if (n <= 2) {   // already tridiagonal
@@ -401,8 +410,8 @@ struct R2 {
         d[0] = A[0];
         e[n - 1] = 0.0;
         if (n == 2) {
-          e[0] = A[1];
-          d[1] = A[3];
+          e[0] = load(A, ld, lower, 0, 1);
+          d[1] = A[ld + 1];
         }
         for (int k = 0; k < n; k++) tau[k] = 0.0;
       }
@@ -415,12 +424,12 @@ struct R2 {
 #pragma unroll
       for (int j = 0; j < NJ; j++) {
         const int c = l + WAVE * j;
-        a[i][j] = (r < n && c < n) ? A[(size_t)r * n + c] : 0.0;
+        a[i][j] = (r < n && c < n) ? load(A, ld, lower, r, c) : 0.0;
       }
       __builtin_amdgcn_sched_barrier(0);   // one row of loads in flight at a time: the registers hold the matrix
     }
     double cr = 0.0;   // corner: element (CF + w, CF + l), lanes 0 .. 15
-    if (CORNER && CF + w < n && l < 16 && CF + l < n) cr = A[(size_t)(CF + w) * n + CF + l];
+    if (CORNER && CF + w < n && l < 16 && CF + l < n) cr = load(A, ld, lower, CF + w, CF + l);
     // v and w by row: the padding slots and the rows >= 16 NI are read (as zeros) but never written
     for (int q = tid; q < R2_W * R2_NP; q += SM_T) {
       (&S.vt[0][0])[q] = 0.0;
@@ -433,10 +442,10 @@ struct R2 {
 #pragma unroll
       for (int j = 0; j < NJ; j++) x[j] = a[0][j];
       if constexpr (CORNER) x[NJ] = l < 16 ? S.fold[l] : 0.0;
-      reflector(n, 0, x, S, A, d, e, tau, l);
+      reflector(n, ld, 0, x, S, A, d, e, tau, l);
     }
     __syncthreads();
-    stages<0>(n, a, cr, S, A, d, e, tau, w);   // the last one also writes the trailing 2 x 2 block
+    stages<0>(n, ld, a, cr, S, A, d, e, tau, w);   // the last one also writes the trailing 2 x 2 block
     if (tid == 0) {
       e[n - 1] = 0.0;
       tau[n - 2] = 0.0;
@@ -463,14 +472,15 @@ struct R2 {
 // to 64, 96, 128, 160, 192, 208 rows) keep the slots past n out of the work.
 // The reflector is written to row k of A (the back-transformation reads it
 // there), with d, e and tau.
-__device__ __forceinline__ void tridiag_resident(const int n, double* __restrict__ A, double* __restrict__ d, double* __restrict__ e,
-                                 double* __restrict__ tau, R2Lds& S) {
-  if (n <= 64) R2<4, 1, false>::run(n, A, d, e, tau, S);
-  else if (n <= 96) R2<6, 2, false>::run(n, A, d, e, tau, S);
-  else if (n <= 128) R2<8, 2, false>::run(n, A, d, e, tau, S);
-  else if (n <= 160) R2<10, 3, false>::run(n, A, d, e, tau, S);
-  else if (n <= 192) R2<12, 3, false>::run(n, A, d, e, tau, S);
-  else R2<R2_NI, 3, true>::run(n, A, d, e, tau, S);
+__device__ __forceinline__ void tridiag_resident(const int n, double* __restrict__ A, const int ld, const bool lower,
+                                                 double* __restrict__ d, double* __restrict__ e,
+                                                 double* __restrict__ tau, R2Lds& S) {
+  if (n <= 64) R2<4, 1, false>::run(n, A, ld, lower, d, e, tau, S);
+  else if (n <= 96) R2<6, 2, false>::run(n, A, ld, lower, d, e, tau, S);
+  else if (n <= 128) R2<8, 2, false>::run(n, A, ld, lower, d, e, tau, S);
+  else if (n <= 160) R2<10, 3, false>::run(n, A, ld, lower, d, e, tau, S);
+  else if (n <= 192) R2<12, 3, false>::run(n, A, ld, lower, d, e, tau, S);
+  else R2<R2_NI, 3, true>::run(n, A, ld, lower, d, e, tau, S);
 }
 
 __global__ __launch_bounds__(SM_T) void tridiag_small_kernel(const E2Block* __restrict__ blks,
@@ -499,7 +509,7 @@ __global__ __launch_bounds__(SM_T) void tridiag_small_kernel(const E2Block* __re
   double* e = d + n;
   double* tau = e + n;
   if (n <= RS_MAXN) {
-    tridiag_resident(n, A, d, e, tau, sm.r2);
+    tridiag_resident(n, A, n, false, d, e, tau, sm.r2);
     return;
   }
   bool pending = false;   // vsh/wsh hold the update of the previous column
@@ -604,6 +614,23 @@ __global__ __launch_bounds__(SM_T) void tridiag_small_kernel(const E2Block* __re
     e[n - 1] = 0.0;
     for (int k = (n >= 2 ? n - 2 : 0); k < n; k++) tau[k] = 0.0;
   }
+}
+
+// The trailing matrix of a large block whose column chain stopped at panel
+// boundary k0 (n - k0 <= RS_MAXN): its lower triangle is current there (the
+// panel products keep it so) and the resident reduction continues from it,
+// the upper triangle read by symmetry, writing reflectors k0 .. into their rows
+// and d, e, tau from k0 on.
+__global__ __launch_bounds__(SM_T) void tridiag_hand_kernel(const E2Block* __restrict__ blks,
+                                                             const E2Hand* __restrict__ hands, double* __restrict__ A0,
+                                                             double* __restrict__ WS) {
+  __shared__ R2Lds S;
+  const E2Hand H = hands[blockIdx.x];
+  const E2Block B = blks[H.blk];
+  const int n = B.n, k0 = H.k0;
+  double* A = A0 + B.a_off + (size_t)k0 * n + k0;
+  double* d = WS + B.v_off;
+  tridiag_resident(n - k0, A, n, true, d + k0, d + n + k0, d + 2 * (size_t)n + k0, S);
 }
 
 // ---------------------------------------------------------------------------
@@ -989,25 +1016,6 @@ __global__ __launch_bounds__(CHUNK * FG) void tdb_fin_kernel(const E2Block* __re
   if (panel_end) return;
   const double sq = block_sum((g == 0 && i > k + 1 && in) ? a * a : 0.0, red);
   if (tid == 0) P.npart[c] = sq;
-}
-
-__global__ void tridiag_tail_kernel(const E2Block* __restrict__ blks, const int* __restrict__ list,
-                                    const double* __restrict__ A0, double* __restrict__ WS) {
-  const E2Block B = blks[list[blockIdx.x]];
-  if (threadIdx.x != 0) return;
-  const int n = B.n;
-  const double* A = A0 + B.a_off;
-  double* d = WS + B.v_off;
-  double* e = d + n;
-  double* tau = e + n;
-  if (n >= 2) {
-    d[n - 2] = A[(size_t)(n - 2) * n + (n - 2)];
-    e[n - 2] = A[(size_t)(n - 2) * n + (n - 1)];
-    tau[n - 2] = 0.0;
-  }
-  d[n - 1] = A[(size_t)(n - 1) * n + (n - 1)];
-  e[n - 1] = 0.0;
-  tau[n - 1] = 0.0;
 }
 
 // ---------------------------------------------------------------------------
@@ -1858,6 +1866,12 @@ hipError_t e2_launch_tridiag_small(const E2Block* b, const int* list, int nlist,
   return hipGetLastError();
 }
 
+hipError_t e2_launch_tridiag_hand(const E2Block* b, const E2Hand* hands, int nh, double* A, double* WS, hipStream_t s) {
+  if (nh <= 0) return hipSuccess;
+  hipLaunchKernelGGL(e2::tridiag_hand_kernel, dim3(nh), dim3(SM_T), 0, s, b, hands, A, WS);
+  return hipGetLastError();
+}
+
 hipError_t e2_launch_tdb_norm(const E2Block* b, const int* list, int nlist, int nmax, int k, double* A, double* WS,
                               hipStream_t s) {
   if (nlist <= 0 || k + 2 >= nmax) return hipSuccess;
@@ -1902,12 +1916,6 @@ hipError_t e2_launch_gemm3(const E2Gemm* probs, const E2Tile* tiles, int ntiles,
   return hipGetLastError();
 }
 
-hipError_t e2_launch_tridiag_tail(const E2Block* b, const int* list, int nlist, const double* A, double* WS,
-                                  hipStream_t s) {
-  if (nlist <= 0) return hipSuccess;
-  hipLaunchKernelGGL(e2::tridiag_tail_kernel, dim3(nlist), dim3(64), 0, s, b, list, A, WS);
-  return hipGetLastError();
-}
 
 hipError_t e2_launch_dc_init(const E2Block* b, int nb, int nmax, const E2Merge* merges, int nmerge, const int* mbeg,
                              double* WS, hipStream_t s) {

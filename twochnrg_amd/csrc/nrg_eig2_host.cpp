@@ -244,6 +244,13 @@ static int e2_phase_a(nrg_ctx* c, const std::vector<E2Block>& blk, double* dA, d
   // column launches runs on a stream of its own, so that one group's
   // latency-bound launches overlap another's bandwidth-bound products
   std::sort(large.begin(), large.end(), [&](int x, int y) { return blk[x].n > blk[y].n; });
+  // each large block's column chain stops at the panel boundary khand, from
+  // which its trailing matrix (at most E2_RS_MAXN rows) is reduced by the
+  // register-resident kernel: its last columns would otherwise be two
+  // latency-bound launches each.  khand grows with n, so in a group sorted by
+  // n the blocks still on the chain at column k are a prefix of the group.
+  std::vector<int> khand(nb, 0);
+  for (int b : large) khand[b] = kPanel * ((blk[b].n - E2_RS_MAXN + kPanel - 1) / kPanel);
   const int ngrp = std::min<int>(c->plain ? 1 : c->red_streams, (int)large.size());
   std::vector<std::vector<int>> grp(ngrp);
   {
@@ -263,13 +270,13 @@ static int e2_phase_a(nrg_ctx* c, const std::vector<E2Block>& blk, double* dA, d
   std::vector<E2Tile> ttiles;
   std::vector<std::vector<int>> tev_k(ngrp), tev_pbeg(ngrp), tev_tbeg(ngrp), tev_tcnt(ngrp);
   for (int g = 0; g < ngrp; g++) {
-    const int nmax_g = grp[g].empty() ? 0 : blk[grp[g][0]].n;
-    for (int k = 0; k + 2 < nmax_g; k++) {
+    const int kend_g = grp[g].empty() ? 0 : khand[grp[g][0]];
+    for (int k = 0; k < kend_g; k++) {
       const int j = k % kPanel, k0 = k - j;
       const int pbase = (int)tprobs.size(), tbase = (int)ttiles.size();
       for (int b : grp[g]) {
         const int nb_ = blk[b].n;
-        if (k + 2 >= nb_ || !(j == kPanel - 1 || k + 3 >= nb_)) continue;
+        if (k >= khand[b] || j != kPanel - 1) continue;   // the panel ending at khand - 1 is the last
         const int s0 = k0 + std::min(kPanel, nb_ - 2 - k0);
         const int64_t nn = nb_;
         E2Gemm gm{};
@@ -360,6 +367,10 @@ static int e2_phase_a(nrg_ctx* c, const std::vector<E2Block>& blk, double* dA, d
     goff[g + 1] = (int)glist.size();
   }
   if ((rc = upload(c, "e2_large", glist, &d_large))) return rc;
+  std::vector<E2Hand> hands;
+  for (int b : glist) hands.push_back(E2Hand{b, khand[b]});
+  E2Hand* d_hand;
+  if ((rc = upload(c, "e2_hand", hands, &d_hand))) return rc;
   if ((rc = upload(c, "e2_leaf", leaves, &d_leaf))) return rc;
   if ((rc = upload(c, "e2_all", all_merges, &d_all))) return rc;
   if ((rc = upload(c, "e2_mbeg", mbeg, &d_mbeg))) return rc;
@@ -390,32 +401,38 @@ static int e2_phase_a(nrg_ctx* c, const std::vector<E2Block>& blk, double* dA, d
     std::vector<hipStream_t> gs(ngrp, s);
     for (int g = 0; g < ngrp; g++) gs[g] = lane_stream(lane_g0 + g);
     std::vector<std::vector<int>> gn(ngrp);
-    std::vector<int> nmax_g(ngrp);
+    std::vector<int> nmax_g(ngrp), kend_g(ngrp);
     std::vector<size_t> ev(ngrp, 0);
     int kmax = 0;
     for (int g = 0; g < ngrp; g++) {
       for (int b : grp[g]) gn[g].push_back(blk[b].n);
       nmax_g[g] = gn[g][0];
-      kmax = std::max(kmax, nmax_g[g]);
+      kend_g[g] = khand[grp[g][0]];
+      kmax = std::max(kmax, kend_g[g]);
       HIPCHK(e2_launch_tdb_norm(d_blk, d_large + goff[g], (int)grp[g].size(), nmax_g[g], 0, dA, WS, gs[g]));
     }
-    for (int k = 0; k + 2 < kmax; k++) {
+    // blocks of group g still on the chain at column k: a prefix of the group
+    auto live = [&](int g, int k) {
+      int nl = 0;
+      while (nl < (int)grp[g].size() && khand[grp[g][nl]] > k) nl++;
+      return nl;
+    };
+    for (int k = 0; k < kmax; k++) {
       for (int g = 0; g < ngrp; g++) {
-        if (k + 2 >= nmax_g[g]) continue;
+        if (k >= kend_g[g]) continue;
         const hipStream_t sg = gs[g];
-        const int nl = (int)grp[g].size();
+        const int nl = live(g, k);
         const int* bl = grp[g].data();
         const int* bn = gn[g].data();
         if (k % 8 == 0) {
           // every 8th column is timed (the events cost more than a short launch):
           // algorithmic bytes = the lower triangle (diagonal included) of the
-          // symmetric trailing matrix of every active large block of the group
+          // symmetric trailing matrix of every block of the group on the chain
           double by = 0.0;
-          for (int b : grp[g])
-            if (k + 2 < blk[b].n) {
-              const double mm = (double)(blk[b].n - k - 1);
-              by += 8.0 * mm * (mm + 1.0) * 0.5;
-            }
+          for (int q = 0; q < nl; q++) {
+            const double mm = (double)(blk[bl[q]].n - k - 1);
+            by += 8.0 * mm * (mm + 1.0) * 0.5;
+          }
           PendingTimer tm;
           if ((rc = c->timer_begin(tm, NRG_K_SYMV, 0.0, by, sg))) return rc;
           HIPCHK(e2_launch_tdb_symv(d_blk, bl, bn, nl, k, dA, WS, sg));
@@ -430,13 +447,16 @@ static int e2_phase_a(nrg_ctx* c, const std::vector<E2Block>& blk, double* dA, d
           e++;
         }
         if ((k + 1) % kPanel == 0) {
-          HIPCHK(e2_launch_tdb_refresh(d_blk, bl, bn, nl, k + 1, dA, sg));
-          HIPCHK(e2_launch_tdb_norm(d_blk, d_large + goff[g], nl, nmax_g[g], k + 1, dA, WS, sg));
+          const int nl2 = live(g, k + 1);   // blocks that go on with the next panel
+          if (nl2 > 0) {
+            HIPCHK(e2_launch_tdb_refresh(d_blk, bl, bn, nl2, k + 1, dA, sg));
+            HIPCHK(e2_launch_tdb_norm(d_blk, d_large + goff[g], nl2, nmax_g[g], k + 1, dA, WS, sg));
+          }
         }
       }
     }
     for (int g = 0; g < ngrp; g++)
-      HIPCHK(e2_launch_tridiag_tail(d_blk, d_large + goff[g], (int)grp[g].size(), dA, WS, gs[g]));
+      HIPCHK(e2_launch_tridiag_hand(d_blk, d_hand + goff[g], (int)grp[g].size(), dA, WS, gs[g]));
   }
   // s continues after every group's chain and the small blocks
   if (nfork > 0 && (rc = c->join(nfork))) return rc;
