@@ -111,6 +111,26 @@ def test_nrg_like_graded(ctx):
     _check(a, W[0], Z[0], tol=1e-12)
 
 
+@pytest.mark.parametrize("kind", ["rand", "degen", "tri"])
+def test_resident_hand_off_boundaries(ctx, kind):
+    # blocks above 256 rows leave their column chain at the panel boundary
+    # 32 ceil((n - 208) / 32) and the register-resident kernel reduces the
+    # trailing matrix from its lower triangle: n = 257 hands over 193 rows (the
+    # 16 x 16 corner variant), 272 exactly 208, 273 and 305 177 (12 row slots),
+    # 288 192, 465 177 after 9 panels; 208 is resident from the start, 209 and
+    # 241 take the streamed small-block path
+    if ctx.impl != 2:
+        pytest.skip("the hand-off belongs to the batched solver")
+    rng = np.random.default_rng(77)
+    sizes = [257, 272, 273, 288, 305, 465, 241, 208, 209]
+    blocks = [_sym(rng, n, kind) for n in sizes]
+    print("eig hand-off", kind, sizes, flush=True)
+    W, Z, info = ctx.diag_blocks(blocks)
+    assert not info.any()
+    for a, w, z in zip(blocks, W, Z):
+        _check(a, w, z)
+
+
 def test_reduction_streams_give_identical_results(ctx):
     # the large blocks split over 1..4 reduction streams: every block runs the
     # same kernels on the same data, so the results are bit-identical
