@@ -68,8 +68,8 @@ struct alignas(16) R2Lds {
   double vn[256];                       // v by column
   double vw[256][2];                    // (v, w) by column
   double pw[R2_W];                      // per-wave partials of v . (A v)
-  double ycol[R2_W];                    // (A v) at the corner columns, one per corner row
-  double fold[R2_W];                    // corner rows' elements at the next reflector's column
+  double ycol[R2_W][4];                 // (A v) at the corner columns: four 16-lane partials per corner row
+  double fold[R2_W][WAVE];              // corner rows' elements at the next reflector's column (its lane)
   double ts[2];                         // t of the current reflector
   double qp[R2_W / 2][256][2];          // per-wave partial sums of (A v)_c, waves 2h and 2h+1 side by side
 };
@@ -219,7 +219,7 @@ struct R2 {
         cr = __fma_rn(-wr, vc[NJ], cr);
         x[NJ] = cr;
       } else {   // elements (CF + m, kk) of the corner rows, updated as their holders will
-        double y = l < 16 ? S.fold[l] : 0.0;
+        double y = l < 16 ? S.fold[l][kk & (WAVE - 1)] : 0.0;
         y = __fma_rn(-vc[NJ], wr, y);
         y = __fma_rn(-wc[NJ], vr, y);
         x[NJ] = y;
@@ -265,17 +265,17 @@ struct R2 {
           S.qp[w >> 1][c][w & 1] = q[j];
           ss = __fma_rn(S.vn[c], q[j], ss);   // v_c = 0 for c <= k and c >= n
         }
-        double pv;
         if constexpr (CORNER) {
-          // (A v) at column CF + w: row CF + w (this wave's corner row) times v
+          // (A v) at column CF + w is row CF + w (this wave's corner row) times
+          // v: four 16-lane partials for the wave forming w, and its term
+          // v_(CF+w) (A v)_(CF+w) of v.(A v) lane by lane
           double yr = 0.0;
 #pragma unroll
           for (int j = J0; j < NJ; j++) yr = __fma_rn(a[IC][j], S.vn[l + WAVE * j], yr);
           yr = __fma_rn(cr, S.vn[CF + l], yr);   // lanes 16 .. 63: cr = 0
-          pv = ss;
-          nrgw::wave_sum_bc2(pv, yr);
-          pv = __fma_rn(S.vn[CF + w], yr, pv);
-          if (l == 0) S.ycol[w] = yr;
+          ss = __fma_rn(S.vn[CF + w], yr, ss);
+          yr = r2_sum16(yr);
+          if ((l & 15) == 0) S.ycol[w][l >> 4] = yr;
           // element (CF + w, k + 1) for the wave forming reflector k + 1
           const int kk = k + 1;
           if (kk < CF && l == (kk & (WAVE - 1))) {
@@ -283,11 +283,10 @@ struct R2 {
 #pragma unroll
             for (int j = 0; j < NJ; j++)
               if (j == kk / WAVE) x = a[IC][j];
-            S.fold[w] = x;
+            S.fold[w][l] = x;
           }
-        } else {
-          pv = nrgw::wave_sum_bc(ss);
         }
+        const double pv = nrgw::wave_sum_bc(ss);
         if (l == 0) S.pw[w] = pv;
       }
       __syncthreads();
@@ -322,7 +321,8 @@ struct R2 {
               for (int h = 0; h < R2_W / 2; h += 2 * m) p[h] += p[h + m];
             y = p[0];
           } else {
-            y = S.ycol[l & (R2_W - 1)];   // lanes 16 .. 63 are past n
+            const double* yc = S.ycol[l & (R2_W - 1)];   // lanes 16 .. 63 are past n
+            y = (yc[0] + yc[1]) + (yc[2] + yc[3]);
           }
           vc[j] = S.vn[c];
           wc[j] = (c > k && c < n) ? t * y - K * vc[j] : 0.0;
@@ -435,13 +435,13 @@ struct R2 {
       (&S.vt[0][0])[q] = 0.0;
       st2(&S.vwt[0][0][0] + 2 * q, 0.0, 0.0);
     }
-    if (CORNER && l == 0) S.fold[w] = a[IC][0];   // element (CF + w, 0) for reflector 0
+    if (CORNER && l == 0) S.fold[w][0] = a[IC][0];   // element (CF + w, 0) for reflector 0
     __syncthreads();
     if (w == 0) {   // reflector 0 from row 0 (slot 0 of wave 0)
       double x[NJL];
 #pragma unroll
       for (int j = 0; j < NJ; j++) x[j] = a[0][j];
-      if constexpr (CORNER) x[NJ] = l < 16 ? S.fold[l] : 0.0;
+      if constexpr (CORNER) x[NJ] = l < 16 ? S.fold[l][0] : 0.0;
       reflector(n, ld, 0, x, S, A, d, e, tau, l);
     }
     __syncthreads();

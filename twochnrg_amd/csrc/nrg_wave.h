@@ -53,23 +53,4 @@ __device__ __forceinline__ double wave_sum_bc(double v) {
   return readlane_d(v, 63);
 }
 
-// two independent sums, their DPP steps interleaved so each hides the
-// other's latency
-__device__ __forceinline__ void wave_sum_bc2(double& a, double& b) {
-  a += dpp_d<0xB1>(a);
-  b += dpp_d<0xB1>(b);
-  a += dpp_d<0x4E>(a);
-  b += dpp_d<0x4E>(b);
-  a += dpp_d<0x141>(a);
-  b += dpp_d<0x141>(b);
-  a += dpp_d<0x140>(a);
-  b += dpp_d<0x140>(b);
-  a += dpp_rows_d<0x142, 0xA>(a);
-  b += dpp_rows_d<0x142, 0xA>(b);
-  a += dpp_rows_d<0x143, 0xC>(a);
-  b += dpp_rows_d<0x143, 0xC>(b);
-  a = readlane_d(a, 63);
-  b = readlane_d(b, 63);
-}
-
 }  // namespace nrgw
