@@ -537,23 +537,26 @@ __device__ __forceinline__ void tick_cols(int s0, int n, int tk, int& lo, int& h
 __global__ __launch_bounds__(SB_T) void chase_kernel(const E2Block* __restrict__ blks, const E2SbGrid G,
                                                      double* __restrict__ WS, double* __restrict__ V2,
                                                      int* __restrict__ FL) {
-  // the ring: band column c at slot c mod SBW, LDB+1 doubles per slot; after
-  // it one dummy double per lane, the target of masked stores (so that the
-  // task code has no per-element branches)
-  __shared__ double Wf[SBW * (LDB + 1) + 64];
+  // the ring: band column c at slot c mod SBW, RS = LDB + 2 doubles per slot
+  // (a task's lanes walk a column, stride 1, or a diagonal (c + t, d - t),
+  // stride RS - 1: both odd, so 16 lanes hit 16 different banks); after it
+  // one dummy double per lane, the target of masked stores (so that the task
+  // code has no per-element branches)
+  constexpr int RS = LDB + 2;
+  __shared__ double Wf[SBW * RS + 64];
   __shared__ int sh_abort;
   const int bi = blockIdx.x / G.G, gq = blockIdx.x % G.G;
   if (bi >= G.nb) return;
   const E2Block B = blks[G.blk[bi]];
   const int n = B.n, tid = threadIdx.x, grp = tid >> 4, t = tid & 15;
-  const int DUM = SBW * (LDB + 1) + (tid & 63);
+  const int DUM = SBW * RS + (tid & 63);
   double* AB = WS + B.q_off;
   double* VB = V2 + B.v2_off;
   int* abortf = FL + B.f_off;
   int* ret = abortf + 1;
   const int nbatch = e2_sb_nbatch(n);
   constexpr int PF = (TB * LDB) / SB_T;   // prefetched doubles per thread per tick
-#define WA(c, d) ((((c) & (SBW - 1)) * (LDB + 1)) + (d))
+#define WA(c, d) ((((c) & (SBW - 1)) * RS) + (d))
   for (int q = gq; q < nbatch; q += G.G) {
     const int s0 = q * SBP, s1 = min(s0 + SBP, n - 2);
     const int s = s0 + grp;   // this group's sweep
