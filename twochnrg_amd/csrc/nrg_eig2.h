@@ -15,7 +15,7 @@ struct E2Block {
   int64_t u_off;   // n*n  rank-one eigenvectors
   int64_t dl_off;  // n*n  d_l - lambda_j
   int64_t t_off;   // n*n  merge output / reorder scratch
-  int64_t p_off;   // panel scratch of the reduction: 3 x 32 rows of n
+  int64_t p_off;   // panel scratch of the reduction: 3 x E2_PANEL rows of n
   int64_t v_off;   // 20*n doubles of vectors
   int64_t i_off;   // 8*n ints
   int ts;          // 1: two-stage reduction (dense -> band of width E2_TSB -> tridiagonal)
@@ -140,6 +140,9 @@ struct E2BtRows {
 // a block that is smaller than the largest or already reduced.
 #define E2_TDB_MAXB 32
 #define E2_RS_MAXN 208     // largest matrix the register-resident reduction holds
+#ifndef E2_PANEL
+#define E2_PANEL 16        // columns per panel of the large blocks' reduction
+#endif
 // a large block's column chain stops at panel boundary k0 (n - k0 <= E2_RS_MAXN)
 // and the resident reduction finishes its trailing matrix
 struct E2Hand {

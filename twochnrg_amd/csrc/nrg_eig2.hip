@@ -29,7 +29,7 @@
 #define E2_T 256
 #define WAVE 64
 #define LEAF 16
-#define PANEL 32   // reduction panel
+#define PANEL E2_PANEL   // reduction panel
 #define BTP 64     // back-transformation panel
 
 namespace e2 {
@@ -654,6 +654,11 @@ __global__ __launch_bounds__(SM_T) void tridiag_hand_kernel(const E2Block* __res
 // ---------------------------------------------------------------------------
 #define SYMV_R 128   // rows and columns of a symv tile
 #define CHUNK 256    // threads of a symv tile / elements per chunk
+#ifndef E2_FIN_E
+#define E2_FIN_E 128
+#endif
+#define FIN_E E2_FIN_E   // elements per finish workgroup, and per partial norm of a column
+static_assert(CHUNK % FIN_E == 0 && FIN_E % WAVE == 0, "partial-norm chunks");
 
 struct TdbPtr {
   double *A, *X, *d, *e, *tau, *refl, *npart, *part, *tpart, *cpart;
@@ -674,10 +679,14 @@ __device__ __forceinline__ TdbPtr tdb_ptrs(const E2Block& B, double* A0, double*
   return p;
 }
 
+// the reflector of column k from its partial norms; called by whole waves
+// (every lane gets the result): lane l adds partials l, l + 64, ... and the
+// wave sums the lanes, so the loads are issued together
 __device__ __forceinline__ void tdb_reflector(const TdbPtr& P, int k, int n, double& t, double& v0, double& alpha) {
-  const int nch = (n - k + CHUNK - 1) / CHUNK;
-  double nrm2 = 0.0;
-  for (int c = 0; c < nch; c++) nrm2 += P.npart[c];
+  const int nch = (n - k + FIN_E - 1) / FIN_E;
+  double part = 0.0;
+  for (int c = (int)(threadIdx.x & (WAVE - 1)); c < nch; c += WAVE) part += P.npart[c];
+  const double nrm2 = wave_sum(part);
   const double x0 = P.A[(size_t)k * n + k + 1];
   if (nrm2 == 0.0) { t = 0.0; v0 = 0.0; alpha = 0.0; return; }
   const double nrm = sqrt(nrm2);
@@ -686,9 +695,9 @@ __device__ __forceinline__ void tdb_reflector(const TdbPtr& P, int k, int n, dou
   t = 2.0 / (nrm2 - x0 * x0 + v0 * v0);
 }
 
-// partial norms of column k (entries i > k) at a panel start; for the last,
-// partial panel also clear the unused panel rows in the two columns the
-// final GEMM reads
+// partial norms of column k (entries i > k) at a panel start, one per FIN_E
+// entries (the finish kernel's granularity); for the last, partial panel
+// also clear the unused panel rows in the two columns the final GEMM reads
 __global__ __launch_bounds__(CHUNK) void tdb_norm_kernel(const E2Block* __restrict__ blks,
                                                          const int* __restrict__ list, int k, double* __restrict__ A0,
                                                          double* __restrict__ WS) {
@@ -701,8 +710,15 @@ __global__ __launch_bounds__(CHUNK) void tdb_norm_kernel(const E2Block* __restri
   const TdbPtr P = tdb_ptrs(B, A0, WS);
   const int i = k + c * CHUNK + threadIdx.x;
   const double a = (i > k && i < n) ? P.A[(size_t)k * n + i] : 0.0;
-  const double s = block_sum(a * a, red);
-  if (threadIdx.x == 0) P.npart[c] = s;
+  const double ws = wave_sum(a * a);
+  if ((threadIdx.x & (WAVE - 1)) == 0) red[threadIdx.x / WAVE] = ws;
+  __syncthreads();
+  constexpr int SUB = CHUNK / FIN_E, WPS = FIN_E / WAVE;
+  if (threadIdx.x < SUB && k + c * CHUNK + (int)threadIdx.x * FIN_E < n) {
+    double s = 0.0;
+    for (int w = 0; w < WPS; w++) s += red[threadIdx.x * WPS + w];
+    P.npart[c * SUB + threadIdx.x] = s;
+  }
   const int nbp = min(PANEL, n - 2 - k);
   if (c == 0 && nbp < PANEL && threadIdx.x < 3 * PANEL) {
     const int p = threadIdx.x % PANEL, sect = threadIdx.x / PANEL;
@@ -884,6 +900,10 @@ __global__ __launch_bounds__(256) void tdb_refresh_kernel(const E2Block* __restr
   }
 }
 
+// One workgroup finishes FIN_E consecutive elements (128 by default): the
+// kernel reads about RT + 2j doubles per element, and at large m a
+// 256-element chunk is bound by the bandwidth of the one CU that runs it, so
+// short chunks spread a column's reads over more CUs.
 // FG thread groups share a chunk's elements: group g sums the row-tile
 // partials rt = g (mod FG) and the panel terms p = g (mod FG) of each element;
 // group 0 adds the four sums in a fixed order and finishes the element.  Every
@@ -893,26 +913,30 @@ __global__ __launch_bounds__(256) void tdb_refresh_kernel(const E2Block* __restr
 // about once.
 #define FG 4
 #define FIN_PR 12   // row-tile partials per thread loaded at once (all of them while RT <= FG * FIN_PR)
-__global__ __launch_bounds__(CHUNK * FG) void tdb_fin_kernel(const E2Block* __restrict__ blks, const E2TdbGrid G,
+__global__ __launch_bounds__(FIN_E * FG) void tdb_fin_kernel(const E2Block* __restrict__ blks, const E2TdbGrid G,
                                                              int k, double* __restrict__ A0, double* __restrict__ WS) {
   constexpr int NQ = PANEL / FG;   // panel terms per thread
-  __shared__ double red[CHUNK * FG / WAVE];
+  constexpr int NTH = FIN_E * FG;
+  static_assert(NTH >= CHUNK, "the panel dot products are summed by CHUNK threads");
+  __shared__ double red[NTH / WAVE];
   __shared__ double c1s[PANEL], c2s[PANEL], vk1[PANEL], wk1[PANEL];
-  __shared__ double ys[FG][CHUNK], as[FG][CHUNK];
+  __shared__ double ys[FG][FIN_E], as[FG][FIN_E];
   __shared__ double cred[2][CHUNK / PANEL][PANEL];
   const int gi = tdb_find(G, blockIdx.x);
   const E2Block B = blks[G.blk[gi]];
   const int n = B.n;
   if (k + 2 >= n) return;
   const int m = n - k - 1;
+  // CT: the symv's column chunks (of its panel dot products); FT: this kernel's chunks
   const int CT = (m + CHUNK - 1) / CHUNK, RT = (m + SYMV_R - 1) / SYMV_R, NT = RT * (RT + 1) / 2;
+  const int FT = (m + FIN_E - 1) / FIN_E;
   const int c = blockIdx.x - G.cum[gi];
-  if (c >= CT) return;
+  if (c >= FT) return;
   const TdbPtr P = tdb_ptrs(B, A0, WS);
   const int tid = threadIdx.x, j = k % PANEL;
-  const int e = tid % CHUNK, g = tid / CHUNK;
+  const int e = tid % FIN_E, g = tid / FIN_E;
   const double* rowk = P.A + (size_t)k * n;
-  const int i = k + 1 + c * CHUNK + e;
+  const int i = k + 1 + c * FIN_E + e;
   const bool panel_end = (j == PANEL - 1) || (k + 3 >= n);
   const bool in = i < n;
   // ---- loads
@@ -953,7 +977,7 @@ __global__ __launch_bounds__(CHUNK * FG) void tdb_fin_kernel(const E2Block* __re
     cred[1][gg][pp] = a2;
   }
   double tv = 0.0;
-  for (int q = tid; q < NT; q += CHUNK * FG) tv += P.tpart[q];   // one per lower-triangle tile
+  for (int q = tid; q < NT; q += NTH) tv += P.tpart[q];   // one per lower-triangle tile
   const double t = P.refl[0], v0 = P.refl[1];
   // this group's share of y_i = (A v)_i
   double yp = 0.0;
@@ -1902,9 +1926,9 @@ hipError_t e2_launch_tdb_refresh(const E2Block* b, const int* blk_idx, const int
 
 hipError_t e2_launch_tdb_fin(const E2Block* b, const int* blk_idx, const int* blk_n, int nlist, int k, double* A,
                              double* WS, hipStream_t s) {
-  auto per = [](int m) { return (m + CHUNK - 1) / CHUNK; };
+  auto per = [](int m) { return (m + FIN_E - 1) / FIN_E; };
   return tdb_launch_all(blk_idx, blk_n, nlist, k, per, [&](const E2TdbGrid& G) {
-    hipLaunchKernelGGL(e2::tdb_fin_kernel, dim3(G.cum[G.nb]), dim3(CHUNK * FG), 0, s, b, G, k, A, WS);
+    hipLaunchKernelGGL(e2::tdb_fin_kernel, dim3(G.cum[G.nb]), dim3(FIN_E * FG), 0, s, b, G, k, A, WS);
   });
 }
 

@@ -66,7 +66,7 @@ struct GraphScope {
 };
 
 static const int kLeaf = 16;   // = LEAF in nrg_eig2.hip
-static const int kPanel = 32;  // = PANEL in nrg_eig2.hip
+static const int kPanel = E2_PANEL;
 static const int kBtp = 64;    // = BTP in nrg_eig2.hip
 static const int kSmall = 256; // single-workgroup reduction up to this size
 static const int kBtKs = 256;  // K chunk of the back-transformation products Z V^T
