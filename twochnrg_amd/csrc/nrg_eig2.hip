@@ -777,9 +777,12 @@ __global__ __launch_bounds__(CHUNK) void tdb_symv_kernel(const E2Block* __restri
   const TdbPtr P = tdb_ptrs(B, A0, WS);
   const double* rowk = P.A + (size_t)k * n;
   const int tid = threadIdx.x;
-  if (x < NT) {
+  // workgroups [0, CT): the panel dot products, one column chunk each (they
+  // are dispatched first); [CT, CT + NT): the lower-triangle tiles
+  if (x >= CT) {
+    const int xt = x - CT;
     int I, J;
-    tri_tile(x, I, J);
+    tri_tile(xt, I, J);
     const int r0 = k + 1 + I * T, c0 = k + 1 + J * T;
     const int lane = tid & (WAVE - 1), w = tid / WAVE;
     const int cl = lane & 15, rl = lane >> 4;
@@ -801,7 +804,7 @@ __global__ __launch_bounds__(CHUNK) void tdb_symv_kernel(const E2Block* __restri
       double t, v0, alpha;
       tdb_reflector(P, k, n, t, v0, alpha);
       if (q == k + 1) vq = v0;
-      if (x == 0 && tid == 0) {
+      if (xt == 0 && tid == 0) {
         P.d[k] = rowk[k];
         P.e[k] = alpha;
         P.tau[k] = t;
@@ -845,25 +848,43 @@ __global__ __launch_bounds__(CHUNK) void tdb_symv_kernel(const E2Block* __restri
       tv = y * vs[e];
     }
     tv = block_sum(tv, red);
-    if (tid == 0) P.tpart[x] = tv;
+    if (tid == 0) P.tpart[xt] = tv;
   } else {
-    // panel dot products over one column chunk: cpart[c][p] = W_p . v, cpart[c][P+p] = V_p . v
-    double t, v0, alpha;
-    tdb_reflector(P, k, n, t, v0, alpha);
-    const int c = x - NT;
+    // panel dot products over one column chunk: cpart[c][p] = W_p . v,
+    // cpart[c][P+p] = V_p . v; wave w takes rows p = w, w + 4, ... and lane l
+    // elements l, l + 64, ... of the chunk.  The panel rows are loaded before
+    // the reflector and v are known, all in one round trip.
+    constexpr int NW = CHUNK / WAVE, PW = (PANEL + NW - 1) / NW, EW = CHUNK / WAVE;
+    const int c = x;
     const int j = k % PANEL;
     const int c0 = k + 1 + c * CHUNK;
+    const int lane = tid & (WAVE - 1), wv = tid / WAVE;
+    double xw[PW][EW], xv[PW][EW];
+#pragma unroll
+    for (int r = 0; r < PW; r++) {
+      const int p = wv + r * NW;
+#pragma unroll
+      for (int u = 0; u < EW; u++) {
+        const int ii = c0 + lane + u * WAVE;
+        const bool ok = p < j && ii < n;
+        xw[r][u] = ok ? P.X[(size_t)(PANEL + p) * n + ii] : 0.0;
+        xv[r][u] = ok ? P.X[(size_t)p * n + ii] : 0.0;
+      }
+    }
+    double t, v0, alpha;
+    tdb_reflector(P, k, n, t, v0, alpha);
     const int i = c0 + tid;
     vs[tid] = (i < n) ? ((i == k + 1) ? v0 : rowk[i]) : 0.0;
     __syncthreads();
-    const int lane = tid & (WAVE - 1), wv = tid / WAVE;
-    for (int p = wv; p < j; p += CHUNK / WAVE) {
-      const double* Vp = P.X + (size_t)p * n;
-      const double* Wp = P.X + (size_t)(PANEL + p) * n;
+#pragma unroll
+    for (int r = 0; r < PW; r++) {
+      const int p = wv + r * NW;
+      if (p >= j) break;
       double a1 = 0.0, a2 = 0.0;
-      for (int q = lane; q < CHUNK; q += WAVE) {
-        const int ii = c0 + q;
-        if (ii < n) { a1 += Wp[ii] * vs[q]; a2 += Vp[ii] * vs[q]; }
+#pragma unroll
+      for (int u = 0; u < EW; u++) {
+        a1 += xw[r][u] * vs[lane + u * WAVE];
+        a2 += xv[r][u] * vs[lane + u * WAVE];
       }
       a1 = wave_sum(a1);
       a2 = wave_sum(a2);
