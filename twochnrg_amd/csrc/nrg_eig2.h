@@ -154,6 +154,11 @@ struct E2TdbGrid {
   int cum[E2_TDB_MAXB + 1];
   int blk[E2_TDB_MAXB];
 };
+// the panel updates of one launch: problem p owns workgroups cum[p] .. cum[p+1]-1
+struct E2PanelGrid {
+  int np;
+  int cum[E2_TDB_MAXB + 1];
+};
 
 extern "C" {
 hipError_t e2_launch_tridiag_small(const E2Block* b, const int* list, int nlist, double* A, double* WS, hipStream_t s);
@@ -170,6 +175,11 @@ hipError_t e2_launch_tdb_fin(const E2Block* b, const int* blk_idx, const int* bl
                              double* WS, hipStream_t s);
 hipError_t e2_launch_gemm3(const E2Gemm* probs, const E2Tile* tiles, int ntiles, const double* A, const double* B,
                            double* C, hipStream_t s);
+// the products of the reduction's panel updates: each problem's m x m result
+// over its lower triangle and a band of two 64 x 64 tiles above it; dprobs:
+// the np problems on the device, hprobs their host copy (for the launch sizes)
+hipError_t e2_launch_panel_gemm(const E2Gemm* dprobs, const E2Gemm* hprobs, int np, const double* A, const double* B,
+                                double* C, hipStream_t s);
 hipError_t e2_launch_tridiag_hand(const E2Block* b, const E2Hand* hands, int nh, double* A, double* WS, hipStream_t s);
 hipError_t e2_launch_dc_init(const E2Block* b, int nb, int nmax, const E2Merge* merges, int nmerge, const int* mbeg,
                              double* WS, hipStream_t s);

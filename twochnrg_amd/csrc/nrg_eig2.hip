@@ -1778,16 +1778,10 @@ typedef double v4d __attribute__((ext_vector_type(4)));
 #define GT 64
 #define GK 16
 #define GP 4   // LDS row padding (doubles)
-__global__ __launch_bounds__(256) void gemm_kernel(const E2Gemm* __restrict__ probs, const E2Tile* __restrict__ tiles,
-                                                   const double* __restrict__ Abase, const double* __restrict__ Bbase,
-                                                   double* __restrict__ Cbase, const E2MState* __restrict__ dyns) {
-  __shared__ double As[GK][GT + GP];
-  __shared__ double Bs[GK][GT + GP];
-  const E2Tile Tl = tiles[blockIdx.x];
-  const E2Gemm P = probs[Tl.prob];
-  int M = P.m, K = P.k;
-  if (P.dyn >= 0) { M = dyns[P.dyn].k; K = M; }
-  const int m0 = Tl.tm * GT, n0 = Tl.tn * GT;
+__device__ __forceinline__ void gemm_tile(const E2Gemm& P, int M, int K, int m0, int n0,
+                                          const double* __restrict__ Abase, const double* __restrict__ Bbase,
+                                          double* __restrict__ Cbase, double (&As)[GK][GT + GP],
+                                          double (&Bs)[GK][GT + GP]) {
   if (m0 >= M || n0 >= P.n) return;
   const double* A = Abase + P.a_off;
   const double* Bm = Bbase + P.b_off;
@@ -1856,6 +1850,54 @@ __global__ __launch_bounds__(256) void gemm_kernel(const E2Gemm* __restrict__ pr
           *cp = v;
         }
       }
+}
+
+__global__ __launch_bounds__(256) void gemm_kernel(const E2Gemm* __restrict__ probs, const E2Tile* __restrict__ tiles,
+                                                   const double* __restrict__ Abase, const double* __restrict__ Bbase,
+                                                   double* __restrict__ Cbase, const E2MState* __restrict__ dyns) {
+  __shared__ double As[GK][GT + GP];
+  __shared__ double Bs[GK][GT + GP];
+  const E2Tile Tl = tiles[blockIdx.x];
+  const E2Gemm P = probs[Tl.prob];
+  int M = P.m, K = P.k;
+  if (P.dyn >= 0) { M = dyns[P.dyn].k; K = M; }
+  gemm_tile(P, M, K, Tl.tm * GT, Tl.tn * GT, Abase, Bbase, Cbase, As, Bs);
+}
+
+// The panel updates of the large blocks' reduction: problem p of the launch
+// (an m x m result, m = probs[p].m) covers tile rows a < tt = ceil(m / 64) and,
+// in row a, tile columns 0 .. min(a + 2, tt - 1): the lower triangle and the
+// band of two tiles above it.  Row a then has min(a + 3, tt) tiles, rows
+// a < tt - 2 have a + 3 and start at tile a (a + 5) / 2, the last two rows
+// (or all rows when tt <= 2) have tt.  Workgroup x of problem p (cum[p] <= x <
+// cum[p + 1]) takes tile x - cum[p] in row order, so no tile list is built.
+__device__ __forceinline__ void band_tile(int tt, int t, int& a, int& b) {
+  const int a0 = tt > 2 ? tt - 2 : 0;      // first full row
+  const int s0 = a0 * (a0 + 5) / 2;        // tiles before it
+  if (t >= s0) {
+    a = a0 + (t - s0) / tt;
+    b = (t - s0) % tt;
+    return;
+  }
+  int r = (int)((sqrt(25.0 + 8.0 * t) - 5.0) * 0.5);   // largest r with r (r + 5) / 2 <= t
+  while ((r + 1) * (r + 6) / 2 <= t) r++;
+  while (r * (r + 5) / 2 > t) r--;
+  a = r;
+  b = t - r * (r + 5) / 2;
+}
+__global__ __launch_bounds__(256) void panel_gemm_kernel(const E2Gemm* __restrict__ probs, const E2PanelGrid G,
+                                                         const double* __restrict__ Abase,
+                                                         const double* __restrict__ Bbase, double* __restrict__ Cbase) {
+  __shared__ double As[GK][GT + GP];
+  __shared__ double Bs[GK][GT + GP];
+  const int x = blockIdx.x;
+  int p = 0;
+#pragma unroll
+  for (int i = 1; i < E2_TDB_MAXB; i++) p += (i < G.np && G.cum[i] <= x) ? 1 : 0;
+  const E2Gemm P = probs[p];
+  int a, b;
+  band_tile((P.m + GT - 1) / GT, x - G.cum[p], a, b);
+  gemm_tile(P, P.m, P.k, a * GT, b * GT, Abase, Bbase, Cbase, As, Bs);
 }
 
 // copy eigenvalues (what & 1) and the first krow eigenvector rows (what & 2)
@@ -1951,6 +1993,27 @@ hipError_t e2_launch_tdb_fin(const E2Block* b, const int* blk_idx, const int* bl
   return tdb_launch_all(blk_idx, blk_n, nlist, k, per, [&](const E2TdbGrid& G) {
     hipLaunchKernelGGL(e2::tdb_fin_kernel, dim3(G.cum[G.nb]), dim3(FIN_E * FG), 0, s, b, G, k, A, WS);
   });
+}
+
+// tiles of the band product of an m x m result (see band_tile)
+static int band_tiles(int m) {
+  const int tt = (m + GT - 1) / GT, a0 = tt > 2 ? tt - 2 : 0;
+  return a0 * (a0 + 5) / 2 + (tt - a0) * tt;
+}
+
+hipError_t e2_launch_panel_gemm(const E2Gemm* dprobs, const E2Gemm* hprobs, int np, const double* A, const double* B,
+                                double* C, hipStream_t s) {
+  for (int p0 = 0; p0 < np; p0 += E2_TDB_MAXB) {
+    E2PanelGrid G = {};
+    G.np = np - p0 < E2_TDB_MAXB ? np - p0 : E2_TDB_MAXB;
+    G.cum[0] = 0;
+    for (int i = 0; i < G.np; i++) G.cum[i + 1] = G.cum[i] + band_tiles(hprobs[p0 + i].m);
+    if (G.cum[G.np] <= 0) continue;
+    hipLaunchKernelGGL(e2::panel_gemm_kernel, dim3(G.cum[G.np]), dim3(256), 0, s, dprobs + p0, G, A, B, C);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 hipError_t e2_launch_gemm3(const E2Gemm* probs, const E2Tile* tiles, int ntiles, const double* A, const double* B,

@@ -267,13 +267,12 @@ static int e2_phase_a(nrg_ctx* c, const std::vector<E2Block>& blk, double* dA, d
   // panel are rebuilt from their columns before it starts.  Per group: the
   // launch at column k covers the group's blocks whose panel ends there.
   std::vector<E2Gemm> tprobs;
-  std::vector<E2Tile> ttiles;
-  std::vector<std::vector<int>> tev_k(ngrp), tev_pbeg(ngrp), tev_tbeg(ngrp), tev_tcnt(ngrp);
+  std::vector<std::vector<int>> tev_k(ngrp), tev_pbeg(ngrp), tev_np(ngrp);
   for (int g = 0; g < ngrp; g++) {
     const int kend_g = grp[g].empty() ? 0 : khand[grp[g][0]];
     for (int k = 0; k < kend_g; k++) {
       const int j = k % kPanel, k0 = k - j;
-      const int pbase = (int)tprobs.size(), tbase = (int)ttiles.size();
+      const int pbase = (int)tprobs.size();
       for (int b : grp[g]) {
         const int nb_ = blk[b].n;
         if (k >= khand[b] || j != kPanel - 1) continue;   // the panel ending at khand - 1 is the last
@@ -285,15 +284,10 @@ static int e2_phase_a(nrg_ctx* c, const std::vector<E2Block>& blk, double* dA, d
         gm.b_off = blk[b].p_off + (int64_t)kPanel * nn + s0; gm.bs0 = nn; gm.bs1 = 1;
         gm.c_off = blk[b].a_off + (int64_t)s0 * nn + s0; gm.cs0 = nn;
         gm.alpha = -1.0; gm.accumulate = 1;
-        const int pi = (int)tprobs.size() - pbase;
         tprobs.push_back(gm);
-        const int tt = (gm.m + 63) / 64;
-        for (int a = 0; a < tt; a++)
-          for (int bb = 0; bb < tt && bb <= a + 2; bb++) ttiles.push_back(E2Tile{pi, a, bb, 0});
       }
       if ((int)tprobs.size() > pbase) {
-        tev_k[g].push_back(k); tev_pbeg[g].push_back(pbase); tev_tbeg[g].push_back(tbase);
-        tev_tcnt[g].push_back((int)ttiles.size() - tbase);
+        tev_k[g].push_back(k); tev_pbeg[g].push_back(pbase); tev_np[g].push_back((int)tprobs.size() - pbase);
       }
     }
   }
@@ -379,9 +373,7 @@ static int e2_phase_a(nrg_ctx* c, const std::vector<E2Block>& blk, double* dA, d
   if ((rc = upload(c, "e2_gp", gprobs, &d_gp))) return rc;
   if ((rc = upload(c, "e2_gt", gtiles, &d_gt))) return rc;
   E2Gemm* d_tp;
-  E2Tile* d_tt;
   if ((rc = upload(c, "e2_tp", tprobs, &d_tp))) return rc;
-  if ((rc = upload(c, "e2_tt", ttiles, &d_tt))) return rc;
   // ---- launches
   hipStream_t s = c->stream;
   GraphScope gs_(c, c->graph && tsl.empty());
@@ -443,7 +435,8 @@ static int e2_phase_a(nrg_ctx* c, const std::vector<E2Block>& blk, double* dA, d
         HIPCHK(e2_launch_tdb_fin(d_blk, bl, bn, nl, k, dA, WS, sg));
         size_t& e = ev[g];
         if (e < tev_k[g].size() && tev_k[g][e] == k) {
-          HIPCHK(e2_launch_gemm3(d_tp + tev_pbeg[g][e], d_tt + tev_tbeg[g][e], tev_tcnt[g][e], WS, WS, dA, sg));
+          HIPCHK(e2_launch_panel_gemm(d_tp + tev_pbeg[g][e], tprobs.data() + tev_pbeg[g][e], tev_np[g][e], WS, WS, dA,
+                                      sg));
           e++;
         }
         if ((k + 1) % kPanel == 0) {
