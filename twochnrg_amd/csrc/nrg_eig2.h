@@ -138,7 +138,9 @@ struct E2BtRows {
 // The live blocks of one reduction launch, passed by value: workgroup x works
 // on block blk[i] for cum[i] <= x < cum[i+1], so no workgroup is launched for
 // a block that is smaller than the largest or already reduced.
-#define E2_TDB_MAXB 32
+#ifndef E2_TDB_MAXB
+#define E2_TDB_MAXB 128
+#endif
 #define E2_RS_MAXN 208     // largest matrix the register-resident reduction holds
 #ifndef E2_PANEL
 #define E2_PANEL 16        // columns per panel of the large blocks' reduction
@@ -155,9 +157,10 @@ struct E2TdbGrid {
   int blk[E2_TDB_MAXB];
 };
 // the panel updates of one launch: problem p owns workgroups cum[p] .. cum[p+1]-1
+#define E2_PANEL_MAXP 32
 struct E2PanelGrid {
   int np;
-  int cum[E2_TDB_MAXB + 1];
+  int cum[E2_PANEL_MAXP + 1];
 };
 
 extern "C" {

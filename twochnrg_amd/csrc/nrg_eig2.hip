@@ -1893,7 +1893,7 @@ __global__ __launch_bounds__(256) void panel_gemm_kernel(const E2Gemm* __restric
   const int x = blockIdx.x;
   int p = 0;
 #pragma unroll
-  for (int i = 1; i < E2_TDB_MAXB; i++) p += (i < G.np && G.cum[i] <= x) ? 1 : 0;
+  for (int i = 1; i < E2_PANEL_MAXP; i++) p += (i < G.np && G.cum[i] <= x) ? 1 : 0;
   const E2Gemm P = probs[p];
   int a, b;
   band_tile((P.m + GT - 1) / GT, x - G.cum[p], a, b);
@@ -2003,9 +2003,9 @@ static int band_tiles(int m) {
 
 hipError_t e2_launch_panel_gemm(const E2Gemm* dprobs, const E2Gemm* hprobs, int np, const double* A, const double* B,
                                 double* C, hipStream_t s) {
-  for (int p0 = 0; p0 < np; p0 += E2_TDB_MAXB) {
+  for (int p0 = 0; p0 < np; p0 += E2_PANEL_MAXP) {
     E2PanelGrid G = {};
-    G.np = np - p0 < E2_TDB_MAXB ? np - p0 : E2_TDB_MAXB;
+    G.np = np - p0 < E2_PANEL_MAXP ? np - p0 : E2_PANEL_MAXP;
     G.cum[0] = 0;
     for (int i = 0; i < G.np; i++) G.cum[i + 1] = G.cum[i] + band_tiles(hprobs[p0 + i].m);
     if (G.cum[G.np] <= 0) continue;
