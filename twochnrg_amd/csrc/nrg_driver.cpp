@@ -21,6 +21,7 @@
 #include <map>
 #include <memory>
 #include <numeric>
+#include <thread>
 #include <tuple>
 #include <utility>
 #include <vector>
@@ -723,6 +724,21 @@ struct Chain {
     return 0;
   }
 
+  // the eigenvalues from a host copy of the whole eigenvalue buffer (wall[i] =
+  // dWn[i]); no device call, so instances can do this on host threads
+  void collect_host(const double* wall, double* dZn, double* dWn) {
+    const int ns = Abasis.Nstates();
+    Aeig = CNRGarray();
+    Aeig.Nshell = Abasis.Nshell; Aeig.NQNumbers = 2;
+    Aeig.QNumbers = Abasis.QNumbers;
+    Aeig.BlockBegEnd = Abasis.BlockBegEnd;
+    const int64_t w0 = L_woff.empty() ? 0 : L_woff[0] - Abasis.GetBlockLimit(0, 0);
+    Aeig.dEn.assign(wall + w0, wall + w0 + ns);
+    zrec.clear();
+    Aeig_zoff = L_hoff;
+    dZ = dZn;
+    dW = dWn;
+  }
   int collect(double* dZn, double* dWn) {
     const int ns = Abasis.Nstates();
     Aeig = CNRGarray();
@@ -1865,6 +1881,31 @@ static int run_onechqs(nrg_ctx* c, nrg_onechqs_params p, nrg_step_cb cb, void* u
 // builds its basis and assembles H_N into one shared buffer, one eigensolve
 // covers the blocks of all instances, then each instance truncates, rotates
 // its operators and reports.  All instances run the same number of sites.
+// f(0), ..., f(n-1) on up to 16 host threads (instance i on thread i mod T);
+// the first nonzero result, or 0.  For host work that touches only the
+// instance's own arrays.
+template <class F>
+static int host_par_for(int n, F f) {
+  const int hw = (int)std::thread::hardware_concurrency();
+  const int nt = std::max(1, std::min(n, std::min(hw > 0 ? hw : 1, 16)));
+  std::vector<int> res(nt, 0);
+  auto part = [&](int t) {
+    for (int i = t; i < n && !res[t]; i += nt) res[t] = f(i);
+  };
+  std::vector<std::thread> th;
+  int spawned = 0;
+  try {
+    for (int t = 1; t < nt; t++, spawned++) th.emplace_back(part, t);
+  } catch (...) {
+  }
+  part(0);
+  for (int t = 1 + spawned; t < nt; t++) part(t);   // shares of threads that could not start
+  for (auto& x : th) x.join();
+  for (int r : res)
+    if (r) return r;
+  return 0;
+}
+
 struct MultiCbCtx {
   nrg_multi_cb cb;
   void* user;
@@ -1925,17 +1966,33 @@ static int run_onechqs_multi(nrg_ctx* c, int ninst, const nrg_onechqs_params* ps
     }
     std::vector<int> krow(n.size(), 0);
     int base = 0;
-    for (int i = 0; i < ninst; i++) {
-      if ((rc = R[i]->ch.collect((double*)zb.p, (double*)wb.p))) return rc;
-      if ((rc = R[i]->post_cut(N, nullptr, nullptr))) return rc;
-      R[i]->ch.kept_rows(krow, base);
-      base += R[i]->ch.Abasis.NumBlocks();
+    if (defer) {
+      // every instance's eigenvalues in one copy; the truncations (host work on
+      // each instance's own arrays) on host threads, while nothing else runs
+      std::vector<double> wall((size_t)wb0);
+      HIPCHK(hipMemcpyAsync(wall.data(), wb.p, sizeof(double) * (size_t)wb0, hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(hipStreamSynchronize(c->stream));
+      if ((rc = host_par_for(ninst, [&](int i) {
+             R[i]->ch.collect_host(wall.data(), (double*)zb.p, (double*)wb.p);
+             return R[i]->post_cut(N, nullptr, nullptr);
+           })))
+        return rc;
+      for (int i = 0; i < ninst; i++) {
+        R[i]->ch.kept_rows(krow, base);
+        base += R[i]->ch.Abasis.NumBlocks();
+      }
+      if ((rc = eig_finish(c, krow.data()))) return rc;
+      // the thermodynamics on host threads while the kept rows are formed
+      if ((rc = host_par_for(ninst, [&](int i) { return R[i]->post_thermo(N, nullptr); }))) return rc;
+    } else {
+      for (int i = 0; i < ninst; i++) {
+        if ((rc = R[i]->ch.collect((double*)zb.p, (double*)wb.p))) return rc;
+        if ((rc = R[i]->post_cut(N, nullptr, nullptr))) return rc;
+        if ((rc = R[i]->post_thermo(N, nullptr))) return rc;
+      }
     }
-    if (defer && (rc = eig_finish(c, krow.data()))) return rc;
-    for (int i = 0; i < ninst; i++) {
-      if ((rc = R[i]->post_thermo(N, nullptr))) return rc;
+    for (int i = 0; i < ninst; i++)
       if ((rc = R[i]->post_rest(N, multi_tramp, &cbc[i], nullptr, nullptr))) return rc;
-    }
   }
   HIPCHK(hipStreamSynchronize(c->stream));
   auto t1 = std::chrono::steady_clock::now();
