@@ -964,7 +964,8 @@ struct Chain {
       M.MatBlockMap = up_nmap[m];
       M.MatBlockBegEnd = up_nbe[m];
     }
-    HIPCHK(hipStreamSynchronize(c->stream));
+    // no wait here: what reads the new operators (the next step's f-basis, the
+    // averages' gather) is ordered after the rotations on the stream
     if ((rc = c->resolve())) return rc;
     // the new operators live on AeigCut
     return make_basis_dev(c, AeigCut, (tag + "oldb").c_str(), oldb);
@@ -972,52 +973,77 @@ struct Chain {
 
   // CalcOpAvg (src/CalcOpAvg.cpp:13-84) on AeigCut for the operators with CalcAvg.
   // Only the diagonals of the diagonal blocks enter; they are gathered on the
-  // device for every such operator and come back in one transfer.
-  int op_avgs(double bbar, std::vector<double>& out) {
-    out.clear();
+  // device for every such operator and come back in one transfer.  In two
+  // halves for many chains at once: op_avgs_issue gathers them into ddst
+  // (op_avgs_len() doubles, stream-ordered), op_avgs_sum adds them up from a
+  // host copy.
+  std::vector<int> oa_avg;
+  std::vector<unsigned char> oa_has;
+  int64_t op_avgs_len() const {
+    int na = 0;
+    for (size_t m = 0; m < mats.size(); m++) na += mats[m].CalcAvg ? 1 : 0;
+    return AeigCut.dEn.empty() ? 0 : (int64_t)na * (int64_t)AeigCut.dEn.size();
+  }
+  int op_avgs_issue(double* ddst) {
     const int nb = AeigCut.NumBlocks();
     const int64_t ns = AeigCut.dEn.empty() ? 0 : (int64_t)AeigCut.dEn.size();
-    std::vector<int> avg;
+    std::vector<int>& avg = oa_avg;
+    avg.clear();
     for (size_t m = 0; m < mats.size(); m++)
       if (mats[m].CalcAvg) avg.push_back((int)m);
+    oa_has.assign((size_t)avg.size() * nb, 0);
     if (avg.empty()) return 0;
     std::vector<DiagJob> jobs;
-    std::vector<unsigned char> has((size_t)avg.size() * nb, 0);
     for (size_t a = 0; a < avg.size(); a++) {
       const CNRGmatrix& M = mats[avg[a]];
       const double* base = (const double*)c->buf(M.buf[M.cur]).p;
       for (size_t q = 0; q + 1 < M.MatBlockMap.size(); q += 2) {
         const int b = M.MatBlockMap[q];
-        if (b != M.MatBlockMap[q + 1] || b < 0 || b >= nb || has[a * nb + b]) continue;
-        has[a * nb + b] = 1;   // the first (b, b) entry of the map, as a linear search finds it
+        if (b != M.MatBlockMap[q + 1] || b < 0 || b >= nb || oa_has[a * nb + b]) continue;
+        oa_has[a * nb + b] = 1;   // the first (b, b) entry of the map, as a linear search finds it
         jobs.push_back(DiagJob{base + M.MatBlockBegEnd[q], (int64_t)a * ns + AeigCut.GetBlockLimit(b, 0),
                                AeigCut.GetBlockSize(b), 0});
       }
     }
-    std::vector<double> h((size_t)avg.size() * ns + 1, 0.0);
     if (!jobs.empty() && ns > 0) {
       DiagJob* dj;
-      int rc = upload(c, "opavg_jobs", jobs, &dj);
+      int rc = upload(c, (tag + "opavg_jobs").c_str(), jobs, &dj);
       if (rc) return rc;
-      DBuf& db = c->buf("opavg_diag");
-      if ((rc = db.reserve(sizeof(double) * h.size()))) return rc;
-      HIPCHK(nrgk_launch_diag_gather(dj, (int)jobs.size(), (double*)db.p, c->stream));
-      HIPCHK(hipMemcpyAsync(h.data(), db.p, sizeof(double) * (h.size() - 1), hipMemcpyDeviceToHost, c->stream));
-      HIPCHK(hipStreamSynchronize(c->stream));
+      HIPCHK(nrgk_launch_diag_gather(dj, (int)jobs.size(), ddst, c->stream));
     }
-    for (size_t a = 0; a < avg.size(); a++) {
+    return 0;
+  }
+  // h: the op_avgs_len() doubles op_avgs_issue gathered (entries no job wrote are not read)
+  void op_avgs_sum(double bbar, const double* h, std::vector<double>& out) const {
+    out.clear();
+    const int nb = AeigCut.NumBlocks();
+    const int64_t ns = AeigCut.dEn.empty() ? 0 : (int64_t)AeigCut.dEn.size();
+    for (size_t a = 0; a < oa_avg.size(); a++) {
       double s = 0, Z = 0;
       for (int b = 0; b < nb; b++) {
         const double dg = 2.0 * AeigCut.GetQNumber(b, 1) + 1.0;
         const int n = AeigCut.GetBlockSize(b), i0 = AeigCut.GetBlockLimit(b, 0);
         for (int i = 0; i < n; i++) {
           const double x = std::exp(-bbar * AeigCut.dEn[i0 + i]);
-          const double el = has[a * nb + b] ? h[(size_t)a * ns + i0 + i] : 0.0;
+          const double el = oa_has[a * nb + b] ? h[(size_t)a * ns + i0 + i] : 0.0;
           s += dg * el * x; Z += dg * x;
         }
       }
       out.push_back(std::fabs(Z) > 1e-20 ? s / Z : s);
     }
+  }
+  int op_avgs(double bbar, std::vector<double>& out) {
+    const int64_t len = op_avgs_len();
+    DBuf& db = c->buf("opavg_diag");
+    int rc;
+    if ((rc = db.reserve(sizeof(double) * (size_t)(len + 1)))) return rc;
+    if ((rc = op_avgs_issue((double*)db.p))) return rc;
+    std::vector<double> h((size_t)len + 1, 0.0);
+    if (len > 0 && !oa_avg.empty()) {
+      HIPCHK(hipMemcpyAsync(h.data(), db.p, sizeof(double) * (size_t)len, hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(hipStreamSynchronize(c->stream));
+    }
+    op_avgs_sum(bbar, h.data(), out);
     return 0;
   }
 
@@ -1804,11 +1830,13 @@ struct OneChRun {
   // UpdateMatrices, the DM-NRG capture, CalcStuff, the record, chi for the next site
   int post_rest(int N, nrg_step_cb cb, void* user, std::vector<DmShell>* dm, Tangent* tg) {
     int rc;
-    const double DN = s_DN, emin = s_emin, ecut = s_ecut;
-    const int nkept = s_nkept;
-    double* thT = s_thT;
-    double* sus = s_sus;
-    double* ent = s_ent;
+    if ((rc = rest_update(N, dm, tg))) return rc;
+    if ((rc = ch.op_avgs(bbar, ops))) return rc;
+    return rest_emit(N, cb, user, tg);
+  }
+  // post_rest before the operator averages: UpdateMatrices and the DM-NRG capture
+  int rest_update(int N, std::vector<DmShell>* dm, Tangent* tg) {
+    int rc;
     if (tg) {
       if ((rc = tg->cut(ch))) return rc;
       if ((rc = ch.plan_update())) return rc;
@@ -1822,7 +1850,16 @@ struct OneChRun {
       if ((rc = dm_capture(ch, N, 1, dm->back()))) return rc;
       if (tg && (rc = tg->capture(dm->back(), 1))) return rc;
     }
-    if ((rc = ch.op_avgs(bbar, ops))) return rc;
+    return 0;
+  }
+  // post_rest after the operator averages (in ops): the record, chi for the next site
+  int rest_emit(int N, nrg_step_cb cb, void* user, Tangent* tg) {
+    int rc;
+    const double DN = s_DN, emin = s_emin, ecut = s_ecut;
+    const int nkept = s_nkept;
+    double* thT = s_thT;
+    double* sus = s_sus;
+    double* ent = s_ent;
     if (tg && (rc = tg->op_avgs(ch, bbar))) return rc;
     if (befcut) ch.cut_states(p.ncut, false, p.myeps);
     ch.emit(cb, user, emin, ecut, nkept, thT, sus, ent, DN / bbar, ops);
@@ -1991,8 +2028,25 @@ static int run_onechqs_multi(nrg_ctx* c, int ninst, const nrg_onechqs_params* ps
         if ((rc = R[i]->post_thermo(N, nullptr))) return rc;
       }
     }
+    // every instance's operator updates, then the diagonals for their averages
+    // gathered into one buffer and copied back with one wait
     for (int i = 0; i < ninst; i++)
-      if ((rc = R[i]->post_rest(N, multi_tramp, &cbc[i], nullptr, nullptr))) return rc;
+      if ((rc = R[i]->rest_update(N, nullptr, nullptr))) return rc;
+    std::vector<int64_t> aoff(ninst + 1, 0);
+    for (int i = 0; i < ninst; i++) aoff[i + 1] = aoff[i] + R[i]->ch.op_avgs_len();
+    DBuf& ab = c->buf("mOpavg");
+    if ((rc = ab.reserve(sizeof(double) * (size_t)(aoff[ninst] + 1)))) return rc;
+    for (int i = 0; i < ninst; i++)
+      if ((rc = R[i]->ch.op_avgs_issue((double*)ab.p + aoff[i]))) return rc;
+    std::vector<double> hall((size_t)aoff[ninst] + 1, 0.0);
+    if (aoff[ninst] > 0)
+      HIPCHK(hipMemcpyAsync(hall.data(), ab.p, sizeof(double) * (size_t)aoff[ninst], hipMemcpyDeviceToHost,
+                            c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int i = 0; i < ninst; i++) {
+      R[i]->ch.op_avgs_sum(R[i]->bbar, hall.data() + aoff[i], R[i]->ops);
+      if ((rc = R[i]->rest_emit(N, multi_tramp, &cbc[i], nullptr))) return rc;
+    }
   }
   HIPCHK(hipStreamSynchronize(c->stream));
   auto t1 = std::chrono::steady_clock::now();
