@@ -493,15 +493,14 @@ int nrg::rotate_dev(nrg_ctx* c, const std::vector<RotJob>& jobs, const double* d
   size_t n1t = 0;
   if (rot_plan(jobs, pall, tall, n1t) == 0) return NRG_OK;
   const size_t n1tiles = n1t;
-  DBuf& dp = c->buf("rot_probs");
-  DBuf& dt = c->buf("rot_tiles");
-  const size_t np = pall.size(), nt = tall.size();
+  const size_t nt = tall.size();
   int rc;
-  if ((rc = dp.reserve(sizeof(E2Gemm) * np))) return rc;
-  if ((rc = dt.reserve(sizeof(E2Tile) * nt))) return rc;
-  HIPCHK(hipMemcpyAsync(dp.p, pall.data(), sizeof(E2Gemm) * np, hipMemcpyHostToDevice, c->stream));
-  HIPCHK(hipMemcpyAsync(dt.p, tall.data(), sizeof(E2Tile) * nt, hipMemcpyHostToDevice, c->stream));
-  HIPCHK(hipStreamSynchronize(c->stream));
+  // through the pinned ring, with no wait: a later call's copies into the same
+  // buffers run after these launches on the stream
+  E2Gemm* d_p;
+  E2Tile* d_t;
+  if ((rc = upload(c, "rot_probs", pall, &d_p))) return rc;
+  if ((rc = upload(c, "rot_tiles", tall, &d_t))) return rc;
   double fl = 0.0, by = 0.0;
   for (auto& g : pall) {
     fl += 2.0 * g.m * (double)g.n * g.k;
@@ -509,8 +508,8 @@ int nrg::rotate_dev(nrg_ctx* c, const std::vector<RotJob>& jobs, const double* d
   }
   PendingTimer tm;
   if ((rc = c->timer_begin(tm, NRG_K_GEMM, fl, by))) return rc;
-  HIPCHK(e2_launch_gemm3((E2Gemm*)dp.p, (E2Tile*)dt.p, (int)n1tiles, dZ, dF, dAux, c->stream));
-  HIPCHK(e2_launch_gemm3((E2Gemm*)dp.p, (E2Tile*)dt.p + n1tiles, (int)(nt - n1tiles), dZ, dAux, dOut, c->stream));
+  HIPCHK(e2_launch_gemm3(d_p, d_t, (int)n1tiles, dZ, dF, dAux, c->stream));
+  HIPCHK(e2_launch_gemm3(d_p, d_t + n1tiles, (int)(nt - n1tiles), dZ, dAux, dOut, c->stream));
   if ((rc = c->timer_end(tm))) return rc;
   return NRG_OK;
 }
