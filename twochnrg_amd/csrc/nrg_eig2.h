@@ -171,7 +171,7 @@ hipError_t e2_launch_tdb_norm(const E2Block* b, const int* list, int nlist, int 
 // each block's index in b and its order
 hipError_t e2_launch_tdb_symv(const E2Block* b, const int* blk_idx, const int* blk_n, int nlist, int k, double* A,
                               double* WS, hipStream_t s);
-// rows s0..s0+31 of every block still being reduced rebuilt from their columns
+// rows s0..s0+E2_PANEL-1 of every block still being reduced rebuilt from their columns
 hipError_t e2_launch_tdb_refresh(const E2Block* b, const int* blk_idx, const int* blk_n, int nlist, int s0, double* A,
                                  hipStream_t s);
 hipError_t e2_launch_tdb_fin(const E2Block* b, const int* blk_idx, const int* blk_n, int nlist, int k, double* A,

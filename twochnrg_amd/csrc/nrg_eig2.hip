@@ -2,13 +2,15 @@
 // blocks, built from three stages that each run over all blocks at once:
 //
 //   1. Householder reduction to tridiagonal form.  Blocks with n <= 256: one
-//      256-thread workgroup per block.  Larger blocks: blocked reduction in
-//      panels of 32 columns -- per column a 2-D tiled y = A v (a thread owns a
-//      column, so each row access of a wavefront is one contiguous 512-byte
-//      segment) with the panel corrections, per panel one rank-64 GEMM update
-//      of the trailing matrix.
+//      1024-thread workgroup per block, the matrix held in registers.  Larger
+//      blocks: blocked reduction in panels of E2_PANEL columns -- per column
+//      y = A v over the lower-triangle tiles of the symmetric trailing matrix
+//      with the panel corrections, then a finish launch (w, the next column
+//      and its partial norms); per panel one rank-2 E2_PANEL GEMM update of
+//      the lower triangle; the last <= E2_RS_MAXN rows of each block go to the
+//      register-resident kernel.
 //   2. Divide and conquer on the tridiagonal matrix (Cuppen's tearing): leaves
-//      of <= 32 rows by implicit QL; merges by z-vector deflation (small
+//      of <= 16 rows by implicit QL; merges by z-vector deflation (small
 //      components, and Givens rotation of nearly equal poles), secular-equation
 //      roots by safeguarded Newton in the distance to the nearer pole,
 //      Gu-Eisenstat recomputation of z for orthogonal vectors, and a GEMM for
