@@ -40,6 +40,21 @@ def test_truncate_matches_oracle_random():
         assert np.array_equal(kept, o <= ec) and nk == int(np.sum(o <= ec))
 
 
+def test_truncate_matches_oracle_large():
+    # tens of thousands of states: the library sorts on several host threads
+    # there (chunk sorts and stable merges); the result must still be the oracle's
+    rng = np.random.default_rng(11)
+    for n, eps in ((40000, 1e-4), (40000, 1.5e-3), (20000, 1e-13)):
+        e = np.round(rng.random(n) * 400.0, 3)
+        e[rng.integers(0, n)] = 0.0
+        ncut = n // 7
+        e2, kept, ec, nk = t.truncate(e, ncut, eps)
+        o = po.setdegen(e, eps)
+        assert np.array_equal(e2, o)
+        assert ec == po.ecut(o, ncut) == t.ecut(o, ncut)
+        assert np.array_equal(kept, o <= ec) and nk == int(np.sum(o <= ec))
+
+
 def test_ecut_window_and_edges():
     # 1% window extends the cut over near-degenerate levels
     e = np.array([0.0, 1.0, 1.001, 1.005, 1.2, 2.0])

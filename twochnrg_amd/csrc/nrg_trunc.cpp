@@ -3,8 +3,10 @@
 // built on its own under AddressSanitizer/UBSan by the tests.
 #include <algorithm>
 #include <cmath>
+#include <functional>
 #include <numeric>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/twochnrg_amd.h"
@@ -17,6 +19,45 @@ thread_local std::string g_err;
 int fail(int code, const std::string& msg) {
   g_err = msg;
   return code;
+}
+
+// v sorted by comp, on up to 8 host threads when it is long (an NRG step of
+// tens of thousands of states): each thread sorts a contiguous chunk, then
+// neighbouring chunks are merged.  std::merge takes from the left range on
+// ties, so with stable chunk sorts the result is std::stable_sort's; for a
+// plain sort of values the sorted sequence is the same whatever the method.
+template <class T, class C>
+static void par_sort(std::vector<T>& v, C comp, bool stable) {
+  const size_t n = v.size();
+  const int hw = (int)std::thread::hardware_concurrency();
+  const int nt = std::min(8, hw > 0 ? hw : 1);
+  auto serial = [&](size_t a, size_t b) {
+    if (stable) std::stable_sort(v.begin() + a, v.begin() + b, comp);
+    else std::sort(v.begin() + a, v.begin() + b, comp);
+  };
+  if (n < 16384 || nt < 2) {
+    serial(0, n);
+    return;
+  }
+  std::vector<size_t> bd(nt + 1);
+  for (int t = 0; t <= nt; t++) bd[t] = n * t / nt;
+  std::vector<std::thread> th;
+  int spawned = 0;
+  try {
+    for (int t = 1; t < nt; t++, spawned++) th.emplace_back(serial, bd[t], bd[t + 1]);
+  } catch (...) {
+  }
+  serial(bd[0], bd[1]);
+  for (int t = 1 + spawned; t < nt; t++) serial(bd[t], bd[t + 1]);   // chunks of threads that could not start
+  for (auto& x : th) x.join();
+  std::vector<T> tmp(n);
+  for (int w = 1; w < nt; w *= 2) {
+    for (int i = 0; i < nt; i += 2 * w) {
+      const size_t a = bd[i], m = bd[std::min(i + w, nt)], b = bd[std::min(i + 2 * w, nt)];
+      std::merge(v.begin() + a, v.begin() + m, v.begin() + m, v.begin() + b, tmp.begin() + a, comp);
+    }
+    v.swap(tmp);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -32,7 +73,7 @@ void set_degen(std::vector<double>& en, double eps) {
   if (n < 2) return;
   std::vector<int> idx(n);
   std::iota(idx.begin(), idx.end(), 0);
-  std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return en[a] < en[b]; });
+  par_sort(idx, [&](int a, int b) { return en[a] < en[b]; }, true);
   int s = 0;
   while (s < n) {
     int e = s + 1;
@@ -58,7 +99,7 @@ void set_degen(std::vector<double>& en, double eps) {
 // as a level far above the cut, which keeps every remaining state.
 double ecut_of(const std::vector<double>& en, int ncut) {
   std::vector<double> e(en);
-  std::sort(e.begin(), e.end());
+  par_sort(e, std::less<double>(), false);
   const int n = (int)e.size();
   if (n == 0) return 0.0;
   if (ncut >= n - 1) return e[n - 1];
